@@ -1,0 +1,300 @@
+/*
+ * nemo_hip.h -- C-ABI of the MI355X (gfx950) implementation of the NEMO 4.0
+ * ocean time-step hot path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (NEMO_HIP_OK == 0) and never throws or aborts across the ABI.  On
+ * failure the message is available through nemo_hip_last_error().
+ *
+ * Array conventions (identical to NEMO's Fortran arrays, so a Fortran caller
+ * passes C_LOC() of its own arrays with no transposition):
+ *   - 3D fields are (jpi,jpj,jpk) column-major, i fastest: element (i,j,k)
+ *     (1-based) lives at offset (i-1) + jpi*((j-1) + jpj*(k-1));
+ *   - 4D tracer fields are (jpi,jpj,jpk,kjpt), tracer index slowest;
+ *   - 2D fields are (jpi,jpj).
+ *
+ * Pointer spaces.  With nemo_hip_set_pointer_mode(NEMO_HIP_PTR_HOST) (the
+ * default, and the drop-in mode a Fortran model uses) array arguments are host
+ * memory: the library stages inputs into HBM, runs the kernels and copies the
+ * outputs back before returning.  With NEMO_HIP_PTR_DEVICE they are device
+ * pointers already resident in HBM; calls are then asynchronous on the
+ * library's stream and nemo_hip_sync() waits for them.
+ *
+ * Domain state that NEMO keeps in module variables (sizes, scale factors,
+ * masks, k-indices, namelist switches) is registered once with
+ * nemo_hip_domain_register(); the 3D masks are not passed, they are derived on
+ * the device from k_top/k_bot exactly as DOM/dommsk.F90 and DOM/domzgr.F90 do.
+ *
+ * Module arrays a reference routine reads or writes without having them in its
+ * dummy-argument list (avt, the iso-neutral slopes, ah_wslp2/akz, ...) appear
+ * as extra trailing pointer arguments.
+ *
+ * All references below are relative to the NEMO 4.0 source tree (src/OCE).
+ */
+#ifndef NEMO_HIP_H
+#define NEMO_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* status codes                                                             */
+/* ------------------------------------------------------------------------ */
+#define NEMO_HIP_OK              0
+#define NEMO_HIP_ERR_ARG         1   /* bad argument / inconsistent sizes       */
+#define NEMO_HIP_ERR_UNSUPPORTED 2   /* namelist option not implemented         */
+#define NEMO_HIP_ERR_STATE       3   /* not initialised / no domain registered  */
+#define NEMO_HIP_ERR_HIP         4   /* HIP runtime error                       */
+#define NEMO_HIP_ERR_NUMERIC     5   /* stp_ctl-style blow-up detected          */
+
+#define NEMO_HIP_PTR_HOST   0
+#define NEMO_HIP_PTR_DEVICE 1
+
+/* vorticity schemes: DYN/dynvor.F90:65-70 */
+#define NEMO_NP_ENS 0
+#define NEMO_NP_ENE 1
+#define NEMO_NP_ENT 2
+#define NEMO_NP_EET 3
+#define NEMO_NP_EEN 4
+#define NEMO_NP_MIX 5
+/* momentum advection forms: DYN/dynadv.F90:41-44 */
+#define NEMO_NP_LIN_DYN 0
+#define NEMO_NP_VEC_C2  1
+#define NEMO_NP_FLX_C2  2
+#define NEMO_NP_FLX_UBS 3
+
+/* ------------------------------------------------------------------------ */
+/* domain: the module state of par_oce.F90, dom_oce.F90, phycst.F90 and the   */
+/* namelist switches the hot path reads                                     */
+/* ------------------------------------------------------------------------ */
+typedef struct nemo_hip_domain {
+    /* sizes, par_oce.F90:50-76 */
+    int32_t jpi, jpj, jpk;
+    /* lateral boundary type, DOM/dom_oce.F90:68 (0 closed, 1 cyclic E-W,
+       2 cyclic N-S, 7 cyclic both; north folds 3-6 are not supported) */
+    int32_t jperio;
+    /* vertical coordinate / free surface, DOM/dom_oce.F90 */
+    int32_t ln_linssh;          /* must be 1: e3 fixed in time              */
+    int32_t ln_isfcav;          /* must be 0: no ice-shelf cavities         */
+    int32_t ln_zps;             /* must be 0: full steps                    */
+    int32_t ln_sco;             /* must be 0                                */
+    /* tracer physics, LDF/ldftra.F90, ZDF/zdf_oce.F90 */
+    int32_t ln_traldf_lap;      /* laplacian (1) or bilaplacian (0)         */
+    int32_t ln_traldf_msc;      /* method of stabilizing correction         */
+    int32_t ln_zdfddm;          /* double diffusion: separate S matrix      */
+    int32_t ln_zad_Aimp;        /* must be 0                                */
+    /* momentum, DYN/dynadv.F90, DYN/dynkeg.F90, DYN/dynvor.F90 */
+    int32_t n_dynadv;           /* NEMO_NP_*_DYN / NEMO_NP_VEC_C2 / ...     */
+    int32_t nn_dynkeg;          /* 0 C2, 1 Hollingsworth                    */
+    int32_t nvor_scheme;        /* NEMO_NP_ENS ... NEMO_NP_MIX              */
+    int32_t nn_een_e3f;         /* 0 or 1                                   */
+    /* vertical momentum diffusion, ZDF/zdfdrg.F90 */
+    int32_t ln_drgimp;          /* implicit bottom friction                 */
+    /* split-explicit free surface, DYN/dynspg_ts.F90 (namdyn_spg)          */
+    int32_t ln_dynspg_ts;
+    int32_t ln_bt_fw, ln_bt_av, nn_bt_flt, nn_baro;
+    /* time stepping, DOM/dom_oce.F90:62-63 */
+    int32_t neuler, nit000;
+    int32_t reserved0;
+    double  rdt;                /* baroclinic time step [s]                 */
+    double  rn_atfp;            /* Asselin filter parameter                 */
+    double  rau0;               /* reference density, TRA/eosbn2.F90:1244   */
+    double  grav;               /* DOM/phycst.F90:38                        */
+    double  rn_bt_alpha;        /* must be 0                                */
+    double  rn_scal_load;       /* 0 without tides (SBC/tideini.F90:84)     */
+    /* first / last wet level per column (0 = land), DOM/dommsk.F90 input */
+    const int32_t *k_top, *k_bot;
+    /* horizontal scale factors and their derived forms, DOM/domhgr.F90:133-150 */
+    const double *e1t, *e2t, *e1u, *e2u, *e1v, *e2v, *e1f, *e2f;
+    const double *e1e2t, *r1_e1e2t, *e1e2u, *r1_e1e2u, *e1e2v, *r1_e1e2v;
+    const double *e1e2f, *r1_e1e2f;
+    const double *e2_e1u, *e1_e2v, *r1_e1u, *r1_e2u, *r1_e1v, *r1_e2v;
+    /* Coriolis parameter at f- and t-points */
+    const double *ff_f, *ff_t;
+    /* vertical scale factors (now == before == after when ln_linssh) */
+    const double *e3t_n, *e3u_n, *e3v_n, *e3w_n, *e3f_n, *e3uw_n, *e3vw_n;
+} nemo_hip_domain_t;
+
+/* ------------------------------------------------------------------------ */
+/* split-explicit surface pressure gradient: the module state dyn_spg_ts    */
+/* reads and writes (oce.F90, sbc_oce.F90, zdfdrg.F90, dynspg_ts.F90:73)    */
+/* ------------------------------------------------------------------------ */
+typedef struct nemo_hip_spg_ts {
+    /* 3D (jpi,jpj,jpk): in/out momentum trends and now velocities */
+    void *ua, *va, *un, *vn;
+    /* 2D (jpi,jpj), always FP64 */
+    const double *un_b, *vn_b, *ub_b, *vb_b;       /* now / before barotropic   */
+    const double *sshn;                            /* now sea surface height    */
+    const double *hu_n, *hv_n, *r1_hu_n, *r1_hv_n, *ht_n;
+    const double *rCdU_bot;                        /* bottom drag (t-points)    */
+    const double *utau, *vtau;                     /* wind stress               */
+    const double *emp, *rnf;                       /* fresh-water fluxes        */
+    double *ua_b, *va_b, *ssha;                    /* out: after barotropic     */
+    double *un_adv, *vn_adv;                       /* out: advective transport  */
+    double *ub2_b, *vb2_b, *un_bf, *vn_bf;         /* in/out: time-filter state */
+} nemo_hip_spg_ts_t;
+
+/* ------------------------------------------------------------------------ */
+/* library control                                                          */
+/* ------------------------------------------------------------------------ */
+int nemo_hip_init(int device);
+int nemo_hip_finalize(void);
+/* copies the last error message (NUL terminated, truncated to len) */
+int nemo_hip_last_error(char *buf, int len);
+int nemo_hip_set_pointer_mode(int mode);
+int nemo_hip_sync(void);
+/* the library's HIP stream, as an opaque handle (hipStream_t) */
+void *nemo_hip_stream(void);
+
+/* replaces the module initialisation done by dom_init (DOM/domain.F90) for
+   the arrays the hot path reads; call again if the domain changes */
+int nemo_hip_domain_register(const nemo_hip_domain_t *dom);
+
+/* per-routine GPU timers keyed by reference routine name (timing.F90:89-212):
+   when enabled every kernel launch is bracketed by HIP events on the
+   library's stream and accumulated by name */
+int nemo_hip_timing_enable(int on);
+int nemo_hip_timing_reset(void);
+/* total milliseconds and launch count accumulated for `name` */
+int nemo_hip_timing_get(const char *name, double *total_ms, int64_t *count);
+
+/* ------------------------------------------------------------------------ */
+/* a2: masks and k-indices, DOM/dommsk.F90:134-210 and DOM/domzgr.F90:267-316 */
+/* every output pointer may be NULL                                         */
+/* ------------------------------------------------------------------------ */
+int nemo_hip_dom_msk(double *tmask, double *umask, double *vmask, double *fmask,
+                     double *wmask, double *wumask, double *wvmask,
+                     double *ssmask, double *ssumask, double *ssvmask,
+                     int32_t *mikt, int32_t *mbkt, int32_t *miku, int32_t *mikv,
+                     int32_t *mbku, int32_t *mbkv);
+
+/* a19: lbc_lnk, serial path, LBC/lbclnk.F90:81-89 + LBC/lbc_lnk_generic.h90:48-108.
+   ptab is (jpi,jpj,nk); cd_nat in 'T','U','V','W','F'; psgn only matters at a
+   north fold (unsupported); pval is the closed-boundary value. */
+int nemo_hip_lbc_lnk(double *ptab, int nk, char cd_nat, double psgn, double pval);
+
+/* ------------------------------------------------------------------------ */
+/* tracers                                                                  */
+/* ------------------------------------------------------------------------ */
+/* a3: effective transport of tra_adv, TRA/traadv.F90:100-124 */
+int nemo_hip_tra_adv_trp(const double *un, const double *vn, const double *wn,
+                         double *zun, double *zvn, double *zwn);
+
+/* a4-a5: FCT advection, TRA/traadv_fct.F90:54-80 (kn_fct_h, kn_fct_v in {2,4}) */
+int nemo_hip_tra_adv_fct(int kt, int kit000, const char *cdtype, double p2dt,
+                         const double *pun, const double *pvn, const double *pwn,
+                         const double *ptb, const double *ptn, double *pta,
+                         int kjpt, int kn_fct_h, int kn_fct_v);
+
+/* a6: 4th order compact T->w interpolation, TRA/traadv_fct.F90:517 */
+int nemo_hip_interp_4th_cpt(const double *pt_in, double *pt_out);
+
+/* a8: iso-level laplacian, TRA/traldf_lap_blp.F90:47-76 */
+int nemo_hip_tra_ldf_lap(int kt, int kit000, const char *cdtype,
+                         const double *pahu, const double *pahv,
+                         const double *pgu, const double *pgv,
+                         const double *pgui, const double *pgvi,
+                         const double *ptb, double *pta, int kjpt, int kpass);
+
+/* a9: rotated (iso-neutral) laplacian, TRA/traldf_iso.F90:50-103.
+   Trailing arguments: slopes of LDF/ldfslp.F90:57-58 (in) and the module
+   arrays ah_wslp2, akz of LDF/ldfslp.F90:64-65 (written when kpass == 1). */
+int nemo_hip_tra_ldf_iso(int kt, int kit000, const char *cdtype,
+                         const double *pahu, const double *pahv,
+                         const double *pgu, const double *pgv,
+                         const double *pgui, const double *pgvi,
+                         const double *ptb, const double *ptbb, double *pta,
+                         int kjpt, int kpass,
+                         const double *uslp, const double *vslp,
+                         const double *wslpi, const double *wslpj,
+                         double *ah_wslp2, double *akz);
+
+/* a11: implicit vertical diffusion, TRA/trazdf.F90:109-135.  pta is the trend
+   on entry and the after tracer on exit.  avs may equal avt.  l_ldfslp != 0
+   adds ah_wslp2 (or akz when ln_traldf_msc) to the diffusivity. */
+int nemo_hip_tra_zdf_imp(int kt, int kit000, const char *cdtype, double p2dt,
+                         const double *ptb, double *pta, int kjpt,
+                         const double *avt, const double *avs, int l_ldfslp,
+                         const double *ah_wslp2, const double *akz);
+
+/* a10: tra_zdf, TRA/trazdf.F90:46-106: r2dt from kt, tra_zdf_imp on (T,S),
+   then negative salinity reset to 0.1 */
+int nemo_hip_tra_zdf(int kt, const double *tsb, double *tsa,
+                     const double *avt, const double *avs, int l_ldfslp,
+                     const double *ah_wslp2, const double *akz);
+
+/* ------------------------------------------------------------------------ */
+/* momentum                                                                 */
+/* ------------------------------------------------------------------------ */
+/* a13: DYN/dynkeg.F90:46-192 (kscheme 0 C2, 1 Hollingsworth) */
+int nemo_hip_dyn_keg(int kt, int kscheme, const double *un, const double *vn,
+                     double *ua, double *va);
+/* a14: DYN/dynzad.F90:38-121 */
+int nemo_hip_dyn_zad(int kt, const double *un, const double *vn, const double *wn,
+                     double *ua, double *va);
+/* a15: DYN/dynadv_cen2.F90:37-144 */
+int nemo_hip_dyn_adv_cen2(int kt, const double *un, const double *vn,
+                          const double *wn, double *ua, double *va);
+/* a12: DYN/dynadv.F90:55-85, dispatch on the registered n_dynadv */
+int nemo_hip_dyn_adv(int kt, const double *un, const double *vn, const double *wn,
+                     double *ua, double *va);
+/* a16: DYN/dynvor.F90:98-186, scheme from the registered nvor_scheme */
+int nemo_hip_dyn_vor(int kt, const double *un, const double *vn,
+                     double *ua, double *va);
+
+/* a17: DYN/dynzdf.F90:47-499.  ua/va are trends on entry and after
+   velocities on exit. */
+int nemo_hip_dyn_zdf(int kt, const double *ub, const double *vb,
+                     double *ua, double *va, const double *avm,
+                     const double *rCdU_bot, const double *ua_b, const double *va_b,
+                     const double *utau, const double *utau_b,
+                     const double *vtau, const double *vtau_b);
+
+/* a18: DYN/dynspg_ts.F90:121-1310 (3D arrays in st are double*) */
+int nemo_hip_dyn_spg_ts(int kt, nemo_hip_spg_ts_t *st);
+/* ts_wgt, DYN/dynspg_ts.F90:1313-1388: weights and cycle length */
+int nemo_hip_ts_wgt(int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2);
+
+/* ------------------------------------------------------------------------ */
+/* FP32-storage precision form: same arguments with float 3D fields.         */
+/* Stencils run in FP32; the Thomas eliminations of tra_zdf/dyn_zdf, the     */
+/* barotropic subcycles and all 2D barotropic state stay FP64.               */
+/* ------------------------------------------------------------------------ */
+int nemo_hip_tra_adv_trp_sp(const float *un, const float *vn, const float *wn,
+                            float *zun, float *zvn, float *zwn);
+int nemo_hip_tra_adv_fct_sp(int kt, int kit000, const char *cdtype, double p2dt,
+                            const float *pun, const float *pvn, const float *pwn,
+                            const float *ptb, const float *ptn, float *pta,
+                            int kjpt, int kn_fct_h, int kn_fct_v);
+int nemo_hip_tra_ldf_lap_sp(int kt, int kit000, const char *cdtype,
+                            const float *pahu, const float *pahv,
+                            const float *ptb, float *pta, int kjpt, int kpass);
+int nemo_hip_tra_ldf_iso_sp(int kt, int kit000, const char *cdtype,
+                            const float *pahu, const float *pahv,
+                            const float *ptb, const float *ptbb, float *pta,
+                            int kjpt, int kpass,
+                            const float *uslp, const float *vslp,
+                            const float *wslpi, const float *wslpj,
+                            float *ah_wslp2, float *akz);
+int nemo_hip_tra_zdf_sp(int kt, const float *tsb, float *tsa,
+                        const float *avt, const float *avs, int l_ldfslp,
+                        const float *ah_wslp2, const float *akz);
+int nemo_hip_dyn_adv_sp(int kt, const float *un, const float *vn, const float *wn,
+                        float *ua, float *va);
+int nemo_hip_dyn_vor_sp(int kt, const float *un, const float *vn,
+                        float *ua, float *va);
+int nemo_hip_dyn_zdf_sp(int kt, const float *ub, const float *vb,
+                        float *ua, float *va, const float *avm,
+                        const double *rCdU_bot, const double *ua_b, const double *va_b,
+                        const double *utau, const double *utau_b,
+                        const double *vtau, const double *vtau_b);
+/* 3D arrays in st are float* */
+int nemo_hip_dyn_spg_ts_sp(int kt, nemo_hip_spg_ts_t *st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEMO_HIP_H */

@@ -1,0 +1,949 @@
+// api.hip -- the extern "C" boundary of include/nemo_hip.h.
+//
+// Library state: one HIP stream, the registered domain (masks as per-column
+// level intervals, k-indices, FP64 and FP32 copies of the metrics), library
+// owned scratch arrays, the per-routine timers and, in host-pointer mode, a
+// pool of staging buffers.  Every entry point validates its arguments,
+// stages host arrays if needed, launches the kernels of tra.hip / dyn.hip /
+// spg.hip and returns a status; no exception or abort crosses the ABI.
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace nh {
+static char g_err[NH_MAXSTR] = "";
+static bool g_init = false;
+static int g_device = -1;
+static hipStream_t g_stream = nullptr;
+static int g_ptr_mode = NEMO_HIP_PTR_HOST;
+static bool g_have_domain = false;
+static DevDomain g_dom;
+static nemo_hip_domain_t g_hdom;
+static std::vector<void *> g_dom_allocs;
+static void *g_scratch3[S_NSCRATCH3] = {};
+static double *g_scratch2[32] = {};
+static double g_r2dt_tra, g_r2dt_zdf, g_r2dt_dyn;   // module r2dt of tra_adv / tra_zdf / dyn_zdf
+// staging pool (host-pointer mode)
+static std::vector<std::pair<void *, size_t>> g_pool;
+// timing
+struct Pending {
+    std::string name;
+    hipEvent_t a, b;
+};
+static bool g_timing = false;
+static std::vector<Pending> g_pend;
+static std::vector<hipEvent_t> g_evfree;
+static std::map<std::string, std::pair<double, int64_t>> g_tot;
+static std::string g_open_name;
+static hipEvent_t g_open_ev = nullptr;
+
+int set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+hipStream_t stream() { return g_stream; }
+const DevDomain &dom() { return g_dom; }
+bool have_domain() { return g_have_domain; }
+void *scratch3(int which) { return g_scratch3[which]; }
+double *scratch2(int which) { return g_scratch2[which]; }
+
+static hipEvent_t ev_get()
+{
+    if (!g_evfree.empty()) {
+        hipEvent_t e = g_evfree.back();
+        g_evfree.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+void timing_begin(const char *name)
+{
+    if (!g_timing) return;
+    g_open_ev = ev_get();
+    g_open_name = name;
+    if (g_open_ev) (void)hipEventRecord(g_open_ev, g_stream);
+}
+void timing_end(const char *name)
+{
+    if (!g_timing || !g_open_ev) return;
+    hipEvent_t b = ev_get();
+    if (!b) return;
+    (void)hipEventRecord(b, g_stream);
+    g_pend.push_back({std::string(name), g_open_ev, b});
+    g_open_ev = nullptr;
+    if (g_pend.size() > 4096) {
+        // fold early so the event pool stays bounded
+        (void)hipStreamSynchronize(g_stream);
+        for (auto &p : g_pend) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, p.a, p.b);
+            auto &t = g_tot[p.name];
+            t.first += ms;
+            t.second += 1;
+            g_evfree.push_back(p.a);
+            g_evfree.push_back(p.b);
+        }
+        g_pend.clear();
+    }
+}
+static int fold_timing()
+{
+    HIPCHK(hipStreamSynchronize(g_stream));
+    for (auto &p : g_pend) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
+        auto &t = g_tot[p.name];
+        t.first += ms;
+        t.second += 1;
+        g_evfree.push_back(p.a);
+        g_evfree.push_back(p.b);
+    }
+    g_pend.clear();
+    return 0;
+}
+
+static int dev_alloc(void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (bytes == 0) bytes = 8;
+    HIPCHK(hipMalloc(p, bytes));
+    g_dom_allocs.push_back(*p);
+    return 0;
+}
+static void free_domain()
+{
+    for (void *p : g_dom_allocs) (void)hipFree(p);
+    g_dom_allocs.clear();
+    for (auto &s : g_scratch3) s = nullptr;
+    for (auto &s : g_scratch2) s = nullptr;
+    g_have_domain = false;
+}
+
+// ---------------------------------------------------------------------------
+// staging of host arrays (NEMO_HIP_PTR_HOST)
+// ---------------------------------------------------------------------------
+struct Stager {
+    struct A {
+        void *user, *dev;
+        size_t bytes;
+        bool out;
+    };
+    std::vector<A> args;
+    int status = 0;
+    size_t slot = 0;
+    bool host = (g_ptr_mode == NEMO_HIP_PTR_HOST);
+
+    void *get(const void *user, size_t bytes, bool in, bool out)
+    {
+        if (!user || !host || status) return (void *)user;
+        if (slot >= g_pool.size()) g_pool.push_back({nullptr, 0});
+        auto &pb = g_pool[slot++];
+        if (pb.second < bytes) {
+            if (pb.first) (void)hipFree(pb.first);
+            pb.first = nullptr;
+            pb.second = 0;
+            if (hipMalloc(&pb.first, bytes) != hipSuccess) {
+                status = set_error(NEMO_HIP_ERR_HIP, "staging allocation of %zu bytes failed", bytes);
+                return nullptr;
+            }
+            pb.second = bytes;
+        }
+        if (in && hipMemcpyAsync(pb.first, user, bytes, hipMemcpyHostToDevice, g_stream) != hipSuccess)
+            status = set_error(NEMO_HIP_ERR_HIP, "host to device copy failed");
+        args.push_back({(void *)user, pb.first, bytes, out});
+        return pb.first;
+    }
+    template <typename T> const T *in(const T *p, size_t n) { return (const T *)get(p, n * sizeof(T), true, false); }
+    template <typename T> T *inout(T *p, size_t n) { return (T *)get(p, n * sizeof(T), true, true); }
+    template <typename T> T *out(T *p, size_t n) { return (T *)get(p, n * sizeof(T), false, true); }
+
+    // copies outputs back (host mode); rc is the status of the launches
+    int finish(int rc)
+    {
+        if (status) return status;
+        if (rc) return rc;
+        if (!host) return 0;
+        for (auto &a : args)
+            if (a.out) HIPCHK(hipMemcpyAsync(a.user, a.dev, a.bytes, hipMemcpyDeviceToHost, g_stream));
+        HIPCHK(hipStreamSynchronize(g_stream));
+        return 0;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// domain registration
+// ---------------------------------------------------------------------------
+struct Ival {
+    int k0, k1;
+};
+static Ival inter(Ival a, Ival b)
+{
+    Ival r{a.k0 > b.k0 ? a.k0 : b.k0, a.k1 < b.k1 ? a.k1 : b.k1};
+    if (r.k1 < r.k0) r = Ival{1, 0};
+    return r;
+}
+
+template <typename V>
+static void lbc_host(std::vector<V> &a, int jpi, int jpj, int iper, int jper, char nat, V pval)
+{
+    auto at = [&](int i, int j) -> V & { return a[(size_t)(j - 1) * jpi + (i - 1)]; };
+    for (int j = 1; j <= jpj; j++) {
+        if (iper) {
+            V x = at(jpi - 1, j), y = at(2, j);
+            at(1, j) = x;
+            at(jpi, j) = y;
+        } else {
+            if (nat != 'F') at(1, j) = pval;
+            at(jpi, j) = pval;
+        }
+    }
+    for (int i = 1; i <= jpi; i++) {
+        if (jper) {
+            V x = at(i, jpj - 1), y = at(i, 2);
+            at(i, 1) = x;
+            at(i, jpj) = y;
+        } else {
+            if (nat != 'F') at(i, 1) = pval;
+            at(i, jpj) = pval;
+        }
+    }
+}
+
+template <typename T>
+static int upload(const T *src, size_t n, const T **dst)
+{
+    void *p;
+    int rc = dev_alloc(&p, n * sizeof(T));
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    *dst = (const T *)p;
+    return 0;
+}
+static int upload_f(const double *src, size_t n, const float **dst)
+{
+    std::vector<float> f(n);
+    for (size_t q = 0; q < n; q++) f[q] = (float)src[q];
+    return upload(f.data(), n, dst);
+}
+
+static int check_options(const nemo_hip_domain_t *h)
+{
+    if (h->jpi < 4 || h->jpj < 4 || h->jpk < 3) return set_error(NEMO_HIP_ERR_ARG, "domain too small: %d x %d x %d", h->jpi, h->jpj, h->jpk);
+    if (!(h->jperio == 0 || h->jperio == 1 || h->jperio == 2 || h->jperio == 7))
+        return set_error(NEMO_HIP_ERR_UNSUPPORTED, "jperio = %d (north folds are not implemented)", h->jperio);
+    if (!h->ln_linssh) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "ln_linssh = F (variable volume) is not implemented");
+    if (h->ln_isfcav) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "ln_isfcav = T is not implemented");
+    if (h->ln_zps) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "ln_zps = T (partial steps) is not implemented");
+    if (h->ln_sco) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "ln_sco = T is not implemented");
+    if (h->ln_zad_Aimp) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "ln_zad_Aimp = T is not implemented");
+    if (h->nvor_scheme < 0 || h->nvor_scheme > 5) return set_error(NEMO_HIP_ERR_ARG, "nvor_scheme = %d", h->nvor_scheme);
+    if (h->n_dynadv < 0 || h->n_dynadv > 2)
+        return set_error(NEMO_HIP_ERR_UNSUPPORTED, "n_dynadv = %d (UBS flux form is not implemented)", h->n_dynadv);
+    if (h->nn_dynkeg != 0 && h->nn_dynkeg != 1) return set_error(NEMO_HIP_ERR_ARG, "nn_dynkeg = %d", h->nn_dynkeg);
+    if (h->nn_een_e3f != 0 && h->nn_een_e3f != 1) return set_error(NEMO_HIP_ERR_ARG, "nn_een_e3f = %d", h->nn_een_e3f);
+    if (h->rn_bt_alpha != 0.0) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "rn_bt_alpha /= 0 is not implemented");
+    if (h->nn_baro < 1 || h->nn_baro > 300) return set_error(NEMO_HIP_ERR_ARG, "nn_baro = %d", h->nn_baro);
+    const void *req[] = {h->k_top, h->k_bot, h->e1t, h->e2t, h->e1u, h->e2u, h->e1v, h->e2v, h->e1e2t, h->r1_e1e2t,
+                         h->e1e2u, h->r1_e1e2u, h->e1e2v, h->r1_e1e2v, h->r1_e1e2f, h->e2_e1u, h->e1_e2v, h->r1_e1u,
+                         h->r1_e2u, h->r1_e1v, h->r1_e2v, h->ff_f, h->ff_t, h->e3t_n, h->e3u_n, h->e3v_n, h->e3w_n,
+                         h->e3f_n, h->e3uw_n, h->e3vw_n};
+    for (const void *p : req)
+        if (!p) return set_error(NEMO_HIP_ERR_ARG, "domain: a required array pointer is NULL");
+    return 0;
+}
+
+static int register_domain(const nemo_hip_domain_t *h)
+{
+    int rc = check_options(h);
+    if (rc) return rc;
+    free_domain();
+    g_hdom = *h;
+    DevDomain d;
+    memset(&d, 0, sizeof d);
+    const int jpi = h->jpi, jpj = h->jpj, jpk = h->jpk;
+    d.jpi = jpi; d.jpj = jpj; d.jpk = jpk;
+    d.iper = (h->jperio == 1 || h->jperio == 7);
+    d.jper = (h->jperio == 2 || h->jperio == 7);
+    d.n2 = (size_t)jpi * jpj;
+    d.n3 = d.n2 * jpk;
+    d.ln_traldf_lap = h->ln_traldf_lap; d.ln_traldf_msc = h->ln_traldf_msc; d.ln_zdfddm = h->ln_zdfddm;
+    d.n_dynadv = h->n_dynadv; d.nn_dynkeg = h->nn_dynkeg; d.nvor_scheme = h->nvor_scheme; d.nn_een_e3f = h->nn_een_e3f;
+    d.ln_drgimp = h->ln_drgimp; d.ln_dynspg_ts = h->ln_dynspg_ts; d.ln_bt_fw = h->ln_bt_fw; d.ln_bt_av = h->ln_bt_av;
+    d.nn_bt_flt = h->nn_bt_flt; d.nn_baro = h->nn_baro; d.neuler = h->neuler; d.nit000 = h->nit000;
+    d.rdt = h->rdt; d.rn_atfp = h->rn_atfp; d.rau0 = h->rau0; d.grav = h->grav; d.rn_scal_load = h->rn_scal_load;
+    const size_t n2 = d.n2, n3 = d.n3;
+    auto I2 = [&](int i, int j) { return (size_t)(j - 1) * jpi + (i - 1); };
+
+    // masks as level intervals: DOM/dommsk.F90:134-197 (tmask, lbc, u/v/f products, lbc)
+    std::vector<Ival> T(n2), U(n2, Ival{1, 0}), V(n2, Ival{1, 0}), F(n2, Ival{1, 0});
+    for (size_t q = 0; q < n2; q++) {
+        const int kt = h->k_top[q], kb = h->k_bot[q];
+        if (kt < 0 || kb > jpk || (kt != 0 && kb < kt && kb != 0))
+            return set_error(NEMO_HIP_ERR_ARG, "k_top/k_bot out of range at column %zu: %d %d", q, kt, kb);
+        Ival r = (kt != 0) ? Ival{kt, kb} : Ival{1, 0};
+        if (r.k0 < 1) r.k0 = 1;
+        if (r.k1 > jpk) r.k1 = jpk;
+        if (r.k1 < r.k0) r = Ival{1, 0};
+        T[q] = r;
+    }
+    lbc_host(T, jpi, jpj, d.iper, d.jper, 'T', Ival{1, 0});
+    for (int j = 1; j <= jpj - 1; j++)
+        for (int i = 1; i <= jpi - 1; i++) {
+            U[I2(i, j)] = inter(T[I2(i, j)], T[I2(i + 1, j)]);
+            V[I2(i, j)] = inter(T[I2(i, j)], T[I2(i, j + 1)]);
+            F[I2(i, j)] = inter(inter(T[I2(i, j)], T[I2(i + 1, j)]), inter(T[I2(i, j + 1)], T[I2(i + 1, j + 1)]));
+        }
+    lbc_host(U, jpi, jpj, d.iper, d.jper, 'U', Ival{1, 0});
+    lbc_host(V, jpi, jpj, d.iper, d.jper, 'V', Ival{1, 0});
+    lbc_host(F, jpi, jpj, d.iper, d.jper, 'F', Ival{1, 0});
+    auto up_ival = [&](const std::vector<Ival> &v, KRange &kr) -> int {
+        std::vector<int32_t> a(n2), b(n2);
+        for (size_t q = 0; q < n2; q++) { a[q] = v[q].k0; b[q] = v[q].k1; }
+        int r = upload(a.data(), n2, &kr.k0);
+        return r ? r : upload(b.data(), n2, &kr.k1);
+    };
+    if ((rc = up_ival(T, d.kT)) || (rc = up_ival(U, d.kU)) || (rc = up_ival(V, d.kV)) || (rc = up_ival(F, d.kF))) return rc;
+    {
+        std::vector<double> s(n2), su(n2), sv(n2);
+        for (size_t q = 0; q < n2; q++) {
+            s[q] = T[q].k1 >= T[q].k0 ? 1.0 : 0.0;
+            su[q] = U[q].k1 >= U[q].k0 ? 1.0 : 0.0;
+            sv[q] = V[q].k1 >= V[q].k0 ? 1.0 : 0.0;
+        }
+        if ((rc = upload(s.data(), n2, &d.ssmask)) || (rc = upload(su.data(), n2, &d.ssumask)) ||
+            (rc = upload(sv.data(), n2, &d.ssvmask)) || (rc = upload_f(s.data(), n2, &d.ssmask_f)) ||
+            (rc = upload_f(su.data(), n2, &d.ssumask_f)) || (rc = upload_f(sv.data(), n2, &d.ssvmask_f)))
+            return rc;
+    }
+    // k-indices, DOM/domzgr.F90:292-314
+    {
+        std::vector<int32_t> mikt(n2), mbkt(n2);
+        std::vector<double> zu(n2, 0.0), zv(n2, 0.0), zbu(n2, 0.0), zbv(n2, 0.0);
+        for (size_t q = 0; q < n2; q++) {
+            mikt[q] = h->k_top[q] > 1 ? h->k_top[q] : 1;
+            mbkt[q] = h->k_bot[q] > 1 ? h->k_bot[q] : 1;
+        }
+        for (int j = 1; j <= jpj - 1; j++)
+            for (int i = 1; i <= jpi - 1; i++) {
+                zu[I2(i, j)] = std::max(mikt[I2(i + 1, j)], mikt[I2(i, j)]);
+                zv[I2(i, j)] = std::max(mikt[I2(i, j + 1)], mikt[I2(i, j)]);
+                zbu[I2(i, j)] = std::min(mbkt[I2(i + 1, j)], mbkt[I2(i, j)]);
+                zbv[I2(i, j)] = std::min(mbkt[I2(i, j + 1)], mbkt[I2(i, j)]);
+            }
+        lbc_host(zu, jpi, jpj, d.iper, d.jper, 'U', 0.0);
+        lbc_host(zv, jpi, jpj, d.iper, d.jper, 'V', 0.0);
+        lbc_host(zbu, jpi, jpj, d.iper, d.jper, 'U', 0.0);
+        lbc_host(zbv, jpi, jpj, d.iper, d.jper, 'V', 0.0);
+        std::vector<int32_t> miku(n2), mikv(n2), mbku(n2), mbkv(n2);
+        for (size_t q = 0; q < n2; q++) {
+            miku[q] = std::max((int)std::lround(zu[q]), 1);
+            mikv[q] = std::max((int)std::lround(zv[q]), 1);
+            mbku[q] = std::max((int)std::lround(zbu[q]), 1);
+            mbkv[q] = std::max((int)std::lround(zbv[q]), 1);
+        }
+        if ((rc = upload(mikt.data(), n2, &d.mikt)) || (rc = upload(mbkt.data(), n2, &d.mbkt)) ||
+            (rc = upload(miku.data(), n2, &d.miku)) || (rc = upload(mikv.data(), n2, &d.mikv)) ||
+            (rc = upload(mbku.data(), n2, &d.mbku)) || (rc = upload(mbkv.data(), n2, &d.mbkv)))
+            return rc;
+    }
+    // horizontal metrics and the metric-term pre-computations of DYN/dynvor.F90:919-939
+    struct M2 {
+        const double *src;
+        const double **dd;
+        const float **df;
+    };
+    std::vector<double> di_e2v(n2, 0.0), dj_e1u(n2, 0.0), di_e2u(n2, 0.0), dj_e1v(n2, 0.0);
+    for (int j = 2; j <= jpj - 1; j++)
+        for (int i = 2; i <= jpi - 1; i++) {
+            di_e2u[I2(i, j)] = (h->e2u[I2(i, j)] - h->e2u[I2(i - 1, j)]) * 0.5;
+            dj_e1v[I2(i, j)] = (h->e1v[I2(i, j)] - h->e1v[I2(i, j - 1)]) * 0.5;
+        }
+    lbc_host(di_e2u, jpi, jpj, d.iper, d.jper, 'T', 0.0);
+    lbc_host(dj_e1v, jpi, jpj, d.iper, d.jper, 'T', 0.0);
+    for (int j = 1; j <= jpj - 1; j++)
+        for (int i = 1; i <= jpi - 1; i++) {
+            di_e2v[I2(i, j)] = (h->e2v[I2(i + 1, j)] - h->e2v[I2(i, j)]) * 0.5 * h->r1_e1e2f[I2(i, j)];
+            dj_e1u[I2(i, j)] = (h->e1u[I2(i, j + 1)] - h->e1u[I2(i, j)]) * 0.5 * h->r1_e1e2f[I2(i, j)];
+        }
+    lbc_host(di_e2v, jpi, jpj, d.iper, d.jper, 'F', 0.0);
+    lbc_host(dj_e1u, jpi, jpj, d.iper, d.jper, 'F', 0.0);
+    M2 m2[] = {{h->e1t, &d.md.e1t, &d.mf.e1t}, {h->e2t, &d.md.e2t, &d.mf.e2t}, {h->e1u, &d.md.e1u, &d.mf.e1u},
+               {h->e2u, &d.md.e2u, &d.mf.e2u}, {h->e1v, &d.md.e1v, &d.mf.e1v}, {h->e2v, &d.md.e2v, &d.mf.e2v},
+               {h->e1e2t, &d.md.e1e2t, &d.mf.e1e2t}, {h->r1_e1e2t, &d.md.r1_e1e2t, &d.mf.r1_e1e2t},
+               {h->e1e2u, &d.md.e1e2u, &d.mf.e1e2u}, {h->r1_e1e2u, &d.md.r1_e1e2u, &d.mf.r1_e1e2u},
+               {h->e1e2v, &d.md.e1e2v, &d.mf.e1e2v}, {h->r1_e1e2v, &d.md.r1_e1e2v, &d.mf.r1_e1e2v},
+               {h->r1_e1e2f, &d.md.r1_e1e2f, &d.mf.r1_e1e2f}, {h->e2_e1u, &d.md.e2_e1u, &d.mf.e2_e1u},
+               {h->e1_e2v, &d.md.e1_e2v, &d.mf.e1_e2v}, {h->r1_e1u, &d.md.r1_e1u, &d.mf.r1_e1u},
+               {h->r1_e2u, &d.md.r1_e2u, &d.mf.r1_e2u}, {h->r1_e1v, &d.md.r1_e1v, &d.mf.r1_e1v},
+               {h->r1_e2v, &d.md.r1_e2v, &d.mf.r1_e2v}, {h->ff_f, &d.md.ff_f, &d.mf.ff_f},
+               {h->ff_t, &d.md.ff_t, &d.mf.ff_t}, {di_e2v.data(), &d.md.di_e2v_2e1e2f, &d.mf.di_e2v_2e1e2f},
+               {dj_e1u.data(), &d.md.dj_e1u_2e1e2f, &d.mf.dj_e1u_2e1e2f}, {di_e2u.data(), &d.md.di_e2u_2, &d.mf.di_e2u_2},
+               {dj_e1v.data(), &d.md.dj_e1v_2, &d.mf.dj_e1v_2}};
+    for (auto &e : m2)
+        if ((rc = upload(e.src, n2, e.dd)) || (rc = upload_f(e.src, n2, e.df))) return rc;
+    // vertical scale factors: a jpk table when horizontally uniform, else 3D
+    const double *e3s[7] = {h->e3t_n, h->e3u_n, h->e3v_n, h->e3w_n, h->e3f_n, h->e3uw_n, h->e3vw_n};
+    bool uniform = true;
+    for (int a = 0; a < 7 && uniform; a++)
+        for (int k = 0; k < jpk && uniform; k++) {
+            const double v = e3s[a][(size_t)k * n2];
+            for (size_t q = 1; q < n2; q++)
+                if (e3s[a][(size_t)k * n2 + q] != v) { uniform = false; break; }
+        }
+    const double **e3d[7] = {&d.md.e3t, &d.md.e3u, &d.md.e3v, &d.md.e3w, &d.md.e3f, &d.md.e3uw, &d.md.e3vw};
+    const float **e3f[7] = {&d.mf.e3t, &d.mf.e3u, &d.mf.e3v, &d.mf.e3w, &d.mf.e3f, &d.mf.e3uw, &d.mf.e3vw};
+    if (uniform) {
+        d.e3_sk = 1;
+        d.e3_sij = 0;
+        for (int a = 0; a < 7; a++) {
+            std::vector<double> col(jpk);
+            for (int k = 0; k < jpk; k++) col[k] = e3s[a][(size_t)k * n2];
+            if ((rc = upload(col.data(), (size_t)jpk, e3d[a])) || (rc = upload_f(col.data(), (size_t)jpk, e3f[a]))) return rc;
+        }
+    } else {
+        d.e3_sk = n2;
+        d.e3_sij = 1;
+        for (int a = 0; a < 7; a++)
+            if ((rc = upload(e3s[a], n3, e3d[a])) || (rc = upload_f(e3s[a], n3, e3f[a]))) return rc;
+    }
+    // scratch
+    for (int s = 0; s < S_NSCRATCH3; s++)
+        if ((rc = dev_alloc(&g_scratch3[s], n3 * sizeof(double)))) return rc;
+    for (int s = 0; s < 32; s++)
+        if ((rc = dev_alloc((void **)&g_scratch2[s], n2 * sizeof(double)))) return rc;
+    for (int s = 0; s < S_NSCRATCH3; s++) HIPCHK(hipMemset(g_scratch3[s], 0, n3 * sizeof(double)));
+    for (int s = 0; s < 32; s++) HIPCHK(hipMemset(g_scratch2[s], 0, n2 * sizeof(double)));
+    HIPCHK(hipDeviceSynchronize());
+    g_dom = d;
+    g_have_domain = true;
+    g_r2dt_tra = g_r2dt_zdf = g_r2dt_dyn = 2.0 * h->rdt;
+    spg_ts_reset();
+    return 0;
+}
+
+static double r2dt_update(double &r2dt, int kt)
+{
+    // tra_adv / tra_zdf / dyn_zdf time step, TRA/traadv.F90:95-97
+    if (g_hdom.neuler == 0 && kt == g_hdom.nit000) r2dt = g_hdom.rdt;
+    else if (kt <= g_hdom.nit000 + 1) r2dt = 2.0 * g_hdom.rdt;
+    return r2dt;
+}
+}  // namespace nh
+
+using namespace nh;
+
+#define NEED_DOMAIN()                                                                       \
+    do {                                                                                    \
+        if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called"); \
+        if (!g_have_domain) return set_error(NEMO_HIP_ERR_STATE, "no domain registered");     \
+    } while (0)
+#define NEED_PTR(p)                                                                   \
+    do {                                                                              \
+        if (!(p)) return set_error(NEMO_HIP_ERR_ARG, "%s: argument %s is NULL", __func__, #p); \
+    } while (0)
+
+static int is_tra_type(const char *cdtype, int *is_tra)
+{
+    if (!cdtype) return set_error(NEMO_HIP_ERR_ARG, "cdtype is NULL");
+    if (!strncmp(cdtype, "TRA", 3)) *is_tra = 1;
+    else if (!strncmp(cdtype, "TRC", 3)) *is_tra = 0;
+    else return set_error(NEMO_HIP_ERR_ARG, "cdtype must be 'TRA' or 'TRC'");
+    return 0;
+}
+
+extern "C" {
+
+int nemo_hip_init(int device)
+{
+    if (g_init) {
+        if (device == g_device) return NEMO_HIP_OK;
+        return set_error(NEMO_HIP_ERR_STATE, "already initialised on device %d", g_device);
+    }
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_error(NEMO_HIP_ERR_ARG, "device %d out of range (%d devices)", device, n);
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipStreamCreate(&g_stream));
+    g_device = device;
+    g_init = true;
+    return NEMO_HIP_OK;
+}
+
+int nemo_hip_finalize(void)
+{
+    if (!g_init) return NEMO_HIP_OK;
+    (void)hipStreamSynchronize(g_stream);
+    free_domain();
+    for (auto &p : g_pool) if (p.first) (void)hipFree(p.first);
+    g_pool.clear();
+    for (auto &p : g_pend) { g_evfree.push_back(p.a); g_evfree.push_back(p.b); }
+    g_pend.clear();
+    for (auto e : g_evfree) (void)hipEventDestroy(e);
+    g_evfree.clear();
+    (void)hipStreamDestroy(g_stream);
+    g_stream = nullptr;
+    g_init = false;
+    g_device = -1;
+    return NEMO_HIP_OK;
+}
+
+int nemo_hip_last_error(char *buf, int len)
+{
+    if (!buf || len <= 0) return NEMO_HIP_ERR_ARG;
+    snprintf(buf, (size_t)len, "%s", g_err);
+    return NEMO_HIP_OK;
+}
+
+int nemo_hip_set_pointer_mode(int mode)
+{
+    if (mode != NEMO_HIP_PTR_HOST && mode != NEMO_HIP_PTR_DEVICE) return set_error(NEMO_HIP_ERR_ARG, "pointer mode %d", mode);
+    g_ptr_mode = mode;
+    return NEMO_HIP_OK;
+}
+
+int nemo_hip_sync(void)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "not initialised");
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return NEMO_HIP_OK;
+}
+
+void *nemo_hip_stream(void) { return (void *)g_stream; }
+
+int nemo_hip_domain_register(const nemo_hip_domain_t *dom)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    if (!dom) return set_error(NEMO_HIP_ERR_ARG, "domain is NULL");
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return register_domain(dom);
+}
+
+int nemo_hip_timing_enable(int on)
+{
+    g_timing = on != 0;
+    return NEMO_HIP_OK;
+}
+int nemo_hip_timing_reset(void)
+{
+    if (g_init) {
+        int rc = fold_timing();
+        if (rc) return rc;
+    }
+    g_tot.clear();
+    return NEMO_HIP_OK;
+}
+int nemo_hip_timing_get(const char *name, double *total_ms, int64_t *count)
+{
+    if (!name || !total_ms || !count) return set_error(NEMO_HIP_ERR_ARG, "timing_get: NULL argument");
+    if (g_init) {
+        int rc = fold_timing();
+        if (rc) return rc;
+    }
+    auto it = g_tot.find(name);
+    *total_ms = it == g_tot.end() ? 0.0 : it->second.first;
+    *count = it == g_tot.end() ? 0 : it->second.second;
+    return NEMO_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+int nemo_hip_dom_msk(double *tmask, double *umask, double *vmask, double *fmask, double *wmask, double *wumask,
+                     double *wvmask, double *ssmask, double *ssumask, double *ssvmask, int32_t *mikt, int32_t *mbkt,
+                     int32_t *miku, int32_t *mikv, int32_t *mbku, int32_t *mbkv)
+{
+    NEED_DOMAIN();
+    const DevDomain &d = g_dom;
+    Stager S;
+    double *m3[7] = {S.out(tmask, d.n3), S.out(umask, d.n3), S.out(vmask, d.n3), S.out(fmask, d.n3),
+                     S.out(wmask, d.n3), S.out(wumask, d.n3), S.out(wvmask, d.n3)};
+    double *m2[3] = {S.out(ssmask, d.n2), S.out(ssumask, d.n2), S.out(ssvmask, d.n2)};
+    int32_t *ki[6] = {S.out(mikt, d.n2), S.out(mbkt, d.n2), S.out(miku, d.n2),
+                      S.out(mikv, d.n2), S.out(mbku, d.n2), S.out(mbkv, d.n2)};
+    const int32_t *src[6] = {d.mikt, d.mbkt, d.miku, d.mikv, d.mbku, d.mbkv};
+    int rc = 0;
+    if (!S.status) rc = launch_dom_msk(m3[0], m3[1], m3[2], m3[3], m3[4], m3[5], m3[6], m2[0], m2[1], m2[2]);
+    for (int q = 0; q < 6 && !rc && !S.status; q++)
+        if (ki[q]) HIPCHK(hipMemcpyAsync(ki[q], src[q], d.n2 * sizeof(int32_t), hipMemcpyDeviceToDevice, g_stream));
+    return S.finish(rc);
+}
+
+int nemo_hip_lbc_lnk(double *ptab, int nk, char cd_nat, double psgn, double pval)
+{
+    (void)psgn;
+    NEED_DOMAIN();
+    NEED_PTR(ptab);
+    if (nk < 1) return set_error(NEMO_HIP_ERR_ARG, "lbc_lnk: nk = %d", nk);
+    if (!strchr("TUVWF", cd_nat)) return set_error(NEMO_HIP_ERR_ARG, "lbc_lnk: grid type '%c'", cd_nat);
+    Stager S;
+    double *p = S.inout(ptab, g_dom.n2 * (size_t)nk);
+    int rc = S.status ? S.status : launch_lbc<double>(p, nk, cd_nat, pval);
+    return S.finish(rc);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// tracers (templated bodies shared by the FP64 and FP32 entry points)
+// ---------------------------------------------------------------------------
+template <typename T>
+static int do_tra_adv_trp(const T *un, const T *vn, const T *wn, T *zun, T *zvn, T *zwn)
+{
+    NEED_DOMAIN();
+    NEED_PTR(un); NEED_PTR(vn); NEED_PTR(wn); NEED_PTR(zun); NEED_PTR(zvn); NEED_PTR(zwn);
+    const size_t n3 = g_dom.n3;
+    Stager S;
+    const T *a = S.in(un, n3), *b = S.in(vn, n3), *c = S.in(wn, n3);
+    T *x = S.out(zun, n3), *y = S.out(zvn, n3), *z = S.out(zwn, n3);
+    int rc = S.status ? S.status : launch_tra_adv_trp<T>(a, b, c, x, y, z);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_tra_adv_fct(int kt, int kit000, const char *cdtype, double p2dt, const T *pun, const T *pvn,
+                          const T *pwn, const T *ptb, const T *ptn, T *pta, int kjpt, int kn_fct_h, int kn_fct_v)
+{
+    (void)kt; (void)kit000;
+    NEED_DOMAIN();
+    int is_tra, rc = is_tra_type(cdtype, &is_tra);
+    if (rc) return rc;
+    NEED_PTR(pun); NEED_PTR(pvn); NEED_PTR(pwn); NEED_PTR(ptb); NEED_PTR(ptn); NEED_PTR(pta);
+    if (kjpt < 1) return set_error(NEMO_HIP_ERR_ARG, "kjpt = %d", kjpt);
+    if ((kn_fct_h != 2 && kn_fct_h != 4) || (kn_fct_v != 2 && kn_fct_v != 4))
+        return set_error(NEMO_HIP_ERR_UNSUPPORTED, "tra_adv_fct: kn_fct_h = %d, kn_fct_v = %d (2 or 4)", kn_fct_h, kn_fct_v);
+    if (!(p2dt > 0.0)) return set_error(NEMO_HIP_ERR_ARG, "p2dt = %g", p2dt);
+    const size_t n3 = g_dom.n3, n4 = n3 * (size_t)kjpt;
+    Stager S;
+    const T *u = S.in(pun, n3), *v = S.in(pvn, n3), *w = S.in(pwn, n3), *tb = S.in(ptb, n4), *tn = S.in(ptn, n4);
+    T *ta = S.inout(pta, n4);
+    rc = S.status ? S.status : launch_tra_adv_fct<T>(p2dt, u, v, w, tb, tn, ta, kjpt, kn_fct_h, kn_fct_v);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_tra_ldf_lap(int kt, int kit000, const char *cdtype, const T *pahu, const T *pahv, const T *ptb, T *pta,
+                          int kjpt, int kpass)
+{
+    (void)kt; (void)kit000;
+    NEED_DOMAIN();
+    int is_tra, rc = is_tra_type(cdtype, &is_tra);
+    if (rc) return rc;
+    NEED_PTR(pahu); NEED_PTR(pahv); NEED_PTR(ptb); NEED_PTR(pta);
+    if (kjpt < 1 || (kpass != 1 && kpass != 2)) return set_error(NEMO_HIP_ERR_ARG, "kjpt = %d, kpass = %d", kjpt, kpass);
+    const size_t n3 = g_dom.n3, n4 = n3 * (size_t)kjpt;
+    Stager S;
+    const T *a = S.in(pahu, n3), *b = S.in(pahv, n3), *tb = S.in(ptb, n4);
+    T *ta = S.inout(pta, n4);
+    rc = S.status ? S.status : launch_tra_ldf_lap<T>(a, b, tb, ta, kjpt, kpass);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_tra_ldf_iso(int kt, int kit000, const char *cdtype, const T *pahu, const T *pahv, const T *ptb,
+                          const T *ptbb, T *pta, int kjpt, int kpass, const T *uslp, const T *vslp, const T *wslpi,
+                          const T *wslpj, T *ah_wslp2, T *akz)
+{
+    NEED_DOMAIN();
+    int is_tra, rc = is_tra_type(cdtype, &is_tra);
+    if (rc) return rc;
+    NEED_PTR(pahu); NEED_PTR(pahv); NEED_PTR(ptb); NEED_PTR(pta); NEED_PTR(uslp); NEED_PTR(vslp);
+    NEED_PTR(wslpi); NEED_PTR(wslpj); NEED_PTR(ah_wslp2); NEED_PTR(akz);
+    if (kjpt < 1 || (kpass != 1 && kpass != 2)) return set_error(NEMO_HIP_ERR_ARG, "kjpt = %d, kpass = %d", kjpt, kpass);
+    if (kpass == 2 && !ptbb) return set_error(NEMO_HIP_ERR_ARG, "tra_ldf_iso: kpass = 2 needs ptbb");
+    if (!ptbb) ptbb = ptb;
+    // time step of traldf_iso.F90:131-133
+    const double z2dt = (g_hdom.neuler == 0 && kt == g_hdom.nit000) ? g_hdom.rdt : 2.0 * g_hdom.rdt;
+    const size_t n3 = g_dom.n3, n4 = n3 * (size_t)kjpt;
+    Stager S;
+    const T *a = S.in(pahu, n3), *b = S.in(pahv, n3), *tb = S.in(ptb, n4);
+    const T *tbb = (ptbb == ptb) ? tb : S.in(ptbb, n4);
+    T *ta = S.inout(pta, n4);
+    const T *us = S.in(uslp, n3), *vs = S.in(vslp, n3), *wi = S.in(wslpi, n3), *wj = S.in(wslpj, n3);
+    T *ah = S.inout(ah_wslp2, n3), *kz = S.inout(akz, n3);
+    rc = S.status ? S.status
+                  : launch_tra_ldf_iso<T>(kpass == 1 && kt == kit000, z2dt, a, b, tb, tbb, ta, kjpt, kpass, us, vs,
+                                          wi, wj, ah, kz);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_tra_zdf_imp(int kt, int kit000, const char *cdtype, double p2dt, const T *ptb, T *pta, int kjpt,
+                          const T *avt, const T *avs, int l_ldfslp, const T *ah_wslp2, const T *akz, int clamp_sal)
+{
+    (void)kt; (void)kit000;
+    NEED_DOMAIN();
+    int is_tra, rc = is_tra_type(cdtype, &is_tra);
+    if (rc) return rc;
+    NEED_PTR(ptb); NEED_PTR(pta);
+    if (kjpt < 1) return set_error(NEMO_HIP_ERR_ARG, "kjpt = %d", kjpt);
+    if (is_tra && !avt) return set_error(NEMO_HIP_ERR_ARG, "tra_zdf_imp: avt is NULL");
+    if ((!is_tra || g_dom.ln_zdfddm) && !avs) return set_error(NEMO_HIP_ERR_ARG, "tra_zdf_imp: avs is NULL");
+    const T *add_user = g_dom.ln_traldf_msc ? akz : ah_wslp2;
+    if (l_ldfslp && !add_user) return set_error(NEMO_HIP_ERR_ARG, "tra_zdf_imp: iso-neutral coefficient is NULL");
+    const size_t n3 = g_dom.n3, n4 = n3 * (size_t)kjpt;
+    Stager S;
+    const T *tb = S.in(ptb, n4);
+    T *ta = S.inout(pta, n4);
+    const T *t = S.in(avt, n3);
+    const T *s = (avs == avt) ? t : S.in(avs, n3);
+    const T *add = l_ldfslp ? S.in(add_user, n3) : nullptr;
+    rc = S.status ? S.status : launch_tra_zdf_imp<T>(p2dt, is_tra, tb, ta, kjpt, t, s, l_ldfslp, add, clamp_sal);
+    return S.finish(rc);
+}
+
+// ---------------------------------------------------------------------------
+// momentum
+// ---------------------------------------------------------------------------
+template <typename T>
+static int do_dyn_adv(int which, int kscheme, const T *un, const T *vn, const T *wn, T *ua, T *va)
+{
+    NEED_DOMAIN();
+    NEED_PTR(un); NEED_PTR(vn); NEED_PTR(ua); NEED_PTR(va);
+    if ((which & 2 || which == 4) && !wn) return set_error(NEMO_HIP_ERR_ARG, "wn is NULL");
+    if (kscheme != 0 && kscheme != 1) return set_error(NEMO_HIP_ERR_ARG, "kscheme = %d", kscheme);
+    const size_t n3 = g_dom.n3;
+    Stager S;
+    const T *u = S.in(un, n3), *v = S.in(vn, n3), *w = wn ? S.in(wn, n3) : nullptr;
+    T *a = S.inout(ua, n3), *b = S.inout(va, n3);
+    int rc = S.status ? S.status : launch_dyn_adv<T>(which, kscheme, u, v, w, a, b);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int dyn_adv_dispatch(const T *un, const T *vn, const T *wn, T *ua, T *va)
+{
+    NEED_DOMAIN();
+    switch (g_dom.n_dynadv) {
+    case NEMO_NP_LIN_DYN: return NEMO_HIP_OK;
+    case NEMO_NP_VEC_C2: return do_dyn_adv<T>(3, g_dom.nn_dynkeg, un, vn, wn, ua, va);
+    case NEMO_NP_FLX_C2: return do_dyn_adv<T>(4, 0, un, vn, wn, ua, va);
+    default: return set_error(NEMO_HIP_ERR_UNSUPPORTED, "dyn_adv: n_dynadv = %d", g_dom.n_dynadv);
+    }
+}
+
+template <typename T>
+static int do_dyn_vor(const T *un, const T *vn, T *ua, T *va)
+{
+    NEED_DOMAIN();
+    NEED_PTR(un); NEED_PTR(vn); NEED_PTR(ua); NEED_PTR(va);
+    const size_t n3 = g_dom.n3;
+    Stager S;
+    const T *u = S.in(un, n3), *v = S.in(vn, n3);
+    T *a = S.inout(ua, n3), *b = S.inout(va, n3);
+    int rc = S.status ? S.status : launch_dyn_vor<T>(u, v, a, b);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_dyn_zdf(int kt, const T *ub, const T *vb, T *ua, T *va, const T *avm, const double *rCdU_bot,
+                      const double *ua_b, const double *va_b, const double *utau, const double *utau_b,
+                      const double *vtau, const double *vtau_b)
+{
+    NEED_DOMAIN();
+    NEED_PTR(ub); NEED_PTR(vb); NEED_PTR(ua); NEED_PTR(va); NEED_PTR(avm); NEED_PTR(rCdU_bot);
+    NEED_PTR(utau); NEED_PTR(utau_b); NEED_PTR(vtau); NEED_PTR(vtau_b);
+    if (!g_dom.ln_drgimp) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "dyn_zdf: explicit drag (ln_drgimp = F) is not implemented");
+    if (g_dom.ln_dynspg_ts && (!ua_b || !va_b)) return set_error(NEMO_HIP_ERR_ARG, "dyn_zdf: ua_b/va_b are NULL");
+    const double r2dt = r2dt_update(g_r2dt_dyn, kt);
+    const size_t n3 = g_dom.n3, n2 = g_dom.n2;
+    Stager S;
+    const T *b1 = S.in(ub, n3), *b2 = S.in(vb, n3), *m = S.in(avm, n3);
+    T *a1 = S.inout(ua, n3), *a2 = S.inout(va, n3);
+    const double *cd = S.in(rCdU_bot, n2), *uab = S.in(ua_b, n2), *vab = S.in(va_b, n2), *t1 = S.in(utau, n2),
+                 *t2 = S.in(utau_b, n2), *t3 = S.in(vtau, n2), *t4 = S.in(vtau_b, n2);
+    int rc = S.status ? S.status : launch_dyn_zdf<T>(r2dt, b1, b2, a1, a2, m, cd, uab, vab, t1, t2, t3, t4);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_dyn_spg_ts(int kt, nemo_hip_spg_ts_t *st)
+{
+    NEED_DOMAIN();
+    if (!st) return set_error(NEMO_HIP_ERR_ARG, "dyn_spg_ts: state is NULL");
+    if (!g_dom.ln_dynspg_ts) return set_error(NEMO_HIP_ERR_STATE, "dyn_spg_ts called with ln_dynspg_ts = F");
+    const void *req[] = {st->ua, st->va, st->un, st->vn, st->un_b, st->vn_b, st->ub_b, st->vb_b, st->sshn, st->hu_n,
+                         st->hv_n, st->r1_hu_n, st->r1_hv_n, st->ht_n, st->rCdU_bot, st->utau, st->vtau, st->emp,
+                         st->ua_b, st->va_b, st->ssha, st->un_adv, st->vn_adv, st->ub2_b, st->vb2_b, st->un_bf,
+                         st->vn_bf};
+    for (const void *p : req)
+        if (!p) return set_error(NEMO_HIP_ERR_ARG, "dyn_spg_ts: a required field is NULL");
+    const size_t n3 = g_dom.n3, n2 = g_dom.n2;
+    Stager S;
+    nemo_hip_spg_ts_t D;
+    D.ua = S.inout((T *)st->ua, n3); D.va = S.inout((T *)st->va, n3);
+    D.un = S.inout((T *)st->un, n3); D.vn = S.inout((T *)st->vn, n3);
+    D.un_b = S.in(st->un_b, n2); D.vn_b = S.in(st->vn_b, n2); D.ub_b = S.in(st->ub_b, n2); D.vb_b = S.in(st->vb_b, n2);
+    D.sshn = S.in(st->sshn, n2); D.hu_n = S.in(st->hu_n, n2); D.hv_n = S.in(st->hv_n, n2);
+    D.r1_hu_n = S.in(st->r1_hu_n, n2); D.r1_hv_n = S.in(st->r1_hv_n, n2); D.ht_n = S.in(st->ht_n, n2);
+    D.rCdU_bot = S.in(st->rCdU_bot, n2); D.utau = S.in(st->utau, n2); D.vtau = S.in(st->vtau, n2);
+    D.emp = S.in(st->emp, n2); D.rnf = st->rnf ? S.in(st->rnf, n2) : nullptr;
+    D.ua_b = S.out(st->ua_b, n2); D.va_b = S.out(st->va_b, n2); D.ssha = S.out(st->ssha, n2);
+    D.un_adv = S.out(st->un_adv, n2); D.vn_adv = S.out(st->vn_adv, n2);
+    D.ub2_b = S.inout(st->ub2_b, n2); D.vb2_b = S.inout(st->vb2_b, n2);
+    D.un_bf = S.inout(st->un_bf, n2); D.vn_bf = S.inout(st->vn_bf, n2);
+    int rc = S.status ? S.status : launch_dyn_spg_ts<T>(kt, &D);
+    return S.finish(rc);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 entry points
+// ---------------------------------------------------------------------------
+extern "C" {
+int nemo_hip_tra_adv_trp(const double *un, const double *vn, const double *wn, double *zun, double *zvn, double *zwn)
+{
+    return do_tra_adv_trp<double>(un, vn, wn, zun, zvn, zwn);
+}
+
+int nemo_hip_tra_adv_fct(int kt, int kit000, const char *cdtype, double p2dt, const double *pun, const double *pvn,
+                         const double *pwn, const double *ptb, const double *ptn, double *pta, int kjpt, int kn_fct_h,
+                         int kn_fct_v)
+{
+    return do_tra_adv_fct<double>(kt, kit000, cdtype, p2dt, pun, pvn, pwn, ptb, ptn, pta, kjpt, kn_fct_h, kn_fct_v);
+}
+
+int nemo_hip_interp_4th_cpt(const double *pt_in, double *pt_out)
+{
+    NEED_DOMAIN();
+    NEED_PTR(pt_in); NEED_PTR(pt_out);
+    Stager S;
+    const double *a = S.in(pt_in, g_dom.n3);
+    double *b = S.inout(pt_out, g_dom.n3);
+    int rc = S.status ? S.status : launch_interp_4th_cpt(a, b);
+    return S.finish(rc);
+}
+
+int nemo_hip_tra_ldf_lap(int kt, int kit000, const char *cdtype, const double *pahu, const double *pahv,
+                         const double *pgu, const double *pgv, const double *pgui, const double *pgvi,
+                         const double *ptb, double *pta, int kjpt, int kpass)
+{
+    (void)pgu; (void)pgv; (void)pgui; (void)pgvi;   // partial-step gradients: ln_zps = F
+    return do_tra_ldf_lap<double>(kt, kit000, cdtype, pahu, pahv, ptb, pta, kjpt, kpass);
+}
+
+int nemo_hip_tra_ldf_iso(int kt, int kit000, const char *cdtype, const double *pahu, const double *pahv,
+                         const double *pgu, const double *pgv, const double *pgui, const double *pgvi,
+                         const double *ptb, const double *ptbb, double *pta, int kjpt, int kpass,
+                         const double *uslp, const double *vslp, const double *wslpi, const double *wslpj,
+                         double *ah_wslp2, double *akz)
+{
+    (void)pgu; (void)pgv; (void)pgui; (void)pgvi;
+    return do_tra_ldf_iso<double>(kt, kit000, cdtype, pahu, pahv, ptb, ptbb, pta, kjpt, kpass, uslp, vslp, wslpi,
+                                  wslpj, ah_wslp2, akz);
+}
+
+int nemo_hip_tra_zdf_imp(int kt, int kit000, const char *cdtype, double p2dt, const double *ptb, double *pta, int kjpt,
+                         const double *avt, const double *avs, int l_ldfslp, const double *ah_wslp2, const double *akz)
+{
+    return do_tra_zdf_imp<double>(kt, kit000, cdtype, p2dt, ptb, pta, kjpt, avt, avs, l_ldfslp, ah_wslp2, akz, 0);
+}
+
+int nemo_hip_tra_zdf(int kt, const double *tsb, double *tsa, const double *avt, const double *avs, int l_ldfslp,
+                     const double *ah_wslp2, const double *akz)
+{
+    NEED_DOMAIN();
+    const double r2dt = r2dt_update(g_r2dt_zdf, kt);
+    return do_tra_zdf_imp<double>(kt, g_hdom.nit000, "TRA", r2dt, tsb, tsa, 2, avt, avs, l_ldfslp, ah_wslp2, akz, 1);
+}
+
+int nemo_hip_dyn_keg(int kt, int kscheme, const double *un, const double *vn, double *ua, double *va)
+{
+    (void)kt;
+    return do_dyn_adv<double>(1, kscheme, un, vn, nullptr, ua, va);
+}
+int nemo_hip_dyn_zad(int kt, const double *un, const double *vn, const double *wn, double *ua, double *va)
+{
+    (void)kt;
+    return do_dyn_adv<double>(2, 0, un, vn, wn, ua, va);
+}
+int nemo_hip_dyn_adv_cen2(int kt, const double *un, const double *vn, const double *wn, double *ua, double *va)
+{
+    (void)kt;
+    return do_dyn_adv<double>(4, 0, un, vn, wn, ua, va);
+}
+int nemo_hip_dyn_adv(int kt, const double *un, const double *vn, const double *wn, double *ua, double *va)
+{
+    (void)kt;
+    return dyn_adv_dispatch<double>(un, vn, wn, ua, va);
+}
+int nemo_hip_dyn_vor(int kt, const double *un, const double *vn, double *ua, double *va)
+{
+    (void)kt;
+    return do_dyn_vor<double>(un, vn, ua, va);
+}
+int nemo_hip_dyn_zdf(int kt, const double *ub, const double *vb, double *ua, double *va, const double *avm,
+                     const double *rCdU_bot, const double *ua_b, const double *va_b, const double *utau,
+                     const double *utau_b, const double *vtau, const double *vtau_b)
+{
+    return do_dyn_zdf<double>(kt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau, utau_b, vtau, vtau_b);
+}
+int nemo_hip_dyn_spg_ts(int kt, nemo_hip_spg_ts_t *st) { return do_dyn_spg_ts<double>(kt, st); }
+
+int nemo_hip_ts_wgt(int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2)
+{
+    NEED_DOMAIN();
+    NEED_PTR(jpit); NEED_PTR(zwgt1); NEED_PTR(zwgt2);
+    return spg_ts_weights(g_dom, ll_av, ll_fw, jpit, zwgt1, zwgt2);
+}
+
+// ---------------------------------------------------------------------------
+// FP32-storage entry points
+// ---------------------------------------------------------------------------
+int nemo_hip_tra_adv_trp_sp(const float *un, const float *vn, const float *wn, float *zun, float *zvn, float *zwn)
+{
+    return do_tra_adv_trp<float>(un, vn, wn, zun, zvn, zwn);
+}
+int nemo_hip_tra_adv_fct_sp(int kt, int kit000, const char *cdtype, double p2dt, const float *pun, const float *pvn,
+                            const float *pwn, const float *ptb, const float *ptn, float *pta, int kjpt, int kn_fct_h,
+                            int kn_fct_v)
+{
+    return do_tra_adv_fct<float>(kt, kit000, cdtype, p2dt, pun, pvn, pwn, ptb, ptn, pta, kjpt, kn_fct_h, kn_fct_v);
+}
+int nemo_hip_tra_ldf_lap_sp(int kt, int kit000, const char *cdtype, const float *pahu, const float *pahv,
+                            const float *ptb, float *pta, int kjpt, int kpass)
+{
+    return do_tra_ldf_lap<float>(kt, kit000, cdtype, pahu, pahv, ptb, pta, kjpt, kpass);
+}
+int nemo_hip_tra_ldf_iso_sp(int kt, int kit000, const char *cdtype, const float *pahu, const float *pahv,
+                            const float *ptb, const float *ptbb, float *pta, int kjpt, int kpass, const float *uslp,
+                            const float *vslp, const float *wslpi, const float *wslpj, float *ah_wslp2, float *akz)
+{
+    return do_tra_ldf_iso<float>(kt, kit000, cdtype, pahu, pahv, ptb, ptbb, pta, kjpt, kpass, uslp, vslp, wslpi, wslpj,
+                                 ah_wslp2, akz);
+}
+int nemo_hip_tra_zdf_sp(int kt, const float *tsb, float *tsa, const float *avt, const float *avs, int l_ldfslp,
+                        const float *ah_wslp2, const float *akz)
+{
+    NEED_DOMAIN();
+    const double r2dt = r2dt_update(g_r2dt_zdf, kt);
+    return do_tra_zdf_imp<float>(kt, g_hdom.nit000, "TRA", r2dt, tsb, tsa, 2, avt, avs, l_ldfslp, ah_wslp2, akz, 1);
+}
+int nemo_hip_dyn_adv_sp(int kt, const float *un, const float *vn, const float *wn, float *ua, float *va)
+{
+    (void)kt;
+    return dyn_adv_dispatch<float>(un, vn, wn, ua, va);
+}
+int nemo_hip_dyn_vor_sp(int kt, const float *un, const float *vn, float *ua, float *va)
+{
+    (void)kt;
+    return do_dyn_vor<float>(un, vn, ua, va);
+}
+int nemo_hip_dyn_zdf_sp(int kt, const float *ub, const float *vb, float *ua, float *va, const float *avm,
+                        const double *rCdU_bot, const double *ua_b, const double *va_b, const double *utau,
+                        const double *utau_b, const double *vtau, const double *vtau_b)
+{
+    return do_dyn_zdf<float>(kt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau, utau_b, vtau, vtau_b);
+}
+int nemo_hip_dyn_spg_ts_sp(int kt, nemo_hip_spg_ts_t *st) { return do_dyn_spg_ts<float>(kt, st); }
+
+}  // extern "C"

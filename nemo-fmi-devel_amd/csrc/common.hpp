@@ -1,0 +1,184 @@
+// common.hpp -- shared device/host definitions of the gfx950 NEMO hot-path library.
+//
+// Layout: every field keeps NEMO's Fortran layout (i fastest, then j, then k,
+// then tracer), addressed with 1-based (i,j,k) exactly as the reference loop
+// nests are written, so each kernel body reads like the loop it replaces.
+//
+// Masks are not stored as 3D arrays.  Under full-step z-coordinates every
+// 3D mask of DOM/dommsk.F90 is, per column, the indicator of one interval of
+// levels (tmask: [k_top,k_bot]; umask: the intersection of the two adjacent
+// t-columns; ...), after the serial lbc_lnk of the ring.  The library keeps
+// one (k0,k1) pair of int32 per column and point type, so a mask costs 0 HBM
+// bytes per cell; the dom_msk entry point expands them to the reference's
+// REAL 0/1 arrays for the bit-exact check.
+//
+// Vertical scale factors are addressed through (base, stride_k, stride_ij):
+// a horizontally uniform z-coordinate registers a jpk-long table (stride_ij 0),
+// anything else the full 3D array (stride_ij 1).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+#include "../../include/nemo_hip.h"
+
+#define NH_MAXSTR 1024
+
+// ---------------------------------------------------------------------------
+// per-column level interval: mask(k) = (k0 <= k && k <= k1), 1-based
+// ---------------------------------------------------------------------------
+struct KRange {
+    const int32_t *k0, *k1;
+};
+
+template <typename T>
+struct Metrics {
+    // 2D (jpi,jpj)
+    const T *e1t, *e2t, *e1u, *e2u, *e1v, *e2v;
+    const T *e1e2t, *r1_e1e2t, *e1e2u, *r1_e1e2u, *e1e2v, *r1_e1e2v, *r1_e1e2f;
+    const T *e2_e1u, *e1_e2v, *r1_e1u, *r1_e2u, *r1_e1v, *r1_e2v;
+    const T *ff_f, *ff_t;
+    const T *di_e2v_2e1e2f, *dj_e1u_2e1e2f, *di_e2u_2, *dj_e1v_2;   // DYN/dynvor.F90:919-939
+    // vertical scale factors
+    const T *e3t, *e3u, *e3v, *e3w, *e3f, *e3uw, *e3vw;
+};
+
+struct DevDomain {
+    int jpi, jpj, jpk;
+    int iper, jper;                 // l_Iperio, l_Jperio
+    size_t n2, n3;
+    size_t e3_sk, e3_sij;           // strides of the e3 arrays
+    // options
+    int ln_traldf_lap, ln_traldf_msc, ln_zdfddm;
+    int n_dynadv, nn_dynkeg, nvor_scheme, nn_een_e3f;
+    int ln_drgimp, ln_dynspg_ts, ln_bt_fw, ln_bt_av, nn_bt_flt, nn_baro;
+    int neuler, nit000;
+    double rdt, rn_atfp, rau0, grav, rn_scal_load;
+    // masks as level intervals, DOM/dommsk.F90
+    KRange kT, kU, kV, kF;
+    const double *ssmask, *ssumask, *ssvmask;
+    const float *ssmask_f, *ssumask_f, *ssvmask_f;
+    // k-indices, DOM/domzgr.F90:292-314
+    const int32_t *mikt, *mbkt, *miku, *mikv, *mbku, *mbkv;
+    Metrics<double> md;
+    Metrics<float> mf;
+};
+
+template <typename T> __host__ __device__ inline const Metrics<T> &met(const DevDomain &d);
+template <> __host__ __device__ inline const Metrics<double> &met<double>(const DevDomain &d) { return d.md; }
+template <> __host__ __device__ inline const Metrics<float> &met<float>(const DevDomain &d) { return d.mf; }
+
+// ---------------------------------------------------------------------------
+// indexing (1-based, Fortran order)
+// ---------------------------------------------------------------------------
+#define IJ(i, j) ((size_t)((j) - 1) * (size_t)d.jpi + (size_t)((i) - 1))
+#define IJK(i, j, k) (((size_t)((k) - 1) * (size_t)d.jpj + (size_t)((j) - 1)) * (size_t)d.jpi + (size_t)((i) - 1))
+#define A2(p, i, j) (p)[IJ(i, j)]
+#define A3(p, i, j, k) (p)[IJK(i, j, k)]
+#define E3(p, ij, k) (p)[(size_t)((k) - 1) * d.e3_sk + (size_t)(ij) * d.e3_sij]
+
+template <typename T>
+__device__ __forceinline__ T kmask(const KRange &r, size_t ij, int k)
+{
+    return (k >= r.k0[ij] && k <= r.k1[ij]) ? T(1) : T(0);
+}
+// w-point masks: wmask(1) = tmask(1), wmask(k) = tmask(k) * tmask(k-1), DOM/dommsk.F90:182-190
+template <typename T>
+__device__ __forceinline__ T wkmask(const KRange &r, size_t ij, int k)
+{
+    int a = r.k0[ij], b = r.k1[ij];
+    bool in = (k >= a && k <= b) && (k == 1 || (k - 1 >= a && k - 1 <= b));
+    return in ? T(1) : T(0);
+}
+
+#define TMASK(T_, i, j, k) kmask<T_>(d.kT, IJ(i, j), k)
+#define UMASK(T_, i, j, k) kmask<T_>(d.kU, IJ(i, j), k)
+#define VMASK(T_, i, j, k) kmask<T_>(d.kV, IJ(i, j), k)
+#define WMASK(T_, i, j, k) wkmask<T_>(d.kT, IJ(i, j), k)
+#define WUMASK(T_, i, j, k) wkmask<T_>(d.kU, IJ(i, j), k)
+#define WVMASK(T_, i, j, k) wkmask<T_>(d.kV, IJ(i, j), k)
+
+// NEMO's SIGN with key_nosignedzero semantics, lib_fortran.F90:336-351
+template <typename T>
+__device__ __forceinline__ T nsign(T a, T b) { return b >= T(0) ? fabs(a) : -fabs(a); }
+template <typename T>
+__device__ __forceinline__ T fmax2(T a, T b) { return a > b ? a : b; }
+template <typename T>
+__device__ __forceinline__ T fmin2(T a, T b) { return a < b ? a : b; }
+
+// ---------------------------------------------------------------------------
+// serial lbc_lnk as an index map, LBC/lbc_lnk_generic.h90:80-103.
+// The value a field holds at (i,j) after lbc_lnk is either the closed
+// boundary value (returns false) or the value computed at (si,sj) before the
+// call.  The east-west fill runs first and the north-south fill then copies
+// whole rows, so a corner takes the E-W rule of its source row.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool lbc_src(const DevDomain &d, char nat, int i, int j, int &si, int &sj)
+{
+    si = i;
+    sj = j;
+    if (j == 1 || j == d.jpj) {
+        if (d.jper) sj = (j == 1) ? d.jpj - 1 : 2;
+        else if (!(nat == 'F' && j == 1)) return false;
+    }
+    if (i == 1 || i == d.jpi) {
+        if (d.iper) si = (i == 1) ? d.jpi - 1 : 2;
+        else if (!(nat == 'F' && i == 1)) return false;
+    }
+    return true;
+}
+
+__device__ __forceinline__ bool is_interior(const DevDomain &d, int i, int j)
+{
+    return i >= 2 && i <= d.jpi - 1 && j >= 2 && j <= d.jpj - 1;
+}
+
+// 2D launch geometry over (i,j) with an explicit index origin
+struct Grid2 {
+    dim3 grid, block;
+};
+inline Grid2 grid2(int ni, int nj, int bx = 64, int by = 4)
+{
+    Grid2 g;
+    g.block = dim3(bx, by, 1);
+    g.grid = dim3((unsigned)((ni + bx - 1) / bx), (unsigned)((nj + by - 1) / by), 1);
+    return g;
+}
+#define THREAD_IJ(i0, j0) \
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + (i0); \
+    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + (j0);
+
+// ---------------------------------------------------------------------------
+// host side: error state, stream, timing, launch wrapper
+// ---------------------------------------------------------------------------
+namespace nh {
+int set_error(int code, const char *fmt, ...);
+hipStream_t stream();
+const DevDomain &dom();
+bool have_domain();
+// per-routine timing: begin/end bracket kernel launches under a name
+void timing_begin(const char *name);
+void timing_end(const char *name);
+// library-owned scratch buffers (HBM), sized at domain registration
+enum Scratch {
+    S_ZWI, S_ZWX, S_ZWY, S_ZWZ, S_ZBETUP, S_ZBETDO, S_ZTW, S_ZWT, S_ZWT2, S_NSCRATCH3
+};
+void *scratch3(int which);     // n3 doubles each
+double *scratch2(int which);   // n2 doubles each (dyn_spg_ts work arrays)
+}  // namespace nh
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess)                                                                      \
+            return nh::set_error(4, "%s:%d: %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH(name, kern, g, ...)                                                \
+    do {                                                                          \
+        nh::timing_begin(name);                                                   \
+        hipLaunchKernelGGL(kern, (g).grid, (g).block, 0, nh::stream(), __VA_ARGS__); \
+        nh::timing_end(name);                                                     \
+        HIPCHK(hipGetLastError());                                                \
+    } while (0)

@@ -1,0 +1,32 @@
+// kernels.hpp -- host launchers of the device kernels (all enqueue on the
+// library stream and return a nemo_hip status code).
+#pragma once
+#include "common.hpp"
+
+int launch_dom_msk(double *tmask, double *umask, double *vmask, double *fmask, double *wmask, double *wumask,
+                   double *wvmask, double *ssmask, double *ssumask, double *ssvmask);
+template <typename T> int launch_lbc(T *p, int nk, char nat, T pval);
+
+template <typename T> int launch_tra_adv_trp(const T *un, const T *vn, const T *wn, T *zun, T *zvn, T *zwn);
+template <typename T>
+int launch_tra_adv_fct(double p2dt, const T *pun, const T *pvn, const T *pwn, const T *ptb, const T *ptn, T *pta,
+                       int kjpt, int kn_fct_h, int kn_fct_v);
+int launch_interp_4th_cpt(const double *tin, double *tout);
+template <typename T> int launch_tra_ldf_lap(const T *pahu, const T *pahv, const T *ptb, T *pta, int kjpt, int kpass);
+template <typename T>
+int launch_tra_ldf_iso(int first, double z2dt, const T *pahu, const T *pahv, const T *ptb, const T *ptbb, T *pta,
+                       int kjpt, int kpass, const T *uslp, const T *vslp, const T *wslpi, const T *wslpj, T *ah_wslp2,
+                       T *akz);
+template <typename T>
+int launch_tra_zdf_imp(double p2dt, int is_tra, const T *ptb, T *pta, int kjpt, const T *avt, const T *avs,
+                       int l_ldfslp, const T *add, int clamp_sal);
+
+template <typename T> int launch_dyn_adv(int which, int kscheme, const T *un, const T *vn, const T *wn, T *ua, T *va);
+template <typename T> int launch_dyn_vor(const T *un, const T *vn, T *ua, T *va);
+template <typename T>
+int launch_dyn_zdf(double r2dt, const T *ub, const T *vb, T *ua, T *va, const T *avm, const double *rCdU_bot,
+                   const double *ua_b, const double *va_b, const double *utau, const double *utau_b,
+                   const double *vtau, const double *vtau_b);
+int spg_ts_weights(const DevDomain &d, int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2);
+template <typename T> int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st);
+void spg_ts_reset();

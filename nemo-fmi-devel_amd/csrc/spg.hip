@@ -1,0 +1,555 @@
+// spg.hip -- split-explicit free surface, DYN/dynspg_ts.F90:121-1310, on gfx950.
+//
+// Configuration implemented: ln_linssh = T, ln_bt_fw = T, no wetting/drying,
+// tides, open boundaries, AGRIF, atmospheric pressure or Stokes drift
+// (every branch those switches select is absent from the path).
+//
+// Phase 1 (3D -> 2D) is one column kernel (vertical sums of the trends,
+// removal of their mean, bottom baroclinic velocity) plus one 2D kernel (the
+// barotropic Coriolis / drag / wind forcing).  Phase 2 runs the `icycle`
+// dependent sub-steps as three 2D kernels each:
+//   S1 extrapolated velocities and fluxes (pointwise) + un_adv sum,
+//   S2 after sea level (flux divergence) with its lbc fill, the AB3-AM4
+//      interpolated level and the ssha sum,
+//   S3 after velocities (Coriolis, drag, pressure gradient) with their lbc
+//      fill and the ua_b sum.
+// The before/now/after sub-step levels rotate through three buffers instead of
+// being copied.  All barotropic state and arithmetic is FP64 in both storage
+// forms.  Phase 3 is one 2D kernel and one column kernel.
+#include "common.hpp"
+#include "kernels.hpp"
+
+namespace {
+struct SpgWork {
+    double *zu_frc, *zv_frc, *zwxb, *zwyb, *zCdU_u, *zCdU_v, *zssh_frc, *zwz, *zhf, *ftne, *ftnw, *ftse, *ftsw;
+    double *ua_e, *va_e, *zwx, *zwy, *zsshp2_e;
+};
+}
+
+// barotropic Coriolis trend at u/v points, phase 1 (lines 390-443) and
+// phase 2 (lines 1000-1056); zwx/zwy are the fluxes, ue/ve the velocities
+// used by the t-point (ENT) scheme, h the t-point depth
+__device__ __forceinline__ void bt_coriolis(const DevDomain &d, int i, int j, const double *zwx, const double *zwy,
+                                            const double *zwz, const double *ftne, const double *ftnw,
+                                            const double *ftse, const double *ftsw, const double *ue,
+                                            const double *ve, const double *ht, double z1_hu, double z1_hv,
+                                            double &zu_trd, double &zv_trd)
+{
+    const Metrics<double> &m = d.md;
+    const double r1_4 = 0.25, r1_8 = 0.125, r1_12 = 1.0 / 12.0;
+    switch (d.nvor_scheme) {
+    case NEMO_NP_ENT: {
+        zu_trd = r1_4 * m.r1_e1e2u[IJ(i, j)] * z1_hu *
+                 (m.e1e2t[IJ(i + 1, j)] * A2(ht, i + 1, j) * m.ff_t[IJ(i + 1, j)] * (A2(ve, i + 1, j) + A2(ve, i + 1, j - 1)) +
+                  m.e1e2t[IJ(i, j)] * A2(ht, i, j) * m.ff_t[IJ(i, j)] * (A2(ve, i, j) + A2(ve, i, j - 1)));
+        zv_trd = -(r1_4 * m.r1_e1e2v[IJ(i, j)] * z1_hv *
+                   (m.e1e2t[IJ(i, j + 1)] * A2(ht, i, j + 1) * m.ff_t[IJ(i, j + 1)] * (A2(ue, i, j + 1) + A2(ue, i - 1, j + 1)) +
+                    m.e1e2t[IJ(i, j)] * A2(ht, i, j) * m.ff_t[IJ(i, j)] * (A2(ue, i, j) + A2(ue, i - 1, j))));
+        break;
+    }
+    case NEMO_NP_ENE:
+    case NEMO_NP_MIX: {
+        const double zy1 = (A2(zwy, i, j - 1) + A2(zwy, i + 1, j - 1)) * m.r1_e1u[IJ(i, j)];
+        const double zy2 = (A2(zwy, i, j) + A2(zwy, i + 1, j)) * m.r1_e1u[IJ(i, j)];
+        const double zx1 = (A2(zwx, i - 1, j) + A2(zwx, i - 1, j + 1)) * m.r1_e2v[IJ(i, j)];
+        const double zx2 = (A2(zwx, i, j) + A2(zwx, i, j + 1)) * m.r1_e2v[IJ(i, j)];
+        zu_trd = r1_4 * (A2(zwz, i, j - 1) * zy1 + A2(zwz, i, j) * zy2);
+        zv_trd = -(r1_4 * (A2(zwz, i - 1, j) * zx1 + A2(zwz, i, j) * zx2));
+        break;
+    }
+    case NEMO_NP_ENS: {
+        const double zy1 = r1_8 * (A2(zwy, i, j - 1) + A2(zwy, i + 1, j - 1) + A2(zwy, i, j) + A2(zwy, i + 1, j)) *
+                           m.r1_e1u[IJ(i, j)];
+        const double zx1 = -(r1_8 * (A2(zwx, i - 1, j) + A2(zwx, i - 1, j + 1) + A2(zwx, i, j) + A2(zwx, i, j + 1)) *
+                             m.r1_e2v[IJ(i, j)]);
+        zu_trd = zy1 * (A2(zwz, i, j - 1) + A2(zwz, i, j));
+        zv_trd = zx1 * (A2(zwz, i - 1, j) + A2(zwz, i, j));
+        break;
+    }
+    default: {   // EET, EEN
+        zu_trd = r1_12 * m.r1_e1u[IJ(i, j)] *
+                 (A2(ftne, i, j) * A2(zwy, i, j) + A2(ftnw, i + 1, j) * A2(zwy, i + 1, j) +
+                  A2(ftse, i, j) * A2(zwy, i, j - 1) + A2(ftsw, i + 1, j) * A2(zwy, i + 1, j - 1));
+        zv_trd = -(r1_12 * m.r1_e2v[IJ(i, j)] *
+                   (A2(ftsw, i, j + 1) * A2(zwx, i - 1, j + 1) + A2(ftse, i, j + 1) * A2(zwx, i, j + 1) +
+                    A2(ftnw, i, j) * A2(zwx, i - 1, j) + A2(ftne, i, j) * A2(zwx, i, j)));
+        break;
+    }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Coriolis pre-computations, lines 241-342 (recomputed every call: with a
+// linear free surface they are constant, so this equals the kt == nit000 set)
+// ---------------------------------------------------------------------------
+__global__ void k_spg_cor1(DevDomain d, const double *ht_n, SpgWork w)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const Metrics<double> &m = d.md;
+    const size_t ij = IJ(i, j);
+    switch (d.nvor_scheme) {
+    case NEMO_NP_EEN:
+        if (i <= d.jpi - 1 && j <= d.jpj - 1) {
+            double z;
+            const double s = A2(ht_n, i, j + 1) + A2(ht_n, i + 1, j + 1) + A2(ht_n, i, j) + A2(ht_n, i + 1, j);
+            if (d.nn_een_e3f == 0) {
+                z = s * 0.25;
+            } else {
+                z = s / fmax2(1.0, A2(d.ssmask, i, j + 1) + A2(d.ssmask, i + 1, j + 1) + A2(d.ssmask, i, j) +
+                                       A2(d.ssmask, i + 1, j));
+            }
+            if (z != 0.0) z = m.ff_f[ij] / z;
+            w.zhf[ij] = z;      // zwz before lbc_lnk
+        }
+        break;
+    case NEMO_NP_ENE:
+    case NEMO_NP_ENS:
+    case NEMO_NP_MIX:
+        if (j <= d.jpj - 1) {
+            // zhf = 0 * (1 - umask*umask) + sum_k e3f * umask(j) * umask(j+1)
+            double z = 0.0;
+            z = z * (1.0 - UMASK(double, i, j, 1) * UMASK(double, i, j + 1, 1));
+            for (int k = 1; k <= d.jpk - 1; k++)
+                z = z + E3(m.e3f, ij, k) * UMASK(double, i, j, k) * UMASK(double, i, j + 1, k);
+            w.zhf[ij] = z;
+        }
+        break;
+    default: break;
+    }
+}
+
+__global__ void k_spg_cor2(DevDomain d, const double *ht_n, SpgWork w)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const Metrics<double> &m = d.md;
+    const size_t ij = IJ(i, j);
+    int si, sj;
+    switch (d.nvor_scheme) {
+    case NEMO_NP_ENE:
+    case NEMO_NP_ENS:
+    case NEMO_NP_MIX: {
+        double zhf = lbc_src(d, 'F', i, j, si, sj) ? A2(w.zhf, si, sj) : 0.0;
+        double z = 0.0;
+        if (zhf != 0.0) z = 1.0 / zhf;
+        w.zwz[ij] = m.ff_f[ij] * z;
+        break;
+    }
+    case NEMO_NP_EEN:
+        w.zwz[ij] = lbc_src(d, 'F', i, j, si, sj) ? A2(w.zhf, si, sj) : 0.0;
+        break;
+    case NEMO_NP_EET:
+        if (i >= 2 && j >= 2) {
+            const double ss = d.ssmask[ij];
+            const double z1_ht = ss / (A2(ht_n, i, j) + 1.0 - ss);
+            const double *f = m.ff_f;
+            A2(w.ftne, i, j) = (A2(f, i - 1, j) + A2(f, i, j) + A2(f, i, j - 1)) * z1_ht;
+            A2(w.ftnw, i, j) = (A2(f, i - 1, j - 1) + A2(f, i - 1, j) + A2(f, i, j)) * z1_ht;
+            A2(w.ftse, i, j) = (A2(f, i, j) + A2(f, i, j - 1) + A2(f, i - 1, j - 1)) * z1_ht;
+            A2(w.ftsw, i, j) = (A2(f, i, j - 1) + A2(f, i - 1, j - 1) + A2(f, i - 1, j)) * z1_ht;
+        } else if (i == 1) {
+            A2(w.ftne, i, j) = A2(w.ftnw, i, j) = A2(w.ftse, i, j) = A2(w.ftsw, i, j) = 0.0;
+        }
+        break;
+    default: break;
+    }
+}
+
+__global__ void k_spg_cor3(DevDomain d, SpgWork w)
+{
+    // EEN triads from the lbc-filled zwz, lines 267-275
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    if (i == 1) {
+        A2(w.ftne, i, j) = A2(w.ftnw, i, j) = A2(w.ftse, i, j) = A2(w.ftsw, i, j) = 0.0;
+        return;
+    }
+    if (j < 2) return;
+    const double *z = w.zwz;
+    A2(w.ftne, i, j) = A2(z, i - 1, j) + A2(z, i, j) + A2(z, i, j - 1);
+    A2(w.ftnw, i, j) = A2(z, i - 1, j - 1) + A2(z, i - 1, j) + A2(z, i, j);
+    A2(w.ftse, i, j) = A2(z, i, j) + A2(z, i, j - 1) + A2(z, i - 1, j - 1);
+    A2(w.ftsw, i, j) = A2(z, i, j - 1) + A2(z, i - 1, j - 1) + A2(z, i - 1, j);
+}
+
+// ---------------------------------------------------------------------------
+// Phase 1a: vertical sums of the trends and their removal (lines 358-378);
+// bottom baroclinic velocities (lines 519-527)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_spg_p1a(DevDomain d, T *ua, T *va, const T *un, const T *vn, const double *r1_hu_n,
+                          const double *r1_hv_n, const double *un_b, const double *vn_b, SpgWork w)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const Metrics<T> &m = met<T>(d);
+    const size_t ij = IJ(i, j);
+    const int jpk = d.jpk;
+    double su = 0.0, sv = 0.0;
+    for (int k = 1; k <= jpk - 1; k++) {
+        const size_t p = IJK(i, j, k);
+        su = su + (double)E3(m.e3u, ij, k) * (double)ua[p] * UMASK(double, i, j, k);
+        sv = sv + (double)E3(m.e3v, ij, k) * (double)va[p] * VMASK(double, i, j, k);
+    }
+    su = su * r1_hu_n[ij];
+    sv = sv * r1_hv_n[ij];
+    w.zu_frc[ij] = su;
+    w.zv_frc[ij] = sv;
+    if (!is_interior(d, i, j)) return;
+    for (int k = 1; k <= jpk - 1; k++) {
+        const size_t p = IJK(i, j, k);
+        ua[p] = (T)((double)ua[p] - su * UMASK(double, i, j, k));
+        va[p] = (T)((double)va[p] - sv * VMASK(double, i, j, k));
+    }
+    w.zwxb[ij] = (double)A3(un, i, j, d.mbku[ij]) - un_b[ij];
+    w.zwyb[ij] = (double)A3(vn, i, j, d.mbkv[ij]) - vn_b[ij];
+}
+
+// Phase 1b: barotropic forcing (lines 386-639)
+__global__ void k_spg_p1b(DevDomain d, const double *un_b, const double *vn_b, const double *hu_n,
+                          const double *hv_n, const double *r1_hu_n, const double *r1_hv_n, const double *ht_n,
+                          const double *rCdU_bot, const double *utau, const double *vtau, const double *emp,
+                          const double *rnf, SpgWork w)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const Metrics<double> &m = d.md;
+    const size_t ij = IJ(i, j);
+    const double r1_rau0 = 1.0 / d.rau0, r1_2 = 0.5;
+    // zssh_frc = r1_rau0 * ( emp - rnf + fwfisf ), fwfisf = 0 without cavities
+    w.zssh_frc[ij] = r1_rau0 * (emp[ij] - (rnf ? rnf[ij] : 0.0) + 0.0);
+    if (!is_interior(d, i, j)) return;
+    w.zCdU_u[ij] = r1_2 * (A2(rCdU_bot, i + 1, j) + A2(rCdU_bot, i, j));
+    w.zCdU_v[ij] = r1_2 * (A2(rCdU_bot, i, j + 1) + A2(rCdU_bot, i, j));
+    // now fluxes zwx = un_b * hu_n * e2u at the 6 points the scheme reads
+    double zwx_[2][3], zwy_[3][2];   // zwx(i-1..i, j..j+1) / zwy(i..i+1, j-1..j)
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) {
+            const int p = i - 1 + a, q = j + b;
+            zwx_[a][b] = A2(un_b, p, q) * A2(hu_n, p, q) * m.e2u[IJ(p, q)];
+            const int p2 = i + a, q2 = j - 1 + b;
+            zwy_[a][b] = A2(vn_b, p2, q2) * A2(hv_n, p2, q2) * m.e1v[IJ(p2, q2)];
+        }
+    double zu_trd, zv_trd;
+    {
+        const double *zwz = w.zwz;
+        switch (d.nvor_scheme) {
+        case NEMO_NP_ENT: {
+            const double r1_4 = 0.25;
+            zu_trd = r1_4 * m.r1_e1e2u[ij] * r1_hu_n[ij] *
+                     (m.e1e2t[IJ(i + 1, j)] * A2(ht_n, i + 1, j) * m.ff_t[IJ(i + 1, j)] * (A2(vn_b, i + 1, j) + A2(vn_b, i + 1, j - 1)) +
+                      m.e1e2t[ij] * A2(ht_n, i, j) * m.ff_t[ij] * (A2(vn_b, i, j) + A2(vn_b, i, j - 1)));
+            zv_trd = -(r1_4 * m.r1_e1e2v[ij] * r1_hv_n[ij] *
+                       (m.e1e2t[IJ(i, j + 1)] * A2(ht_n, i, j + 1) * m.ff_t[IJ(i, j + 1)] * (A2(un_b, i, j + 1) + A2(un_b, i - 1, j + 1)) +
+                        m.e1e2t[ij] * A2(ht_n, i, j) * m.ff_t[ij] * (A2(un_b, i, j) + A2(un_b, i - 1, j))));
+            break;
+        }
+        case NEMO_NP_ENE:
+        case NEMO_NP_MIX: {
+            const double zy1 = (zwy_[0][0] + zwy_[1][0]) * m.r1_e1u[ij];
+            const double zy2 = (zwy_[0][1] + zwy_[1][1]) * m.r1_e1u[ij];
+            const double zx1 = (zwx_[0][0] + zwx_[0][1]) * m.r1_e2v[ij];
+            const double zx2 = (zwx_[1][0] + zwx_[1][1]) * m.r1_e2v[ij];
+            zu_trd = 0.25 * (A2(zwz, i, j - 1) * zy1 + A2(zwz, i, j) * zy2);
+            zv_trd = -(0.25 * (A2(zwz, i - 1, j) * zx1 + A2(zwz, i, j) * zx2));
+            break;
+        }
+        case NEMO_NP_ENS: {
+            const double zy1 = 0.125 * (zwy_[0][0] + zwy_[1][0] + zwy_[0][1] + zwy_[1][1]) * m.r1_e1u[ij];
+            const double zx1 = -(0.125 * (zwx_[0][0] + zwx_[0][1] + zwx_[1][0] + zwx_[1][1]) * m.r1_e2v[ij]);
+            zu_trd = zy1 * (A2(zwz, i, j - 1) + A2(zwz, i, j));
+            zv_trd = zx1 * (A2(zwz, i - 1, j) + A2(zwz, i, j));
+            break;
+        }
+        default: {
+            const double r1_12 = 1.0 / 12.0;
+            zu_trd = r1_12 * m.r1_e1u[ij] *
+                     (A2(w.ftne, i, j) * zwy_[0][1] + A2(w.ftnw, i + 1, j) * zwy_[1][1] + A2(w.ftse, i, j) * zwy_[0][0] +
+                      A2(w.ftsw, i + 1, j) * zwy_[1][0]);
+            zv_trd = -(r1_12 * m.r1_e2v[ij] *
+                       (A2(w.ftsw, i, j + 1) * zwx_[0][1] + A2(w.ftse, i, j + 1) * zwx_[1][1] +
+                        A2(w.ftnw, i, j) * zwx_[0][0] + A2(w.ftne, i, j) * zwx_[1][0]));
+            break;
+        }
+        }
+    }
+    double zu = w.zu_frc[ij], zv = w.zv_frc[ij];
+    zu = zu - zu_trd * d.ssumask[ij];
+    zv = zv - zv_trd * d.ssvmask[ij];
+    zu = zu + r1_hu_n[ij] * r1_2 * (A2(rCdU_bot, i + 1, j) + A2(rCdU_bot, i, j)) * w.zwxb[ij];
+    zv = zv + r1_hv_n[ij] * r1_2 * (A2(rCdU_bot, i, j + 1) + A2(rCdU_bot, i, j)) * w.zwyb[ij];
+    zu = zu + r1_rau0 * utau[ij] * r1_hu_n[ij];
+    zv = zv + r1_rau0 * vtau[ij] * r1_hv_n[ij];
+    w.zu_frc[ij] = zu;
+    w.zv_frc[ij] = zv;
+}
+
+// ---------------------------------------------------------------------------
+// Phase 2 sub-steps
+// ---------------------------------------------------------------------------
+__global__ void k_spg_s1(DevDomain d, double za1, double za2, double za3, double wgt2, const double *un_e,
+                         const double *ub_e, const double *ubb_e, const double *vn_e, const double *vb_e,
+                         const double *vbb_e, const double *hu_n, const double *hv_n, double *un_adv, double *vn_adv,
+                         SpgWork w)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const Metrics<double> &m = d.md;
+    const size_t ij = IJ(i, j);
+    const double ue = za1 * un_e[ij] + za2 * ub_e[ij] + za3 * ubb_e[ij];
+    const double ve = za1 * vn_e[ij] + za2 * vb_e[ij] + za3 * vbb_e[ij];
+    w.ua_e[ij] = ue;
+    w.va_e[ij] = ve;
+    const double zwx = m.e2u[ij] * ue * hu_n[ij];
+    const double zwy = m.e1v[ij] * ve * hv_n[ij];
+    w.zwx[ij] = zwx;
+    w.zwy[ij] = zwy;
+    un_adv[ij] = un_adv[ij] + wgt2 * zwx * m.r1_e2u[ij];
+    vn_adv[ij] = vn_adv[ij] + wgt2 * zwy * m.r1_e1v[ij];
+}
+
+__global__ void k_spg_s2(DevDomain d, double rdtbt, double za0, double za1, double za2, double za3, double wgt1,
+                         const double *sshn_e, const double *sshb_e, double *sshbb_to_a, double *ssha, SpgWork w)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const Metrics<double> &m = d.md;
+    const size_t ij = IJ(i, j);
+    int si, sj;
+    double ssha_e = 0.0;
+    if (lbc_src(d, 'T', i, j, si, sj)) {
+        const size_t s = IJ(si, sj);
+        const double zhdiv = (A2(w.zwx, si, sj) - A2(w.zwx, si - 1, sj) + A2(w.zwy, si, sj) - A2(w.zwy, si, sj - 1)) *
+                             m.r1_e1e2t[s];
+        ssha_e = (sshn_e[s] - rdtbt * (w.zssh_frc[s] + zhdiv)) * d.ssmask[s];
+    }
+    const double sshbb = sshbb_to_a[ij];
+    w.zsshp2_e[ij] = za0 * ssha_e + za1 * sshn_e[ij] + za2 * sshb_e[ij] + za3 * sshbb;
+    sshbb_to_a[ij] = ssha_e;
+    ssha[ij] = ssha[ij] + wgt1 * ssha_e;
+}
+
+__device__ __forceinline__ void spg_new_vel(const DevDomain &d, int i, int j, double rdtbt, const double *un_e,
+                                            const double *vn_e, const double *r1_hu_n, const double *r1_hv_n,
+                                            const double *hu_n, const double *hv_n, const double *ht_n,
+                                            const SpgWork &w, double &ua, double &va)
+{
+    const Metrics<double> &m = d.md;
+    const size_t ij = IJ(i, j);
+    double z1_hu = 0.0, z1_hv = 0.0;
+    if (d.nvor_scheme == NEMO_NP_ENT) {
+        const double su = d.ssumask[ij], sv = d.ssvmask[ij];
+        z1_hu = su / (hu_n[ij] + 1.0 - su);
+        z1_hv = sv / (hv_n[ij] + 1.0 - sv);
+    }
+    double zu_trd, zv_trd;
+    bt_coriolis(d, i, j, w.zwx, w.zwy, w.zwz, w.ftne, w.ftnw, w.ftse, w.ftsw, w.ua_e, w.va_e, ht_n, z1_hu, z1_hv,
+                zu_trd, zv_trd);
+    zu_trd = zu_trd + w.zCdU_u[ij] * un_e[ij] * r1_hu_n[ij];
+    zv_trd = zv_trd + w.zCdU_v[ij] * vn_e[ij] * r1_hv_n[ij];
+    const double zu_spg = -(d.grav * (A2(w.zsshp2_e, i + 1, j) - A2(w.zsshp2_e, i, j)) * m.r1_e1u[ij]);
+    const double zv_spg = -(d.grav * (A2(w.zsshp2_e, i, j + 1) - A2(w.zsshp2_e, i, j)) * m.r1_e2v[ij]);
+    const double zwx = (1.0 - d.rn_scal_load) * zu_spg;
+    const double zwy = (1.0 - d.rn_scal_load) * zv_spg;
+    ua = (un_e[ij] + rdtbt * (zwx + zu_trd + w.zu_frc[ij])) * d.ssumask[ij];
+    va = (vn_e[ij] + rdtbt * (zwy + zv_trd + w.zv_frc[ij])) * d.ssvmask[ij];
+}
+
+__global__ void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double *un_e, const double *vn_e,
+                         double *ubb_to_a, double *vbb_to_a, const double *r1_hu_n, const double *r1_hv_n,
+                         const double *hu_n, const double *hv_n, const double *ht_n, double *ua_b, double *va_b,
+                         SpgWork w)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const size_t ij = IJ(i, j);
+    int si, sj;
+    double ua = 0.0, va = 0.0, t;
+    if (lbc_src(d, 'U', i, j, si, sj)) spg_new_vel(d, si, sj, rdtbt, un_e, vn_e, r1_hu_n, r1_hv_n, hu_n, hv_n, ht_n, w, ua, t);
+    if (lbc_src(d, 'V', i, j, si, sj)) spg_new_vel(d, si, sj, rdtbt, un_e, vn_e, r1_hu_n, r1_hv_n, hu_n, hv_n, ht_n, w, t, va);
+    ubb_to_a[ij] = ua;
+    vbb_to_a[ij] = va;
+    ua_b[ij] = ua_b[ij] + wgt1 * ua;
+    va_b[ij] = va_b[ij] + wgt1 * va;
+}
+
+// ---------------------------------------------------------------------------
+// Phase 3, lines 1214-1269
+// ---------------------------------------------------------------------------
+__global__ void k_spg_p3a(DevDomain d, int first_euler, double *un_adv, double *vn_adv, double *ub2_b,
+                          double *vb2_b, double *un_bf, double *vn_bf)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const size_t ij = IJ(i, j);
+    const double r1_2 = 0.5, atfp = d.rn_atfp;
+    const double zwx = un_adv[ij], zwy = vn_adv[ij];
+    if (!first_euler) {
+        un_adv[ij] = r1_2 * (ub2_b[ij] + zwx - atfp * un_bf[ij]);
+        vn_adv[ij] = r1_2 * (vb2_b[ij] + zwy - atfp * vn_bf[ij]);
+        un_bf[ij] = atfp * un_bf[ij] + (zwx - ub2_b[ij]);
+        vn_bf[ij] = atfp * vn_bf[ij] + (zwy - vb2_b[ij]);
+    } else {
+        un_bf[ij] = 0.0;
+        vn_bf[ij] = 0.0;
+    }
+    ub2_b[ij] = zwx;
+    vb2_b[ij] = zwy;
+}
+
+template <typename T>
+__global__ void k_spg_p3b(DevDomain d, double r1_2dt_b, T *ua, T *va, T *un, T *vn, const double *ua_b,
+                          const double *va_b, const double *ub_b, const double *vb_b, const double *un_adv,
+                          const double *vn_adv, const double *r1_hu_n, const double *r1_hv_n, const double *un_b,
+                          const double *vn_b)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const size_t ij = IJ(i, j);
+    const double du = (ua_b[ij] - ub_b[ij]) * r1_2dt_b, dv = (va_b[ij] - vb_b[ij]) * r1_2dt_b;
+    const double cu = un_adv[ij] * r1_hu_n[ij], cv = vn_adv[ij] * r1_hv_n[ij];
+    const double ubn = un_b[ij], vbn = vn_b[ij];
+    for (int k = 1; k <= d.jpk - 1; k++) {
+        const size_t p = IJK(i, j, k);
+        ua[p] = (T)((double)ua[p] + du);
+        va[p] = (T)((double)va[p] + dv);
+        un[p] = (T)(((double)un[p] + cu - ubn) * UMASK(double, i, j, k));
+        vn[p] = (T)(((double)vn[p] + cv - vbn) * VMASK(double, i, j, k));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+namespace {
+// the *_e before / now levels: three rotating buffers per variable, kept
+// between calls (they carry over when ln_bt_av = F)
+struct SpgState {
+    int rot_u[3] = {0, 1, 2}, rot_s[3] = {0, 1, 2};   // [now, before, before-before]
+} g_spg;
+}
+
+int spg_ts_weights(const DevDomain &d, int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2)
+{
+    const int nn_baro = d.nn_baro;
+    for (int jn = 0; jn < 3 * nn_baro; jn++) zwgt1[jn] = zwgt2[jn] = 0.0;
+    const int jic = ll_fw ? nn_baro : 2 * nn_baro;
+    if (ll_av) {
+        if (d.nn_bt_flt == 0) {
+            zwgt1[jic - 1] = 1.0;
+            *jpit = jic;
+        } else if (d.nn_bt_flt == 1 || d.nn_bt_flt == 2) {
+            // float(): default REAL arithmetic, DYN/dynspg_ts.F90:1350 and 1359
+            const float lim = d.nn_bt_flt == 1 ? 0.5f : 1.0f;
+            for (int jn = 1; jn <= 3 * nn_baro; jn++) {
+                const float za1 = fabsf((float)(jn - jic)) / (float)nn_baro;
+                if ((double)za1 < (double)lim) {
+                    zwgt1[jn - 1] = 1.0;
+                    *jpit = jn;
+                }
+            }
+        } else {
+            return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "ts_wgt: nn_bt_flt = %d", d.nn_bt_flt);
+        }
+    } else {
+        zwgt1[jic - 1] = 1.0;
+        *jpit = jic;
+    }
+    for (int jn = 1; jn <= *jpit; jn++)
+        for (int ji = jn; ji <= *jpit; ji++) zwgt2[jn - 1] = zwgt2[jn - 1] + zwgt1[ji - 1];
+    double s1 = 0.0, s2 = 0.0;
+    for (int jn = 1; jn <= *jpit; jn++) {
+        s1 += zwgt1[jn - 1];
+        s2 += zwgt2[jn - 1];
+    }
+    const double za1 = 1.0 / s1, za2 = 1.0 / s2;
+    for (int jn = 1; jn <= *jpit; jn++) {
+        zwgt1[jn - 1] = zwgt1[jn - 1] * za1;
+        zwgt2[jn - 1] = zwgt2[jn - 1] * za2;
+    }
+    return 0;
+}
+
+template <typename T>
+int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st)
+{
+    const DevDomain &d = nh::dom();
+    if (!d.ln_bt_fw) return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "dyn_spg_ts: ln_bt_fw = F is not implemented");
+    T *ua = (T *)st->ua, *va = (T *)st->va, *un = (T *)st->un, *vn = (T *)st->vn;
+    SpgWork w;
+    double *s2[32];
+    for (int q = 0; q < 32; q++) s2[q] = nh::scratch2(q);
+    w.zu_frc = s2[0]; w.zv_frc = s2[1]; w.zwxb = s2[2]; w.zwyb = s2[3]; w.zCdU_u = s2[4]; w.zCdU_v = s2[5];
+    w.zssh_frc = s2[6]; w.zwz = s2[7]; w.zhf = s2[8]; w.ftne = s2[9]; w.ftnw = s2[10]; w.ftse = s2[11];
+    w.ftsw = s2[12]; w.ua_e = s2[13]; w.va_e = s2[14]; w.zwx = s2[15]; w.zwy = s2[16]; w.zsshp2_e = s2[17];
+    double *ube[3] = {s2[18], s2[19], s2[20]}, *vbe[3] = {s2[21], s2[22], s2[23]}, *sse[3] = {s2[24], s2[25], s2[26]};
+    const size_t b2 = d.n2 * sizeof(double);
+    hipStream_t s = nh::stream();
+
+    const double z2dt_bf = (kt == d.nit000 && d.neuler == 0) ? d.rdt : 2.0 * d.rdt;
+    const double r1_2dt_b = 1.0 / z2dt_bf;
+    const double rdtbt = d.rdt / (double)d.nn_baro;
+    int ll_init = d.ln_bt_av;
+    if (kt == d.nit000 && d.neuler == 0) ll_init = 1;
+    double wgt1[1024], wgt2[1024];
+    int icycle = 0;
+    if (3 * d.nn_baro > 1024) return nh::set_error(NEMO_HIP_ERR_ARG, "nn_baro too large");
+    int rc = spg_ts_weights(d, d.ln_bt_av, 1, &icycle, wgt1, wgt2);
+    if (rc) return rc;
+
+    Grid2 g = grid2(d.jpi, d.jpj);
+    LAUNCH("dyn_spg_ts", k_spg_cor1, g, d, st->ht_n, w);
+    LAUNCH("dyn_spg_ts", k_spg_cor2, g, d, st->ht_n, w);
+    if (d.nvor_scheme == NEMO_NP_EEN) LAUNCH("dyn_spg_ts", k_spg_cor3, g, d, w);
+    LAUNCH("dyn_spg_ts", k_spg_p1a<T>, g, d, ua, va, (const T *)un, (const T *)vn, st->r1_hu_n, st->r1_hv_n, st->un_b,
+           st->vn_b, w);
+    LAUNCH("dyn_spg_ts", k_spg_p1b, g, d, st->un_b, st->vn_b, st->hu_n, st->hv_n, st->r1_hu_n, st->r1_hv_n, st->ht_n,
+           st->rCdU_bot, st->utau, st->vtau, st->emp, st->rnf, w);
+
+    // phase 2 initialisation, lines 665-702
+    int *ru = g_spg.rot_u, *rs = g_spg.rot_s;
+    if (ll_init) {
+        for (int q = 1; q <= 2; q++) {
+            HIPCHK(hipMemsetAsync(ube[ru[q]], 0, b2, s));
+            HIPCHK(hipMemsetAsync(vbe[ru[q]], 0, b2, s));
+            HIPCHK(hipMemsetAsync(sse[rs[q]], 0, b2, s));
+        }
+    }
+    HIPCHK(hipMemcpyAsync(sse[rs[0]], st->sshn, b2, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(ube[ru[0]], st->un_b, b2, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemcpyAsync(vbe[ru[0]], st->vn_b, b2, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipMemsetAsync(st->ua_b, 0, b2, s));
+    HIPCHK(hipMemsetAsync(st->va_b, 0, b2, s));
+    HIPCHK(hipMemsetAsync(st->ssha, 0, b2, s));
+    HIPCHK(hipMemsetAsync(st->un_adv, 0, b2, s));
+    HIPCHK(hipMemsetAsync(st->vn_adv, 0, b2, s));
+
+    for (int jn = 1; jn <= icycle; jn++) {
+        double za0, za1, za2, za3;
+        if (jn < 3 && ll_init) { za1 = 1.0; za2 = 0.0; za3 = 0.0; }
+        else { za1 = 1.781105; za2 = -1.06221; za3 = 0.281105; }
+        LAUNCH("dyn_spg_ts", k_spg_s1, g, d, za1, za2, za3, wgt2[jn - 1], ube[ru[0]], ube[ru[1]], ube[ru[2]],
+               vbe[ru[0]], vbe[ru[1]], vbe[ru[2]], st->hu_n, st->hv_n, st->un_adv, st->vn_adv, w);
+        if (jn == 1 && ll_init) { za0 = 1.0; za1 = 0.0; za2 = 0.0; za3 = 0.0; }
+        else if (jn == 2 && ll_init) { za0 = 1.0833333333333; za1 = -0.1666666666666; za2 = 0.0833333333333; za3 = 0.0; }
+        else { za0 = 0.614; za1 = 0.285; za2 = 0.088; za3 = 0.013; }
+        LAUNCH("dyn_spg_ts", k_spg_s2, g, d, rdtbt, za0, za1, za2, za3, wgt1[jn - 1], sse[rs[0]], sse[rs[1]],
+               sse[rs[2]], st->ssha, w);
+        LAUNCH("dyn_spg_ts", k_spg_s3, g, d, rdtbt, wgt1[jn - 1], ube[ru[0]], vbe[ru[0]], ube[ru[2]], vbe[ru[2]],
+               st->r1_hu_n, st->r1_hv_n, st->hu_n, st->hv_n, st->ht_n, st->ua_b, st->va_b, w);
+        // swap: bb <- b, b <- n, n <- a (the after level was written over bb)
+        int t;
+        t = ru[2]; ru[2] = ru[1]; ru[1] = ru[0]; ru[0] = t;
+        t = rs[2]; rs[2] = rs[1]; rs[1] = rs[0]; rs[0] = t;
+    }
+    LAUNCH("dyn_spg_ts", k_spg_p3a, g, d, (kt == d.nit000 && d.neuler == 0) ? 1 : 0, st->un_adv, st->vn_adv, st->ub2_b,
+           st->vb2_b, st->un_bf, st->vn_bf);
+    LAUNCH("dyn_spg_ts", k_spg_p3b<T>, g, d, r1_2dt_b, ua, va, un, vn, st->ua_b, st->va_b, st->ub_b, st->vb_b,
+           st->un_adv, st->vn_adv, st->r1_hu_n, st->r1_hv_n, st->un_b, st->vn_b);
+    return 0;
+}
+template int launch_dyn_spg_ts<double>(int, const nemo_hip_spg_ts_t *);
+template int launch_dyn_spg_ts<float>(int, const nemo_hip_spg_ts_t *);
+
+void spg_ts_reset() { g_spg = SpgState(); }

@@ -1,0 +1,255 @@
+"""Host-side mirror of NEMO's hot-path routines over the gfx950 C-ABI.
+
+Every method keeps the reference routine's name and dummy-argument order
+(e.g. ``tra_adv_fct(kt, kit000, cdtype, p2dt, pun, pvn, pwn, ptb, ptn, pta,
+kjpt, kn_fct_h, kn_fct_v)``, TRA/traadv_fct.F90:54) followed by the module
+arrays the routine reads, and raises ``NemoHipError`` carrying the library's
+message where the reference would call ctl_stop.
+
+Arrays are either numpy arrays in Fortran order (host-pointer mode: the
+library stages them through HBM) or 1-D torch tensors on the GPU holding the
+same Fortran-ordered data (device mode: zero copy, asynchronous on the
+library's stream).  There is no CPU fallback: without libnemo_hip.so every
+call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .abi import (NEMO_HIP_PTR_DEVICE, NEMO_HIP_PTR_HOST, NP_EEN, NP_EET, NP_ENE, NP_ENS, NP_ENT,  # noqa: F401
+                  NP_MIX, NP_FLX_C2, NP_LIN_DYN, NP_VEC_C2)
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libnemo_hip.so")
+HEADER = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include", "nemo_hip.h")
+
+
+class NemoHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nemo_hip error {code}: {msg}")
+        self.code = code
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise NemoHipError(abi.NEMO_HIP_ERR_STATE,
+                           f"{path} is missing: build it with __graft_entry__.build() (no CPU fallback exists)")
+    return C.CDLL(path)
+
+
+def _is_torch(a):
+    return type(a).__module__.startswith("torch")
+
+
+class NemoHip:
+    """One process-wide handle on the library (the library itself is a singleton)."""
+
+    def __init__(self, device=0, mode=NEMO_HIP_PTR_DEVICE, path=LIB_PATH):
+        self.L = load_library(path)
+        self.L.nemo_hip_stream.restype = C.c_void_p
+        self._check(self.L.nemo_hip_init(C.c_int(device)))
+        self.set_pointer_mode(mode)
+        self.dom = None
+
+    # ------------------------------------------------------------------
+    def last_error(self):
+        buf = C.create_string_buffer(1024)
+        self.L.nemo_hip_last_error(buf, C.c_int(1024))
+        return buf.value.decode(errors="replace")
+
+    def _check(self, rc):
+        if rc != 0:
+            raise NemoHipError(rc, self.last_error())
+
+    def set_pointer_mode(self, mode):
+        self._check(self.L.nemo_hip_set_pointer_mode(C.c_int(mode)))
+        self.mode = mode
+
+    def sync(self):
+        self._check(self.L.nemo_hip_sync())
+
+    def stream(self):
+        return self.L.nemo_hip_stream()
+
+    def finalize(self):
+        self._check(self.L.nemo_hip_finalize())
+
+    def register(self, dom):
+        """dom: configs.Domain (host arrays); the library copies what it needs."""
+        self._dstruct = dom.struct()
+        self._check(self.L.nemo_hip_domain_register(C.byref(self._dstruct)))
+        self.dom = dom
+
+    # pointers ----------------------------------------------------------
+    def _p(self, a, dtype=np.float64):
+        if a is None:
+            return None
+        if _is_torch(a):
+            if self.mode != NEMO_HIP_PTR_DEVICE:
+                raise NemoHipError(abi.NEMO_HIP_ERR_ARG, "torch tensor passed in host-pointer mode")
+            if not a.is_cuda or not a.is_contiguous():
+                raise NemoHipError(abi.NEMO_HIP_ERR_ARG, "device tensors must be contiguous GPU tensors")
+            return C.c_void_p(a.data_ptr())
+        if self.mode != NEMO_HIP_PTR_HOST:
+            raise NemoHipError(abi.NEMO_HIP_ERR_ARG, "numpy array passed in device-pointer mode")
+        if a.dtype != dtype:
+            raise NemoHipError(abi.NEMO_HIP_ERR_ARG, f"expected {dtype}, got {a.dtype}")
+        if not (a.flags.f_contiguous or a.ndim == 1):
+            raise NemoHipError(abi.NEMO_HIP_ERR_ARG, "host arrays must be Fortran-contiguous")
+        return C.c_void_p(a.ctypes.data)
+
+    # timing ------------------------------------------------------------
+    def timing_enable(self, on=True):
+        self._check(self.L.nemo_hip_timing_enable(C.c_int(1 if on else 0)))
+
+    def timing_reset(self):
+        self._check(self.L.nemo_hip_timing_reset())
+
+    def timing_get(self, name):
+        ms, n = C.c_double(0), C.c_int64(0)
+        self._check(self.L.nemo_hip_timing_get(name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    # a2 / a19 ----------------------------------------------------------
+    def dom_msk(self, tmask=None, umask=None, vmask=None, fmask=None, wmask=None, wumask=None, wvmask=None,
+                ssmask=None, ssumask=None, ssvmask=None, mikt=None, mbkt=None, miku=None, mikv=None, mbku=None,
+                mbkv=None):
+        f = self._p
+        i = lambda a: self._p(a, np.int32)
+        self._check(self.L.nemo_hip_dom_msk(f(tmask), f(umask), f(vmask), f(fmask), f(wmask), f(wumask), f(wvmask),
+                                            f(ssmask), f(ssumask), f(ssvmask), i(mikt), i(mbkt), i(miku), i(mikv),
+                                            i(mbku), i(mbkv)))
+
+    def lbc_lnk(self, ptab, nk, cd_nat, psgn=1.0, pval=0.0):
+        self._check(self.L.nemo_hip_lbc_lnk(self._p(ptab), C.c_int(nk), C.c_char(cd_nat.encode()),
+                                            C.c_double(psgn), C.c_double(pval)))
+
+    # tracers -------------------------------------------------------------
+    def tra_adv_trp(self, un, vn, wn, zun, zvn, zwn, sp=False):
+        fn = self.L.nemo_hip_tra_adv_trp_sp if sp else self.L.nemo_hip_tra_adv_trp
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        self._check(fn(p(un), p(vn), p(wn), p(zun), p(zvn), p(zwn)))
+
+    def tra_adv_fct(self, kt, kit000, cdtype, p2dt, pun, pvn, pwn, ptb, ptn, pta, kjpt, kn_fct_h, kn_fct_v, sp=False):
+        fn = self.L.nemo_hip_tra_adv_fct_sp if sp else self.L.nemo_hip_tra_adv_fct
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        self._check(fn(C.c_int(kt), C.c_int(kit000), cdtype.encode(), C.c_double(p2dt), p(pun), p(pvn), p(pwn),
+                       p(ptb), p(ptn), p(pta), C.c_int(kjpt), C.c_int(kn_fct_h), C.c_int(kn_fct_v)))
+
+    def interp_4th_cpt(self, pt_in, pt_out):
+        self._check(self.L.nemo_hip_interp_4th_cpt(self._p(pt_in), self._p(pt_out)))
+
+    def tra_ldf_lap(self, kt, kit000, cdtype, pahu, pahv, ptb, pta, kjpt, kpass, sp=False):
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        if sp:
+            rc = self.L.nemo_hip_tra_ldf_lap_sp(C.c_int(kt), C.c_int(kit000), cdtype.encode(), p(pahu), p(pahv),
+                                                p(ptb), p(pta), C.c_int(kjpt), C.c_int(kpass))
+        else:
+            rc = self.L.nemo_hip_tra_ldf_lap(C.c_int(kt), C.c_int(kit000), cdtype.encode(), p(pahu), p(pahv), None,
+                                             None, None, None, p(ptb), p(pta), C.c_int(kjpt), C.c_int(kpass))
+        self._check(rc)
+
+    def tra_ldf_iso(self, kt, kit000, cdtype, pahu, pahv, ptb, ptbb, pta, kjpt, kpass, uslp, vslp, wslpi, wslpj,
+                    ah_wslp2, akz, sp=False):
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        if sp:
+            rc = self.L.nemo_hip_tra_ldf_iso_sp(C.c_int(kt), C.c_int(kit000), cdtype.encode(), p(pahu), p(pahv),
+                                                p(ptb), p(ptbb), p(pta), C.c_int(kjpt), C.c_int(kpass), p(uslp),
+                                                p(vslp), p(wslpi), p(wslpj), p(ah_wslp2), p(akz))
+        else:
+            rc = self.L.nemo_hip_tra_ldf_iso(C.c_int(kt), C.c_int(kit000), cdtype.encode(), p(pahu), p(pahv), None,
+                                             None, None, None, p(ptb), p(ptbb), p(pta), C.c_int(kjpt), C.c_int(kpass),
+                                             p(uslp), p(vslp), p(wslpi), p(wslpj), p(ah_wslp2), p(akz))
+        self._check(rc)
+
+    def tra_zdf_imp(self, kt, kit000, cdtype, p2dt, ptb, pta, kjpt, avt, avs, l_ldfslp=0, ah_wslp2=None, akz=None):
+        p = self._p
+        self._check(self.L.nemo_hip_tra_zdf_imp(C.c_int(kt), C.c_int(kit000), cdtype.encode(), C.c_double(p2dt),
+                                                p(ptb), p(pta), C.c_int(kjpt), p(avt), p(avs), C.c_int(l_ldfslp),
+                                                p(ah_wslp2), p(akz)))
+
+    def tra_zdf(self, kt, tsb, tsa, avt, avs, l_ldfslp=0, ah_wslp2=None, akz=None, sp=False):
+        fn = self.L.nemo_hip_tra_zdf_sp if sp else self.L.nemo_hip_tra_zdf
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        self._check(fn(C.c_int(kt), p(tsb), p(tsa), p(avt), p(avs), C.c_int(l_ldfslp), p(ah_wslp2), p(akz)))
+
+    # momentum ------------------------------------------------------------
+    def dyn_keg(self, kt, kscheme, un, vn, ua, va):
+        p = self._p
+        self._check(self.L.nemo_hip_dyn_keg(C.c_int(kt), C.c_int(kscheme), p(un), p(vn), p(ua), p(va)))
+
+    def dyn_zad(self, kt, un, vn, wn, ua, va):
+        p = self._p
+        self._check(self.L.nemo_hip_dyn_zad(C.c_int(kt), p(un), p(vn), p(wn), p(ua), p(va)))
+
+    def dyn_adv_cen2(self, kt, un, vn, wn, ua, va):
+        p = self._p
+        self._check(self.L.nemo_hip_dyn_adv_cen2(C.c_int(kt), p(un), p(vn), p(wn), p(ua), p(va)))
+
+    def dyn_adv(self, kt, un, vn, wn, ua, va, sp=False):
+        fn = self.L.nemo_hip_dyn_adv_sp if sp else self.L.nemo_hip_dyn_adv
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        self._check(fn(C.c_int(kt), p(un), p(vn), p(wn), p(ua), p(va)))
+
+    def dyn_vor(self, kt, un, vn, ua, va, sp=False):
+        fn = self.L.nemo_hip_dyn_vor_sp if sp else self.L.nemo_hip_dyn_vor
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        self._check(fn(C.c_int(kt), p(un), p(vn), p(ua), p(va)))
+
+    def dyn_zdf(self, kt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau, utau_b, vtau, vtau_b, sp=False):
+        fn = self.L.nemo_hip_dyn_zdf_sp if sp else self.L.nemo_hip_dyn_zdf
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        q = self._p
+        self._check(fn(C.c_int(kt), p(ub), p(vb), p(ua), p(va), p(avm), q(rCdU_bot), q(ua_b), q(va_b), q(utau),
+                       q(utau_b), q(vtau), q(vtau_b)))
+
+    def ts_wgt(self, ll_av, ll_fw):
+        n = 3 * self.dom.opts["nn_baro"]
+        w1, w2 = np.zeros(n), np.zeros(n)
+        jpit = C.c_int(0)
+        self._check(self.L.nemo_hip_ts_wgt(C.c_int(ll_av), C.c_int(ll_fw), C.byref(jpit),
+                                           w1.ctypes.data_as(C.c_void_p), w2.ctypes.data_as(C.c_void_p)))
+        return jpit.value, w1, w2
+
+    def dyn_spg_ts(self, kt, fields, sp=False):
+        """fields: dict named as nemo_hip_spg_ts_t (3D: ua, va, un, vn; 2D FP64 others)."""
+        s = abi.SpgTsStruct()
+        for n, _ in abi.SpgTsStruct._fields_:
+            a = fields.get(n)
+            if a is None:
+                setattr(s, n, None)
+                continue
+            dt = np.float32 if (sp and n in abi.SPG_3D_FIELDS) else np.float64
+            setattr(s, n, self._p(a, dt).value)
+        fn = self.L.nemo_hip_dyn_spg_ts_sp if sp else self.L.nemo_hip_dyn_spg_ts
+        self._check(fn(C.c_int(kt), C.byref(s)))
+
+
+_handle = None
+
+
+def get(device=0, mode=NEMO_HIP_PTR_DEVICE):
+    """The process-wide handle (created on first use)."""
+    global _handle
+    if _handle is None:
+        _handle = NemoHip(device, mode)
+    else:
+        _handle.set_pointer_mode(mode)
+    return _handle
+
+
+def exported_symbols(header=HEADER):
+    """Names of the entry points declared in include/nemo_hip.h."""
+    import re
+    txt = open(header).read()
+    return sorted(set(re.findall(r"\b(?:int|void\s*\*)\s+(nemo_hip_\w+)\s*\(", txt)))

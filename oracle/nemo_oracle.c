@@ -1,0 +1,1736 @@
+/*
+ * nemo_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement, in plain C, of the NEMO 4.0 ocean time-step hot path
+ * (reference tree fmidev/nemo-fmi-devel, paths relative to src/OCE).  It is
+ * the checker the HIP library is compared against and the CPU baseline of the
+ * benchmark; no product code links it.
+ *
+ * Rules followed so that the arithmetic is the reference's:
+ *   - every loop nest keeps the reference's index ranges, loop order and
+ *     temporaries (Fortran column-major, 1-based indexing through A2/A3/A4);
+ *   - every expression keeps Fortran's evaluation order (left to right for
+ *     operators of equal precedence, parentheses honoured, unary minus applied
+ *     to the whole product), and the file is compiled with -ffp-contract=off
+ *     so no multiply-add is fused;
+ *   - SIGN(a,b) is NEMO's (b >= 0 ? |a| : -|a|), lib_fortran.F90:336-351,
+ *     the semantics nemogcm.F90:599 requires;
+ *   - automatic arrays the reference leaves undefined outside the loops that
+ *     fill them start at zero here (calloc); the reference never uses those
+ *     values in a result, except where noted.
+ *
+ * Options: the routines implement the switches named in nemo_hip.h
+ * (ln_linssh=T, ln_isfcav=F, ln_zps=F, ln_sco=F, ln_zad_Aimp=F, serial lbc
+ * with jperio 0/1/2/7, no wetting/drying, tides, BDY or AGRIF).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/nemo_hip.h"
+
+/* ------------------------------------------------------------------------ */
+static nemo_hip_domain_t D;               /* registered domain (caller owns arrays) */
+static int JPI, JPJ, JPK;
+static double *tmask, *umask, *vmask, *fmask, *wmask, *wumask, *wvmask;
+static double *ssmask, *ssumask, *ssvmask;
+static int *mikt, *mbkt, *miku, *mikv, *mbku, *mbkv, *mikf;
+/* F-point metric term pre-computations, DYN/dynvor.F90:919-939 */
+static double *di_e2v_2e1e2f, *dj_e1u_2e1e2f, *di_e2u_2, *dj_e1v_2;
+/* tra_adv / tra_zdf / dyn_zdf module time step r2dt (persists between calls) */
+static double r2dt_tra, r2dt_zdf, r2dt_dyn;
+
+#define A2(a, i, j) (a)[((size_t)(j)-1) * JPI + (size_t)(i)-1]
+#define A3(a, i, j, k) (a)[(((size_t)(k)-1) * JPJ + (size_t)(j)-1) * JPI + (size_t)(i)-1]
+#define A4(a, i, j, k, n) (a)[((((size_t)(n)-1) * JPK + (size_t)(k)-1) * JPJ + (size_t)(j)-1) * JPI + (size_t)(i)-1]
+
+/* read before the Fortran-name macros below shadow the field names */
+static void dom_sizes(const nemo_hip_domain_t *d) { JPI = d->jpi; JPJ = d->jpj; JPK = d->jpk; }
+
+#define jpi JPI
+#define jpj JPJ
+#define jpk JPK
+#define jpim1 (JPI - 1)
+#define jpjm1 (JPJ - 1)
+#define jpkm1 (JPK - 1)
+
+static double MAX2(double a, double b) { return a > b ? a : b; }
+static double MIN2(double a, double b) { return a < b ? a : b; }
+static double MAX3(double a, double b, double c) { return MAX2(MAX2(a, b), c); }
+static double MIN3(double a, double b, double c) { return MIN2(MIN2(a, b), c); }
+static double SIGN(double a, double b) { return b >= 0.0 ? fabs(a) : -fabs(a); }
+static int IMAX(int a, int b) { return a > b ? a : b; }
+static int IMIN(int a, int b) { return a < b ? a : b; }
+
+static size_t n2(void) { return (size_t)JPI * JPJ; }
+static size_t n3(void) { return (size_t)JPI * JPJ * JPK; }
+static double *z3(void) { return (double *)calloc(n3(), sizeof(double)); }
+static double *z2(void) { return (double *)calloc(n2(), sizeof(double)); }
+
+/* metric shorthands */
+#define e1t(i, j) A2(D.e1t, i, j)
+#define e2t(i, j) A2(D.e2t, i, j)
+#define e1u(i, j) A2(D.e1u, i, j)
+#define e2u(i, j) A2(D.e2u, i, j)
+#define e1v(i, j) A2(D.e1v, i, j)
+#define e2v(i, j) A2(D.e2v, i, j)
+#define e1e2t(i, j) A2(D.e1e2t, i, j)
+#define r1_e1e2t(i, j) A2(D.r1_e1e2t, i, j)
+#define e1e2u(i, j) A2(D.e1e2u, i, j)
+#define r1_e1e2u(i, j) A2(D.r1_e1e2u, i, j)
+#define e1e2v(i, j) A2(D.e1e2v, i, j)
+#define r1_e1e2v(i, j) A2(D.r1_e1e2v, i, j)
+#define r1_e1e2f(i, j) A2(D.r1_e1e2f, i, j)
+#define e2_e1u(i, j) A2(D.e2_e1u, i, j)
+#define e1_e2v(i, j) A2(D.e1_e2v, i, j)
+#define r1_e1u(i, j) A2(D.r1_e1u, i, j)
+#define r1_e2u(i, j) A2(D.r1_e2u, i, j)
+#define r1_e1v(i, j) A2(D.r1_e1v, i, j)
+#define r1_e2v(i, j) A2(D.r1_e2v, i, j)
+#define ff_f(i, j) A2(D.ff_f, i, j)
+#define ff_t(i, j) A2(D.ff_t, i, j)
+#define e3t_n(i, j, k) A3(D.e3t_n, i, j, k)
+#define e3u_n(i, j, k) A3(D.e3u_n, i, j, k)
+#define e3v_n(i, j, k) A3(D.e3v_n, i, j, k)
+#define e3w_n(i, j, k) A3(D.e3w_n, i, j, k)
+#define e3f_n(i, j, k) A3(D.e3f_n, i, j, k)
+#define e3uw_n(i, j, k) A3(D.e3uw_n, i, j, k)
+#define e3vw_n(i, j, k) A3(D.e3vw_n, i, j, k)
+/* linear free surface: before / after scale factors equal the now ones */
+#define e3t_b(i, j, k) e3t_n(i, j, k)
+#define e3t_a(i, j, k) e3t_n(i, j, k)
+
+/* ------------------------------------------------------------------------ */
+/* lbc_lnk, serial path: LBC/lbc_lnk_generic.h90:80-103                    */
+/* ------------------------------------------------------------------------ */
+static int l_Iperio(void) { return D.jperio == 1 || D.jperio == 4 || D.jperio == 6 || D.jperio == 7; }
+static int l_Jperio(void) { return D.jperio == 2 || D.jperio == 7; }
+
+static void lbc3(double *p, int nk, char nat, double pval)
+{
+    int i, j, k;
+    for (k = 1; k <= nk; k++) {
+        if (l_Iperio()) {
+            for (j = 1; j <= jpj; j++) {
+                A3(p, 1, j, k) = A3(p, jpim1, j, k);
+                A3(p, jpi, j, k) = A3(p, 2, j, k);
+            }
+        } else {
+            if (nat != 'F')
+                for (j = 1; j <= jpj; j++) A3(p, 1, j, k) = pval;
+            for (j = 1; j <= jpj; j++) A3(p, jpi, j, k) = pval;
+        }
+        if (l_Jperio()) {
+            for (i = 1; i <= jpi; i++) {
+                A3(p, i, 1, k) = A3(p, i, jpjm1, k);
+                A3(p, i, jpj, k) = A3(p, i, 2, k);
+            }
+        } else {
+            if (nat != 'F')
+                for (i = 1; i <= jpi; i++) A3(p, i, 1, k) = pval;
+            for (i = 1; i <= jpi; i++) A3(p, i, jpj, k) = pval;
+        }
+    }
+}
+
+int ora_lbc_lnk(double *ptab, int nk, char cd_nat, double psgn, double pval)
+{
+    (void)psgn;
+    lbc3(ptab, nk, cd_nat, pval);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* domain: masks DOM/dommsk.F90:134-210, k-indices DOM/domzgr.F90:292-314,   */
+/* metric terms DYN/dynvor.F90:919-939                                       */
+/* ------------------------------------------------------------------------ */
+static void free_domain(void)
+{
+    double **d[] = {&tmask, &umask, &vmask, &fmask, &wmask, &wumask, &wvmask, &ssmask, &ssumask, &ssvmask,
+                    &di_e2v_2e1e2f, &dj_e1u_2e1e2f, &di_e2u_2, &dj_e1v_2};
+    int **ii[] = {&mikt, &mbkt, &miku, &mikv, &mbku, &mbkv, &mikf};
+    for (size_t q = 0; q < sizeof d / sizeof d[0]; q++) { free(*d[q]); *d[q] = NULL; }
+    for (size_t q = 0; q < sizeof ii / sizeof ii[0]; q++) { free(*ii[q]); *ii[q] = NULL; }
+}
+
+int ora_domain_register(const nemo_hip_domain_t *dom)
+{
+    int i, j, k;
+    free_domain();
+    D = *dom;
+    dom_sizes(dom);
+    tmask = z3(); umask = z3(); vmask = z3(); fmask = z3();
+    wmask = z3(); wumask = z3(); wvmask = z3();
+    ssmask = z2(); ssumask = z2(); ssvmask = z2();
+    /* tmask from k_top / k_bot, dommsk.F90:134-144 */
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) {
+            int iktop = A2(D.k_top, i, j), ikbot = A2(D.k_bot, i, j);
+            if (iktop != 0)
+                for (k = iktop; k <= ikbot; k++) A3(tmask, i, j, k) = 1.0;
+        }
+    lbc3(tmask, jpk, 'T', 0.0);
+    /* u-, v-, f-masks, dommsk.F90:167-178 */
+    for (k = 1; k <= jpk; k++)
+        for (j = 1; j <= jpjm1; j++) {
+            for (i = 1; i <= jpim1; i++) {
+                A3(umask, i, j, k) = A3(tmask, i, j, k) * A3(tmask, i + 1, j, k);
+                A3(vmask, i, j, k) = A3(tmask, i, j, k) * A3(tmask, i, j + 1, k);
+            }
+            for (i = 1; i <= jpim1; i++)
+                A3(fmask, i, j, k) = A3(tmask, i, j, k) * A3(tmask, i + 1, j, k) * A3(tmask, i, j + 1, k) *
+                                     A3(tmask, i + 1, j + 1, k);
+        }
+    lbc3(umask, jpk, 'U', 0.0);
+    lbc3(vmask, jpk, 'V', 0.0);
+    lbc3(fmask, jpk, 'F', 0.0);
+    /* w-masks, dommsk.F90:182-190 */
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) {
+            A3(wmask, i, j, 1) = A3(tmask, i, j, 1);
+            A3(wumask, i, j, 1) = A3(umask, i, j, 1);
+            A3(wvmask, i, j, 1) = A3(vmask, i, j, 1);
+        }
+    for (k = 2; k <= jpk; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A3(wmask, i, j, k) = A3(tmask, i, j, k) * A3(tmask, i, j, k - 1);
+                A3(wumask, i, j, k) = A3(umask, i, j, k) * A3(umask, i, j, k - 1);
+                A3(wvmask, i, j, k) = A3(vmask, i, j, k) * A3(vmask, i, j, k - 1);
+            }
+    /* column masks: MAXVAL over k, dommsk.F90:195-197 */
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) {
+            double a = A3(tmask, i, j, 1), b = A3(umask, i, j, 1), c = A3(vmask, i, j, 1);
+            for (k = 2; k <= jpk; k++) {
+                a = MAX2(a, A3(tmask, i, j, k));
+                b = MAX2(b, A3(umask, i, j, k));
+                c = MAX2(c, A3(vmask, i, j, k));
+            }
+            A2(ssmask, i, j) = a; A2(ssumask, i, j) = b; A2(ssvmask, i, j) = c;
+        }
+    /* k-indices, domzgr.F90:292-314 */
+    mikt = calloc(n2(), sizeof(int)); mbkt = calloc(n2(), sizeof(int));
+    miku = calloc(n2(), sizeof(int)); mikv = calloc(n2(), sizeof(int));
+    mbku = calloc(n2(), sizeof(int)); mbkv = calloc(n2(), sizeof(int)); mikf = calloc(n2(), sizeof(int));
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) {
+            A2(mikt, i, j) = IMAX(A2(D.k_top, i, j), 1);
+            A2(mbkt, i, j) = IMAX(A2(D.k_bot, i, j), 1);
+        }
+    for (j = 1; j <= jpjm1; j++)
+        for (i = 1; i <= jpim1; i++) {
+            A2(miku, i, j) = IMAX(A2(mikt, i + 1, j), A2(mikt, i, j));
+            A2(mikv, i, j) = IMAX(A2(mikt, i, j + 1), A2(mikt, i, j));
+            A2(mikf, i, j) = IMAX(IMAX(A2(mikt, i, j + 1), A2(mikt, i, j)),
+                                  IMAX(A2(mikt, i + 1, j), A2(mikt, i + 1, j + 1)));
+            A2(mbku, i, j) = IMIN(A2(mbkt, i + 1, j), A2(mbkt, i, j));
+            A2(mbkv, i, j) = IMIN(A2(mbkt, i, j + 1), A2(mbkt, i, j));
+        }
+    {
+        /* REAL round trip through lbc_lnk then MAX(NINT(.),1) */
+        int *arr[] = {miku, mikv, mikf, mbku, mbkv};
+        char nat[] = {'U', 'V', 'F', 'U', 'V'};
+        double *zk = z2();
+        for (int q = 0; q < 5; q++) {
+            for (size_t s = 0; s < n2(); s++) zk[s] = (double)arr[q][s];
+            lbc3(zk, 1, nat[q], 0.0);
+            for (size_t s = 0; s < n2(); s++) arr[q][s] = IMAX((int)lround(zk[s]), 1);
+        }
+        free(zk);
+    }
+    /* flux-form metric terms, dynvor.F90:920-939 */
+    di_e2v_2e1e2f = z2(); dj_e1u_2e1e2f = z2(); di_e2u_2 = z2(); dj_e1v_2 = z2();
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            A2(di_e2u_2, i, j) = (e2u(i, j) - e2u(i - 1, j)) * 0.5;
+            A2(dj_e1v_2, i, j) = (e1v(i, j) - e1v(i, j - 1)) * 0.5;
+        }
+    lbc3(di_e2u_2, 1, 'T', 0.0);
+    lbc3(dj_e1v_2, 1, 'T', 0.0);
+    for (j = 1; j <= jpjm1; j++)
+        for (i = 1; i <= jpim1; i++) {
+            A2(di_e2v_2e1e2f, i, j) = (e2v(i + 1, j) - e2v(i, j)) * 0.5 * r1_e1e2f(i, j);
+            A2(dj_e1u_2e1e2f, i, j) = (e1u(i, j + 1) - e1u(i, j)) * 0.5 * r1_e1e2f(i, j);
+        }
+    lbc3(di_e2v_2e1e2f, 1, 'F', 0.0);
+    lbc3(dj_e1u_2e1e2f, 1, 'F', 0.0);
+    r2dt_tra = r2dt_zdf = r2dt_dyn = 2.0 * D.rdt;
+    return 0;
+}
+
+int ora_dom_msk(double *ptmask, double *pumask, double *pvmask, double *pfmask, double *pwmask, double *pwumask,
+                double *pwvmask, double *pssmask, double *pssumask, double *pssvmask, int *pmikt, int *pmbkt,
+                int *pmiku, int *pmikv, int *pmbku, int *pmbkv)
+{
+    size_t b3 = n3() * sizeof(double), b2 = n2() * sizeof(double), bi = n2() * sizeof(int);
+    if (ptmask) memcpy(ptmask, tmask, b3);
+    if (pumask) memcpy(pumask, umask, b3);
+    if (pvmask) memcpy(pvmask, vmask, b3);
+    if (pfmask) memcpy(pfmask, fmask, b3);
+    if (pwmask) memcpy(pwmask, wmask, b3);
+    if (pwumask) memcpy(pwumask, wumask, b3);
+    if (pwvmask) memcpy(pwvmask, wvmask, b3);
+    if (pssmask) memcpy(pssmask, ssmask, b2);
+    if (pssumask) memcpy(pssumask, ssumask, b2);
+    if (pssvmask) memcpy(pssvmask, ssvmask, b2);
+    if (pmikt) memcpy(pmikt, mikt, bi);
+    if (pmbkt) memcpy(pmbkt, mbkt, bi);
+    if (pmiku) memcpy(pmiku, miku, bi);
+    if (pmikv) memcpy(pmikv, mikv, bi);
+    if (pmbku) memcpy(pmbku, mbku, bi);
+    if (pmbkv) memcpy(pmbkv, mbkv, bi);
+    return 0;
+}
+
+#define TMASK(i, j, k) A3(tmask, i, j, k)
+#define UMASK(i, j, k) A3(umask, i, j, k)
+#define VMASK(i, j, k) A3(vmask, i, j, k)
+#define WMASK(i, j, k) A3(wmask, i, j, k)
+#define WUMASK(i, j, k) A3(wumask, i, j, k)
+#define WVMASK(i, j, k) A3(wvmask, i, j, k)
+
+/* ------------------------------------------------------------------------ */
+/* a3: tra_adv effective transport, TRA/traadv.F90:94-124                  */
+/* ------------------------------------------------------------------------ */
+int ora_tra_adv_trp(const double *un, const double *vn, const double *wn, double *zun, double *zvn, double *zwn)
+{
+    int i, j, k;
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) A3(zun, i, j, jpk) = A3(zvn, i, j, jpk) = A3(zwn, i, j, jpk) = 0.0;
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A3(zun, i, j, k) = e2u(i, j) * e3u_n(i, j, k) * A3(un, i, j, k);
+                A3(zvn, i, j, k) = e1v(i, j) * e3v_n(i, j, k) * A3(vn, i, j, k);
+                A3(zwn, i, j, k) = e1e2t(i, j) * A3(wn, i, j, k);
+            }
+    return 0;
+}
+
+/* tra_adv time step, traadv.F90:95-97 (module r2dt, updated only at the start) */
+double ora_tra_adv_r2dt(int kt)
+{
+    if (D.neuler == 0 && kt == D.nit000) r2dt_tra = D.rdt;
+    else if (kt <= D.nit000 + 1) r2dt_tra = 2.0 * D.rdt;
+    return r2dt_tra;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a6: interp_4th_cpt, TRA/traadv_fct.F90:517-616                           */
+/* ------------------------------------------------------------------------ */
+int ora_interp_4th_cpt(const double *pt_in, double *pt_out)
+{
+    int i, j, k, ikt, ikb;
+    double *zwd = z3(), *zwi = z3(), *zws = z3(), *zwrm = z3(), *zwt = z3();
+    for (k = 3; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A3(zwd, i, j, k) = 3.0 * WMASK(i, j, k) + 1.0;
+                A3(zwi, i, j, k) = WMASK(i, j, k);
+                A3(zws, i, j, k) = WMASK(i, j, k);
+                A3(zwrm, i, j, k) = 3.0 * WMASK(i, j, k) * (A3(pt_in, i, j, k) + A3(pt_in, i, j, k - 1));
+            }
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            ikt = A2(mikt, i, j) + 1;
+            ikb = A2(mbkt, i, j);
+            A3(zwd, i, j, ikt) = 1.0;
+            A3(zwi, i, j, ikt) = 0.0;
+            A3(zws, i, j, ikt) = 0.0;
+            A3(zwrm, i, j, ikt) = 0.5 * (A3(pt_in, i, j, ikt - 1) + A3(pt_in, i, j, ikt));
+            A3(zwd, i, j, ikb) = 1.0;
+            A3(zwi, i, j, ikb) = 0.0;
+            A3(zws, i, j, ikb) = 0.0;
+            /* ikb-1 == 0 over one-level columns: the reference then reads
+               pt_in(:,:,0), outside the array; those columns are land here */
+            A3(zwrm, i, j, ikb) = 0.5 * ((ikb >= 2 ? A3(pt_in, i, j, ikb - 1) : 0.0) + A3(pt_in, i, j, ikb));
+        }
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) A3(zwt, i, j, 2) = A3(zwd, i, j, 2);
+    for (k = 3; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++)
+                A3(zwt, i, j, k) = A3(zwd, i, j, k) - A3(zwi, i, j, k) * A3(zws, i, j, k - 1) / A3(zwt, i, j, k - 1);
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) A3(pt_out, i, j, 2) = A3(zwrm, i, j, 2);
+    for (k = 3; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++)
+                A3(pt_out, i, j, k) =
+                    A3(zwrm, i, j, k) - A3(zwi, i, j, k) / A3(zwt, i, j, k - 1) * A3(pt_out, i, j, k - 1);
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) A3(pt_out, i, j, jpkm1) = A3(pt_out, i, j, jpkm1) / A3(zwt, i, j, jpkm1);
+    for (k = jpk - 2; k >= 2; k--)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++)
+                A3(pt_out, i, j, k) =
+                    (A3(pt_out, i, j, k) - A3(zws, i, j, k) * A3(pt_out, i, j, k + 1)) / A3(zwt, i, j, k);
+    free(zwd); free(zwi); free(zws); free(zwrm); free(zwt);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a5: nonosc, TRA/traadv_fct.F90:330-428                                   */
+/* ------------------------------------------------------------------------ */
+static void nonosc(const double *pbef, double *paa, double *pbb, double *pcc, const double *paft, double p2dt)
+{
+    int i, j, k, ikm1;
+    const double zbig = 1.e+40, zrtrn = 1.e-15;
+    double *zbetup = z3(), *zbetdo = z3(), *zbup = z3(), *zbdo = z3();
+    for (size_t s = 0; s < n3(); s++) {
+        double tm = tmask[s];
+        zbup[s] = MAX2(pbef[s] * tm - zbig * (1.0 - tm), paft[s] * tm - zbig * (1.0 - tm));
+        zbdo[s] = MIN2(pbef[s] * tm + zbig * (1.0 - tm), paft[s] * tm + zbig * (1.0 - tm));
+    }
+    for (k = 1; k <= jpkm1; k++) {
+        ikm1 = IMAX(k - 1, 1);
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                /* Fortran MAX/MIN of 7 arguments, evaluated left to right */
+                double zup = A3(zbup, i, j, k);
+                zup = MAX2(zup, A3(zbup, i - 1, j, k));
+                zup = MAX2(zup, A3(zbup, i + 1, j, k));
+                zup = MAX2(zup, A3(zbup, i, j - 1, k));
+                zup = MAX2(zup, A3(zbup, i, j + 1, k));
+                zup = MAX2(zup, A3(zbup, i, j, ikm1));
+                zup = MAX2(zup, A3(zbup, i, j, k + 1));
+                double zdo = A3(zbdo, i, j, k);
+                zdo = MIN2(zdo, A3(zbdo, i - 1, j, k));
+                zdo = MIN2(zdo, A3(zbdo, i + 1, j, k));
+                zdo = MIN2(zdo, A3(zbdo, i, j - 1, k));
+                zdo = MIN2(zdo, A3(zbdo, i, j + 1, k));
+                zdo = MIN2(zdo, A3(zbdo, i, j, ikm1));
+                zdo = MIN2(zdo, A3(zbdo, i, j, k + 1));
+                double zpos = MAX2(0., A3(paa, i - 1, j, k)) - MIN2(0., A3(paa, i, j, k)) +
+                              MAX2(0., A3(pbb, i, j - 1, k)) - MIN2(0., A3(pbb, i, j, k)) +
+                              MAX2(0., A3(pcc, i, j, k + 1)) - MIN2(0., A3(pcc, i, j, k));
+                double zneg = MAX2(0., A3(paa, i, j, k)) - MIN2(0., A3(paa, i - 1, j, k)) +
+                              MAX2(0., A3(pbb, i, j, k)) - MIN2(0., A3(pbb, i, j - 1, k)) +
+                              MAX2(0., A3(pcc, i, j, k)) - MIN2(0., A3(pcc, i, j, k + 1));
+                double zbt = e1e2t(i, j) * e3t_n(i, j, k) / p2dt;
+                A3(zbetup, i, j, k) = (zup - A3(paft, i, j, k)) / (zpos + zrtrn) * zbt;
+                A3(zbetdo, i, j, k) = (A3(paft, i, j, k) - zdo) / (zneg + zrtrn) * zbt;
+            }
+    }
+    lbc3(zbetup, jpk, 'T', 0.0);
+    lbc3(zbetdo, jpk, 'T', 0.0);
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                double zau = MIN3(1.0, A3(zbetdo, i, j, k), A3(zbetup, i + 1, j, k));
+                double zbu = MIN3(1.0, A3(zbetup, i, j, k), A3(zbetdo, i + 1, j, k));
+                double zcu = (0.5 + SIGN(0.5, A3(paa, i, j, k)));
+                A3(paa, i, j, k) = A3(paa, i, j, k) * (zcu * zau + (1.0 - zcu) * zbu);
+                double zav = MIN3(1.0, A3(zbetdo, i, j, k), A3(zbetup, i, j + 1, k));
+                double zbv = MIN3(1.0, A3(zbetup, i, j, k), A3(zbetdo, i, j + 1, k));
+                double zcv = (0.5 + SIGN(0.5, A3(pbb, i, j, k)));
+                A3(pbb, i, j, k) = A3(pbb, i, j, k) * (zcv * zav + (1.0 - zcv) * zbv);
+                double za = MIN3(1., A3(zbetdo, i, j, k + 1), A3(zbetup, i, j, k));
+                double zb = MIN3(1., A3(zbetup, i, j, k + 1), A3(zbetdo, i, j, k));
+                double zc = (0.5 + SIGN(0.5, A3(pcc, i, j, k + 1)));
+                A3(pcc, i, j, k + 1) = A3(pcc, i, j, k + 1) * (zc * za + (1.0 - zc) * zb);
+            }
+    lbc3(paa, jpk, 'U', 0.0);
+    lbc3(pbb, jpk, 'V', 0.0);
+    free(zbetup); free(zbetdo); free(zbup); free(zbdo);
+}
+
+/* ------------------------------------------------------------------------ */
+/* a4: tra_adv_fct, TRA/traadv_fct.F90:54-327                              */
+/* ------------------------------------------------------------------------ */
+int ora_tra_adv_fct(int kt, int kit000, const char *cdtype, double p2dt, const double *pun, const double *pvn,
+                    const double *pwn, const double *ptb, const double *ptn, double *pta, int kjpt, int kn_fct_h,
+                    int kn_fct_v)
+{
+    int i, j, k, jn;
+    const double r1_6 = 1.0 / 6.0;
+    (void)kt; (void)kit000; (void)cdtype;
+    if ((kn_fct_h != 2 && kn_fct_h != 4) || (kn_fct_v != 2 && kn_fct_v != 4)) return NEMO_HIP_ERR_UNSUPPORTED;
+    double *zwi = z3(), *zwx = z3(), *zwy = z3(), *zwz = z3(), *ztu = z3(), *ztv = z3(), *zltu = z3(),
+           *zltv = z3(), *ztw = z3();
+    size_t off = n3();
+    for (jn = 1; jn <= kjpt; jn++) {
+        const double *tb = ptb + (jn - 1) * off, *tn = ptn + (jn - 1) * off;
+        double *ta = pta + (jn - 1) * off;
+        /* surface / bottom flux zero (lines 114-115); recomputed interior below */
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A3(zwz, i, j, 1) = 0.0;
+                A3(zwx, i, j, jpk) = 0.0; A3(zwy, i, j, jpk) = 0.0; A3(zwz, i, j, jpk) = 0.0;
+            }
+        if (jn == 1) memset(zwi, 0, off * sizeof(double));
+        /* upstream fluxes, lines 123-135 */
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 1; j <= jpjm1; j++)
+                for (i = 1; i <= jpim1; i++) {
+                    double zfp_ui = A3(pun, i, j, k) + fabs(A3(pun, i, j, k));
+                    double zfm_ui = A3(pun, i, j, k) - fabs(A3(pun, i, j, k));
+                    double zfp_vj = A3(pvn, i, j, k) + fabs(A3(pvn, i, j, k));
+                    double zfm_vj = A3(pvn, i, j, k) - fabs(A3(pvn, i, j, k));
+                    A3(zwx, i, j, k) = 0.5 * (zfp_ui * A3(tb, i, j, k) + zfm_ui * A3(tb, i + 1, j, k));
+                    A3(zwy, i, j, k) = 0.5 * (zfp_vj * A3(tb, i, j, k) + zfm_vj * A3(tb, i, j + 1, k));
+                }
+        /* lines 137-145 */
+        for (k = 2; k <= jpkm1; k++)
+            for (j = 1; j <= jpj; j++)
+                for (i = 1; i <= jpi; i++) {
+                    double zfp_wk = A3(pwn, i, j, k) + fabs(A3(pwn, i, j, k));
+                    double zfm_wk = A3(pwn, i, j, k) - fabs(A3(pwn, i, j, k));
+                    A3(zwz, i, j, k) =
+                        0.5 * (zfp_wk * A3(tb, i, j, k) + zfm_wk * A3(tb, i, j, k - 1)) * WMASK(i, j, k);
+                }
+        /* linear free surface, lines 146-156 (no cavities) */
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) A3(zwz, i, j, 1) = A3(pwn, i, j, 1) * A3(tb, i, j, 1);
+        /* lines 158-170 */
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++) {
+                    double ztra = -(A3(zwx, i, j, k) - A3(zwx, i - 1, j, k) + A3(zwy, i, j, k) -
+                                    A3(zwy, i, j - 1, k) + A3(zwz, i, j, k) - A3(zwz, i, j, k + 1)) *
+                                  r1_e1e2t(i, j);
+                    A3(ta, i, j, k) = A3(ta, i, j, k) + ztra / e3t_n(i, j, k) * TMASK(i, j, k);
+                    A3(zwi, i, j, k) =
+                        (e3t_b(i, j, k) * A3(tb, i, j, k) + p2dt * ztra) / e3t_a(i, j, k) * TMASK(i, j, k);
+                }
+        /* horizontal anti-diffusive fluxes, lines 180-251 */
+        if (kn_fct_h == 2) {
+            for (k = 1; k <= jpkm1; k++)
+                for (j = 1; j <= jpjm1; j++)
+                    for (i = 1; i <= jpim1; i++) {
+                        A3(zwx, i, j, k) =
+                            0.5 * A3(pun, i, j, k) * (A3(tn, i, j, k) + A3(tn, i + 1, j, k)) - A3(zwx, i, j, k);
+                        A3(zwy, i, j, k) =
+                            0.5 * A3(pvn, i, j, k) * (A3(tn, i, j, k) + A3(tn, i, j + 1, k)) - A3(zwy, i, j, k);
+                    }
+        } else {
+            for (j = 1; j <= jpj; j++)
+                for (i = 1; i <= jpi; i++) A3(zltu, i, j, jpk) = A3(zltv, i, j, jpk) = 0.0;
+            for (k = 1; k <= jpkm1; k++) {
+                for (j = 1; j <= jpjm1; j++)
+                    for (i = 1; i <= jpim1; i++) {
+                        A3(ztu, i, j, k) = (A3(tn, i + 1, j, k) - A3(tn, i, j, k)) * UMASK(i, j, k);
+                        A3(ztv, i, j, k) = (A3(tn, i, j + 1, k) - A3(tn, i, j, k)) * VMASK(i, j, k);
+                    }
+                for (j = 2; j <= jpjm1; j++)
+                    for (i = 2; i <= jpim1; i++) {
+                        A3(zltu, i, j, k) = (A3(ztu, i, j, k) + A3(ztu, i - 1, j, k)) * r1_6;
+                        A3(zltv, i, j, k) = (A3(ztv, i, j, k) + A3(ztv, i, j - 1, k)) * r1_6;
+                    }
+            }
+            lbc3(zltu, jpk, 'T', 0.0);
+            lbc3(zltv, jpk, 'T', 0.0);
+            for (k = 1; k <= jpkm1; k++)
+                for (j = 1; j <= jpjm1; j++)
+                    for (i = 1; i <= jpim1; i++) {
+                        double zC2t_u = A3(tn, i, j, k) + A3(tn, i + 1, j, k);
+                        double zC2t_v = A3(tn, i, j, k) + A3(tn, i, j + 1, k);
+                        A3(zwx, i, j, k) = 0.5 * A3(pun, i, j, k) * (zC2t_u + A3(zltu, i, j, k) - A3(zltu, i + 1, j, k)) -
+                                           A3(zwx, i, j, k);
+                        A3(zwy, i, j, k) = 0.5 * A3(pvn, i, j, k) * (zC2t_v + A3(zltv, i, j, k) - A3(zltv, i, j + 1, k)) -
+                                           A3(zwy, i, j, k);
+                    }
+        }
+        /* vertical anti-diffusive fluxes, lines 253-275 */
+        if (kn_fct_v == 2) {
+            for (k = 2; k <= jpkm1; k++)
+                for (j = 2; j <= jpjm1; j++)
+                    for (i = 2; i <= jpim1; i++)
+                        A3(zwz, i, j, k) =
+                            (A3(pwn, i, j, k) * 0.5 * (A3(tn, i, j, k) + A3(tn, i, j, k - 1)) - A3(zwz, i, j, k)) *
+                            WMASK(i, j, k);
+        } else {
+            ora_interp_4th_cpt(tn, ztw);
+            for (k = 2; k <= jpkm1; k++)
+                for (j = 2; j <= jpjm1; j++)
+                    for (i = 2; i <= jpim1; i++)
+                        A3(zwz, i, j, k) = (A3(pwn, i, j, k) * A3(ztw, i, j, k) - A3(zwz, i, j, k)) * WMASK(i, j, k);
+        }
+        /* linssh: surface anti-diffusive flux is zero, lines 276-278 */
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) A3(zwz, i, j, 1) = 0.0;
+        lbc3(zwi, jpk, 'T', 0.0);
+        lbc3(zwx, jpk, 'U', 0.0);
+        lbc3(zwy, jpk, 'V', 0.0);
+        lbc3(zwz, jpk, 'W', 0.0);
+        nonosc(tb, zwx, zwy, zwz, zwi, p2dt);
+        /* final trend, lines 288-297 */
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++)
+                    A3(ta, i, j, k) = A3(ta, i, j, k) - (A3(zwx, i, j, k) - A3(zwx, i - 1, j, k) + A3(zwy, i, j, k) -
+                                                         A3(zwy, i, j - 1, k) + A3(zwz, i, j, k) - A3(zwz, i, j, k + 1)) *
+                                                            r1_e1e2t(i, j) / e3t_n(i, j, k);
+    }
+    free(zwi); free(zwx); free(zwy); free(zwz); free(ztu); free(ztv); free(zltu); free(zltv); free(ztw);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a8: tra_ldf_lap, TRA/traldf_lap_blp.F90:47-158 (ln_zps = F)              */
+/* ------------------------------------------------------------------------ */
+int ora_tra_ldf_lap(int kt, int kit000, const char *cdtype, const double *pahu, const double *pahv,
+                    const double *pgu, const double *pgv, const double *pgui, const double *pgvi, const double *ptb,
+                    double *pta, int kjpt, int kpass)
+{
+    int i, j, k, jn;
+    (void)kt; (void)kit000; (void)cdtype; (void)pgu; (void)pgv; (void)pgui; (void)pgvi;
+    double zsign = (kpass == 1) ? 1.0 : -1.0;
+    double *ztu = z3(), *ztv = z3(), *zaheeu = z3(), *zaheev = z3();
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpjm1; j++)
+            for (i = 1; i <= jpim1; i++) {
+                A3(zaheeu, i, j, k) = zsign * A3(pahu, i, j, k) * e2_e1u(i, j) * e3u_n(i, j, k);
+                A3(zaheev, i, j, k) = zsign * A3(pahv, i, j, k) * e1_e2v(i, j) * e3v_n(i, j, k);
+            }
+    for (jn = 1; jn <= kjpt; jn++) {
+        const double *tb = ptb + (jn - 1) * n3();
+        double *ta = pta + (jn - 1) * n3();
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 1; j <= jpjm1; j++)
+                for (i = 1; i <= jpim1; i++) {
+                    A3(ztu, i, j, k) = A3(zaheeu, i, j, k) * (A3(tb, i + 1, j, k) - A3(tb, i, j, k));
+                    A3(ztv, i, j, k) = A3(zaheev, i, j, k) * (A3(tb, i, j + 1, k) - A3(tb, i, j, k));
+                }
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++)
+                    A3(ta, i, j, k) = A3(ta, i, j, k) + (A3(ztu, i, j, k) - A3(ztu, i - 1, j, k) + A3(ztv, i, j, k) -
+                                                         A3(ztv, i, j - 1, k)) /
+                                                            (e1e2t(i, j) * e3t_n(i, j, k));
+    }
+    free(ztu); free(ztv); free(zaheeu); free(zaheev);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a9: tra_ldf_iso, TRA/traldf_iso.F90:50-382 (ln_zps = F)                  */
+/* ------------------------------------------------------------------------ */
+int ora_tra_ldf_iso(int kt, int kit000, const char *cdtype, const double *pahu, const double *pahv,
+                    const double *pgu, const double *pgv, const double *pgui, const double *pgvi, const double *ptb,
+                    const double *ptbb, double *pta, int kjpt, int kpass, const double *uslp, const double *vslp,
+                    const double *wslpi, const double *wslpj, double *ah_wslp2, double *akz)
+{
+    int i, j, k, jn;
+    (void)cdtype; (void)pgu; (void)pgv; (void)pgui; (void)pgvi;
+    if (kpass == 1 && kt == kit000) {
+        memset(akz, 0, n3() * sizeof(double));
+        memset(ah_wslp2, 0, n3() * sizeof(double));
+    }
+    double z2dt = (D.neuler == 0 && kt == D.nit000) ? D.rdt : 2. * D.rdt;
+    double z1_2dt = 1.0 / z2dt;
+    double zsign = (kpass == 1) ? 1.0 : -1.0;
+    if (kpass == 1) {
+        for (k = 2; k <= jpkm1; k++)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++) {
+                    double zmsku = WMASK(i, j, k) / MAX2(UMASK(i, j, k - 1) + UMASK(i - 1, j, k) +
+                                                             UMASK(i - 1, j, k - 1) + UMASK(i, j, k),
+                                                         1.0);
+                    double zmskv = WMASK(i, j, k) / MAX2(VMASK(i, j, k - 1) + VMASK(i, j - 1, k) +
+                                                             VMASK(i, j - 1, k - 1) + VMASK(i, j, k),
+                                                         1.0);
+                    double zahu_w = (A3(pahu, i, j, k - 1) + A3(pahu, i - 1, j, k) + A3(pahu, i - 1, j, k - 1) +
+                                     A3(pahu, i, j, k)) *
+                                    zmsku;
+                    double zahv_w = (A3(pahv, i, j, k - 1) + A3(pahv, i, j - 1, k) + A3(pahv, i, j - 1, k - 1) +
+                                     A3(pahv, i, j, k)) *
+                                    zmskv;
+                    A3(ah_wslp2, i, j, k) = zahu_w * A3(wslpi, i, j, k) * A3(wslpi, i, j, k) +
+                                            zahv_w * A3(wslpj, i, j, k) * A3(wslpj, i, j, k);
+                }
+        if (D.ln_traldf_msc) {
+            for (k = 2; k <= jpkm1; k++)
+                for (j = 2; j <= jpjm1; j++)
+                    for (i = 2; i <= jpim1; i++)
+                        A3(akz, i, j, k) =
+                            0.25 * ((A3(pahu, i, j, k) + A3(pahu, i, j, k - 1)) / (e1u(i, j) * e1u(i, j)) +
+                                    (A3(pahu, i - 1, j, k) + A3(pahu, i - 1, j, k - 1)) / (e1u(i - 1, j) * e1u(i - 1, j)) +
+                                    (A3(pahv, i, j, k) + A3(pahv, i, j, k - 1)) / (e2v(i, j) * e2v(i, j)) +
+                                    (A3(pahv, i, j - 1, k) + A3(pahv, i, j - 1, k - 1)) / (e2v(i, j - 1) * e2v(i, j - 1)));
+            if (!D.ln_traldf_lap) {
+                for (k = 2; k <= jpkm1; k++)
+                    for (j = 1; j <= jpjm1; j++)
+                        for (i = 1; i <= jpim1; i++)
+                            A3(akz, i, j, k) = 16.0 * A3(ah_wslp2, i, j, k) *
+                                               (A3(akz, i, j, k) + A3(ah_wslp2, i, j, k) / (e3w_n(i, j, k) * e3w_n(i, j, k)));
+            } else {
+                for (k = 2; k <= jpkm1; k++)
+                    for (j = 1; j <= jpjm1; j++)
+                        for (i = 1; i <= jpim1; i++) {
+                            double ze3w_2 = e3w_n(i, j, k) * e3w_n(i, j, k);
+                            double zcoef0 = z2dt * (A3(akz, i, j, k) + A3(ah_wslp2, i, j, k) / ze3w_2);
+                            A3(akz, i, j, k) = MAX2(zcoef0 - 0.5, 0.0) * ze3w_2 * z1_2dt;
+                        }
+            }
+        } else {
+            memcpy(akz, ah_wslp2, n3() * sizeof(double));
+        }
+    }
+    double *zdit = z3(), *zdjt = z3(), *zftu = z3(), *zftv = z3(), *ztfw = z3();
+    double *zdkt = z2(), *zdk1t = z2();
+    for (jn = 1; jn <= kjpt; jn++) {
+        const double *tb = ptb + (jn - 1) * n3(), *tbb = ptbb + (jn - 1) * n3();
+        double *ta = pta + (jn - 1) * n3();
+        for (k = 1; k <= jpk; k++)
+            for (j = 1; j <= jpj; j++) {
+                A3(zdit, 1, j, k) = 0.0; A3(zdit, jpi, j, k) = 0.0;
+                A3(zdjt, 1, j, k) = 0.0; A3(zdjt, jpi, j, k) = 0.0;
+            }
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 1; j <= jpjm1; j++)
+                for (i = 1; i <= jpim1; i++) {
+                    A3(zdit, i, j, k) = (A3(tb, i + 1, j, k) - A3(tb, i, j, k)) * UMASK(i, j, k);
+                    A3(zdjt, i, j, k) = (A3(tb, i, j + 1, k) - A3(tb, i, j, k)) * VMASK(i, j, k);
+                }
+        /* horizontal trend, lines 247-285 */
+        for (k = 1; k <= jpkm1; k++) {
+            for (j = 1; j <= jpj; j++)
+                for (i = 1; i <= jpi; i++)
+                    A2(zdk1t, i, j) = (A3(tb, i, j, k) - A3(tb, i, j, k + 1)) * WMASK(i, j, k + 1);
+            if (k == 1) memcpy(zdkt, zdk1t, n2() * sizeof(double));
+            else
+                for (j = 1; j <= jpj; j++)
+                    for (i = 1; i <= jpi; i++)
+                        A2(zdkt, i, j) = (A3(tb, i, j, k - 1) - A3(tb, i, j, k)) * WMASK(i, j, k);
+            for (j = 1; j <= jpjm1; j++)
+                for (i = 1; i <= jpim1; i++) {
+                    double zabe1 = A3(pahu, i, j, k) * e2_e1u(i, j) * e3u_n(i, j, k);
+                    double zabe2 = A3(pahv, i, j, k) * e1_e2v(i, j) * e3v_n(i, j, k);
+                    double zmsku =
+                        1. / MAX2(WMASK(i + 1, j, k) + WMASK(i, j, k + 1) + WMASK(i + 1, j, k + 1) + WMASK(i, j, k), 1.);
+                    double zmskv =
+                        1. / MAX2(WMASK(i, j + 1, k) + WMASK(i, j, k + 1) + WMASK(i, j + 1, k + 1) + WMASK(i, j, k), 1.);
+                    double zcof1 = -A3(pahu, i, j, k) * e2u(i, j) * A3(uslp, i, j, k) * zmsku;
+                    double zcof2 = -A3(pahv, i, j, k) * e1v(i, j) * A3(vslp, i, j, k) * zmskv;
+                    A3(zftu, i, j, k) = (zabe1 * A3(zdit, i, j, k) +
+                                         zcof1 * (A2(zdkt, i + 1, j) + A2(zdk1t, i, j) + A2(zdk1t, i + 1, j) + A2(zdkt, i, j))) *
+                                        UMASK(i, j, k);
+                    A3(zftv, i, j, k) = (zabe2 * A3(zdjt, i, j, k) +
+                                         zcof2 * (A2(zdkt, i, j + 1) + A2(zdk1t, i, j) + A2(zdk1t, i, j + 1) + A2(zdkt, i, j))) *
+                                        VMASK(i, j, k);
+                }
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++)
+                    A3(ta, i, j, k) = A3(ta, i, j, k) + zsign * (A3(zftu, i, j, k) - A3(zftu, i - 1, j, k) +
+                                                                 A3(zftv, i, j, k) - A3(zftv, i, j - 1, k)) *
+                                                            r1_e1e2t(i, j) / e3t_n(i, j, k);
+        }
+        /* vertical trend, lines 291-366 */
+        memset(ztfw, 0, n3() * sizeof(double));
+        for (k = 2; k <= jpkm1; k++)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++) {
+                    double zmsku = WMASK(i, j, k) / MAX2(UMASK(i, j, k - 1) + UMASK(i - 1, j, k) +
+                                                             UMASK(i - 1, j, k - 1) + UMASK(i, j, k),
+                                                         1.0);
+                    double zmskv = WMASK(i, j, k) / MAX2(VMASK(i, j, k - 1) + VMASK(i, j - 1, k) +
+                                                             VMASK(i, j - 1, k - 1) + VMASK(i, j, k),
+                                                         1.0);
+                    double zahu_w = (A3(pahu, i, j, k - 1) + A3(pahu, i - 1, j, k) + A3(pahu, i - 1, j, k - 1) +
+                                     A3(pahu, i, j, k)) *
+                                    zmsku;
+                    double zahv_w = (A3(pahv, i, j, k - 1) + A3(pahv, i, j - 1, k) + A3(pahv, i, j - 1, k - 1) +
+                                     A3(pahv, i, j, k)) *
+                                    zmskv;
+                    double zcoef3 = -zahu_w * e2t(i, j) * zmsku * A3(wslpi, i, j, k);
+                    double zcoef4 = -zahv_w * e1t(i, j) * zmskv * A3(wslpj, i, j, k);
+                    A3(ztfw, i, j, k) = zcoef3 * (A3(zdit, i, j, k - 1) + A3(zdit, i - 1, j, k) +
+                                                  A3(zdit, i - 1, j, k - 1) + A3(zdit, i, j, k)) +
+                                        zcoef4 * (A3(zdjt, i, j, k - 1) + A3(zdjt, i, j - 1, k) +
+                                                  A3(zdjt, i, j - 1, k - 1) + A3(zdjt, i, j, k));
+                }
+        if (D.ln_traldf_lap) {
+            for (k = 2; k <= jpkm1; k++)
+                for (j = 1; j <= jpjm1; j++)
+                    for (i = 2; i <= jpim1; i++)
+                        A3(ztfw, i, j, k) = A3(ztfw, i, j, k) + e1e2t(i, j) / e3w_n(i, j, k) * WMASK(i, j, k) *
+                                                                    (A3(ah_wslp2, i, j, k) - A3(akz, i, j, k)) *
+                                                                    (A3(tb, i, j, k - 1) - A3(tb, i, j, k));
+        } else if (kpass == 1) {
+            for (k = 2; k <= jpkm1; k++)
+                for (j = 1; j <= jpjm1; j++)
+                    for (i = 2; i <= jpim1; i++)
+                        A3(ztfw, i, j, k) = A3(ztfw, i, j, k) + A3(ah_wslp2, i, j, k) * e1e2t(i, j) *
+                                                                    (A3(tb, i, j, k - 1) - A3(tb, i, j, k)) /
+                                                                    e3w_n(i, j, k) * WMASK(i, j, k);
+        } else {
+            for (k = 2; k <= jpkm1; k++)
+                for (j = 1; j <= jpjm1; j++)
+                    for (i = 2; i <= jpim1; i++)
+                        A3(ztfw, i, j, k) =
+                            A3(ztfw, i, j, k) + e1e2t(i, j) / e3w_n(i, j, k) * WMASK(i, j, k) *
+                                                    (A3(ah_wslp2, i, j, k) * (A3(tb, i, j, k - 1) - A3(tb, i, j, k)) +
+                                                     A3(akz, i, j, k) * (A3(tbb, i, j, k - 1) - A3(tbb, i, j, k)));
+        }
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++)
+                    A3(ta, i, j, k) = A3(ta, i, j, k) + zsign * (A3(ztfw, i, j, k) - A3(ztfw, i, j, k + 1)) *
+                                                            r1_e1e2t(i, j) / e3t_n(i, j, k);
+    }
+    free(zdit); free(zdjt); free(zftu); free(zftv); free(ztfw); free(zdkt); free(zdk1t);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a11: tra_zdf_imp, TRA/trazdf.F90:109-268 (ln_zad_Aimp = F)               */
+/* ------------------------------------------------------------------------ */
+int ora_tra_zdf_imp(int kt, int kit000, const char *cdtype, double p2dt, const double *ptb, double *pta, int kjpt,
+                    const double *avt, const double *avs, int l_ldfslp, const double *ah_wslp2, const double *akz)
+{
+    int i, j, k, jn;
+    (void)kt; (void)kit000;
+    int is_tra = (cdtype[0] == 'T' && cdtype[1] == 'R' && cdtype[2] == 'A');
+    double *zwi = z3(), *zwt = z3(), *zwd = z3(), *zws = z3();
+    for (jn = 1; jn <= kjpt; jn++) {
+        const double *tb = ptb + (jn - 1) * n3();
+        double *ta = pta + (jn - 1) * n3();
+        if ((is_tra && (jn == 1 || (jn == 2 && D.ln_zdfddm))) || (!is_tra && jn == 1)) {
+            const double *src = (is_tra && jn == 1) ? avt : avs;
+            for (k = 2; k <= jpk; k++)
+                for (j = 1; j <= jpj; j++)
+                    for (i = 1; i <= jpi; i++) A3(zwt, i, j, k) = A3(src, i, j, k);
+            for (j = 1; j <= jpj; j++)
+                for (i = 1; i <= jpi; i++) A3(zwt, i, j, 1) = 0.0;
+            if (l_ldfslp) {
+                const double *add = D.ln_traldf_msc ? akz : ah_wslp2;
+                for (k = 2; k <= jpkm1; k++)
+                    for (j = 2; j <= jpjm1; j++)
+                        for (i = 2; i <= jpim1; i++) A3(zwt, i, j, k) = A3(zwt, i, j, k) + A3(add, i, j, k);
+            }
+            for (k = 1; k <= jpkm1; k++)
+                for (j = 2; j <= jpjm1; j++)
+                    for (i = 2; i <= jpim1; i++) {
+                        A3(zwi, i, j, k) = -p2dt * A3(zwt, i, j, k) / e3w_n(i, j, k);
+                        A3(zws, i, j, k) = -p2dt * A3(zwt, i, j, k + 1) / e3w_n(i, j, k + 1);
+                        A3(zwd, i, j, k) = e3t_a(i, j, k) - A3(zwi, i, j, k) - A3(zws, i, j, k);
+                    }
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++) A3(zwt, i, j, 1) = A3(zwd, i, j, 1);
+            for (k = 2; k <= jpkm1; k++)
+                for (j = 2; j <= jpjm1; j++)
+                    for (i = 2; i <= jpim1; i++)
+                        A3(zwt, i, j, k) = A3(zwd, i, j, k) - A3(zwi, i, j, k) * A3(zws, i, j, k - 1) / A3(zwt, i, j, k - 1);
+        }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++)
+                A3(ta, i, j, 1) = e3t_b(i, j, 1) * A3(tb, i, j, 1) + p2dt * e3t_n(i, j, 1) * A3(ta, i, j, 1);
+        for (k = 2; k <= jpkm1; k++)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++) {
+                    double zrhs = e3t_b(i, j, k) * A3(tb, i, j, k) + p2dt * e3t_n(i, j, k) * A3(ta, i, j, k);
+                    A3(ta, i, j, k) = zrhs - A3(zwi, i, j, k) / A3(zwt, i, j, k - 1) * A3(ta, i, j, k - 1);
+                }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++)
+                A3(ta, i, j, jpkm1) = A3(ta, i, j, jpkm1) / A3(zwt, i, j, jpkm1) * TMASK(i, j, jpkm1);
+        for (k = jpk - 2; k >= 1; k--)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++)
+                    A3(ta, i, j, k) =
+                        (A3(ta, i, j, k) - A3(zws, i, j, k) * A3(ta, i, j, k + 1)) / A3(zwt, i, j, k) * TMASK(i, j, k);
+    }
+    free(zwi); free(zwt); free(zwd); free(zws);
+    return 0;
+}
+
+/* a10: tra_zdf, TRA/trazdf.F90:46-106 */
+int ora_tra_zdf(int kt, const double *tsb, double *tsa, const double *avt, const double *avs, int l_ldfslp,
+                const double *ah_wslp2, const double *akz)
+{
+    if (D.neuler == 0 && kt == D.nit000) r2dt_zdf = D.rdt;
+    else if (kt <= D.nit000 + 1) r2dt_zdf = 2. * D.rdt;
+    int st = ora_tra_zdf_imp(kt, D.nit000, "TRA", r2dt_zdf, tsb, tsa, 2, avt, avs, l_ldfslp, ah_wslp2, akz);
+    double *sa = tsa + n3();
+    for (size_t s = 0; s < n3(); s++)
+        if (sa[s] < 0.0) sa[s] = 0.1;
+    return st;
+}
+
+
+/* ------------------------------------------------------------------------ */
+/* a13: dyn_keg, DYN/dynkeg.F90:46-192                                      */
+/* ------------------------------------------------------------------------ */
+int ora_dyn_keg(int kt, int kscheme, const double *un, const double *vn, double *ua, double *va)
+{
+    int i, j, k;
+    const double r1_48 = 1.0 / 48.0;
+    (void)kt;
+    double *zhke = z3();
+    if (kscheme == 0) {
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 2; j <= jpj; j++)
+                for (i = 2; i <= jpi; i++) {
+                    double zu = A3(un, i - 1, j, k) * A3(un, i - 1, j, k) + A3(un, i, j, k) * A3(un, i, j, k);
+                    double zv = A3(vn, i, j - 1, k) * A3(vn, i, j - 1, k) + A3(vn, i, j, k) * A3(vn, i, j, k);
+                    A3(zhke, i, j, k) = 0.25 * (zv + zu);
+                }
+    } else if (kscheme == 1) {
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++) {
+                    double a = A3(un, i - 1, j - 1, k) + A3(un, i - 1, j + 1, k);
+                    double b = A3(un, i, j - 1, k) + A3(un, i, j + 1, k);
+                    double zu = 8.0 * (A3(un, i - 1, j, k) * A3(un, i - 1, j, k) + A3(un, i, j, k) * A3(un, i, j, k)) +
+                                a * a + b * b;
+                    double c = A3(vn, i - 1, j - 1, k) + A3(vn, i + 1, j - 1, k);
+                    double d = A3(vn, i - 1, j, k) + A3(vn, i + 1, j, k);
+                    double zv = 8.0 * (A3(vn, i, j - 1, k) * A3(vn, i, j - 1, k) + A3(vn, i, j, k) * A3(vn, i, j, k)) +
+                                c * c + d * d;
+                    A3(zhke, i, j, k) = r1_48 * (zv + zu);
+                }
+        lbc3(zhke, jpk, 'T', 0.0);
+    } else {
+        free(zhke);
+        return NEMO_HIP_ERR_UNSUPPORTED;
+    }
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A3(ua, i, j, k) = A3(ua, i, j, k) - (A3(zhke, i + 1, j, k) - A3(zhke, i, j, k)) / e1u(i, j);
+                A3(va, i, j, k) = A3(va, i, j, k) - (A3(zhke, i, j + 1, k) - A3(zhke, i, j, k)) / e2v(i, j);
+            }
+    free(zhke);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a14: dyn_zad, DYN/dynzad.F90:38-121                                      */
+/* ------------------------------------------------------------------------ */
+int ora_dyn_zad(int kt, const double *un, const double *vn, const double *wn, double *ua, double *va)
+{
+    int i, j, k;
+    (void)kt;
+    double *zww = z2(), *zwuw = z3(), *zwvw = z3();
+    for (k = 2; k <= jpkm1; k++) {
+        for (j = 2; j <= jpj; j++)
+            for (i = 2; i <= jpi; i++) A2(zww, i, j) = 0.25 * e1e2t(i, j) * A3(wn, i, j, k);
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A3(zwuw, i, j, k) = (A2(zww, i + 1, j) + A2(zww, i, j)) * (A3(un, i, j, k - 1) - A3(un, i, j, k));
+                A3(zwvw, i, j, k) = (A2(zww, i, j + 1) + A2(zww, i, j)) * (A3(vn, i, j, k - 1) - A3(vn, i, j, k));
+            }
+    }
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            A3(zwuw, i, j, 1) = 0.0; A3(zwvw, i, j, 1) = 0.0;
+            A3(zwuw, i, j, jpk) = 0.0; A3(zwvw, i, j, jpk) = 0.0;
+        }
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A3(ua, i, j, k) = A3(ua, i, j, k) - (A3(zwuw, i, j, k) + A3(zwuw, i, j, k + 1)) * r1_e1e2u(i, j) / e3u_n(i, j, k);
+                A3(va, i, j, k) = A3(va, i, j, k) - (A3(zwvw, i, j, k) + A3(zwvw, i, j, k + 1)) * r1_e1e2v(i, j) / e3v_n(i, j, k);
+            }
+    free(zww); free(zwuw); free(zwvw);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a15: dyn_adv_cen2, DYN/dynadv_cen2.F90:37-144                            */
+/* ------------------------------------------------------------------------ */
+int ora_dyn_adv_cen2(int kt, const double *un, const double *vn, const double *wn, double *ua, double *va)
+{
+    int i, j, k;
+    (void)kt;
+    double *zfu_t = z3(), *zfu_f = z3(), *zfu_uw = z3(), *zfu = z3();
+    double *zfv_t = z3(), *zfv_f = z3(), *zfv_vw = z3(), *zfv = z3(), *zfw = z3();
+    for (k = 1; k <= jpkm1; k++) {
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A3(zfu, i, j, k) = 0.25 * e2u(i, j) * e3u_n(i, j, k) * A3(un, i, j, k);
+                A3(zfv, i, j, k) = 0.25 * e1v(i, j) * e3v_n(i, j, k) * A3(vn, i, j, k);
+            }
+        for (j = 1; j <= jpjm1; j++)
+            for (i = 1; i <= jpim1; i++) {
+                A3(zfu_t, i + 1, j, k) = (A3(zfu, i, j, k) + A3(zfu, i + 1, j, k)) * (A3(un, i, j, k) + A3(un, i + 1, j, k));
+                A3(zfv_f, i, j, k) = (A3(zfv, i, j, k) + A3(zfv, i + 1, j, k)) * (A3(un, i, j, k) + A3(un, i, j + 1, k));
+                A3(zfu_f, i, j, k) = (A3(zfu, i, j, k) + A3(zfu, i, j + 1, k)) * (A3(vn, i, j, k) + A3(vn, i + 1, j, k));
+                A3(zfv_t, i, j + 1, k) = (A3(zfv, i, j, k) + A3(zfv, i, j + 1, k)) * (A3(vn, i, j, k) + A3(vn, i, j + 1, k));
+            }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A3(ua, i, j, k) = A3(ua, i, j, k) - (A3(zfu_t, i + 1, j, k) - A3(zfu_t, i, j, k) + A3(zfv_f, i, j, k) -
+                                                     A3(zfv_f, i, j - 1, k)) *
+                                                        r1_e1e2u(i, j) / e3u_n(i, j, k);
+                A3(va, i, j, k) = A3(va, i, j, k) - (A3(zfu_f, i, j, k) - A3(zfu_f, i - 1, j, k) + A3(zfv_t, i, j + 1, k) -
+                                                     A3(zfv_t, i, j, k)) *
+                                                        r1_e1e2v(i, j) / e3v_n(i, j, k);
+            }
+    }
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            A3(zfu_uw, i, j, jpk) = 0.0; A3(zfv_vw, i, j, jpk) = 0.0;
+            A3(zfu_uw, i, j, 1) = 0.0; A3(zfv_vw, i, j, 1) = 0.0;
+        }
+    /* linear free surface: advection through the surface */
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            A3(zfu_uw, i, j, 1) =
+                0.5 * (e1e2t(i, j) * A3(wn, i, j, 1) + e1e2t(i + 1, j) * A3(wn, i + 1, j, 1)) * A3(un, i, j, 1);
+            A3(zfv_vw, i, j, 1) =
+                0.5 * (e1e2t(i, j) * A3(wn, i, j, 1) + e1e2t(i, j + 1) * A3(wn, i, j + 1, 1)) * A3(vn, i, j, 1);
+        }
+    for (k = 2; k <= jpkm1; k++) {
+        for (j = 2; j <= jpj; j++)
+            for (i = 2; i <= jpi; i++) A3(zfw, i, j, k) = 0.25 * e1e2t(i, j) * A3(wn, i, j, k);
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A3(zfu_uw, i, j, k) = (A3(zfw, i, j, k) + A3(zfw, i + 1, j, k)) * (A3(un, i, j, k) + A3(un, i, j, k - 1));
+                A3(zfv_vw, i, j, k) = (A3(zfw, i, j, k) + A3(zfw, i, j + 1, k)) * (A3(vn, i, j, k) + A3(vn, i, j, k - 1));
+            }
+    }
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A3(ua, i, j, k) = A3(ua, i, j, k) - (A3(zfu_uw, i, j, k) - A3(zfu_uw, i, j, k + 1)) * r1_e1e2u(i, j) / e3u_n(i, j, k);
+                A3(va, i, j, k) = A3(va, i, j, k) - (A3(zfv_vw, i, j, k) - A3(zfv_vw, i, j, k + 1)) * r1_e1e2v(i, j) / e3v_n(i, j, k);
+            }
+    free(zfu_t); free(zfu_f); free(zfu_uw); free(zfu); free(zfv_t); free(zfv_f); free(zfv_vw); free(zfv); free(zfw);
+    return 0;
+}
+
+/* a12: dyn_adv dispatch, DYN/dynadv.F90:73-81 */
+int ora_dyn_adv(int kt, const double *un, const double *vn, const double *wn, double *ua, double *va)
+{
+    switch (D.n_dynadv) {
+    case NEMO_NP_LIN_DYN: return 0;
+    case NEMO_NP_VEC_C2: {
+        int st = ora_dyn_keg(kt, D.nn_dynkeg, un, vn, ua, va);
+        return st ? st : ora_dyn_zad(kt, un, vn, wn, ua, va);
+    }
+    case NEMO_NP_FLX_C2: return ora_dyn_adv_cen2(kt, un, vn, wn, ua, va);
+    default: return NEMO_HIP_ERR_UNSUPPORTED;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* a16: dyn_vor, DYN/dynvor.F90:98-692                                      */
+/* ------------------------------------------------------------------------ */
+enum { np_COR = 1, np_RVO = 2, np_MET = 3, np_CRV = 4, np_CME = 5 };
+static const double r1_4 = 0.250, r1_8 = 0.125;
+#define R1_12 (1.0 / 12.0)
+
+/* relative vorticity numerator at f-point, the common sub-expression of every scheme */
+#define CURL(pu, pv, i, j, k)                                                                               \
+    (e2v(i + 1, j) * A3(pv, i + 1, j, k) - e2v(i, j) * A3(pv, i, j, k) - e1u(i, j + 1) * A3(pu, i, j + 1, k) + \
+     e1u(i, j) * A3(pu, i, j, k))
+
+static void vor_enT(int kvor, const double *pu, const double *pv, double *pu_rhs, double *pv_rhs)
+{
+    int i, j, k;
+    double *zwz = z3(), *zwt = z2();
+    if (kvor == np_RVO || kvor == np_CRV) {
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 1; j <= jpjm1; j++)
+                for (i = 1; i <= jpim1; i++) A3(zwz, i, j, k) = CURL(pu, pv, i, j, k) * r1_e1e2f(i, j);
+        lbc3(zwz, jpk, 'F', 0.0);
+    }
+    for (k = 1; k <= jpkm1; k++) {
+        switch (kvor) {
+        case np_COR:
+            for (j = 1; j <= jpj; j++)
+                for (i = 1; i <= jpi; i++) A2(zwt, i, j) = ff_t(i, j) * e1e2t(i, j) * e3t_n(i, j, k);
+            break;
+        case np_RVO:
+            for (j = 2; j <= jpj; j++)
+                for (i = 2; i <= jpi; i++)
+                    A2(zwt, i, j) = r1_4 * (A3(zwz, i - 1, j, k) + A3(zwz, i, j, k) + A3(zwz, i - 1, j - 1, k) + A3(zwz, i, j - 1, k)) *
+                                    e1e2t(i, j) * e3t_n(i, j, k);
+            break;
+        case np_MET:
+            for (j = 2; j <= jpj; j++)
+                for (i = 2; i <= jpi; i++)
+                    A2(zwt, i, j) = ((A3(pv, i, j, k) + A3(pv, i, j - 1, k)) * A2(di_e2u_2, i, j) -
+                                     (A3(pu, i, j, k) + A3(pu, i - 1, j, k)) * A2(dj_e1v_2, i, j)) *
+                                    e3t_n(i, j, k);
+            break;
+        case np_CRV:
+            for (j = 2; j <= jpj; j++)
+                for (i = 2; i <= jpi; i++)
+                    A2(zwt, i, j) = (ff_t(i, j) + r1_4 * (A3(zwz, i - 1, j, k) + A3(zwz, i, j, k) + A3(zwz, i - 1, j - 1, k) +
+                                                          A3(zwz, i, j - 1, k))) *
+                                    e1e2t(i, j) * e3t_n(i, j, k);
+            break;
+        case np_CME:
+            for (j = 2; j <= jpj; j++)
+                for (i = 2; i <= jpi; i++)
+                    A2(zwt, i, j) = (ff_t(i, j) * e1e2t(i, j) + (A3(pv, i, j, k) + A3(pv, i, j - 1, k)) * A2(di_e2u_2, i, j) -
+                                     (A3(pu, i, j, k) + A3(pu, i - 1, j, k)) * A2(dj_e1v_2, i, j)) *
+                                    e3t_n(i, j, k);
+            break;
+        }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A3(pu_rhs, i, j, k) = A3(pu_rhs, i, j, k) +
+                                      r1_4 * r1_e1e2u(i, j) / e3u_n(i, j, k) *
+                                          (A2(zwt, i + 1, j) * (A3(pv, i + 1, j, k) + A3(pv, i + 1, j - 1, k)) +
+                                           A2(zwt, i, j) * (A3(pv, i, j, k) + A3(pv, i, j - 1, k)));
+                A3(pv_rhs, i, j, k) = A3(pv_rhs, i, j, k) -
+                                      r1_4 * r1_e1e2v(i, j) / e3v_n(i, j, k) *
+                                          (A2(zwt, i, j + 1) * (A3(pu, i, j + 1, k) + A3(pu, i - 1, j + 1, k)) +
+                                           A2(zwt, i, j) * (A3(pu, i, j, k) + A3(pu, i - 1, j, k)));
+            }
+    }
+    free(zwz); free(zwt);
+}
+
+/* f-point vorticity of vor_ene / vor_ens / vor_eeT for one level into zwz2 */
+static void fvor_level(int kvor, const double *pu, const double *pv, int k, double *zwz2)
+{
+    int i, j;
+    switch (kvor) {
+    case np_COR: memcpy(zwz2, D.ff_f, n2() * sizeof(double)); break;
+    case np_RVO:
+        for (j = 1; j <= jpjm1; j++)
+            for (i = 1; i <= jpim1; i++) A2(zwz2, i, j) = CURL(pu, pv, i, j, k) * r1_e1e2f(i, j);
+        break;
+    case np_MET:
+        for (j = 1; j <= jpjm1; j++)
+            for (i = 1; i <= jpim1; i++)
+                A2(zwz2, i, j) = (A3(pv, i + 1, j, k) + A3(pv, i, j, k)) * A2(di_e2v_2e1e2f, i, j) -
+                                 (A3(pu, i, j + 1, k) + A3(pu, i, j, k)) * A2(dj_e1u_2e1e2f, i, j);
+        break;
+    case np_CRV:
+        for (j = 1; j <= jpjm1; j++)
+            for (i = 1; i <= jpim1; i++) A2(zwz2, i, j) = ff_f(i, j) + CURL(pu, pv, i, j, k) * r1_e1e2f(i, j);
+        break;
+    case np_CME:
+        for (j = 1; j <= jpjm1; j++)
+            for (i = 1; i <= jpim1; i++)
+                A2(zwz2, i, j) = ff_f(i, j) + (A3(pv, i + 1, j, k) + A3(pv, i, j, k)) * A2(di_e2v_2e1e2f, i, j) -
+                                 (A3(pu, i, j + 1, k) + A3(pu, i, j, k)) * A2(dj_e1u_2e1e2f, i, j);
+        break;
+    }
+}
+
+static void vor_ene(int kvor, const double *pun, const double *pvn, double *pua, double *pva)
+{
+    int i, j, k;
+    double *zwx = z2(), *zwy = z2(), *zwz = z2();
+    for (k = 1; k <= jpkm1; k++) {
+        fvor_level(kvor, pun, pvn, k, zwz);
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A2(zwx, i, j) = e2u(i, j) * A3(pun, i, j, k);
+                A2(zwy, i, j) = e1v(i, j) * A3(pvn, i, j, k);
+            }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                double zy1 = A2(zwy, i, j - 1) + A2(zwy, i + 1, j - 1);
+                double zy2 = A2(zwy, i, j) + A2(zwy, i + 1, j);
+                double zx1 = A2(zwx, i - 1, j) + A2(zwx, i - 1, j + 1);
+                double zx2 = A2(zwx, i, j) + A2(zwx, i, j + 1);
+                A3(pua, i, j, k) = A3(pua, i, j, k) + r1_4 * r1_e1u(i, j) * (A2(zwz, i, j - 1) * zy1 + A2(zwz, i, j) * zy2);
+                A3(pva, i, j, k) = A3(pva, i, j, k) - r1_4 * r1_e2v(i, j) * (A2(zwz, i - 1, j) * zx1 + A2(zwz, i, j) * zx2);
+            }
+    }
+    free(zwx); free(zwy); free(zwz);
+}
+
+static void vor_ens(int kvor, const double *pun, const double *pvn, double *pua, double *pva)
+{
+    int i, j, k;
+    double *zwx = z2(), *zwy = z2(), *zwz = z2();
+    for (k = 1; k <= jpkm1; k++) {
+        fvor_level(kvor, pun, pvn, k, zwz);
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A2(zwx, i, j) = e2u(i, j) * A3(pun, i, j, k);
+                A2(zwy, i, j) = e1v(i, j) * A3(pvn, i, j, k);
+            }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                double zuav = r1_8 * r1_e1u(i, j) * (A2(zwy, i, j - 1) + A2(zwy, i + 1, j - 1) + A2(zwy, i, j) + A2(zwy, i + 1, j));
+                double zvau = -(r1_8 * r1_e2v(i, j) * (A2(zwx, i - 1, j) + A2(zwx, i - 1, j + 1) + A2(zwx, i, j) + A2(zwx, i, j + 1)));
+                A3(pua, i, j, k) = A3(pua, i, j, k) + zuav * (A2(zwz, i, j - 1) + A2(zwz, i, j));
+                A3(pva, i, j, k) = A3(pva, i, j, k) + zvau * (A2(zwz, i - 1, j) + A2(zwz, i, j));
+            }
+    }
+    free(zwx); free(zwy); free(zwz);
+}
+
+/* vor_een (use_e3f = 1, DYN/dynvor.F90:539-692) and vor_eeT (use_e3f = 0, 696-828) */
+static void vor_een_eet(int use_e3f, int kvor, const double *pun, const double *pvn, double *pua, double *pva)
+{
+    int i, j, k;
+    double *zwz = z3(), *z1_e3f = z2(), *zwx = z2(), *zwy = z2();
+    double *ztne = z2(), *ztnw = z2(), *ztse = z2(), *ztsw = z2(), *zlev = z2();
+    for (k = 1; k <= jpkm1; k++) {
+        if (use_e3f) {
+            for (j = 1; j <= jpjm1; j++)
+                for (i = 1; i <= jpim1; i++) {
+                    double ze3f = e3t_n(i, j + 1, k) * TMASK(i, j + 1, k) + e3t_n(i + 1, j + 1, k) * TMASK(i + 1, j + 1, k) +
+                                  e3t_n(i, j, k) * TMASK(i, j, k) + e3t_n(i + 1, j, k) * TMASK(i + 1, j, k);
+                    double znum;
+                    if (D.nn_een_e3f == 0) znum = 4.0;
+                    else znum = TMASK(i, j + 1, k) + TMASK(i + 1, j + 1, k) + TMASK(i, j, k) + TMASK(i + 1, j, k);
+                    A2(z1_e3f, i, j) = (ze3f != 0.0) ? znum / ze3f : 0.0;
+                }
+            switch (kvor) {
+            case np_COR:
+                for (j = 1; j <= jpjm1; j++)
+                    for (i = 1; i <= jpim1; i++) A3(zwz, i, j, k) = ff_f(i, j) * A2(z1_e3f, i, j);
+                break;
+            case np_RVO:
+                for (j = 1; j <= jpjm1; j++)
+                    for (i = 1; i <= jpim1; i++) A3(zwz, i, j, k) = CURL(pun, pvn, i, j, k) * r1_e1e2f(i, j) * A2(z1_e3f, i, j);
+                break;
+            case np_MET:
+                for (j = 1; j <= jpjm1; j++)
+                    for (i = 1; i <= jpim1; i++)
+                        A3(zwz, i, j, k) = ((A3(pvn, i + 1, j, k) + A3(pvn, i, j, k)) * A2(di_e2v_2e1e2f, i, j) -
+                                            (A3(pun, i, j + 1, k) + A3(pun, i, j, k)) * A2(dj_e1u_2e1e2f, i, j)) *
+                                           A2(z1_e3f, i, j);
+                break;
+            case np_CRV:
+                for (j = 1; j <= jpjm1; j++)
+                    for (i = 1; i <= jpim1; i++)
+                        A3(zwz, i, j, k) = (ff_f(i, j) + CURL(pun, pvn, i, j, k) * r1_e1e2f(i, j)) * A2(z1_e3f, i, j);
+                break;
+            case np_CME:
+                for (j = 1; j <= jpjm1; j++)
+                    for (i = 1; i <= jpim1; i++)
+                        A3(zwz, i, j, k) = (ff_f(i, j) + (A3(pvn, i + 1, j, k) + A3(pvn, i, j, k)) * A2(di_e2v_2e1e2f, i, j) -
+                                            (A3(pun, i, j + 1, k) + A3(pun, i, j, k)) * A2(dj_e1u_2e1e2f, i, j)) *
+                                           A2(z1_e3f, i, j);
+                break;
+            }
+        } else {
+            /* vor_eeT: same vorticities as fvor_level but only on 1..jpim1 x 1..jpjm1 */
+            fvor_level(kvor, pun, pvn, k, zlev);
+            for (j = 1; j <= jpjm1; j++)
+                for (i = 1; i <= jpim1; i++) A3(zwz, i, j, k) = A2(zlev, i, j);
+        }
+    }
+    lbc3(zwz, jpk, 'F', 0.0);
+    for (k = 1; k <= jpkm1; k++) {
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A2(zwx, i, j) = e2u(i, j) * e3u_n(i, j, k) * A3(pun, i, j, k);
+                A2(zwy, i, j) = e1v(i, j) * e3v_n(i, j, k) * A3(pvn, i, j, k);
+            }
+        for (j = 1; j <= jpj; j++) A2(ztne, 1, j) = A2(ztnw, 1, j) = A2(ztse, 1, j) = A2(ztsw, 1, j) = 0.0;
+        for (j = 2; j <= jpj; j++)
+            for (i = 2; i <= jpi; i++) {
+                double a = A3(zwz, i - 1, j, k) + A3(zwz, i, j, k) + A3(zwz, i, j - 1, k);
+                double b = A3(zwz, i - 1, j - 1, k) + A3(zwz, i - 1, j, k) + A3(zwz, i, j, k);
+                double c = A3(zwz, i, j, k) + A3(zwz, i, j - 1, k) + A3(zwz, i - 1, j - 1, k);
+                double d = A3(zwz, i, j - 1, k) + A3(zwz, i - 1, j - 1, k) + A3(zwz, i - 1, j, k);
+                if (!use_e3f) {
+                    double z1_e3t = 1.0 / e3t_n(i, j, k);
+                    a = a * z1_e3t; b = b * z1_e3t; c = c * z1_e3t; d = d * z1_e3t;
+                }
+                A2(ztne, i, j) = a; A2(ztnw, i, j) = b; A2(ztse, i, j) = c; A2(ztsw, i, j) = d;
+            }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                double zua = R1_12 * r1_e1u(i, j) *
+                             (A2(ztne, i, j) * A2(zwy, i, j) + A2(ztnw, i + 1, j) * A2(zwy, i + 1, j) +
+                              A2(ztse, i, j) * A2(zwy, i, j - 1) + A2(ztsw, i + 1, j) * A2(zwy, i + 1, j - 1));
+                double zva = -(R1_12 * r1_e2v(i, j) *
+                               (A2(ztsw, i, j + 1) * A2(zwx, i - 1, j + 1) + A2(ztse, i, j + 1) * A2(zwx, i, j + 1) +
+                                A2(ztnw, i, j) * A2(zwx, i - 1, j) + A2(ztne, i, j) * A2(zwx, i, j)));
+                A3(pua, i, j, k) = A3(pua, i, j, k) + zua;
+                A3(pva, i, j, k) = A3(pva, i, j, k) + zva;
+            }
+    }
+    free(zwz); free(z1_e3f); free(zwx); free(zwy); free(ztne); free(ztnw); free(ztse); free(ztsw); free(zlev);
+}
+
+int ora_dyn_vor(int kt, const double *un, const double *vn, double *ua, double *va)
+{
+    (void)kt;
+    int ncor = np_COR, nrvm, ntot;
+    switch (D.n_dynadv) {
+    case NEMO_NP_LIN_DYN: nrvm = np_COR; ntot = np_COR; break;
+    case NEMO_NP_VEC_C2: nrvm = np_RVO; ntot = np_CRV; break;
+    default: nrvm = np_MET; ntot = np_CME; break;
+    }
+    switch (D.nvor_scheme) {
+    case NEMO_NP_ENT: vor_enT(ntot, un, vn, ua, va); break;
+    case NEMO_NP_EET: vor_een_eet(0, ntot, un, vn, ua, va); break;
+    case NEMO_NP_ENE: vor_ene(ntot, un, vn, ua, va); break;
+    case NEMO_NP_ENS: vor_ens(ntot, un, vn, ua, va); break;
+    case NEMO_NP_MIX: vor_ens(nrvm, un, vn, ua, va); vor_ene(ncor, un, vn, ua, va); break;
+    case NEMO_NP_EEN: vor_een_eet(1, ntot, un, vn, ua, va); break;
+    default: return NEMO_HIP_ERR_UNSUPPORTED;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a17: dyn_zdf, DYN/dynzdf.F90:47-499 (ln_zad_Aimp = F, iso-level         */
+/* momentum mixing, r_vvl = 0 under ln_linssh)                               */
+/* ------------------------------------------------------------------------ */
+int ora_dyn_zdf(int kt, const double *ub, const double *vb, double *ua, double *va, const double *avm,
+                const double *rCdU_bot, const double *ua_b, const double *va_b, const double *utau,
+                const double *utau_b, const double *vtau, const double *vtau_b)
+{
+    int i, j, k, iku, ikv;
+    const double r_vvl = 0.0;
+    if (D.neuler == 0 && kt == D.nit000) r2dt_dyn = D.rdt;
+    else if (kt <= D.nit000 + 1) r2dt_dyn = 2. * D.rdt;
+    const double r2dt = r2dt_dyn;
+    double *zwi = z3(), *zwd = z3(), *zws = z3();
+    /* leapfrog on all trends but vertical mixing, lines 110-114 */
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A3(ua, i, j, k) = (A3(ub, i, j, k) + r2dt * A3(ua, i, j, k)) * UMASK(i, j, k);
+                A3(va, i, j, k) = (A3(vb, i, j, k) + r2dt * A3(va, i, j, k)) * VMASK(i, j, k);
+            }
+    if (D.ln_drgimp && D.ln_dynspg_ts) {
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 1; j <= jpj; j++)
+                for (i = 1; i <= jpi; i++) {
+                    A3(ua, i, j, k) = (A3(ua, i, j, k) - A2(ua_b, i, j)) * UMASK(i, j, k);
+                    A3(va, i, j, k) = (A3(va, i, j, k) - A2(va_b, i, j)) * VMASK(i, j, k);
+                }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                iku = A2(mbku, i, j);
+                ikv = A2(mbkv, i, j);
+                double ze3ua = (1.0 - r_vvl) * e3u_n(i, j, iku) + r_vvl * e3u_n(i, j, iku);
+                double ze3va = (1.0 - r_vvl) * e3v_n(i, j, ikv) + r_vvl * e3v_n(i, j, ikv);
+                A3(ua, i, j, iku) = A3(ua, i, j, iku) +
+                                    r2dt * 0.5 * (A2(rCdU_bot, i + 1, j) + A2(rCdU_bot, i, j)) * A2(ua_b, i, j) / ze3ua;
+                A3(va, i, j, ikv) = A3(va, i, j, ikv) +
+                                    r2dt * 0.5 * (A2(rCdU_bot, i, j + 1) + A2(rCdU_bot, i, j)) * A2(va_b, i, j) / ze3va;
+            }
+    }
+    const double zdt = r2dt * 0.5;
+    for (int comp = 0; comp < 2; comp++) {
+        double *pa = comp ? va : ua;
+        const double *e3x = comp ? D.e3v_n : D.e3u_n;
+        const double *e3xw = comp ? D.e3vw_n : D.e3uw_n;
+        const double *wxm = comp ? wvmask : wumask;
+        const double *xmask = comp ? vmask : umask;
+        const double *tau = comp ? vtau : utau, *tau_b = comp ? vtau_b : utau_b;
+        const int *mbkx = comp ? mbkv : mbku;
+        int di = comp ? 0 : 1, dj = comp ? 1 : 0;
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++) {
+                    double ze3 = (1.0 - r_vvl) * A3(e3x, i, j, k) + r_vvl * A3(e3x, i, j, k);
+                    double zzwi = -(zdt * (A3(avm, i + di, j + dj, k) + A3(avm, i, j, k)) / (ze3 * A3(e3xw, i, j, k)) *
+                                    A3(wxm, i, j, k));
+                    double zzws = -(zdt * (A3(avm, i + di, j + dj, k + 1) + A3(avm, i, j, k + 1)) /
+                                    (ze3 * A3(e3xw, i, j, k + 1)) * A3(wxm, i, j, k + 1));
+                    A3(zwi, i, j, k) = zzwi;
+                    A3(zws, i, j, k) = zzws;
+                    A3(zwd, i, j, k) = 1.0 - zzwi - zzws;
+                }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A3(zwi, i, j, 1) = 0.0;
+                A3(zwd, i, j, 1) = 1.0 - A3(zws, i, j, 1);
+            }
+        if (D.ln_drgimp) {
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++) {
+                    int ik = A2(mbkx, i, j);
+                    double ze3 = (1.0 - r_vvl) * A3(e3x, i, j, ik) + r_vvl * A3(e3x, i, j, ik);
+                    A3(zwd, i, j, ik) =
+                        A3(zwd, i, j, ik) - r2dt * 0.5 * (A2(rCdU_bot, i + di, j + dj) + A2(rCdU_bot, i, j)) / ze3;
+                }
+        }
+        for (k = 2; k <= jpkm1; k++)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++)
+                    A3(zwd, i, j, k) = A3(zwd, i, j, k) - A3(zwi, i, j, k) * A3(zws, i, j, k - 1) / A3(zwd, i, j, k - 1);
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                double ze3 = (1.0 - r_vvl) * A3(e3x, i, j, 1) + r_vvl * A3(e3x, i, j, 1);
+                A3(pa, i, j, 1) = A3(pa, i, j, 1) + r2dt * 0.5 * (A2(tau_b, i, j) + A2(tau, i, j)) / (ze3 * D.rau0) *
+                                                        A3(xmask, i, j, 1);
+            }
+        for (k = 2; k <= jpkm1; k++)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++)
+                    A3(pa, i, j, k) = A3(pa, i, j, k) - A3(zwi, i, j, k) / A3(zwd, i, j, k - 1) * A3(pa, i, j, k - 1);
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) A3(pa, i, j, jpkm1) = A3(pa, i, j, jpkm1) / A3(zwd, i, j, jpkm1);
+        for (k = jpk - 2; k >= 1; k--)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++)
+                    A3(pa, i, j, k) = (A3(pa, i, j, k) - A3(zws, i, j, k) * A3(pa, i, j, k + 1)) / A3(zwd, i, j, k);
+    }
+    free(zwi); free(zwd); free(zws);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* a18: dyn_spg_ts, DYN/dynspg_ts.F90:121-1388                              */
+/* ------------------------------------------------------------------------ */
+int ora_ts_wgt(int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2)
+{
+    int nn_baro = D.nn_baro, jic, jn, ji;
+    for (jn = 0; jn < 3 * nn_baro; jn++) zwgt1[jn] = zwgt2[jn] = 0.0;
+    jic = ll_fw ? nn_baro : 2 * nn_baro;
+    if (ll_av) {
+        if (D.nn_bt_flt == 0) {
+            zwgt1[jic - 1] = 1.0; *jpit = jic;
+        } else {
+            float lim = (D.nn_bt_flt == 1) ? 0.5f : 1.0f;
+            if (D.nn_bt_flt != 1 && D.nn_bt_flt != 2) return NEMO_HIP_ERR_UNSUPPORTED;
+            for (jn = 1; jn <= 3 * nn_baro; jn++) {
+                /* float(): default REAL division, compared with a REAL(wp) limit */
+                float za1 = fabsf((float)(jn - jic)) / (float)nn_baro;
+                if ((double)za1 < (double)lim) { zwgt1[jn - 1] = 1.0; *jpit = jn; }
+            }
+        }
+    } else {
+        zwgt1[jic - 1] = 1.0; *jpit = jic;
+    }
+    for (jn = 1; jn <= *jpit; jn++)
+        for (ji = jn; ji <= *jpit; ji++) zwgt2[jn - 1] = zwgt2[jn - 1] + zwgt1[ji - 1];
+    double s1 = 0.0, s2 = 0.0;
+    for (jn = 1; jn <= *jpit; jn++) { s1 += zwgt1[jn - 1]; s2 += zwgt2[jn - 1]; }
+    double za1 = 1.0 / s1, za2 = 1.0 / s2;
+    for (jn = 1; jn <= *jpit; jn++) { zwgt1[jn - 1] = zwgt1[jn - 1] * za1; zwgt2[jn - 1] = zwgt2[jn - 1] * za2; }
+    return 0;
+}
+
+/* barotropic state kept between calls: the *_e module arrays (dynspg_ts.F90 uses them
+   from sbc_oce / oce); with ln_bt_av they are reset every call */
+static double *sshbb_e, *sshb_e, *ubb_e, *ub_e, *vbb_e, *vb_e;
+
+static void coriolis_bt(const double *zwx, const double *zwy, const double *zwz, const double *ftne,
+                        const double *ftnw, const double *ftse, const double *ftsw, double *zu_trd, double *zv_trd)
+{
+    int i, j;
+    switch (D.nvor_scheme) {
+    case NEMO_NP_ENE: case NEMO_NP_MIX:
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                double zy1 = (A2(zwy, i, j - 1) + A2(zwy, i + 1, j - 1)) * r1_e1u(i, j);
+                double zy2 = (A2(zwy, i, j) + A2(zwy, i + 1, j)) * r1_e1u(i, j);
+                double zx1 = (A2(zwx, i - 1, j) + A2(zwx, i - 1, j + 1)) * r1_e2v(i, j);
+                double zx2 = (A2(zwx, i, j) + A2(zwx, i, j + 1)) * r1_e2v(i, j);
+                A2(zu_trd, i, j) = r1_4 * (A2(zwz, i, j - 1) * zy1 + A2(zwz, i, j) * zy2);
+                A2(zv_trd, i, j) = -(r1_4 * (A2(zwz, i - 1, j) * zx1 + A2(zwz, i, j) * zx2));
+            }
+        break;
+    case NEMO_NP_ENS:
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                double zy1 = r1_8 * (A2(zwy, i, j - 1) + A2(zwy, i + 1, j - 1) + A2(zwy, i, j) + A2(zwy, i + 1, j)) * r1_e1u(i, j);
+                double zx1 = -(r1_8 * (A2(zwx, i - 1, j) + A2(zwx, i - 1, j + 1) + A2(zwx, i, j) + A2(zwx, i, j + 1)) * r1_e2v(i, j));
+                A2(zu_trd, i, j) = zy1 * (A2(zwz, i, j - 1) + A2(zwz, i, j));
+                A2(zv_trd, i, j) = zx1 * (A2(zwz, i - 1, j) + A2(zwz, i, j));
+            }
+        break;
+    case NEMO_NP_EET: case NEMO_NP_EEN:
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A2(zu_trd, i, j) = R1_12 * r1_e1u(i, j) *
+                                   (A2(ftne, i, j) * A2(zwy, i, j) + A2(ftnw, i + 1, j) * A2(zwy, i + 1, j) +
+                                    A2(ftse, i, j) * A2(zwy, i, j - 1) + A2(ftsw, i + 1, j) * A2(zwy, i + 1, j - 1));
+                A2(zv_trd, i, j) = -(R1_12 * r1_e2v(i, j) *
+                                     (A2(ftsw, i, j + 1) * A2(zwx, i - 1, j + 1) + A2(ftse, i, j + 1) * A2(zwx, i, j + 1) +
+                                      A2(ftnw, i, j) * A2(zwx, i - 1, j) + A2(ftne, i, j) * A2(zwx, i, j)));
+            }
+        break;
+    }
+}
+
+int ora_dyn_spg_ts(int kt, nemo_hip_spg_ts_t *st)
+{
+    int i, j, k, jn;
+    double *ua = (double *)st->ua, *va = (double *)st->va, *un = (double *)st->un, *vn = (double *)st->vn;
+    if (!D.ln_bt_fw) return NEMO_HIP_ERR_UNSUPPORTED;
+    const double r1_2 = 0.5, r1_rau0 = 1.0 / D.rau0;
+    double z2dt_bf = (kt == D.nit000 && D.neuler == 0) ? D.rdt : 2.0 * D.rdt;
+    double r1_2dt_b = 1.0 / z2dt_bf;
+    double rdtbt = D.rdt / (double)D.nn_baro;
+    int ll_init = D.ln_bt_av, ll_fw_start = 0, icycle = 0;
+    if (kt == D.nit000) {
+        if (D.neuler == 0) ll_init = 1;
+    }
+    ll_fw_start = 1; /* ln_bt_fw */
+    double *wgtbtp1 = calloc(3 * D.nn_baro, sizeof(double)), *wgtbtp2 = calloc(3 * D.nn_baro, sizeof(double));
+    int stw = ora_ts_wgt(D.ln_bt_av, ll_fw_start, &icycle, wgtbtp1, wgtbtp2);
+    if (stw) { free(wgtbtp1); free(wgtbtp2); return stw; }
+    if (!sshbb_e) {
+        sshbb_e = z2(); sshb_e = z2(); ubb_e = z2(); ub_e = z2(); vbb_e = z2(); vb_e = z2();
+    }
+    double *zCdU_u = z2(), *zCdU_v = z2(), *zwz = z2(), *zhf = z2(), *ftne = z2(), *ftnw = z2(), *ftse = z2(),
+           *ftsw = z2();
+    double *zu_frc = z2(), *zv_frc = z2(), *zu_trd = z2(), *zv_trd = z2(), *zwx = z2(), *zwy = z2(), *zssh_frc = z2();
+    double *zhdiv = z2(), *zsshp2_e = z2(), *ua_e = z2(), *va_e = z2(), *sshn_e = z2(), *un_e = z2(), *vn_e = z2();
+    double *hur_e = z2(), *hvr_e = z2(), *ssha_e = z2();
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            A2(zCdU_u, i, j) = r1_2 * (A2(st->rCdU_bot, i + 1, j) + A2(st->rCdU_bot, i, j));
+            A2(zCdU_v, i, j) = r1_2 * (A2(st->rCdU_bot, i, j + 1) + A2(st->rCdU_bot, i, j));
+        }
+    /* Coriolis arrays, lines 241-342 */
+    switch (D.nvor_scheme) {
+    case NEMO_NP_EEN:
+        for (j = 1; j <= jpjm1; j++)
+            for (i = 1; i <= jpim1; i++) {
+                if (D.nn_een_e3f == 0) {
+                    A2(zwz, i, j) = (A2(st->ht_n, i, j + 1) + A2(st->ht_n, i + 1, j + 1) + A2(st->ht_n, i, j) +
+                                     A2(st->ht_n, i + 1, j)) *
+                                    0.25;
+                } else {
+                    A2(zwz, i, j) = (A2(st->ht_n, i, j + 1) + A2(st->ht_n, i + 1, j + 1) + A2(st->ht_n, i, j) +
+                                     A2(st->ht_n, i + 1, j)) /
+                                    (MAX2(1.0, A2(ssmask, i, j + 1) + A2(ssmask, i + 1, j + 1) + A2(ssmask, i, j) +
+                                                   A2(ssmask, i + 1, j)));
+                }
+                if (A2(zwz, i, j) != 0.0) A2(zwz, i, j) = ff_f(i, j) / A2(zwz, i, j);
+            }
+        lbc3(zwz, 1, 'F', 0.0);
+        for (j = 2; j <= jpj; j++)
+            for (i = 2; i <= jpi; i++) {
+                A2(ftne, i, j) = A2(zwz, i - 1, j) + A2(zwz, i, j) + A2(zwz, i, j - 1);
+                A2(ftnw, i, j) = A2(zwz, i - 1, j - 1) + A2(zwz, i - 1, j) + A2(zwz, i, j);
+                A2(ftse, i, j) = A2(zwz, i, j) + A2(zwz, i, j - 1) + A2(zwz, i - 1, j - 1);
+                A2(ftsw, i, j) = A2(zwz, i, j - 1) + A2(zwz, i - 1, j - 1) + A2(zwz, i - 1, j);
+            }
+        break;
+    case NEMO_NP_EET:
+        for (j = 2; j <= jpj; j++)
+            for (i = 2; i <= jpi; i++) {
+                double z1_ht = A2(ssmask, i, j) / (A2(st->ht_n, i, j) + 1.0 - A2(ssmask, i, j));
+                A2(ftne, i, j) = (ff_f(i - 1, j) + ff_f(i, j) + ff_f(i, j - 1)) * z1_ht;
+                A2(ftnw, i, j) = (ff_f(i - 1, j - 1) + ff_f(i - 1, j) + ff_f(i, j)) * z1_ht;
+                A2(ftse, i, j) = (ff_f(i, j) + ff_f(i, j - 1) + ff_f(i - 1, j - 1)) * z1_ht;
+                A2(ftsw, i, j) = (ff_f(i, j - 1) + ff_f(i - 1, j - 1) + ff_f(i - 1, j)) * z1_ht;
+            }
+        break;
+    case NEMO_NP_ENE: case NEMO_NP_ENS: case NEMO_NP_MIX:
+        for (j = 1; j <= jpjm1; j++)
+            for (i = 1; i <= jpi; i++) A2(zhf, i, j) = A2(zhf, i, j) * (1.0 - UMASK(i, j, 1) * UMASK(i, j + 1, 1));
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 1; j <= jpjm1; j++)
+                for (i = 1; i <= jpi; i++)
+                    A2(zhf, i, j) = A2(zhf, i, j) + e3f_n(i, j, k) * UMASK(i, j, k) * UMASK(i, j + 1, k);
+        lbc3(zhf, 1, 'F', 0.0);
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++)
+                if (A2(zhf, i, j) != 0.0) A2(zwz, i, j) = 1.0 / A2(zhf, i, j);
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) A2(zwz, i, j) = ff_f(i, j) * A2(zwz, i, j);
+        break;
+    default: break; /* ENT: t-point Coriolis, nothing to pre-compute */
+    }
+    /* Phase 1, lines 358-639 */
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A2(zu_frc, i, j) = A2(zu_frc, i, j) + e3u_n(i, j, k) * A3(ua, i, j, k) * UMASK(i, j, k);
+                A2(zv_frc, i, j) = A2(zv_frc, i, j) + e3v_n(i, j, k) * A3(va, i, j, k) * VMASK(i, j, k);
+            }
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) {
+            A2(zu_frc, i, j) = A2(zu_frc, i, j) * A2(st->r1_hu_n, i, j);
+            A2(zv_frc, i, j) = A2(zv_frc, i, j) * A2(st->r1_hv_n, i, j);
+        }
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A3(ua, i, j, k) = A3(ua, i, j, k) - A2(zu_frc, i, j) * UMASK(i, j, k);
+                A3(va, i, j, k) = A3(va, i, j, k) - A2(zv_frc, i, j) * VMASK(i, j, k);
+            }
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) {
+            A2(zwx, i, j) = A2(st->un_b, i, j) * A2(st->hu_n, i, j) * e2u(i, j);
+            A2(zwy, i, j) = A2(st->vn_b, i, j) * A2(st->hv_n, i, j) * e1v(i, j);
+        }
+    if (D.nvor_scheme == NEMO_NP_ENT) {
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A2(zu_trd, i, j) = r1_4 * r1_e1e2u(i, j) * A2(st->r1_hu_n, i, j) *
+                                   (e1e2t(i + 1, j) * A2(st->ht_n, i + 1, j) * ff_t(i + 1, j) *
+                                        (A2(st->vn_b, i + 1, j) + A2(st->vn_b, i + 1, j - 1)) +
+                                    e1e2t(i, j) * A2(st->ht_n, i, j) * ff_t(i, j) * (A2(st->vn_b, i, j) + A2(st->vn_b, i, j - 1)));
+                A2(zv_trd, i, j) = -(r1_4 * r1_e1e2v(i, j) * A2(st->r1_hv_n, i, j) *
+                                     (e1e2t(i, j + 1) * A2(st->ht_n, i, j + 1) * ff_t(i, j + 1) *
+                                          (A2(st->un_b, i, j + 1) + A2(st->un_b, i - 1, j + 1)) +
+                                      e1e2t(i, j) * A2(st->ht_n, i, j) * ff_t(i, j) * (A2(st->un_b, i, j) + A2(st->un_b, i - 1, j))));
+            }
+    } else {
+        coriolis_bt(zwx, zwy, zwz, ftne, ftnw, ftse, ftsw, zu_trd, zv_trd);
+    }
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            A2(zu_frc, i, j) = A2(zu_frc, i, j) - A2(zu_trd, i, j) * A2(ssumask, i, j);
+            A2(zv_frc, i, j) = A2(zv_frc, i, j) - A2(zv_trd, i, j) * A2(ssvmask, i, j);
+        }
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            A2(zwx, i, j) = A3(un, i, j, A2(mbku, i, j)) - A2(st->un_b, i, j);
+            A2(zwy, i, j) = A3(vn, i, j, A2(mbkv, i, j)) - A2(st->vn_b, i, j);
+        }
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            A2(zu_frc, i, j) = A2(zu_frc, i, j) + A2(st->r1_hu_n, i, j) * r1_2 *
+                                                      (A2(st->rCdU_bot, i + 1, j) + A2(st->rCdU_bot, i, j)) * A2(zwx, i, j);
+            A2(zv_frc, i, j) = A2(zv_frc, i, j) + A2(st->r1_hv_n, i, j) * r1_2 *
+                                                      (A2(st->rCdU_bot, i, j + 1) + A2(st->rCdU_bot, i, j)) * A2(zwy, i, j);
+        }
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            A2(zu_frc, i, j) = A2(zu_frc, i, j) + r1_rau0 * A2(st->utau, i, j) * A2(st->r1_hu_n, i, j);
+            A2(zv_frc, i, j) = A2(zv_frc, i, j) + r1_rau0 * A2(st->vtau, i, j) * A2(st->r1_hv_n, i, j);
+        }
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++)
+            A2(zssh_frc, i, j) = r1_rau0 * (A2(st->emp, i, j) - (st->rnf ? A2(st->rnf, i, j) : 0.0) + 0.0);
+    /* Phase 2, lines 665-1207 */
+    if (ll_init) {
+        memset(sshbb_e, 0, n2() * sizeof(double)); memset(ubb_e, 0, n2() * sizeof(double));
+        memset(vbb_e, 0, n2() * sizeof(double)); memset(sshb_e, 0, n2() * sizeof(double));
+        memset(ub_e, 0, n2() * sizeof(double)); memset(vb_e, 0, n2() * sizeof(double));
+    }
+    memcpy(sshn_e, st->sshn, n2() * sizeof(double));
+    memcpy(un_e, st->un_b, n2() * sizeof(double));
+    memcpy(vn_e, st->vn_b, n2() * sizeof(double));
+    memcpy(hur_e, st->r1_hu_n, n2() * sizeof(double));
+    memcpy(hvr_e, st->r1_hv_n, n2() * sizeof(double));
+    memset(st->ua_b, 0, n2() * sizeof(double)); memset(st->va_b, 0, n2() * sizeof(double));
+    memset(st->ssha, 0, n2() * sizeof(double));
+    memset(st->un_adv, 0, n2() * sizeof(double)); memset(st->vn_adv, 0, n2() * sizeof(double));
+    for (jn = 1; jn <= icycle; jn++) {
+        double za0, za1, za2, za3;
+        if (jn < 3 && ll_init) { za1 = 1.0; za2 = 0.0; za3 = 0.0; }
+        else { za1 = 1.781105; za2 = -1.06221; za3 = 0.281105; }
+        for (size_t s = 0; s < n2(); s++) {
+            ua_e[s] = za1 * un_e[s] + za2 * ub_e[s] + za3 * ubb_e[s];
+            va_e[s] = za1 * vn_e[s] + za2 * vb_e[s] + za3 * vbb_e[s];
+        }
+        /* linssh: zhup2_e = hu_n, zhvp2_e = hv_n, zhtp2_e = ht_n */
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A2(zwx, i, j) = e2u(i, j) * A2(ua_e, i, j) * A2(st->hu_n, i, j);
+                A2(zwy, i, j) = e1v(i, j) * A2(va_e, i, j) * A2(st->hv_n, i, j);
+            }
+        za2 = wgtbtp2[jn - 1];
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A2(st->un_adv, i, j) = A2(st->un_adv, i, j) + za2 * A2(zwx, i, j) * r1_e2u(i, j);
+                A2(st->vn_adv, i, j) = A2(st->vn_adv, i, j) + za2 * A2(zwy, i, j) * r1_e1v(i, j);
+            }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++)
+                A2(zhdiv, i, j) = (A2(zwx, i, j) - A2(zwx, i - 1, j) + A2(zwy, i, j) - A2(zwy, i, j - 1)) * r1_e1e2t(i, j);
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++)
+                A2(ssha_e, i, j) = (A2(sshn_e, i, j) - rdtbt * (A2(zssh_frc, i, j) + A2(zhdiv, i, j))) * A2(ssmask, i, j);
+        lbc3(ssha_e, 1, 'T', 0.0);
+        if (jn == 1 && ll_init) { za0 = 1.0; za1 = 0.0; za2 = 0.0; za3 = 0.0; }
+        else if (jn == 2 && ll_init) { za0 = 1.0833333333333; za1 = -0.1666666666666; za2 = 0.0833333333333; za3 = 0.0; }
+        else { za0 = 0.614; za1 = 0.285; za2 = 0.088; za3 = 0.013; }
+        for (size_t s = 0; s < n2(); s++)
+            zsshp2_e[s] = za0 * ssha_e[s] + za1 * sshn_e[s] + za2 * sshb_e[s] + za3 * sshbb_e[s];
+        if (D.nvor_scheme == NEMO_NP_ENT) {
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++) {
+                    double z1_hu = A2(ssumask, i, j) / (A2(st->hu_n, i, j) + 1.0 - A2(ssumask, i, j));
+                    double z1_hv = A2(ssvmask, i, j) / (A2(st->hv_n, i, j) + 1.0 - A2(ssvmask, i, j));
+                    A2(zu_trd, i, j) = r1_4 * r1_e1e2u(i, j) * z1_hu *
+                                       (e1e2t(i + 1, j) * A2(st->ht_n, i + 1, j) * ff_t(i + 1, j) *
+                                            (A2(va_e, i + 1, j) + A2(va_e, i + 1, j - 1)) +
+                                        e1e2t(i, j) * A2(st->ht_n, i, j) * ff_t(i, j) * (A2(va_e, i, j) + A2(va_e, i, j - 1)));
+                    A2(zv_trd, i, j) = -(r1_4 * r1_e1e2v(i, j) * z1_hv *
+                                         (e1e2t(i, j + 1) * A2(st->ht_n, i, j + 1) * ff_t(i, j + 1) *
+                                              (A2(ua_e, i, j + 1) + A2(ua_e, i - 1, j + 1)) +
+                                          e1e2t(i, j) * A2(st->ht_n, i, j) * ff_t(i, j) * (A2(ua_e, i, j) + A2(ua_e, i - 1, j))));
+                }
+        } else {
+            coriolis_bt(zwx, zwy, zwz, ftne, ftnw, ftse, ftsw, zu_trd, zv_trd);
+        }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A2(zu_trd, i, j) = A2(zu_trd, i, j) + A2(zCdU_u, i, j) * A2(un_e, i, j) * A2(hur_e, i, j);
+                A2(zv_trd, i, j) = A2(zv_trd, i, j) + A2(zCdU_v, i, j) * A2(vn_e, i, j) * A2(hvr_e, i, j);
+            }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                double zu_spg = -(D.grav * (A2(zsshp2_e, i + 1, j) - A2(zsshp2_e, i, j)) * r1_e1u(i, j));
+                double zv_spg = -(D.grav * (A2(zsshp2_e, i, j + 1) - A2(zsshp2_e, i, j)) * r1_e2v(i, j));
+                A2(zwx, i, j) = (1.0 - D.rn_scal_load) * zu_spg;
+                A2(zwy, i, j) = (1.0 - D.rn_scal_load) * zv_spg;
+            }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A2(ua_e, i, j) = (A2(un_e, i, j) + rdtbt * (A2(zwx, i, j) + A2(zu_trd, i, j) + A2(zu_frc, i, j))) *
+                                 A2(ssumask, i, j);
+                A2(va_e, i, j) = (A2(vn_e, i, j) + rdtbt * (A2(zwy, i, j) + A2(zv_trd, i, j) + A2(zv_frc, i, j))) *
+                                 A2(ssvmask, i, j);
+            }
+        lbc3(ua_e, 1, 'U', 0.0);
+        lbc3(va_e, 1, 'V', 0.0);
+        memcpy(ubb_e, ub_e, n2() * sizeof(double)); memcpy(ub_e, un_e, n2() * sizeof(double));
+        memcpy(un_e, ua_e, n2() * sizeof(double));
+        memcpy(vbb_e, vb_e, n2() * sizeof(double)); memcpy(vb_e, vn_e, n2() * sizeof(double));
+        memcpy(vn_e, va_e, n2() * sizeof(double));
+        memcpy(sshbb_e, sshb_e, n2() * sizeof(double)); memcpy(sshb_e, sshn_e, n2() * sizeof(double));
+        memcpy(sshn_e, ssha_e, n2() * sizeof(double));
+        za1 = wgtbtp1[jn - 1];
+        for (size_t s = 0; s < n2(); s++) {
+            st->ua_b[s] = st->ua_b[s] + za1 * ua_e[s];
+            st->va_b[s] = st->va_b[s] + za1 * va_e[s];
+            st->ssha[s] = st->ssha[s] + za1 * ssha_e[s];
+        }
+    }
+    /* Phase 3, lines 1214-1269 */
+    for (size_t s = 0; s < n2(); s++) {
+        double zx = st->un_adv[s], zy = st->vn_adv[s];
+        if (!(kt == D.nit000 && D.neuler == 0)) {
+            st->un_adv[s] = r1_2 * (st->ub2_b[s] + zx - D.rn_atfp * st->un_bf[s]);
+            st->vn_adv[s] = r1_2 * (st->vb2_b[s] + zy - D.rn_atfp * st->vn_bf[s]);
+            st->un_bf[s] = D.rn_atfp * st->un_bf[s] + (zx - st->ub2_b[s]);
+            st->vn_bf[s] = D.rn_atfp * st->vn_bf[s] + (zy - st->vb2_b[s]);
+        } else {
+            st->un_bf[s] = 0.0; st->vn_bf[s] = 0.0;
+        }
+        st->ub2_b[s] = zx; st->vb2_b[s] = zy;
+    }
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A3(ua, i, j, k) = A3(ua, i, j, k) + (A2(st->ua_b, i, j) - A2(st->ub_b, i, j)) * r1_2dt_b;
+                A3(va, i, j, k) = A3(va, i, j, k) + (A2(st->va_b, i, j) - A2(st->vb_b, i, j)) * r1_2dt_b;
+            }
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A3(un, i, j, k) = (A3(un, i, j, k) + A2(st->un_adv, i, j) * A2(st->r1_hu_n, i, j) - A2(st->un_b, i, j)) *
+                                  UMASK(i, j, k);
+                A3(vn, i, j, k) = (A3(vn, i, j, k) + A2(st->vn_adv, i, j) * A2(st->r1_hv_n, i, j) - A2(st->vn_b, i, j)) *
+                                  VMASK(i, j, k);
+            }
+    double *fr[] = {zCdU_u, zCdU_v, zwz, zhf, ftne, ftnw, ftse, ftsw, zu_frc, zv_frc, zu_trd, zv_trd, zwx, zwy,
+                    zssh_frc, zhdiv, zsshp2_e, ua_e, va_e, sshn_e, un_e, vn_e, hur_e, hvr_e, ssha_e, wgtbtp1, wgtbtp2};
+    for (size_t q = 0; q < sizeof fr / sizeof fr[0]; q++) free(fr[q]);
+    return 0;
+}
+
+/* reset of the barotropic *_e state (a new run) */
+int ora_spg_ts_reset(void)
+{
+    free(sshbb_e); free(sshb_e); free(ubb_e); free(ub_e); free(vbb_e); free(vb_e);
+    sshbb_e = sshb_e = ubb_e = ub_e = vbb_e = vb_e = NULL;
+    return 0;
+}
+
+/* layout of the ABI structures as this compiler lays them out (checked
+   against the ctypes mirror by the tests) */
+#include <stddef.h>
+int ora_abi_layout(size_t *out, int n)
+{
+    size_t v[] = {sizeof(nemo_hip_domain_t), offsetof(nemo_hip_domain_t, rdt), offsetof(nemo_hip_domain_t, k_top),
+                  offsetof(nemo_hip_domain_t, e1t), offsetof(nemo_hip_domain_t, e3t_n),
+                  offsetof(nemo_hip_domain_t, e3vw_n), sizeof(nemo_hip_spg_ts_t),
+                  offsetof(nemo_hip_spg_ts_t, un_b), offsetof(nemo_hip_spg_ts_t, vn_bf)};
+    int m = (int)(sizeof v / sizeof v[0]);
+    for (int q = 0; q < n && q < m; q++) out[q] = v[q];
+    return m;
+}

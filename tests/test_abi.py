@@ -1,0 +1,43 @@
+"""CPU checks of the C-ABI boundary: the library loads, exports every entry
+point include/nemo_hip.h declares, and the ctypes mirror of the structures has
+the C layout (no GPU call is made)."""
+import ctypes as C
+
+import nemo_hip
+from nemo_hip import abi
+from oracle import oracle
+
+
+def test_header_declares_entry_points():
+    names = nemo_hip.exported_symbols()
+    for n in ["nemo_hip_init", "nemo_hip_domain_register", "nemo_hip_tra_adv_fct", "nemo_hip_tra_ldf_iso",
+              "nemo_hip_tra_zdf_imp", "nemo_hip_dyn_vor", "nemo_hip_dyn_zdf", "nemo_hip_dyn_spg_ts",
+              "nemo_hip_lbc_lnk", "nemo_hip_dom_msk"]:
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = nemo_hip.load_library()
+    missing = [n for n in nemo_hip.exported_symbols() if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_struct_layout_matches_c():
+    out = (C.c_size_t * 16)()
+    n = oracle.lib().ora_abi_layout(out, 16)
+    got = list(out[:n])
+    D, S = abi.DomainStruct, abi.SpgTsStruct
+    want = [C.sizeof(D), D.rdt.offset, D.k_top.offset, D.e1t.offset, D.e3t_n.offset, D.e3vw_n.offset,
+            C.sizeof(S), S.un_b.offset, S.vn_bf.offset]
+    assert got == want
+
+
+def test_init_without_gpu_fails_loudly():
+    import torch
+    if torch.cuda.is_available():
+        return
+    L = nemo_hip.load_library()
+    assert L.nemo_hip_init(0) != 0
+    buf = C.create_string_buffer(512)
+    L.nemo_hip_last_error(buf, 512)
+    assert b"device" in buf.value
