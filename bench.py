@@ -1,0 +1,326 @@
+#!/usr/bin/env python3
+"""Benchmark of the NEMO ocean time-step hot path on one MI355X.
+
+Workload: 1024 x 1024 x 75 seeded basin (about 30% land, lognormal bathymetry
+on full-step z-levels, 10 km grid), rdt = 600 s, nn_baro = 30, FCT 4/4,
+iso-neutral laplacian, ENE vorticity / vector-form momentum advection,
+implicit vertical diffusion and bottom drag.  One "step" is the hot path of
+step.F90 over the whole domain, with every input already resident in HBM:
+
+  ua = va = 0;  dyn_adv;  dyn_vor;  dyn_spg_ts;  dyn_zdf
+  tsa = 0;      tra_adv (transport + tra_adv_fct);  tra_ldf_iso;  tra_zdf
+
+Two storage forms are timed: the plain FP64 form and the FP32-storage form
+(FP32 stencils; FP64 tridiagonal eliminations and barotropic subcycles).
+value = grid-cell updates/s of the FP32 form = jpi*jpj*jpk / t_step.
+
+Also reported: per-routine / per-kernel times from HIP events on the
+library's stream, the roofline of the dominant kernel, the step's fraction of
+HBM peak from the algorithmic bytes, and the CPU oracle (plain C restatement
+of the reference loops, one core) timed on a bounded sub-domain.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "nemo-fmi-devel_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = json.load(open(os.path.join(ROOT, "BASELINE.json")))["metric"] if os.path.exists(
+    os.path.join(ROOT, "BASELINE.json")) else "grid-cell updates/s per step"
+HBM_PEAK = 8.0e12
+
+# Algorithmic HBM bytes per grid cell of each routine (3D arrays the routine's
+# arithmetic reads plus those it writes, once each, at storage width; masks,
+# 1-D e3 tables and 2D metrics count 0; temporaries count 0).
+#   dyn_adv (keg+zad)  r un vn wn ua va            w ua va          7
+#   dyn_vor            r un vn ua va               w ua va          6
+#   dyn_spg_ts         r ua va un vn (+ua va again for phase 3)     w ua va un vn   8
+#   dyn_zdf            r ub vb ua va avm           w ua va          7
+#   tra_adv_trp        r un vn wn                  w zun zvn zwn    6
+#   tra_adv_fct (x2)   r zun zvn zwn tb tn ta      w ta             3 + 2*4 = 11
+#   tra_ldf_iso        r ahtu ahtv uslp vslp wslpi wslpj tb*2 ta*2  w ta*2 ah_wslp2 akz  14
+#   tra_zdf            r avt ah_wslp2 tb*2 ta*2    w ta*2           8
+#   zeroing ua va tsa                              w 4              4
+ROUTINE_ARRAYS = {"dyn_adv": 7, "dyn_vor": 6, "dyn_spg_ts": 8, "dyn_zdf": 7, "tra_adv_trp": 6,
+                  "tra_adv_fct": 11, "tra_ldf_iso": 14, "tra_zdf": 8, "zero": 4}
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", default="1024x1024x75")
+    ap.add_argument("--forms", default="fp64,fp32")
+    ap.add_argument("--cpu-sample", default="256x256x75")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=20261204)
+    ap.add_argument("--timing-steps", type=int, default=2)
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_case(jpi, jpj, jpk, seed):
+    from nemo_hip import configs
+    opts = dict(nvor_scheme=1, n_dynadv=1, nn_dynkeg=0, ln_traldf_msc=0, nn_baro=30, rdt=600.0, neuler=0, nit000=1)
+    d = configs.basin(jpi, jpj, jpk, seed=seed, opts=opts)
+    return d
+
+
+class Stepper:
+    """The hot path of one time step, in one storage form, on resident arrays."""
+
+    def __init__(self, h, d, S, sp):
+        import torch
+        self.h, self.d, self.sp = h, d, sp
+        dt = torch.float32 if sp else torch.float64
+        n3 = d.jpi * d.jpj * d.jpk
+        c = lambda k: S[k].to(dt).contiguous()
+        self.f3 = {k: c(k) for k in ("un", "vn", "wn", "ub", "vb", "avt", "avm", "ahtu", "ahtv", "uslp", "vslp",
+                                     "wslpi", "wslpj")}
+        self.f3["avs"] = self.f3["avt"]
+        self.tsb, self.tsn = c("tsb"), c("tsn")
+        z = lambda n: __import__("torch").zeros(n, dtype=dt, device="cuda")
+        self.tsa, self.ua, self.va = z(2 * n3), z(n3), z(n3)
+        self.zun, self.zvn, self.zwn = z(n3), z(n3), z(n3)
+        self.ah, self.akz = z(n3), z(n3)
+        self.f2 = {k: S[k].clone() for k in ("un_b", "vn_b", "ub_b", "vb_b", "sshn", "hu_n", "hv_n", "r1_hu_n",
+                                             "r1_hv_n", "ht_n", "rCdU_bot", "utau", "vtau", "emp", "rnf", "ub2_b",
+                                             "vb2_b", "un_bf", "vn_bf", "utau_b", "vtau_b")}
+        n2 = d.jpi * d.jpj
+        for k in ("ua_b", "va_b", "ssha", "un_adv", "vn_adv"):
+            self.f2[k] = __import__("torch").zeros(n2, dtype=__import__("torch").float64, device="cuda")
+        self.un_dyn, self.vn_dyn = self.f3["un"].clone(), self.f3["vn"].clone()
+        self.spg = dict(self.f2)
+        self.spg.update(ua=self.ua, va=self.va, un=self.un_dyn, vn=self.vn_dyn)
+        self.r2dt = 2.0 * d.opts["rdt"]
+
+    def step(self, kt):
+        h, f, sp = self.h, self.f3, self.sp
+        # step.F90:179-180 and 234: trends start at zero
+        self.ua.zero_()
+        self.va.zero_()
+        self.tsa.zero_()
+        h.dyn_adv(kt, f["un"], f["vn"], f["wn"], self.ua, self.va, sp=sp)
+        h.dyn_vor(kt, f["un"], f["vn"], self.ua, self.va, sp=sp)
+        h.dyn_spg_ts(kt, self.spg, sp=sp)
+        h.dyn_zdf(kt, f["ub"], f["vb"], self.ua, self.va, f["avm"], self.f2["rCdU_bot"], self.f2["ua_b"],
+                  self.f2["va_b"], self.f2["utau"], self.f2["utau_b"], self.f2["vtau"], self.f2["vtau_b"], sp=sp)
+        h.tra_adv_trp(f["un"], f["vn"], f["wn"], self.zun, self.zvn, self.zwn, sp=sp)
+        h.tra_adv_fct(kt, 1, "TRA", self.r2dt, self.zun, self.zvn, self.zwn, self.tsb, self.tsn, self.tsa, 2, 4, 4,
+                      sp=sp)
+        h.tra_ldf_iso(kt, 1, "TRA", f["ahtu"], f["ahtv"], self.tsb, self.tsb, self.tsa, 2, 1, f["uslp"], f["vslp"],
+                      f["wslpi"], f["wslpj"], self.ah, self.akz, sp=sp)
+        h.tra_zdf(kt, self.tsb, self.tsa, f["avt"], f["avs"], 1, self.ah, self.akz, sp=sp)
+
+
+def time_form(h, d, S, sp, steps, warmup, timing_steps):
+    import torch
+    st = Stepper(h, d, S, sp)
+    ext = torch.cuda.ExternalStream(h.stream())
+    with torch.cuda.stream(ext):
+        kt = 2
+        for _ in range(warmup):
+            st.step(kt)
+            kt += 1
+        h.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            st.step(kt)
+            kt += 1
+        h.sync()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        # per-routine / per-kernel HIP-event timers, separate (short) run
+        h.timing_reset()
+        h.timing_enable(True)
+        for _ in range(timing_steps):
+            st.step(kt)
+            kt += 1
+        h.sync()
+        h.timing_enable(False)
+    ms = (t1 - t0) * 1e3 / steps
+    finite = bool(torch.isfinite(st.tsa).all().item() and torch.isfinite(st.ua).all().item())
+    del st
+    torch.cuda.empty_cache()
+    return ms, finite
+
+
+def kernel_times(h):
+    """{name: (total_ms, count)} of every timer the library accumulated."""
+    import ctypes as C
+    out = {}
+    names = ["dyn_adv", "dyn_vor", "dyn_spg_ts", "dyn_zdf", "tra_adv_trp", "tra_adv_fct", "tra_ldf_iso", "tra_zdf",
+             "lbc_lnk"]
+    kern = {
+        "dyn_adv": ["k_dyn_keg_zad"], "dyn_vor": ["k_dyn_vor"], "dyn_zdf": ["k_dyn_zdf"],
+        "dyn_spg_ts": ["k_spg_cor1", "k_spg_cor2", "k_spg_p1a", "k_spg_p1b", "k_spg_s1", "k_spg_s2", "k_spg_s3",
+                       "k_spg_p3a", "k_spg_p3b"],
+        "tra_adv_trp": ["k_tra_adv_trp"],
+        "tra_adv_fct": ["k_fct_low", "k_fct_ad", "k_fct_beta", "k_fct_final"],
+        "tra_ldf_iso": ["k_ldf_iso_coef", "k_tra_ldf_iso"], "tra_zdf": ["k_tra_zdf", "k_clamp_ring"],
+        "lbc_lnk": ["k_lbc"],
+    }
+    for n in names:
+        out[n] = h.timing_get(n)
+        for k in kern.get(n, []):
+            key = f"{n}/{k}"
+            out[key] = h.timing_get(key)
+    return out
+
+
+def cpu_baseline(sample, steps, seed):
+    """The oracle (one core, -O2, no FMA) on a bounded sub-domain of the same
+    physics: cells/s of the same step sequence."""
+    import numpy as np
+    from nemo_hip import configs
+    from oracle import oracle as ora
+    jpi, jpj, jpk = [int(x) for x in sample.split("x")]
+    d = build_case(jpi, jpj, jpk, seed)
+    S = configs.make_state(d, seed=seed + 1)
+    o = ora.Oracle(d)
+    s3 = (jpi, jpj, jpk)
+    F = {k: S[k].copy(order="F") for k in ("un_b", "vn_b", "ub_b", "vb_b", "sshn", "hu_n", "hv_n", "r1_hu_n",
+                                           "r1_hv_n", "ht_n", "rCdU_bot", "utau", "vtau", "emp", "rnf", "ub2_b",
+                                           "vb2_b", "un_bf", "vn_bf")}
+    for k in ("ua_b", "va_b", "ssha", "un_adv", "vn_adv"):
+        F[k] = np.zeros(s3[:2], order="F")
+    un, vn = S["un"].copy(order="F"), S["vn"].copy(order="F")
+    z = [np.zeros(s3, order="F") for _ in range(3)]
+    ah, akz = np.zeros(s3, order="F"), np.zeros(s3, order="F")
+    t0 = time.perf_counter()
+    for kt in range(2, 2 + steps):
+        ua, va = np.zeros(s3, order="F"), np.zeros(s3, order="F")
+        tsa = np.zeros(s3 + (2,), order="F")
+        o.dyn_adv(kt, S["un"], S["vn"], S["wn"], ua, va)
+        o.dyn_vor(kt, S["un"], S["vn"], ua, va)
+        F.update(ua=ua, va=va, un=un, vn=vn)
+        o.dyn_spg_ts(kt, F)
+        o.dyn_zdf(kt, S["ub"], S["vb"], ua, va, S["avm"], S["rCdU_bot"], F["ua_b"], F["va_b"], S["utau"],
+                  S["utau_b"], S["vtau"], S["vtau_b"])
+        o.tra_adv_trp(S["un"], S["vn"], S["wn"], *z)
+        o.tra_adv_fct(kt, 1, "TRA", 1200.0, *z, S["tsb"], S["tsn"], tsa, 2, 4, 4)
+        o.tra_ldf_iso(kt, 1, "TRA", S["ahtu"], S["ahtv"], S["tsb"], S["tsb"], tsa, 2, 1, S["uslp"], S["vslp"],
+                      S["wslpi"], S["wslpj"], ah, akz)
+        o.tra_zdf(kt, S["tsb"], tsa, S["avt"], S["avs"], 1, ah, akz)
+    t = time.perf_counter() - t0
+    cells = jpi * jpj * jpk
+    return {"value": cells * steps / t, "unit": "grid-cell updates/s", "cores": 1, "kind": "port",
+            "sample": f"{jpi}x{jpj}x{jpk} sub-domain of the same seeded physics, {steps} steps, "
+                      f"{t:.1f} s, plain C oracle -O2 -ffp-contract=off, 1 thread"}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
+              f"(the path shards as NEMO's (i,j) domain decomposition, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import nemo_hip
+    from nemo_hip import configs
+    jpi, jpj, jpk = [int(x) for x in a.size.split("x")]
+    cells = jpi * jpj * jpk
+    t0 = time.perf_counter()
+    d = build_case(jpi, jpj, jpk, a.seed)
+    log(f"domain {jpi}x{jpj}x{jpk} built in {time.perf_counter() - t0:.1f} s, wet fraction "
+        f"{float((d.arrays['k_top'] > 0).mean()):.3f}")
+    h = nemo_hip.get(0, nemo_hip.NEMO_HIP_PTR_DEVICE)
+    h.register(d)
+    t0 = time.perf_counter()
+    S = configs.make_state_device(d, seed=a.seed + 1)
+    torch.cuda.synchronize()
+    log(f"state generated on the device in {time.perf_counter() - t0:.1f} s")
+    res = {}
+    for form in a.forms.split(","):
+        sp = form == "fp32"
+        ms, finite = time_form(h, d, S, sp, a.steps, a.warmup, a.timing_steps)
+        kt = kernel_times(h)
+        res[form] = {"ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3), "finite": finite,
+                     "routines_ms_per_step": {k: v[0] / a.timing_steps for k, v in kt.items()
+                                              if "/" not in k and v[1] > 0},
+                     "kernels_ms_per_step": {k: v[0] / a.timing_steps for k, v in kt.items()
+                                             if "/" in k and v[1] > 0},
+                     "kernel_launches_per_step": {k: v[1] // a.timing_steps for k, v in kt.items() if v[1] > 0}}
+        width = 4 if sp else 8
+        alg = sum(ROUTINE_ARRAYS.values()) * width * cells
+        res[form]["algorithmic_bytes_per_step"] = alg
+        res[form]["hbm_frac_step"] = alg / (ms * 1e-3) / HBM_PEAK
+        log(f"{form}: {ms:.2f} ms/step, {cells / (ms * 1e-3):.3e} cell-updates/s, "
+            f"HBM fraction of step {res[form]['hbm_frac_step']:.3f}, finite={finite}")
+        for k, v in sorted(res[form]["routines_ms_per_step"].items(), key=lambda x: -x[1]):
+            log(f"   {k:14s} {v:8.3f} ms")
+    # roofline of the dominant single kernel of the headline form
+    head = "fp32" if "fp32" in res else list(res)[0]
+    width = 4 if head == "fp32" else 8
+    kms = res[head]["kernels_ms_per_step"]
+    klaunch = res[head]["kernel_launches_per_step"]
+    roof = None
+    if kms:
+        kname = max(kms, key=kms.get)
+        n_launch = max(1, klaunch.get(kname, 1))
+        avg_ms = kms[kname] / n_launch
+        arrays = KERNEL_BYTES.get(kname.split("/")[1]) if "/" in kname else None
+        if arrays is not None:
+            alg_bytes = arrays(width, jpi, jpj, jpk)
+            roof = {"kernel": kname, "bound": "hbm", "achieved": alg_bytes / (avg_ms * 1e-3) / 1e9,
+                    "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": alg_bytes / (avg_ms * 1e-3) / HBM_PEAK,
+                    "traffic": None, "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": alg_bytes}
+    cpu = None
+    if not a.no_cpu:
+        t0 = time.perf_counter()
+        cpu = cpu_baseline(a.cpu_sample, a.cpu_steps, a.seed)
+        log(f"cpu baseline {cpu['value']:.3e} cells/s ({time.perf_counter() - t0:.1f} s incl. setup)")
+    h_ms = res[head]["ms_per_step"]
+    line = {
+        "metric": METRIC,
+        "value": res[head]["cell_updates_per_s"],
+        "unit": "grid-cell updates/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": h_ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 storage (f64 tridiagonal + barotropic)" if head == "fp32" else "f64",
+        "data": "synthetic: seeded land mask, lognormal bathymetry, streamfunction velocities",
+        "config": {"workload": f"NEMO hot-path step (dyn_adv+dyn_vor+dyn_spg_ts+dyn_zdf+tra_adv_fct+tra_ldf_iso"
+                               f"+tra_zdf), {jpi}x{jpj}x{jpk}, FCT 4/4, nn_baro 30, rdt 600 s",
+                   "jpi": jpi, "jpj": jpj, "jpk": jpk, "form": head},
+        "hbm_frac_step": res[head]["hbm_frac_step"],
+        "forms": res,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if "fp32" in res and "fp64" in res:
+        line["fp32_speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"]
+    print(json.dumps(line))
+
+
+# algorithmic bytes per launch of the kernels the roofline line may name
+# (width = bytes per stored value; see DESIGN.md for the per-kernel accounting)
+def _b(n_arrays_interior):
+    return lambda w, i, j, k: n_arrays_interior * w * (i - 2) * (j - 2) * (k - 1)
+
+
+KERNEL_BYTES = {
+    "k_tra_zdf": _b(8),          # r avt ah_wslp2 tb*2 ta*2, w ta*2
+    "k_tra_ldf_iso": _b(11),     # r ahtu ahtv uslp vslp wslpi wslpj ah_wslp2 akz tb ta, w ta (one tracer)
+    "k_dyn_zdf": _b(7),          # r ub vb ua va avm, w ua va
+    "k_dyn_vor": _b(6),          # r un vn ua va, w ua va
+    "k_dyn_keg_zad": _b(7),      # r un vn wn ua va, w ua va
+    "k_fct_final": _b(7),        # r zwx zwy zwz zbetup zbetdo ta, w ta
+    "k_fct_beta": _b(7),         # r tb zwi zwx zwy zwz, w zbetup zbetdo
+    "k_fct_ad": _b(8),           # r pun pvn pwn tb tn, w zwx zwy zwz
+    "k_fct_low": _b(7),          # r pun pvn pwn tb ta, w ta zwi
+    "k_spg_p1a": _b(6),          # r ua va un vn (bottom), w ua va
+    "k_spg_p3b": _b(8),          # r ua va un vn, w ua va un vn
+    "k_tra_adv_trp": lambda w, i, j, k: 6 * w * i * j * (k - 1),
+}
+
+if __name__ == "__main__":
+    main()

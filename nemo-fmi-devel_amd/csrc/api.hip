@@ -44,6 +44,29 @@ static std::map<std::string, std::pair<double, int64_t>> g_tot;
 static std::string g_open_name;
 static hipEvent_t g_open_ev = nullptr;
 
+// accumulate completed (start, end) pairs and recycle their events; a start
+// event shared by two entries is recycled once
+static void release_pending()
+{
+    std::vector<hipEvent_t> seen;
+    for (auto &p : g_pend) {
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, p.a, p.b);
+        auto &t = g_tot[p.name];
+        t.first += ms;
+        t.second += 1;
+        bool dup = false;
+        for (auto e : seen) dup = dup || (e == p.a);
+        if (!dup) {
+            seen.push_back(p.a);
+            g_evfree.push_back(p.a);
+        }
+        g_evfree.push_back(p.b);
+        if (seen.size() > 8) seen.erase(seen.begin());
+    }
+    g_pend.clear();
+}
+
 int set_error(int code, const char *fmt, ...)
 {
     va_list ap;
@@ -76,42 +99,43 @@ void timing_begin(const char *name)
     g_open_name = name;
     if (g_open_ev) (void)hipEventRecord(g_open_ev, g_stream);
 }
-void timing_end(const char *name)
+// timers are kept per routine ("tra_adv_fct") and per kernel of the routine
+// ("tra_adv_fct/k_fct_low"); the kernel text is the launch expression with
+// its template arguments and parentheses removed
+static std::string kernel_key(const char *name, const char *kernel)
+{
+    std::string k;
+    int depth = 0;
+    for (const char *c = kernel; *c; c++) {
+        if (*c == '<') depth++;
+        else if (*c == '>') depth--;
+        else if (depth == 0 && *c != '(' && *c != ')' && *c != ' ') k += *c;
+    }
+    return std::string(name) + "/" + k;
+}
+void timing_end(const char *name, const char *kernel)
 {
     if (!g_timing || !g_open_ev) return;
     hipEvent_t b = ev_get();
-    if (!b) return;
+    hipEvent_t b2 = ev_get();
+    if (!b || !b2) return;
     (void)hipEventRecord(b, g_stream);
+    hipEvent_t a2 = g_open_ev;
+    // the routine total and the per-kernel entry share the start event
     g_pend.push_back({std::string(name), g_open_ev, b});
+    (void)hipEventRecord(b2, g_stream);
+    g_pend.push_back({kernel_key(name, kernel), a2, b2});
     g_open_ev = nullptr;
     if (g_pend.size() > 4096) {
         // fold early so the event pool stays bounded
         (void)hipStreamSynchronize(g_stream);
-        for (auto &p : g_pend) {
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, p.a, p.b);
-            auto &t = g_tot[p.name];
-            t.first += ms;
-            t.second += 1;
-            g_evfree.push_back(p.a);
-            g_evfree.push_back(p.b);
-        }
-        g_pend.clear();
+        release_pending();
     }
 }
 static int fold_timing()
 {
     HIPCHK(hipStreamSynchronize(g_stream));
-    for (auto &p : g_pend) {
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
-        auto &t = g_tot[p.name];
-        t.first += ms;
-        t.second += 1;
-        g_evfree.push_back(p.a);
-        g_evfree.push_back(p.b);
-    }
-    g_pend.clear();
+    release_pending();
     return 0;
 }
 
@@ -489,8 +513,7 @@ int nemo_hip_finalize(void)
     free_domain();
     for (auto &p : g_pool) if (p.first) (void)hipFree(p.first);
     g_pool.clear();
-    for (auto &p : g_pend) { g_evfree.push_back(p.a); g_evfree.push_back(p.b); }
-    g_pend.clear();
+    release_pending();
     for (auto e : g_evfree) (void)hipEventDestroy(e);
     g_evfree.clear();
     (void)hipStreamDestroy(g_stream);

@@ -159,7 +159,7 @@ const DevDomain &dom();
 bool have_domain();
 // per-routine timing: begin/end bracket kernel launches under a name
 void timing_begin(const char *name);
-void timing_end(const char *name);
+void timing_end(const char *name, const char *kernel);
 // library-owned scratch buffers (HBM), sized at domain registration
 enum Scratch {
     S_ZWI, S_ZWX, S_ZWY, S_ZWZ, S_ZBETUP, S_ZBETDO, S_ZTW, S_ZWT, S_ZWT2, S_NSCRATCH3
@@ -179,6 +179,6 @@ double *scratch2(int which);   // n2 doubles each (dyn_spg_ts work arrays)
     do {                                                                          \
         nh::timing_begin(name);                                                   \
         hipLaunchKernelGGL(kern, (g).grid, (g).block, 0, nh::stream(), __VA_ARGS__); \
-        nh::timing_end(name);                                                     \
+        nh::timing_end(name, #kern);                                              \
         HIPCHK(hipGetLastError());                                                \
     } while (0)

@@ -275,8 +275,19 @@ def make_state(d, seed=1, umax=0.2, divergent=0.2, kz_range=(1e-5, 1e-1), kz_con
     S["tsn"] = np.asfortranarray(tn)
     S["tsb"] = np.asfortranarray(tb)
     # velocities
-    psi = np.stack([_smooth(rng.standard_normal((jpi, jpj)), sig) for _ in range(jpk)], axis=2)
-    phi = np.stack([_smooth(rng.standard_normal((jpi, jpj)), sig) for _ in range(jpk)], axis=2)
+    separable = jpi * jpj * jpk > 4_000_000
+    if separable:
+        # large grids: two smooth horizontal modes times random vertical
+        # structure (the 3D fields of the small cases cost one FFT per level)
+        def field3d():
+            a, b = _smooth(rng.standard_normal((jpi, jpj)), sig), _smooth(rng.standard_normal((jpi, jpj)), sig)
+            za, zb = rng.standard_normal(jpk), rng.standard_normal(jpk)
+            return a[:, :, None] * za[None, None, :] + b[:, :, None] * zb[None, None, :]
+    else:
+        def field3d():
+            return np.stack([_smooth(rng.standard_normal((jpi, jpj)), sig) for _ in range(jpk)], axis=2)
+    psi = field3d()
+    phi = field3d()
     psi *= f
     e1, e2 = A["e1u"][:, :, None], A["e2u"][:, :, None]
     un = fzeros((jpi, jpj, jpk))
@@ -323,7 +334,7 @@ def make_state(d, seed=1, umax=0.2, divergent=0.2, kz_range=(1e-5, 1e-1), kz_con
     S["ahtv"] = np.asfortranarray(aht * v)
     # iso-neutral slopes (smooth, |slope| <= slope_max), LDF/ldfslp.F90 masking
     def slope(mask):
-        s = np.stack([_smooth(rng.standard_normal((jpi, jpj)), sig) for _ in range(jpk)], axis=2)
+        s = field3d()
         s = s / max(1e-30, np.abs(s).max()) * slope_max
         return np.asfortranarray(s * mask)
     S["uslp"], S["vslp"], S["wslpi"], S["wslpj"] = slope(u), slope(v), slope(w), slope(w)
@@ -364,3 +375,147 @@ def make_state(d, seed=1, umax=0.2, divergent=0.2, kz_range=(1e-5, 1e-1), kz_con
     S["va"] = np.asfortranarray(1e-6 * rng.standard_normal((jpi, jpj, jpk)) * v)
     S["masks"] = (t, u, v, w, f)
     return S
+
+
+# ----------------------------------------------------------------------
+# the same construction built directly in HBM (benchmark sizes)
+# ----------------------------------------------------------------------
+def make_state_device(d, seed=1, umax=0.2, divergent=0.2, kz_range=(1e-5, 1e-1), aht=1000.0, slope_max=0.01,
+                      sst_noise=0.01, device="cuda"):
+    """make_state's fields generated with torch on the GPU: every 3D field is a
+    tensor of shape (jpk, jpj, jpi), i.e. NEMO's Fortran layout, flattened by
+    .reshape(-1).  Horizontal structure: FFT-smoothed noise; vertical
+    structure: two random modes per field (as make_state on large grids)."""
+    import torch
+    g = torch.Generator(device=device)
+    g.manual_seed(int(seed))
+    jpi, jpj, jpk = d.jpi, d.jpj, d.jpk
+    f64 = torch.float64
+    A = d.arrays
+    T2 = lambda a: torch.from_numpy(np.ascontiguousarray(np.asarray(a).T)).to(device)   # (jpj, jpi)
+    sig = max(1.5, min(jpi, jpj) / 16.0)
+
+    def randn(*shape):
+        return torch.randn(*shape, generator=g, device=device, dtype=f64)
+
+    def smooth2(noise):
+        kx = torch.fft.fftfreq(jpi, device=device, dtype=f64)[None, :]
+        ky = torch.fft.fftfreq(jpj, device=device, dtype=f64)[:, None]
+        gk = torch.exp(-2.0 * (np.pi * sig) ** 2 * (kx ** 2 + ky ** 2))
+        return torch.real(torch.fft.ifft2(torch.fft.fft2(noise) * gk))
+
+    def field3d():
+        a, b = smooth2(randn(jpj, jpi)), smooth2(randn(jpj, jpi))
+        za, zb = randn(jpk), randn(jpk)
+        return a[None] * za[:, None, None] + b[None] * zb[:, None, None]
+
+    # masks, DOM/dommsk.F90 (closed or cyclic ring as in the domain's k_top/k_bot)
+    kt, kb = T2(A["k_top"]).long(), T2(A["k_bot"]).long()
+    k1 = torch.arange(1, jpk + 1, device=device)[:, None, None]
+    t = ((kt[None] != 0) & (k1 >= kt[None]) & (k1 <= kb[None])).to(f64)
+    iper, jper = d.jperio in (1, 4, 6, 7), d.jperio in (2, 7)
+
+    def lbc(a, nat):
+        a = a.clone()
+        if iper:
+            a[..., 0], a[..., -1] = a[..., -2].clone(), a[..., 1].clone()
+        else:
+            if nat != "F":
+                a[..., 0] = 0
+            a[..., -1] = 0
+        if jper:
+            a[..., 0, :], a[..., -1, :] = a[..., -2, :].clone(), a[..., 1, :].clone()
+        else:
+            if nat != "F":
+                a[..., 0, :] = 0
+            a[..., -1, :] = 0
+        return a
+    t = lbc(t, "T")
+    u = torch.zeros_like(t)
+    v = torch.zeros_like(t)
+    f = torch.zeros_like(t)
+    u[:, :-1, :-1] = t[:, :-1, :-1] * t[:, :-1, 1:]
+    v[:, :-1, :-1] = t[:, :-1, :-1] * t[:, 1:, :-1]
+    f[:, :-1, :-1] = t[:, :-1, :-1] * t[:, :-1, 1:] * t[:, 1:, :-1] * t[:, 1:, 1:]
+    u, v, f = lbc(u, "U"), lbc(v, "V"), lbc(f, "F")
+    w = t.clone()
+    w[1:] = t[1:] * t[:-1]
+    S = {}
+    gdept = torch.from_numpy(np.cumsum(d.e3t_1d) - 0.5 * d.e3t_1d).to(device)
+    prof = 4.0 + 21.0 * torch.exp(-gdept / 800.0)
+    tn = torch.empty((2, jpk, jpj, jpi), dtype=f64, device=device)
+    tn[0] = (prof[:, None, None] + sst_noise * randn(jpk, jpj, jpi)) * t
+    tn[1] = (35.0 + sst_noise * randn(jpk, jpj, jpi)) * t
+    S["tsn"] = tn
+    S["tsb"] = tn + 0.01 * randn(2, jpk, jpj, jpi) * t[None]
+    psi = field3d() * f
+    phi = field3d()
+    un = torch.zeros_like(t)
+    vn = torch.zeros_like(t)
+    un[:, 1:, :] = -(psi[:, 1:, :] - psi[:, :-1, :])
+    vn[:, :, 1:] = psi[:, :, 1:] - psi[:, :, :-1]
+    un[:, :, :-1] += divergent * (phi[:, :, 1:] - phi[:, :, :-1])
+    vn[:, :-1, :] += divergent * (phi[:, 1:, :] - phi[:, :-1, :])
+    un *= u
+    vn *= v
+    sc = umax / torch.sqrt(torch.mean(un[u > 0] ** 2)).item()
+    un *= sc
+    vn *= sc
+    e3t = torch.from_numpy(d.e3t_1d).to(device)[:, None, None]
+    e2u, e1v, r1_e1e2t = T2(A["e2u"]), T2(A["e1v"]), T2(A["r1_e1e2t"])
+    zu = e2u[None] * e3t * un
+    zv = e1v[None] * e3t * vn
+    hdiv = torch.zeros_like(t)
+    hdiv[:, 1:-1, 1:-1] = ((zu[:, 1:-1, 1:-1] - zu[:, 1:-1, :-2] + zv[:, 1:-1, 1:-1] - zv[:, :-2, 1:-1])
+                           * r1_e1e2t[None, 1:-1, 1:-1] / e3t)
+    # wn(k) = wn(k+1) - e3t*hdiv (masked), wn(jpk) = 0
+    inc = (e3t * hdiv * t)[:-1]
+    wn = torch.zeros_like(t)
+    wn[:-1] = torch.flip(torch.cumsum(torch.flip(inc, [0]), 0), [0]) * t[:-1]
+    S["un"], S["vn"], S["wn"] = un, vn, wn
+    S["ub"] = un * (1.0 + 0.05 * randn(jpk, jpj, jpi)) * u
+    S["vb"] = vn * (1.0 + 0.05 * randn(jpk, jpj, jpi)) * v
+    lo, hi = np.log(kz_range[0]), np.log(kz_range[1])
+    wk = w.clone()
+    wk[0] = 0
+    unif = lambda: torch.rand(jpk, jpj, jpi, generator=g, device=device, dtype=f64)
+    S["avt"] = torch.exp(lo + (hi - lo) * unif()) * wk
+    S["avs"] = S["avt"]
+    S["avm"] = torch.exp(lo + (hi - lo) * unif()) * wk
+    S["ahtu"] = aht * u
+    S["ahtv"] = aht * v
+
+    def slope(mask):
+        s = field3d()
+        return s / s.abs().max() * slope_max * mask
+    S["uslp"], S["vslp"], S["wslpi"], S["wslpj"] = slope(u), slope(v), slope(w), slope(w)
+    ssm = (t.amax(0) > 0).to(f64)
+    ssum = (u.amax(0) > 0).to(f64)
+    ssvm = (v.amax(0) > 0).to(f64)
+    hu = (e3t * u).sum(0)
+    hv = (e3t * v).sum(0)
+    S["hu_n"], S["hv_n"], S["ht_n"] = hu, hv, (e3t * t).sum(0)
+    S["r1_hu_n"] = ssum / (hu + 1.0 - ssum)
+    S["r1_hv_n"] = ssvm / (hv + 1.0 - ssvm)
+    S["un_b"] = (e3t * un * u).sum(0) * S["r1_hu_n"]
+    S["vn_b"] = (e3t * vn * v).sum(0) * S["r1_hv_n"]
+    S["ub_b"] = (e3t * S["ub"] * u).sum(0) * S["r1_hu_n"]
+    S["vb_b"] = (e3t * S["vb"] * v).sum(0) * S["r1_hv_n"]
+    S["sshn"] = 0.1 * smooth2(randn(jpj, jpi)) * ssm
+    S["rCdU_bot"] = -(1e-4 + 9e-4 * torch.rand(jpj, jpi, generator=g, device=device, dtype=f64)) * ssm
+    ly = jpj * float(A["e2t"][0, 0])
+    yy = (torch.arange(jpj, device=device, dtype=f64) - 0.5) * float(A["e2t"][0, 0])
+    S["utau"] = -0.1 * torch.cos(2 * np.pi * yy / ly)[:, None] * u[0]
+    S["vtau"] = torch.zeros_like(hu)
+    S["utau_b"] = S["utau"] * 0.98
+    S["vtau_b"] = torch.zeros_like(hu)
+    S["emp"] = 1e-5 * smooth2(randn(jpj, jpi)) * ssm
+    S["rnf"] = torch.zeros_like(hu)
+    S["ub2_b"] = S["un_b"] * hu * e2u
+    S["vb2_b"] = S["vn_b"] * hv * e1v
+    S["un_bf"] = 0.01 * S["ub2_b"]
+    S["vn_bf"] = 0.01 * S["vb2_b"]
+    S["ua_b"] = S["un_b"] * 1.01
+    S["va_b"] = S["vn_b"] * 1.01
+    S["tmask"] = t
+    return {k: (v.contiguous().reshape(-1) if k != "tmask" else v) for k, v in S.items()}
