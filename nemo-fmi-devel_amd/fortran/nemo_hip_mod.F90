@@ -1,0 +1,320 @@
+MODULE nemo_hip_mod
+   !!======================================================================
+   !!                     ***  MODULE  nemo_hip_mod  ***
+   !! Drop-in replacements of NEMO 4.0 hot-path routines that run on an
+   !! MI355X through the C-ABI of include/nemo_hip.h.
+   !!
+   !! Each public routine keeps the dummy-argument list of the routine it
+   !! replaces (paths relative to src/OCE):
+   !!   tra_adv_fct    TRA/traadv_fct.F90:54        interp_4th_cpt TRA/traadv_fct.F90:517
+   !!   tra_ldf_lap    TRA/traldf_lap_blp.F90:47    tra_ldf_iso    TRA/traldf_iso.F90:50
+   !!   tra_zdf_imp    TRA/trazdf.F90:109
+   !!   dyn_keg        DYN/dynkeg.F90:46            dyn_zad        DYN/dynzad.F90:38
+   !!   dyn_adv_cen2   DYN/dynadv_cen2.F90:37       dyn_vor        DYN/dynvor.F90:98
+   !!   dyn_zdf        DYN/dynzdf.F90:47            dyn_spg_ts     DYN/dynspg_ts.F90:121
+   !! The arrays those routines take from NEMO modules (oce, zdf_oce, ldfslp,
+   !! sbc_oce, zdfdrg, dynspg_ts) are bound once, after allocation, with the
+   !! nemo_hip_bind_* routines: module arrays never move, so their addresses
+   !! stay valid for the run.  A nonzero status is turned into ctl_stop by
+   !! the caller-supplied error hook (default: print and ERROR STOP).
+   !!======================================================================
+   USE, INTRINSIC :: ISO_C_BINDING
+   IMPLICIT NONE
+   PRIVATE
+
+   INTEGER, PARAMETER :: wp = C_DOUBLE
+
+   PUBLIC :: nemo_hip_start, nemo_hip_stop, nemo_hip_domain
+   PUBLIC :: nemo_hip_bind_zdf, nemo_hip_bind_ldfslp, nemo_hip_bind_dyn, nemo_hip_bind_spg
+   PUBLIC :: tra_adv_fct, interp_4th_cpt, tra_ldf_lap, tra_ldf_iso, tra_zdf_imp
+   PUBLIC :: dyn_keg, dyn_zad, dyn_adv_cen2, dyn_vor, dyn_zdf, dyn_spg_ts
+   PUBLIC :: nemo_hip_domain_t, nemo_hip_spg_ts_t
+
+   !! mirror of nemo_hip_domain_t (same field order and kinds)
+   TYPE, BIND(C) :: nemo_hip_domain_t
+      INTEGER(C_INT32_T) :: jpi, jpj, jpk, jperio
+      INTEGER(C_INT32_T) :: ln_linssh, ln_isfcav, ln_zps, ln_sco
+      INTEGER(C_INT32_T) :: ln_traldf_lap, ln_traldf_msc, ln_zdfddm, ln_zad_Aimp
+      INTEGER(C_INT32_T) :: n_dynadv, nn_dynkeg, nvor_scheme, nn_een_e3f
+      INTEGER(C_INT32_T) :: ln_drgimp
+      INTEGER(C_INT32_T) :: ln_dynspg_ts, ln_bt_fw, ln_bt_av, nn_bt_flt, nn_baro
+      INTEGER(C_INT32_T) :: neuler, nit000, reserved0
+      REAL(C_DOUBLE)     :: rdt, rn_atfp, rau0, grav, rn_bt_alpha, rn_scal_load
+      TYPE(C_PTR) :: k_top, k_bot
+      TYPE(C_PTR) :: e1t, e2t, e1u, e2u, e1v, e2v, e1f, e2f
+      TYPE(C_PTR) :: e1e2t, r1_e1e2t, e1e2u, r1_e1e2u, e1e2v, r1_e1e2v, e1e2f, r1_e1e2f
+      TYPE(C_PTR) :: e2_e1u, e1_e2v, r1_e1u, r1_e2u, r1_e1v, r1_e2v
+      TYPE(C_PTR) :: ff_f, ff_t
+      TYPE(C_PTR) :: e3t_n, e3u_n, e3v_n, e3w_n, e3f_n, e3uw_n, e3vw_n
+   END TYPE nemo_hip_domain_t
+
+   !! mirror of nemo_hip_spg_ts_t
+   TYPE, BIND(C) :: nemo_hip_spg_ts_t
+      TYPE(C_PTR) :: ua, va, un, vn
+      TYPE(C_PTR) :: un_b, vn_b, ub_b, vb_b, sshn, hu_n, hv_n, r1_hu_n, r1_hv_n, ht_n
+      TYPE(C_PTR) :: rCdU_bot, utau, vtau, emp, rnf
+      TYPE(C_PTR) :: ua_b, va_b, ssha, un_adv, vn_adv
+      TYPE(C_PTR) :: ub2_b, vb2_b, un_bf, vn_bf
+   END TYPE nemo_hip_spg_ts_t
+
+   !! module arrays bound at initialisation
+   TYPE(C_PTR), SAVE :: p_avt = C_NULL_PTR, p_avs = C_NULL_PTR, p_avm = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_uslp = C_NULL_PTR, p_vslp = C_NULL_PTR, p_wslpi = C_NULL_PTR, p_wslpj = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_ah_wslp2 = C_NULL_PTR, p_akz = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_un = C_NULL_PTR, p_vn = C_NULL_PTR, p_wn = C_NULL_PTR, p_ub = C_NULL_PTR, p_vb = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_ua = C_NULL_PTR, p_va = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_rCdU_bot = C_NULL_PTR, p_ua_b = C_NULL_PTR, p_va_b = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_utau = C_NULL_PTR, p_utau_b = C_NULL_PTR, p_vtau = C_NULL_PTR, p_vtau_b = C_NULL_PTR
+   INTEGER, SAVE :: l_ldfslp = 0
+   TYPE(nemo_hip_spg_ts_t), SAVE :: spg_state
+
+   INTERFACE
+      INTEGER(C_INT) FUNCTION c_init(device) BIND(C, name="nemo_hip_init")
+         IMPORT :: C_INT
+         INTEGER(C_INT), VALUE :: device
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_finalize() BIND(C, name="nemo_hip_finalize")
+         IMPORT :: C_INT
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_last_error(buf, len) BIND(C, name="nemo_hip_last_error")
+         IMPORT :: C_INT, C_CHAR
+         CHARACTER(KIND=C_CHAR), DIMENSION(*) :: buf
+         INTEGER(C_INT), VALUE :: len
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_domain_register(dom) BIND(C, name="nemo_hip_domain_register")
+         IMPORT :: C_INT, nemo_hip_domain_t
+         TYPE(nemo_hip_domain_t), INTENT(in) :: dom
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_tra_adv_fct(kt, kit000, cdtype, p2dt, pun, pvn, pwn, ptb, ptn, pta, kjpt,   &
+         &                                  kn_fct_h, kn_fct_v) BIND(C, name="nemo_hip_tra_adv_fct")
+         IMPORT :: C_INT, C_CHAR, C_DOUBLE, C_PTR
+         INTEGER(C_INT), VALUE :: kt, kit000, kjpt, kn_fct_h, kn_fct_v
+         CHARACTER(KIND=C_CHAR), DIMENSION(*) :: cdtype
+         REAL(C_DOUBLE), VALUE :: p2dt
+         TYPE(C_PTR), VALUE :: pun, pvn, pwn, ptb, ptn, pta
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_interp_4th_cpt(pin, pout) BIND(C, name="nemo_hip_interp_4th_cpt")
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: pin, pout
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_tra_ldf_lap(kt, kit000, cdtype, pahu, pahv, pgu, pgv, pgui, pgvi, ptb, pta,   &
+         &                                  kjpt, kpass) BIND(C, name="nemo_hip_tra_ldf_lap")
+         IMPORT :: C_INT, C_CHAR, C_PTR
+         INTEGER(C_INT), VALUE :: kt, kit000, kjpt, kpass
+         CHARACTER(KIND=C_CHAR), DIMENSION(*) :: cdtype
+         TYPE(C_PTR), VALUE :: pahu, pahv, pgu, pgv, pgui, pgvi, ptb, pta
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_tra_ldf_iso(kt, kit000, cdtype, pahu, pahv, pgu, pgv, pgui, pgvi, ptb, ptbb, pta, &
+         &                                  kjpt, kpass, uslp, vslp, wslpi, wslpj, ah_wslp2, akz)             &
+         &                                  BIND(C, name="nemo_hip_tra_ldf_iso")
+         IMPORT :: C_INT, C_CHAR, C_PTR
+         INTEGER(C_INT), VALUE :: kt, kit000, kjpt, kpass
+         CHARACTER(KIND=C_CHAR), DIMENSION(*) :: cdtype
+         TYPE(C_PTR), VALUE :: pahu, pahv, pgu, pgv, pgui, pgvi, ptb, ptbb, pta
+         TYPE(C_PTR), VALUE :: uslp, vslp, wslpi, wslpj, ah_wslp2, akz
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_tra_zdf_imp(kt, kit000, cdtype, p2dt, ptb, pta, kjpt, avt, avs, l_ldfslp,   &
+         &                                  ah_wslp2, akz) BIND(C, name="nemo_hip_tra_zdf_imp")
+         IMPORT :: C_INT, C_CHAR, C_DOUBLE, C_PTR
+         INTEGER(C_INT), VALUE :: kt, kit000, kjpt, l_ldfslp
+         CHARACTER(KIND=C_CHAR), DIMENSION(*) :: cdtype
+         REAL(C_DOUBLE), VALUE :: p2dt
+         TYPE(C_PTR), VALUE :: ptb, pta, avt, avs, ah_wslp2, akz
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_dyn_keg(kt, kscheme, un, vn, ua, va) BIND(C, name="nemo_hip_dyn_keg")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt, kscheme
+         TYPE(C_PTR), VALUE :: un, vn, ua, va
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_dyn_zad(kt, un, vn, wn, ua, va) BIND(C, name="nemo_hip_dyn_zad")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: un, vn, wn, ua, va
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_dyn_adv_cen2(kt, un, vn, wn, ua, va) BIND(C, name="nemo_hip_dyn_adv_cen2")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: un, vn, wn, ua, va
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_dyn_vor(kt, un, vn, ua, va) BIND(C, name="nemo_hip_dyn_vor")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: un, vn, ua, va
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_dyn_zdf(kt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau, utau_b, vtau, vtau_b) &
+         &                              BIND(C, name="nemo_hip_dyn_zdf")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau, utau_b, vtau, vtau_b
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_dyn_spg_ts(kt, st) BIND(C, name="nemo_hip_dyn_spg_ts")
+         IMPORT :: C_INT, nemo_hip_spg_ts_t
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(nemo_hip_spg_ts_t), INTENT(inout) :: st
+      END FUNCTION
+   END INTERFACE
+
+CONTAINS
+
+   SUBROUTINE check( kstat, cdname )
+      !! turns a library status into a stop, the way ctl_stop ends a NEMO run
+      INTEGER(C_INT)  , INTENT(in) :: kstat
+      CHARACTER(len=*), INTENT(in) :: cdname
+      CHARACTER(KIND=C_CHAR) :: buf(512)
+      CHARACTER(len=512) :: msg
+      INTEGER :: i, ist
+      IF( kstat == 0 ) RETURN
+      ist = c_last_error( buf, 512_C_INT )
+      msg = ' '
+      DO i = 1, 512
+         IF( buf(i) == C_NULL_CHAR ) EXIT
+         msg(i:i) = buf(i)
+      END DO
+      WRITE(*,'(a,a,a,i0,a,a)') 'nemo_hip: ', cdname, ' failed (status ', kstat, '): ', TRIM(msg)
+      ERROR STOP 'nemo_hip'
+   END SUBROUTINE check
+
+   FUNCTION cstr( cd ) RESULT( c )
+      CHARACTER(len=*), INTENT(in) :: cd
+      CHARACTER(KIND=C_CHAR) :: c(LEN(cd)+1)
+      INTEGER :: i
+      DO i = 1, LEN(cd)
+         c(i) = cd(i:i)
+      END DO
+      c(LEN(cd)+1) = C_NULL_CHAR
+   END FUNCTION cstr
+
+   SUBROUTINE nemo_hip_start( kdevice )
+      INTEGER, INTENT(in) :: kdevice
+      CALL check( c_init( INT(kdevice, C_INT) ), 'nemo_hip_init' )
+   END SUBROUTINE nemo_hip_start
+
+   SUBROUTINE nemo_hip_stop
+      CALL check( c_finalize(), 'nemo_hip_finalize' )
+   END SUBROUTINE nemo_hip_stop
+
+   SUBROUTINE nemo_hip_domain( sdom )
+      !! sdom: sizes, switches and C_LOC of the dom_oce arrays (see nemo_hip.h)
+      TYPE(nemo_hip_domain_t), INTENT(in) :: sdom
+      CALL check( c_domain_register( sdom ), 'nemo_hip_domain_register' )
+   END SUBROUTINE nemo_hip_domain
+
+   SUBROUTINE nemo_hip_bind_zdf( pavt, pavs, pavm )
+      !! zdf_oce module arrays (avt, avs, avm)
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(in) :: pavt, pavs, pavm
+      p_avt = C_LOC( pavt ) ; p_avs = C_LOC( pavs ) ; p_avm = C_LOC( pavm )
+   END SUBROUTINE nemo_hip_bind_zdf
+
+   SUBROUTINE nemo_hip_bind_ldfslp( puslp, pvslp, pwslpi, pwslpj, pah_wslp2, pakz )
+      !! ldfslp module arrays; binding them switches on l_ldfslp for tra_zdf_imp
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(in) :: puslp, pvslp, pwslpi, pwslpj, pah_wslp2, pakz
+      p_uslp = C_LOC( puslp ) ; p_vslp = C_LOC( pvslp ) ; p_wslpi = C_LOC( pwslpi ) ; p_wslpj = C_LOC( pwslpj )
+      p_ah_wslp2 = C_LOC( pah_wslp2 ) ; p_akz = C_LOC( pakz )
+      l_ldfslp = 1
+   END SUBROUTINE nemo_hip_bind_ldfslp
+
+   SUBROUTINE nemo_hip_bind_dyn( pun, pvn, pwn, pub, pvb, pua, pva, prCdU_bot, pua_b, pva_b, putau, putau_b, pvtau, pvtau_b )
+      !! oce (un, vn, wn, ub, vb, ua, va), zdfdrg (rCdU_bot), oce (ua_b, va_b), sbc_oce (utau, vtau, ...)
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(in) :: pun, pvn, pwn, pub, pvb, pua, pva
+      REAL(wp), DIMENSION(:,:)  , TARGET, CONTIGUOUS, INTENT(in) :: prCdU_bot, pua_b, pva_b, putau, putau_b, pvtau, pvtau_b
+      p_un = C_LOC( pun ) ; p_vn = C_LOC( pvn ) ; p_wn = C_LOC( pwn ) ; p_ub = C_LOC( pub ) ; p_vb = C_LOC( pvb )
+      p_ua = C_LOC( pua ) ; p_va = C_LOC( pva )
+      p_rCdU_bot = C_LOC( prCdU_bot ) ; p_ua_b = C_LOC( pua_b ) ; p_va_b = C_LOC( pva_b )
+      p_utau = C_LOC( putau ) ; p_utau_b = C_LOC( putau_b ) ; p_vtau = C_LOC( pvtau ) ; p_vtau_b = C_LOC( pvtau_b )
+   END SUBROUTINE nemo_hip_bind_dyn
+
+   SUBROUTINE nemo_hip_bind_spg( sstate )
+      !! C_LOC of every array dyn_spg_ts reads or writes (oce, dom_oce, sbc_oce, dynspg_ts)
+      TYPE(nemo_hip_spg_ts_t), INTENT(in) :: sstate
+      spg_state = sstate
+   END SUBROUTINE nemo_hip_bind_spg
+
+   !!----------------------------------------------------------------------
+   !! drop-in routines
+   !!----------------------------------------------------------------------
+   SUBROUTINE tra_adv_fct( kt, kit000, cdtype, p2dt, pun, pvn, pwn, ptb, ptn, pta, kjpt, kn_fct_h, kn_fct_v )
+      INTEGER                            , INTENT(in   ) ::   kt, kit000, kjpt, kn_fct_h, kn_fct_v
+      CHARACTER(len=3)                   , INTENT(in   ) ::   cdtype
+      REAL(wp)                           , INTENT(in   ) ::   p2dt
+      REAL(wp), DIMENSION(:,:,:)  , TARGET, CONTIGUOUS, INTENT(in   ) ::   pun, pvn, pwn
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(in   ) ::   ptb, ptn
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(inout) ::   pta
+      CALL check( c_tra_adv_fct( INT(kt,C_INT), INT(kit000,C_INT), cstr(cdtype), REAL(p2dt,C_DOUBLE),         &
+         &        C_LOC(pun), C_LOC(pvn), C_LOC(pwn), C_LOC(ptb), C_LOC(ptn), C_LOC(pta),                         &
+         &        INT(kjpt,C_INT), INT(kn_fct_h,C_INT), INT(kn_fct_v,C_INT) ), 'tra_adv_fct' )
+   END SUBROUTINE tra_adv_fct
+
+   SUBROUTINE interp_4th_cpt( pt_in, pt_out )
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(in   ) ::   pt_in
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(inout) ::   pt_out
+      CALL check( c_interp_4th_cpt( C_LOC(pt_in), C_LOC(pt_out) ), 'interp_4th_cpt' )
+   END SUBROUTINE interp_4th_cpt
+
+   SUBROUTINE tra_ldf_lap( kt, kit000, cdtype, pahu, pahv, pgu, pgv, pgui, pgvi, ptb, pta, kjpt, kpass )
+      INTEGER                            , INTENT(in   ) ::   kt, kit000, kjpt, kpass
+      CHARACTER(len=3)                   , INTENT(in   ) ::   cdtype
+      REAL(wp), DIMENSION(:,:,:)  , TARGET, CONTIGUOUS, INTENT(in   ) ::   pahu, pahv
+      REAL(wp), DIMENSION(:,:,:)  , TARGET, CONTIGUOUS, INTENT(in   ) ::   pgu, pgv, pgui, pgvi
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(in   ) ::   ptb
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(inout) ::   pta
+      CALL check( c_tra_ldf_lap( INT(kt,C_INT), INT(kit000,C_INT), cstr(cdtype), C_LOC(pahu), C_LOC(pahv),     &
+         &        C_LOC(pgu), C_LOC(pgv), C_LOC(pgui), C_LOC(pgvi), C_LOC(ptb), C_LOC(pta),                       &
+         &        INT(kjpt,C_INT), INT(kpass,C_INT) ), 'tra_ldf_lap' )
+   END SUBROUTINE tra_ldf_lap
+
+   SUBROUTINE tra_ldf_iso( kt, kit000, cdtype, pahu, pahv, pgu, pgv, pgui, pgvi, ptb, ptbb, pta, kjpt, kpass )
+      INTEGER                            , INTENT(in   ) ::   kt, kit000, kjpt, kpass
+      CHARACTER(len=3)                   , INTENT(in   ) ::   cdtype
+      REAL(wp), DIMENSION(:,:,:)  , TARGET, CONTIGUOUS, INTENT(in   ) ::   pahu, pahv
+      REAL(wp), DIMENSION(:,:,:)  , TARGET, CONTIGUOUS, INTENT(in   ) ::   pgu, pgv, pgui, pgvi
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(in   ) ::   ptb, ptbb
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(inout) ::   pta
+      CALL check( c_tra_ldf_iso( INT(kt,C_INT), INT(kit000,C_INT), cstr(cdtype), C_LOC(pahu), C_LOC(pahv),     &
+         &        C_LOC(pgu), C_LOC(pgv), C_LOC(pgui), C_LOC(pgvi), C_LOC(ptb), C_LOC(ptbb), C_LOC(pta),          &
+         &        INT(kjpt,C_INT), INT(kpass,C_INT), p_uslp, p_vslp, p_wslpi, p_wslpj, p_ah_wslp2, p_akz ),      &
+         &        'tra_ldf_iso' )
+   END SUBROUTINE tra_ldf_iso
+
+   SUBROUTINE tra_zdf_imp( kt, kit000, cdtype, p2dt, ptb, pta, kjpt )
+      INTEGER                            , INTENT(in   ) ::   kt, kit000, kjpt
+      CHARACTER(len=3)                   , INTENT(in   ) ::   cdtype
+      REAL(wp)                           , INTENT(in   ) ::   p2dt
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(in   ) ::   ptb
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(inout) ::   pta
+      CALL check( c_tra_zdf_imp( INT(kt,C_INT), INT(kit000,C_INT), cstr(cdtype), REAL(p2dt,C_DOUBLE),         &
+         &        C_LOC(ptb), C_LOC(pta), INT(kjpt,C_INT), p_avt, p_avs, INT(l_ldfslp,C_INT),                      &
+         &        p_ah_wslp2, p_akz ), 'tra_zdf_imp' )
+   END SUBROUTINE tra_zdf_imp
+
+   SUBROUTINE dyn_keg( kt, kscheme )
+      INTEGER, INTENT(in) ::   kt, kscheme
+      CALL check( c_dyn_keg( INT(kt,C_INT), INT(kscheme,C_INT), p_un, p_vn, p_ua, p_va ), 'dyn_keg' )
+   END SUBROUTINE dyn_keg
+
+   SUBROUTINE dyn_zad( kt )
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_dyn_zad( INT(kt,C_INT), p_un, p_vn, p_wn, p_ua, p_va ), 'dyn_zad' )
+   END SUBROUTINE dyn_zad
+
+   SUBROUTINE dyn_adv_cen2( kt )
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_dyn_adv_cen2( INT(kt,C_INT), p_un, p_vn, p_wn, p_ua, p_va ), 'dyn_adv_cen2' )
+   END SUBROUTINE dyn_adv_cen2
+
+   SUBROUTINE dyn_vor( kt )
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_dyn_vor( INT(kt,C_INT), p_un, p_vn, p_ua, p_va ), 'dyn_vor' )
+   END SUBROUTINE dyn_vor
+
+   SUBROUTINE dyn_zdf( kt )
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_dyn_zdf( INT(kt,C_INT), p_ub, p_vb, p_ua, p_va, p_avm, p_rCdU_bot, p_ua_b, p_va_b,        &
+         &        p_utau, p_utau_b, p_vtau, p_vtau_b ), 'dyn_zdf' )
+   END SUBROUTINE dyn_zdf
+
+   SUBROUTINE dyn_spg_ts( kt )
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_dyn_spg_ts( INT(kt,C_INT), spg_state ), 'dyn_spg_ts' )
+   END SUBROUTINE dyn_spg_ts
+
+END MODULE nemo_hip_mod
