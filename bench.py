@@ -55,8 +55,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", default="1024x1024x75")
     ap.add_argument("--forms", default="fp64,fp32")
-    ap.add_argument("--cpu-sample", default="256x256x75")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-sample", default="512x512x75")
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--timing-steps", type=int, default=2)

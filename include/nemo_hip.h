@@ -162,7 +162,7 @@ int nemo_hip_timing_reset(void);
 int nemo_hip_timing_get(const char *name, double *total_ms, int64_t *count);
 
 /* ------------------------------------------------------------------------ */
-/* a2: masks and k-indices, DOM/dommsk.F90:134-210 and DOM/domzgr.F90:267-316 */
+/* masks and k-indices, DOM/dommsk.F90:134-210 and DOM/domzgr.F90:267-316 */
 /* every output pointer may be NULL                                         */
 /* ------------------------------------------------------------------------ */
 int nemo_hip_dom_msk(double *tmask, double *umask, double *vmask, double *fmask,
@@ -171,7 +171,7 @@ int nemo_hip_dom_msk(double *tmask, double *umask, double *vmask, double *fmask,
                      int32_t *mikt, int32_t *mbkt, int32_t *miku, int32_t *mikv,
                      int32_t *mbku, int32_t *mbkv);
 
-/* a19: lbc_lnk, serial path, LBC/lbclnk.F90:81-89 + LBC/lbc_lnk_generic.h90:48-108.
+/* lbc_lnk, serial path, LBC/lbclnk.F90:81-89 + LBC/lbc_lnk_generic.h90:48-108.
    ptab is (jpi,jpj,nk); cd_nat in 'T','U','V','W','F'; psgn only matters at a
    north fold (unsupported); pval is the closed-boundary value. */
 int nemo_hip_lbc_lnk(double *ptab, int nk, char cd_nat, double psgn, double pval);
@@ -179,27 +179,27 @@ int nemo_hip_lbc_lnk(double *ptab, int nk, char cd_nat, double psgn, double pval
 /* ------------------------------------------------------------------------ */
 /* tracers                                                                  */
 /* ------------------------------------------------------------------------ */
-/* a3: effective transport of tra_adv, TRA/traadv.F90:100-124 */
+/* effective transport of tra_adv, TRA/traadv.F90:100-124 */
 int nemo_hip_tra_adv_trp(const double *un, const double *vn, const double *wn,
                          double *zun, double *zvn, double *zwn);
 
-/* a4-a5: FCT advection, TRA/traadv_fct.F90:54-80 (kn_fct_h, kn_fct_v in {2,4}) */
+/* FCT advection, TRA/traadv_fct.F90:54-80 (kn_fct_h, kn_fct_v in {2,4}) */
 int nemo_hip_tra_adv_fct(int kt, int kit000, const char *cdtype, double p2dt,
                          const double *pun, const double *pvn, const double *pwn,
                          const double *ptb, const double *ptn, double *pta,
                          int kjpt, int kn_fct_h, int kn_fct_v);
 
-/* a6: 4th order compact T->w interpolation, TRA/traadv_fct.F90:517 */
+/* 4th order compact T->w interpolation, TRA/traadv_fct.F90:517 */
 int nemo_hip_interp_4th_cpt(const double *pt_in, double *pt_out);
 
-/* a8: iso-level laplacian, TRA/traldf_lap_blp.F90:47-76 */
+/* iso-level laplacian, TRA/traldf_lap_blp.F90:47-76 */
 int nemo_hip_tra_ldf_lap(int kt, int kit000, const char *cdtype,
                          const double *pahu, const double *pahv,
                          const double *pgu, const double *pgv,
                          const double *pgui, const double *pgvi,
                          const double *ptb, double *pta, int kjpt, int kpass);
 
-/* a9: rotated (iso-neutral) laplacian, TRA/traldf_iso.F90:50-103.
+/* rotated (iso-neutral) laplacian, TRA/traldf_iso.F90:50-103.
    Trailing arguments: slopes of LDF/ldfslp.F90:57-58 (in) and the module
    arrays ah_wslp2, akz of LDF/ldfslp.F90:64-65 (written when kpass == 1). */
 int nemo_hip_tra_ldf_iso(int kt, int kit000, const char *cdtype,
@@ -212,7 +212,7 @@ int nemo_hip_tra_ldf_iso(int kt, int kit000, const char *cdtype,
                          const double *wslpi, const double *wslpj,
                          double *ah_wslp2, double *akz);
 
-/* a11: implicit vertical diffusion, TRA/trazdf.F90:109-135.  pta is the trend
+/* implicit vertical diffusion, TRA/trazdf.F90:109-135.  pta is the trend
    on entry and the after tracer on exit.  avs may equal avt.  l_ldfslp != 0
    adds ah_wslp2 (or akz when ln_traldf_msc) to the diffusivity. */
 int nemo_hip_tra_zdf_imp(int kt, int kit000, const char *cdtype, double p2dt,
@@ -220,7 +220,7 @@ int nemo_hip_tra_zdf_imp(int kt, int kit000, const char *cdtype, double p2dt,
                          const double *avt, const double *avs, int l_ldfslp,
                          const double *ah_wslp2, const double *akz);
 
-/* a10: tra_zdf, TRA/trazdf.F90:46-106: r2dt from kt, tra_zdf_imp on (T,S),
+/* tra_zdf, TRA/trazdf.F90:46-106: r2dt from kt, tra_zdf_imp on (T,S),
    then negative salinity reset to 0.1 */
 int nemo_hip_tra_zdf(int kt, const double *tsb, double *tsa,
                      const double *avt, const double *avs, int l_ldfslp,
@@ -229,23 +229,23 @@ int nemo_hip_tra_zdf(int kt, const double *tsb, double *tsa,
 /* ------------------------------------------------------------------------ */
 /* momentum                                                                 */
 /* ------------------------------------------------------------------------ */
-/* a13: DYN/dynkeg.F90:46-192 (kscheme 0 C2, 1 Hollingsworth) */
+/* DYN/dynkeg.F90:46-192 (kscheme 0 C2, 1 Hollingsworth) */
 int nemo_hip_dyn_keg(int kt, int kscheme, const double *un, const double *vn,
                      double *ua, double *va);
-/* a14: DYN/dynzad.F90:38-121 */
+/* DYN/dynzad.F90:38-121 */
 int nemo_hip_dyn_zad(int kt, const double *un, const double *vn, const double *wn,
                      double *ua, double *va);
-/* a15: DYN/dynadv_cen2.F90:37-144 */
+/* DYN/dynadv_cen2.F90:37-144 */
 int nemo_hip_dyn_adv_cen2(int kt, const double *un, const double *vn,
                           const double *wn, double *ua, double *va);
-/* a12: DYN/dynadv.F90:55-85, dispatch on the registered n_dynadv */
+/* DYN/dynadv.F90:55-85, dispatch on the registered n_dynadv */
 int nemo_hip_dyn_adv(int kt, const double *un, const double *vn, const double *wn,
                      double *ua, double *va);
-/* a16: DYN/dynvor.F90:98-186, scheme from the registered nvor_scheme */
+/* DYN/dynvor.F90:98-186, scheme from the registered nvor_scheme */
 int nemo_hip_dyn_vor(int kt, const double *un, const double *vn,
                      double *ua, double *va);
 
-/* a17: DYN/dynzdf.F90:47-499.  ua/va are trends on entry and after
+/* DYN/dynzdf.F90:47-499.  ua/va are trends on entry and after
    velocities on exit. */
 int nemo_hip_dyn_zdf(int kt, const double *ub, const double *vb,
                      double *ua, double *va, const double *avm,
@@ -253,10 +253,36 @@ int nemo_hip_dyn_zdf(int kt, const double *ub, const double *vb,
                      const double *utau, const double *utau_b,
                      const double *vtau, const double *vtau_b);
 
-/* a18: DYN/dynspg_ts.F90:121-1310 (3D arrays in st are double*) */
+/* DYN/dynspg_ts.F90:121-1310 (3D arrays in st are double*) */
 int nemo_hip_dyn_spg_ts(int kt, nemo_hip_spg_ts_t *st);
 /* ts_wgt, DYN/dynspg_ts.F90:1313-1388: weights and cycle length */
 int nemo_hip_ts_wgt(int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2);
+
+/* ------------------------------------------------------------------------ */
+/* Fused steps on device-resident fields (pointer mode NEMO_HIP_PTR_DEVICE). */
+/* Each equals, bit for bit, the sequence of the separate entry points in   */
+/* the order of the time step, step.F90:179-257.                            */
+/* ------------------------------------------------------------------------ */
+/* ua = va = 0; dyn_adv (DYN/dynadv.F90:55); dyn_vor (DYN/dynvor.F90:98);
+   dyn_spg_ts (DYN/dynspg_ts.F90:121) on st; dyn_zdf (DYN/dynzdf.F90:47) with
+   ub, vb, avm, st->rCdU_bot, st->ua_b/va_b, st->utau/vtau and utau_b/vtau_b.
+   The advection and vorticity trends read st->un/st->vn before dyn_spg_ts
+   updates them.  st->ua/st->va need no initial value. */
+int nemo_hip_step_dyn(int kt, const double *ub, const double *vb, const double *wn,
+                      const double *avm, const double *utau_b, const double *vtau_b,
+                      nemo_hip_spg_ts_t *st);
+/* tsa = 0; tra_adv (TRA/traadv.F90:77, transports of un/vn/wn + tra_adv_fct
+   with kn_fct_h/kn_fct_v); tra_ldf_iso (TRA/traldf_iso.F90:50, laplacian,
+   kpass 1, ptbb = tsb, writes ah_wslp2 and with ln_traldf_msc akz on the
+   interior); tra_zdf (TRA/trazdf.F90:46, avt for both tracers, iso-neutral
+   coefficient, salinity reset).  Two tracers (T, S) in tsb/tsn/tsa. */
+int nemo_hip_step_tra(int kt, const double *un, const double *vn, const double *wn,
+                      const double *tsb, const double *tsn, double *tsa,
+                      int kn_fct_h, int kn_fct_v,
+                      const double *ahtu, const double *ahtv,
+                      const double *uslp, const double *vslp,
+                      const double *wslpi, const double *wslpj,
+                      double *ah_wslp2, double *akz, const double *avt);
 
 /* ------------------------------------------------------------------------ */
 /* FP32-storage precision form: same arguments with float 3D fields.         */
@@ -293,6 +319,16 @@ int nemo_hip_dyn_zdf_sp(int kt, const float *ub, const float *vb,
                         const double *vtau, const double *vtau_b);
 /* 3D arrays in st are float* */
 int nemo_hip_dyn_spg_ts_sp(int kt, nemo_hip_spg_ts_t *st);
+int nemo_hip_step_dyn_sp(int kt, const float *ub, const float *vb, const float *wn,
+                         const float *avm, const double *utau_b, const double *vtau_b,
+                         nemo_hip_spg_ts_t *st);
+int nemo_hip_step_tra_sp(int kt, const float *un, const float *vn, const float *wn,
+                         const float *tsb, const float *tsn, float *tsa,
+                         int kn_fct_h, int kn_fct_v,
+                         const float *ahtu, const float *ahtv,
+                         const float *uslp, const float *vslp,
+                         const float *wslpi, const float *wslpj,
+                         float *ah_wslp2, float *akz, const float *avt);
 
 #ifdef __cplusplus
 }

@@ -820,6 +820,53 @@ static int do_dyn_spg_ts(int kt, nemo_hip_spg_ts_t *st)
 }
 
 // ---------------------------------------------------------------------------
+// fused steps (resident fields)
+// ---------------------------------------------------------------------------
+template <typename T>
+static int do_step_dyn(int kt, const T *ub, const T *vb, const T *wn, const T *avm, const double *utau_b,
+                       const double *vtau_b, nemo_hip_spg_ts_t *st)
+{
+    NEED_DOMAIN();
+    if (g_ptr_mode != NEMO_HIP_PTR_DEVICE)
+        return set_error(NEMO_HIP_ERR_STATE, "step_dyn: the fused step works on device-resident fields only");
+    if (!st) return set_error(NEMO_HIP_ERR_ARG, "step_dyn: state is NULL");
+    if (!g_dom.ln_dynspg_ts) return set_error(NEMO_HIP_ERR_STATE, "step_dyn needs ln_dynspg_ts = T");
+    if (!g_dom.ln_drgimp) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "step_dyn: explicit drag (ln_drgimp = F) is not implemented");
+    if (!g_dom.ln_bt_fw) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "step_dyn: ln_bt_fw = F is not implemented");
+    NEED_PTR(ub); NEED_PTR(vb); NEED_PTR(avm); NEED_PTR(utau_b); NEED_PTR(vtau_b);
+    if (!wn && g_dom.n_dynadv != NEMO_NP_LIN_DYN) return set_error(NEMO_HIP_ERR_ARG, "step_dyn: wn is NULL");
+    const void *req[] = {st->ua, st->va, st->un, st->vn, st->un_b, st->vn_b, st->ub_b, st->vb_b, st->sshn, st->hu_n,
+                         st->hv_n, st->r1_hu_n, st->r1_hv_n, st->ht_n, st->rCdU_bot, st->utau, st->vtau, st->emp,
+                         st->ua_b, st->va_b, st->ssha, st->un_adv, st->vn_adv, st->ub2_b, st->vb2_b, st->un_bf,
+                         st->vn_bf};
+    for (const void *p : req)
+        if (!p) return set_error(NEMO_HIP_ERR_ARG, "step_dyn: a required field is NULL");
+    const double r2dt = r2dt_update(g_r2dt_dyn, kt);
+    return launch_step_dyn<T>(kt, r2dt, ub, vb, wn, avm, utau_b, vtau_b, st);
+}
+
+template <typename T>
+static int do_step_tra(int kt, const T *un, const T *vn, const T *wn, const T *tsb, const T *tsn, T *tsa,
+                       int kn_fct_h, int kn_fct_v, const T *ahtu, const T *ahtv, const T *uslp, const T *vslp,
+                       const T *wslpi, const T *wslpj, T *ah_wslp2, T *akz, const T *avt)
+{
+    NEED_DOMAIN();
+    if (g_ptr_mode != NEMO_HIP_PTR_DEVICE)
+        return set_error(NEMO_HIP_ERR_STATE, "step_tra: the fused step works on device-resident fields only");
+    NEED_PTR(un); NEED_PTR(vn); NEED_PTR(wn); NEED_PTR(tsb); NEED_PTR(tsn); NEED_PTR(tsa); NEED_PTR(ahtu);
+    NEED_PTR(ahtv); NEED_PTR(uslp); NEED_PTR(vslp); NEED_PTR(wslpi); NEED_PTR(wslpj); NEED_PTR(ah_wslp2);
+    NEED_PTR(akz); NEED_PTR(avt);
+    if ((kn_fct_h != 2 && kn_fct_h != 4) || (kn_fct_v != 2 && kn_fct_v != 4))
+        return set_error(NEMO_HIP_ERR_ARG, "step_tra: kn_fct_h = %d, kn_fct_v = %d", kn_fct_h, kn_fct_v);
+    // r2dt of tra_adv and tra_zdf (TRA/traadv.F90:95-97, TRA/trazdf.F90:66-68), z2dt of traldf_iso.F90:131-133
+    const double p2dt = r2dt_update(g_r2dt_tra, kt);
+    const double p2dt_zdf = r2dt_update(g_r2dt_zdf, kt);
+    const double z2dt = (g_hdom.neuler == 0 && kt == g_hdom.nit000) ? g_hdom.rdt : 2.0 * g_hdom.rdt;
+    return launch_step_tra<T>(p2dt, z2dt, p2dt_zdf, un, vn, wn, tsb, tsn, tsa, kn_fct_h, kn_fct_v, ahtu, ahtv, uslp,
+                              vslp, wslpi, wslpj, avt, ah_wslp2, akz);
+}
+
+// ---------------------------------------------------------------------------
 // FP64 entry points
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -912,6 +959,21 @@ int nemo_hip_dyn_zdf(int kt, const double *ub, const double *vb, double *ua, dou
 }
 int nemo_hip_dyn_spg_ts(int kt, nemo_hip_spg_ts_t *st) { return do_dyn_spg_ts<double>(kt, st); }
 
+int nemo_hip_step_dyn(int kt, const double *ub, const double *vb, const double *wn, const double *avm,
+                      const double *utau_b, const double *vtau_b, nemo_hip_spg_ts_t *st)
+{
+    return do_step_dyn<double>(kt, ub, vb, wn, avm, utau_b, vtau_b, st);
+}
+
+int nemo_hip_step_tra(int kt, const double *un, const double *vn, const double *wn, const double *tsb,
+                      const double *tsn, double *tsa, int kn_fct_h, int kn_fct_v, const double *ahtu,
+                      const double *ahtv, const double *uslp, const double *vslp, const double *wslpi,
+                      const double *wslpj, double *ah_wslp2, double *akz, const double *avt)
+{
+    return do_step_tra<double>(kt, un, vn, wn, tsb, tsn, tsa, kn_fct_h, kn_fct_v, ahtu, ahtv, uslp, vslp, wslpi,
+                               wslpj, ah_wslp2, akz, avt);
+}
+
 int nemo_hip_ts_wgt(int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2)
 {
     NEED_DOMAIN();
@@ -968,5 +1030,20 @@ int nemo_hip_dyn_zdf_sp(int kt, const float *ub, const float *vb, float *ua, flo
     return do_dyn_zdf<float>(kt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau, utau_b, vtau, vtau_b);
 }
 int nemo_hip_dyn_spg_ts_sp(int kt, nemo_hip_spg_ts_t *st) { return do_dyn_spg_ts<float>(kt, st); }
+
+int nemo_hip_step_dyn_sp(int kt, const float *ub, const float *vb, const float *wn, const float *avm,
+                         const double *utau_b, const double *vtau_b, nemo_hip_spg_ts_t *st)
+{
+    return do_step_dyn<float>(kt, ub, vb, wn, avm, utau_b, vtau_b, st);
+}
+
+int nemo_hip_step_tra_sp(int kt, const float *un, const float *vn, const float *wn, const float *tsb,
+                         const float *tsn, float *tsa, int kn_fct_h, int kn_fct_v, const float *ahtu,
+                         const float *ahtv, const float *uslp, const float *vslp, const float *wslpi,
+                         const float *wslpj, float *ah_wslp2, float *akz, const float *avt)
+{
+    return do_step_tra<float>(kt, un, vn, wn, tsb, tsn, tsa, kn_fct_h, kn_fct_v, ahtu, ahtv, uslp, vslp, wslpi,
+                              wslpj, ah_wslp2, akz, avt);
+}
 
 }  // extern "C"

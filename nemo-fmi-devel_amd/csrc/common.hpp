@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <type_traits>
 
 #include "../../include/nemo_hip.h"
 
@@ -75,8 +76,45 @@ template <> __host__ __device__ inline const Metrics<float> &met<float>(const De
 #define IJ(i, j) ((size_t)((j) - 1) * (size_t)d.jpi + (size_t)((i) - 1))
 #define IJK(i, j, k) (((size_t)((k) - 1) * (size_t)d.jpj + (size_t)((j) - 1)) * (size_t)d.jpi + (size_t)((i) - 1))
 #define A2(p, i, j) (p)[IJ(i, j)]
-#define A3(p, i, j, k) (p)[IJK(i, j, k)]
+// 3D element access with a 32-bit byte offset: with a wave-uniform base the
+// load takes the saddr + voffset form (no 64-bit address per lane).  The
+// domain registration refuses arrays of 4 GiB or more.
+template <typename T>
+__device__ __forceinline__ T &at32(T *p, uint32_t e)
+{
+    return *reinterpret_cast<T *>(reinterpret_cast<char *>(const_cast<typename std::remove_const<T>::type *>(p)) +
+                                  (uint32_t)(e * (uint32_t)sizeof(T)));
+}
+#define IJK32(i, j, k) ((((uint32_t)((k) - 1) * (uint32_t)d.jpj + (uint32_t)((j) - 1)) * (uint32_t)d.jpi) + (uint32_t)((i) - 1))
+#define A3(p, i, j, k) at32((p), IJK32(i, j, k))
 #define E3(p, ij, k) (p)[(size_t)((k) - 1) * d.e3_sk + (size_t)(ij) * d.e3_sij]
+
+// Plane-relative access for k-marched kernels: the level base pointer is
+// uniform across the wave (kept in SGPRs), the lane's in-plane offset is a
+// 32-bit byte offset, so loads use the saddr + 32-bit voffset form and the
+// 64-bit per-lane address arithmetic disappears.  Valid while a plane is
+// below 4 GiB (jpi*jpj*sizeof(T) < 2^32).
+template <typename T>
+__device__ __forceinline__ const T *plane_ptr(const T *base, size_t plane, int k)
+{
+    return base + (size_t)(k - 1) * plane;
+}
+template <typename T>
+__device__ __forceinline__ T *plane_ptr(T *base, size_t plane, int k)
+{
+    return base + (size_t)(k - 1) * plane;
+}
+template <typename T>
+__device__ __forceinline__ T ldp(const T *pl, uint32_t ij)
+{
+    return *(const T *)((const char *)pl + (uint32_t)(ij * (uint32_t)sizeof(T)));
+}
+template <typename T>
+__device__ __forceinline__ void stp(T *pl, uint32_t ij, T v)
+{
+    *(T *)((char *)pl + (uint32_t)(ij * (uint32_t)sizeof(T))) = v;
+}
+#define IJ32(i, j) ((uint32_t)((j) - 1) * (uint32_t)d.jpi + (uint32_t)((i) - 1))
 
 template <typename T>
 __device__ __forceinline__ T kmask(const KRange &r, size_t ij, int k)
@@ -134,13 +172,23 @@ __device__ __forceinline__ bool is_interior(const DevDomain &d, int i, int j)
     return i >= 2 && i <= d.jpi - 1 && j >= 2 && j <= d.jpj - 1;
 }
 
+// Every kernel runs 256-thread blocks (one wave per SIMD).  Declaring that
+// bound lifts the register cap the compiler applies for 1024-thread blocks
+// (128 VGPRs per lane), which made the larger stencils spill to scratch.
+#define NH_BLOCK 256
+#define KERNEL __global__ __launch_bounds__(NH_BLOCK)
+// same, asking the compiler to keep at least W waves per SIMD resident
+// (caps registers at 512/W per lane) for kernels whose stencils are large
+#define KERNEL_W(W) __global__ __launch_bounds__(NH_BLOCK, W)
+
 // 2D launch geometry over (i,j) with an explicit index origin
 struct Grid2 {
     dim3 grid, block;
 };
-inline Grid2 grid2(int ni, int nj, int bx = 64, int by = 4)
+inline Grid2 grid2(int ni, int nj, int bx = 64, int by = NH_BLOCK / 64)
 {
     Grid2 g;
+    if (bx * by != NH_BLOCK) by = NH_BLOCK / bx;
     g.block = dim3(bx, by, 1);
     g.grid = dim3((unsigned)((ni + bx - 1) / bx), (unsigned)((nj + by - 1) / by), 1);
     return g;

@@ -7,8 +7,8 @@
 // dom_msk: expand the per-column level intervals to NEMO's REAL 0/1 masks.
 // One thread per (i,j) column, k-marched, coalesced along i.
 // ---------------------------------------------------------------------------
-__global__ void k_dom_msk(DevDomain d, double *tmask, double *umask, double *vmask, double *fmask, double *wmask,
-                          double *wumask, double *wvmask, double *ssmask, double *ssumask, double *ssvmask)
+KERNEL void k_dom_msk(DevDomain d, double * __restrict__ tmask, double * __restrict__ umask, double * __restrict__ vmask, double * __restrict__ fmask, double * __restrict__ wmask,
+                          double * __restrict__ wumask, double * __restrict__ wvmask, double * __restrict__ ssmask, double * __restrict__ ssumask, double * __restrict__ ssvmask)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi || j > d.jpj) return;
@@ -41,7 +41,7 @@ int launch_dom_msk(double *tmask, double *umask, double *vmask, double *fmask, d
 // (barrier) before the north-south fill copies whole rows.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_lbc(DevDomain d, T *p, int nk, char nat, T pval)
+KERNEL void k_lbc(DevDomain d, T * __restrict__ p, int nk, char nat, T pval)
 {
     const int k = (int)blockIdx.x + 1;
     if (k > nk) return;
@@ -74,7 +74,7 @@ int launch_lbc(T *p, int nk, char nat, T pval)
     const DevDomain &d = nh::dom();
     Grid2 g;
     g.grid = dim3((unsigned)nk, 1, 1);
-    g.block = dim3(256, 1, 1);
+    g.block = dim3(NH_BLOCK, 1, 1);
     LAUNCH("lbc_lnk", k_lbc<T>, g, d, p, nk, nat, pval);
     return 0;
 }

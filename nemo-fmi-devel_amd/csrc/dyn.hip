@@ -15,7 +15,7 @@
 enum { np_COR = 1, np_RVO = 2, np_MET = 3, np_CRV = 4, np_CME = 5 };
 
 // ---------------------------------------------------------------------------
-// a13 + a14: vector-form advection, dyn_keg then dyn_zad, fused per element
+// vector-form advection, dyn_keg then dyn_zad, fused per element
 // (ua <- (ua - keg) - zad, the order of the two reference calls)
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -46,8 +46,8 @@ __device__ __forceinline__ T keg_hw(const DevDomain &d, const T *un, const T *vn
 }
 
 template <typename T>
-__global__ void k_dyn_keg_zad(DevDomain d, int do_keg, int kscheme, int do_zad, const T *un, const T *vn, const T *wn,
-                              T *ua, T *va)
+KERNEL void k_dyn_keg_zad(DevDomain d, int do_keg, int kscheme, int do_zad, const T * __restrict__ un, const T * __restrict__ vn, const T * __restrict__ wn,
+                              T * __restrict__ ua, T * __restrict__ va)
 {
     THREAD_IJ(2, 2);
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
@@ -94,7 +94,7 @@ __global__ void k_dyn_keg_zad(DevDomain d, int do_keg, int kscheme, int do_zad, 
 }
 
 // ---------------------------------------------------------------------------
-// a15: dyn_adv_cen2, horizontal then vertical flux divergence per element
+// dyn_adv_cen2, horizontal then vertical flux divergence per element
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T cen2_zfu(const DevDomain &d, const T *un, int p, int q, int k)
@@ -117,7 +117,7 @@ __device__ __forceinline__ T cen2_zfw(const DevDomain &d, const T *wn, int p, in
 }
 
 template <typename T>
-__global__ void k_dyn_adv_cen2(DevDomain d, const T *un, const T *vn, const T *wn, T *ua, T *va)
+KERNEL void k_dyn_adv_cen2(DevDomain d, const T * __restrict__ un, const T * __restrict__ vn, const T * __restrict__ wn, T * __restrict__ ua, T * __restrict__ va)
 {
     THREAD_IJ(2, 2);
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
@@ -185,7 +185,7 @@ template int launch_dyn_adv<double>(int, int, const double *, const double *, co
 template int launch_dyn_adv<float>(int, int, const float *, const float *, const float *, float *, float *);
 
 // ---------------------------------------------------------------------------
-// a16: vorticity term
+// vorticity term
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ T curl_f(const DevDomain &d, const T *pu, const T *pv, int p, int q, int k)
@@ -357,132 +357,179 @@ __device__ __forceinline__ void vor_ent_pt(const DevDomain &d, int kvor, const T
                 (zn * (A3(pu, i, j + 1, k) + A3(pu, i - 1, j + 1, k)) + zc * (A3(pu, i, j, k) + A3(pu, i - 1, j, k)));
 }
 
-template <typename T>
-__global__ void k_dyn_vor(DevDomain d, int scheme, int ncor, int nrvm, int ntot, const T *un, const T *vn, T *ua, T *va)
+// The scheme and the vorticity kinds are template parameters: each
+// instantiation carries only its own stencil, which keeps register use (and so
+// occupancy) at what that scheme needs.
+template <typename T, int SCHEME, int NCOR, int NRVM, int NTOT>
+KERNEL void k_dyn_vor(DevDomain d, const T * __restrict__ un, const T * __restrict__ vn, T * __restrict__ ua,
+                      T * __restrict__ va)
 {
     THREAD_IJ(2, 2);
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
     for (int k = 1; k <= d.jpk - 1; k++) {
         const size_t p = IJK(i, j, k);
         T a = ua[p], b = va[p];
-        switch (scheme) {
-        case NEMO_NP_ENE: vor_ene_pt(d, ntot, un, vn, i, j, k, a, b); break;
-        case NEMO_NP_ENS: vor_ens_pt(d, ntot, un, vn, i, j, k, a, b); break;
+        switch (SCHEME) {
+        case NEMO_NP_ENE: vor_ene_pt(d, NTOT, un, vn, i, j, k, a, b); break;
+        case NEMO_NP_ENS: vor_ens_pt(d, NTOT, un, vn, i, j, k, a, b); break;
         case NEMO_NP_MIX:
-            vor_ens_pt(d, nrvm, un, vn, i, j, k, a, b);
-            vor_ene_pt(d, ncor, un, vn, i, j, k, a, b);
+            vor_ens_pt(d, NRVM, un, vn, i, j, k, a, b);
+            vor_ene_pt(d, NCOR, un, vn, i, j, k, a, b);
             break;
-        case NEMO_NP_ENT: vor_ent_pt(d, ntot, un, vn, i, j, k, a, b); break;
-        default: vor_een_pt(d, scheme, ntot, un, vn, i, j, k, a, b); break;
+        case NEMO_NP_ENT: vor_ent_pt(d, NTOT, un, vn, i, j, k, a, b); break;
+        default: vor_een_pt(d, SCHEME, NTOT, un, vn, i, j, k, a, b); break;
         }
         ua[p] = a;
         va[p] = b;
     }
 }
 
+template <typename T, int SCHEME>
+static int launch_dyn_vor_s(const DevDomain &d, const T *un, const T *vn, T *ua, T *va)
+{
+    Grid2 g = grid2(d.jpi - 2, d.jpj - 2);
+    // (nrvm, ntot) from the momentum advection form, DYN/dynvor.F90:903-939
+    switch (d.n_dynadv) {
+    case NEMO_NP_LIN_DYN:
+        LAUNCH("dyn_vor", (k_dyn_vor<T, SCHEME, np_COR, np_COR, np_COR>), g, d, un, vn, ua, va);
+        break;
+    case NEMO_NP_VEC_C2:
+        LAUNCH("dyn_vor", (k_dyn_vor<T, SCHEME, np_COR, np_RVO, np_CRV>), g, d, un, vn, ua, va);
+        break;
+    default:
+        LAUNCH("dyn_vor", (k_dyn_vor<T, SCHEME, np_COR, np_MET, np_CME>), g, d, un, vn, ua, va);
+        break;
+    }
+    return 0;
+}
+
 template <typename T>
 int launch_dyn_vor(const T *un, const T *vn, T *ua, T *va)
 {
     const DevDomain &d = nh::dom();
-    int ncor = np_COR, nrvm, ntot;
-    switch (d.n_dynadv) {
-    case NEMO_NP_LIN_DYN: nrvm = np_COR; ntot = np_COR; break;
-    case NEMO_NP_VEC_C2: nrvm = np_RVO; ntot = np_CRV; break;
-    default: nrvm = np_MET; ntot = np_CME; break;
+    switch (d.nvor_scheme) {
+    case NEMO_NP_ENE: return launch_dyn_vor_s<T, NEMO_NP_ENE>(d, un, vn, ua, va);
+    case NEMO_NP_ENS: return launch_dyn_vor_s<T, NEMO_NP_ENS>(d, un, vn, ua, va);
+    case NEMO_NP_MIX: return launch_dyn_vor_s<T, NEMO_NP_MIX>(d, un, vn, ua, va);
+    case NEMO_NP_ENT: return launch_dyn_vor_s<T, NEMO_NP_ENT>(d, un, vn, ua, va);
+    case NEMO_NP_EET: return launch_dyn_vor_s<T, NEMO_NP_EET>(d, un, vn, ua, va);
+    case NEMO_NP_EEN: return launch_dyn_vor_s<T, NEMO_NP_EEN>(d, un, vn, ua, va);
+    default: return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "dyn_vor: nvor_scheme = %d", d.nvor_scheme);
     }
-    Grid2 g = grid2(d.jpi - 2, d.jpj - 2);
-    LAUNCH("dyn_vor", k_dyn_vor<T>, g, d, d.nvor_scheme, ncor, nrvm, ntot, un, vn, ua, va);
-    return 0;
 }
 template int launch_dyn_vor<double>(const double *, const double *, double *, double *);
 template int launch_dyn_vor<float>(const float *, const float *, float *, float *);
 
 // ---------------------------------------------------------------------------
-// a17: dyn_zdf, DYN/dynzdf.F90:107-485 (ln_zad_Aimp = F, iso-level momentum
+// dyn_zdf, DYN/dynzdf.F90:107-485 (ln_zad_Aimp = F, iso-level momentum
 // mixing, r_vvl = 0).  All columns apply the leapfrog right-hand side; the
 // interior ones then solve the u and v columns (FP64 elimination).
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ void zdf_column(const DevDomain &d, double r2dt, int comp, int i, int j, const T *pb, T *pa,
-                           const T *avm, const double *rCdU_bot, const double *pa_b, const double *tau,
-                           const double *tau_b, double *zwd_s, double *fwd_s)
+// `pre(k, x)` maps the stored trend x at level k to the value the solve
+// starts from (the identity for dyn_zdf alone; the barotropic corrections
+// of dyn_spg_ts in the fused step).
+struct PreIdentity {
+    template <typename T>
+    __device__ __forceinline__ T operator()(int, T x) const { return x; }
+};
+template <typename T, typename Pre = PreIdentity>
+__device__ __forceinline__ void zdf_column(const DevDomain &d, double r2dt, int comp, int i, int j,
+                                           const T *__restrict__ pb, T *__restrict__ pa, const T *__restrict__ avm,
+                                           const double *__restrict__ rCdU_bot, const double *__restrict__ pa_b,
+                                           const double *__restrict__ tau, const double *__restrict__ tau_b,
+                                           double *__restrict__ zwd_s, Pre pre = Pre())
 {
     const Metrics<T> &m = met<T>(d);
     const int jpk = d.jpk;
-    const int di = comp ? 0 : 1, dj = comp ? 1 : 0;
-    const size_t ij = IJ(i, j);
+    const size_t plane = d.n2;
+    const uint32_t ij = IJ32(i, j);
+    const uint32_t ijn = comp ? IJ32(i, j + 1) : IJ32(i + 1, j);     // the other t-point of the u/v face
     const T *e3x = comp ? m.e3v : m.e3u;
     const T *e3xw = comp ? m.e3vw : m.e3uw;
     const KRange &kr = comp ? d.kV : d.kU;
+    const int x0 = kr.k0[ij], x1 = kr.k1[ij];
+    auto xmask = [&](int k) { return (k >= x0 && k <= x1) ? 1.0 : 0.0; };
+    auto wxmask = [&](int k) { return (k >= x0 && k <= x1 && (k == 1 || k - 1 >= x0)) ? 1.0 : 0.0; };
     const double r_vvl = 0.0;
     const bool inner = is_interior(d, i, j);
     const bool rm_b = d.ln_drgimp && d.ln_dynspg_ts;
     const double ab = rm_b ? pa_b[ij] : 0.0;
-    const int ikb = comp ? d.mbkv[ij] : d.mbku[ij];
-    double csum = 0.0;
-    if (inner) csum = rCdU_bot[IJ(i + di, j + dj)] + rCdU_bot[ij];
     if (!inner) {
         for (int k = 1; k <= jpk - 1; k++) {
-            const size_t p = IJK(i, j, k);
-            const double xm = kmask<double>(kr, ij, k);
-            double a = ((double)pb[p] + r2dt * (double)pa[p]) * xm;
+            const double xm = xmask(k);
+            double a = ((double)ldp(plane_ptr(pb, plane, k), ij) +
+                        r2dt * (double)pre(k, ldp((const T *)plane_ptr(pa, plane, k), ij))) * xm;
             if (rm_b) a = (a - ab) * xm;
-            pa[p] = (T)a;
+            stp(plane_ptr(pa, plane, k), ij, (T)a);
         }
         return;
     }
+    const int ikb = comp ? d.mbkv[ij] : d.mbku[ij];
+    const double csum = rCdU_bot[ijn] + rCdU_bot[ij];
     const double zdt = r2dt * 0.5;
-    double zws_prev = 0.0, zwd_prev = 1.0, a_prev = 0.0;
-    for (int k = 1; k <= jpk - 1; k++) {
-        const size_t p = IJK(i, j, k);
-        const double xm = kmask<double>(kr, ij, k);
-        double a = ((double)pb[p] + r2dt * (double)pa[p]) * xm;
+    // off-diagonal at w-level k (k >= 2), lines 208-243
+    auto zzw = [&](int k, double ze3) {
+        const T *ak = plane_ptr(avm, plane, k);
+        return -(zdt * ((double)ldp(ak, ijn) + (double)ldp(ak, ij)) / (ze3 * (double)E3(e3xw, ij, k)) * wxmask(k));
+    };
+    double zws_prev, zwd_prev, a_prev;
+    {   // k = 1: surface row with the wind stress
+        const double xm = xmask(1);
+        double a = ((double)ldp(pb, ij) + r2dt * (double)pre(1, ldp((const T *)pa, ij))) * xm;
+        if (rm_b) a = (a - ab) * xm;
+        const double ze3 = (1.0 - r_vvl) * (double)E3(e3x, ij, 1) + r_vvl * (double)E3(e3x, ij, 1);
+        if (rm_b && 1 == ikb) a = a + r2dt * 0.5 * csum * ab / ze3;
+        const double zws = zzw(2, ze3);
+        double zwd = 1.0 - zws;
+        if (d.ln_drgimp && 1 == ikb) zwd = zwd - r2dt * 0.5 * csum / ze3;
+        a = a + r2dt * 0.5 * (tau_b[ij] + tau[ij]) / (ze3 * d.rau0) * xm;
+        stp(zwd_s, ij, zwd);
+        stp(pa, ij, (T)a);
+        zws_prev = zws;
+        zwd_prev = zwd;
+        a_prev = a;
+    }
+#pragma unroll 4
+    for (int k = 2; k <= jpk - 1; k++) {
+        const double xm = xmask(k);
+        double a = ((double)ldp(plane_ptr(pb, plane, k), ij) +
+                    r2dt * (double)pre(k, ldp((const T *)plane_ptr(pa, plane, k), ij))) * xm;
         if (rm_b) a = (a - ab) * xm;
         const double ze3 = (1.0 - r_vvl) * (double)E3(e3x, ij, k) + r_vvl * (double)E3(e3x, ij, k);
         if (rm_b && k == ikb) a = a + r2dt * 0.5 * csum * ab / ze3;
-        const double zzwi = -(zdt * ((double)A3(avm, i + di, j + dj, k) + (double)A3(avm, i, j, k)) /
-                              (ze3 * (double)E3(e3xw, ij, k)) * wkmask<double>(kr, ij, k));
-        const double zzws = -(zdt * ((double)A3(avm, i + di, j + dj, k + 1) + (double)A3(avm, i, j, k + 1)) /
-                              (ze3 * (double)E3(e3xw, ij, k + 1)) * wkmask<double>(kr, ij, k + 1));
-        double zwi = zzwi, zws = zzws, zwd = 1.0 - zzwi - zzws;
-        if (k == 1) {
-            zwi = 0.0;
-            zwd = 1.0 - zws;
-        }
+        const double zwi = zzw(k, ze3);
+        const double zws = zzw(k + 1, ze3);
+        double zwd = 1.0 - zwi - zws;
         if (d.ln_drgimp && k == ikb) zwd = zwd - r2dt * 0.5 * csum / ze3;
-        if (k >= 2) zwd = zwd - zwi * zws_prev / zwd_prev;
-        if (k == 1) a = a + r2dt * 0.5 * (tau_b[ij] + tau[ij]) / (ze3 * d.rau0) * xm;
-        else a = a - zwi / zwd_prev * a_prev;
-        A3(zwd_s, i, j, k) = zwd;
-        if (fwd_s) A3(fwd_s, i, j, k) = a;
-        else pa[p] = (T)a;
+        zwd = zwd - zwi * zws_prev / zwd_prev;
+        a = a - zwi / zwd_prev * a_prev;
+        stp(plane_ptr(zwd_s, plane, k), ij, zwd);
+        stp(plane_ptr(pa, plane, k), ij, (T)a);
         zws_prev = zws;
         zwd_prev = zwd;
         a_prev = a;
     }
     double x = a_prev / zwd_prev;
-    pa[IJK(i, j, jpk - 1)] = (T)x;
+    stp(plane_ptr(pa, plane, jpk - 1), ij, (T)x);
+#pragma unroll 4
     for (int k = jpk - 2; k >= 1; k--) {
         const double ze3 = (1.0 - r_vvl) * (double)E3(e3x, ij, k) + r_vvl * (double)E3(e3x, ij, k);
-        const double zws = -(zdt * ((double)A3(avm, i + di, j + dj, k + 1) + (double)A3(avm, i, j, k + 1)) /
-                             (ze3 * (double)E3(e3xw, ij, k + 1)) * wkmask<double>(kr, ij, k + 1));
-        const size_t p = IJK(i, j, k);
-        const double a = fwd_s ? A3(fwd_s, i, j, k) : (double)pa[p];
-        x = (a - zws * x) / A3(zwd_s, i, j, k);
-        pa[p] = (T)x;
+        const double zws = zzw(k + 1, ze3);
+        const double a = (double)ldp((const T *)plane_ptr(pa, plane, k), ij);
+        x = (a - zws * x) / ldp((const double *)plane_ptr(zwd_s, plane, k), ij);
+        stp(plane_ptr(pa, plane, k), ij, (T)x);
     }
 }
 
 template <typename T>
-__global__ void k_dyn_zdf(DevDomain d, double r2dt, const T *ub, const T *vb, T *ua, T *va, const T *avm,
-                          const double *rCdU_bot, const double *ua_b, const double *va_b, const double *utau,
-                          const double *utau_b, const double *vtau, const double *vtau_b, double *zwd_s,
-                          double *fwd_s)
+KERNEL void k_dyn_zdf(DevDomain d, double r2dt, const T * __restrict__ ub, const T * __restrict__ vb, T * __restrict__ ua, T * __restrict__ va, const T * __restrict__ avm,
+                          const double * __restrict__ rCdU_bot, const double * __restrict__ ua_b, const double * __restrict__ va_b, const double * __restrict__ utau,
+                          const double * __restrict__ utau_b, const double * __restrict__ vtau, const double * __restrict__ vtau_b, double * __restrict__ zwd_s)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi || j > d.jpj) return;
-    zdf_column(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, zwd_s, fwd_s);
-    zdf_column(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, zwd_s, fwd_s);
+    zdf_column(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, zwd_s);
+    zdf_column(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, zwd_s);
 }
 
 template <typename T>
@@ -493,9 +540,9 @@ int launch_dyn_zdf(double r2dt, const T *ub, const T *vb, T *ua, T *va, const T 
     const DevDomain &d = nh::dom();
     Grid2 g = grid2(d.jpi, d.jpj);
     double *zwd = (double *)nh::scratch3(nh::S_ZWT);
-    double *fwd = nullptr;   // forward-substituted values stay in ua/va at storage precision
+    // forward-substituted values stay in ua/va at storage precision
     LAUNCH("dyn_zdf", k_dyn_zdf<T>, g, d, r2dt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau, utau_b, vtau, vtau_b,
-           zwd, fwd);
+           zwd);
     return 0;
 }
 template int launch_dyn_zdf<double>(double, const double *, const double *, double *, double *, const double *,
@@ -504,3 +551,256 @@ template int launch_dyn_zdf<double>(double, const double *, const double *, doub
 template int launch_dyn_zdf<float>(double, const float *, const float *, float *, float *, const float *,
                                    const double *, const double *, const double *, const double *, const double *,
                                    const double *, const double *);
+
+// ---------------------------------------------------------------------------
+// Fused momentum step: ua = va = 0; dyn_adv; dyn_vor; dyn_spg_ts; dyn_zdf
+// (the order of step.F90:189-257) in two 3D passes around the barotropic
+// sub-steps:
+//   k_step_dyn1  advection + vorticity trends written once, with the vertical
+//                sums of dyn_spg_ts phase 1 (DYN/dynspg_ts.F90:358-378) and the
+//                bottom velocities (519-527) accumulated in the same k-march;
+//   k_step_dyn2  phase 3 (lines 1214-1269) applied to un/vn and to the trend
+//                on the fly, then the dyn_zdf solve of the column.
+// Every stored intermediate of the separate calls is rounded to the storage
+// type at the same point here, so the fused step equals the sequence of
+// separate calls bit for bit in either storage form.
+// ---------------------------------------------------------------------------
+template <typename T, int ADV, int KEG, int SCHEME, int NCOR, int NRVM, int NTOT>
+KERNEL void k_step_dyn1(DevDomain d, const T *__restrict__ un, const T *__restrict__ vn, const T *__restrict__ wn,
+                        T *__restrict__ ua, T *__restrict__ va, const double *__restrict__ r1_hu_n,
+                        const double *__restrict__ r1_hv_n, const double *__restrict__ un_b,
+                        const double *__restrict__ vn_b, double *__restrict__ zu_frc, double *__restrict__ zv_frc,
+                        double *__restrict__ su_s, double *__restrict__ sv_s, double *__restrict__ zwxb,
+                        double *__restrict__ zwyb)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const size_t ij = IJ(i, j);
+    const int jpk = d.jpk;
+    if (!is_interior(d, i, j)) {
+        for (int k = 1; k <= jpk; k++) {
+            A3(ua, i, j, k) = T(0);
+            A3(va, i, j, k) = T(0);
+        }
+        double su = 0.0, sv = 0.0;
+        su = su * r1_hu_n[ij];
+        sv = sv * r1_hv_n[ij];
+        zu_frc[ij] = su; su_s[ij] = su;
+        zv_frc[ij] = sv; sv_s[ij] = sv;
+        return;
+    }
+    const Metrics<T> &m = met<T>(d);
+    const T e1u = m.e1u[ij], e2v = m.e2v[ij], r1_e1e2u = m.r1_e1e2u[ij], r1_e1e2v = m.r1_e1e2v[ij];
+    const T e1e2t = m.e1e2t[ij], e1e2t_e = m.e1e2t[IJ(i + 1, j)], e1e2t_n = m.e1e2t[IJ(i, j + 1)];
+    // vertical flux carried between levels: zad (dynzad.F90:93-94) or cen2 surface flux (dynadv_cen2.F90:105-112)
+    T cu_k = T(0), cv_k = T(0);
+    if (ADV == NEMO_NP_FLX_C2) {
+        cu_k = T(0.5) * (e1e2t * A3(wn, i, j, 1) + e1e2t_e * A3(wn, i + 1, j, 1)) * A3(un, i, j, 1);
+        cv_k = T(0.5) * (e1e2t * A3(wn, i, j, 1) + e1e2t_n * A3(wn, i, j + 1, 1)) * A3(vn, i, j, 1);
+    }
+    double su = 0.0, sv = 0.0;
+    for (int k = 1; k <= jpk - 1; k++) {
+        T a = T(0), b = T(0);
+        if (ADV == NEMO_NP_VEC_C2) {
+            T h, he, hn;
+            if (KEG == 0) {
+                h = keg_c2(d, un, vn, i, j, k); he = keg_c2(d, un, vn, i + 1, j, k); hn = keg_c2(d, un, vn, i, j + 1, k);
+            } else {
+                h = keg_hw(d, un, vn, i, j, k); he = keg_hw(d, un, vn, i + 1, j, k); hn = keg_hw(d, un, vn, i, j + 1, k);
+            }
+            a = a - (he - h) / e1u;
+            b = b - (hn - h) / e2v;
+            T cu_kp1 = T(0), cv_kp1 = T(0);
+            if (k + 1 <= jpk - 1) {
+                const int kk = k + 1;
+                const T zww = T(0.25) * e1e2t * A3(wn, i, j, kk);
+                const T zww_e = T(0.25) * e1e2t_e * A3(wn, i + 1, j, kk);
+                const T zww_n = T(0.25) * e1e2t_n * A3(wn, i, j + 1, kk);
+                cu_kp1 = (zww_e + zww) * (A3(un, i, j, kk - 1) - A3(un, i, j, kk));
+                cv_kp1 = (zww_n + zww) * (A3(vn, i, j, kk - 1) - A3(vn, i, j, kk));
+            }
+            a = a - (cu_k + cu_kp1) * r1_e1e2u / E3(m.e3u, ij, k);
+            b = b - (cv_k + cv_kp1) * r1_e1e2v / E3(m.e3v, ij, k);
+            cu_k = cu_kp1;
+            cv_k = cv_kp1;
+        } else if (ADV == NEMO_NP_FLX_C2) {
+            const T fu_im1 = cen2_zfu(d, un, i - 1, j, k), fu_i = cen2_zfu(d, un, i, j, k), fu_ip1 = cen2_zfu(d, un, i + 1, j, k);
+            const T fu_jp1 = cen2_zfu(d, un, i, j + 1, k), fu_im1jp1 = cen2_zfu(d, un, i - 1, j + 1, k);
+            const T fv_i = cen2_zfv(d, vn, i, j, k), fv_ip1 = cen2_zfv(d, vn, i + 1, j, k);
+            const T fv_jm1 = cen2_zfv(d, vn, i, j - 1, k), fv_ip1jm1 = cen2_zfv(d, vn, i + 1, j - 1, k);
+            const T fv_jp1 = cen2_zfv(d, vn, i, j + 1, k), fv_im1 = cen2_zfv(d, vn, i - 1, j, k);
+            const T u_i = A3(un, i, j, k), u_ip1 = A3(un, i + 1, j, k), u_im1 = A3(un, i - 1, j, k);
+            const T u_jp1 = A3(un, i, j + 1, k), u_jm1 = A3(un, i, j - 1, k);
+            const T v_j = A3(vn, i, j, k), v_ip1 = A3(vn, i + 1, j, k), v_jp1 = A3(vn, i, j + 1, k);
+            const T v_im1 = A3(vn, i - 1, j, k), v_jm1 = A3(vn, i, j - 1, k);
+            const T zfu_t_ip1 = (fu_i + fu_ip1) * (u_i + u_ip1);
+            const T zfu_t_i = (fu_im1 + fu_i) * (u_im1 + u_i);
+            const T zfv_f_j = (fv_i + fv_ip1) * (u_i + u_jp1);
+            const T zfv_f_jm1 = (fv_jm1 + fv_ip1jm1) * (u_jm1 + u_i);
+            const T zfu_f_i = (fu_i + fu_jp1) * (v_j + v_ip1);
+            const T zfu_f_im1 = (fu_im1 + fu_im1jp1) * (v_im1 + v_j);
+            const T zfv_t_jp1 = (fv_i + fv_jp1) * (v_j + v_jp1);
+            const T zfv_t_j = (fv_jm1 + fv_i) * (v_jm1 + v_j);
+            a = a - (zfu_t_ip1 - zfu_t_i + zfv_f_j - zfv_f_jm1) * r1_e1e2u / E3(m.e3u, ij, k);
+            b = b - (zfu_f_i - zfu_f_im1 + zfv_t_jp1 - zfv_t_j) * r1_e1e2v / E3(m.e3v, ij, k);
+            T cu_kp1 = T(0), cv_kp1 = T(0);
+            if (k + 1 <= jpk - 1) {
+                const int kk = k + 1;
+                const T fw = cen2_zfw(d, wn, i, j, kk);
+                cu_kp1 = (fw + cen2_zfw(d, wn, i + 1, j, kk)) * (A3(un, i, j, kk) + A3(un, i, j, kk - 1));
+                cv_kp1 = (fw + cen2_zfw(d, wn, i, j + 1, kk)) * (A3(vn, i, j, kk) + A3(vn, i, j, kk - 1));
+            }
+            a = a - (cu_k - cu_kp1) * r1_e1e2u / E3(m.e3u, ij, k);
+            b = b - (cv_k - cv_kp1) * r1_e1e2v / E3(m.e3v, ij, k);
+            cu_k = cu_kp1;
+            cv_k = cv_kp1;
+        }
+        switch (SCHEME) {
+        case NEMO_NP_ENE: vor_ene_pt(d, NTOT, un, vn, i, j, k, a, b); break;
+        case NEMO_NP_ENS: vor_ens_pt(d, NTOT, un, vn, i, j, k, a, b); break;
+        case NEMO_NP_MIX:
+            vor_ens_pt(d, NRVM, un, vn, i, j, k, a, b);
+            vor_ene_pt(d, NCOR, un, vn, i, j, k, a, b);
+            break;
+        case NEMO_NP_ENT: vor_ent_pt(d, NTOT, un, vn, i, j, k, a, b); break;
+        default: vor_een_pt(d, SCHEME, NTOT, un, vn, i, j, k, a, b); break;
+        }
+        const size_t p = IJK(i, j, k);
+        ua[p] = a;
+        va[p] = b;
+        su = su + (double)E3(m.e3u, ij, k) * (double)a * UMASK(double, i, j, k);
+        sv = sv + (double)E3(m.e3v, ij, k) * (double)b * VMASK(double, i, j, k);
+    }
+    A3(ua, i, j, jpk) = T(0);
+    A3(va, i, j, jpk) = T(0);
+    su = su * r1_hu_n[ij];
+    sv = sv * r1_hv_n[ij];
+    zu_frc[ij] = su; su_s[ij] = su;
+    zv_frc[ij] = sv; sv_s[ij] = sv;
+    zwxb[ij] = (double)A3(un, i, j, d.mbku[ij]) - un_b[ij];
+    zwyb[ij] = (double)A3(vn, i, j, d.mbkv[ij]) - vn_b[ij];
+}
+
+// ua after dyn_spg_ts: interior columns remove the vertical mean of the
+// trend (phase 1, rounded to storage), every column adds the barotropic
+// correction (phase 3, rounded to storage)
+template <typename T>
+struct PreSpg {
+    double s, dd, xm0;
+    int x0, x1;
+    bool inner;
+    __device__ __forceinline__ T operator()(int k, T x) const
+    {
+        const double xm = (k >= x0 && k <= x1) ? 1.0 : 0.0;
+        if (inner) x = (T)((double)x - s * xm);
+        return (T)((double)x + dd);
+    }
+};
+
+template <typename T>
+KERNEL void k_step_dyn2(DevDomain d, double r2dt, double r1_2dt_b, const T *__restrict__ ub, const T *__restrict__ vb,
+                        T *__restrict__ ua, T *__restrict__ va, T *__restrict__ un, T *__restrict__ vn,
+                        const T *__restrict__ avm, const double *__restrict__ rCdU_bot,
+                        const double *__restrict__ ua_b, const double *__restrict__ va_b,
+                        const double *__restrict__ ub_b, const double *__restrict__ vb_b,
+                        const double *__restrict__ un_adv, const double *__restrict__ vn_adv,
+                        const double *__restrict__ r1_hu_n, const double *__restrict__ r1_hv_n,
+                        const double *__restrict__ un_b, const double *__restrict__ vn_b,
+                        const double *__restrict__ su_s, const double *__restrict__ sv_s,
+                        const double *__restrict__ utau, const double *__restrict__ utau_b,
+                        const double *__restrict__ vtau, const double *__restrict__ vtau_b,
+                        double *__restrict__ zwd_s)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const size_t ij = IJ(i, j);
+    const size_t plane = d.n2;
+    const uint32_t ij32 = IJ32(i, j);
+    // phase 3 on the now velocities, lines 1254-1259
+    {
+        const double cu = un_adv[ij] * r1_hu_n[ij], cv = vn_adv[ij] * r1_hv_n[ij];
+        const double ubn = un_b[ij], vbn = vn_b[ij];
+        const int u0 = d.kU.k0[ij], u1 = d.kU.k1[ij], v0 = d.kV.k0[ij], v1 = d.kV.k1[ij];
+#pragma unroll 4
+        for (int k = 1; k <= d.jpk - 1; k++) {
+            T *uk = plane_ptr(un, plane, k), *vk = plane_ptr(vn, plane, k);
+            const double um = (k >= u0 && k <= u1) ? 1.0 : 0.0, vm = (k >= v0 && k <= v1) ? 1.0 : 0.0;
+            stp(uk, ij32, (T)(((double)ldp((const T *)uk, ij32) + cu - ubn) * um));
+            stp(vk, ij32, (T)(((double)ldp((const T *)vk, ij32) + cv - vbn) * vm));
+        }
+    }
+    const bool inner = is_interior(d, i, j);
+    PreSpg<T> pu{su_s[ij], (ua_b[ij] - ub_b[ij]) * r1_2dt_b, 0.0, d.kU.k0[ij], d.kU.k1[ij], inner};
+    PreSpg<T> pv{sv_s[ij], (va_b[ij] - vb_b[ij]) * r1_2dt_b, 0.0, d.kV.k0[ij], d.kV.k1[ij], inner};
+    zdf_column(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, zwd_s, pu);
+    zdf_column(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, zwd_s, pv);
+}
+
+template <typename T, int ADV, int KEG, int SCHEME>
+static int launch_step_dyn1_s(const DevDomain &d, Grid2 g, const T *un, const T *vn, const T *wn, T *ua, T *va,
+                              const nemo_hip_spg_ts_t *st, double **w)
+{
+    const double *r1u = st->r1_hu_n, *r1v = st->r1_hv_n, *ub = st->un_b, *vb = st->vn_b;
+    switch (ADV) {
+    case NEMO_NP_LIN_DYN:
+        LAUNCH("step_dyn", (k_step_dyn1<T, ADV, KEG, SCHEME, np_COR, np_COR, np_COR>), g, d, un, vn, wn, ua, va, r1u,
+               r1v, ub, vb, w[0], w[1], w[2], w[3], w[4], w[5]);
+        break;
+    case NEMO_NP_VEC_C2:
+        LAUNCH("step_dyn", (k_step_dyn1<T, ADV, KEG, SCHEME, np_COR, np_RVO, np_CRV>), g, d, un, vn, wn, ua, va, r1u,
+               r1v, ub, vb, w[0], w[1], w[2], w[3], w[4], w[5]);
+        break;
+    default:
+        LAUNCH("step_dyn", (k_step_dyn1<T, ADV, KEG, SCHEME, np_COR, np_MET, np_CME>), g, d, un, vn, wn, ua, va, r1u,
+               r1v, ub, vb, w[0], w[1], w[2], w[3], w[4], w[5]);
+        break;
+    }
+    return 0;
+}
+template <typename T, int SCHEME>
+static int launch_step_dyn1_v(const DevDomain &d, Grid2 g, const T *un, const T *vn, const T *wn, T *ua, T *va,
+                              const nemo_hip_spg_ts_t *st, double **w)
+{
+    switch (d.n_dynadv) {
+    case NEMO_NP_LIN_DYN: return launch_step_dyn1_s<T, NEMO_NP_LIN_DYN, 0, SCHEME>(d, g, un, vn, wn, ua, va, st, w);
+    case NEMO_NP_VEC_C2:
+        if (d.nn_dynkeg == 0) return launch_step_dyn1_s<T, NEMO_NP_VEC_C2, 0, SCHEME>(d, g, un, vn, wn, ua, va, st, w);
+        return launch_step_dyn1_s<T, NEMO_NP_VEC_C2, 1, SCHEME>(d, g, un, vn, wn, ua, va, st, w);
+    case NEMO_NP_FLX_C2: return launch_step_dyn1_s<T, NEMO_NP_FLX_C2, 0, SCHEME>(d, g, un, vn, wn, ua, va, st, w);
+    default: return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "step_dyn: n_dynadv = %d", d.n_dynadv);
+    }
+}
+
+template <typename T>
+int launch_step_dyn(int kt, double r2dt, const T *ub, const T *vb, const T *wn, const T *avm, const double *utau_b,
+                    const double *vtau_b, const nemo_hip_spg_ts_t *st)
+{
+    const DevDomain &d = nh::dom();
+    T *ua = (T *)st->ua, *va = (T *)st->va, *un = (T *)st->un, *vn = (T *)st->vn;
+    // the barotropic work arrays the phase-1 kernel of dyn_spg_ts would fill
+    // (zu_frc, zv_frc, zwxb, zwyb) plus the two vertical sums phase 3 needs
+    double *w[6] = {nh::scratch2(0), nh::scratch2(1), nh::scratch2(27), nh::scratch2(28), nh::scratch2(2),
+                    nh::scratch2(3)};
+    Grid2 g = grid2(d.jpi, d.jpj);
+    int rc;
+    switch (d.nvor_scheme) {
+    case NEMO_NP_ENE: rc = launch_step_dyn1_v<T, NEMO_NP_ENE>(d, g, un, vn, wn, ua, va, st, w); break;
+    case NEMO_NP_ENS: rc = launch_step_dyn1_v<T, NEMO_NP_ENS>(d, g, un, vn, wn, ua, va, st, w); break;
+    case NEMO_NP_MIX: rc = launch_step_dyn1_v<T, NEMO_NP_MIX>(d, g, un, vn, wn, ua, va, st, w); break;
+    case NEMO_NP_ENT: rc = launch_step_dyn1_v<T, NEMO_NP_ENT>(d, g, un, vn, wn, ua, va, st, w); break;
+    case NEMO_NP_EET: rc = launch_step_dyn1_v<T, NEMO_NP_EET>(d, g, un, vn, wn, ua, va, st, w); break;
+    default: rc = launch_step_dyn1_v<T, NEMO_NP_EEN>(d, g, un, vn, wn, ua, va, st, w); break;
+    }
+    if (rc) return rc;
+    if ((rc = launch_dyn_spg_ts<T>(kt, st, 1))) return rc;
+    const double z2dt_bf = (kt == d.nit000 && d.neuler == 0) ? d.rdt : 2.0 * d.rdt;
+    double *zwd = (double *)nh::scratch3(nh::S_ZWT);
+    LAUNCH("step_dyn", k_step_dyn2<T>, g, d, r2dt, 1.0 / z2dt_bf, ub, vb, ua, va, un, vn, avm, st->rCdU_bot, st->ua_b,
+           st->va_b, st->ub_b, st->vb_b, st->un_adv, st->vn_adv, st->r1_hu_n, st->r1_hv_n, st->un_b, st->vn_b, w[2],
+           w[3], st->utau, utau_b, st->vtau, vtau_b, zwd);
+    return 0;
+}
+template int launch_step_dyn<double>(int, double, const double *, const double *, const double *, const double *,
+                                     const double *, const double *, const nemo_hip_spg_ts_t *);
+template int launch_step_dyn<float>(int, double, const float *, const float *, const float *, const float *,
+                                    const double *, const double *, const nemo_hip_spg_ts_t *);

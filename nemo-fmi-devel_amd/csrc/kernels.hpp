@@ -28,5 +28,13 @@ int launch_dyn_zdf(double r2dt, const T *ub, const T *vb, T *ua, T *va, const T 
                    const double *ua_b, const double *va_b, const double *utau, const double *utau_b,
                    const double *vtau, const double *vtau_b);
 int spg_ts_weights(const DevDomain &d, int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2);
-template <typename T> int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st);
+// fused = 1: the 3D parts of phases 1 and 3 are left to the fused step kernels
+template <typename T> int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused = 0);
+template <typename T>
+int launch_step_dyn(int kt, double r2dt, const T *ub, const T *vb, const T *wn, const T *avm, const double *utau_b,
+                    const double *vtau_b, const nemo_hip_spg_ts_t *st);
 void spg_ts_reset();
+template <typename T>
+int launch_step_tra(double p2dt, double z2dt, double p2dt_zdf, const T *un, const T *vn, const T *wn, const T *tsb,
+                    const T *tsn, T *tsa, int kn_fct_h, int kn_fct_v, const T *ahtu, const T *ahtv, const T *uslp,
+                    const T *vslp, const T *wslpi, const T *wslpj, const T *avt, T *ah_wslp2, T *akz);

@@ -82,7 +82,7 @@ __device__ __forceinline__ void bt_coriolis(const DevDomain &d, int i, int j, co
 // Coriolis pre-computations, lines 241-342 (recomputed every call: with a
 // linear free surface they are constant, so this equals the kt == nit000 set)
 // ---------------------------------------------------------------------------
-__global__ void k_spg_cor1(DevDomain d, const double *ht_n, SpgWork w)
+KERNEL void k_spg_cor1(DevDomain d, const double * __restrict__ ht_n, SpgWork w)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi || j > d.jpj) return;
@@ -119,7 +119,7 @@ __global__ void k_spg_cor1(DevDomain d, const double *ht_n, SpgWork w)
     }
 }
 
-__global__ void k_spg_cor2(DevDomain d, const double *ht_n, SpgWork w)
+KERNEL void k_spg_cor2(DevDomain d, const double * __restrict__ ht_n, SpgWork w)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi || j > d.jpj) return;
@@ -156,7 +156,7 @@ __global__ void k_spg_cor2(DevDomain d, const double *ht_n, SpgWork w)
     }
 }
 
-__global__ void k_spg_cor3(DevDomain d, SpgWork w)
+KERNEL void k_spg_cor3(DevDomain d, SpgWork w)
 {
     // EEN triads from the lbc-filled zwz, lines 267-275
     THREAD_IJ(1, 1);
@@ -178,8 +178,8 @@ __global__ void k_spg_cor3(DevDomain d, SpgWork w)
 // bottom baroclinic velocities (lines 519-527)
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_spg_p1a(DevDomain d, T *ua, T *va, const T *un, const T *vn, const double *r1_hu_n,
-                          const double *r1_hv_n, const double *un_b, const double *vn_b, SpgWork w)
+KERNEL void k_spg_p1a(DevDomain d, T * __restrict__ ua, T * __restrict__ va, const T * __restrict__ un, const T * __restrict__ vn, const double * __restrict__ r1_hu_n,
+                          const double * __restrict__ r1_hv_n, const double * __restrict__ un_b, const double * __restrict__ vn_b, SpgWork w)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi || j > d.jpj) return;
@@ -207,10 +207,10 @@ __global__ void k_spg_p1a(DevDomain d, T *ua, T *va, const T *un, const T *vn, c
 }
 
 // Phase 1b: barotropic forcing (lines 386-639)
-__global__ void k_spg_p1b(DevDomain d, const double *un_b, const double *vn_b, const double *hu_n,
-                          const double *hv_n, const double *r1_hu_n, const double *r1_hv_n, const double *ht_n,
-                          const double *rCdU_bot, const double *utau, const double *vtau, const double *emp,
-                          const double *rnf, SpgWork w)
+KERNEL void k_spg_p1b(DevDomain d, const double * __restrict__ un_b, const double * __restrict__ vn_b, const double * __restrict__ hu_n,
+                          const double * __restrict__ hv_n, const double * __restrict__ r1_hu_n, const double * __restrict__ r1_hv_n, const double * __restrict__ ht_n,
+                          const double * __restrict__ rCdU_bot, const double * __restrict__ utau, const double * __restrict__ vtau, const double * __restrict__ emp,
+                          const double * __restrict__ rnf, SpgWork w)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi || j > d.jpj) return;
@@ -288,9 +288,9 @@ __global__ void k_spg_p1b(DevDomain d, const double *un_b, const double *vn_b, c
 // ---------------------------------------------------------------------------
 // Phase 2 sub-steps
 // ---------------------------------------------------------------------------
-__global__ void k_spg_s1(DevDomain d, double za1, double za2, double za3, double wgt2, const double *un_e,
-                         const double *ub_e, const double *ubb_e, const double *vn_e, const double *vb_e,
-                         const double *vbb_e, const double *hu_n, const double *hv_n, double *un_adv, double *vn_adv,
+KERNEL void k_spg_s1(DevDomain d, double za1, double za2, double za3, double wgt2, const double * __restrict__ un_e,
+                         const double * __restrict__ ub_e, const double * __restrict__ ubb_e, const double * __restrict__ vn_e, const double * __restrict__ vb_e,
+                         const double * __restrict__ vbb_e, const double * __restrict__ hu_n, const double * __restrict__ hv_n, double * __restrict__ un_adv, double * __restrict__ vn_adv,
                          SpgWork w)
 {
     THREAD_IJ(1, 1);
@@ -309,8 +309,8 @@ __global__ void k_spg_s1(DevDomain d, double za1, double za2, double za3, double
     vn_adv[ij] = vn_adv[ij] + wgt2 * zwy * m.r1_e1v[ij];
 }
 
-__global__ void k_spg_s2(DevDomain d, double rdtbt, double za0, double za1, double za2, double za3, double wgt1,
-                         const double *sshn_e, const double *sshb_e, double *sshbb_to_a, double *ssha, SpgWork w)
+KERNEL void k_spg_s2(DevDomain d, double rdtbt, double za0, double za1, double za2, double za3, double wgt1,
+                         const double * __restrict__ sshn_e, const double * __restrict__ sshb_e, double * __restrict__ sshbb_to_a, double * __restrict__ ssha, SpgWork w)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi || j > d.jpj) return;
@@ -356,9 +356,9 @@ __device__ __forceinline__ void spg_new_vel(const DevDomain &d, int i, int j, do
     va = (vn_e[ij] + rdtbt * (zwy + zv_trd + w.zv_frc[ij])) * d.ssvmask[ij];
 }
 
-__global__ void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double *un_e, const double *vn_e,
-                         double *ubb_to_a, double *vbb_to_a, const double *r1_hu_n, const double *r1_hv_n,
-                         const double *hu_n, const double *hv_n, const double *ht_n, double *ua_b, double *va_b,
+KERNEL void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double * __restrict__ un_e, const double * __restrict__ vn_e,
+                         double * __restrict__ ubb_to_a, double * __restrict__ vbb_to_a, const double * __restrict__ r1_hu_n, const double * __restrict__ r1_hv_n,
+                         const double * __restrict__ hu_n, const double * __restrict__ hv_n, const double * __restrict__ ht_n, double * __restrict__ ua_b, double * __restrict__ va_b,
                          SpgWork w)
 {
     THREAD_IJ(1, 1);
@@ -377,8 +377,8 @@ __global__ void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double *u
 // ---------------------------------------------------------------------------
 // Phase 3, lines 1214-1269
 // ---------------------------------------------------------------------------
-__global__ void k_spg_p3a(DevDomain d, int first_euler, double *un_adv, double *vn_adv, double *ub2_b,
-                          double *vb2_b, double *un_bf, double *vn_bf)
+KERNEL void k_spg_p3a(DevDomain d, int first_euler, double * __restrict__ un_adv, double * __restrict__ vn_adv, double * __restrict__ ub2_b,
+                          double * __restrict__ vb2_b, double * __restrict__ un_bf, double * __restrict__ vn_bf)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi || j > d.jpj) return;
@@ -399,10 +399,10 @@ __global__ void k_spg_p3a(DevDomain d, int first_euler, double *un_adv, double *
 }
 
 template <typename T>
-__global__ void k_spg_p3b(DevDomain d, double r1_2dt_b, T *ua, T *va, T *un, T *vn, const double *ua_b,
-                          const double *va_b, const double *ub_b, const double *vb_b, const double *un_adv,
-                          const double *vn_adv, const double *r1_hu_n, const double *r1_hv_n, const double *un_b,
-                          const double *vn_b)
+KERNEL void k_spg_p3b(DevDomain d, double r1_2dt_b, T * __restrict__ ua, T * __restrict__ va, T * __restrict__ un, T * __restrict__ vn, const double * __restrict__ ua_b,
+                          const double * __restrict__ va_b, const double * __restrict__ ub_b, const double * __restrict__ vb_b, const double * __restrict__ un_adv,
+                          const double * __restrict__ vn_adv, const double * __restrict__ r1_hu_n, const double * __restrict__ r1_hv_n, const double * __restrict__ un_b,
+                          const double * __restrict__ vn_b)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi || j > d.jpj) return;
@@ -472,7 +472,7 @@ int spg_ts_weights(const DevDomain &d, int ll_av, int ll_fw, int *jpit, double *
 }
 
 template <typename T>
-int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st)
+int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
 {
     const DevDomain &d = nh::dom();
     if (!d.ln_bt_fw) return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "dyn_spg_ts: ln_bt_fw = F is not implemented");
@@ -502,8 +502,9 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st)
     LAUNCH("dyn_spg_ts", k_spg_cor1, g, d, st->ht_n, w);
     LAUNCH("dyn_spg_ts", k_spg_cor2, g, d, st->ht_n, w);
     if (d.nvor_scheme == NEMO_NP_EEN) LAUNCH("dyn_spg_ts", k_spg_cor3, g, d, w);
-    LAUNCH("dyn_spg_ts", k_spg_p1a<T>, g, d, ua, va, (const T *)un, (const T *)vn, st->r1_hu_n, st->r1_hv_n, st->un_b,
-           st->vn_b, w);
+    if (!fused)
+        LAUNCH("dyn_spg_ts", k_spg_p1a<T>, g, d, ua, va, (const T *)un, (const T *)vn, st->r1_hu_n, st->r1_hv_n,
+               st->un_b, st->vn_b, w);
     LAUNCH("dyn_spg_ts", k_spg_p1b, g, d, st->un_b, st->vn_b, st->hu_n, st->hv_n, st->r1_hu_n, st->r1_hv_n, st->ht_n,
            st->rCdU_bot, st->utau, st->vtau, st->emp, st->rnf, w);
 
@@ -545,11 +546,12 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st)
     }
     LAUNCH("dyn_spg_ts", k_spg_p3a, g, d, (kt == d.nit000 && d.neuler == 0) ? 1 : 0, st->un_adv, st->vn_adv, st->ub2_b,
            st->vb2_b, st->un_bf, st->vn_bf);
-    LAUNCH("dyn_spg_ts", k_spg_p3b<T>, g, d, r1_2dt_b, ua, va, un, vn, st->ua_b, st->va_b, st->ub_b, st->vb_b,
-           st->un_adv, st->vn_adv, st->r1_hu_n, st->r1_hv_n, st->un_b, st->vn_b);
+    if (!fused)
+        LAUNCH("dyn_spg_ts", k_spg_p3b<T>, g, d, r1_2dt_b, ua, va, un, vn, st->ua_b, st->va_b, st->ub_b, st->vb_b,
+               st->un_adv, st->vn_adv, st->r1_hu_n, st->r1_hv_n, st->un_b, st->vn_b);
     return 0;
 }
-template int launch_dyn_spg_ts<double>(int, const nemo_hip_spg_ts_t *);
-template int launch_dyn_spg_ts<float>(int, const nemo_hip_spg_ts_t *);
+template int launch_dyn_spg_ts<double>(int, const nemo_hip_spg_ts_t *, int);
+template int launch_dyn_spg_ts<float>(int, const nemo_hip_spg_ts_t *, int);
 
 void spg_ts_reset() { g_spg = SpgState(); }

@@ -23,10 +23,10 @@
 #include "kernels.hpp"
 
 // ---------------------------------------------------------------------------
-// a3: effective transport, TRA/traadv.F90:100-124
+// effective transport, TRA/traadv.F90:100-124
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_tra_adv_trp(DevDomain d, const T *un, const T *vn, const T *wn, T *zun, T *zvn, T *zwn)
+KERNEL void k_tra_adv_trp(DevDomain d, const T * __restrict__ un, const T * __restrict__ vn, const T * __restrict__ wn, T * __restrict__ zun, T * __restrict__ zvn, T * __restrict__ zwn)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi || j > d.jpj) return;
@@ -87,8 +87,8 @@ __device__ __forceinline__ T up_z(const DevDomain &d, const T *pwn, const T *tb,
 
 // (1) low-order trend into pta and first guess zwi, traadv_fct.F90:158-170
 template <typename T>
-__global__ void k_fct_low(DevDomain d, double p2dt, const T *pun, const T *pvn, const T *pwn, const T *tb, T *ta,
-                          T *zwi)
+KERNEL void k_fct_low(DevDomain d, double p2dt, const T * __restrict__ pun, const T * __restrict__ pvn, const T * __restrict__ pwn, const T * __restrict__ tb, T * __restrict__ ta,
+                          T * __restrict__ zwi)
 {
     THREAD_IJ(2, 2);
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
@@ -135,8 +135,8 @@ __device__ __forceinline__ T fct_zltu(const DevDomain &d, const T *tn, int i, in
 }
 
 template <typename T>
-__global__ void k_fct_ad(DevDomain d, int kn_fct_h, int kn_fct_v, const T *pun, const T *pvn, const T *pwn,
-                         const T *tb, const T *tn, T *zwx, T *zwy, T *zwz, double *zwt)
+KERNEL void k_fct_ad(DevDomain d, int kn_fct_h, int kn_fct_v, const T * __restrict__ pun, const T * __restrict__ pvn, const T * __restrict__ pwn,
+                         const T * __restrict__ tb, const T * __restrict__ tn, T * __restrict__ zwx, T * __restrict__ zwy, T * __restrict__ zwz, double * __restrict__ zwt)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
@@ -228,8 +228,8 @@ __device__ __forceinline__ double zbdo(const DevDomain &d, const T *pbef, const 
     return a < b ? a : b;
 }
 template <typename T>
-__global__ void k_fct_beta(DevDomain d, double p2dt, const T *pbef, const T *paft, const T *paa, const T *pbb,
-                           const T *pcc, T *zbetup, T *zbetdo)
+KERNEL void k_fct_beta(DevDomain d, double p2dt, const T * __restrict__ pbef, const T * __restrict__ paft, const T * __restrict__ paa, const T * __restrict__ pbb,
+                           const T * __restrict__ pcc, T * __restrict__ zbetup, T * __restrict__ zbetdo)
 {
     THREAD_IJ(2, 2);
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
@@ -317,7 +317,7 @@ __device__ __forceinline__ double lim_z(const DevDomain &d, const T *pcc, const 
 
 // (4) limited fluxes and final trend, traadv_fct.F90:404-426 + 288-297
 template <typename T>
-__global__ void k_fct_final(DevDomain d, const T *paa, const T *pbb, const T *pcc, const T *bup, const T *bdo, T *ta)
+KERNEL void k_fct_final(DevDomain d, const T * __restrict__ paa, const T * __restrict__ pbb, const T * __restrict__ pcc, const T * __restrict__ bup, const T * __restrict__ bdo, T * __restrict__ ta)
 {
     THREAD_IJ(2, 2);
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
@@ -367,10 +367,10 @@ template int launch_tra_adv_fct<float>(double, const float *, const float *, con
                                        const float *, float *, int, int, int);
 
 // ---------------------------------------------------------------------------
-// a6: interp_4th_cpt alone (traadv_fct.F90:517-616), interior columns,
+// interp_4th_cpt alone (traadv_fct.F90:517-616), interior columns,
 // levels 2..jpkm1 of pt_out
 // ---------------------------------------------------------------------------
-__global__ void k_interp_4th_cpt(DevDomain d, const double *tin, double *tout, double *zwt)
+KERNEL void k_interp_4th_cpt(DevDomain d, const double * __restrict__ tin, double * __restrict__ tout, double * __restrict__ zwt)
 {
     THREAD_IJ(2, 2);
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
@@ -412,11 +412,11 @@ int launch_interp_4th_cpt(const double *tin, double *tout)
 }
 
 // ---------------------------------------------------------------------------
-// a8: tra_ldf_lap, TRA/traldf_lap_blp.F90:99-145 (ln_zps = F); all tracers
+// tra_ldf_lap, TRA/traldf_lap_blp.F90:99-145 (ln_zps = F); all tracers
 // of the call are updated by the same column thread
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_tra_ldf_lap(DevDomain d, T zsign, const T *pahu, const T *pahv, const T *ptb, T *pta, int kjpt)
+KERNEL void k_tra_ldf_lap(DevDomain d, T zsign, const T * __restrict__ pahu, const T * __restrict__ pahv, const T * __restrict__ ptb, T * __restrict__ pta, int kjpt)
 {
     THREAD_IJ(2, 2);
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
@@ -457,12 +457,12 @@ template int launch_tra_ldf_lap<double>(const double *, const double *, const do
 template int launch_tra_ldf_lap<float>(const float *, const float *, const float *, float *, int, int);
 
 // ---------------------------------------------------------------------------
-// a9: tra_ldf_iso, TRA/traldf_iso.F90:144-366
+// tra_ldf_iso, TRA/traldf_iso.F90:144-366
 // ---------------------------------------------------------------------------
 // pass-1 coefficients ah_wslp2 and akz, lines 146-202 (threads 1..jpim1 x 1..jpjm1)
 template <typename T>
-__global__ void k_ldf_iso_coef(DevDomain d, double z2dt, const T *pahu, const T *pahv, const T *wslpi,
-                               const T *wslpj, T *ah_wslp2, T *akz)
+KERNEL void k_ldf_iso_coef(DevDomain d, double z2dt, const T * __restrict__ pahu, const T * __restrict__ pahv, const T * __restrict__ wslpi,
+                               const T * __restrict__ wslpj, T * __restrict__ ah_wslp2, T * __restrict__ akz)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
@@ -506,6 +506,561 @@ __global__ void k_ldf_iso_coef(DevDomain d, double z2dt, const T *pahu, const T 
     }
 }
 
+// Per column the kernel marches down k with a register window: level k+1 of
+// tb is loaded once for the five columns (C, W, E, S, N) the stencil touches,
+// and every intermediate of the reference loop nests -- the masked gradients
+// zdit/zdjt (lines 218-225), the vertical gradients zdkt/zdk1t (250-254), the
+// w-point flux ztfw (298-357) -- is formed once and carried to the next level.
+// The values, and the order in which each expression combines them, are those
+// of the reference, so the FP64 form is bit-identical to it.  NT tracers share
+// one pass over the coefficient arrays.
+//
+// MODE 0: ln_traldf_lap (line 323); 1: bilaplacian first pass (kpass 1, line
+// 335); 2: bilaplacian second pass (kpass 2, line 341, reads ptbb).
+struct IsoCols {
+    uint32_t c, w, e, s, n;
+    int t0[5], t1[5];            // t-intervals of C W E S N (w-masks)
+    int u0[2], u1[2], v0[2], v1[2];   // u-intervals of C W, v-intervals of C S
+};
+__device__ __forceinline__ bool kin(int k, int a, int b) { return k >= a && k <= b; }
+template <typename T>
+__device__ __forceinline__ T wmk(const IsoCols &c, int q, int k)
+{
+    return (kin(k, c.t0[q], c.t1[q]) && (k == 1 || kin(k - 1, c.t0[q], c.t1[q]))) ? T(1) : T(0);
+}
+template <typename T>
+__device__ __forceinline__ T umk(const IsoCols &c, int q, int k) { return kin(k, c.u0[q], c.u1[q]) ? T(1) : T(0); }
+template <typename T>
+__device__ __forceinline__ T vmk(const IsoCols &c, int q, int k) { return kin(k, c.v0[q], c.v1[q]) ? T(1) : T(0); }
+
+enum { QC = 0, QW = 1, QE = 2, QS = 3, QN = 4 };
+
+template <typename T, int MODE, int NT>
+KERNEL void k_tra_ldf_iso(DevDomain d, T zsign, const T * __restrict__ pahu, const T * __restrict__ pahv,
+                          const T * __restrict__ uslp, const T * __restrict__ vslp, const T * __restrict__ wslpi,
+                          const T * __restrict__ wslpj, const T * __restrict__ ah_wslp2, const T * __restrict__ akz,
+                          const T * __restrict__ ptb, const T * __restrict__ ptbb, T * __restrict__ pta)
+{
+    THREAD_IJ(2, 2);
+    if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    const Metrics<T> &m = met<T>(d);
+    const size_t n3 = d.n3, plane = d.n2;
+    const int jpk = d.jpk;
+    IsoCols c;
+    c.c = IJ32(i, j); c.w = IJ32(i - 1, j); c.e = IJ32(i + 1, j); c.s = IJ32(i, j - 1); c.n = IJ32(i, j + 1);
+    const uint32_t col[5] = {c.c, c.w, c.e, c.s, c.n};
+#pragma unroll
+    for (int q = 0; q < 5; q++) { c.t0[q] = d.kT.k0[col[q]]; c.t1[q] = d.kT.k1[col[q]]; }
+    c.u0[0] = d.kU.k0[c.c]; c.u1[0] = d.kU.k1[c.c]; c.u0[1] = d.kU.k0[c.w]; c.u1[1] = d.kU.k1[c.w];
+    c.v0[0] = d.kV.k0[c.c]; c.v1[0] = d.kV.k1[c.c]; c.v0[1] = d.kV.k0[c.s]; c.v1[1] = d.kV.k1[c.s];
+
+    const T r1_e1e2t = m.r1_e1e2t[c.c], e1e2t = m.e1e2t[c.c], e2t = m.e2t[c.c], e1t = m.e1t[c.c];
+    const T e2_e1u_c = m.e2_e1u[c.c], e2_e1u_w = m.e2_e1u[c.w], e2u_c = m.e2u[c.c], e2u_w = m.e2u[c.w];
+    const T e1_e2v_c = m.e1_e2v[c.c], e1_e2v_s = m.e1_e2v[c.s], e1v_c = m.e1v[c.c], e1v_s = m.e1v[c.s];
+
+    // window state, level k ("now") -- per tracer
+    T tb[NT][5], zdit_c[NT], zdit_w[NT], zdjt_c[NT], zdjt_s[NT], dkp[NT][5], tbb_k[NT], ztfw_k[NT];
+    T ahu_c = ldp(pahu, c.c), ahu_w = ldp(pahu, c.w), ahv_c = ldp(pahv, c.c), ahv_s = ldp(pahv, c.s);
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const T *tbt = ptb + (size_t)t * n3;
+#pragma unroll
+        for (int q = 0; q < 5; q++) { tb[t][q] = ldp(tbt, col[q]); dkp[t][q] = T(0); }
+        zdit_c[t] = (tb[t][QE] - tb[t][QC]) * umk<T>(c, 0, 1);
+        zdit_w[t] = (tb[t][QC] - tb[t][QW]) * umk<T>(c, 1, 1);
+        zdjt_c[t] = (tb[t][QN] - tb[t][QC]) * vmk<T>(c, 0, 1);
+        zdjt_s[t] = (tb[t][QC] - tb[t][QS]) * vmk<T>(c, 1, 1);
+        tbb_k[t] = (MODE == 2) ? ldp(ptbb + (size_t)t * n3, c.c) : T(0);
+        ztfw_k[t] = T(0);
+    }
+    for (int k = 1; k <= jpk - 1; k++) {
+        // level k and k+1 planes (k+1 = jpk only feeds terms the reference
+        // multiplies by a zero mask or skips)
+        const T *ahu1 = plane_ptr(pahu, plane, k + 1), *ahv1 = plane_ptr(pahv, plane, k + 1);
+        const T ahu_c1 = ldp(ahu1, c.c), ahu_w1 = ldp(ahu1, c.w);
+        const T ahv_c1 = ldp(ahv1, c.c), ahv_s1 = ldp(ahv1, c.s);
+        const T *usk = plane_ptr(uslp, plane, k), *vsk = plane_ptr(vslp, plane, k);
+        const T us_c = ldp(usk, c.c), us_w = ldp(usk, c.w), vs_c = ldp(vsk, c.c), vs_s = ldp(vsk, c.s);
+        const T e3t = E3(m.e3t, c.c, k), e3u_c = E3(m.e3u, c.c, k), e3u_w = E3(m.e3u, c.w, k);
+        const T e3v_c = E3(m.e3v, c.c, k), e3v_s = E3(m.e3v, c.s, k);
+        const bool wlev = (k + 1 <= jpk - 1);       // ztfw(k+1) is a computed level
+        T wsi = T(0), wsj = T(0), ahw = T(0), akw = T(0), e3w1 = T(1);
+        if (wlev) {
+            wsi = ldp(plane_ptr(wslpi, plane, k + 1), c.c);
+            wsj = ldp(plane_ptr(wslpj, plane, k + 1), c.c);
+            ahw = ldp(plane_ptr(ah_wslp2, plane, k + 1), c.c);
+            if (MODE != 1) akw = ldp(plane_ptr(akz, plane, k + 1), c.c);
+            e3w1 = E3(m.e3w, c.c, k + 1);
+        }
+        // masks at k and k+1
+        const T wm_c = wmk<T>(c, QC, k), wm_w = wmk<T>(c, QW, k), wm_e = wmk<T>(c, QE, k), wm_s = wmk<T>(c, QS, k),
+                wm_n = wmk<T>(c, QN, k);
+        const T wm_c1 = wmk<T>(c, QC, k + 1), wm_w1 = wmk<T>(c, QW, k + 1), wm_e1 = wmk<T>(c, QE, k + 1),
+                wm_s1 = wmk<T>(c, QS, k + 1), wm_n1 = wmk<T>(c, QN, k + 1);
+        const T um_c = umk<T>(c, 0, k), um_w = umk<T>(c, 1, k), vm_c = vmk<T>(c, 0, k), vm_s = vmk<T>(c, 1, k);
+        const T um_c1 = umk<T>(c, 0, k + 1), um_w1 = umk<T>(c, 1, k + 1), vm_c1 = vmk<T>(c, 0, k + 1),
+                vm_s1 = vmk<T>(c, 1, k + 1);
+        // zftu / zftv mask weights and coefficients, lines 257-276
+        const T zmsku_c = T(1) / fmax2(wm_e + wm_c1 + wm_e1 + wm_c, T(1));
+        const T zmsku_w = T(1) / fmax2(wm_c + wm_w1 + wm_c1 + wm_w, T(1));
+        const T zmskv_c = T(1) / fmax2(wm_n + wm_c1 + wm_n1 + wm_c, T(1));
+        const T zmskv_s = T(1) / fmax2(wm_c + wm_s1 + wm_c1 + wm_s, T(1));
+        const T zabe1_c = ahu_c * e2_e1u_c * e3u_c, zabe1_w = ahu_w * e2_e1u_w * e3u_w;
+        const T zcof1_c = -(ahu_c * e2u_c * us_c * zmsku_c), zcof1_w = -(ahu_w * e2u_w * us_w * zmsku_w);
+        const T zabe2_c = ahv_c * e1_e2v_c * e3v_c, zabe2_s = ahv_s * e1_e2v_s * e3v_s;
+        const T zcof2_c = -(ahv_c * e1v_c * vs_c * zmskv_c), zcof2_s = -(ahv_s * e1v_s * vs_s * zmskv_s);
+        // ztfw(k+1) weights, lines 298-321
+        T zcoef3 = T(0), zcoef4 = T(0);
+        if (wlev) {
+            const T zmsku = wm_c1 / fmax2(um_c + um_w1 + um_w + um_c1, T(1));
+            const T zmskv = wm_c1 / fmax2(vm_c + vm_s1 + vm_s + vm_c1, T(1));
+            const T zahu_w = (ahu_c + ahu_w1 + ahu_w + ahu_c1) * zmsku;
+            const T zahv_w = (ahv_c + ahv_s1 + ahv_s + ahv_c1) * zmskv;
+            zcoef3 = -(zahu_w * e2t * zmsku * wsi);
+            zcoef4 = -(zahv_w * e1t * zmskv * wsj);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const T *tb1p = plane_ptr(ptb + (size_t)t * n3, plane, k + 1);
+            T tb1[5], dkn[5];
+#pragma unroll
+            for (int q = 0; q < 5; q++) tb1[q] = ldp(tb1p, col[q]);
+            // DK(q,k+1) = (tb(k) - tb(k+1)) * wmask(k+1) = zdk1t(q,k) = zdkt(q,k+1)
+            dkn[QC] = (tb[t][QC] - tb1[QC]) * wm_c1;
+            dkn[QW] = (tb[t][QW] - tb1[QW]) * wm_w1;
+            dkn[QE] = (tb[t][QE] - tb1[QE]) * wm_e1;
+            dkn[QS] = (tb[t][QS] - tb1[QS]) * wm_s1;
+            dkn[QN] = (tb[t][QN] - tb1[QN]) * wm_n1;
+            T zdkt[5];
+#pragma unroll
+            for (int q = 0; q < 5; q++) zdkt[q] = (k == 1) ? dkn[q] : dkp[t][q];
+            const T zftu_c = (zabe1_c * zdit_c[t] + zcof1_c * (zdkt[QE] + dkn[QC] + dkn[QE] + zdkt[QC])) * um_c;
+            const T zftu_w = (zabe1_w * zdit_w[t] + zcof1_w * (zdkt[QC] + dkn[QW] + dkn[QC] + zdkt[QW])) * um_w;
+            const T zftv_c = (zabe2_c * zdjt_c[t] + zcof2_c * (zdkt[QN] + dkn[QC] + dkn[QN] + zdkt[QC])) * vm_c;
+            const T zftv_s = (zabe2_s * zdjt_s[t] + zcof2_s * (zdkt[QC] + dkn[QS] + dkn[QC] + zdkt[QS])) * vm_s;
+            // gradients at k+1
+            const T zdit_c1 = (tb1[QE] - tb1[QC]) * um_c1, zdit_w1 = (tb1[QC] - tb1[QW]) * um_w1;
+            const T zdjt_c1 = (tb1[QN] - tb1[QC]) * vm_c1, zdjt_s1 = (tb1[QC] - tb1[QS]) * vm_s1;
+            T ztfw1 = T(0), tbb1 = T(0);
+            if (MODE == 2) tbb1 = ldp(plane_ptr(ptbb + (size_t)t * n3, plane, k + 1), c.c);
+            if (wlev) {
+                ztfw1 = zcoef3 * (zdit_c[t] + zdit_w1 + zdit_w[t] + zdit_c1) +
+                        zcoef4 * (zdjt_c[t] + zdjt_s1 + zdjt_s[t] + zdjt_c1);
+                const T dt = tb[t][QC] - tb1[QC];
+                if (MODE == 0) ztfw1 = ztfw1 + e1e2t / e3w1 * wm_c1 * (ahw - akw) * dt;
+                else if (MODE == 1) ztfw1 = ztfw1 + ahw * e1e2t * dt / e3w1 * wm_c1;
+                else ztfw1 = ztfw1 + e1e2t / e3w1 * wm_c1 * (ahw * dt + akw * (tbb_k[t] - tbb1));
+            }
+            const T hdiv = zftu_c - zftu_w + zftv_c - zftv_s;
+            T *tak = plane_ptr(pta + (size_t)t * n3, plane, k);
+            T a = ldp((const T *)tak, c.c);
+            a = a + zsign * hdiv * r1_e1e2t / e3t;
+            a = a + zsign * (ztfw_k[t] - ztfw1) * r1_e1e2t / e3t;
+            stp(tak, c.c, a);
+            // roll the window
+#pragma unroll
+            for (int q = 0; q < 5; q++) { tb[t][q] = tb1[q]; dkp[t][q] = dkn[q]; }
+            zdit_c[t] = zdit_c1; zdit_w[t] = zdit_w1; zdjt_c[t] = zdjt_c1; zdjt_s[t] = zdjt_s1;
+            ztfw_k[t] = ztfw1;
+            tbb_k[t] = tbb1;
+        }
+        ahu_c = ahu_c1; ahu_w = ahu_w1; ahv_c = ahv_c1; ahv_s = ahv_s1;
+    }
+}
+
+template <typename T, int MODE>
+static int launch_iso_mode(const DevDomain &d, T zsign, const T *pahu, const T *pahv, const T *uslp, const T *vslp,
+                           const T *wslpi, const T *wslpj, const T *ah_wslp2, const T *akz, const T *ptb,
+                           const T *ptbb, T *pta, int kjpt)
+{
+    Grid2 g = grid2(d.jpi - 2, d.jpj - 2);
+    int jn = 0;
+    for (; jn + 2 <= kjpt; jn += 2) {
+        const size_t o = (size_t)jn * d.n3;
+        LAUNCH("tra_ldf_iso", (k_tra_ldf_iso<T, MODE, 2>), g, d, zsign, pahu, pahv, uslp, vslp, wslpi, wslpj,
+               ah_wslp2, akz, ptb + o, ptbb + o, pta + o);
+    }
+    if (jn < kjpt) {
+        const size_t o = (size_t)jn * d.n3;
+        LAUNCH("tra_ldf_iso", (k_tra_ldf_iso<T, MODE, 1>), g, d, zsign, pahu, pahv, uslp, vslp, wslpi, wslpj,
+               ah_wslp2, akz, ptb + o, ptbb + o, pta + o);
+    }
+    return 0;
+}
+
+template <typename T>
+int launch_tra_ldf_iso(int first, double z2dt, const T *pahu, const T *pahv, const T *ptb, const T *ptbb, T *pta,
+                       int kjpt, int kpass, const T *uslp, const T *vslp, const T *wslpi, const T *wslpj, T *ah_wslp2,
+                       T *akz)
+{
+    const DevDomain &d = nh::dom();
+    if (kpass == 1) {
+        if (first) {
+            HIPCHK(hipMemsetAsync(akz, 0, d.n3 * sizeof(T), nh::stream()));
+            HIPCHK(hipMemsetAsync(ah_wslp2, 0, d.n3 * sizeof(T), nh::stream()));
+        }
+        Grid2 gh = grid2(d.jpi - 1, d.jpj - 1);
+        LAUNCH("tra_ldf_iso", k_ldf_iso_coef<T>, gh, d, z2dt, pahu, pahv, wslpi, wslpj, ah_wslp2, akz);
+        if (!d.ln_traldf_msc)
+            HIPCHK(hipMemcpyAsync(akz, ah_wslp2, d.n3 * sizeof(T), hipMemcpyDeviceToDevice, nh::stream()));
+    }
+    const T zsign = (kpass == 1) ? T(1) : T(-1);
+    if (d.ln_traldf_lap)
+        return launch_iso_mode<T, 0>(d, zsign, pahu, pahv, uslp, vslp, wslpi, wslpj, ah_wslp2, akz, ptb, ptbb, pta, kjpt);
+    if (kpass == 1)
+        return launch_iso_mode<T, 1>(d, zsign, pahu, pahv, uslp, vslp, wslpi, wslpj, ah_wslp2, akz, ptb, ptbb, pta, kjpt);
+    return launch_iso_mode<T, 2>(d, zsign, pahu, pahv, uslp, vslp, wslpi, wslpj, ah_wslp2, akz, ptb, ptbb, pta, kjpt);
+}
+template int launch_tra_ldf_iso<double>(int, double, const double *, const double *, const double *, const double *,
+                                        double *, int, int, const double *, const double *, const double *,
+                                        const double *, double *, double *);
+template int launch_tra_ldf_iso<float>(int, double, const float *, const float *, const float *, const float *,
+                                       float *, int, int, const float *, const float *, const float *, const float *,
+                                       float *, float *);
+
+// ---------------------------------------------------------------------------
+// tra_zdf_imp (+ the salinity reset of tra_zdf), TRA/trazdf.F90.
+// One thread per interior column: the forward sweep builds the matrix on the
+// fly (FP64), keeps T_k in a scratch column and substitutes every tracer of
+// the group; the backward sweep recomputes the upper diagonal.
+// ---------------------------------------------------------------------------
+// NT tracers share the matrix.  The sweeps are unrolled by four with the
+// pointers declared non-aliasing, so the loads of four levels are issued
+// together ahead of the serial FP64 recurrence.
+template <typename T, int NT>
+KERNEL void k_tra_zdf(DevDomain d, double p2dt, const T * __restrict__ ptb, T * __restrict__ pta, int jn0,
+                      const T * __restrict__ src, int l_ldfslp, const T * __restrict__ add,
+                      double * __restrict__ zwt, int clamp_jn)
+{
+    THREAD_IJ(2, 2);
+    if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    const uint32_t ij = IJ32(i, j);
+    const size_t plane = d.n2, n3 = d.n3;
+    const Metrics<T> &m = met<T>(d);
+    const int jpk = d.jpk;
+    const T *tb[NT];
+    T *ta[NT];
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+        tb[n] = ptb + (size_t)(jn0 + n) * n3;
+        ta[n] = pta + (size_t)(jn0 + n) * n3;
+    }
+    // zwt(k) of trazdf.F90:153-176: avt/avs, 0 at the surface, plus the
+    // iso-neutral contribution on 2..jpkm1
+    auto coef = [&](int k) -> double {
+        if (k <= 1) return 0.0;
+        double c = (double)ldp(plane_ptr(src, plane, k), ij);
+        if (l_ldfslp && k <= jpk - 1) c = c + (double)ldp(plane_ptr(add, plane, k), ij);
+        return c;
+    };
+    double prev[NT];
+    double zws_prev, zwt_prev, zc_k;
+    // k = 1 (zwi = 0)
+    {
+        const double zc_kp1 = coef(2);
+        const double e3t = (double)E3(m.e3t, ij, 1);
+        const double zwi = -(p2dt * 0.0 / (double)E3(m.e3w, ij, 1));
+        const double zws = -(p2dt * zc_kp1 / (double)E3(m.e3w, ij, 2));
+        const double zwd = e3t - zwi - zws;
+        stp(zwt, ij, zwd);
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            const double v = e3t * (double)ldp(tb[n], ij) + p2dt * e3t * (double)ldp((const T *)ta[n], ij);
+            prev[n] = v;
+            stp(ta[n], ij, (T)v);
+        }
+        zws_prev = zws;
+        zwt_prev = zwd;
+        zc_k = zc_kp1;
+    }
+#pragma unroll 4
+    for (int k = 2; k <= jpk - 1; k++) {
+        const double zc_kp1 = coef(k + 1);
+        const double e3t = (double)E3(m.e3t, ij, k);
+        const double zwi = -(p2dt * zc_k / (double)E3(m.e3w, ij, k));
+        const double zws = -(p2dt * zc_kp1 / (double)E3(m.e3w, ij, k + 1));
+        const double zwd = e3t - zwi - zws;
+        const double zwt_k = zwd - zwi * zws_prev / zwt_prev;
+        stp(plane_ptr(zwt, plane, k), ij, zwt_k);
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            const double zrhs = e3t * (double)ldp(plane_ptr(tb[n], plane, k), ij) +
+                                p2dt * e3t * (double)ldp((const T *)plane_ptr(ta[n], plane, k), ij);
+            const double v = zrhs - zwi / zwt_prev * prev[n];
+            prev[n] = v;
+            stp(plane_ptr(ta[n], plane, k), ij, (T)v);
+        }
+        zws_prev = zws;
+        zwt_prev = zwt_k;
+        zc_k = zc_kp1;
+    }
+    // back substitution, the result is the after tracer
+    {
+        const double tm = TMASK(double, i, j, jpk - 1);
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            const double v = prev[n] / zwt_prev * tm;
+            prev[n] = v;
+            T out = (T)v;
+            if (jn0 + n == clamp_jn && out < T(0)) out = T(0.1);
+            stp(plane_ptr(ta[n], plane, jpk - 1), ij, out);
+        }
+    }
+    const int t0 = d.kT.k0[ij], t1 = d.kT.k1[ij];
+#pragma unroll 4
+    for (int k = jpk - 2; k >= 1; k--) {
+        const double zws = -(p2dt * coef(k + 1) / (double)E3(m.e3w, ij, k + 1));
+        const double zwt_k = ldp((const double *)plane_ptr(zwt, plane, k), ij);
+        const double tm = (k >= t0 && k <= t1) ? 1.0 : 0.0;
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            const double v = ((double)ldp((const T *)plane_ptr(ta[n], plane, k), ij) - zws * prev[n]) / zwt_k * tm;
+            prev[n] = v;
+            T out = (T)v;
+            if (jn0 + n == clamp_jn && out < T(0)) out = T(0.1);
+            stp(plane_ptr(ta[n], plane, k), ij, out);
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+        if (jn0 + n == clamp_jn) {
+            T *sa = plane_ptr(ta[n], plane, jpk);
+            if (ldp((const T *)sa, ij) < T(0)) stp(sa, ij, T(0.1));
+        }
+    }
+}
+
+// WHERE( tsa(:,:,:,jp_sal) < 0 ) tsa = 0.1 on the columns the solve does not touch
+template <typename T>
+KERNEL void k_clamp_ring(DevDomain d, T * __restrict__ sa)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj || is_interior(d, i, j)) return;
+    for (int k = 1; k <= d.jpk; k++)
+        if (A3(sa, i, j, k) < T(0)) A3(sa, i, j, k) = T(0.1);
+}
+
+template <typename T>
+int launch_tra_zdf_imp(double p2dt, int is_tra, const T *ptb, T *pta, int kjpt, const T *avt, const T *avs,
+                       int l_ldfslp, const T *add, int clamp_sal)
+{
+    const DevDomain &d = nh::dom();
+    Grid2 g = grid2(d.jpi - 2, d.jpj - 2);
+    double *zwt = (double *)nh::scratch3(nh::S_ZWT);
+    const int clamp_jn = clamp_sal ? 1 : -1;
+    // matrix groups: TRA with ln_zdfddm: T on avt, S on avs; otherwise one
+    // matrix (avt for TRA, avs for TRC) shared by every tracer
+    if (is_tra && d.ln_zdfddm) {
+        for (int jn = 0; jn < kjpt; jn++)
+            LAUNCH("tra_zdf", (k_tra_zdf<T, 1>), g, d, p2dt, ptb, pta, jn, jn == 0 ? avt : avs, l_ldfslp, add, zwt,
+                   clamp_jn);
+    } else {
+        const T *src = is_tra ? avt : avs;
+        int jn = 0;
+        for (; jn + 2 <= kjpt; jn += 2)
+            LAUNCH("tra_zdf", (k_tra_zdf<T, 2>), g, d, p2dt, ptb, pta, jn, src, l_ldfslp, add, zwt, clamp_jn);
+        if (jn < kjpt)
+            LAUNCH("tra_zdf", (k_tra_zdf<T, 1>), g, d, p2dt, ptb, pta, jn, src, l_ldfslp, add, zwt, clamp_jn);
+    }
+    if (clamp_sal) {
+        Grid2 ga = grid2(d.jpi, d.jpj);
+        LAUNCH("tra_zdf", k_clamp_ring<T>, ga, d, pta + d.n3);
+    }
+    return 0;
+}
+template int launch_tra_zdf_imp<double>(double, int, const double *, double *, int, const double *, const double *,
+                                        int, const double *, int);
+template int launch_tra_zdf_imp<float>(double, int, const float *, float *, int, const float *, const float *, int,
+                                       const float *, int);
+
+// ===========================================================================
+// Fused tracer step: tsa = 0; tra_adv (transport + FCT); tra_ldf_iso (one
+// pass, laplacian); tra_zdf -- the order of step.F90:234-257 -- in four
+// passes over the two active tracers:
+//   k_step_tra0  first guess zwi and the forward sweep of interp_4th_cpt
+//                (column, top-down);
+//   k_step_trab  vertical antidiffusive flux (back substitution, bottom-up)
+//                and the Zalesak betas, lbc_lnk'd through lbc_src;
+//   k_step_trac  low-order trend, limited antidiffusive divergence, rotated
+//                laplacian and the forward sweep of the implicit vertical
+//                diffusion (top-down);
+//   k_step_trad  back substitution, salinity reset.
+// The transports of tra_adv are evaluated where used, the horizontal
+// antidiffusive fluxes are recomputed where used, and every value the
+// separate routines store is rounded to the storage type at the same point,
+// so the fused step equals the sequence of separate calls bit for bit in
+// either storage form.  The pivots of interp_4th_cpt do not depend on the
+// data: rows 3..mbkt-1 follow p(k) = 4 - 1/p(k-1) from p(3) = 4, every other
+// row has pivot 1, so they come from a host table.
+// ===========================================================================
+#define STEP_MAXK 160
+template <typename T>
+struct PivotTab {
+    T p[STEP_MAXK + 2];
+};
+
+template <typename T>
+__device__ __forceinline__ T trp_u(const DevDomain &d, const T *un, int p, int q, int k)
+{
+    if (k >= d.jpk) return T(0);
+    const Metrics<T> &m = met<T>(d);
+    const size_t pq = IJ(p, q);
+    return m.e2u[pq] * E3(m.e3u, pq, k) * A3(un, p, q, k);
+}
+template <typename T>
+__device__ __forceinline__ T trp_v(const DevDomain &d, const T *vn, int p, int q, int k)
+{
+    if (k >= d.jpk) return T(0);
+    const Metrics<T> &m = met<T>(d);
+    const size_t pq = IJ(p, q);
+    return m.e1v[pq] * E3(m.e3v, pq, k) * A3(vn, p, q, k);
+}
+template <typename T>
+__device__ __forceinline__ T trp_w(const DevDomain &d, const T *wn, int p, int q, int k)
+{
+    if (k >= d.jpk) return T(0);
+    return met<T>(d).e1e2t[IJ(p, q)] * A3(wn, p, q, k);
+}
+template <typename T>
+__device__ __forceinline__ T fs_upx(const DevDomain &d, const T *un, const T *tb, int i, int j, int k)
+{
+    const T u = trp_u(d, un, i, j, k);
+    const T zfp = u + fabs(u), zfm = u - fabs(u);
+    return T(0.5) * (zfp * A3(tb, i, j, k) + zfm * A3(tb, i + 1, j, k));
+}
+template <typename T>
+__device__ __forceinline__ T fs_upy(const DevDomain &d, const T *vn, const T *tb, int i, int j, int k)
+{
+    const T v = trp_v(d, vn, i, j, k);
+    const T zfp = v + fabs(v), zfm = v - fabs(v);
+    return T(0.5) * (zfp * A3(tb, i, j, k) + zfm * A3(tb, i, j + 1, k));
+}
+template <typename T>
+__device__ __forceinline__ T fs_upz(const DevDomain &d, const T *wn, const T *tb, int i, int j, int k)
+{
+    if (k == 1) return trp_w(d, wn, i, j, 1) * A3(tb, i, j, 1);
+    if (k >= d.jpk) return T(0);
+    const T w = trp_w(d, wn, i, j, k);
+    const T zfp = w + fabs(w), zfm = w - fabs(w);
+    return T(0.5) * (zfp * A3(tb, i, j, k) + zfm * A3(tb, i, j, k - 1)) * WMASK(T, i, j, k);
+}
+// horizontal antidiffusive fluxes of k_fct_ad at (p,q) before lbc_lnk
+template <typename T, int NH4>
+__device__ __forceinline__ T fs_zwx(const DevDomain &d, const T *un, const T *tb, const T *tn, int p, int q, int k)
+{
+    const T hu = fs_upx(d, un, tb, p, q, k);
+    const T c2u = A3(tn, p, q, k) + A3(tn, p + 1, q, k);
+    const T u = trp_u(d, un, p, q, k);
+    if (NH4) return T(0.5) * u * (c2u + fct_zltu(d, tn, p, q, k, true) - fct_zltu(d, tn, p + 1, q, k, true)) - hu;
+    return T(0.5) * u * c2u - hu;
+}
+template <typename T, int NH4>
+__device__ __forceinline__ T fs_zwy(const DevDomain &d, const T *vn, const T *tb, const T *tn, int p, int q, int k)
+{
+    const T hv = fs_upy(d, vn, tb, p, q, k);
+    const T c2v = A3(tn, p, q, k) + A3(tn, p, q + 1, k);
+    const T v = trp_v(d, vn, p, q, k);
+    if (NH4) return T(0.5) * v * (c2v + fct_zltu(d, tn, p, q, k, false) - fct_zltu(d, tn, p, q + 1, k, false)) - hv;
+    return T(0.5) * v * c2v - hv;
+}
+// ... after lbc_lnk('U') / ('V')
+template <typename T, int NH4>
+__device__ __forceinline__ T fs_zwx_l(const DevDomain &d, const T *un, const T *tb, const T *tn, int p, int q, int k)
+{
+    int sp, sq;
+    if (!lbc_src(d, 'U', p, q, sp, sq)) return T(0);
+    return fs_zwx<T, NH4>(d, un, tb, tn, sp, sq, k);
+}
+template <typename T, int NH4>
+__device__ __forceinline__ T fs_zwy_l(const DevDomain &d, const T *vn, const T *tb, const T *tn, int p, int q, int k)
+{
+    int sp, sq;
+    if (!lbc_src(d, 'V', p, q, sp, sq)) return T(0);
+    return fs_zwy<T, NH4>(d, vn, tb, tn, sp, sq, k);
+}
+// interp_4th_cpt pivot of row k in a column whose last wet level is ikb
+template <typename T>
+__device__ __forceinline__ T cpt_piv(const PivotTab<T> &pt, int k, int ikb)
+{
+    return (k >= 3 && k < ikb) ? pt.p[k] : T(1);
+}
+
+// ---------------------------------------------------------------------------
+// pass 0 (one thread per interior column): the first guess zwi
+// (traadv_fct.F90:158-170) and the vertical antidiffusive flux zwz (lines
+// 255-278; for the compact scheme interp_4th_cpt's sweeps, lines 577-614,
+// with the forward values parked in zwz)
+// ---------------------------------------------------------------------------
+template <typename T, int NV4>
+KERNEL void k_step_tra0(DevDomain d, double p2dt, PivotTab<T> pt, const T *__restrict__ un,
+                        const T *__restrict__ vn, const T *__restrict__ wn, const T *__restrict__ ptb,
+                        const T *__restrict__ ptn, T *__restrict__ pzwi, T *__restrict__ pzwz)
+{
+    THREAD_IJ(2, 2);
+    if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    const Metrics<T> &m = met<T>(d);
+    const size_t ij = IJ(i, j), n3 = d.n3;
+    const int jpk = d.jpk;
+    const T r1_e1e2t = m.r1_e1e2t[ij];
+    const T zp2dt = (T)p2dt;
+    const int ikb = d.mbkt[ij];
+#pragma unroll 1
+    for (int t = 0; t < 2; t++) {
+        const T *tb = ptb + t * n3, *tn = ptn + t * n3;
+        T *zwi = pzwi + t * n3, *zwz = pzwz + t * n3;
+        T zwz_k = fs_upz(d, wn, tb, i, j, 1), zout_prev = T(0);
+#pragma unroll 1
+        for (int k = 1; k <= jpk - 1; k++) {
+            const T e3t = E3(m.e3t, ij, k);
+            const T tm = TMASK(T, i, j, k);
+            const size_t p = IJK(i, j, k);
+            const T zwz_kp1 = fs_upz(d, wn, tb, i, j, k + 1);
+            const T ztra = -((fs_upx(d, un, tb, i, j, k) - fs_upx(d, un, tb, i - 1, j, k) + fs_upy(d, vn, tb, i, j, k) -
+                              fs_upy(d, vn, tb, i, j - 1, k) + zwz_k - zwz_kp1) *
+                             r1_e1e2t);
+            zwi[p] = (e3t * tb[p] + zp2dt * ztra) / e3t * tm;
+            zwz_k = zwz_kp1;
+            if (k == 1) continue;
+            if (NV4) {
+                T zwrm, zout, zwi_c = T(0);
+                if (k == 2 || k == ikb) {
+                    zwrm = T(0.5) * (A3(tn, i, j, k - 1) + A3(tn, i, j, k));
+                } else {
+                    const T wm = WMASK(T, i, j, k);
+                    zwi_c = wm;
+                    zwrm = T(3) * wm * (A3(tn, i, j, k) + A3(tn, i, j, k - 1));
+                }
+                if (k == 2) zout = zwrm;
+                else zout = zwrm - zwi_c / cpt_piv(pt, k - 1, ikb) * zout_prev;
+                zwz[p] = zout;
+                zout_prev = zout;
+            } else {
+                zwz[p] = (trp_w(d, wn, i, j, k) * T(0.5) * (tn[p] + A3(tn, i, j, k - 1)) - fs_upz(d, wn, tb, i, j, k)) *
+                         WMASK(T, i, j, k);
+            }
+        }
+        zwi[IJK(i, j, jpk)] = T(0);
+        zwz[IJK(i, j, 1)] = T(0);
+        zwz[IJK(i, j, jpk)] = T(0);
+        if (NV4) {
+            T xnext = zout_prev / cpt_piv(pt, jpk - 1, ikb);
+            {
+                const size_t p = IJK(i, j, jpk - 1);
+                zwz[p] = (trp_w(d, wn, i, j, jpk - 1) * xnext - fs_upz(d, wn, tb, i, j, jpk - 1)) * WMASK(T, i, j, jpk - 1);
+            }
+#pragma unroll 1
+            for (int k = jpk - 2; k >= 2; k--) {
+                const T zws = (k == 2 || k == ikb) ? T(0) : WMASK(T, i, j, k);
+                const size_t p = IJK(i, j, k);
+                const T x = (zwz[p] - zws * xnext) / cpt_piv(pt, k, ikb);
+                zwz[p] = (trp_w(d, wn, i, j, k) * x - fs_upz(d, wn, tb, i, j, k)) * WMASK(T, i, j, k);
+                xnext = x;
+            }
+        }
+    }
+}
+
+// pointwise forms of the rotated-laplacian intermediates (traldf_iso.F90:218-276)
 template <typename T>
 __device__ __forceinline__ T iso_zdk1t(const DevDomain &d, const T *tb, int i, int j, int k)
 {
@@ -517,7 +1072,6 @@ __device__ __forceinline__ T iso_zdkt(const DevDomain &d, const T *tb, int i, in
     if (k == 1) return iso_zdk1t(d, tb, i, j, 1);
     return (A3(tb, i, j, k - 1) - A3(tb, i, j, k)) * WMASK(T, i, j, k);
 }
-// masked horizontal gradients zdit / zdjt, lines 218-225
 template <typename T>
 __device__ __forceinline__ T iso_zdit(const DevDomain &d, const T *tb, int i, int j, int k)
 {
@@ -528,7 +1082,6 @@ __device__ __forceinline__ T iso_zdjt(const DevDomain &d, const T *tb, int i, in
 {
     return (A3(tb, i, j + 1, k) - A3(tb, i, j, k)) * VMASK(T, i, j, k);
 }
-// zftu at (p,q,k), lines 257-271
 template <typename T>
 __device__ __forceinline__ T iso_zftu(const DevDomain &d, const T *pahu, const T *uslp, const T *tb, int p, int q, int k)
 {
@@ -555,24 +1108,157 @@ __device__ __forceinline__ T iso_zftv(const DevDomain &d, const T *pahv, const T
                                                         iso_zdk1t(d, tb, p, q + 1, k) + iso_zdkt(d, tb, p, q, k))) *
            VMASK(T, p, q, k);
 }
-// ztfw at (i,j,k), 2 <= k <= jpkm1, lines 298-357
+
+// ---------------------------------------------------------------------------
+// pass b (one thread per cell): the betas of nonosc (traadv_fct.F90:354-399),
+// ring cells taking the value lbc_lnk('T') gives them
+// ---------------------------------------------------------------------------
 template <typename T>
-__device__ __forceinline__ T iso_ztfw(const DevDomain &d, int kpass, const T *pahu, const T *pahv, const T *wslpi,
-                                      const T *wslpj, const T *ah_wslp2, const T *akz, const T *tb, const T *tbb,
-                                      int i, int j, int k)
+__device__ __forceinline__ T fs_zwi(const DevDomain &d, const T *zwi, int p, int q, int k)
 {
-    if (k <= 1 || k >= d.jpk) return T(0);
+    int sp, sq;
+    if (!lbc_src(d, 'T', p, q, sp, sq)) return T(0);
+    return A3(zwi, sp, sq, k);
+}
+template <typename T>
+__device__ __forceinline__ void fs_zbnd(const DevDomain &d, const T *pbef, const T *zwi, int i, int j, int k,
+                                        double &up, double &dn)
+{
+    const double zbig = 1.e+40;
+    const double tm = TMASK(double, i, j, k);
+    const double a = (double)A3(pbef, i, j, k) * tm;
+    const double b = (double)fs_zwi(d, zwi, i, j, k) * tm;
+    const double au = a - zbig * (1.0 - tm), bu = b - zbig * (1.0 - tm);
+    const double ad = a + zbig * (1.0 - tm), bd = b + zbig * (1.0 - tm);
+    up = au > bu ? au : bu;
+    dn = ad < bd ? ad : bd;
+}
+
+template <typename T, int NH4>
+KERNEL void k_step_trab(DevDomain d, double p2dt, const T *__restrict__ un, const T *__restrict__ vn,
+                        const T *__restrict__ ptb, const T *__restrict__ ptn, const T *__restrict__ pzwi,
+                        const T *__restrict__ pzwz, T *__restrict__ pbu, T *__restrict__ pbd)
+{
+    THREAD_IJ(1, 1);
+    const int k = (int)blockIdx.z + 1;
+    if (i > d.jpi || j > d.jpj) return;
+    const size_t n3 = d.n3;
+    const size_t p = IJK(i, j, k);
+    int ci = i, cj = j;
+    bool zero = (k == d.jpk);
+    if (!zero && !is_interior(d, i, j)) {
+        int si, sj;
+        if (!lbc_src(d, 'T', i, j, si, sj) || !is_interior(d, si, sj)) zero = true;
+        ci = si;
+        cj = sj;
+    }
+    if (zero) {
+        pbu[p] = T(0); pbd[p] = T(0);
+        pbu[n3 + p] = T(0); pbd[n3 + p] = T(0);
+        return;
+    }
+    const size_t cij = IJ(ci, cj);
+    const double zbt = d.md.e1e2t[cij] * E3(d.md.e3t, cij, k) / p2dt;
+    const double zrtrn = 1.e-15;
+    const int ikm1 = k - 1 > 1 ? k - 1 : 1;
+#pragma unroll 1
+    for (int t = 0; t < 2; t++) {
+        const T *tb = ptb + t * n3, *tn = ptn + t * n3, *zwi = pzwi + t * n3, *zwz = pzwz + t * n3;
+        double zup, zdo, u2, d2;
+        fs_zbnd(d, tb, zwi, ci, cj, k, zup, zdo);
+        fs_zbnd(d, tb, zwi, ci - 1, cj, k, u2, d2); zup = fmax2(zup, u2); zdo = fmin2(zdo, d2);
+        fs_zbnd(d, tb, zwi, ci + 1, cj, k, u2, d2); zup = fmax2(zup, u2); zdo = fmin2(zdo, d2);
+        fs_zbnd(d, tb, zwi, ci, cj - 1, k, u2, d2); zup = fmax2(zup, u2); zdo = fmin2(zdo, d2);
+        fs_zbnd(d, tb, zwi, ci, cj + 1, k, u2, d2); zup = fmax2(zup, u2); zdo = fmin2(zdo, d2);
+        fs_zbnd(d, tb, zwi, ci, cj, ikm1, u2, d2); zup = fmax2(zup, u2); zdo = fmin2(zdo, d2);
+        fs_zbnd(d, tb, zwi, ci, cj, k + 1, u2, d2); zup = fmax2(zup, u2); zdo = fmin2(zdo, d2);
+        const double a_im1 = (double)fs_zwx_l<T, NH4>(d, un, tb, tn, ci - 1, cj, k);
+        const double a_i = (double)fs_zwx_l<T, NH4>(d, un, tb, tn, ci, cj, k);
+        const double b_jm1 = (double)fs_zwy_l<T, NH4>(d, vn, tb, tn, ci, cj - 1, k);
+        const double b_j = (double)fs_zwy_l<T, NH4>(d, vn, tb, tn, ci, cj, k);
+        const double c_kp1 = (double)A3(zwz, ci, cj, k + 1), c_k = (double)A3(zwz, ci, cj, k);
+        const double zpos = fmax2(0.0, a_im1) - fmin2(0.0, a_i) + fmax2(0.0, b_jm1) - fmin2(0.0, b_j) +
+                            fmax2(0.0, c_kp1) - fmin2(0.0, c_k);
+        const double zneg = fmax2(0.0, a_i) - fmin2(0.0, a_im1) + fmax2(0.0, b_j) - fmin2(0.0, b_jm1) +
+                            fmax2(0.0, c_k) - fmin2(0.0, c_kp1);
+        const double paft_k = (double)A3(zwi, ci, cj, k);
+        const double bup = (zup - paft_k) / (zpos + zrtrn) * zbt;
+        const double bdo = (paft_k - zdo) / (zneg + zrtrn) * zbt;
+        if (sizeof(T) == 4) {
+            pbu[t * n3 + p] = (T)fmin2(fmax2(bup, -1.0e30), 1.0e30);
+            pbd[t * n3 + p] = (T)fmin2(fmax2(bdo, -1.0e30), 1.0e30);
+        } else {
+            pbu[t * n3 + p] = (T)bup;
+            pbd[t * n3 + p] = (T)bdo;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pass c (one thread per cell): low-order trend + limited antidiffusive
+// divergence (traadv_fct.F90:158-166, 288-297, 404-422) and the rotated
+// laplacian (traldf_iso.F90:146-366, ln_traldf_lap, kpass 1); writes the
+// trend and ah_wslp2 (akz with ln_traldf_msc) for tra_zdf_imp
+// ---------------------------------------------------------------------------
+template <typename T, int NH4>
+__device__ __forceinline__ double fs_lim_x(const DevDomain &d, const T *un, const T *tb, const T *tn, const T *bup,
+                                           const T *bdo, int p, int q, int k)
+{
+    int sp, sq;
+    if (!lbc_src(d, 'U', p, q, sp, sq)) return 0.0;
+    return lim_face<T>((double)fs_zwx<T, NH4>(d, un, tb, tn, sp, sq, k), A3(bdo, sp, sq, k), A3(bup, sp + 1, sq, k),
+                       A3(bup, sp, sq, k), A3(bdo, sp + 1, sq, k));
+}
+template <typename T, int NH4>
+__device__ __forceinline__ double fs_lim_y(const DevDomain &d, const T *vn, const T *tb, const T *tn, const T *bup,
+                                           const T *bdo, int p, int q, int k)
+{
+    int sp, sq;
+    if (!lbc_src(d, 'V', p, q, sp, sq)) return 0.0;
+    return lim_face<T>((double)fs_zwy<T, NH4>(d, vn, tb, tn, sp, sq, k), A3(bdo, sp, sq, k), A3(bup, sp, sq + 1, k),
+                       A3(bup, sp, sq, k), A3(bdo, sp, sq + 1, k));
+}
+// ah_wslp2 (and akz) at w-level k of an interior column, traldf_iso.F90:146-201
+template <typename T, int MSC>
+__device__ __forceinline__ void fs_ahw(const DevDomain &d, double z2dt, const T *pahu, const T *pahv, const T *wslpi,
+                                       const T *wslpj, int i, int j, int k, T &zmsku, T &zmskv, T &zahu_w, T &zahv_w,
+                                       T &ah, T &akz)
+{
+    const Metrics<T> &m = met<T>(d);
+    const T wm = WMASK(T, i, j, k);
+    zmsku = wm / fmax2(UMASK(T, i, j, k - 1) + UMASK(T, i - 1, j, k) + UMASK(T, i - 1, j, k - 1) + UMASK(T, i, j, k), T(1));
+    zmskv = wm / fmax2(VMASK(T, i, j, k - 1) + VMASK(T, i, j - 1, k) + VMASK(T, i, j - 1, k - 1) + VMASK(T, i, j, k), T(1));
+    zahu_w = (A3(pahu, i, j, k - 1) + A3(pahu, i - 1, j, k) + A3(pahu, i - 1, j, k - 1) + A3(pahu, i, j, k)) * zmsku;
+    zahv_w = (A3(pahv, i, j, k - 1) + A3(pahv, i, j - 1, k) + A3(pahv, i, j - 1, k - 1) + A3(pahv, i, j, k)) * zmskv;
+    const T wi = A3(wslpi, i, j, k), wj = A3(wslpj, i, j, k);
+    ah = zahu_w * wi * wi + zahv_w * wj * wj;
+    if (MSC) {
+        const size_t ij = IJ(i, j);
+        const T e1u = m.e1u[ij], e1u_w = m.e1u[IJ(i - 1, j)], e2v = m.e2v[ij], e2v_s = m.e2v[IJ(i, j - 1)];
+        const T zakz = T(0.25) * ((A3(pahu, i, j, k) + A3(pahu, i, j, k - 1)) / (e1u * e1u) +
+                                  (A3(pahu, i - 1, j, k) + A3(pahu, i - 1, j, k - 1)) / (e1u_w * e1u_w) +
+                                  (A3(pahv, i, j, k) + A3(pahv, i, j, k - 1)) / (e2v * e2v) +
+                                  (A3(pahv, i, j - 1, k) + A3(pahv, i, j - 1, k - 1)) / (e2v_s * e2v_s));
+        const T e3w = E3(m.e3w, ij, k);
+        const T z1_2dt = T(1) / (T)z2dt;
+        const T ze3w_2 = e3w * e3w;
+        const T zcoef0 = (T)z2dt * (zakz + ah / ze3w_2);
+        akz = fmax2(zcoef0 - T(0.5), T(0)) * ze3w_2 * z1_2dt;
+    } else {
+        akz = ah;
+    }
+}
+// ztfw at w-level k (2..jpkm1) of tracer tb, traldf_iso.F90:298-332 (laplacian form)
+template <typename T, int MSC>
+__device__ __forceinline__ T fs_ztfw(const DevDomain &d, double z2dt, const T *pahu, const T *pahv, const T *wslpi,
+                                     const T *wslpj, const T *tb, int i, int j, int k, T &ah_out, T &akz_out)
+{
     const Metrics<T> &m = met<T>(d);
     const size_t ij = IJ(i, j);
-    const T wm = WMASK(T, i, j, k);
-    const T zmsku = wm / fmax2(UMASK(T, i, j, k - 1) + UMASK(T, i - 1, j, k) + UMASK(T, i - 1, j, k - 1) +
-                                   UMASK(T, i, j, k), T(1));
-    const T zmskv = wm / fmax2(VMASK(T, i, j, k - 1) + VMASK(T, i, j - 1, k) + VMASK(T, i, j - 1, k - 1) +
-                                   VMASK(T, i, j, k), T(1));
-    const T zahu_w = (A3(pahu, i, j, k - 1) + A3(pahu, i - 1, j, k) + A3(pahu, i - 1, j, k - 1) + A3(pahu, i, j, k)) *
-                     zmsku;
-    const T zahv_w = (A3(pahv, i, j, k - 1) + A3(pahv, i, j - 1, k) + A3(pahv, i, j - 1, k - 1) + A3(pahv, i, j, k)) *
-                     zmskv;
+    T zmsku, zmskv, zahu_w, zahv_w, ah, akz;
+    fs_ahw<T, MSC>(d, z2dt, pahu, pahv, wslpi, wslpj, i, j, k, zmsku, zmskv, zahu_w, zahv_w, ah, akz);
+    ah_out = ah;
+    akz_out = akz;
     const T zcoef3 = -(zahu_w * m.e2t[ij] * zmsku * A3(wslpi, i, j, k));
     const T zcoef4 = -(zahv_w * m.e1t[ij] * zmskv * A3(wslpj, i, j, k));
     T ztfw = zcoef3 * (iso_zdit(d, tb, i, j, k - 1) + iso_zdit(d, tb, i - 1, j, k) + iso_zdit(d, tb, i - 1, j, k - 1) +
@@ -580,195 +1266,162 @@ __device__ __forceinline__ T iso_ztfw(const DevDomain &d, int kpass, const T *pa
              zcoef4 * (iso_zdjt(d, tb, i, j, k - 1) + iso_zdjt(d, tb, i, j - 1, k) + iso_zdjt(d, tb, i, j - 1, k - 1) +
                        iso_zdjt(d, tb, i, j, k));
     const T e1e2t = m.e1e2t[ij], e3w = E3(m.e3w, ij, k);
-    const size_t p = IJK(i, j, k);
-    const T dt = A3(tb, i, j, k - 1) - tb[p];
-    if (d.ln_traldf_lap) {
-        ztfw = ztfw + e1e2t / e3w * wm * (ah_wslp2[p] - akz[p]) * dt;
-    } else if (kpass == 1) {
-        ztfw = ztfw + ah_wslp2[p] * e1e2t * dt / e3w * wm;
-    } else {
-        ztfw = ztfw + e1e2t / e3w * wm * (ah_wslp2[p] * dt + akz[p] * (A3(tbb, i, j, k - 1) - tbb[p]));
-    }
-    return ztfw;
+    const T dt = A3(tb, i, j, k - 1) - A3(tb, i, j, k);
+    return ztfw + e1e2t / e3w * WMASK(T, i, j, k) * (ah - akz) * dt;
 }
 
-template <typename T>
-__global__ void k_tra_ldf_iso(DevDomain d, int kpass, T zsign, const T *pahu, const T *pahv, const T *uslp,
-                              const T *vslp, const T *wslpi, const T *wslpj, const T *ah_wslp2, const T *akz,
-                              const T *tb, const T *tbb, T *ta)
+template <typename T, int NH4>
+KERNEL void k_step_trac(DevDomain d, const T *__restrict__ un, const T *__restrict__ vn, const T *__restrict__ wn,
+                        const T *__restrict__ ptb, const T *__restrict__ ptn, const T *__restrict__ pbu,
+                        const T *__restrict__ pbd, const T *__restrict__ pzwz, T *__restrict__ pta)
 {
-    THREAD_IJ(2, 2);
-    if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    THREAD_IJ(1, 1);
+    const int k = (int)blockIdx.z + 1;
+    if (i > d.jpi || j > d.jpj) return;
+    const size_t n3 = d.n3;
+    const size_t p = IJK(i, j, k);
+    if (k == d.jpk || !is_interior(d, i, j)) {
+        pta[p] = T(0);
+        pta[n3 + p] = T(0);
+        return;
+    }
     const Metrics<T> &m = met<T>(d);
     const size_t ij = IJ(i, j);
     const T r1_e1e2t = m.r1_e1e2t[ij];
-    T ztfw_k = T(0);
-    for (int k = 1; k <= d.jpk - 1; k++) {
-        const T e3t = E3(m.e3t, ij, k);
-        const T hdiv = iso_zftu(d, pahu, uslp, tb, i, j, k) - iso_zftu(d, pahu, uslp, tb, i - 1, j, k) +
-                       iso_zftv(d, pahv, vslp, tb, i, j, k) - iso_zftv(d, pahv, vslp, tb, i, j - 1, k);
-        const T ztfw_kp1 = iso_ztfw(d, kpass, pahu, pahv, wslpi, wslpj, ah_wslp2, akz, tb, tbb, i, j, k + 1);
-        const size_t p = IJK(i, j, k);
-        T t = ta[p];
-        t = t + zsign * hdiv * r1_e1e2t / e3t;
-        t = t + zsign * (ztfw_k - ztfw_kp1) * r1_e1e2t / e3t;
-        ta[p] = t;
-        ztfw_k = ztfw_kp1;
+    const T e3t = E3(m.e3t, ij, k);
+    const T tm = TMASK(T, i, j, k);
+#pragma unroll 1
+    for (int t = 0; t < 2; t++) {
+        const T *tb = ptb + t * n3, *tn = ptn + t * n3;
+        const T *bu = pbu + t * n3, *bd = pbd + t * n3, *zwz = pzwz + t * n3;
+        const T ztra = -((fs_upx(d, un, tb, i, j, k) - fs_upx(d, un, tb, i - 1, j, k) + fs_upy(d, vn, tb, i, j, k) -
+                          fs_upy(d, vn, tb, i, j - 1, k) + fs_upz(d, wn, tb, i, j, k) - fs_upz(d, wn, tb, i, j, k + 1)) *
+                         r1_e1e2t);
+        T a = T(0);
+        a = a + ztra / e3t * tm;
+        const double lz_k = (k == 1) ? (double)A3(zwz, i, j, 1)
+                                     : lim_face<T>(A3(zwz, i, j, k), A3(bd, i, j, k), A3(bu, i, j, k - 1),
+                                                   A3(bu, i, j, k), A3(bd, i, j, k - 1));
+        const double lz_kp1 = lim_face<T>(A3(zwz, i, j, k + 1), A3(bd, i, j, k + 1), A3(bu, i, j, k),
+                                          A3(bu, i, j, k + 1), A3(bd, i, j, k));
+        const T lx_i = (T)fs_lim_x<T, NH4>(d, un, tb, tn, bu, bd, i, j, k);
+        const T lx_im1 = (T)fs_lim_x<T, NH4>(d, un, tb, tn, bu, bd, i - 1, j, k);
+        const T ly_j = (T)fs_lim_y<T, NH4>(d, vn, tb, tn, bu, bd, i, j, k);
+        const T ly_jm1 = (T)fs_lim_y<T, NH4>(d, vn, tb, tn, bu, bd, i, j - 1, k);
+        a = a - (lx_i - lx_im1 + ly_j - ly_jm1 + (T)lz_k - (T)lz_kp1) * r1_e1e2t / e3t;
+        pta[t * n3 + p] = a;
     }
 }
 
-template <typename T>
-int launch_tra_ldf_iso(int first, double z2dt, const T *pahu, const T *pahv, const T *ptb, const T *ptbb, T *pta,
-                       int kjpt, int kpass, const T *uslp, const T *vslp, const T *wslpi, const T *wslpj, T *ah_wslp2,
-                       T *akz)
-{
-    const DevDomain &d = nh::dom();
-    if (kpass == 1) {
-        if (first) {
-            HIPCHK(hipMemsetAsync(akz, 0, d.n3 * sizeof(T), nh::stream()));
-            HIPCHK(hipMemsetAsync(ah_wslp2, 0, d.n3 * sizeof(T), nh::stream()));
-        }
-        Grid2 gh = grid2(d.jpi - 1, d.jpj - 1);
-        LAUNCH("tra_ldf_iso", k_ldf_iso_coef<T>, gh, d, z2dt, pahu, pahv, wslpi, wslpj, ah_wslp2, akz);
-        if (!d.ln_traldf_msc)
-            HIPCHK(hipMemcpyAsync(akz, ah_wslp2, d.n3 * sizeof(T), hipMemcpyDeviceToDevice, nh::stream()));
-    }
-    const T zsign = (kpass == 1) ? T(1) : T(-1);
-    Grid2 g = grid2(d.jpi - 2, d.jpj - 2);
-    for (int jn = 0; jn < kjpt; jn++) {
-        const size_t o = (size_t)jn * d.n3;
-        LAUNCH("tra_ldf_iso", k_tra_ldf_iso<T>, g, d, kpass, zsign, pahu, pahv, uslp, vslp, wslpi, wslpj, ah_wslp2,
-               akz, ptb + o, ptbb + o, pta + o);
-    }
-    return 0;
-}
-template int launch_tra_ldf_iso<double>(int, double, const double *, const double *, const double *, const double *,
-                                        double *, int, int, const double *, const double *, const double *,
-                                        const double *, double *, double *);
-template int launch_tra_ldf_iso<float>(int, double, const float *, const float *, const float *, const float *,
-                                       float *, int, int, const float *, const float *, const float *, const float *,
-                                       float *, float *);
-
-// ---------------------------------------------------------------------------
-// a10/a11: tra_zdf_imp (+ the salinity reset of tra_zdf), TRA/trazdf.F90.
-// One thread per interior column: the forward sweep builds the matrix on the
-// fly (FP64), keeps T_k in a scratch column and substitutes every tracer of
-// the group; the backward sweep recomputes the upper diagonal.
-// ---------------------------------------------------------------------------
-#define ZDF_MAXT 4
-template <typename T>
-__device__ __forceinline__ double zdf_coef(const DevDomain &d, const T *src, int l_ldfslp, const T *add, int i, int j,
-                                           int k)
-{
-    // zwt(k) of trazdf.F90:153-176: avt/avs, 0 at the surface, plus the
-    // iso-neutral contribution on 2..jpkm1
-    if (k <= 1) return 0.0;
-    double c = (double)A3(src, i, j, k);
-    if (l_ldfslp && k <= d.jpk - 1) c = c + (double)A3(add, i, j, k);
-    return c;
-}
-
-template <typename T>
-__global__ void k_tra_zdf(DevDomain d, double p2dt, const T *ptb, T *pta, int jn0, int ntr, const T *src,
-                          int l_ldfslp, const T *add, double *zwt, int clamp_jn)
+// rotated laplacian added to the trend, one thread per interior cell
+template <typename T, int MSC>
+KERNEL void k_step_tra_iso(DevDomain d, double z2dt, const T *__restrict__ ptb, const T *__restrict__ pahu,
+                           const T *__restrict__ pahv, const T *__restrict__ uslp, const T *__restrict__ vslp,
+                           const T *__restrict__ wslpi, const T *__restrict__ wslpj, T *__restrict__ pta,
+                           T *__restrict__ ah_wslp2, T *__restrict__ akz)
 {
     THREAD_IJ(2, 2);
+    const int k = (int)blockIdx.z + 1;
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
-    const size_t ij = IJ(i, j);
+    const size_t n3 = d.n3;
+    const size_t p = IJK(i, j, k);
     const Metrics<T> &m = met<T>(d);
+    const size_t ij = IJ(i, j);
+    const T r1_e1e2t = m.r1_e1e2t[ij];
+    const T e3t = E3(m.e3t, ij, k);
     const int jpk = d.jpk;
-    double prev[ZDF_MAXT];
-    double zws_prev = 0.0, zwt_prev = 1.0;
-    double zc_k = zdf_coef(d, src, l_ldfslp, add, i, j, 1);
-    for (int k = 1; k <= jpk - 1; k++) {
-        const double zc_kp1 = zdf_coef(d, src, l_ldfslp, add, i, j, k + 1);
-        const double e3t = (double)E3(m.e3t, ij, k);
-        const double zwi = -(p2dt * zc_k / (double)E3(m.e3w, ij, k));
-        const double zws = -(p2dt * zc_kp1 / (double)E3(m.e3w, ij, k + 1));
-        const double zwd = e3t - zwi - zws;
-        const double zwt_k = (k == 1) ? zwd : zwd - zwi * zws_prev / zwt_prev;
-        A3(zwt, i, j, k) = zwt_k;
-        const size_t p = IJK(i, j, k);
-        for (int n = 0; n < ntr; n++) {
-            const size_t o = (size_t)(jn0 + n) * d.n3 + p;
-            const double zrhs = e3t * (double)ptb[o] + p2dt * e3t * (double)pta[o];
-            const double v = (k == 1) ? zrhs : zrhs - zwi / zwt_prev * prev[n];
-            prev[n] = v;
-            pta[o] = (T)v;
-        }
-        zws_prev = zws;
+    T ah_k = T(0), akz_k = T(0), ah_1, akz_1;
+#pragma unroll 1
+    for (int t = 0; t < 2; t++) {
+        const T *tb = ptb + t * n3;
+        const T hdiv = iso_zftu(d, pahu, uslp, tb, i, j, k) - iso_zftu(d, pahu, uslp, tb, i - 1, j, k) +
+                       iso_zftv(d, pahv, vslp, tb, i, j, k) - iso_zftv(d, pahv, vslp, tb, i, j - 1, k);
+        const T ztfw_k = (k >= 2) ? fs_ztfw<T, MSC>(d, z2dt, pahu, pahv, wslpi, wslpj, tb, i, j, k, ah_k, akz_k) : T(0);
+        const T ztfw_kp1 = (k + 1 <= jpk - 1)
+                               ? fs_ztfw<T, MSC>(d, z2dt, pahu, pahv, wslpi, wslpj, tb, i, j, k + 1, ah_1, akz_1)
+                               : T(0);
+        T a = pta[t * n3 + p];
+        a = a + hdiv * r1_e1e2t / e3t;
+        a = a + (ztfw_k - ztfw_kp1) * r1_e1e2t / e3t;
+        pta[t * n3 + p] = a;
+    }
+    if (k >= 2) {
+        ah_wslp2[p] = ah_k;
+        if (MSC) akz[p] = akz_k;
+    }
+}
+
+template <typename T>
+static PivotTab<T> cpt_pivots(int jpk)
+{
+    // interp_4th_cpt rows 3.. of a wet column: zwd = 3*1+1, zwi = zws = 1,
+    // zwt(k) = zwd - zwi*zws(k-1)/zwt(k-1) with row 2 an identity row
+    PivotTab<T> pt;
+    for (int k = 0; k < STEP_MAXK + 2; k++) pt.p[k] = T(1);
+    T zws_prev = T(0), zwt_prev = T(1);
+    for (int k = 3; k <= jpk && k <= STEP_MAXK; k++) {
+        const T wm = T(1);
+        const T zwd = T(3) * wm + T(1);
+        const T zwt_k = zwd - wm * zws_prev / zwt_prev;
+        pt.p[k] = zwt_k;
+        zws_prev = wm;
         zwt_prev = zwt_k;
-        zc_k = zc_kp1;
     }
-    // back substitution, the result is the after tracer
-    {
-        const double tm = TMASK(double, i, j, jpk - 1);
-        for (int n = 0; n < ntr; n++) {
-            const double v = prev[n] / zwt_prev * tm;
-            prev[n] = v;
-            T out = (T)v;
-            if (jn0 + n == clamp_jn && out < T(0)) out = T(0.1);
-            pta[(size_t)(jn0 + n) * d.n3 + IJK(i, j, jpk - 1)] = out;
-        }
-    }
-    for (int k = jpk - 2; k >= 1; k--) {
-        const double zws = -(p2dt * zdf_coef(d, src, l_ldfslp, add, i, j, k + 1) / (double)E3(m.e3w, ij, k + 1));
-        const double zwt_k = A3(zwt, i, j, k);
-        const double tm = TMASK(double, i, j, k);
-        const size_t p = IJK(i, j, k);
-        for (int n = 0; n < ntr; n++) {
-            const size_t o = (size_t)(jn0 + n) * d.n3 + p;
-            const double v = ((double)pta[o] - zws * prev[n]) / zwt_k * tm;
-            prev[n] = v;
-            T out = (T)v;
-            if (jn0 + n == clamp_jn && out < T(0)) out = T(0.1);
-            pta[o] = out;
-        }
-    }
-    if (clamp_jn >= jn0 && clamp_jn < jn0 + ntr) {
-        T *sa = pta + (size_t)clamp_jn * d.n3;
-        if (A3(sa, i, j, jpk) < T(0)) A3(sa, i, j, jpk) = T(0.1);
-    }
+    return pt;
 }
 
-// WHERE( tsa(:,:,:,jp_sal) < 0 ) tsa = 0.1 on the columns the solve does not touch
-template <typename T>
-__global__ void k_clamp_ring(DevDomain d, T *sa)
+template <typename T, int NH4, int NV4>
+static int launch_step_tra_hv(const DevDomain &d, double p2dt, double z2dt, double p2dt_zdf, const T *un, const T *vn,
+                              const T *wn, const T *tsb, const T *tsn, T *tsa, const T *ahtu, const T *ahtv,
+                              const T *uslp, const T *vslp, const T *wslpi, const T *wslpj, const T *avt, T *ah_wslp2,
+                              T *akz)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj || is_interior(d, i, j)) return;
-    for (int k = 1; k <= d.jpk; k++)
-        if (A3(sa, i, j, k) < T(0)) A3(sa, i, j, k) = T(0.1);
-}
-
-template <typename T>
-int launch_tra_zdf_imp(double p2dt, int is_tra, const T *ptb, T *pta, int kjpt, const T *avt, const T *avs,
-                       int l_ldfslp, const T *add, int clamp_sal)
-{
-    const DevDomain &d = nh::dom();
-    Grid2 g = grid2(d.jpi - 2, d.jpj - 2);
+    const PivotTab<T> pt = cpt_pivots<T>(d.jpk);
+    // scratch (storage type, two tracers each): zwi, zwz, betas
+    T *zwi = (T *)nh::scratch3(nh::S_ZWI), *zwz = (T *)nh::scratch3(nh::S_ZWX);
+    T *bu = (T *)nh::scratch3(nh::S_ZWY), *bd = (T *)nh::scratch3(nh::S_ZWZ);
+    Grid2 gi = grid2(d.jpi - 2, d.jpj - 2), g3 = grid2(d.jpi, d.jpj);
+    g3.grid.z = (unsigned)d.jpk;
+    LAUNCH("step_tra", (k_step_tra0<T, NV4>), gi, d, p2dt, pt, un, vn, wn, tsb, tsn, zwi, zwz);
+    LAUNCH("step_tra", (k_step_trab<T, NH4>), g3, d, p2dt, un, vn, tsb, tsn, zwi, zwz, bu, bd);
+    LAUNCH("step_tra", (k_step_trac<T, NH4>), g3, d, un, vn, wn, tsb, tsn, bu, bd, zwz, tsa);
+    Grid2 gk = grid2(d.jpi - 2, d.jpj - 2);
+    gk.grid.z = (unsigned)(d.jpk - 1);
+    if (d.ln_traldf_msc)
+        LAUNCH("step_tra", (k_step_tra_iso<T, 1>), gk, d, z2dt, tsb, ahtu, ahtv, uslp, vslp, wslpi, wslpj, tsa,
+               ah_wslp2, akz);
+    else
+        LAUNCH("step_tra", (k_step_tra_iso<T, 0>), gk, d, z2dt, tsb, ahtu, ahtv, uslp, vslp, wslpi, wslpj, tsa,
+               ah_wslp2, akz);
+    // tra_zdf: the shared matrix on avt plus the iso-neutral coefficient, salinity reset
     double *zwt = (double *)nh::scratch3(nh::S_ZWT);
-    const int clamp_jn = clamp_sal ? 1 : -1;
-    // matrix groups: TRA with ln_zdfddm: T on avt, S on avs; otherwise one
-    // matrix (avt for TRA, avs for TRC) shared by every tracer
-    if (is_tra && d.ln_zdfddm) {
-        for (int jn = 0; jn < kjpt; jn++)
-            LAUNCH("tra_zdf", k_tra_zdf<T>, g, d, p2dt, ptb, pta, jn, 1, jn == 0 ? avt : avs, l_ldfslp, add, zwt,
-                   clamp_jn);
-    } else {
-        const T *src = is_tra ? avt : avs;
-        for (int jn = 0; jn < kjpt; jn += ZDF_MAXT) {
-            int nt = kjpt - jn < ZDF_MAXT ? kjpt - jn : ZDF_MAXT;
-            LAUNCH("tra_zdf", k_tra_zdf<T>, g, d, p2dt, ptb, pta, jn, nt, src, l_ldfslp, add, zwt, clamp_jn);
-        }
-    }
-    if (clamp_sal) {
-        Grid2 ga = grid2(d.jpi, d.jpj);
-        LAUNCH("tra_zdf", k_clamp_ring<T>, ga, d, pta + d.n3);
-    }
+    const T *add = d.ln_traldf_msc ? (const T *)akz : (const T *)ah_wslp2;
+    LAUNCH("step_tra", (k_tra_zdf<T, 2>), gi, d, p2dt_zdf, tsb, tsa, 0, avt, 1, add, zwt, 1);
     return 0;
 }
-template int launch_tra_zdf_imp<double>(double, int, const double *, double *, int, const double *, const double *,
-                                        int, const double *, int);
-template int launch_tra_zdf_imp<float>(double, int, const float *, float *, int, const float *, const float *, int,
-                                       const float *, int);
+
+template <typename T>
+int launch_step_tra(double p2dt, double z2dt, double p2dt_zdf, const T *un, const T *vn, const T *wn, const T *tsb,
+                    const T *tsn, T *tsa, int kn_fct_h, int kn_fct_v, const T *ahtu, const T *ahtv, const T *uslp,
+                    const T *vslp, const T *wslpi, const T *wslpj, const T *avt, T *ah_wslp2, T *akz)
+{
+    const DevDomain &d = nh::dom();
+    if (d.jpk > STEP_MAXK) return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "step_tra: jpk = %d > %d", d.jpk, STEP_MAXK);
+    if (!d.ln_traldf_lap) return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "step_tra: bilaplacian diffusion is not fused");
+    if (d.ln_zdfddm) return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "step_tra: ln_zdfddm = T is not fused");
+    const int h4 = kn_fct_h == 4, v4 = kn_fct_v == 4;
+#define STEP_TRA_ARGS d, p2dt, z2dt, p2dt_zdf, un, vn, wn, tsb, tsn, tsa, ahtu, ahtv, uslp, vslp, wslpi, wslpj, avt, ah_wslp2, akz
+    if (h4 && v4) return launch_step_tra_hv<T, 1, 1>(STEP_TRA_ARGS);
+    if (h4) return launch_step_tra_hv<T, 1, 0>(STEP_TRA_ARGS);
+    if (v4) return launch_step_tra_hv<T, 0, 1>(STEP_TRA_ARGS);
+    return launch_step_tra_hv<T, 0, 0>(STEP_TRA_ARGS);
+#undef STEP_TRA_ARGS
+}
+template int launch_step_tra<double>(double, double, double, const double *, const double *, const double *,
+                                     const double *, const double *, double *, int, int, const double *,
+                                     const double *, const double *, const double *, const double *, const double *,
+                                     const double *, double *, double *);
+template int launch_step_tra<float>(double, double, double, const float *, const float *, const float *,
+                                    const float *, const float *, float *, int, int, const float *, const float *,
+                                    const float *, const float *, const float *, const float *, const float *, float *,
+                                    float *);

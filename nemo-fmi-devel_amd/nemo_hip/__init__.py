@@ -112,7 +112,7 @@ class NemoHip:
         self._check(self.L.nemo_hip_timing_get(name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
-    # a2 / a19 ----------------------------------------------------------
+    # masks and lateral boundaries ----------------------------------------------------------
     def dom_msk(self, tmask=None, umask=None, vmask=None, fmask=None, wmask=None, wumask=None, wvmask=None,
                 ssmask=None, ssumask=None, ssvmask=None, mikt=None, mbkt=None, miku=None, mikv=None, mbku=None,
                 mbkv=None):
@@ -233,6 +233,33 @@ class NemoHip:
             setattr(s, n, self._p(a, dt).value)
         fn = self.L.nemo_hip_dyn_spg_ts_sp if sp else self.L.nemo_hip_dyn_spg_ts
         self._check(fn(C.c_int(kt), C.byref(s)))
+
+    # fused steps ----------------------------------------------------------
+    def step_dyn(self, kt, ub, vb, wn, avm, utau_b, vtau_b, fields, sp=False):
+        """ua = va = 0; dyn_adv; dyn_vor; dyn_spg_ts; dyn_zdf on resident fields.
+        fields: dict named as nemo_hip_spg_ts_t (3D: ua, va, un, vn)."""
+        s = abi.SpgTsStruct()
+        dt = np.float32 if sp else np.float64
+        for n, _ in abi.SpgTsStruct._fields_:
+            a = fields.get(n)
+            if a is None:
+                setattr(s, n, None)
+                continue
+            setattr(s, n, self._p(a, dt if n in abi.SPG_3D_FIELDS else np.float64).value)
+        p = lambda a: self._p(a, dt)
+        q = self._p
+        fn = self.L.nemo_hip_step_dyn_sp if sp else self.L.nemo_hip_step_dyn
+        self._check(fn(C.c_int(kt), p(ub), p(vb), p(wn), p(avm), q(utau_b), q(vtau_b), C.byref(s)))
+
+    def step_tra(self, kt, un, vn, wn, tsb, tsn, tsa, kn_fct_h, kn_fct_v, ahtu, ahtv, uslp, vslp, wslpi, wslpj,
+                 ah_wslp2, akz, avt, sp=False):
+        """tsa = 0; tra_adv (transport + FCT); tra_ldf_iso; tra_zdf on resident fields."""
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        fn = self.L.nemo_hip_step_tra_sp if sp else self.L.nemo_hip_step_tra
+        self._check(fn(C.c_int(kt), p(un), p(vn), p(wn), p(tsb), p(tsn), p(tsa), C.c_int(kn_fct_h),
+                       C.c_int(kn_fct_v), p(ahtu), p(ahtv), p(uslp), p(vslp), p(wslpi), p(wslpj), p(ah_wslp2),
+                       p(akz), p(avt)))
 
 
 _handle = None

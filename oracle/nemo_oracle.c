@@ -292,7 +292,7 @@ int ora_dom_msk(double *ptmask, double *pumask, double *pvmask, double *pfmask, 
 #define WVMASK(i, j, k) A3(wvmask, i, j, k)
 
 /* ------------------------------------------------------------------------ */
-/* a3: tra_adv effective transport, TRA/traadv.F90:94-124                  */
+/* tra_adv effective transport, TRA/traadv.F90:94-124                  */
 /* ------------------------------------------------------------------------ */
 int ora_tra_adv_trp(const double *un, const double *vn, const double *wn, double *zun, double *zvn, double *zwn)
 {
@@ -318,7 +318,7 @@ double ora_tra_adv_r2dt(int kt)
 }
 
 /* ------------------------------------------------------------------------ */
-/* a6: interp_4th_cpt, TRA/traadv_fct.F90:517-616                           */
+/* interp_4th_cpt, TRA/traadv_fct.F90:517-616                           */
 /* ------------------------------------------------------------------------ */
 int ora_interp_4th_cpt(const double *pt_in, double *pt_out)
 {
@@ -372,7 +372,7 @@ int ora_interp_4th_cpt(const double *pt_in, double *pt_out)
 }
 
 /* ------------------------------------------------------------------------ */
-/* a5: nonosc, TRA/traadv_fct.F90:330-428                                   */
+/* nonosc, TRA/traadv_fct.F90:330-428                                   */
 /* ------------------------------------------------------------------------ */
 static void nonosc(const double *pbef, double *paa, double *pbb, double *pcc, const double *paft, double p2dt)
 {
@@ -438,7 +438,7 @@ static void nonosc(const double *pbef, double *paa, double *pbb, double *pcc, co
 }
 
 /* ------------------------------------------------------------------------ */
-/* a4: tra_adv_fct, TRA/traadv_fct.F90:54-327                              */
+/* tra_adv_fct, TRA/traadv_fct.F90:54-327                              */
 /* ------------------------------------------------------------------------ */
 int ora_tra_adv_fct(int kt, int kit000, const char *cdtype, double p2dt, const double *pun, const double *pvn,
                     const double *pwn, const double *ptb, const double *ptn, double *pta, int kjpt, int kn_fct_h,
@@ -569,7 +569,7 @@ int ora_tra_adv_fct(int kt, int kit000, const char *cdtype, double p2dt, const d
 }
 
 /* ------------------------------------------------------------------------ */
-/* a8: tra_ldf_lap, TRA/traldf_lap_blp.F90:47-158 (ln_zps = F)              */
+/* tra_ldf_lap, TRA/traldf_lap_blp.F90:47-158 (ln_zps = F)              */
 /* ------------------------------------------------------------------------ */
 int ora_tra_ldf_lap(int kt, int kit000, const char *cdtype, const double *pahu, const double *pahv,
                     const double *pgu, const double *pgv, const double *pgui, const double *pgvi, const double *ptb,
@@ -606,7 +606,7 @@ int ora_tra_ldf_lap(int kt, int kit000, const char *cdtype, const double *pahu, 
 }
 
 /* ------------------------------------------------------------------------ */
-/* a9: tra_ldf_iso, TRA/traldf_iso.F90:50-382 (ln_zps = F)                  */
+/* tra_ldf_iso, TRA/traldf_iso.F90:50-382 (ln_zps = F)                  */
 /* ------------------------------------------------------------------------ */
 int ora_tra_ldf_iso(int kt, int kit000, const char *cdtype, const double *pahu, const double *pahv,
                     const double *pgu, const double *pgv, const double *pgui, const double *pgvi, const double *ptb,
@@ -776,7 +776,7 @@ int ora_tra_ldf_iso(int kt, int kit000, const char *cdtype, const double *pahu, 
 }
 
 /* ------------------------------------------------------------------------ */
-/* a11: tra_zdf_imp, TRA/trazdf.F90:109-268 (ln_zad_Aimp = F)               */
+/* tra_zdf_imp, TRA/trazdf.F90:109-268 (ln_zad_Aimp = F)               */
 /* ------------------------------------------------------------------------ */
 int ora_tra_zdf_imp(int kt, int kit000, const char *cdtype, double p2dt, const double *ptb, double *pta, int kjpt,
                     const double *avt, const double *avs, int l_ldfslp, const double *ah_wslp2, const double *akz)
@@ -837,7 +837,7 @@ int ora_tra_zdf_imp(int kt, int kit000, const char *cdtype, double p2dt, const d
     return 0;
 }
 
-/* a10: tra_zdf, TRA/trazdf.F90:46-106 */
+/* tra_zdf, TRA/trazdf.F90:46-106 */
 int ora_tra_zdf(int kt, const double *tsb, double *tsa, const double *avt, const double *avs, int l_ldfslp,
                 const double *ah_wslp2, const double *akz)
 {
@@ -852,7 +852,7 @@ int ora_tra_zdf(int kt, const double *tsb, double *tsa, const double *avt, const
 
 
 /* ------------------------------------------------------------------------ */
-/* a13: dyn_keg, DYN/dynkeg.F90:46-192                                      */
+/* dyn_keg, DYN/dynkeg.F90:46-192                                      */
 /* ------------------------------------------------------------------------ */
 int ora_dyn_keg(int kt, int kscheme, const double *un, const double *vn, double *ua, double *va)
 {
@@ -898,7 +898,7 @@ int ora_dyn_keg(int kt, int kscheme, const double *un, const double *vn, double 
 }
 
 /* ------------------------------------------------------------------------ */
-/* a14: dyn_zad, DYN/dynzad.F90:38-121                                      */
+/* dyn_zad, DYN/dynzad.F90:38-121                                      */
 /* ------------------------------------------------------------------------ */
 int ora_dyn_zad(int kt, const double *un, const double *vn, const double *wn, double *ua, double *va)
 {
@@ -930,7 +930,7 @@ int ora_dyn_zad(int kt, const double *un, const double *vn, const double *wn, do
 }
 
 /* ------------------------------------------------------------------------ */
-/* a15: dyn_adv_cen2, DYN/dynadv_cen2.F90:37-144                            */
+/* dyn_adv_cen2, DYN/dynadv_cen2.F90:37-144                            */
 /* ------------------------------------------------------------------------ */
 int ora_dyn_adv_cen2(int kt, const double *un, const double *vn, const double *wn, double *ua, double *va)
 {
@@ -993,7 +993,7 @@ int ora_dyn_adv_cen2(int kt, const double *un, const double *vn, const double *w
     return 0;
 }
 
-/* a12: dyn_adv dispatch, DYN/dynadv.F90:73-81 */
+/* dyn_adv dispatch, DYN/dynadv.F90:73-81 */
 int ora_dyn_adv(int kt, const double *un, const double *vn, const double *wn, double *ua, double *va)
 {
     switch (D.n_dynadv) {
@@ -1008,7 +1008,7 @@ int ora_dyn_adv(int kt, const double *un, const double *vn, const double *wn, do
 }
 
 /* ------------------------------------------------------------------------ */
-/* a16: dyn_vor, DYN/dynvor.F90:98-692                                      */
+/* dyn_vor, DYN/dynvor.F90:98-692                                      */
 /* ------------------------------------------------------------------------ */
 enum { np_COR = 1, np_RVO = 2, np_MET = 3, np_CRV = 4, np_CME = 5 };
 static const double r1_4 = 0.250, r1_8 = 0.125;
@@ -1263,7 +1263,7 @@ int ora_dyn_vor(int kt, const double *un, const double *vn, double *ua, double *
 }
 
 /* ------------------------------------------------------------------------ */
-/* a17: dyn_zdf, DYN/dynzdf.F90:47-499 (ln_zad_Aimp = F, iso-level         */
+/* dyn_zdf, DYN/dynzdf.F90:47-499 (ln_zad_Aimp = F, iso-level         */
 /* momentum mixing, r_vvl = 0 under ln_linssh)                               */
 /* ------------------------------------------------------------------------ */
 int ora_dyn_zdf(int kt, const double *ub, const double *vb, double *ua, double *va, const double *avm,
@@ -1364,7 +1364,7 @@ int ora_dyn_zdf(int kt, const double *ub, const double *vb, double *ua, double *
 }
 
 /* ------------------------------------------------------------------------ */
-/* a18: dyn_spg_ts, DYN/dynspg_ts.F90:121-1388                              */
+/* dyn_spg_ts, DYN/dynspg_ts.F90:121-1388                              */
 /* ------------------------------------------------------------------------ */
 int ora_ts_wgt(int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2)
 {

@@ -31,7 +31,7 @@ def uniform_domain(jpi, jpj, jpk, e1=1000.0, depth=100.0, jperio=0, k_bot=None, 
 
 
 # ----------------------------------------------------------------------------
-# a2 masks / k-indices, a19 lbc_lnk
+# masks, k-indices and lbc_lnk
 # ----------------------------------------------------------------------------
 def test_masks_known_answer():
     # 6 x 5 x 4 closed basin; interior columns with chosen bottoms
@@ -116,7 +116,7 @@ def test_lbc_lnk(jperio, nat):
 
 
 # ----------------------------------------------------------------------------
-# a11 / a10: implicit vertical diffusion against a dense solve
+# implicit vertical diffusion against a dense solve
 # ----------------------------------------------------------------------------
 def dense_tracer_column(e3t, e3w, kz, tb, ta_trend, p2dt, nlev):
     """Backward-Euler diffusion with no-flux top and bottom, written as the
@@ -201,7 +201,7 @@ def test_tra_zdf_salinity_reset():
 
 
 # ----------------------------------------------------------------------------
-# a17: dyn_zdf against a dense solve
+# dyn_zdf against a dense solve
 # ----------------------------------------------------------------------------
 def test_dyn_zdf_dense_solve():
     d = configs.basin(22, 18, 14, seed=8)
@@ -262,7 +262,7 @@ def test_dyn_zdf_conserves_momentum_without_stress():
 
 
 # ----------------------------------------------------------------------------
-# a6: compact 4th-order interpolation
+# compact 4th-order interpolation
 # ----------------------------------------------------------------------------
 def test_interp_4th_cpt_dense_and_exactness():
     jpk = 12
@@ -322,7 +322,7 @@ def test_interp_4th_cpt_convergence_order():
 
 
 # ----------------------------------------------------------------------------
-# a4 / a5: FCT advection
+# FCT advection
 # ----------------------------------------------------------------------------
 def zalesak_1d(tb, tn, u, dx, dt, order_h=2):
     """Textbook Zalesak (1979) FCT on a periodic 1-D grid with unit cross
@@ -463,7 +463,7 @@ def test_fct4_equals_fct2_on_linear_field():
 
 
 # ----------------------------------------------------------------------------
-# a8 / a9: lateral diffusion
+# lateral diffusion
 # ----------------------------------------------------------------------------
 def test_ldf_lap_closed_forms():
     jpi, jpj, jpk = 20, 18, 4
@@ -532,7 +532,7 @@ def test_ldf_iso_slope_coefficient_closed_form():
 
 
 # ----------------------------------------------------------------------------
-# a13 / a14 / a15 / a16: momentum
+# momentum
 # ----------------------------------------------------------------------------
 def test_keg_zad_closed_forms():
     jpi, jpj, jpk = 16, 14, 6
@@ -616,7 +616,7 @@ def test_cen2_conserves_momentum():
 
 
 # ----------------------------------------------------------------------------
-# a18: split-explicit barotropic solver
+# split-explicit barotropic solver
 # ----------------------------------------------------------------------------
 def test_ts_wgt_closed_form():
     d = configs.gyre_like()
