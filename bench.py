@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", default="1024x1024x75")
-    ap.add_argument("--forms", default="fp64,fp32")
+    ap.add_argument("--forms", default="fp64,fp32,fp64_fused,fp32_fused")
     ap.add_argument("--cpu-sample", default="512x512x75")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -75,11 +75,13 @@ def build_case(jpi, jpj, jpk, seed):
 
 
 class Stepper:
-    """The hot path of one time step, in one storage form, on resident arrays."""
+    """The hot path of one time step, in one storage form, on resident arrays.
+    fused=False: the separate drop-in routines in step order; fused=True: the
+    two fused step entry points (same results, bit for bit)."""
 
-    def __init__(self, h, d, S, sp):
+    def __init__(self, h, d, S, sp, fused=False):
         import torch
-        self.h, self.d, self.sp = h, d, sp
+        self.h, self.d, self.sp, self.fused = h, d, sp, fused
         dt = torch.float32 if sp else torch.float64
         n3 = d.jpi * d.jpj * d.jpk
         c = lambda k: S[k].to(dt).contiguous()
@@ -87,16 +89,17 @@ class Stepper:
                                      "wslpi", "wslpj")}
         self.f3["avs"] = self.f3["avt"]
         self.tsb, self.tsn = c("tsb"), c("tsn")
-        z = lambda n: __import__("torch").zeros(n, dtype=dt, device="cuda")
+        z = lambda n: torch.zeros(n, dtype=dt, device="cuda")
         self.tsa, self.ua, self.va = z(2 * n3), z(n3), z(n3)
-        self.zun, self.zvn, self.zwn = z(n3), z(n3), z(n3)
+        self.zun, self.zvn, self.zwn = (None, None, None) if fused else (z(n3), z(n3), z(n3))
         self.ah, self.akz = z(n3), z(n3)
         self.f2 = {k: S[k].clone() for k in ("un_b", "vn_b", "ub_b", "vb_b", "sshn", "hu_n", "hv_n", "r1_hu_n",
                                              "r1_hv_n", "ht_n", "rCdU_bot", "utau", "vtau", "emp", "rnf", "ub2_b",
                                              "vb2_b", "un_bf", "vn_bf", "utau_b", "vtau_b")}
         n2 = d.jpi * d.jpj
         for k in ("ua_b", "va_b", "ssha", "un_adv", "vn_adv"):
-            self.f2[k] = __import__("torch").zeros(n2, dtype=__import__("torch").float64, device="cuda")
+            self.f2[k] = torch.zeros(n2, dtype=torch.float64, device="cuda")
+        # the momentum fields dyn_spg_ts updates; the tracer transport reads the original un/vn
         self.un_dyn, self.vn_dyn = self.f3["un"].clone(), self.f3["vn"].clone()
         self.spg = dict(self.f2)
         self.spg.update(ua=self.ua, va=self.va, un=self.un_dyn, vn=self.vn_dyn)
@@ -104,12 +107,17 @@ class Stepper:
 
     def step(self, kt):
         h, f, sp = self.h, self.f3, self.sp
+        if self.fused:
+            h.step_dyn(kt, f["ub"], f["vb"], f["wn"], f["avm"], self.f2["utau_b"], self.f2["vtau_b"], self.spg, sp=sp)
+            h.step_tra(kt, f["un"], f["vn"], f["wn"], self.tsb, self.tsn, self.tsa, 4, 4, f["ahtu"], f["ahtv"],
+                       f["uslp"], f["vslp"], f["wslpi"], f["wslpj"], self.ah, self.akz, f["avt"], sp=sp)
+            return
         # step.F90:179-180 and 234: trends start at zero
         self.ua.zero_()
         self.va.zero_()
         self.tsa.zero_()
-        h.dyn_adv(kt, f["un"], f["vn"], f["wn"], self.ua, self.va, sp=sp)
-        h.dyn_vor(kt, f["un"], f["vn"], self.ua, self.va, sp=sp)
+        h.dyn_adv(kt, self.un_dyn, self.vn_dyn, f["wn"], self.ua, self.va, sp=sp)
+        h.dyn_vor(kt, self.un_dyn, self.vn_dyn, self.ua, self.va, sp=sp)
         h.dyn_spg_ts(kt, self.spg, sp=sp)
         h.dyn_zdf(kt, f["ub"], f["vb"], self.ua, self.va, f["avm"], self.f2["rCdU_bot"], self.f2["ua_b"],
                   self.f2["va_b"], self.f2["utau"], self.f2["utau_b"], self.f2["vtau"], self.f2["vtau_b"], sp=sp)
@@ -121,9 +129,9 @@ class Stepper:
         h.tra_zdf(kt, self.tsb, self.tsa, f["avt"], f["avs"], 1, self.ah, self.akz, sp=sp)
 
 
-def time_form(h, d, S, sp, steps, warmup, timing_steps):
+def time_form(h, d, S, sp, fused, steps, warmup, timing_steps):
     import torch
-    st = Stepper(h, d, S, sp)
+    st = Stepper(h, d, S, sp, fused)
     ext = torch.cuda.ExternalStream(h.stream())
     with torch.cuda.stream(ext):
         kt = 2
@@ -159,15 +167,17 @@ def kernel_times(h):
     import ctypes as C
     out = {}
     names = ["dyn_adv", "dyn_vor", "dyn_spg_ts", "dyn_zdf", "tra_adv_trp", "tra_adv_fct", "tra_ldf_iso", "tra_zdf",
-             "lbc_lnk"]
+             "lbc_lnk", "step_dyn", "step_tra"]
     kern = {
         "dyn_adv": ["k_dyn_keg_zad"], "dyn_vor": ["k_dyn_vor"], "dyn_zdf": ["k_dyn_zdf"],
         "dyn_spg_ts": ["k_spg_cor1", "k_spg_cor2", "k_spg_p1a", "k_spg_p1b", "k_spg_s1", "k_spg_s2", "k_spg_s3",
                        "k_spg_p3a", "k_spg_p3b"],
         "tra_adv_trp": ["k_tra_adv_trp"],
-        "tra_adv_fct": ["k_fct_low", "k_fct_ad", "k_fct_beta", "k_fct_final"],
+        "tra_adv_fct": ["k_fct_pre", "k_fct_beta", "k_fct_final", "k_fct_low", "k_fct_ad"],
         "tra_ldf_iso": ["k_ldf_iso_coef", "k_tra_ldf_iso"], "tra_zdf": ["k_tra_zdf", "k_clamp_ring"],
         "lbc_lnk": ["k_lbc"],
+        "step_dyn": ["k_step_dyn1", "k_step_dyn2"],
+        "step_tra": ["k_fct_pre", "k_fct_beta", "k_fct_final", "k_zero_ring", "k_tra_zdf"],
     }
     for n in names:
         out[n] = h.timing_get(n)
@@ -241,8 +251,9 @@ def main():
     log(f"state generated on the device in {time.perf_counter() - t0:.1f} s")
     res = {}
     for form in a.forms.split(","):
-        sp = form == "fp32"
-        ms, finite = time_form(h, d, S, sp, a.steps, a.warmup, a.timing_steps)
+        sp = form.startswith("fp32")
+        fused = form.endswith("_fused")
+        ms, finite = time_form(h, d, S, sp, fused, a.steps, a.warmup, a.timing_steps)
         kt = kernel_times(h)
         res[form] = {"ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3), "finite": finite,
                      "routines_ms_per_step": {k: v[0] / a.timing_steps for k, v in kt.items()
@@ -259,8 +270,8 @@ def main():
         for k, v in sorted(res[form]["routines_ms_per_step"].items(), key=lambda x: -x[1]):
             log(f"   {k:14s} {v:8.3f} ms")
     # roofline of the dominant single kernel of the headline form
-    head = "fp32" if "fp32" in res else list(res)[0]
-    width = 4 if head == "fp32" else 8
+    head = next((f for f in ("fp32_fused", "fp32") if f in res), list(res)[0])
+    width = 4 if head.startswith("fp32") else 8
     kms = res[head]["kernels_ms_per_step"]
     klaunch = res[head]["kernel_launches_per_step"]
     roof = None
@@ -286,7 +297,7 @@ def main():
         "unit": "grid-cell updates/s",
         "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": h_ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32 storage (f64 tridiagonal + barotropic)" if head == "fp32" else "f64",
+        "dtype": "f32 storage (f64 tridiagonal + barotropic)" if head.startswith("fp32") else "f64",
         "data": "synthetic: seeded land mask, lognormal bathymetry, streamfunction velocities",
         "config": {"workload": f"NEMO hot-path step (dyn_adv+dyn_vor+dyn_spg_ts+dyn_zdf+tra_adv_fct+tra_ldf_iso"
                                f"+tra_zdf), {jpi}x{jpj}x{jpk}, FCT 4/4, nn_baro 30, rdt 600 s",
@@ -296,8 +307,10 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    if "fp32" in res and "fp64" in res:
-        line["fp32_speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / res["fp32"]["ms_per_step"]
+    if "fp64" in res:
+        for f in ("fp32", "fp32_fused", "fp64_fused"):
+            if f in res:
+                line[f"{f}_speedup_vs_fp64"] = res["fp64"]["ms_per_step"] / res[f]["ms_per_step"]
     print(json.dumps(line))
 
 
@@ -320,6 +333,13 @@ KERNEL_BYTES = {
     "k_spg_p1a": _b(6),          # r ua va un vn (bottom), w ua va
     "k_spg_p3b": _b(8),          # r ua va un vn, w ua va un vn
     "k_tra_adv_trp": lambda w, i, j, k: 6 * w * i * j * (k - 1),
+    # FCT column pre-pass (two tracers per launch): the routine's own fields;
+    # its temporaries (zwi, zwx, zwy, zwz), as in the per-routine accounting,
+    # are not counted
+    "k_fct_pre": _b(11),         # r pun pvn pwn tb*2 tn*2, r+w ta*2 (w only in the fused step)
+    # fused momentum step
+    "k_step_dyn1": _b(5),        # r un vn wn, w ua va
+    "k_step_dyn2": _b(11),       # r ua va un vn ub vb avm, w ua va un vn
 }
 
 if __name__ == "__main__":

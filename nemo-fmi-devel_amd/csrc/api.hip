@@ -28,7 +28,7 @@ static DevDomain g_dom;
 static nemo_hip_domain_t g_hdom;
 static std::vector<void *> g_dom_allocs;
 static void *g_scratch3[S_NSCRATCH3] = {};
-static double *g_scratch2[32] = {};
+static double *g_scratch2[NH_NSCRATCH2] = {};
 static double g_r2dt_tra, g_r2dt_zdf, g_r2dt_dyn;   // module r2dt of tra_adv / tra_zdf / dyn_zdf
 // staging pool (host-pointer mode)
 static std::vector<std::pair<void *, size_t>> g_pool;
@@ -266,6 +266,11 @@ static int upload_f(const double *src, size_t n, const float **dst)
 static int check_options(const nemo_hip_domain_t *h)
 {
     if (h->jpi < 4 || h->jpj < 4 || h->jpk < 3) return set_error(NEMO_HIP_ERR_ARG, "domain too small: %d x %d x %d", h->jpi, h->jpj, h->jpk);
+    // mask intervals are packed as bytes
+    if (h->jpk > 255) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "jpk = %d > 255", h->jpk);
+    // kernels address one 3D array (one tracer of a 4D array) with 32-bit byte offsets
+    if ((uint64_t)h->jpi * (uint64_t)h->jpj * (uint64_t)h->jpk * sizeof(double) >= (1ull << 32))
+        return set_error(NEMO_HIP_ERR_UNSUPPORTED, "domain too large: one 3D FP64 array must stay below 4 GiB");
     if (!(h->jperio == 0 || h->jperio == 1 || h->jperio == 2 || h->jperio == 7))
         return set_error(NEMO_HIP_ERR_UNSUPPORTED, "jperio = %d (north folds are not implemented)", h->jperio);
     if (!h->ln_linssh) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "ln_linssh = F (variable volume) is not implemented");
@@ -340,6 +345,15 @@ static int register_domain(const nemo_hip_domain_t *h)
         return r ? r : upload(b.data(), n2, &kr.k1);
     };
     if ((rc = up_ival(T, d.kT)) || (rc = up_ival(U, d.kU)) || (rc = up_ival(V, d.kV)) || (rc = up_ival(F, d.kF))) return rc;
+    {
+        std::vector<uint32_t> tu(n2), vf(n2);
+        auto b = [](int x) { return (uint32_t)(x & 255); };
+        for (size_t q = 0; q < n2; q++) {
+            tu[q] = b(T[q].k0) | b(T[q].k1) << 8 | b(U[q].k0) << 16 | b(U[q].k1) << 24;
+            vf[q] = b(V[q].k0) | b(V[q].k1) << 8 | b(F[q].k0) << 16 | b(F[q].k1) << 24;
+        }
+        if ((rc = upload(tu.data(), n2, &d.ktu)) || (rc = upload(vf.data(), n2, &d.kvf))) return rc;
+    }
     {
         std::vector<double> s(n2), su(n2), sv(n2);
         for (size_t q = 0; q < n2; q++) {
@@ -443,13 +457,17 @@ static int register_domain(const nemo_hip_domain_t *h)
         for (int a = 0; a < 7; a++)
             if ((rc = upload(e3s[a], n3, e3d[a])) || (rc = upload_f(e3s[a], n3, e3f[a]))) return rc;
     }
-    // scratch
-    for (int s = 0; s < S_NSCRATCH3; s++)
-        if ((rc = dev_alloc(&g_scratch3[s], n3 * sizeof(double)))) return rc;
-    for (int s = 0; s < 32; s++)
+    // scratch: one allocation, slots of n3 doubles back to back, so a pair of
+    // consecutive slots holds a two-tracer array of either storage type
+    {
+        void *base;
+        if ((rc = dev_alloc(&base, (size_t)S_NSCRATCH3 * n3 * sizeof(double)))) return rc;
+        for (int s = 0; s < S_NSCRATCH3; s++) g_scratch3[s] = (double *)base + (size_t)s * n3;
+        HIPCHK(hipMemset(base, 0, (size_t)S_NSCRATCH3 * n3 * sizeof(double)));
+    }
+    for (int s = 0; s < NH_NSCRATCH2; s++)
         if ((rc = dev_alloc((void **)&g_scratch2[s], n2 * sizeof(double)))) return rc;
-    for (int s = 0; s < S_NSCRATCH3; s++) HIPCHK(hipMemset(g_scratch3[s], 0, n3 * sizeof(double)));
-    for (int s = 0; s < 32; s++) HIPCHK(hipMemset(g_scratch2[s], 0, n2 * sizeof(double)));
+    for (int s = 0; s < NH_NSCRATCH2; s++) HIPCHK(hipMemset(g_scratch2[s], 0, n2 * sizeof(double)));
     HIPCHK(hipDeviceSynchronize());
     g_dom = d;
     g_have_domain = true;

@@ -58,6 +58,9 @@ struct DevDomain {
     double rdt, rn_atfp, rau0, grav, rn_scal_load;
     // masks as level intervals, DOM/dommsk.F90
     KRange kT, kU, kV, kF;
+    // the same intervals packed as bytes, one 32-bit load per column for two
+    // point types: ktu = T0 | T1 << 8 | U0 << 16 | U1 << 24, kvf = V0 | V1 << 8 | F0 << 16 | F1 << 24
+    const uint32_t *ktu, *kvf;
     const double *ssmask, *ssumask, *ssvmask;
     const float *ssmask_f, *ssumask_f, *ssvmask_f;
     // k-indices, DOM/domzgr.F90:292-314
@@ -130,12 +133,27 @@ __device__ __forceinline__ T wkmask(const KRange &r, size_t ij, int k)
     return in ? T(1) : T(0);
 }
 
-#define TMASK(T_, i, j, k) kmask<T_>(d.kT, IJ(i, j), k)
-#define UMASK(T_, i, j, k) kmask<T_>(d.kU, IJ(i, j), k)
-#define VMASK(T_, i, j, k) kmask<T_>(d.kV, IJ(i, j), k)
-#define WMASK(T_, i, j, k) wkmask<T_>(d.kT, IJ(i, j), k)
-#define WUMASK(T_, i, j, k) wkmask<T_>(d.kU, IJ(i, j), k)
-#define WVMASK(T_, i, j, k) wkmask<T_>(d.kV, IJ(i, j), k)
+template <typename T, int SH>
+__device__ __forceinline__ T pmask(const uint32_t *p, uint32_t ij, int k)
+{
+    const uint32_t w = p[ij] >> SH;
+    const int a = (int)(w & 255u), b = (int)((w >> 8) & 255u);
+    return (k >= a && k <= b) ? T(1) : T(0);
+}
+template <typename T, int SH>
+__device__ __forceinline__ T wpmask(const uint32_t *p, uint32_t ij, int k)
+{
+    const uint32_t w = p[ij] >> SH;
+    const int a = (int)(w & 255u), b = (int)((w >> 8) & 255u);
+    return (k >= a && k <= b && (k == 1 || k - 1 >= a)) ? T(1) : T(0);
+}
+
+#define TMASK(T_, i, j, k) pmask<T_, 0>(d.ktu, IJ32(i, j), k)
+#define UMASK(T_, i, j, k) pmask<T_, 16>(d.ktu, IJ32(i, j), k)
+#define VMASK(T_, i, j, k) pmask<T_, 0>(d.kvf, IJ32(i, j), k)
+#define WMASK(T_, i, j, k) wpmask<T_, 0>(d.ktu, IJ32(i, j), k)
+#define WUMASK(T_, i, j, k) wpmask<T_, 16>(d.ktu, IJ32(i, j), k)
+#define WVMASK(T_, i, j, k) wpmask<T_, 0>(d.kvf, IJ32(i, j), k)
 
 // NEMO's SIGN with key_nosignedzero semantics, lib_fortran.F90:336-351
 template <typename T>
@@ -210,10 +228,11 @@ void timing_begin(const char *name);
 void timing_end(const char *name, const char *kernel);
 // library-owned scratch buffers (HBM), sized at domain registration
 enum Scratch {
-    S_ZWI, S_ZWX, S_ZWY, S_ZWZ, S_ZBETUP, S_ZBETDO, S_ZTW, S_ZWT, S_ZWT2, S_NSCRATCH3
+    S_ZWI, S_ZWX, S_ZWY, S_ZWZ, S_ZBETUP, S_ZBETDO, S_ZTW, S_ZWT, S_ZWT2, S_ZUN, S_ZVN, S_ZWN, S_NSCRATCH3
 };
-void *scratch3(int which);     // n3 doubles each
-double *scratch2(int which);   // n2 doubles each (dyn_spg_ts work arrays)
+void *scratch3(int which);     // n3 doubles each, slot s+1 directly after slot s
+#define NH_NSCRATCH2 40
+double *scratch2(int which);   // n2 doubles each (dyn_spg_ts work arrays), NH_NSCRATCH2 of them
 }  // namespace nh
 
 #define HIPCHK(x)                                                                                  \

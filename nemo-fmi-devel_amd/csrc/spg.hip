@@ -24,6 +24,16 @@ struct SpgWork {
     double *zu_frc, *zv_frc, *zwxb, *zwyb, *zCdU_u, *zCdU_v, *zssh_frc, *zwz, *zhf, *ftne, *ftnw, *ftse, *ftsw;
     double *ua_e, *va_e, *zwx, *zwy, *zsshp2_e;
 };
+// the next sub-step's extrapolation, folded into the velocity kernel: its
+// coefficients and weight, the velocity level two steps back, and the
+// second set of ua_e/va_e/zwx/zwy buffers it writes (the current set is still
+// read by neighbouring points of this sub-step)
+struct SpgNext {
+    int on;
+    double za1, za2, za3, wgt2;
+    const double *ub_before, *vb_before;
+    double *ua_e, *va_e, *zwx, *zwy, *un_adv, *vn_adv;
+};
 }
 
 // barotropic Coriolis trend at u/v points, phase 1 (lines 390-443) and
@@ -357,9 +367,9 @@ __device__ __forceinline__ void spg_new_vel(const DevDomain &d, int i, int j, do
 }
 
 KERNEL void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double * __restrict__ un_e, const double * __restrict__ vn_e,
-                         double * __restrict__ ubb_to_a, double * __restrict__ vbb_to_a, const double * __restrict__ r1_hu_n, const double * __restrict__ r1_hv_n,
-                         const double * __restrict__ hu_n, const double * __restrict__ hv_n, const double * __restrict__ ht_n, double * __restrict__ ua_b, double * __restrict__ va_b,
-                         SpgWork w)
+                          double * __restrict__ ubb_to_a, double * __restrict__ vbb_to_a, const double * __restrict__ r1_hu_n, const double * __restrict__ r1_hv_n,
+                          const double * __restrict__ hu_n, const double * __restrict__ hv_n, const double * __restrict__ ht_n, double * __restrict__ ua_b, double * __restrict__ va_b,
+                          SpgWork w, SpgNext nx)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi || j > d.jpj) return;
@@ -368,10 +378,25 @@ KERNEL void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double * __re
     double ua = 0.0, va = 0.0, t;
     if (lbc_src(d, 'U', i, j, si, sj)) spg_new_vel(d, si, sj, rdtbt, un_e, vn_e, r1_hu_n, r1_hv_n, hu_n, hv_n, ht_n, w, ua, t);
     if (lbc_src(d, 'V', i, j, si, sj)) spg_new_vel(d, si, sj, rdtbt, un_e, vn_e, r1_hu_n, r1_hv_n, hu_n, hv_n, ht_n, w, t, va);
+    const double ub_now = un_e[ij], vb_now = vn_e[ij];
     ubb_to_a[ij] = ua;
     vbb_to_a[ij] = va;
     ua_b[ij] = ua_b[ij] + wgt1 * ua;
     va_b[ij] = va_b[ij] + wgt1 * va;
+    if (!nx.on) return;
+    // the next sub-step's extrapolation, fluxes and un_adv sum (pointwise in
+    // the levels now, before and after of this point), as k_spg_s1 computes them
+    const Metrics<double> &m = d.md;
+    const double ue = nx.za1 * ua + nx.za2 * ub_now + nx.za3 * nx.ub_before[ij];
+    const double ve = nx.za1 * va + nx.za2 * vb_now + nx.za3 * nx.vb_before[ij];
+    nx.ua_e[ij] = ue;
+    nx.va_e[ij] = ve;
+    const double zwx = m.e2u[ij] * ue * hu_n[ij];
+    const double zwy = m.e1v[ij] * ve * hv_n[ij];
+    nx.zwx[ij] = zwx;
+    nx.zwy[ij] = zwy;
+    nx.un_adv[ij] = nx.un_adv[ij] + nx.wgt2 * zwx * m.r1_e2u[ij];
+    nx.vn_adv[ij] = nx.vn_adv[ij] + nx.wgt2 * zwy * m.r1_e1v[ij];
 }
 
 // ---------------------------------------------------------------------------
@@ -526,19 +551,43 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
     HIPCHK(hipMemsetAsync(st->un_adv, 0, b2, s));
     HIPCHK(hipMemsetAsync(st->vn_adv, 0, b2, s));
 
-    for (int jn = 1; jn <= icycle; jn++) {
-        double za0, za1, za2, za3;
+    // two sets of the sub-step fluxes: sub-step jn reads set (jn-1)&1 and its
+    // velocity kernel writes the extrapolation of sub-step jn+1 into the other
+    double *ua_e2[2] = {w.ua_e, nh::scratch2(29)}, *va_e2[2] = {w.va_e, nh::scratch2(30)};
+    double *zwx2[2] = {w.zwx, nh::scratch2(31)}, *zwy2[2] = {w.zwy, nh::scratch2(32)};
+    auto ab3 = [&](int jn, double &za1, double &za2, double &za3) {
         if (jn < 3 && ll_init) { za1 = 1.0; za2 = 0.0; za3 = 0.0; }
         else { za1 = 1.781105; za2 = -1.06221; za3 = 0.281105; }
-        LAUNCH("dyn_spg_ts", k_spg_s1, g, d, za1, za2, za3, wgt2[jn - 1], ube[ru[0]], ube[ru[1]], ube[ru[2]],
+    };
+    {
+        double za1, za2, za3;
+        ab3(1, za1, za2, za3);
+        LAUNCH("dyn_spg_ts", k_spg_s1, g, d, za1, za2, za3, wgt2[0], ube[ru[0]], ube[ru[1]], ube[ru[2]],
                vbe[ru[0]], vbe[ru[1]], vbe[ru[2]], st->hu_n, st->hv_n, st->un_adv, st->vn_adv, w);
+    }
+    for (int jn = 1; jn <= icycle; jn++) {
+        const int cur = (jn - 1) & 1, nxt = jn & 1;
+        SpgWork wc = w;
+        wc.ua_e = ua_e2[cur]; wc.va_e = va_e2[cur]; wc.zwx = zwx2[cur]; wc.zwy = zwy2[cur];
+        double za0, za1, za2, za3;
         if (jn == 1 && ll_init) { za0 = 1.0; za1 = 0.0; za2 = 0.0; za3 = 0.0; }
         else if (jn == 2 && ll_init) { za0 = 1.0833333333333; za1 = -0.1666666666666; za2 = 0.0833333333333; za3 = 0.0; }
         else { za0 = 0.614; za1 = 0.285; za2 = 0.088; za3 = 0.013; }
         LAUNCH("dyn_spg_ts", k_spg_s2, g, d, rdtbt, za0, za1, za2, za3, wgt1[jn - 1], sse[rs[0]], sse[rs[1]],
-               sse[rs[2]], st->ssha, w);
+               sse[rs[2]], st->ssha, wc);
+        SpgNext nx{};
+        nx.on = jn < icycle;
+        if (nx.on) {
+            ab3(jn + 1, nx.za1, nx.za2, nx.za3);
+            nx.wgt2 = wgt2[jn];
+            // after the rotation: now <- the level written here, before <- now, before-before <- before
+            nx.ub_before = ube[ru[1]];
+            nx.vb_before = vbe[ru[1]];
+            nx.ua_e = ua_e2[nxt]; nx.va_e = va_e2[nxt]; nx.zwx = zwx2[nxt]; nx.zwy = zwy2[nxt];
+            nx.un_adv = st->un_adv; nx.vn_adv = st->vn_adv;
+        }
         LAUNCH("dyn_spg_ts", k_spg_s3, g, d, rdtbt, wgt1[jn - 1], ube[ru[0]], vbe[ru[0]], ube[ru[2]], vbe[ru[2]],
-               st->r1_hu_n, st->r1_hv_n, st->hu_n, st->hv_n, st->ht_n, st->ua_b, st->va_b, w);
+               st->r1_hu_n, st->r1_hv_n, st->hu_n, st->hv_n, st->ht_n, st->ua_b, st->va_b, wc, nx);
         // swap: bb <- b, b <- n, n <- a (the after level was written over bb)
         int t;
         t = ru[2]; ru[2] = ru[1]; ru[1] = ru[0]; ru[0] = t;

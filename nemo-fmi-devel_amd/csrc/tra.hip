@@ -22,6 +22,12 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
+// deepest column the data-independent pivot table of interp_4th_cpt covers
+#define STEP_MAXK 160
+template <int TRP, typename T>
+static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const T *pv, const T *pw, const T *ptb,
+                            const T *ptn, T *pta, int kjpt, int kn_fct_h, int kn_fct_v, int ta0);
+
 // ---------------------------------------------------------------------------
 // effective transport, TRA/traadv.F90:100-124
 // ---------------------------------------------------------------------------
@@ -340,6 +346,8 @@ int launch_tra_adv_fct(double p2dt, const T *pun, const T *pvn, const T *pwn, co
                        int kjpt, int kn_fct_h, int kn_fct_v)
 {
     const DevDomain &d = nh::dom();
+    if (d.jpk <= STEP_MAXK)
+        return launch_fct_pairs<0, T>(d, p2dt, pun, pvn, pwn, ptb, ptn, pta, kjpt, kn_fct_h, kn_fct_v, 0);
     T *zwi = (T *)nh::scratch3(nh::S_ZWI), *zwx = (T *)nh::scratch3(nh::S_ZWX), *zwy = (T *)nh::scratch3(nh::S_ZWY);
     T *zwz = (T *)nh::scratch3(nh::S_ZWZ), *zbu = (T *)nh::scratch3(nh::S_ZBETUP), *zbd = (T *)nh::scratch3(nh::S_ZBETDO);
     double *zwt = (double *)nh::scratch3(nh::S_ZTW);
@@ -893,7 +901,6 @@ template int launch_tra_zdf_imp<float>(double, int, const float *, float *, int,
 // data: rows 3..mbkt-1 follow p(k) = 4 - 1/p(k-1) from p(3) = 4, every other
 // row has pivot 1, so they come from a host table.
 // ===========================================================================
-#define STEP_MAXK 160
 template <typename T>
 struct PivotTab {
     T p[STEP_MAXK + 2];
@@ -985,19 +992,66 @@ __device__ __forceinline__ T cpt_piv(const PivotTab<T> &pt, int k, int ikb)
     return (k >= 3 && k < ikb) ? pt.p[k] : T(1);
 }
 
-// ---------------------------------------------------------------------------
-// pass 0 (one thread per interior column): the first guess zwi
-// (traadv_fct.F90:158-170) and the vertical antidiffusive flux zwz (lines
-// 255-278; for the compact scheme interp_4th_cpt's sweeps, lines 577-614,
-// with the forward values parked in zwz)
-// ---------------------------------------------------------------------------
-template <typename T, int NV4>
-KERNEL void k_step_tra0(DevDomain d, double p2dt, PivotTab<T> pt, const T *__restrict__ un,
-                        const T *__restrict__ vn, const T *__restrict__ wn, const T *__restrict__ ptb,
-                        const T *__restrict__ ptn, T *__restrict__ pzwi, T *__restrict__ pzwz)
+template <typename T>
+static PivotTab<T> cpt_pivots(int jpk);
+
+// transports: TRP = 0 reads the arrays of tra_adv (pun, pvn, pwn, zero at
+// jpk); TRP = 1 forms them from un, vn, wn where used (traadv.F90:100-124)
+template <int TRP, typename T>
+__device__ __forceinline__ T tr_u(const DevDomain &d, const T *pu, int p, int q, int k)
 {
-    THREAD_IJ(2, 2);
-    if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    if (TRP) return trp_u(d, pu, p, q, k);
+    return A3(pu, p, q, k);
+}
+template <int TRP, typename T>
+__device__ __forceinline__ T tr_v(const DevDomain &d, const T *pv, int p, int q, int k)
+{
+    if (TRP) return trp_v(d, pv, p, q, k);
+    return A3(pv, p, q, k);
+}
+template <int TRP, typename T>
+__device__ __forceinline__ T tr_w(const DevDomain &d, const T *pw, int p, int q, int k)
+{
+    if (TRP) return trp_w(d, pw, p, q, k);
+    return A3(pw, p, q, k);
+}
+// upstream fluxes on the transport provider
+template <int TRP, typename T>
+__device__ __forceinline__ T tp_upx(const DevDomain &d, const T *pu, T tb_i, T tb_ip1, int i, int j, int k)
+{
+    const T u = tr_u<TRP>(d, pu, i, j, k);
+    const T zfp = u + fabs(u), zfm = u - fabs(u);
+    return T(0.5) * (zfp * tb_i + zfm * tb_ip1);
+}
+template <int TRP, typename T>
+__device__ __forceinline__ T tp_upz(const DevDomain &d, const T *pw, const T *tb, int i, int j, int k)
+{
+    if (k == 1) return tr_w<TRP>(d, pw, i, j, 1) * A3(tb, i, j, 1);
+    if (k >= d.jpk) return T(0);
+    const T w = tr_w<TRP>(d, pw, i, j, k);
+    const T zfp = w + fabs(w), zfm = w - fabs(w);
+    return T(0.5) * (zfp * A3(tb, i, j, k) + zfm * A3(tb, i, j, k - 1)) * WMASK(T, i, j, k);
+}
+
+// ---------------------------------------------------------------------------
+// FCT for pairs of tracers.  One column pass (k_fct_pre) replaces the first
+// two loop groups of tra_adv_fct: per level it forms the horizontal
+// antidiffusive fluxes of the column's faces (traadv_fct.F90:180-221), the
+// low-order trend and first guess zwi (158-170) and the vertical
+// antidiffusive flux zwz (255-278), whose compact-scheme sweeps (577-614) take
+// their pivots from the data-independent table.  lbc_lnk, the Zalesak betas
+// (k_fct_beta) and the limited divergence (k_fct_final) follow as in the
+// reference.
+// ---------------------------------------------------------------------------
+template <int TRP, typename T, int NH4, int NV4, int TA0>
+KERNEL void k_fct_pre(DevDomain d, double p2dt, PivotTab<T> pt, const T *__restrict__ pu, const T *__restrict__ pv,
+                      const T *__restrict__ pw, const T *__restrict__ ptb, const T *__restrict__ ptn,
+                      T *__restrict__ pta, T *__restrict__ pzwi, T *__restrict__ pzwx, T *__restrict__ pzwy,
+                      T *__restrict__ pzwz, int kjpt)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi - 1 || j > d.jpj - 1) return;     // the flux loops: 1..jpim1 x 1..jpjm1
+    const bool inner = (i >= 2 && j >= 2);
     const Metrics<T> &m = met<T>(d);
     const size_t ij = IJ(i, j), n3 = d.n3;
     const int jpk = d.jpk;
@@ -1005,20 +1059,41 @@ KERNEL void k_step_tra0(DevDomain d, double p2dt, PivotTab<T> pt, const T *__res
     const T zp2dt = (T)p2dt;
     const int ikb = d.mbkt[ij];
 #pragma unroll 1
-    for (int t = 0; t < 2; t++) {
+    for (int t = 0; t < kjpt; t++) {
         const T *tb = ptb + t * n3, *tn = ptn + t * n3;
-        T *zwi = pzwi + t * n3, *zwz = pzwz + t * n3;
-        T zwz_k = fs_upz(d, wn, tb, i, j, 1), zout_prev = T(0);
+        T *ta = pta + t * n3, *zwi = pzwi + t * n3, *zwx = pzwx + t * n3, *zwy = pzwy + t * n3, *zwz = pzwz + t * n3;
+        T zwz_k = inner ? tp_upz<TRP>(d, pw, tb, i, j, 1) : T(0), zout_prev = T(0);
 #pragma unroll 1
         for (int k = 1; k <= jpk - 1; k++) {
+            const T tbc = A3(tb, i, j, k), tbe = A3(tb, i + 1, j, k), tbn = A3(tb, i, j + 1, k);
+            // horizontal fluxes of the (i+1/2, j) and (i, j+1/2) faces: upstream and antidiffusive
+            const T hu = tp_upx<TRP>(d, pu, tbc, tbe, i, j, k);
+            const T v = tr_v<TRP>(d, pv, i, j, k);
+            const T hv = T(0.5) * ((v + fabs(v)) * tbc + (v - fabs(v)) * tbn);
+            {
+                const T c2u = A3(tn, i, j, k) + A3(tn, i + 1, j, k);
+                const T c2v = A3(tn, i, j, k) + A3(tn, i, j + 1, k);
+                const T u = tr_u<TRP>(d, pu, i, j, k);
+                if (NH4) {
+                    A3(zwx, i, j, k) = T(0.5) * u * (c2u + fct_zltu(d, tn, i, j, k, true) - fct_zltu(d, tn, i + 1, j, k, true)) - hu;
+                    A3(zwy, i, j, k) = T(0.5) * v * (c2v + fct_zltu(d, tn, i, j, k, false) - fct_zltu(d, tn, i, j + 1, k, false)) - hv;
+                } else {
+                    A3(zwx, i, j, k) = T(0.5) * u * c2u - hu;
+                    A3(zwy, i, j, k) = T(0.5) * v * c2v - hv;
+                }
+            }
+            if (!inner) continue;
+            // low-order trend and first guess
             const T e3t = E3(m.e3t, ij, k);
             const T tm = TMASK(T, i, j, k);
-            const size_t p = IJK(i, j, k);
-            const T zwz_kp1 = fs_upz(d, wn, tb, i, j, k + 1);
-            const T ztra = -((fs_upx(d, un, tb, i, j, k) - fs_upx(d, un, tb, i - 1, j, k) + fs_upy(d, vn, tb, i, j, k) -
-                              fs_upy(d, vn, tb, i, j - 1, k) + zwz_k - zwz_kp1) *
-                             r1_e1e2t);
-            zwi[p] = (e3t * tb[p] + zp2dt * ztra) / e3t * tm;
+            const T hum = tp_upx<TRP>(d, pu, A3(tb, i - 1, j, k), tbc, i - 1, j, k);
+            const T vs = tr_v<TRP>(d, pv, i, j - 1, k);
+            const T hvm = T(0.5) * ((vs + fabs(vs)) * A3(tb, i, j - 1, k) + (vs - fabs(vs)) * tbc);
+            const T zwz_kp1 = tp_upz<TRP>(d, pw, tb, i, j, k + 1);
+            const T ztra = -((hu - hum + hv - hvm + zwz_k - zwz_kp1) * r1_e1e2t);
+            T a = TA0 ? T(0) : A3(ta, i, j, k);
+            A3(ta, i, j, k) = a + ztra / e3t * tm;
+            A3(zwi, i, j, k) = (e3t * tbc + zp2dt * ztra) / e3t * tm;
             zwz_k = zwz_kp1;
             if (k == 1) continue;
             if (NV4) {
@@ -1032,32 +1107,94 @@ KERNEL void k_step_tra0(DevDomain d, double p2dt, PivotTab<T> pt, const T *__res
                 }
                 if (k == 2) zout = zwrm;
                 else zout = zwrm - zwi_c / cpt_piv(pt, k - 1, ikb) * zout_prev;
-                zwz[p] = zout;
+                A3(zwz, i, j, k) = zout;
                 zout_prev = zout;
             } else {
-                zwz[p] = (trp_w(d, wn, i, j, k) * T(0.5) * (tn[p] + A3(tn, i, j, k - 1)) - fs_upz(d, wn, tb, i, j, k)) *
-                         WMASK(T, i, j, k);
+                A3(zwz, i, j, k) = (tr_w<TRP>(d, pw, i, j, k) * T(0.5) * (A3(tn, i, j, k) + A3(tn, i, j, k - 1)) -
+                                    tp_upz<TRP>(d, pw, tb, i, j, k)) *
+                                   WMASK(T, i, j, k);
             }
         }
-        zwi[IJK(i, j, jpk)] = T(0);
-        zwz[IJK(i, j, 1)] = T(0);
-        zwz[IJK(i, j, jpk)] = T(0);
+        if (!inner) continue;
+        A3(zwi, i, j, jpk) = T(0);
+        A3(zwz, i, j, 1) = T(0);
+        A3(zwz, i, j, jpk) = T(0);
         if (NV4) {
             T xnext = zout_prev / cpt_piv(pt, jpk - 1, ikb);
-            {
-                const size_t p = IJK(i, j, jpk - 1);
-                zwz[p] = (trp_w(d, wn, i, j, jpk - 1) * xnext - fs_upz(d, wn, tb, i, j, jpk - 1)) * WMASK(T, i, j, jpk - 1);
-            }
+            A3(zwz, i, j, jpk - 1) = (tr_w<TRP>(d, pw, i, j, jpk - 1) * xnext - tp_upz<TRP>(d, pw, tb, i, j, jpk - 1)) *
+                                     WMASK(T, i, j, jpk - 1);
 #pragma unroll 1
             for (int k = jpk - 2; k >= 2; k--) {
                 const T zws = (k == 2 || k == ikb) ? T(0) : WMASK(T, i, j, k);
-                const size_t p = IJK(i, j, k);
-                const T x = (zwz[p] - zws * xnext) / cpt_piv(pt, k, ikb);
-                zwz[p] = (trp_w(d, wn, i, j, k) * x - fs_upz(d, wn, tb, i, j, k)) * WMASK(T, i, j, k);
+                const T x = (A3(zwz, i, j, k) - zws * xnext) / cpt_piv(pt, k, ikb);
+                A3(zwz, i, j, k) = (tr_w<TRP>(d, pw, i, j, k) * x - tp_upz<TRP>(d, pw, tb, i, j, k)) * WMASK(T, i, j, k);
                 xnext = x;
             }
         }
     }
+}
+
+// ring and bottom level of a trend that starts from zero (fused step)
+template <typename T>
+KERNEL void k_zero_ring(DevDomain d, T *__restrict__ pta, int ntr)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const bool ring = !is_interior(d, i, j);
+    for (int t = 0; t < ntr; t++) {
+        T *ta = pta + t * d.n3;
+        if (ring)
+            for (int k = 1; k <= d.jpk; k++) A3(ta, i, j, k) = T(0);
+        else
+            A3(ta, i, j, d.jpk) = T(0);
+    }
+}
+
+template <int TRP, typename T>
+static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const T *pv, const T *pw, const T *ptb,
+                            const T *ptn, T *pta, int kjpt, int kn_fct_h, int kn_fct_v, int ta0)
+{
+    const PivotTab<T> pt = cpt_pivots<T>(d.jpk);
+    // scratch slot pairs (two tracers of either storage type each)
+    T *zwi = (T *)nh::scratch3(0), *zwx = (T *)nh::scratch3(2), *zwy = (T *)nh::scratch3(4);
+    T *zwz = (T *)nh::scratch3(6), *bu = (T *)nh::scratch3(8), *bd = (T *)nh::scratch3(10);
+    const char *nm = TRP ? "step_tra" : "tra_adv_fct";
+    Grid2 gf = grid2(d.jpi - 1, d.jpj - 1), gi = grid2(d.jpi - 2, d.jpj - 2);
+    const size_t n3 = d.n3;
+    int st;
+    for (int t0 = 0; t0 < kjpt; t0 += 2) {
+        const int nt = kjpt - t0 < 2 ? kjpt - t0 : 2;
+        const size_t o = (size_t)t0 * n3;
+#define FP_LAUNCH(H, V, Z) \
+    LAUNCH(nm, (k_fct_pre<TRP, T, H, V, Z>), gf, d, p2dt, pt, pu, pv, pw, ptb + o, ptn + o, pta + o, zwi, zwx, zwy, zwz, nt)
+        const int h4 = kn_fct_h == 4, v4 = kn_fct_v == 4;
+        if (ta0) {
+            if (h4 && v4) FP_LAUNCH(1, 1, 1); else if (h4) FP_LAUNCH(1, 0, 1); else if (v4) FP_LAUNCH(0, 1, 1); else FP_LAUNCH(0, 0, 1);
+        } else {
+            if (h4 && v4) FP_LAUNCH(1, 1, 0); else if (h4) FP_LAUNCH(1, 0, 0); else if (v4) FP_LAUNCH(0, 1, 0); else FP_LAUNCH(0, 0, 0);
+        }
+#undef FP_LAUNCH
+        const int nk = nt * d.jpk;
+        if ((st = launch_lbc<T>(zwi, nk, 'T', T(0)))) return st;
+        if ((st = launch_lbc<T>(zwx, nk, 'U', T(0)))) return st;
+        if ((st = launch_lbc<T>(zwy, nk, 'V', T(0)))) return st;
+        if ((st = launch_lbc<T>(zwz, nk, 'W', T(0)))) return st;
+        for (int t = 0; t < nt; t++) {
+            const size_t q = (size_t)t * n3;
+            LAUNCH(nm, (k_fct_beta<T>), gi, d, p2dt, ptb + o + q, zwi + q, zwx + q, zwy + q, zwz + q, bu + q, bd + q);
+        }
+        if ((st = launch_lbc<T>(bu, nk, 'T', T(0)))) return st;
+        if ((st = launch_lbc<T>(bd, nk, 'T', T(0)))) return st;
+        for (int t = 0; t < nt; t++) {
+            const size_t q = (size_t)t * n3;
+            LAUNCH(nm, (k_fct_final<T>), gi, d, zwx + q, zwy + q, zwz + q, bu + q, bd + q, pta + o + q);
+        }
+    }
+    if (ta0) {
+        Grid2 ga = grid2(d.jpi, d.jpj);
+        LAUNCH(nm, k_zero_ring<T>, ga, d, pta, kjpt);
+    }
+    return 0;
 }
 
 // pointwise forms of the rotated-laplacian intermediates (traldf_iso.F90:218-276)
@@ -1109,248 +1246,6 @@ __device__ __forceinline__ T iso_zftv(const DevDomain &d, const T *pahv, const T
            VMASK(T, p, q, k);
 }
 
-// ---------------------------------------------------------------------------
-// pass b (one thread per cell): the betas of nonosc (traadv_fct.F90:354-399),
-// ring cells taking the value lbc_lnk('T') gives them
-// ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ T fs_zwi(const DevDomain &d, const T *zwi, int p, int q, int k)
-{
-    int sp, sq;
-    if (!lbc_src(d, 'T', p, q, sp, sq)) return T(0);
-    return A3(zwi, sp, sq, k);
-}
-template <typename T>
-__device__ __forceinline__ void fs_zbnd(const DevDomain &d, const T *pbef, const T *zwi, int i, int j, int k,
-                                        double &up, double &dn)
-{
-    const double zbig = 1.e+40;
-    const double tm = TMASK(double, i, j, k);
-    const double a = (double)A3(pbef, i, j, k) * tm;
-    const double b = (double)fs_zwi(d, zwi, i, j, k) * tm;
-    const double au = a - zbig * (1.0 - tm), bu = b - zbig * (1.0 - tm);
-    const double ad = a + zbig * (1.0 - tm), bd = b + zbig * (1.0 - tm);
-    up = au > bu ? au : bu;
-    dn = ad < bd ? ad : bd;
-}
-
-template <typename T, int NH4>
-KERNEL void k_step_trab(DevDomain d, double p2dt, const T *__restrict__ un, const T *__restrict__ vn,
-                        const T *__restrict__ ptb, const T *__restrict__ ptn, const T *__restrict__ pzwi,
-                        const T *__restrict__ pzwz, T *__restrict__ pbu, T *__restrict__ pbd)
-{
-    THREAD_IJ(1, 1);
-    const int k = (int)blockIdx.z + 1;
-    if (i > d.jpi || j > d.jpj) return;
-    const size_t n3 = d.n3;
-    const size_t p = IJK(i, j, k);
-    int ci = i, cj = j;
-    bool zero = (k == d.jpk);
-    if (!zero && !is_interior(d, i, j)) {
-        int si, sj;
-        if (!lbc_src(d, 'T', i, j, si, sj) || !is_interior(d, si, sj)) zero = true;
-        ci = si;
-        cj = sj;
-    }
-    if (zero) {
-        pbu[p] = T(0); pbd[p] = T(0);
-        pbu[n3 + p] = T(0); pbd[n3 + p] = T(0);
-        return;
-    }
-    const size_t cij = IJ(ci, cj);
-    const double zbt = d.md.e1e2t[cij] * E3(d.md.e3t, cij, k) / p2dt;
-    const double zrtrn = 1.e-15;
-    const int ikm1 = k - 1 > 1 ? k - 1 : 1;
-#pragma unroll 1
-    for (int t = 0; t < 2; t++) {
-        const T *tb = ptb + t * n3, *tn = ptn + t * n3, *zwi = pzwi + t * n3, *zwz = pzwz + t * n3;
-        double zup, zdo, u2, d2;
-        fs_zbnd(d, tb, zwi, ci, cj, k, zup, zdo);
-        fs_zbnd(d, tb, zwi, ci - 1, cj, k, u2, d2); zup = fmax2(zup, u2); zdo = fmin2(zdo, d2);
-        fs_zbnd(d, tb, zwi, ci + 1, cj, k, u2, d2); zup = fmax2(zup, u2); zdo = fmin2(zdo, d2);
-        fs_zbnd(d, tb, zwi, ci, cj - 1, k, u2, d2); zup = fmax2(zup, u2); zdo = fmin2(zdo, d2);
-        fs_zbnd(d, tb, zwi, ci, cj + 1, k, u2, d2); zup = fmax2(zup, u2); zdo = fmin2(zdo, d2);
-        fs_zbnd(d, tb, zwi, ci, cj, ikm1, u2, d2); zup = fmax2(zup, u2); zdo = fmin2(zdo, d2);
-        fs_zbnd(d, tb, zwi, ci, cj, k + 1, u2, d2); zup = fmax2(zup, u2); zdo = fmin2(zdo, d2);
-        const double a_im1 = (double)fs_zwx_l<T, NH4>(d, un, tb, tn, ci - 1, cj, k);
-        const double a_i = (double)fs_zwx_l<T, NH4>(d, un, tb, tn, ci, cj, k);
-        const double b_jm1 = (double)fs_zwy_l<T, NH4>(d, vn, tb, tn, ci, cj - 1, k);
-        const double b_j = (double)fs_zwy_l<T, NH4>(d, vn, tb, tn, ci, cj, k);
-        const double c_kp1 = (double)A3(zwz, ci, cj, k + 1), c_k = (double)A3(zwz, ci, cj, k);
-        const double zpos = fmax2(0.0, a_im1) - fmin2(0.0, a_i) + fmax2(0.0, b_jm1) - fmin2(0.0, b_j) +
-                            fmax2(0.0, c_kp1) - fmin2(0.0, c_k);
-        const double zneg = fmax2(0.0, a_i) - fmin2(0.0, a_im1) + fmax2(0.0, b_j) - fmin2(0.0, b_jm1) +
-                            fmax2(0.0, c_k) - fmin2(0.0, c_kp1);
-        const double paft_k = (double)A3(zwi, ci, cj, k);
-        const double bup = (zup - paft_k) / (zpos + zrtrn) * zbt;
-        const double bdo = (paft_k - zdo) / (zneg + zrtrn) * zbt;
-        if (sizeof(T) == 4) {
-            pbu[t * n3 + p] = (T)fmin2(fmax2(bup, -1.0e30), 1.0e30);
-            pbd[t * n3 + p] = (T)fmin2(fmax2(bdo, -1.0e30), 1.0e30);
-        } else {
-            pbu[t * n3 + p] = (T)bup;
-            pbd[t * n3 + p] = (T)bdo;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// pass c (one thread per cell): low-order trend + limited antidiffusive
-// divergence (traadv_fct.F90:158-166, 288-297, 404-422) and the rotated
-// laplacian (traldf_iso.F90:146-366, ln_traldf_lap, kpass 1); writes the
-// trend and ah_wslp2 (akz with ln_traldf_msc) for tra_zdf_imp
-// ---------------------------------------------------------------------------
-template <typename T, int NH4>
-__device__ __forceinline__ double fs_lim_x(const DevDomain &d, const T *un, const T *tb, const T *tn, const T *bup,
-                                           const T *bdo, int p, int q, int k)
-{
-    int sp, sq;
-    if (!lbc_src(d, 'U', p, q, sp, sq)) return 0.0;
-    return lim_face<T>((double)fs_zwx<T, NH4>(d, un, tb, tn, sp, sq, k), A3(bdo, sp, sq, k), A3(bup, sp + 1, sq, k),
-                       A3(bup, sp, sq, k), A3(bdo, sp + 1, sq, k));
-}
-template <typename T, int NH4>
-__device__ __forceinline__ double fs_lim_y(const DevDomain &d, const T *vn, const T *tb, const T *tn, const T *bup,
-                                           const T *bdo, int p, int q, int k)
-{
-    int sp, sq;
-    if (!lbc_src(d, 'V', p, q, sp, sq)) return 0.0;
-    return lim_face<T>((double)fs_zwy<T, NH4>(d, vn, tb, tn, sp, sq, k), A3(bdo, sp, sq, k), A3(bup, sp, sq + 1, k),
-                       A3(bup, sp, sq, k), A3(bdo, sp, sq + 1, k));
-}
-// ah_wslp2 (and akz) at w-level k of an interior column, traldf_iso.F90:146-201
-template <typename T, int MSC>
-__device__ __forceinline__ void fs_ahw(const DevDomain &d, double z2dt, const T *pahu, const T *pahv, const T *wslpi,
-                                       const T *wslpj, int i, int j, int k, T &zmsku, T &zmskv, T &zahu_w, T &zahv_w,
-                                       T &ah, T &akz)
-{
-    const Metrics<T> &m = met<T>(d);
-    const T wm = WMASK(T, i, j, k);
-    zmsku = wm / fmax2(UMASK(T, i, j, k - 1) + UMASK(T, i - 1, j, k) + UMASK(T, i - 1, j, k - 1) + UMASK(T, i, j, k), T(1));
-    zmskv = wm / fmax2(VMASK(T, i, j, k - 1) + VMASK(T, i, j - 1, k) + VMASK(T, i, j - 1, k - 1) + VMASK(T, i, j, k), T(1));
-    zahu_w = (A3(pahu, i, j, k - 1) + A3(pahu, i - 1, j, k) + A3(pahu, i - 1, j, k - 1) + A3(pahu, i, j, k)) * zmsku;
-    zahv_w = (A3(pahv, i, j, k - 1) + A3(pahv, i, j - 1, k) + A3(pahv, i, j - 1, k - 1) + A3(pahv, i, j, k)) * zmskv;
-    const T wi = A3(wslpi, i, j, k), wj = A3(wslpj, i, j, k);
-    ah = zahu_w * wi * wi + zahv_w * wj * wj;
-    if (MSC) {
-        const size_t ij = IJ(i, j);
-        const T e1u = m.e1u[ij], e1u_w = m.e1u[IJ(i - 1, j)], e2v = m.e2v[ij], e2v_s = m.e2v[IJ(i, j - 1)];
-        const T zakz = T(0.25) * ((A3(pahu, i, j, k) + A3(pahu, i, j, k - 1)) / (e1u * e1u) +
-                                  (A3(pahu, i - 1, j, k) + A3(pahu, i - 1, j, k - 1)) / (e1u_w * e1u_w) +
-                                  (A3(pahv, i, j, k) + A3(pahv, i, j, k - 1)) / (e2v * e2v) +
-                                  (A3(pahv, i, j - 1, k) + A3(pahv, i, j - 1, k - 1)) / (e2v_s * e2v_s));
-        const T e3w = E3(m.e3w, ij, k);
-        const T z1_2dt = T(1) / (T)z2dt;
-        const T ze3w_2 = e3w * e3w;
-        const T zcoef0 = (T)z2dt * (zakz + ah / ze3w_2);
-        akz = fmax2(zcoef0 - T(0.5), T(0)) * ze3w_2 * z1_2dt;
-    } else {
-        akz = ah;
-    }
-}
-// ztfw at w-level k (2..jpkm1) of tracer tb, traldf_iso.F90:298-332 (laplacian form)
-template <typename T, int MSC>
-__device__ __forceinline__ T fs_ztfw(const DevDomain &d, double z2dt, const T *pahu, const T *pahv, const T *wslpi,
-                                     const T *wslpj, const T *tb, int i, int j, int k, T &ah_out, T &akz_out)
-{
-    const Metrics<T> &m = met<T>(d);
-    const size_t ij = IJ(i, j);
-    T zmsku, zmskv, zahu_w, zahv_w, ah, akz;
-    fs_ahw<T, MSC>(d, z2dt, pahu, pahv, wslpi, wslpj, i, j, k, zmsku, zmskv, zahu_w, zahv_w, ah, akz);
-    ah_out = ah;
-    akz_out = akz;
-    const T zcoef3 = -(zahu_w * m.e2t[ij] * zmsku * A3(wslpi, i, j, k));
-    const T zcoef4 = -(zahv_w * m.e1t[ij] * zmskv * A3(wslpj, i, j, k));
-    T ztfw = zcoef3 * (iso_zdit(d, tb, i, j, k - 1) + iso_zdit(d, tb, i - 1, j, k) + iso_zdit(d, tb, i - 1, j, k - 1) +
-                       iso_zdit(d, tb, i, j, k)) +
-             zcoef4 * (iso_zdjt(d, tb, i, j, k - 1) + iso_zdjt(d, tb, i, j - 1, k) + iso_zdjt(d, tb, i, j - 1, k - 1) +
-                       iso_zdjt(d, tb, i, j, k));
-    const T e1e2t = m.e1e2t[ij], e3w = E3(m.e3w, ij, k);
-    const T dt = A3(tb, i, j, k - 1) - A3(tb, i, j, k);
-    return ztfw + e1e2t / e3w * WMASK(T, i, j, k) * (ah - akz) * dt;
-}
-
-template <typename T, int NH4>
-KERNEL void k_step_trac(DevDomain d, const T *__restrict__ un, const T *__restrict__ vn, const T *__restrict__ wn,
-                        const T *__restrict__ ptb, const T *__restrict__ ptn, const T *__restrict__ pbu,
-                        const T *__restrict__ pbd, const T *__restrict__ pzwz, T *__restrict__ pta)
-{
-    THREAD_IJ(1, 1);
-    const int k = (int)blockIdx.z + 1;
-    if (i > d.jpi || j > d.jpj) return;
-    const size_t n3 = d.n3;
-    const size_t p = IJK(i, j, k);
-    if (k == d.jpk || !is_interior(d, i, j)) {
-        pta[p] = T(0);
-        pta[n3 + p] = T(0);
-        return;
-    }
-    const Metrics<T> &m = met<T>(d);
-    const size_t ij = IJ(i, j);
-    const T r1_e1e2t = m.r1_e1e2t[ij];
-    const T e3t = E3(m.e3t, ij, k);
-    const T tm = TMASK(T, i, j, k);
-#pragma unroll 1
-    for (int t = 0; t < 2; t++) {
-        const T *tb = ptb + t * n3, *tn = ptn + t * n3;
-        const T *bu = pbu + t * n3, *bd = pbd + t * n3, *zwz = pzwz + t * n3;
-        const T ztra = -((fs_upx(d, un, tb, i, j, k) - fs_upx(d, un, tb, i - 1, j, k) + fs_upy(d, vn, tb, i, j, k) -
-                          fs_upy(d, vn, tb, i, j - 1, k) + fs_upz(d, wn, tb, i, j, k) - fs_upz(d, wn, tb, i, j, k + 1)) *
-                         r1_e1e2t);
-        T a = T(0);
-        a = a + ztra / e3t * tm;
-        const double lz_k = (k == 1) ? (double)A3(zwz, i, j, 1)
-                                     : lim_face<T>(A3(zwz, i, j, k), A3(bd, i, j, k), A3(bu, i, j, k - 1),
-                                                   A3(bu, i, j, k), A3(bd, i, j, k - 1));
-        const double lz_kp1 = lim_face<T>(A3(zwz, i, j, k + 1), A3(bd, i, j, k + 1), A3(bu, i, j, k),
-                                          A3(bu, i, j, k + 1), A3(bd, i, j, k));
-        const T lx_i = (T)fs_lim_x<T, NH4>(d, un, tb, tn, bu, bd, i, j, k);
-        const T lx_im1 = (T)fs_lim_x<T, NH4>(d, un, tb, tn, bu, bd, i - 1, j, k);
-        const T ly_j = (T)fs_lim_y<T, NH4>(d, vn, tb, tn, bu, bd, i, j, k);
-        const T ly_jm1 = (T)fs_lim_y<T, NH4>(d, vn, tb, tn, bu, bd, i, j - 1, k);
-        a = a - (lx_i - lx_im1 + ly_j - ly_jm1 + (T)lz_k - (T)lz_kp1) * r1_e1e2t / e3t;
-        pta[t * n3 + p] = a;
-    }
-}
-
-// rotated laplacian added to the trend, one thread per interior cell
-template <typename T, int MSC>
-KERNEL void k_step_tra_iso(DevDomain d, double z2dt, const T *__restrict__ ptb, const T *__restrict__ pahu,
-                           const T *__restrict__ pahv, const T *__restrict__ uslp, const T *__restrict__ vslp,
-                           const T *__restrict__ wslpi, const T *__restrict__ wslpj, T *__restrict__ pta,
-                           T *__restrict__ ah_wslp2, T *__restrict__ akz)
-{
-    THREAD_IJ(2, 2);
-    const int k = (int)blockIdx.z + 1;
-    if (i > d.jpi - 1 || j > d.jpj - 1) return;
-    const size_t n3 = d.n3;
-    const size_t p = IJK(i, j, k);
-    const Metrics<T> &m = met<T>(d);
-    const size_t ij = IJ(i, j);
-    const T r1_e1e2t = m.r1_e1e2t[ij];
-    const T e3t = E3(m.e3t, ij, k);
-    const int jpk = d.jpk;
-    T ah_k = T(0), akz_k = T(0), ah_1, akz_1;
-#pragma unroll 1
-    for (int t = 0; t < 2; t++) {
-        const T *tb = ptb + t * n3;
-        const T hdiv = iso_zftu(d, pahu, uslp, tb, i, j, k) - iso_zftu(d, pahu, uslp, tb, i - 1, j, k) +
-                       iso_zftv(d, pahv, vslp, tb, i, j, k) - iso_zftv(d, pahv, vslp, tb, i, j - 1, k);
-        const T ztfw_k = (k >= 2) ? fs_ztfw<T, MSC>(d, z2dt, pahu, pahv, wslpi, wslpj, tb, i, j, k, ah_k, akz_k) : T(0);
-        const T ztfw_kp1 = (k + 1 <= jpk - 1)
-                               ? fs_ztfw<T, MSC>(d, z2dt, pahu, pahv, wslpi, wslpj, tb, i, j, k + 1, ah_1, akz_1)
-                               : T(0);
-        T a = pta[t * n3 + p];
-        a = a + hdiv * r1_e1e2t / e3t;
-        a = a + (ztfw_k - ztfw_kp1) * r1_e1e2t / e3t;
-        pta[t * n3 + p] = a;
-    }
-    if (k >= 2) {
-        ah_wslp2[p] = ah_k;
-        if (MSC) akz[p] = akz_k;
-    }
-}
-
 template <typename T>
 static PivotTab<T> cpt_pivots(int jpk)
 {
@@ -1370,36 +1265,6 @@ static PivotTab<T> cpt_pivots(int jpk)
     return pt;
 }
 
-template <typename T, int NH4, int NV4>
-static int launch_step_tra_hv(const DevDomain &d, double p2dt, double z2dt, double p2dt_zdf, const T *un, const T *vn,
-                              const T *wn, const T *tsb, const T *tsn, T *tsa, const T *ahtu, const T *ahtv,
-                              const T *uslp, const T *vslp, const T *wslpi, const T *wslpj, const T *avt, T *ah_wslp2,
-                              T *akz)
-{
-    const PivotTab<T> pt = cpt_pivots<T>(d.jpk);
-    // scratch (storage type, two tracers each): zwi, zwz, betas
-    T *zwi = (T *)nh::scratch3(nh::S_ZWI), *zwz = (T *)nh::scratch3(nh::S_ZWX);
-    T *bu = (T *)nh::scratch3(nh::S_ZWY), *bd = (T *)nh::scratch3(nh::S_ZWZ);
-    Grid2 gi = grid2(d.jpi - 2, d.jpj - 2), g3 = grid2(d.jpi, d.jpj);
-    g3.grid.z = (unsigned)d.jpk;
-    LAUNCH("step_tra", (k_step_tra0<T, NV4>), gi, d, p2dt, pt, un, vn, wn, tsb, tsn, zwi, zwz);
-    LAUNCH("step_tra", (k_step_trab<T, NH4>), g3, d, p2dt, un, vn, tsb, tsn, zwi, zwz, bu, bd);
-    LAUNCH("step_tra", (k_step_trac<T, NH4>), g3, d, un, vn, wn, tsb, tsn, bu, bd, zwz, tsa);
-    Grid2 gk = grid2(d.jpi - 2, d.jpj - 2);
-    gk.grid.z = (unsigned)(d.jpk - 1);
-    if (d.ln_traldf_msc)
-        LAUNCH("step_tra", (k_step_tra_iso<T, 1>), gk, d, z2dt, tsb, ahtu, ahtv, uslp, vslp, wslpi, wslpj, tsa,
-               ah_wslp2, akz);
-    else
-        LAUNCH("step_tra", (k_step_tra_iso<T, 0>), gk, d, z2dt, tsb, ahtu, ahtv, uslp, vslp, wslpi, wslpj, tsa,
-               ah_wslp2, akz);
-    // tra_zdf: the shared matrix on avt plus the iso-neutral coefficient, salinity reset
-    double *zwt = (double *)nh::scratch3(nh::S_ZWT);
-    const T *add = d.ln_traldf_msc ? (const T *)akz : (const T *)ah_wslp2;
-    LAUNCH("step_tra", (k_tra_zdf<T, 2>), gi, d, p2dt_zdf, tsb, tsa, 0, avt, 1, add, zwt, 1);
-    return 0;
-}
-
 template <typename T>
 int launch_step_tra(double p2dt, double z2dt, double p2dt_zdf, const T *un, const T *vn, const T *wn, const T *tsb,
                     const T *tsn, T *tsa, int kn_fct_h, int kn_fct_v, const T *ahtu, const T *ahtv, const T *uslp,
@@ -1409,13 +1274,17 @@ int launch_step_tra(double p2dt, double z2dt, double p2dt_zdf, const T *un, cons
     if (d.jpk > STEP_MAXK) return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "step_tra: jpk = %d > %d", d.jpk, STEP_MAXK);
     if (!d.ln_traldf_lap) return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "step_tra: bilaplacian diffusion is not fused");
     if (d.ln_zdfddm) return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "step_tra: ln_zdfddm = T is not fused");
-    const int h4 = kn_fct_h == 4, v4 = kn_fct_v == 4;
-#define STEP_TRA_ARGS d, p2dt, z2dt, p2dt_zdf, un, vn, wn, tsb, tsn, tsa, ahtu, ahtv, uslp, vslp, wslpi, wslpj, avt, ah_wslp2, akz
-    if (h4 && v4) return launch_step_tra_hv<T, 1, 1>(STEP_TRA_ARGS);
-    if (h4) return launch_step_tra_hv<T, 1, 0>(STEP_TRA_ARGS);
-    if (v4) return launch_step_tra_hv<T, 0, 1>(STEP_TRA_ARGS);
-    return launch_step_tra_hv<T, 0, 0>(STEP_TRA_ARGS);
-#undef STEP_TRA_ARGS
+    int st;
+    // FCT on transports formed where used, trend from zero
+    if ((st = launch_fct_pairs<1, T>(d, p2dt, un, vn, wn, tsb, tsn, tsa, 2, kn_fct_h, kn_fct_v, 1))) return st;
+    if ((st = launch_tra_ldf_iso<T>(0, z2dt, ahtu, ahtv, tsb, tsb, tsa, 2, 1, uslp, vslp, wslpi, wslpj, ah_wslp2, akz)))
+        return st;
+    // tra_zdf: the shared matrix on avt plus the iso-neutral coefficient, salinity reset
+    double *zwt = (double *)nh::scratch3(8);
+    const T *add = d.ln_traldf_msc ? (const T *)akz : (const T *)ah_wslp2;
+    Grid2 gi = grid2(d.jpi - 2, d.jpj - 2);
+    LAUNCH("step_tra", (k_tra_zdf<T, 2>), gi, d, p2dt_zdf, tsb, tsa, 0, avt, 1, add, zwt, 1);
+    return 0;
 }
 template int launch_step_tra<double>(double, double, double, const double *, const double *, const double *,
                                      const double *, const double *, double *, int, int, const double *,

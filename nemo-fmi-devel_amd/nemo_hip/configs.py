@@ -362,8 +362,10 @@ def make_state(d, seed=1, umax=0.2, divergent=0.2, kz_range=(1e-5, 1e-1), kz_con
     S["vtau_b"] = fzeros((jpi, jpj))
     S["emp"] = np.asfortranarray(1e-5 * _smooth(rng.standard_normal((jpi, jpj)), sig) * ssm)
     S["rnf"] = fzeros((jpi, jpj))
-    S["ub2_b"] = np.asfortranarray(S["un_b"] * hu * A["e2u"])
-    S["vb2_b"] = np.asfortranarray(S["vn_b"] * hv * A["e1v"])
+    # previous-step time-averaged transports per unit width (m2/s), as
+    # dyn_spg_ts stores them in un_adv / ub2_b (DYN/dynspg_ts.F90:1214-1230)
+    S["ub2_b"] = np.asfortranarray(S["un_b"] * hu)
+    S["vb2_b"] = np.asfortranarray(S["vn_b"] * hv)
     S["un_bf"] = np.asfortranarray(0.01 * S["ub2_b"])
     S["vn_bf"] = np.asfortranarray(0.01 * S["vb2_b"])
     # after-level barotropic velocities used by dyn_zdf (from a dyn_spg_ts call)
@@ -511,8 +513,8 @@ def make_state_device(d, seed=1, umax=0.2, divergent=0.2, kz_range=(1e-5, 1e-1),
     S["vtau_b"] = torch.zeros_like(hu)
     S["emp"] = 1e-5 * smooth2(randn(jpj, jpi)) * ssm
     S["rnf"] = torch.zeros_like(hu)
-    S["ub2_b"] = S["un_b"] * hu * e2u
-    S["vb2_b"] = S["vn_b"] * hv * e1v
+    S["ub2_b"] = S["un_b"] * hu
+    S["vb2_b"] = S["vn_b"] * hv
     S["un_bf"] = 0.01 * S["ub2_b"]
     S["vn_bf"] = 0.01 * S["vb2_b"]
     S["ua_b"] = S["un_b"] * 1.01

@@ -22,6 +22,19 @@
  * Options: the routines implement the switches named in nemo_hip.h
  * (ln_linssh=T, ln_isfcav=F, ln_zps=F, ln_sco=F, ln_zad_Aimp=F, serial lbc
  * with jperio 0/1/2/7, no wetting/drying, tides, BDY or AGRIF).
+ *
+ * Pinning (tests/test_oracle_pins.py).  The reference ships no golden vectors
+ * or known-answer tests for these routines and cannot be compiled here, so
+ * each function is checked against what the mathematics fixes: dense solves
+ * (tra_zdf_imp, dyn_zdf, interp_4th_cpt), closed forms (masks, lbc_lnk,
+ * tra_ldf_lap, dyn_keg, dyn_zad, ts_wgt, ah_wslp2), a textbook 1-D Zalesak
+ * FCT (tra_adv_fct + nonosc), reductions (tra_ldf_iso -> tra_ldf_lap at zero
+ * slope) and conservation / invariance laws (tracer and momentum budgets, the
+ * no-work property of the ENE/EEN/EET/ENT vorticity terms, barotropic volume,
+ * rest state, linearity).
+ * Parity partially pinned: dyn_vor ENS and MIX near land; the forcing and
+ * time-filter terms of dyn_spg_ts (drag, wind, un_bf/ub2_b); nonosc on
+ * masked coasts (bounds invariant only).
  */
 #include <math.h>
 #include <stdio.h>
