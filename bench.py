@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--timing-steps", type=int, default=2)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (three times): the reference kernel the "
+                         "FETCH_SIZE / WRITE_SIZE counters are scaled by")
     return ap.parse_args()
 
 
@@ -173,11 +176,11 @@ def kernel_times(h):
         "dyn_spg_ts": ["k_spg_cor1", "k_spg_cor2", "k_spg_p1a", "k_spg_p1b", "k_spg_s1", "k_spg_s2", "k_spg_s3",
                        "k_spg_p3a", "k_spg_p3b"],
         "tra_adv_trp": ["k_tra_adv_trp"],
-        "tra_adv_fct": ["k_fct_pre", "k_fct_beta", "k_fct_final", "k_fct_low", "k_fct_ad"],
+        "tra_adv_fct": ["k_fct_h", "k_fct_v", "k_fct_beta", "k_fct_final", "k_fct_low", "k_fct_ad"],
         "tra_ldf_iso": ["k_ldf_iso_coef", "k_tra_ldf_iso"], "tra_zdf": ["k_tra_zdf", "k_clamp_ring"],
         "lbc_lnk": ["k_lbc"],
         "step_dyn": ["k_step_dyn1", "k_step_dyn2"],
-        "step_tra": ["k_fct_pre", "k_fct_beta", "k_fct_final", "k_zero_ring", "k_tra_zdf"],
+        "step_tra": ["k_fct_h", "k_fct_v", "k_fct_beta", "k_fct_final", "k_zero_ring", "k_tra_zdf"],
     }
     for n in names:
         out[n] = h.timing_get(n)
@@ -228,8 +231,24 @@ def cpu_baseline(sample, steps, seed):
                       f"{t:.1f} s, plain C oracle -O2 -ffp-contract=off, 1 thread"}
 
 
+def copy_calibration(mib):
+    """A plain copy kernel of known size, for scaling the HBM byte counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.rand(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4, "launches": 3}))
+
+
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration(a.copy_calibration)
+        return
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               f"(the path shards as NEMO's (i,j) domain decomposition, see DESIGN.md)", file=sys.stderr)
@@ -333,10 +352,11 @@ KERNEL_BYTES = {
     "k_spg_p1a": _b(6),          # r ua va un vn (bottom), w ua va
     "k_spg_p3b": _b(8),          # r ua va un vn, w ua va un vn
     "k_tra_adv_trp": lambda w, i, j, k: 6 * w * i * j * (k - 1),
-    # FCT column pre-pass (two tracers per launch): the routine's own fields;
+    # FCT first passes (two tracers per launch): the routine's own fields;
     # its temporaries (zwi, zwx, zwy, zwz), as in the per-routine accounting,
     # are not counted
-    "k_fct_pre": _b(11),         # r pun pvn pwn tb*2 tn*2, r+w ta*2 (w only in the fused step)
+    "k_fct_h": _b(11),           # r pun pvn pwn tb*2 tn*2, r+w ta*2 (w only in the fused step)
+    "k_fct_v": _b(5),            # r pwn tb*2 tn*2
     # fused momentum step
     "k_step_dyn1": _b(5),        # r un vn wn, w ua va
     "k_step_dyn2": _b(11),       # r ua va un vn ub vb avm, w ua va un vn

@@ -237,13 +237,15 @@ template <typename T>
 KERNEL void k_fct_beta(DevDomain d, double p2dt, const T * __restrict__ pbef, const T * __restrict__ paft, const T * __restrict__ paa, const T * __restrict__ pbb,
                            const T * __restrict__ pcc, T * __restrict__ zbetup, T * __restrict__ zbetdo)
 {
+    // one thread per interior cell, k = blockIdx.z + 1 (no recurrence in k)
     THREAD_IJ(2, 2);
+    const int k = (int)blockIdx.z + 1;
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
     const double zrtrn = 1.e-15;
     const size_t ij = IJ(i, j);
     const double e1e2t = d.md.e1e2t[ij];
     const int jpk = d.jpk;
-    for (int k = 1; k <= jpk - 1; k++) {
+    {
         const int ikm1 = k - 1 > 1 ? k - 1 : 1;
         double zup = zbup(d, pbef, paft, i, j, k);
         zup = fmax2(zup, zbup(d, pbef, paft, i - 1, j, k));
@@ -280,8 +282,10 @@ KERNEL void k_fct_beta(DevDomain d, double p2dt, const T * __restrict__ pbef, co
             A3(zbetdo, i, j, k) = (T)bd;
         }
     }
-    A3(zbetup, i, j, jpk) = T(0);
-    A3(zbetdo, i, j, jpk) = T(0);
+    if (k == jpk - 1) {
+        A3(zbetup, i, j, jpk) = T(0);
+        A3(zbetdo, i, j, jpk) = T(0);
+    }
 }
 
 // limited fluxes of nonosc (traadv_fct.F90:404-422), FP64
@@ -325,20 +329,19 @@ __device__ __forceinline__ double lim_z(const DevDomain &d, const T *pcc, const 
 template <typename T>
 KERNEL void k_fct_final(DevDomain d, const T * __restrict__ paa, const T * __restrict__ pbb, const T * __restrict__ pcc, const T * __restrict__ bup, const T * __restrict__ bdo, T * __restrict__ ta)
 {
+    // one thread per interior cell, k = blockIdx.z + 1
     THREAD_IJ(2, 2);
+    const int k = (int)blockIdx.z + 1;
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
     const Metrics<T> &m = met<T>(d);
     const size_t ij = IJ(i, j);
     const T r1_e1e2t = m.r1_e1e2t[ij];
-    double lz_k = lim_z(d, pcc, bup, bdo, i, j, 1);
-    for (int k = 1; k <= d.jpk - 1; k++) {
-        const double lz_kp1 = lim_z(d, pcc, bup, bdo, i, j, k + 1);
-        const T lx_i = (T)lim_x(d, paa, bup, bdo, i, j, k), lx_im1 = (T)lim_x(d, paa, bup, bdo, i - 1, j, k);
-        const T ly_j = (T)lim_y(d, pbb, bup, bdo, i, j, k), ly_jm1 = (T)lim_y(d, pbb, bup, bdo, i, j - 1, k);
-        const size_t p = IJK(i, j, k);
-        ta[p] = ta[p] - (lx_i - lx_im1 + ly_j - ly_jm1 + (T)lz_k - (T)lz_kp1) * r1_e1e2t / E3(m.e3t, ij, k);
-        lz_k = lz_kp1;
-    }
+    const double lz_k = lim_z(d, pcc, bup, bdo, i, j, k);
+    const double lz_kp1 = lim_z(d, pcc, bup, bdo, i, j, k + 1);
+    const T lx_i = (T)lim_x(d, paa, bup, bdo, i, j, k), lx_im1 = (T)lim_x(d, paa, bup, bdo, i - 1, j, k);
+    const T ly_j = (T)lim_y(d, pbb, bup, bdo, i, j, k), ly_jm1 = (T)lim_y(d, pbb, bup, bdo, i, j - 1, k);
+    const size_t p = IJK(i, j, k);
+    ta[p] = ta[p] - (lx_i - lx_im1 + ly_j - ly_jm1 + (T)lz_k - (T)lz_kp1) * r1_e1e2t / E3(m.e3t, ij, k);
 }
 
 template <typename T>
@@ -352,6 +355,8 @@ int launch_tra_adv_fct(double p2dt, const T *pun, const T *pvn, const T *pwn, co
     T *zwz = (T *)nh::scratch3(nh::S_ZWZ), *zbu = (T *)nh::scratch3(nh::S_ZBETUP), *zbd = (T *)nh::scratch3(nh::S_ZBETDO);
     double *zwt = (double *)nh::scratch3(nh::S_ZTW);
     Grid2 gi = grid2(d.jpi - 2, d.jpj - 2), gh = grid2(d.jpi - 1, d.jpj - 1);
+    Grid2 gk = gi;
+    gk.grid.z = (unsigned)(d.jpk - 1);
     int st;
     for (int jn = 0; jn < kjpt; jn++) {
         const T *tb = ptb + (size_t)jn * d.n3, *tn = ptn + (size_t)jn * d.n3;
@@ -362,10 +367,10 @@ int launch_tra_adv_fct(double p2dt, const T *pun, const T *pvn, const T *pwn, co
         if ((st = launch_lbc<T>(zwx, d.jpk, 'U', T(0)))) return st;
         if ((st = launch_lbc<T>(zwy, d.jpk, 'V', T(0)))) return st;
         if ((st = launch_lbc<T>(zwz, d.jpk, 'W', T(0)))) return st;
-        LAUNCH("tra_adv_fct", (k_fct_beta<T>), gi, d, p2dt, tb, zwi, zwx, zwy, zwz, zbu, zbd);
+        LAUNCH("tra_adv_fct", (k_fct_beta<T>), gk, d, p2dt, tb, zwi, zwx, zwy, zwz, zbu, zbd);
         if ((st = launch_lbc<T>(zbu, d.jpk, 'T', T(0)))) return st;
         if ((st = launch_lbc<T>(zbd, d.jpk, 'T', T(0)))) return st;
-        LAUNCH("tra_adv_fct", (k_fct_final<T>), gi, d, zwx, zwy, zwz, zbu, zbd, ta);
+        LAUNCH("tra_adv_fct", (k_fct_final<T>), gk, d, zwx, zwy, zwz, zbu, zbd, ta);
     }
     return 0;
 }
@@ -472,13 +477,15 @@ template <typename T>
 KERNEL void k_ldf_iso_coef(DevDomain d, double z2dt, const T * __restrict__ pahu, const T * __restrict__ pahv, const T * __restrict__ wslpi,
                                const T * __restrict__ wslpj, T * __restrict__ ah_wslp2, T * __restrict__ akz)
 {
+    // one thread per (i,j) and level k = blockIdx.z + 2 (no recurrence in k)
     THREAD_IJ(1, 1);
+    const int k = (int)blockIdx.z + 2;
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
     const Metrics<T> &m = met<T>(d);
     const bool inner = is_interior(d, i, j);
     const size_t ij = IJ(i, j);
     const T z1_2dt = T(1) / (T)z2dt;
-    for (int k = 2; k <= d.jpk - 1; k++) {
+    {
         const size_t p = IJK(i, j, k);
         if (inner) {
             const T wm = WMASK(T, i, j, k);
@@ -708,6 +715,7 @@ int launch_tra_ldf_iso(int first, double z2dt, const T *pahu, const T *pahv, con
             HIPCHK(hipMemsetAsync(ah_wslp2, 0, d.n3 * sizeof(T), nh::stream()));
         }
         Grid2 gh = grid2(d.jpi - 1, d.jpj - 1);
+        gh.grid.z = (unsigned)(d.jpk - 2);
         LAUNCH("tra_ldf_iso", k_ldf_iso_coef<T>, gh, d, z2dt, pahu, pahv, wslpi, wslpj, ah_wslp2, akz);
         if (!d.ln_traldf_msc)
             HIPCHK(hipMemcpyAsync(akz, ah_wslp2, d.n3 * sizeof(T), hipMemcpyDeviceToDevice, nh::stream()));
@@ -1043,93 +1051,113 @@ __device__ __forceinline__ T tp_upz(const DevDomain &d, const T *pw, const T *tb
 // (k_fct_beta) and the limited divergence (k_fct_final) follow as in the
 // reference.
 // ---------------------------------------------------------------------------
-template <int TRP, typename T, int NH4, int NV4, int TA0>
-KERNEL void k_fct_pre(DevDomain d, double p2dt, PivotTab<T> pt, const T *__restrict__ pu, const T *__restrict__ pv,
-                      const T *__restrict__ pw, const T *__restrict__ ptb, const T *__restrict__ ptn,
-                      T *__restrict__ pta, T *__restrict__ pzwi, T *__restrict__ pzwx, T *__restrict__ pzwy,
-                      T *__restrict__ pzwz, int kjpt)
+// per cell (one thread per (i,j,k), k = blockIdx.z + 1): the horizontal
+// fluxes of the (i+1/2, j) and (i, j+1/2) faces (upstream and antidiffusive,
+// lines 123-145, 180-221) and, on the interior, the low-order trend and the
+// first guess zwi (158-170)
+template <int TRP, typename T, int NH4, int TA0>
+KERNEL void k_fct_h(DevDomain d, double p2dt, const T *__restrict__ pu, const T *__restrict__ pv,
+                    const T *__restrict__ pw, const T *__restrict__ ptb, const T *__restrict__ ptn, T *__restrict__ pta,
+                    T *__restrict__ pzwi, T *__restrict__ pzwx, T *__restrict__ pzwy, int kjpt)
 {
     THREAD_IJ(1, 1);
+    const int k = (int)blockIdx.z + 1;
     if (i > d.jpi - 1 || j > d.jpj - 1) return;     // the flux loops: 1..jpim1 x 1..jpjm1
     const bool inner = (i >= 2 && j >= 2);
     const Metrics<T> &m = met<T>(d);
-    const size_t ij = IJ(i, j), n3 = d.n3;
-    const int jpk = d.jpk;
-    const T r1_e1e2t = m.r1_e1e2t[ij];
+    const size_t n3 = d.n3;
     const T zp2dt = (T)p2dt;
-    const int ikb = d.mbkt[ij];
+    const T u = tr_u<TRP>(d, pu, i, j, k), v = tr_v<TRP>(d, pv, i, j, k);
+    T um = T(0), vm = T(0), e3t = T(1), tm = T(0), r1_e1e2t = T(0);
+    if (inner) {
+        um = tr_u<TRP>(d, pu, i - 1, j, k);
+        vm = tr_v<TRP>(d, pv, i, j - 1, k);
+        const size_t ij = IJ(i, j);
+        e3t = E3(m.e3t, ij, k);
+        tm = TMASK(T, i, j, k);
+        r1_e1e2t = m.r1_e1e2t[ij];
+    }
 #pragma unroll 1
     for (int t = 0; t < kjpt; t++) {
         const T *tb = ptb + t * n3, *tn = ptn + t * n3;
-        T *ta = pta + t * n3, *zwi = pzwi + t * n3, *zwx = pzwx + t * n3, *zwy = pzwy + t * n3, *zwz = pzwz + t * n3;
-        T zwz_k = inner ? tp_upz<TRP>(d, pw, tb, i, j, 1) : T(0), zout_prev = T(0);
+        const T tbc = A3(tb, i, j, k), tbe = A3(tb, i + 1, j, k), tbn = A3(tb, i, j + 1, k);
+        const T hu = T(0.5) * ((u + fabs(u)) * tbc + (u - fabs(u)) * tbe);
+        const T hv = T(0.5) * ((v + fabs(v)) * tbc + (v - fabs(v)) * tbn);
+        const T c2u = A3(tn, i, j, k) + A3(tn, i + 1, j, k);
+        const T c2v = A3(tn, i, j, k) + A3(tn, i, j + 1, k);
+        T *zwx = pzwx + t * n3, *zwy = pzwy + t * n3;
+        if (NH4) {
+            A3(zwx, i, j, k) = T(0.5) * u * (c2u + fct_zltu(d, tn, i, j, k, true) - fct_zltu(d, tn, i + 1, j, k, true)) - hu;
+            A3(zwy, i, j, k) = T(0.5) * v * (c2v + fct_zltu(d, tn, i, j, k, false) - fct_zltu(d, tn, i, j + 1, k, false)) - hv;
+        } else {
+            A3(zwx, i, j, k) = T(0.5) * u * c2u - hu;
+            A3(zwy, i, j, k) = T(0.5) * v * c2v - hv;
+        }
+        if (!inner) continue;
+        const T hum = T(0.5) * ((um + fabs(um)) * A3(tb, i - 1, j, k) + (um - fabs(um)) * tbc);
+        const T hvm = T(0.5) * ((vm + fabs(vm)) * A3(tb, i, j - 1, k) + (vm - fabs(vm)) * tbc);
+        const T ztra = -((hu - hum + hv - hvm + tp_upz<TRP>(d, pw, tb, i, j, k) - tp_upz<TRP>(d, pw, tb, i, j, k + 1)) *
+                         r1_e1e2t);
+        T *ta = pta + t * n3, *zwi = pzwi + t * n3;
+        const T a = TA0 ? T(0) : A3(ta, i, j, k);
+        A3(ta, i, j, k) = a + ztra / e3t * tm;
+        A3(zwi, i, j, k) = (e3t * tbc + zp2dt * ztra) / e3t * tm;
+        if (k == d.jpk - 1) A3(zwi, i, j, d.jpk) = T(0);
+    }
+}
+
+// per interior column: the vertical antidiffusive flux zwz (lines 255-278),
+// the compact scheme's sweeps (577-614) with the table pivots
+template <int TRP, typename T, int NV4>
+KERNEL void k_fct_v(DevDomain d, PivotTab<T> pt, const T *__restrict__ pw, const T *__restrict__ ptb,
+                    const T *__restrict__ ptn, T *__restrict__ pzwz, int kjpt)
+{
+    THREAD_IJ(2, 2);
+    if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    const size_t n3 = d.n3;
+    const int jpk = d.jpk;
+    const int ikb = d.mbkt[IJ(i, j)];
 #pragma unroll 1
-        for (int k = 1; k <= jpk - 1; k++) {
-            const T tbc = A3(tb, i, j, k), tbe = A3(tb, i + 1, j, k), tbn = A3(tb, i, j + 1, k);
-            // horizontal fluxes of the (i+1/2, j) and (i, j+1/2) faces: upstream and antidiffusive
-            const T hu = tp_upx<TRP>(d, pu, tbc, tbe, i, j, k);
-            const T v = tr_v<TRP>(d, pv, i, j, k);
-            const T hv = T(0.5) * ((v + fabs(v)) * tbc + (v - fabs(v)) * tbn);
-            {
-                const T c2u = A3(tn, i, j, k) + A3(tn, i + 1, j, k);
-                const T c2v = A3(tn, i, j, k) + A3(tn, i, j + 1, k);
-                const T u = tr_u<TRP>(d, pu, i, j, k);
-                if (NH4) {
-                    A3(zwx, i, j, k) = T(0.5) * u * (c2u + fct_zltu(d, tn, i, j, k, true) - fct_zltu(d, tn, i + 1, j, k, true)) - hu;
-                    A3(zwy, i, j, k) = T(0.5) * v * (c2v + fct_zltu(d, tn, i, j, k, false) - fct_zltu(d, tn, i, j + 1, k, false)) - hv;
-                } else {
-                    A3(zwx, i, j, k) = T(0.5) * u * c2u - hu;
-                    A3(zwy, i, j, k) = T(0.5) * v * c2v - hv;
-                }
-            }
-            if (!inner) continue;
-            // low-order trend and first guess
-            const T e3t = E3(m.e3t, ij, k);
-            const T tm = TMASK(T, i, j, k);
-            const T hum = tp_upx<TRP>(d, pu, A3(tb, i - 1, j, k), tbc, i - 1, j, k);
-            const T vs = tr_v<TRP>(d, pv, i, j - 1, k);
-            const T hvm = T(0.5) * ((vs + fabs(vs)) * A3(tb, i, j - 1, k) + (vs - fabs(vs)) * tbc);
-            const T zwz_kp1 = tp_upz<TRP>(d, pw, tb, i, j, k + 1);
-            const T ztra = -((hu - hum + hv - hvm + zwz_k - zwz_kp1) * r1_e1e2t);
-            T a = TA0 ? T(0) : A3(ta, i, j, k);
-            A3(ta, i, j, k) = a + ztra / e3t * tm;
-            A3(zwi, i, j, k) = (e3t * tbc + zp2dt * ztra) / e3t * tm;
-            zwz_k = zwz_kp1;
-            if (k == 1) continue;
-            if (NV4) {
-                T zwrm, zout, zwi_c = T(0);
-                if (k == 2 || k == ikb) {
-                    zwrm = T(0.5) * (A3(tn, i, j, k - 1) + A3(tn, i, j, k));
-                } else {
-                    const T wm = WMASK(T, i, j, k);
-                    zwi_c = wm;
-                    zwrm = T(3) * wm * (A3(tn, i, j, k) + A3(tn, i, j, k - 1));
-                }
-                if (k == 2) zout = zwrm;
-                else zout = zwrm - zwi_c / cpt_piv(pt, k - 1, ikb) * zout_prev;
-                A3(zwz, i, j, k) = zout;
-                zout_prev = zout;
-            } else {
+    for (int t = 0; t < kjpt; t++) {
+        const T *tb = ptb + t * n3, *tn = ptn + t * n3;
+        T *zwz = pzwz + t * n3;
+        A3(zwz, i, j, 1) = T(0);
+        A3(zwz, i, j, jpk) = T(0);
+        if (!NV4) {
+#pragma unroll 4
+            for (int k = 2; k <= jpk - 1; k++)
                 A3(zwz, i, j, k) = (tr_w<TRP>(d, pw, i, j, k) * T(0.5) * (A3(tn, i, j, k) + A3(tn, i, j, k - 1)) -
                                     tp_upz<TRP>(d, pw, tb, i, j, k)) *
                                    WMASK(T, i, j, k);
-            }
+            continue;
         }
-        if (!inner) continue;
-        A3(zwi, i, j, jpk) = T(0);
-        A3(zwz, i, j, 1) = T(0);
-        A3(zwz, i, j, jpk) = T(0);
-        if (NV4) {
-            T xnext = zout_prev / cpt_piv(pt, jpk - 1, ikb);
-            A3(zwz, i, j, jpk - 1) = (tr_w<TRP>(d, pw, i, j, jpk - 1) * xnext - tp_upz<TRP>(d, pw, tb, i, j, jpk - 1)) *
-                                     WMASK(T, i, j, jpk - 1);
-#pragma unroll 1
-            for (int k = jpk - 2; k >= 2; k--) {
-                const T zws = (k == 2 || k == ikb) ? T(0) : WMASK(T, i, j, k);
-                const T x = (A3(zwz, i, j, k) - zws * xnext) / cpt_piv(pt, k, ikb);
-                A3(zwz, i, j, k) = (tr_w<TRP>(d, pw, i, j, k) * x - tp_upz<TRP>(d, pw, tb, i, j, k)) * WMASK(T, i, j, k);
-                xnext = x;
+        T zout_prev = T(0), tn_km1 = A3(tn, i, j, 1);
+#pragma unroll 4
+        for (int k = 2; k <= jpk - 1; k++) {
+            const T tnk = A3(tn, i, j, k);
+            T zwrm, zout, zwi_c = T(0);
+            if (k == 2 || k == ikb) {
+                zwrm = T(0.5) * (tn_km1 + tnk);
+            } else {
+                const T wm = WMASK(T, i, j, k);
+                zwi_c = wm;
+                zwrm = T(3) * wm * (tnk + tn_km1);
             }
+            if (k == 2) zout = zwrm;
+            else zout = zwrm - zwi_c / cpt_piv(pt, k - 1, ikb) * zout_prev;
+            A3(zwz, i, j, k) = zout;
+            zout_prev = zout;
+            tn_km1 = tnk;
+        }
+        T xnext = zout_prev / cpt_piv(pt, jpk - 1, ikb);
+        A3(zwz, i, j, jpk - 1) = (tr_w<TRP>(d, pw, i, j, jpk - 1) * xnext - tp_upz<TRP>(d, pw, tb, i, j, jpk - 1)) *
+                                 WMASK(T, i, j, jpk - 1);
+#pragma unroll 4
+        for (int k = jpk - 2; k >= 2; k--) {
+            const T zws = (k == 2 || k == ikb) ? T(0) : WMASK(T, i, j, k);
+            const T x = (A3(zwz, i, j, k) - zws * xnext) / cpt_piv(pt, k, ikb);
+            A3(zwz, i, j, k) = (tr_w<TRP>(d, pw, i, j, k) * x - tp_upz<TRP>(d, pw, tb, i, j, k)) * WMASK(T, i, j, k);
+            xnext = x;
         }
     }
 }
@@ -1159,21 +1187,25 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
     T *zwi = (T *)nh::scratch3(0), *zwx = (T *)nh::scratch3(2), *zwy = (T *)nh::scratch3(4);
     T *zwz = (T *)nh::scratch3(6), *bu = (T *)nh::scratch3(8), *bd = (T *)nh::scratch3(10);
     const char *nm = TRP ? "step_tra" : "tra_adv_fct";
-    Grid2 gf = grid2(d.jpi - 1, d.jpj - 1), gi = grid2(d.jpi - 2, d.jpj - 2);
+    Grid2 gi = grid2(d.jpi - 2, d.jpj - 2), gk = gi;
+    gk.grid.z = (unsigned)(d.jpk - 1);
     const size_t n3 = d.n3;
     int st;
     for (int t0 = 0; t0 < kjpt; t0 += 2) {
         const int nt = kjpt - t0 < 2 ? kjpt - t0 : 2;
         const size_t o = (size_t)t0 * n3;
-#define FP_LAUNCH(H, V, Z) \
-    LAUNCH(nm, (k_fct_pre<TRP, T, H, V, Z>), gf, d, p2dt, pt, pu, pv, pw, ptb + o, ptn + o, pta + o, zwi, zwx, zwy, zwz, nt)
-        const int h4 = kn_fct_h == 4, v4 = kn_fct_v == 4;
-        if (ta0) {
-            if (h4 && v4) FP_LAUNCH(1, 1, 1); else if (h4) FP_LAUNCH(1, 0, 1); else if (v4) FP_LAUNCH(0, 1, 1); else FP_LAUNCH(0, 0, 1);
+        Grid2 g3 = grid2(d.jpi - 1, d.jpj - 1);
+        g3.grid.z = (unsigned)(d.jpk - 1);
+#define FH_LAUNCH(H, Z) \
+    LAUNCH(nm, (k_fct_h<TRP, T, H, Z>), g3, d, p2dt, pu, pv, pw, ptb + o, ptn + o, pta + o, zwi, zwx, zwy, nt)
+        if (kn_fct_h == 4) {
+            if (ta0) FH_LAUNCH(1, 1); else FH_LAUNCH(1, 0);
         } else {
-            if (h4 && v4) FP_LAUNCH(1, 1, 0); else if (h4) FP_LAUNCH(1, 0, 0); else if (v4) FP_LAUNCH(0, 1, 0); else FP_LAUNCH(0, 0, 0);
+            if (ta0) FH_LAUNCH(0, 1); else FH_LAUNCH(0, 0);
         }
-#undef FP_LAUNCH
+#undef FH_LAUNCH
+        if (kn_fct_v == 4) LAUNCH(nm, (k_fct_v<TRP, T, 1>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
+        else LAUNCH(nm, (k_fct_v<TRP, T, 0>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
         const int nk = nt * d.jpk;
         if ((st = launch_lbc<T>(zwi, nk, 'T', T(0)))) return st;
         if ((st = launch_lbc<T>(zwx, nk, 'U', T(0)))) return st;
@@ -1181,13 +1213,13 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
         if ((st = launch_lbc<T>(zwz, nk, 'W', T(0)))) return st;
         for (int t = 0; t < nt; t++) {
             const size_t q = (size_t)t * n3;
-            LAUNCH(nm, (k_fct_beta<T>), gi, d, p2dt, ptb + o + q, zwi + q, zwx + q, zwy + q, zwz + q, bu + q, bd + q);
+            LAUNCH(nm, (k_fct_beta<T>), gk, d, p2dt, ptb + o + q, zwi + q, zwx + q, zwy + q, zwz + q, bu + q, bd + q);
         }
         if ((st = launch_lbc<T>(bu, nk, 'T', T(0)))) return st;
         if ((st = launch_lbc<T>(bd, nk, 'T', T(0)))) return st;
         for (int t = 0; t < nt; t++) {
             const size_t q = (size_t)t * n3;
-            LAUNCH(nm, (k_fct_final<T>), gi, d, zwx + q, zwy + q, zwz + q, bu + q, bd + q, pta + o + q);
+            LAUNCH(nm, (k_fct_final<T>), gk, d, zwx + q, zwy + q, zwz + q, bu + q, bd + q, pta + o + q);
         }
     }
     if (ta0) {
