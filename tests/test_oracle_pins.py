@@ -703,3 +703,72 @@ def test_spg_ts_linear_in_the_free_surface():
         outs.append(F)
     for k in ("ssha", "ua_b", "un_adv"):
         assert np.allclose(outs[1][k], 3.0 * outs[0][k], rtol=1e-11, atol=1e-14 * np.abs(outs[1][k]).max())
+
+
+def test_spg_ts_gravity_mode_matches_scalar_recurrence():
+    """Flat closed basin, f = 0, no drag, wind or fresh-water flux and no
+    baroclinic trend: dyn_spg_ts advances one discrete gravity-wave eigenmode
+    of the C-grid.  With N = jpi - 2 wet columns, eta_i = E cos(theta (i - 3/2))
+    on T-points and u_{i+1/2} = U sin(theta (i - 1)) on U-points,
+    theta = pi / N, the flux divergence acts on the amplitudes as H delta U and
+    the pressure gradient as -delta E, delta = 2 sin(theta/2) / dx.  The
+    generalized forward-backward sub-step of Shchepetkin & McWilliams (2005)
+    -- AB3 extrapolation of u for the fluxes, forward sea level, AB3-AM4
+    interpolation of the sea level for the pressure gradient, forward-backward
+    start-up -- with the boxcar averaging of ts_wgt then reduces to a scalar
+    recurrence, written here in mode space.  The oracle's averaged sea level
+    ssha, velocity ua_b and transport un_adv must be that recurrence times the
+    mode shapes."""
+    jpi, jpj, jpk = 24, 8, 5
+    H, dx, nn_baro, rdt = 100.0, 1000.0, 30, 300.0
+    d = uniform_domain(jpi, jpj, jpk, e1=dx, depth=H, f0=0.0,
+                       opts={"nvor_scheme": abi.NP_ENE, "nn_baro": nn_baro, "rdt": rdt})
+    S = configs.make_state(d, seed=21)
+    o = Oracle(d)
+    F = spg_inputs(d, S, zero=True)
+    F["rCdU_bot"][...] = 0.0
+    N = jpi - 2
+    theta = np.pi / N
+    A = 0.05
+    ii = np.arange(jpi)                         # 0-based index of i = ii + 1
+    eta_shape = np.where((ii >= 1) & (ii <= jpi - 2), np.cos(theta * (ii - 0.5)), 0.0)
+    u_shape = np.where(ii <= jpi - 2, np.sin(theta * ii), 0.0)
+    wet_j = np.zeros(jpj)
+    wet_j[1:-1] = 1.0
+    F["sshn"][...] = A * eta_shape[:, None] * wet_j[None, :]
+    assert np.allclose(S["hu_n"][1:-2, 1:-1], H) and np.all(S["hu_n"][jpi - 2, :] == 0.0)
+    o.dyn_spg_ts(2, F)
+
+    # scalar recurrence
+    g, dt = d.opts["grav"], rdt / nn_baro
+    icycle, wgt1, wgt2 = o.ts_wgt(1, 1)
+    delta = 2.0 * np.sin(theta / 2.0) / dx
+    En, Eb, Ebb = A, 0.0, 0.0
+    Un, Ub, Ubb = 0.0, 0.0, 0.0
+    ssha = ua_b = adv = 0.0
+    for jn in range(1, icycle + 1):
+        a1, a2, a3 = (1.0, 0.0, 0.0) if jn < 3 else (1.781105, -1.06221, 0.281105)
+        ue = a1 * Un + a2 * Ub + a3 * Ubb
+        Ea = En - dt * H * delta * ue
+        if jn == 1:
+            b = (1.0, 0.0, 0.0, 0.0)
+        elif jn == 2:
+            b = (1.0833333333333, -0.1666666666666, 0.0833333333333, 0.0)
+        else:
+            b = (0.614, 0.285, 0.088, 0.013)
+        Ep2 = b[0] * Ea + b[1] * En + b[2] * Eb + b[3] * Ebb
+        Ua = Un + dt * g * delta * Ep2
+        ssha += wgt1[jn - 1] * Ea
+        ua_b += wgt1[jn - 1] * Ua
+        adv += wgt2[jn - 1] * ue * H
+        Ebb, Eb, En = Eb, En, Ea
+        Ubb, Ub, Un = Ub, Un, Ua
+    want_ssh = ssha * eta_shape[:, None] * wet_j[None, :]
+    want_u = ua_b * u_shape[:, None] * wet_j[None, :]
+    want_adv = 0.5 * adv * u_shape[:, None] * wet_j[None, :]   # phase 3 with ub2_b = un_bf = 0
+    assert abs(ssha - A) > 1e-3 * A                            # the mode did evolve
+    assert np.allclose(F["ssha"], want_ssh, rtol=0, atol=1e-11 * A)
+    scale_u = max(abs(ua_b), 1e-30)
+    assert np.allclose(F["ua_b"], want_u, rtol=0, atol=1e-10 * scale_u)
+    assert np.allclose(F["un_adv"], want_adv, rtol=0, atol=1e-10 * max(abs(0.5 * adv), 1e-30))
+    assert np.abs(F["va_b"]).max() < 1e-12 * scale_u
