@@ -30,8 +30,7 @@ for p in (ROOT, os.path.join(ROOT, "nemo-fmi-devel_amd")):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-METRIC = json.load(open(os.path.join(ROOT, "BASELINE.json")))["metric"] if os.path.exists(
-    os.path.join(ROOT, "BASELINE.json")) else "grid-cell updates/s per step"
+METRIC = "grid-cell updates/s per step + % HBM peak, 1024\u00b2\u00d775, FP32 vs FP64 form"
 HBM_PEAK = 8.0e12
 
 # Algorithmic HBM bytes per grid cell of each routine (3D arrays the routine's
@@ -132,7 +131,12 @@ class Stepper:
         h.tra_zdf(kt, self.tsb, self.tsa, f["avt"], f["avs"], 1, self.ah, self.akz, sp=sp)
 
 
-def time_form(h, d, S, sp, fused, steps, warmup, timing_steps):
+def time_form(h, d, S, sp, fused, steps, warmup, timing_steps, live=False):
+    """Warm up, time every kernel over `timing_steps` steps with HIP events,
+    then time `steps` steps with the wall clock.  live=True also brackets the
+    launches of the dominant kernel (the one with the most time per step that
+    has a byte count) with HIP events on the library stream inside the timed
+    region; its measured launch durations are returned for the roofline."""
     import torch
     st = Stepper(h, d, S, sp, fused)
     ext = torch.cuda.ExternalStream(h.stream())
@@ -142,6 +146,25 @@ def time_form(h, d, S, sp, fused, steps, warmup, timing_steps):
             st.step(kt)
             kt += 1
         h.sync()
+        # per-routine / per-kernel HIP-event timers over a short run
+        h.timing_reset()
+        h.timing_only(None)
+        h.timing_enable(True)
+        for _ in range(timing_steps):
+            st.step(kt)
+            kt += 1
+        h.sync()
+        h.timing_enable(False)
+        ktimes = kernel_times(h)
+        key = None
+        if live:
+            cand = {k: v[0] for k, v in ktimes.items() if "/" in k and v[1] > 0 and k.split("/")[1] in KERNEL_BYTES}
+            key = max(cand, key=cand.get) if cand else None
+            if key:
+                h.timing_reset()
+                h.timing_only(key)
+                h.timing_enable(True)
+        h.sync()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
@@ -150,19 +173,17 @@ def time_form(h, d, S, sp, fused, steps, warmup, timing_steps):
         h.sync()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        # per-routine / per-kernel HIP-event timers, separate (short) run
-        h.timing_reset()
-        h.timing_enable(True)
-        for _ in range(timing_steps):
-            st.step(kt)
-            kt += 1
-        h.sync()
-        h.timing_enable(False)
+        live_res = None
+        if key:
+            h.timing_enable(False)
+            tot, cnt = h.timing_get(key)
+            h.timing_only(None)
+            live_res = {"kernel": key, "total_ms": tot, "launches": cnt}
     ms = (t1 - t0) * 1e3 / steps
     finite = bool(torch.isfinite(st.tsa).all().item() and torch.isfinite(st.ua).all().item())
     del st
     torch.cuda.empty_cache()
-    return ms, finite
+    return ms, finite, ktimes, live_res
 
 
 def kernel_times(h):
@@ -231,6 +252,28 @@ def cpu_baseline(sample, steps, seed):
                       f"{t:.1f} s, plain C oracle -O2 -ffp-contract=off, 1 thread"}
 
 
+TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic.md")
+
+
+def traffic_per_launch(kname, form, jpi, jpj, jpk):
+    """HBM bytes per launch of a kernel, from the table of calibrated
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counts (one run per counter, each scaled
+    by the same counter's reading for a copy kernel of known size); None when
+    the kernel, form and domain have no row."""
+    if not os.path.exists(TRAFFIC_TABLE):
+        return None
+    kernel = kname.split("/")[-1]
+    want = f"{jpi}x{jpj}x{jpk}"
+    for line in open(TRAFFIC_TABLE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 8 and cells[0] == form and cells[1] == want and cells[2] == kernel:
+            try:
+                return int(float(cells[7]))
+            except ValueError:
+                return None
+    return None
+
+
 def copy_calibration(mib):
     """A plain copy kernel of known size, for scaling the HBM byte counters."""
     import torch
@@ -269,11 +312,15 @@ def main():
     torch.cuda.synchronize()
     log(f"state generated on the device in {time.perf_counter() - t0:.1f} s")
     res = {}
-    for form in a.forms.split(","):
+    forms = a.forms.split(",")
+    head = next((f for f in ("fp32_fused", "fp32") if f in forms), forms[0])
+    live = None
+    for form in forms:
         sp = form.startswith("fp32")
         fused = form.endswith("_fused")
-        ms, finite = time_form(h, d, S, sp, fused, a.steps, a.warmup, a.timing_steps)
-        kt = kernel_times(h)
+        ms, finite, kt, lv = time_form(h, d, S, sp, fused, a.steps, a.warmup, a.timing_steps, live=(form == head))
+        if form == head:
+            live = lv
         res[form] = {"ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3), "finite": finite,
                      "routines_ms_per_step": {k: v[0] / a.timing_steps for k, v in kt.items()
                                               if "/" not in k and v[1] > 0},
@@ -288,22 +335,18 @@ def main():
             f"HBM fraction of step {res[form]['hbm_frac_step']:.3f}, finite={finite}")
         for k, v in sorted(res[form]["routines_ms_per_step"].items(), key=lambda x: -x[1]):
             log(f"   {k:14s} {v:8.3f} ms")
-    # roofline of the dominant single kernel of the headline form
-    head = next((f for f in ("fp32_fused", "fp32") if f in res), list(res)[0])
+    # roofline of the dominant kernel of the headline form: its launches timed
+    # with HIP events inside the timed region
     width = 4 if head.startswith("fp32") else 8
-    kms = res[head]["kernels_ms_per_step"]
-    klaunch = res[head]["kernel_launches_per_step"]
     roof = None
-    if kms:
-        kname = max(kms, key=kms.get)
-        n_launch = max(1, klaunch.get(kname, 1))
-        avg_ms = kms[kname] / n_launch
-        arrays = KERNEL_BYTES.get(kname.split("/")[1]) if "/" in kname else None
-        if arrays is not None:
-            alg_bytes = arrays(width, jpi, jpj, jpk)
-            roof = {"kernel": kname, "bound": "hbm", "achieved": alg_bytes / (avg_ms * 1e-3) / 1e9,
-                    "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": alg_bytes / (avg_ms * 1e-3) / HBM_PEAK,
-                    "traffic": None, "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": alg_bytes}
+    if live and live["launches"] > 0:
+        kname = live["kernel"]
+        avg_ms = live["total_ms"] / live["launches"]
+        alg_bytes = KERNEL_BYTES[kname.split("/")[1]](width, jpi, jpj, jpk)
+        roof = {"kernel": kname, "bound": "hbm", "achieved": alg_bytes / (avg_ms * 1e-3) / 1e9,
+                "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": alg_bytes / (avg_ms * 1e-3) / HBM_PEAK,
+                "traffic": traffic_per_launch(kname, head, jpi, jpj, jpk), "avg_launch_ms": avg_ms,
+                "launches_timed": live["launches"], "algorithmic_bytes_per_launch": alg_bytes}
     cpu = None
     if not a.no_cpu:
         t0 = time.perf_counter()
@@ -334,7 +377,8 @@ def main():
 
 
 # algorithmic bytes per launch of the kernels the roofline line may name
-# (width = bytes per stored value; see DESIGN.md for the per-kernel accounting)
+# (width = bytes per stored value; each 3D array the kernel's arithmetic reads or
+# writes counts once per interior cell, masks and temporaries count 0)
 def _b(n_arrays_interior):
     return lambda w, i, j, k: n_arrays_interior * w * (i - 2) * (j - 2) * (k - 1)
 
@@ -357,6 +401,15 @@ KERNEL_BYTES = {
     # are not counted
     "k_fct_h": _b(11),           # r pun pvn pwn tb*2 tn*2, r+w ta*2 (w only in the fused step)
     "k_fct_v": _b(5),            # r pwn tb*2 tn*2
+    # barotropic sub-step kernels (2D, FP64 in both forms; ENE scheme)
+    # S3: reads zwx zwy zwz zCdU_u zCdU_v un_e vn_e zu_frc zv_frc zsshp2_e
+    # ub_e(-1) vb_e(-1) hu_n hv_n r1_hu_n r1_hv_n r1_e1u r1_e2v e2u e1v r1_e2u
+    # r1_e1v and the sums ua_b va_b un_adv vn_adv; writes ua_e va_e zwx' zwy'
+    # and the four sums: 26 + 8 arrays
+    "k_spg_s3": lambda w, i, j, k: 34 * 8 * i * j,
+    # S2: reads zwx zwy zssh_frc r1_e1e2t sshn_e sshb_e sshbb_e ssha; writes
+    # ssha_e ssha zsshp2_e: 8 + 3 arrays
+    "k_spg_s2": lambda w, i, j, k: 11 * 8 * i * j,
     # fused momentum step
     "k_step_dyn1": _b(5),        # r un vn wn, w ua va
     "k_step_dyn2": _b(11),       # r ua va un vn ub vb avm, w ua va un vn

@@ -157,6 +157,11 @@ int nemo_hip_domain_register(const nemo_hip_domain_t *dom);
    when enabled every kernel launch is bracketed by HIP events on the
    library's stream and accumulated by name */
 int nemo_hip_timing_enable(int on);
+/* restrict the timers to the launches of one kernel, keyed "routine/kernel"
+   (e.g. "step_tra/k_fct_h"); NULL or "" times every launch.  Lets a caller
+   time one kernel inside a production loop at the cost of three events per
+   launch of it */
+int nemo_hip_timing_only(const char *key);
 int nemo_hip_timing_reset(void);
 /* total milliseconds and launch count accumulated for `name` */
 int nemo_hip_timing_get(const char *name, double *total_ms, int64_t *count);

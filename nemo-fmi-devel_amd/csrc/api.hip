@@ -43,6 +43,7 @@ static std::vector<hipEvent_t> g_evfree;
 static std::map<std::string, std::pair<double, int64_t>> g_tot;
 static std::string g_open_name;
 static hipEvent_t g_open_ev = nullptr;
+static std::string g_only;   // when set, only launches with this "routine/kernel" key are timed
 
 // accumulate completed (start, end) pairs and recycle their events; a start
 // event shared by two entries is recycled once
@@ -92,13 +93,6 @@ static hipEvent_t ev_get()
     if (hipEventCreate(&e) != hipSuccess) return nullptr;
     return e;
 }
-void timing_begin(const char *name)
-{
-    if (!g_timing) return;
-    g_open_ev = ev_get();
-    g_open_name = name;
-    if (g_open_ev) (void)hipEventRecord(g_open_ev, g_stream);
-}
 // timers are kept per routine ("tra_adv_fct") and per kernel of the routine
 // ("tra_adv_fct/k_fct_low"); the kernel text is the launch expression with
 // its template arguments and parentheses removed
@@ -112,6 +106,14 @@ static std::string kernel_key(const char *name, const char *kernel)
         else if (depth == 0 && *c != '(' && *c != ')' && *c != ' ') k += *c;
     }
     return std::string(name) + "/" + k;
+}
+void timing_begin(const char *name, const char *kernel)
+{
+    if (!g_timing) return;
+    if (!g_only.empty() && kernel_key(name, kernel) != g_only) return;
+    g_open_ev = ev_get();
+    g_open_name = name;
+    if (g_open_ev) (void)hipEventRecord(g_open_ev, g_stream);
 }
 void timing_end(const char *name, const char *kernel)
 {
@@ -575,6 +577,11 @@ int nemo_hip_domain_register(const nemo_hip_domain_t *dom)
 int nemo_hip_timing_enable(int on)
 {
     g_timing = on != 0;
+    return NEMO_HIP_OK;
+}
+int nemo_hip_timing_only(const char *key)
+{
+    g_only = key ? key : "";
     return NEMO_HIP_OK;
 }
 int nemo_hip_timing_reset(void)

@@ -185,6 +185,18 @@ __device__ __forceinline__ bool lbc_src(const DevDomain &d, char nat, int i, int
     return true;
 }
 
+// branch-free form for stencils: when the point has no source (closed ring)
+// the source is moved to the interior point (2,2), so the caller can load its
+// stencil unconditionally (every load stays in bounds) and select zero at the
+// end; the compiler can then issue all loads of a stencil together
+__device__ __forceinline__ bool lbc_src_safe(const DevDomain &d, char nat, int i, int j, int &si, int &sj)
+{
+    const bool ok = lbc_src(d, nat, i, j, si, sj);
+    si = ok ? si : 2;
+    sj = ok ? sj : 2;
+    return ok;
+}
+
 __device__ __forceinline__ bool is_interior(const DevDomain &d, int i, int j)
 {
     return i >= 2 && i <= d.jpi - 1 && j >= 2 && j <= d.jpj - 1;
@@ -211,9 +223,27 @@ inline Grid2 grid2(int ni, int nj, int bx = 64, int by = NH_BLOCK / 64)
     g.grid = dim3((unsigned)((ni + bx - 1) / bx), (unsigned)((nj + by - 1) / by), 1);
     return g;
 }
+// XCD-aware tile order.  The dispatcher deals the workgroups of a launch
+// round-robin to the 8 XCDs, each with its own L2, so with the plain order the
+// i-neighbour tiles of a tile sit in other L2s.  Within each level plane of
+// the grid, workgroup b takes tile l = (b mod 8) * (tiles/8) + b div 8
+// instead: XCD e then works through the contiguous band e of tile rows, where
+// the i- and j-neighbours of a tile (and, for per-cell kernels, the band one
+// level up) are cached in its own L2.  Planes whose tile count is not a
+// multiple of 8 keep the plain order.
+__device__ __forceinline__ void xcd_tile(unsigned &bx, unsigned &by)
+{
+    const unsigned nx = gridDim.x, np = gridDim.x * gridDim.y;
+    const unsigned b = blockIdx.x + nx * blockIdx.y;
+    const unsigned l = (np & 7u) ? b : (b & 7u) * (np >> 3) + (b >> 3);
+    bx = l % nx;
+    by = l / nx;
+}
 #define THREAD_IJ(i0, j0) \
-    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) + (i0); \
-    const int j = (int)(blockIdx.y * blockDim.y + threadIdx.y) + (j0);
+    unsigned bx_, by_; \
+    xcd_tile(bx_, by_); \
+    const int i = (int)(bx_ * blockDim.x + threadIdx.x) + (i0); \
+    const int j = (int)(by_ * blockDim.y + threadIdx.y) + (j0);
 
 // ---------------------------------------------------------------------------
 // host side: error state, stream, timing, launch wrapper
@@ -224,7 +254,7 @@ hipStream_t stream();
 const DevDomain &dom();
 bool have_domain();
 // per-routine timing: begin/end bracket kernel launches under a name
-void timing_begin(const char *name);
+void timing_begin(const char *name, const char *kernel);
 void timing_end(const char *name, const char *kernel);
 // library-owned scratch buffers (HBM), sized at domain registration
 enum Scratch {
@@ -244,7 +274,7 @@ double *scratch2(int which);   // n2 doubles each (dyn_spg_ts work arrays), NH_N
 
 #define LAUNCH(name, kern, g, ...)                                                \
     do {                                                                          \
-        nh::timing_begin(name);                                                   \
+        nh::timing_begin(name, #kern);                                            \
         hipLaunchKernelGGL(kern, (g).grid, (g).block, 0, nh::stream(), __VA_ARGS__); \
         nh::timing_end(name, #kern);                                              \
         HIPCHK(hipGetLastError());                                                \

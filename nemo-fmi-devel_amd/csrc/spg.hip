@@ -7,14 +7,17 @@
 // Phase 1 (3D -> 2D) is one column kernel (vertical sums of the trends,
 // removal of their mean, bottom baroclinic velocity) plus one 2D kernel (the
 // barotropic Coriolis / drag / wind forcing).  Phase 2 runs the `icycle`
-// dependent sub-steps as three 2D kernels each:
-//   S1 extrapolated velocities and fluxes (pointwise) + un_adv sum,
-//   S2 after sea level (flux divergence) with its lbc fill, the AB3-AM4
-//      interpolated level and the ssha sum,
-//   S3 after velocities (Coriolis, drag, pressure gradient) with their lbc
-//      fill and the ua_b sum.
-// The before/now/after sub-step levels rotate through three buffers instead of
-// being copied.  All barotropic state and arithmetic is FP64 in both storage
+// dependent sub-steps as two 2D kernels each:
+//   S2 (k_spg_s2)  the after sea level (flux divergence, lbc fill), its ssha
+//                  sum and the AB3-AM4 interpolated level zsshp2_e;
+//   S3 (k_spg_s3)  the after velocities (Coriolis, drag, pressure gradient,
+//                  lbc fill) and the ua_b sum, then the next sub-step's
+//                  extrapolated velocities, fluxes and un_adv sum, pointwise.
+// Only the first extrapolation (k_spg_s1) has a kernel of its own.  (One
+// kernel per sub-step that recomputes zsshp2_e at the east and north
+// neighbours moves as many bytes and runs slower.)  The sub-step levels rotate
+// through buffers instead of being copied: three for the velocities, four for
+// the sea level.  All barotropic state and arithmetic is FP64 in both storage
 // forms.  Phase 3 is one 2D kernel and one column kernel.
 #include "common.hpp"
 #include "kernels.hpp"
@@ -309,8 +312,10 @@ KERNEL void k_spg_s1(DevDomain d, double za1, double za2, double za3, double wgt
     const size_t ij = IJ(i, j);
     const double ue = za1 * un_e[ij] + za2 * ub_e[ij] + za3 * ubb_e[ij];
     const double ve = za1 * vn_e[ij] + za2 * vb_e[ij] + za3 * vbb_e[ij];
-    w.ua_e[ij] = ue;
-    w.va_e[ij] = ve;
+    if (d.nvor_scheme == NEMO_NP_ENT) {   // only the t-point Coriolis reads them
+        w.ua_e[ij] = ue;
+        w.va_e[ij] = ve;
+    }
     const double zwx = m.e2u[ij] * ue * hu_n[ij];
     const double zwy = m.e1v[ij] * ve * hv_n[ij];
     w.zwx[ij] = zwx;
@@ -319,31 +324,39 @@ KERNEL void k_spg_s1(DevDomain d, double za1, double za2, double za3, double wgt
     vn_adv[ij] = vn_adv[ij] + wgt2 * zwy * m.r1_e1v[ij];
 }
 
-KERNEL void k_spg_s2(DevDomain d, double rdtbt, double za0, double za1, double za2, double za3, double wgt1,
-                         const double * __restrict__ sshn_e, const double * __restrict__ sshb_e, double * __restrict__ sshbb_to_a, double * __restrict__ ssha, SpgWork w)
+// the sub-step's sea-level levels and the AB3-AM4 interpolation weights
+// (lines 930-940)
+struct SpgSsh {
+    double za0, za1, za2, za3;
+    const double *sshn_e, *sshb_e, *sshbb_e;
+};
+
+// after sea level of the sub-step at (p,q) after lbc_lnk('T'), lines 905-925:
+// the flux divergence evaluated at the fill's source point
+__device__ __forceinline__ double spg_ssha(const DevDomain &d, double rdtbt, const SpgSsh &c, const SpgWork &w, int p,
+                                           int q)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const Metrics<double> &m = d.md;
-    const size_t ij = IJ(i, j);
     int si, sj;
-    double ssha_e = 0.0;
-    if (lbc_src(d, 'T', i, j, si, sj)) {
-        const size_t s = IJ(si, sj);
-        const double zhdiv = (A2(w.zwx, si, sj) - A2(w.zwx, si - 1, sj) + A2(w.zwy, si, sj) - A2(w.zwy, si, sj - 1)) *
-                             m.r1_e1e2t[s];
-        ssha_e = (sshn_e[s] - rdtbt * (w.zssh_frc[s] + zhdiv)) * d.ssmask[s];
-    }
-    const double sshbb = sshbb_to_a[ij];
-    w.zsshp2_e[ij] = za0 * ssha_e + za1 * sshn_e[ij] + za2 * sshb_e[ij] + za3 * sshbb;
-    sshbb_to_a[ij] = ssha_e;
-    ssha[ij] = ssha[ij] + wgt1 * ssha_e;
+    const bool ok = lbc_src_safe(d, 'T', p, q, si, sj);
+    const size_t s = IJ(si, sj);
+    const double zhdiv = (A2(w.zwx, si, sj) - A2(w.zwx, si - 1, sj) + A2(w.zwy, si, sj) - A2(w.zwy, si, sj - 1)) *
+                         d.md.r1_e1e2t[s];
+    const double a = (c.sshn_e[s] - rdtbt * (w.zssh_frc[s] + zhdiv)) * d.ssmask[s];
+    return ok ? a : 0.0;
+}
+// zsshp2_e at (p,q) from the after level there
+__device__ __forceinline__ double spg_p2(const SpgSsh &c, double ssha_e, size_t pq)
+{
+    return c.za0 * ssha_e + c.za1 * c.sshn_e[pq] + c.za2 * c.sshb_e[pq] + c.za3 * c.sshbb_e[pq];
 }
 
+// after velocities at (i,j), lines 1000-1110; p2c, p2e, p2n are zsshp2_e at
+// (i,j), (i+1,j) and (i,j+1)
 __device__ __forceinline__ void spg_new_vel(const DevDomain &d, int i, int j, double rdtbt, const double *un_e,
                                             const double *vn_e, const double *r1_hu_n, const double *r1_hv_n,
                                             const double *hu_n, const double *hv_n, const double *ht_n,
-                                            const SpgWork &w, double &ua, double &va)
+                                            const SpgWork &w, double p2c, double p2e, double p2n, double &ua,
+                                            double &va)
 {
     const Metrics<double> &m = d.md;
     const size_t ij = IJ(i, j);
@@ -358,26 +371,60 @@ __device__ __forceinline__ void spg_new_vel(const DevDomain &d, int i, int j, do
                 zu_trd, zv_trd);
     zu_trd = zu_trd + w.zCdU_u[ij] * un_e[ij] * r1_hu_n[ij];
     zv_trd = zv_trd + w.zCdU_v[ij] * vn_e[ij] * r1_hv_n[ij];
-    const double zu_spg = -(d.grav * (A2(w.zsshp2_e, i + 1, j) - A2(w.zsshp2_e, i, j)) * m.r1_e1u[ij]);
-    const double zv_spg = -(d.grav * (A2(w.zsshp2_e, i, j + 1) - A2(w.zsshp2_e, i, j)) * m.r1_e2v[ij]);
+    const double zu_spg = -(d.grav * (p2e - p2c) * m.r1_e1u[ij]);
+    const double zv_spg = -(d.grav * (p2n - p2c) * m.r1_e2v[ij]);
     const double zwx = (1.0 - d.rn_scal_load) * zu_spg;
     const double zwy = (1.0 - d.rn_scal_load) * zv_spg;
     ua = (un_e[ij] + rdtbt * (zwx + zu_trd + w.zu_frc[ij])) * d.ssumask[ij];
     va = (vn_e[ij] + rdtbt * (zwy + zv_trd + w.zv_frc[ij])) * d.ssvmask[ij];
 }
-
-KERNEL void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double * __restrict__ un_e, const double * __restrict__ vn_e,
-                          double * __restrict__ ubb_to_a, double * __restrict__ vbb_to_a, const double * __restrict__ r1_hu_n, const double * __restrict__ r1_hv_n,
-                          const double * __restrict__ hu_n, const double * __restrict__ hv_n, const double * __restrict__ ht_n, double * __restrict__ ua_b, double * __restrict__ va_b,
-                          SpgWork w, SpgNext nx)
+// S2: after sea level at (i,j), its time sum and zsshp2_e
+KERNEL void k_spg_s2(DevDomain d, double rdtbt, double wgt1, SpgSsh c, double *__restrict__ ssha_to,
+                     double *__restrict__ ssha, SpgWork w)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi || j > d.jpj) return;
     const size_t ij = IJ(i, j);
-    int si, sj;
-    double ua = 0.0, va = 0.0, t;
-    if (lbc_src(d, 'U', i, j, si, sj)) spg_new_vel(d, si, sj, rdtbt, un_e, vn_e, r1_hu_n, r1_hv_n, hu_n, hv_n, ht_n, w, ua, t);
-    if (lbc_src(d, 'V', i, j, si, sj)) spg_new_vel(d, si, sj, rdtbt, un_e, vn_e, r1_hu_n, r1_hv_n, hu_n, hv_n, ht_n, w, t, va);
+    const double ssha_e = spg_ssha(d, rdtbt, c, w, i, j);
+    ssha_to[ij] = ssha_e;
+    ssha[ij] = ssha[ij] + wgt1 * ssha_e;
+    w.zsshp2_e[ij] = spg_p2(c, ssha_e, ij);
+}
+
+__device__ __forceinline__ void spg_new_vel_p2(const DevDomain &d, int p, int q, double rdtbt, const double *un_e,
+                                               const double *vn_e, const double *r1_hu_n, const double *r1_hv_n,
+                                               const double *hu_n, const double *hv_n, const double *ht_n,
+                                               const SpgWork &w, double &ua, double &va)
+{
+    spg_new_vel(d, p, q, rdtbt, un_e, vn_e, r1_hu_n, r1_hv_n, hu_n, hv_n, ht_n, w, A2(w.zsshp2_e, p, q),
+                A2(w.zsshp2_e, p + 1, q), A2(w.zsshp2_e, p, q + 1), ua, va);
+}
+
+// S3: after velocities and the next sub-step's extrapolation
+KERNEL void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double *__restrict__ un_e,
+                     const double *__restrict__ vn_e, double *__restrict__ ubb_to_a, double *__restrict__ vbb_to_a,
+                     const double *__restrict__ r1_hu_n, const double *__restrict__ r1_hv_n,
+                     const double *__restrict__ hu_n, const double *__restrict__ hv_n, const double *__restrict__ ht_n,
+                     double *__restrict__ ua_b, double *__restrict__ va_b, SpgWork w, SpgNext nx)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const size_t ij = IJ(i, j);
+    // the after velocities at (i,j) after lbc_lnk('U') / ('V'): evaluated at
+    // their source points; on the interior both sources are (i,j) and one
+    // evaluation serves both components
+    int ui, uj, vi, vj;
+    const bool oku = lbc_src_safe(d, 'U', i, j, ui, uj);
+    const bool okv = lbc_src_safe(d, 'V', i, j, vi, vj);
+    double ua1, va1, ua2, va2;
+    spg_new_vel_p2(d, ui, uj, rdtbt, un_e, vn_e, r1_hu_n, r1_hv_n, hu_n, hv_n, ht_n, w, ua1, va1);
+    if (ui == vi && uj == vj) {
+        ua2 = ua1;
+        va2 = va1;
+    } else {
+        spg_new_vel_p2(d, vi, vj, rdtbt, un_e, vn_e, r1_hu_n, r1_hv_n, hu_n, hv_n, ht_n, w, ua2, va2);
+    }
+    const double ua = oku ? ua1 : 0.0, va = okv ? va2 : 0.0;
     const double ub_now = un_e[ij], vb_now = vn_e[ij];
     ubb_to_a[ij] = ua;
     vbb_to_a[ij] = va;
@@ -389,8 +436,10 @@ KERNEL void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double * __re
     const Metrics<double> &m = d.md;
     const double ue = nx.za1 * ua + nx.za2 * ub_now + nx.za3 * nx.ub_before[ij];
     const double ve = nx.za1 * va + nx.za2 * vb_now + nx.za3 * nx.vb_before[ij];
-    nx.ua_e[ij] = ue;
-    nx.va_e[ij] = ve;
+    if (d.nvor_scheme == NEMO_NP_ENT) {   // only the t-point Coriolis reads them
+        nx.ua_e[ij] = ue;
+        nx.va_e[ij] = ve;
+    }
     const double zwx = m.e2u[ij] * ue * hu_n[ij];
     const double zwy = m.e1v[ij] * ve * hv_n[ij];
     nx.zwx[ij] = zwx;
@@ -451,7 +500,8 @@ namespace {
 // the *_e before / now levels: three rotating buffers per variable, kept
 // between calls (they carry over when ln_bt_av = F)
 struct SpgState {
-    int rot_u[3] = {0, 1, 2}, rot_s[3] = {0, 1, 2};   // [now, before, before-before]
+    int rot_u[3] = {0, 1, 2};      // [now, before, before-before]
+    int rot_s[4] = {0, 1, 2, 3};   // [now, before, before-before, after]
 } g_spg;
 }
 
@@ -508,7 +558,7 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
     w.zu_frc = s2[0]; w.zv_frc = s2[1]; w.zwxb = s2[2]; w.zwyb = s2[3]; w.zCdU_u = s2[4]; w.zCdU_v = s2[5];
     w.zssh_frc = s2[6]; w.zwz = s2[7]; w.zhf = s2[8]; w.ftne = s2[9]; w.ftnw = s2[10]; w.ftse = s2[11];
     w.ftsw = s2[12]; w.ua_e = s2[13]; w.va_e = s2[14]; w.zwx = s2[15]; w.zwy = s2[16]; w.zsshp2_e = s2[17];
-    double *ube[3] = {s2[18], s2[19], s2[20]}, *vbe[3] = {s2[21], s2[22], s2[23]}, *sse[3] = {s2[24], s2[25], s2[26]};
+    double *ube[3] = {s2[18], s2[19], s2[20]}, *vbe[3] = {s2[21], s2[22], s2[23]}, *sse[4] = {s2[24], s2[25], s2[26], nh::scratch2(33)};
     const size_t b2 = d.n2 * sizeof(double);
     hipStream_t s = nh::stream();
 
@@ -573,8 +623,8 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
         if (jn == 1 && ll_init) { za0 = 1.0; za1 = 0.0; za2 = 0.0; za3 = 0.0; }
         else if (jn == 2 && ll_init) { za0 = 1.0833333333333; za1 = -0.1666666666666; za2 = 0.0833333333333; za3 = 0.0; }
         else { za0 = 0.614; za1 = 0.285; za2 = 0.088; za3 = 0.013; }
-        LAUNCH("dyn_spg_ts", k_spg_s2, g, d, rdtbt, za0, za1, za2, za3, wgt1[jn - 1], sse[rs[0]], sse[rs[1]],
-               sse[rs[2]], st->ssha, wc);
+        const SpgSsh c{za0, za1, za2, za3, sse[rs[0]], sse[rs[1]], sse[rs[2]]};
+        LAUNCH("dyn_spg_ts", k_spg_s2, g, d, rdtbt, wgt1[jn - 1], c, sse[rs[3]], st->ssha, wc);
         SpgNext nx{};
         nx.on = jn < icycle;
         if (nx.on) {
@@ -588,10 +638,11 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
         }
         LAUNCH("dyn_spg_ts", k_spg_s3, g, d, rdtbt, wgt1[jn - 1], ube[ru[0]], vbe[ru[0]], ube[ru[2]], vbe[ru[2]],
                st->r1_hu_n, st->r1_hv_n, st->hu_n, st->hv_n, st->ht_n, st->ua_b, st->va_b, wc, nx);
-        // swap: bb <- b, b <- n, n <- a (the after level was written over bb)
+        // swap: bb <- b, b <- n, n <- a (the after velocities were written
+        // over bb; the after sea level into the fourth buffer)
         int t;
         t = ru[2]; ru[2] = ru[1]; ru[1] = ru[0]; ru[0] = t;
-        t = rs[2]; rs[2] = rs[1]; rs[1] = rs[0]; rs[0] = t;
+        t = rs[3]; rs[3] = rs[2]; rs[2] = rs[1]; rs[1] = rs[0]; rs[0] = t;
     }
     LAUNCH("dyn_spg_ts", k_spg_p3a, g, d, (kt == d.nit000 && d.neuler == 0) ? 1 : 0, st->un_adv, st->vn_adv, st->ub2_b,
            st->vb2_b, st->un_bf, st->vn_bf);

@@ -24,6 +24,12 @@
 
 // deepest column the data-independent pivot table of interp_4th_cpt covers
 #define STEP_MAXK 160
+// levels per thread of k_fct_beta: a short k-march reuses the vertical
+// neighbours the thread has loaded, where one level per thread re-fetches
+// them through the fabric.  It pays for the FP64 betas; for FP32 storage one
+// level per thread is faster (measured at 1024x1024x75), and so it is for the
+// flux kernel k_fct_h, whose k-march needs too many registers
+template <typename T> struct FctKB { static constexpr int beta = sizeof(T) == 8 ? 8 : 1; };
 template <int TRP, typename T>
 static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const T *pv, const T *pw, const T *ptb,
                             const T *ptn, T *pta, int kjpt, int kn_fct_h, int kn_fct_v, int ta0);
@@ -127,17 +133,19 @@ __device__ __forceinline__ T fct_zltu(const DevDomain &d, const T *tn, int i, in
 {
     // zltu / zltv after lbc_lnk('T'), traadv_fct.F90:195-209
     int si, sj;
-    if (!lbc_src(d, 'T', i, j, si, sj)) return T(0);
+    const bool ok = lbc_src_safe(d, 'T', i, j, si, sj);
     const T r1_6 = T(1) / T(6);
+    T r;
     if (xdir) {
         const T a = (A3(tn, si + 1, sj, k) - A3(tn, si, sj, k)) * UMASK(T, si, sj, k);
         const T b = (A3(tn, si, sj, k) - A3(tn, si - 1, sj, k)) * UMASK(T, si - 1, sj, k);
-        return (a + b) * r1_6;
+        r = (a + b) * r1_6;
     } else {
         const T a = (A3(tn, si, sj + 1, k) - A3(tn, si, sj, k)) * VMASK(T, si, sj, k);
         const T b = (A3(tn, si, sj, k) - A3(tn, si, sj - 1, k)) * VMASK(T, si, sj - 1, k);
-        return (a + b) * r1_6;
+        r = (a + b) * r1_6;
     }
+    return ok ? r : T(0);
 }
 
 template <typename T>
@@ -233,37 +241,45 @@ __device__ __forceinline__ double zbdo(const DevDomain &d, const T *pbef, const 
     const double b = (double)paft[p] * tm + zbig * (1.0 - tm);
     return a < b ? a : b;
 }
-template <typename T>
+template <typename T, int KB>
 KERNEL void k_fct_beta(DevDomain d, double p2dt, const T * __restrict__ pbef, const T * __restrict__ paft, const T * __restrict__ paa, const T * __restrict__ pbb,
                            const T * __restrict__ pcc, T * __restrict__ zbetup, T * __restrict__ zbetdo)
 {
-    // one thread per interior cell, k = blockIdx.z + 1 (no recurrence in k)
+    // one thread per interior cell column and block of KB levels
     THREAD_IJ(2, 2);
-    const int k = (int)blockIdx.z + 1;
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    const int jpk = d.jpk;
+    const int kb0 = (int)blockIdx.z * KB + 1;
+    const int kb1 = kb0 + KB - 1 < jpk - 1 ? kb0 + KB - 1 : jpk - 1;
     const double zrtrn = 1.e-15;
     const size_t ij = IJ(i, j);
     const double e1e2t = d.md.e1e2t[ij];
-    const int jpk = d.jpk;
-    {
-        const int ikm1 = k - 1 > 1 ? k - 1 : 1;
-        double zup = zbup(d, pbef, paft, i, j, k);
+    // the bounds of the column itself at the levels above, at and below k
+    // (above level 1 is level 1 itself, as ikm1 = MAX(1, jk-1))
+    double up_m = zbup(d, pbef, paft, i, j, kb0 > 1 ? kb0 - 1 : 1);
+    double do_m = zbdo(d, pbef, paft, i, j, kb0 > 1 ? kb0 - 1 : 1);
+    double up_c = zbup(d, pbef, paft, i, j, kb0), do_c = zbdo(d, pbef, paft, i, j, kb0);
+    double c_k = A3(pcc, i, j, kb0);
+#pragma unroll 1
+    for (int k = kb0; k <= kb1; k++) {
+        const double up_p = zbup(d, pbef, paft, i, j, k + 1), do_p = zbdo(d, pbef, paft, i, j, k + 1);
+        double zup = up_c;
         zup = fmax2(zup, zbup(d, pbef, paft, i - 1, j, k));
         zup = fmax2(zup, zbup(d, pbef, paft, i + 1, j, k));
         zup = fmax2(zup, zbup(d, pbef, paft, i, j - 1, k));
         zup = fmax2(zup, zbup(d, pbef, paft, i, j + 1, k));
-        zup = fmax2(zup, zbup(d, pbef, paft, i, j, ikm1));
-        zup = fmax2(zup, zbup(d, pbef, paft, i, j, k + 1));
-        double zdo = zbdo(d, pbef, paft, i, j, k);
+        zup = fmax2(zup, up_m);
+        zup = fmax2(zup, up_p);
+        double zdo = do_c;
         zdo = fmin2(zdo, zbdo(d, pbef, paft, i - 1, j, k));
         zdo = fmin2(zdo, zbdo(d, pbef, paft, i + 1, j, k));
         zdo = fmin2(zdo, zbdo(d, pbef, paft, i, j - 1, k));
         zdo = fmin2(zdo, zbdo(d, pbef, paft, i, j + 1, k));
-        zdo = fmin2(zdo, zbdo(d, pbef, paft, i, j, ikm1));
-        zdo = fmin2(zdo, zbdo(d, pbef, paft, i, j, k + 1));
+        zdo = fmin2(zdo, do_m);
+        zdo = fmin2(zdo, do_p);
         const double a_im1 = A3(paa, i - 1, j, k), a_i = A3(paa, i, j, k);
         const double b_jm1 = A3(pbb, i, j - 1, k), b_j = A3(pbb, i, j, k);
-        const double c_kp1 = A3(pcc, i, j, k + 1), c_k = A3(pcc, i, j, k);
+        const double c_kp1 = A3(pcc, i, j, k + 1);
         const double zpos = fmax2(0.0, a_im1) - fmin2(0.0, a_i) + fmax2(0.0, b_jm1) - fmin2(0.0, b_j) +
                             fmax2(0.0, c_kp1) - fmin2(0.0, c_k);
         const double zneg = fmax2(0.0, a_i) - fmin2(0.0, a_im1) + fmax2(0.0, b_j) - fmin2(0.0, b_jm1) +
@@ -281,10 +297,15 @@ KERNEL void k_fct_beta(DevDomain d, double p2dt, const T * __restrict__ pbef, co
             A3(zbetup, i, j, k) = (T)bu;
             A3(zbetdo, i, j, k) = (T)bd;
         }
-    }
-    if (k == jpk - 1) {
-        A3(zbetup, i, j, jpk) = T(0);
-        A3(zbetdo, i, j, jpk) = T(0);
+        if (k == jpk - 1) {
+            A3(zbetup, i, j, jpk) = T(0);
+            A3(zbetdo, i, j, jpk) = T(0);
+        }
+        up_m = up_c;
+        up_c = up_p;
+        do_m = do_c;
+        do_c = do_p;
+        c_k = c_kp1;
     }
 }
 
@@ -303,17 +324,19 @@ template <typename T>
 __device__ __forceinline__ double lim_x(const DevDomain &d, const T *paa, const T *bup, const T *bdo, int p, int q, int k)
 {
     int sp, sq;
-    if (!lbc_src(d, 'U', p, q, sp, sq)) return 0.0;
-    return lim_face<T>(A3(paa, sp, sq, k), A3(bdo, sp, sq, k), A3(bup, sp + 1, sq, k), A3(bup, sp, sq, k),
-                       A3(bdo, sp + 1, sq, k));
+    const bool ok = lbc_src_safe(d, 'U', p, q, sp, sq);
+    const double r = lim_face<T>(A3(paa, sp, sq, k), A3(bdo, sp, sq, k), A3(bup, sp + 1, sq, k), A3(bup, sp, sq, k),
+                                 A3(bdo, sp + 1, sq, k));
+    return ok ? r : 0.0;
 }
 template <typename T>
 __device__ __forceinline__ double lim_y(const DevDomain &d, const T *pbb, const T *bup, const T *bdo, int p, int q, int k)
 {
     int sp, sq;
-    if (!lbc_src(d, 'V', p, q, sp, sq)) return 0.0;
-    return lim_face<T>(A3(pbb, sp, sq, k), A3(bdo, sp, sq, k), A3(bup, sp, sq + 1, k), A3(bup, sp, sq, k),
-                       A3(bdo, sp, sq + 1, k));
+    const bool ok = lbc_src_safe(d, 'V', p, q, sp, sq);
+    const double r = lim_face<T>(A3(pbb, sp, sq, k), A3(bdo, sp, sq, k), A3(bup, sp, sq + 1, k), A3(bup, sp, sq, k),
+                                 A3(bdo, sp, sq + 1, k));
+    return ok ? r : 0.0;
 }
 // pcc at level kk (2..jpk) after limiting: set by the k = kk-1 iteration
 template <typename T>
@@ -329,19 +352,22 @@ __device__ __forceinline__ double lim_z(const DevDomain &d, const T *pcc, const 
 template <typename T>
 KERNEL void k_fct_final(DevDomain d, const T * __restrict__ paa, const T * __restrict__ pbb, const T * __restrict__ pcc, const T * __restrict__ bup, const T * __restrict__ bdo, T * __restrict__ ta)
 {
-    // one thread per interior cell, k = blockIdx.z + 1
+    // one thread per interior column, k-marched: the limited vertical flux of
+    // the lower face is carried to the next level
     THREAD_IJ(2, 2);
-    const int k = (int)blockIdx.z + 1;
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
     const Metrics<T> &m = met<T>(d);
     const size_t ij = IJ(i, j);
     const T r1_e1e2t = m.r1_e1e2t[ij];
-    const double lz_k = lim_z(d, pcc, bup, bdo, i, j, k);
-    const double lz_kp1 = lim_z(d, pcc, bup, bdo, i, j, k + 1);
-    const T lx_i = (T)lim_x(d, paa, bup, bdo, i, j, k), lx_im1 = (T)lim_x(d, paa, bup, bdo, i - 1, j, k);
-    const T ly_j = (T)lim_y(d, pbb, bup, bdo, i, j, k), ly_jm1 = (T)lim_y(d, pbb, bup, bdo, i, j - 1, k);
-    const size_t p = IJK(i, j, k);
-    ta[p] = ta[p] - (lx_i - lx_im1 + ly_j - ly_jm1 + (T)lz_k - (T)lz_kp1) * r1_e1e2t / E3(m.e3t, ij, k);
+    double lz_k = lim_z(d, pcc, bup, bdo, i, j, 1);
+    for (int k = 1; k <= d.jpk - 1; k++) {
+        const double lz_kp1 = lim_z(d, pcc, bup, bdo, i, j, k + 1);
+        const T lx_i = (T)lim_x(d, paa, bup, bdo, i, j, k), lx_im1 = (T)lim_x(d, paa, bup, bdo, i - 1, j, k);
+        const T ly_j = (T)lim_y(d, pbb, bup, bdo, i, j, k), ly_jm1 = (T)lim_y(d, pbb, bup, bdo, i, j - 1, k);
+        const size_t p = IJK(i, j, k);
+        ta[p] = ta[p] - (lx_i - lx_im1 + ly_j - ly_jm1 + (T)lz_k - (T)lz_kp1) * r1_e1e2t / E3(m.e3t, ij, k);
+        lz_k = lz_kp1;
+    }
 }
 
 template <typename T>
@@ -356,7 +382,7 @@ int launch_tra_adv_fct(double p2dt, const T *pun, const T *pvn, const T *pwn, co
     double *zwt = (double *)nh::scratch3(nh::S_ZTW);
     Grid2 gi = grid2(d.jpi - 2, d.jpj - 2), gh = grid2(d.jpi - 1, d.jpj - 1);
     Grid2 gk = gi;
-    gk.grid.z = (unsigned)(d.jpk - 1);
+    gk.grid.z = (unsigned)((d.jpk - 1 + FctKB<T>::beta - 1) / FctKB<T>::beta);
     int st;
     for (int jn = 0; jn < kjpt; jn++) {
         const T *tb = ptb + (size_t)jn * d.n3, *tn = ptn + (size_t)jn * d.n3;
@@ -367,10 +393,10 @@ int launch_tra_adv_fct(double p2dt, const T *pun, const T *pvn, const T *pwn, co
         if ((st = launch_lbc<T>(zwx, d.jpk, 'U', T(0)))) return st;
         if ((st = launch_lbc<T>(zwy, d.jpk, 'V', T(0)))) return st;
         if ((st = launch_lbc<T>(zwz, d.jpk, 'W', T(0)))) return st;
-        LAUNCH("tra_adv_fct", (k_fct_beta<T>), gk, d, p2dt, tb, zwi, zwx, zwy, zwz, zbu, zbd);
+        LAUNCH("tra_adv_fct", (k_fct_beta<T, FctKB<T>::beta>), gk, d, p2dt, tb, zwi, zwx, zwy, zwz, zbu, zbd);
         if ((st = launch_lbc<T>(zbu, d.jpk, 'T', T(0)))) return st;
         if ((st = launch_lbc<T>(zbd, d.jpk, 'T', T(0)))) return st;
-        LAUNCH("tra_adv_fct", (k_fct_final<T>), gk, d, zwx, zwy, zwz, zbu, zbd, ta);
+        LAUNCH("tra_adv_fct", (k_fct_final<T>), gi, d, zwx, zwy, zwz, zbu, zbd, ta);
     }
     return 0;
 }
@@ -690,12 +716,15 @@ static int launch_iso_mode(const DevDomain &d, T zsign, const T *pahu, const T *
 {
     Grid2 g = grid2(d.jpi - 2, d.jpj - 2);
     int jn = 0;
-    for (; jn + 2 <= kjpt; jn += 2) {
+    // FP64: one tracer per pass (the two-tracer window needs 255+ VGPRs and
+    // runs one wave per SIMD); FP32: tracer pairs share the coefficient reads
+    const int pair = sizeof(T) == 4;
+    for (; pair && jn + 2 <= kjpt; jn += 2) {
         const size_t o = (size_t)jn * d.n3;
         LAUNCH("tra_ldf_iso", (k_tra_ldf_iso<T, MODE, 2>), g, d, zsign, pahu, pahv, uslp, vslp, wslpi, wslpj,
                ah_wslp2, akz, ptb + o, ptbb + o, pta + o);
     }
-    if (jn < kjpt) {
+    for (; jn < kjpt; jn++) {
         const size_t o = (size_t)jn * d.n3;
         LAUNCH("tra_ldf_iso", (k_tra_ldf_iso<T, MODE, 1>), g, d, zsign, pahu, pahv, uslp, vslp, wslpi, wslpj,
                ah_wslp2, akz, ptb + o, ptbb + o, pta + o);
@@ -890,108 +919,57 @@ template int launch_tra_zdf_imp<float>(double, int, const float *, float *, int,
                                        const float *, int);
 
 // ===========================================================================
-// Fused tracer step: tsa = 0; tra_adv (transport + FCT); tra_ldf_iso (one
-// pass, laplacian); tra_zdf -- the order of step.F90:234-257 -- in four
-// passes over the two active tracers:
-//   k_step_tra0  first guess zwi and the forward sweep of interp_4th_cpt
-//                (column, top-down);
-//   k_step_trab  vertical antidiffusive flux (back substitution, bottom-up)
-//                and the Zalesak betas, lbc_lnk'd through lbc_src;
-//   k_step_trac  low-order trend, limited antidiffusive divergence, rotated
-//                laplacian and the forward sweep of the implicit vertical
-//                diffusion (top-down);
-//   k_step_trad  back substitution, salinity reset.
-// The transports of tra_adv are evaluated where used, the horizontal
-// antidiffusive fluxes are recomputed where used, and every value the
-// separate routines store is rounded to the storage type at the same point,
-// so the fused step equals the sequence of separate calls bit for bit in
-// either storage form.  The pivots of interp_4th_cpt do not depend on the
-// data: rows 3..mbkt-1 follow p(k) = 4 - 1/p(k-1) from p(3) = 4, every other
-// row has pivot 1, so they come from a host table.
+// FCT kernels shared by tra_adv_fct and the fused tracer step, and the fused
+// tracer step itself: tsa = 0; tra_adv (transports + FCT); tra_ldf_iso (one
+// pass, laplacian); tra_zdf -- the order of step.F90:234-257.  Both tracers
+// of a pair go through each kernel together:
+//   k_fct_h      per cell: horizontal upstream and antidiffusive fluxes,
+//                low-order trend, first guess zwi;
+//   k_fct_v      per column: vertical upstream flux, compact interpolation
+//                and vertical antidiffusive flux zwz;
+//   k_lbc        lbc_lnk of zwi, zwx, zwy, zwz;
+//   k_fct_beta   per cell: the Zalesak betas (nonosc), then k_lbc;
+//   k_fct_final  per column: limited antidiffusive divergence;
+// then k_ldf_iso_coef / k_tra_ldf_iso and k_tra_zdf.  In the fused step the
+// transports of tra_adv are formed where used and the trend starts from zero
+// without a fill pass; every value the separate routines store is rounded to
+// the storage type at the same point, so the fused step equals the sequence
+// of separate calls bit for bit in either storage form.  The pivots of
+// interp_4th_cpt do not depend on the data: rows 3..mbkt-1 follow
+// p(k) = 4 - 1/p(k-1) from p(3) = 4, every other row has pivot 1, so they
+// come from a host table.
 // ===========================================================================
 template <typename T>
 struct PivotTab {
     T p[STEP_MAXK + 2];
 };
 
+// (k is uniform across a wave: the level tests select, they do not branch,
+// so the loads of a stencil can be issued together)
 template <typename T>
 __device__ __forceinline__ T trp_u(const DevDomain &d, const T *un, int p, int q, int k)
 {
-    if (k >= d.jpk) return T(0);
     const Metrics<T> &m = met<T>(d);
     const size_t pq = IJ(p, q);
-    return m.e2u[pq] * E3(m.e3u, pq, k) * A3(un, p, q, k);
+    const int kc = k < d.jpk ? k : d.jpk;
+    const T v = m.e2u[pq] * E3(m.e3u, pq, kc) * A3(un, p, q, kc);
+    return k >= d.jpk ? T(0) : v;
 }
 template <typename T>
 __device__ __forceinline__ T trp_v(const DevDomain &d, const T *vn, int p, int q, int k)
 {
-    if (k >= d.jpk) return T(0);
     const Metrics<T> &m = met<T>(d);
     const size_t pq = IJ(p, q);
-    return m.e1v[pq] * E3(m.e3v, pq, k) * A3(vn, p, q, k);
+    const int kc = k < d.jpk ? k : d.jpk;
+    const T v = m.e1v[pq] * E3(m.e3v, pq, kc) * A3(vn, p, q, kc);
+    return k >= d.jpk ? T(0) : v;
 }
 template <typename T>
 __device__ __forceinline__ T trp_w(const DevDomain &d, const T *wn, int p, int q, int k)
 {
-    if (k >= d.jpk) return T(0);
-    return met<T>(d).e1e2t[IJ(p, q)] * A3(wn, p, q, k);
-}
-template <typename T>
-__device__ __forceinline__ T fs_upx(const DevDomain &d, const T *un, const T *tb, int i, int j, int k)
-{
-    const T u = trp_u(d, un, i, j, k);
-    const T zfp = u + fabs(u), zfm = u - fabs(u);
-    return T(0.5) * (zfp * A3(tb, i, j, k) + zfm * A3(tb, i + 1, j, k));
-}
-template <typename T>
-__device__ __forceinline__ T fs_upy(const DevDomain &d, const T *vn, const T *tb, int i, int j, int k)
-{
-    const T v = trp_v(d, vn, i, j, k);
-    const T zfp = v + fabs(v), zfm = v - fabs(v);
-    return T(0.5) * (zfp * A3(tb, i, j, k) + zfm * A3(tb, i, j + 1, k));
-}
-template <typename T>
-__device__ __forceinline__ T fs_upz(const DevDomain &d, const T *wn, const T *tb, int i, int j, int k)
-{
-    if (k == 1) return trp_w(d, wn, i, j, 1) * A3(tb, i, j, 1);
-    if (k >= d.jpk) return T(0);
-    const T w = trp_w(d, wn, i, j, k);
-    const T zfp = w + fabs(w), zfm = w - fabs(w);
-    return T(0.5) * (zfp * A3(tb, i, j, k) + zfm * A3(tb, i, j, k - 1)) * WMASK(T, i, j, k);
-}
-// horizontal antidiffusive fluxes of k_fct_ad at (p,q) before lbc_lnk
-template <typename T, int NH4>
-__device__ __forceinline__ T fs_zwx(const DevDomain &d, const T *un, const T *tb, const T *tn, int p, int q, int k)
-{
-    const T hu = fs_upx(d, un, tb, p, q, k);
-    const T c2u = A3(tn, p, q, k) + A3(tn, p + 1, q, k);
-    const T u = trp_u(d, un, p, q, k);
-    if (NH4) return T(0.5) * u * (c2u + fct_zltu(d, tn, p, q, k, true) - fct_zltu(d, tn, p + 1, q, k, true)) - hu;
-    return T(0.5) * u * c2u - hu;
-}
-template <typename T, int NH4>
-__device__ __forceinline__ T fs_zwy(const DevDomain &d, const T *vn, const T *tb, const T *tn, int p, int q, int k)
-{
-    const T hv = fs_upy(d, vn, tb, p, q, k);
-    const T c2v = A3(tn, p, q, k) + A3(tn, p, q + 1, k);
-    const T v = trp_v(d, vn, p, q, k);
-    if (NH4) return T(0.5) * v * (c2v + fct_zltu(d, tn, p, q, k, false) - fct_zltu(d, tn, p, q + 1, k, false)) - hv;
-    return T(0.5) * v * c2v - hv;
-}
-// ... after lbc_lnk('U') / ('V')
-template <typename T, int NH4>
-__device__ __forceinline__ T fs_zwx_l(const DevDomain &d, const T *un, const T *tb, const T *tn, int p, int q, int k)
-{
-    int sp, sq;
-    if (!lbc_src(d, 'U', p, q, sp, sq)) return T(0);
-    return fs_zwx<T, NH4>(d, un, tb, tn, sp, sq, k);
-}
-template <typename T, int NH4>
-__device__ __forceinline__ T fs_zwy_l(const DevDomain &d, const T *vn, const T *tb, const T *tn, int p, int q, int k)
-{
-    int sp, sq;
-    if (!lbc_src(d, 'V', p, q, sp, sq)) return T(0);
-    return fs_zwy<T, NH4>(d, vn, tb, tn, sp, sq, k);
+    const int kc = k < d.jpk ? k : d.jpk;
+    const T v = met<T>(d).e1e2t[IJ(p, q)] * A3(wn, p, q, kc);
+    return k >= d.jpk ? T(0) : v;
 }
 // interp_4th_cpt pivot of row k in a column whose last wet level is ikb
 template <typename T>
@@ -1023,34 +1001,19 @@ __device__ __forceinline__ T tr_w(const DevDomain &d, const T *pw, int p, int q,
     if (TRP) return trp_w(d, pw, p, q, k);
     return A3(pw, p, q, k);
 }
-// upstream fluxes on the transport provider
-template <int TRP, typename T>
-__device__ __forceinline__ T tp_upx(const DevDomain &d, const T *pu, T tb_i, T tb_ip1, int i, int j, int k)
-{
-    const T u = tr_u<TRP>(d, pu, i, j, k);
-    const T zfp = u + fabs(u), zfm = u - fabs(u);
-    return T(0.5) * (zfp * tb_i + zfm * tb_ip1);
-}
 template <int TRP, typename T>
 __device__ __forceinline__ T tp_upz(const DevDomain &d, const T *pw, const T *tb, int i, int j, int k)
 {
-    if (k == 1) return tr_w<TRP>(d, pw, i, j, 1) * A3(tb, i, j, 1);
-    if (k >= d.jpk) return T(0);
-    const T w = tr_w<TRP>(d, pw, i, j, k);
+    // k == 1: linear free surface flux; k >= jpk: zero (selects on the uniform k)
+    const int kc = k < d.jpk ? k : d.jpk, km1 = kc > 1 ? kc - 1 : 1;
+    const T w = tr_w<TRP>(d, pw, i, j, kc);
+    const T tbk = A3(tb, i, j, kc);
     const T zfp = w + fabs(w), zfm = w - fabs(w);
-    return T(0.5) * (zfp * A3(tb, i, j, k) + zfm * A3(tb, i, j, k - 1)) * WMASK(T, i, j, k);
+    const T gen = T(0.5) * (zfp * tbk + zfm * A3(tb, i, j, km1)) * WMASK(T, i, j, kc);
+    const T sfc = w * tbk;
+    return k == 1 ? sfc : (k >= d.jpk ? T(0) : gen);
 }
 
-// ---------------------------------------------------------------------------
-// FCT for pairs of tracers.  One column pass (k_fct_pre) replaces the first
-// two loop groups of tra_adv_fct: per level it forms the horizontal
-// antidiffusive fluxes of the column's faces (traadv_fct.F90:180-221), the
-// low-order trend and first guess zwi (158-170) and the vertical
-// antidiffusive flux zwz (255-278), whose compact-scheme sweeps (577-614) take
-// their pivots from the data-independent table.  lbc_lnk, the Zalesak betas
-// (k_fct_beta) and the limited divergence (k_fct_final) follow as in the
-// reference.
-// ---------------------------------------------------------------------------
 // per cell (one thread per (i,j,k), k = blockIdx.z + 1): the horizontal
 // fluxes of the (i+1/2, j) and (i, j+1/2) faces (upstream and antidiffusive,
 // lines 123-145, 180-221) and, on the interior, the low-order trend and the
@@ -1188,7 +1151,7 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
     T *zwz = (T *)nh::scratch3(6), *bu = (T *)nh::scratch3(8), *bd = (T *)nh::scratch3(10);
     const char *nm = TRP ? "step_tra" : "tra_adv_fct";
     Grid2 gi = grid2(d.jpi - 2, d.jpj - 2), gk = gi;
-    gk.grid.z = (unsigned)(d.jpk - 1);
+    gk.grid.z = (unsigned)((d.jpk - 1 + FctKB<T>::beta - 1) / FctKB<T>::beta);
     const size_t n3 = d.n3;
     int st;
     for (int t0 = 0; t0 < kjpt; t0 += 2) {
@@ -1213,13 +1176,13 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
         if ((st = launch_lbc<T>(zwz, nk, 'W', T(0)))) return st;
         for (int t = 0; t < nt; t++) {
             const size_t q = (size_t)t * n3;
-            LAUNCH(nm, (k_fct_beta<T>), gk, d, p2dt, ptb + o + q, zwi + q, zwx + q, zwy + q, zwz + q, bu + q, bd + q);
+            LAUNCH(nm, (k_fct_beta<T, FctKB<T>::beta>), gk, d, p2dt, ptb + o + q, zwi + q, zwx + q, zwy + q, zwz + q, bu + q, bd + q);
         }
         if ((st = launch_lbc<T>(bu, nk, 'T', T(0)))) return st;
         if ((st = launch_lbc<T>(bd, nk, 'T', T(0)))) return st;
         for (int t = 0; t < nt; t++) {
             const size_t q = (size_t)t * n3;
-            LAUNCH(nm, (k_fct_final<T>), gk, d, zwx + q, zwy + q, zwz + q, bu + q, bd + q, pta + o + q);
+            LAUNCH(nm, (k_fct_final<T>), gi, d, zwx + q, zwy + q, zwz + q, bu + q, bd + q, pta + o + q);
         }
     }
     if (ta0) {
@@ -1229,54 +1192,6 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
     return 0;
 }
 
-// pointwise forms of the rotated-laplacian intermediates (traldf_iso.F90:218-276)
-template <typename T>
-__device__ __forceinline__ T iso_zdk1t(const DevDomain &d, const T *tb, int i, int j, int k)
-{
-    return (A3(tb, i, j, k) - A3(tb, i, j, k + 1)) * WMASK(T, i, j, k + 1);
-}
-template <typename T>
-__device__ __forceinline__ T iso_zdkt(const DevDomain &d, const T *tb, int i, int j, int k)
-{
-    if (k == 1) return iso_zdk1t(d, tb, i, j, 1);
-    return (A3(tb, i, j, k - 1) - A3(tb, i, j, k)) * WMASK(T, i, j, k);
-}
-template <typename T>
-__device__ __forceinline__ T iso_zdit(const DevDomain &d, const T *tb, int i, int j, int k)
-{
-    return (A3(tb, i + 1, j, k) - A3(tb, i, j, k)) * UMASK(T, i, j, k);
-}
-template <typename T>
-__device__ __forceinline__ T iso_zdjt(const DevDomain &d, const T *tb, int i, int j, int k)
-{
-    return (A3(tb, i, j + 1, k) - A3(tb, i, j, k)) * VMASK(T, i, j, k);
-}
-template <typename T>
-__device__ __forceinline__ T iso_zftu(const DevDomain &d, const T *pahu, const T *uslp, const T *tb, int p, int q, int k)
-{
-    const Metrics<T> &m = met<T>(d);
-    const size_t pq = IJ(p, q);
-    const T zabe1 = A3(pahu, p, q, k) * m.e2_e1u[pq] * E3(m.e3u, pq, k);
-    const T zmsku = T(1) / fmax2(WMASK(T, p + 1, q, k) + WMASK(T, p, q, k + 1) + WMASK(T, p + 1, q, k + 1) +
-                                     WMASK(T, p, q, k), T(1));
-    const T zcof1 = -(A3(pahu, p, q, k) * m.e2u[pq] * A3(uslp, p, q, k) * zmsku);
-    return (zabe1 * iso_zdit(d, tb, p, q, k) + zcof1 * (iso_zdkt(d, tb, p + 1, q, k) + iso_zdk1t(d, tb, p, q, k) +
-                                                        iso_zdk1t(d, tb, p + 1, q, k) + iso_zdkt(d, tb, p, q, k))) *
-           UMASK(T, p, q, k);
-}
-template <typename T>
-__device__ __forceinline__ T iso_zftv(const DevDomain &d, const T *pahv, const T *vslp, const T *tb, int p, int q, int k)
-{
-    const Metrics<T> &m = met<T>(d);
-    const size_t pq = IJ(p, q);
-    const T zabe2 = A3(pahv, p, q, k) * m.e1_e2v[pq] * E3(m.e3v, pq, k);
-    const T zmskv = T(1) / fmax2(WMASK(T, p, q + 1, k) + WMASK(T, p, q, k + 1) + WMASK(T, p, q + 1, k + 1) +
-                                     WMASK(T, p, q, k), T(1));
-    const T zcof2 = -(A3(pahv, p, q, k) * m.e1v[pq] * A3(vslp, p, q, k) * zmskv);
-    return (zabe2 * iso_zdjt(d, tb, p, q, k) + zcof2 * (iso_zdkt(d, tb, p, q + 1, k) + iso_zdk1t(d, tb, p, q, k) +
-                                                        iso_zdk1t(d, tb, p, q + 1, k) + iso_zdkt(d, tb, p, q, k))) *
-           VMASK(T, p, q, k);
-}
 
 template <typename T>
 static PivotTab<T> cpt_pivots(int jpk)

@@ -104,6 +104,10 @@ class NemoHip:
     def timing_enable(self, on=True):
         self._check(self.L.nemo_hip_timing_enable(C.c_int(1 if on else 0)))
 
+    def timing_only(self, key=None):
+        """Time only the launches of one kernel, keyed "routine/kernel"; None times every launch."""
+        self._check(self.L.nemo_hip_timing_only(key.encode() if key else None))
+
     def timing_reset(self):
         self._check(self.L.nemo_hip_timing_reset())
 
