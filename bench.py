@@ -342,7 +342,7 @@ def main():
     if live and live["launches"] > 0:
         kname = live["kernel"]
         avg_ms = live["total_ms"] / live["launches"]
-        alg_bytes = KERNEL_BYTES[kname.split("/")[1]](width, jpi, jpj, jpk)
+        alg_bytes = KERNEL_BYTES[kname.split("/")[1]](width, jpi, jpj, jpk, head)
         roof = {"kernel": kname, "bound": "hbm", "achieved": alg_bytes / (avg_ms * 1e-3) / 1e9,
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": alg_bytes / (avg_ms * 1e-3) / HBM_PEAK,
                 "traffic": traffic_per_launch(kname, head, jpi, jpj, jpk), "avg_launch_ms": avg_ms,
@@ -376,16 +376,30 @@ def main():
     print(json.dumps(line))
 
 
-# algorithmic bytes per launch of the kernels the roofline line may name
-# (width = bytes per stored value; each 3D array the kernel's arithmetic reads or
-# writes counts once per interior cell, masks and temporaries count 0)
+# Algorithmic bytes per launch of each library kernel, for the roofline of the
+# dominant one and for the traffic table: every 3D array the kernel reads or
+# writes, its inputs and outputs between kernels included, counts once per
+# interior cell at storage width w (masks, 1-D e3 tables and 2D metrics 0);
+# the barotropic sub-steps count their 2D FP64 arrays.  Arguments: w, jpi,
+# jpj, jpk and the form ("fp64", "fp32", "fp64_fused", "fp32_fused").
 def _b(n_arrays_interior):
-    return lambda w, i, j, k: n_arrays_interior * w * (i - 2) * (j - 2) * (k - 1)
+    return lambda w, i, j, k, f: n_arrays_interior * w * (i - 2) * (j - 2) * (k - 1)
+
+
+def _fused(f):
+    return f.endswith("_fused")
+
+
+def _iso_nt(f):
+    # tra_ldf_iso runs one tracer per launch in FP64 storage, both in FP32
+    return 1 if f.startswith("fp64") else 2
 
 
 KERNEL_BYTES = {
     "k_tra_zdf": _b(8),          # r avt ah_wslp2 tb*2 ta*2, w ta*2
-    "k_tra_ldf_iso": _b(11),     # r ahtu ahtv uslp vslp wslpi wslpj ah_wslp2 akz tb ta, w ta (one tracer)
+    # r ahtu ahtv uslp vslp wslpi wslpj ah_wslp2 akz + (tb, ta) per tracer, w ta per tracer
+    "k_tra_ldf_iso": lambda w, i, j, k, f: (8 + 3 * _iso_nt(f)) * w * (i - 2) * (j - 2) * (k - 1),
+    "k_ldf_iso_coef": _b(5),     # r ahtu ahtv wslpi wslpj, w ah_wslp2
     "k_dyn_zdf": _b(7),          # r ub vb ua va avm, w ua va
     "k_dyn_vor": _b(6),          # r un vn ua va, w ua va
     "k_dyn_keg_zad": _b(7),      # r un vn wn ua va, w ua va
@@ -393,23 +407,23 @@ KERNEL_BYTES = {
     "k_fct_beta": _b(7),         # r tb zwi zwx zwy zwz, w zbetup zbetdo
     "k_fct_ad": _b(8),           # r pun pvn pwn tb tn, w zwx zwy zwz
     "k_fct_low": _b(7),          # r pun pvn pwn tb ta, w ta zwi
-    "k_spg_p1a": _b(6),          # r ua va un vn (bottom), w ua va
+    "k_spg_p1a": _b(6),          # r ua va un vn, w ua va
     "k_spg_p3b": _b(8),          # r ua va un vn, w ua va un vn
-    "k_tra_adv_trp": lambda w, i, j, k: 6 * w * i * j * (k - 1),
-    # FCT first passes (two tracers per launch): the routine's own fields;
-    # its temporaries (zwi, zwx, zwy, zwz), as in the per-routine accounting,
-    # are not counted
-    "k_fct_h": _b(11),           # r pun pvn pwn tb*2 tn*2, r+w ta*2 (w only in the fused step)
-    "k_fct_v": _b(5),            # r pwn tb*2 tn*2
+    "k_tra_adv_trp": lambda w, i, j, k, f: 6 * w * i * j * (k - 1),   # r un vn wn, w zun zvn zwn
+    # FCT flux pass, two tracers per launch: r pun pvn pwn tb*2 tn*2 ta*2,
+    # w ta*2 zwi*2 zwx*2 zwy*2; in the fused step the transports are formed
+    # from un vn wn and the trend starts from zero (no ta read)
+    "k_fct_h": lambda w, i, j, k, f: (15 if _fused(f) else 17) * w * (i - 2) * (j - 2) * (k - 1),
+    "k_fct_v": _b(7),            # r pwn tb*2 tn*2, w zwz*2
     # barotropic sub-step kernels (2D, FP64 in both forms; ENE scheme)
     # S3: reads zwx zwy zwz zCdU_u zCdU_v un_e vn_e zu_frc zv_frc zsshp2_e
     # ub_e(-1) vb_e(-1) hu_n hv_n r1_hu_n r1_hv_n r1_e1u r1_e2v e2u e1v r1_e2u
     # r1_e1v and the sums ua_b va_b un_adv vn_adv; writes ua_e va_e zwx' zwy'
     # and the four sums: 26 + 8 arrays
-    "k_spg_s3": lambda w, i, j, k: 34 * 8 * i * j,
+    "k_spg_s3": lambda w, i, j, k, f: 34 * 8 * i * j,
     # S2: reads zwx zwy zssh_frc r1_e1e2t sshn_e sshb_e sshbb_e ssha; writes
     # ssha_e ssha zsshp2_e: 8 + 3 arrays
-    "k_spg_s2": lambda w, i, j, k: 11 * 8 * i * j,
+    "k_spg_s2": lambda w, i, j, k, f: 11 * 8 * i * j,
     # fused momentum step
     "k_step_dyn1": _b(5),        # r un vn wn, w ua va
     "k_step_dyn2": _b(11),       # r ua va un vn ub vb avm, w ua va un vn

@@ -32,38 +32,31 @@ static double *g_scratch2[NH_NSCRATCH2] = {};
 static double g_r2dt_tra, g_r2dt_zdf, g_r2dt_dyn;   // module r2dt of tra_adv / tra_zdf / dyn_zdf
 // staging pool (host-pointer mode)
 static std::vector<std::pair<void *, size_t>> g_pool;
-// timing
+// timing: every timed launch records its own start and end events (taken by
+// the dispatch packet, so launch gaps are not counted) and adds the elapsed
+// time to the routine's total and to the "routine/kernel" total
 struct Pending {
-    std::string name;
+    std::string name, key;
     hipEvent_t a, b;
 };
 static bool g_timing = false;
 static std::vector<Pending> g_pend;
 static std::vector<hipEvent_t> g_evfree;
 static std::map<std::string, std::pair<double, int64_t>> g_tot;
-static std::string g_open_name;
-static hipEvent_t g_open_ev = nullptr;
 static std::string g_only;   // when set, only launches with this "routine/kernel" key are timed
 
-// accumulate completed (start, end) pairs and recycle their events; a start
-// event shared by two entries is recycled once
 static void release_pending()
 {
-    std::vector<hipEvent_t> seen;
     for (auto &p : g_pend) {
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, p.a, p.b);
-        auto &t = g_tot[p.name];
-        t.first += ms;
-        t.second += 1;
-        bool dup = false;
-        for (auto e : seen) dup = dup || (e == p.a);
-        if (!dup) {
-            seen.push_back(p.a);
-            g_evfree.push_back(p.a);
+        for (const std::string *n : {&p.name, &p.key}) {
+            auto &t = g_tot[*n];
+            t.first += ms;
+            t.second += 1;
         }
+        g_evfree.push_back(p.a);
         g_evfree.push_back(p.b);
-        if (seen.size() > 8) seen.erase(seen.begin());
     }
     g_pend.clear();
 }
@@ -107,32 +100,22 @@ static std::string kernel_key(const char *name, const char *kernel)
     }
     return std::string(name) + "/" + k;
 }
-void timing_begin(const char *name, const char *kernel)
+bool timing_pair(const char *name, const char *kernel, hipEvent_t *start, hipEvent_t *stop)
 {
-    if (!g_timing) return;
-    if (!g_only.empty() && kernel_key(name, kernel) != g_only) return;
-    g_open_ev = ev_get();
-    g_open_name = name;
-    if (g_open_ev) (void)hipEventRecord(g_open_ev, g_stream);
-}
-void timing_end(const char *name, const char *kernel)
-{
-    if (!g_timing || !g_open_ev) return;
-    hipEvent_t b = ev_get();
-    hipEvent_t b2 = ev_get();
-    if (!b || !b2) return;
-    (void)hipEventRecord(b, g_stream);
-    hipEvent_t a2 = g_open_ev;
-    // the routine total and the per-kernel entry share the start event
-    g_pend.push_back({std::string(name), g_open_ev, b});
-    (void)hipEventRecord(b2, g_stream);
-    g_pend.push_back({kernel_key(name, kernel), a2, b2});
-    g_open_ev = nullptr;
-    if (g_pend.size() > 4096) {
+    if (!g_timing) return false;
+    std::string key = kernel_key(name, kernel);
+    if (!g_only.empty() && key != g_only) return false;
+    if (g_pend.size() >= 4096) {
         // fold early so the event pool stays bounded
-        (void)hipStreamSynchronize(g_stream);
+        if (hipStreamSynchronize(g_stream) != hipSuccess) return false;
         release_pending();
     }
+    hipEvent_t a = ev_get(), b = ev_get();
+    if (!a || !b) return false;
+    g_pend.push_back({std::string(name), key, a, b});
+    *start = a;
+    *stop = b;
+    return true;
 }
 static int fold_timing()
 {

@@ -18,6 +18,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <stdint.h>
 #include <stddef.h>
 #include <type_traits>
@@ -155,6 +156,13 @@ __device__ __forceinline__ T wpmask(const uint32_t *p, uint32_t ij, int k)
 #define WUMASK(T_, i, j, k) wpmask<T_, 16>(d.ktu, IJ32(i, j), k)
 #define WVMASK(T_, i, j, k) wpmask<T_, 0>(d.kvf, IJ32(i, j), k)
 
+// A column Thomas solve may keep the FP64 pivot of every ZDF_SEG-th level
+// only (a checkpoint in scratch): the back substitution then recomputes the
+// pivots of each segment in registers from the checkpoint above it with the
+// operations of the forward sweep, so they are the same values, and the
+// per-level pivot store and reload (16 bytes per cell) disappear.
+#define ZDF_SEG 4
+
 // NEMO's SIGN with key_nosignedzero semantics, lib_fortran.F90:336-351
 template <typename T>
 __device__ __forceinline__ T nsign(T a, T b) { return b >= T(0) ? fabs(a) : -fabs(a); }
@@ -253,9 +261,9 @@ int set_error(int code, const char *fmt, ...);
 hipStream_t stream();
 const DevDomain &dom();
 bool have_domain();
-// per-routine timing: begin/end bracket kernel launches under a name
-void timing_begin(const char *name, const char *kernel);
-void timing_end(const char *name, const char *kernel);
+// per-routine timing: when this launch is timed, returns true with the two
+// events its dispatch packet records at kernel start and end
+bool timing_pair(const char *name, const char *kernel, hipEvent_t *start, hipEvent_t *stop);
 // library-owned scratch buffers (HBM), sized at domain registration
 enum Scratch {
     S_ZWI, S_ZWX, S_ZWY, S_ZWZ, S_ZBETUP, S_ZBETDO, S_ZTW, S_ZWT, S_ZWT2, S_ZUN, S_ZVN, S_ZWN, S_NSCRATCH3
@@ -272,10 +280,12 @@ double *scratch2(int which);   // n2 doubles each (dyn_spg_ts work arrays), NH_N
             return nh::set_error(4, "%s:%d: %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
     } while (0)
 
-#define LAUNCH(name, kern, g, ...)                                                \
-    do {                                                                          \
-        nh::timing_begin(name, #kern);                                            \
-        hipLaunchKernelGGL(kern, (g).grid, (g).block, 0, nh::stream(), __VA_ARGS__); \
-        nh::timing_end(name, #kern);                                              \
-        HIPCHK(hipGetLastError());                                                \
+#define LAUNCH(name, kern, g, ...)                                                                    \
+    do {                                                                                              \
+        hipEvent_t ev0_, ev1_;                                                                        \
+        if (nh::timing_pair(name, #kern, &ev0_, &ev1_))                                               \
+            hipExtLaunchKernelGGL(kern, (g).grid, (g).block, 0, nh::stream(), ev0_, ev1_, 0, __VA_ARGS__); \
+        else                                                                                          \
+            hipLaunchKernelGGL(kern, (g).grid, (g).block, 0, nh::stream(), __VA_ARGS__);              \
+        HIPCHK(hipGetLastError());                                                                    \
     } while (0)

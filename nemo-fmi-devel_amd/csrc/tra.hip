@@ -798,6 +798,11 @@ KERNEL void k_tra_zdf(DevDomain d, double p2dt, const T * __restrict__ ptb, T * 
         if (l_ldfslp && k <= jpk - 1) c = c + (double)ldp(plane_ptr(add, plane, k), ij);
         return c;
     };
+    // FP64 storage keeps only checkpoint pivots (every ZDF_SEG-th level) and
+    // recomputes the others in the back substitution; with FP32 storage the
+    // kernel is latency-bound and storing every pivot is faster (measured at
+    // 1024x1024x75)
+    constexpr bool ckpt = sizeof(T) == 8;
     double prev[NT];
     double zws_prev, zwt_prev, zc_k;
     // k = 1 (zwi = 0)
@@ -807,7 +812,7 @@ KERNEL void k_tra_zdf(DevDomain d, double p2dt, const T * __restrict__ ptb, T * 
         const double zwi = -(p2dt * 0.0 / (double)E3(m.e3w, ij, 1));
         const double zws = -(p2dt * zc_kp1 / (double)E3(m.e3w, ij, 2));
         const double zwd = e3t - zwi - zws;
-        stp(zwt, ij, zwd);
+        if (!ckpt) stp(zwt, ij, zwd);
 #pragma unroll
         for (int n = 0; n < NT; n++) {
             const double v = e3t * (double)ldp(tb[n], ij) + p2dt * e3t * (double)ldp((const T *)ta[n], ij);
@@ -826,7 +831,7 @@ KERNEL void k_tra_zdf(DevDomain d, double p2dt, const T * __restrict__ ptb, T * 
         const double zws = -(p2dt * zc_kp1 / (double)E3(m.e3w, ij, k + 1));
         const double zwd = e3t - zwi - zws;
         const double zwt_k = zwd - zwi * zws_prev / zwt_prev;
-        stp(plane_ptr(zwt, plane, k), ij, zwt_k);
+        if (!ckpt || (k % ZDF_SEG == 0 && k <= jpk - 2)) stp(plane_ptr(zwt, plane, k), ij, zwt_k);
 #pragma unroll
         for (int n = 0; n < NT; n++) {
             const double zrhs = e3t * (double)ldp(plane_ptr(tb[n], plane, k), ij) +
@@ -852,18 +857,66 @@ KERNEL void k_tra_zdf(DevDomain d, double p2dt, const T * __restrict__ ptb, T * 
         }
     }
     const int t0 = d.kT.k0[ij], t1 = d.kT.k1[ij];
+    if constexpr (!ckpt) {
 #pragma unroll 4
-    for (int k = jpk - 2; k >= 1; k--) {
-        const double zws = -(p2dt * coef(k + 1) / (double)E3(m.e3w, ij, k + 1));
-        const double zwt_k = ldp((const double *)plane_ptr(zwt, plane, k), ij);
-        const double tm = (k >= t0 && k <= t1) ? 1.0 : 0.0;
+        for (int k = jpk - 2; k >= 1; k--) {
+            const double zws = -(p2dt * coef(k + 1) / (double)E3(m.e3w, ij, k + 1));
+            const double zwt_k = ldp((const double *)plane_ptr(zwt, plane, k), ij);
+            const double tm = (k >= t0 && k <= t1) ? 1.0 : 0.0;
 #pragma unroll
-        for (int n = 0; n < NT; n++) {
-            const double v = ((double)ldp((const T *)plane_ptr(ta[n], plane, k), ij) - zws * prev[n]) / zwt_k * tm;
-            prev[n] = v;
-            T out = (T)v;
-            if (jn0 + n == clamp_jn && out < T(0)) out = T(0.1);
-            stp(plane_ptr(ta[n], plane, k), ij, out);
+            for (int n = 0; n < NT; n++) {
+                const double v = ((double)ldp((const T *)plane_ptr(ta[n], plane, k), ij) - zws * prev[n]) / zwt_k * tm;
+                prev[n] = v;
+                T out = (T)v;
+                if (jn0 + n == clamp_jn && out < T(0)) out = T(0.1);
+                stp(plane_ptr(ta[n], plane, k), ij, out);
+            }
+        }
+    }
+    // levels jpk-2 .. 1 by segments of ZDF_SEG, bottom segment first; the
+    // lower diagonal of level k equals the upper one of level k-1 (same
+    // expression), so a segment restarts from its checkpoint pivot alone
+    const int kl = ckpt ? jpk - 2 : 0;
+    for (int s = (kl + ZDF_SEG - 1) / ZDF_SEG - 1; s >= 0; s--) {
+        const int k0 = s * ZDF_SEG + 1;
+        double piv[ZDF_SEG], zsv[ZDF_SEG];
+        double zc = coef(k0);
+        double zwt_p = 1.0, zws_p = 0.0;
+        if (k0 > 1) {
+            zwt_p = ldp((const double *)plane_ptr(zwt, plane, k0 - 1), ij);
+            zws_p = -(p2dt * zc / (double)E3(m.e3w, ij, k0));
+        }
+#pragma unroll
+        for (int q = 0; q < ZDF_SEG; q++) {
+            const int k = k0 + q;
+            if (k <= kl) {
+                const double zc_kp1 = coef(k + 1);
+                const double e3t = (double)E3(m.e3t, ij, k);
+                const double zwi = -(p2dt * zc / (double)E3(m.e3w, ij, k));
+                const double zws = -(p2dt * zc_kp1 / (double)E3(m.e3w, ij, k + 1));
+                const double zwd = e3t - zwi - zws;
+                piv[q] = k == 1 ? zwd : zwd - zwi * zws_p / zwt_p;
+                zsv[q] = zws;
+                zws_p = zws;
+                zwt_p = piv[q];
+                zc = zc_kp1;
+            }
+        }
+#pragma unroll
+        for (int q = ZDF_SEG - 1; q >= 0; q--) {
+            const int k = k0 + q;
+            if (k <= kl) {
+                const double tm = (k >= t0 && k <= t1) ? 1.0 : 0.0;
+#pragma unroll
+                for (int n = 0; n < NT; n++) {
+                    const double v =
+                        ((double)ldp((const T *)plane_ptr(ta[n], plane, k), ij) - zsv[q] * prev[n]) / piv[q] * tm;
+                    prev[n] = v;
+                    T out = (T)v;
+                    if (jn0 + n == clamp_jn && out < T(0)) out = T(0.1);
+                    stp(plane_ptr(ta[n], plane, k), ij, out);
+                }
+            }
         }
     }
 #pragma unroll
