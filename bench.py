@@ -274,6 +274,28 @@ def traffic_per_launch(kname, form, jpi, jpj, jpk):
     return None
 
 
+def stream_copy(mib=2048, reps=10):
+    """Device-to-device copy bandwidth (read + write bytes / time) of torch's
+    copy kernel over `mib` MiB: the measured ceiling a streaming kernel meets."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
 def copy_calibration(mib):
     """A plain copy kernel of known size, for scaling the HBM byte counters."""
     import torch
@@ -347,6 +369,10 @@ def main():
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": alg_bytes / (avg_ms * 1e-3) / HBM_PEAK,
                 "traffic": traffic_per_launch(kname, head, jpi, jpj, jpk), "avg_launch_ms": avg_ms,
                 "launches_timed": live["launches"], "algorithmic_bytes_per_launch": alg_bytes}
+    copy_gbs = stream_copy()
+    log(f"device copy (2 GiB, read + write): {copy_gbs:.0f} GB/s")
+    kt_, kb_ = d.arrays["k_top"], d.arrays["k_bot"]
+    wet_cells = int(((kb_ - kt_ + 1) * (kt_ > 0)).sum())
     cpu = None
     if not a.no_cpu:
         t0 = time.perf_counter()
@@ -365,6 +391,9 @@ def main():
                                f"+tra_zdf), {jpi}x{jpj}x{jpk}, FCT 4/4, nn_baro 30, rdt 600 s",
                    "jpi": jpi, "jpj": jpj, "jpk": jpk, "form": head},
         "hbm_frac_step": res[head]["hbm_frac_step"],
+        "wet_cells": wet_cells,
+        "wet_cell_updates_per_s": wet_cells / (h_ms * 1e-3),
+        "stream_copy_GBps": copy_gbs,
         "forms": res,
         "roofline": roof,
         "cpu_baseline": cpu,
