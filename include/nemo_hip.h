@@ -149,6 +149,19 @@ int nemo_hip_sync(void);
 /* the library's HIP stream, as an opaque handle (hipStream_t) */
 void *nemo_hip_stream(void);
 
+/* Resident fields for device mode (no counterpart routine in the reference:
+   this is the field bind / push / pull of a resident-mode caller, which keeps
+   NEMO's prognostic arrays in HBM between calls and touches host memory only
+   for I/O, restarts and diagnostics).  alloc returns a device address that
+   the *_DEVICE-mode entry points accept; push copies a host array to it and
+   returns when the copy is done; pull waits for the work queued on the
+   library stream, then copies the device array to host memory; free
+   releases an address alloc returned (others are rejected). */
+int nemo_hip_field_alloc(size_t bytes, void **dev);
+int nemo_hip_field_free(void *dev);
+int nemo_hip_field_push(void *dev, const void *host, size_t bytes);
+int nemo_hip_field_pull(void *host, const void *dev, size_t bytes);
+
 /* replaces the module initialisation done by dom_init (DOM/domain.F90) for
    the arrays the hot path reads; call again if the domain changes */
 int nemo_hip_domain_register(const nemo_hip_domain_t *dom);

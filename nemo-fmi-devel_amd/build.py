@@ -65,9 +65,27 @@ def build_fortran():
     return obj
 
 
+def build_fortran_driver(modobj):
+    """The Fortran test program tests/fortran/drive_routines: the drop-in
+    module linked against libnemo_hip.so, which it finds next to itself
+    through a relative run path."""
+    fc = shutil.which("amdflang") or shutil.which("flang")
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "fortran", "drive_routines.F90")
+    if not fc or not modobj or not os.path.exists(src):
+        return None
+    exe = os.path.join(root, "tests", "fortran", "drive_routines")
+    cmd = [fc, "-O2", "-I", BUILD, src, modobj, "-o", exe, "-L", HERE, "-lnemo_hip",
+           "-Wl,-rpath,$ORIGIN/../../" + os.path.basename(HERE)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran driver build failed:\n{r.stdout}\n{r.stderr}")
+    return exe
+
+
 def build(verbose=False):
     lib = build_hip(verbose)
-    build_fortran()
+    build_fortran_driver(build_fortran())
     return lib
 
 

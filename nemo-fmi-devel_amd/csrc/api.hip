@@ -44,6 +44,7 @@ static std::vector<Pending> g_pend;
 static std::vector<hipEvent_t> g_evfree;
 static std::map<std::string, std::pair<double, int64_t>> g_tot;
 static std::string g_only;   // when set, only launches with this "routine/kernel" key are timed
+static std::map<void *, size_t> g_fields;   // resident fields of field_alloc: address -> bytes
 
 static void release_pending()
 {
@@ -516,6 +517,8 @@ int nemo_hip_finalize(void)
     free_domain();
     for (auto &p : g_pool) if (p.first) (void)hipFree(p.first);
     g_pool.clear();
+    for (auto &f : g_fields) (void)hipFree(f.first);
+    g_fields.clear();
     release_pending();
     for (auto e : g_evfree) (void)hipEventDestroy(e);
     g_evfree.clear();
@@ -548,6 +551,63 @@ int nemo_hip_sync(void)
 }
 
 void *nemo_hip_stream(void) { return (void *)g_stream; }
+
+int nemo_hip_field_alloc(size_t bytes, void **dev)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "not initialised");
+    if (!dev) return set_error(NEMO_HIP_ERR_ARG, "field_alloc: NULL result pointer");
+    *dev = nullptr;
+    if (bytes == 0) return set_error(NEMO_HIP_ERR_ARG, "field_alloc: zero bytes");
+    void *p = nullptr;
+    HIPCHK(hipMalloc(&p, bytes));
+    g_fields[p] = bytes;
+    *dev = p;
+    return NEMO_HIP_OK;
+}
+int nemo_hip_field_free(void *dev)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "not initialised");
+    auto it = g_fields.find(dev);
+    if (it == g_fields.end()) return set_error(NEMO_HIP_ERR_ARG, "field_free: %p was not returned by field_alloc", dev);
+    HIPCHK(hipStreamSynchronize(g_stream));
+    g_fields.erase(it);
+    HIPCHK(hipFree(dev));
+    return NEMO_HIP_OK;
+}
+// the field alloc returned that contains [p, p + bytes), or an error
+static int field_check(const void *p, size_t bytes, const char *who)
+{
+    if (!p) return set_error(NEMO_HIP_ERR_ARG, "%s: NULL pointer", who);
+    auto it = g_fields.upper_bound(const_cast<void *>(p));
+    if (it == g_fields.begin()) return set_error(NEMO_HIP_ERR_ARG, "%s: %p is not a field of field_alloc", who, p);
+    --it;
+    const char *b = (const char *)it->first, *q = (const char *)p;
+    if (q >= b + it->second) return set_error(NEMO_HIP_ERR_ARG, "%s: %p is not a field of field_alloc", who, p);
+    if (q + bytes > b + it->second)
+        return set_error(NEMO_HIP_ERR_ARG, "%s: %zu bytes at %p exceed the field at %p (%zu bytes)", who, bytes, p,
+                         it->first, it->second);
+    return NEMO_HIP_OK;
+}
+int nemo_hip_field_push(void *dev, const void *host, size_t bytes)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "not initialised");
+    if (!host) return set_error(NEMO_HIP_ERR_ARG, "field_push: NULL host pointer");
+    int rc = field_check(dev, bytes, "field_push");
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return NEMO_HIP_OK;
+}
+int nemo_hip_field_pull(void *host, const void *dev, size_t bytes)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "not initialised");
+    if (!host) return set_error(NEMO_HIP_ERR_ARG, "field_pull: NULL host pointer");
+    int rc = field_check(dev, bytes, "field_pull");
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    return NEMO_HIP_OK;
+}
 
 int nemo_hip_domain_register(const nemo_hip_domain_t *dom)
 {

@@ -29,6 +29,7 @@ MODULE nemo_hip_mod
    PUBLIC :: tra_adv_fct, interp_4th_cpt, tra_ldf_lap, tra_ldf_iso, tra_zdf_imp
    PUBLIC :: dyn_keg, dyn_zad, dyn_adv_cen2, dyn_vor, dyn_zdf, dyn_spg_ts
    PUBLIC :: nemo_hip_domain_t, nemo_hip_spg_ts_t
+   PUBLIC :: nemo_hip_field_alloc, nemo_hip_field_free, nemo_hip_field_push, nemo_hip_field_pull
 
    !! mirror of nemo_hip_domain_t (same field order and kinds)
    TYPE, BIND(C) :: nemo_hip_domain_t
@@ -80,6 +81,25 @@ MODULE nemo_hip_mod
          IMPORT :: C_INT, C_CHAR
          CHARACTER(KIND=C_CHAR), DIMENSION(*) :: buf
          INTEGER(C_INT), VALUE :: len
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_field_alloc(nbytes, dev) BIND(C, name="nemo_hip_field_alloc")
+         IMPORT :: C_INT, C_SIZE_T, C_PTR
+         INTEGER(C_SIZE_T), VALUE :: nbytes
+         TYPE(C_PTR), INTENT(out) :: dev
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_field_free(dev) BIND(C, name="nemo_hip_field_free")
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: dev
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_field_push(dev, host, nbytes) BIND(C, name="nemo_hip_field_push")
+         IMPORT :: C_INT, C_SIZE_T, C_PTR
+         TYPE(C_PTR), VALUE :: dev, host
+         INTEGER(C_SIZE_T), VALUE :: nbytes
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_field_pull(host, dev, nbytes) BIND(C, name="nemo_hip_field_pull")
+         IMPORT :: C_INT, C_SIZE_T, C_PTR
+         TYPE(C_PTR), VALUE :: host, dev
+         INTEGER(C_SIZE_T), VALUE :: nbytes
       END FUNCTION
       INTEGER(C_INT) FUNCTION c_domain_register(dom) BIND(C, name="nemo_hip_domain_register")
          IMPORT :: C_INT, nemo_hip_domain_t
@@ -198,6 +218,35 @@ CONTAINS
       TYPE(nemo_hip_domain_t), INTENT(in) :: sdom
       CALL check( c_domain_register( sdom ), 'nemo_hip_domain_register' )
    END SUBROUTINE nemo_hip_domain
+
+   !!----------------------------------------------------------------------
+   !! resident fields (device mode): a device mirror per prognostic array,
+   !! pushed once, stepped in HBM, pulled for output and restarts
+   !!----------------------------------------------------------------------
+   FUNCTION nemo_hip_field_alloc( kbytes ) RESULT( pdev )
+      INTEGER(C_SIZE_T), INTENT(in) :: kbytes
+      TYPE(C_PTR) :: pdev
+      CALL check( c_field_alloc( kbytes, pdev ), 'nemo_hip_field_alloc' )
+   END FUNCTION nemo_hip_field_alloc
+
+   SUBROUTINE nemo_hip_field_free( pdev )
+      TYPE(C_PTR), INTENT(in) :: pdev
+      CALL check( c_field_free( pdev ), 'nemo_hip_field_free' )
+   END SUBROUTINE nemo_hip_field_free
+
+   SUBROUTINE nemo_hip_field_push( pdev, ptab )
+      !! whole array, host -> device
+      TYPE(C_PTR), INTENT(in) :: pdev
+      REAL(wp), DIMENSION(..), TARGET, CONTIGUOUS, INTENT(in) :: ptab
+      CALL check( c_field_push( pdev, C_LOC(ptab), INT(SIZE(ptab), C_SIZE_T) * 8_C_SIZE_T ), 'nemo_hip_field_push' )
+   END SUBROUTINE nemo_hip_field_push
+
+   SUBROUTINE nemo_hip_field_pull( ptab, pdev )
+      !! whole array, device -> host, after the work queued on the library stream
+      REAL(wp), DIMENSION(..), TARGET, CONTIGUOUS, INTENT(inout) :: ptab
+      TYPE(C_PTR), INTENT(in) :: pdev
+      CALL check( c_field_pull( C_LOC(ptab), pdev, INT(SIZE(ptab), C_SIZE_T) * 8_C_SIZE_T ), 'nemo_hip_field_pull' )
+   END SUBROUTINE nemo_hip_field_pull
 
    SUBROUTINE nemo_hip_bind_zdf( pavt, pavs, pavm )
       !! zdf_oce module arrays (avt, avs, avm)

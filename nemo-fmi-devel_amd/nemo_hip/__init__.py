@@ -43,6 +43,14 @@ def _is_torch(a):
     return type(a).__module__.startswith("torch")
 
 
+class DeviceField:
+    """A resident field of nemo_hip_field_alloc: a device address and its size
+    in bytes; device-mode entry points accept it where they accept a tensor."""
+
+    def __init__(self, ptr, nbytes):
+        self.ptr, self.nbytes = ptr, nbytes
+
+
 class NemoHip:
     """One process-wide handle on the library (the library itself is a singleton)."""
 
@@ -86,6 +94,10 @@ class NemoHip:
     def _p(self, a, dtype=np.float64):
         if a is None:
             return None
+        if isinstance(a, DeviceField):
+            if self.mode != NEMO_HIP_PTR_DEVICE:
+                raise NemoHipError(abi.NEMO_HIP_ERR_ARG, "resident field passed in host-pointer mode")
+            return C.c_void_p(a.ptr)
         if _is_torch(a):
             if self.mode != NEMO_HIP_PTR_DEVICE:
                 raise NemoHipError(abi.NEMO_HIP_ERR_ARG, "torch tensor passed in host-pointer mode")
@@ -99,6 +111,28 @@ class NemoHip:
         if not (a.flags.f_contiguous or a.ndim == 1):
             raise NemoHipError(abi.NEMO_HIP_ERR_ARG, "host arrays must be Fortran-contiguous")
         return C.c_void_p(a.ctypes.data)
+
+    # resident fields -----------------------------------------------------
+    def field_alloc(self, nbytes):
+        p = C.c_void_p()
+        self._check(self.L.nemo_hip_field_alloc(C.c_size_t(nbytes), C.byref(p)))
+        return DeviceField(p.value, nbytes)
+
+    def field_free(self, f):
+        self._check(self.L.nemo_hip_field_free(C.c_void_p(f.ptr if isinstance(f, DeviceField) else f)))
+
+    def field_push(self, f, host, nbytes=None):
+        """Copy a Fortran-ordered host array into the field (whole array unless nbytes)."""
+        host = np.asfortranarray(host)
+        n = host.nbytes if nbytes is None else nbytes
+        self._check(self.L.nemo_hip_field_push(C.c_void_p(f.ptr), C.c_void_p(host.ctypes.data), C.c_size_t(n)))
+
+    def field_pull(self, host, f, nbytes=None):
+        """Copy the field into a Fortran-contiguous host array, after the queued work."""
+        if not (host.flags.f_contiguous or host.ndim == 1):
+            raise NemoHipError(abi.NEMO_HIP_ERR_ARG, "host arrays must be Fortran-contiguous")
+        n = host.nbytes if nbytes is None else nbytes
+        self._check(self.L.nemo_hip_field_pull(C.c_void_p(host.ctypes.data), C.c_void_p(f.ptr), C.c_size_t(n)))
 
     # timing ------------------------------------------------------------
     def timing_enable(self, on=True):

@@ -41,3 +41,26 @@ def test_init_without_gpu_fails_loudly():
     buf = C.create_string_buffer(512)
     L.nemo_hip_last_error(buf, 512)
     assert b"device" in buf.value
+
+
+def test_fortran_module_binds_declared_entry_points(tmp_path):
+    """Every BIND(C) name of the Fortran drop-in module is an entry point the
+    header declares, and the module compiles with the image's Fortran
+    compiler (skipped when there is none)."""
+    import os
+    import re
+    import shutil
+    import subprocess
+    import pytest
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = os.path.join(root, "nemo-fmi-devel_amd", "fortran", "nemo_hip_mod.F90")
+    bound = set(re.findall(r'BIND\(C,\s*name="(\w+)"\)', open(src).read()))
+    assert len(bound) >= 15
+    missing = sorted(bound - set(nemo_hip.exported_symbols()))
+    assert not missing, missing
+    fc = shutil.which("amdflang") or shutil.which("flang")
+    if not fc:
+        pytest.skip("no Fortran compiler")
+    r = subprocess.run([fc, "-c", src, "-o", str(tmp_path / "m.o"), "-module-dir", str(tmp_path)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
