@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--cpu-sample", default="512x512x75")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--host-mode", type=int, default=0, metavar="STEPS",
+                    help="also time STEPS steps of the FP64 routines in host-pointer mode (NEMO's host arrays, "
+                         "every call staged over PCIe): reported apart, never the headline value")
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -115,9 +118,11 @@ class Stepper:
                        f["uslp"], f["vslp"], f["wslpi"], f["wslpj"], self.ah, self.akz, f["avt"], sp=sp)
             return
         # step.F90:179-180 and 234: trends start at zero
-        self.ua.zero_()
-        self.va.zero_()
-        self.tsa.zero_()
+        for a in (self.ua, self.va, self.tsa):
+            if hasattr(a, "zero_"):
+                a.zero_()
+            else:
+                a.fill(0.0)
         h.dyn_adv(kt, self.un_dyn, self.vn_dyn, f["wn"], self.ua, self.va, sp=sp)
         h.dyn_vor(kt, self.un_dyn, self.vn_dyn, self.ua, self.va, sp=sp)
         h.dyn_spg_ts(kt, self.spg, sp=sp)
@@ -184,6 +189,34 @@ def time_form(h, d, S, sp, fused, steps, warmup, timing_steps, live=False):
     del st
     torch.cuda.empty_cache()
     return ms, finite, ktimes, live_res
+
+
+def time_host_mode(h, d, S, steps):
+    """ms per step of the separate FP64 routines called on host arrays (the
+    drop-in path a NEMO build without resident fields takes)."""
+    import nemo_hip
+    import torch
+    st = Stepper(h, d, S, False, False)
+    host = lambda t: t.detach().cpu().numpy().copy()
+    for k in st.f3:
+        st.f3[k] = host(st.f3[k])
+    st.tsb, st.tsn, st.tsa, st.ua, st.va = (host(t) for t in (st.tsb, st.tsn, st.tsa, st.ua, st.va))
+    st.zun, st.zvn, st.zwn, st.ah, st.akz = (host(t) for t in (st.zun, st.zvn, st.zwn, st.ah, st.akz))
+    st.f2 = {k: host(v) for k, v in st.f2.items()}
+    st.un_dyn, st.vn_dyn = host(st.un_dyn), host(st.vn_dyn)
+    st.spg = dict(st.f2)
+    st.spg.update(ua=st.ua, va=st.va, un=st.un_dyn, vn=st.vn_dyn)
+    torch.cuda.synchronize()
+    h.set_pointer_mode(nemo_hip.NEMO_HIP_PTR_HOST)
+    try:
+        st.step(2)
+        t0 = time.perf_counter()
+        for kt in range(3, 3 + steps):
+            st.step(kt)
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+    finally:
+        h.set_pointer_mode(nemo_hip.NEMO_HIP_PTR_DEVICE)
+    return ms
 
 
 def kernel_times(h):
@@ -369,6 +402,11 @@ def main():
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": alg_bytes / (avg_ms * 1e-3) / HBM_PEAK,
                 "traffic": traffic_per_launch(kname, head, jpi, jpj, jpk), "avg_launch_ms": avg_ms,
                 "launches_timed": live["launches"], "algorithmic_bytes_per_launch": alg_bytes}
+    host_ms = None
+    if a.host_mode:
+        host_ms = time_host_mode(h, d, S, a.host_mode)
+        log(f"host-pointer mode (FP64 routines, PCIe-inclusive): {host_ms:.1f} ms/step, "
+            f"{cells / (host_ms * 1e-3):.3e} cell-updates/s")
     copy_gbs = stream_copy()
     log(f"device copy (2 GiB, read + write): {copy_gbs:.0f} GB/s")
     kt_, kb_ = d.arrays["k_top"], d.arrays["k_bot"]
@@ -394,6 +432,7 @@ def main():
         "wet_cells": wet_cells,
         "wet_cell_updates_per_s": wet_cells / (h_ms * 1e-3),
         "stream_copy_GBps": copy_gbs,
+        "host_mode_fp64_ms_per_step": host_ms,
         "forms": res,
         "roofline": roof,
         "cpu_baseline": cpu,
