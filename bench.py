@@ -230,11 +230,11 @@ def kernel_times(h):
         "dyn_spg_ts": ["k_spg_cor1", "k_spg_cor2", "k_spg_p1a", "k_spg_p1b", "k_spg_s1", "k_spg_s2", "k_spg_s3",
                        "k_spg_p3a", "k_spg_p3b"],
         "tra_adv_trp": ["k_tra_adv_trp"],
-        "tra_adv_fct": ["k_fct_h", "k_fct_v", "k_fct_beta", "k_fct_final", "k_fct_low", "k_fct_ad"],
+        "tra_adv_fct": ["k_fct_h", "k_fct_v", "k_fct_lim", "k_fct_beta", "k_fct_final", "k_fct_low", "k_fct_ad"],
         "tra_ldf_iso": ["k_ldf_iso_coef", "k_tra_ldf_iso"], "tra_zdf": ["k_tra_zdf", "k_clamp_ring"],
         "lbc_lnk": ["k_lbc"],
         "step_dyn": ["k_step_dyn1", "k_step_dyn2"],
-        "step_tra": ["k_fct_h", "k_fct_v", "k_fct_beta", "k_fct_final", "k_zero_ring", "k_tra_zdf"],
+        "step_tra": ["k_fct_h", "k_fct_v", "k_fct_lim", "k_fct_beta", "k_fct_final", "k_zero_ring", "k_tra_zdf"],
     }
     for n in names:
         out[n] = h.timing_get(n)
@@ -473,6 +473,7 @@ KERNEL_BYTES = {
     "k_dyn_keg_zad": _b(7),      # r un vn wn ua va, w ua va
     "k_fct_final": _b(7),        # r zwx zwy zwz zbetup zbetdo ta, w ta
     "k_fct_beta": _b(7),         # r tb zwi zwx zwy zwz, w zbetup zbetdo
+    "k_fct_lim": _b(14),         # two tracers: r tb zwi zwx zwy zwz ta, w ta
     "k_fct_ad": _b(8),           # r pun pvn pwn tb tn, w zwx zwy zwz
     "k_fct_low": _b(7),          # r pun pvn pwn tb ta, w ta zwi
     "k_spg_p1a": _b(6),          # r ua va un vn, w ua va

@@ -370,6 +370,245 @@ KERNEL void k_fct_final(DevDomain d, const T * __restrict__ paa, const T * __res
     }
 }
 
+// (3)+(4) fused for closed domains: the Zalesak betas and the limited
+// divergence in one k-march per 64x4 tile.  The bounds zbup/zbdo (FP64) of
+// the tile and a 2-point halo, for three levels, and the betas of the tile
+// and a 1-point halo, for two levels, live in LDS, so the betas never go
+// through HBM.  Every global load is issued one level ahead into registers
+// and consumed after the next barrier, so the loads of the march overlap
+// its LDS work.  Every value is the one k_fct_beta and k_fct_final compute
+// (same expressions, same rounding to the storage type), and the ring values
+// lbc_lnk gives a closed domain (zero betas and limited fluxes) are selected
+// where they apply.
+#define LIM_TX 64
+#define LIM_TY 4
+#define LIM_ZX (LIM_TX + 4)
+#define LIM_ZY (LIM_TY + 4)
+#define LIM_BX (LIM_TX + 2)
+#define LIM_BY (LIM_TY + 2)
+#define LIM_NZ ((LIM_ZX * LIM_ZY + NH_BLOCK - 1) / NH_BLOCK)
+#define LIM_NB ((LIM_BX * LIM_BY + NH_BLOCK - 1) / NH_BLOCK)
+template <typename T>
+KERNEL_W(4) void k_fct_lim(DevDomain d, double p2dt, const T *__restrict__ pbef_, const T *__restrict__ paft_,
+                      const T *__restrict__ paa_, const T *__restrict__ pbb_, const T *__restrict__ pcc_,
+                      T *__restrict__ ta_)
+{
+    // bounds: FP64 with the reference's +-1e40 land sentinels; with FP32
+    // storage they are exact in FP32 (max/min of stored values) and the
+    // sentinels become +-inf, which give the same betas after the +-1e30
+    // clamp of the FP32 form
+    using ZB = typename std::conditional<sizeof(T) == 4, float, double>::type;
+    const ZB zsent = sizeof(T) == 4 ? (ZB)INFINITY : (ZB)1.e+40;
+    __shared__ ZB zup_s[3][LIM_ZY][LIM_ZX], zdo_s[3][LIM_ZY][LIM_ZX];
+    __shared__ T bup_s[2][LIM_BY][LIM_BX], bdo_s[2][LIM_BY][LIM_BX];
+    // blockIdx.z selects the tracer of the pair
+    const size_t o = (size_t)blockIdx.z * d.n3;
+    const T *pbef = pbef_ + o, *paft = paft_ + o, *paa = paa_ + o, *pbb = pbb_ + o, *pcc = pcc_ + o;
+    T *ta = ta_ + o;
+    unsigned bx, by;
+    xcd_tile(bx, by);
+    const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, tid = ty * LIM_TX + tx;
+    const int i0 = 2 + (int)bx * LIM_TX, j0 = 2 + (int)by * LIM_TY;
+    const int i = i0 + tx, j = j0 + ty;
+    const int jpi = d.jpi, jpj = d.jpj, jpk = d.jpk;
+    const bool own = (i <= jpi - 1 && j <= jpj - 1);
+    const double zrtrn = 1.e-15;
+    const Metrics<T> &m = met<T>(d);
+
+    // ---- this thread's points of the bounds region (tile + 2)
+    int zpx[LIM_NZ], zpy[LIM_NZ];
+    bool zin[LIM_NZ], zon[LIM_NZ];
+    T ztb[LIM_NZ], zzw[LIM_NZ];
+#pragma unroll
+    for (int r = 0; r < LIM_NZ; r++) {
+        const int q = tid + r * NH_BLOCK;
+        zon[r] = q < LIM_ZX * LIM_ZY;
+        zpx[r] = q % LIM_ZX;
+        zpy[r] = q / LIM_ZX;
+        const int pi = i0 - 2 + zpx[r], pj = j0 - 2 + zpy[r];
+        zin[r] = zon[r] && pi >= 1 && pi <= jpi && pj >= 1 && pj <= jpj;
+    }
+    auto zb_fetch = [&](int kk) {
+#pragma unroll
+        for (int r = 0; r < LIM_NZ; r++) {
+            if (zin[r] && kk <= jpk) {
+                const int pi = i0 - 2 + zpx[r], pj = j0 - 2 + zpy[r];
+                ztb[r] = A3(pbef, pi, pj, kk);
+                zzw[r] = A3(paft, pi, pj, kk);
+            }
+        }
+    };
+    // zbup / zbdo of traadv_fct.F90:361-364 from the fetched values: with
+    // tmask 0 or 1, b*tmask -+ zbig*(1-tmask) is b or -+zbig exactly
+    auto zb_store = [&](int kk) {
+        const int sl = kk % 3;
+#pragma unroll
+        for (int r = 0; r < LIM_NZ; r++) {
+            if (!zon[r]) continue;
+            ZB u = 0, w = 0;
+            if (zin[r] && kk <= jpk) {
+                const int pi = i0 - 2 + zpx[r], pj = j0 - 2 + zpy[r];
+                if constexpr (sizeof(T) == 4) {
+                    const bool wet = TMASK(int, pi, pj, kk) != 0;
+                    const ZB a = (ZB)ztb[r], b = (ZB)zzw[r];
+                    u = wet ? (a > b ? a : b) : -zsent;
+                    w = wet ? (a < b ? a : b) : zsent;
+                } else {
+                    const double tm = TMASK(double, pi, pj, kk);
+                    const double au = (double)ztb[r] * tm - zsent * (1.0 - tm);
+                    const double bu = (double)zzw[r] * tm - zsent * (1.0 - tm);
+                    u = au > bu ? au : bu;
+                    const double ad = (double)ztb[r] * tm + zsent * (1.0 - tm);
+                    const double bd = (double)zzw[r] * tm + zsent * (1.0 - tm);
+                    w = ad < bd ? ad : bd;
+                }
+            }
+            zup_s[sl][zpy[r]][zpx[r]] = u;
+            zdo_s[sl][zpy[r]][zpx[r]] = w;
+        }
+    };
+
+    // ---- this thread's points of the beta region (tile + 1)
+    int bpx[LIM_NB], bpy[LIM_NB];
+    bool bon[LIM_NB], bin[LIM_NB];
+    T fa0[LIM_NB], fa1[LIM_NB], fb0[LIM_NB], fb1[LIM_NB], fc0[LIM_NB], fc1[LIM_NB], fzw[LIM_NB];
+    double fe1e2t[LIM_NB];
+#pragma unroll
+    for (int r = 0; r < LIM_NB; r++) {
+        const int q = tid + r * NH_BLOCK;
+        bon[r] = q < LIM_BX * LIM_BY;
+        bpx[r] = q % LIM_BX;
+        bpy[r] = q / LIM_BX;
+        const int pi = i0 - 1 + bpx[r], pj = j0 - 1 + bpy[r];
+        bin[r] = bon[r] && pi >= 2 && pi <= jpi - 1 && pj >= 2 && pj <= jpj - 1;
+        fe1e2t[r] = bin[r] ? d.md.e1e2t[IJ(pi, pj)] : 0.0;
+    }
+    auto beta_fetch = [&](int kk) {
+#pragma unroll
+        for (int r = 0; r < LIM_NB; r++) {
+            if (bin[r] && kk <= jpk - 1) {
+                const int pi = i0 - 1 + bpx[r], pj = j0 - 1 + bpy[r];
+                fa0[r] = A3(paa, pi - 1, pj, kk);
+                fa1[r] = A3(paa, pi, pj, kk);
+                fb0[r] = A3(pbb, pi, pj - 1, kk);
+                fb1[r] = A3(pbb, pi, pj, kk);
+                fc0[r] = A3(pcc, pi, pj, kk);
+                fc1[r] = A3(pcc, pi, pj, kk + 1);
+                fzw[r] = A3(paft, pi, pj, kk);
+            }
+        }
+    };
+    // nonosc betas, lines 366-399, then the closed-ring lbc_lnk (zero off
+    // the interior and at level jpk)
+    auto beta_make = [&](int kk) {
+        const int sb = kk & 1;
+        const int sa = (kk > 1 ? kk - 1 : 1) % 3, sc = kk % 3, sn = (kk + 1) % 3;
+#pragma unroll
+        for (int r = 0; r < LIM_NB; r++) {
+            if (!bon[r]) continue;
+            T bu_t = T(0), bd_t = T(0);
+            if (bin[r] && kk <= jpk - 1) {
+                const int zx = bpx[r] + 1, zy = bpy[r] + 1;
+                ZB zup = zup_s[sc][zy][zx];
+                zup = fmax2(zup, zup_s[sc][zy][zx - 1]);
+                zup = fmax2(zup, zup_s[sc][zy][zx + 1]);
+                zup = fmax2(zup, zup_s[sc][zy - 1][zx]);
+                zup = fmax2(zup, zup_s[sc][zy + 1][zx]);
+                zup = fmax2(zup, zup_s[sa][zy][zx]);
+                zup = fmax2(zup, zup_s[sn][zy][zx]);
+                ZB zdo = zdo_s[sc][zy][zx];
+                zdo = fmin2(zdo, zdo_s[sc][zy][zx - 1]);
+                zdo = fmin2(zdo, zdo_s[sc][zy][zx + 1]);
+                zdo = fmin2(zdo, zdo_s[sc][zy - 1][zx]);
+                zdo = fmin2(zdo, zdo_s[sc][zy + 1][zx]);
+                zdo = fmin2(zdo, zdo_s[sa][zy][zx]);
+                zdo = fmin2(zdo, zdo_s[sn][zy][zx]);
+                const double a_im1 = fa0[r], a_i = fa1[r], b_jm1 = fb0[r], b_j = fb1[r];
+                const double c_k = fc0[r], c_kp1 = fc1[r];
+                const double zpos = fmax2(0.0, a_im1) - fmin2(0.0, a_i) + fmax2(0.0, b_jm1) - fmin2(0.0, b_j) +
+                                    fmax2(0.0, c_kp1) - fmin2(0.0, c_k);
+                const double zneg = fmax2(0.0, a_i) - fmin2(0.0, a_im1) + fmax2(0.0, b_j) - fmin2(0.0, b_jm1) +
+                                    fmax2(0.0, c_k) - fmin2(0.0, c_kp1);
+                const int pi = i0 - 1 + bpx[r], pj = j0 - 1 + bpy[r];
+                const double zbt = fe1e2t[r] * E3(d.md.e3t, IJ(pi, pj), kk) / p2dt;
+                const double paft_k = fzw[r];
+                const double bu = ((double)zup - paft_k) / (zpos + zrtrn) * zbt;
+                const double bd = (paft_k - (double)zdo) / (zneg + zrtrn) * zbt;
+                if (sizeof(T) == 4) {
+                    bu_t = (T)fmin2(fmax2(bu, -1.0e30), 1.0e30);
+                    bd_t = (T)fmin2(fmax2(bd, -1.0e30), 1.0e30);
+                } else {
+                    bu_t = (T)bu;
+                    bd_t = (T)bd;
+                }
+            }
+            bup_s[sb][bpy[r]][bpx[r]] = bu_t;
+            bdo_s[sb][bpy[r]][bpx[r]] = bd_t;
+        }
+    };
+
+    // ---- this thread's cell: the limited divergence (lines 404-422, 288-297)
+    T ga0 = T(0), ga1 = T(0), gb0 = T(0), gb1 = T(0), gc1 = T(0), gta = T(0);
+    auto fin_fetch = [&](int k) {
+        if (own && k <= jpk - 1) {
+            ga0 = A3(paa, i - 1, j, k);
+            ga1 = A3(paa, i, j, k);
+            gb0 = A3(pbb, i, j - 1, k);
+            gb1 = A3(pbb, i, j, k);
+            gc1 = A3(pcc, i, j, k + 1);
+            gta = A3(ta, i, j, k);
+        }
+    };
+    const size_t ij = own ? IJ(i, j) : 0;
+    const T r1_e1e2t = own ? m.r1_e1e2t[ij] : T(0);
+    double lz_k = own ? (double)A3(pcc, i, j, 1) : 0.0;
+    const int cx = tx + 1, cy = ty + 1;
+    const bool okxm = i - 1 >= 2, okym = j - 1 >= 2;
+    auto fin_do = [&](int k) {
+        if (!own) return;
+        const int s0 = k & 1, s1 = (k + 1) & 1;
+        const double bu_c = bup_s[s0][cy][cx], bd_c = bdo_s[s0][cy][cx];
+        const double lx_i = lim_face<T>(ga1, bd_c, bup_s[s0][cy][cx + 1], bu_c, bdo_s[s0][cy][cx + 1]);
+        const double lx_im1 = okxm ? lim_face<T>(ga0, bdo_s[s0][cy][cx - 1], bu_c, bup_s[s0][cy][cx - 1], bd_c) : 0.0;
+        const double ly_j = lim_face<T>(gb1, bd_c, bup_s[s0][cy + 1][cx], bu_c, bdo_s[s0][cy + 1][cx]);
+        const double ly_jm1 = okym ? lim_face<T>(gb0, bdo_s[s0][cy - 1][cx], bu_c, bup_s[s0][cy - 1][cx], bd_c) : 0.0;
+        const double lz_kp1 = lim_face<T>(gc1, bdo_s[s1][cy][cx], bu_c, bup_s[s1][cy][cx], bd_c);
+        A3(ta, i, j, k) = gta - ((T)lx_i - (T)lx_im1 + (T)ly_j - (T)ly_jm1 + (T)lz_k - (T)lz_kp1) * r1_e1e2t /
+                                    E3(m.e3t, ij, k);
+        lz_k = lz_kp1;
+    };
+
+    // ---- prologue: bounds of levels 1-3, betas of level 1, loads of the
+    // first iteration in flight
+    zb_fetch(1);
+    zb_store(1);
+    zb_fetch(2);
+    zb_store(2);
+    zb_fetch(3);
+    zb_store(3);
+    beta_fetch(1);
+    __syncthreads();
+    beta_make(1);
+    beta_fetch(2);
+    fin_fetch(1);
+    zb_fetch(4);
+    __syncthreads();
+    // ---- march: at the top of iteration k the LDS holds the bounds of
+    // levels k..k+2 and the betas of level k; the registers hold the loads
+    // for the betas of level k+1, the cell of level k and the bounds of
+    // level k+3
+    for (int k = 1; k <= jpk - 1; k++) {
+        beta_make(k + 1);
+        beta_fetch(k + 2);
+        __syncthreads();
+        fin_do(k);
+        fin_fetch(k + 1);
+        zb_store(k + 3);
+        zb_fetch(k + 4);
+        __syncthreads();
+    }
+}
+
 template <typename T>
 int launch_tra_adv_fct(double p2dt, const T *pun, const T *pvn, const T *pwn, const T *ptb, const T *ptn, T *pta,
                        int kjpt, int kn_fct_h, int kn_fct_v)
@@ -1227,6 +1466,13 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
         if ((st = launch_lbc<T>(zwx, nk, 'U', T(0)))) return st;
         if ((st = launch_lbc<T>(zwy, nk, 'V', T(0)))) return st;
         if ((st = launch_lbc<T>(zwz, nk, 'W', T(0)))) return st;
+        if (!d.iper && !d.jper) {
+            // closed domain: betas and limited divergence fused, both tracers in one launch
+            Grid2 gl = grid2(d.jpi - 2, d.jpj - 2);
+            gl.grid.z = (unsigned)nt;
+            LAUNCH(nm, (k_fct_lim<T>), gl, d, p2dt, ptb + o, zwi, zwx, zwy, zwz, pta + o);
+            continue;
+        }
         for (int t = 0; t < nt; t++) {
             const size_t q = (size_t)t * n3;
             LAUNCH(nm, (k_fct_beta<T, FctKB<T>::beta>), gk, d, p2dt, ptb + o + q, zwi + q, zwx + q, zwy + q, zwz + q, bu + q, bd + q);
