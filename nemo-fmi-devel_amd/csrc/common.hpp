@@ -149,6 +149,19 @@ __device__ __forceinline__ T wpmask(const uint32_t *p, uint32_t ij, int k)
     return (k >= a && k <= b && (k == 1 || k - 1 >= a)) ? T(1) : T(0);
 }
 
+// surface masks ssmask / ssumask / ssvmask (1 where the column has a wet
+// level, DOM/dommsk.F90) from the same packed intervals: the values the
+// registration uploads as 2D arrays, read as 4 bytes per column
+template <int SH>
+__device__ __forceinline__ double psurf(const uint32_t *p, uint32_t ij)
+{
+    const uint32_t w = p[ij] >> SH;
+    return ((w >> 8) & 255u) >= (w & 255u) ? 1.0 : 0.0;
+}
+#define SSMASK(ij) psurf<0>(d.ktu, (uint32_t)(ij))
+#define SSUMASK(ij) psurf<16>(d.ktu, (uint32_t)(ij))
+#define SSVMASK(ij) psurf<0>(d.kvf, (uint32_t)(ij))
+
 #define TMASK(T_, i, j, k) pmask<T_, 0>(d.ktu, IJ32(i, j), k)
 #define UMASK(T_, i, j, k) pmask<T_, 16>(d.ktu, IJ32(i, j), k)
 #define VMASK(T_, i, j, k) pmask<T_, 0>(d.kvf, IJ32(i, j), k)

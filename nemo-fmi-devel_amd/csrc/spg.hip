@@ -341,7 +341,7 @@ __device__ __forceinline__ double spg_ssha(const DevDomain &d, double rdtbt, con
     const size_t s = IJ(si, sj);
     const double zhdiv = (A2(w.zwx, si, sj) - A2(w.zwx, si - 1, sj) + A2(w.zwy, si, sj) - A2(w.zwy, si, sj - 1)) *
                          d.md.r1_e1e2t[s];
-    const double a = (c.sshn_e[s] - rdtbt * (w.zssh_frc[s] + zhdiv)) * d.ssmask[s];
+    const double a = (c.sshn_e[s] - rdtbt * (w.zssh_frc[s] + zhdiv)) * SSMASK(s);
     return ok ? a : 0.0;
 }
 // zsshp2_e at (p,q) from the after level there
@@ -362,7 +362,7 @@ __device__ __forceinline__ void spg_new_vel(const DevDomain &d, int i, int j, do
     const size_t ij = IJ(i, j);
     double z1_hu = 0.0, z1_hv = 0.0;
     if (d.nvor_scheme == NEMO_NP_ENT) {
-        const double su = d.ssumask[ij], sv = d.ssvmask[ij];
+        const double su = SSUMASK(ij), sv = SSVMASK(ij);
         z1_hu = su / (hu_n[ij] + 1.0 - su);
         z1_hv = sv / (hv_n[ij] + 1.0 - sv);
     }
@@ -375,8 +375,8 @@ __device__ __forceinline__ void spg_new_vel(const DevDomain &d, int i, int j, do
     const double zv_spg = -(d.grav * (p2n - p2c) * m.r1_e2v[ij]);
     const double zwx = (1.0 - d.rn_scal_load) * zu_spg;
     const double zwy = (1.0 - d.rn_scal_load) * zv_spg;
-    ua = (un_e[ij] + rdtbt * (zwx + zu_trd + w.zu_frc[ij])) * d.ssumask[ij];
-    va = (vn_e[ij] + rdtbt * (zwy + zv_trd + w.zv_frc[ij])) * d.ssvmask[ij];
+    ua = (un_e[ij] + rdtbt * (zwx + zu_trd + w.zu_frc[ij])) * SSUMASK(ij);
+    va = (vn_e[ij] + rdtbt * (zwy + zv_trd + w.zv_frc[ij])) * SSVMASK(ij);
 }
 // S2: after sea level at (i,j), its time sum and zsshp2_e
 KERNEL void k_spg_s2(DevDomain d, double rdtbt, double wgt1, SpgSsh c, double *__restrict__ ssha_to,
@@ -387,7 +387,8 @@ KERNEL void k_spg_s2(DevDomain d, double rdtbt, double wgt1, SpgSsh c, double *_
     const size_t ij = IJ(i, j);
     const double ssha_e = spg_ssha(d, rdtbt, c, w, i, j);
     ssha_to[ij] = ssha_e;
-    ssha[ij] = ssha[ij] + wgt1 * ssha_e;
+    // a zero weight leaves the sum (+0 from its reset) as it is
+    if (wgt1 != 0.0) ssha[ij] = ssha[ij] + wgt1 * ssha_e;
     w.zsshp2_e[ij] = spg_p2(c, ssha_e, ij);
 }
 
@@ -428,8 +429,10 @@ KERNEL void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double *__res
     const double ub_now = un_e[ij], vb_now = vn_e[ij];
     ubb_to_a[ij] = ua;
     vbb_to_a[ij] = va;
-    ua_b[ij] = ua_b[ij] + wgt1 * ua;
-    va_b[ij] = va_b[ij] + wgt1 * va;
+    if (wgt1 != 0.0) {   // a zero weight leaves the sums (+0 from their reset) as they are
+        ua_b[ij] = ua_b[ij] + wgt1 * ua;
+        va_b[ij] = va_b[ij] + wgt1 * va;
+    }
     if (!nx.on) return;
     // the next sub-step's extrapolation, fluxes and un_adv sum (pointwise in
     // the levels now, before and after of this point), as k_spg_s1 computes them
