@@ -459,8 +459,8 @@ def _fused(f):
 
 
 def _iso_nt(f):
-    # tra_ldf_iso runs one tracer per launch in FP64 storage, both in FP32
-    return 1 if f.startswith("fp64") else 2
+    # tra_ldf_iso covers a tracer pair per launch in both storage forms
+    return 2
 
 
 KERNEL_BYTES = {

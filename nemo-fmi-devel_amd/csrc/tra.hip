@@ -815,13 +815,27 @@ __device__ __forceinline__ T vmk(const IsoCols &c, int q, int k) { return kin(k,
 
 enum { QC = 0, QW = 1, QE = 2, QS = 3, QN = 4 };
 
-template <typename T, int MODE, int NT>
+// NT tracers per thread; TSPLIT > 1 spreads TSPLIT tracers over
+// neighbouring workgroups instead (the tracer is the lowest digit of the
+// XCD-ordered tile index, so the workgroups of one tile run on one XCD at
+// the same time and share the coefficient loads in its L2)
+template <typename T, int MODE, int NT, int TSPLIT>
 KERNEL void k_tra_ldf_iso(DevDomain d, T zsign, const T * __restrict__ pahu, const T * __restrict__ pahv,
                           const T * __restrict__ uslp, const T * __restrict__ vslp, const T * __restrict__ wslpi,
                           const T * __restrict__ wslpj, const T * __restrict__ ah_wslp2, const T * __restrict__ akz,
                           const T * __restrict__ ptb, const T * __restrict__ ptbb, T * __restrict__ pta)
 {
-    THREAD_IJ(2, 2);
+    unsigned bx_, by_;
+    xcd_tile(bx_, by_);
+    if (TSPLIT > 1) {
+        const size_t toff = (size_t)(bx_ % TSPLIT) * d.n3;
+        ptb += toff;
+        ptbb += toff;
+        pta += toff;
+        bx_ /= TSPLIT;
+    }
+    const int i = (int)(bx_ * blockDim.x + threadIdx.x) + 2;
+    const int j = (int)(by_ * blockDim.y + threadIdx.y) + 2;
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
     const Metrics<T> &m = met<T>(d);
     const size_t n3 = d.n3, plane = d.n2;
@@ -955,17 +969,19 @@ static int launch_iso_mode(const DevDomain &d, T zsign, const T *pahu, const T *
 {
     Grid2 g = grid2(d.jpi - 2, d.jpj - 2);
     int jn = 0;
-    // FP64: one tracer per pass (the two-tracer window needs 255+ VGPRs and
-    // runs one wave per SIMD); FP32: tracer pairs share the coefficient reads
-    const int pair = sizeof(T) == 4;
-    for (; pair && jn + 2 <= kjpt; jn += 2) {
+    // tracer pairs: one tracer per thread (a two-tracer register window needs
+    // 170-255 VGPRs and runs two waves per SIMD or fewer), the pair in one
+    // launch so that its two workgroups of a tile share the coefficients
+    Grid2 g2 = g;
+    g2.grid.x *= 2;
+    for (; jn + 2 <= kjpt; jn += 2) {
         const size_t o = (size_t)jn * d.n3;
-        LAUNCH("tra_ldf_iso", (k_tra_ldf_iso<T, MODE, 2>), g, d, zsign, pahu, pahv, uslp, vslp, wslpi, wslpj,
+        LAUNCH("tra_ldf_iso", (k_tra_ldf_iso<T, MODE, 1, 2>), g2, d, zsign, pahu, pahv, uslp, vslp, wslpi, wslpj,
                ah_wslp2, akz, ptb + o, ptbb + o, pta + o);
     }
     for (; jn < kjpt; jn++) {
         const size_t o = (size_t)jn * d.n3;
-        LAUNCH("tra_ldf_iso", (k_tra_ldf_iso<T, MODE, 1>), g, d, zsign, pahu, pahv, uslp, vslp, wslpi, wslpj,
+        LAUNCH("tra_ldf_iso", (k_tra_ldf_iso<T, MODE, 1, 1>), g, d, zsign, pahu, pahv, uslp, vslp, wslpi, wslpj,
                ah_wslp2, akz, ptb + o, ptbb + o, pta + o);
     }
     return 0;
