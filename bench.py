@@ -227,11 +227,11 @@ def kernel_times(h):
              "lbc_lnk", "step_dyn", "step_tra"]
     kern = {
         "dyn_adv": ["k_dyn_keg_zad"], "dyn_vor": ["k_dyn_vor"], "dyn_zdf": ["k_dyn_zdf"],
-        "dyn_spg_ts": ["k_spg_cor1", "k_spg_cor2", "k_spg_p1a", "k_spg_p1b", "k_spg_s1", "k_spg_s2", "k_spg_s3",
+        "dyn_spg_ts": ["k_spg_cor1", "k_spg_cor2", "k_spg_init", "k_spg_p1a", "k_spg_p1b", "k_spg_s1", "k_spg_s2", "k_spg_s3",
                        "k_spg_p3a", "k_spg_p3b"],
         "tra_adv_trp": ["k_tra_adv_trp"],
         "tra_adv_fct": ["k_fct_h", "k_fct_v", "k_fct_lim", "k_fct_beta", "k_fct_final", "k_fct_low", "k_fct_ad"],
-        "tra_ldf_iso": ["k_ldf_iso_coef", "k_tra_ldf_iso"], "tra_zdf": ["k_tra_zdf", "k_clamp_ring"],
+        "tra_ldf_iso": ["k_ldf_iso_coef", "k_tra_ldf_iso", "k_fill3", "k_copy3"], "tra_zdf": ["k_tra_zdf", "k_clamp_ring"],
         "lbc_lnk": ["k_lbc"],
         "step_dyn": ["k_step_dyn1", "k_step_dyn2"],
         "step_tra": ["k_fct_h", "k_fct_v", "k_fct_lim", "k_fct_beta", "k_fct_final", "k_zero_ring", "k_tra_zdf"],

@@ -166,6 +166,33 @@ int nemo_hip_field_pull(void *host, const void *dev, size_t bytes);
    the arrays the hot path reads; call again if the domain changes */
 int nemo_hip_domain_register(const nemo_hip_domain_t *dom);
 
+/* Contexts (no counterpart routine in the reference, which runs one domain
+   per process): an ensemble of independent domains in one process, each
+   with its own registered domain, scratch and the state the reference keeps
+   in module variables between calls (r2dt, the barotropic levels).  Context
+   0 exists from nemo_hip_init and lives until nemo_hip_finalize; create
+   returns a new, empty one; select makes a context current for every call
+   that follows (registration and entry points); destroy frees a context
+   that is not current. */
+int nemo_hip_context_create(int *ctx);
+int nemo_hip_context_select(int ctx);
+int nemo_hip_context_destroy(int ctx);
+
+/* Graphs (no counterpart routine in the reference: they serve callers that
+   step the same fields many times, such as an ensemble).  begin records the
+   launches of the device-mode calls that follow on the library stream, in
+   any contexts; end turns the record into an executable graph and returns
+   its id; launch replays it on the library stream; free releases it.  A
+   recorded call replays with the arguments and time step it was recorded
+   with, so recording refuses what would change between replays: the first
+   two steps (kt <= nit000 + 1), the first tra_ldf_iso call (kt = kit000),
+   dyn_spg_ts without time averaging (ln_bt_av = F), host-pointer mode, and
+   calls that synchronise or allocate. */
+int nemo_hip_graph_begin(void);
+int nemo_hip_graph_end(int *graph);
+int nemo_hip_graph_launch(int graph);
+int nemo_hip_graph_free(int graph);
+
 /* per-routine GPU timers keyed by reference routine name (timing.F90:89-212):
    when enabled every kernel launch is bracketed by HIP events on the
    library's stream and accumulated by name */

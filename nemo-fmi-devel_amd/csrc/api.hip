@@ -23,13 +23,25 @@ static bool g_init = false;
 static int g_device = -1;
 static hipStream_t g_stream = nullptr;
 static int g_ptr_mode = NEMO_HIP_PTR_HOST;
-static bool g_have_domain = false;
-static DevDomain g_dom;
-static nemo_hip_domain_t g_hdom;
-static std::vector<void *> g_dom_allocs;
-static void *g_scratch3[S_NSCRATCH3] = {};
-static double *g_scratch2[NH_NSCRATCH2] = {};
-static double g_r2dt_tra, g_r2dt_zdf, g_r2dt_dyn;   // module r2dt of tra_adv / tra_zdf / dyn_zdf
+// a context: one registered domain with its scratch and the state the
+// reference keeps in module variables between calls; context 0 exists from
+// init, others come from nemo_hip_context_create (ensembles of domains)
+struct Ctx {
+    bool have_domain = false;
+    DevDomain dom{};
+    nemo_hip_domain_t hdom{};
+    std::vector<void *> dom_allocs;
+    void *scratch3[S_NSCRATCH3] = {};
+    double *scratch2[NH_NSCRATCH2] = {};
+    double r2dt_tra = 0, r2dt_zdf = 0, r2dt_dyn = 0;   // module r2dt of tra_adv / tra_zdf / dyn_zdf
+    int spg_rot_u[3] = {0, 1, 2};                       // dyn_spg_ts level buffers: [now, before, before-before]
+    int spg_rot_s[4] = {0, 1, 2, 3};                    // ... and sea level: [now, before, before-before, after]
+};
+static std::vector<Ctx *> g_ctxs;
+static Ctx *g_c = nullptr;   // the current context
+// recording of a graph on the library stream (nemo_hip_graph_begin / end)
+static bool g_capturing = false;
+static std::vector<hipGraphExec_t> g_graphs;
 // staging pool (host-pointer mode)
 static std::vector<std::pair<void *, size_t>> g_pool;
 // timing: every timed launch records its own start and end events (taken by
@@ -71,10 +83,18 @@ int set_error(int code, const char *fmt, ...)
     return code;
 }
 hipStream_t stream() { return g_stream; }
-const DevDomain &dom() { return g_dom; }
-bool have_domain() { return g_have_domain; }
-void *scratch3(int which) { return g_scratch3[which]; }
-double *scratch2(int which) { return g_scratch2[which]; }
+const DevDomain &dom() { return g_c->dom; }
+bool capturing() { return g_capturing; }
+int *spg_rot_u() { return g_c->spg_rot_u; }
+int *spg_rot_s() { return g_c->spg_rot_s; }
+void spg_ts_reset()
+{
+    for (int q = 0; q < 3; q++) g_c->spg_rot_u[q] = q;
+    for (int q = 0; q < 4; q++) g_c->spg_rot_s[q] = q;
+}
+bool have_domain() { return g_c->have_domain; }
+void *scratch3(int which) { return g_c->scratch3[which]; }
+double *scratch2(int which) { return g_c->scratch2[which]; }
 
 static hipEvent_t ev_get()
 {
@@ -103,7 +123,7 @@ static std::string kernel_key(const char *name, const char *kernel)
 }
 bool timing_pair(const char *name, const char *kernel, hipEvent_t *start, hipEvent_t *stop)
 {
-    if (!g_timing) return false;
+    if (!g_timing || g_capturing) return false;
     std::string key = kernel_key(name, kernel);
     if (!g_only.empty() && key != g_only) return false;
     if (g_pend.size() >= 4096) {
@@ -130,16 +150,16 @@ static int dev_alloc(void **p, size_t bytes)
     *p = nullptr;
     if (bytes == 0) bytes = 8;
     HIPCHK(hipMalloc(p, bytes));
-    g_dom_allocs.push_back(*p);
+    g_c->dom_allocs.push_back(*p);
     return 0;
 }
 static void free_domain()
 {
-    for (void *p : g_dom_allocs) (void)hipFree(p);
-    g_dom_allocs.clear();
-    for (auto &s : g_scratch3) s = nullptr;
-    for (auto &s : g_scratch2) s = nullptr;
-    g_have_domain = false;
+    for (void *p : g_c->dom_allocs) (void)hipFree(p);
+    g_c->dom_allocs.clear();
+    for (auto &s : g_c->scratch3) s = nullptr;
+    for (auto &s : g_c->scratch2) s = nullptr;
+    g_c->have_domain = false;
 }
 
 // ---------------------------------------------------------------------------
@@ -159,6 +179,10 @@ struct Stager {
     void *get(const void *user, size_t bytes, bool in, bool out)
     {
         if (!user || !host || status) return (void *)user;
+        if (g_capturing) {
+            status = set_error(NEMO_HIP_ERR_STATE, "host-pointer calls cannot be recorded into a graph");
+            return nullptr;
+        }
         if (slot >= g_pool.size()) g_pool.push_back({nullptr, 0});
         auto &pb = g_pool[slot++];
         if (pb.second < bytes) {
@@ -285,7 +309,7 @@ static int register_domain(const nemo_hip_domain_t *h)
     int rc = check_options(h);
     if (rc) return rc;
     free_domain();
-    g_hdom = *h;
+    g_c->hdom = *h;
     DevDomain d;
     memset(&d, 0, sizeof d);
     const int jpi = h->jpi, jpj = h->jpj, jpk = h->jpk;
@@ -448,16 +472,16 @@ static int register_domain(const nemo_hip_domain_t *h)
     {
         void *base;
         if ((rc = dev_alloc(&base, (size_t)S_NSCRATCH3 * n3 * sizeof(double)))) return rc;
-        for (int s = 0; s < S_NSCRATCH3; s++) g_scratch3[s] = (double *)base + (size_t)s * n3;
+        for (int s = 0; s < S_NSCRATCH3; s++) g_c->scratch3[s] = (double *)base + (size_t)s * n3;
         HIPCHK(hipMemset(base, 0, (size_t)S_NSCRATCH3 * n3 * sizeof(double)));
     }
     for (int s = 0; s < NH_NSCRATCH2; s++)
-        if ((rc = dev_alloc((void **)&g_scratch2[s], n2 * sizeof(double)))) return rc;
-    for (int s = 0; s < NH_NSCRATCH2; s++) HIPCHK(hipMemset(g_scratch2[s], 0, n2 * sizeof(double)));
+        if ((rc = dev_alloc((void **)&g_c->scratch2[s], n2 * sizeof(double)))) return rc;
+    for (int s = 0; s < NH_NSCRATCH2; s++) HIPCHK(hipMemset(g_c->scratch2[s], 0, n2 * sizeof(double)));
     HIPCHK(hipDeviceSynchronize());
-    g_dom = d;
-    g_have_domain = true;
-    g_r2dt_tra = g_r2dt_zdf = g_r2dt_dyn = 2.0 * h->rdt;
+    g_c->dom = d;
+    g_c->have_domain = true;
+    g_c->r2dt_tra = g_c->r2dt_zdf = g_c->r2dt_dyn = 2.0 * h->rdt;
     spg_ts_reset();
     return 0;
 }
@@ -465,8 +489,8 @@ static int register_domain(const nemo_hip_domain_t *h)
 static double r2dt_update(double &r2dt, int kt)
 {
     // tra_adv / tra_zdf / dyn_zdf time step, TRA/traadv.F90:95-97
-    if (g_hdom.neuler == 0 && kt == g_hdom.nit000) r2dt = g_hdom.rdt;
-    else if (kt <= g_hdom.nit000 + 1) r2dt = 2.0 * g_hdom.rdt;
+    if (g_c->hdom.neuler == 0 && kt == g_c->hdom.nit000) r2dt = g_c->hdom.rdt;
+    else if (kt <= g_c->hdom.nit000 + 1) r2dt = 2.0 * g_c->hdom.rdt;
     return r2dt;
 }
 }  // namespace nh
@@ -476,7 +500,18 @@ using namespace nh;
 #define NEED_DOMAIN()                                                                       \
     do {                                                                                    \
         if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called"); \
-        if (!g_have_domain) return set_error(NEMO_HIP_ERR_STATE, "no domain registered");     \
+        if (!g_c->have_domain) return set_error(NEMO_HIP_ERR_STATE, "no domain registered");     \
+    } while (0)
+// calls recorded into a graph replay with the same arguments and time step:
+// the first steps (Euler step and the switch to leapfrog) are refused
+#define NEED_STEADY(kt)                                                                                   \
+    do {                                                                                                  \
+        if (g_capturing && (kt) <= g_c->hdom.nit000 + 1)                                                   \
+            return set_error(NEMO_HIP_ERR_STATE, "kt = %d: the first two steps cannot be recorded into a graph", (kt)); \
+    } while (0)
+#define NEED_NOT_CAPTURING(who)                                                                           \
+    do {                                                                                                  \
+        if (g_capturing) return set_error(NEMO_HIP_ERR_STATE, "%s cannot be called while recording a graph", who); \
     } while (0)
 #define NEED_PTR(p)                                                                   \
     do {                                                                              \
@@ -506,6 +541,8 @@ int nemo_hip_init(int device)
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreate(&g_stream));
     g_device = device;
+    g_ctxs.assign(1, new Ctx());
+    g_c = g_ctxs[0];
     g_init = true;
     return NEMO_HIP_OK;
 }
@@ -513,8 +550,24 @@ int nemo_hip_init(int device)
 int nemo_hip_finalize(void)
 {
     if (!g_init) return NEMO_HIP_OK;
+    if (g_capturing) {
+        hipGraph_t gr = nullptr;
+        (void)hipStreamEndCapture(g_stream, &gr);
+        if (gr) (void)hipGraphDestroy(gr);
+        g_capturing = false;
+    }
     (void)hipStreamSynchronize(g_stream);
-    free_domain();
+    for (auto &e : g_graphs)
+        if (e) (void)hipGraphExecDestroy(e);
+    g_graphs.clear();
+    for (Ctx *c : g_ctxs) {
+        if (!c) continue;
+        g_c = c;
+        free_domain();
+        delete c;
+    }
+    g_ctxs.clear();
+    g_c = nullptr;
     for (auto &p : g_pool) if (p.first) (void)hipFree(p.first);
     g_pool.clear();
     for (auto &f : g_fields) (void)hipFree(f.first);
@@ -546,6 +599,7 @@ int nemo_hip_set_pointer_mode(int mode)
 int nemo_hip_sync(void)
 {
     if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "not initialised");
+    NEED_NOT_CAPTURING("nemo_hip_sync");
     HIPCHK(hipStreamSynchronize(g_stream));
     return NEMO_HIP_OK;
 }
@@ -555,6 +609,7 @@ void *nemo_hip_stream(void) { return (void *)g_stream; }
 int nemo_hip_field_alloc(size_t bytes, void **dev)
 {
     if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "not initialised");
+    NEED_NOT_CAPTURING("nemo_hip_field_alloc");
     if (!dev) return set_error(NEMO_HIP_ERR_ARG, "field_alloc: NULL result pointer");
     *dev = nullptr;
     if (bytes == 0) return set_error(NEMO_HIP_ERR_ARG, "field_alloc: zero bytes");
@@ -567,6 +622,7 @@ int nemo_hip_field_alloc(size_t bytes, void **dev)
 int nemo_hip_field_free(void *dev)
 {
     if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "not initialised");
+    NEED_NOT_CAPTURING("nemo_hip_field_free");
     auto it = g_fields.find(dev);
     if (it == g_fields.end()) return set_error(NEMO_HIP_ERR_ARG, "field_free: %p was not returned by field_alloc", dev);
     HIPCHK(hipStreamSynchronize(g_stream));
@@ -591,6 +647,7 @@ static int field_check(const void *p, size_t bytes, const char *who)
 int nemo_hip_field_push(void *dev, const void *host, size_t bytes)
 {
     if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "not initialised");
+    NEED_NOT_CAPTURING("nemo_hip_field_push");
     if (!host) return set_error(NEMO_HIP_ERR_ARG, "field_push: NULL host pointer");
     int rc = field_check(dev, bytes, "field_push");
     if (rc) return rc;
@@ -601,6 +658,7 @@ int nemo_hip_field_push(void *dev, const void *host, size_t bytes)
 int nemo_hip_field_pull(void *host, const void *dev, size_t bytes)
 {
     if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "not initialised");
+    NEED_NOT_CAPTURING("nemo_hip_field_pull");
     if (!host) return set_error(NEMO_HIP_ERR_ARG, "field_pull: NULL host pointer");
     int rc = field_check(dev, bytes, "field_pull");
     if (rc) return rc;
@@ -612,9 +670,101 @@ int nemo_hip_field_pull(void *host, const void *dev, size_t bytes)
 int nemo_hip_domain_register(const nemo_hip_domain_t *dom)
 {
     if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    NEED_NOT_CAPTURING("nemo_hip_domain_register");
     if (!dom) return set_error(NEMO_HIP_ERR_ARG, "domain is NULL");
     HIPCHK(hipStreamSynchronize(g_stream));
     return register_domain(dom);
+}
+
+// ---------------------------------------------------------------------------
+// contexts
+int nemo_hip_context_create(int *ctx)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    NEED_NOT_CAPTURING("nemo_hip_context_create");
+    if (!ctx) return set_error(NEMO_HIP_ERR_ARG, "context_create: NULL result pointer");
+    for (size_t q = 1; q < g_ctxs.size(); q++)
+        if (!g_ctxs[q]) {
+            g_ctxs[q] = new Ctx();
+            *ctx = (int)q;
+            return NEMO_HIP_OK;
+        }
+    g_ctxs.push_back(new Ctx());
+    *ctx = (int)g_ctxs.size() - 1;
+    return NEMO_HIP_OK;
+}
+int nemo_hip_context_select(int ctx)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    if (ctx < 0 || ctx >= (int)g_ctxs.size() || !g_ctxs[ctx]) return set_error(NEMO_HIP_ERR_ARG, "no context %d", ctx);
+    g_c = g_ctxs[ctx];
+    return NEMO_HIP_OK;
+}
+int nemo_hip_context_destroy(int ctx)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    NEED_NOT_CAPTURING("nemo_hip_context_destroy");
+    if (ctx <= 0 || ctx >= (int)g_ctxs.size() || !g_ctxs[ctx])
+        return set_error(NEMO_HIP_ERR_ARG, "context %d cannot be destroyed (context 0 lives until finalize)", ctx);
+    if (g_ctxs[ctx] == g_c) return set_error(NEMO_HIP_ERR_STATE, "context %d is the current context", ctx);
+    HIPCHK(hipStreamSynchronize(g_stream));
+    Ctx *keep = g_c;
+    g_c = g_ctxs[ctx];
+    free_domain();
+    delete g_c;
+    g_ctxs[ctx] = nullptr;
+    g_c = keep;
+    return NEMO_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// graphs: the launches of device-mode calls recorded on the library stream
+int nemo_hip_graph_begin(void)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    if (g_capturing) return set_error(NEMO_HIP_ERR_STATE, "a graph is already being recorded");
+    if (g_ptr_mode != NEMO_HIP_PTR_DEVICE) return set_error(NEMO_HIP_ERR_STATE, "graphs record device-pointer calls only");
+    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    g_capturing = true;
+    return NEMO_HIP_OK;
+}
+int nemo_hip_graph_end(int *graph)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    if (!g_capturing) return set_error(NEMO_HIP_ERR_STATE, "no graph is being recorded");
+    g_capturing = false;
+    hipGraph_t gr = nullptr;
+    HIPCHK(hipStreamEndCapture(g_stream, &gr));
+    if (!graph) {
+        (void)hipGraphDestroy(gr);
+        return set_error(NEMO_HIP_ERR_ARG, "graph_end: NULL result pointer");
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(gr);
+    if (e != hipSuccess) return set_error(NEMO_HIP_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+    g_graphs.push_back(ex);
+    *graph = (int)g_graphs.size() - 1;
+    return NEMO_HIP_OK;
+}
+int nemo_hip_graph_launch(int graph)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    NEED_NOT_CAPTURING("nemo_hip_graph_launch");
+    if (graph < 0 || graph >= (int)g_graphs.size() || !g_graphs[graph]) return set_error(NEMO_HIP_ERR_ARG, "no graph %d", graph);
+    HIPCHK(hipGraphLaunch(g_graphs[graph], g_stream));
+    return NEMO_HIP_OK;
+}
+int nemo_hip_graph_free(int graph)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    NEED_NOT_CAPTURING("nemo_hip_graph_free");
+    if (graph < 0 || graph >= (int)g_graphs.size() || !g_graphs[graph]) return set_error(NEMO_HIP_ERR_ARG, "no graph %d", graph);
+    HIPCHK(hipStreamSynchronize(g_stream));
+    HIPCHK(hipGraphExecDestroy(g_graphs[graph]));
+    g_graphs[graph] = nullptr;
+    return NEMO_HIP_OK;
 }
 
 int nemo_hip_timing_enable(int on)
@@ -655,7 +805,7 @@ int nemo_hip_dom_msk(double *tmask, double *umask, double *vmask, double *fmask,
                      int32_t *miku, int32_t *mikv, int32_t *mbku, int32_t *mbkv)
 {
     NEED_DOMAIN();
-    const DevDomain &d = g_dom;
+    const DevDomain &d = g_c->dom;
     Stager S;
     double *m3[7] = {S.out(tmask, d.n3), S.out(umask, d.n3), S.out(vmask, d.n3), S.out(fmask, d.n3),
                      S.out(wmask, d.n3), S.out(wumask, d.n3), S.out(wvmask, d.n3)};
@@ -678,7 +828,7 @@ int nemo_hip_lbc_lnk(double *ptab, int nk, char cd_nat, double psgn, double pval
     if (nk < 1) return set_error(NEMO_HIP_ERR_ARG, "lbc_lnk: nk = %d", nk);
     if (!strchr("TUVWF", cd_nat)) return set_error(NEMO_HIP_ERR_ARG, "lbc_lnk: grid type '%c'", cd_nat);
     Stager S;
-    double *p = S.inout(ptab, g_dom.n2 * (size_t)nk);
+    double *p = S.inout(ptab, g_c->dom.n2 * (size_t)nk);
     int rc = S.status ? S.status : launch_lbc<double>(p, nk, cd_nat, pval);
     return S.finish(rc);
 }
@@ -693,7 +843,7 @@ static int do_tra_adv_trp(const T *un, const T *vn, const T *wn, T *zun, T *zvn,
 {
     NEED_DOMAIN();
     NEED_PTR(un); NEED_PTR(vn); NEED_PTR(wn); NEED_PTR(zun); NEED_PTR(zvn); NEED_PTR(zwn);
-    const size_t n3 = g_dom.n3;
+    const size_t n3 = g_c->dom.n3;
     Stager S;
     const T *a = S.in(un, n3), *b = S.in(vn, n3), *c = S.in(wn, n3);
     T *x = S.out(zun, n3), *y = S.out(zvn, n3), *z = S.out(zwn, n3);
@@ -707,6 +857,7 @@ static int do_tra_adv_fct(int kt, int kit000, const char *cdtype, double p2dt, c
 {
     (void)kt; (void)kit000;
     NEED_DOMAIN();
+    NEED_STEADY(kt);
     int is_tra, rc = is_tra_type(cdtype, &is_tra);
     if (rc) return rc;
     NEED_PTR(pun); NEED_PTR(pvn); NEED_PTR(pwn); NEED_PTR(ptb); NEED_PTR(ptn); NEED_PTR(pta);
@@ -714,7 +865,7 @@ static int do_tra_adv_fct(int kt, int kit000, const char *cdtype, double p2dt, c
     if ((kn_fct_h != 2 && kn_fct_h != 4) || (kn_fct_v != 2 && kn_fct_v != 4))
         return set_error(NEMO_HIP_ERR_UNSUPPORTED, "tra_adv_fct: kn_fct_h = %d, kn_fct_v = %d (2 or 4)", kn_fct_h, kn_fct_v);
     if (!(p2dt > 0.0)) return set_error(NEMO_HIP_ERR_ARG, "p2dt = %g", p2dt);
-    const size_t n3 = g_dom.n3, n4 = n3 * (size_t)kjpt;
+    const size_t n3 = g_c->dom.n3, n4 = n3 * (size_t)kjpt;
     Stager S;
     const T *u = S.in(pun, n3), *v = S.in(pvn, n3), *w = S.in(pwn, n3), *tb = S.in(ptb, n4), *tn = S.in(ptn, n4);
     T *ta = S.inout(pta, n4);
@@ -732,7 +883,7 @@ static int do_tra_ldf_lap(int kt, int kit000, const char *cdtype, const T *pahu,
     if (rc) return rc;
     NEED_PTR(pahu); NEED_PTR(pahv); NEED_PTR(ptb); NEED_PTR(pta);
     if (kjpt < 1 || (kpass != 1 && kpass != 2)) return set_error(NEMO_HIP_ERR_ARG, "kjpt = %d, kpass = %d", kjpt, kpass);
-    const size_t n3 = g_dom.n3, n4 = n3 * (size_t)kjpt;
+    const size_t n3 = g_c->dom.n3, n4 = n3 * (size_t)kjpt;
     Stager S;
     const T *a = S.in(pahu, n3), *b = S.in(pahv, n3), *tb = S.in(ptb, n4);
     T *ta = S.inout(pta, n4);
@@ -746,6 +897,9 @@ static int do_tra_ldf_iso(int kt, int kit000, const char *cdtype, const T *pahu,
                           const T *wslpj, T *ah_wslp2, T *akz)
 {
     NEED_DOMAIN();
+    NEED_STEADY(kt);
+    if (g_capturing && kt == kit000)
+        return set_error(NEMO_HIP_ERR_STATE, "tra_ldf_iso: the first call (kt = kit000) cannot be recorded into a graph");
     int is_tra, rc = is_tra_type(cdtype, &is_tra);
     if (rc) return rc;
     NEED_PTR(pahu); NEED_PTR(pahv); NEED_PTR(ptb); NEED_PTR(pta); NEED_PTR(uslp); NEED_PTR(vslp);
@@ -754,8 +908,8 @@ static int do_tra_ldf_iso(int kt, int kit000, const char *cdtype, const T *pahu,
     if (kpass == 2 && !ptbb) return set_error(NEMO_HIP_ERR_ARG, "tra_ldf_iso: kpass = 2 needs ptbb");
     if (!ptbb) ptbb = ptb;
     // time step of traldf_iso.F90:131-133
-    const double z2dt = (g_hdom.neuler == 0 && kt == g_hdom.nit000) ? g_hdom.rdt : 2.0 * g_hdom.rdt;
-    const size_t n3 = g_dom.n3, n4 = n3 * (size_t)kjpt;
+    const double z2dt = (g_c->hdom.neuler == 0 && kt == g_c->hdom.nit000) ? g_c->hdom.rdt : 2.0 * g_c->hdom.rdt;
+    const size_t n3 = g_c->dom.n3, n4 = n3 * (size_t)kjpt;
     Stager S;
     const T *a = S.in(pahu, n3), *b = S.in(pahv, n3), *tb = S.in(ptb, n4);
     const T *tbb = (ptbb == ptb) ? tb : S.in(ptbb, n4);
@@ -774,15 +928,16 @@ static int do_tra_zdf_imp(int kt, int kit000, const char *cdtype, double p2dt, c
 {
     (void)kt; (void)kit000;
     NEED_DOMAIN();
+    NEED_STEADY(kt);
     int is_tra, rc = is_tra_type(cdtype, &is_tra);
     if (rc) return rc;
     NEED_PTR(ptb); NEED_PTR(pta);
     if (kjpt < 1) return set_error(NEMO_HIP_ERR_ARG, "kjpt = %d", kjpt);
     if (is_tra && !avt) return set_error(NEMO_HIP_ERR_ARG, "tra_zdf_imp: avt is NULL");
-    if ((!is_tra || g_dom.ln_zdfddm) && !avs) return set_error(NEMO_HIP_ERR_ARG, "tra_zdf_imp: avs is NULL");
-    const T *add_user = g_dom.ln_traldf_msc ? akz : ah_wslp2;
+    if ((!is_tra || g_c->dom.ln_zdfddm) && !avs) return set_error(NEMO_HIP_ERR_ARG, "tra_zdf_imp: avs is NULL");
+    const T *add_user = g_c->dom.ln_traldf_msc ? akz : ah_wslp2;
     if (l_ldfslp && !add_user) return set_error(NEMO_HIP_ERR_ARG, "tra_zdf_imp: iso-neutral coefficient is NULL");
-    const size_t n3 = g_dom.n3, n4 = n3 * (size_t)kjpt;
+    const size_t n3 = g_c->dom.n3, n4 = n3 * (size_t)kjpt;
     Stager S;
     const T *tb = S.in(ptb, n4);
     T *ta = S.inout(pta, n4);
@@ -803,7 +958,7 @@ static int do_dyn_adv(int which, int kscheme, const T *un, const T *vn, const T 
     NEED_PTR(un); NEED_PTR(vn); NEED_PTR(ua); NEED_PTR(va);
     if ((which & 2 || which == 4) && !wn) return set_error(NEMO_HIP_ERR_ARG, "wn is NULL");
     if (kscheme != 0 && kscheme != 1) return set_error(NEMO_HIP_ERR_ARG, "kscheme = %d", kscheme);
-    const size_t n3 = g_dom.n3;
+    const size_t n3 = g_c->dom.n3;
     Stager S;
     const T *u = S.in(un, n3), *v = S.in(vn, n3), *w = wn ? S.in(wn, n3) : nullptr;
     T *a = S.inout(ua, n3), *b = S.inout(va, n3);
@@ -815,11 +970,11 @@ template <typename T>
 static int dyn_adv_dispatch(const T *un, const T *vn, const T *wn, T *ua, T *va)
 {
     NEED_DOMAIN();
-    switch (g_dom.n_dynadv) {
+    switch (g_c->dom.n_dynadv) {
     case NEMO_NP_LIN_DYN: return NEMO_HIP_OK;
-    case NEMO_NP_VEC_C2: return do_dyn_adv<T>(3, g_dom.nn_dynkeg, un, vn, wn, ua, va);
+    case NEMO_NP_VEC_C2: return do_dyn_adv<T>(3, g_c->dom.nn_dynkeg, un, vn, wn, ua, va);
     case NEMO_NP_FLX_C2: return do_dyn_adv<T>(4, 0, un, vn, wn, ua, va);
-    default: return set_error(NEMO_HIP_ERR_UNSUPPORTED, "dyn_adv: n_dynadv = %d", g_dom.n_dynadv);
+    default: return set_error(NEMO_HIP_ERR_UNSUPPORTED, "dyn_adv: n_dynadv = %d", g_c->dom.n_dynadv);
     }
 }
 
@@ -828,7 +983,7 @@ static int do_dyn_vor(const T *un, const T *vn, T *ua, T *va)
 {
     NEED_DOMAIN();
     NEED_PTR(un); NEED_PTR(vn); NEED_PTR(ua); NEED_PTR(va);
-    const size_t n3 = g_dom.n3;
+    const size_t n3 = g_c->dom.n3;
     Stager S;
     const T *u = S.in(un, n3), *v = S.in(vn, n3);
     T *a = S.inout(ua, n3), *b = S.inout(va, n3);
@@ -842,12 +997,13 @@ static int do_dyn_zdf(int kt, const T *ub, const T *vb, T *ua, T *va, const T *a
                       const double *vtau, const double *vtau_b)
 {
     NEED_DOMAIN();
+    NEED_STEADY(kt);
     NEED_PTR(ub); NEED_PTR(vb); NEED_PTR(ua); NEED_PTR(va); NEED_PTR(avm); NEED_PTR(rCdU_bot);
     NEED_PTR(utau); NEED_PTR(utau_b); NEED_PTR(vtau); NEED_PTR(vtau_b);
-    if (!g_dom.ln_drgimp) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "dyn_zdf: explicit drag (ln_drgimp = F) is not implemented");
-    if (g_dom.ln_dynspg_ts && (!ua_b || !va_b)) return set_error(NEMO_HIP_ERR_ARG, "dyn_zdf: ua_b/va_b are NULL");
-    const double r2dt = r2dt_update(g_r2dt_dyn, kt);
-    const size_t n3 = g_dom.n3, n2 = g_dom.n2;
+    if (!g_c->dom.ln_drgimp) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "dyn_zdf: explicit drag (ln_drgimp = F) is not implemented");
+    if (g_c->dom.ln_dynspg_ts && (!ua_b || !va_b)) return set_error(NEMO_HIP_ERR_ARG, "dyn_zdf: ua_b/va_b are NULL");
+    const double r2dt = r2dt_update(g_c->r2dt_dyn, kt);
+    const size_t n3 = g_c->dom.n3, n2 = g_c->dom.n2;
     Stager S;
     const T *b1 = S.in(ub, n3), *b2 = S.in(vb, n3), *m = S.in(avm, n3);
     T *a1 = S.inout(ua, n3), *a2 = S.inout(va, n3);
@@ -861,15 +1017,16 @@ template <typename T>
 static int do_dyn_spg_ts(int kt, nemo_hip_spg_ts_t *st)
 {
     NEED_DOMAIN();
+    NEED_STEADY(kt);
     if (!st) return set_error(NEMO_HIP_ERR_ARG, "dyn_spg_ts: state is NULL");
-    if (!g_dom.ln_dynspg_ts) return set_error(NEMO_HIP_ERR_STATE, "dyn_spg_ts called with ln_dynspg_ts = F");
+    if (!g_c->dom.ln_dynspg_ts) return set_error(NEMO_HIP_ERR_STATE, "dyn_spg_ts called with ln_dynspg_ts = F");
     const void *req[] = {st->ua, st->va, st->un, st->vn, st->un_b, st->vn_b, st->ub_b, st->vb_b, st->sshn, st->hu_n,
                          st->hv_n, st->r1_hu_n, st->r1_hv_n, st->ht_n, st->rCdU_bot, st->utau, st->vtau, st->emp,
                          st->ua_b, st->va_b, st->ssha, st->un_adv, st->vn_adv, st->ub2_b, st->vb2_b, st->un_bf,
                          st->vn_bf};
     for (const void *p : req)
         if (!p) return set_error(NEMO_HIP_ERR_ARG, "dyn_spg_ts: a required field is NULL");
-    const size_t n3 = g_dom.n3, n2 = g_dom.n2;
+    const size_t n3 = g_c->dom.n3, n2 = g_c->dom.n2;
     Stager S;
     nemo_hip_spg_ts_t D;
     D.ua = S.inout((T *)st->ua, n3); D.va = S.inout((T *)st->va, n3);
@@ -895,21 +1052,22 @@ static int do_step_dyn(int kt, const T *ub, const T *vb, const T *wn, const T *a
                        const double *vtau_b, nemo_hip_spg_ts_t *st)
 {
     NEED_DOMAIN();
+    NEED_STEADY(kt);
     if (g_ptr_mode != NEMO_HIP_PTR_DEVICE)
         return set_error(NEMO_HIP_ERR_STATE, "step_dyn: the fused step works on device-resident fields only");
     if (!st) return set_error(NEMO_HIP_ERR_ARG, "step_dyn: state is NULL");
-    if (!g_dom.ln_dynspg_ts) return set_error(NEMO_HIP_ERR_STATE, "step_dyn needs ln_dynspg_ts = T");
-    if (!g_dom.ln_drgimp) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "step_dyn: explicit drag (ln_drgimp = F) is not implemented");
-    if (!g_dom.ln_bt_fw) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "step_dyn: ln_bt_fw = F is not implemented");
+    if (!g_c->dom.ln_dynspg_ts) return set_error(NEMO_HIP_ERR_STATE, "step_dyn needs ln_dynspg_ts = T");
+    if (!g_c->dom.ln_drgimp) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "step_dyn: explicit drag (ln_drgimp = F) is not implemented");
+    if (!g_c->dom.ln_bt_fw) return set_error(NEMO_HIP_ERR_UNSUPPORTED, "step_dyn: ln_bt_fw = F is not implemented");
     NEED_PTR(ub); NEED_PTR(vb); NEED_PTR(avm); NEED_PTR(utau_b); NEED_PTR(vtau_b);
-    if (!wn && g_dom.n_dynadv != NEMO_NP_LIN_DYN) return set_error(NEMO_HIP_ERR_ARG, "step_dyn: wn is NULL");
+    if (!wn && g_c->dom.n_dynadv != NEMO_NP_LIN_DYN) return set_error(NEMO_HIP_ERR_ARG, "step_dyn: wn is NULL");
     const void *req[] = {st->ua, st->va, st->un, st->vn, st->un_b, st->vn_b, st->ub_b, st->vb_b, st->sshn, st->hu_n,
                          st->hv_n, st->r1_hu_n, st->r1_hv_n, st->ht_n, st->rCdU_bot, st->utau, st->vtau, st->emp,
                          st->ua_b, st->va_b, st->ssha, st->un_adv, st->vn_adv, st->ub2_b, st->vb2_b, st->un_bf,
                          st->vn_bf};
     for (const void *p : req)
         if (!p) return set_error(NEMO_HIP_ERR_ARG, "step_dyn: a required field is NULL");
-    const double r2dt = r2dt_update(g_r2dt_dyn, kt);
+    const double r2dt = r2dt_update(g_c->r2dt_dyn, kt);
     return launch_step_dyn<T>(kt, r2dt, ub, vb, wn, avm, utau_b, vtau_b, st);
 }
 
@@ -919,6 +1077,7 @@ static int do_step_tra(int kt, const T *un, const T *vn, const T *wn, const T *t
                        const T *wslpi, const T *wslpj, T *ah_wslp2, T *akz, const T *avt)
 {
     NEED_DOMAIN();
+    NEED_STEADY(kt);
     if (g_ptr_mode != NEMO_HIP_PTR_DEVICE)
         return set_error(NEMO_HIP_ERR_STATE, "step_tra: the fused step works on device-resident fields only");
     NEED_PTR(un); NEED_PTR(vn); NEED_PTR(wn); NEED_PTR(tsb); NEED_PTR(tsn); NEED_PTR(tsa); NEED_PTR(ahtu);
@@ -927,9 +1086,9 @@ static int do_step_tra(int kt, const T *un, const T *vn, const T *wn, const T *t
     if ((kn_fct_h != 2 && kn_fct_h != 4) || (kn_fct_v != 2 && kn_fct_v != 4))
         return set_error(NEMO_HIP_ERR_ARG, "step_tra: kn_fct_h = %d, kn_fct_v = %d", kn_fct_h, kn_fct_v);
     // r2dt of tra_adv and tra_zdf (TRA/traadv.F90:95-97, TRA/trazdf.F90:66-68), z2dt of traldf_iso.F90:131-133
-    const double p2dt = r2dt_update(g_r2dt_tra, kt);
-    const double p2dt_zdf = r2dt_update(g_r2dt_zdf, kt);
-    const double z2dt = (g_hdom.neuler == 0 && kt == g_hdom.nit000) ? g_hdom.rdt : 2.0 * g_hdom.rdt;
+    const double p2dt = r2dt_update(g_c->r2dt_tra, kt);
+    const double p2dt_zdf = r2dt_update(g_c->r2dt_zdf, kt);
+    const double z2dt = (g_c->hdom.neuler == 0 && kt == g_c->hdom.nit000) ? g_c->hdom.rdt : 2.0 * g_c->hdom.rdt;
     return launch_step_tra<T>(p2dt, z2dt, p2dt_zdf, un, vn, wn, tsb, tsn, tsa, kn_fct_h, kn_fct_v, ahtu, ahtv, uslp,
                               vslp, wslpi, wslpj, avt, ah_wslp2, akz);
 }
@@ -955,8 +1114,8 @@ int nemo_hip_interp_4th_cpt(const double *pt_in, double *pt_out)
     NEED_DOMAIN();
     NEED_PTR(pt_in); NEED_PTR(pt_out);
     Stager S;
-    const double *a = S.in(pt_in, g_dom.n3);
-    double *b = S.inout(pt_out, g_dom.n3);
+    const double *a = S.in(pt_in, g_c->dom.n3);
+    double *b = S.inout(pt_out, g_c->dom.n3);
     int rc = S.status ? S.status : launch_interp_4th_cpt(a, b);
     return S.finish(rc);
 }
@@ -990,8 +1149,8 @@ int nemo_hip_tra_zdf(int kt, const double *tsb, double *tsa, const double *avt, 
                      const double *ah_wslp2, const double *akz)
 {
     NEED_DOMAIN();
-    const double r2dt = r2dt_update(g_r2dt_zdf, kt);
-    return do_tra_zdf_imp<double>(kt, g_hdom.nit000, "TRA", r2dt, tsb, tsa, 2, avt, avs, l_ldfslp, ah_wslp2, akz, 1);
+    const double r2dt = r2dt_update(g_c->r2dt_zdf, kt);
+    return do_tra_zdf_imp<double>(kt, g_c->hdom.nit000, "TRA", r2dt, tsb, tsa, 2, avt, avs, l_ldfslp, ah_wslp2, akz, 1);
 }
 
 int nemo_hip_dyn_keg(int kt, int kscheme, const double *un, const double *vn, double *ua, double *va)
@@ -1046,7 +1205,7 @@ int nemo_hip_ts_wgt(int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt
 {
     NEED_DOMAIN();
     NEED_PTR(jpit); NEED_PTR(zwgt1); NEED_PTR(zwgt2);
-    return spg_ts_weights(g_dom, ll_av, ll_fw, jpit, zwgt1, zwgt2);
+    return spg_ts_weights(g_c->dom, ll_av, ll_fw, jpit, zwgt1, zwgt2);
 }
 
 // ---------------------------------------------------------------------------
@@ -1078,8 +1237,8 @@ int nemo_hip_tra_zdf_sp(int kt, const float *tsb, float *tsa, const float *avt, 
                         const float *ah_wslp2, const float *akz)
 {
     NEED_DOMAIN();
-    const double r2dt = r2dt_update(g_r2dt_zdf, kt);
-    return do_tra_zdf_imp<float>(kt, g_hdom.nit000, "TRA", r2dt, tsb, tsa, 2, avt, avs, l_ldfslp, ah_wslp2, akz, 1);
+    const double r2dt = r2dt_update(g_c->r2dt_zdf, kt);
+    return do_tra_zdf_imp<float>(kt, g_c->hdom.nit000, "TRA", r2dt, tsb, tsa, 2, avt, avs, l_ldfslp, ah_wslp2, akz, 1);
 }
 int nemo_hip_dyn_adv_sp(int kt, const float *un, const float *vn, const float *wn, float *ua, float *va)
 {

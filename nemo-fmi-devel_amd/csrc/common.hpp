@@ -273,6 +273,9 @@ namespace nh {
 int set_error(int code, const char *fmt, ...);
 hipStream_t stream();
 const DevDomain &dom();
+bool capturing();              // a graph is being recorded on the library stream
+int *spg_rot_u();              // dyn_spg_ts level buffers of the current context
+int *spg_rot_s();
 bool have_domain();
 // per-routine timing: when this launch is timed, returns true with the two
 // events its dispatch packet records at kernel start and end

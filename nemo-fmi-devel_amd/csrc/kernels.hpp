@@ -33,7 +33,6 @@ template <typename T> int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st,
 template <typename T>
 int launch_step_dyn(int kt, double r2dt, const T *ub, const T *vb, const T *wn, const T *avm, const double *utau_b,
                     const double *vtau_b, const nemo_hip_spg_ts_t *st);
-void spg_ts_reset();
 template <typename T>
 int launch_step_tra(double p2dt, double z2dt, double p2dt_zdf, const T *un, const T *vn, const T *wn, const T *tsb,
                     const T *tsn, T *tsa, int kn_fct_h, int kn_fct_v, const T *ahtu, const T *ahtv, const T *uslp,

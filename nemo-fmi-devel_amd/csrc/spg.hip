@@ -496,17 +496,33 @@ KERNEL void k_spg_p3b(DevDomain d, double r1_2dt_b, T * __restrict__ ua, T * __r
     }
 }
 
+// phase 2 initialisation, lines 665-702, and the reset of the time sums: the
+// now levels from un_b, vn_b, sshn; with ll_init the before levels at zero
+// (one kernel instead of memset/memcpy calls, so the call can be recorded
+// into a graph)
+KERNEL void k_spg_init(DevDomain d, int ll_init, const double *__restrict__ un_b, const double *__restrict__ vn_b,
+                       const double *__restrict__ sshn, double *__restrict__ ub0, double *__restrict__ vb0,
+                       double *__restrict__ ss0, double *__restrict__ ub1, double *__restrict__ vb1,
+                       double *__restrict__ ss1, double *__restrict__ ub2, double *__restrict__ vb2,
+                       double *__restrict__ ss2, double *__restrict__ ua_b, double *__restrict__ va_b,
+                       double *__restrict__ ssha, double *__restrict__ un_adv, double *__restrict__ vn_adv)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const size_t ij = IJ(i, j);
+    if (ll_init) {
+        ub1[ij] = 0.0; vb1[ij] = 0.0; ss1[ij] = 0.0;
+        ub2[ij] = 0.0; vb2[ij] = 0.0; ss2[ij] = 0.0;
+    }
+    ss0[ij] = sshn[ij];
+    ub0[ij] = un_b[ij];
+    vb0[ij] = vn_b[ij];
+    ua_b[ij] = 0.0; va_b[ij] = 0.0; ssha[ij] = 0.0; un_adv[ij] = 0.0; vn_adv[ij] = 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
-namespace {
-// the *_e before / now levels: three rotating buffers per variable, kept
-// between calls (they carry over when ln_bt_av = F)
-struct SpgState {
-    int rot_u[3] = {0, 1, 2};      // [now, before, before-before]
-    int rot_s[4] = {0, 1, 2, 3};   // [now, before, before-before, after]
-} g_spg;
-}
 
 int spg_ts_weights(const DevDomain &d, int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2)
 {
@@ -562,8 +578,6 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
     w.zssh_frc = s2[6]; w.zwz = s2[7]; w.zhf = s2[8]; w.ftne = s2[9]; w.ftnw = s2[10]; w.ftse = s2[11];
     w.ftsw = s2[12]; w.ua_e = s2[13]; w.va_e = s2[14]; w.zwx = s2[15]; w.zwy = s2[16]; w.zsshp2_e = s2[17];
     double *ube[3] = {s2[18], s2[19], s2[20]}, *vbe[3] = {s2[21], s2[22], s2[23]}, *sse[4] = {s2[24], s2[25], s2[26], nh::scratch2(33)};
-    const size_t b2 = d.n2 * sizeof(double);
-    hipStream_t s = nh::stream();
 
     const double z2dt_bf = (kt == d.nit000 && d.neuler == 0) ? d.rdt : 2.0 * d.rdt;
     const double r1_2dt_b = 1.0 / z2dt_bf;
@@ -586,23 +600,22 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
     LAUNCH("dyn_spg_ts", k_spg_p1b, g, d, st->un_b, st->vn_b, st->hu_n, st->hv_n, st->r1_hu_n, st->r1_hv_n, st->ht_n,
            st->rCdU_bot, st->utau, st->vtau, st->emp, st->rnf, w);
 
-    // phase 2 initialisation, lines 665-702
-    int *ru = g_spg.rot_u, *rs = g_spg.rot_s;
+    // phase 2 initialisation, lines 665-702.  The before levels are set
+    // afresh when ll_init, so the buffers restart from the first rotation
+    // (the same values in the same buffers at every such call, which a
+    // recorded graph relies on); without it the levels of the last call
+    // carry over and the rotation continues
+    int *ru = nh::spg_rot_u(), *rs = nh::spg_rot_s();
     if (ll_init) {
-        for (int q = 1; q <= 2; q++) {
-            HIPCHK(hipMemsetAsync(ube[ru[q]], 0, b2, s));
-            HIPCHK(hipMemsetAsync(vbe[ru[q]], 0, b2, s));
-            HIPCHK(hipMemsetAsync(sse[rs[q]], 0, b2, s));
-        }
+        for (int q = 0; q < 3; q++) ru[q] = q;
+        for (int q = 0; q < 4; q++) rs[q] = q;
+    } else if (nh::capturing()) {
+        return nh::set_error(NEMO_HIP_ERR_STATE,
+                             "dyn_spg_ts: ln_bt_av = F carries the barotropic levels between calls; not recordable");
     }
-    HIPCHK(hipMemcpyAsync(sse[rs[0]], st->sshn, b2, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(ube[ru[0]], st->un_b, b2, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemcpyAsync(vbe[ru[0]], st->vn_b, b2, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipMemsetAsync(st->ua_b, 0, b2, s));
-    HIPCHK(hipMemsetAsync(st->va_b, 0, b2, s));
-    HIPCHK(hipMemsetAsync(st->ssha, 0, b2, s));
-    HIPCHK(hipMemsetAsync(st->un_adv, 0, b2, s));
-    HIPCHK(hipMemsetAsync(st->vn_adv, 0, b2, s));
+    LAUNCH("dyn_spg_ts", k_spg_init, g, d, ll_init, st->un_b, st->vn_b, st->sshn, ube[ru[0]], vbe[ru[0]], sse[rs[0]],
+           ube[ru[1]], vbe[ru[1]], sse[rs[1]], ube[ru[2]], vbe[ru[2]], sse[rs[2]], st->ua_b, st->va_b, st->ssha,
+           st->un_adv, st->vn_adv);
 
     // two sets of the sub-step fluxes: sub-step jn reads set (jn-1)&1 and its
     // velocity kernel writes the extrapolation of sub-step jn+1 into the other
@@ -657,4 +670,4 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
 template int launch_dyn_spg_ts<double>(int, const nemo_hip_spg_ts_t *, int);
 template int launch_dyn_spg_ts<float>(int, const nemo_hip_spg_ts_t *, int);
 
-void spg_ts_reset() { g_spg = SpgState(); }
+

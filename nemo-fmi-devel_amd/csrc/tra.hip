@@ -987,6 +987,21 @@ static int launch_iso_mode(const DevDomain &d, T zsign, const T *pahu, const T *
     return 0;
 }
 
+// whole-array fill and copy (kernels rather than memset/memcpy calls, so
+// the calls can be recorded into a graph)
+template <typename T>
+KERNEL void k_fill3(T *__restrict__ p, size_t n, T v)
+{
+    const size_t q = (size_t)blockIdx.x * NH_BLOCK + threadIdx.x;
+    if (q < n) p[q] = v;
+}
+template <typename T>
+KERNEL void k_copy3(T *__restrict__ dst, const T *__restrict__ src, size_t n)
+{
+    const size_t q = (size_t)blockIdx.x * NH_BLOCK + threadIdx.x;
+    if (q < n) dst[q] = src[q];
+}
+
 template <typename T>
 int launch_tra_ldf_iso(int first, double z2dt, const T *pahu, const T *pahv, const T *ptb, const T *ptbb, T *pta,
                        int kjpt, int kpass, const T *uslp, const T *vslp, const T *wslpi, const T *wslpj, T *ah_wslp2,
@@ -994,15 +1009,18 @@ int launch_tra_ldf_iso(int first, double z2dt, const T *pahu, const T *pahv, con
 {
     const DevDomain &d = nh::dom();
     if (kpass == 1) {
+        const unsigned nb = (unsigned)((d.n3 + NH_BLOCK - 1) / NH_BLOCK);
+        Grid2 g1;
+        g1.block = dim3(NH_BLOCK, 1, 1);
+        g1.grid = dim3(nb, 1, 1);
         if (first) {
-            HIPCHK(hipMemsetAsync(akz, 0, d.n3 * sizeof(T), nh::stream()));
-            HIPCHK(hipMemsetAsync(ah_wslp2, 0, d.n3 * sizeof(T), nh::stream()));
+            LAUNCH("tra_ldf_iso", k_fill3<T>, g1, akz, d.n3, T(0));
+            LAUNCH("tra_ldf_iso", k_fill3<T>, g1, ah_wslp2, d.n3, T(0));
         }
         Grid2 gh = grid2(d.jpi - 1, d.jpj - 1);
         gh.grid.z = (unsigned)(d.jpk - 2);
         LAUNCH("tra_ldf_iso", k_ldf_iso_coef<T>, gh, d, z2dt, pahu, pahv, wslpi, wslpj, ah_wslp2, akz);
-        if (!d.ln_traldf_msc)
-            HIPCHK(hipMemcpyAsync(akz, ah_wslp2, d.n3 * sizeof(T), hipMemcpyDeviceToDevice, nh::stream()));
+        if (!d.ln_traldf_msc) LAUNCH("tra_ldf_iso", k_copy3<T>, g1, akz, (const T *)ah_wslp2, d.n3);
     }
     const T zsign = (kpass == 1) ? T(1) : T(-1);
     if (d.ln_traldf_lap)

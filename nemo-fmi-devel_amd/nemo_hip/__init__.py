@@ -112,6 +112,32 @@ class NemoHip:
             raise NemoHipError(abi.NEMO_HIP_ERR_ARG, "host arrays must be Fortran-contiguous")
         return C.c_void_p(a.ctypes.data)
 
+    # contexts and graphs -------------------------------------------------
+    def context_create(self):
+        c = C.c_int()
+        self._check(self.L.nemo_hip_context_create(C.byref(c)))
+        return c.value
+
+    def context_select(self, ctx):
+        self._check(self.L.nemo_hip_context_select(C.c_int(ctx)))
+
+    def context_destroy(self, ctx):
+        self._check(self.L.nemo_hip_context_destroy(C.c_int(ctx)))
+
+    def graph_begin(self):
+        self._check(self.L.nemo_hip_graph_begin())
+
+    def graph_end(self):
+        g = C.c_int()
+        self._check(self.L.nemo_hip_graph_end(C.byref(g)))
+        return g.value
+
+    def graph_launch(self, g):
+        self._check(self.L.nemo_hip_graph_launch(C.c_int(g)))
+
+    def graph_free(self, g):
+        self._check(self.L.nemo_hip_graph_free(C.c_int(g)))
+
     # resident fields -----------------------------------------------------
     def field_alloc(self, nbytes):
         p = C.c_void_p()

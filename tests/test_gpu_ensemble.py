@@ -1,0 +1,86 @@
+"""An ensemble of domains of different sizes (one library context each),
+stepped in FP32 storage by the fused step entry points, recorded once into
+a HIP graph and replayed: every replay must give, bit for bit, what the same
+calls give when issued one by one.  The replays are interleaved with eager
+work and synchronisations, and the recording refuses the calls it cannot
+replay faithfully."""
+import numpy as np
+import pytest
+
+import nemo_hip
+from nemo_hip import abi, configs
+
+torch = pytest.importorskip("torch")
+
+# (jpi, jpj, jpk, jperio, seed): closed and cyclic members of different shapes
+MEMBERS = [(40, 30, 12, 0, 3), (36, 28, 10, 7, 5), (52, 20, 14, 0, 6)]
+OPTS = dict(nvor_scheme=abi.NP_ENE, n_dynadv=1, nn_dynkeg=0, ln_traldf_msc=0, nn_baro=30, rdt=600.0, neuler=0,
+            nit000=1)
+
+
+def _stepper(h, d, seed):
+    from bench import Stepper
+    S = configs.make_state_device(d, seed=seed)
+    return Stepper(h, d, S, True, True)
+
+
+def _state(st):
+    out = {"tsa": st.tsa, "ua": st.ua, "va": st.va, "un": st.un_dyn, "vn": st.vn_dyn}
+    for k in ("ua_b", "va_b", "ssha", "un_adv", "vn_adv", "ub2_b", "vb2_b", "un_bf", "vn_bf"):
+        out[k] = st.f2[k]
+    return {k: v.detach().cpu().numpy().copy() for k, v in out.items()}
+
+
+@pytest.mark.gpu
+def test_ensemble_graph_replays_match_eager_calls(hip):
+    ctxs, eager, graph = [], [], []
+    g = None
+    try:
+        for (jpi, jpj, jpk, jperio, seed) in MEMBERS:
+            c = hip.context_create()
+            ctxs.append(c)
+            hip.context_select(c)
+            d = configs.basin(jpi, jpj, jpk, seed=seed, jperio=jperio, opts=dict(OPTS))
+            hip.register(d)
+            eager.append(_stepper(hip, d, seed + 100))
+            graph.append(_stepper(hip, d, seed + 100))
+        # steps 1 and 2 (Euler start, switch to leapfrog) are issued eagerly
+        for kt in (1, 2):
+            for c, a, b in zip(ctxs, eager, graph):
+                hip.context_select(c)
+                a.step(kt)
+                b.step(kt)
+        hip.sync()
+        # what recording refuses
+        hip.graph_begin()
+        hip.context_select(ctxs[0])
+        with pytest.raises(nemo_hip.NemoHipError, match="first two steps"):
+            graph[0].step(2)
+        with pytest.raises(nemo_hip.NemoHipError, match="recording"):
+            hip.sync()
+        hip.graph_free(hip.graph_end())
+        # one ensemble step recorded, then replayed for kt = 3, 4, 5
+        hip.graph_begin()
+        for c, b in zip(ctxs, graph):
+            hip.context_select(c)
+            b.step(3)
+        g = hip.graph_end()
+        for kt in (3, 4, 5):
+            for c, a in zip(ctxs, eager):
+                hip.context_select(c)
+                a.step(kt)
+            hip.graph_launch(g)
+            hip.sync()
+            torch.cuda.synchronize()
+            for m, (a, b) in enumerate(zip(eager, graph)):
+                sa, sb = _state(a), _state(b)
+                for k in sa:
+                    assert np.all(np.isfinite(sa[k])), f"member {m} {k} kt {kt}: non-finite"
+                    assert np.array_equal(sa[k], sb[k]), f"member {m} field {k} differs after replay at kt {kt}"
+    finally:
+        if g is not None:
+            hip.graph_free(g)
+        hip.context_select(0)
+        del eager, graph
+        for c in ctxs:
+            hip.context_destroy(c)
