@@ -57,6 +57,10 @@ def parse():
     ap.add_argument("--cpu-sample", default="512x512x75")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ensemble", type=int, default=0, metavar="MEMBERS",
+                    help="also step an ensemble of MEMBERS domains (i x j drawn from 256x320 .. 768x640, 56 levels, "
+                         "FP32 fused step, one library context each), eagerly and as one recorded HIP graph per "
+                         "step: reported apart, never the headline value")
     ap.add_argument("--host-mode", type=int, default=0, metavar="STEPS",
                     help="also time STEPS steps of the FP64 routines in host-pointer mode (NEMO's host arrays, "
                          "every call staged over PCIe): reported apart, never the headline value")
@@ -217,6 +221,61 @@ def time_host_mode(h, d, S, steps):
     finally:
         h.set_pointer_mode(nemo_hip.NEMO_HIP_PTR_DEVICE)
     return ms
+
+
+def time_ensemble(h, members, steps, seed):
+    """ms per ensemble step, issued call by call and replayed from one
+    recorded graph, and the ensemble's cells."""
+    import numpy as np
+    import torch
+    from nemo_hip import configs
+    rng = np.random.default_rng(seed)
+    ctxs, st, cells = [], [], 0
+    for m in range(members):
+        jpi = int(rng.integers(256, 769)) // 8 * 8
+        jpj = int(rng.integers(320, 641)) // 8 * 8
+        c = h.context_create()
+        ctxs.append(c)
+        h.context_select(c)
+        d = build_case(jpi, jpj, 56, seed + 7 * m)
+        h.register(d)
+        st.append(Stepper(h, d, configs.make_state_device(d, seed=seed + 7 * m + 1), True, True))
+        cells += jpi * jpj * 56
+    ext = torch.cuda.ExternalStream(h.stream())
+    try:
+        with torch.cuda.stream(ext):
+            for kt in (1, 2, 3):
+                for c, s_ in zip(ctxs, st):
+                    h.context_select(c)
+                    s_.step(kt)
+            h.sync()
+            t0 = time.perf_counter()
+            for kt in range(4, 4 + steps):
+                for c, s_ in zip(ctxs, st):
+                    h.context_select(c)
+                    s_.step(kt)
+            h.sync()
+            eager_ms = (time.perf_counter() - t0) * 1e3 / steps
+            h.graph_begin()
+            for c, s_ in zip(ctxs, st):
+                h.context_select(c)
+                s_.step(4 + steps)
+            g = h.graph_end()
+            h.graph_launch(g)
+            h.sync()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                h.graph_launch(g)
+            h.sync()
+            graph_ms = (time.perf_counter() - t0) * 1e3 / steps
+            h.graph_free(g)
+    finally:
+        h.context_select(0)
+        del st
+        torch.cuda.empty_cache()
+        for c in ctxs:
+            h.context_destroy(c)
+    return eager_ms, graph_ms, cells
 
 
 def kernel_times(h):
@@ -407,6 +466,13 @@ def main():
         host_ms = time_host_mode(h, d, S, a.host_mode)
         log(f"host-pointer mode (FP64 routines, PCIe-inclusive): {host_ms:.1f} ms/step, "
             f"{cells / (host_ms * 1e-3):.3e} cell-updates/s")
+    ens = None
+    if a.ensemble:
+        e_ms, g_ms, e_cells = time_ensemble(h, a.ensemble, 5, a.seed + 11)
+        ens = {"members": a.ensemble, "cells": e_cells, "eager_ms_per_step": e_ms, "graph_ms_per_step": g_ms,
+               "graph_cell_updates_per_s": e_cells / (g_ms * 1e-3)}
+        log(f"ensemble of {a.ensemble} (FP32 fused, {e_cells:.3e} cells): {e_ms:.1f} ms/step call by call, "
+            f"{g_ms:.1f} ms/step from one graph, {e_cells / (g_ms * 1e-3):.3e} cell-updates/s")
     copy_gbs = stream_copy()
     log(f"device copy (2 GiB, read + write): {copy_gbs:.0f} GB/s")
     kt_, kb_ = d.arrays["k_top"], d.arrays["k_bot"]
@@ -433,6 +499,7 @@ def main():
         "wet_cell_updates_per_s": wet_cells / (h_ms * 1e-3),
         "stream_copy_GBps": copy_gbs,
         "host_mode_fp64_ms_per_step": host_ms,
+        "ensemble": ens,
         "forms": res,
         "roofline": roof,
         "cpu_baseline": cpu,
