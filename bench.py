@@ -554,9 +554,9 @@ KERNEL_BYTES = {
     # barotropic sub-step kernels (2D, FP64 in both forms; ENE scheme)
     # S3: reads zwx zwy zwz zCdU_u zCdU_v un_e vn_e zu_frc zv_frc zsshp2_e
     # ub_e(-1) vb_e(-1) hu_n hv_n r1_hu_n r1_hv_n r1_e1u r1_e2v e2u e1v r1_e2u
-    # r1_e1v and the sums ua_b va_b un_adv vn_adv; writes ua_e va_e zwx' zwy'
-    # and the four sums: 26 + 8 arrays
-    "k_spg_s3": lambda w, i, j, k, f: 34 * 8 * i * j,
+    # r1_e1v and the sums ua_b va_b un_adv vn_adv; writes zwx' zwy' and the
+    # four sums (ua_e va_e only for the t-point scheme): 26 + 6 arrays
+    "k_spg_s3": lambda w, i, j, k, f: 32 * 8 * i * j,
     # S2: reads zwx zwy zssh_frc r1_e1e2t sshn_e sshb_e sshbb_e ssha; writes
     # ssha_e ssha zsshp2_e: 8 + 3 arrays
     "k_spg_s2": lambda w, i, j, k, f: 11 * 8 * i * j,
