@@ -304,6 +304,34 @@ int nemo_hip_dyn_spg_ts(int kt, nemo_hip_spg_ts_t *st);
 int nemo_hip_ts_wgt(int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2);
 
 /* ------------------------------------------------------------------------ */
+/* time-step closure around the hot path (linear free surface)              */
+/* ------------------------------------------------------------------------ */
+/* div_hor, DYN/divhor.F90:50-110: hdivn (jpi,jpj,jpk) on levels 1..jpkm1,
+   then lbc_lnk('T') */
+int nemo_hip_div_hor(int kt, const double *un, const double *vn, double *hdivn);
+/* ssh_nxt, DYN/sshwzv.F90:56-126: div_hor into hdivn, then the after sea
+   level ssha from sshb, emp_b, emp (2D, FP64) */
+int nemo_hip_ssh_nxt(int kt, const double *un, const double *vn, double *hdivn,
+                     const double *sshb, const double *emp_b, const double *emp,
+                     double *ssha);
+/* wzv, DYN/sshwzv.F90:129-213: wn from hdivn (wn(jpk) = 0 set at nit000) */
+int nemo_hip_wzv(int kt, const double *hdivn, double *wn);
+/* ssh_swp, DYN/sshwzv.F90:216-267: Asselin filter of sshb, sshn = ssha */
+int nemo_hip_ssh_swp(int kt, double *sshb, double *sshn, const double *ssha);
+/* tra_nxt, TRA/tranxt.F90:66-192 (tra_nxt_fix, lines 195-237) on the two
+   active tracers (jpi,jpj,jpk,2) */
+int nemo_hip_tra_nxt(int kt, double *tsb, double *tsn, double *tsa);
+/* dyn_nxt, DYN/dynnxt.F90:66-377.  ua_b/va_b (and with ln_bt_fw = F
+   un_adv/vn_adv) are the dyn_spg_ts outputs; ub_b, vb_b, un_b, vn_b are
+   updated; r1_hu_n/r1_hv_n stand for r1_hu_a and r1_hu_b, which equal them
+   when the scale factors are fixed */
+int nemo_hip_dyn_nxt(int kt, double *ub, double *vb, double *un, double *vn,
+                     double *ua, double *va, const double *ua_b, const double *va_b,
+                     const double *un_adv, const double *vn_adv,
+                     double *ub_b, double *vb_b, double *un_b, double *vn_b,
+                     const double *r1_hu_n, const double *r1_hv_n);
+
+/* ------------------------------------------------------------------------ */
 /* Fused steps on device-resident fields (pointer mode NEMO_HIP_PTR_DEVICE). */
 /* Each equals, bit for bit, the sequence of the separate entry points in   */
 /* the order of the time step, step.F90:179-257.                            */
@@ -374,6 +402,18 @@ int nemo_hip_step_tra_sp(int kt, const float *un, const float *vn, const float *
                          const float *uslp, const float *vslp,
                          const float *wslpi, const float *wslpj,
                          float *ah_wslp2, float *akz, const float *avt);
+/* time-step closure: float 3D fields, FP64 2D fields and arithmetic */
+int nemo_hip_div_hor_sp(int kt, const float *un, const float *vn, float *hdivn);
+int nemo_hip_ssh_nxt_sp(int kt, const float *un, const float *vn, float *hdivn,
+                        const double *sshb, const double *emp_b, const double *emp,
+                        double *ssha);
+int nemo_hip_wzv_sp(int kt, const float *hdivn, float *wn);
+int nemo_hip_tra_nxt_sp(int kt, float *tsb, float *tsn, float *tsa);
+int nemo_hip_dyn_nxt_sp(int kt, float *ub, float *vb, float *un, float *vn,
+                        float *ua, float *va, const double *ua_b, const double *va_b,
+                        const double *un_adv, const double *vn_adv,
+                        double *ub_b, double *vb_b, double *un_b, double *vn_b,
+                        const double *r1_hu_n, const double *r1_hv_n);
 
 #ifdef __cplusplus
 }

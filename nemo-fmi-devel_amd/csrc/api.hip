@@ -1094,6 +1094,102 @@ static int do_step_tra(int kt, const T *un, const T *vn, const T *wn, const T *t
 }
 
 // ---------------------------------------------------------------------------
+// time-step closure: div_hor, ssh_nxt, wzv, ssh_swp, tra_nxt, dyn_nxt
+// ---------------------------------------------------------------------------
+static bool euler_step(int kt) { return g_c->hdom.neuler == 0 && kt == g_c->hdom.nit000; }
+
+template <typename T>
+static int do_div_hor(int kt, const T *un, const T *vn, T *hdivn)
+{
+    (void)kt;
+    NEED_DOMAIN();
+    NEED_PTR(un); NEED_PTR(vn); NEED_PTR(hdivn);
+    const size_t n3 = g_c->dom.n3;
+    Stager S;
+    const T *u = S.in(un, n3), *v = S.in(vn, n3);
+    T *h = S.inout(hdivn, n3);
+    int rc = S.status ? S.status : launch_div_hor<T>(u, v, h);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_ssh_nxt(int kt, const T *un, const T *vn, T *hdivn, const double *sshb, const double *emp_b,
+                      const double *emp, double *ssha)
+{
+    NEED_DOMAIN();
+    NEED_STEADY(kt);
+    NEED_PTR(un); NEED_PTR(vn); NEED_PTR(hdivn); NEED_PTR(sshb); NEED_PTR(emp_b); NEED_PTR(emp); NEED_PTR(ssha);
+    // DYN/sshwzv.F90:85-87
+    const double z2dt = euler_step(kt) ? g_c->hdom.rdt : 2.0 * g_c->hdom.rdt;
+    const double zcoef = 0.5 * (1.0 / g_c->hdom.rau0);
+    const size_t n3 = g_c->dom.n3, n2 = g_c->dom.n2;
+    Stager S;
+    const T *u = S.in(un, n3), *v = S.in(vn, n3);
+    T *h = S.inout(hdivn, n3);
+    const double *b = S.in(sshb, n2), *e0 = S.in(emp_b, n2), *e1 = S.in(emp, n2);
+    double *a = S.out(ssha, n2);
+    int rc = S.status ? S.status : launch_ssh_nxt<T>(z2dt, zcoef, u, v, h, b, e0, e1, a);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_wzv(int kt, const T *hdivn, T *wn)
+{
+    NEED_DOMAIN();
+    NEED_STEADY(kt);
+    NEED_PTR(hdivn); NEED_PTR(wn);
+    // DYN/sshwzv.F90:163-164
+    const double z1_2dt = euler_step(kt) ? 1. / g_c->hdom.rdt : 1. / (2. * g_c->hdom.rdt);
+    const size_t n3 = g_c->dom.n3;
+    Stager S;
+    const T *h = S.in(hdivn, n3);
+    T *w = S.inout(wn, n3);
+    int rc = S.status ? S.status : launch_wzv<T>(kt == g_c->hdom.nit000, z1_2dt, h, w);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_tra_nxt(int kt, T *tsb, T *tsn, T *tsa)
+{
+    NEED_DOMAIN();
+    NEED_STEADY(kt);
+    NEED_PTR(tsb); NEED_PTR(tsn); NEED_PTR(tsa);
+    const size_t n4 = g_c->dom.n3 * 2;
+    Stager S;
+    T *b = S.inout(tsb, n4), *n = S.inout(tsn, n4), *a = S.inout(tsa, n4);
+    int rc = S.status ? S.status : launch_tra_nxt<T>(euler_step(kt), b, n, a);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_dyn_nxt(int kt, T *ub, T *vb, T *un, T *vn, T *ua, T *va, const double *ua_b, const double *va_b,
+                      const double *un_adv, const double *vn_adv, double *ub_b, double *vb_b, double *un_b,
+                      double *vn_b, const double *r1_hu_n, const double *r1_hv_n)
+{
+    NEED_DOMAIN();
+    NEED_STEADY(kt);
+    NEED_PTR(ub); NEED_PTR(vb); NEED_PTR(un); NEED_PTR(vn); NEED_PTR(ua); NEED_PTR(va);
+    NEED_PTR(ub_b); NEED_PTR(vb_b); NEED_PTR(un_b); NEED_PTR(vn_b); NEED_PTR(r1_hu_n); NEED_PTR(r1_hv_n);
+    const int ts = g_c->dom.ln_dynspg_ts;
+    if (ts && (!ua_b || !va_b)) return set_error(NEMO_HIP_ERR_ARG, "dyn_nxt: ua_b/va_b are NULL");
+    if (ts && !g_c->dom.ln_bt_fw && (!un_adv || !vn_adv))
+        return set_error(NEMO_HIP_ERR_ARG, "dyn_nxt: un_adv/vn_adv are NULL (ln_bt_fw = F)");
+    const size_t n3 = g_c->dom.n3, n2 = g_c->dom.n2;
+    Stager S;
+    T *b1 = S.inout(ub, n3), *b2 = S.inout(vb, n3), *n1 = S.inout(un, n3), *n2_ = S.inout(vn, n3);
+    T *a1 = S.inout(ua, n3), *a2 = S.inout(va, n3);
+    const double *uab = ts ? S.in(ua_b, n2) : nullptr, *vab = ts ? S.in(va_b, n2) : nullptr;
+    const bool adv = ts && !g_c->dom.ln_bt_fw;
+    const double *uad = adv ? S.in(un_adv, n2) : nullptr, *vad = adv ? S.in(vn_adv, n2) : nullptr;
+    double *ubb = S.inout(ub_b, n2), *vbb = S.inout(vb_b, n2), *unb = S.inout(un_b, n2), *vnb = S.inout(vn_b, n2);
+    const double *rh = S.in(r1_hu_n, n2), *rv = S.in(r1_hv_n, n2);
+    int rc = S.status ? S.status
+                      : launch_dyn_nxt<T>(euler_step(kt), b1, b2, n1, n2_, a1, a2, uab, vab, uad, vad, ubb, vbb, unb,
+                                          vnb, rh, rv);
+    return S.finish(rc);
+}
+
+// ---------------------------------------------------------------------------
 // FP64 entry points
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -1201,6 +1297,38 @@ int nemo_hip_step_tra(int kt, const double *un, const double *vn, const double *
                                wslpj, ah_wslp2, akz, avt);
 }
 
+int nemo_hip_div_hor(int kt, const double *un, const double *vn, double *hdivn)
+{
+    return do_div_hor<double>(kt, un, vn, hdivn);
+}
+int nemo_hip_ssh_nxt(int kt, const double *un, const double *vn, double *hdivn, const double *sshb,
+                     const double *emp_b, const double *emp, double *ssha)
+{
+    return do_ssh_nxt<double>(kt, un, vn, hdivn, sshb, emp_b, emp, ssha);
+}
+int nemo_hip_wzv(int kt, const double *hdivn, double *wn) { return do_wzv<double>(kt, hdivn, wn); }
+int nemo_hip_ssh_swp(int kt, double *sshb, double *sshn, const double *ssha)
+{
+    NEED_DOMAIN();
+    NEED_STEADY(kt);
+    NEED_PTR(sshb); NEED_PTR(sshn); NEED_PTR(ssha);
+    const size_t n2 = g_c->dom.n2;
+    Stager S;
+    double *b = S.inout(sshb, n2), *n = S.inout(sshn, n2);
+    const double *a = S.in(ssha, n2);
+    int rc = S.status ? S.status : launch_ssh_swp(euler_step(kt), b, n, a);
+    return S.finish(rc);
+}
+int nemo_hip_tra_nxt(int kt, double *tsb, double *tsn, double *tsa) { return do_tra_nxt<double>(kt, tsb, tsn, tsa); }
+int nemo_hip_dyn_nxt(int kt, double *ub, double *vb, double *un, double *vn, double *ua, double *va,
+                     const double *ua_b, const double *va_b, const double *un_adv, const double *vn_adv,
+                     double *ub_b, double *vb_b, double *un_b, double *vn_b, const double *r1_hu_n,
+                     const double *r1_hv_n)
+{
+    return do_dyn_nxt<double>(kt, ub, vb, un, vn, ua, va, ua_b, va_b, un_adv, vn_adv, ub_b, vb_b, un_b, vn_b,
+                              r1_hu_n, r1_hv_n);
+}
+
 int nemo_hip_ts_wgt(int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2)
 {
     NEED_DOMAIN();
@@ -1271,6 +1399,26 @@ int nemo_hip_step_tra_sp(int kt, const float *un, const float *vn, const float *
 {
     return do_step_tra<float>(kt, un, vn, wn, tsb, tsn, tsa, kn_fct_h, kn_fct_v, ahtu, ahtv, uslp, vslp, wslpi,
                               wslpj, ah_wslp2, akz, avt);
+}
+
+int nemo_hip_div_hor_sp(int kt, const float *un, const float *vn, float *hdivn)
+{
+    return do_div_hor<float>(kt, un, vn, hdivn);
+}
+int nemo_hip_ssh_nxt_sp(int kt, const float *un, const float *vn, float *hdivn, const double *sshb,
+                        const double *emp_b, const double *emp, double *ssha)
+{
+    return do_ssh_nxt<float>(kt, un, vn, hdivn, sshb, emp_b, emp, ssha);
+}
+int nemo_hip_wzv_sp(int kt, const float *hdivn, float *wn) { return do_wzv<float>(kt, hdivn, wn); }
+int nemo_hip_tra_nxt_sp(int kt, float *tsb, float *tsn, float *tsa) { return do_tra_nxt<float>(kt, tsb, tsn, tsa); }
+int nemo_hip_dyn_nxt_sp(int kt, float *ub, float *vb, float *un, float *vn, float *ua, float *va,
+                        const double *ua_b, const double *va_b, const double *un_adv, const double *vn_adv,
+                        double *ub_b, double *vb_b, double *un_b, double *vn_b, const double *r1_hu_n,
+                        const double *r1_hv_n)
+{
+    return do_dyn_nxt<float>(kt, ub, vb, un, vn, ua, va, ua_b, va_b, un_adv, vn_adv, ub_b, vb_b, un_b, vn_b,
+                             r1_hu_n, r1_hv_n);
 }
 
 }  // extern "C"

@@ -37,3 +37,16 @@ template <typename T>
 int launch_step_tra(double p2dt, double z2dt, double p2dt_zdf, const T *un, const T *vn, const T *wn, const T *tsb,
                     const T *tsn, T *tsa, int kn_fct_h, int kn_fct_v, const T *ahtu, const T *ahtv, const T *uslp,
                     const T *vslp, const T *wslpi, const T *wslpj, const T *avt, T *ah_wslp2, T *akz);
+
+// time-step closure (nxt.hip)
+template <typename T> int launch_div_hor(const T *un, const T *vn, T *hdivn);
+template <typename T>
+int launch_ssh_nxt(double z2dt, double zcoef, const T *un, const T *vn, T *hdivn, const double *sshb,
+                   const double *emp_b, const double *emp, double *ssha);
+template <typename T> int launch_wzv(int first, double z1_2dt, const T *hdivn, T *wn);
+int launch_ssh_swp(int euler, double *sshb, double *sshn, const double *ssha);
+template <typename T> int launch_tra_nxt(int euler, T *tsb, T *tsn, T *tsa);
+template <typename T>
+int launch_dyn_nxt(int euler, T *ub, T *vb, T *un, T *vn, T *ua, T *va, const double *ua_b, const double *va_b,
+                   const double *un_adv, const double *vn_adv, double *ub_b, double *vb_b, double *un_b,
+                   double *vn_b, const double *r1_hu, const double *r1_hv);

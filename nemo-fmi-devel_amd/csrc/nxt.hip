@@ -1,0 +1,382 @@
+// nxt.hip -- the routines that close a time step around the hot path:
+// horizontal divergence (DYN/divhor.F90), sea-level prediction, vertical
+// velocity and sea-level swap (DYN/sshwzv.F90), the tracer and momentum
+// Asselin filters and swaps (TRA/tranxt.F90, DYN/dynnxt.F90), for the linear
+// free surface (e3 fixed in time).
+//
+// All of them are HBM-bound streams of a few fields.  Arithmetic is done in
+// FP64 and rounded once on store, so the FP64 form repeats the reference's
+// operations and the FP32 storage form loses nothing beyond its storage
+// rounding.
+//
+// lbc_lnk is folded into the kernels.  A read-only input (un, vn for the
+// divergence) lets a ring point compute its value at its lbc source point.
+// Fields updated in place (tsb/tsn/tsa, ua) are written to the ring by the
+// thread of the interior source point (lbc_targets), so no thread reads a
+// value another thread of the same launch overwrites.
+#include "common.hpp"
+#include "kernels.hpp"
+
+// the ring points whose serial lbc_lnk source ('T', 'U' or 'V' grid) is the
+// interior point (i,j): the inverse of lbc_src, at most three points
+template <typename F>
+__device__ __forceinline__ void lbc_targets(const DevDomain &d, int i, int j, F f)
+{
+    if (i != 2 && i != d.jpi - 1 && j != 2 && j != d.jpj - 1) return;
+    const int ci[3] = {i, 1, d.jpi}, cj[3] = {j, 1, d.jpj};
+    for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) {
+            if (a == 0 && b == 0) continue;
+            int si, sj;
+            if (lbc_src(d, 'T', ci[a], cj[b], si, sj) && si == i && sj == j) f(ci[a], cj[b]);
+        }
+}
+
+// horizontal divergence at an interior point, DYN/divhor.F90:79-83
+template <typename T>
+__device__ __forceinline__ double hdiv_at(const DevDomain &d, const T *un, const T *vn, int i, int j, int k)
+{
+    const Metrics<double> &m = d.md;
+    const uint32_t ij = IJ32(i, j);
+    return ((double)A2(m.e2u, i, j) * E3(m.e3u, ij, k) * (double)A3(un, i, j, k) -
+            (double)A2(m.e2u, i - 1, j) * E3(m.e3u, ij - 1, k) * (double)A3(un, i - 1, j, k) +
+            (double)A2(m.e1v, i, j) * E3(m.e3v, ij, k) * (double)A3(vn, i, j, k) -
+            (double)A2(m.e1v, i, j - 1) * E3(m.e3v, ij - d.jpi, k) * (double)A3(vn, i, j - 1, k)) *
+           A2(m.r1_e1e2t, i, j) / E3(m.e3t, ij, k);
+}
+
+// ---------------------------------------------------------------------------
+// div_hor, DYN/divhor.F90:50-110: hdivn on levels 1..jpkm1 of the interior,
+// then lbc_lnk('T') of all jpk levels.  One thread per cell.
+// ---------------------------------------------------------------------------
+template <typename T>
+KERNEL void k_div_hor(DevDomain d, const T *__restrict__ un, const T *__restrict__ vn, T *__restrict__ hdivn)
+{
+    THREAD_IJ(1, 1);
+    const int k = (int)blockIdx.z + 1;
+    if (i > d.jpi || j > d.jpj) return;
+    int si, sj;
+    const bool ok = lbc_src(d, 'T', i, j, si, sj);
+    if (k == d.jpk) {   // level jpk is not computed: lbc_lnk only
+        if (si != i || sj != j || !ok) A3(hdivn, i, j, k) = ok ? A3(hdivn, si, sj, k) : T(0);
+        return;
+    }
+    A3(hdivn, i, j, k) = ok ? (T)hdiv_at<T>(d, un, vn, si, sj, k) : T(0);
+}
+
+// ---------------------------------------------------------------------------
+// ssh_nxt, DYN/sshwzv.F90:56-126: div_hor, then the after sea level from the
+// depth-integrated divergence.  One thread per column: the column computes
+// its hdivn (at its lbc source point on the ring), the integral and ssha.
+// ---------------------------------------------------------------------------
+template <typename T>
+KERNEL void k_ssh_nxt(DevDomain d, double z2dt, double zcoef, const T *__restrict__ un, const T *__restrict__ vn,
+                      T *__restrict__ hdivn, const double *__restrict__ sshb, const double *__restrict__ emp_b,
+                      const double *__restrict__ emp, double *__restrict__ ssha)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    int si, sj;
+    const bool ok = lbc_src(d, 'T', i, j, si, sj);
+    const uint32_t ij = IJ32(i, j);
+    double zhdiv = 0.0;
+    for (int k = 1; k <= d.jpk - 1; k++) {
+        const T h = ok ? (T)hdiv_at<T>(d, un, vn, si, sj, k) : T(0);
+        A3(hdivn, i, j, k) = h;
+        zhdiv = zhdiv + E3(d.md.e3t, ij, k) * (double)h;
+    }
+    if (si != i || sj != j || !ok) A3(hdivn, i, j, d.jpk) = ok ? A3(hdivn, si, sj, d.jpk) : T(0);
+    ssha[ij] = (sshb[ij] - z2dt * (zcoef * (emp_b[ij] + emp[ij]) + zhdiv)) * SSMASK(ij);
+}
+
+// ---------------------------------------------------------------------------
+// wzv, DYN/sshwzv.F90:129-213 (z-star / linear free surface branch): w from
+// the bottom up.  e3t_a - e3t_b is formed as the reference forms it (zero
+// when the scale factors are fixed).  One thread per column.
+// ---------------------------------------------------------------------------
+template <typename T>
+KERNEL void k_wzv(DevDomain d, int first, double z1_2dt, const T *__restrict__ hdivn, T *__restrict__ wn)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const uint32_t ij = IJ32(i, j);
+    if (first) A3(wn, i, j, d.jpk) = T(0);
+    double w = (double)A3(wn, i, j, d.jpk);
+    for (int k = d.jpk - 1; k >= 1; k--) {
+        const double e3 = E3(d.md.e3t, ij, k);
+        w = w - (e3 * (double)A3(hdivn, i, j, k) + z1_2dt * (e3 - e3)) * TMASK(double, i, j, k);
+        A3(wn, i, j, k) = (T)w;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ssh_swp, DYN/sshwzv.F90:216-267 (linear free surface)
+// ---------------------------------------------------------------------------
+KERNEL void k_ssh_swp(DevDomain d, int euler, double atfp, double *__restrict__ sshb, double *__restrict__ sshn,
+                      const double *__restrict__ ssha)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const uint32_t ij = IJ32(i, j);
+    if (!euler) sshb[ij] = sshn[ij] + atfp * (sshb[ij] - 2 * sshn[ij] + ssha[ij]);
+    sshn[ij] = ssha[ij];
+}
+
+// ---------------------------------------------------------------------------
+// tra_nxt, TRA/tranxt.F90:66-192 with tra_nxt_fix, lines 195-237: lbc_lnk of
+// tsa; Euler: tsn = tsa on levels 1..jpkm1; leapfrog: Asselin filter on the
+// interior, then lbc_lnk of tsb, tsn, tsa.  One thread per cell and tracer
+// (grid.z = jpk * 2).
+// ---------------------------------------------------------------------------
+template <typename T>
+KERNEL void k_tra_nxt(DevDomain d, int euler, double atfp, T *__restrict__ tsb, T *__restrict__ tsn,
+                      T *__restrict__ tsa)
+{
+    THREAD_IJ(1, 1);
+    const int k = (int)(blockIdx.z % (unsigned)d.jpk) + 1;
+    const size_t off = (size_t)(blockIdx.z / (unsigned)d.jpk) * d.n3;
+    if (i > d.jpi || j > d.jpj) return;
+    T *pb = tsb + off, *pn = tsn + off, *pa = tsa + off;
+    if (!is_interior(d, i, j)) {
+        int si, sj;
+        if (lbc_src(d, 'T', i, j, si, sj)) return;   // written by the thread of (si,sj)
+        A3(pa, i, j, k) = T(0);
+        if (euler) {
+            if (k < d.jpk) A3(pn, i, j, k) = T(0);
+        } else {
+            A3(pb, i, j, k) = T(0);
+            A3(pn, i, j, k) = T(0);
+        }
+        return;
+    }
+    const T ta = A3(pa, i, j, k);
+    T tb, tn;
+    if (euler) {
+        tb = A3(pb, i, j, k);   // only copied to ring points by the leapfrog branch
+        tn = k < d.jpk ? ta : A3(pn, i, j, k);
+        if (k < d.jpk) A3(pn, i, j, k) = tn;
+    } else if (k < d.jpk) {
+        const double ztn = (double)A3(pn, i, j, k);
+        const double ztd = (double)ta - 2.0 * ztn + (double)A3(pb, i, j, k);
+        tb = (T)(ztn + atfp * ztd);
+        tn = ta;
+        A3(pb, i, j, k) = tb;
+        A3(pn, i, j, k) = tn;
+    } else {
+        tb = A3(pb, i, j, k);
+        tn = A3(pn, i, j, k);
+    }
+    lbc_targets(d, i, j, [&](int ti, int tj) {
+        A3(pa, ti, tj, k) = ta;
+        if (euler) {
+            if (k < d.jpk) A3(pn, ti, tj, k) = tn;
+        } else {
+            A3(pb, ti, tj, k) = tb;
+            A3(pn, ti, tj, k) = tn;
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
+// dyn_nxt, DYN/dynnxt.F90:66-377 (linear free surface), in two column
+// kernels.
+// Part 1 (lines 111-135): with dyn_spg_ts the baroclinic ua/va get the
+// barotropic after velocity (ua - depth mean + ua_b) on the interior, written
+// also to the ring points the lbc_lnk('U'/'V') of line 139 fills from them;
+// ring points without a source are zeroed.  Without ln_bt_fw, un/vn get the
+// advective barotropic velocity (lines 127-133), every column on itself.
+// ---------------------------------------------------------------------------
+template <typename T>
+KERNEL void k_dyn_nxt1(DevDomain d, int spg_ts, int bt_fw, T *__restrict__ ua, T *__restrict__ va,
+                       T *__restrict__ un, T *__restrict__ vn, const double *__restrict__ ua_b,
+                       const double *__restrict__ va_b, const double *__restrict__ un_adv,
+                       const double *__restrict__ vn_adv, const double *__restrict__ un_b,
+                       const double *__restrict__ vn_b, const double *__restrict__ r1_hu, const double *__restrict__ r1_hv)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const Metrics<double> &m = d.md;
+    const uint32_t ij = IJ32(i, j);
+    const int jpk = d.jpk;
+    if (spg_ts && !bt_fw) {
+        const double cu = un_adv[ij] * r1_hu[ij], cv = vn_adv[ij] * r1_hv[ij];
+        for (int k = 1; k <= jpk - 1; k++) {
+            A3(un, i, j, k) = (T)(((double)A3(un, i, j, k) - cu + un_b[ij]) * UMASK(double, i, j, k));
+            A3(vn, i, j, k) = (T)(((double)A3(vn, i, j, k) - cv + vn_b[ij]) * VMASK(double, i, j, k));
+        }
+    }
+    if (!is_interior(d, i, j)) {
+        int si, sj;
+        if (lbc_src(d, 'U', i, j, si, sj)) return;   // written by the thread of (si,sj)
+        for (int k = 1; k <= jpk; k++) {
+            A3(ua, i, j, k) = T(0);
+            A3(va, i, j, k) = T(0);
+        }
+        return;
+    }
+    double zue = 0.0, zve = 0.0;
+    if (spg_ts) {
+        zue = E3(m.e3u, ij, 1) * (double)A3(ua, i, j, 1) * UMASK(double, i, j, 1);
+        zve = E3(m.e3v, ij, 1) * (double)A3(va, i, j, 1) * VMASK(double, i, j, 1);
+        for (int k = 2; k <= jpk - 1; k++) {
+            zue = zue + E3(m.e3u, ij, k) * (double)A3(ua, i, j, k) * UMASK(double, i, j, k);
+            zve = zve + E3(m.e3v, ij, k) * (double)A3(va, i, j, k) * VMASK(double, i, j, k);
+        }
+        zue = zue * r1_hu[ij];
+        zve = zve * r1_hv[ij];
+    }
+    for (int k = 1; k <= jpk; k++) {
+        T a = A3(ua, i, j, k), b = A3(va, i, j, k);
+        if (spg_ts && k < jpk) {
+            a = (T)(((double)a - zue + ua_b[ij]) * UMASK(double, i, j, k));
+            b = (T)(((double)b - zve + va_b[ij]) * VMASK(double, i, j, k));
+            A3(ua, i, j, k) = a;
+            A3(va, i, j, k) = b;
+        }
+        lbc_targets(d, i, j, [&](int ti, int tj) {
+            A3(ua, ti, tj, k) = a;
+            A3(va, ti, tj, k) = b;
+        });
+    }
+}
+
+// Part 2 (lines 147-176, 286-297, 299-347): Euler swap or Asselin filter and
+// swap on every column, the forward time-splitting correction of ub/vb, and
+// the barotropic now and before velocities.  Each column reads and writes
+// only itself.  The depth mean of part 1 multiplies by r1_hu once (zue *
+// r1_hu_a) exactly as lines 119-120 do, and the same holds for line 292.
+template <typename T>
+KERNEL void k_dyn_nxt2(DevDomain d, int euler, int bt_fix, double atfp, T *__restrict__ ub, T *__restrict__ vb,
+                       T *__restrict__ un, T *__restrict__ vn, const T *__restrict__ ua, const T *__restrict__ va,
+                       double *__restrict__ ub_b, double *__restrict__ vb_b, double *__restrict__ un_b,
+                       double *__restrict__ vn_b, const double *__restrict__ r1_hu, const double *__restrict__ r1_hv)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const Metrics<double> &m = d.md;
+    const uint32_t ij = IJ32(i, j);
+    const int jpkm1 = d.jpk - 1;
+    double sun = 0.0, svn = 0.0, sub = 0.0, svb = 0.0;   // depth sums of un, vn, ub, vb
+    for (int k = 1; k <= jpkm1; k++) {
+        const double a = (double)A3(ua, i, j, k), b = (double)A3(va, i, j, k);
+        const double e3u = E3(m.e3u, ij, k), e3v = E3(m.e3v, ij, k);
+        const double mu = UMASK(double, i, j, k), mv = VMASK(double, i, j, k);
+        double fu, fv;
+        if (euler) {
+            fu = (double)A3(ub, i, j, k);
+            fv = (double)A3(vb, i, j, k);
+        } else {
+            const double n1 = (double)A3(un, i, j, k), n2 = (double)A3(vn, i, j, k);
+            const T zuf = (T)(n1 + atfp * ((double)A3(ub, i, j, k) - 2.0 * n1 + a));
+            const T zvf = (T)(n2 + atfp * ((double)A3(vb, i, j, k) - 2.0 * n2 + b));
+            A3(ub, i, j, k) = zuf;
+            A3(vb, i, j, k) = zvf;
+            fu = (double)zuf;
+            fv = (double)zvf;
+        }
+        A3(un, i, j, k) = A3(ua, i, j, k);
+        A3(vn, i, j, k) = A3(va, i, j, k);
+        const double tu = e3u * (double)A3(ua, i, j, k) * mu, tv = e3v * (double)A3(va, i, j, k) * mv;
+        const double bu = e3u * fu * mu, bv = e3v * fv * mv;
+        sun = k == 1 ? tu : sun + tu;
+        svn = k == 1 ? tv : svn + tv;
+        sub = k == 1 ? bu : sub + bu;
+        svb = k == 1 ? bv : svb + bv;
+    }
+    if (!euler && bt_fix) {
+        // lines 286-297: the depth mean of ub/vb set to the input un_b/vn_b
+        const double cu = sub * r1_hu[ij] - un_b[ij], cv = svb * r1_hv[ij] - vn_b[ij];
+        for (int k = 1; k <= jpkm1; k++) {
+            const double e3u = E3(m.e3u, ij, k), e3v = E3(m.e3v, ij, k);
+            const double mu = UMASK(double, i, j, k), mv = VMASK(double, i, j, k);
+            const T nb = (T)((double)A3(ub, i, j, k) - cu * mu);
+            const T nv = (T)((double)A3(vb, i, j, k) - cv * mv);
+            A3(ub, i, j, k) = nb;
+            A3(vb, i, j, k) = nv;
+            const double bu = e3u * (double)nb * mu, bv = e3v * (double)nv * mv;
+            sub = k == 1 ? bu : sub + bu;
+            svb = k == 1 ? bv : svb + bv;
+        }
+    }
+    un_b[ij] = sun * r1_hu[ij];
+    vn_b[ij] = svn * r1_hv[ij];
+    ub_b[ij] = sub * r1_hu[ij];
+    vb_b[ij] = svb * r1_hv[ij];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename T>
+int launch_div_hor(const T *un, const T *vn, T *hdivn)
+{
+    const DevDomain &d = nh::dom();
+    Grid2 g = grid2(d.jpi, d.jpj);
+    g.grid.z = (unsigned)d.jpk;
+    LAUNCH("div_hor", k_div_hor<T>, g, d, un, vn, hdivn);
+    return 0;
+}
+
+template <typename T>
+int launch_ssh_nxt(double z2dt, double zcoef, const T *un, const T *vn, T *hdivn, const double *sshb,
+                   const double *emp_b, const double *emp, double *ssha)
+{
+    const DevDomain &d = nh::dom();
+    Grid2 g = grid2(d.jpi, d.jpj);
+    LAUNCH("ssh_nxt", k_ssh_nxt<T>, g, d, z2dt, zcoef, un, vn, hdivn, sshb, emp_b, emp, ssha);
+    return 0;
+}
+
+template <typename T>
+int launch_wzv(int first, double z1_2dt, const T *hdivn, T *wn)
+{
+    const DevDomain &d = nh::dom();
+    Grid2 g = grid2(d.jpi, d.jpj);
+    LAUNCH("wzv", k_wzv<T>, g, d, first, z1_2dt, hdivn, wn);
+    return 0;
+}
+
+int launch_ssh_swp(int euler, double *sshb, double *sshn, const double *ssha)
+{
+    const DevDomain &d = nh::dom();
+    Grid2 g = grid2(d.jpi, d.jpj);
+    LAUNCH("ssh_swp", k_ssh_swp, g, d, euler, d.rn_atfp, sshb, sshn, ssha);
+    return 0;
+}
+
+template <typename T>
+int launch_tra_nxt(int euler, T *tsb, T *tsn, T *tsa)
+{
+    const DevDomain &d = nh::dom();
+    Grid2 g = grid2(d.jpi, d.jpj);
+    g.grid.z = (unsigned)(2 * d.jpk);
+    LAUNCH("tra_nxt", k_tra_nxt<T>, g, d, euler, d.rn_atfp, tsb, tsn, tsa);
+    return 0;
+}
+
+template <typename T>
+int launch_dyn_nxt(int euler, T *ub, T *vb, T *un, T *vn, T *ua, T *va, const double *ua_b, const double *va_b,
+                   const double *un_adv, const double *vn_adv, double *ub_b, double *vb_b, double *un_b,
+                   double *vn_b, const double *r1_hu, const double *r1_hv)
+{
+    const DevDomain &d = nh::dom();
+    Grid2 g = grid2(d.jpi, d.jpj);
+    LAUNCH("dyn_nxt", k_dyn_nxt1<T>, g, d, d.ln_dynspg_ts, d.ln_bt_fw, ua, va, un, vn, ua_b, va_b, un_adv, vn_adv,
+           (const double *)un_b, (const double *)vn_b, r1_hu, r1_hv);
+    LAUNCH("dyn_nxt", k_dyn_nxt2<T>, g, d, euler, d.ln_dynspg_ts && d.ln_bt_fw, d.rn_atfp, ub, vb, un, vn,
+           (const T *)ua, (const T *)va, ub_b, vb_b, un_b, vn_b, r1_hu, r1_hv);
+    return 0;
+}
+
+#define INST(T)                                                                                                  \
+    template int launch_div_hor<T>(const T *, const T *, T *);                                                   \
+    template int launch_ssh_nxt<T>(double, double, const T *, const T *, T *, const double *, const double *,    \
+                                   const double *, double *);                                                    \
+    template int launch_wzv<T>(int, double, const T *, T *);                                                     \
+    template int launch_tra_nxt<T>(int, T *, T *, T *);                                                          \
+    template int launch_dyn_nxt<T>(int, T *, T *, T *, T *, T *, T *, const double *, const double *,            \
+                                   const double *, const double *, double *, double *, double *, double *,      \
+                                   const double *, const double *);
+INST(double)
+INST(float)
+#undef INST

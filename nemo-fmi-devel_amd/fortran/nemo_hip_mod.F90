@@ -12,8 +12,11 @@ MODULE nemo_hip_mod
    !!   dyn_keg        DYN/dynkeg.F90:46            dyn_zad        DYN/dynzad.F90:38
    !!   dyn_adv_cen2   DYN/dynadv_cen2.F90:37       dyn_vor        DYN/dynvor.F90:98
    !!   dyn_zdf        DYN/dynzdf.F90:47            dyn_spg_ts     DYN/dynspg_ts.F90:121
+   !!   div_hor        DYN/divhor.F90:50            ssh_nxt        DYN/sshwzv.F90:56
+   !!   wzv            DYN/sshwzv.F90:129           ssh_swp        DYN/sshwzv.F90:216
+   !!   tra_nxt        TRA/tranxt.F90:66            dyn_nxt        DYN/dynnxt.F90:66
    !! The arrays those routines take from NEMO modules (oce, zdf_oce, ldfslp,
-   !! sbc_oce, zdfdrg, dynspg_ts) are bound once, after allocation, with the
+   !! sbc_oce, zdfdrg, dynspg_ts, dom_oce) are bound once, after allocation, with the
    !! nemo_hip_bind_* routines: module arrays never move, so their addresses
    !! stay valid for the run.  A nonzero status is turned into ctl_stop by
    !! the caller-supplied error hook (default: print and ERROR STOP).
@@ -28,6 +31,7 @@ MODULE nemo_hip_mod
    PUBLIC :: nemo_hip_bind_zdf, nemo_hip_bind_ldfslp, nemo_hip_bind_dyn, nemo_hip_bind_spg
    PUBLIC :: tra_adv_fct, interp_4th_cpt, tra_ldf_lap, tra_ldf_iso, tra_zdf_imp
    PUBLIC :: dyn_keg, dyn_zad, dyn_adv_cen2, dyn_vor, dyn_zdf, dyn_spg_ts
+   PUBLIC :: nemo_hip_bind_nxt, div_hor, ssh_nxt, wzv, ssh_swp, tra_nxt, dyn_nxt
    PUBLIC :: nemo_hip_domain_t, nemo_hip_spg_ts_t
    PUBLIC :: nemo_hip_field_alloc, nemo_hip_field_free, nemo_hip_field_push, nemo_hip_field_pull
 
@@ -66,6 +70,11 @@ MODULE nemo_hip_mod
    TYPE(C_PTR), SAVE :: p_ua = C_NULL_PTR, p_va = C_NULL_PTR
    TYPE(C_PTR), SAVE :: p_rCdU_bot = C_NULL_PTR, p_ua_b = C_NULL_PTR, p_va_b = C_NULL_PTR
    TYPE(C_PTR), SAVE :: p_utau = C_NULL_PTR, p_utau_b = C_NULL_PTR, p_vtau = C_NULL_PTR, p_vtau_b = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_hdivn = C_NULL_PTR, p_sshb = C_NULL_PTR, p_sshn = C_NULL_PTR, p_ssha = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_emp_b = C_NULL_PTR, p_emp = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_tsb = C_NULL_PTR, p_tsn = C_NULL_PTR, p_tsa = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_ub_b = C_NULL_PTR, p_vb_b = C_NULL_PTR, p_un_b = C_NULL_PTR, p_vn_b = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_un_adv = C_NULL_PTR, p_vn_adv = C_NULL_PTR, p_r1_hu_n = C_NULL_PTR, p_r1_hv_n = C_NULL_PTR
    INTEGER, SAVE :: l_ldfslp = 0
    TYPE(nemo_hip_spg_ts_t), SAVE :: spg_state
 
@@ -172,6 +181,38 @@ MODULE nemo_hip_mod
          INTEGER(C_INT), VALUE :: kt
          TYPE(nemo_hip_spg_ts_t), INTENT(inout) :: st
       END FUNCTION
+      INTEGER(C_INT) FUNCTION c_div_hor(kt, un, vn, hdivn) BIND(C, name="nemo_hip_div_hor")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: un, vn, hdivn
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_ssh_nxt(kt, un, vn, hdivn, sshb, emp_b, emp, ssha) BIND(C, name="nemo_hip_ssh_nxt")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: un, vn, hdivn, sshb, emp_b, emp, ssha
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_wzv(kt, hdivn, wn) BIND(C, name="nemo_hip_wzv")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: hdivn, wn
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_ssh_swp(kt, sshb, sshn, ssha) BIND(C, name="nemo_hip_ssh_swp")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: sshb, sshn, ssha
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_tra_nxt(kt, tsb, tsn, tsa) BIND(C, name="nemo_hip_tra_nxt")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: tsb, tsn, tsa
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_dyn_nxt(kt, ub, vb, un, vn, ua, va, ua_b, va_b, un_adv, vn_adv, ub_b, vb_b,   &
+         &                              un_b, vn_b, r1_hu_n, r1_hv_n) BIND(C, name="nemo_hip_dyn_nxt")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: ub, vb, un, vn, ua, va, ua_b, va_b, un_adv, vn_adv, ub_b, vb_b, un_b, vn_b
+         TYPE(C_PTR), VALUE :: r1_hu_n, r1_hv_n
+      END FUNCTION
    END INTERFACE
 
 CONTAINS
@@ -272,6 +313,23 @@ CONTAINS
       p_utau = C_LOC( putau ) ; p_utau_b = C_LOC( putau_b ) ; p_vtau = C_LOC( pvtau ) ; p_vtau_b = C_LOC( pvtau_b )
    END SUBROUTINE nemo_hip_bind_dyn
 
+   SUBROUTINE nemo_hip_bind_nxt( phdivn, psshb, psshn, pssha, pemp_b, pemp, ptsb, ptsn, ptsa,   &
+      &                          pub_b, pvb_b, pun_b, pvn_b, pun_adv, pvn_adv, pr1_hu_n, pr1_hv_n )
+      !! oce (hdivn, sshb/n/a, tsb/n/a, ub_b ... vn_b), sbc_oce (emp_b, emp), dynspg_ts (un_adv, vn_adv),
+      !! dom_oce (r1_hu_n, r1_hv_n) for the time-step closure
+      REAL(wp), DIMENSION(:,:,:)  , TARGET, CONTIGUOUS, INTENT(in) :: phdivn
+      REAL(wp), DIMENSION(:,:)    , TARGET, CONTIGUOUS, INTENT(in) :: psshb, psshn, pssha, pemp_b, pemp
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(in) :: ptsb, ptsn, ptsa
+      REAL(wp), DIMENSION(:,:)    , TARGET, CONTIGUOUS, INTENT(in) :: pub_b, pvb_b, pun_b, pvn_b, pun_adv, pvn_adv
+      REAL(wp), DIMENSION(:,:)    , TARGET, CONTIGUOUS, INTENT(in) :: pr1_hu_n, pr1_hv_n
+      p_hdivn = C_LOC( phdivn ) ; p_sshb = C_LOC( psshb ) ; p_sshn = C_LOC( psshn ) ; p_ssha = C_LOC( pssha )
+      p_emp_b = C_LOC( pemp_b ) ; p_emp = C_LOC( pemp )
+      p_tsb = C_LOC( ptsb ) ; p_tsn = C_LOC( ptsn ) ; p_tsa = C_LOC( ptsa )
+      p_ub_b = C_LOC( pub_b ) ; p_vb_b = C_LOC( pvb_b ) ; p_un_b = C_LOC( pun_b ) ; p_vn_b = C_LOC( pvn_b )
+      p_un_adv = C_LOC( pun_adv ) ; p_vn_adv = C_LOC( pvn_adv )
+      p_r1_hu_n = C_LOC( pr1_hu_n ) ; p_r1_hv_n = C_LOC( pr1_hv_n )
+   END SUBROUTINE nemo_hip_bind_nxt
+
    SUBROUTINE nemo_hip_bind_spg( sstate )
       !! C_LOC of every array dyn_spg_ts reads or writes (oce, dom_oce, sbc_oce, dynspg_ts)
       TYPE(nemo_hip_spg_ts_t), INTENT(in) :: sstate
@@ -365,5 +423,36 @@ CONTAINS
       INTEGER, INTENT(in) ::   kt
       CALL check( c_dyn_spg_ts( INT(kt,C_INT), spg_state ), 'dyn_spg_ts' )
    END SUBROUTINE dyn_spg_ts
+
+   SUBROUTINE div_hor( kt )
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_div_hor( INT(kt,C_INT), p_un, p_vn, p_hdivn ), 'div_hor' )
+   END SUBROUTINE div_hor
+
+   SUBROUTINE ssh_nxt( kt )
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_ssh_nxt( INT(kt,C_INT), p_un, p_vn, p_hdivn, p_sshb, p_emp_b, p_emp, p_ssha ), 'ssh_nxt' )
+   END SUBROUTINE ssh_nxt
+
+   SUBROUTINE wzv( kt )
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_wzv( INT(kt,C_INT), p_hdivn, p_wn ), 'wzv' )
+   END SUBROUTINE wzv
+
+   SUBROUTINE ssh_swp( kt )
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_ssh_swp( INT(kt,C_INT), p_sshb, p_sshn, p_ssha ), 'ssh_swp' )
+   END SUBROUTINE ssh_swp
+
+   SUBROUTINE tra_nxt( kt )
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_tra_nxt( INT(kt,C_INT), p_tsb, p_tsn, p_tsa ), 'tra_nxt' )
+   END SUBROUTINE tra_nxt
+
+   SUBROUTINE dyn_nxt( kt )
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_dyn_nxt( INT(kt,C_INT), p_ub, p_vb, p_un, p_vn, p_ua, p_va, p_ua_b, p_va_b,               &
+         &        p_un_adv, p_vn_adv, p_ub_b, p_vb_b, p_un_b, p_vn_b, p_r1_hu_n, p_r1_hv_n ), 'dyn_nxt' )
+   END SUBROUTINE dyn_nxt
 
 END MODULE nemo_hip_mod

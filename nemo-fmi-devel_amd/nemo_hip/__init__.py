@@ -326,6 +326,44 @@ class NemoHip:
                        p(akz), p(avt)))
 
 
+    # time-step closure ------------------------------------------------------
+    def div_hor(self, kt, un, vn, hdivn, sp=False):
+        fn = self.L.nemo_hip_div_hor_sp if sp else self.L.nemo_hip_div_hor
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        self._check(fn(C.c_int(kt), p(un), p(vn), p(hdivn)))
+
+    def ssh_nxt(self, kt, un, vn, hdivn, sshb, emp_b, emp, ssha, sp=False):
+        fn = self.L.nemo_hip_ssh_nxt_sp if sp else self.L.nemo_hip_ssh_nxt
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        q = self._p
+        self._check(fn(C.c_int(kt), p(un), p(vn), p(hdivn), q(sshb), q(emp_b), q(emp), q(ssha)))
+
+    def wzv(self, kt, hdivn, wn, sp=False):
+        fn = self.L.nemo_hip_wzv_sp if sp else self.L.nemo_hip_wzv
+        dt = np.float32 if sp else np.float64
+        self._check(fn(C.c_int(kt), self._p(hdivn, dt), self._p(wn, dt)))
+
+    def ssh_swp(self, kt, sshb, sshn, ssha):
+        self._check(self.L.nemo_hip_ssh_swp(C.c_int(kt), self._p(sshb), self._p(sshn), self._p(ssha)))
+
+    def tra_nxt(self, kt, tsb, tsn, tsa, sp=False):
+        fn = self.L.nemo_hip_tra_nxt_sp if sp else self.L.nemo_hip_tra_nxt
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        self._check(fn(C.c_int(kt), p(tsb), p(tsn), p(tsa)))
+
+    def dyn_nxt(self, kt, ub, vb, un, vn, ua, va, ua_b, va_b, un_adv, vn_adv, ub_b, vb_b, un_b, vn_b, r1_hu_n,
+                r1_hv_n, sp=False):
+        fn = self.L.nemo_hip_dyn_nxt_sp if sp else self.L.nemo_hip_dyn_nxt
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        q = self._p
+        self._check(fn(C.c_int(kt), p(ub), p(vb), p(un), p(vn), p(ua), p(va), q(ua_b), q(va_b), q(un_adv),
+                       q(vn_adv), q(ub_b), q(vb_b), q(un_b), q(vn_b), q(r1_hu_n), q(r1_hv_n)))
+
+
 _handle = None
 
 

@@ -1737,6 +1737,222 @@ int ora_spg_ts_reset(void)
 /* layout of the ABI structures as this compiler lays them out (checked
    against the ctypes mirror by the tests) */
 #include <stddef.h>
+/* ------------------------------------------------------------------------ */
+/* step glue: div_hor, ssh_nxt, wzv, ssh_swp, tra_nxt, dyn_nxt               */
+/* (linear free surface: e3*_a = e3*_b = e3*_n, r1_hu_a = r1_hu_b = r1_hu_n)  */
+/* ------------------------------------------------------------------------ */
+/* div_hor, DYN/divhor.F90:50-110 (no runoff, ice shelf or AGRIF terms) */
+int ora_div_hor(int kt, const double *un, const double *vn, double *hdivn)
+{
+    int i, j, k;
+    (void)kt;
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++)
+                A3(hdivn, i, j, k) = (e2u(i, j) * e3u_n(i, j, k) * A3(un, i, j, k) -
+                                      e2u(i - 1, j) * e3u_n(i - 1, j, k) * A3(un, i - 1, j, k) +
+                                      e1v(i, j) * e3v_n(i, j, k) * A3(vn, i, j, k) -
+                                      e1v(i, j - 1) * e3v_n(i, j - 1, k) * A3(vn, i, j - 1, k)) *
+                                     r1_e1e2t(i, j) / e3t_n(i, j, k);
+    lbc3(hdivn, jpk, 'T', 0.0);
+    return 0;
+}
+
+/* ssh_nxt, DYN/sshwzv.F90:56-126 (no wetting/drying, BDY or AGRIF) */
+int ora_ssh_nxt(int kt, const double *un, const double *vn, double *hdivn, const double *sshb, const double *emp_b,
+                const double *emp, double *ssha)
+{
+    int i, j, k;
+    double z2dt = 2.0 * D.rdt;
+    if (D.neuler == 0 && kt == D.nit000) z2dt = D.rdt;
+    const double zcoef = 0.5 * (1.0 / D.rau0);
+    ora_div_hor(kt, un, vn, hdivn);
+    double *zhdiv = calloc((size_t)jpi * jpj, sizeof(double));
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) A2(zhdiv, i, j) = A2(zhdiv, i, j) + e3t_n(i, j, k) * A3(hdivn, i, j, k);
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++)
+            A2(ssha, i, j) =
+                (A2(sshb, i, j) - z2dt * (zcoef * (A2(emp_b, i, j) + A2(emp, i, j)) + A2(zhdiv, i, j))) *
+                A2(ssmask, i, j);
+    free(zhdiv);
+    return 0;
+}
+
+/* wzv, DYN/sshwzv.F90:129-213 (z-star and linear free surface case) */
+int ora_wzv(int kt, const double *hdivn, double *wn)
+{
+    int i, j, k;
+    if (kt == D.nit000)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) A3(wn, i, j, jpk) = 0.0;
+    double z1_2dt = 1. / (2. * D.rdt);
+    if (D.neuler == 0 && kt == D.nit000) z1_2dt = 1. / D.rdt;
+    for (k = jpkm1; k >= 1; k--)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++)
+                A3(wn, i, j, k) = A3(wn, i, j, k + 1) - (e3t_n(i, j, k) * A3(hdivn, i, j, k) +
+                                                          z1_2dt * (e3t_a(i, j, k) - e3t_b(i, j, k))) *
+                                                             TMASK(i, j, k);
+    return 0;
+}
+
+/* ssh_swp, DYN/sshwzv.F90:216-267 (linear free surface) */
+int ora_ssh_swp(int kt, double *sshb, double *sshn, const double *ssha)
+{
+    int i, j;
+    const int euler = (D.neuler == 0 && kt == D.nit000);
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) {
+            if (!euler)
+                A2(sshb, i, j) = A2(sshn, i, j) + D.rn_atfp * (A2(sshb, i, j) - 2 * A2(sshn, i, j) + A2(ssha, i, j));
+            A2(sshn, i, j) = A2(ssha, i, j);
+        }
+    return 0;
+}
+
+/* tra_nxt with tra_nxt_fix, TRA/tranxt.F90:66-237 (linear free surface; no
+   trend diagnostics, BDY or AGRIF) on the two active tracers */
+int ora_tra_nxt(int kt, double *tsb, double *tsn, double *tsa)
+{
+    int i, j, k, n;
+    const size_t n3 = (size_t)jpi * jpj * jpk;
+    for (n = 0; n < 2; n++) lbc3(tsa + n * n3, jpk, 'T', 0.0);
+    if (D.neuler == 0 && kt == D.nit000) {
+        for (n = 1; n <= 2; n++)
+            for (k = 1; k <= jpkm1; k++)
+                for (j = 1; j <= jpj; j++)
+                    for (i = 1; i <= jpi; i++) A4(tsn, i, j, k, n) = A4(tsa, i, j, k, n);
+        return 0;
+    }
+    for (n = 1; n <= 2; n++)
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++) {
+                    const double ztn = A4(tsn, i, j, k, n);
+                    const double ztd = A4(tsa, i, j, k, n) - 2.0 * ztn + A4(tsb, i, j, k, n);
+                    A4(tsb, i, j, k, n) = ztn + D.rn_atfp * ztd;
+                    A4(tsn, i, j, k, n) = A4(tsa, i, j, k, n);
+                }
+    for (n = 0; n < 2; n++) {
+        lbc3(tsb + n * n3, jpk, 'T', 0.0);
+        lbc3(tsn + n * n3, jpk, 'T', 0.0);
+        lbc3(tsa + n * n3, jpk, 'T', 0.0);
+    }
+    return 0;
+}
+
+/* dyn_nxt, DYN/dynnxt.F90:66-377 (linear free surface, vector or flux form;
+   with dyn_spg_ts the forward time-splitting corrections) */
+int ora_dyn_nxt(int kt, double *ub, double *vb, double *un, double *vn, double *ua, double *va, const double *ua_b,
+                const double *va_b, const double *un_adv, const double *vn_adv, double *ub_b, double *vb_b,
+                double *un_b, double *vn_b, const double *r1_hu_n, const double *r1_hv_n)
+{
+    int i, j, k;
+    const size_t n2 = (size_t)jpi * jpj;
+    double *zue = calloc(n2, sizeof(double)), *zve = calloc(n2, sizeof(double));
+    if (D.ln_dynspg_ts) {
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A2(zue, i, j) = e3u_n(i, j, 1) * A3(ua, i, j, 1) * UMASK(i, j, 1);
+                A2(zve, i, j) = e3v_n(i, j, 1) * A3(va, i, j, 1) * VMASK(i, j, 1);
+            }
+        for (k = 2; k <= jpkm1; k++)
+            for (j = 1; j <= jpj; j++)
+                for (i = 1; i <= jpi; i++) {
+                    A2(zue, i, j) = A2(zue, i, j) + e3u_n(i, j, k) * A3(ua, i, j, k) * UMASK(i, j, k);
+                    A2(zve, i, j) = A2(zve, i, j) + e3v_n(i, j, k) * A3(va, i, j, k) * VMASK(i, j, k);
+                }
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 1; j <= jpj; j++)
+                for (i = 1; i <= jpi; i++) {
+                    A3(ua, i, j, k) = (A3(ua, i, j, k) - A2(zue, i, j) * A2(r1_hu_n, i, j) + A2(ua_b, i, j)) *
+                                      UMASK(i, j, k);
+                    A3(va, i, j, k) = (A3(va, i, j, k) - A2(zve, i, j) * A2(r1_hv_n, i, j) + A2(va_b, i, j)) *
+                                      VMASK(i, j, k);
+                }
+        if (!D.ln_bt_fw)
+            for (k = 1; k <= jpkm1; k++)
+                for (j = 1; j <= jpj; j++)
+                    for (i = 1; i <= jpi; i++) {
+                        A3(un, i, j, k) = (A3(un, i, j, k) - A2(un_adv, i, j) * A2(r1_hu_n, i, j) +
+                                           A2(un_b, i, j)) * UMASK(i, j, k);
+                        A3(vn, i, j, k) = (A3(vn, i, j, k) - A2(vn_adv, i, j) * A2(r1_hv_n, i, j) +
+                                           A2(vn_b, i, j)) * VMASK(i, j, k);
+                    }
+    }
+    lbc3(ua, jpk, 'U', 0.0);
+    lbc3(va, jpk, 'V', 0.0);
+    if (D.neuler == 0 && kt == D.nit000) {
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 1; j <= jpj; j++)
+                for (i = 1; i <= jpi; i++) {
+                    A3(un, i, j, k) = A3(ua, i, j, k);
+                    A3(vn, i, j, k) = A3(va, i, j, k);
+                }
+    } else {
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 1; j <= jpj; j++)
+                for (i = 1; i <= jpi; i++) {
+                    const double zuf =
+                        A3(un, i, j, k) + D.rn_atfp * (A3(ub, i, j, k) - 2.0 * A3(un, i, j, k) + A3(ua, i, j, k));
+                    const double zvf =
+                        A3(vn, i, j, k) + D.rn_atfp * (A3(vb, i, j, k) - 2.0 * A3(vn, i, j, k) + A3(va, i, j, k));
+                    A3(ub, i, j, k) = zuf;
+                    A3(vb, i, j, k) = zvf;
+                    A3(un, i, j, k) = A3(ua, i, j, k);
+                    A3(vn, i, j, k) = A3(va, i, j, k);
+                }
+        if (D.ln_dynspg_ts && D.ln_bt_fw) {
+            for (j = 1; j <= jpj; j++)
+                for (i = 1; i <= jpi; i++) {
+                    A2(zue, i, j) = e3u_n(i, j, 1) * A3(ub, i, j, 1) * UMASK(i, j, 1);
+                    A2(zve, i, j) = e3v_n(i, j, 1) * A3(vb, i, j, 1) * VMASK(i, j, 1);
+                }
+            for (k = 2; k <= jpkm1; k++)
+                for (j = 1; j <= jpj; j++)
+                    for (i = 1; i <= jpi; i++) {
+                        A2(zue, i, j) = A2(zue, i, j) + e3u_n(i, j, k) * A3(ub, i, j, k) * UMASK(i, j, k);
+                        A2(zve, i, j) = A2(zve, i, j) + e3v_n(i, j, k) * A3(vb, i, j, k) * VMASK(i, j, k);
+                    }
+            for (k = 1; k <= jpkm1; k++)
+                for (j = 1; j <= jpj; j++)
+                    for (i = 1; i <= jpi; i++) {
+                        A3(ub, i, j, k) = A3(ub, i, j, k) -
+                                          (A2(zue, i, j) * A2(r1_hu_n, i, j) - A2(un_b, i, j)) * UMASK(i, j, k);
+                        A3(vb, i, j, k) = A3(vb, i, j, k) -
+                                          (A2(zve, i, j) * A2(r1_hv_n, i, j) - A2(vn_b, i, j)) * VMASK(i, j, k);
+                    }
+        }
+    }
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) {
+            A2(un_b, i, j) = e3u_n(i, j, 1) * A3(un, i, j, 1) * UMASK(i, j, 1);
+            A2(ub_b, i, j) = e3u_n(i, j, 1) * A3(ub, i, j, 1) * UMASK(i, j, 1);
+            A2(vn_b, i, j) = e3v_n(i, j, 1) * A3(vn, i, j, 1) * VMASK(i, j, 1);
+            A2(vb_b, i, j) = e3v_n(i, j, 1) * A3(vb, i, j, 1) * VMASK(i, j, 1);
+        }
+    for (k = 2; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A2(un_b, i, j) = A2(un_b, i, j) + e3u_n(i, j, k) * A3(un, i, j, k) * UMASK(i, j, k);
+                A2(ub_b, i, j) = A2(ub_b, i, j) + e3u_n(i, j, k) * A3(ub, i, j, k) * UMASK(i, j, k);
+                A2(vn_b, i, j) = A2(vn_b, i, j) + e3v_n(i, j, k) * A3(vn, i, j, k) * VMASK(i, j, k);
+                A2(vb_b, i, j) = A2(vb_b, i, j) + e3v_n(i, j, k) * A3(vb, i, j, k) * VMASK(i, j, k);
+            }
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) {
+            A2(un_b, i, j) = A2(un_b, i, j) * A2(r1_hu_n, i, j);
+            A2(vn_b, i, j) = A2(vn_b, i, j) * A2(r1_hv_n, i, j);
+            A2(ub_b, i, j) = A2(ub_b, i, j) * A2(r1_hu_n, i, j);
+            A2(vb_b, i, j) = A2(vb_b, i, j) * A2(r1_hv_n, i, j);
+        }
+    free(zue);
+    free(zve);
+    return 0;
+}
+
 int ora_abi_layout(size_t *out, int n)
 {
     size_t v[] = {sizeof(nemo_hip_domain_t), offsetof(nemo_hip_domain_t, rdt), offsetof(nemo_hip_domain_t, k_top),

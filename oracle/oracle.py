@@ -146,3 +146,26 @@ class Oracle:
             a = fields.get(n)
             setattr(s, n, None if a is None else a.ctypes.data)
         _chk(self.L.ora_dyn_spg_ts(C.c_int(kt), C.byref(s)), "dyn_spg_ts")
+
+    # time-step closure ----------------------------------------------------
+    def div_hor(self, kt, un, vn, hdivn):
+        _chk(self.L.ora_div_hor(C.c_int(kt), _p(un), _p(vn), _p(hdivn)), "div_hor")
+
+    def ssh_nxt(self, kt, un, vn, hdivn, sshb, emp_b, emp, ssha):
+        _chk(self.L.ora_ssh_nxt(C.c_int(kt), _p(un), _p(vn), _p(hdivn), _p(sshb), _p(emp_b), _p(emp), _p(ssha)),
+             "ssh_nxt")
+
+    def wzv(self, kt, hdivn, wn):
+        _chk(self.L.ora_wzv(C.c_int(kt), _p(hdivn), _p(wn)), "wzv")
+
+    def ssh_swp(self, kt, sshb, sshn, ssha):
+        _chk(self.L.ora_ssh_swp(C.c_int(kt), _p(sshb), _p(sshn), _p(ssha)), "ssh_swp")
+
+    def tra_nxt(self, kt, tsb, tsn, tsa):
+        _chk(self.L.ora_tra_nxt(C.c_int(kt), _p(tsb), _p(tsn), _p(tsa)), "tra_nxt")
+
+    def dyn_nxt(self, kt, ub, vb, un, vn, ua, va, ua_b, va_b, un_adv, vn_adv, ub_b, vb_b, un_b, vn_b, r1_hu_n,
+                r1_hv_n):
+        _chk(self.L.ora_dyn_nxt(C.c_int(kt), _p(ub), _p(vb), _p(un), _p(vn), _p(ua), _p(va), _p(ua_b), _p(va_b),
+                                _p(un_adv), _p(vn_adv), _p(ub_b), _p(vb_b), _p(un_b), _p(vn_b), _p(r1_hu_n),
+                                _p(r1_hv_n)), "dyn_nxt")

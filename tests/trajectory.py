@@ -1,0 +1,116 @@
+"""A whole NEMO time step, step.F90:161-279, for a configuration without an
+equation of state, pressure gradient, lateral viscosity or surface module
+(constant forcing, the before forcing equal to the now forcing):
+
+    ssh_nxt, wzv                      first guess of ssha, now w
+    ua = va = 0, dyn_adv, dyn_vor,
+    dyn_spg_ts                        barotropic sub-steps, ssha, ua_b, ...
+    div_hor, wzv                      w of the corrected now velocities
+    dyn_zdf                           after velocities
+    tsa = 0, tra_adv (fct),
+    tra_ldf_iso, tra_zdf              after tracers
+    tra_nxt, dyn_nxt, ssh_swp         Asselin filters and swaps
+
+oracle_run repeats it with the oracle's separate routines; gpu_run with the
+library on device-resident fields (the fused step_dyn / step_tra entry points
+plus the closure routines), in FP64 or the FP32 storage form.  Both return the
+prognostic state after the steps listed in `marks`.
+"""
+import numpy as np
+
+SPG_2D = ["un_b", "vn_b", "ub_b", "vb_b", "sshn", "hu_n", "hv_n", "r1_hu_n", "r1_hv_n", "ht_n", "rCdU_bot", "utau",
+          "vtau", "emp", "rnf", "ub2_b", "vb2_b", "un_bf", "vn_bf"]
+FIX_3D = ["avm", "avt", "ahtu", "ahtv", "uslp", "vslp", "wslpi", "wslpj"]
+OUT_3D = ["un", "vn", "ub", "vb", "wn"]
+OUT_4D = ["tsn", "tsb"]
+OUT_2D = ["sshn", "sshb", "un_b", "vn_b", "ub_b", "vb_b", "ub2_b", "un_bf"]
+KN_FCT = (4, 4)
+
+
+def r2dt(d, kt):
+    o = d.opts
+    return o["rdt"] if (o["neuler"] == 0 and kt == o["nit000"]) else 2.0 * o["rdt"]
+
+
+def initial(S):
+    """The prognostic fields of the generator's state, with sshb = sshn and
+    the before forcing equal to the now forcing."""
+    I = {k: S[k] for k in SPG_2D + FIX_3D + ["un", "vn", "ub", "vb", "wn", "tsb", "tsn"]}
+    I["sshb"] = S["sshn"]
+    I["emp_b"], I["utau_b"], I["vtau_b"] = S["emp"], S["utau"], S["vtau"]
+    return I
+
+
+def oracle_run(d, S, marks):
+    from oracle.oracle import Oracle
+    o = Oracle(d)
+    s2, s3, s4 = (d.jpi, d.jpj), (d.jpi, d.jpj, d.jpk), (d.jpi, d.jpj, d.jpk, 2)
+    X = {k: np.array(v, order="F", copy=True) for k, v in initial(S).items()}
+    for k in ("ua_b", "va_b", "ssha", "un_adv", "vn_adv"):
+        X[k] = np.zeros(s2, order="F")
+    hdivn = np.zeros(s3, order="F")
+    ah, akz = np.zeros(s3, order="F"), np.zeros(s3, order="F")
+    out = {}
+    for kt in range(1, max(marks) + 1):
+        o.ssh_nxt(kt, X["un"], X["vn"], hdivn, X["sshb"], X["emp_b"], X["emp"], X["ssha"])
+        o.wzv(kt, hdivn, X["wn"])
+        ua, va = np.zeros(s3, order="F"), np.zeros(s3, order="F")
+        o.dyn_adv(kt, X["un"], X["vn"], X["wn"], ua, va)
+        o.dyn_vor(kt, X["un"], X["vn"], ua, va)
+        F = {k: X[k] for k in SPG_2D + ["ua_b", "va_b", "ssha", "un_adv", "vn_adv"]}
+        F.update(ua=ua, va=va, un=X["un"], vn=X["vn"])
+        o.dyn_spg_ts(kt, F)
+        o.div_hor(kt, X["un"], X["vn"], hdivn)
+        o.wzv(kt, hdivn, X["wn"])
+        o.dyn_zdf(kt, X["ub"], X["vb"], ua, va, X["avm"], X["rCdU_bot"], X["ua_b"], X["va_b"], X["utau"],
+                  X["utau_b"], X["vtau"], X["vtau_b"])
+        z = [np.zeros(s3, order="F") for _ in range(3)]
+        tsa = np.zeros(s4, order="F")
+        o.tra_adv_trp(X["un"], X["vn"], X["wn"], *z)
+        o.tra_adv_fct(kt, 1, "TRA", r2dt(d, kt), *z, X["tsb"], X["tsn"], tsa, 2, *KN_FCT)
+        o.tra_ldf_iso(kt, 1, "TRA", X["ahtu"], X["ahtv"], X["tsb"], X["tsb"], tsa, 2, 1, X["uslp"], X["vslp"],
+                      X["wslpi"], X["wslpj"], ah, akz)
+        o.tra_zdf(kt, X["tsb"], tsa, X["avt"], X["avt"], 1, ah, akz)
+        o.tra_nxt(kt, X["tsb"], X["tsn"], tsa)
+        o.dyn_nxt(kt, X["ub"], X["vb"], X["un"], X["vn"], ua, va, X["ua_b"], X["va_b"], X["un_adv"], X["vn_adv"],
+                  X["ub_b"], X["vb_b"], X["un_b"], X["vn_b"], X["r1_hu_n"], X["r1_hv_n"])
+        o.ssh_swp(kt, X["sshb"], X["sshn"], X["ssha"])
+        if kt in marks:
+            out[kt] = {k: X[k].copy(order="F") for k in OUT_3D + OUT_4D + OUT_2D}
+    return out
+
+
+def gpu_run(h, d, S, marks, sp=False):
+    """h: the library handle in device-pointer mode with d registered."""
+    import torch
+    dt = torch.float32 if sp else torch.float64
+    s2, s3, s4 = (d.jpi, d.jpj), (d.jpi, d.jpj, d.jpk), (d.jpi, d.jpj, d.jpk, 2)
+    n2, n3 = d.jpi * d.jpj, d.jpi * d.jpj * d.jpk
+    ndim = {k: len(np.shape(v)) for k, v in initial(S).items()}
+    dev = lambda a, t: torch.from_numpy(np.ascontiguousarray(np.asarray(a).ravel(order="F"))).to("cuda", t)
+    X = {k: dev(v, dt if ndim[k] >= 3 else torch.float64) for k, v in initial(S).items()}
+    for k in ("ua_b", "va_b", "ssha", "un_adv", "vn_adv"):
+        X[k] = torch.zeros(n2, dtype=torch.float64, device="cuda")
+    z = lambda n: torch.zeros(n, dtype=dt, device="cuda")
+    hdivn, ua, va, tsa, ah, akz = z(n3), z(n3), z(n3), z(2 * n3), z(n3), z(n3)
+    F = {k: X[k] for k in SPG_2D + ["ua_b", "va_b", "ssha", "un_adv", "vn_adv"]}
+    F.update(ua=ua, va=va, un=X["un"], vn=X["vn"])
+    shape = {k: (s4 if k in OUT_4D else s3 if k in OUT_3D else s2) for k in OUT_3D + OUT_4D + OUT_2D}
+    out = {}
+    for kt in range(1, max(marks) + 1):
+        h.ssh_nxt(kt, X["un"], X["vn"], hdivn, X["sshb"], X["emp_b"], X["emp"], X["ssha"], sp=sp)
+        h.wzv(kt, hdivn, X["wn"], sp=sp)
+        h.step_dyn(kt, X["ub"], X["vb"], X["wn"], X["avm"], X["utau_b"], X["vtau_b"], F, sp=sp)
+        h.div_hor(kt, X["un"], X["vn"], hdivn, sp=sp)
+        h.wzv(kt, hdivn, X["wn"], sp=sp)
+        h.step_tra(kt, X["un"], X["vn"], X["wn"], X["tsb"], X["tsn"], tsa, *KN_FCT, X["ahtu"], X["ahtv"],
+                   X["uslp"], X["vslp"], X["wslpi"], X["wslpj"], ah, akz, X["avt"], sp=sp)
+        h.tra_nxt(kt, X["tsb"], X["tsn"], tsa, sp=sp)
+        h.dyn_nxt(kt, X["ub"], X["vb"], X["un"], X["vn"], ua, va, X["ua_b"], X["va_b"], X["un_adv"], X["vn_adv"],
+                  X["ub_b"], X["vb_b"], X["un_b"], X["vn_b"], X["r1_hu_n"], X["r1_hv_n"], sp=sp)
+        h.ssh_swp(kt, X["sshb"], X["sshn"], X["ssha"])
+        if kt in marks:
+            torch.cuda.synchronize()
+            h.sync()
+            out[kt] = {k: X[k].cpu().numpy().astype(np.float64).reshape(shape[k], order="F") for k in shape}
+    return out
