@@ -64,6 +64,9 @@ def parse():
     ap.add_argument("--host-mode", type=int, default=0, metavar="STEPS",
                     help="also time STEPS steps of the FP64 routines in host-pointer mode (NEMO's host arrays, "
                          "every call staged over PCIe): reported apart, never the headline value")
+    ap.add_argument("--whole-step", default="fp32_fused,fp64_fused", metavar="FORMS",
+                    help="also time, for these fused forms, the whole time step: the hot path closed by ssh_nxt, "
+                         "wzv, div_hor, tra_nxt, dyn_nxt and ssh_swp (reported apart; '' skips)")
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -93,7 +96,7 @@ class Stepper:
         self.h, self.d, self.sp, self.fused = h, d, sp, fused
         dt = torch.float32 if sp else torch.float64
         n3 = d.jpi * d.jpj * d.jpk
-        c = lambda k: S[k].to(dt).contiguous()
+        c = lambda k: S[k].to(dt, copy=True).contiguous()   # the steps update their own copies
         self.f3 = {k: c(k) for k in ("un", "vn", "wn", "ub", "vb", "avt", "avm", "ahtu", "ahtv", "uslp", "vslp",
                                      "wslpi", "wslpj")}
         self.f3["avs"] = self.f3["avt"]
@@ -113,6 +116,27 @@ class Stepper:
         self.spg = dict(self.f2)
         self.spg.update(ua=self.ua, va=self.va, un=self.un_dyn, vn=self.vn_dyn)
         self.r2dt = 2.0 * d.opts["rdt"]
+        # the closure's fields: divergence work array and the before sea level
+        self.hdivn = z(n3)
+        self.sshb = self.f2["sshn"].clone()
+
+    def whole_step(self, kt):
+        """The whole time step, step.F90:161-279: the fused hot path closed by
+        the sea-level, vertical-velocity and time-filter routines, so every
+        prognostic field (u, v, w, T, S, ssh) advances on the device."""
+        h, f, f2, sp = self.h, self.f3, self.f2, self.sp
+        h.ssh_nxt(kt, self.un_dyn, self.vn_dyn, self.hdivn, self.sshb, f2["emp"], f2["emp"], f2["ssha"], sp=sp)
+        h.wzv(kt, self.hdivn, f["wn"], sp=sp)
+        h.step_dyn(kt, f["ub"], f["vb"], f["wn"], f["avm"], f2["utau_b"], f2["vtau_b"], self.spg, sp=sp)
+        h.div_hor(kt, self.un_dyn, self.vn_dyn, self.hdivn, sp=sp)
+        h.wzv(kt, self.hdivn, f["wn"], sp=sp)
+        h.step_tra(kt, self.un_dyn, self.vn_dyn, f["wn"], self.tsb, self.tsn, self.tsa, 4, 4, f["ahtu"], f["ahtv"],
+                   f["uslp"], f["vslp"], f["wslpi"], f["wslpj"], self.ah, self.akz, f["avt"], sp=sp)
+        h.tra_nxt(kt, self.tsb, self.tsn, self.tsa, sp=sp)
+        h.dyn_nxt(kt, f["ub"], f["vb"], self.un_dyn, self.vn_dyn, self.ua, self.va, f2["ua_b"], f2["va_b"],
+                  f2["un_adv"], f2["vn_adv"], f2["ub_b"], f2["vb_b"], f2["un_b"], f2["vn_b"], f2["r1_hu_n"],
+                  f2["r1_hv_n"], sp=sp)
+        h.ssh_swp(kt, self.sshb, f2["sshn"], f2["ssha"])
 
     def step(self, kt):
         h, f, sp = self.h, self.f3, self.sp
@@ -140,7 +164,7 @@ class Stepper:
         h.tra_zdf(kt, self.tsb, self.tsa, f["avt"], f["avs"], 1, self.ah, self.akz, sp=sp)
 
 
-def time_form(h, d, S, sp, fused, steps, warmup, timing_steps, live=False):
+def time_form(h, d, S, sp, fused, steps, warmup, timing_steps, live=False, whole=False):
     """Warm up, time every kernel over `timing_steps` steps with HIP events,
     then time `steps` steps with the wall clock.  live=True also brackets the
     launches of the dominant kernel (the one with the most time per step that
@@ -148,11 +172,12 @@ def time_form(h, d, S, sp, fused, steps, warmup, timing_steps, live=False):
     region; its measured launch durations are returned for the roofline."""
     import torch
     st = Stepper(h, d, S, sp, fused)
+    step = st.whole_step if whole else st.step
     ext = torch.cuda.ExternalStream(h.stream())
     with torch.cuda.stream(ext):
         kt = 2
         for _ in range(warmup):
-            st.step(kt)
+            step(kt)
             kt += 1
         h.sync()
         # per-routine / per-kernel HIP-event timers over a short run
@@ -160,7 +185,7 @@ def time_form(h, d, S, sp, fused, steps, warmup, timing_steps, live=False):
         h.timing_only(None)
         h.timing_enable(True)
         for _ in range(timing_steps):
-            st.step(kt)
+            step(kt)
             kt += 1
         h.sync()
         h.timing_enable(False)
@@ -177,7 +202,7 @@ def time_form(h, d, S, sp, fused, steps, warmup, timing_steps, live=False):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
-            st.step(kt)
+            step(kt)
             kt += 1
         h.sync()
         torch.cuda.synchronize()
@@ -283,7 +308,7 @@ def kernel_times(h):
     import ctypes as C
     out = {}
     names = ["dyn_adv", "dyn_vor", "dyn_spg_ts", "dyn_zdf", "tra_adv_trp", "tra_adv_fct", "tra_ldf_iso", "tra_zdf",
-             "lbc_lnk", "step_dyn", "step_tra"]
+             "lbc_lnk", "step_dyn", "step_tra", "div_hor", "ssh_nxt", "wzv", "ssh_swp", "tra_nxt", "dyn_nxt"]
     kern = {
         "dyn_adv": ["k_dyn_keg_zad"], "dyn_vor": ["k_dyn_vor"], "dyn_zdf": ["k_dyn_zdf"],
         "dyn_spg_ts": ["k_spg_cor1", "k_spg_cor2", "k_spg_init", "k_spg_p1a", "k_spg_p1b", "k_spg_s1", "k_spg_s2", "k_spg_s3",
@@ -294,6 +319,8 @@ def kernel_times(h):
         "lbc_lnk": ["k_lbc"],
         "step_dyn": ["k_step_dyn1", "k_step_dyn2"],
         "step_tra": ["k_fct_h", "k_fct_v", "k_fct_lim", "k_fct_beta", "k_fct_final", "k_zero_ring", "k_tra_zdf"],
+        "div_hor": ["k_div_hor"], "ssh_nxt": ["k_ssh_nxt"], "wzv": ["k_wzv"], "ssh_swp": ["k_ssh_swp"],
+        "tra_nxt": ["k_tra_nxt"], "dyn_nxt": ["k_dyn_nxt1", "k_dyn_nxt2"],
     }
     for n in names:
         out[n] = h.timing_get(n)
@@ -449,6 +476,17 @@ def main():
             f"HBM fraction of step {res[form]['hbm_frac_step']:.3f}, finite={finite}")
         for k, v in sorted(res[form]["routines_ms_per_step"].items(), key=lambda x: -x[1]):
             log(f"   {k:14s} {v:8.3f} ms")
+    whole = {}
+    for form in [f for f in a.whole_step.split(",") if f]:
+        ms, finite, kt, _ = time_form(h, d, S, form.startswith("fp32"), True, a.steps, a.warmup, a.timing_steps,
+                                      whole=True)
+        closure = ("div_hor", "ssh_nxt", "wzv", "ssh_swp", "tra_nxt", "dyn_nxt")
+        whole[form] = {"ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3), "finite": finite,
+                       "closure_ms_per_step": {k: kt[k][0] / a.timing_steps for k in closure if kt[k][1] > 0},
+                       "closure_kernels_ms_per_step": {k: v[0] / a.timing_steps for k, v in kt.items()
+                                                       if "/" in k and k.split("/")[0] in closure and v[1] > 0}}
+        log(f"whole step {form}: {ms:.2f} ms/step, {cells / (ms * 1e-3):.3e} cell-updates/s, finite={finite}, "
+            f"closure {sum(whole[form]['closure_ms_per_step'].values()):.3f} ms")
     # roofline of the dominant kernel of the headline form: its launches timed
     # with HIP events inside the timed region
     width = 4 if head.startswith("fp32") else 8
@@ -500,6 +538,7 @@ def main():
         "stream_copy_GBps": copy_gbs,
         "host_mode_fp64_ms_per_step": host_ms,
         "ensemble": ens,
+        "whole_step": whole,
         "forms": res,
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -563,6 +602,13 @@ KERNEL_BYTES = {
     # fused momentum step
     "k_step_dyn1": _b(5),        # r un vn wn, w ua va
     "k_step_dyn2": _b(11),       # r ua va un vn ub vb avm, w ua va un vn
+    # time-step closure
+    "k_div_hor": _b(3),          # r un vn, w hdivn
+    "k_ssh_nxt": lambda w, i, j, k, f: 3 * w * (i - 2) * (j - 2) * (k - 1) + 4 * 8 * i * j,   # + sshb emp_b emp ssha
+    "k_wzv": _b(2),              # r hdivn, w wn
+    "k_tra_nxt": _b(10),         # two tracers: r tsb tsn tsa, w tsb tsn
+    "k_dyn_nxt1": _b(4),         # r ua va, w ua va
+    "k_dyn_nxt2": _b(10),        # r ua va un vn ub vb, w un vn ub vb
 }
 
 if __name__ == "__main__":

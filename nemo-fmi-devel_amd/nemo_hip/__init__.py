@@ -36,6 +36,15 @@ def load_library(path=LIB_PATH):
     if not os.path.exists(path):
         raise NemoHipError(abi.NEMO_HIP_ERR_STATE,
                            f"{path} is missing: build it with __graft_entry__.build() (no CPU fallback exists)")
+    # One HIP runtime per process: the library binds to whichever
+    # libamdhip64 is loaded first.  When torch is installed its bundled
+    # runtime is loaded before the library, so that device tensors and the
+    # library share it (loading the system runtime first leaves torch
+    # without a device).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     return C.CDLL(path)
 
 

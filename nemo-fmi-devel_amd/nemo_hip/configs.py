@@ -463,6 +463,12 @@ def make_state_device(d, seed=1, umax=0.2, divergent=0.2, kz_range=(1e-5, 1e-1),
     sc = umax / torch.sqrt(torch.mean(un[u > 0] ** 2)).item()
     un *= sc
     vn *= sc
+    # The masked streamfunction jumps to zero at the coast, and on wide grids
+    # (smoothing scale jpi/16) those jumps reach 60 times the rms speed.  The
+    # cap at 5 umax keeps the advective Courant number of a stepped state
+    # below 0.15 at 10 km and 2 rdt = 1200 s (w follows from continuity below).
+    un.clamp_(-5.0 * umax, 5.0 * umax)
+    vn.clamp_(-5.0 * umax, 5.0 * umax)
     e3t = torch.from_numpy(d.e3t_1d).to(device)[:, None, None]
     e2u, e1v, r1_e1e2t = T2(A["e2u"]), T2(A["e1v"]), T2(A["r1_e1e2t"])
     zu = e2u[None] * e3t * un

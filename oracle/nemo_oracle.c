@@ -31,7 +31,13 @@
  * FCT (tra_adv_fct + nonosc), reductions (tra_ldf_iso -> tra_ldf_lap at zero
  * slope) and conservation / invariance laws (tracer and momentum budgets, the
  * no-work property of the ENE/EEN/EET/ENT vorticity terms, barotropic volume,
- * rest state, linearity).
+ * rest state, linearity).  The time-step closure (tests/test_oracle_nxt.py):
+ * the discrete Gauss theorem and streamfunction flows (div_hor), volume
+ * conservation and the fresh-water closed form (ssh_nxt), the closed form of
+ * the vertical integral and its agreement with the sea-level tendency (wzv),
+ * the Asselin filter on linear and quadratic sequences (ssh_swp, tra_nxt,
+ * dyn_nxt) and the depth-mean constraints of the time-splitting corrections
+ * (dyn_nxt).
  * Parity partially pinned: dyn_vor ENS and MIX near land; the forcing and
  * time-filter terms of dyn_spg_ts (drag, wind, un_bf/ub2_b); nonosc on
  * masked coasts (bounds invariant only).
