@@ -32,18 +32,37 @@ __device__ __forceinline__ void lbc_targets(const DevDomain &d, int i, int j, F 
         }
 }
 
-// horizontal divergence at an interior point, DYN/divhor.F90:79-83
+// horizontal divergence at the interior point (i,j), DYN/divhor.F90:79-83.
+// The column's metrics are loaded once; at(k) evaluates one level with the
+// reference's operation order.
 template <typename T>
-__device__ __forceinline__ double hdiv_at(const DevDomain &d, const T *un, const T *vn, int i, int j, int k)
-{
-    const Metrics<double> &m = d.md;
-    const uint32_t ij = IJ32(i, j);
-    return ((double)A2(m.e2u, i, j) * E3(m.e3u, ij, k) * (double)A3(un, i, j, k) -
-            (double)A2(m.e2u, i - 1, j) * E3(m.e3u, ij - 1, k) * (double)A3(un, i - 1, j, k) +
-            (double)A2(m.e1v, i, j) * E3(m.e3v, ij, k) * (double)A3(vn, i, j, k) -
-            (double)A2(m.e1v, i, j - 1) * E3(m.e3v, ij - d.jpi, k) * (double)A3(vn, i, j - 1, k)) *
-           A2(m.r1_e1e2t, i, j) / E3(m.e3t, ij, k);
-}
+struct HdivCol {
+    double a1, a0, b1, b0, r;
+    uint32_t ij;
+    const T *un, *vn;
+    __device__ __forceinline__ HdivCol(const DevDomain &d, const T *un_, const T *vn_, int i, int j) : un(un_), vn(vn_)
+    {
+        const Metrics<double> &m = d.md;
+        ij = IJ32(i, j);
+        a1 = A2(m.e2u, i, j);
+        a0 = A2(m.e2u, i - 1, j);
+        b1 = A2(m.e1v, i, j);
+        b0 = A2(m.e1v, i, j - 1);
+        r = A2(m.r1_e1e2t, i, j);
+    }
+    __device__ __forceinline__ double at(const DevDomain &d, int k) const
+    {
+        const Metrics<double> &m = d.md;
+        const T *u = plane_ptr(un, d.n2, k), *v = plane_ptr(vn, d.n2, k);
+        return (a1 * E3(m.e3u, ij, k) * (double)ldp(u, ij) - a0 * E3(m.e3u, ij - 1, k) * (double)ldp(u, ij - 1) +
+                b1 * E3(m.e3v, ij, k) * (double)ldp(v, ij) -
+                b0 * E3(m.e3v, ij - (uint32_t)d.jpi, k) * (double)ldp(v, ij - (uint32_t)d.jpi)) *
+               r / E3(m.e3t, ij, k);
+    }
+};
+
+// levels per thread of the per-cell closure kernels
+#define NXT_KB 8
 
 // ---------------------------------------------------------------------------
 // div_hor, DYN/divhor.F90:50-110: hdivn on levels 1..jpkm1 of the interior,
@@ -53,15 +72,24 @@ template <typename T>
 KERNEL void k_div_hor(DevDomain d, const T *__restrict__ un, const T *__restrict__ vn, T *__restrict__ hdivn)
 {
     THREAD_IJ(1, 1);
-    const int k = (int)blockIdx.z + 1;
+    const int k0 = (int)blockIdx.z * NXT_KB + 1;
     if (i > d.jpi || j > d.jpj) return;
     int si, sj;
     const bool ok = lbc_src(d, 'T', i, j, si, sj);
-    if (k == d.jpk) {   // level jpk is not computed: lbc_lnk only
-        if (si != i || sj != j || !ok) A3(hdivn, i, j, k) = ok ? A3(hdivn, si, sj, k) : T(0);
-        return;
+    const bool ring = si != i || sj != j || !ok;
+    const HdivCol<T> c(d, un, vn, ok ? si : 2, ok ? sj : 2);
+    const uint32_t ij = IJ32(i, j);
+#pragma unroll
+    for (int q = 0; q < NXT_KB; q++) {
+        const int k = k0 + q;
+        if (k > d.jpk) break;
+        T *h = plane_ptr(hdivn, d.n2, k);
+        if (k == d.jpk) {   // level jpk is not computed: lbc_lnk only
+            if (ring) stp(h, ij, ok ? ldp((const T *)h, c.ij) : T(0));
+        } else {
+            stp(h, ij, ok ? (T)c.at(d, k) : T(0));
+        }
     }
-    A3(hdivn, i, j, k) = ok ? (T)hdiv_at<T>(d, un, vn, si, sj, k) : T(0);
 }
 
 // ---------------------------------------------------------------------------
@@ -79,10 +107,12 @@ KERNEL void k_ssh_nxt(DevDomain d, double z2dt, double zcoef, const T *__restric
     int si, sj;
     const bool ok = lbc_src(d, 'T', i, j, si, sj);
     const uint32_t ij = IJ32(i, j);
+    const HdivCol<T> c(d, un, vn, ok ? si : 2, ok ? sj : 2);
     double zhdiv = 0.0;
+#pragma unroll 2
     for (int k = 1; k <= d.jpk - 1; k++) {
-        const T h = ok ? (T)hdiv_at<T>(d, un, vn, si, sj, k) : T(0);
-        A3(hdivn, i, j, k) = h;
+        const T h = ok ? (T)c.at(d, k) : T(0);
+        stp(plane_ptr(hdivn, d.n2, k), ij, h);
         zhdiv = zhdiv + E3(d.md.e3t, ij, k) * (double)h;
     }
     if (si != i || sj != j || !ok) A3(hdivn, i, j, d.jpk) = ok ? A3(hdivn, si, sj, d.jpk) : T(0);
@@ -102,6 +132,7 @@ KERNEL void k_wzv(DevDomain d, int first, double z1_2dt, const T *__restrict__ h
     const uint32_t ij = IJ32(i, j);
     if (first) A3(wn, i, j, d.jpk) = T(0);
     double w = (double)A3(wn, i, j, d.jpk);
+#pragma unroll 4
     for (int k = d.jpk - 1; k >= 1; k--) {
         const double e3 = E3(d.md.e3t, ij, k);
         w = w - (e3 * (double)A3(hdivn, i, j, k) + z1_2dt * (e3 - e3)) * TMASK(double, i, j, k);
@@ -218,6 +249,7 @@ KERNEL void k_dyn_nxt1(DevDomain d, int spg_ts, int bt_fw, T *__restrict__ ua, T
     if (spg_ts) {
         zue = E3(m.e3u, ij, 1) * (double)A3(ua, i, j, 1) * UMASK(double, i, j, 1);
         zve = E3(m.e3v, ij, 1) * (double)A3(va, i, j, 1) * VMASK(double, i, j, 1);
+#pragma unroll 4
         for (int k = 2; k <= jpk - 1; k++) {
             zue = zue + E3(m.e3u, ij, k) * (double)A3(ua, i, j, k) * UMASK(double, i, j, k);
             zve = zve + E3(m.e3v, ij, k) * (double)A3(va, i, j, k) * VMASK(double, i, j, k);
@@ -225,6 +257,7 @@ KERNEL void k_dyn_nxt1(DevDomain d, int spg_ts, int bt_fw, T *__restrict__ ua, T
         zue = zue * r1_hu[ij];
         zve = zve * r1_hv[ij];
     }
+#pragma unroll 4
     for (int k = 1; k <= jpk; k++) {
         T a = A3(ua, i, j, k), b = A3(va, i, j, k);
         if (spg_ts && k < jpk) {
@@ -257,6 +290,7 @@ KERNEL void k_dyn_nxt2(DevDomain d, int euler, int bt_fix, double atfp, T *__res
     const uint32_t ij = IJ32(i, j);
     const int jpkm1 = d.jpk - 1;
     double sun = 0.0, svn = 0.0, sub = 0.0, svb = 0.0;   // depth sums of un, vn, ub, vb
+#pragma unroll 2
     for (int k = 1; k <= jpkm1; k++) {
         const double a = (double)A3(ua, i, j, k), b = (double)A3(va, i, j, k);
         const double e3u = E3(m.e3u, ij, k), e3v = E3(m.e3v, ij, k);
@@ -286,6 +320,7 @@ KERNEL void k_dyn_nxt2(DevDomain d, int euler, int bt_fix, double atfp, T *__res
     if (!euler && bt_fix) {
         // lines 286-297: the depth mean of ub/vb set to the input un_b/vn_b
         const double cu = sub * r1_hu[ij] - un_b[ij], cv = svb * r1_hv[ij] - vn_b[ij];
+#pragma unroll 4
         for (int k = 1; k <= jpkm1; k++) {
             const double e3u = E3(m.e3u, ij, k), e3v = E3(m.e3v, ij, k);
             const double mu = UMASK(double, i, j, k), mv = VMASK(double, i, j, k);
@@ -312,7 +347,7 @@ int launch_div_hor(const T *un, const T *vn, T *hdivn)
 {
     const DevDomain &d = nh::dom();
     Grid2 g = grid2(d.jpi, d.jpj);
-    g.grid.z = (unsigned)d.jpk;
+    g.grid.z = (unsigned)((d.jpk + NXT_KB - 1) / NXT_KB);
     LAUNCH("div_hor", k_div_hor<T>, g, d, un, vn, hdivn);
     return 0;
 }

@@ -380,16 +380,16 @@ KERNEL void k_fct_final(DevDomain d, const T * __restrict__ paa, const T * __res
 // (same expressions, same rounding to the storage type), and the ring values
 // lbc_lnk gives a closed domain (zero betas and limited fluxes) are selected
 // where they apply.
+// k_fct_lim tiles: LIM_TX x TY cells per 64x4 workgroup, so each thread owns
+// TY/4 cells of its i column (rows ty, ty+4, ...); taller tiles spread the
+// halo of the bounds (2 cells) and of the betas (1 cell) over more cells and
+// do more work between two barriers of the k-march
 #define LIM_TX 64
-#define LIM_TY 4
-#define LIM_ZX (LIM_TX + 4)
-#define LIM_ZY (LIM_TY + 4)
-#define LIM_BX (LIM_TX + 2)
-#define LIM_BY (LIM_TY + 2)
-#define LIM_NZ ((LIM_ZX * LIM_ZY + NH_BLOCK - 1) / NH_BLOCK)
-#define LIM_NB ((LIM_BX * LIM_BY + NH_BLOCK - 1) / NH_BLOCK)
-template <typename T>
-KERNEL_W(4) void k_fct_lim(DevDomain d, double p2dt, const T *__restrict__ pbef_, const T *__restrict__ paft_,
+template <typename T> struct FctLimTY {
+    static constexpr int value = sizeof(T) == 4 ? 8 : 4;   // 64x8 tiles for FP32, 64x4 for FP64
+};
+template <typename T, int TY>
+__global__ __launch_bounds__(NH_BLOCK, 3) void k_fct_lim(DevDomain d, double p2dt, const T *__restrict__ pbef_, const T *__restrict__ paft_,
                       const T *__restrict__ paa_, const T *__restrict__ pbb_, const T *__restrict__ pcc_,
                       T *__restrict__ ta_)
 {
@@ -399,8 +399,12 @@ KERNEL_W(4) void k_fct_lim(DevDomain d, double p2dt, const T *__restrict__ pbef_
     // clamp of the FP32 form
     using ZB = typename std::conditional<sizeof(T) == 4, float, double>::type;
     const ZB zsent = sizeof(T) == 4 ? (ZB)INFINITY : (ZB)1.e+40;
-    __shared__ ZB zup_s[3][LIM_ZY][LIM_ZX], zdo_s[3][LIM_ZY][LIM_ZX];
-    __shared__ T bup_s[2][LIM_BY][LIM_BX], bdo_s[2][LIM_BY][LIM_BX];
+    constexpr int ZX = LIM_TX + 4, ZY = TY + 4, BX = LIM_TX + 2, BY = TY + 2;
+    constexpr int NZ = (ZX * ZY + NH_BLOCK - 1) / NH_BLOCK, NB = (BX * BY + NH_BLOCK - 1) / NH_BLOCK;
+    constexpr int WY = NH_BLOCK / LIM_TX, NR = TY / WY;   // workgroup rows, cells per thread
+    static_assert(TY % WY == 0, "tile height must be a multiple of the workgroup height");
+    __shared__ ZB zup_s[3][ZY][ZX], zdo_s[3][ZY][ZX];
+    __shared__ T bup_s[2][BY][BX], bdo_s[2][BY][BX];
     // blockIdx.z selects the tracer of the pair
     const size_t o = (size_t)blockIdx.z * d.n3;
     const T *pbef = pbef_ + o, *paft = paft_ + o, *paa = paa_ + o, *pbb = pbb_ + o, *pcc = pcc_ + o;
@@ -408,34 +412,41 @@ KERNEL_W(4) void k_fct_lim(DevDomain d, double p2dt, const T *__restrict__ pbef_
     unsigned bx, by;
     xcd_tile(bx, by);
     const int tx = (int)threadIdx.x, ty = (int)threadIdx.y, tid = ty * LIM_TX + tx;
-    const int i0 = 2 + (int)bx * LIM_TX, j0 = 2 + (int)by * LIM_TY;
-    const int i = i0 + tx, j = j0 + ty;
+    const int i0 = 2 + (int)bx * LIM_TX, j0 = 2 + (int)by * TY;
+    const int i = i0 + tx;
     const int jpi = d.jpi, jpj = d.jpj, jpk = d.jpk;
-    const bool own = (i <= jpi - 1 && j <= jpj - 1);
     const double zrtrn = 1.e-15;
     const Metrics<T> &m = met<T>(d);
 
     // ---- this thread's points of the bounds region (tile + 2)
-    int zpx[LIM_NZ], zpy[LIM_NZ];
-    bool zin[LIM_NZ], zon[LIM_NZ];
-    T ztb[LIM_NZ], zzw[LIM_NZ];
+    // Every global load of the march is unconditional, from an address
+    // clamped into the domain (values of points outside it are loaded and
+    // ignored), so the compiler can count the loads in flight and wait for a
+    // prefetched level without waiting for the next one.
+    int zpx[NZ], zpy[NZ];
+    bool zin[NZ], zon[NZ];
+    T ztb[NZ], zzw[NZ];
+    uint32_t zkt[NZ], zof[NZ];   // packed tmask interval (loaded once), in-plane offset
 #pragma unroll
-    for (int r = 0; r < LIM_NZ; r++) {
+    for (int r = 0; r < NZ; r++) {
         const int q = tid + r * NH_BLOCK;
-        zon[r] = q < LIM_ZX * LIM_ZY;
-        zpx[r] = q % LIM_ZX;
-        zpy[r] = q / LIM_ZX;
+        zon[r] = q < ZX * ZY;
+        zpx[r] = q % ZX;
+        zpy[r] = q / ZX;
         const int pi = i0 - 2 + zpx[r], pj = j0 - 2 + zpy[r];
         zin[r] = zon[r] && pi >= 1 && pi <= jpi && pj >= 1 && pj <= jpj;
+        zof[r] = IJ32(min(max(pi, 1), jpi), min(max(pj, 1), jpj));
+        zkt[r] = zin[r] ? d.ktu[zof[r]] : 0u;
     }
+    // tmask(kk) of zb point r from its interval (TMASK without a load)
+    auto zwet = [&](int r, int kk) { return kk >= (int)(zkt[r] & 255u) && kk <= (int)((zkt[r] >> 8) & 255u); };
     auto zb_fetch = [&](int kk) {
+        const int kc = min(kk, jpk);
+        const T *pb = plane_ptr(pbef, d.n2, kc), *pa = plane_ptr(paft, d.n2, kc);
 #pragma unroll
-        for (int r = 0; r < LIM_NZ; r++) {
-            if (zin[r] && kk <= jpk) {
-                const int pi = i0 - 2 + zpx[r], pj = j0 - 2 + zpy[r];
-                ztb[r] = A3(pbef, pi, pj, kk);
-                zzw[r] = A3(paft, pi, pj, kk);
-            }
+        for (int r = 0; r < NZ; r++) {
+            ztb[r] = ldp(pb, zof[r]);
+            zzw[r] = ldp(pa, zof[r]);
         }
     };
     // zbup / zbdo of traadv_fct.F90:361-364 from the fetched values: with
@@ -443,18 +454,17 @@ KERNEL_W(4) void k_fct_lim(DevDomain d, double p2dt, const T *__restrict__ pbef_
     auto zb_store = [&](int kk) {
         const int sl = kk % 3;
 #pragma unroll
-        for (int r = 0; r < LIM_NZ; r++) {
+        for (int r = 0; r < NZ; r++) {
             if (!zon[r]) continue;
             ZB u = 0, w = 0;
             if (zin[r] && kk <= jpk) {
-                const int pi = i0 - 2 + zpx[r], pj = j0 - 2 + zpy[r];
                 if constexpr (sizeof(T) == 4) {
-                    const bool wet = TMASK(int, pi, pj, kk) != 0;
+                    const bool wet = zwet(r, kk);
                     const ZB a = (ZB)ztb[r], b = (ZB)zzw[r];
                     u = wet ? (a > b ? a : b) : -zsent;
                     w = wet ? (a < b ? a : b) : zsent;
                 } else {
-                    const double tm = TMASK(double, pi, pj, kk);
+                    const double tm = zwet(r, kk) ? 1.0 : 0.0;
                     const double au = (double)ztb[r] * tm - zsent * (1.0 - tm);
                     const double bu = (double)zzw[r] * tm - zsent * (1.0 - tm);
                     u = au > bu ? au : bu;
@@ -469,33 +479,38 @@ KERNEL_W(4) void k_fct_lim(DevDomain d, double p2dt, const T *__restrict__ pbef_
     };
 
     // ---- this thread's points of the beta region (tile + 1)
-    int bpx[LIM_NB], bpy[LIM_NB];
-    bool bon[LIM_NB], bin[LIM_NB];
-    T fa0[LIM_NB], fa1[LIM_NB], fb0[LIM_NB], fb1[LIM_NB], fc0[LIM_NB], fc1[LIM_NB], fzw[LIM_NB];
-    double fe1e2t[LIM_NB];
+    int bpx[NB], bpy[NB];
+    bool bon[NB], bin[NB];
+    uint32_t bof[NB];
+    T fa0[NB], fa1[NB], fb0[NB], fb1[NB], fc0[NB], fc1[NB], fzw[NB];
+    double fe1e2t[NB], fe3t[NB];
 #pragma unroll
-    for (int r = 0; r < LIM_NB; r++) {
+    for (int r = 0; r < NB; r++) {
         const int q = tid + r * NH_BLOCK;
-        bon[r] = q < LIM_BX * LIM_BY;
-        bpx[r] = q % LIM_BX;
-        bpy[r] = q / LIM_BX;
+        bon[r] = q < BX * BY;
+        bpx[r] = q % BX;
+        bpy[r] = q / BX;
         const int pi = i0 - 1 + bpx[r], pj = j0 - 1 + bpy[r];
         bin[r] = bon[r] && pi >= 2 && pi <= jpi - 1 && pj >= 2 && pj <= jpj - 1;
-        fe1e2t[r] = bin[r] ? d.md.e1e2t[IJ(pi, pj)] : 0.0;
+        bof[r] = IJ32(min(max(pi, 2), jpi - 1), min(max(pj, 2), jpj - 1));
+        fe1e2t[r] = bin[r] ? d.md.e1e2t[bof[r]] : 0.0;
     }
     auto beta_fetch = [&](int kk) {
+        const int kc = min(kk, jpk - 1);
+        const T *pa = plane_ptr(paa, d.n2, kc), *pb = plane_ptr(pbb, d.n2, kc), *pc = plane_ptr(pcc, d.n2, kc);
+        const T *pc1 = plane_ptr(pcc, d.n2, kc + 1), *pz = plane_ptr(paft, d.n2, kc);
+        const uint32_t w = (uint32_t)jpi;
 #pragma unroll
-        for (int r = 0; r < LIM_NB; r++) {
-            if (bin[r] && kk <= jpk - 1) {
-                const int pi = i0 - 1 + bpx[r], pj = j0 - 1 + bpy[r];
-                fa0[r] = A3(paa, pi - 1, pj, kk);
-                fa1[r] = A3(paa, pi, pj, kk);
-                fb0[r] = A3(pbb, pi, pj - 1, kk);
-                fb1[r] = A3(pbb, pi, pj, kk);
-                fc0[r] = A3(pcc, pi, pj, kk);
-                fc1[r] = A3(pcc, pi, pj, kk + 1);
-                fzw[r] = A3(paft, pi, pj, kk);
-            }
+        for (int r = 0; r < NB; r++) {
+            const uint32_t q = bof[r];
+            fa0[r] = ldp(pa, q - 1u);
+            fa1[r] = ldp(pa, q);
+            fb0[r] = ldp(pb, q - w);
+            fb1[r] = ldp(pb, q);
+            fc0[r] = ldp(pc, q);
+            fc1[r] = ldp(pc1, q);
+            fzw[r] = ldp(pz, q);
+            fe3t[r] = E3(d.md.e3t, q, kc);
         }
     };
     // nonosc betas, lines 366-399, then the closed-ring lbc_lnk (zero off
@@ -504,7 +519,7 @@ KERNEL_W(4) void k_fct_lim(DevDomain d, double p2dt, const T *__restrict__ pbef_
         const int sb = kk & 1;
         const int sa = (kk > 1 ? kk - 1 : 1) % 3, sc = kk % 3, sn = (kk + 1) % 3;
 #pragma unroll
-        for (int r = 0; r < LIM_NB; r++) {
+        for (int r = 0; r < NB; r++) {
             if (!bon[r]) continue;
             T bu_t = T(0), bd_t = T(0);
             if (bin[r] && kk <= jpk - 1) {
@@ -529,8 +544,7 @@ KERNEL_W(4) void k_fct_lim(DevDomain d, double p2dt, const T *__restrict__ pbef_
                                     fmax2(0.0, c_kp1) - fmin2(0.0, c_k);
                 const double zneg = fmax2(0.0, a_i) - fmin2(0.0, a_im1) + fmax2(0.0, b_j) - fmin2(0.0, b_jm1) +
                                     fmax2(0.0, c_k) - fmin2(0.0, c_kp1);
-                const int pi = i0 - 1 + bpx[r], pj = j0 - 1 + bpy[r];
-                const double zbt = fe1e2t[r] * E3(d.md.e3t, IJ(pi, pj), kk) / p2dt;
+                const double zbt = fe1e2t[r] * fe3t[r] / p2dt;
                 const double paft_k = fzw[r];
                 const double bu = ((double)zup - paft_k) / (zpos + zrtrn) * zbt;
                 const double bd = (paft_k - (double)zdo) / (zneg + zrtrn) * zbt;
@@ -547,35 +561,60 @@ KERNEL_W(4) void k_fct_lim(DevDomain d, double p2dt, const T *__restrict__ pbef_
         }
     };
 
-    // ---- this thread's cell: the limited divergence (lines 404-422, 288-297)
-    T ga0 = T(0), ga1 = T(0), gb0 = T(0), gb1 = T(0), gc1 = T(0), gta = T(0);
+    // ---- this thread's cells: the limited divergence (lines 404-422, 288-297)
+    int jr[NR];
+    bool own[NR];
+    uint32_t ijr[NR], fof[NR];
+    T ga0[NR], ga1[NR], gb0[NR], gb1[NR], gc1[NR], gta[NR], ge3t[NR], r1_e1e2t[NR];
+    double lz_k[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        jr[r] = j0 + ty + WY * r;
+        own[r] = (i <= jpi - 1 && jr[r] <= jpj - 1);
+        ijr[r] = own[r] ? IJ32(i, jr[r]) : 0u;
+        fof[r] = IJ32(min(i, jpi - 1), min(jr[r], jpj - 1));
+        r1_e1e2t[r] = own[r] ? m.r1_e1e2t[ijr[r]] : T(0);
+        lz_k[r] = own[r] ? (double)A3(pcc, i, jr[r], 1) : 0.0;
+        ga0[r] = ga1[r] = gb0[r] = gb1[r] = gc1[r] = gta[r] = ge3t[r] = T(1);
+    }
     auto fin_fetch = [&](int k) {
-        if (own && k <= jpk - 1) {
-            ga0 = A3(paa, i - 1, j, k);
-            ga1 = A3(paa, i, j, k);
-            gb0 = A3(pbb, i, j - 1, k);
-            gb1 = A3(pbb, i, j, k);
-            gc1 = A3(pcc, i, j, k + 1);
-            gta = A3(ta, i, j, k);
+        const int kc = min(k, jpk - 1);
+        const T *pa = plane_ptr(paa, d.n2, kc), *pb = plane_ptr(pbb, d.n2, kc), *pc1 = plane_ptr(pcc, d.n2, kc + 1);
+        const T *pt = plane_ptr((const T *)ta, d.n2, kc);
+        const uint32_t w = (uint32_t)jpi;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            const uint32_t q = fof[r];
+            ga0[r] = ldp(pa, q - 1u);
+            ga1[r] = ldp(pa, q);
+            gb0[r] = ldp(pb, q - w);
+            gb1[r] = ldp(pb, q);
+            gc1[r] = ldp(pc1, q);
+            gta[r] = ldp(pt, q);
+            ge3t[r] = E3(m.e3t, q, kc);
         }
     };
-    const size_t ij = own ? IJ(i, j) : 0;
-    const T r1_e1e2t = own ? m.r1_e1e2t[ij] : T(0);
-    double lz_k = own ? (double)A3(pcc, i, j, 1) : 0.0;
-    const int cx = tx + 1, cy = ty + 1;
-    const bool okxm = i - 1 >= 2, okym = j - 1 >= 2;
+    const int cx = tx + 1;
+    const bool okxm = i - 1 >= 2;
     auto fin_do = [&](int k) {
-        if (!own) return;
         const int s0 = k & 1, s1 = (k + 1) & 1;
-        const double bu_c = bup_s[s0][cy][cx], bd_c = bdo_s[s0][cy][cx];
-        const double lx_i = lim_face<T>(ga1, bd_c, bup_s[s0][cy][cx + 1], bu_c, bdo_s[s0][cy][cx + 1]);
-        const double lx_im1 = okxm ? lim_face<T>(ga0, bdo_s[s0][cy][cx - 1], bu_c, bup_s[s0][cy][cx - 1], bd_c) : 0.0;
-        const double ly_j = lim_face<T>(gb1, bd_c, bup_s[s0][cy + 1][cx], bu_c, bdo_s[s0][cy + 1][cx]);
-        const double ly_jm1 = okym ? lim_face<T>(gb0, bdo_s[s0][cy - 1][cx], bu_c, bup_s[s0][cy - 1][cx], bd_c) : 0.0;
-        const double lz_kp1 = lim_face<T>(gc1, bdo_s[s1][cy][cx], bu_c, bup_s[s1][cy][cx], bd_c);
-        A3(ta, i, j, k) = gta - ((T)lx_i - (T)lx_im1 + (T)ly_j - (T)ly_jm1 + (T)lz_k - (T)lz_kp1) * r1_e1e2t /
-                                    E3(m.e3t, ij, k);
-        lz_k = lz_kp1;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            if (!own[r]) continue;
+            const int j = jr[r], cy = ty + WY * r + 1;
+            const bool okym = j - 1 >= 2;
+            const double bu_c = bup_s[s0][cy][cx], bd_c = bdo_s[s0][cy][cx];
+            const double lx_i = lim_face<T>(ga1[r], bd_c, bup_s[s0][cy][cx + 1], bu_c, bdo_s[s0][cy][cx + 1]);
+            const double lx_im1 =
+                okxm ? lim_face<T>(ga0[r], bdo_s[s0][cy][cx - 1], bu_c, bup_s[s0][cy][cx - 1], bd_c) : 0.0;
+            const double ly_j = lim_face<T>(gb1[r], bd_c, bup_s[s0][cy + 1][cx], bu_c, bdo_s[s0][cy + 1][cx]);
+            const double ly_jm1 =
+                okym ? lim_face<T>(gb0[r], bdo_s[s0][cy - 1][cx], bu_c, bup_s[s0][cy - 1][cx], bd_c) : 0.0;
+            const double lz_kp1 = lim_face<T>(gc1[r], bdo_s[s1][cy][cx], bu_c, bup_s[s1][cy][cx], bd_c);
+            A3(ta, i, j, k) = gta[r] - ((T)lx_i - (T)lx_im1 + (T)ly_j - (T)ly_jm1 + (T)lz_k[r] - (T)lz_kp1) *
+                                           r1_e1e2t[r] / ge3t[r];
+            lz_k[r] = lz_kp1;
+        }
     };
 
     // ---- prologue: bounds of levels 1-3, betas of level 1, loads of the
@@ -1502,9 +1541,11 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
         if ((st = launch_lbc<T>(zwz, nk, 'W', T(0)))) return st;
         if (!d.iper && !d.jper) {
             // closed domain: betas and limited divergence fused, both tracers in one launch
-            Grid2 gl = grid2(d.jpi - 2, d.jpj - 2);
+            constexpr int TY = FctLimTY<T>::value;
+            Grid2 gl = grid2(d.jpi - 2, d.jpj - 2, LIM_TX, NH_BLOCK / LIM_TX);
+            gl.grid.y = (unsigned)((d.jpj - 2 + TY - 1) / TY);
             gl.grid.z = (unsigned)nt;
-            LAUNCH(nm, (k_fct_lim<T>), gl, d, p2dt, ptb + o, zwi, zwx, zwy, zwz, pta + o);
+            LAUNCH(nm, (k_fct_lim<T, TY>), gl, d, p2dt, ptb + o, zwi, zwx, zwy, zwz, pta + o);
             continue;
         }
         for (int t = 0; t < nt; t++) {
