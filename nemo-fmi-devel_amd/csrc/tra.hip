@@ -906,25 +906,49 @@ KERNEL void k_tra_ldf_iso(DevDomain d, T zsign, const T * __restrict__ pahu, con
         tbb_k[t] = (MODE == 2) ? ldp(ptbb + (size_t)t * n3, c.c) : T(0);
         ztfw_k[t] = T(0);
     }
+    // Every load of iteration k is issued one iteration ahead (Lv), without
+    // conditions, so the march waits for a level's data while the next
+    // level's loads are in flight.  All of them stay inside the arrays:
+    // iteration k reads levels k and k+1 <= jpk.
+    struct Lv {
+        T ahu_c1, ahu_w1, ahv_c1, ahv_s1, us_c, us_w, vs_c, vs_s;
+        T e3t, e3u_c, e3u_w, e3v_c, e3v_s, e3w1, wsi, wsj, ahw, akw;
+        T tb1[NT][5], tbb1[NT], ta[NT];
+    };
+    auto fetch = [&](int k, Lv &v) {
+        const T *ahu1 = plane_ptr(pahu, plane, k + 1), *ahv1 = plane_ptr(pahv, plane, k + 1);
+        v.ahu_c1 = ldp(ahu1, c.c); v.ahu_w1 = ldp(ahu1, c.w);
+        v.ahv_c1 = ldp(ahv1, c.c); v.ahv_s1 = ldp(ahv1, c.s);
+        const T *usk = plane_ptr(uslp, plane, k), *vsk = plane_ptr(vslp, plane, k);
+        v.us_c = ldp(usk, c.c); v.us_w = ldp(usk, c.w); v.vs_c = ldp(vsk, c.c); v.vs_s = ldp(vsk, c.s);
+        v.e3t = E3(m.e3t, c.c, k); v.e3u_c = E3(m.e3u, c.c, k); v.e3u_w = E3(m.e3u, c.w, k);
+        v.e3v_c = E3(m.e3v, c.c, k); v.e3v_s = E3(m.e3v, c.s, k);
+        v.e3w1 = E3(m.e3w, c.c, k + 1);
+        v.wsi = ldp(plane_ptr(wslpi, plane, k + 1), c.c);
+        v.wsj = ldp(plane_ptr(wslpj, plane, k + 1), c.c);
+        v.ahw = ldp(plane_ptr(ah_wslp2, plane, k + 1), c.c);
+        v.akw = (MODE != 1) ? ldp(plane_ptr(akz, plane, k + 1), c.c) : T(0);
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const T *tb1p = plane_ptr(ptb + (size_t)t * n3, plane, k + 1);
+#pragma unroll
+            for (int q = 0; q < 5; q++) v.tb1[t][q] = ldp(tb1p, col[q]);
+            v.tbb1[t] = (MODE == 2) ? ldp(plane_ptr(ptbb + (size_t)t * n3, plane, k + 1), c.c) : T(0);
+            v.ta[t] = ldp((const T *)plane_ptr(pta + (size_t)t * n3, plane, k), c.c);
+        }
+    };
+    Lv cur, nxt;
+    fetch(1, cur);
     for (int k = 1; k <= jpk - 1; k++) {
+        fetch(k + 1 <= jpk - 1 ? k + 1 : jpk - 1, nxt);
         // level k and k+1 planes (k+1 = jpk only feeds terms the reference
         // multiplies by a zero mask or skips)
-        const T *ahu1 = plane_ptr(pahu, plane, k + 1), *ahv1 = plane_ptr(pahv, plane, k + 1);
-        const T ahu_c1 = ldp(ahu1, c.c), ahu_w1 = ldp(ahu1, c.w);
-        const T ahv_c1 = ldp(ahv1, c.c), ahv_s1 = ldp(ahv1, c.s);
-        const T *usk = plane_ptr(uslp, plane, k), *vsk = plane_ptr(vslp, plane, k);
-        const T us_c = ldp(usk, c.c), us_w = ldp(usk, c.w), vs_c = ldp(vsk, c.c), vs_s = ldp(vsk, c.s);
-        const T e3t = E3(m.e3t, c.c, k), e3u_c = E3(m.e3u, c.c, k), e3u_w = E3(m.e3u, c.w, k);
-        const T e3v_c = E3(m.e3v, c.c, k), e3v_s = E3(m.e3v, c.s, k);
+        const T ahu_c1 = cur.ahu_c1, ahu_w1 = cur.ahu_w1, ahv_c1 = cur.ahv_c1, ahv_s1 = cur.ahv_s1;
+        const T us_c = cur.us_c, us_w = cur.us_w, vs_c = cur.vs_c, vs_s = cur.vs_s;
+        const T e3t = cur.e3t, e3u_c = cur.e3u_c, e3u_w = cur.e3u_w, e3v_c = cur.e3v_c, e3v_s = cur.e3v_s;
         const bool wlev = (k + 1 <= jpk - 1);       // ztfw(k+1) is a computed level
-        T wsi = T(0), wsj = T(0), ahw = T(0), akw = T(0), e3w1 = T(1);
-        if (wlev) {
-            wsi = ldp(plane_ptr(wslpi, plane, k + 1), c.c);
-            wsj = ldp(plane_ptr(wslpj, plane, k + 1), c.c);
-            ahw = ldp(plane_ptr(ah_wslp2, plane, k + 1), c.c);
-            if (MODE != 1) akw = ldp(plane_ptr(akz, plane, k + 1), c.c);
-            e3w1 = E3(m.e3w, c.c, k + 1);
-        }
+        const T wsi = wlev ? cur.wsi : T(0), wsj = wlev ? cur.wsj : T(0), ahw = wlev ? cur.ahw : T(0);
+        const T akw = (wlev && MODE != 1) ? cur.akw : T(0), e3w1 = wlev ? cur.e3w1 : T(1);
         // masks at k and k+1
         const T wm_c = wmk<T>(c, QC, k), wm_w = wmk<T>(c, QW, k), wm_e = wmk<T>(c, QE, k), wm_s = wmk<T>(c, QS, k),
                 wm_n = wmk<T>(c, QN, k);
@@ -954,10 +978,9 @@ KERNEL void k_tra_ldf_iso(DevDomain d, T zsign, const T * __restrict__ pahu, con
         }
 #pragma unroll
         for (int t = 0; t < NT; t++) {
-            const T *tb1p = plane_ptr(ptb + (size_t)t * n3, plane, k + 1);
             T tb1[5], dkn[5];
 #pragma unroll
-            for (int q = 0; q < 5; q++) tb1[q] = ldp(tb1p, col[q]);
+            for (int q = 0; q < 5; q++) tb1[q] = cur.tb1[t][q];
             // DK(q,k+1) = (tb(k) - tb(k+1)) * wmask(k+1) = zdk1t(q,k) = zdkt(q,k+1)
             dkn[QC] = (tb[t][QC] - tb1[QC]) * wm_c1;
             dkn[QW] = (tb[t][QW] - tb1[QW]) * wm_w1;
@@ -975,7 +998,7 @@ KERNEL void k_tra_ldf_iso(DevDomain d, T zsign, const T * __restrict__ pahu, con
             const T zdit_c1 = (tb1[QE] - tb1[QC]) * um_c1, zdit_w1 = (tb1[QC] - tb1[QW]) * um_w1;
             const T zdjt_c1 = (tb1[QN] - tb1[QC]) * vm_c1, zdjt_s1 = (tb1[QC] - tb1[QS]) * vm_s1;
             T ztfw1 = T(0), tbb1 = T(0);
-            if (MODE == 2) tbb1 = ldp(plane_ptr(ptbb + (size_t)t * n3, plane, k + 1), c.c);
+            if (MODE == 2) tbb1 = cur.tbb1[t];
             if (wlev) {
                 ztfw1 = zcoef3 * (zdit_c[t] + zdit_w1 + zdit_w[t] + zdit_c1) +
                         zcoef4 * (zdjt_c[t] + zdjt_s1 + zdjt_s[t] + zdjt_c1);
@@ -986,7 +1009,7 @@ KERNEL void k_tra_ldf_iso(DevDomain d, T zsign, const T * __restrict__ pahu, con
             }
             const T hdiv = zftu_c - zftu_w + zftv_c - zftv_s;
             T *tak = plane_ptr(pta + (size_t)t * n3, plane, k);
-            T a = ldp((const T *)tak, c.c);
+            T a = cur.ta[t];
             a = a + zsign * hdiv * r1_e1e2t / e3t;
             a = a + zsign * (ztfw_k[t] - ztfw1) * r1_e1e2t / e3t;
             stp(tak, c.c, a);
@@ -998,6 +1021,7 @@ KERNEL void k_tra_ldf_iso(DevDomain d, T zsign, const T * __restrict__ pahu, con
             tbb_k[t] = tbb1;
         }
         ahu_c = ahu_c1; ahu_w = ahu_w1; ahv_c = ahv_c1; ahv_s = ahv_s1;
+        cur = nxt;
     }
 }
 
