@@ -1407,11 +1407,14 @@ __device__ __forceinline__ T tp_upz(const DevDomain &d, const T *pw, const T *tb
 // fluxes of the (i+1/2, j) and (i, j+1/2) faces (upstream and antidiffusive,
 // lines 123-145, 180-221) and, on the interior, the low-order trend and the
 // first guess zwi (158-170)
-template <int TRP, typename T, int NH4, int TA0>
+// NT: the number of tracers of the launch, a template constant so that the
+// loads of both tracers of a pair are in flight together
+template <int TRP, typename T, int NH4, int TA0, int NT>
 KERNEL void k_fct_h(DevDomain d, double p2dt, const T *__restrict__ pu, const T *__restrict__ pv,
                     const T *__restrict__ pw, const T *__restrict__ ptb, const T *__restrict__ ptn, T *__restrict__ pta,
                     T *__restrict__ pzwi, T *__restrict__ pzwx, T *__restrict__ pzwy, int kjpt)
 {
+    (void)kjpt;
     THREAD_IJ(1, 1);
     const int k = (int)blockIdx.z + 1;
     if (i > d.jpi - 1 || j > d.jpj - 1) return;     // the flux loops: 1..jpim1 x 1..jpjm1
@@ -1429,8 +1432,8 @@ KERNEL void k_fct_h(DevDomain d, double p2dt, const T *__restrict__ pu, const T 
         tm = TMASK(T, i, j, k);
         r1_e1e2t = m.r1_e1e2t[ij];
     }
-#pragma unroll 1
-    for (int t = 0; t < kjpt; t++) {
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
         const T *tb = ptb + t * n3, *tn = ptn + t * n3;
         const T tbc = A3(tb, i, j, k), tbe = A3(tb, i + 1, j, k), tbn = A3(tb, i, j + 1, k);
         const T hu = T(0.5) * ((u + fabs(u)) * tbc + (u - fabs(u)) * tbe);
@@ -1548,14 +1551,17 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
         const size_t o = (size_t)t0 * n3;
         Grid2 g3 = grid2(d.jpi - 1, d.jpj - 1);
         g3.grid.z = (unsigned)(d.jpk - 1);
+#define FH_LAUNCH2(H, Z, N) \
+    LAUNCH(nm, (k_fct_h<TRP, T, H, Z, N>), g3, d, p2dt, pu, pv, pw, ptb + o, ptn + o, pta + o, zwi, zwx, zwy, nt)
 #define FH_LAUNCH(H, Z) \
-    LAUNCH(nm, (k_fct_h<TRP, T, H, Z>), g3, d, p2dt, pu, pv, pw, ptb + o, ptn + o, pta + o, zwi, zwx, zwy, nt)
+    do { if (nt == 2) FH_LAUNCH2(H, Z, 2); else FH_LAUNCH2(H, Z, 1); } while (0)
         if (kn_fct_h == 4) {
             if (ta0) FH_LAUNCH(1, 1); else FH_LAUNCH(1, 0);
         } else {
             if (ta0) FH_LAUNCH(0, 1); else FH_LAUNCH(0, 0);
         }
 #undef FH_LAUNCH
+#undef FH_LAUNCH2
         if (kn_fct_v == 4) LAUNCH(nm, (k_fct_v<TRP, T, 1>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
         else LAUNCH(nm, (k_fct_v<TRP, T, 0>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
         const int nk = nt * d.jpk;
