@@ -1463,17 +1463,18 @@ KERNEL void k_fct_h(DevDomain d, double p2dt, const T *__restrict__ pu, const T 
 
 // per interior column: the vertical antidiffusive flux zwz (lines 255-278),
 // the compact scheme's sweeps (577-614) with the table pivots
-template <int TRP, typename T, int NV4>
+template <int TRP, typename T, int NV4, int NT>
 KERNEL void k_fct_v(DevDomain d, PivotTab<T> pt, const T *__restrict__ pw, const T *__restrict__ ptb,
                     const T *__restrict__ ptn, T *__restrict__ pzwz, int kjpt)
 {
+    (void)kjpt;
     THREAD_IJ(2, 2);
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
     const size_t n3 = d.n3;
     const int jpk = d.jpk;
     const int ikb = d.mbkt[IJ(i, j)];
-#pragma unroll 1
-    for (int t = 0; t < kjpt; t++) {
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
         const T *tb = ptb + t * n3, *tn = ptn + t * n3;
         T *zwz = pzwz + t * n3;
         A3(zwz, i, j, 1) = T(0);
@@ -1562,8 +1563,13 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
         }
 #undef FH_LAUNCH
 #undef FH_LAUNCH2
-        if (kn_fct_v == 4) LAUNCH(nm, (k_fct_v<TRP, T, 1>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
-        else LAUNCH(nm, (k_fct_v<TRP, T, 0>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
+        if (kn_fct_v == 4) {
+            if (nt == 2) LAUNCH(nm, (k_fct_v<TRP, T, 1, 2>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
+            else LAUNCH(nm, (k_fct_v<TRP, T, 1, 1>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
+        } else {
+            if (nt == 2) LAUNCH(nm, (k_fct_v<TRP, T, 0, 2>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
+            else LAUNCH(nm, (k_fct_v<TRP, T, 0, 1>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
+        }
         const int nk = nt * d.jpk;
         if ((st = launch_lbc<T>(zwi, nk, 'T', T(0)))) return st;
         if ((st = launch_lbc<T>(zwx, nk, 'U', T(0)))) return st;
