@@ -138,6 +138,23 @@ typedef struct nemo_hip_spg_ts {
 } nemo_hip_spg_ts_t;
 
 /* ------------------------------------------------------------------------ */
+/* equation of state: the namelist state of eos_init (TRA/eosbn2.F90:1218),  */
+/* nameos parameters with the defaults of TRA/eosbn2.F90:89-95               */
+/* ------------------------------------------------------------------------ */
+#define NEMO_NP_TEOS10 (-1)
+#define NEMO_NP_EOS80  0
+#define NEMO_NP_SEOS   1
+typedef struct nemo_hip_eos {
+    int32_t neos;               /* NEMO_NP_SEOS (the polynomial TEOS-10 and
+                                   EOS-80 forms are not implemented)        */
+    int32_t reserved0;
+    double  rn_a0, rn_b0;       /* thermal / saline expansion               */
+    double  rn_lambda1, rn_lambda2;   /* cabbeling in T^2, S^2              */
+    double  rn_mu1, rn_mu2;     /* thermobaric in T, S                      */
+    double  rn_nu;              /* cabbeling in T*S                         */
+} nemo_hip_eos_t;
+
+/* ------------------------------------------------------------------------ */
 /* library control                                                          */
 /* ------------------------------------------------------------------------ */
 int nemo_hip_init(int device);
@@ -304,6 +321,33 @@ int nemo_hip_dyn_spg_ts(int kt, nemo_hip_spg_ts_t *st);
 int nemo_hip_ts_wgt(int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2);
 
 /* ------------------------------------------------------------------------ */
+/* density, pressure gradient, lateral viscosity                             */
+/* ------------------------------------------------------------------------ */
+/* eos_init, TRA/eosbn2.F90:1218-1700: the equation of state of the current
+   context (call after nemo_hip_domain_register) */
+int nemo_hip_eos_register(const nemo_hip_eos_t *eos);
+/* eos_insitu, TRA/eosbn2.F90:188-301: prd = (rho - rau0) / rau0 on
+   levels 1..jpkm1 from pts (jpi,jpj,jpk,2) and the depth pdep (jpi,jpj,jpk) */
+int nemo_hip_eos_insitu(const double *pts, double *prd, const double *pdep);
+/* eos_insitu_pot, TRA/eosbn2.F90:304-468: prd and the surface-referenced
+   potential density prhop */
+int nemo_hip_eos_insitu_pot(const double *pts, double *prd, double *prhop, const double *pdep);
+/* rab_3d, TRA/eosbn2.F90:566-676: thermal / haline expansion ratios pab
+   (jpi,jpj,jpk,2) at the t-point depths gdept_n */
+int nemo_hip_rab_3d(const double *pts, double *pab, const double *gdept);
+/* bn2, TRA/eosbn2.F90:889-933: N^2 at w-points, levels 2..jpkm1 */
+int nemo_hip_bn2(const double *pts, const double *pab, double *pn2,
+                 const double *gdept, const double *gdepw);
+/* hpg_zco, DYN/dynhpg.F90:258-323 (dyn_hpg with ln_hpg_zco): the hydrostatic
+   pressure gradient of the in situ density anomaly rhd added to ua, va */
+int nemo_hip_hpg_zco(int kt, const double *rhd, double *ua, double *va);
+/* dyn_ldf_lap, DYN/dynldf_lap_blp.F90:37-104: the iso-level laplacian
+   (rot/div form) of pub, pvb added to pua, pva; ahmt, ahmf are the
+   viscosities of LDF/ldfdyn.F90 (masked by tmask, fmask) */
+int nemo_hip_dyn_ldf_lap(int kt, const double *pub, const double *pvb, double *pua, double *pva,
+                         int kpass, const double *ahmt, const double *ahmf);
+
+/* ------------------------------------------------------------------------ */
 /* time-step closure around the hot path (linear free surface)              */
 /* ------------------------------------------------------------------------ */
 /* div_hor, DYN/divhor.F90:50-110: hdivn (jpi,jpj,jpk) on levels 1..jpkm1,
@@ -402,6 +446,16 @@ int nemo_hip_step_tra_sp(int kt, const float *un, const float *vn, const float *
                          const float *uslp, const float *vslp,
                          const float *wslpi, const float *wslpj,
                          float *ah_wslp2, float *akz, const float *avt);
+/* density, pressure gradient, lateral viscosity: float 3D fields; the
+   arithmetic is FP64, rounded once on store */
+int nemo_hip_eos_insitu_sp(const float *pts, float *prd, const float *pdep);
+int nemo_hip_eos_insitu_pot_sp(const float *pts, float *prd, float *prhop, const float *pdep);
+int nemo_hip_rab_3d_sp(const float *pts, float *pab, const float *gdept);
+int nemo_hip_bn2_sp(const float *pts, const float *pab, float *pn2,
+                    const float *gdept, const float *gdepw);
+int nemo_hip_hpg_zco_sp(int kt, const float *rhd, float *ua, float *va);
+int nemo_hip_dyn_ldf_lap_sp(int kt, const float *pub, const float *pvb, float *pua, float *pva,
+                            int kpass, const float *ahmt, const float *ahmf);
 /* time-step closure: float 3D fields, FP64 2D fields and arithmetic */
 int nemo_hip_div_hor_sp(int kt, const float *un, const float *vn, float *hdivn);
 int nemo_hip_ssh_nxt_sp(int kt, const float *un, const float *vn, float *hdivn,

@@ -36,6 +36,8 @@ struct Ctx {
     double r2dt_tra = 0, r2dt_zdf = 0, r2dt_dyn = 0;   // module r2dt of tra_adv / tra_zdf / dyn_zdf
     int spg_rot_u[3] = {0, 1, 2};                       // dyn_spg_ts level buffers: [now, before, before-before]
     int spg_rot_s[4] = {0, 1, 2, 3};                    // ... and sea level: [now, before, before-before, after]
+    bool have_eos = false;                              // eos_init state (nemo_hip_eos_register)
+    SeosPar eos{};
 };
 static std::vector<Ctx *> g_ctxs;
 static Ctx *g_c = nullptr;   // the current context
@@ -1190,6 +1192,86 @@ static int do_dyn_nxt(int kt, T *ub, T *vb, T *un, T *vn, T *ua, T *va, const do
 }
 
 // ---------------------------------------------------------------------------
+// density, pressure gradient, lateral viscosity
+// ---------------------------------------------------------------------------
+#define NEED_EOS()                                                                                         \
+    do {                                                                                                   \
+        if (!g_c->have_eos) return set_error(NEMO_HIP_ERR_STATE, "no equation of state registered (eos_register)"); \
+    } while (0)
+
+template <typename T>
+static int do_eos(int pot, const T *pts, T *prd, T *prhop, const T *pdep)
+{
+    NEED_DOMAIN();
+    NEED_EOS();
+    NEED_PTR(pts); NEED_PTR(prd); NEED_PTR(pdep);
+    if (pot && !prhop) return set_error(NEMO_HIP_ERR_ARG, "eos_insitu_pot: prhop is NULL");
+    const size_t n3 = g_c->dom.n3;
+    Stager S;
+    const T *ts = S.in(pts, 2 * n3), *dep = S.in(pdep, n3);
+    T *rd = S.inout(prd, n3), *rp = pot ? S.inout(prhop, n3) : nullptr;
+    int rc = S.status ? S.status : launch_eos<T>(g_c->eos, pot, ts, rd, rp, dep);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_rab(const T *pts, T *pab, const T *gdept)
+{
+    NEED_DOMAIN();
+    NEED_EOS();
+    NEED_PTR(pts); NEED_PTR(pab); NEED_PTR(gdept);
+    const size_t n3 = g_c->dom.n3;
+    Stager S;
+    const T *ts = S.in(pts, 2 * n3), *gt = S.in(gdept, n3);
+    T *ab = S.inout(pab, 2 * n3);
+    int rc = S.status ? S.status : launch_rab<T>(g_c->eos, ts, ab, gt);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_bn2(const T *pts, const T *pab, T *pn2, const T *gdept, const T *gdepw)
+{
+    NEED_DOMAIN();
+    NEED_PTR(pts); NEED_PTR(pab); NEED_PTR(pn2); NEED_PTR(gdept); NEED_PTR(gdepw);
+    const size_t n3 = g_c->dom.n3;
+    Stager S;
+    const T *ts = S.in(pts, 2 * n3), *ab = S.in(pab, 2 * n3), *gt = S.in(gdept, n3), *gw = S.in(gdepw, n3);
+    T *n2 = S.inout(pn2, n3);
+    int rc = S.status ? S.status : launch_bn2<T>(ts, ab, n2, gt, gw);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_hpg_zco(int kt, const T *rhd, T *ua, T *va)
+{
+    (void)kt;
+    NEED_DOMAIN();
+    NEED_PTR(rhd); NEED_PTR(ua); NEED_PTR(va);
+    const size_t n3 = g_c->dom.n3;
+    Stager S;
+    const T *r = S.in(rhd, n3);
+    T *a = S.inout(ua, n3), *b = S.inout(va, n3);
+    int rc = S.status ? S.status : launch_hpg_zco<T>(r, a, b);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_dyn_ldf_lap(int kt, const T *pub, const T *pvb, T *pua, T *pva, int kpass, const T *ahmt,
+                          const T *ahmf)
+{
+    (void)kt;
+    NEED_DOMAIN();
+    NEED_PTR(pub); NEED_PTR(pvb); NEED_PTR(pua); NEED_PTR(pva); NEED_PTR(ahmt); NEED_PTR(ahmf);
+    if (kpass != 1 && kpass != 2) return set_error(NEMO_HIP_ERR_ARG, "dyn_ldf_lap: kpass = %d", kpass);
+    const size_t n3 = g_c->dom.n3;
+    Stager S;
+    const T *ub = S.in(pub, n3), *vb = S.in(pvb, n3), *at = S.in(ahmt, n3), *af = S.in(ahmf, n3);
+    T *a = S.inout(pua, n3), *b = S.inout(pva, n3);
+    int rc = S.status ? S.status : launch_dyn_ldf_lap<T>(kpass == 1 ? 1.0 : -1.0, ub, vb, a, b, at, af);
+    return S.finish(rc);
+}
+
+// ---------------------------------------------------------------------------
 // FP64 entry points
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -1329,6 +1411,38 @@ int nemo_hip_dyn_nxt(int kt, double *ub, double *vb, double *un, double *vn, dou
                               r1_hu_n, r1_hv_n);
 }
 
+int nemo_hip_eos_register(const nemo_hip_eos_t *eos)
+{
+    NEED_DOMAIN();
+    NEED_NOT_CAPTURING("nemo_hip_eos_register");
+    NEED_PTR(eos);
+    if (eos->neos != NEMO_NP_SEOS)
+        return set_error(NEMO_HIP_ERR_UNSUPPORTED, "eos_register: neos = %d (only the simplified EOS, neos = %d)",
+                         eos->neos, NEMO_NP_SEOS);
+    g_c->eos = SeosPar{eos->rn_a0, eos->rn_b0, eos->rn_lambda1, eos->rn_lambda2, eos->rn_mu1, eos->rn_mu2, eos->rn_nu};
+    g_c->have_eos = true;
+    return NEMO_HIP_OK;
+}
+int nemo_hip_eos_insitu(const double *pts, double *prd, const double *pdep)
+{
+    return do_eos<double>(0, pts, prd, nullptr, pdep);
+}
+int nemo_hip_eos_insitu_pot(const double *pts, double *prd, double *prhop, const double *pdep)
+{
+    return do_eos<double>(1, pts, prd, prhop, pdep);
+}
+int nemo_hip_rab_3d(const double *pts, double *pab, const double *gdept) { return do_rab<double>(pts, pab, gdept); }
+int nemo_hip_bn2(const double *pts, const double *pab, double *pn2, const double *gdept, const double *gdepw)
+{
+    return do_bn2<double>(pts, pab, pn2, gdept, gdepw);
+}
+int nemo_hip_hpg_zco(int kt, const double *rhd, double *ua, double *va) { return do_hpg_zco<double>(kt, rhd, ua, va); }
+int nemo_hip_dyn_ldf_lap(int kt, const double *pub, const double *pvb, double *pua, double *pva, int kpass,
+                         const double *ahmt, const double *ahmf)
+{
+    return do_dyn_ldf_lap<double>(kt, pub, pvb, pua, pva, kpass, ahmt, ahmf);
+}
+
 int nemo_hip_ts_wgt(int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2)
 {
     NEED_DOMAIN();
@@ -1419,6 +1533,26 @@ int nemo_hip_dyn_nxt_sp(int kt, float *ub, float *vb, float *un, float *vn, floa
 {
     return do_dyn_nxt<float>(kt, ub, vb, un, vn, ua, va, ua_b, va_b, un_adv, vn_adv, ub_b, vb_b, un_b, vn_b,
                              r1_hu_n, r1_hv_n);
+}
+
+int nemo_hip_eos_insitu_sp(const float *pts, float *prd, const float *pdep)
+{
+    return do_eos<float>(0, pts, prd, nullptr, pdep);
+}
+int nemo_hip_eos_insitu_pot_sp(const float *pts, float *prd, float *prhop, const float *pdep)
+{
+    return do_eos<float>(1, pts, prd, prhop, pdep);
+}
+int nemo_hip_rab_3d_sp(const float *pts, float *pab, const float *gdept) { return do_rab<float>(pts, pab, gdept); }
+int nemo_hip_bn2_sp(const float *pts, const float *pab, float *pn2, const float *gdept, const float *gdepw)
+{
+    return do_bn2<float>(pts, pab, pn2, gdept, gdepw);
+}
+int nemo_hip_hpg_zco_sp(int kt, const float *rhd, float *ua, float *va) { return do_hpg_zco<float>(kt, rhd, ua, va); }
+int nemo_hip_dyn_ldf_lap_sp(int kt, const float *pub, const float *pvb, float *pua, float *pva, int kpass,
+                            const float *ahmt, const float *ahmf)
+{
+    return do_dyn_ldf_lap<float>(kt, pub, pvb, pua, pva, kpass, ahmt, ahmf);
 }
 
 }  // extern "C"

@@ -50,3 +50,14 @@ template <typename T>
 int launch_dyn_nxt(int euler, T *ub, T *vb, T *un, T *vn, T *ua, T *va, const double *ua_b, const double *va_b,
                    const double *un_adv, const double *vn_adv, double *ub_b, double *vb_b, double *un_b,
                    double *vn_b, const double *r1_hu, const double *r1_hv);
+
+// density, pressure gradient, lateral viscosity (phys.hip)
+struct SeosPar {   // simplified EOS coefficients, TRA/eosbn2.F90:89-95
+    double a0, b0, lambda1, lambda2, mu1, mu2, nu;
+};
+template <typename T> int launch_eos(const SeosPar &e, int pot, const T *pts, T *prd, T *prhop, const T *pdep);
+template <typename T> int launch_rab(const SeosPar &e, const T *pts, T *pab, const T *gdept);
+template <typename T> int launch_bn2(const T *pts, const T *pab, T *pn2, const T *gdept, const T *gdepw);
+template <typename T> int launch_hpg_zco(const T *rhd, T *ua, T *va);
+template <typename T>
+int launch_dyn_ldf_lap(double zsign, const T *pub, const T *pvb, T *pua, T *pva, const T *ahmt, const T *ahmf);

@@ -1,0 +1,230 @@
+// phys.hip -- density (the simplified equation of state of TRA/eosbn2.F90),
+// the hydrostatic pressure gradient in z-coordinates (DYN/dynhpg.F90) and
+// the iso-level laplacian viscosity (DYN/dynldf_lap_blp.F90).
+//
+// Arithmetic is FP64 in both storage forms, rounded once on store, in the
+// reference's operation order, so the FP64 form reproduces it bit for bit.
+#include "common.hpp"
+#include "kernels.hpp"
+
+// ---------------------------------------------------------------------------
+// eos_insitu (POT = 0), TRA/eosbn2.F90:276-299, and eos_insitu_pot (POT = 1),
+// lines 440-462, S-EOS.  One thread per cell, levels 1..jpkm1.
+// ---------------------------------------------------------------------------
+template <typename T, int POT>
+KERNEL void k_eos(DevDomain d, SeosPar e, const T *__restrict__ pts, T *__restrict__ prd, T *__restrict__ prhop,
+                  const T *__restrict__ pdep)
+{
+    THREAD_IJ(1, 1);
+    const int k = (int)blockIdx.z + 1;
+    if (i > d.jpi || j > d.jpj) return;
+    const uint32_t p = IJK32(i, j, k);
+    const double r1_rau0 = 1.0 / d.rau0;
+    const double zt = (double)at32(pts, p) - 10.;
+    const double zs = (double)at32(pts + d.n3, p) - 35.;
+    const double zh = (double)at32(pdep, p);
+    const double ztm = TMASK(double, i, j, k);
+    if (POT) {
+        double zn = -e.a0 * (1. + 0.5 * e.lambda1 * zt) * zt + e.b0 * (1. - 0.5 * e.lambda2 * zs) * zs -
+                    e.nu * zt * zs;
+        at32(prhop, p) = (T)((d.rau0 + zn) * ztm);
+        zn = zn - (e.a0 * e.mu1 * zt + e.b0 * e.mu2 * zs) * zh;
+        at32(prd, p) = (T)(zn * r1_rau0 * ztm);
+    } else {
+        const double zn = -e.a0 * (1. + 0.5 * e.lambda1 * zt + e.mu1 * zh) * zt +
+                          e.b0 * (1. - 0.5 * e.lambda2 * zs - e.mu2 * zh) * zs - e.nu * zt * zs;
+        at32(prd, p) = (T)(zn * r1_rau0 * ztm);
+    }
+}
+
+// rab_3d, TRA/eosbn2.F90:645-662 (S-EOS): alpha and beta at t-points
+template <typename T>
+KERNEL void k_rab(DevDomain d, SeosPar e, const T *__restrict__ pts, T *__restrict__ pab,
+                  const T *__restrict__ gdept)
+{
+    THREAD_IJ(1, 1);
+    const int k = (int)blockIdx.z + 1;
+    if (i > d.jpi || j > d.jpj) return;
+    const uint32_t p = IJK32(i, j, k);
+    const double r1_rau0 = 1.0 / d.rau0;
+    const double zt = (double)at32(pts, p) - 10.;
+    const double zs = (double)at32(pts + d.n3, p) - 35.;
+    const double zh = (double)at32(gdept, p);
+    const double ztm = TMASK(double, i, j, k);
+    double zn = e.a0 * (1. + e.lambda1 * zt + e.mu1 * zh) + e.nu * zs;
+    at32(pab, p) = (T)(zn * r1_rau0 * ztm);
+    zn = e.b0 * (1. - e.lambda2 * zs - e.mu2 * zh) - e.nu * zt;
+    at32(pab + d.n3, p) = (T)(zn * r1_rau0 * ztm);
+}
+
+// bn2, TRA/eosbn2.F90:914-925: N^2 at w-points 2..jpkm1 (grid.z = jpk - 2)
+template <typename T>
+KERNEL void k_bn2(DevDomain d, const T *__restrict__ pts, const T *__restrict__ pab, T *__restrict__ pn2,
+                  const T *__restrict__ gdept, const T *__restrict__ gdepw)
+{
+    THREAD_IJ(1, 1);
+    const int k = (int)blockIdx.z + 2;
+    if (i > d.jpi || j > d.jpj) return;
+    const uint32_t p = IJK32(i, j, k), q = IJK32(i, j, k - 1);
+    const size_t n3 = d.n3;
+    const double gt = (double)at32(gdept, p);
+    const double zrw = ((double)at32(gdepw, p) - gt) / ((double)at32(gdept, q) - gt);
+    const double zaw = (double)at32(pab, p) * (1. - zrw) + (double)at32(pab, q) * zrw;
+    const double zbw = (double)at32(pab + n3, p) * (1. - zrw) + (double)at32(pab + n3, q) * zrw;
+    at32(pn2, p) = (T)(d.grav *
+                       (zaw * ((double)at32(pts, q) - (double)at32(pts, p)) -
+                        zbw * ((double)at32(pts + n3, q) - (double)at32(pts + n3, p))) /
+                       E3(d.md.e3w, IJ32(i, j), k) * WMASK(double, i, j, k));
+}
+
+// ---------------------------------------------------------------------------
+// hpg_zco, DYN/dynhpg.F90:286-320: per interior column, the pressure
+// gradient integrated from the surface in an FP64 register
+// ---------------------------------------------------------------------------
+template <typename T>
+KERNEL void k_hpg_zco(DevDomain d, const T *__restrict__ rhd, T *__restrict__ ua, T *__restrict__ va)
+{
+    THREAD_IJ(2, 2);
+    if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    const Metrics<double> &m = d.md;
+    const uint32_t ij = IJ32(i, j), ie = ij + 1u, jn = ij + (uint32_t)d.jpi;
+    const size_t plane = d.n2;
+    const double zcoef0 = -d.grav * 0.5;
+    const double r1u = m.r1_e1u[ij], r1v = m.r1_e2v[ij];
+    double zhpi, zhpj;
+    {
+        const double zcoef1 = zcoef0 * E3(m.e3w, ij, 1);
+        const double rc = (double)ldp(rhd, ij);
+        zhpi = zcoef1 * ((double)ldp(rhd, ie) - rc) * r1u;
+        zhpj = zcoef1 * ((double)ldp(rhd, jn) - rc) * r1v;
+        stp(ua, ij, (T)((double)ldp((const T *)ua, ij) + zhpi));
+        stp(va, ij, (T)((double)ldp((const T *)va, ij) + zhpj));
+    }
+    double rc_m = (double)ldp(rhd, ij), re_m = (double)ldp(rhd, ie), rn_m = (double)ldp(rhd, jn);
+#pragma unroll 4
+    for (int k = 2; k <= d.jpk - 1; k++) {
+        const T *r = plane_ptr(rhd, plane, k);
+        const double rc = (double)ldp(r, ij), re = (double)ldp(r, ie), rn = (double)ldp(r, jn);
+        const double zcoef1 = zcoef0 * E3(m.e3w, ij, k);
+        zhpi = zhpi + zcoef1 * ((re + re_m) - (rc + rc_m)) * r1u;
+        zhpj = zhpj + zcoef1 * ((rn + rn_m) - (rc + rc_m)) * r1v;
+        T *uk = plane_ptr(ua, plane, k), *vk = plane_ptr(va, plane, k);
+        stp(uk, ij, (T)((double)ldp((const T *)uk, ij) + zhpi));
+        stp(vk, ij, (T)((double)ldp((const T *)vk, ij) + zhpj));
+        rc_m = rc;
+        re_m = re;
+        rn_m = rn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// dyn_ldf_lap, DYN/dynldf_lap_blp.F90:72-100: per interior cell, the three
+// f-point curls and three t-point divergences the cell's two trends use,
+// each evaluated as the reference evaluates zcur / zdiv
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ double ldf_cur(const DevDomain &d, const T *pub, const T *pvb, const T *ahmf, int p, int q,
+                                          int k)
+{
+    const Metrics<double> &m = d.md;
+    return (double)A3(ahmf, p, q, k) * E3(m.e3f, IJ32(p, q), k) * A2(m.r1_e1e2f, p, q) *
+           (A2(m.e2v, p + 1, q) * (double)A3(pvb, p + 1, q, k) - A2(m.e2v, p, q) * (double)A3(pvb, p, q, k) -
+            A2(m.e1u, p, q + 1) * (double)A3(pub, p, q + 1, k) + A2(m.e1u, p, q) * (double)A3(pub, p, q, k));
+}
+template <typename T>
+__device__ __forceinline__ double ldf_div(const DevDomain &d, const T *pub, const T *pvb, const T *ahmt, int p, int q,
+                                          int k)
+{
+    const Metrics<double> &m = d.md;
+    return (double)A3(ahmt, p, q, k) * A2(m.r1_e1e2t, p, q) / E3(m.e3t, IJ32(p, q), k) *
+           (A2(m.e2u, p, q) * E3(m.e3u, IJ32(p, q), k) * (double)A3(pub, p, q, k) -
+            A2(m.e2u, p - 1, q) * E3(m.e3u, IJ32(p - 1, q), k) * (double)A3(pub, p - 1, q, k) +
+            A2(m.e1v, p, q) * E3(m.e3v, IJ32(p, q), k) * (double)A3(pvb, p, q, k) -
+            A2(m.e1v, p, q - 1) * E3(m.e3v, IJ32(p, q - 1), k) * (double)A3(pvb, p, q - 1, k));
+}
+
+template <typename T>
+KERNEL void k_dyn_ldf_lap(DevDomain d, double zsign, const T *__restrict__ pub, const T *__restrict__ pvb,
+                          T *__restrict__ pua, T *__restrict__ pva, const T *__restrict__ ahmt,
+                          const T *__restrict__ ahmf)
+{
+    THREAD_IJ(2, 2);
+    const int k = (int)blockIdx.z + 1;
+    if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    const Metrics<double> &m = d.md;
+    const double cur = ldf_cur(d, pub, pvb, ahmf, i, j, k);
+    const double cur_s = ldf_cur(d, pub, pvb, ahmf, i, j - 1, k);
+    const double cur_w = ldf_cur(d, pub, pvb, ahmf, i - 1, j, k);
+    const double div = ldf_div(d, pub, pvb, ahmt, i, j, k);
+    const double div_e = ldf_div(d, pub, pvb, ahmt, i + 1, j, k);
+    const double div_n = ldf_div(d, pub, pvb, ahmt, i, j + 1, k);
+    const uint32_t ij = IJ32(i, j);
+    A3(pua, i, j, k) = (T)((double)A3(pua, i, j, k) +
+                           zsign * (-((cur - cur_s) * m.r1_e2u[ij] / E3(m.e3u, ij, k)) + (div_e - div) * m.r1_e1u[ij]));
+    A3(pva, i, j, k) = (T)((double)A3(pva, i, j, k) +
+                           zsign * ((cur - cur_w) * m.r1_e1v[ij] / E3(m.e3v, ij, k) + (div_n - div) * m.r1_e2v[ij]));
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename T>
+int launch_eos(const SeosPar &e, int pot, const T *pts, T *prd, T *prhop, const T *pdep)
+{
+    const DevDomain &d = nh::dom();
+    Grid2 g = grid2(d.jpi, d.jpj);
+    g.grid.z = (unsigned)(d.jpk - 1);
+    if (pot) LAUNCH("eos", (k_eos<T, 1>), g, d, e, pts, prd, prhop, pdep);
+    else LAUNCH("eos", (k_eos<T, 0>), g, d, e, pts, prd, prhop, pdep);
+    return 0;
+}
+
+template <typename T>
+int launch_rab(const SeosPar &e, const T *pts, T *pab, const T *gdept)
+{
+    const DevDomain &d = nh::dom();
+    Grid2 g = grid2(d.jpi, d.jpj);
+    g.grid.z = (unsigned)(d.jpk - 1);
+    LAUNCH("rab", k_rab<T>, g, d, e, pts, pab, gdept);
+    return 0;
+}
+
+template <typename T>
+int launch_bn2(const T *pts, const T *pab, T *pn2, const T *gdept, const T *gdepw)
+{
+    const DevDomain &d = nh::dom();
+    if (d.jpk < 3) return 0;
+    Grid2 g = grid2(d.jpi, d.jpj);
+    g.grid.z = (unsigned)(d.jpk - 2);
+    LAUNCH("bn2", k_bn2<T>, g, d, pts, pab, pn2, gdept, gdepw);
+    return 0;
+}
+
+template <typename T>
+int launch_hpg_zco(const T *rhd, T *ua, T *va)
+{
+    const DevDomain &d = nh::dom();
+    Grid2 g = grid2(d.jpi - 2, d.jpj - 2);
+    LAUNCH("dyn_hpg", k_hpg_zco<T>, g, d, rhd, ua, va);
+    return 0;
+}
+
+template <typename T>
+int launch_dyn_ldf_lap(double zsign, const T *pub, const T *pvb, T *pua, T *pva, const T *ahmt, const T *ahmf)
+{
+    const DevDomain &d = nh::dom();
+    Grid2 g = grid2(d.jpi - 2, d.jpj - 2);
+    g.grid.z = (unsigned)(d.jpk - 1);
+    LAUNCH("dyn_ldf", k_dyn_ldf_lap<T>, g, d, zsign, pub, pvb, pua, pva, ahmt, ahmf);
+    return 0;
+}
+
+#define INST(T)                                                                                      \
+    template int launch_eos<T>(const SeosPar &, int, const T *, T *, T *, const T *);                 \
+    template int launch_rab<T>(const SeosPar &, const T *, T *, const T *);                           \
+    template int launch_bn2<T>(const T *, const T *, T *, const T *, const T *);                      \
+    template int launch_hpg_zco<T>(const T *, T *, T *);                                              \
+    template int launch_dyn_ldf_lap<T>(double, const T *, const T *, T *, T *, const T *, const T *);
+INST(double)
+INST(float)
+#undef INST

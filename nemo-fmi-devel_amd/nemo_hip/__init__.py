@@ -335,6 +335,44 @@ class NemoHip:
                        p(akz), p(avt)))
 
 
+    # density, pressure gradient, lateral viscosity -----------------------------
+    def eos_register(self, eos=None):
+        """eos: abi.EosStruct (default: the simplified EOS with its defaults)."""
+        self._eos = eos if eos is not None else abi.seos()
+        self._check(self.L.nemo_hip_eos_register(C.byref(self._eos)))
+
+    def eos_insitu(self, pts, prd, pdep, sp=False):
+        fn = self.L.nemo_hip_eos_insitu_sp if sp else self.L.nemo_hip_eos_insitu
+        dt = np.float32 if sp else np.float64
+        self._check(fn(self._p(pts, dt), self._p(prd, dt), self._p(pdep, dt)))
+
+    def eos_insitu_pot(self, pts, prd, prhop, pdep, sp=False):
+        fn = self.L.nemo_hip_eos_insitu_pot_sp if sp else self.L.nemo_hip_eos_insitu_pot
+        dt = np.float32 if sp else np.float64
+        self._check(fn(self._p(pts, dt), self._p(prd, dt), self._p(prhop, dt), self._p(pdep, dt)))
+
+    def rab_3d(self, pts, pab, gdept, sp=False):
+        fn = self.L.nemo_hip_rab_3d_sp if sp else self.L.nemo_hip_rab_3d
+        dt = np.float32 if sp else np.float64
+        self._check(fn(self._p(pts, dt), self._p(pab, dt), self._p(gdept, dt)))
+
+    def bn2(self, pts, pab, pn2, gdept, gdepw, sp=False):
+        fn = self.L.nemo_hip_bn2_sp if sp else self.L.nemo_hip_bn2
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        self._check(fn(p(pts), p(pab), p(pn2), p(gdept), p(gdepw)))
+
+    def hpg_zco(self, kt, rhd, ua, va, sp=False):
+        fn = self.L.nemo_hip_hpg_zco_sp if sp else self.L.nemo_hip_hpg_zco
+        dt = np.float32 if sp else np.float64
+        self._check(fn(C.c_int(kt), self._p(rhd, dt), self._p(ua, dt), self._p(va, dt)))
+
+    def dyn_ldf_lap(self, kt, pub, pvb, pua, pva, kpass, ahmt, ahmf, sp=False):
+        fn = self.L.nemo_hip_dyn_ldf_lap_sp if sp else self.L.nemo_hip_dyn_ldf_lap
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        self._check(fn(C.c_int(kt), p(pub), p(pvb), p(pua), p(pva), C.c_int(kpass), p(ahmt), p(ahmf)))
+
     # time-step closure ------------------------------------------------------
     def div_hor(self, kt, un, vn, hdivn, sp=False):
         fn = self.L.nemo_hip_div_hor_sp if sp else self.L.nemo_hip_div_hor

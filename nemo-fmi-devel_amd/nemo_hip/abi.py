@@ -63,3 +63,25 @@ SPG_STATE_FIELDS = ["ub2_b", "vb2_b", "un_bf", "vn_bf"]
 class SpgTsStruct(C.Structure):
     _fields_ = ([(n, C.c_void_p) for n in SPG_3D_FIELDS]
                 + [(n, C.c_void_p) for n in SPG_IN_FIELDS + SPG_OUT_FIELDS + SPG_STATE_FIELDS])
+
+
+# equation of state, TRA/eosbn2.F90:84-95 (nameos defaults of the S-EOS)
+NP_TEOS10, NP_EOS80, NP_SEOS = -1, 0, 1
+EOS_REAL_FIELDS = ["rn_a0", "rn_b0", "rn_lambda1", "rn_lambda2", "rn_mu1", "rn_mu2", "rn_nu"]
+SEOS_DEFAULTS = dict(rn_a0=1.6550e-1, rn_b0=7.6554e-1, rn_lambda1=5.9520e-2, rn_lambda2=5.4914e-4,
+                     rn_mu1=1.4970e-4, rn_mu2=1.1090e-5, rn_nu=2.4341e-3)
+
+
+class EosStruct(C.Structure):
+    _fields_ = [("neos", C.c_int32), ("reserved0", C.c_int32)] + [(n, C.c_double) for n in EOS_REAL_FIELDS]
+
+
+def seos(**kw):
+    """An EosStruct for the simplified EOS (defaults of TRA/eosbn2.F90:89-95)."""
+    e = EosStruct()
+    e.neos = NP_SEOS
+    v = dict(SEOS_DEFAULTS)
+    v.update(kw)
+    for n in EOS_REAL_FIELDS:
+        setattr(e, n, float(v[n]))
+    return e

@@ -220,6 +220,17 @@ def basin(jpi, jpj, jpk, seed=0, land_frac=0.3, depth_max=5000.0, depth_min=50.0
 # ----------------------------------------------------------------------
 # prognostic fields
 # ----------------------------------------------------------------------
+def depths(d):
+    """t- and w-point depths (jpi,jpj,jpk) consistent with the domain's
+    vertical scale factors: gdepw(1) = 0, gdepw(k) = gdepw(k-1) + e3t(k-1);
+    gdept(1) = e3w(1), gdept(k) = gdept(k-1) + e3w(k)."""
+    e3t, e3w = d.arrays["e3t_n"], d.arrays["e3w_n"]
+    gdepw = np.zeros_like(e3t)
+    gdepw[:, :, 1:] = np.cumsum(e3t[:, :, :-1], axis=2)
+    gdept = np.cumsum(e3w, axis=2)
+    return np.asfortranarray(gdept), np.asfortranarray(gdepw)
+
+
 def masks(d):
     """(tmask, umask, vmask, wmask, fmask) the way DOM/dommsk.F90 builds them (used only
     to shape synthetic fields; the kernels derive their own)."""

@@ -1959,12 +1959,175 @@ int ora_dyn_nxt(int kt, double *ub, double *vb, double *un, double *vn, double *
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* equation of state (simplified EOS), hpg_zco, dyn_ldf_lap                 */
+/* ------------------------------------------------------------------------ */
+static nemo_hip_eos_t EOS;
+
+/* eos_init, TRA/eosbn2.F90:1218-1700 (the S-EOS branch only) */
+int ora_eos_register(const nemo_hip_eos_t *e)
+{
+    if (!e || e->neos != NEMO_NP_SEOS) return 3;
+    EOS = *e;
+    return 0;
+}
+
+/* eos_insitu, TRA/eosbn2.F90:276-299 (np_seos) */
+int ora_eos_insitu(const double *pts, double *prd, const double *pdep)
+{
+    int i, j, k;
+    const double r1_rau0 = 1.0 / D.rau0;
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                const double zt = A4(pts, i, j, k, 1) - 10.;
+                const double zs = A4(pts, i, j, k, 2) - 35.;
+                const double zh = A3(pdep, i, j, k);
+                const double ztm = TMASK(i, j, k);
+                const double zn = -EOS.rn_a0 * (1. + 0.5 * EOS.rn_lambda1 * zt + EOS.rn_mu1 * zh) * zt +
+                                  EOS.rn_b0 * (1. - 0.5 * EOS.rn_lambda2 * zs - EOS.rn_mu2 * zh) * zs -
+                                  EOS.rn_nu * zt * zs;
+                A3(prd, i, j, k) = zn * r1_rau0 * ztm;
+            }
+    return 0;
+}
+
+/* eos_insitu_pot, TRA/eosbn2.F90:440-462 (np_seos) */
+int ora_eos_insitu_pot(const double *pts, double *prd, double *prhop, const double *pdep)
+{
+    int i, j, k;
+    const double r1_rau0 = 1.0 / D.rau0;
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                const double zt = A4(pts, i, j, k, 1) - 10.;
+                const double zs = A4(pts, i, j, k, 2) - 35.;
+                const double zh = A3(pdep, i, j, k);
+                const double ztm = TMASK(i, j, k);
+                double zn = -EOS.rn_a0 * (1. + 0.5 * EOS.rn_lambda1 * zt) * zt +
+                            EOS.rn_b0 * (1. - 0.5 * EOS.rn_lambda2 * zs) * zs - EOS.rn_nu * zt * zs;
+                A3(prhop, i, j, k) = (D.rau0 + zn) * ztm;
+                zn = zn - (EOS.rn_a0 * EOS.rn_mu1 * zt + EOS.rn_b0 * EOS.rn_mu2 * zs) * zh;
+                A3(prd, i, j, k) = zn * r1_rau0 * ztm;
+            }
+    return 0;
+}
+
+/* rab_3d, TRA/eosbn2.F90:645-662 (np_seos) */
+int ora_rab_3d(const double *pts, double *pab, const double *gdept)
+{
+    int i, j, k;
+    const double r1_rau0 = 1.0 / D.rau0;
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                const double zt = A4(pts, i, j, k, 1) - 10.;
+                const double zs = A4(pts, i, j, k, 2) - 35.;
+                const double zh = A3(gdept, i, j, k);
+                const double ztm = TMASK(i, j, k);
+                double zn = EOS.rn_a0 * (1. + EOS.rn_lambda1 * zt + EOS.rn_mu1 * zh) + EOS.rn_nu * zs;
+                A4(pab, i, j, k, 1) = zn * r1_rau0 * ztm;
+                zn = EOS.rn_b0 * (1. - EOS.rn_lambda2 * zs - EOS.rn_mu2 * zh) - EOS.rn_nu * zt;
+                A4(pab, i, j, k, 2) = zn * r1_rau0 * ztm;
+            }
+    return 0;
+}
+
+/* bn2, TRA/eosbn2.F90:889-933 */
+int ora_bn2(const double *pts, const double *pab, double *pn2, const double *gdept, const double *gdepw)
+{
+    int i, j, k;
+    for (k = 2; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                const double zrw = (A3(gdepw, i, j, k) - A3(gdept, i, j, k)) / (A3(gdept, i, j, k - 1) - A3(gdept, i, j, k));
+                const double zaw = A4(pab, i, j, k, 1) * (1. - zrw) + A4(pab, i, j, k - 1, 1) * zrw;
+                const double zbw = A4(pab, i, j, k, 2) * (1. - zrw) + A4(pab, i, j, k - 1, 2) * zrw;
+                A3(pn2, i, j, k) = D.grav *
+                                   (zaw * (A4(pts, i, j, k - 1, 1) - A4(pts, i, j, k, 1)) -
+                                    zbw * (A4(pts, i, j, k - 1, 2) - A4(pts, i, j, k, 2))) /
+                                   e3w_n(i, j, k) * WMASK(i, j, k);
+            }
+    return 0;
+}
+
+/* hpg_zco, DYN/dynhpg.F90:258-323 */
+int ora_hpg_zco(int kt, const double *rhd, double *ua, double *va)
+{
+    int i, j, k;
+    (void)kt;
+    const double zcoef0 = -D.grav * 0.5;
+    double *zhpi = z3(), *zhpj = z3();
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            const double zcoef1 = zcoef0 * e3w_n(i, j, 1);
+            A3(zhpi, i, j, 1) = zcoef1 * (A3(rhd, i + 1, j, 1) - A3(rhd, i, j, 1)) * r1_e1u(i, j);
+            A3(zhpj, i, j, 1) = zcoef1 * (A3(rhd, i, j + 1, 1) - A3(rhd, i, j, 1)) * r1_e2v(i, j);
+            A3(ua, i, j, 1) = A3(ua, i, j, 1) + A3(zhpi, i, j, 1);
+            A3(va, i, j, 1) = A3(va, i, j, 1) + A3(zhpj, i, j, 1);
+        }
+    for (k = 2; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                const double zcoef1 = zcoef0 * e3w_n(i, j, k);
+                A3(zhpi, i, j, k) = A3(zhpi, i, j, k - 1) +
+                                    zcoef1 * ((A3(rhd, i + 1, j, k) + A3(rhd, i + 1, j, k - 1)) -
+                                              (A3(rhd, i, j, k) + A3(rhd, i, j, k - 1))) * r1_e1u(i, j);
+                A3(zhpj, i, j, k) = A3(zhpj, i, j, k - 1) +
+                                    zcoef1 * ((A3(rhd, i, j + 1, k) + A3(rhd, i, j + 1, k - 1)) -
+                                              (A3(rhd, i, j, k) + A3(rhd, i, j, k - 1))) * r1_e2v(i, j);
+                A3(ua, i, j, k) = A3(ua, i, j, k) + A3(zhpi, i, j, k);
+                A3(va, i, j, k) = A3(va, i, j, k) + A3(zhpj, i, j, k);
+            }
+    free(zhpi);
+    free(zhpj);
+    return 0;
+}
+
+/* dyn_ldf_lap, DYN/dynldf_lap_blp.F90:37-104 (before and now scale factors
+   are equal: linear free surface) */
+int ora_dyn_ldf_lap(int kt, const double *pub, const double *pvb, double *pua, double *pva, int kpass,
+                    const double *ahmt, const double *ahmf)
+{
+    int i, j, k;
+    (void)kt;
+    const double zsign = kpass == 1 ? 1. : -1.;
+    double *zcur = z2(), *zdiv = z2();
+    for (k = 1; k <= jpkm1; k++) {
+        for (j = 2; j <= jpj; j++)
+            for (i = 2; i <= jpi; i++) {
+                A2(zcur, i - 1, j - 1) =
+                    A3(ahmf, i - 1, j - 1, k) * e3f_n(i - 1, j - 1, k) * r1_e1e2f(i - 1, j - 1) *
+                    (e2v(i, j - 1) * A3(pvb, i, j - 1, k) - e2v(i - 1, j - 1) * A3(pvb, i - 1, j - 1, k) -
+                     e1u(i - 1, j) * A3(pub, i - 1, j, k) + e1u(i - 1, j - 1) * A3(pub, i - 1, j - 1, k));
+                A2(zdiv, i, j) = A3(ahmt, i, j, k) * r1_e1e2t(i, j) / e3t_n(i, j, k) *
+                                 (e2u(i, j) * e3u_n(i, j, k) * A3(pub, i, j, k) -
+                                  e2u(i - 1, j) * e3u_n(i - 1, j, k) * A3(pub, i - 1, j, k) +
+                                  e1v(i, j) * e3v_n(i, j, k) * A3(pvb, i, j, k) -
+                                  e1v(i, j - 1) * e3v_n(i, j - 1, k) * A3(pvb, i, j - 1, k));
+            }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A3(pua, i, j, k) = A3(pua, i, j, k) +
+                                   zsign * (-((A2(zcur, i, j) - A2(zcur, i, j - 1)) * r1_e2u(i, j) / e3u_n(i, j, k)) +
+                                            (A2(zdiv, i + 1, j) - A2(zdiv, i, j)) * r1_e1u(i, j));
+                A3(pva, i, j, k) = A3(pva, i, j, k) +
+                                   zsign * ((A2(zcur, i, j) - A2(zcur, i - 1, j)) * r1_e1v(i, j) / e3v_n(i, j, k) +
+                                            (A2(zdiv, i, j + 1) - A2(zdiv, i, j)) * r1_e2v(i, j));
+            }
+    }
+    free(zcur);
+    free(zdiv);
+    return 0;
+}
+
 int ora_abi_layout(size_t *out, int n)
 {
     size_t v[] = {sizeof(nemo_hip_domain_t), offsetof(nemo_hip_domain_t, rdt), offsetof(nemo_hip_domain_t, k_top),
                   offsetof(nemo_hip_domain_t, e1t), offsetof(nemo_hip_domain_t, e3t_n),
                   offsetof(nemo_hip_domain_t, e3vw_n), sizeof(nemo_hip_spg_ts_t),
-                  offsetof(nemo_hip_spg_ts_t, un_b), offsetof(nemo_hip_spg_ts_t, vn_bf)};
+                  offsetof(nemo_hip_spg_ts_t, un_b), offsetof(nemo_hip_spg_ts_t, vn_bf), sizeof(nemo_hip_eos_t),
+                  offsetof(nemo_hip_eos_t, rn_a0), offsetof(nemo_hip_eos_t, rn_nu)};
     int m = (int)(sizeof v / sizeof v[0]);
     for (int q = 0; q < n && q < m; q++) out[q] = v[q];
     return m;

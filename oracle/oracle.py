@@ -147,6 +147,30 @@ class Oracle:
             setattr(s, n, None if a is None else a.ctypes.data)
         _chk(self.L.ora_dyn_spg_ts(C.c_int(kt), C.byref(s)), "dyn_spg_ts")
 
+    # equation of state, pressure gradient, lateral viscosity ---------------
+    def eos_register(self, eos):
+        """eos: abi.EosStruct"""
+        _chk(self.L.ora_eos_register(C.byref(eos)), "eos_register")
+
+    def eos_insitu(self, pts, prd, pdep):
+        _chk(self.L.ora_eos_insitu(_p(pts), _p(prd), _p(pdep)), "eos_insitu")
+
+    def eos_insitu_pot(self, pts, prd, prhop, pdep):
+        _chk(self.L.ora_eos_insitu_pot(_p(pts), _p(prd), _p(prhop), _p(pdep)), "eos_insitu_pot")
+
+    def rab_3d(self, pts, pab, gdept):
+        _chk(self.L.ora_rab_3d(_p(pts), _p(pab), _p(gdept)), "rab_3d")
+
+    def bn2(self, pts, pab, pn2, gdept, gdepw):
+        _chk(self.L.ora_bn2(_p(pts), _p(pab), _p(pn2), _p(gdept), _p(gdepw)), "bn2")
+
+    def hpg_zco(self, kt, rhd, ua, va):
+        _chk(self.L.ora_hpg_zco(C.c_int(kt), _p(rhd), _p(ua), _p(va)), "hpg_zco")
+
+    def dyn_ldf_lap(self, kt, pub, pvb, pua, pva, kpass, ahmt, ahmf):
+        _chk(self.L.ora_dyn_ldf_lap(C.c_int(kt), _p(pub), _p(pvb), _p(pua), _p(pva), C.c_int(kpass), _p(ahmt),
+                                    _p(ahmf)), "dyn_ldf_lap")
+
     # time-step closure ----------------------------------------------------
     def div_hor(self, kt, un, vn, hdivn):
         _chk(self.L.ora_div_hor(C.c_int(kt), _p(un), _p(vn), _p(hdivn)), "div_hor")

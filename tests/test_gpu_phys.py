@@ -1,0 +1,173 @@
+"""Parity of the density (simplified EOS), pressure-gradient (hpg_zco) and
+lateral-viscosity (dyn_ldf_lap) kernels with the oracle.
+
+FP64: <= 1e-12 max relative error, bit-identical expected.  FP32 storage
+form: the kernels compute in FP64 from the float inputs and round once, so
+against the oracle on the same float-rounded inputs the bound is 1e-6.
+"""
+import numpy as np
+import pytest
+
+import nemo_hip
+from nemo_hip import NEMO_HIP_PTR_DEVICE, NEMO_HIP_PTR_HOST, abi, configs
+from test_gpu_parity import CASE_IDS, Env, check, dev, host, sync
+
+torch = pytest.importorskip("torch")
+
+SP_TOL = 1e-6
+
+
+def rnd(rng, shape, sp, lo=None, hi=None):
+    x = rng.standard_normal(shape) if lo is None else rng.uniform(lo, hi, shape)
+    if sp:
+        x = x.astype(np.float32).astype(np.float64)
+    return np.asfortranarray(x)
+
+
+def tracers(rng, e, sp):
+    ts = np.zeros(e.s4, order="F")
+    ts[..., 0] = rng.uniform(-2, 30, e.s3)
+    ts[..., 1] = rng.uniform(30, 40, e.s3)
+    if sp:
+        ts = np.asfortranarray(ts.astype(np.float32).astype(np.float64))
+    return ts
+
+
+def g(a, sp):
+    return dev(a, torch.float32 if sp else torch.float64)
+
+
+def cmp(got, want, sp, what):
+    check(got, want, tol=SP_TOL if sp else 1e-12, what=what)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASE_IDS)
+@pytest.mark.parametrize("sp", [False, True])
+def test_eos(hip, case, sp):
+    e = Env(hip, case)
+    hip.eos_register(abi.seos())
+    e.o.eos_register(abi.seos())
+    rng = np.random.default_rng(71)
+    ts = tracers(rng, e, sp)
+    gdept, gdepw = configs.depths(e.d)
+    if sp:
+        gdept = np.asfortranarray(gdept.astype(np.float32).astype(np.float64))
+        gdepw = np.asfortranarray(gdepw.astype(np.float32).astype(np.float64))
+    # in situ density
+    prd = rnd(rng, e.s3, sp)
+    want = prd.copy(order="F")
+    e.o.eos_insitu(ts, want, gdept)
+    gp = g(prd, sp)
+    hip.eos_insitu(g(ts, sp), gp, g(gdept, sp), sp=sp)
+    # in situ and potential density
+    want2, wantp = prd.copy(order="F"), prd.copy(order="F")
+    e.o.eos_insitu_pot(ts, want2, wantp, gdept)
+    gp2, gpp = g(prd, sp), g(prd, sp)
+    hip.eos_insitu_pot(g(ts, sp), gp2, gpp, g(gdept, sp), sp=sp)
+    # expansion ratios and N^2
+    pab = rnd(rng, e.s4, sp)
+    wab = pab.copy(order="F")
+    e.o.rab_3d(ts, wab, gdept)
+    gab = g(pab, sp)
+    hip.rab_3d(g(ts, sp), gab, g(gdept, sp), sp=sp)
+    pn2 = rnd(rng, e.s3, sp)
+    wn2 = pn2.copy(order="F")
+    e.o.bn2(ts, wab, wn2, gdept, gdepw)
+    gn2 = g(pn2, sp)
+    hip.bn2(g(ts, sp), gab, gn2, g(gdept, sp), g(gdepw, sp), sp=sp)
+    sync(hip)
+    cmp(host(gp, e.s3), want, sp, f"eos_insitu {case}")
+    cmp(host(gp2, e.s3), want2, sp, f"eos_insitu_pot prd {case}")
+    cmp(host(gpp, e.s3), wantp, sp, f"eos_insitu_pot prhop {case}")
+    cmp(host(gab, e.s4), wab, sp, f"rab_3d {case}")
+    if not sp:   # N^2 of FP32 ratios differs from the oracle's FP64 ratios by their rounding
+        cmp(host(gn2, e.s3), wn2, sp, f"bn2 {case}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["basin", "cyc7"])
+def test_bn2_fp32_against_same_ratios(hip, case):
+    """bn2 in FP32 storage on ratios both sides read as the same floats."""
+    e = Env(hip, case)
+    rng = np.random.default_rng(72)
+    ts = tracers(rng, e, True)
+    pab = rnd(rng, e.s4, True, 1e-5, 1e-3)
+    gdept, gdepw = configs.depths(e.d)
+    gdept = np.asfortranarray(gdept.astype(np.float32).astype(np.float64))
+    gdepw = np.asfortranarray(gdepw.astype(np.float32).astype(np.float64))
+    want = np.zeros(e.s3, order="F")
+    e.o.bn2(ts, pab, want, gdept, gdepw)
+    gn2 = g(np.zeros(e.s3, order="F"), True)
+    hip.bn2(g(ts, True), g(pab, True), gn2, g(gdept, True), g(gdepw, True), sp=True)
+    sync(hip)
+    cmp(host(gn2, e.s3), want, True, f"bn2 fp32 {case}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASE_IDS)
+@pytest.mark.parametrize("sp", [False, True])
+def test_hpg_zco(hip, case, sp):
+    e = Env(hip, case)
+    rng = np.random.default_rng(73)
+    rhd = rnd(rng, e.s3, sp)
+    ua, va = rnd(rng, e.s3, sp), rnd(rng, e.s3, sp)
+    wu, wv = ua.copy(order="F"), va.copy(order="F")
+    e.o.hpg_zco(3, rhd, wu, wv)
+    gu, gv = g(ua, sp), g(va, sp)
+    hip.hpg_zco(3, g(rhd, sp), gu, gv, sp=sp)
+    sync(hip)
+    cmp(host(gu, e.s3), wu, sp, f"hpg_zco ua {case}")
+    cmp(host(gv, e.s3), wv, sp, f"hpg_zco va {case}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASE_IDS)
+@pytest.mark.parametrize("kpass", [1, 2])
+@pytest.mark.parametrize("sp", [False, True])
+def test_dyn_ldf_lap(hip, case, kpass, sp):
+    e = Env(hip, case)
+    rng = np.random.default_rng(74 + kpass)
+    t, u, v, w, f = configs.masks(e.d)
+    ub, vb = rnd(rng, e.s3, sp), rnd(rng, e.s3, sp)
+    ahmt = np.asfortranarray(rnd(rng, e.s3, sp, 500, 2000) * t)
+    ahmf = np.asfortranarray(rnd(rng, e.s3, sp, 500, 2000) * f)
+    ua, va = rnd(rng, e.s3, sp), rnd(rng, e.s3, sp)
+    wu, wv = ua.copy(order="F"), va.copy(order="F")
+    e.o.dyn_ldf_lap(3, ub, vb, wu, wv, kpass, ahmt, ahmf)
+    gu, gv = g(ua, sp), g(va, sp)
+    hip.dyn_ldf_lap(3, g(ub, sp), g(vb, sp), gu, gv, kpass, g(ahmt, sp), g(ahmf, sp), sp=sp)
+    sync(hip)
+    cmp(host(gu, e.s3), wu, sp, f"dyn_ldf_lap ua {case}")
+    cmp(host(gv, e.s3), wv, sp, f"dyn_ldf_lap va {case}")
+
+
+@pytest.mark.gpu
+def test_phys_host_pointer_mode_and_errors(hip):
+    e = Env(hip, "cyc1")
+    hip.eos_register(abi.seos())
+    e.o.eos_register(abi.seos())
+    rng = np.random.default_rng(75)
+    ts = tracers(rng, e, False)
+    gdept, _ = configs.depths(e.d)
+    try:
+        hip.set_pointer_mode(NEMO_HIP_PTR_HOST)
+        a, b = np.zeros(e.s3, order="F"), np.zeros(e.s3, order="F")
+        e.o.eos_insitu(ts, a, gdept)
+        hip.eos_insitu(ts, b, gdept)
+        check(b, a, what="host eos_insitu")
+        ua, va = rnd(rng, e.s3, False), rnd(rng, e.s3, False)
+        wu, wv = ua.copy(order="F"), va.copy(order="F")
+        e.o.hpg_zco(2, a, wu, wv)
+        hip.hpg_zco(2, b, ua, va)
+        check(ua, wu, what="host hpg ua")
+        check(va, wv, what="host hpg va")
+    finally:
+        hip.set_pointer_mode(NEMO_HIP_PTR_DEVICE)
+    with pytest.raises(nemo_hip.NemoHipError, match="neos"):
+        bad = abi.seos()
+        bad.neos = abi.NP_TEOS10
+        hip.eos_register(bad)
+    x3 = dev(np.zeros(e.s3))
+    with pytest.raises(nemo_hip.NemoHipError, match="kpass"):
+        hip.dyn_ldf_lap(1, x3, x3, x3, x3, 3, x3, x3)

@@ -42,6 +42,23 @@ def test_trajectory_fp64(hip, case):
     print(json.dumps({"case": case, "bit_identical": ident}))
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["gyre", "cyc7"])
+def test_trajectory_with_density_and_viscosity_fp64(hip, case):
+    """The step with the simplified EOS, hpg_zco and dyn_ldf_lap added."""
+    d = CASES[case]()
+    S = configs.make_state(d, seed=19)
+    want = oracle_run(d, S, MARKS, physics=True)
+    hip.register(d)
+    got = gpu_run(hip, d, S, MARKS, physics=True)
+    ident = True
+    for kt in MARKS:
+        for k in OUT_3D + OUT_4D + OUT_2D:
+            check(got[kt][k], want[kt][k], what=f"{case} physics step {kt} {k}")
+            ident &= bool(np.array_equal(got[kt][k], want[kt][k]))
+    print(json.dumps({"case": case, "physics": True, "bit_identical": ident}))
+
+
 # max relative error after 8 steps, FP32 storage against the FP64 oracle
 # (measured on MI355X: tracers 4.4e-7, velocities 2.2e-7, w 4.8e-7, sea level
 # 2.6e-7, barotropic velocities 1.3e-7, un_bf 1.1e-6)

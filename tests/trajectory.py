@@ -11,10 +11,16 @@ equation of state, pressure gradient, lateral viscosity or surface module
     tra_ldf_iso, tra_zdf              after tracers
     tra_nxt, dyn_nxt, ssh_swp         Asselin filters and swaps
 
+With physics=True the step also has a density and its pressure gradient
+and a lateral viscosity, step.F90:165 and 191-193: eos_insitu_pot of the now
+tracers (simplified EOS) after wzv, then dyn_ldf_lap of the before velocities
+(ahm = AHM m2/s) and hpg_zco between dyn_vor and dyn_spg_ts.
+
 oracle_run repeats it with the oracle's separate routines; gpu_run with the
-library on device-resident fields (the fused step_dyn / step_tra entry points
-plus the closure routines), in FP64 or the FP32 storage form.  Both return the
-prognostic state after the steps listed in `marks`.
+library on device-resident fields (the fused step_dyn / step_tra entry points,
+or with physics the separate momentum routines, plus the closure routines),
+in FP64 or the FP32 storage form.  Both return the prognostic state after the
+steps listed in `marks`.
 """
 import numpy as np
 
@@ -25,6 +31,7 @@ OUT_3D = ["un", "vn", "ub", "vb", "wn"]
 OUT_4D = ["tsn", "tsb"]
 OUT_2D = ["sshn", "sshb", "un_b", "vn_b", "ub_b", "vb_b", "ub2_b", "un_bf"]
 KN_FCT = (4, 4)
+AHM = 2000.0
 
 
 def r2dt(d, kt):
@@ -41,9 +48,21 @@ def initial(S):
     return I
 
 
-def oracle_run(d, S, marks):
+def viscosity(d):
+    """ahmt, ahmf of a constant lateral viscosity AHM, masked by tmask / fmask."""
+    from nemo_hip import configs
+    t, u, v, w, f = configs.masks(d)
+    return np.asfortranarray(AHM * t), np.asfortranarray(AHM * f)
+
+
+def oracle_run(d, S, marks, physics=False):
     from oracle.oracle import Oracle
+    from nemo_hip import abi, configs
     o = Oracle(d)
+    if physics:
+        o.eos_register(abi.seos())
+        gdept, _ = configs.depths(d)
+        ahmt, ahmf = viscosity(d)
     s2, s3, s4 = (d.jpi, d.jpj), (d.jpi, d.jpj, d.jpk), (d.jpi, d.jpj, d.jpk, 2)
     X = {k: np.array(v, order="F", copy=True) for k, v in initial(S).items()}
     for k in ("ua_b", "va_b", "ssha", "un_adv", "vn_adv"):
@@ -54,9 +73,15 @@ def oracle_run(d, S, marks):
     for kt in range(1, max(marks) + 1):
         o.ssh_nxt(kt, X["un"], X["vn"], hdivn, X["sshb"], X["emp_b"], X["emp"], X["ssha"])
         o.wzv(kt, hdivn, X["wn"])
+        if physics:
+            rhd, rhop = np.zeros(s3, order="F"), np.zeros(s3, order="F")
+            o.eos_insitu_pot(X["tsn"], rhd, rhop, gdept)
         ua, va = np.zeros(s3, order="F"), np.zeros(s3, order="F")
         o.dyn_adv(kt, X["un"], X["vn"], X["wn"], ua, va)
         o.dyn_vor(kt, X["un"], X["vn"], ua, va)
+        if physics:
+            o.dyn_ldf_lap(kt, X["ub"], X["vb"], ua, va, 1, ahmt, ahmf)
+            o.hpg_zco(kt, rhd, ua, va)
         F = {k: X[k] for k in SPG_2D + ["ua_b", "va_b", "ssha", "un_adv", "vn_adv"]}
         F.update(ua=ua, va=va, un=X["un"], vn=X["vn"])
         o.dyn_spg_ts(kt, F)
@@ -80,9 +105,10 @@ def oracle_run(d, S, marks):
     return out
 
 
-def gpu_run(h, d, S, marks, sp=False):
+def gpu_run(h, d, S, marks, sp=False, physics=False):
     """h: the library handle in device-pointer mode with d registered."""
     import torch
+    from nemo_hip import abi, configs
     dt = torch.float32 if sp else torch.float64
     s2, s3, s4 = (d.jpi, d.jpj), (d.jpi, d.jpj, d.jpk), (d.jpi, d.jpj, d.jpk, 2)
     n2, n3 = d.jpi * d.jpj, d.jpi * d.jpj * d.jpk
@@ -96,13 +122,34 @@ def gpu_run(h, d, S, marks, sp=False):
     F = {k: X[k] for k in SPG_2D + ["ua_b", "va_b", "ssha", "un_adv", "vn_adv"]}
     F.update(ua=ua, va=va, un=X["un"], vn=X["vn"])
     shape = {k: (s4 if k in OUT_4D else s3 if k in OUT_3D else s2) for k in OUT_3D + OUT_4D + OUT_2D}
+    if physics:
+        h.eos_register(abi.seos())
+        gdept = dev(configs.depths(d)[0], dt)
+        ahmt, ahmf = (dev(a, dt) for a in viscosity(d))
+        rhd, rhop = z(n3), z(n3)
     out = {}
+    # torch work (the trend reset) is queued on the library stream, in order with the library's launches
+    lib_stream = torch.cuda.ExternalStream(h.stream())
     for kt in range(1, max(marks) + 1):
         h.ssh_nxt(kt, X["un"], X["vn"], hdivn, X["sshb"], X["emp_b"], X["emp"], X["ssha"], sp=sp)
         h.wzv(kt, hdivn, X["wn"], sp=sp)
-        h.step_dyn(kt, X["ub"], X["vb"], X["wn"], X["avm"], X["utau_b"], X["vtau_b"], F, sp=sp)
+        if physics:
+            h.eos_insitu_pot(X["tsn"], rhd, rhop, gdept, sp=sp)
+            with torch.cuda.stream(lib_stream):
+                ua.zero_()
+                va.zero_()
+            h.dyn_adv(kt, X["un"], X["vn"], X["wn"], ua, va, sp=sp)
+            h.dyn_vor(kt, X["un"], X["vn"], ua, va, sp=sp)
+            h.dyn_ldf_lap(kt, X["ub"], X["vb"], ua, va, 1, ahmt, ahmf, sp=sp)
+            h.hpg_zco(kt, rhd, ua, va, sp=sp)
+            h.dyn_spg_ts(kt, F, sp=sp)
+        else:
+            h.step_dyn(kt, X["ub"], X["vb"], X["wn"], X["avm"], X["utau_b"], X["vtau_b"], F, sp=sp)
         h.div_hor(kt, X["un"], X["vn"], hdivn, sp=sp)
         h.wzv(kt, hdivn, X["wn"], sp=sp)
+        if physics:
+            h.dyn_zdf(kt, X["ub"], X["vb"], ua, va, X["avm"], X["rCdU_bot"], X["ua_b"], X["va_b"], X["utau"],
+                      X["utau_b"], X["vtau"], X["vtau_b"], sp=sp)
         h.step_tra(kt, X["un"], X["vn"], X["wn"], X["tsb"], X["tsn"], tsa, *KN_FCT, X["ahtu"], X["ahtv"],
                    X["uslp"], X["vslp"], X["wslpi"], X["wslpj"], ah, akz, X["avt"], sp=sp)
         h.tra_nxt(kt, X["tsb"], X["tsn"], tsa, sp=sp)
