@@ -347,6 +347,19 @@ int nemo_hip_hpg_zco(int kt, const double *rhd, double *ua, double *va);
 int nemo_hip_dyn_ldf_lap(int kt, const double *pub, const double *pvb, double *pua, double *pva,
                          int kpass, const double *ahmt, const double *ahmf);
 
+/* stp_ctl, stpctl.F90:55-190: the run control of one time step.  zmax[0..5]
+   = max|sshn|, max|un|, max(-S), max(S), max(-T), max(T) (the tracers over
+   tmask = 1), as lines 113-121 define them; a NaN anywhere in sshn, un or S
+   makes the corresponding maximum NaN.  When one of the tests of lines
+   149-155 fails, *kindic = -3, loc holds the first (Fortran order) positions
+   of max|ssh| (i,j), max|u| (i,j,k), min S and max S (i,j,k), 1-based, and
+   the call returns NEMO_HIP_ERR_NUMERIC with the reference's message;
+   otherwise *kindic is unchanged.  The call waits for the queued work. */
+int nemo_hip_stp_ctl(int kt, const double *sshn, const double *un, const double *tsn,
+                     double zmax[6], int32_t loc[11], int *kindic);
+int nemo_hip_stp_ctl_sp(int kt, const double *sshn, const float *un, const float *tsn,
+                        double zmax[6], int32_t loc[11], int *kindic);
+
 /* ------------------------------------------------------------------------ */
 /* time-step closure around the hot path (linear free surface)              */
 /* ------------------------------------------------------------------------ */

@@ -38,6 +38,7 @@ struct Ctx {
     int spg_rot_s[4] = {0, 1, 2, 3};                    // ... and sea level: [now, before, before-before, after]
     bool have_eos = false;                              // eos_init state (nemo_hip_eos_register)
     SeosPar eos{};
+    double *stp_buf = nullptr;                          // stp_ctl partial maxima and results
 };
 static std::vector<Ctx *> g_ctxs;
 static Ctx *g_c = nullptr;   // the current context
@@ -161,6 +162,7 @@ static void free_domain()
     g_c->dom_allocs.clear();
     for (auto &s : g_c->scratch3) s = nullptr;
     for (auto &s : g_c->scratch2) s = nullptr;
+    g_c->stp_buf = nullptr;
     g_c->have_domain = false;
 }
 
@@ -480,6 +482,7 @@ static int register_domain(const nemo_hip_domain_t *h)
     for (int s = 0; s < NH_NSCRATCH2; s++)
         if ((rc = dev_alloc((void **)&g_c->scratch2[s], n2 * sizeof(double)))) return rc;
     for (int s = 0; s < NH_NSCRATCH2; s++) HIPCHK(hipMemset(g_c->scratch2[s], 0, n2 * sizeof(double)));
+    if ((rc = dev_alloc((void **)&g_c->stp_buf, stp_ctl_buffer_bytes()))) return rc;
     HIPCHK(hipDeviceSynchronize());
     g_c->dom = d;
     g_c->have_domain = true;
@@ -1271,6 +1274,51 @@ static int do_dyn_ldf_lap(int kt, const T *pub, const T *pvb, T *pua, T *pva, in
     return S.finish(rc);
 }
 
+// stp_ctl: maxima on the device, the tests and the message on the host
+template <typename T>
+static int do_stp_ctl(int kt, const double *sshn, const T *un, const T *tsn, double zmax[6], int32_t loc[11],
+                      int *kindic)
+{
+    NEED_DOMAIN();
+    NEED_NOT_CAPTURING("stp_ctl");
+    NEED_PTR(sshn); NEED_PTR(un); NEED_PTR(tsn); NEED_PTR(zmax); NEED_PTR(loc); NEED_PTR(kindic);
+    const DevDomain &d = g_c->dom;
+    Stager S;
+    const double *ss = S.in(sshn, d.n2);
+    const T *u = S.in(un, d.n3), *ts = S.in(tsn, 2 * d.n3);
+    if (S.status) return S.status;
+    int rc = launch_stp_ctl<T>(ss, u, ts, g_c->stp_buf, 0);
+    if (rc) return rc;
+    double *dz = g_c->stp_buf + (size_t)1024 * 6;
+    HIPCHK(hipMemcpyAsync(zmax, dz, 6 * sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    // stpctl.F90:149-155
+    const bool bad = zmax[0] > 20. || zmax[1] > 10. || zmax[2] >= 0. || zmax[3] >= 100. || zmax[3] < 0. ||
+                     std::isnan(zmax[0] + zmax[1] + zmax[2]);
+    if (!bad) return NEMO_HIP_OK;
+    if ((rc = launch_stp_ctl<T>(ss, u, ts, g_c->stp_buf, 1))) return rc;
+    uint32_t q[4];
+    HIPCHK(hipMemcpyAsync(q, (const void *)(dz + 8), sizeof q, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    const size_t n2 = d.n2;
+    auto ijk = [&](uint32_t v, int32_t *o, int nd) {
+        if (v == 0xffffffffu) { for (int c = 0; c < nd; c++) o[c] = 0; return; }
+        o[0] = (int32_t)(v % n2 % (size_t)d.jpi) + 1;
+        o[1] = (int32_t)(v % n2 / (size_t)d.jpi) + 1;
+        if (nd == 3) o[2] = (int32_t)(v / n2) + 1;
+    };
+    ijk(q[0], loc, 2);
+    ijk(q[1], loc + 2, 3);
+    ijk(q[2], loc + 5, 3);
+    ijk(q[3], loc + 8, 3);
+    *kindic = -3;
+    return set_error(NEMO_HIP_ERR_NUMERIC,
+                     "stp_ctl: |ssh| > 20 m  or  |U| > 10 m/s  or  S <= 0  or  S >= 100  or  NaN encounter in the tests; "
+                     "kt=%d |ssh| max %g at (%d,%d); |U| max %g at (%d,%d,%d); S min %g at (%d,%d,%d); "
+                     "S max %g at (%d,%d,%d)", kt, zmax[0], loc[0], loc[1], zmax[1], loc[2], loc[3], loc[4], -zmax[2],
+                     loc[5], loc[6], loc[7], zmax[3], loc[8], loc[9], loc[10]);
+}
+
 // ---------------------------------------------------------------------------
 // FP64 entry points
 // ---------------------------------------------------------------------------
@@ -1443,6 +1491,12 @@ int nemo_hip_dyn_ldf_lap(int kt, const double *pub, const double *pvb, double *p
     return do_dyn_ldf_lap<double>(kt, pub, pvb, pua, pva, kpass, ahmt, ahmf);
 }
 
+int nemo_hip_stp_ctl(int kt, const double *sshn, const double *un, const double *tsn, double zmax[6],
+                     int32_t loc[11], int *kindic)
+{
+    return do_stp_ctl<double>(kt, sshn, un, tsn, zmax, loc, kindic);
+}
+
 int nemo_hip_ts_wgt(int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2)
 {
     NEED_DOMAIN();
@@ -1553,6 +1607,12 @@ int nemo_hip_dyn_ldf_lap_sp(int kt, const float *pub, const float *pvb, float *p
                             const float *ahmt, const float *ahmf)
 {
     return do_dyn_ldf_lap<float>(kt, pub, pvb, pua, pva, kpass, ahmt, ahmf);
+}
+
+int nemo_hip_stp_ctl_sp(int kt, const double *sshn, const float *un, const float *tsn, double zmax[6],
+                        int32_t loc[11], int *kindic)
+{
+    return do_stp_ctl<float>(kt, sshn, un, tsn, zmax, loc, kindic);
 }
 
 }  // extern "C"

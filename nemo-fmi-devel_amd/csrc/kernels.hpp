@@ -61,3 +61,5 @@ template <typename T> int launch_bn2(const T *pts, const T *pab, T *pn2, const T
 template <typename T> int launch_hpg_zco(const T *rhd, T *ua, T *va);
 template <typename T>
 int launch_dyn_ldf_lap(double zsign, const T *pub, const T *pvb, T *pua, T *pva, const T *ahmt, const T *ahmf);
+template <typename T> int launch_stp_ctl(const double *sshn, const T *un, const T *tsn, double *buf, int want_loc);
+size_t stp_ctl_buffer_bytes();

@@ -403,6 +403,115 @@ int launch_dyn_nxt(int euler, T *ub, T *vb, T *un, T *vn, T *ua, T *va, const do
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// stp_ctl, stpctl.F90:113-121: the six maxima as a two-pass reduction
+// (per-block partials, then one block), NaN-propagating; and, when a test
+// fails, the first Fortran-order position of each reported extremum
+// (atomic minimum of the linear index over the matching cells)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double nanmax(double a, double b) { return (a > b || a != a) ? a : b; }
+
+template <typename T>
+KERNEL void k_stp_max(DevDomain d, const double *__restrict__ sshn, const T *__restrict__ un,
+                      const T *__restrict__ tsn, double *__restrict__ part)
+{
+    __shared__ double red[6][NH_BLOCK];
+    double m[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) m[c] = -INFINITY;
+    const size_t n2 = d.n2, n3 = d.n3;
+    const size_t stride = (size_t)gridDim.x * NH_BLOCK;
+    for (size_t q = (size_t)blockIdx.x * NH_BLOCK + threadIdx.x; q < n3; q += stride) {
+        if (q < n2) m[0] = nanmax(m[0], fabs(sshn[q]));
+        m[1] = nanmax(m[1], fabs((double)un[q]));
+        const int k = (int)(q / n2) + 1;
+        if (pmask<double, 0>(d.ktu, (uint32_t)(q % n2), k) == 1.0) {
+            const double sv = (double)tsn[n3 + q], tv = (double)tsn[q];
+            m[2] = nanmax(m[2], -sv);
+            m[3] = nanmax(m[3], sv);
+            m[4] = nanmax(m[4], -tv);
+            m[5] = nanmax(m[5], tv);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 6; c++) red[c][threadIdx.x] = m[c];
+    __syncthreads();
+    for (int h = NH_BLOCK / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+#pragma unroll
+            for (int c = 0; c < 6; c++) red[c][threadIdx.x] = nanmax(red[c][threadIdx.x], red[c][threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) part[(size_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+KERNEL void k_stp_final(const double *__restrict__ part, int nb, double *__restrict__ zmax,
+                        uint32_t *__restrict__ loc)
+{
+    __shared__ double red[NH_BLOCK];
+    for (int c = 0; c < 6; c++) {
+        double m = -INFINITY;
+        for (int b = (int)threadIdx.x; b < nb; b += NH_BLOCK) m = nanmax(m, part[(size_t)b * 6 + c]);
+        red[threadIdx.x] = m;
+        __syncthreads();
+        for (int h = NH_BLOCK / 2; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) red[threadIdx.x] = nanmax(red[threadIdx.x], red[threadIdx.x + h]);
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) zmax[c] = red[0];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) loc[threadIdx.x] = 0xffffffffu;
+}
+
+// loc[0]: |ssh| max, loc[1]: |u| max, loc[2]: S min, loc[3]: S max (linear indices)
+__device__ __forceinline__ bool hit(double v, double m) { return v == m || (m != m && v != v); }
+template <typename T>
+KERNEL void k_stp_loc(DevDomain d, const double *__restrict__ sshn, const T *__restrict__ un,
+                      const T *__restrict__ tsn, const double *__restrict__ zmax, uint32_t *__restrict__ loc)
+{
+    const size_t n2 = d.n2, n3 = d.n3;
+    const size_t stride = (size_t)gridDim.x * NH_BLOCK;
+    const double m0 = zmax[0], m1 = zmax[1], m2 = zmax[2], m3 = zmax[3];
+    for (size_t q = (size_t)blockIdx.x * NH_BLOCK + threadIdx.x; q < n3; q += stride) {
+        if (q < n2) {
+            const double s = sshn[q];
+            if (hit(fabs(s), m0) || (m0 != m0 && s != s)) atomicMin(&loc[0], (uint32_t)q);
+        }
+        const double u = (double)un[q];
+        if (hit(fabs(u), m1) || (m1 != m1 && u != u)) atomicMin(&loc[1], (uint32_t)q);
+        const int k = (int)(q / n2) + 1;
+        if (pmask<double, 0>(d.ktu, (uint32_t)(q % n2), k) == 1.0) {
+            const double sv = (double)tsn[n3 + q];
+            if (hit(-sv, m2) || (m2 != m2 && sv != sv)) atomicMin(&loc[2], (uint32_t)q);
+            if (hit(sv, m3) || (m3 != m3 && sv != sv)) atomicMin(&loc[3], (uint32_t)q);
+        }
+    }
+}
+
+#define STP_NB 1024
+template <typename T>
+int launch_stp_ctl(const double *sshn, const T *un, const T *tsn, double *buf, int want_loc)
+{
+    const DevDomain &d = nh::dom();
+    double *part = buf, *zmax = buf + (size_t)STP_NB * 6;
+    uint32_t *loc = (uint32_t *)(zmax + 8);
+    Grid2 g;
+    g.block = dim3(NH_BLOCK, 1, 1);
+    g.grid = dim3(STP_NB, 1, 1);
+    if (!want_loc) {
+        LAUNCH("stp_ctl", k_stp_max<T>, g, d, sshn, un, tsn, part);
+        Grid2 g1;
+        g1.block = dim3(NH_BLOCK, 1, 1);
+        g1.grid = dim3(1, 1, 1);
+        LAUNCH("stp_ctl", k_stp_final, g1, (const double *)part, STP_NB, zmax, loc);
+    } else {
+        LAUNCH("stp_ctl", k_stp_loc<T>, g, d, sshn, un, tsn, (const double *)zmax, loc);
+    }
+    return 0;
+}
+size_t stp_ctl_buffer_bytes() { return ((size_t)STP_NB * 6 + 8) * sizeof(double) + 4 * sizeof(uint32_t); }
+
 #define INST(T)                                                                                                  \
     template int launch_div_hor<T>(const T *, const T *, T *);                                                   \
     template int launch_ssh_nxt<T>(double, double, const T *, const T *, T *, const double *, const double *,    \
@@ -411,7 +520,8 @@ int launch_dyn_nxt(int euler, T *ub, T *vb, T *un, T *vn, T *ua, T *va, const do
     template int launch_tra_nxt<T>(int, T *, T *, T *);                                                          \
     template int launch_dyn_nxt<T>(int, T *, T *, T *, T *, T *, T *, const double *, const double *,            \
                                    const double *, const double *, double *, double *, double *, double *,      \
-                                   const double *, const double *);
+                                   const double *, const double *);                                              \
+    template int launch_stp_ctl<T>(const double *, const T *, const T *, double *, int);
 INST(double)
 INST(float)
 #undef INST

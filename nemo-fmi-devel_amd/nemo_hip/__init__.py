@@ -335,6 +335,22 @@ class NemoHip:
                        p(akz), p(avt)))
 
 
+    # run control ----------------------------------------------------------------
+    def stp_ctl(self, kt, sshn, un, tsn, sp=False):
+        """stp_ctl: returns (status, zmax[6], loc[11], kindic, message).  status
+        is 0, or NEMO_HIP_ERR_NUMERIC when a test fails (the reference's
+        ctl_stop); other errors raise."""
+        fn = self.L.nemo_hip_stp_ctl_sp if sp else self.L.nemo_hip_stp_ctl
+        dt = np.float32 if sp else np.float64
+        zmax = np.zeros(6)
+        loc = np.zeros(11, np.int32)
+        kind = C.c_int(0)
+        rc = fn(C.c_int(kt), self._p(sshn), self._p(un, dt), self._p(tsn, dt), zmax.ctypes.data_as(C.c_void_p),
+                loc.ctypes.data_as(C.c_void_p), C.byref(kind))
+        if rc not in (0, abi.NEMO_HIP_ERR_NUMERIC):
+            self._check(rc)
+        return rc, zmax, loc, kind.value, (self.last_error() if rc else "")
+
     # density, pressure gradient, lateral viscosity -----------------------------
     def eos_register(self, eos=None):
         """eos: abi.EosStruct (default: the simplified EOS with its defaults)."""

@@ -2121,6 +2121,57 @@ int ora_dyn_ldf_lap(int kt, const double *pub, const double *pvb, double *pua, d
     return 0;
 }
 
+/* stp_ctl, stpctl.F90:113-121 and 149-165: the maxima, the tests and the
+   first (column-major) positions of the extrema; NaNs propagate */
+static double nmax(double a, double b) { return (a > b || a != a) ? a : b; }
+int ora_stp_ctl(int kt, const double *sshn, const double *un, const double *tsn, double *zmax, int *loc, int *kindic)
+{
+    int i, j, k, q;
+    (void)kt;
+    for (q = 0; q < 6; q++) zmax[q] = -INFINITY;
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) zmax[0] = nmax(zmax[0], fabs(A2(sshn, i, j)));
+    for (k = 1; k <= jpk; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                zmax[1] = nmax(zmax[1], fabs(A3(un, i, j, k)));
+                if (TMASK(i, j, k) == 1.0) {
+                    zmax[2] = nmax(zmax[2], -A4(tsn, i, j, k, 2));
+                    zmax[3] = nmax(zmax[3], A4(tsn, i, j, k, 2));
+                    zmax[4] = nmax(zmax[4], -A4(tsn, i, j, k, 1));
+                    zmax[5] = nmax(zmax[5], A4(tsn, i, j, k, 1));
+                }
+            }
+    const int bad = zmax[0] > 20. || zmax[1] > 10. || zmax[2] >= 0. || zmax[3] >= 100. || zmax[3] < 0. ||
+                    isnan(zmax[0] + zmax[1] + zmax[2]);
+    if (!bad) return 0;
+    for (q = 0; q < 11; q++) loc[q] = 0;
+    int f0 = 0, f1 = 0, f2 = 0, f3 = 0;
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++)
+            if (!f0 && (fabs(A2(sshn, i, j)) == zmax[0] || (zmax[0] != zmax[0] && A2(sshn, i, j) != A2(sshn, i, j)))) {
+                loc[0] = i; loc[1] = j; f0 = 1;
+            }
+    for (k = 1; k <= jpk; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                const double u = A3(un, i, j, k);
+                if (!f1 && (fabs(u) == zmax[1] || (zmax[1] != zmax[1] && u != u))) {
+                    loc[2] = i; loc[3] = j; loc[4] = k; f1 = 1;
+                }
+                if (TMASK(i, j, k) != 1.0) continue;
+                const double sv = A4(tsn, i, j, k, 2);
+                if (!f2 && (-sv == zmax[2] || (zmax[2] != zmax[2] && sv != sv))) {
+                    loc[5] = i; loc[6] = j; loc[7] = k; f2 = 1;
+                }
+                if (!f3 && (sv == zmax[3] || (zmax[3] != zmax[3] && sv != sv))) {
+                    loc[8] = i; loc[9] = j; loc[10] = k; f3 = 1;
+                }
+            }
+    *kindic = -3;
+    return 5;
+}
+
 int ora_abi_layout(size_t *out, int n)
 {
     size_t v[] = {sizeof(nemo_hip_domain_t), offsetof(nemo_hip_domain_t, rdt), offsetof(nemo_hip_domain_t, k_top),

@@ -171,6 +171,14 @@ class Oracle:
         _chk(self.L.ora_dyn_ldf_lap(C.c_int(kt), _p(pub), _p(pvb), _p(pua), _p(pva), C.c_int(kpass), _p(ahmt),
                                     _p(ahmf)), "dyn_ldf_lap")
 
+    def stp_ctl(self, kt, sshn, un, tsn):
+        """(status, zmax[6], loc[11], kindic) of the oracle's stp_ctl."""
+        zmax = np.zeros(6)
+        loc = np.zeros(11, np.int32)
+        kind = C.c_int(0)
+        st = self.L.ora_stp_ctl(C.c_int(kt), _p(sshn), _p(un), _p(tsn), _p(zmax), _p(loc), C.byref(kind))
+        return st, zmax, loc, kind.value
+
     # time-step closure ----------------------------------------------------
     def div_hor(self, kt, un, vn, hdivn):
         _chk(self.L.ora_div_hor(C.c_int(kt), _p(un), _p(vn), _p(hdivn)), "div_hor")

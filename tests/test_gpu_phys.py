@@ -171,3 +171,42 @@ def test_phys_host_pointer_mode_and_errors(hip):
     x3 = dev(np.zeros(e.s3))
     with pytest.raises(nemo_hip.NemoHipError, match="kpass"):
         hip.dyn_ldf_lap(1, x3, x3, x3, x3, 3, x3, x3)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["gyre", "cyc7"])
+@pytest.mark.parametrize("sp", [False, True])
+def test_stp_ctl(hip, case, sp):
+    """Maxima, tests, status and positions equal the oracle's (a maximum is
+    exact in any order), for a healthy state, a blow-up and a NaN."""
+    e = Env(hip, case)
+    t, u, v, w, f = configs.masks(e.d)
+    rng = np.random.default_rng(76)
+    ssh = np.asfortranarray(rng.uniform(-1, 1, e.s2))
+    un = rnd(rng, e.s3, sp)
+    ts = tracers(rng, e, sp)
+    ts[..., 0] *= t
+    ts[..., 1] *= t
+    wet = t == 1
+    iw = np.argwhere(wet.transpose(2, 1, 0))[0][::-1]
+
+    def both():
+        want = e.o.stp_ctl(5, ssh, un, ts)
+        got = hip.stp_ctl(5, dev(ssh), g(un, sp), g(ts, sp), sp=sp)
+        return want, got
+
+    for scenario in ("healthy", "blow-up", "nan"):
+        if scenario == "blow-up":
+            un[3, 4, 1] = 12.0
+            ts[iw[0], iw[1], iw[2], 1] = -0.5
+        if scenario == "nan":
+            un[3, 4, 1] = 0.0
+            ts[iw[0], iw[1], iw[2], 1] = 35.0
+            ssh[5, 3] = np.nan
+        (st0, z0, l0, k0), (st1, z1, l1, k1, msg) = both()
+        assert st1 == st0, (scenario, st0, st1, msg)
+        assert np.array_equal(z1, z0, equal_nan=True), (scenario, z0, z1)
+        assert k1 == k0
+        if st0:
+            assert np.array_equal(l1, l0), (scenario, l0, l1)
+            assert "stp_ctl" in msg

@@ -238,3 +238,37 @@ def test_dyn_ldf_lap_potential_and_streamfunction_flows():
     assert np.allclose(ua, ua0, rtol=0, atol=1e-12 * np.abs(ua0).max())
     assert np.allclose(va, va0, rtol=0, atol=1e-12 * np.abs(va0).max())
     assert np.abs(ua0).max() > 0
+
+
+# ----------------------------------------------------------------------------
+def test_stp_ctl_maxima_tests_and_positions():
+    d = domain()
+    o = Oracle(d)
+    t, u, v, w, f = configs.masks(d)
+    s2, s3 = (d.jpi, d.jpj), (d.jpi, d.jpj, d.jpk)
+    rng = np.random.default_rng(9)
+    ssh = np.asfortranarray(rng.uniform(-1, 1, s2))
+    un = np.asfortranarray(rng.uniform(-1, 1, s3))
+    ts = np.zeros(s3 + (2,), order="F")
+    ts[..., 0] = rng.uniform(0, 20, s3) * t
+    ts[..., 1] = rng.uniform(33, 36, s3) * t
+    st, zmax, loc, kind = o.stp_ctl(5, ssh, un, ts)
+    assert st == 0 and kind == 0
+    wet = t == 1
+    assert zmax[0] == np.abs(ssh).max() and zmax[1] == np.abs(un).max()
+    assert zmax[2] == -ts[..., 1][wet].min() and zmax[3] == ts[..., 1][wet].max()
+    assert zmax[4] == -ts[..., 0][wet].min() and zmax[5] == ts[..., 0][wet].max()
+    # a velocity of 12 m/s at (4,5,2) and a fresh cell at the first wet point in Fortran order
+    un[3, 4, 1] = -12.0
+    iw = np.argwhere(wet.transpose(2, 1, 0))[0][::-1]      # (i,j,k) 0-based, first in Fortran order
+    ts[iw[0], iw[1], iw[2], 1] = -0.5
+    st, zmax, loc, kind = o.stp_ctl(5, ssh, un, ts)
+    assert st == 5 and kind == -3
+    assert zmax[1] == 12.0 and list(loc[2:5]) == [4, 5, 2]
+    assert zmax[2] == 0.5 and list(loc[5:8]) == [iw[0] + 1, iw[1] + 1, iw[2] + 1]
+    # a NaN in the sea level is caught and located
+    un[3, 4, 1] = 0.0
+    ts[iw[0], iw[1], iw[2], 1] = 35.0
+    ssh[7, 3] = np.nan
+    st, zmax, loc, kind = o.stp_ctl(5, ssh, un, ts)
+    assert st == 5 and np.isnan(zmax[0]) and list(loc[0:2]) == [8, 4]
