@@ -347,6 +347,18 @@ int nemo_hip_hpg_zco(int kt, const double *rhd, double *ua, double *va);
 int nemo_hip_dyn_ldf_lap(int kt, const double *pub, const double *pvb, double *pua, double *pva,
                          int kpass, const double *ahmt, const double *ahmf);
 
+/* tra_sbc, TRA/trasbc.F90:62-238, linear free surface, no penetrative solar
+   flux (ln_traqsr = F), no ice shelf, runoff or assimilation terms: qns +=
+   qsr and qsr = 0 (lines 108-111); the swap of the surface content trends
+   sbc_tsc_b <- sbc_tsc and the new sbc_tsc (jpi,jpj,2) from qns, sfx and the
+   concentration/dilution of emp; the trend added to level 1 of tsa.  At
+   kt = nit000 the run starts from rest of the forcing (no restart): both
+   trends reset and zfact = 1.  rcp = 3991.86795711963 (TRA/eosbn2.F90:1245). */
+int nemo_hip_tra_sbc(int kt, double *qns, double *qsr, const double *sfx, const double *emp,
+                     const double *tsn, double *tsa, double *sbc_tsc, double *sbc_tsc_b);
+int nemo_hip_tra_sbc_sp(int kt, double *qns, double *qsr, const double *sfx, const double *emp,
+                        const float *tsn, float *tsa, double *sbc_tsc, double *sbc_tsc_b);
+
 /* stp_ctl, stpctl.F90:55-190: the run control of one time step.  zmax[0..5]
    = max|sshn|, max|un|, max(-S), max(S), max(-T), max(T) (the tracers over
    tmask = 1), as lines 113-121 define them; a NaN anywhere in sshn, un or S

@@ -1274,6 +1274,25 @@ static int do_dyn_ldf_lap(int kt, const T *pub, const T *pvb, T *pua, T *pva, in
     return S.finish(rc);
 }
 
+// tra_sbc (linear free surface, no penetrative solar flux)
+template <typename T>
+static int do_tra_sbc(int kt, double *qns, double *qsr, const double *sfx, const double *emp, const T *tsn, T *tsa,
+                      double *sbc, double *sbc_b)
+{
+    NEED_DOMAIN();
+    NEED_STEADY(kt);
+    NEED_PTR(qns); NEED_PTR(qsr); NEED_PTR(sfx); NEED_PTR(emp); NEED_PTR(tsn); NEED_PTR(tsa); NEED_PTR(sbc);
+    NEED_PTR(sbc_b);
+    const size_t n2 = g_c->dom.n2, n3 = g_c->dom.n3;
+    Stager S;
+    double *q1 = S.inout(qns, n2), *q2 = S.inout(qsr, n2), *c = S.inout(sbc, 2 * n2), *cb = S.inout(sbc_b, 2 * n2);
+    const double *sf = S.in(sfx, n2), *em = S.in(emp, n2);
+    const T *tn = S.in(tsn, 2 * n3);
+    T *ta = S.inout(tsa, 2 * n3);
+    int rc = S.status ? S.status : launch_tra_sbc<T>(kt == g_c->hdom.nit000, q1, q2, sf, em, tn, ta, c, cb);
+    return S.finish(rc);
+}
+
 // stp_ctl: maxima on the device, the tests and the message on the host
 template <typename T>
 static int do_stp_ctl(int kt, const double *sshn, const T *un, const T *tsn, double zmax[6], int32_t loc[11],
@@ -1497,6 +1516,12 @@ int nemo_hip_stp_ctl(int kt, const double *sshn, const double *un, const double 
     return do_stp_ctl<double>(kt, sshn, un, tsn, zmax, loc, kindic);
 }
 
+int nemo_hip_tra_sbc(int kt, double *qns, double *qsr, const double *sfx, const double *emp, const double *tsn,
+                     double *tsa, double *sbc_tsc, double *sbc_tsc_b)
+{
+    return do_tra_sbc<double>(kt, qns, qsr, sfx, emp, tsn, tsa, sbc_tsc, sbc_tsc_b);
+}
+
 int nemo_hip_ts_wgt(int ll_av, int ll_fw, int *jpit, double *zwgt1, double *zwgt2)
 {
     NEED_DOMAIN();
@@ -1613,6 +1638,12 @@ int nemo_hip_stp_ctl_sp(int kt, const double *sshn, const float *un, const float
                         int32_t loc[11], int *kindic)
 {
     return do_stp_ctl<float>(kt, sshn, un, tsn, zmax, loc, kindic);
+}
+
+int nemo_hip_tra_sbc_sp(int kt, double *qns, double *qsr, const double *sfx, const double *emp, const float *tsn,
+                        float *tsa, double *sbc_tsc, double *sbc_tsc_b)
+{
+    return do_tra_sbc<float>(kt, qns, qsr, sfx, emp, tsn, tsa, sbc_tsc, sbc_tsc_b);
 }
 
 }  // extern "C"

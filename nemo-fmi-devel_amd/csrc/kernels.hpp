@@ -63,3 +63,6 @@ template <typename T>
 int launch_dyn_ldf_lap(double zsign, const T *pub, const T *pvb, T *pua, T *pva, const T *ahmt, const T *ahmf);
 template <typename T> int launch_stp_ctl(const double *sshn, const T *un, const T *tsn, double *buf, int want_loc);
 size_t stp_ctl_buffer_bytes();
+template <typename T>
+int launch_tra_sbc(int first, double *qns, double *qsr, const double *sfx, const double *emp, const T *tsn, T *tsa,
+                   double *sbc, double *sbc_b);

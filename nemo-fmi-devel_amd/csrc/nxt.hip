@@ -512,6 +512,59 @@ int launch_stp_ctl(const double *sshn, const T *un, const T *tsn, double *buf, i
 }
 size_t stp_ctl_buffer_bytes() { return ((size_t)STP_NB * 6 + 8) * sizeof(double) + 4 * sizeof(uint32_t); }
 
+// ---------------------------------------------------------------------------
+// tra_sbc, TRA/trasbc.F90:108-160 (linear free surface, ln_traqsr = F): one
+// thread per (i,j); every point swaps or resets its trends, the points of
+// 2..jpim1 x 2..jpj get the new trend and update level 1 of tsa
+// ---------------------------------------------------------------------------
+template <typename T>
+KERNEL void k_tra_sbc(DevDomain d, int first, double *__restrict__ qns, double *__restrict__ qsr,
+                      const double *__restrict__ sfx, const double *__restrict__ emp, const T *__restrict__ tsn,
+                      T *__restrict__ tsa, double *__restrict__ sbc, double *__restrict__ sbc_b)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const uint32_t ij = IJ32(i, j);
+    const size_t n2 = d.n2, n3 = d.n3;
+    const double rcp = 3991.86795711963, r1_rau0 = 1.0 / d.rau0, r1_rau0_rcp = 1.0 / (d.rau0 * rcp);
+    const double q = qns[ij] + qsr[ij];
+    qns[ij] = q;
+    qsr[ij] = 0.0;
+    double bt, bs, ct, cs;
+    if (first) {
+        bt = bs = ct = cs = 0.0;
+    } else {
+        bt = sbc[ij];
+        bs = sbc[n2 + ij];
+        ct = bt;
+        cs = bs;
+    }
+    const double zfact = first ? 1.0 : 0.5;
+    if (j >= 2 && i >= 2 && i <= d.jpi - 1) {
+        ct = r1_rau0_rcp * q;
+        cs = r1_rau0 * sfx[ij];
+        ct = ct + r1_rau0 * emp[ij] * (double)tsn[ij];
+        cs = cs + r1_rau0 * emp[ij] * (double)tsn[n3 + ij];
+        const double e3 = E3(d.md.e3t, ij, 1);
+        tsa[ij] = (T)((double)tsa[ij] + zfact * (bt + ct) / e3);
+        tsa[n3 + ij] = (T)((double)tsa[n3 + ij] + zfact * (bs + cs) / e3);
+    }
+    sbc_b[ij] = bt;
+    sbc_b[n2 + ij] = bs;
+    sbc[ij] = ct;
+    sbc[n2 + ij] = cs;
+}
+
+template <typename T>
+int launch_tra_sbc(int first, double *qns, double *qsr, const double *sfx, const double *emp, const T *tsn, T *tsa,
+                   double *sbc, double *sbc_b)
+{
+    const DevDomain &d = nh::dom();
+    Grid2 g = grid2(d.jpi, d.jpj);
+    LAUNCH("tra_sbc", k_tra_sbc<T>, g, d, first, qns, qsr, sfx, emp, tsn, tsa, sbc, sbc_b);
+    return 0;
+}
+
 #define INST(T)                                                                                                  \
     template int launch_div_hor<T>(const T *, const T *, T *);                                                   \
     template int launch_ssh_nxt<T>(double, double, const T *, const T *, T *, const double *, const double *,    \
@@ -521,7 +574,9 @@ size_t stp_ctl_buffer_bytes() { return ((size_t)STP_NB * 6 + 8) * sizeof(double)
     template int launch_dyn_nxt<T>(int, T *, T *, T *, T *, T *, T *, const double *, const double *,            \
                                    const double *, const double *, double *, double *, double *, double *,      \
                                    const double *, const double *);                                              \
-    template int launch_stp_ctl<T>(const double *, const T *, const T *, double *, int);
+    template int launch_stp_ctl<T>(const double *, const T *, const T *, double *, int);                        \
+    template int launch_tra_sbc<T>(int, double *, double *, const double *, const double *, const T *, T *,       \
+                                   double *, double *);
 INST(double)
 INST(float)
 #undef INST

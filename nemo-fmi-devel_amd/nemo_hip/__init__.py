@@ -335,6 +335,13 @@ class NemoHip:
                        p(akz), p(avt)))
 
 
+    def tra_sbc(self, kt, qns, qsr, sfx, emp, tsn, tsa, sbc_tsc, sbc_tsc_b, sp=False):
+        fn = self.L.nemo_hip_tra_sbc_sp if sp else self.L.nemo_hip_tra_sbc
+        dt = np.float32 if sp else np.float64
+        q = self._p
+        self._check(fn(C.c_int(kt), q(qns), q(qsr), q(sfx), q(emp), q(tsn, dt), q(tsa, dt), q(sbc_tsc),
+                       q(sbc_tsc_b)))
+
     # run control ----------------------------------------------------------------
     def stp_ctl(self, kt, sshn, un, tsn, sp=False):
         """stp_ctl: returns (status, zmax[6], loc[11], kindic, message).  status

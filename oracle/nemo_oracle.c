@@ -2172,6 +2172,48 @@ int ora_stp_ctl(int kt, const double *sshn, const double *un, const double *tsn,
     return 5;
 }
 
+/* tra_sbc, TRA/trasbc.F90:108-160 (see the entry point's restrictions) */
+int ora_tra_sbc(int kt, double *qns, double *qsr, const double *sfx, const double *emp, const double *tsn,
+                double *tsa, double *sbc_tsc, double *sbc_tsc_b)
+{
+    int i, j, n;
+    const double rcp = 3991.86795711963, r1_rau0 = 1.0 / D.rau0, r1_rau0_rcp = 1.0 / (D.rau0 * rcp);
+    const size_t nn = (size_t)jpi * jpj;
+    double zfact;
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) {
+            A2(qns, i, j) = A2(qns, i, j) + A2(qsr, i, j);
+            A2(qsr, i, j) = 0.0;
+        }
+    if (kt == D.nit000) {
+        zfact = 1.0;
+        for (size_t q = 0; q < 2 * nn; q++) {
+            sbc_tsc[q] = 0.0;
+            sbc_tsc_b[q] = 0.0;
+        }
+    } else {
+        zfact = 0.5;
+        for (size_t q = 0; q < 2 * nn; q++) sbc_tsc_b[q] = sbc_tsc[q];
+    }
+    for (j = 2; j <= jpj; j++)
+        for (i = 2; i <= jpim1; i++) {
+            A2(sbc_tsc, i, j) = r1_rau0_rcp * A2(qns, i, j);
+            A2(sbc_tsc + nn, i, j) = r1_rau0 * A2(sfx, i, j);
+        }
+    for (j = 2; j <= jpj; j++)
+        for (i = 2; i <= jpim1; i++) {
+            A2(sbc_tsc, i, j) = A2(sbc_tsc, i, j) + r1_rau0 * A2(emp, i, j) * A4(tsn, i, j, 1, 1);
+            A2(sbc_tsc + nn, i, j) = A2(sbc_tsc + nn, i, j) + r1_rau0 * A2(emp, i, j) * A4(tsn, i, j, 1, 2);
+        }
+    for (n = 1; n <= 2; n++)
+        for (j = 2; j <= jpj; j++)
+            for (i = 2; i <= jpim1; i++)
+                A4(tsa, i, j, 1, n) = A4(tsa, i, j, 1, n) +
+                                      zfact * (A2(sbc_tsc_b + (n - 1) * nn, i, j) + A2(sbc_tsc + (n - 1) * nn, i, j)) /
+                                          e3t_n(i, j, 1);
+    return 0;
+}
+
 int ora_abi_layout(size_t *out, int n)
 {
     size_t v[] = {sizeof(nemo_hip_domain_t), offsetof(nemo_hip_domain_t, rdt), offsetof(nemo_hip_domain_t, k_top),

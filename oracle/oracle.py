@@ -179,6 +179,10 @@ class Oracle:
         st = self.L.ora_stp_ctl(C.c_int(kt), _p(sshn), _p(un), _p(tsn), _p(zmax), _p(loc), C.byref(kind))
         return st, zmax, loc, kind.value
 
+    def tra_sbc(self, kt, qns, qsr, sfx, emp, tsn, tsa, sbc_tsc, sbc_tsc_b):
+        _chk(self.L.ora_tra_sbc(C.c_int(kt), _p(qns), _p(qsr), _p(sfx), _p(emp), _p(tsn), _p(tsa), _p(sbc_tsc),
+                                _p(sbc_tsc_b)), "tra_sbc")
+
     # time-step closure ----------------------------------------------------
     def div_hor(self, kt, un, vn, hdivn):
         _chk(self.L.ora_div_hor(C.c_int(kt), _p(un), _p(vn), _p(hdivn)), "div_hor")

@@ -210,3 +210,33 @@ def test_stp_ctl(hip, case, sp):
         if st0:
             assert np.array_equal(l1, l0), (scenario, l0, l1)
             assert "stp_ctl" in msg
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["basin", "cyc7"])
+@pytest.mark.parametrize("sp", [False, True])
+def test_tra_sbc(hip, case, sp):
+    e = Env(hip, case)
+    rng = np.random.default_rng(77)
+    ts = tracers(rng, e, sp)
+    fields = {n: np.asfortranarray(rng.standard_normal(e.s2) * sc) for n, sc in
+              (("qns", 100.0), ("qsr", 50.0), ("sfx", 1e-3), ("emp", 1e-5))}
+    sbc, sbc_b = np.zeros(e.s2 + (2,), order="F"), np.zeros(e.s2 + (2,), order="F")
+    o = {n: v.copy(order="F") for n, v in fields.items()}
+    o.update(sbc=sbc.copy(order="F"), sbc_b=sbc_b.copy(order="F"))
+    gd = {n: dev(v) for n, v in o.items()}
+    tsa = rnd(rng, e.s4, sp)
+    wta, gta = tsa.copy(order="F"), g(tsa, sp)
+    for kt in (1, 2, 3):
+        for n in ("qns", "qsr"):            # the forcing module refreshes the fluxes every step
+            o[n][:] = fields[n]
+            gd[n].copy_(dev(fields[n]))
+        torch.cuda.synchronize()            # the refresh is on torch's stream, the library runs on its own
+        e.o.tra_sbc(kt, o["qns"], o["qsr"], o["sfx"], o["emp"], ts, wta, o["sbc"], o["sbc_b"])
+        hip.tra_sbc(kt, gd["qns"], gd["qsr"], gd["sfx"], gd["emp"], g(ts, sp), gta, gd["sbc"], gd["sbc_b"], sp=sp)
+        sync(hip)
+        cmp(host(gta, e.s4), wta, sp, f"tra_sbc {case} kt={kt} tsa")
+        for n in ("qns", "qsr"):
+            check(host(gd[n], e.s2), o[n], what=f"tra_sbc {n}")
+        for n in ("sbc", "sbc_b"):
+            check(host(gd[n], e.s2 + (2,)), o[n], tol=SP_TOL if sp else 1e-12, what=f"tra_sbc {n}")

@@ -272,3 +272,45 @@ def test_stp_ctl_maxima_tests_and_positions():
     ssh[7, 3] = np.nan
     st, zmax, loc, kind = o.stp_ctl(5, ssh, un, ts)
     assert st == 5 and np.isnan(zmax[0]) and list(loc[0:2]) == [8, 4]
+
+
+# ----------------------------------------------------------------------------
+def test_tra_sbc_closed_forms():
+    d = domain()
+    o = Oracle(d)
+    s2, s3 = (d.jpi, d.jpj), (d.jpi, d.jpj, d.jpk)
+    rau0, rcp = d.opts["rau0"], 3991.86795711963
+    e3 = d.arrays["e3t_n"][:, :, 0]
+    R = (slice(1, -1), slice(1, None))          # ji = 2..jpim1, jj = 2..jpj
+    rng = np.random.default_rng(10)
+    tsn = np.zeros(s3 + (2,), order="F")
+    tsn[..., 0] = rng.uniform(0, 20, s3)
+    tsn[..., 1] = rng.uniform(33, 36, s3)
+
+    def run(kt, qns, qsr, sfx, emp, sbc, sbc_b):
+        tsa = np.zeros(s3 + (2,), order="F")
+        o.tra_sbc(kt, qns, qsr, sfx, emp, tsn, tsa, sbc, sbc_b)
+        return tsa
+
+    Q, Qs, F = 150.0, 50.0, 1.0e-3
+    qns, qsr = np.full(s2, Q, order="F"), np.full(s2, Qs, order="F")
+    sfx, emp = np.full(s2, F, order="F"), np.zeros(s2, order="F")
+    sbc, sbc_b = np.zeros(s2 + (2,), order="F"), np.zeros(s2 + (2,), order="F")
+    tsa = run(1, qns, qsr, sfx, emp, sbc, sbc_b)            # Euler start: zfact = 1, no before trend
+    assert np.all(qns == Q + Qs) and np.all(qsr == 0)
+    assert np.allclose(tsa[R + (0, 0)], (Q + Qs) / (rau0 * rcp) / e3[R], rtol=1e-14)
+    assert np.allclose(tsa[R + (0, 1)], F / rau0 / e3[R], rtol=1e-14)
+    out = np.ones(s2, bool)
+    out[R] = False
+    assert np.all(tsa[:, :, 0][out] == 0) and np.all(tsa[:, :, 1:] == 0)
+    # next step, same forcing: the trapezoid of equal trends is the trend itself
+    qns[:], qsr[:] = Q, Qs
+    tsa2 = run(2, qns, qsr, sfx, emp, sbc, sbc_b)
+    assert np.allclose(tsa2[R + (0,)], tsa[R + (0,)], rtol=1e-14)
+    # constant-volume concentration/dilution by the fresh-water flux
+    qns[:], qsr[:], sfx[:] = 0.0, 0.0, 0.0
+    emp[:] = 2.0e-5
+    sbc[:], sbc_b[:] = 0.0, 0.0
+    tsa3 = run(1, qns, qsr, sfx, emp, sbc, sbc_b)
+    for n in range(2):
+        assert np.allclose(tsa3[R + (0, n)], 2.0e-5 / rau0 * tsn[R + (0, n)] / e3[R], rtol=1e-14)

@@ -11,10 +11,12 @@ equation of state, pressure gradient, lateral viscosity or surface module
     tra_ldf_iso, tra_zdf              after tracers
     tra_nxt, dyn_nxt, ssh_swp         Asselin filters and swaps
 
-With physics=True the step also has a density and its pressure gradient
-and a lateral viscosity, step.F90:165 and 191-193: eos_insitu_pot of the now
-tracers (simplified EOS) after wzv, then dyn_ldf_lap of the before velocities
-(ahm = AHM m2/s) and hpg_zco between dyn_vor and dyn_spg_ts.
+With physics=True the step also has a density and its pressure gradient, a
+lateral viscosity and the surface tracer fluxes, step.F90:165, 191-193 and
+238: eos_insitu_pot of the now tracers (simplified EOS) after wzv,
+dyn_ldf_lap of the before velocities (ahm = AHM m2/s) and hpg_zco between
+dyn_vor and dyn_spg_ts, and tra_sbc (an analytic heat flux, the generator's
+fresh-water flux) before tra_adv.
 
 oracle_run repeats it with the oracle's separate routines; gpu_run with the
 library on device-resident fields (the fused step_dyn / step_tra entry points,
@@ -48,6 +50,14 @@ def initial(S):
     return I
 
 
+def surface_fluxes(d):
+    """Analytic non-solar and solar heat fluxes (W/m2), salt flux 0."""
+    y = np.arange(d.jpj) / max(d.jpj - 1, 1)
+    qns = np.asfortranarray(np.broadcast_to((-40.0 + 80.0 * np.cos(np.pi * y))[None, :], (d.jpi, d.jpj)).copy())
+    qsr = np.asfortranarray(np.full((d.jpi, d.jpj), 30.0))
+    return qns, qsr, np.zeros((d.jpi, d.jpj), order="F")
+
+
 def viscosity(d):
     """ahmt, ahmf of a constant lateral viscosity AHM, masked by tmask / fmask."""
     from nemo_hip import configs
@@ -63,6 +73,8 @@ def oracle_run(d, S, marks, physics=False):
         o.eos_register(abi.seos())
         gdept, _ = configs.depths(d)
         ahmt, ahmf = viscosity(d)
+        qns0, qsr0, sfx = surface_fluxes(d)
+        sbc, sbc_b = np.zeros((d.jpi, d.jpj, 2), order="F"), np.zeros((d.jpi, d.jpj, 2), order="F")
     s2, s3, s4 = (d.jpi, d.jpj), (d.jpi, d.jpj, d.jpk), (d.jpi, d.jpj, d.jpk, 2)
     X = {k: np.array(v, order="F", copy=True) for k, v in initial(S).items()}
     for k in ("ua_b", "va_b", "ssha", "un_adv", "vn_adv"):
@@ -91,6 +103,9 @@ def oracle_run(d, S, marks, physics=False):
                   X["utau_b"], X["vtau"], X["vtau_b"])
         z = [np.zeros(s3, order="F") for _ in range(3)]
         tsa = np.zeros(s4, order="F")
+        if physics:
+            qns, qsr = qns0.copy(order="F"), qsr0.copy(order="F")
+            o.tra_sbc(kt, qns, qsr, sfx, X["emp"], X["tsn"], tsa, sbc, sbc_b)
         o.tra_adv_trp(X["un"], X["vn"], X["wn"], *z)
         o.tra_adv_fct(kt, 1, "TRA", r2dt(d, kt), *z, X["tsb"], X["tsn"], tsa, 2, *KN_FCT)
         o.tra_ldf_iso(kt, 1, "TRA", X["ahtu"], X["ahtv"], X["tsb"], X["tsb"], tsa, 2, 1, X["uslp"], X["vslp"],
@@ -127,6 +142,11 @@ def gpu_run(h, d, S, marks, sp=False, physics=False):
         gdept = dev(configs.depths(d)[0], dt)
         ahmt, ahmf = (dev(a, dt) for a in viscosity(d))
         rhd, rhop = z(n3), z(n3)
+        qns0, qsr0, sfx = (dev(a, torch.float64) for a in surface_fluxes(d))
+        qns, qsr = qns0.clone(), qsr0.clone()
+        sbc, sbc_b = torch.zeros(2 * n2, dtype=torch.float64, device="cuda"), \
+            torch.zeros(2 * n2, dtype=torch.float64, device="cuda")
+        zun, zvn, zwn = z(n3), z(n3), z(n3)
     out = {}
     # torch work (the trend reset) is queued on the library stream, in order with the library's launches
     lib_stream = torch.cuda.ExternalStream(h.stream())
@@ -150,8 +170,20 @@ def gpu_run(h, d, S, marks, sp=False, physics=False):
         if physics:
             h.dyn_zdf(kt, X["ub"], X["vb"], ua, va, X["avm"], X["rCdU_bot"], X["ua_b"], X["va_b"], X["utau"],
                       X["utau_b"], X["vtau"], X["vtau_b"], sp=sp)
-        h.step_tra(kt, X["un"], X["vn"], X["wn"], X["tsb"], X["tsn"], tsa, *KN_FCT, X["ahtu"], X["ahtv"],
-                   X["uslp"], X["vslp"], X["wslpi"], X["wslpj"], ah, akz, X["avt"], sp=sp)
+        if physics:
+            with torch.cuda.stream(lib_stream):
+                tsa.zero_()
+                qns.copy_(qns0)
+                qsr.copy_(qsr0)
+            h.tra_sbc(kt, qns, qsr, sfx, X["emp"], X["tsn"], tsa, sbc, sbc_b, sp=sp)
+            h.tra_adv_trp(X["un"], X["vn"], X["wn"], zun, zvn, zwn, sp=sp)
+            h.tra_adv_fct(kt, 1, "TRA", r2dt(d, kt), zun, zvn, zwn, X["tsb"], X["tsn"], tsa, 2, *KN_FCT, sp=sp)
+            h.tra_ldf_iso(kt, 1, "TRA", X["ahtu"], X["ahtv"], X["tsb"], X["tsb"], tsa, 2, 1, X["uslp"], X["vslp"],
+                          X["wslpi"], X["wslpj"], ah, akz, sp=sp)
+            h.tra_zdf(kt, X["tsb"], tsa, X["avt"], X["avt"], 1, ah, akz, sp=sp)
+        else:
+            h.step_tra(kt, X["un"], X["vn"], X["wn"], X["tsb"], X["tsn"], tsa, *KN_FCT, X["ahtu"], X["ahtv"],
+                       X["uslp"], X["vslp"], X["wslpi"], X["wslpj"], ah, akz, X["avt"], sp=sp)
         h.tra_nxt(kt, X["tsb"], X["tsn"], tsa, sp=sp)
         h.dyn_nxt(kt, X["ub"], X["vb"], X["un"], X["vn"], ua, va, X["ua_b"], X["va_b"], X["un_adv"], X["vn_adv"],
                   X["ub_b"], X["vb_b"], X["un_b"], X["vn_b"], X["r1_hu_n"], X["r1_hv_n"], sp=sp)
