@@ -1127,12 +1127,16 @@ KERNEL void k_tra_zdf(DevDomain d, double p2dt, const T * __restrict__ ptb, T * 
         ta[n] = pta + (size_t)(jn0 + n) * n3;
     }
     // zwt(k) of trazdf.F90:153-176: avt/avs, 0 at the surface, plus the
-    // iso-neutral contribution on 2..jpkm1
+    // iso-neutral contribution on 2..jpkm1.  Both loads are unconditional
+    // (levels 1 <= k <= jpk, the second clamped to jpkm1), so the march can
+    // keep several levels of loads in flight; the conditions only select.
+    const T *addp = l_ldfslp ? add : src;
     auto coef = [&](int k) -> double {
-        if (k <= 1) return 0.0;
-        double c = (double)ldp(plane_ptr(src, plane, k), ij);
-        if (l_ldfslp && k <= jpk - 1) c = c + (double)ldp(plane_ptr(add, plane, k), ij);
-        return c;
+        const double sv = (double)ldp(plane_ptr(src, plane, k), ij);
+        const double av = (double)ldp(plane_ptr(addp, plane, k <= jpk - 1 ? k : jpk - 1), ij);
+        double c = sv;
+        if (l_ldfslp && k <= jpk - 1) c = c + av;
+        return k <= 1 ? 0.0 : c;
     };
     // FP64 storage keeps only checkpoint pivots (every ZDF_SEG-th level) and
     // recomputes the others in the back substitution; with FP32 storage the
