@@ -15,6 +15,10 @@ MODULE nemo_hip_mod
    !!   div_hor        DYN/divhor.F90:50            ssh_nxt        DYN/sshwzv.F90:56
    !!   wzv            DYN/sshwzv.F90:129           ssh_swp        DYN/sshwzv.F90:216
    !!   tra_nxt        TRA/tranxt.F90:66            dyn_nxt        DYN/dynnxt.F90:66
+   !!   eos_insitu     TRA/eosbn2.F90:188           eos_insitu_pot TRA/eosbn2.F90:304
+   !!   rab_3d         TRA/eosbn2.F90:566           bn2            TRA/eosbn2.F90:889
+   !!   hpg_zco        DYN/dynhpg.F90:258           dyn_ldf_lap    DYN/dynldf_lap_blp.F90:37
+   !!   tra_sbc        TRA/trasbc.F90:62            stp_ctl        stpctl.F90:55
    !! The arrays those routines take from NEMO modules (oce, zdf_oce, ldfslp,
    !! sbc_oce, zdfdrg, dynspg_ts, dom_oce) are bound once, after allocation, with the
    !! nemo_hip_bind_* routines: module arrays never move, so their addresses
@@ -32,6 +36,8 @@ MODULE nemo_hip_mod
    PUBLIC :: tra_adv_fct, interp_4th_cpt, tra_ldf_lap, tra_ldf_iso, tra_zdf_imp
    PUBLIC :: dyn_keg, dyn_zad, dyn_adv_cen2, dyn_vor, dyn_zdf, dyn_spg_ts
    PUBLIC :: nemo_hip_bind_nxt, div_hor, ssh_nxt, wzv, ssh_swp, tra_nxt, dyn_nxt
+   PUBLIC :: nemo_hip_eos_t, nemo_hip_eos_init, nemo_hip_bind_phys
+   PUBLIC :: eos_insitu, eos_insitu_pot, rab_3d, bn2, hpg_zco, dyn_ldf_lap, tra_sbc, stp_ctl
    PUBLIC :: nemo_hip_domain_t, nemo_hip_spg_ts_t
    PUBLIC :: nemo_hip_field_alloc, nemo_hip_field_free, nemo_hip_field_push, nemo_hip_field_pull
 
@@ -52,6 +58,12 @@ MODULE nemo_hip_mod
       TYPE(C_PTR) :: ff_f, ff_t
       TYPE(C_PTR) :: e3t_n, e3u_n, e3v_n, e3w_n, e3f_n, e3uw_n, e3vw_n
    END TYPE nemo_hip_domain_t
+
+   !! mirror of nemo_hip_eos_t (namelist nameos of the simplified EOS)
+   TYPE, BIND(C) :: nemo_hip_eos_t
+      INTEGER(C_INT32_T) :: neos, reserved0
+      REAL(C_DOUBLE)     :: rn_a0, rn_b0, rn_lambda1, rn_lambda2, rn_mu1, rn_mu2, rn_nu
+   END TYPE nemo_hip_eos_t
 
    !! mirror of nemo_hip_spg_ts_t
    TYPE, BIND(C) :: nemo_hip_spg_ts_t
@@ -75,6 +87,10 @@ MODULE nemo_hip_mod
    TYPE(C_PTR), SAVE :: p_tsb = C_NULL_PTR, p_tsn = C_NULL_PTR, p_tsa = C_NULL_PTR
    TYPE(C_PTR), SAVE :: p_ub_b = C_NULL_PTR, p_vb_b = C_NULL_PTR, p_un_b = C_NULL_PTR, p_vn_b = C_NULL_PTR
    TYPE(C_PTR), SAVE :: p_un_adv = C_NULL_PTR, p_vn_adv = C_NULL_PTR, p_r1_hu_n = C_NULL_PTR, p_r1_hv_n = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_gdept = C_NULL_PTR, p_gdepw = C_NULL_PTR, p_rhd = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_ahmt = C_NULL_PTR, p_ahmf = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_qns = C_NULL_PTR, p_qsr = C_NULL_PTR, p_sfx = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_sbc_tsc = C_NULL_PTR, p_sbc_tsc_b = C_NULL_PTR
    INTEGER, SAVE :: l_ldfslp = 0
    TYPE(nemo_hip_spg_ts_t), SAVE :: spg_state
 
@@ -213,6 +229,49 @@ MODULE nemo_hip_mod
          TYPE(C_PTR), VALUE :: ub, vb, un, vn, ua, va, ua_b, va_b, un_adv, vn_adv, ub_b, vb_b, un_b, vn_b
          TYPE(C_PTR), VALUE :: r1_hu_n, r1_hv_n
       END FUNCTION
+      INTEGER(C_INT) FUNCTION c_eos_register(eos) BIND(C, name="nemo_hip_eos_register")
+         IMPORT :: C_INT, nemo_hip_eos_t
+         TYPE(nemo_hip_eos_t), INTENT(in) :: eos
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_eos_insitu(pts, prd, pdep) BIND(C, name="nemo_hip_eos_insitu")
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: pts, prd, pdep
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_eos_insitu_pot(pts, prd, prhop, pdep) BIND(C, name="nemo_hip_eos_insitu_pot")
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: pts, prd, prhop, pdep
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_rab_3d(pts, pab, gdept) BIND(C, name="nemo_hip_rab_3d")
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: pts, pab, gdept
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_bn2(pts, pab, pn2, gdept, gdepw) BIND(C, name="nemo_hip_bn2")
+         IMPORT :: C_INT, C_PTR
+         TYPE(C_PTR), VALUE :: pts, pab, pn2, gdept, gdepw
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_hpg_zco(kt, rhd, ua, va) BIND(C, name="nemo_hip_hpg_zco")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: rhd, ua, va
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_dyn_ldf_lap(kt, pub, pvb, pua, pva, kpass, ahmt, ahmf) BIND(C, name="nemo_hip_dyn_ldf_lap")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt, kpass
+         TYPE(C_PTR), VALUE :: pub, pvb, pua, pva, ahmt, ahmf
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_tra_sbc(kt, qns, qsr, sfx, emp, tsn, tsa, sbc_tsc, sbc_tsc_b) BIND(C, name="nemo_hip_tra_sbc")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: qns, qsr, sfx, emp, tsn, tsa, sbc_tsc, sbc_tsc_b
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_stp_ctl(kt, sshn, un, tsn, zmax, loc, kindic) BIND(C, name="nemo_hip_stp_ctl")
+         IMPORT :: C_INT, C_PTR, C_INT32_T, C_DOUBLE
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: sshn, un, tsn
+         REAL(C_DOUBLE), INTENT(out) :: zmax(6)
+         INTEGER(C_INT32_T), INTENT(out) :: loc(11)
+         INTEGER(C_INT), INTENT(inout) :: kindic
+      END FUNCTION
    END INTERFACE
 
 CONTAINS
@@ -329,6 +388,23 @@ CONTAINS
       p_un_adv = C_LOC( pun_adv ) ; p_vn_adv = C_LOC( pvn_adv )
       p_r1_hu_n = C_LOC( pr1_hu_n ) ; p_r1_hv_n = C_LOC( pr1_hv_n )
    END SUBROUTINE nemo_hip_bind_nxt
+
+   SUBROUTINE nemo_hip_eos_init( seos )
+      !! eos_init (TRA/eosbn2.F90:1218): the simplified equation of state
+      TYPE(nemo_hip_eos_t), INTENT(in) :: seos
+      CALL check( c_eos_register( seos ), 'eos_init' )
+   END SUBROUTINE nemo_hip_eos_init
+
+   SUBROUTINE nemo_hip_bind_phys( pgdept, pgdepw, prhd, pahmt, pahmf, pqns, pqsr, psfx, psbc_tsc, psbc_tsc_b )
+      !! dom_oce (gdept_n, gdepw_n), oce (rhd), ldfdyn (ahmt, ahmf), sbc_oce (qns, qsr, sfx, sbc_tsc, sbc_tsc_b)
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(in) :: pgdept, pgdepw, prhd, pahmt, pahmf
+      REAL(wp), DIMENSION(:,:)  , TARGET, CONTIGUOUS, INTENT(in) :: pqns, pqsr, psfx
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(in) :: psbc_tsc, psbc_tsc_b
+      p_gdept = C_LOC( pgdept ) ; p_gdepw = C_LOC( pgdepw ) ; p_rhd = C_LOC( prhd )
+      p_ahmt = C_LOC( pahmt ) ; p_ahmf = C_LOC( pahmf )
+      p_qns = C_LOC( pqns ) ; p_qsr = C_LOC( pqsr ) ; p_sfx = C_LOC( psfx )
+      p_sbc_tsc = C_LOC( psbc_tsc ) ; p_sbc_tsc_b = C_LOC( psbc_tsc_b )
+   END SUBROUTINE nemo_hip_bind_phys
 
    SUBROUTINE nemo_hip_bind_spg( sstate )
       !! C_LOC of every array dyn_spg_ts reads or writes (oce, dom_oce, sbc_oce, dynspg_ts)
@@ -454,5 +530,61 @@ CONTAINS
       CALL check( c_dyn_nxt( INT(kt,C_INT), p_ub, p_vb, p_un, p_vn, p_ua, p_va, p_ua_b, p_va_b,               &
          &        p_un_adv, p_vn_adv, p_ub_b, p_vb_b, p_un_b, p_vn_b, p_r1_hu_n, p_r1_hv_n ), 'dyn_nxt' )
    END SUBROUTINE dyn_nxt
+
+   SUBROUTINE eos_insitu( pts, prd, pdep )
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(in   ) ::   pts
+      REAL(wp), DIMENSION(:,:,:)  , TARGET, CONTIGUOUS, INTENT(  out) ::   prd
+      REAL(wp), DIMENSION(:,:,:)  , TARGET, CONTIGUOUS, INTENT(in   ) ::   pdep
+      CALL check( c_eos_insitu( C_LOC(pts), C_LOC(prd), C_LOC(pdep) ), 'eos_insitu' )
+   END SUBROUTINE eos_insitu
+
+   SUBROUTINE eos_insitu_pot( pts, prd, prhop, pdep )
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(in   ) ::   pts
+      REAL(wp), DIMENSION(:,:,:)  , TARGET, CONTIGUOUS, INTENT(  out) ::   prd, prhop
+      REAL(wp), DIMENSION(:,:,:)  , TARGET, CONTIGUOUS, INTENT(in   ) ::   pdep
+      CALL check( c_eos_insitu_pot( C_LOC(pts), C_LOC(prd), C_LOC(prhop), C_LOC(pdep) ), 'eos_insitu_pot' )
+   END SUBROUTINE eos_insitu_pot
+
+   SUBROUTINE rab_3d( pts, pab )
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(in   ) ::   pts
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(  out) ::   pab
+      CALL check( c_rab_3d( C_LOC(pts), C_LOC(pab), p_gdept ), 'rab_3d' )
+   END SUBROUTINE rab_3d
+
+   SUBROUTINE bn2( pts, pab, pn2 )
+      REAL(wp), DIMENSION(:,:,:,:), TARGET, CONTIGUOUS, INTENT(in   ) ::   pts, pab
+      REAL(wp), DIMENSION(:,:,:)  , TARGET, CONTIGUOUS, INTENT(  out) ::   pn2
+      CALL check( c_bn2( C_LOC(pts), C_LOC(pab), C_LOC(pn2), p_gdept, p_gdepw ), 'bn2' )
+   END SUBROUTINE bn2
+
+   SUBROUTINE hpg_zco( kt )
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_hpg_zco( INT(kt,C_INT), p_rhd, p_ua, p_va ), 'hpg_zco' )
+   END SUBROUTINE hpg_zco
+
+   SUBROUTINE dyn_ldf_lap( kt, pub, pvb, pua, pva, kpass )
+      INTEGER                         , INTENT(in   ) ::   kt, kpass
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(in   ) ::   pub, pvb
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(inout) ::   pua, pva
+      CALL check( c_dyn_ldf_lap( INT(kt,C_INT), C_LOC(pub), C_LOC(pvb), C_LOC(pua), C_LOC(pva), INT(kpass,C_INT), &
+         &        p_ahmt, p_ahmf ), 'dyn_ldf_lap' )
+   END SUBROUTINE dyn_ldf_lap
+
+   SUBROUTINE tra_sbc( kt )
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_tra_sbc( INT(kt,C_INT), p_qns, p_qsr, p_sfx, p_emp, p_tsn, p_tsa, p_sbc_tsc, p_sbc_tsc_b ),   &
+         &        'tra_sbc' )
+   END SUBROUTINE tra_sbc
+
+   SUBROUTINE stp_ctl( kt, kindic )
+      INTEGER, INTENT(in   ) ::   kt
+      INTEGER, INTENT(inout) ::   kindic
+      REAL(C_DOUBLE) :: zmax(6)
+      INTEGER(C_INT32_T) :: iloc(11)
+      INTEGER(C_INT) :: ikind
+      ikind = INT(kindic, C_INT)
+      CALL check( c_stp_ctl( INT(kt,C_INT), p_sshn, p_un, p_tsn, zmax, iloc, ikind ), 'stp_ctl' )
+      kindic = ikind
+   END SUBROUTINE stp_ctl
 
 END MODULE nemo_hip_mod
