@@ -145,8 +145,10 @@ typedef struct nemo_hip_spg_ts {
 #define NEMO_NP_EOS80  0
 #define NEMO_NP_SEOS   1
 typedef struct nemo_hip_eos {
-    int32_t neos;               /* NEMO_NP_SEOS (the polynomial TEOS-10 and
-                                   EOS-80 forms are not implemented)        */
+    int32_t neos;               /* NEMO_NP_TEOS10 / NEMO_NP_EOS80 (the
+                                   polynomials of TRA/eosbn2.F90:1266-1647;
+                                   the rn_* fields are unused) or
+                                   NEMO_NP_SEOS                             */
     int32_t reserved0;
     double  rn_a0, rn_b0;       /* thermal / saline expansion               */
     double  rn_lambda1, rn_lambda2;   /* cabbeling in T^2, S^2              */

@@ -38,6 +38,7 @@ struct Ctx {
     int spg_rot_s[4] = {0, 1, 2, 3};                    // ... and sea level: [now, before, before-before, after]
     bool have_eos = false;                              // eos_init state (nemo_hip_eos_register)
     SeosPar eos{};
+    int neos = NEMO_NP_SEOS;
     double *stp_buf = nullptr;                          // stp_ctl partial maxima and results
 };
 static std::vector<Ctx *> g_ctxs;
@@ -1213,7 +1214,7 @@ static int do_eos(int pot, const T *pts, T *prd, T *prhop, const T *pdep)
     Stager S;
     const T *ts = S.in(pts, 2 * n3), *dep = S.in(pdep, n3);
     T *rd = S.inout(prd, n3), *rp = pot ? S.inout(prhop, n3) : nullptr;
-    int rc = S.status ? S.status : launch_eos<T>(g_c->eos, pot, ts, rd, rp, dep);
+    int rc = S.status ? S.status : launch_eos<T>(g_c->neos, g_c->eos, pot, ts, rd, rp, dep);
     return S.finish(rc);
 }
 
@@ -1227,7 +1228,7 @@ static int do_rab(const T *pts, T *pab, const T *gdept)
     Stager S;
     const T *ts = S.in(pts, 2 * n3), *gt = S.in(gdept, n3);
     T *ab = S.inout(pab, 2 * n3);
-    int rc = S.status ? S.status : launch_rab<T>(g_c->eos, ts, ab, gt);
+    int rc = S.status ? S.status : launch_rab<T>(g_c->neos, g_c->eos, ts, ab, gt);
     return S.finish(rc);
 }
 
@@ -1483,9 +1484,10 @@ int nemo_hip_eos_register(const nemo_hip_eos_t *eos)
     NEED_DOMAIN();
     NEED_NOT_CAPTURING("nemo_hip_eos_register");
     NEED_PTR(eos);
-    if (eos->neos != NEMO_NP_SEOS)
-        return set_error(NEMO_HIP_ERR_UNSUPPORTED, "eos_register: neos = %d (only the simplified EOS, neos = %d)",
-                         eos->neos, NEMO_NP_SEOS);
+    if (eos->neos != NEMO_NP_SEOS && eos->neos != NEMO_NP_TEOS10 && eos->neos != NEMO_NP_EOS80)
+        return set_error(NEMO_HIP_ERR_ARG, "eos_register: neos = %d (TEOS-10 %d, EOS-80 %d or S-EOS %d)", eos->neos,
+                         NEMO_NP_TEOS10, NEMO_NP_EOS80, NEMO_NP_SEOS);
+    g_c->neos = eos->neos;
     g_c->eos = SeosPar{eos->rn_a0, eos->rn_b0, eos->rn_lambda1, eos->rn_lambda2, eos->rn_mu1, eos->rn_mu2, eos->rn_nu};
     g_c->have_eos = true;
     return NEMO_HIP_OK;

@@ -55,8 +55,8 @@ int launch_dyn_nxt(int euler, T *ub, T *vb, T *un, T *vn, T *ua, T *va, const do
 struct SeosPar {   // simplified EOS coefficients, TRA/eosbn2.F90:89-95
     double a0, b0, lambda1, lambda2, mu1, mu2, nu;
 };
-template <typename T> int launch_eos(const SeosPar &e, int pot, const T *pts, T *prd, T *prhop, const T *pdep);
-template <typename T> int launch_rab(const SeosPar &e, const T *pts, T *pab, const T *gdept);
+template <typename T> int launch_eos(int neos, const SeosPar &e, int pot, const T *pts, T *prd, T *prhop, const T *pdep);
+template <typename T> int launch_rab(int neos, const SeosPar &e, const T *pts, T *pab, const T *gdept);
 template <typename T> int launch_bn2(const T *pts, const T *pab, T *pn2, const T *gdept, const T *gdepw);
 template <typename T> int launch_hpg_zco(const T *rhd, T *ua, T *va);
 template <typename T>

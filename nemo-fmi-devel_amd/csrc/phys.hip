@@ -1,11 +1,91 @@
-// phys.hip -- density (the simplified equation of state of TRA/eosbn2.F90),
+// phys.hip -- density (the polynomial TEOS-10 / EOS-80 and the simplified
+// equations of state of TRA/eosbn2.F90),
 // the hydrostatic pressure gradient in z-coordinates (DYN/dynhpg.F90) and
 // the iso-level laplacian viscosity (DYN/dynldf_lap_blp.F90).
 //
 // Arithmetic is FP64 in both storage forms, rounded once on store, in the
 // reference's operation order, so the FP64 form reproduces it bit for bit.
 #include "common.hpp"
+#include "eos_poly.h"
 #include "kernels.hpp"
+
+// {TEOS-10, EOS-80} coefficient tables; a kernel reads one scheme with
+// compile-time offsets, so the loads are wave-uniform scalar loads
+__constant__ const nemo_eos_poly_t EOS_POLY[2] = NEMO_EOS_POLY_TABLES;
+
+// one depth coefficient (zn0..zn3) of TRA/eosbn2.F90:246-266 / 600-637:
+// total degree N in (zs, zt), nested over zt; each zt-power's salinity
+// polynomial c0 + zs (c1 + zs (c2 + ...)) adds its constant term last
+template <int N>
+__device__ __forceinline__ double poly_st(const double (&c)[7][7], double zt, double zs)
+{
+    double acc = c[N][0];
+#pragma unroll
+    for (int j = N - 1; j >= 0; j--) {
+        double ts = c[j][N - j];
+#pragma unroll
+        for (int i = N - j - 1; i >= 1; i--) ts = ts * zs + c[j][i];
+        acc = acc * zt + ts * zs + c[j][0];
+    }
+    return acc;
+}
+
+// zn = ((zn3 zh + zn2) zh + zn1) zh + zn0 (TRA/eosbn2.F90:268), degrees
+// (D0, D1, D2, D3) of the four depth coefficients
+template <int D0, int D1, int D2, int D3>
+__device__ __forceinline__ double poly_z(const double (&c)[4][7][7], double zt, double zs, double zh, double &zn0)
+{
+    const double z3 = poly_st<D3>(c[3], zt, zs), z2 = poly_st<D2>(c[2], zt, zs);
+    const double z1 = poly_st<D1>(c[1], zt, zs);
+    zn0 = poly_st<D0>(c[0], zt, zs);
+    return ((z3 * zh + z2) * zh + z1) * zh + zn0;
+}
+
+// ---------------------------------------------------------------------------
+// eos_insitu (POT = 0), TRA/eosbn2.F90:236-274, and eos_insitu_pot (POT = 1),
+// lines 399-437: polynomial TEOS-10 (SCH = 0) / EOS-80 (SCH = 1).  One
+// thread per cell, levels 1..jpkm1.
+// ---------------------------------------------------------------------------
+template <typename T, int POT, int SCH>
+KERNEL void k_eos_poly(DevDomain d, const T *__restrict__ pts, T *__restrict__ prd, T *__restrict__ prhop,
+                       const T *__restrict__ pdep)
+{
+    THREAD_IJ(1, 1);
+    const int k = (int)blockIdx.z + 1;
+    if (i > d.jpi || j > d.jpj) return;
+    const nemo_eos_poly_t &P = EOS_POLY[SCH];
+    const uint32_t p = IJK32(i, j, k);
+    const double r1_rau0 = 1.0 / d.rau0;
+    const double zh = (double)at32(pdep, p) * P.r1_Z0;
+    const double zt = (double)at32(pts, p) * P.r1_T0;
+    const double zs = sqrt(fabs((double)at32(pts + d.n3, p) + P.rdeltaS) * P.r1_S0);
+    const double ztm = TMASK(double, i, j, k);
+    double zn0;
+    const double zn = poly_z<6, 4, 2, 1>(P.rho, zt, zs, zh, zn0);
+    if (POT) at32(prhop, p) = (T)(zn0 * ztm);
+    at32(prd, p) = (T)((zn * r1_rau0 - 1.) * ztm);
+}
+
+// rab_3d, TRA/eosbn2.F90:588-643: alpha and beta of the polynomial schemes
+template <typename T, int SCH>
+KERNEL void k_rab_poly(DevDomain d, const T *__restrict__ pts, T *__restrict__ pab, const T *__restrict__ gdept)
+{
+    THREAD_IJ(1, 1);
+    const int k = (int)blockIdx.z + 1;
+    if (i > d.jpi || j > d.jpj) return;
+    const nemo_eos_poly_t &P = EOS_POLY[SCH];
+    const uint32_t p = IJK32(i, j, k);
+    const double r1_rau0 = 1.0 / d.rau0;
+    const double zh = (double)at32(gdept, p) * P.r1_Z0;
+    const double zt = (double)at32(pts, p) * P.r1_T0;
+    const double zs = sqrt(fabs((double)at32(pts + d.n3, p) + P.rdeltaS) * P.r1_S0);
+    const double ztm = TMASK(double, i, j, k);
+    double z0;
+    const double za = poly_z<5, 3, 1, 0>(P.alp, zt, zs, zh, z0);
+    at32(pab, p) = (T)(za * r1_rau0 * ztm);
+    const double zb = poly_z<5, 3, 1, 0>(P.bet, zt, zs, zh, z0);
+    at32(pab + d.n3, p) = (T)(zb / zs * r1_rau0 * ztm);
+}
 
 // ---------------------------------------------------------------------------
 // eos_insitu (POT = 0), TRA/eosbn2.F90:276-299, and eos_insitu_pot (POT = 1),
@@ -169,23 +249,33 @@ KERNEL void k_dyn_ldf_lap(DevDomain d, double zsign, const T *__restrict__ pub, 
 // launchers
 // ---------------------------------------------------------------------------
 template <typename T>
-int launch_eos(const SeosPar &e, int pot, const T *pts, T *prd, T *prhop, const T *pdep)
+int launch_eos(int neos, const SeosPar &e, int pot, const T *pts, T *prd, T *prhop, const T *pdep)
 {
     const DevDomain &d = nh::dom();
     Grid2 g = grid2(d.jpi, d.jpj);
     g.grid.z = (unsigned)(d.jpk - 1);
-    if (pot) LAUNCH("eos", (k_eos<T, 1>), g, d, e, pts, prd, prhop, pdep);
-    else LAUNCH("eos", (k_eos<T, 0>), g, d, e, pts, prd, prhop, pdep);
+    if (neos == NEMO_NP_TEOS10) {
+        if (pot) LAUNCH("eos", (k_eos_poly<T, 1, NEMO_EOS_POLY_TEOS10>), g, d, pts, prd, prhop, pdep);
+        else LAUNCH("eos", (k_eos_poly<T, 0, NEMO_EOS_POLY_TEOS10>), g, d, pts, prd, prhop, pdep);
+    } else if (neos == NEMO_NP_EOS80) {
+        if (pot) LAUNCH("eos", (k_eos_poly<T, 1, NEMO_EOS_POLY_EOS80>), g, d, pts, prd, prhop, pdep);
+        else LAUNCH("eos", (k_eos_poly<T, 0, NEMO_EOS_POLY_EOS80>), g, d, pts, prd, prhop, pdep);
+    } else {
+        if (pot) LAUNCH("eos", (k_eos<T, 1>), g, d, e, pts, prd, prhop, pdep);
+        else LAUNCH("eos", (k_eos<T, 0>), g, d, e, pts, prd, prhop, pdep);
+    }
     return 0;
 }
 
 template <typename T>
-int launch_rab(const SeosPar &e, const T *pts, T *pab, const T *gdept)
+int launch_rab(int neos, const SeosPar &e, const T *pts, T *pab, const T *gdept)
 {
     const DevDomain &d = nh::dom();
     Grid2 g = grid2(d.jpi, d.jpj);
     g.grid.z = (unsigned)(d.jpk - 1);
-    LAUNCH("rab", k_rab<T>, g, d, e, pts, pab, gdept);
+    if (neos == NEMO_NP_TEOS10) LAUNCH("rab", (k_rab_poly<T, NEMO_EOS_POLY_TEOS10>), g, d, pts, pab, gdept);
+    else if (neos == NEMO_NP_EOS80) LAUNCH("rab", (k_rab_poly<T, NEMO_EOS_POLY_EOS80>), g, d, pts, pab, gdept);
+    else LAUNCH("rab", k_rab<T>, g, d, e, pts, pab, gdept);
     return 0;
 }
 
@@ -220,8 +310,8 @@ int launch_dyn_ldf_lap(double zsign, const T *pub, const T *pvb, T *pua, T *pva,
 }
 
 #define INST(T)                                                                                      \
-    template int launch_eos<T>(const SeosPar &, int, const T *, T *, T *, const T *);                 \
-    template int launch_rab<T>(const SeosPar &, const T *, T *, const T *);                           \
+    template int launch_eos<T>(int, const SeosPar &, int, const T *, T *, T *, const T *);            \
+    template int launch_rab<T>(int, const SeosPar &, const T *, T *, const T *);                      \
     template int launch_bn2<T>(const T *, const T *, T *, const T *, const T *);                      \
     template int launch_hpg_zco<T>(const T *, T *, T *);                                              \
     template int launch_dyn_ldf_lap<T>(double, const T *, const T *, T *, T *, const T *, const T *);

@@ -390,7 +390,9 @@ CONTAINS
    END SUBROUTINE nemo_hip_bind_nxt
 
    SUBROUTINE nemo_hip_eos_init( seos )
-      !! eos_init (TRA/eosbn2.F90:1218): the simplified equation of state
+      !! eos_init (TRA/eosbn2.F90:1218): seos%neos selects the polynomial
+      !! TEOS-10 (-1) or EOS-80 (0), or the simplified EOS (1) with the
+      !! nameos parameters in the rn_* fields
       TYPE(nemo_hip_eos_t), INTENT(in) :: seos
       CALL check( c_eos_register( seos ), 'eos_init' )
    END SUBROUTINE nemo_hip_eos_init

@@ -85,3 +85,17 @@ def seos(**kw):
     for n in EOS_REAL_FIELDS:
         setattr(e, n, float(v[n]))
     return e
+
+
+def teos10():
+    """An EosStruct selecting the polynomial TEOS-10 (ln_TEOS10, TRA/eosbn2.F90:1266)."""
+    e = seos()
+    e.neos = NP_TEOS10
+    return e
+
+
+def eos80():
+    """An EosStruct selecting the polynomial EOS-80 (ln_EOS80, TRA/eosbn2.F90:1457)."""
+    e = seos()
+    e.neos = NP_EOS80
+    return e

@@ -48,6 +48,7 @@
 #include <string.h>
 
 #include "../include/nemo_hip.h"
+#include "../nemo-fmi-devel_amd/csrc/eos_poly.h"
 
 /* ------------------------------------------------------------------------ */
 static nemo_hip_domain_t D;               /* registered domain (caller owns arrays) */
@@ -1963,19 +1964,82 @@ int ora_dyn_nxt(int kt, double *ub, double *vb, double *un, double *vn, double *
 /* equation of state (simplified EOS), hpg_zco, dyn_ldf_lap                 */
 /* ------------------------------------------------------------------------ */
 static nemo_hip_eos_t EOS;
+static const nemo_eos_poly_t EOS_POLY[2] = NEMO_EOS_POLY_TABLES;
 
-/* eos_init, TRA/eosbn2.F90:1218-1700 (the S-EOS branch only) */
+/* eos_init, TRA/eosbn2.F90:1218-1700: S-EOS keeps the namelist parameters;
+   TEOS-10 and EOS-80 select the coefficient tables of lines 1266-1647 */
 int ora_eos_register(const nemo_hip_eos_t *e)
 {
-    if (!e || e->neos != NEMO_NP_SEOS) return 3;
+    if (!e || (e->neos != NEMO_NP_SEOS && e->neos != NEMO_NP_TEOS10 && e->neos != NEMO_NP_EOS80)) return 3;
     EOS = *e;
     return 0;
+}
+
+/* the coefficient table of a polynomial scheme (0 TEOS-10, 1 EOS-80) */
+int ora_eos_poly_table(int scheme, double *out)
+{
+    if (scheme < 0 || scheme > 1) return 3;
+    memcpy(out, &EOS_POLY[scheme], sizeof(nemo_eos_poly_t));
+    return 0;
+}
+
+static const nemo_eos_poly_t *poly_tab(void)
+{
+    return EOS.neos == NEMO_NP_TEOS10 ? &EOS_POLY[0] : EOS.neos == NEMO_NP_EOS80 ? &EOS_POLY[1] : NULL;
+}
+
+/* one depth coefficient (zn0..zn3) of TRA/eosbn2.F90:246-266 / 600-616:
+   total degree n in (zs, zt), nested over zt; each zt-power's salinity
+   polynomial c0 + zs (c1 + zs (c2 + ...)) adds its constant term last */
+static double poly_st(const double c[7][7], int n, double zt, double zs)
+{
+    double acc = c[n][0];
+    for (int j = n - 1; j >= 0; j--) {
+        double ts = c[j][n - j];
+        for (int i = n - j - 1; i >= 1; i--) ts = ts * zs + c[j][i];
+        acc = acc * zt + ts * zs + c[j][0];
+    }
+    return acc;
+}
+
+/* zn = ((zn3 zh + zn2) zh + zn1) zh + zn0, TRA/eosbn2.F90:268 */
+static double poly_z(const double c[4][7][7], const int deg[4], double zt, double zs, double zh, double *zn0)
+{
+    const double z0 = poly_st(c[0], deg[0], zt, zs), z1 = poly_st(c[1], deg[1], zt, zs);
+    const double z2 = poly_st(c[2], deg[2], zt, zs), z3 = poly_st(c[3], deg[3], zt, zs);
+    if (zn0) *zn0 = z0;
+    return ((z3 * zh + z2) * zh + z1) * zh + z0;
+}
+
+static const int DEG_RHO[4] = {6, 4, 2, 1}, DEG_AB[4] = {5, 3, 1, 0};
+
+/* eos_insitu / eos_insitu_pot, TRA/eosbn2.F90:236-274 and 399-437 (np_teos10, np_eos80) */
+static void poly_insitu(const nemo_eos_poly_t *P, const double *pts, double *prd, double *prhop, const double *pdep)
+{
+    int i, j, k;
+    const double r1_rau0 = 1.0 / D.rau0;
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                const double zh = A3(pdep, i, j, k) * P->r1_Z0;
+                const double zt = A4(pts, i, j, k, 1) * P->r1_T0;
+                const double zs = sqrt(fabs(A4(pts, i, j, k, 2) + P->rdeltaS) * P->r1_S0);
+                const double ztm = TMASK(i, j, k);
+                double zn0;
+                const double zn = poly_z(P->rho, DEG_RHO, zt, zs, zh, &zn0);
+                if (prhop) A3(prhop, i, j, k) = zn0 * ztm;
+                A3(prd, i, j, k) = (zn * r1_rau0 - 1.) * ztm;
+            }
 }
 
 /* eos_insitu, TRA/eosbn2.F90:276-299 (np_seos) */
 int ora_eos_insitu(const double *pts, double *prd, const double *pdep)
 {
     int i, j, k;
+    if (poly_tab()) {
+        poly_insitu(poly_tab(), pts, prd, NULL, pdep);
+        return 0;
+    }
     const double r1_rau0 = 1.0 / D.rau0;
     for (k = 1; k <= jpkm1; k++)
         for (j = 1; j <= jpj; j++)
@@ -1996,6 +2060,10 @@ int ora_eos_insitu(const double *pts, double *prd, const double *pdep)
 int ora_eos_insitu_pot(const double *pts, double *prd, double *prhop, const double *pdep)
 {
     int i, j, k;
+    if (poly_tab()) {
+        poly_insitu(poly_tab(), pts, prd, prhop, pdep);
+        return 0;
+    }
     const double r1_rau0 = 1.0 / D.rau0;
     for (k = 1; k <= jpkm1; k++)
         for (j = 1; j <= jpj; j++)
@@ -2013,11 +2081,25 @@ int ora_eos_insitu_pot(const double *pts, double *prd, double *prhop, const doub
     return 0;
 }
 
-/* rab_3d, TRA/eosbn2.F90:645-662 (np_seos) */
+/* rab_3d, TRA/eosbn2.F90:588-643 (np_teos10, np_eos80) and 645-662 (np_seos) */
 int ora_rab_3d(const double *pts, double *pab, const double *gdept)
 {
     int i, j, k;
     const double r1_rau0 = 1.0 / D.rau0;
+    const nemo_eos_poly_t *P = poly_tab();
+    if (P) {
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 1; j <= jpj; j++)
+                for (i = 1; i <= jpi; i++) {
+                    const double zh = A3(gdept, i, j, k) * P->r1_Z0;
+                    const double zt = A4(pts, i, j, k, 1) * P->r1_T0;
+                    const double zs = sqrt(fabs(A4(pts, i, j, k, 2) + P->rdeltaS) * P->r1_S0);
+                    const double ztm = TMASK(i, j, k);
+                    A4(pab, i, j, k, 1) = poly_z(P->alp, DEG_AB, zt, zs, zh, NULL) * r1_rau0 * ztm;
+                    A4(pab, i, j, k, 2) = poly_z(P->bet, DEG_AB, zt, zs, zh, NULL) / zs * r1_rau0 * ztm;
+                }
+        return 0;
+    }
     for (k = 1; k <= jpkm1; k++)
         for (j = 1; j <= jpj; j++)
             for (i = 1; i <= jpi; i++) {

@@ -152,6 +152,16 @@ class Oracle:
         """eos: abi.EosStruct"""
         _chk(self.L.ora_eos_register(C.byref(eos)), "eos_register")
 
+    def eos_poly_table(self, scheme):
+        """The coefficient table of polynomial scheme 0 (TEOS-10) or 1 (EOS-80):
+        dict of rdeltaS, r1_S0, r1_T0, r1_Z0 and rho/alp/bet arrays [k][j][i]."""
+        buf = np.zeros(4 + 3 * 196)
+        _chk(self.L.ora_eos_poly_table(C.c_int(scheme), _p(buf)), "eos_poly_table")
+        t = dict(zip(["rdeltaS", "r1_S0", "r1_T0", "r1_Z0"], buf[:4]))
+        for n, name in enumerate(["rho", "alp", "bet"]):
+            t[name] = buf[4 + 196 * n: 4 + 196 * (n + 1)].reshape(4, 7, 7).copy()
+        return t
+
     def eos_insitu(self, pts, prd, pdep):
         _chk(self.L.ora_eos_insitu(_p(pts), _p(prd), _p(pdep)), "eos_insitu")
 

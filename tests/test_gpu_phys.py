@@ -86,6 +86,53 @@ def test_eos(hip, case, sp):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("case", ["gyre", "basin", "cyc7"])
+@pytest.mark.parametrize("scheme", ["teos10", "eos80"])
+@pytest.mark.parametrize("sp", [False, True])
+def test_eos_poly(hip, case, scheme, sp):
+    """Polynomial TEOS-10 / EOS-80: eos_insitu, eos_insitu_pot and rab_3d
+    against the oracle, then N^2 from the kernel's own ratios."""
+    e = Env(hip, case)
+    mk = abi.teos10 if scheme == "teos10" else abi.eos80
+    hip.eos_register(mk())
+    e.o.eos_register(mk())
+    rng = np.random.default_rng(72)
+    ts = tracers(rng, e, sp)
+    gdept, gdepw = configs.depths(e.d)
+    if sp:
+        gdept = np.asfortranarray(gdept.astype(np.float32).astype(np.float64))
+        gdepw = np.asfortranarray(gdepw.astype(np.float32).astype(np.float64))
+    prd = rnd(rng, e.s3, sp)
+    want = prd.copy(order="F")
+    e.o.eos_insitu(ts, want, gdept)
+    gp = g(prd, sp)
+    hip.eos_insitu(g(ts, sp), gp, g(gdept, sp), sp=sp)
+    want2, wantp = prd.copy(order="F"), prd.copy(order="F")
+    e.o.eos_insitu_pot(ts, want2, wantp, gdept)
+    gp2, gpp = g(prd, sp), g(prd, sp)
+    hip.eos_insitu_pot(g(ts, sp), gp2, gpp, g(gdept, sp), sp=sp)
+    pab = rnd(rng, e.s4, sp)
+    wab = pab.copy(order="F")
+    e.o.rab_3d(ts, wab, gdept)
+    gab = g(pab, sp)
+    hip.rab_3d(g(ts, sp), gab, g(gdept, sp), sp=sp)
+    sync(hip)
+    cmp(host(gp, e.s3), want, sp, f"{scheme} eos_insitu {case}")
+    cmp(host(gp2, e.s3), want2, sp, f"{scheme} eos_insitu_pot prd {case}")
+    cmp(host(gpp, e.s3), wantp, sp, f"{scheme} eos_insitu_pot prhop {case}")
+    cmp(host(gab, e.s4), wab, sp, f"{scheme} rab_3d {case}")
+    if not sp:
+        pn2 = rnd(rng, e.s3, sp)
+        wn2 = pn2.copy(order="F")
+        e.o.bn2(ts, wab, wn2, gdept, gdepw)
+        gn2 = g(pn2, sp)
+        hip.bn2(g(ts, sp), gab, gn2, g(gdept, sp), g(gdepw, sp), sp=sp)
+        sync(hip)
+        cmp(host(gn2, e.s3), wn2, sp, f"{scheme} bn2 {case}")
+    hip.eos_register(abi.seos())
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("case", ["basin", "cyc7"])
 def test_bn2_fp32_against_same_ratios(hip, case):
     """bn2 in FP32 storage on ratios both sides read as the same floats."""
@@ -166,7 +213,7 @@ def test_phys_host_pointer_mode_and_errors(hip):
         hip.set_pointer_mode(NEMO_HIP_PTR_DEVICE)
     with pytest.raises(nemo_hip.NemoHipError, match="neos"):
         bad = abi.seos()
-        bad.neos = abi.NP_TEOS10
+        bad.neos = 5
         hip.eos_register(bad)
     x3 = dev(np.zeros(e.s3))
     with pytest.raises(nemo_hip.NemoHipError, match="kpass"):

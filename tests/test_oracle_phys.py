@@ -314,3 +314,154 @@ def test_tra_sbc_closed_forms():
     tsa3 = run(1, qns, qsr, sfx, emp, sbc, sbc_b)
     for n in range(2):
         assert np.allclose(tsa3[R + (0, n)], 2.0e-5 / rau0 * tsn[R + (0, n)] / e3[R], rtol=1e-14)
+
+
+# ----------------------------------------------------------------------------
+# polynomial TEOS-10 / EOS-80 (TRA/eosbn2.F90:236-274, 399-437, 588-643)
+# ----------------------------------------------------------------------------
+POLY = [("teos10", abi.teos10, 0, 1028.21993233072), ("eos80", abi.eos80, 1, 1028.35011066567)]
+DEG_RHO, DEG_AB = (6, 4, 2, 1), (5, 3, 1, 0)
+
+
+@pytest.mark.parametrize("name,mk,scheme,check", POLY, ids=[p[0] for p in POLY])
+def test_poly_eos_check_value(name, mk, scheme, check):
+    """eos_insitu at z = 3000, T = 3, S = 35.5 gives the check value the
+    reference documents (TRA/eosbn2.F90:205 and 208, quoted to 1e-11 kg/m3);
+    every monomial is nonzero there, so a misplaced or mistyped coefficient
+    above that size fails."""
+    d = domain()
+    t, *_ = configs.masks(d)
+    s3 = (d.jpi, d.jpj, d.jpk)
+    ts = np.zeros(s3 + (2,), order="F")
+    ts[..., 0], ts[..., 1] = 3.0, 35.5
+    o = Oracle(d)
+    o.eos_register(mk())
+    prd = np.full(s3, 7.0, order="F")
+    o.eos_insitu(ts, prd, np.full(s3, 3000.0, order="F"))
+    wet = t[:, :, :-1] > 0
+    rho = (prd[:, :, :-1][wet] + 1.0) * d.opts["rau0"]
+    assert np.abs(rho - check).max() < 1e-10
+    assert np.all(prd[:, :, :-1][~wet] == 0.0)
+    assert np.all(prd[:, :, -1] == 7.0)          # level jpk untouched
+
+
+@pytest.mark.parametrize("name,mk,scheme,check", POLY, ids=[p[0] for p in POLY])
+def test_poly_eos_potential_density(name, mk, scheme, check):
+    """eos_insitu_pot: its prd is eos_insitu's; prhop does not depend on
+    depth and is the in situ density at the surface."""
+    d = domain()
+    t, *_ = configs.masks(d)
+    s3 = (d.jpi, d.jpj, d.jpk)
+    rng = np.random.default_rng(21)
+    ts = np.zeros(s3 + (2,), order="F")
+    ts[..., 0] = rng.uniform(-2, 30, s3)
+    ts[..., 1] = rng.uniform(30, 40, s3)
+    dep = np.asfortranarray(rng.uniform(0, 5000, s3))
+    o = Oracle(d)
+    o.eos_register(mk())
+    a, b, p = fz(s3), fz(s3), fz(s3)
+    o.eos_insitu(ts, a, dep)
+    o.eos_insitu_pot(ts, b, p, dep)
+    assert np.array_equal(a, b)
+    a0, p0 = fz(s3), fz(s3)
+    o.eos_insitu_pot(ts, a0, p0, fz(s3))
+    assert np.array_equal(p0, p)
+    r1 = 1.0 / d.opts["rau0"]
+    assert np.array_equal(a0[:, :, :-1], (p0[:, :, :-1] * r1 - 1.0) * t[:, :, :-1])
+    assert np.all((p[:, :, :-1][t[:, :, :-1] > 0] > 990) & (p[:, :, :-1][t[:, :, :-1] > 0] < 1035))
+
+
+@pytest.mark.parametrize("name,mk,scheme,check", POLY, ids=[p[0] for p in POLY])
+def test_poly_rab_is_density_derivative(name, mk, scheme, check):
+    """rab_3d's alpha = -1/rau0 drho/dT and beta = 1/rau0 drho/dS of
+    eos_insitu (central differences, h = 1e-3; the alp/bet tables are the
+    derivative tables rounded to 11 digits)."""
+    d = domain()
+    t, *_ = configs.masks(d)
+    s3 = (d.jpi, d.jpj, d.jpk)
+    rng = np.random.default_rng(22)
+    ts = np.zeros(s3 + (2,), order="F")
+    ts[..., 0] = rng.uniform(-2, 30, s3)
+    ts[..., 1] = rng.uniform(30, 40, s3)
+    dep = np.asfortranarray(rng.uniform(0, 5000, s3))
+    o = Oracle(d)
+    o.eos_register(mk())
+    pab = np.zeros(s3 + (2,), order="F")
+    o.rab_3d(ts, pab, dep)
+    h = 1e-3
+
+    def rd(dt, dsal):
+        x = ts.copy(order="F")
+        x[..., 0] += dt
+        x[..., 1] += dsal
+        r = fz(s3)
+        o.eos_insitu(x, r, dep)
+        return r
+
+    alpha = -(rd(h, 0) - rd(-h, 0)) / (2 * h)
+    beta = (rd(0, h) - rd(0, -h)) / (2 * h)
+    sl = (slice(None), slice(None), slice(0, -1))
+    assert np.allclose(pab[..., 0][sl], alpha[sl], rtol=1e-7, atol=1e-12)
+    assert np.allclose(pab[..., 1][sl], beta[sl], rtol=1e-7, atol=1e-12)
+    assert np.all(pab[..., 1][sl][t[sl] > 0] > 5e-4)      # haline contraction ~7.5e-4
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_poly_tables_are_derivative_tables(scheme):
+    """The alp and bet tables are -d/dT and 2 zs d/dS of the rho table in the
+    scaled variables (alp_ijk = -(j+1) rho_i(j+1)k r1_T0, bet_ijk = (i+1)
+    rho_(i+1)jk r1_S0 / 2), to the 11 digits they are tabulated with, and
+    every table is zero beyond its degree."""
+    d = domain()
+    T = Oracle(d).eos_poly_table(scheme)
+    rho, alp, bet = T["rho"], T["alp"], T["bet"]
+    for k in range(4):
+        for j in range(7):
+            for i in range(7):
+                if i + j > DEG_RHO[k]:
+                    assert rho[k, j, i] == 0.0
+                if i + j > DEG_AB[k]:
+                    assert alp[k, j, i] == 0.0 and bet[k, j, i] == 0.0
+                    continue
+                da = -(j + 1) * rho[k, j + 1, i] * T["r1_T0"]
+                db = (i + 1) * rho[k, j, i + 1] * T["r1_S0"] / 2
+                assert abs(alp[k, j, i] - da) <= 2e-10 * abs(da) + 1e-14, (k, j, i)
+                assert abs(bet[k, j, i] - db) <= 2e-10 * abs(db) + 1e-14, (k, j, i)
+    assert T["r1_T0"] == 1 / 40 and T["r1_Z0"] == 1e-4
+
+
+REF_EOS = "/root/reference/src/OCE/TRA/eosbn2.F90"
+
+
+@pytest.mark.skipif(not __import__("os").path.exists(REF_EOS), reason="reference sources not present")
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_poly_tables_match_reference_text(scheme):
+    """Every coefficient eos_init assigns for the scheme equals the table entry
+    (read as text from the reference's source, where it is present)."""
+    import re
+    lines = open(REF_EOS).read().splitlines()
+    start = "CASE( np_teos10 )" if scheme == 0 else "CASE( np_eos80 )"
+    stop = "CASE( np_eos80 )" if scheme == 0 else "CASE( np_seos )"
+    got, on = {}, False
+    for ln in lines[1200:]:
+        if start in ln:
+            on = True
+            continue
+        if on and stop in ln:
+            break
+        m = on and re.match(r"\s*(EOS|ALP|BET|rdeltaS|r1_S0)(\d?)(\d?)(\d?)\s*=\s*([-+0-9.eE]+)_wp(\s*/\s*([-+0-9.eE]+)_wp)?", ln)
+        if m:
+            v = float(m.group(5)) / (float(m.group(7)) if m.group(7) else 1.0)
+            got[(m.group(1),) + tuple(int(c) for c in m.group(2, 3, 4) if c)] = v
+    T = Oracle(domain()).eos_poly_table(scheme)
+    names = {"EOS": "rho", "ALP": "alp", "BET": "bet"}
+    n = 0
+    for key, v in got.items():
+        if key[0] in names:
+            i, j, k = key[1:]
+            assert T[names[key[0]]][k, j, i] == v, key
+            n += 1
+    assert n == 122
+    assert T["rdeltaS"] == got[("rdeltaS",)] and T["r1_S0"] == got[("r1_S0",)]
+    nz = sum(int(np.count_nonzero(T[x])) for x in names.values())
+    assert nz == 122
