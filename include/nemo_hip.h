@@ -340,6 +340,23 @@ int nemo_hip_rab_3d(const double *pts, double *pab, const double *gdept);
 /* bn2, TRA/eosbn2.F90:889-933: N^2 at w-points, levels 2..jpkm1 */
 int nemo_hip_bn2(const double *pts, const double *pab, double *pn2,
                  const double *gdept, const double *gdepw);
+/* zdf_mxl, ZDF/zdfmxl.F90:64-140: the mixed-layer base level nmln
+   (jpi,jpj), its w-depth hmlp and the depth hmlpt of its last t-point, from
+   rn2b (jpi,jpj,jpk) and the rho_c density criterion (0.01 in the
+   reference, ZDF/zdfmxl.F90:34); nlb10 is the shallowest w-level below 10 m
+   (DOM/domzgr.F90:144-145).  The turbocline depth hmld (a diagnostic) is
+   not computed.  2-D outputs are FP64 in both storage forms. */
+int nemo_hip_zdf_mxl(int kt, const double *rn2b, const double *gdept, const double *gdepw, int nlb10, double rho_c,
+                     int32_t *nmln, double *hmlp, double *hmlpt);
+/* ldf_slp, LDF/ldfslp.F90:83-374 with ldf_slp_mxl (620-729): iso-neutral
+   slopes uslp, vslp, wslpi, wslpj (jpi,jpj,jpk) from the in situ density
+   prd and N^2 pn2 (the reference passes rhd and rn2b, OCE/step.F90:150),
+   with the mixed layer of nemo_hip_zdf_mxl and the nam_traldf limit
+   rn_slpmax.  Levels 2..jpkm1 are computed; levels 1 and jpk keep their
+   interior values and every level's ring is set by lbc_lnk. */
+int nemo_hip_ldf_slp(int kt, const double *prd, const double *pn2, const double *gdept, const double *gdepw,
+                     const int32_t *nmln, const double *hmlp, const double *hmlpt, double rn_slpmax, double *uslp,
+                     double *vslp, double *wslpi, double *wslpj);
 /* hpg_zco, DYN/dynhpg.F90:258-323 (dyn_hpg with ln_hpg_zco): the hydrostatic
    pressure gradient of the in situ density anomaly rhd added to ua, va */
 int nemo_hip_hpg_zco(int kt, const double *rhd, double *ua, double *va);
@@ -477,6 +494,11 @@ int nemo_hip_step_tra_sp(int kt, const float *un, const float *vn, const float *
    arithmetic is FP64, rounded once on store */
 int nemo_hip_eos_insitu_sp(const float *pts, float *prd, const float *pdep);
 int nemo_hip_eos_insitu_pot_sp(const float *pts, float *prd, float *prhop, const float *pdep);
+int nemo_hip_zdf_mxl_sp(int kt, const float *rn2b, const float *gdept, const float *gdepw, int nlb10, double rho_c,
+                        int32_t *nmln, double *hmlp, double *hmlpt);
+int nemo_hip_ldf_slp_sp(int kt, const float *prd, const float *pn2, const float *gdept, const float *gdepw,
+                        const int32_t *nmln, const double *hmlp, const double *hmlpt, double rn_slpmax, float *uslp,
+                        float *vslp, float *wslpi, float *wslpj);
 int nemo_hip_rab_3d_sp(const float *pts, float *pab, const float *gdept);
 int nemo_hip_bn2_sp(const float *pts, const float *pab, float *pn2,
                     const float *gdept, const float *gdepw);

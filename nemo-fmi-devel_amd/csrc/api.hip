@@ -1275,6 +1275,45 @@ static int do_dyn_ldf_lap(int kt, const T *pub, const T *pvb, T *pua, T *pva, in
     return S.finish(rc);
 }
 
+// zdf_mxl (nmln, hmlp, hmlpt) and ldf_slp
+template <typename T>
+static int do_zdf_mxl(int kt, const T *rn2b, const T *gdept, const T *gdepw, int nlb10, double rho_c,
+                      int32_t *nmln, double *hmlp, double *hmlpt)
+{
+    (void)kt;
+    NEED_DOMAIN();
+    NEED_PTR(rn2b); NEED_PTR(gdept); NEED_PTR(gdepw); NEED_PTR(nmln); NEED_PTR(hmlp); NEED_PTR(hmlpt);
+    if (nlb10 < 2 || nlb10 > g_c->dom.jpk - 1)
+        return set_error(NEMO_HIP_ERR_ARG, "zdf_mxl: nlb10 = %d outside 2..jpk-1 = %d", nlb10, g_c->dom.jpk - 1);
+    const size_t n2 = g_c->dom.n2, n3 = g_c->dom.n3;
+    Stager S;
+    const T *r = S.in(rn2b, n3), *gt = S.in(gdept, n3), *gw = S.in(gdepw, n3);
+    int32_t *nm = S.out(nmln, n2);
+    double *h1 = S.out(hmlp, n2), *h2 = S.out(hmlpt, n2);
+    int rc = S.status ? S.status : launch_zdf_mxl<T>(r, gt, gw, nlb10, rho_c, nm, h1, h2);
+    return S.finish(rc);
+}
+
+template <typename T>
+static int do_ldf_slp(int kt, const T *prd, const T *pn2, const T *gdept, const T *gdepw, const int32_t *nmln,
+                      const double *hmlp, const double *hmlpt, double rn_slpmax, T *uslp, T *vslp, T *wslpi,
+                      T *wslpj)
+{
+    (void)kt;
+    NEED_DOMAIN();
+    NEED_PTR(prd); NEED_PTR(pn2); NEED_PTR(gdept); NEED_PTR(gdepw); NEED_PTR(nmln); NEED_PTR(hmlp);
+    NEED_PTR(hmlpt); NEED_PTR(uslp); NEED_PTR(vslp); NEED_PTR(wslpi); NEED_PTR(wslpj);
+    if (!(rn_slpmax > 0.)) return set_error(NEMO_HIP_ERR_ARG, "ldf_slp: rn_slpmax = %g", rn_slpmax);
+    const size_t n2 = g_c->dom.n2, n3 = g_c->dom.n3;
+    Stager S;
+    const T *rd = S.in(prd, n3), *n2p = S.in(pn2, n3), *gt = S.in(gdept, n3), *gw = S.in(gdepw, n3);
+    const int32_t *nm = S.in(nmln, n2);
+    const double *h1 = S.in(hmlp, n2), *h2 = S.in(hmlpt, n2);
+    T *us = S.inout(uslp, n3), *vs = S.inout(vslp, n3), *wi = S.inout(wslpi, n3), *wj = S.inout(wslpj, n3);
+    int rc = S.status ? S.status : launch_ldf_slp<T>(rd, n2p, gt, gw, nm, h1, h2, rn_slpmax, us, vs, wi, wj);
+    return S.finish(rc);
+}
+
 // tra_sbc (linear free surface, no penetrative solar flux)
 template <typename T>
 static int do_tra_sbc(int kt, double *qns, double *qsr, const double *sfx, const double *emp, const T *tsn, T *tsa,
@@ -1500,6 +1539,17 @@ int nemo_hip_eos_insitu_pot(const double *pts, double *prd, double *prhop, const
 {
     return do_eos<double>(1, pts, prd, prhop, pdep);
 }
+int nemo_hip_zdf_mxl(int kt, const double *rn2b, const double *gdept, const double *gdepw, int nlb10, double rho_c,
+                     int32_t *nmln, double *hmlp, double *hmlpt)
+{
+    return do_zdf_mxl<double>(kt, rn2b, gdept, gdepw, nlb10, rho_c, nmln, hmlp, hmlpt);
+}
+int nemo_hip_ldf_slp(int kt, const double *prd, const double *pn2, const double *gdept, const double *gdepw,
+                     const int32_t *nmln, const double *hmlp, const double *hmlpt, double rn_slpmax, double *uslp,
+                     double *vslp, double *wslpi, double *wslpj)
+{
+    return do_ldf_slp<double>(kt, prd, pn2, gdept, gdepw, nmln, hmlp, hmlpt, rn_slpmax, uslp, vslp, wslpi, wslpj);
+}
 int nemo_hip_rab_3d(const double *pts, double *pab, const double *gdept) { return do_rab<double>(pts, pab, gdept); }
 int nemo_hip_bn2(const double *pts, const double *pab, double *pn2, const double *gdept, const double *gdepw)
 {
@@ -1623,6 +1673,17 @@ int nemo_hip_eos_insitu_sp(const float *pts, float *prd, const float *pdep)
 int nemo_hip_eos_insitu_pot_sp(const float *pts, float *prd, float *prhop, const float *pdep)
 {
     return do_eos<float>(1, pts, prd, prhop, pdep);
+}
+int nemo_hip_zdf_mxl_sp(int kt, const float *rn2b, const float *gdept, const float *gdepw, int nlb10, double rho_c,
+                        int32_t *nmln, double *hmlp, double *hmlpt)
+{
+    return do_zdf_mxl<float>(kt, rn2b, gdept, gdepw, nlb10, rho_c, nmln, hmlp, hmlpt);
+}
+int nemo_hip_ldf_slp_sp(int kt, const float *prd, const float *pn2, const float *gdept, const float *gdepw,
+                        const int32_t *nmln, const double *hmlp, const double *hmlpt, double rn_slpmax, float *uslp,
+                        float *vslp, float *wslpi, float *wslpj)
+{
+    return do_ldf_slp<float>(kt, prd, pn2, gdept, gdepw, nmln, hmlp, hmlpt, rn_slpmax, uslp, vslp, wslpi, wslpj);
 }
 int nemo_hip_rab_3d_sp(const float *pts, float *pab, const float *gdept) { return do_rab<float>(pts, pab, gdept); }
 int nemo_hip_bn2_sp(const float *pts, const float *pab, float *pn2, const float *gdept, const float *gdepw)

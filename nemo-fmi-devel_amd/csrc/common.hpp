@@ -287,6 +287,7 @@ enum Scratch {
 void *scratch3(int which);     // n3 doubles each, slot s+1 directly after slot s
 #define NH_NSCRATCH2 40
 double *scratch2(int which);   // n2 doubles each (dyn_spg_ts work arrays), NH_NSCRATCH2 of them
+#define NH_S2_SLP 34           // slots 34..37: ldf_slp mixed-layer slopes (dyn_spg_ts uses 0..33)
 }  // namespace nh
 
 #define HIPCHK(x)                                                                                  \

@@ -57,6 +57,12 @@ struct SeosPar {   // simplified EOS coefficients, TRA/eosbn2.F90:89-95
 };
 template <typename T> int launch_eos(int neos, const SeosPar &e, int pot, const T *pts, T *prd, T *prhop, const T *pdep);
 template <typename T> int launch_rab(int neos, const SeosPar &e, const T *pts, T *pab, const T *gdept);
+template <typename T>
+int launch_zdf_mxl(const T *rn2b, const T *gdept, const T *gdepw, int nlb10, double rho_c, int32_t *nmln,
+                   double *hmlp, double *hmlpt);
+template <typename T>
+int launch_ldf_slp(const T *prd, const T *pn2, const T *gdept, const T *gdepw, const int32_t *nmln,
+                   const double *hmlp, const double *hmlpt, double rn_slpmax, T *uslp, T *vslp, T *wslpi, T *wslpj);
 template <typename T> int launch_bn2(const T *pts, const T *pab, T *pn2, const T *gdept, const T *gdepw);
 template <typename T> int launch_hpg_zco(const T *rhd, T *ua, T *va);
 template <typename T>

@@ -385,6 +385,22 @@ class NemoHip:
         p = lambda a: self._p(a, dt)
         self._check(fn(p(pts), p(pab), p(pn2), p(gdept), p(gdepw)))
 
+    def zdf_mxl(self, kt, rn2b, gdept, gdepw, nlb10, nmln, hmlp, hmlpt, rho_c=0.01, sp=False):
+        """zdf_mxl: nmln (int32), hmlp, hmlpt (FP64, jpi x jpj) from rn2b."""
+        fn = self.L.nemo_hip_zdf_mxl_sp if sp else self.L.nemo_hip_zdf_mxl
+        dt = np.float32 if sp else np.float64
+        self._check(fn(C.c_int(kt), self._p(rn2b, dt), self._p(gdept, dt), self._p(gdepw, dt), C.c_int(nlb10),
+                       C.c_double(rho_c), self._p(nmln, np.int32), self._p(hmlp), self._p(hmlpt)))
+
+    def ldf_slp(self, kt, prd, pn2, gdept, gdepw, nmln, hmlp, hmlpt, rn_slpmax, uslp, vslp, wslpi, wslpj,
+                sp=False):
+        """ldf_slp: the iso-neutral slopes (updated in place)."""
+        fn = self.L.nemo_hip_ldf_slp_sp if sp else self.L.nemo_hip_ldf_slp
+        dt = np.float32 if sp else np.float64
+        self._check(fn(C.c_int(kt), self._p(prd, dt), self._p(pn2, dt), self._p(gdept, dt), self._p(gdepw, dt),
+                       self._p(nmln, np.int32), self._p(hmlp), self._p(hmlpt), C.c_double(rn_slpmax),
+                       self._p(uslp, dt), self._p(vslp, dt), self._p(wslpi, dt), self._p(wslpj, dt)))
+
     def hpg_zco(self, kt, rhd, ua, va, sp=False):
         fn = self.L.nemo_hip_hpg_zco_sp if sp else self.L.nemo_hip_hpg_zco
         dt = np.float32 if sp else np.float64

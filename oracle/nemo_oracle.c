@@ -2296,6 +2296,189 @@ int ora_tra_sbc(int kt, double *qns, double *qsr, const double *sfx, const doubl
     return 0;
 }
 
+/* ------------------------------------------------------------------------ */
+/* mixed layer and iso-neutral slopes                                       */
+/* ------------------------------------------------------------------------ */
+static double DMIN3(double a, double b, double c) { return fmin(fmin(a, b), c); }
+static double DMAX(double a, double b) { return a > b ? a : b; }
+static double DMIN(double a, double b) { return a < b ? a : b; }
+
+/* zdf_mxl, ZDF/zdfmxl.F90:92-125: nmln, hmlp, hmlpt (the turbocline depth
+   hmld, a diagnostic, is left out) */
+int ora_zdf_mxl(int kt, const double *rn2b, const double *gdept, const double *gdepw, int nlb10, double rho_c,
+                int *nmln, double *hmlp, double *hmlpt)
+{
+    int i, j, k;
+    (void)kt;
+    if (nlb10 < 2 || nlb10 > jpkm1) return 3;
+    const double zN2_c = D.grav * rho_c * (1.0 / D.rau0);
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) {
+            A2(nmln, i, j) = nlb10;
+            A2(hmlp, i, j) = 0.;
+        }
+    for (k = nlb10; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                const int ikt = A2(mbkt, i, j);
+                A2(hmlp, i, j) = A2(hmlp, i, j) + DMAX(A3(rn2b, i, j, k), 0.) * e3w_n(i, j, k);
+                if (A2(hmlp, i, j) < zN2_c) A2(nmln, i, j) = IMIN(k, ikt) + 1;
+            }
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) {
+            const int iikn = A2(nmln, i, j);
+            A2(hmlp, i, j) = A3(gdepw, i, j, iikn) * A2(ssmask, i, j);
+            A2(hmlpt, i, j) = A3(gdept, i, j, iikn - 1) * A2(ssmask, i, j);
+        }
+    return 0;
+}
+
+/* ldf_slp, LDF/ldfslp.F90:83-374, with ldf_slp_mxl (620-729); no partial
+   steps, no ice-shelf cavities (risfdep = 0, mikt = miku = mikv = 1), no
+   s-coordinate level slopes.  Levels 1 and jpk of the slopes keep their
+   interior values; every level's ring is reset by the final lbc_lnk. */
+int ora_ldf_slp(int kt, const double *prd, const double *pn2, const double *gdept, const double *gdepw,
+                const int *nmln, const double *hmlp, const double *hmlpt, double rn_slpmax, double *uslp,
+                double *vslp, double *wslpi, double *wslpj)
+{
+    int i, j, k;
+    (void)kt;
+    const double zeps = 1.e-20, z1_16 = 1.0 / 16., zm1_g = -1.0 / D.grav, zm1_2g = -0.5 / D.grav;
+    const double z1_slpmax = 1. / rn_slpmax;
+    double *zgru = z3(), *zgrv = z3(), *zdzr = z3(), *zwz = z3(), *zww = z3(), *oml = z3();
+    double *uslpml = z2(), *vslpml = z2(), *wslpiml = z2(), *wslpjml = z2();
+    double *zslpml_hmlpu = z2(), *zslpml_hmlpv = z2();
+    for (k = 1; k <= jpk; k++)
+        for (j = 1; j <= jpjm1; j++)
+            for (i = 1; i <= jpim1; i++) {
+                A3(zgru, i, j, k) = UMASK(i, j, k) * (A3(prd, i + 1, j, k) - A3(prd, i, j, k));
+                A3(zgrv, i, j, k) = VMASK(i, j, k) * (A3(prd, i, j + 1, k) - A3(prd, i, j, k));
+            }
+    for (k = 2; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++)
+                A3(zdzr, i, j, k) = zm1_g * (A3(prd, i, j, k) + 1.) * (A3(pn2, i, j, k) + A3(pn2, i, j, k + 1)) *
+                                    (1. - 0.5 * TMASK(i, j, k + 1));
+    /* ldf_slp_mxl */
+    for (k = 1; k <= jpk; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) A3(oml, i, j, k) = (k <= A2(nmln, i, j) - 1) ? 1. : 0.;
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            const int iku = IMIN(IMAX(1, IMAX(A2(nmln, i, j), A2(nmln, i + 1, j))), jpkm1);
+            const int ikv = IMIN(IMAX(1, IMAX(A2(nmln, i, j), A2(nmln, i, j + 1))), jpkm1);
+            double zbu = 0.5 * (A3(zdzr, i, j, iku) + A3(zdzr, i + 1, j, iku));
+            double zbv = 0.5 * (A3(zdzr, i, j, ikv) + A3(zdzr, i, j + 1, ikv));
+            const double zau = A3(zgru, i, j, iku) * r1_e1u(i, j);
+            const double zav = A3(zgrv, i, j, ikv) * r1_e2v(i, j);
+            zbu = DMIN3(zbu, -z1_slpmax * fabs(zau), -7.e+3 / e3u_n(i, j, iku) * fabs(zau));
+            zbv = DMIN3(zbv, -z1_slpmax * fabs(zav), -7.e+3 / e3v_n(i, j, ikv) * fabs(zav));
+            A2(uslpml, i, j) = zau / (zbu - zeps) * UMASK(i, j, iku);
+            A2(vslpml, i, j) = zav / (zbv - zeps) * VMASK(i, j, ikv);
+            const int ik = IMIN(A2(nmln, i, j) + 1, jpk), ikm1 = IMAX(1, ik - 1);
+            const double zbw = zm1_2g * A3(pn2, i, j, ik) * (A3(prd, i, j, ik) + A3(prd, i, j, ikm1) + 2.);
+            const double zci = DMAX(UMASK(i - 1, j, ik) + UMASK(i, j, ik) + UMASK(i - 1, j, ikm1) + UMASK(i, j, ikm1),
+                                    zeps) * e1t(i, j);
+            const double zcj = DMAX(VMASK(i, j - 1, ik) + VMASK(i, j, ik) + VMASK(i, j - 1, ikm1) + VMASK(i, j, ikm1),
+                                    zeps) * e2t(i, j);
+            const double zai = (A3(zgru, i - 1, j, ik) + A3(zgru, i, j, ik) + A3(zgru, i - 1, j, ikm1) +
+                                A3(zgru, i, j, ikm1)) / zci * TMASK(i, j, ik);
+            const double zaj = (A3(zgrv, i, j - 1, ik) + A3(zgrv, i, j, ik) + A3(zgrv, i, j - 1, ikm1) +
+                                A3(zgrv, i, j, ikm1)) / zcj * TMASK(i, j, ik);
+            const double zbi = DMIN3(zbw, -100. * fabs(zai), -7.e+3 / e3w_n(i, j, ik) * fabs(zai));
+            const double zbj = DMIN3(zbw, -100. * fabs(zaj), -7.e+3 / e3w_n(i, j, ik) * fabs(zaj));
+            A2(wslpiml, i, j) = zai / (zbi - zeps) * TMASK(i, j, ik);
+            A2(wslpjml, i, j) = zaj / (zbj - zeps) * TMASK(i, j, ik);
+        }
+    /* I. slopes at u- and v-points */
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            A2(zslpml_hmlpu, i, j) = A2(uslpml, i, j) / DMAX(DMAX(A2(hmlpt, i, j), A2(hmlpt, i + 1, j)), 5.);
+            A2(zslpml_hmlpv, i, j) = A2(vslpml, i, j) / DMAX(DMAX(A2(hmlpt, i, j), A2(hmlpt, i, j + 1)), 5.);
+        }
+    for (k = 2; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                const double zau = A3(zgru, i, j, k) * r1_e1u(i, j);
+                const double zav = A3(zgrv, i, j, k) * r1_e2v(i, j);
+                double zbu = 0.5 * (A3(zdzr, i, j, k) + A3(zdzr, i + 1, j, k));
+                double zbv = 0.5 * (A3(zdzr, i, j, k) + A3(zdzr, i, j + 1, k));
+                zbu = DMIN3(zbu, -z1_slpmax * fabs(zau), -7.e+3 / e3u_n(i, j, k) * fabs(zau));
+                zbv = DMIN3(zbv, -z1_slpmax * fabs(zav), -7.e+3 / e3v_n(i, j, k) * fabs(zav));
+                const double zfi = DMAX(A3(oml, i, j, k), A3(oml, i + 1, j, k));
+                const double zfj = DMAX(A3(oml, i, j, k), A3(oml, i, j + 1, k));
+                const double zdepu = 0.5 * ((A3(gdept, i, j, k) + A3(gdept, i + 1, j, k)) - e3u_n(i, j, A2(miku, i, j)));
+                const double zdepv = 0.5 * ((A3(gdept, i, j, k) + A3(gdept, i, j + 1, k)) - e3v_n(i, j, A2(mikv, i, j)));
+                A3(zwz, i, j, k) = ((1. - zfi) * zau / (zbu - zeps) + zfi * zdepu * A2(zslpml_hmlpu, i, j)) * UMASK(i, j, k);
+                A3(zww, i, j, k) = ((1. - zfj) * zav / (zbv - zeps) + zfj * zdepv * A2(zslpml_hmlpv, i, j)) * VMASK(i, j, k);
+            }
+    lbc3(zwz, jpk, 'U', 0.);
+    lbc3(zww, jpk, 'V', 0.);
+    for (k = 2; k <= jpkm1; k++) {
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+#define SHAPIRO(z)                                                                                         \
+    (A3(z, i - 1, j - 1, k) + A3(z, i + 1, j - 1, k) + A3(z, i - 1, j + 1, k) + A3(z, i + 1, j + 1, k) +    \
+     2. * (A3(z, i, j - 1, k) + A3(z, i - 1, j, k) + A3(z, i + 1, j, k) + A3(z, i, j + 1, k)) + 4. * A3(z, i, j, k))
+                A3(uslp, i, j, k) = z1_16 * SHAPIRO(zwz);
+                A3(vslp, i, j, k) = z1_16 * SHAPIRO(zww);
+            }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A3(uslp, i, j, k) = A3(uslp, i, j, k) * (UMASK(i, j + 1, k) + UMASK(i, j - 1, k)) * 0.5 *
+                                    (UMASK(i, j, k) + UMASK(i, j, k + 1)) * 0.5;
+                A3(vslp, i, j, k) = A3(vslp, i, j, k) * (VMASK(i + 1, j, k) + VMASK(i - 1, j, k)) * 0.5 *
+                                    (VMASK(i, j, k) + VMASK(i, j, k + 1)) * 0.5;
+            }
+    }
+    /* II. slopes at w-points */
+    for (k = 2; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                const double zbw = zm1_2g * A3(pn2, i, j, k) * (A3(prd, i, j, k) + A3(prd, i, j, k - 1) + 2.);
+                const double zci = DMAX(UMASK(i - 1, j, k) + UMASK(i, j, k) + UMASK(i - 1, j, k - 1) + UMASK(i, j, k - 1),
+                                        zeps) * e1t(i, j);
+                const double zcj = DMAX(VMASK(i, j - 1, k) + VMASK(i, j, k - 1) + VMASK(i, j - 1, k - 1) + VMASK(i, j, k),
+                                        zeps) * e2t(i, j);
+                const double zai = (A3(zgru, i - 1, j, k) + A3(zgru, i, j, k - 1) + A3(zgru, i - 1, j, k - 1) +
+                                    A3(zgru, i, j, k)) / zci * WMASK(i, j, k);
+                const double zaj = (A3(zgrv, i, j - 1, k) + A3(zgrv, i, j, k - 1) + A3(zgrv, i, j - 1, k - 1) +
+                                    A3(zgrv, i, j, k)) / zcj * WMASK(i, j, k);
+                const double zbi = DMIN3(zbw, -100. * fabs(zai), -7.e+3 / e3w_n(i, j, k) * fabs(zai));
+                const double zbj = DMIN3(zbw, -100. * fabs(zaj), -7.e+3 / e3w_n(i, j, k) * fabs(zaj));
+                const double zfk = DMAX(A3(oml, i, j, k), A3(oml, i, j, k - 1));
+                const int ikt = A2(mikt, i, j);
+                const double zck = (A3(gdepw, i, j, k) - A3(gdepw, i, j, ikt)) /
+                                   DMAX(A2(hmlp, i, j) - A3(gdepw, i, j, ikt), 10.);
+                A3(zwz, i, j, k) = (zai / (zbi - zeps) * (1. - zfk) + zck * A2(wslpiml, i, j) * zfk) * WMASK(i, j, k);
+                A3(zww, i, j, k) = (zaj / (zbj - zeps) * (1. - zfk) + zck * A2(wslpjml, i, j) * zfk) * WMASK(i, j, k);
+            }
+    lbc3(zwz, jpk, 'T', 0.);
+    lbc3(zww, jpk, 'T', 0.);
+    for (k = 2; k <= jpkm1; k++) {
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                const double zcofw = WMASK(i, j, k) * z1_16;
+                A3(wslpi, i, j, k) = SHAPIRO(zwz) * zcofw;
+                A3(wslpj, i, j, k) = SHAPIRO(zww) * zcofw;
+            }
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                const double zck = (UMASK(i, j, k) + UMASK(i - 1, j, k)) * (VMASK(i, j, k) + VMASK(i, j - 1, k)) * 0.25;
+                A3(wslpi, i, j, k) = A3(wslpi, i, j, k) * zck;
+                A3(wslpj, i, j, k) = A3(wslpj, i, j, k) * zck;
+            }
+    }
+#undef SHAPIRO
+    lbc3(uslp, jpk, 'U', 0.);
+    lbc3(vslp, jpk, 'V', 0.);
+    lbc3(wslpi, jpk, 'W', 0.);
+    lbc3(wslpj, jpk, 'W', 0.);
+    free(zgru); free(zgrv); free(zdzr); free(zwz); free(zww); free(oml);
+    free(uslpml); free(vslpml); free(wslpiml); free(wslpjml); free(zslpml_hmlpu); free(zslpml_hmlpv);
+    return 0;
+}
+
 int ora_abi_layout(size_t *out, int n)
 {
     size_t v[] = {sizeof(nemo_hip_domain_t), offsetof(nemo_hip_domain_t, rdt), offsetof(nemo_hip_domain_t, k_top),

@@ -174,6 +174,21 @@ class Oracle:
     def bn2(self, pts, pab, pn2, gdept, gdepw):
         _chk(self.L.ora_bn2(_p(pts), _p(pab), _p(pn2), _p(gdept), _p(gdepw)), "bn2")
 
+    def zdf_mxl(self, kt, rn2b, gdept, gdepw, nlb10, rho_c=0.01):
+        """(nmln int32, hmlp, hmlpt) of zdf_mxl."""
+        jpi, jpj = rn2b.shape[:2]
+        nmln = np.zeros((jpi, jpj), dtype=np.int32, order="F")
+        hmlp, hmlpt = np.zeros((jpi, jpj), order="F"), np.zeros((jpi, jpj), order="F")
+        _chk(self.L.ora_zdf_mxl(C.c_int(kt), _p(rn2b), _p(gdept), _p(gdepw), C.c_int(nlb10), C.c_double(rho_c),
+                                nmln.ctypes.data_as(C.POINTER(C.c_int)), _p(hmlp), _p(hmlpt)), "zdf_mxl")
+        return nmln, hmlp, hmlpt
+
+    def ldf_slp(self, kt, prd, pn2, gdept, gdepw, nmln, hmlp, hmlpt, rn_slpmax, uslp, vslp, wslpi, wslpj):
+        nm = np.asfortranarray(nmln, dtype=np.int32)
+        _chk(self.L.ora_ldf_slp(C.c_int(kt), _p(prd), _p(pn2), _p(gdept), _p(gdepw),
+                                nm.ctypes.data_as(C.POINTER(C.c_int)), _p(hmlp), _p(hmlpt), C.c_double(rn_slpmax),
+                                _p(uslp), _p(vslp), _p(wslpi), _p(wslpj)), "ldf_slp")
+
     def hpg_zco(self, kt, rhd, ua, va):
         _chk(self.L.ora_hpg_zco(C.c_int(kt), _p(rhd), _p(ua), _p(va)), "hpg_zco")
 

@@ -1,0 +1,190 @@
+"""CPU checks that pin the oracle's mixed-layer level (zdf_mxl) and
+iso-neutral slopes (ldf_slp with ldf_slp_mxl) to closed forms:
+
+* zdf_mxl: an unstratified column is mixed to its bottom (nmln = mbkt + 1);
+  a density step at level L puts the base at L; negative N^2 counts as
+  zero; a uniform N^2 crosses the criterion where its depth integral does;
+  hmlp and hmlpt are the w- and t-depths of the base;
+* ldf_slp below the mixed layer: for a density with a uniform horizontal
+  gradient and uniform N^2 in a flat-bottomed box, every slope equals
+  -g drho/dx / (N^2 (1 + rho)) (the Shapiro filter keeps a nearly linear
+  field); with a steep gradient the limiter gives min(rn_slpmax, e3/7000)
+  exactly; a horizontally uniform density gives zero slopes;
+* in the mixed layer the u/v slopes taper linearly with depth from the
+  slope at its base, (gdept - e3u(1)/2) / hmlpt, and the w slopes as
+  gdepw / hmlp (the ldf_slp_mxl construction);
+* the lateral boundary: the ring of a cyclic domain repeats the interior.
+The coastal tapering factors are covered only through the GPU parity tests
+(parity partially pinned there).
+"""
+import numpy as np
+import pytest
+
+from nemo_hip import configs
+from oracle.oracle import Oracle
+
+GRAV = 9.80665
+
+
+def box(jperio=0, jpi=30, jpj=24, jpk=12):
+    """A flat-bottomed box, every interior column wet to level jpk-1."""
+    return configs.basin(jpi, jpj, jpk, seed=3, land_frac=0.0, depth_min=5000.0, depth_max=5000.0, jperio=jperio)
+
+
+def fz(shape):
+    return np.zeros(shape, order="F")
+
+
+def nlb10_of(d):
+    """Shallowest w-level below ~10 m, DOM/domzgr.F90:144-145."""
+    e3w, gdepw = d.e3w_1d, np.concatenate([[0.0], np.cumsum(d.e3t_1d[:-1])])
+    zref = 10.0 - 0.1 * e3w.min()
+    return int(np.argmax(gdepw > zref)) + 1
+
+
+# ----------------------------------------------------------------------------
+def test_zdf_mxl_closed_forms():
+    d = configs.basin(26, 20, 14, seed=9)
+    o = Oracle(d)
+    t, *_ = configs.masks(d)
+    gdept, gdepw = configs.depths(d)
+    s2, s3 = (d.jpi, d.jpj), (d.jpi, d.jpj, d.jpk)
+    nlb10 = nlb10_of(d)
+    mbkt = np.maximum(d.arrays["k_bot"], 1)
+    ss = t[:, :, 0]
+    # unstratified (and unstable) columns are mixed to the bottom
+    for val in (0.0, -3e-4):
+        nm, hp, hpt = o.zdf_mxl(1, np.full(s3, val, order="F"), gdept, gdepw, nlb10)
+        want = np.minimum(d.jpk - 1, mbkt) + 1
+        assert np.array_equal(nm, np.maximum(want, nlb10))
+        ii, jj = np.indices(s2)
+        assert np.array_equal(hp, gdepw[ii, jj, nm - 1] * ss)
+        assert np.array_equal(hpt, gdept[ii, jj, nm - 2] * ss)
+    # a density step at level L
+    L = 6
+    rn2 = fz(s3)
+    rn2[:, :, L - 1:] = 1.0
+    nm, _, _ = o.zdf_mxl(1, rn2, gdept, gdepw, nlb10)
+    assert np.array_equal(nm, np.maximum(np.minimum(L - 1, mbkt) + 1, nlb10))
+    # uniform N^2: the base where the integral from nlb10 first reaches the criterion
+    c = GRAV * 0.01 / d.opts["rau0"]
+    e3w = d.e3w_1d
+    n0 = 0.37 * c / e3w[nlb10 - 1:nlb10 + 3].sum()
+    nm, _, _ = o.zdf_mxl(1, np.full(s3, n0, order="F"), gdept, gdepw, nlb10)
+    integ = np.cumsum(n0 * e3w[nlb10 - 1:d.jpk - 1])
+    below = np.nonzero(integ < c)[0]
+    klast = nlb10 + below[-1]
+    assert abs(integ[below[-1]] - c) > 1e-6 * c and 3 <= len(below) <= 5
+    assert np.array_equal(nm, np.maximum(np.minimum(klast, mbkt) + 1, nlb10))
+    with pytest.raises(RuntimeError):
+        o.zdf_mxl(1, rn2, gdept, gdepw, 1)
+
+
+def uniform_gradient(d, axis, alpha, n0=1e-5, r0=2e-3):
+    s3 = (d.jpi, d.jpj, d.jpk)
+    idx = np.indices(s3)[axis] + 1.0
+    prd = np.asfortranarray(r0 + alpha * idx)
+    pn2 = fz(s3)
+    pn2[:, :, 1:-1] = n0
+    return prd, pn2
+
+
+def slopes(o, d, prd, pn2, nmln, rn_slpmax=0.01):
+    gdept, gdepw = configs.depths(d)
+    ii, jj = np.indices((d.jpi, d.jpj))
+    nm = np.full((d.jpi, d.jpj), nmln, dtype=np.int32, order="F")
+    hmlp = np.asfortranarray(gdepw[ii, jj, nm - 1])
+    hmlpt = np.asfortranarray(gdept[ii, jj, nm - 2])
+    s3 = (d.jpi, d.jpj, d.jpk)
+    out = [np.full(s3, 9.0, order="F") for _ in range(4)]
+    for a in out:
+        a[:, :, 0] = a[:, :, -1] = 0.0
+    o.ldf_slp(1, prd, pn2, gdept, gdepw, nm, hmlp, hmlpt, rn_slpmax, *out)
+    return out, gdept, gdepw
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_ldf_slp_uniform_gradient_below_mixed_layer(axis):
+    d = box()
+    o = Oracle(d)
+    E, n0, r0 = 10000.0, 1e-5, 2e-3
+    alpha = 1e-4 * E * n0 / GRAV                      # slope ~ -1e-4
+    prd, pn2 = uniform_gradient(d, axis, alpha, n0, r0)
+    (us, vs, wi, wj), _, _ = slopes(o, d, prd, pn2, nmln=2)
+    along, across = (us, vs) if axis == 0 else (vs, us)
+    walong, wacross = (wi, wj) if axis == 0 else (wj, wi)
+    c = np.arange(d.jpi if axis == 0 else d.jpj) + 1.0
+    sl = (slice(2, d.jpi - 3), slice(2, d.jpj - 3))   # 3x3 neighbourhood wet, away from the taper
+    for k in range(2, d.jpk - 2):                      # 1-based levels 2..jpk-2
+        idx = c[sl[axis]]
+        su = -alpha * GRAV / (E * n0 * (1 + r0 + alpha * (idx + 0.5)))
+        sw = -alpha * GRAV / (E * n0 * (1 + r0 + alpha * idx))
+        shape = [1, 1]
+        shape[axis] = -1
+        got_u = along[sl[0], sl[1], k - 1]
+        got_w = walong[sl[0], sl[1], k - 1]
+        assert np.allclose(got_u, su.reshape(shape), rtol=1e-9, atol=0), k
+        assert np.allclose(got_w, sw.reshape(shape), rtol=1e-9, atol=0), k
+        assert np.all(across[:, :, k - 1] == 0.0) and np.all(wacross[:, :, k - 1] == 0.0)
+    # a horizontally uniform density has no slope
+    prd0, _ = uniform_gradient(d, axis, 0.0, n0, r0)
+    out, _, _ = slopes(o, d, prd0, pn2, nmln=2)
+    for a in out:
+        assert np.all(a == 0.0)
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_ldf_slp_limiter(axis):
+    """A gradient 100x steeper than rn_slpmax: |slope| = min(rn_slpmax, e3/7000)."""
+    d = box()
+    o = Oracle(d)
+    E, n0 = 10000.0, 1e-5
+    alpha = -1.0 * E * n0 / GRAV                     # raw slope ~ +1
+    prd, pn2 = uniform_gradient(d, axis, alpha, n0)
+    (us, vs, wi, wj), _, _ = slopes(o, d, prd, pn2, nmln=2, rn_slpmax=0.01)
+    along, walong = (us, wi) if axis == 0 else (vs, wj)
+    sl = (slice(2, d.jpi - 3), slice(2, d.jpj - 3))
+    for k in range(2, d.jpk - 2):
+        wu = min(0.01, d.e3t_1d[k - 1] / 7.0e3)
+        # at k = 2 (= nmln) the w-slope is the mixed-layer base value, taken at level 3
+        ww = min(0.01, d.e3w_1d[max(k, 3) - 1] / 7.0e3)
+        assert np.allclose(along[sl[0], sl[1], k - 1], wu, rtol=1e-12, atol=0), k
+        assert np.allclose(walong[sl[0], sl[1], k - 1], ww, rtol=1e-12, atol=0), k
+
+
+def test_ldf_slp_mixed_layer_taper():
+    d = box()
+    o = Oracle(d)
+    E, n0, r0, M = 10000.0, 1e-5, 2e-3, 6
+    alpha = 1e-4 * E * n0 / GRAV
+    prd, pn2 = uniform_gradient(d, 0, alpha, n0, r0)
+    (us, vs, wi, wj), gdept, gdepw = slopes(o, d, prd, pn2, nmln=M)
+    i = np.arange(2, d.jpi - 3)
+    su = -alpha * GRAV / (E * n0 * (1 + r0 + alpha * (i + 1 + 0.5)))
+    sw = -alpha * GRAV / (E * n0 * (1 + r0 + alpha * (i + 1)))
+    e3u1 = d.arrays["e3u_n"][0, 0, 0]
+    for k in range(2, M):              # inside the mixed layer: u-slope tapers with gdept
+        f = (gdept[0, 0, k - 1] - 0.5 * e3u1) / gdept[0, 0, M - 2]
+        assert np.allclose(us[2:-3, 2:-3, k - 1], (f * su)[:, None], rtol=1e-9, atol=0), k
+    for k in range(2, M + 1):          # w-slope tapers with gdepw down to the base
+        f = gdepw[0, 0, k - 1] / gdepw[0, 0, M - 1]
+        assert np.allclose(wi[2:-3, 2:-3, k - 1], (f * sw)[:, None], rtol=1e-9, atol=0), k
+    for k in range(M, d.jpk - 2):      # below: the interior slope
+        assert np.allclose(us[2:-3, 2:-3, k - 1], su[:, None], rtol=1e-9, atol=0), k
+    for k in range(M + 1, d.jpk - 2):
+        assert np.allclose(wi[2:-3, 2:-3, k - 1], sw[:, None], rtol=1e-9, atol=0), k
+
+
+def test_ldf_slp_cyclic_ring_and_untouched_levels():
+    d = configs.basin(28, 22, 10, seed=4, jperio=1)
+    o = Oracle(d)
+    rng = np.random.default_rng(5)
+    s3 = (d.jpi, d.jpj, d.jpk)
+    prd = np.asfortranarray(rng.uniform(-2e-3, 4e-3, s3))
+    pn2 = np.asfortranarray(rng.uniform(1e-6, 1e-4, s3))
+    (us, vs, wi, wj), _, _ = slopes(o, d, prd, pn2, nmln=3)
+    for a in (us, vs, wi, wj):
+        assert np.array_equal(a[0], a[-2]) and np.array_equal(a[-1], a[1])
+        assert np.all(a[:, 0] == 0.0) and np.all(a[:, -1] == 0.0)      # closed north/south
+        assert np.all(a[:, :, 0] == 0.0) and np.all(a[:, :, -1] == 0.0)
+        assert np.isfinite(a).all() and np.abs(a).max() <= 0.01 * (1 + 1e-12)
