@@ -231,6 +231,13 @@ def depths(d):
     return np.asfortranarray(gdept), np.asfortranarray(gdepw)
 
 
+def nlb10(d):
+    """The shallowest w-level below ~10 m (1-based), DOM/domzgr.F90:144-145."""
+    gdepw = np.concatenate([[0.0], np.cumsum(d.e3t_1d[:-1])])
+    zref = 10.0 - 0.1 * np.min(d.e3w_1d)
+    return int(np.argmax(gdepw > zref)) + 1
+
+
 def masks(d):
     """(tmask, umask, vmask, wmask, fmask) the way DOM/dommsk.F90 builds them (used only
     to shape synthetic fields; the kernels derive their own)."""

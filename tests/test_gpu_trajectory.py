@@ -12,7 +12,7 @@ import json
 import numpy as np
 import pytest
 
-from nemo_hip import configs
+from nemo_hip import abi, configs
 from test_gpu_parity import check
 from trajectory import OUT_2D, OUT_3D, OUT_4D, gpu_run, oracle_run
 
@@ -43,20 +43,23 @@ def test_trajectory_fp64(hip, case):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["gyre", "cyc7"])
-def test_trajectory_with_density_and_viscosity_fp64(hip, case):
-    """The step with the simplified EOS, hpg_zco and dyn_ldf_lap added."""
+@pytest.mark.parametrize("case,eos", [("gyre", "seos"), ("cyc7", "seos"), ("cyc7", "teos10")])
+def test_trajectory_with_density_and_viscosity_fp64(hip, case, eos):
+    """The step with the iso-neutral slopes of the before state (rab_3d, bn2,
+    zdf_mxl, eos_insitu, ldf_slp), the density and its pressure gradient,
+    dyn_ldf_lap and tra_sbc added."""
     d = CASES[case]()
     S = configs.make_state(d, seed=19)
-    want = oracle_run(d, S, MARKS, physics=True)
+    mk = abi.seos if eos == "seos" else abi.teos10
+    want = oracle_run(d, S, MARKS, physics=True, eos=mk())
     hip.register(d)
-    got = gpu_run(hip, d, S, MARKS, physics=True)
+    got = gpu_run(hip, d, S, MARKS, physics=True, eos=mk())
     ident = True
     for kt in MARKS:
         for k in OUT_3D + OUT_4D + OUT_2D:
-            check(got[kt][k], want[kt][k], what=f"{case} physics step {kt} {k}")
+            check(got[kt][k], want[kt][k], what=f"{case} {eos} physics step {kt} {k}")
             ident &= bool(np.array_equal(got[kt][k], want[kt][k]))
-    print(json.dumps({"case": case, "physics": True, "bit_identical": ident}))
+    print(json.dumps({"case": case, "eos": eos, "physics": True, "bit_identical": ident}))
 
 
 # max relative error after 8 steps, FP32 storage against the FP64 oracle

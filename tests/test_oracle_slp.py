@@ -36,10 +36,7 @@ def fz(shape):
 
 
 def nlb10_of(d):
-    """Shallowest w-level below ~10 m, DOM/domzgr.F90:144-145."""
-    e3w, gdepw = d.e3w_1d, np.concatenate([[0.0], np.cumsum(d.e3t_1d[:-1])])
-    zref = 10.0 - 0.1 * e3w.min()
-    return int(np.argmax(gdepw > zref)) + 1
+    return configs.nlb10(d)
 
 
 # ----------------------------------------------------------------------------

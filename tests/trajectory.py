@@ -11,9 +11,12 @@ equation of state, pressure gradient, lateral viscosity or surface module
     tra_ldf_iso, tra_zdf              after tracers
     tra_nxt, dyn_nxt, ssh_swp         Asselin filters and swaps
 
-With physics=True the step also has a density and its pressure gradient, a
-lateral viscosity and the surface tracer fluxes, step.F90:165, 191-193 and
-238: eos_insitu_pot of the now tracers (simplified EOS) after wzv,
+With physics=True the step also has the iso-neutral slopes, a density and
+its pressure gradient, a lateral viscosity and the surface tracer fluxes,
+step.F90:127-150, 165, 191-193 and 238: rab_3d, bn2, zdf_mxl, eos_insitu
+and ldf_slp of the before tracers first (the slopes tra_ldf_iso uses),
+eos_insitu_pot of the now tracers after wzv (the simplified EOS unless
+`eos` names another),
 dyn_ldf_lap of the before velocities (ahm = AHM m2/s) and hpg_zco between
 dyn_vor and dyn_spg_ts, and tra_sbc (an analytic heat flux, the generator's
 fresh-water flux) before tra_adv.
@@ -34,6 +37,7 @@ OUT_4D = ["tsn", "tsb"]
 OUT_2D = ["sshn", "sshb", "un_b", "vn_b", "ub_b", "vb_b", "ub2_b", "un_bf"]
 KN_FCT = (4, 4)
 AHM = 2000.0
+RN_SLPMAX = 0.01
 
 
 def r2dt(d, kt):
@@ -65,13 +69,14 @@ def viscosity(d):
     return np.asfortranarray(AHM * t), np.asfortranarray(AHM * f)
 
 
-def oracle_run(d, S, marks, physics=False):
+def oracle_run(d, S, marks, physics=False, eos=None):
     from oracle.oracle import Oracle
     from nemo_hip import abi, configs
     o = Oracle(d)
     if physics:
-        o.eos_register(abi.seos())
-        gdept, _ = configs.depths(d)
+        o.eos_register(eos or abi.seos())
+        gdept, gdepw = configs.depths(d)
+        nlb10 = configs.nlb10(d)
         ahmt, ahmf = viscosity(d)
         qns0, qsr0, sfx = surface_fluxes(d)
         sbc, sbc_b = np.zeros((d.jpi, d.jpj, 2), order="F"), np.zeros((d.jpi, d.jpj, 2), order="F")
@@ -83,6 +88,14 @@ def oracle_run(d, S, marks, physics=False):
     ah, akz = np.zeros(s3, order="F"), np.zeros(s3, order="F")
     out = {}
     for kt in range(1, max(marks) + 1):
+        if physics:
+            pab, rn2b, rhd = np.zeros(s4, order="F"), np.zeros(s3, order="F"), np.zeros(s3, order="F")
+            o.rab_3d(X["tsb"], pab, gdept)
+            o.bn2(X["tsb"], pab, rn2b, gdept, gdepw)
+            nmln, hmlp, hmlpt = o.zdf_mxl(kt, rn2b, gdept, gdepw, nlb10)
+            o.eos_insitu(X["tsb"], rhd, gdept)
+            o.ldf_slp(kt, rhd, rn2b, gdept, gdepw, nmln, hmlp, hmlpt, RN_SLPMAX, X["uslp"], X["vslp"], X["wslpi"],
+                      X["wslpj"])
         o.ssh_nxt(kt, X["un"], X["vn"], hdivn, X["sshb"], X["emp_b"], X["emp"], X["ssha"])
         o.wzv(kt, hdivn, X["wn"])
         if physics:
@@ -120,7 +133,7 @@ def oracle_run(d, S, marks, physics=False):
     return out
 
 
-def gpu_run(h, d, S, marks, sp=False, physics=False):
+def gpu_run(h, d, S, marks, sp=False, physics=False, eos=None):
     """h: the library handle in device-pointer mode with d registered."""
     import torch
     from nemo_hip import abi, configs
@@ -138,8 +151,12 @@ def gpu_run(h, d, S, marks, sp=False, physics=False):
     F.update(ua=ua, va=va, un=X["un"], vn=X["vn"])
     shape = {k: (s4 if k in OUT_4D else s3 if k in OUT_3D else s2) for k in OUT_3D + OUT_4D + OUT_2D}
     if physics:
-        h.eos_register(abi.seos())
-        gdept = dev(configs.depths(d)[0], dt)
+        h.eos_register(eos or abi.seos())
+        gdept, gdepw = (dev(a, dt) for a in configs.depths(d))
+        nlb10 = configs.nlb10(d)
+        pab, rn2b = z(2 * n3), z(n3)
+        nmln = torch.zeros(n2, dtype=torch.int32, device="cuda")
+        hmlp, hmlpt = (torch.zeros(n2, dtype=torch.float64, device="cuda") for _ in range(2))
         ahmt, ahmf = (dev(a, dt) for a in viscosity(d))
         rhd, rhop = z(n3), z(n3)
         qns0, qsr0, sfx = (dev(a, torch.float64) for a in surface_fluxes(d))
@@ -151,6 +168,13 @@ def gpu_run(h, d, S, marks, sp=False, physics=False):
     # torch work (the trend reset) is queued on the library stream, in order with the library's launches
     lib_stream = torch.cuda.ExternalStream(h.stream())
     for kt in range(1, max(marks) + 1):
+        if physics:
+            h.rab_3d(X["tsb"], pab, gdept, sp=sp)
+            h.bn2(X["tsb"], pab, rn2b, gdept, gdepw, sp=sp)
+            h.zdf_mxl(kt, rn2b, gdept, gdepw, nlb10, nmln, hmlp, hmlpt, sp=sp)
+            h.eos_insitu(X["tsb"], rhd, gdept, sp=sp)
+            h.ldf_slp(kt, rhd, rn2b, gdept, gdepw, nmln, hmlp, hmlpt, RN_SLPMAX, X["uslp"], X["vslp"], X["wslpi"],
+                      X["wslpj"], sp=sp)
         h.ssh_nxt(kt, X["un"], X["vn"], hdivn, X["sshb"], X["emp_b"], X["emp"], X["ssha"], sp=sp)
         h.wzv(kt, hdivn, X["wn"], sp=sp)
         if physics:
