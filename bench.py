@@ -67,6 +67,10 @@ def parse():
     ap.add_argument("--whole-step", default="fp32_fused,fp64_fused", metavar="FORMS",
                     help="also time, for these fused forms, the whole time step: the hot path closed by ssh_nxt, "
                          "wzv, div_hor, tra_nxt, dyn_nxt and ssh_swp (reported apart; '' skips)")
+    ap.add_argument("--physics-step", default="fp32,fp64", metavar="FORMS",
+                    help="also time, for these forms, the whole step with its physics: the iso-neutral slopes of "
+                         "the state (rab_3d, bn2, zdf_mxl, eos_insitu, ldf_slp), the density, hpg_zco, "
+                         "dyn_ldf_lap, tra_sbc and the separate hot-path routines (TEOS-10)")
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -91,7 +95,7 @@ class Stepper:
     fused=False: the separate drop-in routines in step order; fused=True: the
     two fused step entry points (same results, bit for bit)."""
 
-    def __init__(self, h, d, S, sp, fused=False):
+    def __init__(self, h, d, S, sp, fused=False, physics=False):
         import torch
         self.h, self.d, self.sp, self.fused = h, d, sp, fused
         dt = torch.float32 if sp else torch.float64
@@ -119,6 +123,80 @@ class Stepper:
         # the closure's fields: divergence work array and the before sea level
         self.hdivn = z(n3)
         self.sshb = self.f2["sshn"].clone()
+        if physics:
+            self._physics_fields(S, dt, z, n2, n3)
+
+    def _physics_fields(self, S, dt, z, n2, n3):
+        """The fields of the physics step: depths, expansion ratios, N^2, the
+        mixed layer, densities, a 2000 m2/s lateral viscosity and analytic
+        surface fluxes (TEOS-10 equation of state)."""
+        import numpy as np
+        import torch
+        from nemo_hip import abi, configs
+        d, h = self.d, self.h
+        dev = lambda a, t: torch.from_numpy(np.ascontiguousarray(np.asarray(a).ravel(order="F"))).to("cuda", t)
+        gt, gw = configs.depths(d)
+        self.gdept, self.gdepw = dev(gt, dt), dev(gw, dt)
+        del gt, gw
+        t, _, _, _, f = configs.masks(d)
+        self.ahmt, self.ahmf = dev(2000.0 * t, dt), dev(2000.0 * f, dt)
+        del t, f
+        self.pab, self.rn2b, self.rhd, self.rhop = z(2 * n3), z(n3), z(n3), z(n3)
+        self.nmln = torch.zeros(n2, dtype=torch.int32, device="cuda")
+        f64 = lambda: torch.zeros(n2, dtype=torch.float64, device="cuda")
+        self.hmlp, self.hmlpt, self.sbc, self.sbc_b = f64(), f64(), torch.zeros(2 * n2, dtype=torch.float64,
+                                                                              device="cuda"), \
+            torch.zeros(2 * n2, dtype=torch.float64, device="cuda")
+        y = np.arange(d.jpj) / max(d.jpj - 1, 1)
+        self.qns0 = dev(np.broadcast_to((-40.0 + 80.0 * np.cos(np.pi * y))[None, :], (d.jpi, d.jpj)), torch.float64)
+        self.qsr0 = torch.full((n2,), 30.0, dtype=torch.float64, device="cuda")
+        self.sfx = f64()
+        self.qns, self.qsr = self.qns0.clone(), self.qsr0.clone()
+        self.nlb10 = configs.nlb10(d)
+        h.eos_register(abi.teos10())
+
+    def physics_step(self, kt):
+        """The whole time step with its physics, step.F90:127-279: the iso-
+        neutral slopes of the before state (rab_3d, bn2, zdf_mxl, eos_insitu,
+        ldf_slp), the now density, the pressure gradient and lateral
+        viscosity, the surface tracer fluxes, the separate hot-path routines
+        and the closure."""
+        h, f, f2, sp = self.h, self.f3, self.f2, self.sp
+        h.rab_3d(self.tsb, self.pab, self.gdept, sp=sp)
+        h.bn2(self.tsb, self.pab, self.rn2b, self.gdept, self.gdepw, sp=sp)
+        h.zdf_mxl(kt, self.rn2b, self.gdept, self.gdepw, self.nlb10, self.nmln, self.hmlp, self.hmlpt, sp=sp)
+        h.eos_insitu(self.tsb, self.rhd, self.gdept, sp=sp)
+        h.ldf_slp(kt, self.rhd, self.rn2b, self.gdept, self.gdepw, self.nmln, self.hmlp, self.hmlpt, 0.01, f["uslp"],
+                  f["vslp"], f["wslpi"], f["wslpj"], sp=sp)
+        h.ssh_nxt(kt, self.un_dyn, self.vn_dyn, self.hdivn, self.sshb, f2["emp"], f2["emp"], f2["ssha"], sp=sp)
+        h.wzv(kt, self.hdivn, f["wn"], sp=sp)
+        h.eos_insitu_pot(self.tsn, self.rhd, self.rhop, self.gdept, sp=sp)
+        self.ua.zero_()
+        self.va.zero_()
+        h.dyn_adv(kt, self.un_dyn, self.vn_dyn, f["wn"], self.ua, self.va, sp=sp)
+        h.dyn_vor(kt, self.un_dyn, self.vn_dyn, self.ua, self.va, sp=sp)
+        h.dyn_ldf_lap(kt, f["ub"], f["vb"], self.ua, self.va, 1, self.ahmt, self.ahmf, sp=sp)
+        h.hpg_zco(kt, self.rhd, self.ua, self.va, sp=sp)
+        h.dyn_spg_ts(kt, self.spg, sp=sp)
+        h.div_hor(kt, self.un_dyn, self.vn_dyn, self.hdivn, sp=sp)
+        h.wzv(kt, self.hdivn, f["wn"], sp=sp)
+        h.dyn_zdf(kt, f["ub"], f["vb"], self.ua, self.va, f["avm"], f2["rCdU_bot"], f2["ua_b"], f2["va_b"],
+                  f2["utau"], f2["utau_b"], f2["vtau"], f2["vtau_b"], sp=sp)
+        self.tsa.zero_()
+        self.qns.copy_(self.qns0)
+        self.qsr.copy_(self.qsr0)
+        h.tra_sbc(kt, self.qns, self.qsr, self.sfx, f2["emp"], self.tsn, self.tsa, self.sbc, self.sbc_b, sp=sp)
+        h.tra_adv_trp(self.un_dyn, self.vn_dyn, f["wn"], self.zun, self.zvn, self.zwn, sp=sp)
+        h.tra_adv_fct(kt, 1, "TRA", self.r2dt, self.zun, self.zvn, self.zwn, self.tsb, self.tsn, self.tsa, 2, 4, 4,
+                      sp=sp)
+        h.tra_ldf_iso(kt, 1, "TRA", f["ahtu"], f["ahtv"], self.tsb, self.tsb, self.tsa, 2, 1, f["uslp"], f["vslp"],
+                      f["wslpi"], f["wslpj"], self.ah, self.akz, sp=sp)
+        h.tra_zdf(kt, self.tsb, self.tsa, f["avt"], f["avs"], 1, self.ah, self.akz, sp=sp)
+        h.tra_nxt(kt, self.tsb, self.tsn, self.tsa, sp=sp)
+        h.dyn_nxt(kt, f["ub"], f["vb"], self.un_dyn, self.vn_dyn, self.ua, self.va, f2["ua_b"], f2["va_b"],
+                  f2["un_adv"], f2["vn_adv"], f2["ub_b"], f2["vb_b"], f2["un_b"], f2["vn_b"], f2["r1_hu_n"],
+                  f2["r1_hv_n"], sp=sp)
+        h.ssh_swp(kt, self.sshb, f2["sshn"], f2["ssha"])
 
     def whole_step(self, kt):
         """The whole time step, step.F90:161-279: the fused hot path closed by
@@ -164,15 +242,15 @@ class Stepper:
         h.tra_zdf(kt, self.tsb, self.tsa, f["avt"], f["avs"], 1, self.ah, self.akz, sp=sp)
 
 
-def time_form(h, d, S, sp, fused, steps, warmup, timing_steps, live=False, whole=False):
+def time_form(h, d, S, sp, fused, steps, warmup, timing_steps, live=False, whole=False, physics=False):
     """Warm up, time every kernel over `timing_steps` steps with HIP events,
     then time `steps` steps with the wall clock.  live=True also brackets the
     launches of the dominant kernel (the one with the most time per step that
     has a byte count) with HIP events on the library stream inside the timed
     region; its measured launch durations are returned for the roofline."""
     import torch
-    st = Stepper(h, d, S, sp, fused)
-    step = st.whole_step if whole else st.step
+    st = Stepper(h, d, S, sp, fused, physics=physics)
+    step = st.physics_step if physics else st.whole_step if whole else st.step
     ext = torch.cuda.ExternalStream(h.stream())
     with torch.cuda.stream(ext):
         kt = 2
@@ -303,12 +381,17 @@ def time_ensemble(h, members, steps, seed):
     return eager_ms, graph_ms, cells
 
 
+# the physics step's routines beside the hot path (their timer names)
+PHYSICS_ROUTINES = ["rab", "bn2", "zdf_mxl", "eos", "ldf_slp", "dyn_hpg", "dyn_ldf", "tra_sbc"]
+
+
 def kernel_times(h):
     """{name: (total_ms, count)} of every timer the library accumulated."""
     import ctypes as C
     out = {}
     names = ["dyn_adv", "dyn_vor", "dyn_spg_ts", "dyn_zdf", "tra_adv_trp", "tra_adv_fct", "tra_ldf_iso", "tra_zdf",
-             "lbc_lnk", "step_dyn", "step_tra", "div_hor", "ssh_nxt", "wzv", "ssh_swp", "tra_nxt", "dyn_nxt"]
+             "lbc_lnk", "step_dyn", "step_tra", "div_hor", "ssh_nxt", "wzv", "ssh_swp", "tra_nxt", "dyn_nxt"] + \
+        PHYSICS_ROUTINES
     kern = {
         "dyn_adv": ["k_dyn_keg_zad"], "dyn_vor": ["k_dyn_vor"], "dyn_zdf": ["k_dyn_zdf"],
         "dyn_spg_ts": ["k_spg_cor1", "k_spg_cor2", "k_spg_init", "k_spg_p1a", "k_spg_p1b", "k_spg_s1", "k_spg_s2", "k_spg_s3",
@@ -321,6 +404,7 @@ def kernel_times(h):
         "step_tra": ["k_fct_h", "k_fct_v", "k_fct_lim", "k_fct_beta", "k_fct_final", "k_zero_ring", "k_tra_zdf"],
         "div_hor": ["k_div_hor"], "ssh_nxt": ["k_ssh_nxt"], "wzv": ["k_wzv"], "ssh_swp": ["k_ssh_swp"],
         "tra_nxt": ["k_tra_nxt"], "dyn_nxt": ["k_dyn_nxt1", "k_dyn_nxt2"],
+        "ldf_slp": ["k_slp_mxl", "k_slp_uv_raw", "k_slp_uv_filt", "k_slp_w_raw", "k_slp_w_filt"],
     }
     for n in names:
         out[n] = h.timing_get(n)
@@ -487,6 +571,18 @@ def main():
                                                        if "/" in k and k.split("/")[0] in closure and v[1] > 0}}
         log(f"whole step {form}: {ms:.2f} ms/step, {cells / (ms * 1e-3):.3e} cell-updates/s, finite={finite}, "
             f"closure {sum(whole[form]['closure_ms_per_step'].values()):.3f} ms")
+    physics = {}
+    for form in [f for f in a.physics_step.split(",") if f]:
+        ms, finite, kt, _ = time_form(h, d, S, form.startswith("fp32"), False, a.steps, a.warmup, a.timing_steps,
+                                      physics=True)
+        phys = {k: kt[k][0] / a.timing_steps for k in PHYSICS_ROUTINES if kt[k][1] > 0}
+        physics[form] = {"ms_per_step": ms, "cell_updates_per_s": cells / (ms * 1e-3), "finite": finite,
+                         "physics_ms_per_step": phys,
+                         "ldf_slp_kernels_ms_per_step": {k: v[0] / a.timing_steps for k, v in kt.items()
+                                                         if k.startswith("ldf_slp/") and v[1] > 0}}
+        log(f"physics step {form}: {ms:.2f} ms/step, {cells / (ms * 1e-3):.3e} cell-updates/s, finite={finite}, "
+            f"physics routines {sum(phys.values()):.3f} ms: " +
+            ", ".join(f"{k} {v:.3f}" for k, v in sorted(phys.items(), key=lambda x: -x[1])))
     # roofline of the dominant kernel of the headline form: its launches timed
     # with HIP events inside the timed region
     width = 4 if head.startswith("fp32") else 8
@@ -539,6 +635,7 @@ def main():
         "host_mode_fp64_ms_per_step": host_ms,
         "ensemble": ens,
         "whole_step": whole,
+        "physics_step": physics,
         "forms": res,
         "roofline": roof,
         "cpu_baseline": cpu,
