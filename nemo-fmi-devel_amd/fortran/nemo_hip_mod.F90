@@ -19,6 +19,7 @@ MODULE nemo_hip_mod
    !!   rab_3d         TRA/eosbn2.F90:566           bn2            TRA/eosbn2.F90:889
    !!   hpg_zco        DYN/dynhpg.F90:258           dyn_ldf_lap    DYN/dynldf_lap_blp.F90:37
    !!   tra_sbc        TRA/trasbc.F90:62            stp_ctl        stpctl.F90:55
+   !!   zdf_mxl        ZDF/zdfmxl.F90:64            ldf_slp        LDF/ldfslp.F90:83
    !! The arrays those routines take from NEMO modules (oce, zdf_oce, ldfslp,
    !! sbc_oce, zdfdrg, dynspg_ts, dom_oce) are bound once, after allocation, with the
    !! nemo_hip_bind_* routines: module arrays never move, so their addresses
@@ -38,6 +39,7 @@ MODULE nemo_hip_mod
    PUBLIC :: nemo_hip_bind_nxt, div_hor, ssh_nxt, wzv, ssh_swp, tra_nxt, dyn_nxt
    PUBLIC :: nemo_hip_eos_t, nemo_hip_eos_init, nemo_hip_bind_phys
    PUBLIC :: eos_insitu, eos_insitu_pot, rab_3d, bn2, hpg_zco, dyn_ldf_lap, tra_sbc, stp_ctl
+   PUBLIC :: nemo_hip_bind_mxl, zdf_mxl, ldf_slp
    PUBLIC :: nemo_hip_domain_t, nemo_hip_spg_ts_t
    PUBLIC :: nemo_hip_field_alloc, nemo_hip_field_free, nemo_hip_field_push, nemo_hip_field_pull
 
@@ -88,6 +90,9 @@ MODULE nemo_hip_mod
    TYPE(C_PTR), SAVE :: p_ub_b = C_NULL_PTR, p_vb_b = C_NULL_PTR, p_un_b = C_NULL_PTR, p_vn_b = C_NULL_PTR
    TYPE(C_PTR), SAVE :: p_un_adv = C_NULL_PTR, p_vn_adv = C_NULL_PTR, p_r1_hu_n = C_NULL_PTR, p_r1_hv_n = C_NULL_PTR
    TYPE(C_PTR), SAVE :: p_gdept = C_NULL_PTR, p_gdepw = C_NULL_PTR, p_rhd = C_NULL_PTR
+   TYPE(C_PTR), SAVE :: p_rn2b = C_NULL_PTR, p_nmln = C_NULL_PTR, p_hmlp = C_NULL_PTR, p_hmlpt = C_NULL_PTR
+   INTEGER(C_INT), SAVE :: m_nlb10 = 0
+   REAL(wp), SAVE :: m_rho_c = 0.01_wp, m_slpmax = 0.01_wp
    TYPE(C_PTR), SAVE :: p_ahmt = C_NULL_PTR, p_ahmf = C_NULL_PTR
    TYPE(C_PTR), SAVE :: p_qns = C_NULL_PTR, p_qsr = C_NULL_PTR, p_sfx = C_NULL_PTR
    TYPE(C_PTR), SAVE :: p_sbc_tsc = C_NULL_PTR, p_sbc_tsc_b = C_NULL_PTR
@@ -249,6 +254,20 @@ MODULE nemo_hip_mod
          IMPORT :: C_INT, C_PTR
          TYPE(C_PTR), VALUE :: pts, pab, pn2, gdept, gdepw
       END FUNCTION
+      INTEGER(C_INT) FUNCTION c_zdf_mxl(kt, rn2b, gdept, gdepw, nlb10, rho_c, nmln, hmlp, hmlpt) &
+         &                          BIND(C, name="nemo_hip_zdf_mxl")
+         IMPORT :: C_INT, C_PTR, C_DOUBLE
+         INTEGER(C_INT), VALUE :: kt, nlb10
+         REAL(C_DOUBLE), VALUE :: rho_c
+         TYPE(C_PTR), VALUE :: rn2b, gdept, gdepw, nmln, hmlp, hmlpt
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_ldf_slp(kt, prd, pn2, gdept, gdepw, nmln, hmlp, hmlpt, rn_slpmax, uslp, vslp, &
+         &                              wslpi, wslpj) BIND(C, name="nemo_hip_ldf_slp")
+         IMPORT :: C_INT, C_PTR, C_DOUBLE
+         INTEGER(C_INT), VALUE :: kt
+         REAL(C_DOUBLE), VALUE :: rn_slpmax
+         TYPE(C_PTR), VALUE :: prd, pn2, gdept, gdepw, nmln, hmlp, hmlpt, uslp, vslp, wslpi, wslpj
+      END FUNCTION
       INTEGER(C_INT) FUNCTION c_hpg_zco(kt, rhd, ua, va) BIND(C, name="nemo_hip_hpg_zco")
          IMPORT :: C_INT, C_PTR
          INTEGER(C_INT), VALUE :: kt
@@ -396,6 +415,18 @@ CONTAINS
       TYPE(nemo_hip_eos_t), INTENT(in) :: seos
       CALL check( c_eos_register( seos ), 'eos_init' )
    END SUBROUTINE nemo_hip_eos_init
+
+   SUBROUTINE nemo_hip_bind_mxl( prn2b, knmln, phmlp, phmlpt, knlb10, prho_c, pslpmax )
+      !! zdfmxl (nmln, hmlp, hmlpt) and zdf_oce (rn2b) arrays, dom_oce nlb10,
+      !! the density criterion rho_c and the nam_traldf slope limit rn_slpmax
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(in) :: prn2b
+      INTEGER(C_INT32_T), DIMENSION(:,:), TARGET, CONTIGUOUS, INTENT(in) :: knmln
+      REAL(wp), DIMENSION(:,:), TARGET, CONTIGUOUS, INTENT(in) :: phmlp, phmlpt
+      INTEGER, INTENT(in) :: knlb10
+      REAL(wp), INTENT(in) :: prho_c, pslpmax
+      p_rn2b = C_LOC( prn2b ) ; p_nmln = C_LOC( knmln ) ; p_hmlp = C_LOC( phmlp ) ; p_hmlpt = C_LOC( phmlpt )
+      m_nlb10 = knlb10 ; m_rho_c = prho_c ; m_slpmax = pslpmax
+   END SUBROUTINE nemo_hip_bind_mxl
 
    SUBROUTINE nemo_hip_bind_phys( pgdept, pgdepw, prhd, pahmt, pahmf, pqns, pqsr, psfx, psbc_tsc, psbc_tsc_b )
       !! dom_oce (gdept_n, gdepw_n), oce (rhd), ldfdyn (ahmt, ahmf), sbc_oce (qns, qsr, sfx, sbc_tsc, sbc_tsc_b)
@@ -558,6 +589,20 @@ CONTAINS
       REAL(wp), DIMENSION(:,:,:)  , TARGET, CONTIGUOUS, INTENT(  out) ::   pn2
       CALL check( c_bn2( C_LOC(pts), C_LOC(pab), C_LOC(pn2), p_gdept, p_gdepw ), 'bn2' )
    END SUBROUTINE bn2
+
+   SUBROUTINE zdf_mxl( kt )
+      !! ZDF/zdfmxl.F90:64: nmln, hmlp, hmlpt from rn2b (hmld is not computed)
+      INTEGER, INTENT(in) :: kt
+      CALL check( c_zdf_mxl( kt, p_rn2b, p_gdept, p_gdepw, m_nlb10, m_rho_c, p_nmln, p_hmlp, p_hmlpt ), 'zdf_mxl' )
+   END SUBROUTINE zdf_mxl
+
+   SUBROUTINE ldf_slp( kt, prd, pn2 )
+      !! LDF/ldfslp.F90:83: uslp, vslp, wslpi, wslpj (the bound ldfslp arrays)
+      INTEGER, INTENT(in) :: kt
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(in) :: prd, pn2
+      CALL check( c_ldf_slp( kt, C_LOC(prd), C_LOC(pn2), p_gdept, p_gdepw, p_nmln, p_hmlp, p_hmlpt, m_slpmax,  &
+         &                   p_uslp, p_vslp, p_wslpi, p_wslpj ), 'ldf_slp' )
+   END SUBROUTINE ldf_slp
 
    SUBROUTINE hpg_zco( kt )
       INTEGER, INTENT(in) ::   kt

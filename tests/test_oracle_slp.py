@@ -26,9 +26,26 @@ from oracle.oracle import Oracle
 GRAV = 9.80665
 
 
+E1, E2 = 10000.0, 15000.0
+
+
 def box(jperio=0, jpi=30, jpj=24, jpk=12):
-    """A flat-bottomed box, every interior column wet to level jpk-1."""
-    return configs.basin(jpi, jpj, jpk, seed=3, land_frac=0.0, depth_min=5000.0, depth_max=5000.0, jperio=jperio)
+    """A flat-bottomed box, every interior column wet to level jpk-1, with
+    e1 = 10 km and e2 = 15 km (so an e1/e2 mix-up changes the slopes)."""
+    d = configs.basin(jpi, jpj, jpk, seed=3, land_frac=0.0, depth_min=5000.0, depth_max=5000.0, jperio=jperio,
+                      e1=E1)
+    A = d.arrays
+    for n in ("e2t", "e2u", "e2v", "e2f"):
+        A[n] = np.asfortranarray(np.full_like(A[n], E2))
+    A["r1_e2u"], A["r1_e2v"] = 1.0 / A["e2u"], 1.0 / A["e2v"]
+    for p in ("t", "u", "v", "f"):
+        A["e1e2" + p] = A["e1" + p] * A["e2" + p]
+        if "r1_e1e2" + p in A:
+            A["r1_e1e2" + p] = 1.0 / A["e1e2" + p]
+    A["e2_e1u"], A["e1_e2v"] = A["e2u"] / A["e1u"], A["e1v"] / A["e2v"]
+    for n in A:
+        A[n] = np.asfortranarray(A[n])
+    return d
 
 
 def fz(shape):
@@ -57,6 +74,12 @@ def test_zdf_mxl_closed_forms():
         ii, jj = np.indices(s2)
         assert np.array_equal(hp, gdepw[ii, jj, nm - 1] * ss)
         assert np.array_equal(hpt, gdept[ii, jj, nm - 2] * ss)
+    # unstable water above a strong step: negative N^2 counts as zero, so the
+    # base is the step, not deeper
+    rn2 = np.full(s3, -1.0, order="F")
+    rn2[:, :, 5:] = 1.0
+    nm, _, _ = o.zdf_mxl(1, rn2, gdept, gdepw, nlb10)
+    assert np.array_equal(nm, np.maximum(np.minimum(5, mbkt) + 1, nlb10))
     # a density step at level L
     L = 6
     rn2 = fz(s3)
@@ -77,13 +100,26 @@ def test_zdf_mxl_closed_forms():
         o.zdf_mxl(1, rn2, gdept, gdepw, 1)
 
 
-def uniform_gradient(d, axis, alpha, n0=1e-5, r0=2e-3):
+def uniform_gradient(d, axis, alpha, n0=1e-5, r0=2e-3, beta=0.0):
+    """prd = r0 + alpha * (index along axis) + beta * level, N^2 = n0 on
+    levels 2..jpk-1 (zero at 1 and jpk, as bn2 leaves them)."""
     s3 = (d.jpi, d.jpj, d.jpk)
-    idx = np.indices(s3)[axis] + 1.0
-    prd = np.asfortranarray(r0 + alpha * idx)
+    ix = np.indices(s3)
+    prd = np.asfortranarray(r0 + alpha * (ix[axis] + 1.0) + beta * (ix[2] + 1.0))
     pn2 = fz(s3)
     pn2[:, :, 1:-1] = n0
     return prd, pn2
+
+
+def slope_u(alpha, E, n0, r0, beta, c, k):
+    """The u/v-point slope at 1-based coordinate c, level k: density at the
+    face mean of the two t-points, zdzr = -(prd + 1) n0 / g."""
+    return -alpha * GRAV / (E * n0 * (1 + r0 + alpha * (c + 0.5) + beta * k))
+
+
+def slope_w(alpha, E, n0, r0, beta, c, k):
+    """The w-point slope: zbw from the mean density of levels k and k-1."""
+    return -alpha * GRAV / (E * n0 * (1 + r0 + alpha * c + beta * (k - 0.5)))
 
 
 def slopes(o, d, prd, pn2, nmln, rn_slpmax=0.01):
@@ -104,25 +140,31 @@ def slopes(o, d, prd, pn2, nmln, rn_slpmax=0.01):
 def test_ldf_slp_uniform_gradient_below_mixed_layer(axis):
     d = box()
     o = Oracle(d)
-    E, n0, r0 = 10000.0, 1e-5, 2e-3
+    E, n0, r0, beta = (E1, E2)[axis], 1e-5, 2e-3, 1e-4
     alpha = 1e-4 * E * n0 / GRAV                      # slope ~ -1e-4
-    prd, pn2 = uniform_gradient(d, axis, alpha, n0, r0)
+    prd, pn2 = uniform_gradient(d, axis, alpha, n0, r0, beta)
     (us, vs, wi, wj), _, _ = slopes(o, d, prd, pn2, nmln=2)
     along, across = (us, vs) if axis == 0 else (vs, us)
     walong, wacross = (wi, wj) if axis == 0 else (wj, wi)
     c = np.arange(d.jpi if axis == 0 else d.jpj) + 1.0
     sl = (slice(2, d.jpi - 3), slice(2, d.jpj - 3))   # 3x3 neighbourhood wet, away from the taper
-    for k in range(2, d.jpk - 2):                      # 1-based levels 2..jpk-2
-        idx = c[sl[axis]]
-        su = -alpha * GRAV / (E * n0 * (1 + r0 + alpha * (idx + 0.5)))
-        sw = -alpha * GRAV / (E * n0 * (1 + r0 + alpha * idx))
-        shape = [1, 1]
-        shape[axis] = -1
+    shape = [1, 1]
+    shape[axis] = -1
+    idx = c[sl[axis]]
+    for k in range(2, d.jpk):                          # 1-based levels 2..jpk-1
+        # at the bottom level (k = jpk-1) N^2 below is zero and zdzr divides
+        # by 1 instead of 2; the vertical taper then halves the u/v slope
+        taper = 0.5 if k == d.jpk - 1 else 1.0
+        su = taper * slope_u(alpha, E, n0, r0, beta, idx, k)
         got_u = along[sl[0], sl[1], k - 1]
-        got_w = walong[sl[0], sl[1], k - 1]
         assert np.allclose(got_u, su.reshape(shape), rtol=1e-9, atol=0), k
-        assert np.allclose(got_w, sw.reshape(shape), rtol=1e-9, atol=0), k
+        if k >= 3:                                     # level 2 takes the mixed-layer base value
+            sw = slope_w(alpha, E, n0, r0, beta, idx, k)
+            got_w = walong[sl[0], sl[1], k - 1]
+            assert np.allclose(got_w, sw.reshape(shape), rtol=1e-9, atol=0), k
         assert np.all(across[:, :, k - 1] == 0.0) and np.all(wacross[:, :, k - 1] == 0.0)
+    sw3 = slope_w(alpha, E, n0, r0, beta, idx, 3)
+    assert np.allclose(walong[sl[0], sl[1], 1], sw3.reshape(shape), rtol=1e-9, atol=0)
     # a horizontally uniform density has no slope
     prd0, _ = uniform_gradient(d, axis, 0.0, n0, r0)
     out, _, _ = slopes(o, d, prd0, pn2, nmln=2)
@@ -135,7 +177,7 @@ def test_ldf_slp_limiter(axis):
     """A gradient 100x steeper than rn_slpmax: |slope| = min(rn_slpmax, e3/7000)."""
     d = box()
     o = Oracle(d)
-    E, n0 = 10000.0, 1e-5
+    E, n0 = (E1, E2)[axis], 1e-5
     alpha = -1.0 * E * n0 / GRAV                     # raw slope ~ +1
     prd, pn2 = uniform_gradient(d, axis, alpha, n0)
     (us, vs, wi, wj), _, _ = slopes(o, d, prd, pn2, nmln=2, rn_slpmax=0.01)
@@ -152,24 +194,26 @@ def test_ldf_slp_limiter(axis):
 def test_ldf_slp_mixed_layer_taper():
     d = box()
     o = Oracle(d)
-    E, n0, r0, M = 10000.0, 1e-5, 2e-3, 6
+    E, n0, r0, beta, M = E1, 1e-5, 2e-3, 1e-4, 6
     alpha = 1e-4 * E * n0 / GRAV
-    prd, pn2 = uniform_gradient(d, 0, alpha, n0, r0)
+    prd, pn2 = uniform_gradient(d, 0, alpha, n0, r0, beta)
     (us, vs, wi, wj), gdept, gdepw = slopes(o, d, prd, pn2, nmln=M)
-    i = np.arange(2, d.jpi - 3)
-    su = -alpha * GRAV / (E * n0 * (1 + r0 + alpha * (i + 1 + 0.5)))
-    sw = -alpha * GRAV / (E * n0 * (1 + r0 + alpha * (i + 1)))
+    c = np.arange(2, d.jpi - 3) + 1.0
+    su_base = slope_u(alpha, E, n0, r0, beta, c, M)         # iku = nmln
+    sw_base = slope_w(alpha, E, n0, r0, beta, c, M + 1)     # ik = nmln + 1
     e3u1 = d.arrays["e3u_n"][0, 0, 0]
     for k in range(2, M):              # inside the mixed layer: u-slope tapers with gdept
         f = (gdept[0, 0, k - 1] - 0.5 * e3u1) / gdept[0, 0, M - 2]
-        assert np.allclose(us[2:-3, 2:-3, k - 1], (f * su)[:, None], rtol=1e-9, atol=0), k
+        assert np.allclose(us[2:-3, 2:-3, k - 1], (f * su_base)[:, None], rtol=1e-9, atol=0), k
     for k in range(2, M + 1):          # w-slope tapers with gdepw down to the base
         f = gdepw[0, 0, k - 1] / gdepw[0, 0, M - 1]
-        assert np.allclose(wi[2:-3, 2:-3, k - 1], (f * sw)[:, None], rtol=1e-9, atol=0), k
+        assert np.allclose(wi[2:-3, 2:-3, k - 1], (f * sw_base)[:, None], rtol=1e-9, atol=0), k
     for k in range(M, d.jpk - 2):      # below: the interior slope
-        assert np.allclose(us[2:-3, 2:-3, k - 1], su[:, None], rtol=1e-9, atol=0), k
-    for k in range(M + 1, d.jpk - 2):
-        assert np.allclose(wi[2:-3, 2:-3, k - 1], sw[:, None], rtol=1e-9, atol=0), k
+        assert np.allclose(us[2:-3, 2:-3, k - 1], slope_u(alpha, E, n0, r0, beta, c, k)[:, None], rtol=1e-9,
+                           atol=0), k
+    for k in range(M + 1, d.jpk - 1):
+        assert np.allclose(wi[2:-3, 2:-3, k - 1], slope_w(alpha, E, n0, r0, beta, c, k)[:, None], rtol=1e-9,
+                           atol=0), k
 
 
 def test_ldf_slp_cyclic_ring_and_untouched_levels():
