@@ -49,6 +49,8 @@ extern "C" {
 #define NEMO_HIP_ERR_STATE       3   /* not initialised / no domain registered  */
 #define NEMO_HIP_ERR_HIP         4   /* HIP runtime error                       */
 #define NEMO_HIP_ERR_NUMERIC     5   /* stp_ctl-style blow-up detected          */
+#define NEMO_HIP_ERR_IO          6   /* restart file missing, unreadable or not
+                                        matching the registered domain          */
 
 #define NEMO_HIP_PTR_HOST   0
 #define NEMO_HIP_PTR_DEVICE 1
@@ -155,6 +157,28 @@ typedef struct nemo_hip_eos {
     double  rn_mu1, rn_mu2;     /* thermobaric in T, S                      */
     double  rn_nu;              /* cabbeling in T*S                         */
 } nemo_hip_eos_t;
+
+/* ------------------------------------------------------------------------ */
+/* restart state: the variables of rst_write / rst_read                     */
+/* (IOM/restart.F90:133-191, 240-309) and ts_rst (DYN/dynspg_ts.F90:        */
+/* 1391-1463), as a directory holding manifest.json and one raw array per   */
+/* variable (<name>.bin, Fortran order, little-endian "f8" or "f4")         */
+/* ------------------------------------------------------------------------ */
+typedef struct nemo_hip_rst {
+    int32_t real_bytes;         /* 8: the 3D fields are double; 4: float
+                                   (the _sp storage form)                   */
+    int32_t reserved0;
+    void   *ub, *vb, *tsb;      /* before fields; tsb (jpi,jpj,jpk,2) holds
+                                   the variables tb and sb                  */
+    void   *un, *vn, *tsn;      /* now fields (tn, sn)                      */
+    void   *rhop;               /* potential density                        */
+    double *sshb, *sshn;        /* (jpi,jpj)                                */
+    double *ub2_b, *vb2_b, *un_bf, *vn_bf;   /* ts_rst, (jpi,jpj)           */
+    /* on read only, optional: the depth-mean velocities DOM/istate.F90:
+       159-178 derives after a restart, from r1_hu_n / r1_hv_n            */
+    double *un_b, *vn_b, *ub_b, *vb_b;
+    const double *r1_hu_n, *r1_hv_n;
+} nemo_hip_rst_t;
 
 /* ------------------------------------------------------------------------ */
 /* library control                                                          */
@@ -340,6 +364,18 @@ int nemo_hip_rab_3d(const double *pts, double *pab, const double *gdept);
 /* bn2, TRA/eosbn2.F90:889-933: N^2 at w-points, levels 2..jpkm1 */
 int nemo_hip_bn2(const double *pts, const double *pab, double *pn2,
                  const double *gdept, const double *gdepw);
+/* rst_write, IOM/restart.F90:133-191, with ts_rst WRITE
+   (DYN/dynspg_ts.F90:1440-1451): writes rdt, kt and the fields of rst into
+   the directory dir (created if missing).  ln_bt_av = F (the barotropic
+   *_e levels) is not supported. */
+int nemo_hip_rst_write(const char *dir, int kt, const nemo_hip_rst_t *rst);
+/* rst_read, IOM/restart.F90:240-309, with ts_rst READ (dynspg_ts.F90:
+   1402-1437): fills the fields of rst from dir.  *neuler is 0 (Euler
+   restart: before fields set to the now fields, the ts_rst fields to zero)
+   when the file's rdt differs from the domain's or it holds no before
+   fields, else 1; *kt is the step the file was written at.  When un_b is
+   not NULL the four depth means are computed as DOM/istate.F90:159-178. */
+int nemo_hip_rst_read(const char *dir, nemo_hip_rst_t *rst, int *kt, int *neuler);
 /* zdf_mxl, ZDF/zdfmxl.F90:64-140: the mixed-layer base level nmln
    (jpi,jpj), its w-depth hmlp and the depth hmlpt of its last t-point, from
    rn2b (jpi,jpj,jpk) and the rho_c density criterion (0.01 in the

@@ -6,6 +6,7 @@
 // pool of staging buffers.  Every entry point validates its arguments,
 // stages host arrays if needed, launches the kernels of tra.hip / dyn.hip /
 // spg.hip and returns a status; no exception or abort crosses the ABI.
+#include <cerrno>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -13,6 +14,8 @@
 #include <map>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "common.hpp"
 #include "kernels.hpp"
@@ -1516,6 +1519,238 @@ int nemo_hip_dyn_nxt(int kt, double *ub, double *vb, double *un, double *vn, dou
 {
     return do_dyn_nxt<double>(kt, ub, vb, un, vn, ua, va, ua_b, va_b, un_adv, vn_adv, ub_b, vb_b, un_b, vn_b,
                               r1_hu_n, r1_hv_n);
+}
+
+// ---------------------------------------------------------------------------
+// restart files (rst_write, rst_read, ts_rst): a directory holding
+// manifest.json and one raw Fortran-ordered array per variable.  The
+// variable names are the reference's restart names.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct RstVar {
+    const char *name;
+    void *ptr;      // caller's pointer, host or device per the pointer mode
+    size_t n;       // elements
+    int width;      // bytes per element in the caller's array
+    bool before;    // a before field (absent from a file: Euler restart)
+    bool ts;        // a ts_rst field
+};
+
+std::vector<RstVar> rst_vars(const nemo_hip_rst_t *r)
+{
+    const size_t n2 = g_c->dom.n2, n3 = g_c->dom.n3;
+    const int w = r->real_bytes;
+    auto tr = [&](void *p, int t) { return p ? (void *)((char *)p + (size_t)t * n3 * (size_t)w) : nullptr; };
+    return {{"ub", r->ub, n3, w, true, false},          {"vb", r->vb, n3, w, true, false},
+            {"tb", tr(r->tsb, 0), n3, w, true, false},  {"sb", tr(r->tsb, 1), n3, w, true, false},
+            {"sshb", r->sshb, n2, 8, true, false},      {"un", r->un, n3, w, false, false},
+            {"vn", r->vn, n3, w, false, false},         {"tn", tr(r->tsn, 0), n3, w, false, false},
+            {"sn", tr(r->tsn, 1), n3, w, false, false}, {"sshn", r->sshn, n2, 8, false, false},
+            {"rhop", r->rhop, n3, w, false, false},     {"ub2_b", r->ub2_b, n2, 8, false, true},
+            {"vb2_b", r->vb2_b, n2, 8, false, true},    {"un_bf", r->un_bf, n2, 8, false, true},
+            {"vn_bf", r->vn_bf, n2, 8, false, true}};
+}
+
+int rst_check(const nemo_hip_rst_t *r, const char *who)
+{
+    if (r->real_bytes != 8 && r->real_bytes != 4)
+        return set_error(NEMO_HIP_ERR_ARG, "%s: real_bytes = %d (8 or 4)", who, r->real_bytes);
+    if (!g_c->dom.ln_bt_av)
+        return set_error(NEMO_HIP_ERR_UNSUPPORTED,
+                         "%s: ln_bt_av = F (the barotropic *_e levels of ts_rst) is not supported", who);
+    for (const auto &v : rst_vars(r))
+        if (!v.ptr) return set_error(NEMO_HIP_ERR_ARG, "%s: %s is NULL", who, v.name);
+    return 0;
+}
+
+// the value after "key": in the manifest text (numbers only)
+bool json_num(const std::string &t, const char *key, double *out)
+{
+    const std::string k = std::string("\"") + key + "\":";
+    const size_t q = t.find(k);
+    if (q == std::string::npos) return false;
+    const char *b = t.c_str() + q + k.size();
+    char *e = nullptr;
+    *out = strtod(b, &e);
+    return e != b;
+}
+
+// the dtype of variable `name` in the manifest, "" when absent
+std::string json_dtype(const std::string &t, const char *name)
+{
+    const std::string k = std::string("\"name\": \"") + name + "\"";
+    const size_t q = t.find(k);
+    if (q == std::string::npos) return "";
+    const size_t end = t.find('}', q);
+    const size_t d = t.find("\"dtype\": \"", q);
+    if (d == std::string::npos || d > end) return "";
+    return t.substr(d + 10, 2);
+}
+
+}  // namespace
+
+int nemo_hip_rst_write(const char *dir, int kt, const nemo_hip_rst_t *r)
+{
+    NEED_DOMAIN();
+    NEED_NOT_CAPTURING("nemo_hip_rst_write");
+    NEED_PTR(dir); NEED_PTR(r);
+    int rc = rst_check(r, "rst_write");
+    if (rc) return rc;
+    if (mkdir(dir, 0755) != 0 && errno != EEXIST)
+        return set_error(NEMO_HIP_ERR_IO, "rst_write: cannot create %s: %s", dir, strerror(errno));
+    HIPCHK(hipStreamSynchronize(g_stream));   // the fields' producers have finished
+    const DevDomain &d = g_c->dom;
+    char head[512];
+    snprintf(head, sizeof head,
+             "{\n  \"format\": \"nemo-hip-restart\",\n  \"version\": 1,\n  \"kt\": %d,\n  \"rdt\": %.17g,\n"
+             "  \"jpi\": %d,\n  \"jpj\": %d,\n  \"jpk\": %d,\n  \"jperio\": %d,\n  \"variables\": [\n",
+             kt, g_c->hdom.rdt, d.jpi, d.jpj, d.jpk, g_c->hdom.jperio);
+    std::string man = head;
+    std::vector<char> buf;
+    const auto vars = rst_vars(r);
+    for (size_t q = 0; q < vars.size(); q++) {
+        const RstVar &v = vars[q];
+        const size_t bytes = v.n * (size_t)v.width;
+        const void *src = v.ptr;
+        if (g_ptr_mode == NEMO_HIP_PTR_DEVICE) {
+            buf.resize(bytes);
+            HIPCHK(hipMemcpy(buf.data(), v.ptr, bytes, hipMemcpyDeviceToHost));
+            src = buf.data();
+        }
+        const std::string path = std::string(dir) + "/" + v.name + ".bin";
+        FILE *f = fopen(path.c_str(), "wb");
+        if (!f) return set_error(NEMO_HIP_ERR_IO, "rst_write: cannot open %s: %s", path.c_str(), strerror(errno));
+        const size_t wrote = fwrite(src, 1, bytes, f);
+        if (fclose(f) != 0 || wrote != bytes)
+            return set_error(NEMO_HIP_ERR_IO, "rst_write: writing %s failed", path.c_str());
+        char ent[256];
+        if (v.n == d.n3)
+            snprintf(ent, sizeof ent, "    {\"name\": \"%s\", \"file\": \"%s.bin\", \"dtype\": \"f%d\", \"shape\": [%d, %d, %d]}",
+                     v.name, v.name, v.width, d.jpi, d.jpj, d.jpk);
+        else
+            snprintf(ent, sizeof ent, "    {\"name\": \"%s\", \"file\": \"%s.bin\", \"dtype\": \"f%d\", \"shape\": [%d, %d]}",
+                     v.name, v.name, v.width, d.jpi, d.jpj);
+        man += ent;
+        man += (q + 1 < vars.size()) ? ",\n" : "\n";
+    }
+    man += "  ]\n}\n";
+    // the manifest last, by rename: a directory with a manifest is complete
+    const std::string mp = std::string(dir) + "/manifest.json", tmp = mp + ".tmp";
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return set_error(NEMO_HIP_ERR_IO, "rst_write: cannot open %s: %s", tmp.c_str(), strerror(errno));
+    const size_t wrote = fwrite(man.data(), 1, man.size(), f);
+    if (fclose(f) != 0 || wrote != man.size() || rename(tmp.c_str(), mp.c_str()) != 0)
+        return set_error(NEMO_HIP_ERR_IO, "rst_write: writing %s failed", mp.c_str());
+    return NEMO_HIP_OK;
+}
+
+int nemo_hip_rst_read(const char *dir, nemo_hip_rst_t *r, int *kt, int *neuler)
+{
+    NEED_DOMAIN();
+    NEED_NOT_CAPTURING("nemo_hip_rst_read");
+    NEED_PTR(dir); NEED_PTR(r); NEED_PTR(kt); NEED_PTR(neuler);
+    int rc = rst_check(r, "rst_read");
+    if (rc) return rc;
+    if (r->un_b && (!r->vn_b || !r->ub_b || !r->vb_b || !r->r1_hu_n || !r->r1_hv_n))
+        return set_error(NEMO_HIP_ERR_ARG, "rst_read: un_b given without vn_b, ub_b, vb_b, r1_hu_n and r1_hv_n");
+    const std::string mp = std::string(dir) + "/manifest.json";
+    std::string man;
+    {
+        FILE *f = fopen(mp.c_str(), "rb");
+        if (!f) return set_error(NEMO_HIP_ERR_IO, "rst_read: cannot open %s: %s", mp.c_str(), strerror(errno));
+        char c[4096];
+        size_t got;
+        while ((got = fread(c, 1, sizeof c, f)) > 0) man.append(c, got);
+        fclose(f);
+    }
+    const DevDomain &d = g_c->dom;
+    double v_kt, v_rdt, v_i, v_j, v_k;
+    if (man.find("\"format\": \"nemo-hip-restart\"") == std::string::npos || !json_num(man, "kt", &v_kt) ||
+        !json_num(man, "rdt", &v_rdt) || !json_num(man, "jpi", &v_i) || !json_num(man, "jpj", &v_j) ||
+        !json_num(man, "jpk", &v_k))
+        return set_error(NEMO_HIP_ERR_IO, "rst_read: %s is not a restart manifest", mp.c_str());
+    if ((int)v_i != d.jpi || (int)v_j != d.jpj || (int)v_k != d.jpk)
+        return set_error(NEMO_HIP_ERR_IO, "rst_read: %s holds a %dx%dx%d domain, %dx%dx%d is registered", dir,
+                         (int)v_i, (int)v_j, (int)v_k, d.jpi, d.jpj, d.jpk);
+    // restart.F90:255-258 and 273-282: a changed time step or no before
+    // fields force an Euler restart
+    int ne = (v_rdt == g_c->hdom.rdt) ? 1 : 0;
+    const bool have_before = !json_dtype(man, "ub").empty();
+    if (!have_before) ne = 0;
+    const bool read_ts = ne != 0 && d.ln_bt_fw;    // dynspg_ts.F90:1402
+    HIPCHK(hipStreamSynchronize(g_stream));          // no queued work still uses the destinations
+    const bool dev = (g_ptr_mode == NEMO_HIP_PTR_DEVICE);
+    std::vector<char> raw, conv;
+    for (const auto &v : rst_vars(r)) {
+        if ((v.before && !have_before) || (v.ts && !read_ts)) continue;
+        const std::string dt = json_dtype(man, v.name);
+        if (dt.empty()) {
+            if (!strcmp(v.name, "rhop")) continue;   // optional (restart.F90:288-292)
+            return set_error(NEMO_HIP_ERR_IO, "rst_read: %s has no variable %s", dir, v.name);
+        }
+        const int fw = dt == "f8" ? 8 : dt == "f4" ? 4 : 0;
+        if (!fw) return set_error(NEMO_HIP_ERR_IO, "rst_read: %s has dtype %s", v.name, dt.c_str());
+        const std::string path = std::string(dir) + "/" + v.name + ".bin";
+        FILE *f = fopen(path.c_str(), "rb");
+        if (!f) return set_error(NEMO_HIP_ERR_IO, "rst_read: cannot open %s: %s", path.c_str(), strerror(errno));
+        raw.resize(v.n * (size_t)fw);
+        const size_t got = fread(raw.data(), 1, raw.size(), f);
+        const bool extra = fgetc(f) != EOF;
+        fclose(f);
+        if (got != raw.size() || extra)
+            return set_error(NEMO_HIP_ERR_IO, "rst_read: %s does not hold %zu values of %d bytes", path.c_str(), v.n, fw);
+        const char *src = raw.data();
+        if (fw != v.width) {   // widen f4 to double, or round f8 to float
+            conv.resize(v.n * (size_t)v.width);
+            for (size_t q = 0; q < v.n; q++) {
+                if (v.width == 8) ((double *)conv.data())[q] = (double)((const float *)raw.data())[q];
+                else ((float *)conv.data())[q] = (float)((const double *)raw.data())[q];
+            }
+            src = conv.data();
+        }
+        const size_t bytes = v.n * (size_t)v.width;
+        if (dev) HIPCHK(hipMemcpy(v.ptr, src, bytes, hipMemcpyHostToDevice));
+        else memcpy(v.ptr, src, bytes);
+    }
+    const size_t n2 = d.n2, n3 = d.n3, w = (size_t)r->real_bytes;
+    auto copy = [&](void *dst, const void *src, size_t bytes) -> int {
+        if (dev) HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+        else memcpy(dst, src, bytes);
+        return 0;
+    };
+    if (ne == 0) {   // restart.F90:296-307: before fields = now fields
+        if ((rc = copy(r->tsb, r->tsn, 2 * n3 * w)) || (rc = copy(r->ub, r->un, n3 * w)) ||
+            (rc = copy(r->vb, r->vn, n3 * w)) || (rc = copy(r->sshb, r->sshn, n2 * 8)))
+            return rc;
+    }
+    if (!read_ts) {  // dynspg_ts.F90:1422-1424: start from rest
+        for (double *p : {r->ub2_b, r->vb2_b, r->un_bf, r->vn_bf}) {
+            if (dev) HIPCHK(hipMemset(p, 0, n2 * 8));
+            else memset(p, 0, n2 * 8);
+        }
+    }
+    if (r->un_b) {   // istate.F90:159-178
+        Stager S;
+        if (r->real_bytes == 8) {
+            const double *un = S.in((const double *)r->un, n3), *vn = S.in((const double *)r->vn, n3);
+            const double *ub = S.in((const double *)r->ub, n3), *vb = S.in((const double *)r->vb, n3);
+            const double *hu = S.in(r->r1_hu_n, n2), *hv = S.in(r->r1_hv_n, n2);
+            double *a = S.out(r->un_b, n2), *b = S.out(r->vn_b, n2), *c = S.out(r->ub_b, n2), *e = S.out(r->vb_b, n2);
+            rc = S.status ? S.status : launch_istate_bt<double>(un, vn, ub, vb, hu, hv, a, b, c, e);
+        } else {
+            const float *un = S.in((const float *)r->un, n3), *vn = S.in((const float *)r->vn, n3);
+            const float *ub = S.in((const float *)r->ub, n3), *vb = S.in((const float *)r->vb, n3);
+            const double *hu = S.in(r->r1_hu_n, n2), *hv = S.in(r->r1_hv_n, n2);
+            double *a = S.out(r->un_b, n2), *b = S.out(r->vn_b, n2), *c = S.out(r->ub_b, n2), *e = S.out(r->vb_b, n2);
+            rc = S.status ? S.status : launch_istate_bt<float>(un, vn, ub, vb, hu, hv, a, b, c, e);
+        }
+        if ((rc = S.finish(rc))) return rc;
+        HIPCHK(hipStreamSynchronize(g_stream));
+    }
+    *kt = (int)v_kt;
+    *neuler = ne;
+    return NEMO_HIP_OK;
 }
 
 int nemo_hip_eos_register(const nemo_hip_eos_t *eos)

@@ -58,6 +58,9 @@ struct SeosPar {   // simplified EOS coefficients, TRA/eosbn2.F90:89-95
 template <typename T> int launch_eos(int neos, const SeosPar &e, int pot, const T *pts, T *prd, T *prhop, const T *pdep);
 template <typename T> int launch_rab(int neos, const SeosPar &e, const T *pts, T *pab, const T *gdept);
 template <typename T>
+int launch_istate_bt(const T *un, const T *vn, const T *ub, const T *vb, const double *r1_hu, const double *r1_hv,
+                     double *un_b, double *vn_b, double *ub_b, double *vb_b);
+template <typename T>
 int launch_zdf_mxl(const T *rn2b, const T *gdept, const T *gdepw, int nlb10, double rho_c, int32_t *nmln,
                    double *hmlp, double *hmlpt);
 template <typename T>

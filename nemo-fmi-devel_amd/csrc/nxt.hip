@@ -565,7 +565,50 @@ int launch_tra_sbc(int first, double *qns, double *qsr, const double *sfx, const
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// the depth-mean velocities after a restart, DOM/istate.F90:159-178: every
+// column, levels 1..jpkm1 summed in order from zero (linear free surface:
+// the before scale factors and inverse depths are the now ones)
+// ---------------------------------------------------------------------------
+template <typename T>
+KERNEL void k_istate_bt(DevDomain d, const T *__restrict__ un, const T *__restrict__ vn, const T *__restrict__ ub,
+                        const T *__restrict__ vb, const double *__restrict__ r1_hu, const double *__restrict__ r1_hv,
+                        double *__restrict__ un_b, double *__restrict__ vn_b, double *__restrict__ ub_b,
+                        double *__restrict__ vb_b)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    const Metrics<double> &m = d.md;
+    const uint32_t ij = IJ32(i, j);
+    double a = 0., b = 0., c = 0., e = 0.;
+    for (int k = 1; k <= d.jpk - 1; k++) {
+        const double eu = E3(m.e3u, ij, k), ev = E3(m.e3v, ij, k);
+        const double um = UMASK(double, i, j, k), vm = VMASK(double, i, j, k);
+        const uint32_t p = IJK32(i, j, k);
+        a = a + eu * (double)at32(un, p) * um;
+        b = b + ev * (double)at32(vn, p) * vm;
+        c = c + eu * (double)at32(ub, p) * um;
+        e = e + ev * (double)at32(vb, p) * vm;
+    }
+    un_b[ij] = a * r1_hu[ij];
+    vn_b[ij] = b * r1_hv[ij];
+    ub_b[ij] = c * r1_hu[ij];
+    vb_b[ij] = e * r1_hv[ij];
+}
+
+template <typename T>
+int launch_istate_bt(const T *un, const T *vn, const T *ub, const T *vb, const double *r1_hu, const double *r1_hv,
+                     double *un_b, double *vn_b, double *ub_b, double *vb_b)
+{
+    const DevDomain &d = nh::dom();
+    Grid2 g = grid2(d.jpi, d.jpj);
+    LAUNCH("rst_read", k_istate_bt<T>, g, d, un, vn, ub, vb, r1_hu, r1_hv, un_b, vn_b, ub_b, vb_b);
+    return 0;
+}
+
 #define INST(T)                                                                                                  \
+    template int launch_istate_bt<T>(const T *, const T *, const T *, const T *, const double *, const double *,  \
+                                     double *, double *, double *, double *);                                    \
     template int launch_div_hor<T>(const T *, const T *, T *);                                                   \
     template int launch_ssh_nxt<T>(double, double, const T *, const T *, T *, const double *, const double *,    \
                                    const double *, double *);                                                    \

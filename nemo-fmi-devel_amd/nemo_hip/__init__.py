@@ -385,6 +385,30 @@ class NemoHip:
         p = lambda a: self._p(a, dt)
         self._check(fn(p(pts), p(pab), p(pn2), p(gdept), p(gdepw)))
 
+    def _rst(self, fields, sp):
+        r = abi.RstStruct()
+        r.real_bytes = 4 if sp else 8
+        dt3 = np.float32 if sp else np.float64
+        for n in abi.RST_FIELDS:
+            a = fields.get(n)
+            if a is not None:
+                two_d = n in ("sshb", "sshn", "ub2_b", "vb2_b", "un_bf", "vn_bf", "un_b", "vn_b", "ub_b", "vb_b",
+                              "r1_hu_n", "r1_hv_n")
+                setattr(r, n, self._p(a, np.float64 if two_d else dt3).value)
+        return r
+
+    def rst_write(self, path, kt, sp=False, **fields):
+        """rst_write: the fields named as in nemo_hip_rst_t into the directory path."""
+        r = self._rst(fields, sp)
+        self._check(self.L.nemo_hip_rst_write(str(path).encode(), C.c_int(kt), C.byref(r)))
+
+    def rst_read(self, path, sp=False, **fields):
+        """rst_read: fills the fields; returns (kt, neuler)."""
+        r = self._rst(fields, sp)
+        kt, ne = C.c_int(), C.c_int()
+        self._check(self.L.nemo_hip_rst_read(str(path).encode(), C.byref(r), C.byref(kt), C.byref(ne)))
+        return kt.value, ne.value
+
     def zdf_mxl(self, kt, rn2b, gdept, gdepw, nlb10, nmln, hmlp, hmlpt, rho_c=0.01, sp=False):
         """zdf_mxl: nmln (int32), hmlp, hmlpt (FP64, jpi x jpj) from rn2b."""
         fn = self.L.nemo_hip_zdf_mxl_sp if sp else self.L.nemo_hip_zdf_mxl

@@ -99,3 +99,12 @@ def eos80():
     e = seos()
     e.neos = NP_EOS80
     return e
+
+
+# restart state, nemo_hip_rst_t (IOM/restart.F90:133-309, DYN/dynspg_ts.F90:1391-1463)
+RST_FIELDS = ["ub", "vb", "tsb", "un", "vn", "tsn", "rhop", "sshb", "sshn", "ub2_b", "vb2_b", "un_bf", "vn_bf",
+              "un_b", "vn_b", "ub_b", "vb_b", "r1_hu_n", "r1_hv_n"]
+
+
+class RstStruct(C.Structure):
+    _fields_ = [("real_bytes", C.c_int32), ("reserved0", C.c_int32)] + [(n, C.c_void_p) for n in RST_FIELDS]

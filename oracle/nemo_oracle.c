@@ -2485,7 +2485,8 @@ int ora_abi_layout(size_t *out, int n)
                   offsetof(nemo_hip_domain_t, e1t), offsetof(nemo_hip_domain_t, e3t_n),
                   offsetof(nemo_hip_domain_t, e3vw_n), sizeof(nemo_hip_spg_ts_t),
                   offsetof(nemo_hip_spg_ts_t, un_b), offsetof(nemo_hip_spg_ts_t, vn_bf), sizeof(nemo_hip_eos_t),
-                  offsetof(nemo_hip_eos_t, rn_a0), offsetof(nemo_hip_eos_t, rn_nu)};
+                  offsetof(nemo_hip_eos_t, rn_a0), offsetof(nemo_hip_eos_t, rn_nu),
+                  sizeof(nemo_hip_rst_t), offsetof(nemo_hip_rst_t, rhop), offsetof(nemo_hip_rst_t, r1_hv_n)};
     int m = (int)(sizeof v / sizeof v[0]);
     for (int q = 0; q < n && q < m; q++) out[q] = v[q];
     return m;

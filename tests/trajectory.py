@@ -133,8 +133,10 @@ def oracle_run(d, S, marks, physics=False, eos=None):
     return out
 
 
-def gpu_run(h, d, S, marks, sp=False, physics=False, eos=None):
-    """h: the library handle in device-pointer mode with d registered."""
+def gpu_run(h, d, S, marks, sp=False, physics=False, eos=None, kt0=1, X0=None, keep=None):
+    """h: the library handle in device-pointer mode with d registered.
+    Steps kt0 .. max(marks); X0 (device tensors by field name) replaces the
+    initial state where given; `keep`, a dict, receives the final fields."""
     import torch
     from nemo_hip import abi, configs
     dt = torch.float32 if sp else torch.float64
@@ -145,6 +147,7 @@ def gpu_run(h, d, S, marks, sp=False, physics=False, eos=None):
     X = {k: dev(v, dt if ndim[k] >= 3 else torch.float64) for k, v in initial(S).items()}
     for k in ("ua_b", "va_b", "ssha", "un_adv", "vn_adv"):
         X[k] = torch.zeros(n2, dtype=torch.float64, device="cuda")
+    X.update(X0 or {})
     z = lambda n: torch.zeros(n, dtype=dt, device="cuda")
     hdivn, ua, va, tsa, ah, akz = z(n3), z(n3), z(n3), z(2 * n3), z(n3), z(n3)
     F = {k: X[k] for k in SPG_2D + ["ua_b", "va_b", "ssha", "un_adv", "vn_adv"]}
@@ -167,7 +170,7 @@ def gpu_run(h, d, S, marks, sp=False, physics=False, eos=None):
     out = {}
     # torch work (the trend reset) is queued on the library stream, in order with the library's launches
     lib_stream = torch.cuda.ExternalStream(h.stream())
-    for kt in range(1, max(marks) + 1):
+    for kt in range(kt0, max(marks) + 1):
         if physics:
             h.rab_3d(X["tsb"], pab, gdept, sp=sp)
             h.bn2(X["tsb"], pab, rn2b, gdept, gdepw, sp=sp)
@@ -216,4 +219,6 @@ def gpu_run(h, d, S, marks, sp=False, physics=False, eos=None):
             torch.cuda.synchronize()
             h.sync()
             out[kt] = {k: X[k].cpu().numpy().astype(np.float64).reshape(shape[k], order="F") for k in shape}
+    if keep is not None:
+        keep.update(X)
     return out
