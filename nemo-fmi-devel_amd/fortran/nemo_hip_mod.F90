@@ -20,6 +20,7 @@ MODULE nemo_hip_mod
    !!   hpg_zco        DYN/dynhpg.F90:258           dyn_ldf_lap    DYN/dynldf_lap_blp.F90:37
    !!   tra_sbc        TRA/trasbc.F90:62            stp_ctl        stpctl.F90:55
    !!   zdf_mxl        ZDF/zdfmxl.F90:64            ldf_slp        LDF/ldfslp.F90:83
+   !!   rst_write      IOM/restart.F90:133          rst_read       IOM/restart.F90:240
    !! The arrays those routines take from NEMO modules (oce, zdf_oce, ldfslp,
    !! sbc_oce, zdfdrg, dynspg_ts, dom_oce) are bound once, after allocation, with the
    !! nemo_hip_bind_* routines: module arrays never move, so their addresses
@@ -40,6 +41,7 @@ MODULE nemo_hip_mod
    PUBLIC :: nemo_hip_eos_t, nemo_hip_eos_init, nemo_hip_bind_phys
    PUBLIC :: eos_insitu, eos_insitu_pot, rab_3d, bn2, hpg_zco, dyn_ldf_lap, tra_sbc, stp_ctl
    PUBLIC :: nemo_hip_bind_mxl, zdf_mxl, ldf_slp
+   PUBLIC :: nemo_hip_rst_t, rst_write, rst_read
    PUBLIC :: nemo_hip_domain_t, nemo_hip_spg_ts_t
    PUBLIC :: nemo_hip_field_alloc, nemo_hip_field_free, nemo_hip_field_push, nemo_hip_field_pull
 
@@ -75,6 +77,12 @@ MODULE nemo_hip_mod
       TYPE(C_PTR) :: ua_b, va_b, ssha, un_adv, vn_adv
       TYPE(C_PTR) :: ub2_b, vb2_b, un_bf, vn_bf
    END TYPE nemo_hip_spg_ts_t
+
+   TYPE, BIND(C) :: nemo_hip_rst_t
+      INTEGER(C_INT32_T) :: real_bytes, reserved0
+      TYPE(C_PTR) :: ub, vb, tsb, un, vn, tsn, rhop, sshb, sshn, ub2_b, vb2_b, un_bf, vn_bf
+      TYPE(C_PTR) :: un_b, vn_b, ub_b, vb_b, r1_hu_n, r1_hv_n
+   END TYPE nemo_hip_rst_t
 
    !! module arrays bound at initialisation
    TYPE(C_PTR), SAVE :: p_avt = C_NULL_PTR, p_avs = C_NULL_PTR, p_avm = C_NULL_PTR
@@ -253,6 +261,18 @@ MODULE nemo_hip_mod
       INTEGER(C_INT) FUNCTION c_bn2(pts, pab, pn2, gdept, gdepw) BIND(C, name="nemo_hip_bn2")
          IMPORT :: C_INT, C_PTR
          TYPE(C_PTR), VALUE :: pts, pab, pn2, gdept, gdepw
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_rst_write(dir, kt, rst) BIND(C, name="nemo_hip_rst_write")
+         IMPORT :: C_INT, C_CHAR, nemo_hip_rst_t
+         CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(in) :: dir
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(nemo_hip_rst_t), INTENT(in) :: rst
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_rst_read(dir, rst, kt, neuler) BIND(C, name="nemo_hip_rst_read")
+         IMPORT :: C_INT, C_CHAR, nemo_hip_rst_t
+         CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(in) :: dir
+         TYPE(nemo_hip_rst_t), INTENT(inout) :: rst
+         INTEGER(C_INT), INTENT(out) :: kt, neuler
       END FUNCTION
       INTEGER(C_INT) FUNCTION c_zdf_mxl(kt, rn2b, gdept, gdepw, nlb10, rho_c, nmln, hmlp, hmlpt) &
          &                          BIND(C, name="nemo_hip_zdf_mxl")
@@ -589,6 +609,40 @@ CONTAINS
       REAL(wp), DIMENSION(:,:,:)  , TARGET, CONTIGUOUS, INTENT(  out) ::   pn2
       CALL check( c_bn2( C_LOC(pts), C_LOC(pab), C_LOC(pn2), p_gdept, p_gdepw ), 'bn2' )
    END SUBROUTINE bn2
+
+   FUNCTION rst_state( prhop ) RESULT( r )
+      !! the restart variables from the bound oce and dynspg_ts arrays
+      TYPE(C_PTR), INTENT(in) :: prhop
+      TYPE(nemo_hip_rst_t) :: r
+      r%real_bytes = 8 ; r%reserved0 = 0
+      r%ub = p_ub ; r%vb = p_vb ; r%tsb = p_tsb ; r%un = p_un ; r%vn = p_vn ; r%tsn = p_tsn ; r%rhop = prhop
+      r%sshb = p_sshb ; r%sshn = p_sshn
+      r%ub2_b = spg_state%ub2_b ; r%vb2_b = spg_state%vb2_b ; r%un_bf = spg_state%un_bf ; r%vn_bf = spg_state%vn_bf
+      r%un_b = spg_state%un_b ; r%vn_b = spg_state%vn_b ; r%ub_b = spg_state%ub_b ; r%vb_b = spg_state%vb_b
+      r%r1_hu_n = spg_state%r1_hu_n ; r%r1_hv_n = spg_state%r1_hv_n
+   END FUNCTION rst_state
+
+   SUBROUTINE rst_write( kt, cdir, prhop )
+      !! IOM/restart.F90:133 with ts_rst WRITE: the bound before and now fields,
+      !! sea levels, barotropic time-filter fields and prhop into directory cdir
+      INTEGER, INTENT(in) :: kt
+      CHARACTER(len=*), INTENT(in) :: cdir
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(in) :: prhop
+      CALL check( c_rst_write( cstr(TRIM(cdir)), INT(kt, C_INT), rst_state( C_LOC(prhop) ) ), 'rst_write' )
+   END SUBROUTINE rst_write
+
+   SUBROUTINE rst_read( cdir, prhop, kt, kneuler )
+      !! IOM/restart.F90:240 with ts_rst READ and the depth means of
+      !! DOM/istate.F90:159-178; kneuler = 0 for an Euler restart
+      CHARACTER(len=*), INTENT(in) :: cdir
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(inout) :: prhop
+      INTEGER, INTENT(out) :: kt, kneuler
+      TYPE(nemo_hip_rst_t) :: r
+      INTEGER(C_INT) :: ikt, ine
+      r = rst_state( C_LOC(prhop) )
+      CALL check( c_rst_read( cstr(TRIM(cdir)), r, ikt, ine ), 'rst_read' )
+      kt = ikt ; kneuler = ine
+   END SUBROUTINE rst_read
 
    SUBROUTINE zdf_mxl( kt )
       !! ZDF/zdfmxl.F90:64: nmln, hmlp, hmlpt from rn2b (hmld is not computed)
