@@ -394,6 +394,12 @@ int ora_interp_4th_cpt(const double *pt_in, double *pt_out)
 /* ------------------------------------------------------------------------ */
 /* nonosc, TRA/traadv_fct.F90:330-428                                   */
 /* ------------------------------------------------------------------------ */
+/* limiter factors of the last nonosc call are recorded here when set (the
+   frozen-limiter tangent linear and adjoint below) */
+static double *fct_rec_x, *fct_rec_y, *fct_rec_z;
+/* when set, tra_adv_fct applies these factors instead of calling nonosc */
+static const double *fct_frz_x, *fct_frz_y, *fct_frz_z;
+
 static void nonosc(const double *pbef, double *paa, double *pbb, double *pcc, const double *paft, double p2dt)
 {
     int i, j, k, ikm1;
@@ -442,14 +448,17 @@ static void nonosc(const double *pbef, double *paa, double *pbb, double *pcc, co
                 double zau = MIN3(1.0, A3(zbetdo, i, j, k), A3(zbetup, i + 1, j, k));
                 double zbu = MIN3(1.0, A3(zbetup, i, j, k), A3(zbetdo, i + 1, j, k));
                 double zcu = (0.5 + SIGN(0.5, A3(paa, i, j, k)));
+                if (fct_rec_x) A3(fct_rec_x, i, j, k) = zcu * zau + (1.0 - zcu) * zbu;
                 A3(paa, i, j, k) = A3(paa, i, j, k) * (zcu * zau + (1.0 - zcu) * zbu);
                 double zav = MIN3(1.0, A3(zbetdo, i, j, k), A3(zbetup, i, j + 1, k));
                 double zbv = MIN3(1.0, A3(zbetup, i, j, k), A3(zbetdo, i, j + 1, k));
                 double zcv = (0.5 + SIGN(0.5, A3(pbb, i, j, k)));
+                if (fct_rec_y) A3(fct_rec_y, i, j, k) = zcv * zav + (1.0 - zcv) * zbv;
                 A3(pbb, i, j, k) = A3(pbb, i, j, k) * (zcv * zav + (1.0 - zcv) * zbv);
                 double za = MIN3(1., A3(zbetdo, i, j, k + 1), A3(zbetup, i, j, k));
                 double zb = MIN3(1., A3(zbetup, i, j, k + 1), A3(zbetdo, i, j, k));
                 double zc = (0.5 + SIGN(0.5, A3(pcc, i, j, k + 1)));
+                if (fct_rec_z) A3(fct_rec_z, i, j, k + 1) = zc * za + (1.0 - zc) * zb;
                 A3(pcc, i, j, k + 1) = A3(pcc, i, j, k + 1) * (zc * za + (1.0 - zc) * zb);
             }
     lbc3(paa, jpk, 'U', 0.0);
@@ -575,7 +584,19 @@ int ora_tra_adv_fct(int kt, int kit000, const char *cdtype, double p2dt, const d
         lbc3(zwx, jpk, 'U', 0.0);
         lbc3(zwy, jpk, 'V', 0.0);
         lbc3(zwz, jpk, 'W', 0.0);
-        nonosc(tb, zwx, zwy, zwz, zwi, p2dt);
+        if (fct_frz_x) {   /* the limiter factors given: the frozen-limiter tangent linear */
+            for (k = 1; k <= jpkm1; k++)
+                for (j = 2; j <= jpjm1; j++)
+                    for (i = 2; i <= jpim1; i++) {
+                        A3(zwx, i, j, k) = A3(zwx, i, j, k) * A3(fct_frz_x, i, j, k);
+                        A3(zwy, i, j, k) = A3(zwy, i, j, k) * A3(fct_frz_y, i, j, k);
+                        A3(zwz, i, j, k + 1) = A3(zwz, i, j, k + 1) * A3(fct_frz_z, i, j, k + 1);
+                    }
+            lbc3(zwx, jpk, 'U', 0.0);
+            lbc3(zwy, jpk, 'V', 0.0);
+        } else {
+            nonosc(tb, zwx, zwy, zwz, zwi, p2dt);
+        }
         /* final trend, lines 288-297 */
         for (k = 1; k <= jpkm1; k++)
             for (j = 2; j <= jpjm1; j++)
@@ -2476,6 +2497,248 @@ int ora_ldf_slp(int kt, const double *prd, const double *pn2, const double *gdep
     lbc3(wslpj, jpk, 'W', 0.);
     free(zgru); free(zgrv); free(zdzr); free(zwz); free(zww); free(oml);
     free(uslpml); free(vslpml); free(wslpiml); free(wslpjml); free(zslpml_hmlpu); free(zslpml_hmlpv);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* tangent linear and adjoint of the tracer step, one tracer: tra_adv_fct   */
+/* (orders 2/2) with the nonosc limiter factors frozen from the forward     */
+/* state, tra_ldf_lap (kpass 1), tra_zdf_imp (avt, no slopes) and the       */
+/* leapfrog tra_nxt.  Each adjoint reverses the exact forward sequence,     */
+/* lbc_lnk copies included (their transpose adds a ring value to its        */
+/* source and clears the ring).  Adjoint arrays are accumulated (+=).       */
+/* ------------------------------------------------------------------------ */
+
+/* the transpose of lbc3: rows first (lbc3 fills them last), then columns */
+static void lbc3_adj(double *p, int nk, char nat)
+{
+    int i, j, k;
+    for (k = 1; k <= nk; k++) {
+        if (l_Jperio()) {
+            for (i = 1; i <= jpi; i++) {
+                A3(p, i, jpjm1, k) += A3(p, i, 1, k);
+                A3(p, i, 2, k) += A3(p, i, jpj, k);
+            }
+        }
+        for (i = 1; i <= jpi; i++) {
+            if (l_Jperio() || nat != 'F') A3(p, i, 1, k) = 0.;
+            A3(p, i, jpj, k) = 0.;
+        }
+        if (l_Iperio()) {
+            for (j = 1; j <= jpj; j++) {
+                A3(p, jpim1, j, k) += A3(p, 1, j, k);
+                A3(p, 2, j, k) += A3(p, jpi, j, k);
+            }
+        }
+        for (j = 1; j <= jpj; j++) {
+            if (l_Iperio() || nat != 'F') A3(p, 1, j, k) = 0.;
+            A3(p, jpi, j, k) = 0.;
+        }
+    }
+}
+
+/* the limiter factors nonosc applies to the antidiffusive fluxes of the
+   state (tb, tn): cx, cy at interior faces, cz(k) for the face above level k
+   (k = 2..jpk); other points are left as they are */
+int ora_fct_coef(double p2dt, const double *pun, const double *pvn, const double *pwn, const double *tb,
+                 const double *tn, double *cx, double *cy, double *cz)
+{
+    double *ta = z3();
+    fct_rec_x = cx; fct_rec_y = cy; fct_rec_z = cz;
+    int st = ora_tra_adv_fct(1, 1, "TRC", p2dt, pun, pvn, pwn, tb, tn, ta, 1, 2, 2);
+    fct_rec_x = fct_rec_y = fct_rec_z = NULL;
+    free(ta);
+    return st;
+}
+
+/* pta += the FCT 2/2 trend with the limiter factors cx, cy, cz: linear in
+   (tb, tn); with the factors of the same state it is tra_adv_fct itself */
+int ora_fct_tl(double p2dt, const double *pun, const double *pvn, const double *pwn, const double *tb,
+               const double *tn, const double *cx, const double *cy, const double *cz, double *pta)
+{
+    fct_frz_x = cx; fct_frz_y = cy; fct_frz_z = cz;
+    int st = ora_tra_adv_fct(1, 1, "TRC", p2dt, pun, pvn, pwn, tb, tn, pta, 1, 2, 2);
+    fct_frz_x = fct_frz_y = fct_frz_z = NULL;
+    return st;
+}
+
+/* adjoint of ora_fct_tl: tb_ad += dJ/dtb and tn_ad += dJ/dtn from
+   ta_ad = dJ/d(pta) (interior cells; ring cells of tb_ad, tn_ad receive
+   the dependence on the ring values of tb, tn) */
+int ora_fct_ad(const double *pun, const double *pvn, const double *pwn, const double *cx, const double *cy,
+               const double *cz, const double *ta_ad, double *tb_ad, double *tn_ad)
+{
+    int i, j, k;
+    double *ax = z3(), *ay = z3(), *az = z3(), *xb = z3(), *yb = z3(), *zb = z3();
+    /* the final trend (lines 288-297): ta -= (dx + dy + dz) r1_e1e2t / e3t */
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                const double t = A3(ta_ad, i, j, k) * r1_e1e2t(i, j) / e3t_n(i, j, k);
+                A3(ax, i, j, k) -= t; A3(ax, i - 1, j, k) += t;
+                A3(ay, i, j, k) -= t; A3(ay, i, j - 1, k) += t;
+                A3(az, i, j, k) -= t; A3(az, i, j, k + 1) += t;
+            }
+    lbc3_adj(ax, jpk, 'U');
+    lbc3_adj(ay, jpk, 'V');
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                A3(ax, i, j, k) *= A3(cx, i, j, k);
+                A3(ay, i, j, k) *= A3(cy, i, j, k);
+                A3(az, i, j, k + 1) *= A3(cz, i, j, k + 1);
+            }
+    /* faces the limiting did not touch were overwritten before use */
+    for (k = 1; k <= jpk; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                const int in = (i >= 2 && i <= jpim1 && j >= 2 && j <= jpjm1);
+                if (!in || k == jpk) { A3(ax, i, j, k) = 0.; A3(ay, i, j, k) = 0.; }
+                if (!in || k == 1) A3(az, i, j, k) = 0.;
+            }
+    /* antidiffusive fluxes (lines 182-190, 255-263): A = high order - upstream */
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpjm1; j++)
+            for (i = 1; i <= jpim1; i++) {
+                A3(tn_ad, i, j, k) += 0.5 * A3(pun, i, j, k) * A3(ax, i, j, k);
+                A3(tn_ad, i + 1, j, k) += 0.5 * A3(pun, i, j, k) * A3(ax, i, j, k);
+                A3(tn_ad, i, j, k) += 0.5 * A3(pvn, i, j, k) * A3(ay, i, j, k);
+                A3(tn_ad, i, j + 1, k) += 0.5 * A3(pvn, i, j, k) * A3(ay, i, j, k);
+                A3(xb, i, j, k) -= A3(ax, i, j, k);
+                A3(yb, i, j, k) -= A3(ay, i, j, k);
+            }
+    for (k = 2; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                const double t = A3(az, i, j, k) * WMASK(i, j, k);
+                A3(tn_ad, i, j, k) += A3(pwn, i, j, k) * 0.5 * t;
+                A3(tn_ad, i, j, k - 1) += A3(pwn, i, j, k) * 0.5 * t;
+                A3(zb, i, j, k) -= t;
+            }
+    /* the low-order trend (lines 158-170) */
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                const double t = A3(ta_ad, i, j, k) * r1_e1e2t(i, j) / e3t_n(i, j, k) * TMASK(i, j, k);
+                A3(xb, i, j, k) -= t; A3(xb, i - 1, j, k) += t;
+                A3(yb, i, j, k) -= t; A3(yb, i, j - 1, k) += t;
+                A3(zb, i, j, k) -= t;
+                if (k + 1 <= jpkm1) A3(zb, i, j, k + 1) += t;
+            }
+    /* upstream fluxes (lines 123-156) */
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpjm1; j++)
+            for (i = 1; i <= jpim1; i++) {
+                const double u = A3(pun, i, j, k), v = A3(pvn, i, j, k);
+                A3(tb_ad, i, j, k) += 0.5 * (u + fabs(u)) * A3(xb, i, j, k);
+                A3(tb_ad, i + 1, j, k) += 0.5 * (u - fabs(u)) * A3(xb, i, j, k);
+                A3(tb_ad, i, j, k) += 0.5 * (v + fabs(v)) * A3(yb, i, j, k);
+                A3(tb_ad, i, j + 1, k) += 0.5 * (v - fabs(v)) * A3(yb, i, j, k);
+            }
+    for (k = 2; k <= jpkm1; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                const double w = A3(pwn, i, j, k), t = 0.5 * A3(zb, i, j, k) * WMASK(i, j, k);
+                A3(tb_ad, i, j, k) += (w + fabs(w)) * t;
+                A3(tb_ad, i, j, k - 1) += (w - fabs(w)) * t;
+            }
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) A3(tb_ad, i, j, 1) += A3(pwn, i, j, 1) * A3(zb, i, j, 1);
+    free(ax); free(ay); free(az); free(xb); free(yb); free(zb);
+    return 0;
+}
+
+/* adjoint of tra_ldf_lap (kpass 1, one tracer): tb_ad += from ta_ad */
+int ora_ldf_lap_ad(const double *pahu, const double *pahv, const double *ta_ad, double *tb_ad)
+{
+    int i, j, k;
+    double *tu = z3(), *tv = z3();
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                const double t = A3(ta_ad, i, j, k) / (e1e2t(i, j) * e3t_n(i, j, k));
+                A3(tu, i, j, k) += t; A3(tu, i - 1, j, k) -= t;
+                A3(tv, i, j, k) += t; A3(tv, i, j - 1, k) -= t;
+            }
+    for (k = 1; k <= jpkm1; k++)
+        for (j = 1; j <= jpjm1; j++)
+            for (i = 1; i <= jpim1; i++) {
+                const double au = A3(pahu, i, j, k) * e2_e1u(i, j) * e3u_n(i, j, k) * A3(tu, i, j, k);
+                const double av = A3(pahv, i, j, k) * e1_e2v(i, j) * e3v_n(i, j, k) * A3(tv, i, j, k);
+                A3(tb_ad, i + 1, j, k) += au; A3(tb_ad, i, j, k) -= au;
+                A3(tb_ad, i, j + 1, k) += av; A3(tb_ad, i, j, k) -= av;
+            }
+    free(tu); free(tv);
+    return 0;
+}
+
+/* adjoint of tra_zdf_imp (one tracer, diffusivity avt, no slopes): on
+   entry ta_ad = dJ/d(after field); on exit ta_ad = dJ/d(trend), and
+   tb_ad += dJ/dtb.  The elimination is reversed step by step. */
+int ora_zdf_imp_ad(double p2dt, const double *avt, double *ta_ad, double *tb_ad)
+{
+    int i, j, k;
+    const int K = jpkm1;
+    double *zwi = malloc(sizeof(double) * (jpk + 1)), *zws = malloc(sizeof(double) * (jpk + 1));
+    double *zwt = malloc(sizeof(double) * (jpk + 1)), *r = malloc(sizeof(double) * (jpk + 1));
+    double *x = malloc(sizeof(double) * (jpk + 1));
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++) {
+            for (k = 1; k <= K; k++) {
+                const double wt_k = (k == 1) ? 0.0 : A3(avt, i, j, k), wt_kp1 = A3(avt, i, j, k + 1);
+                zwi[k] = -p2dt * wt_k / e3w_n(i, j, k);
+                zws[k] = -p2dt * wt_kp1 / e3w_n(i, j, k + 1);
+                const double zwd = e3t_a(i, j, k) - zwi[k] - zws[k];
+                zwt[k] = (k == 1) ? zwd : zwd - zwi[k] * zws[k - 1] / zwt[k - 1];
+            }
+            for (k = 1; k <= K; k++) { x[k] = A3(ta_ad, i, j, k); r[k] = 0.0; }
+            /* back substitution reversed */
+            for (k = 1; k <= K - 1; k++) {
+                const double t = x[k] * TMASK(i, j, k) / zwt[k];
+                r[k] += t;
+                x[k + 1] -= zws[k] * t;
+            }
+            r[K] += x[K] * TMASK(i, j, K) / zwt[K];
+            /* forward substitution reversed */
+            for (k = K; k >= 2; k--) r[k - 1] -= zwi[k] / zwt[k - 1] * r[k];
+            for (k = 1; k <= K; k++) {
+                A3(tb_ad, i, j, k) += e3t_b(i, j, k) * r[k];
+                A3(ta_ad, i, j, k) = p2dt * e3t_n(i, j, k) * r[k];
+            }
+        }
+    free(zwi); free(zws); free(zwt); free(r); free(x);
+    return 0;
+}
+
+/* adjoint of the leapfrog tra_nxt for one tracer (kt > nit000 or a
+   leapfrog restart): on entry tb_ad, tn_ad = dJ/d(new tsb, tsn); on exit
+   tb_ad, tn_ad = dJ/d(old tsb, tsn) and ta_ad = dJ/dtsa (interior cells;
+   rings cleared, their dependence added to the source points) */
+int ora_tra_nxt_ad(double *tb_ad, double *tn_ad, double *ta_ad)
+{
+    int i, j, k;
+    lbc3_adj(tb_ad, jpk, 'T');
+    lbc3_adj(tn_ad, jpk, 'T');
+    const double g = D.rn_atfp;
+    for (k = 1; k <= jpkm1; k++)   /* level jpk is left as it is: identity */
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                const int in = (i >= 2 && i <= jpim1 && j >= 2 && j <= jpjm1);
+                const double b = A3(tb_ad, i, j, k), n = A3(tn_ad, i, j, k);
+                A3(ta_ad, i, j, k) = in ? g * b + n : 0.0;
+                A3(tb_ad, i, j, k) = in ? g * b : 0.0;
+                A3(tn_ad, i, j, k) = in ? b - 2.0 * g * b : 0.0;
+            }
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) A3(ta_ad, i, j, jpk) = 0.0;
+    return 0;
+}
+
+/* the transpose of the state's lbc_lnk ('T'): the gradient with respect to
+   the interior values of a field whose ring is filled from them */
+int ora_lbc_adj(double *p, int nk, char nat)
+{
+    lbc3_adj(p, nk, nat);
     return 0;
 }
 

@@ -189,6 +189,31 @@ class Oracle:
                                 nm.ctypes.data_as(C.POINTER(C.c_int)), _p(hmlp), _p(hmlpt), C.c_double(rn_slpmax),
                                 _p(uslp), _p(vslp), _p(wslpi), _p(wslpj)), "ldf_slp")
 
+    # tangent linear and adjoint of the tracer step (one tracer) -------------
+    def fct_coef(self, p2dt, pun, pvn, pwn, tb, tn, cx, cy, cz):
+        _chk(self.L.ora_fct_coef(C.c_double(p2dt), _p(pun), _p(pvn), _p(pwn), _p(tb), _p(tn), _p(cx), _p(cy),
+                                 _p(cz)), "fct_coef")
+
+    def fct_tl(self, p2dt, pun, pvn, pwn, tb, tn, cx, cy, cz, ta):
+        _chk(self.L.ora_fct_tl(C.c_double(p2dt), _p(pun), _p(pvn), _p(pwn), _p(tb), _p(tn), _p(cx), _p(cy), _p(cz),
+                               _p(ta)), "fct_tl")
+
+    def fct_ad(self, pun, pvn, pwn, cx, cy, cz, ta_ad, tb_ad, tn_ad):
+        _chk(self.L.ora_fct_ad(_p(pun), _p(pvn), _p(pwn), _p(cx), _p(cy), _p(cz), _p(ta_ad), _p(tb_ad), _p(tn_ad)),
+             "fct_ad")
+
+    def ldf_lap_ad(self, pahu, pahv, ta_ad, tb_ad):
+        _chk(self.L.ora_ldf_lap_ad(_p(pahu), _p(pahv), _p(ta_ad), _p(tb_ad)), "ldf_lap_ad")
+
+    def zdf_imp_ad(self, p2dt, avt, ta_ad, tb_ad):
+        _chk(self.L.ora_zdf_imp_ad(C.c_double(p2dt), _p(avt), _p(ta_ad), _p(tb_ad)), "zdf_imp_ad")
+
+    def tra_nxt_ad(self, tb_ad, tn_ad, ta_ad):
+        _chk(self.L.ora_tra_nxt_ad(_p(tb_ad), _p(tn_ad), _p(ta_ad)), "tra_nxt_ad")
+
+    def lbc_adj(self, p, nk, nat):
+        _chk(self.L.ora_lbc_adj(_p(p), C.c_int(nk), C.c_char(nat.encode())), "lbc_adj")
+
     def hpg_zco(self, kt, rhd, ua, va):
         _chk(self.L.ora_hpg_zco(C.c_int(kt), _p(rhd), _p(ua), _p(va)), "hpg_zco")
 
