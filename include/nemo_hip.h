@@ -376,6 +376,38 @@ int nemo_hip_rst_write(const char *dir, int kt, const nemo_hip_rst_t *rst);
    fields, else 1; *kt is the step the file was written at.  When un_b is
    not NULL the four depth means are computed as DOM/istate.F90:159-178. */
 int nemo_hip_rst_read(const char *dir, nemo_hip_rst_t *rst, int *kt, int *neuler);
+/* Tangent linear and adjoint of the tracer step, one tracer, FP64: the
+   step tra_adv_fct (orders 2/2) + tra_ldf_lap (kpass 1) + tra_zdf_imp (avt,
+   no slopes) + leapfrog tra_nxt, with the nonosc limiter factors frozen
+   from the forward state (the step is then linear in the tracer).  The
+   adjoint entry points have no counterpart in the reference; each is the
+   transpose of the routine named, as this library runs it.  Adjoint arrays
+   (*_ad) are accumulated unless stated.
+   tra_adv_fct_coef: the factors nonosc (TRA/traadv_fct.F90:404-426) gives
+   the state (tb, tn): cx, cy at interior faces, cz(k) for the face above
+   level k (k = 2..jpk).  pun/pvn/pwn are the transports tra_adv_fct takes. */
+int nemo_hip_tra_adv_fct_coef(double p2dt, const double *pun, const double *pvn, const double *pwn, const double *ptb,
+                              const double *ptn, double *pcx, double *pcy, double *pcz);
+/* pta += the FCT 2/2 trend with the factors given (tra_adv_fct itself when
+   they are the state's own) */
+int nemo_hip_tra_adv_fct_tl(double p2dt, const double *pun, const double *pvn, const double *pwn, const double *ptb,
+                            const double *ptn, const double *pcx, const double *pcy, const double *pcz, double *pta);
+/* ptb_ad, ptn_ad += the transpose of tra_adv_fct_tl applied to pta_ad */
+int nemo_hip_tra_adv_fct_ad(const double *pun, const double *pvn, const double *pwn, const double *pcx,
+                            const double *pcy, const double *pcz, const double *pta_ad, double *ptb_ad,
+                            double *ptn_ad);
+/* ptb_ad += the transpose of tra_ldf_lap (kpass 1) applied to pta_ad */
+int nemo_hip_tra_ldf_lap_ad(const double *pahu, const double *pahv, const double *pta_ad, double *ptb_ad);
+/* tra_zdf_imp: on entry pta_ad is the adjoint of the after field, on exit
+   that of the trend; ptb_ad += the adjoint of tb */
+int nemo_hip_tra_zdf_imp_ad(double p2dt, const double *avt, double *pta_ad, double *ptb_ad);
+/* leapfrog tra_nxt (kt > nit000): on entry ptb_ad, ptn_ad are the adjoints
+   of the new tsb, tsn; on exit those of the old ones, and pta_ad (set)
+   that of tsa; ring points are folded into their lbc sources */
+int nemo_hip_tra_nxt_ad(double *ptb_ad, double *ptn_ad, double *pta_ad);
+/* the transpose of lbc_lnk 'T' on nk levels: ring values added to their
+   sources, the ring cleared */
+int nemo_hip_lbc_adj(double *p, int nk);
 /* zdf_mxl, ZDF/zdfmxl.F90:64-140: the mixed-layer base level nmln
    (jpi,jpj), its w-depth hmlp and the depth hmlpt of its last t-point, from
    rn2b (jpi,jpj,jpk) and the rho_c density criterion (0.01 in the

@@ -1753,6 +1753,101 @@ int nemo_hip_rst_read(const char *dir, nemo_hip_rst_t *r, int *kt, int *neuler)
     return NEMO_HIP_OK;
 }
 
+// ---------------------------------------------------------------------------
+// tangent linear and adjoint of the tracer step (one tracer, FP64)
+// ---------------------------------------------------------------------------
+int nemo_hip_tra_adv_fct_coef(double p2dt, const double *pun, const double *pvn, const double *pwn, const double *ptb,
+                              const double *ptn, double *pcx, double *pcy, double *pcz)
+{
+    NEED_DOMAIN();
+    NEED_PTR(pun); NEED_PTR(pvn); NEED_PTR(pwn); NEED_PTR(ptb); NEED_PTR(ptn); NEED_PTR(pcx); NEED_PTR(pcy);
+    NEED_PTR(pcz);
+    const size_t n3 = g_c->dom.n3;
+    Stager S;
+    const double *u = S.in(pun, n3), *v = S.in(pvn, n3), *w = S.in(pwn, n3), *tb = S.in(ptb, n3), *tn = S.in(ptn, n3);
+    double *cx = S.inout(pcx, n3), *cy = S.inout(pcy, n3), *cz = S.inout(pcz, n3);
+    int rc = S.status ? S.status : launch_fct_frozen(0, p2dt, u, v, w, tb, tn, cx, cy, cz, nullptr);
+    return S.finish(rc);
+}
+
+int nemo_hip_tra_adv_fct_tl(double p2dt, const double *pun, const double *pvn, const double *pwn, const double *ptb,
+                            const double *ptn, const double *pcx, const double *pcy, const double *pcz, double *pta)
+{
+    NEED_DOMAIN();
+    NEED_PTR(pun); NEED_PTR(pvn); NEED_PTR(pwn); NEED_PTR(ptb); NEED_PTR(ptn); NEED_PTR(pcx); NEED_PTR(pcy);
+    NEED_PTR(pcz); NEED_PTR(pta);
+    const size_t n3 = g_c->dom.n3;
+    Stager S;
+    const double *u = S.in(pun, n3), *v = S.in(pvn, n3), *w = S.in(pwn, n3), *tb = S.in(ptb, n3), *tn = S.in(ptn, n3);
+    const double *cx = S.in(pcx, n3), *cy = S.in(pcy, n3), *cz = S.in(pcz, n3);
+    double *ta = S.inout(pta, n3);
+    int rc = S.status ? S.status
+                      : launch_fct_frozen(1, p2dt, u, v, w, tb, tn, (double *)cx, (double *)cy, (double *)cz, ta);
+    return S.finish(rc);
+}
+
+int nemo_hip_tra_adv_fct_ad(const double *pun, const double *pvn, const double *pwn, const double *pcx,
+                            const double *pcy, const double *pcz, const double *pta_ad, double *ptb_ad,
+                            double *ptn_ad)
+{
+    NEED_DOMAIN();
+    NEED_PTR(pun); NEED_PTR(pvn); NEED_PTR(pwn); NEED_PTR(pcx); NEED_PTR(pcy); NEED_PTR(pcz); NEED_PTR(pta_ad);
+    NEED_PTR(ptb_ad); NEED_PTR(ptn_ad);
+    const size_t n3 = g_c->dom.n3;
+    Stager S;
+    const double *u = S.in(pun, n3), *v = S.in(pvn, n3), *w = S.in(pwn, n3);
+    const double *cx = S.in(pcx, n3), *cy = S.in(pcy, n3), *cz = S.in(pcz, n3), *mu = S.in(pta_ad, n3);
+    double *gb = S.inout(ptb_ad, n3), *gn = S.inout(ptn_ad, n3);
+    int rc = S.status ? S.status : launch_fct_ad(u, v, w, cx, cy, cz, mu, gb, gn);
+    return S.finish(rc);
+}
+
+int nemo_hip_tra_ldf_lap_ad(const double *pahu, const double *pahv, const double *pta_ad, double *ptb_ad)
+{
+    NEED_DOMAIN();
+    NEED_PTR(pahu); NEED_PTR(pahv); NEED_PTR(pta_ad); NEED_PTR(ptb_ad);
+    const size_t n3 = g_c->dom.n3;
+    Stager S;
+    const double *a = S.in(pahu, n3), *b = S.in(pahv, n3), *mu = S.in(pta_ad, n3);
+    double *gb = S.inout(ptb_ad, n3);
+    int rc = S.status ? S.status : launch_ldf_lap_ad(a, b, mu, gb);
+    return S.finish(rc);
+}
+
+int nemo_hip_tra_zdf_imp_ad(double p2dt, const double *avt, double *pta_ad, double *ptb_ad)
+{
+    NEED_DOMAIN();
+    NEED_PTR(avt); NEED_PTR(pta_ad); NEED_PTR(ptb_ad);
+    const size_t n3 = g_c->dom.n3;
+    Stager S;
+    const double *k = S.in(avt, n3);
+    double *ga = S.inout(pta_ad, n3), *gb = S.inout(ptb_ad, n3);
+    int rc = S.status ? S.status : launch_zdf_imp_ad(p2dt, k, ga, gb);
+    return S.finish(rc);
+}
+
+int nemo_hip_tra_nxt_ad(double *ptb_ad, double *ptn_ad, double *pta_ad)
+{
+    NEED_DOMAIN();
+    NEED_PTR(ptb_ad); NEED_PTR(ptn_ad); NEED_PTR(pta_ad);
+    const size_t n3 = g_c->dom.n3;
+    Stager S;
+    double *gb = S.inout(ptb_ad, n3), *gn = S.inout(ptn_ad, n3), *ga = S.out(pta_ad, n3);
+    int rc = S.status ? S.status : launch_tra_nxt_ad(gb, gn, ga);
+    return S.finish(rc);
+}
+
+int nemo_hip_lbc_adj(double *p, int nk)
+{
+    NEED_DOMAIN();
+    NEED_PTR(p);
+    if (nk < 1 || nk > g_c->dom.jpk) return set_error(NEMO_HIP_ERR_ARG, "lbc_adj: nk = %d", nk);
+    Stager S;
+    double *f = S.inout(p, g_c->dom.n2 * (size_t)nk);
+    int rc = S.status ? S.status : launch_lbc_adj(f, nk);
+    return S.finish(rc);
+}
+
 int nemo_hip_eos_register(const nemo_hip_eos_t *eos)
 {
     NEED_DOMAIN();

@@ -57,6 +57,14 @@ struct SeosPar {   // simplified EOS coefficients, TRA/eosbn2.F90:89-95
 };
 template <typename T> int launch_eos(int neos, const SeosPar &e, int pot, const T *pts, T *prd, T *prhop, const T *pdep);
 template <typename T> int launch_rab(int neos, const SeosPar &e, const T *pts, T *pab, const T *gdept);
+int launch_fct_frozen(int mode, double p2dt, const double *pun, const double *pvn, const double *pwn,
+                      const double *tb, const double *tn, double *cx, double *cy, double *cz, double *pta);
+int launch_fct_ad(const double *pun, const double *pvn, const double *pwn, const double *cx, const double *cy,
+                  const double *cz, const double *ta_ad, double *tb_ad, double *tn_ad);
+int launch_ldf_lap_ad(const double *pahu, const double *pahv, const double *ta_ad, double *tb_ad);
+int launch_zdf_imp_ad(double p2dt, const double *avt, double *ta_ad, double *tb_ad);
+int launch_tra_nxt_ad(double *tb_ad, double *tn_ad, double *ta_ad);
+int launch_lbc_adj(double *p, int nk);
 template <typename T>
 int launch_istate_bt(const T *un, const T *vn, const T *ub, const T *vb, const double *r1_hu, const double *r1_hv,
                      double *un_b, double *vn_b, double *ub_b, double *vb_b);

@@ -680,6 +680,98 @@ int launch_tra_adv_fct(double p2dt, const T *pun, const T *pvn, const T *pwn, co
 }
 template int launch_tra_adv_fct<double>(double, const double *, const double *, const double *, const double *,
                                         const double *, double *, int, int, int);
+// ---------------------------------------------------------------------------
+// the FCT 2/2 limiter frozen (the tangent linear of the tracer step): the
+// factors nonosc multiplies the antidiffusive fluxes by (traadv_fct.F90:
+// 404-426), and the FCT trend with given factors.  One tracer, FP64.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double lim_factor(double flux, double bdo_here, double bup_next, double bup_here,
+                                             double bdo_next)
+{
+    const double za = fmin2(fmin2(1.0, bdo_here), bup_next);
+    const double zb = fmin2(fmin2(1.0, bup_here), bdo_next);
+    const double zc = (0.5 + nsign(0.5, flux));
+    return zc * za + (1.0 - zc) * zb;
+}
+
+KERNEL void k_fct_coef(DevDomain d, const double *__restrict__ paa, const double *__restrict__ pbb,
+                       const double *__restrict__ pcc, const double *__restrict__ bup, const double *__restrict__ bdo,
+                       double *__restrict__ cx, double *__restrict__ cy, double *__restrict__ cz)
+{
+    THREAD_IJ(2, 2);
+    if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    for (int k = 1; k <= d.jpk - 1; k++) {
+        const size_t p = IJK(i, j, k);
+        cx[p] = lim_factor(paa[p], bdo[p], A3(bup, i + 1, j, k), bup[p], A3(bdo, i + 1, j, k));
+        cy[p] = lim_factor(pbb[p], bdo[p], A3(bup, i, j + 1, k), bup[p], A3(bdo, i, j + 1, k));
+        cz[IJK(i, j, k + 1)] = lim_factor(A3(pcc, i, j, k + 1), A3(bdo, i, j, k + 1), bup[p], A3(bup, i, j, k + 1),
+                                          bdo[p]);
+    }
+}
+
+// k_fct_final with the factors given: each limited flux is the flux times
+// its factor, at the flux's lbc source, as nonosc and its lbc_lnk give it
+KERNEL void k_fct_final_frozen(DevDomain d, const double *__restrict__ paa, const double *__restrict__ pbb,
+                               const double *__restrict__ pcc, const double *__restrict__ cx,
+                               const double *__restrict__ cy, const double *__restrict__ cz, double *__restrict__ ta)
+{
+    THREAD_IJ(2, 2);
+    if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    const Metrics<double> &m = d.md;
+    const size_t ij = IJ(i, j);
+    const double r1_e1e2t = m.r1_e1e2t[ij];
+    auto lx = [&](int p, int q, int k) {
+        int sp, sq;
+        const bool ok = lbc_src_safe(d, 'U', p, q, sp, sq);
+        return ok ? A3(paa, sp, sq, k) * A3(cx, sp, sq, k) : 0.0;
+    };
+    auto ly = [&](int p, int q, int k) {
+        int sp, sq;
+        const bool ok = lbc_src_safe(d, 'V', p, q, sp, sq);
+        return ok ? A3(pbb, sp, sq, k) * A3(cy, sp, sq, k) : 0.0;
+    };
+    double lz_k = A3(pcc, i, j, 1);
+    for (int k = 1; k <= d.jpk - 1; k++) {
+        const double lz_kp1 = A3(pcc, i, j, k + 1) * A3(cz, i, j, k + 1);
+        const size_t p = IJK(i, j, k);
+        ta[p] = ta[p] - (lx(i, j, k) - lx(i - 1, j, k) + ly(i, j, k) - ly(i, j - 1, k) + lz_k - lz_kp1) * r1_e1e2t /
+                            E3(m.e3t, ij, k);
+        lz_k = lz_kp1;
+    }
+}
+
+// mode 0: the factors of the state (tb, tn) into cx, cy, cz; mode 1: pta +=
+// the FCT trend with the factors cx, cy, cz
+int launch_fct_frozen(int mode, double p2dt, const double *pun, const double *pvn, const double *pwn,
+                      const double *tb, const double *tn, double *cx, double *cy, double *cz, double *pta)
+{
+    const DevDomain &d = nh::dom();
+    double *zwi = (double *)nh::scratch3(nh::S_ZWI), *zwx = (double *)nh::scratch3(nh::S_ZWX);
+    double *zwy = (double *)nh::scratch3(nh::S_ZWY), *zwz = (double *)nh::scratch3(nh::S_ZWZ);
+    double *zbu = (double *)nh::scratch3(nh::S_ZBETUP), *zbd = (double *)nh::scratch3(nh::S_ZBETDO);
+    double *zwt = (double *)nh::scratch3(nh::S_ZTW), *dummy = (double *)nh::scratch3(nh::S_ZUN);
+    Grid2 gi = grid2(d.jpi - 2, d.jpj - 2), gh = grid2(d.jpi - 1, d.jpj - 1);
+    Grid2 gk = gi;
+    gk.grid.z = (unsigned)((d.jpk - 1 + FctKB<double>::beta - 1) / FctKB<double>::beta);
+    int st;
+    double *ta = mode == 0 ? dummy : pta;
+    LAUNCH("tra_adv_fct", (k_fct_low<double>), gi, d, p2dt, pun, pvn, pwn, tb, ta, zwi);
+    if ((st = launch_lbc<double>(zwi, d.jpk, 'T', 0.0))) return st;
+    LAUNCH("tra_adv_fct", (k_fct_ad<double>), gh, d, 2, 2, pun, pvn, pwn, tb, tn, zwx, zwy, zwz, zwt);
+    if ((st = launch_lbc<double>(zwx, d.jpk, 'U', 0.0))) return st;
+    if ((st = launch_lbc<double>(zwy, d.jpk, 'V', 0.0))) return st;
+    if ((st = launch_lbc<double>(zwz, d.jpk, 'W', 0.0))) return st;
+    if (mode == 1) {
+        LAUNCH("tra_adv_fct", k_fct_final_frozen, gi, d, zwx, zwy, zwz, cx, cy, cz, pta);
+        return 0;
+    }
+    LAUNCH("tra_adv_fct", (k_fct_beta<double, FctKB<double>::beta>), gk, d, p2dt, tb, zwi, zwx, zwy, zwz, zbu, zbd);
+    if ((st = launch_lbc<double>(zbu, d.jpk, 'T', 0.0))) return st;
+    if ((st = launch_lbc<double>(zbd, d.jpk, 'T', 0.0))) return st;
+    LAUNCH("tra_adv_fct", k_fct_coef, gi, d, zwx, zwy, zwz, zbu, zbd, cx, cy, cz);
+    return 0;
+}
+
 template int launch_tra_adv_fct<float>(double, const float *, const float *, const float *, const float *,
                                        const float *, float *, int, int, int);
 

@@ -385,6 +385,37 @@ class NemoHip:
         p = lambda a: self._p(a, dt)
         self._check(fn(p(pts), p(pab), p(pn2), p(gdept), p(gdepw)))
 
+    # tangent linear and adjoint of the tracer step (one tracer, FP64) --------
+    def tra_adv_fct_coef(self, p2dt, pun, pvn, pwn, ptb, ptn, cx, cy, cz):
+        p = self._p
+        self._check(self.L.nemo_hip_tra_adv_fct_coef(C.c_double(p2dt), p(pun), p(pvn), p(pwn), p(ptb), p(ptn), p(cx),
+                                                     p(cy), p(cz)))
+
+    def tra_adv_fct_tl(self, p2dt, pun, pvn, pwn, ptb, ptn, cx, cy, cz, pta):
+        p = self._p
+        self._check(self.L.nemo_hip_tra_adv_fct_tl(C.c_double(p2dt), p(pun), p(pvn), p(pwn), p(ptb), p(ptn), p(cx),
+                                                   p(cy), p(cz), p(pta)))
+
+    def tra_adv_fct_ad(self, pun, pvn, pwn, cx, cy, cz, ta_ad, tb_ad, tn_ad):
+        p = self._p
+        self._check(self.L.nemo_hip_tra_adv_fct_ad(p(pun), p(pvn), p(pwn), p(cx), p(cy), p(cz), p(ta_ad), p(tb_ad),
+                                                   p(tn_ad)))
+
+    def tra_ldf_lap_ad(self, pahu, pahv, ta_ad, tb_ad):
+        p = self._p
+        self._check(self.L.nemo_hip_tra_ldf_lap_ad(p(pahu), p(pahv), p(ta_ad), p(tb_ad)))
+
+    def tra_zdf_imp_ad(self, p2dt, avt, ta_ad, tb_ad):
+        p = self._p
+        self._check(self.L.nemo_hip_tra_zdf_imp_ad(C.c_double(p2dt), p(avt), p(ta_ad), p(tb_ad)))
+
+    def tra_nxt_ad(self, tb_ad, tn_ad, ta_ad):
+        p = self._p
+        self._check(self.L.nemo_hip_tra_nxt_ad(p(tb_ad), p(tn_ad), p(ta_ad)))
+
+    def lbc_adj(self, a, nk):
+        self._check(self.L.nemo_hip_lbc_adj(self._p(a), C.c_int(nk)))
+
     def _rst(self, fields, sp):
         r = abi.RstStruct()
         r.real_bytes = 4 if sp else 8
