@@ -71,6 +71,12 @@ def parse():
                     help="also time, for these forms, the whole step with its physics: the iso-neutral slopes of "
                          "the state (rab_3d, bn2, zdf_mxl, eos_insitu, ldf_slp), the density, hpg_zco, "
                          "dyn_ldf_lap, tra_sbc and the separate hot-path routines (TEOS-10)")
+    ap.add_argument("--adjoint", default="512x512x75", metavar="SIZE",
+                    help="also time the adjoint form at this size ('' to skip): the gradient of an SST misfit "
+                         "after --adjoint-steps tracer steps, checkpoints every 10 steps")
+    ap.add_argument("--adjoint-steps", type=int, default=50)
+    ap.add_argument("--adjoint-dirs", type=int, default=64, metavar="N",
+                    help="finite-difference directions for the adjoint gradient check")
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--timing-steps", type=int, default=2)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -324,6 +330,171 @@ def time_host_mode(h, d, S, steps):
     finally:
         h.set_pointer_mode(nemo_hip.NEMO_HIP_PTR_DEVICE)
     return ms
+
+
+def time_adjoint(h, size, steps, ckpt, ndir, seed):
+    """The adjoint form: J = 1/2 sum over wet surface points of (SST - target)^2
+    after `steps` tracer steps (FCT 2/2, laplacian, implicit diffusion,
+    leapfrog filter; velocities and diffusivities fixed), target = SST +
+    N(0, 0.1 degC); its gradient with respect to the initial temperature by
+    the adjoint with the limiter factors frozen, the forward state kept every
+    `ckpt` steps and each segment recomputed before its backward sweep.
+    Checked by a dot-product test of one step and, along `ndir` seeded
+    directions, against the tangent linear model it transposes (the forward
+    trajectory's limiter factors at every step); central differences of the
+    nonlinear model (8 directions) measure what freezing the limiter costs."""
+    import numpy as np
+    import torch
+    from nemo_hip import configs
+    jpi, jpj, jpk = [int(x) for x in size.split("x")]
+    ctx = h.context_create()
+    h.context_select(ctx)
+    ext = torch.cuda.ExternalStream(h.stream())
+    try:
+        with torch.cuda.stream(ext):
+            d = build_case(jpi, jpj, jpk, seed)
+            h.register(restarted_opts(d))
+            S = configs.make_state_device(d, seed=seed + 1)
+            n3 = jpi * jpj * jpk
+            f64 = lambda: torch.zeros(n3, dtype=torch.float64, device="cuda")
+            zun, zvn, zwn = f64(), f64(), f64()
+            h.tra_adv_trp(S["un"], S["vn"], S["wn"], zun, zvn, zwn)
+            p2dt, kt = 2.0 * d.opts["rdt"], 5
+            ahtu, ahtv, avt = S["ahtu"], S["ahtv"], S["avt"]
+            T0 = S["tsn"][:n3].clone()
+            h.lbc_lnk(T0, jpk, "T")
+            B, N, A = (torch.zeros(2 * n3, dtype=torch.float64, device="cuda") for _ in range(3))
+            c = [f64() for _ in range(3)]
+
+            def step(tb, tn, cc, own):
+                """one tracer step from (tb, tn); own: the factors of the state
+                itself (the nonlinear step), else the frozen factors cc"""
+                if own:
+                    h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, tb, tn, *cc)
+                ta = A[:n3]
+                ta.zero_()
+                h.tra_adv_fct_tl(p2dt, zun, zvn, zwn, tb, tn, *cc, ta)
+                h.tra_ldf_lap(kt, 1, "TRA", ahtu, ahtv, tb, ta, 1, 1)
+                h.tra_zdf_imp(kt, 1, "TRA", p2dt, tb, ta, 1, avt, avt)
+                B[:n3].copy_(tb)
+                N[:n3].copy_(tn)
+                h.tra_nxt(kt, B, N, A)
+                return B[:n3].clone(), N[:n3].clone()
+
+            def run(T, keep=None, frozen=None):
+                """the nonlinear model, or with frozen: the linear model with
+                those factors at every step"""
+                b, n = T.clone(), T.clone()
+                for q in range(steps):
+                    if keep is not None and q % ckpt == 0:
+                        keep.append((b.clone(), n.clone()))
+                    b, n = step(b, n, frozen[q] if frozen else c, frozen is None)
+                return n
+
+            t, *_ = configs.masks(d)
+            wgt = np.zeros((jpi, jpj, jpk))
+            wgt[1:-1, 1:-1, 0] = t[1:-1, 1:-1, 0]
+            wgt = torch.from_numpy(wgt.ravel(order="F")).to("cuda")
+            del t
+            nN = run(T0)
+            rng = np.random.default_rng(seed + 2)
+            target = nN + 0.1 * torch.from_numpy(rng.standard_normal(n3)).to("cuda")
+            J = lambda n: 0.5 * float(torch.sum(wgt * (n - target) ** 2))
+            # timed: forward with checkpoints, then the segmented adjoint sweep
+            h.sync()
+            t0 = time.perf_counter()
+            chk = []
+            nN = run(T0, keep=chk)
+            h.sync()
+            t1 = time.perf_counter()
+            lb, ln = torch.zeros_like(nN), wgt * (nN - target)
+            la = f64()
+            for s0 in reversed(range(0, steps, ckpt)):
+                b, n = chk[s0 // ckpt]
+                seg = []
+                for q in range(s0, min(s0 + ckpt, steps)):
+                    cc = [f64() for _ in range(3)]
+                    h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, b, n, *cc)
+                    seg.append(cc)
+                    b, n = step(b, n, cc, False)
+                for cc in reversed(seg):
+                    h.tra_nxt_ad(lb, ln, la)
+                    h.tra_zdf_imp_ad(p2dt, avt, la, lb)
+                    h.tra_ldf_lap_ad(ahtu, ahtv, la, lb)
+                    h.tra_adv_fct_ad(zun, zvn, zwn, *cc, la, lb, ln)
+                del seg
+            grad = lb + ln
+            h.lbc_adj(grad, jpk)
+            h.sync()
+            t2 = time.perf_counter()
+            # dot-product test of one frozen step (factors of the initial state)
+            h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, T0, T0, *c)
+            xb, xn, yb, yn = (torch.from_numpy(rng.standard_normal(n3)).to("cuda") for _ in range(4))
+            b1, n1 = step(xb, xn, c, False)
+            gb, gn = yb.clone(), yn.clone()
+            h.tra_nxt_ad(gb, gn, la)
+            h.tra_zdf_imp_ad(p2dt, avt, la, gb)
+            h.tra_ldf_lap_ad(ahtu, ahtv, la, gb)
+            h.tra_adv_fct_ad(zun, zvn, zwn, *c, la, gb, gn)
+            lhs = float(torch.dot(b1, yb) + torch.dot(n1, yn))
+            rhs = float(torch.dot(xb, gb) + torch.dot(xn, gn))
+            dot_err = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+            # the gradient against the tangent linear model it transposes (the
+            # forward trajectory's limiter factors at every step): <g, d> =
+            # <r, L d>, r the final misfit, L d the linear model run on d (the
+            # exact limit of central differences of that model); and against
+            # central differences of the nonlinear model (8 directions), the
+            # error of freezing the limiter
+            frozen, b, n = [], T0.clone(), T0.clone()
+            for q in range(steps):
+                cc = [f64() for _ in range(3)]
+                h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, b, n, *cc)
+                frozen.append(cc)
+                b, n = step(b, n, cc, False)
+            r = wgt * (n - target)
+            interior = torch.zeros((jpk, jpj, jpi), dtype=torch.float64, device="cuda")
+            interior[:-1, 1:-1, 1:-1] = 1.0
+            interior = interior.reshape(-1)
+            errs, errs_nl, growth = [], [], []
+            t3 = time.perf_counter()
+            for q in range(ndir):
+                dirn = torch.from_numpy(np.random.default_rng(seed + 100 + q).standard_normal(n3)).to("cuda") * interior
+                h.lbc_lnk(dirn, jpk, "T")
+                ad = float(torch.dot(grad, dirn))
+                Ld = run(dirn, frozen=frozen)
+                tl = float(torch.dot(r, Ld))
+                errs.append(abs(tl - ad) / max(abs(tl), 1e-300))
+                growth.append(float(torch.linalg.vector_norm(Ld) / torch.linalg.vector_norm(dirn)))
+                if q < 8:
+                    eps = 1e-3
+                    fd_nl = (J(run(T0 + eps * dirn)) - J(run(T0 - eps * dirn))) / (2 * eps)
+                    errs_nl.append(abs(fd_nl - ad) / max(abs(fd_nl), 1e-300))
+            t4 = time.perf_counter()
+            del frozen
+    finally:
+        h.context_select(0)
+        torch.cuda.empty_cache()
+        h.context_destroy(ctx)
+    errs, errs_nl = np.array(errs), np.array(errs_nl)
+    ndir = len(errs)
+    return {"workload": f"{jpi}x{jpj}x{jpk} FP64 tracer step (FCT 2/2 with the limiter frozen, laplacian, implicit "
+                        f"diffusion, leapfrog filter), {steps} steps, checkpoints every {ckpt}",
+            "forward_ms_per_step": (t1 - t0) * 1e3 / steps,
+            "adjoint_ms_per_step": (t2 - t1) * 1e3 / steps,
+            "gradient_s": t2 - t0, "dot_product_relerr": dot_err, "directions": ndir,
+            "tl_relerr_median": float(np.median(errs)) if ndir else None,
+            "tl_relerr_max": float(errs.max()) if ndir else None, "check_s": t4 - t3,
+            "tl_growth_median": float(np.median(growth)) if ndir else None,
+            "fd_nonlinear_directions": len(errs_nl),
+            "fd_nonlinear_relerr_median": float(np.median(errs_nl)) if len(errs_nl) else None}
+
+
+def restarted_opts(d):
+    """the domain with a leapfrog start (neuler = 1): every step is a leapfrog step"""
+    from dataclasses import replace
+    d2 = replace(d, opts=dict(d.opts, neuler=1))
+    d2.e3t_1d, d2.e3w_1d = d.e3t_1d, d.e3w_1d
+    return d2
 
 
 def time_ensemble(h, members, steps, seed):
@@ -601,6 +772,15 @@ def main():
         log(f"host-pointer mode (FP64 routines, PCIe-inclusive): {host_ms:.1f} ms/step, "
             f"{cells / (host_ms * 1e-3):.3e} cell-updates/s")
     ens = None
+    adj = None
+    if a.adjoint:
+        adj = time_adjoint(h, a.adjoint, a.adjoint_steps, 10, a.adjoint_dirs, a.seed + 21)
+        log(f"adjoint {adj['workload']}: forward {adj['forward_ms_per_step']:.2f} ms/step, adjoint sweep "
+            f"{adj['adjoint_ms_per_step']:.2f} ms/step, gradient {adj['gradient_s']:.2f} s; dot product "
+            f"{adj['dot_product_relerr']:.1e}; against the tangent linear over {adj['directions']} directions: "
+            f"median {adj['tl_relerr_median']:.2e}, max {adj['tl_relerr_max']:.2e} (growth of a perturbation "
+            f"{adj['tl_growth_median']:.2e}); against FD of the nonlinear model over "
+            f"{adj['fd_nonlinear_directions']}: median {adj['fd_nonlinear_relerr_median']:.2e} ({adj['check_s']:.1f} s)")
     if a.ensemble:
         e_ms, g_ms, e_cells = time_ensemble(h, a.ensemble, 5, a.seed + 11)
         ens = {"members": a.ensemble, "cells": e_cells, "eager_ms_per_step": e_ms, "graph_ms_per_step": g_ms,
@@ -634,6 +814,7 @@ def main():
         "stream_copy_GBps": copy_gbs,
         "host_mode_fp64_ms_per_step": host_ms,
         "ensemble": ens,
+        "adjoint": adj,
         "whole_step": whole,
         "physics_step": physics,
         "forms": res,
