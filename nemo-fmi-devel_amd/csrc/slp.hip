@@ -7,12 +7,13 @@
 // reference's operation order, so the FP64 form reproduces it bit for bit.
 //
 // The reference builds the unfiltered slopes in work arrays, applies
-// lbc_lnk to them, filters and applies lbc_lnk again.  Here the unfiltered
-// slopes go to two FP64 scratch arrays with their ring already equal to what
-// lbc_lnk would give (a ring point evaluates the interior formula at its
-// lbc source, or is zero on a closed boundary), and the filter kernel does
-// the same for the output's ring.  The horizontal and vertical density
-// gradients (zgru, zgrv, zdzr) are recomputed from prd and pn2 where used.
+// lbc_lnk to them, filters and applies lbc_lnk again.  Here one kernel per
+// level builds the unfiltered slopes of a tile and its halo in LDS (a halo
+// point on the ring evaluates the interior formula at its lbc source, or is
+// zero on a closed boundary, which is what lbc_lnk gives), filters them and
+// writes the interior; the final lbc_lnk fills the output's ring.  The
+// horizontal and vertical density gradients (zgru, zgrv, zdzr) are
+// recomputed from prd and pn2 where used.
 #include "common.hpp"
 #include "kernels.hpp"
 
@@ -126,24 +127,14 @@ KERNEL void k_slp_mxl(DevDomain d, SlpIn<T> in, const int32_t *__restrict__ nmln
 }
 
 // ---------------------------------------------------------------------------
-// I. unfiltered u/v slopes, lines 197-219 (zwz, zww), levels 2..jpkm1, every
-// point: ring points take their lbc source's value (lbc_lnk, line 231)
+// I. unfiltered u/v slopes at the interior point (si, sj), level k, lines
+// 197-219 (zwz, zww)
 // ---------------------------------------------------------------------------
 template <typename T>
-KERNEL void k_slp_uv_raw(DevDomain d, SlpIn<T> in, const T *__restrict__ gdept, const int32_t *__restrict__ nmln,
-                         const double *__restrict__ zsu, const double *__restrict__ zsv, double z1_slpmax,
-                         double *__restrict__ zwz, double *__restrict__ zww)
+__device__ __forceinline__ void raw_uv(const DevDomain &d, const SlpIn<T> &in, const T *gdept, const int32_t *nmln,
+                                       const double *zsu, const double *zsv, double z1_slpmax, int si, int sj, int k,
+                                       double &zwz, double &zww)
 {
-    THREAD_IJ(1, 1);
-    const int k = (int)blockIdx.z + 2;
-    if (i > d.jpi || j > d.jpj) return;
-    const uint32_t p = IJK32(i, j, k);
-    int si, sj;
-    if (!lbc_src(d, 'U', i, j, si, sj)) {
-        at32(zwz, p) = 0.;
-        at32(zww, p) = 0.;
-        return;
-    }
     const Metrics<double> &m = d.md;
     const T *prd = in.prd, *pn2 = in.pn2;
     const uint32_t ij = IJ32(si, sj);
@@ -160,26 +151,16 @@ KERNEL void k_slp_uv_raw(DevDomain d, SlpIn<T> in, const T *__restrict__ gdept, 
     const double gc = (double)A3(gdept, si, sj, k);
     const double zdepu = 0.5 * ((gc + (double)A3(gdept, si + 1, sj, k)) - E3(m.e3u, ij, d.miku[ij]));
     const double zdepv = 0.5 * ((gc + (double)A3(gdept, si, sj + 1, k)) - E3(m.e3v, ij, d.mikv[ij]));
-    at32(zwz, p) = ((1. - zfi) * zau / (zbu - ZEPS) + zfi * zdepu * zsu[ij]) * UMASK(double, si, sj, k);
-    at32(zww, p) = ((1. - zfj) * zav / (zbv - ZEPS) + zfj * zdepv * zsv[ij]) * VMASK(double, si, sj, k);
+    zwz = ((1. - zfi) * zau / (zbu - ZEPS) + zfi * zdepu * zsu[ij]) * UMASK(double, si, sj, k);
+    zww = ((1. - zfj) * zav / (zbv - ZEPS) + zfj * zdepv * zsv[ij]) * VMASK(double, si, sj, k);
 }
 
-// II. unfiltered w slopes, lines 277-304, levels 2..jpkm1, every point (lbc_lnk 'T', line 315)
+// II. unfiltered w slopes at the interior point (si, sj), level k, lines 277-304
 template <typename T>
-KERNEL void k_slp_w_raw(DevDomain d, SlpIn<T> in, const T *__restrict__ gdepw, const int32_t *__restrict__ nmln,
-                        const double *__restrict__ hmlp, const double *__restrict__ wiml,
-                        const double *__restrict__ wjml, double *__restrict__ zwz, double *__restrict__ zww)
+__device__ __forceinline__ void raw_w(const DevDomain &d, const SlpIn<T> &in, const T *gdepw, const int32_t *nmln,
+                                      const double *hmlp, const double *wiml, const double *wjml, int si, int sj,
+                                      int k, double &zwz, double &zww)
 {
-    THREAD_IJ(1, 1);
-    const int k = (int)blockIdx.z + 2;
-    if (i > d.jpi || j > d.jpj) return;
-    const uint32_t p = IJK32(i, j, k);
-    int si, sj;
-    if (!lbc_src(d, 'T', i, j, si, sj)) {
-        at32(zwz, p) = 0.;
-        at32(zww, p) = 0.;
-        return;
-    }
     const Metrics<double> &m = d.md;
     const T *prd = in.prd, *pn2 = in.pn2;
     const uint32_t ij = IJ32(si, sj);
@@ -204,86 +185,74 @@ KERNEL void k_slp_w_raw(DevDomain d, SlpIn<T> in, const T *__restrict__ gdepw, c
     const int ikt = d.mikt[ij];
     const double gwt = (double)A3(gdepw, si, sj, ikt);
     const double zck = ((double)A3(gdepw, si, sj, k) - gwt) / max2(hmlp[ij] - gwt, 10.);
-    at32(zwz, p) = (zai / (zbi - ZEPS) * (1. - zfk) + zck * wiml[ij] * zfk) * wm;
-    at32(zww, p) = (zaj / (zbj - ZEPS) * (1. - zfk) + zck * wjml[ij] * zfk) * wm;
-}
-
-// the 1-2-1 x 1-2-1 Shapiro sum of lines 234-251 / 318-337 (before its 1/16)
-__device__ __forceinline__ double shapiro(const DevDomain &d, const double *z, int i, int j, int k)
-{
-    return A3(z, i - 1, j - 1, k) + A3(z, i + 1, j - 1, k) + A3(z, i - 1, j + 1, k) + A3(z, i + 1, j + 1, k) +
-           2. * (A3(z, i, j - 1, k) + A3(z, i - 1, j, k) + A3(z, i + 1, j, k) + A3(z, i, j + 1, k)) +
-           4. * A3(z, i, j, k);
+    zwz = (zai / (zbi - ZEPS) * (1. - zfk) + zck * wiml[ij] * zfk) * wm;
+    zww = (zaj / (zbj - ZEPS) * (1. - zfk) + zck * wjml[ij] * zfk) * wm;
 }
 
 // ---------------------------------------------------------------------------
-// filter and coastal taper of the u/v slopes (lines 232-271) and the final
-// lbc_lnk (line 361), every point and level: levels 1 and jpk only take the
-// lbc of their interior values
+// one level of a SLP_TX x SLP_TY tile: the four unfiltered slopes of the
+// tile and a one-point halo in LDS (a halo point on the domain's ring takes
+// its lbc source's value, or zero on a closed side: the reference's
+// lbc_lnk of zwz/zww, lines 231 and 315), then the Shapiro filters and
+// tapers of lines 232-271 and 316-356 for the tile's interior points.  The
+// output's ring is the final lbc_lnk (line 361), a separate pass.
 // ---------------------------------------------------------------------------
+#define SLP_TX 64
+#define SLP_TY 4
 template <typename T>
-KERNEL void k_slp_uv_filt(DevDomain d, const double *__restrict__ zwz, const double *__restrict__ zww,
-                          T *__restrict__ uslp, T *__restrict__ vslp)
+__global__ __launch_bounds__(SLP_TX * SLP_TY) void k_slp(DevDomain d, SlpIn<T> in, const T *__restrict__ gdept,
+                                                          const T *__restrict__ gdepw,
+                                                          const int32_t *__restrict__ nmln,
+                                                          const double *__restrict__ zsu,
+                                                          const double *__restrict__ zsv,
+                                                          const double *__restrict__ hmlp,
+                                                          const double *__restrict__ wiml,
+                                                          const double *__restrict__ wjml, double z1_slpmax,
+                                                          T *__restrict__ uslp, T *__restrict__ vslp,
+                                                          T *__restrict__ wslpi, T *__restrict__ wslpj)
 {
-    THREAD_IJ(1, 1);
-    const int k = (int)blockIdx.z + 1;
-    if (i > d.jpi || j > d.jpj) return;
-    const uint32_t p = IJK32(i, j, k);
-    int si, sj;
-    const bool src = lbc_src(d, 'U', i, j, si, sj);
-    const bool ring = !is_interior(d, i, j);
-    if (k == 1 || k == d.jpk) {
-        if (!ring) return;
-        at32(uslp, p) = src ? A3(uslp, si, sj, k) : (T)0;
-        at32(vslp, p) = src ? A3(vslp, si, sj, k) : (T)0;
-        return;
+    constexpr int LX = SLP_TX + 2, LY = SLP_TY + 2;
+    __shared__ double su[LY][LX], sv[LY][LX], swi[LY][LX], swj[LY][LX];
+    const int k = (int)blockIdx.z + 2;
+    const int ib = 2 + (int)blockIdx.x * SLP_TX, jb = 2 + (int)blockIdx.y * SLP_TY;
+    const int tid = (int)(threadIdx.y * SLP_TX + threadIdx.x);
+    for (int q = tid; q < LX * LY; q += SLP_TX * SLP_TY) {
+        const int li = q % LX, lj = q / LX;
+        const int i = ib - 1 + li, j = jb - 1 + lj;
+        double a = 0., b = 0., c = 0., e = 0.;
+        int si, sj;
+        if (i <= d.jpi && j <= d.jpj && lbc_src(d, 'U', i, j, si, sj)) {
+            raw_uv<T>(d, in, gdept, nmln, zsu, zsv, z1_slpmax, si, sj, k, a, b);
+            raw_w<T>(d, in, gdepw, nmln, hmlp, wiml, wjml, si, sj, k, c, e);
+        }
+        su[lj][li] = a;
+        sv[lj][li] = b;
+        swi[lj][li] = c;
+        swj[lj][li] = e;
     }
-    if (!src) {
-        at32(uslp, p) = (T)0;
-        at32(vslp, p) = (T)0;
-        return;
-    }
+    __syncthreads();
+    const int li = (int)threadIdx.x + 1, lj = (int)threadIdx.y + 1;
+    const int i = ib - 1 + li, j = jb - 1 + lj;
+    if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    auto shapiro = [&](const double (*z)[LX]) {
+        return z[lj - 1][li - 1] + z[lj - 1][li + 1] + z[lj + 1][li - 1] + z[lj + 1][li + 1] +
+               2. * (z[lj - 1][li] + z[lj][li - 1] + z[lj][li + 1] + z[lj + 1][li]) + 4. * z[lj][li];
+    };
     const double z1_16 = 1.0 / 16.;
-    double u = z1_16 * shapiro(d, zwz, si, sj, k);
-    double v = z1_16 * shapiro(d, zww, si, sj, k);
-    u = u * (UMASK(double, si, sj + 1, k) + UMASK(double, si, sj - 1, k)) * 0.5 *
-        (UMASK(double, si, sj, k) + UMASK(double, si, sj, k + 1)) * 0.5;
-    v = v * (VMASK(double, si + 1, sj, k) + VMASK(double, si - 1, sj, k)) * 0.5 *
-        (VMASK(double, si, sj, k) + VMASK(double, si, sj, k + 1)) * 0.5;
+    double u = z1_16 * shapiro(su);
+    double v = z1_16 * shapiro(sv);
+    u = u * (UMASK(double, i, j + 1, k) + UMASK(double, i, j - 1, k)) * 0.5 *
+        (UMASK(double, i, j, k) + UMASK(double, i, j, k + 1)) * 0.5;
+    v = v * (VMASK(double, i + 1, j, k) + VMASK(double, i - 1, j, k)) * 0.5 *
+        (VMASK(double, i, j, k) + VMASK(double, i, j, k + 1)) * 0.5;
+    const double zcofw = WMASK(double, i, j, k) * z1_16;
+    const double zck = (UMASK(double, i, j, k) + UMASK(double, i - 1, j, k)) *
+                       (VMASK(double, i, j, k) + VMASK(double, i, j - 1, k)) * 0.25;
+    const uint32_t p = IJK32(i, j, k);
     at32(uslp, p) = (T)u;
     at32(vslp, p) = (T)v;
-}
-
-// filter and topographic taper of the w slopes (lines 316-356), final lbc_lnk
-template <typename T>
-KERNEL void k_slp_w_filt(DevDomain d, const double *__restrict__ zwz, const double *__restrict__ zww,
-                         T *__restrict__ wslpi, T *__restrict__ wslpj)
-{
-    THREAD_IJ(1, 1);
-    const int k = (int)blockIdx.z + 1;
-    if (i > d.jpi || j > d.jpj) return;
-    const uint32_t p = IJK32(i, j, k);
-    int si, sj;
-    const bool src = lbc_src(d, 'W', i, j, si, sj);
-    const bool ring = !is_interior(d, i, j);
-    if (k == 1 || k == d.jpk) {
-        if (!ring) return;
-        at32(wslpi, p) = src ? A3(wslpi, si, sj, k) : (T)0;
-        at32(wslpj, p) = src ? A3(wslpj, si, sj, k) : (T)0;
-        return;
-    }
-    if (!src) {
-        at32(wslpi, p) = (T)0;
-        at32(wslpj, p) = (T)0;
-        return;
-    }
-    const double zcofw = WMASK(double, si, sj, k) * (1.0 / 16.);
-    double wi = shapiro(d, zwz, si, sj, k) * zcofw;
-    double wj = shapiro(d, zww, si, sj, k) * zcofw;
-    const double zck = (UMASK(double, si, sj, k) + UMASK(double, si - 1, sj, k)) *
-                       (VMASK(double, si, sj, k) + VMASK(double, si, sj - 1, k)) * 0.25;
-    at32(wslpi, p) = (T)(wi * zck);
-    at32(wslpj, p) = (T)(wj * zck);
+    at32(wslpi, p) = (T)(shapiro(swi) * zcofw * zck);
+    at32(wslpj, p) = (T)(shapiro(swj) * zcofw * zck);
 }
 
 }  // namespace
@@ -312,18 +281,18 @@ int launch_ldf_slp(const T *prd, const T *pn2, const T *gdept, const T *gdepw, c
     const SlpIn<T> in{prd, pn2};
     double *zsu = nh::scratch2(NH_S2_SLP), *zsv = nh::scratch2(NH_S2_SLP + 1);
     double *wiml = nh::scratch2(NH_S2_SLP + 2), *wjml = nh::scratch2(NH_S2_SLP + 3);
-    double *zwz = (double *)nh::scratch3(nh::S_ZWX), *zww = (double *)nh::scratch3(nh::S_ZWY);
     Grid2 g2 = grid2(d.jpi, d.jpj);
     LAUNCH("ldf_slp", k_slp_mxl<T>, g2, d, in, nmln, hmlpt, z1_slpmax, zsu, zsv, wiml, wjml);
-    Grid2 gi = grid2(d.jpi, d.jpj);
-    gi.grid.z = (unsigned)(d.jpk - 2);
-    Grid2 ga = grid2(d.jpi, d.jpj);
-    ga.grid.z = (unsigned)d.jpk;
-    LAUNCH("ldf_slp", k_slp_uv_raw<T>, gi, d, in, gdept, nmln, zsu, zsv, z1_slpmax, zwz, zww);
-    LAUNCH("ldf_slp", k_slp_uv_filt<T>, ga, d, (const double *)zwz, (const double *)zww, uslp, vslp);
-    LAUNCH("ldf_slp", k_slp_w_raw<T>, gi, d, in, gdepw, nmln, hmlp, (const double *)wiml, (const double *)wjml, zwz,
-           zww);
-    LAUNCH("ldf_slp", k_slp_w_filt<T>, ga, d, (const double *)zwz, (const double *)zww, wslpi, wslpj);
+    Grid2 g;
+    g.block = dim3(SLP_TX, SLP_TY, 1);
+    g.grid = dim3((unsigned)((d.jpi - 2 + SLP_TX - 1) / SLP_TX), (unsigned)((d.jpj - 2 + SLP_TY - 1) / SLP_TY),
+                  (unsigned)(d.jpk - 2));
+    LAUNCH("ldf_slp", k_slp<T>, g, d, in, gdept, gdepw, nmln, (const double *)zsu, (const double *)zsv, hmlp,
+           (const double *)wiml, (const double *)wjml, z1_slpmax, uslp, vslp, wslpi, wslpj);
+    int st;
+    if ((st = launch_lbc<T>(uslp, d.jpk, 'U', T(0))) || (st = launch_lbc<T>(vslp, d.jpk, 'V', T(0))) ||
+        (st = launch_lbc<T>(wslpi, d.jpk, 'W', T(0))) || (st = launch_lbc<T>(wslpj, d.jpk, 'W', T(0))))
+        return st;
     return 0;
 }
 
