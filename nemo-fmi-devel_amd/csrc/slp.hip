@@ -128,21 +128,38 @@ KERNEL void k_slp_mxl(DevDomain d, SlpIn<T> in, const int32_t *__restrict__ nmln
 
 // ---------------------------------------------------------------------------
 // I. unfiltered u/v slopes at the interior point (si, sj), level k, lines
-// 197-219 (zwz, zww)
+// 197-219 (zwz, zww).  P(i, j, kk) and N(i, j, kk) give prd and pn2 at the
+// levels used (prd: k, k-1; pn2: k, k+1), from global memory or from LDS.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void raw_uv(const DevDomain &d, const SlpIn<T> &in, const T *gdept, const int32_t *nmln,
-                                       const double *zsu, const double *zsv, double z1_slpmax, int si, int sj, int k,
-                                       double &zwz, double &zww)
+template <class P, class N>
+__device__ __forceinline__ double a_dzr(const DevDomain &d, const P &prd, const N &pn2, int i, int j, int k)
+{
+    const double zm1_g = -1.0 / d.grav;
+    return zm1_g * (prd(i, j, k) + 1.) * (pn2(i, j, k) + pn2(i, j, k + 1)) * (1. - 0.5 * TMASK(double, i, j, k + 1));
+}
+template <class P>
+__device__ __forceinline__ double a_gru(const DevDomain &d, const P &prd, int i, int j, int k)
+{
+    return UMASK(double, i, j, k) * (prd(i + 1, j, k) - prd(i, j, k));
+}
+template <class P>
+__device__ __forceinline__ double a_grv(const DevDomain &d, const P &prd, int i, int j, int k)
+{
+    return VMASK(double, i, j, k) * (prd(i, j + 1, k) - prd(i, j, k));
+}
+
+template <typename T, class P, class N>
+__device__ __forceinline__ void raw_uv(const DevDomain &d, const P &prd, const N &pn2, const T *gdept,
+                                       const int32_t *nmln, const double *zsu, const double *zsv, double z1_slpmax,
+                                       int si, int sj, int k, double &zwz, double &zww)
 {
     const Metrics<double> &m = d.md;
-    const T *prd = in.prd, *pn2 = in.pn2;
     const uint32_t ij = IJ32(si, sj);
-    const double zau = gru(d, prd, si, sj, k) * m.r1_e1u[ij];
-    const double zav = grv(d, prd, si, sj, k) * m.r1_e2v[ij];
-    const double dc = dzr(d, prd, pn2, si, sj, k);
-    double zbu = 0.5 * (dc + dzr(d, prd, pn2, si + 1, sj, k));
-    double zbv = 0.5 * (dc + dzr(d, prd, pn2, si, sj + 1, k));
+    const double zau = a_gru(d, prd, si, sj, k) * m.r1_e1u[ij];
+    const double zav = a_grv(d, prd, si, sj, k) * m.r1_e2v[ij];
+    const double dc = a_dzr(d, prd, pn2, si, sj, k);
+    double zbu = 0.5 * (dc + a_dzr(d, prd, pn2, si + 1, sj, k));
+    double zbv = 0.5 * (dc + a_dzr(d, prd, pn2, si, sj + 1, k));
     zbu = min3(zbu, -z1_slpmax * fabs(zau), -7.e+3 / E3(m.e3u, ij, k) * fabs(zau));
     zbv = min3(zbv, -z1_slpmax * fabs(zav), -7.e+3 / E3(m.e3v, ij, k) * fabs(zav));
     const double oc = oml(nmln, ij, k);
@@ -156,17 +173,15 @@ __device__ __forceinline__ void raw_uv(const DevDomain &d, const SlpIn<T> &in, c
 }
 
 // II. unfiltered w slopes at the interior point (si, sj), level k, lines 277-304
-template <typename T>
-__device__ __forceinline__ void raw_w(const DevDomain &d, const SlpIn<T> &in, const T *gdepw, const int32_t *nmln,
-                                      const double *hmlp, const double *wiml, const double *wjml, int si, int sj,
-                                      int k, double &zwz, double &zww)
+template <typename T, class P, class N>
+__device__ __forceinline__ void raw_w(const DevDomain &d, const P &prd, const N &pn2, const T *gdepw,
+                                      const int32_t *nmln, const double *hmlp, const double *wiml,
+                                      const double *wjml, int si, int sj, int k, double &zwz, double &zww)
 {
     const Metrics<double> &m = d.md;
-    const T *prd = in.prd, *pn2 = in.pn2;
     const uint32_t ij = IJ32(si, sj);
     const double zm1_2g = -0.5 / d.grav;
-    const double zbw = zm1_2g * (double)A3(pn2, si, sj, k) *
-                       ((double)A3(prd, si, sj, k) + (double)A3(prd, si, sj, k - 1) + 2.);
+    const double zbw = zm1_2g * pn2(si, sj, k) * (prd(si, sj, k) + prd(si, sj, k - 1) + 2.);
     const double zci = max2(UMASK(double, si - 1, sj, k) + UMASK(double, si, sj, k) + UMASK(double, si - 1, sj, k - 1) +
                                 UMASK(double, si, sj, k - 1),
                             ZEPS) * m.e1t[ij];
@@ -174,10 +189,10 @@ __device__ __forceinline__ void raw_w(const DevDomain &d, const SlpIn<T> &in, co
                                 VMASK(double, si, sj - 1, k - 1) + VMASK(double, si, sj, k),
                             ZEPS) * m.e2t[ij];
     const double wm = WMASK(double, si, sj, k);
-    const double zai = (gru(d, prd, si - 1, sj, k) + gru(d, prd, si, sj, k - 1) + gru(d, prd, si - 1, sj, k - 1) +
-                        gru(d, prd, si, sj, k)) / zci * wm;
-    const double zaj = (grv(d, prd, si, sj - 1, k) + grv(d, prd, si, sj, k - 1) + grv(d, prd, si, sj - 1, k - 1) +
-                        grv(d, prd, si, sj, k)) / zcj * wm;
+    const double zai = (a_gru(d, prd, si - 1, sj, k) + a_gru(d, prd, si, sj, k - 1) +
+                        a_gru(d, prd, si - 1, sj, k - 1) + a_gru(d, prd, si, sj, k)) / zci * wm;
+    const double zaj = (a_grv(d, prd, si, sj - 1, k) + a_grv(d, prd, si, sj, k - 1) +
+                        a_grv(d, prd, si, sj - 1, k - 1) + a_grv(d, prd, si, sj, k)) / zcj * wm;
     const double e3w = E3(m.e3w, ij, k);
     const double zbi = min3(zbw, -100. * fabs(zai), -7.e+3 / e3w * fabs(zai));
     const double zbj = min3(zbw, -100. * fabs(zaj), -7.e+3 / e3w * fabs(zaj));
@@ -188,6 +203,14 @@ __device__ __forceinline__ void raw_w(const DevDomain &d, const SlpIn<T> &in, co
     zwz = (zai / (zbi - ZEPS) * (1. - zfk) + zck * wiml[ij] * zfk) * wm;
     zww = (zaj / (zbj - ZEPS) * (1. - zfk) + zck * wjml[ij] * zfk) * wm;
 }
+
+// prd or pn2 read from global memory
+template <typename T>
+struct GlobalField {
+    const T *p;
+    const DevDomain &d;
+    __device__ __forceinline__ double operator()(int i, int j, int k) const { return (double)A3(p, i, j, k); }
+};
 
 // ---------------------------------------------------------------------------
 // one level of a SLP_TX x SLP_TY tile: the four unfiltered slopes of the
@@ -211,19 +234,51 @@ __global__ __launch_bounds__(SLP_TX * SLP_TY) void k_slp(DevDomain d, SlpIn<T> i
                                                           T *__restrict__ uslp, T *__restrict__ vslp,
                                                           T *__restrict__ wslpi, T *__restrict__ wslpj)
 {
-    constexpr int LX = SLP_TX + 2, LY = SLP_TY + 2;
+    constexpr int LX = SLP_TX + 2, LY = SLP_TY + 2;      // the slopes: tile + 1
+    constexpr int SX = SLP_TX + 4, SY = SLP_TY + 4;      // prd, pn2: tile + 2
     __shared__ double su[LY][LX], sv[LY][LX], swi[LY][LX], swj[LY][LX];
+    __shared__ double p0[SY][SX], pm[SY][SX], n0[SY][SX], np[SY][SX];
     const int k = (int)blockIdx.z + 2;
     const int ib = 2 + (int)blockIdx.x * SLP_TX, jb = 2 + (int)blockIdx.y * SLP_TY;
     const int tid = (int)(threadIdx.y * SLP_TX + threadIdx.x);
+    // prd (k, k-1) and pn2 (k, k+1) of the tile and a two-point halo, as stored
+    for (int q = tid; q < SX * SY; q += SLP_TX * SLP_TY) {
+        const int li = q % SX, lj = q / SX;
+        const int i = ib - 2 + li, j = jb - 2 + lj;
+        double a = 0., b = 0., c = 0., e = 0.;
+        if (i >= 1 && i <= d.jpi && j >= 1 && j <= d.jpj) {
+            const uint32_t p = IJK32(i, j, k);
+            a = (double)at32(in.prd, p);
+            b = (double)at32(in.prd, p - (uint32_t)d.n2);
+            c = (double)at32(in.pn2, p);
+            e = (double)at32(in.pn2, p + (uint32_t)d.n2);
+        }
+        p0[lj][li] = a;
+        pm[lj][li] = b;
+        n0[lj][li] = c;
+        np[lj][li] = e;
+    }
+    __syncthreads();
+    const auto lprd = [&](int i, int j, int kk) {
+        return kk == k ? p0[j - jb + 2][i - ib + 2] : pm[j - jb + 2][i - ib + 2];
+    };
+    const auto ln2 = [&](int i, int j, int kk) {
+        return kk == k ? n0[j - jb + 2][i - ib + 2] : np[j - jb + 2][i - ib + 2];
+    };
+    const GlobalField<T> gprd{in.prd, d}, gn2{in.pn2, d};
     for (int q = tid; q < LX * LY; q += SLP_TX * SLP_TY) {
         const int li = q % LX, lj = q / LX;
         const int i = ib - 1 + li, j = jb - 1 + lj;
         double a = 0., b = 0., c = 0., e = 0.;
         int si, sj;
         if (i <= d.jpi && j <= d.jpj && lbc_src(d, 'U', i, j, si, sj)) {
-            raw_uv<T>(d, in, gdept, nmln, zsu, zsv, z1_slpmax, si, sj, k, a, b);
-            raw_w<T>(d, in, gdepw, nmln, hmlp, wiml, wjml, si, sj, k, c, e);
+            if (si == i && sj == j) {   // an interior point: its neighbours are staged
+                raw_uv<T>(d, lprd, ln2, gdept, nmln, zsu, zsv, z1_slpmax, si, sj, k, a, b);
+                raw_w<T>(d, lprd, ln2, gdepw, nmln, hmlp, wiml, wjml, si, sj, k, c, e);
+            } else {                    // a ring point: its lbc source, from global memory
+                raw_uv<T>(d, gprd, gn2, gdept, nmln, zsu, zsv, z1_slpmax, si, sj, k, a, b);
+                raw_w<T>(d, gprd, gn2, gdepw, nmln, hmlp, wiml, wjml, si, sj, k, c, e);
+            }
         }
         su[lj][li] = a;
         sv[lj][li] = b;
