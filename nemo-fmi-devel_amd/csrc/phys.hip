@@ -198,9 +198,9 @@ KERNEL void k_hpg_zco(DevDomain d, const T *__restrict__ rhd, T *__restrict__ ua
 }
 
 // ---------------------------------------------------------------------------
-// dyn_ldf_lap, DYN/dynldf_lap_blp.F90:72-100: per interior cell, the three
-// f-point curls and three t-point divergences the cell's two trends use,
-// each evaluated as the reference evaluates zcur / zdiv
+// dyn_ldf_lap, DYN/dynldf_lap_blp.F90:72-100: the f-point curls and t-point
+// divergences the trends use, each evaluated as the reference evaluates
+// zcur / zdiv
 // ---------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ double ldf_cur(const DevDomain &d, const T *pub, const T *pvb, const T *ahmf, int p, int q,
@@ -223,21 +223,36 @@ __device__ __forceinline__ double ldf_div(const DevDomain &d, const T *pub, cons
             A2(m.e1v, p, q - 1) * E3(m.e3v, IJ32(p, q - 1), k) * (double)A3(pvb, p, q - 1, k));
 }
 
+// one level of a LDF_TX x LDF_TY tile: the curls (f-points) and divergences
+// (t-points) the tile's trends use are evaluated once each into LDS, then
+// every interior cell forms its two trends from them
+#define LDF_TX 64
+#define LDF_TY 4
 template <typename T>
-KERNEL void k_dyn_ldf_lap(DevDomain d, double zsign, const T *__restrict__ pub, const T *__restrict__ pvb,
-                          T *__restrict__ pua, T *__restrict__ pva, const T *__restrict__ ahmt,
-                          const T *__restrict__ ahmf)
+__global__ __launch_bounds__(LDF_TX * LDF_TY) void k_dyn_ldf_lap(DevDomain d, double zsign, const T *__restrict__ pub,
+                                                                  const T *__restrict__ pvb, T *__restrict__ pua,
+                                                                  T *__restrict__ pva, const T *__restrict__ ahmt,
+                                                                  const T *__restrict__ ahmf)
 {
-    THREAD_IJ(2, 2);
+    constexpr int LX = LDF_TX + 1, LY = LDF_TY + 1;
+    __shared__ double scur[LY][LX], sdiv[LY][LX];
     const int k = (int)blockIdx.z + 1;
+    const int ib = 2 + (int)blockIdx.x * LDF_TX, jb = 2 + (int)blockIdx.y * LDF_TY;
+    const int tid = (int)(threadIdx.y * LDF_TX + threadIdx.x);
+    for (int q = tid; q < LX * LY; q += LDF_TX * LDF_TY) {
+        const int li = q % LX, lj = q / LX;
+        const int pc = ib - 1 + li, qc = jb - 1 + lj;   // f-point (i-1 .. i+TX-1, j-1 .. j+TY-1)
+        const int pd = ib + li, qd = jb + lj;           // t-point (i .. i+TX, j .. j+TY)
+        scur[lj][li] = (pc <= d.jpi - 1 && qc <= d.jpj - 1) ? ldf_cur(d, pub, pvb, ahmf, pc, qc, k) : 0.;
+        sdiv[lj][li] = (pd <= d.jpi && qd <= d.jpj) ? ldf_div(d, pub, pvb, ahmt, pd, qd, k) : 0.;
+    }
+    __syncthreads();
+    const int li = (int)threadIdx.x, lj = (int)threadIdx.y;
+    const int i = ib + li, j = jb + lj;
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
     const Metrics<double> &m = d.md;
-    const double cur = ldf_cur(d, pub, pvb, ahmf, i, j, k);
-    const double cur_s = ldf_cur(d, pub, pvb, ahmf, i, j - 1, k);
-    const double cur_w = ldf_cur(d, pub, pvb, ahmf, i - 1, j, k);
-    const double div = ldf_div(d, pub, pvb, ahmt, i, j, k);
-    const double div_e = ldf_div(d, pub, pvb, ahmt, i + 1, j, k);
-    const double div_n = ldf_div(d, pub, pvb, ahmt, i, j + 1, k);
+    const double cur = scur[lj + 1][li + 1], cur_s = scur[lj][li + 1], cur_w = scur[lj + 1][li];
+    const double div = sdiv[lj][li], div_e = sdiv[lj][li + 1], div_n = sdiv[lj + 1][li];
     const uint32_t ij = IJ32(i, j);
     A3(pua, i, j, k) = (T)((double)A3(pua, i, j, k) +
                            zsign * (-((cur - cur_s) * m.r1_e2u[ij] / E3(m.e3u, ij, k)) + (div_e - div) * m.r1_e1u[ij]));
@@ -303,8 +318,10 @@ template <typename T>
 int launch_dyn_ldf_lap(double zsign, const T *pub, const T *pvb, T *pua, T *pva, const T *ahmt, const T *ahmf)
 {
     const DevDomain &d = nh::dom();
-    Grid2 g = grid2(d.jpi - 2, d.jpj - 2);
-    g.grid.z = (unsigned)(d.jpk - 1);
+    Grid2 g;
+    g.block = dim3(LDF_TX, LDF_TY, 1);
+    g.grid = dim3((unsigned)((d.jpi - 2 + LDF_TX - 1) / LDF_TX), (unsigned)((d.jpj - 2 + LDF_TY - 1) / LDF_TY),
+                  (unsigned)(d.jpk - 1));
     LAUNCH("dyn_ldf", k_dyn_ldf_lap<T>, g, d, zsign, pub, pvb, pua, pva, ahmt, ahmf);
     return 0;
 }
