@@ -2541,11 +2541,11 @@ static void lbc3_adj(double *p, int nk, char nat)
    state (tb, tn): cx, cy at interior faces, cz(k) for the face above level k
    (k = 2..jpk); other points are left as they are */
 int ora_fct_coef(double p2dt, const double *pun, const double *pvn, const double *pwn, const double *tb,
-                 const double *tn, double *cx, double *cy, double *cz)
+                 const double *tn, int kn_h, int kn_v, double *cx, double *cy, double *cz)
 {
     double *ta = z3();
     fct_rec_x = cx; fct_rec_y = cy; fct_rec_z = cz;
-    int st = ora_tra_adv_fct(1, 1, "TRC", p2dt, pun, pvn, pwn, tb, tn, ta, 1, 2, 2);
+    int st = ora_tra_adv_fct(1, 1, "TRC", p2dt, pun, pvn, pwn, tb, tn, ta, 1, kn_h, kn_v);
     fct_rec_x = fct_rec_y = fct_rec_z = NULL;
     free(ta);
     return st;
@@ -2554,10 +2554,11 @@ int ora_fct_coef(double p2dt, const double *pun, const double *pvn, const double
 /* pta += the FCT 2/2 trend with the limiter factors cx, cy, cz: linear in
    (tb, tn); with the factors of the same state it is tra_adv_fct itself */
 int ora_fct_tl(double p2dt, const double *pun, const double *pvn, const double *pwn, const double *tb,
-               const double *tn, const double *cx, const double *cy, const double *cz, double *pta)
+               const double *tn, int kn_h, int kn_v, const double *cx, const double *cy, const double *cz,
+               double *pta)
 {
     fct_frz_x = cx; fct_frz_y = cy; fct_frz_z = cz;
-    int st = ora_tra_adv_fct(1, 1, "TRC", p2dt, pun, pvn, pwn, tb, tn, pta, 1, 2, 2);
+    int st = ora_tra_adv_fct(1, 1, "TRC", p2dt, pun, pvn, pwn, tb, tn, pta, 1, kn_h, kn_v);
     fct_frz_x = fct_frz_y = fct_frz_z = NULL;
     return st;
 }
@@ -2565,10 +2566,39 @@ int ora_fct_tl(double p2dt, const double *pun, const double *pvn, const double *
 /* adjoint of ora_fct_tl: tb_ad += dJ/dtb and tn_ad += dJ/dtn from
    ta_ad = dJ/d(pta) (interior cells; ring cells of tb_ad, tn_ad receive
    the dependence on the ring values of tb, tn) */
-int ora_fct_ad(const double *pun, const double *pvn, const double *pwn, const double *cx, const double *cy,
-               const double *cz, const double *ta_ad, double *tb_ad, double *tn_ad)
+/* the transpose of interp_4th_cpt (above) on one interior column: tn_ad +=
+   from zt_ad, the adjoint of its output at levels 2..jpkm1 */
+static void interp_4th_cpt_ad(int i, int j, const double *zt_ad, double *tn_ad)
+{
+    int k;
+    const int K = jpkm1, ikt = A2(mikt, i, j) + 1, ikb = A2(mbkt, i, j);
+    double zwd[jpk + 1], zwi[jpk + 1], zws[jpk + 1], c[jpk + 1], zwt[jpk + 1], x[jpk + 1], r[jpk + 1];
+    for (k = 2; k <= K; k++) {
+        const double wm = WMASK(i, j, k);
+        zwd[k] = 3.0 * wm + 1.0; zwi[k] = wm; zws[k] = wm; c[k] = 3.0 * wm;
+        if (k == ikt || k == ikb) { zwd[k] = 1.0; zwi[k] = 0.0; zws[k] = 0.0; c[k] = 0.5; }
+    }
+    zwt[2] = zwd[2];
+    for (k = 3; k <= K; k++) zwt[k] = zwd[k] - zwi[k] * zws[k - 1] / zwt[k - 1];
+    for (k = 2; k <= K; k++) { x[k] = A3(zt_ad, i, j, k); r[k] = 0.0; }
+    for (k = 2; k <= K - 1; k++) {       /* back substitution reversed */
+        const double t = x[k] / zwt[k];
+        r[k] += t;
+        x[k + 1] -= zws[k] * t;
+    }
+    r[K] += x[K] / zwt[K];
+    for (k = K; k >= 3; k--) r[k - 1] -= zwi[k] / zwt[k - 1] * r[k];   /* forward substitution reversed */
+    for (k = 2; k <= K; k++) {           /* zwrm = c (tn(k) + tn(k-1)) */
+        A3(tn_ad, i, j, k) += c[k] * r[k];
+        A3(tn_ad, i, j, k - 1) += c[k] * r[k];
+    }
+}
+
+int ora_fct_ad(const double *pun, const double *pvn, const double *pwn, int kn_h, int kn_v, const double *cx,
+               const double *cy, const double *cz, const double *ta_ad, double *tb_ad, double *tn_ad)
 {
     int i, j, k;
+    const double r1_6 = 1.0 / 6.0;
     double *ax = z3(), *ay = z3(), *az = z3(), *xb = z3(), *yb = z3(), *zb = z3();
     /* the final trend (lines 288-297): ta -= (dx + dy + dz) r1_e1e2t / e3t */
     for (k = 1; k <= jpkm1; k++)
@@ -2596,7 +2626,8 @@ int ora_fct_ad(const double *pun, const double *pvn, const double *pwn, const do
                 if (!in || k == jpk) { A3(ax, i, j, k) = 0.; A3(ay, i, j, k) = 0.; }
                 if (!in || k == 1) A3(az, i, j, k) = 0.;
             }
-    /* antidiffusive fluxes (lines 182-190, 255-263): A = high order - upstream */
+    /* antidiffusive fluxes (lines 182-278): A = high order - upstream */
+    double *zlu = z3(), *zlv = z3(), *ztu = z3(), *ztv = z3(), *ztw = z3();
     for (k = 1; k <= jpkm1; k++)
         for (j = 1; j <= jpjm1; j++)
             for (i = 1; i <= jpim1; i++) {
@@ -2606,15 +2637,46 @@ int ora_fct_ad(const double *pun, const double *pvn, const double *pwn, const do
                 A3(tn_ad, i, j + 1, k) += 0.5 * A3(pvn, i, j, k) * A3(ay, i, j, k);
                 A3(xb, i, j, k) -= A3(ax, i, j, k);
                 A3(yb, i, j, k) -= A3(ay, i, j, k);
+                if (kn_h == 4) {   /* + 0.5 u (zltu(i) - zltu(i+1)) */
+                    A3(zlu, i, j, k) += 0.5 * A3(pun, i, j, k) * A3(ax, i, j, k);
+                    A3(zlu, i + 1, j, k) -= 0.5 * A3(pun, i, j, k) * A3(ax, i, j, k);
+                    A3(zlv, i, j, k) += 0.5 * A3(pvn, i, j, k) * A3(ay, i, j, k);
+                    A3(zlv, i, j + 1, k) -= 0.5 * A3(pvn, i, j, k) * A3(ay, i, j, k);
+                }
             }
+    if (kn_h == 4) {       /* zltu = (ztu(i) + ztu(i-1)) / 6, lbc 'T'; ztu = dtn umask */
+        lbc3_adj(zlu, jpk, 'T');
+        lbc3_adj(zlv, jpk, 'T');
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 2; j <= jpjm1; j++)
+                for (i = 2; i <= jpim1; i++) {
+                    A3(ztu, i, j, k) += A3(zlu, i, j, k) * r1_6; A3(ztu, i - 1, j, k) += A3(zlu, i, j, k) * r1_6;
+                    A3(ztv, i, j, k) += A3(zlv, i, j, k) * r1_6; A3(ztv, i, j - 1, k) += A3(zlv, i, j, k) * r1_6;
+                }
+        for (k = 1; k <= jpkm1; k++)
+            for (j = 1; j <= jpjm1; j++)
+                for (i = 1; i <= jpim1; i++) {
+                    const double tu = A3(ztu, i, j, k) * UMASK(i, j, k), tv = A3(ztv, i, j, k) * VMASK(i, j, k);
+                    A3(tn_ad, i + 1, j, k) += tu; A3(tn_ad, i, j, k) -= tu;
+                    A3(tn_ad, i, j + 1, k) += tv; A3(tn_ad, i, j, k) -= tv;
+                }
+    }
     for (k = 2; k <= jpkm1; k++)
         for (j = 2; j <= jpjm1; j++)
             for (i = 2; i <= jpim1; i++) {
                 const double t = A3(az, i, j, k) * WMASK(i, j, k);
-                A3(tn_ad, i, j, k) += A3(pwn, i, j, k) * 0.5 * t;
-                A3(tn_ad, i, j, k - 1) += A3(pwn, i, j, k) * 0.5 * t;
+                if (kn_v == 4) {
+                    A3(ztw, i, j, k) = A3(pwn, i, j, k) * t;
+                } else {
+                    A3(tn_ad, i, j, k) += A3(pwn, i, j, k) * 0.5 * t;
+                    A3(tn_ad, i, j, k - 1) += A3(pwn, i, j, k) * 0.5 * t;
+                }
                 A3(zb, i, j, k) -= t;
             }
+    if (kn_v == 4)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) interp_4th_cpt_ad(i, j, ztw, tn_ad);
+    free(zlu); free(zlv); free(ztu); free(ztv); free(ztw);
     /* the low-order trend (lines 158-170) */
     for (k = 1; k <= jpkm1; k++)
         for (j = 2; j <= jpjm1; j++)

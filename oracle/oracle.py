@@ -190,17 +190,17 @@ class Oracle:
                                 _p(uslp), _p(vslp), _p(wslpi), _p(wslpj)), "ldf_slp")
 
     # tangent linear and adjoint of the tracer step (one tracer) -------------
-    def fct_coef(self, p2dt, pun, pvn, pwn, tb, tn, cx, cy, cz):
-        _chk(self.L.ora_fct_coef(C.c_double(p2dt), _p(pun), _p(pvn), _p(pwn), _p(tb), _p(tn), _p(cx), _p(cy),
-                                 _p(cz)), "fct_coef")
+    def fct_coef(self, p2dt, pun, pvn, pwn, tb, tn, cx, cy, cz, kn=(2, 2)):
+        _chk(self.L.ora_fct_coef(C.c_double(p2dt), _p(pun), _p(pvn), _p(pwn), _p(tb), _p(tn), C.c_int(kn[0]),
+                                 C.c_int(kn[1]), _p(cx), _p(cy), _p(cz)), "fct_coef")
 
-    def fct_tl(self, p2dt, pun, pvn, pwn, tb, tn, cx, cy, cz, ta):
-        _chk(self.L.ora_fct_tl(C.c_double(p2dt), _p(pun), _p(pvn), _p(pwn), _p(tb), _p(tn), _p(cx), _p(cy), _p(cz),
-                               _p(ta)), "fct_tl")
+    def fct_tl(self, p2dt, pun, pvn, pwn, tb, tn, cx, cy, cz, ta, kn=(2, 2)):
+        _chk(self.L.ora_fct_tl(C.c_double(p2dt), _p(pun), _p(pvn), _p(pwn), _p(tb), _p(tn), C.c_int(kn[0]),
+                               C.c_int(kn[1]), _p(cx), _p(cy), _p(cz), _p(ta)), "fct_tl")
 
-    def fct_ad(self, pun, pvn, pwn, cx, cy, cz, ta_ad, tb_ad, tn_ad):
-        _chk(self.L.ora_fct_ad(_p(pun), _p(pvn), _p(pwn), _p(cx), _p(cy), _p(cz), _p(ta_ad), _p(tb_ad), _p(tn_ad)),
-             "fct_ad")
+    def fct_ad(self, pun, pvn, pwn, cx, cy, cz, ta_ad, tb_ad, tn_ad, kn=(2, 2)):
+        _chk(self.L.ora_fct_ad(_p(pun), _p(pvn), _p(pwn), C.c_int(kn[0]), C.c_int(kn[1]), _p(cx), _p(cy), _p(cz),
+                               _p(ta_ad), _p(tb_ad), _p(tn_ad)), "fct_ad")
 
     def ldf_lap_ad(self, pahu, pahv, ta_ad, tb_ad):
         _chk(self.L.ora_ldf_lap_ad(_p(pahu), _p(pahv), _p(ta_ad), _p(tb_ad)), "ldf_lap_ad")

@@ -1,5 +1,6 @@
 """CPU checks that pin the oracle's tangent linear and adjoint of the tracer
-step (one tracer: FCT 2/2 with the nonosc limiter factors frozen,
+step (one tracer: FCT with the nonosc limiter factors frozen, orders 2
+and 4 horizontally and vertically; the composed step uses 2/2 and 4/4,
 tra_ldf_lap, tra_zdf_imp, leapfrog tra_nxt):
 
 * with the limiter factors of the state itself, the frozen-limiter FCT is
@@ -36,10 +37,10 @@ def t1(S, name):
     return np.asfortranarray(S[name][..., 0])
 
 
-def coef(o, d, F, tb, tn):
+def coef(o, d, F, tb, tn, kn=(2, 2)):
     s3 = (d.jpi, d.jpj, d.jpk)
     c = [np.ones(s3, order="F") for _ in range(3)]
-    o.fct_coef(F["p2dt"], F["zun"], F["zvn"], F["zwn"], tb, tn, *c)
+    o.fct_coef(F["p2dt"], F["zun"], F["zvn"], F["zwn"], tb, tn, *c, kn=kn)
     return c
 
 
@@ -84,11 +85,15 @@ def close(lhs, rhs, tol=1e-12):
     return abs(lhs - rhs) <= tol * max(abs(lhs), abs(rhs), 1e-300)
 
 
+KN = [(2, 2), (4, 4), (2, 4), (4, 2)]
+
+
 @pytest.mark.parametrize("jperio", [0, 7])
-def test_frozen_fct_with_own_factors_is_tra_adv_fct(jperio):
+@pytest.mark.parametrize("kn", KN, ids=[f"{a}{b}" for a, b in KN])
+def test_frozen_fct_with_own_factors_is_tra_adv_fct(jperio, kn):
     d, S, o, F = setup(jperio)
     tb, tn = t1(S, "tsb"), t1(S, "tsn")
-    c = coef(o, d, F, tb, tn)
+    c = coef(o, d, F, tb, tn, kn)
     t, u, v, w, f = configs.masks(d)
     inner = (slice(1, -1), slice(1, -1), slice(0, -1))
     wz = w.copy()
@@ -98,10 +103,25 @@ def test_frozen_fct_with_own_factors_is_tra_adv_fct(jperio):
         assert val.min() >= 0.0 and val.max() <= 1.0
     assert (c[0][inner][u[inner] > 0] < 1.0).any(), "the limiter should act somewhere"
     want = np.asfortranarray(S["tsa"][..., 0])
-    o.tra_adv_fct(KT, 1, "TRA", F["p2dt"], F["zun"], F["zvn"], F["zwn"], tb, tn, want, 1, 2, 2)
+    o.tra_adv_fct(KT, 1, "TRA", F["p2dt"], F["zun"], F["zvn"], F["zwn"], tb, tn, want, 1, *kn)
     got = np.asfortranarray(S["tsa"][..., 0])
-    o.fct_tl(F["p2dt"], F["zun"], F["zvn"], F["zwn"], tb, tn, *c, got)
+    o.fct_tl(F["p2dt"], F["zun"], F["zvn"], F["zwn"], tb, tn, *c, got, kn=kn)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("jperio", [0, 7])
+@pytest.mark.parametrize("kn", KN, ids=[f"{a}{b}" for a, b in KN])
+def test_fct_dot_product_all_orders(jperio, kn):
+    d, S, o, F = setup(jperio)
+    rng = np.random.default_rng(50 + jperio)
+    c = coef(o, d, F, t1(S, "tsb"), t1(S, "tsn"), kn)
+    s3 = (d.jpi, d.jpj, d.jpk)
+    xb, xn, y = rnd(rng, d), rnd(rng, d), interior(d, rnd(rng, d))
+    ta = np.zeros(s3, order="F")
+    o.fct_tl(F["p2dt"], F["zun"], F["zvn"], F["zwn"], xb, xn, *c, ta, kn=kn)
+    ab, an = np.zeros(s3, order="F"), np.zeros(s3, order="F")
+    o.fct_ad(F["zun"], F["zvn"], F["zwn"], *c, y, ab, an, kn=kn)
+    assert close(np.vdot(ta, y), np.vdot(xb, ab) + np.vdot(xn, an)), kn
 
 
 @pytest.mark.parametrize("jperio", [0, 7])
