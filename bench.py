@@ -332,9 +332,9 @@ def time_host_mode(h, d, S, steps):
     return ms
 
 
-def time_adjoint(h, size, steps, ckpt, ndir, seed):
+def time_adjoint(h, size, steps, ckpt, ndir, seed, kn=(4, 4)):
     """The adjoint form: J = 1/2 sum over wet surface points of (SST - target)^2
-    after `steps` tracer steps (FCT 2/2, laplacian, implicit diffusion,
+    after `steps` tracer steps (FCT of orders kn, laplacian, implicit diffusion,
     leapfrog filter; velocities and diffusivities fixed), target = SST +
     N(0, 0.1 degC); its gradient with respect to the initial temperature by
     the adjoint with the limiter factors frozen, the forward state kept every
@@ -370,10 +370,10 @@ def time_adjoint(h, size, steps, ckpt, ndir, seed):
                 """one tracer step from (tb, tn); own: the factors of the state
                 itself (the nonlinear step), else the frozen factors cc"""
                 if own:
-                    h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, tb, tn, *cc)
+                    h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, tb, tn, *cc, kn=kn)
                 ta = A[:n3]
                 ta.zero_()
-                h.tra_adv_fct_tl(p2dt, zun, zvn, zwn, tb, tn, *cc, ta)
+                h.tra_adv_fct_tl(p2dt, zun, zvn, zwn, tb, tn, *cc, ta, kn=kn)
                 h.tra_ldf_lap(kt, 1, "TRA", ahtu, ahtv, tb, ta, 1, 1)
                 h.tra_zdf_imp(kt, 1, "TRA", p2dt, tb, ta, 1, avt, avt)
                 B[:n3].copy_(tb)
@@ -414,28 +414,28 @@ def time_adjoint(h, size, steps, ckpt, ndir, seed):
                 seg = []
                 for q in range(s0, min(s0 + ckpt, steps)):
                     cc = [f64() for _ in range(3)]
-                    h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, b, n, *cc)
+                    h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, b, n, *cc, kn=kn)
                     seg.append(cc)
                     b, n = step(b, n, cc, False)
                 for cc in reversed(seg):
                     h.tra_nxt_ad(lb, ln, la)
                     h.tra_zdf_imp_ad(p2dt, avt, la, lb)
                     h.tra_ldf_lap_ad(ahtu, ahtv, la, lb)
-                    h.tra_adv_fct_ad(zun, zvn, zwn, *cc, la, lb, ln)
+                    h.tra_adv_fct_ad(zun, zvn, zwn, *cc, la, lb, ln, kn=kn)
                 del seg
             grad = lb + ln
             h.lbc_adj(grad, jpk)
             h.sync()
             t2 = time.perf_counter()
             # dot-product test of one frozen step (factors of the initial state)
-            h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, T0, T0, *c)
+            h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, T0, T0, *c, kn=kn)
             xb, xn, yb, yn = (torch.from_numpy(rng.standard_normal(n3)).to("cuda") for _ in range(4))
             b1, n1 = step(xb, xn, c, False)
             gb, gn = yb.clone(), yn.clone()
             h.tra_nxt_ad(gb, gn, la)
             h.tra_zdf_imp_ad(p2dt, avt, la, gb)
             h.tra_ldf_lap_ad(ahtu, ahtv, la, gb)
-            h.tra_adv_fct_ad(zun, zvn, zwn, *c, la, gb, gn)
+            h.tra_adv_fct_ad(zun, zvn, zwn, *c, la, gb, gn, kn=kn)
             lhs = float(torch.dot(b1, yb) + torch.dot(n1, yn))
             rhs = float(torch.dot(xb, gb) + torch.dot(xn, gn))
             dot_err = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
@@ -448,7 +448,7 @@ def time_adjoint(h, size, steps, ckpt, ndir, seed):
             frozen, b, n = [], T0.clone(), T0.clone()
             for q in range(steps):
                 cc = [f64() for _ in range(3)]
-                h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, b, n, *cc)
+                h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, b, n, *cc, kn=kn)
                 frozen.append(cc)
                 b, n = step(b, n, cc, False)
             r = wgt * (n - target)
@@ -477,7 +477,7 @@ def time_adjoint(h, size, steps, ckpt, ndir, seed):
         h.context_destroy(ctx)
     errs, errs_nl = np.array(errs), np.array(errs_nl)
     ndir = len(errs)
-    return {"workload": f"{jpi}x{jpj}x{jpk} FP64 tracer step (FCT 2/2 with the limiter frozen, laplacian, implicit "
+    return {"workload": f"{jpi}x{jpj}x{jpk} FP64 tracer step (FCT {kn[0]}/{kn[1]} with the limiter frozen, laplacian, implicit "
                         f"diffusion, leapfrog filter), {steps} steps, checkpoints every {ckpt}",
             "forward_ms_per_step": (t1 - t0) * 1e3 / steps,
             "adjoint_ms_per_step": (t2 - t1) * 1e3 / steps,

@@ -377,7 +377,7 @@ int nemo_hip_rst_write(const char *dir, int kt, const nemo_hip_rst_t *rst);
    not NULL the four depth means are computed as DOM/istate.F90:159-178. */
 int nemo_hip_rst_read(const char *dir, nemo_hip_rst_t *rst, int *kt, int *neuler);
 /* Tangent linear and adjoint of the tracer step, one tracer, FP64: the
-   step tra_adv_fct (orders 2/2) + tra_ldf_lap (kpass 1) + tra_zdf_imp (avt,
+   step tra_adv_fct (orders 2 or 4) + tra_ldf_lap (kpass 1) + tra_zdf_imp (avt,
    no slopes) + leapfrog tra_nxt, with the nonosc limiter factors frozen
    from the forward state (the step is then linear in the tracer).  The
    adjoint entry points have no counterpart in the reference; each is the
@@ -387,15 +387,16 @@ int nemo_hip_rst_read(const char *dir, nemo_hip_rst_t *rst, int *kt, int *neuler
    the state (tb, tn): cx, cy at interior faces, cz(k) for the face above
    level k (k = 2..jpk).  pun/pvn/pwn are the transports tra_adv_fct takes. */
 int nemo_hip_tra_adv_fct_coef(double p2dt, const double *pun, const double *pvn, const double *pwn, const double *ptb,
-                              const double *ptn, double *pcx, double *pcy, double *pcz);
-/* pta += the FCT 2/2 trend with the factors given (tra_adv_fct itself when
-   they are the state's own) */
+                              const double *ptn, int kn_fct_h, int kn_fct_v, double *pcx, double *pcy, double *pcz);
+/* pta += the FCT trend (orders kn_fct_h, kn_fct_v: 2 or 4) with the factors
+   given (tra_adv_fct itself when they are the state's own) */
 int nemo_hip_tra_adv_fct_tl(double p2dt, const double *pun, const double *pvn, const double *pwn, const double *ptb,
-                            const double *ptn, const double *pcx, const double *pcy, const double *pcz, double *pta);
+                            const double *ptn, int kn_fct_h, int kn_fct_v, const double *pcx, const double *pcy,
+                            const double *pcz, double *pta);
 /* ptb_ad, ptn_ad += the transpose of tra_adv_fct_tl applied to pta_ad */
-int nemo_hip_tra_adv_fct_ad(const double *pun, const double *pvn, const double *pwn, const double *pcx,
-                            const double *pcy, const double *pcz, const double *pta_ad, double *ptb_ad,
-                            double *ptn_ad);
+int nemo_hip_tra_adv_fct_ad(const double *pun, const double *pvn, const double *pwn, int kn_fct_h, int kn_fct_v,
+                            const double *pcx, const double *pcy, const double *pcz, const double *pta_ad,
+                            double *ptb_ad, double *ptn_ad);
 /* ptb_ad += the transpose of tra_ldf_lap (kpass 1) applied to pta_ad */
 int nemo_hip_tra_ldf_lap_ad(const double *pahu, const double *pahv, const double *pta_ad, double *ptb_ad);
 /* tra_zdf_imp: on entry pta_ad is the adjoint of the after field, on exit

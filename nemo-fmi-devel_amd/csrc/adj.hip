@@ -1,5 +1,5 @@
 // adj.hip -- the adjoint of the tracer step for one tracer, FP64: tra_adv_fct
-// (orders 2/2, the nonosc limiter factors frozen from the forward state),
+// (orders 2 or 4, the nonosc limiter factors frozen from the forward state),
 // tra_ldf_lap (kpass 1), tra_zdf_imp (avt, no slopes) and the leapfrog
 // tra_nxt, TRA/traadv_fct.F90, traldf_lap_blp.F90, trazdf.F90, tranxt.F90.
 //
@@ -37,14 +37,20 @@ __device__ __forceinline__ void ring_of(const DevDomain &d, int i, int j, F f)
 }
 
 // ---------------------------------------------------------------------------
-// tra_adv_fct, FCT 2/2 with frozen factors.  The forward trend at an
-// interior cell is  -(dX + dY + dZ) r1/e3t tmask  (upstream fluxes of tb)
+// tra_adv_fct with frozen factors.  The forward trend at an interior cell
+// is  -(dX + dY + dZ) r1/e3t tmask  (upstream fluxes of tb)
 // -(dLx + dLy + dLz) r1/e3t  (limited antidiffusive fluxes: factor times
-// the centred flux of tn minus the upstream flux, the horizontal ones
-// through lbc_lnk 'U'/'V').
+// the high-order flux of tn minus the upstream flux, the horizontal ones
+// through lbc_lnk 'U'/'V').  The 4th-order horizontal flux adds
+// 0.5 u (zltu(i) - zltu(i+1)), zltu = (ztu(i) + ztu(i-1))/6 at interior
+// cells then lbc_lnk 'T', ztu = (tn(i+1) - tn(i)) umask
+// (traadv_fct.F90:192-221); the 4th-order vertical one replaces
+// (tn(k) + tn(k-1))/2 by the compact interpolation interp_4th_cpt.
 // ---------------------------------------------------------------------------
 struct FctAd {
     const double *un, *vn, *wn, *cx, *cy, *cz, *mu;
+    const double *zl_u, *zl_v;   // kn_h = 4: the adjoint of zltu / zltv after the lbc fold
+    int kn_h, kn_v;
 };
 
 // mu r1/e3t (hi) and mu r1/e3t tmask (lo) of a cell; zero off the interior
@@ -122,8 +128,24 @@ KERNEL void k_fct_ad(DevDomain d, FctAd a, double *__restrict__ tb_ad, double *_
         gn += 0.5 * v * ay_face(d, a, i, j - 1, k);
         gb += 0.5 * (v - fabs(v)) * yb_face(d, a, i, j - 1, k);
     }
-    // vertical: the antidiffusive flux of w-levels k and k+1 (interior columns)
-    if (interior(d, i, j)) {
+    if (a.kn_h == 4) {   // ztu = dtn umask feeds zltu at cells i and i+1 (r1_6 each)
+        const double r1_6 = 1.0 / 6.0;
+        if (j <= d.jpj - 1) {
+            if (i >= 2)
+                gn += (A3(a.zl_u, i - 1, j, k) * r1_6 + A3(a.zl_u, i, j, k) * r1_6) * UMASK(double, i - 1, j, k);
+            if (i <= d.jpi - 1)
+                gn -= (A3(a.zl_u, i, j, k) * r1_6 + A3(a.zl_u, i + 1, j, k) * r1_6) * UMASK(double, i, j, k);
+        }
+        if (i <= d.jpi - 1) {
+            if (j >= 2)
+                gn += (A3(a.zl_v, i, j - 1, k) * r1_6 + A3(a.zl_v, i, j, k) * r1_6) * VMASK(double, i, j - 1, k);
+            if (j <= d.jpj - 1)
+                gn -= (A3(a.zl_v, i, j, k) * r1_6 + A3(a.zl_v, i, j + 1, k) * r1_6) * VMASK(double, i, j, k);
+        }
+    }
+    // vertical: the antidiffusive flux of w-levels k and k+1 (interior
+    // columns); the 4th-order one is k_fct_ad_v4's
+    if (a.kn_v == 2 && interior(d, i, j)) {
         if (k >= 2) gn += A3(a.wn, i, j, k) * 0.5 * (az_face(d, a, i, j, k) * WMASK(double, i, j, k));
         if (k + 1 <= d.jpk - 1)
             gn += A3(a.wn, i, j, k + 1) * 0.5 * (az_face(d, a, i, j, k + 1) * WMASK(double, i, j, k + 1));
@@ -142,6 +164,81 @@ KERNEL void k_fct_ad(DevDomain d, FctAd a, double *__restrict__ tb_ad, double *_
     const uint32_t p = IJK32(i, j, k);
     at32(tb_ad, p) += gb;
     at32(tn_ad, p) += gn;
+}
+
+// the adjoint of zltu before the lbc: 0.5 u ax of face i minus that of face i-1
+__device__ __forceinline__ double zlu_raw(const DevDomain &d, const FctAd &a, int i, int j, int k)
+{
+    if (j > d.jpj - 1) return 0.;
+    double v = 0.;
+    if (i >= 2) v = -(0.5 * A3(a.un, i - 1, j, k) * ax_face(d, a, i - 1, j, k));
+    if (i <= d.jpi - 1) v += 0.5 * A3(a.un, i, j, k) * ax_face(d, a, i, j, k);
+    return v;
+}
+__device__ __forceinline__ double zlv_raw(const DevDomain &d, const FctAd &a, int i, int j, int k)
+{
+    if (i > d.jpi - 1) return 0.;
+    double v = 0.;
+    if (j >= 2) v = -(0.5 * A3(a.vn, i, j - 1, k) * ay_face(d, a, i, j - 1, k));
+    if (j <= d.jpj - 1) v += 0.5 * A3(a.vn, i, j, k) * ay_face(d, a, i, j, k);
+    return v;
+}
+
+// kn_h = 4: zl_u, zl_v = the adjoint of zltu, zltv with the lbc_lnk 'T'
+// transposed (ring values added to their interior sources); zero off the
+// interior, where the forward computes no zltu of its own
+KERNEL void k_fct_ad_l4(DevDomain d, FctAd a, double *__restrict__ zl_u, double *__restrict__ zl_v)
+{
+    THREAD_IJ(1, 1);
+    const int k = (int)blockIdx.z + 1;
+    if (i > d.jpi || j > d.jpj) return;
+    double lu = 0., lv = 0.;
+    if (interior(d, i, j)) {
+        lu = zlu_raw(d, a, i, j, k);
+        lv = zlv_raw(d, a, i, j, k);
+        ring_of(d, i, j, [&](int ri, int rj) {
+            lu += zlu_raw(d, a, ri, rj, k);
+            lv += zlv_raw(d, a, ri, rj, k);
+        });
+    }
+    A3(zl_u, i, j, k) = lu;
+    A3(zl_v, i, j, k) = lv;
+}
+
+// kn_v = 4: ztw = interp_4th_cpt(tn) enters the vertical flux as w ztw;
+// the transpose of the compact solve (traadv_fct.F90:535-614) on one
+// interior column per thread, the substitutions reversed
+KERNEL void k_fct_ad_v4(DevDomain d, FctAd a, double *__restrict__ tn_ad)
+{
+    THREAD_IJ(2, 2);
+    if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    const int K = d.jpk - 1, ikt = A2(d.mikt, i, j) + 1, ikb = A2(d.mbkt, i, j);
+    double zwt[NH_MAXK], zs[NH_MAXK], cc[NH_MAXK], x[NH_MAXK], r[NH_MAXK];
+    for (int k = 2; k <= K; k++) {
+        const double wm = WMASK(double, i, j, k);
+        double zd = 3.0 * wm + 1.0;
+        zs[k] = wm;           // the sub- and super-diagonals are equal
+        cc[k] = 3.0 * wm;     // right-hand side: cc (tn(k) + tn(k-1))
+        if (k == ikt || k == ikb) {
+            zd = 1.0;
+            zs[k] = 0.0;
+            cc[k] = 0.5;
+        }
+        zwt[k] = (k == 2) ? zd : zd - zs[k] * zs[k - 1] / zwt[k - 1];
+        x[k] = A3(a.wn, i, j, k) * (az_face(d, a, i, j, k) * WMASK(double, i, j, k));
+        r[k] = 0.0;
+    }
+    for (int k = 2; k <= K - 1; k++) {   // back substitution reversed
+        const double t = x[k] / zwt[k];
+        r[k] += t;
+        x[k + 1] -= zs[k] * t;
+    }
+    r[K] += x[K] / zwt[K];
+    for (int k = K; k >= 3; k--) r[k - 1] -= zs[k] / zwt[k - 1] * r[k];   // forward substitution reversed
+    for (int k = 2; k <= K; k++) {
+        A3(tn_ad, i, j, k) += cc[k] * r[k];
+        A3(tn_ad, i, j, k - 1) += cc[k] * r[k];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -275,14 +372,22 @@ KERNEL void k_lbc_adj(DevDomain d, double *__restrict__ f)
 
 }  // namespace
 
-int launch_fct_ad(const double *pun, const double *pvn, const double *pwn, const double *cx, const double *cy,
-                  const double *cz, const double *ta_ad, double *tb_ad, double *tn_ad)
+int launch_fct_ad(const double *pun, const double *pvn, const double *pwn, int kn_h, int kn_v, const double *cx,
+                  const double *cy, const double *cz, const double *ta_ad, double *tb_ad, double *tn_ad)
 {
     const DevDomain &d = nh::dom();
+    if (kn_v == 4 && d.jpk > NH_MAXK - 1)
+        return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "tra_adv_fct_ad: jpk > %d", NH_MAXK - 1);
     Grid2 g = grid2(d.jpi, d.jpj);
     g.grid.z = (unsigned)(d.jpk - 1);
-    const FctAd a{pun, pvn, pwn, cx, cy, cz, ta_ad};
+    double *zl_u = (double *)nh::scratch3(nh::S_ZWI), *zl_v = (double *)nh::scratch3(nh::S_ZWX);
+    const FctAd a{pun, pvn, pwn, cx, cy, cz, ta_ad, zl_u, zl_v, kn_h, kn_v};
+    if (kn_h == 4) LAUNCH("tra_adv_fct_ad", k_fct_ad_l4, g, d, a, zl_u, zl_v);
     LAUNCH("tra_adv_fct_ad", k_fct_ad, g, d, a, tb_ad, tn_ad);
+    if (kn_v == 4) {
+        Grid2 gc = grid2(d.jpi, d.jpj);
+        LAUNCH("tra_adv_fct_ad", k_fct_ad_v4, gc, d, a, tn_ad);
+    }
     return 0;
 }
 

@@ -1756,24 +1756,33 @@ int nemo_hip_rst_read(const char *dir, nemo_hip_rst_t *r, int *kt, int *neuler)
 // ---------------------------------------------------------------------------
 // tangent linear and adjoint of the tracer step (one tracer, FP64)
 // ---------------------------------------------------------------------------
+#define NEED_FCT_ORDERS(h, v)                                                                            \
+    do {                                                                                                 \
+        if (((h) != 2 && (h) != 4) || ((v) != 2 && (v) != 4))                                           \
+            return set_error(NEMO_HIP_ERR_ARG, "kn_fct_h = %d, kn_fct_v = %d (2 or 4)", (h), (v));      \
+    } while (0)
+
 int nemo_hip_tra_adv_fct_coef(double p2dt, const double *pun, const double *pvn, const double *pwn, const double *ptb,
-                              const double *ptn, double *pcx, double *pcy, double *pcz)
+                              const double *ptn, int kn_fct_h, int kn_fct_v, double *pcx, double *pcy, double *pcz)
 {
     NEED_DOMAIN();
+    NEED_FCT_ORDERS(kn_fct_h, kn_fct_v);
     NEED_PTR(pun); NEED_PTR(pvn); NEED_PTR(pwn); NEED_PTR(ptb); NEED_PTR(ptn); NEED_PTR(pcx); NEED_PTR(pcy);
     NEED_PTR(pcz);
     const size_t n3 = g_c->dom.n3;
     Stager S;
     const double *u = S.in(pun, n3), *v = S.in(pvn, n3), *w = S.in(pwn, n3), *tb = S.in(ptb, n3), *tn = S.in(ptn, n3);
     double *cx = S.inout(pcx, n3), *cy = S.inout(pcy, n3), *cz = S.inout(pcz, n3);
-    int rc = S.status ? S.status : launch_fct_frozen(0, p2dt, u, v, w, tb, tn, cx, cy, cz, nullptr);
+    int rc = S.status ? S.status : launch_fct_frozen(0, p2dt, u, v, w, tb, tn, kn_fct_h, kn_fct_v, cx, cy, cz, nullptr);
     return S.finish(rc);
 }
 
 int nemo_hip_tra_adv_fct_tl(double p2dt, const double *pun, const double *pvn, const double *pwn, const double *ptb,
-                            const double *ptn, const double *pcx, const double *pcy, const double *pcz, double *pta)
+                            const double *ptn, int kn_fct_h, int kn_fct_v, const double *pcx, const double *pcy,
+                            const double *pcz, double *pta)
 {
     NEED_DOMAIN();
+    NEED_FCT_ORDERS(kn_fct_h, kn_fct_v);
     NEED_PTR(pun); NEED_PTR(pvn); NEED_PTR(pwn); NEED_PTR(ptb); NEED_PTR(ptn); NEED_PTR(pcx); NEED_PTR(pcy);
     NEED_PTR(pcz); NEED_PTR(pta);
     const size_t n3 = g_c->dom.n3;
@@ -1782,15 +1791,17 @@ int nemo_hip_tra_adv_fct_tl(double p2dt, const double *pun, const double *pvn, c
     const double *cx = S.in(pcx, n3), *cy = S.in(pcy, n3), *cz = S.in(pcz, n3);
     double *ta = S.inout(pta, n3);
     int rc = S.status ? S.status
-                      : launch_fct_frozen(1, p2dt, u, v, w, tb, tn, (double *)cx, (double *)cy, (double *)cz, ta);
+                      : launch_fct_frozen(1, p2dt, u, v, w, tb, tn, kn_fct_h, kn_fct_v, (double *)cx, (double *)cy,
+                                          (double *)cz, ta);
     return S.finish(rc);
 }
 
-int nemo_hip_tra_adv_fct_ad(const double *pun, const double *pvn, const double *pwn, const double *pcx,
-                            const double *pcy, const double *pcz, const double *pta_ad, double *ptb_ad,
-                            double *ptn_ad)
+int nemo_hip_tra_adv_fct_ad(const double *pun, const double *pvn, const double *pwn, int kn_fct_h, int kn_fct_v,
+                            const double *pcx, const double *pcy, const double *pcz, const double *pta_ad,
+                            double *ptb_ad, double *ptn_ad)
 {
     NEED_DOMAIN();
+    NEED_FCT_ORDERS(kn_fct_h, kn_fct_v);
     NEED_PTR(pun); NEED_PTR(pvn); NEED_PTR(pwn); NEED_PTR(pcx); NEED_PTR(pcy); NEED_PTR(pcz); NEED_PTR(pta_ad);
     NEED_PTR(ptb_ad); NEED_PTR(ptn_ad);
     const size_t n3 = g_c->dom.n3;
@@ -1798,7 +1809,7 @@ int nemo_hip_tra_adv_fct_ad(const double *pun, const double *pvn, const double *
     const double *u = S.in(pun, n3), *v = S.in(pvn, n3), *w = S.in(pwn, n3);
     const double *cx = S.in(pcx, n3), *cy = S.in(pcy, n3), *cz = S.in(pcz, n3), *mu = S.in(pta_ad, n3);
     double *gb = S.inout(ptb_ad, n3), *gn = S.inout(ptn_ad, n3);
-    int rc = S.status ? S.status : launch_fct_ad(u, v, w, cx, cy, cz, mu, gb, gn);
+    int rc = S.status ? S.status : launch_fct_ad(u, v, w, kn_fct_h, kn_fct_v, cx, cy, cz, mu, gb, gn);
     return S.finish(rc);
 }
 

@@ -58,9 +58,10 @@ struct SeosPar {   // simplified EOS coefficients, TRA/eosbn2.F90:89-95
 template <typename T> int launch_eos(int neos, const SeosPar &e, int pot, const T *pts, T *prd, T *prhop, const T *pdep);
 template <typename T> int launch_rab(int neos, const SeosPar &e, const T *pts, T *pab, const T *gdept);
 int launch_fct_frozen(int mode, double p2dt, const double *pun, const double *pvn, const double *pwn,
-                      const double *tb, const double *tn, double *cx, double *cy, double *cz, double *pta);
-int launch_fct_ad(const double *pun, const double *pvn, const double *pwn, const double *cx, const double *cy,
-                  const double *cz, const double *ta_ad, double *tb_ad, double *tn_ad);
+                      const double *tb, const double *tn, int kn_h, int kn_v, double *cx, double *cy, double *cz,
+                      double *pta);
+int launch_fct_ad(const double *pun, const double *pvn, const double *pwn, int kn_h, int kn_v, const double *cx,
+                  const double *cy, const double *cz, const double *ta_ad, double *tb_ad, double *tn_ad);
 int launch_ldf_lap_ad(const double *pahu, const double *pahv, const double *ta_ad, double *tb_ad);
 int launch_zdf_imp_ad(double p2dt, const double *avt, double *ta_ad, double *tb_ad);
 int launch_tra_nxt_ad(double *tb_ad, double *tn_ad, double *ta_ad);

@@ -681,7 +681,7 @@ int launch_tra_adv_fct(double p2dt, const T *pun, const T *pvn, const T *pwn, co
 template int launch_tra_adv_fct<double>(double, const double *, const double *, const double *, const double *,
                                         const double *, double *, int, int, int);
 // ---------------------------------------------------------------------------
-// the FCT 2/2 limiter frozen (the tangent linear of the tracer step): the
+// the FCT limiter frozen (the tangent linear of the tracer step): the
 // factors nonosc multiplies the antidiffusive fluxes by (traadv_fct.F90:
 // 404-426), and the FCT trend with given factors.  One tracer, FP64.
 // ---------------------------------------------------------------------------
@@ -743,7 +743,8 @@ KERNEL void k_fct_final_frozen(DevDomain d, const double *__restrict__ paa, cons
 // mode 0: the factors of the state (tb, tn) into cx, cy, cz; mode 1: pta +=
 // the FCT trend with the factors cx, cy, cz
 int launch_fct_frozen(int mode, double p2dt, const double *pun, const double *pvn, const double *pwn,
-                      const double *tb, const double *tn, double *cx, double *cy, double *cz, double *pta)
+                      const double *tb, const double *tn, int kn_h, int kn_v, double *cx, double *cy, double *cz,
+                      double *pta)
 {
     const DevDomain &d = nh::dom();
     double *zwi = (double *)nh::scratch3(nh::S_ZWI), *zwx = (double *)nh::scratch3(nh::S_ZWX);
@@ -757,7 +758,7 @@ int launch_fct_frozen(int mode, double p2dt, const double *pun, const double *pv
     double *ta = mode == 0 ? dummy : pta;
     LAUNCH("tra_adv_fct", (k_fct_low<double>), gi, d, p2dt, pun, pvn, pwn, tb, ta, zwi);
     if ((st = launch_lbc<double>(zwi, d.jpk, 'T', 0.0))) return st;
-    LAUNCH("tra_adv_fct", (k_fct_ad<double>), gh, d, 2, 2, pun, pvn, pwn, tb, tn, zwx, zwy, zwz, zwt);
+    LAUNCH("tra_adv_fct", (k_fct_ad<double>), gh, d, kn_h, kn_v, pun, pvn, pwn, tb, tn, zwx, zwy, zwz, zwt);
     if ((st = launch_lbc<double>(zwx, d.jpk, 'U', 0.0))) return st;
     if ((st = launch_lbc<double>(zwy, d.jpk, 'V', 0.0))) return st;
     if ((st = launch_lbc<double>(zwz, d.jpk, 'W', 0.0))) return st;

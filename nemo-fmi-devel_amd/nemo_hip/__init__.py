@@ -386,20 +386,20 @@ class NemoHip:
         self._check(fn(p(pts), p(pab), p(pn2), p(gdept), p(gdepw)))
 
     # tangent linear and adjoint of the tracer step (one tracer, FP64) --------
-    def tra_adv_fct_coef(self, p2dt, pun, pvn, pwn, ptb, ptn, cx, cy, cz):
+    def tra_adv_fct_coef(self, p2dt, pun, pvn, pwn, ptb, ptn, cx, cy, cz, kn=(2, 2)):
         p = self._p
-        self._check(self.L.nemo_hip_tra_adv_fct_coef(C.c_double(p2dt), p(pun), p(pvn), p(pwn), p(ptb), p(ptn), p(cx),
-                                                     p(cy), p(cz)))
+        self._check(self.L.nemo_hip_tra_adv_fct_coef(C.c_double(p2dt), p(pun), p(pvn), p(pwn), p(ptb), p(ptn),
+                                                     C.c_int(kn[0]), C.c_int(kn[1]), p(cx), p(cy), p(cz)))
 
-    def tra_adv_fct_tl(self, p2dt, pun, pvn, pwn, ptb, ptn, cx, cy, cz, pta):
+    def tra_adv_fct_tl(self, p2dt, pun, pvn, pwn, ptb, ptn, cx, cy, cz, pta, kn=(2, 2)):
         p = self._p
-        self._check(self.L.nemo_hip_tra_adv_fct_tl(C.c_double(p2dt), p(pun), p(pvn), p(pwn), p(ptb), p(ptn), p(cx),
-                                                   p(cy), p(cz), p(pta)))
+        self._check(self.L.nemo_hip_tra_adv_fct_tl(C.c_double(p2dt), p(pun), p(pvn), p(pwn), p(ptb), p(ptn),
+                                                   C.c_int(kn[0]), C.c_int(kn[1]), p(cx), p(cy), p(cz), p(pta)))
 
-    def tra_adv_fct_ad(self, pun, pvn, pwn, cx, cy, cz, ta_ad, tb_ad, tn_ad):
+    def tra_adv_fct_ad(self, pun, pvn, pwn, cx, cy, cz, ta_ad, tb_ad, tn_ad, kn=(2, 2)):
         p = self._p
-        self._check(self.L.nemo_hip_tra_adv_fct_ad(p(pun), p(pvn), p(pwn), p(cx), p(cy), p(cz), p(ta_ad), p(tb_ad),
-                                                   p(tn_ad)))
+        self._check(self.L.nemo_hip_tra_adv_fct_ad(p(pun), p(pvn), p(pwn), C.c_int(kn[0]), C.c_int(kn[1]), p(cx),
+                                                   p(cy), p(cz), p(ta_ad), p(tb_ad), p(tn_ad)))
 
     def tra_ldf_lap_ad(self, pahu, pahv, ta_ad, tb_ad):
         p = self._p

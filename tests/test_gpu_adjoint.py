@@ -1,6 +1,6 @@
 """The tangent linear and adjoint of the tracer step on the GPU (one tracer,
-FP64; FCT 2/2 with frozen limiter factors, tra_ldf_lap, tra_zdf_imp,
-leapfrog tra_nxt):
+FP64; FCT of orders 2 and 4 with frozen limiter factors, tra_ldf_lap,
+tra_zdf_imp, leapfrog tra_nxt):
 
 * the limiter factors equal the oracle's bit for bit; the frozen FCT with
   the state's own factors equals the library's tra_adv_fct bit for bit;
@@ -15,7 +15,7 @@ import pytest
 
 from nemo_hip import configs
 from test_gpu_parity import CASE_IDS, Env, check, dev, host, sync
-from test_oracle_adjoint import KT, coef, interior
+from test_oracle_adjoint import KN, KT, coef, interior
 
 torch = pytest.importorskip("torch")
 
@@ -33,20 +33,20 @@ def prep(e):
 class GpuStep:
     """the frozen-limiter tracer step and its adjoint through the library"""
 
-    def __init__(self, hip, e, G):
-        self.h, self.e, self.G = hip, e, G
+    def __init__(self, hip, e, G, kn=(2, 2)):
+        self.h, self.e, self.G, self.kn = hip, e, G, kn
         self.n3 = int(np.prod(e.s3))
 
     def coef(self, tb, tn):
         c = [torch.ones(self.n3, dtype=torch.float64, device="cuda") for _ in range(3)]
         G = self.G
-        self.h.tra_adv_fct_coef(G["p2dt"], G["zun"], G["zvn"], G["zwn"], tb, tn, *c)
+        self.h.tra_adv_fct_coef(G["p2dt"], G["zun"], G["zvn"], G["zwn"], tb, tn, *c, kn=self.kn)
         return c
 
     def tl(self, c, tb, tn):
         h, G, n3 = self.h, self.G, self.n3
         ta = torch.zeros(n3, dtype=torch.float64, device="cuda")
-        h.tra_adv_fct_tl(G["p2dt"], G["zun"], G["zvn"], G["zwn"], tb, tn, *c, ta)
+        h.tra_adv_fct_tl(G["p2dt"], G["zun"], G["zvn"], G["zwn"], tb, tn, *c, ta, kn=self.kn)
         h.tra_ldf_lap(KT, 1, "TRA", G["ahtu"], G["ahtv"], tb, ta, 1, 1)
         h.tra_zdf_imp(KT, 1, "TRA", G["p2dt"], tb, ta, 1, G["avt"], G["avt"])
         B, N, A = (torch.zeros(2 * n3, dtype=torch.float64, device="cuda") for _ in range(3))
@@ -61,7 +61,7 @@ class GpuStep:
         h.tra_nxt_ad(lb, ln, la)
         h.tra_zdf_imp_ad(G["p2dt"], G["avt"], la, lb)
         h.tra_ldf_lap_ad(G["ahtu"], G["ahtv"], la, lb)
-        h.tra_adv_fct_ad(G["zun"], G["zvn"], G["zwn"], *c, la, lb, ln)
+        h.tra_adv_fct_ad(G["zun"], G["zvn"], G["zwn"], *c, la, lb, ln, kn=self.kn)
         return lb, ln
 
 
@@ -72,28 +72,29 @@ def test_fct_coef_and_tl(hip, case):
         e = Env(hip, case)
         F, G = prep(e)
         tb, tn = np.asfortranarray(e.S["tsb"][..., 0]), np.asfortranarray(e.S["tsn"][..., 0])
-        c_o = coef(e.o, e.d, F, tb, tn)
-        st = GpuStep(hip, e, G)
-        c_g = st.coef(dev(tb), dev(tn))
-        sync(hip)
-        for a, b, n in zip(c_g, c_o, "xyz"):
-            assert np.array_equal(host(a, e.s3), b), f"c{n} {case}"
-        # with the state's own factors: tra_adv_fct itself
-        ta1 = dev(np.zeros(e.s3))
-        hip.tra_adv_fct_tl(G["p2dt"], G["zun"], G["zvn"], G["zwn"], dev(tb), dev(tn), *c_g, ta1)
-        ta2 = dev(np.zeros(e.s3))
-        hip.tra_adv_fct(KT, 1, "TRA", G["p2dt"], G["zun"], G["zvn"], G["zwn"], dev(tb), dev(tn), ta2, 1, 2, 2)
-        sync(hip)
-        assert torch.equal(ta1, ta2), f"frozen FCT with own factors {case}"
-        # perturbations against the oracle
         rng = np.random.default_rng(61)
         xb, xn = (np.asfortranarray(rng.standard_normal(e.s3)) for _ in range(2))
-        want = np.zeros(e.s3, order="F")
-        e.o.fct_tl(F["p2dt"], F["zun"], F["zvn"], F["zwn"], xb, xn, *c_o, want)
-        got = dev(np.zeros(e.s3))
-        hip.tra_adv_fct_tl(G["p2dt"], G["zun"], G["zvn"], G["zwn"], dev(xb), dev(xn), *c_g, got)
-        sync(hip)
-        check(host(got, e.s3), want, what=f"fct_tl {case}")
+        for kn in KN:
+            c_o = coef(e.o, e.d, F, tb, tn, kn=kn)
+            st = GpuStep(hip, e, G, kn)
+            c_g = st.coef(dev(tb), dev(tn))
+            sync(hip)
+            for a, b, n in zip(c_g, c_o, "xyz"):
+                assert np.array_equal(host(a, e.s3), b), f"c{n} {case} {kn}"
+            # with the state's own factors: tra_adv_fct itself
+            ta1 = dev(np.zeros(e.s3))
+            hip.tra_adv_fct_tl(G["p2dt"], G["zun"], G["zvn"], G["zwn"], dev(tb), dev(tn), *c_g, ta1, kn=kn)
+            ta2 = dev(np.zeros(e.s3))
+            hip.tra_adv_fct(KT, 1, "TRA", G["p2dt"], G["zun"], G["zvn"], G["zwn"], dev(tb), dev(tn), ta2, 1, *kn)
+            sync(hip)
+            assert torch.equal(ta1, ta2), f"frozen FCT with own factors {case} {kn}"
+            # perturbations against the oracle
+            want = np.zeros(e.s3, order="F")
+            e.o.fct_tl(F["p2dt"], F["zun"], F["zvn"], F["zwn"], xb, xn, *c_o, want, kn=kn)
+            got = dev(np.zeros(e.s3))
+            hip.tra_adv_fct_tl(G["p2dt"], G["zun"], G["zvn"], G["zwn"], dev(xb), dev(xn), *c_g, got, kn=kn)
+            sync(hip)
+            check(host(got, e.s3), want, what=f"fct_tl {case} {kn}")
 
 
 @pytest.mark.gpu
@@ -108,14 +109,15 @@ def test_adjoint_routines(hip, case):
         rng = np.random.default_rng(62)
         r = lambda: np.asfortranarray(rng.standard_normal(e.s3))
         mu, b0, n0 = interior(e.d, r()), r(), r()
-        # FCT
-        wb, wn = b0.copy(order="F"), n0.copy(order="F")
-        e.o.fct_ad(F["zun"], F["zvn"], F["zwn"], *c_o, mu, wb, wn)
-        gb, gn = dev(b0), dev(n0)
-        hip.tra_adv_fct_ad(G["zun"], G["zvn"], G["zwn"], *c_g, dev(mu), gb, gn)
-        sync(hip)
-        check(host(gb, e.s3), wb, what=f"fct_ad tb {case}")
-        check(host(gn, e.s3), wn, what=f"fct_ad tn {case}")
+        # FCT, every order
+        for kn in KN:
+            wb, wn = b0.copy(order="F"), n0.copy(order="F")
+            e.o.fct_ad(F["zun"], F["zvn"], F["zwn"], *c_o, mu, wb, wn, kn=kn)
+            gb, gn = dev(b0), dev(n0)
+            hip.tra_adv_fct_ad(G["zun"], G["zvn"], G["zwn"], *c_g, dev(mu), gb, gn, kn=kn)
+            sync(hip)
+            check(host(gb, e.s3), wb, what=f"fct_ad tb {case} {kn}")
+            check(host(gn, e.s3), wn, what=f"fct_ad tn {case} {kn}")
         # laplacian
         wb = b0.copy(order="F")
         e.o.ldf_lap_ad(F["ahtu"], F["ahtv"], mu, wb)
@@ -151,12 +153,13 @@ def test_adjoint_routines(hip, case):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("kn", [(2, 2), (4, 4)], ids=["fct22", "fct44"])
 @pytest.mark.parametrize("case", ["basin", "cyc7"])
-def test_step_dot_product_and_gradient(hip, case):
+def test_step_dot_product_and_gradient(hip, case, kn):
     with torch.cuda.stream(torch.cuda.ExternalStream(hip.stream())):   # torch work in order with the library's
         e = Env(hip, case)
         F, G = prep(e)
-        st = GpuStep(hip, e, G)
+        st = GpuStep(hip, e, G, kn)
         rng = np.random.default_rng(63)
         T0 = np.asfortranarray(e.S["tsn"][..., 0])
         e.o.lbc_lnk(T0, "T")
