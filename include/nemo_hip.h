@@ -435,6 +435,21 @@ int nemo_hip_hpg_zco(int kt, const double *rhd, double *ua, double *va);
 int nemo_hip_dyn_ldf_lap(int kt, const double *pub, const double *pvb, double *pua, double *pva,
                          int kpass, const double *ahmt, const double *ahmf);
 
+/* tra_qsr, TRA/traqsr.F90:80-304, two bands (ln_qsr_2bd = T): the
+   penetrative solar heat content qsr_hc (3D, FP64) of levels 1..nksr,
+   added to the temperature trend tsa with its value of the step before,
+   qsr_hc_b, and fraqsr_1lev (2D) = the fraction absorbed in level 1.
+   nksr is trc_oce_ext_lev(rn_si1, 100) of trc_oce.F90, computed once by
+   the caller's tra_qsr_init (traqsr.F90:411).  At kt = nit000, kqsr_b = 1
+   says qsr_hc_b holds the restart's field (leapfrog); kqsr_b = 0 clears it
+   (Euler).  With tra_qsr the caller keeps qsr out of tra_sbc's qns.  The
+   exponentials are the device's exp: FP64 results agree with a host libm
+   to a few units in the last place, not bit for bit. */
+int nemo_hip_tra_qsr(int kt, int kqsr_b, int nksr, double rn_abs, double rn_si0, double rn_si1, const double *qsr,
+                     const double *gdepw, double *qsr_hc, double *qsr_hc_b, double *tsa, double *fraqsr_1lev);
+int nemo_hip_tra_qsr_sp(int kt, int kqsr_b, int nksr, double rn_abs, double rn_si0, double rn_si1, const double *qsr,
+                        const float *gdepw, double *qsr_hc, double *qsr_hc_b, float *tsa, double *fraqsr_1lev);
+
 /* tra_sbc, TRA/trasbc.F90:62-238, linear free surface, no penetrative solar
    flux (ln_traqsr = F), no ice shelf, runoff or assimilation terms: qns +=
    qsr and qsr = 0 (lines 108-111); the swap of the surface content trends

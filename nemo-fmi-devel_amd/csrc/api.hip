@@ -1336,6 +1336,29 @@ static int do_tra_sbc(int kt, double *qns, double *qsr, const double *sfx, const
     return S.finish(rc);
 }
 
+// tra_qsr (two bands)
+template <typename T>
+static int do_tra_qsr(int kt, int kqsr_b, int nksr, double rn_abs, double rn_si0, double rn_si1, const double *qsr,
+                      const T *gdepw, double *qsr_hc, double *qsr_hc_b, T *tsa, double *fraqsr_1lev)
+{
+    NEED_DOMAIN();
+    NEED_STEADY(kt);
+    NEED_PTR(qsr); NEED_PTR(gdepw); NEED_PTR(qsr_hc); NEED_PTR(qsr_hc_b); NEED_PTR(tsa); NEED_PTR(fraqsr_1lev);
+    if (nksr < 1 || nksr > g_c->dom.jpk - 1)
+        return set_error(NEMO_HIP_ERR_ARG, "tra_qsr: nksr = %d outside 1..jpk-1", nksr);
+    if (!(rn_si0 > 0.0) || !(rn_si1 > 0.0))
+        return set_error(NEMO_HIP_ERR_ARG, "tra_qsr: extinction depths must be positive");
+    const size_t n2 = g_c->dom.n2, n3 = g_c->dom.n3;
+    const int mode = kt != g_c->hdom.nit000 ? 0 : (kqsr_b ? 2 : 1);
+    Stager S;
+    const double *q = S.in(qsr, n2);
+    const T *gw = S.in(gdepw, n3);
+    double *hc = S.inout(qsr_hc, n3), *hb = S.inout(qsr_hc_b, n3), *fr = S.inout(fraqsr_1lev, n2);
+    T *ta = S.inout(tsa, 2 * n3);
+    int rc = S.status ? S.status : launch_tra_qsr<T>(mode, nksr, rn_abs, rn_si0, rn_si1, q, gw, hc, hb, ta, fr);
+    return S.finish(rc);
+}
+
 // stp_ctl: maxima on the device, the tests and the message on the host
 template <typename T>
 static int do_stp_ctl(int kt, const double *sshn, const T *un, const T *tsn, double zmax[6], int32_t loc[11],
@@ -1909,6 +1932,13 @@ int nemo_hip_stp_ctl(int kt, const double *sshn, const double *un, const double 
     return do_stp_ctl<double>(kt, sshn, un, tsn, zmax, loc, kindic);
 }
 
+int nemo_hip_tra_qsr(int kt, int kqsr_b, int nksr, double rn_abs, double rn_si0, double rn_si1, const double *qsr,
+                     const double *gdepw, double *qsr_hc, double *qsr_hc_b, double *tsa, double *fraqsr_1lev)
+{
+    return do_tra_qsr<double>(kt, kqsr_b, nksr, rn_abs, rn_si0, rn_si1, qsr, gdepw, qsr_hc, qsr_hc_b, tsa,
+                              fraqsr_1lev);
+}
+
 int nemo_hip_tra_sbc(int kt, double *qns, double *qsr, const double *sfx, const double *emp, const double *tsn,
                      double *tsa, double *sbc_tsc, double *sbc_tsc_b)
 {
@@ -2042,6 +2072,13 @@ int nemo_hip_stp_ctl_sp(int kt, const double *sshn, const float *un, const float
                         int32_t loc[11], int *kindic)
 {
     return do_stp_ctl<float>(kt, sshn, un, tsn, zmax, loc, kindic);
+}
+
+int nemo_hip_tra_qsr_sp(int kt, int kqsr_b, int nksr, double rn_abs, double rn_si0, double rn_si1, const double *qsr,
+                        const float *gdepw, double *qsr_hc, double *qsr_hc_b, float *tsa, double *fraqsr_1lev)
+{
+    return do_tra_qsr<float>(kt, kqsr_b, nksr, rn_abs, rn_si0, rn_si1, qsr, gdepw, qsr_hc, qsr_hc_b, tsa,
+                             fraqsr_1lev);
 }
 
 int nemo_hip_tra_sbc_sp(int kt, double *qns, double *qsr, const double *sfx, const double *emp, const float *tsn,

@@ -82,5 +82,8 @@ int launch_dyn_ldf_lap(double zsign, const T *pub, const T *pvb, T *pua, T *pva,
 template <typename T> int launch_stp_ctl(const double *sshn, const T *un, const T *tsn, double *buf, int want_loc);
 size_t stp_ctl_buffer_bytes();
 template <typename T>
+int launch_tra_qsr(int mode, int nksr, double rn_abs, double rn_si0, double rn_si1, const double *qsr,
+                   const T *gdepw, double *qsr_hc, double *qsr_hc_b, T *tsa, double *fraqsr_1lev);
+template <typename T>
 int launch_tra_sbc(int first, double *qns, double *qsr, const double *sfx, const double *emp, const T *tsn, T *tsa,
                    double *sbc, double *sbc_b);

@@ -555,6 +555,56 @@ KERNEL void k_tra_sbc(DevDomain d, int first, double *__restrict__ qns, double *
     sbc[n2 + ij] = cs;
 }
 
+// ---------------------------------------------------------------------------
+// tra_qsr, two bands (ln_qsr_2bd), TRA/traqsr.F90:134-276: one thread per
+// (i,j) column.  Every column swaps qsr_hc into qsr_hc_b (or clears it at an
+// Euler start); the interior ones form qsr_hc on levels 1..nksr, add the
+// trend to tsa and set fraqsr_1lev.  The attenuation of w-level k+1 for
+// level k is the attenuation of w-level k for level k+1 (the same
+// expression of the same depth), so one pair of exponentials per level.
+// mode 0: swap, 1: Euler start (z1_2 = 1), 2: qsr_hc_b given (a restart).
+// ---------------------------------------------------------------------------
+template <typename T>
+KERNEL void k_tra_qsr(DevDomain d, int mode, int nksr, double zz0, double zz1, double xsi0r, double xsi1r,
+                      const double *__restrict__ qsr, const T *__restrict__ gdepw, double *__restrict__ hc,
+                      double *__restrict__ hc_b, T *__restrict__ tsa, double *__restrict__ fraqsr)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    if (mode != 2)
+        for (int k = 1; k <= d.jpk; k++) A3(hc_b, i, j, k) = mode == 0 ? A3(hc, i, j, k) : 0.0;
+    if (i < 2 || i > d.jpi - 1 || j < 2 || j > d.jpj - 1) return;
+    const uint32_t ij = IJ32(i, j);
+    const double rcp = 3991.86795711963, r1_rau0_rcp = 1.0 / (d.rau0 * rcp);
+    const double z1_2 = mode == 1 ? 1.0 : 0.5;
+    const double q = qsr[ij];
+    double zg = (double)A3(gdepw, i, j, 1);
+    double zc0 = zz0 * exp(-zg * xsi0r) + zz1 * exp(-zg * xsi1r);
+    for (int k = 1; k <= nksr; k++) {
+        zg = (double)A3(gdepw, i, j, k + 1);
+        const double zc1 = zz0 * exp(-zg * xsi0r) + zz1 * exp(-zg * xsi1r);
+        const double h = q * (zc0 * WMASK(double, i, j, k) - zc1 * WMASK(double, i, j, k + 1));
+        const uint32_t p = IJK32(i, j, k);
+        at32(hc, p) = h;
+        at32(tsa, p) = (T)((double)at32(tsa, p) + z1_2 * (at32(hc_b, p) + h) / E3(d.md.e3t, ij, k));
+        if (k == 1) fraqsr[ij] = (q != 0.0) ? h / (r1_rau0_rcp * q) : 1.0;
+        zc0 = zc1;
+    }
+}
+
+template <typename T>
+int launch_tra_qsr(int mode, int nksr, double rn_abs, double rn_si0, double rn_si1, const double *qsr,
+                   const T *gdepw, double *qsr_hc, double *qsr_hc_b, T *tsa, double *fraqsr_1lev)
+{
+    const DevDomain &d = nh::dom();
+    const double rcp = 3991.86795711963, r1_rau0_rcp = 1.0 / (d.rau0 * rcp);
+    const double zz0 = rn_abs * r1_rau0_rcp, zz1 = (1. - rn_abs) * r1_rau0_rcp;
+    Grid2 g = grid2(d.jpi, d.jpj);
+    LAUNCH("tra_qsr", k_tra_qsr<T>, g, d, mode, nksr, zz0, zz1, 1.0 / rn_si0, 1.0 / rn_si1, qsr, gdepw, qsr_hc,
+           qsr_hc_b, tsa, fraqsr_1lev);
+    return launch_lbc<double>(fraqsr_1lev, 1, 'T', 0.0);
+}
+
 template <typename T>
 int launch_tra_sbc(int first, double *qns, double *qsr, const double *sfx, const double *emp, const T *tsn, T *tsa,
                    double *sbc, double *sbc_b)
@@ -618,6 +668,8 @@ int launch_istate_bt(const T *un, const T *vn, const T *ub, const T *vb, const d
                                    const double *, const double *, double *, double *, double *, double *,      \
                                    const double *, const double *);                                              \
     template int launch_stp_ctl<T>(const double *, const T *, const T *, double *, int);                        \
+    template int launch_tra_qsr<T>(int, int, double, double, double, const double *, const T *, double *,      \
+                                   double *, T *, double *);                                                \
     template int launch_tra_sbc<T>(int, double *, double *, const double *, const double *, const T *, T *,       \
                                    double *, double *);
 INST(double)

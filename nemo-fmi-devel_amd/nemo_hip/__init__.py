@@ -335,6 +335,14 @@ class NemoHip:
                        p(akz), p(avt)))
 
 
+    def tra_qsr(self, kt, nksr, qsr, gdepw, qsr_hc, qsr_hc_b, tsa, fraqsr_1lev, kqsr_b=0, rn_abs=0.58, rn_si0=0.35,
+                rn_si1=23.0, sp=False):
+        """tra_qsr, two bands (namelist_ref's rn_abs, rn_si0, rn_si1 by default)"""
+        fn = self.L.nemo_hip_tra_qsr_sp if sp else self.L.nemo_hip_tra_qsr
+        p = self._p
+        self._check(fn(C.c_int(kt), C.c_int(kqsr_b), C.c_int(nksr), C.c_double(rn_abs), C.c_double(rn_si0),
+                       C.c_double(rn_si1), p(qsr), p(gdepw), p(qsr_hc), p(qsr_hc_b), p(tsa), p(fraqsr_1lev)))
+
     def tra_sbc(self, kt, qns, qsr, sfx, emp, tsn, tsa, sbc_tsc, sbc_tsc_b, sp=False):
         fn = self.L.nemo_hip_tra_sbc_sp if sp else self.L.nemo_hip_tra_sbc
         dt = np.float32 if sp else np.float64

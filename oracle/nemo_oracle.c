@@ -2276,6 +2276,75 @@ int ora_stp_ctl(int kt, const double *sshn, const double *un, const double *tsn,
 }
 
 /* tra_sbc, TRA/trasbc.F90:108-160 (see the entry point's restrictions) */
+/* trc_oce_ext_lev (trc_oce.F90): the last T-level that light reaches,
+   the shallowest level whose deepest wet w-depth below reaches
+   prldex (ln(10) 15 + ln(pqsr_frc)), or the first level with no water */
+int ora_qsr_nksr(double prldex, double pqsr_frc, const double *gdepw0)
+{
+    int i, j, k, pjl = jpkm1;
+    const double zhext = prldex * (log(10.0) * 15.0 + log(pqsr_frc));
+    for (k = jpkm1; k >= 1; k--) {
+        double s = 0.0, zem = 0.0;
+        int first = 1;
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                const double v = A3(gdepw0, i, j, k + 1) * TMASK(i, j, k);
+                s += TMASK(i, j, k);
+                if (first || v > zem) zem = v;
+                first = 0;
+            }
+        if (s > 0.0) {
+            if (zem >= zhext) pjl = k;
+        } else {
+            pjl = k;
+        }
+    }
+    return pjl;
+}
+
+/* tra_qsr, two bands (TRA/traqsr.F90:134-276); kqsr_b: qsr_hc_b holds the
+   restart's field at nit000 */
+int ora_tra_qsr(int kt, int kqsr_b, int nksr, double rn_abs, double rn_si0, double rn_si1, const double *qsr,
+                const double *gdepw, double *qsr_hc, double *qsr_hc_b, double *tsa, double *fraqsr_1lev)
+{
+    int i, j, k;
+    const double rcp = 3991.86795711963, r1_rau0_rcp = 1.0 / (D.rau0 * rcp);
+    const double xsi0r = 1.0 / rn_si0, xsi1r = 1.0 / rn_si1;
+    const size_t n3 = (size_t)jpi * jpj * jpk;
+    double z1_2;
+    if (kt == D.nit000) {
+        if (kqsr_b) {
+            z1_2 = 0.5;
+        } else {
+            z1_2 = 1.0;
+            for (size_t q = 0; q < n3; q++) qsr_hc_b[q] = 0.0;
+        }
+    } else {
+        z1_2 = 0.5;
+        for (size_t q = 0; q < n3; q++) qsr_hc_b[q] = qsr_hc[q];
+    }
+    const double zz0 = rn_abs * r1_rau0_rcp, zz1 = (1. - rn_abs) * r1_rau0_rcp;
+    for (k = 1; k <= nksr; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++) {
+                const double zc0 = zz0 * exp(-A3(gdepw, i, j, k) * xsi0r) + zz1 * exp(-A3(gdepw, i, j, k) * xsi1r);
+                const double zc1 =
+                    zz0 * exp(-A3(gdepw, i, j, k + 1) * xsi0r) + zz1 * exp(-A3(gdepw, i, j, k + 1) * xsi1r);
+                A3(qsr_hc, i, j, k) = A2(qsr, i, j) * (zc0 * WMASK(i, j, k) - zc1 * WMASK(i, j, k + 1));
+            }
+    for (k = 1; k <= nksr; k++)
+        for (j = 2; j <= jpjm1; j++)
+            for (i = 2; i <= jpim1; i++)
+                A4(tsa, i, j, k, 1) =
+                    A4(tsa, i, j, k, 1) + z1_2 * (A3(qsr_hc_b, i, j, k) + A3(qsr_hc, i, j, k)) / e3t_n(i, j, k);
+    for (j = 2; j <= jpjm1; j++)
+        for (i = 2; i <= jpim1; i++)
+            A2(fraqsr_1lev, i, j) =
+                A2(qsr, i, j) != 0.0 ? A3(qsr_hc, i, j, 1) / (r1_rau0_rcp * A2(qsr, i, j)) : 1.0;
+    lbc3(fraqsr_1lev, 1, 'T', 0.0);
+    return 0;
+}
+
 int ora_tra_sbc(int kt, double *qns, double *qsr, const double *sfx, const double *emp, const double *tsn,
                 double *tsa, double *sbc_tsc, double *sbc_tsc_b)
 {

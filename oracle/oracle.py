@@ -229,6 +229,16 @@ class Oracle:
         st = self.L.ora_stp_ctl(C.c_int(kt), _p(sshn), _p(un), _p(tsn), _p(zmax), _p(loc), C.byref(kind))
         return st, zmax, loc, kind.value
 
+    def qsr_nksr(self, gdepw0, prldex=23.0, pqsr_frc=100.0):
+        """trc_oce_ext_lev(prldex, pqsr_frc): nksr of the two-band tra_qsr"""
+        return int(self.L.ora_qsr_nksr(C.c_double(prldex), C.c_double(pqsr_frc), _p(gdepw0)))
+
+    def tra_qsr(self, kt, nksr, qsr, gdepw, qsr_hc, qsr_hc_b, tsa, fraqsr_1lev, kqsr_b=0, rn_abs=0.58, rn_si0=0.35,
+                rn_si1=23.0):
+        _chk(self.L.ora_tra_qsr(C.c_int(kt), C.c_int(kqsr_b), C.c_int(nksr), C.c_double(rn_abs), C.c_double(rn_si0),
+                                C.c_double(rn_si1), _p(qsr), _p(gdepw), _p(qsr_hc), _p(qsr_hc_b), _p(tsa),
+                                _p(fraqsr_1lev)), "tra_qsr")
+
     def tra_sbc(self, kt, qns, qsr, sfx, emp, tsn, tsa, sbc_tsc, sbc_tsc_b):
         _chk(self.L.ora_tra_sbc(C.c_int(kt), _p(qns), _p(qsr), _p(sfx), _p(emp), _p(tsn), _p(tsa), _p(sbc_tsc),
                                 _p(sbc_tsc_b)), "tra_sbc")

@@ -287,3 +287,35 @@ def test_tra_sbc(hip, case, sp):
             check(host(gd[n], e.s2), o[n], what=f"tra_sbc {n}")
         for n in ("sbc", "sbc_b"):
             check(host(gd[n], e.s2 + (2,)), o[n], tol=SP_TOL if sp else 1e-12, what=f"tra_sbc {n}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["basin", "cyc7"])
+@pytest.mark.parametrize("sp", [False, True])
+def test_tra_qsr(hip, case, sp):
+    """tra_qsr (two bands) against the oracle over an Euler first step, two
+    leapfrog steps and a restart start; the device's exp differs from the
+    host's libm by a few units in the last place, so FP64 is held to 1e-12"""
+    e = Env(hip, case)
+    rng = np.random.default_rng(78)
+    gdepw = configs.depths(e.d)[1]
+    if sp:
+        gdepw = np.asfortranarray(gdepw.astype(np.float32).astype(np.float64))
+    nksr = e.o.qsr_nksr(gdepw)
+    qsr = np.asfortranarray(rng.uniform(0.0, 300.0, e.s2))
+    qsr[3, 4] = 0.0
+    hc, hb = np.zeros(e.s3, order="F"), np.zeros(e.s3, order="F")
+    fr = np.zeros(e.s2, order="F")
+    ghc, ghb, gfr, gq, gw = dev(hc), dev(hb), dev(fr), dev(qsr), g(gdepw, sp)
+    tsa = rnd(rng, e.s4, sp)
+    wta, gta = tsa.copy(order="F"), g(tsa, sp)
+    for kt, kb in ((1, 0), (2, 0), (3, 0), (1, 1)):
+        e.o.tra_qsr(kt, nksr, qsr, gdepw, hc, hb, wta, fr, kqsr_b=kb)
+        hip.tra_qsr(kt, nksr, gq, gw, ghc, ghb, gta, gfr, kqsr_b=kb, sp=sp)
+        sync(hip)
+        cmp(host(gta, e.s4), wta, sp, f"tra_qsr {case} kt={kt} tsa")
+        for n, a, b in (("qsr_hc", ghc, hc), ("qsr_hc_b", ghb, hb)):
+            check(host(a, e.s3), b, tol=SP_TOL if sp else 1e-12, what=f"tra_qsr {case} kt={kt} {n}")
+        check(host(gfr, e.s2), fr, tol=SP_TOL if sp else 1e-12, what=f"tra_qsr {case} kt={kt} fraqsr_1lev")
+    with pytest.raises(nemo_hip.NemoHipError, match="nksr"):
+        hip.tra_qsr(2, 0, gq, gw, ghc, ghb, gta, gfr, sp=sp)
