@@ -39,7 +39,7 @@ MODULE nemo_hip_mod
    PUBLIC :: dyn_keg, dyn_zad, dyn_adv_cen2, dyn_vor, dyn_zdf, dyn_spg_ts
    PUBLIC :: nemo_hip_bind_nxt, div_hor, ssh_nxt, wzv, ssh_swp, tra_nxt, dyn_nxt
    PUBLIC :: nemo_hip_eos_t, nemo_hip_eos_init, nemo_hip_bind_phys
-   PUBLIC :: eos_insitu, eos_insitu_pot, rab_3d, bn2, hpg_zco, dyn_ldf_lap, tra_sbc, stp_ctl
+   PUBLIC :: eos_insitu, eos_insitu_pot, rab_3d, bn2, hpg_zco, dyn_ldf_lap, tra_sbc, tra_qsr, stp_ctl
    PUBLIC :: nemo_hip_bind_mxl, zdf_mxl, ldf_slp
    PUBLIC :: nemo_hip_rst_t, rst_write, rst_read
    PUBLIC :: nemo_hip_domain_t, nemo_hip_spg_ts_t
@@ -297,6 +297,13 @@ MODULE nemo_hip_mod
          IMPORT :: C_INT, C_PTR
          INTEGER(C_INT), VALUE :: kt, kpass
          TYPE(C_PTR), VALUE :: pub, pvb, pua, pva, ahmt, ahmf
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_tra_qsr(kt, kqsr_b, nksr, rn_abs, rn_si0, rn_si1, qsr, gdepw, qsr_hc, qsr_hc_b, tsa,  &
+         &                              fraqsr_1lev) BIND(C, name="nemo_hip_tra_qsr")
+         IMPORT :: C_INT, C_PTR, C_DOUBLE
+         INTEGER(C_INT), VALUE :: kt, kqsr_b, nksr
+         REAL(C_DOUBLE), VALUE :: rn_abs, rn_si0, rn_si1
+         TYPE(C_PTR), VALUE :: qsr, gdepw, qsr_hc, qsr_hc_b, tsa, fraqsr_1lev
       END FUNCTION
       INTEGER(C_INT) FUNCTION c_tra_sbc(kt, qns, qsr, sfx, emp, tsn, tsa, sbc_tsc, sbc_tsc_b) BIND(C, name="nemo_hip_tra_sbc")
          IMPORT :: C_INT, C_PTR
@@ -676,6 +683,17 @@ CONTAINS
       CALL check( c_tra_sbc( INT(kt,C_INT), p_qns, p_qsr, p_sfx, p_emp, p_tsn, p_tsa, p_sbc_tsc, p_sbc_tsc_b ),   &
          &        'tra_sbc' )
    END SUBROUTINE tra_sbc
+
+   SUBROUTINE tra_qsr( kt, kqsr_b, knksr, pabs, psi0, psi1, pqsr_hc, pqsr_hc_b, pfraqsr_1lev )
+      !! TRA/traqsr.F90:80, two bands: the traqsr module's arrays and namelist values, nksr of tra_qsr_init
+      INTEGER , INTENT(in) ::   kt, kqsr_b, knksr
+      REAL(wp), INTENT(in) ::   pabs, psi0, psi1
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(inout) ::   pqsr_hc, pqsr_hc_b
+      REAL(wp), DIMENSION(:,:)  , TARGET, CONTIGUOUS, INTENT(inout) ::   pfraqsr_1lev
+      CALL check( c_tra_qsr( INT(kt,C_INT), INT(kqsr_b,C_INT), INT(knksr,C_INT), REAL(pabs,C_DOUBLE),         &
+         &                   REAL(psi0,C_DOUBLE), REAL(psi1,C_DOUBLE), p_qsr, p_gdepw, C_LOC(pqsr_hc),     &
+         &                   C_LOC(pqsr_hc_b), p_tsa, C_LOC(pfraqsr_1lev) ), 'tra_qsr' )
+   END SUBROUTINE tra_qsr
 
    SUBROUTINE stp_ctl( kt, kindic )
       INTEGER, INTENT(in   ) ::   kt
