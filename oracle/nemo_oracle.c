@@ -38,9 +38,18 @@
  * the Asselin filter on linear and quadratic sequences (ssh_swp, tra_nxt,
  * dyn_nxt) and the depth-mean constraints of the time-splitting corrections
  * (dyn_nxt).
- * Parity partially pinned: dyn_vor ENS and MIX near land; the forcing and
- * time-filter terms of dyn_spg_ts (drag, wind, un_bf/ub2_b); nonosc on
- * masked coasts (bounds invariant only).
+ * tests/test_oracle_pins_terms.py pins the terms the laws above leave open:
+ * the rotated cross fluxes of tra_ldf_iso with non-zero slopes (the Redi
+ * cross term -2 A s d2T/dxdz exactly, and the annihilation of a tracer
+ * aligned with the slopes by the explicit part plus akz), the order-4
+ * horizontal FCT flux (exact for the cell means of a cubic), the enstrophy
+ * identity of vor_ens and the ENS/ENE split of MIX, the drag, wind,
+ * fresh-water and un_bf/ub2_b time-filter terms of dyn_spg_ts (a scalar
+ * recurrence in a uniform cyclic channel), nonosc next to land (textbook
+ * Zalesak with walls) and vor_een's e3f averaging at a coast.
+ * tests/test_oracle_slp.py pins ldf_slp's coastal and bottom tapers and the
+ * mixed layer between columns of different depth.  No function is left
+ * unpinned.
  */
 #include <math.h>
 #include <stdio.h>

@@ -13,9 +13,12 @@ iso-neutral slopes (ldf_slp with ldf_slp_mxl) to closed forms:
 * in the mixed layer the u/v slopes taper linearly with depth from the
   slope at its base, (gdept - e3u(1)/2) / hmlpt, and the w slopes as
   gdepw / hmlp (the ldf_slp_mxl construction);
-* the lateral boundary: the ring of a cyclic domain repeats the interior.
-The coastal tapering factors are covered only through the GPU parity tests
-(parity partially pinned there).
+* the lateral boundary: the ring of a cyclic domain repeats the interior;
+* the coastal and bottom tapers: with every unfiltered slope at the limiter,
+  the filtered slope is that constant times the wet share of the filter
+  weights times the taper factors;
+* u-points between columns of different mixed-layer depth take the deeper
+  one (level, taper depth and mask).
 """
 import numpy as np
 import pytest
@@ -229,3 +232,101 @@ def test_ldf_slp_cyclic_ring_and_untouched_levels():
         assert np.all(a[:, 0] == 0.0) and np.all(a[:, -1] == 0.0)      # closed north/south
         assert np.all(a[:, :, 0] == 0.0) and np.all(a[:, :, -1] == 0.0)
         assert np.isfinite(a).all() and np.abs(a).max() <= 0.01 * (1 + 1e-12)
+
+
+def conv121(a):
+    """The 1-2-1 x 1-2-1 sum of a 2D field at interior points (ring left 0)."""
+    out = np.zeros_like(a)
+    w = np.array([1.0, 2.0, 1.0])
+    for di in range(3):
+        for dj in range(3):
+            out[1:-1, 1:-1] += w[di] * w[dj] * a[di:di + a.shape[0] - 2, dj:dj + a.shape[1] - 2]
+    return out
+
+
+def test_ldf_slp_coastal_and_topographic_tapers():
+    """In a basin with coasts and a varying bottom, a density gradient 100x
+    steeper than rn_slpmax makes every unfiltered slope the limited constant
+    s0(k) = min(rn_slpmax, e3(k)/7000) on wet points and 0 on land.  The
+    filtered slope is then s0 times the share of the 1-2-1 x 1-2-1 weights that
+    fall on wet points, times the tapers: (umask(j+1) + umask(j-1))/2 for u
+    across the coast and (umask(k) + umask(k+1))/2 at the bottom; for the w
+    slopes wmask times (umask(i) + umask(i-1)) (vmask(j) + vmask(j-1)) / 4.
+    A w-point none of whose four neighbouring u-points is wet has no gradient
+    to average and stays 0."""
+    d = configs.basin(30, 24, 12, seed=3, land_frac=0.3, e1=E1)
+    o = Oracle(d)
+    n0 = 1e-5
+    alpha = -1.0 * E1 * n0 / GRAV
+    prd, pn2 = uniform_gradient(d, 0, alpha, n0)
+    (us, vs, wi, wj), _, _ = slopes(o, d, prd, pn2, nmln=2, rn_slpmax=0.01)
+    _, um, vm, wm, _ = configs.masks(d)
+    e3u, e3w = d.arrays["e3u_n"][0, 0], d.arrays["e3w_n"][0, 0]
+    coast = 0
+    for k in range(3, d.jpk - 1):                         # 1-based levels below the mixed layer
+        q = k - 1
+        s0u = min(0.01, e3u[q] / 7.0e3)
+        s0w = min(0.01, e3w[q] / 7.0e3)
+        u2 = um[:, :, q]
+        tj = np.zeros_like(u2)
+        tj[:, 1:-1] = 0.5 * (u2[:, 2:] + u2[:, :-2])
+        tk = 0.5 * (u2 + um[:, :, q + 1])
+        want_u = s0u * conv121(u2) / 16.0 * tj * tk
+        want_u[0], want_u[-1], want_u[:, 0], want_u[:, -1] = 0, 0, 0, 0
+        assert np.allclose(us[:, :, q], want_u, rtol=1e-12, atol=1e-18), k
+        # unfiltered w slope: s0w where the w-point is wet and one of its four u-points is
+        anyu = np.zeros_like(u2)
+        anyu[1:, :] = (um[1:, :, q] + um[:-1, :, q] + um[1:, :, q - 1] + um[:-1, :, q - 1]) > 0
+        zw = s0w * wm[:, :, q] * anyu
+        zw[0], zw[-1], zw[:, 0], zw[:, -1] = 0, 0, 0, 0
+        zck = np.zeros_like(u2)
+        zck[1:, 1:] = (u2[1:, 1:] + u2[:-1, 1:]) * (vm[1:, 1:, q] + vm[1:, :-1, q]) * 0.25
+        want_w = conv121(zw) / 16.0 * wm[:, :, q] * zck
+        want_w[0], want_w[-1], want_w[:, 0], want_w[:, -1] = 0, 0, 0, 0
+        assert np.allclose(wi[:, :, q], want_w, rtol=1e-12, atol=1e-18), k
+        assert np.all(vs[:, :, q] == 0.0) and np.all(wj[:, :, q] == 0.0)
+        coast += int(((want_u > 0) & (want_u < 0.999 * s0u)).sum())
+    assert coast > 50                                     # many points feel a coast or the bottom
+
+
+def test_ldf_slp_mixed_layer_between_columns_of_different_depth():
+    """A band of columns (i = I0) whose mixed layer is deeper (nmln = M2) than
+    elsewhere (M1): the u-points on both sides of the band take the deeper
+    mixed layer (ldf_slp_mxl: iku = max(nmln(i), nmln(i+1)), the taper depth
+    max(hmlpt(i), hmlpt(i+1)), the mixed-layer mask max of the two columns),
+    and the 1-2-1 filter along i then mixes them with their shallow
+    neighbours: (S + 2 S + D)/4, (S + 2 D + D)/4, (D + 2 D + S)/4, (D + 2 S + S)/4."""
+    d = box()
+    o = Oracle(d)
+    E, n0, r0, beta, M1, M2, I0 = E1, 1e-5, 2e-3, 1e-4, 3, 7, 14
+    alpha = 1e-4 * E * n0 / GRAV
+    prd, pn2 = uniform_gradient(d, 0, alpha, n0, r0, beta)
+    gdept, gdepw = configs.depths(d)
+    nm = np.full((d.jpi, d.jpj), M1, dtype=np.int32, order="F")
+    nm[I0 - 1, :] = M2                                    # 1-based column I0
+    ii, jj = np.indices((d.jpi, d.jpj))
+    hmlp = np.asfortranarray(gdepw[ii, jj, nm - 1])
+    hmlpt = np.asfortranarray(gdept[ii, jj, nm - 2])
+    s3 = (d.jpi, d.jpj, d.jpk)
+    us, vs, wi, wj = [np.full(s3, 9.0, order="F") for _ in range(4)]
+    for a in (us, vs, wi, wj):
+        a[:, :, 0] = a[:, :, -1] = 0.0
+    o.ldf_slp(1, prd, pn2, gdept, gdepw, nm, hmlp, hmlpt, 0.01, us, vs, wi, wj)
+    e3u1 = d.arrays["e3u_n"][0, 0, 0]
+    gt = gdept[0, 0]
+
+    def zwz(c, k, M):
+        """unfiltered u slope at the u-point of 1-based coordinate c whose
+        mixed layer has its base at level M"""
+        if k <= M - 1:
+            return (gt[k - 1] - 0.5 * e3u1) / gt[M - 2] * slope_u(alpha, E, n0, r0, beta, c, M)
+        return slope_u(alpha, E, n0, r0, beta, c, k)
+
+    for k in range(2, d.jpk - 2):
+        for c in range(I0 - 3, I0 + 3):                   # u-points around the band
+            Ms = [M2 if cc in (I0 - 1, I0) else M1 for cc in (c - 1, c, c + 1)]
+            want = (zwz(c - 1, k, Ms[0]) + 2 * zwz(c, k, Ms[1]) + zwz(c + 1, k, Ms[2])) / 4.0
+            got = us[c - 1, 4:-4, k - 1]
+            assert np.allclose(got, want, rtol=1e-9, atol=0), (k, c)
+    # the deep band's own taper differs from the shallow one inside its mixed layer
+    assert abs(us[I0 - 1, 6, 2] - us[I0 + 3, 6, 2]) > 0.1 * abs(us[I0 + 3, 6, 2])
