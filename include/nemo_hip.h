@@ -100,7 +100,9 @@ typedef struct nemo_hip_domain {
     int32_t ln_bt_fw, ln_bt_av, nn_bt_flt, nn_baro;
     /* time stepping, DOM/dom_oce.F90:62-63 */
     int32_t neuler, nit000;
-    int32_t reserved0;
+    /* penetrative solar flux, namsbc (TRA/trasbc.F90:97-100): 1 keeps qsr
+       for tra_qsr, 0 merges it into qns in tra_sbc                         */
+    int32_t ln_traqsr;
     double  rdt;                /* baroclinic time step [s]                 */
     double  rn_atfp;            /* Asselin filter parameter                 */
     double  rau0;               /* reference density, TRA/eosbn2.F90:1244   */
@@ -167,7 +169,9 @@ typedef struct nemo_hip_eos {
 typedef struct nemo_hip_rst {
     int32_t real_bytes;         /* 8: the 3D fields are double; 4: float
                                    (the _sp storage form)                   */
-    int32_t reserved0;
+    int32_t kqsr_b;             /* out of rst_read: 1 when qsr_hc_b was read
+                                   (pass it to nemo_hip_tra_qsr at nit000,
+                                   TRA/traqsr.F90:136-139), else 0          */
     void   *ub, *vb, *tsb;      /* before fields; tsb (jpi,jpj,jpk,2) holds
                                    the variables tb and sb                  */
     void   *un, *vn, *tsn;      /* now fields (tn, sn)                      */
@@ -178,6 +182,12 @@ typedef struct nemo_hip_rst {
        159-178 derives after a restart, from r1_hu_n / r1_hv_n            */
     double *un_b, *vn_b, *ub_b, *vb_b;
     const double *r1_hu_n, *r1_hv_n;
+    /* optional, the two-band tra_qsr's variables (TRA/traqsr.F90:287-292,
+       424-425): rst_write stores qsr_hc (jpi,jpj,jpk) under the name
+       qsr_hc_b and fraqsr_1lev (jpi,jpj); rst_read reads them into qsr_hc_b
+       and fraqsr_1lev when the file holds them.  NULL skips them.          */
+    const double *qsr_hc;
+    double *qsr_hc_b, *fraqsr_1lev;
 } nemo_hip_rst_t;
 
 /* ------------------------------------------------------------------------ */
@@ -442,7 +452,8 @@ int nemo_hip_dyn_ldf_lap(int kt, const double *pub, const double *pvb, double *p
    nksr is trc_oce_ext_lev(rn_si1, 100) of trc_oce.F90, computed once by
    the caller's tra_qsr_init (traqsr.F90:411).  At kt = nit000, kqsr_b = 1
    says qsr_hc_b holds the restart's field (leapfrog); kqsr_b = 0 clears it
-   (Euler).  With tra_qsr the caller keeps qsr out of tra_sbc's qns.  The
+   (Euler).  A run with tra_qsr registers ln_traqsr = 1, so tra_sbc leaves
+   qsr to it.  The
    exponentials are the device's exp: FP64 results agree with a host libm
    to a few units in the last place, not bit for bit. */
 int nemo_hip_tra_qsr(int kt, int kqsr_b, int nksr, double rn_abs, double rn_si0, double rn_si1, const double *qsr,
@@ -450,9 +461,9 @@ int nemo_hip_tra_qsr(int kt, int kqsr_b, int nksr, double rn_abs, double rn_si0,
 int nemo_hip_tra_qsr_sp(int kt, int kqsr_b, int nksr, double rn_abs, double rn_si0, double rn_si1, const double *qsr,
                         const float *gdepw, double *qsr_hc, double *qsr_hc_b, float *tsa, double *fraqsr_1lev);
 
-/* tra_sbc, TRA/trasbc.F90:62-238, linear free surface, no penetrative solar
-   flux (ln_traqsr = F), no ice shelf, runoff or assimilation terms: qns +=
-   qsr and qsr = 0 (lines 108-111); the swap of the surface content trends
+/* tra_sbc, TRA/trasbc.F90:62-238, linear free surface, no ice shelf, runoff
+   or assimilation terms: with the domain's ln_traqsr = 0, qns += qsr and
+   qsr = 0 (lines 97-100); with ln_traqsr = 1 both are left for tra_qsr; the swap of the surface content trends
    sbc_tsc_b <- sbc_tsc and the new sbc_tsc (jpi,jpj,2) from qns, sfx and the
    concentration/dilution of emp; the trend added to level 1 of tsa.  At
    kt = nit000 the run starts from rest of the forcing (no restart): both

@@ -330,6 +330,7 @@ static int register_domain(const nemo_hip_domain_t *h)
     d.n_dynadv = h->n_dynadv; d.nn_dynkeg = h->nn_dynkeg; d.nvor_scheme = h->nvor_scheme; d.nn_een_e3f = h->nn_een_e3f;
     d.ln_drgimp = h->ln_drgimp; d.ln_dynspg_ts = h->ln_dynspg_ts; d.ln_bt_fw = h->ln_bt_fw; d.ln_bt_av = h->ln_bt_av;
     d.nn_bt_flt = h->nn_bt_flt; d.nn_baro = h->nn_baro; d.neuler = h->neuler; d.nit000 = h->nit000;
+    d.ln_traqsr = h->ln_traqsr != 0;
     d.rdt = h->rdt; d.rn_atfp = h->rn_atfp; d.rau0 = h->rau0; d.grav = h->grav; d.rn_scal_load = h->rn_scal_load;
     const size_t n2 = d.n2, n3 = d.n3;
     auto I2 = [&](int i, int j) { return (size_t)(j - 1) * jpi + (i - 1); };
@@ -1558,9 +1559,13 @@ struct RstVar {
     int width;      // bytes per element in the caller's array
     bool before;    // a before field (absent from a file: Euler restart)
     bool ts;        // a ts_rst field
+    bool opt;       // optional: written when given, read when given and present
 };
 
-std::vector<RstVar> rst_vars(const nemo_hip_rst_t *r)
+// reading: the destinations of the variables; writing: their sources (the
+// solar heat content is written from qsr_hc under the name qsr_hc_b, the
+// reference's convention, traqsr.F90:291)
+std::vector<RstVar> rst_vars(const nemo_hip_rst_t *r, bool reading = false)
 {
     const size_t n2 = g_c->dom.n2, n3 = g_c->dom.n3;
     const int w = r->real_bytes;
@@ -1572,18 +1577,20 @@ std::vector<RstVar> rst_vars(const nemo_hip_rst_t *r)
             {"sn", tr(r->tsn, 1), n3, w, false, false}, {"sshn", r->sshn, n2, 8, false, false},
             {"rhop", r->rhop, n3, w, false, false},     {"ub2_b", r->ub2_b, n2, 8, false, true},
             {"vb2_b", r->vb2_b, n2, 8, false, true},    {"un_bf", r->un_bf, n2, 8, false, true},
-            {"vn_bf", r->vn_bf, n2, 8, false, true}};
+            {"vn_bf", r->vn_bf, n2, 8, false, true},
+            {"qsr_hc_b", reading ? (void *)r->qsr_hc_b : (void *)r->qsr_hc, n3, 8, false, false, true},
+            {"fraqsr_1lev", r->fraqsr_1lev, n2, 8, false, false, true}};
 }
 
-int rst_check(const nemo_hip_rst_t *r, const char *who)
+int rst_check(const nemo_hip_rst_t *r, const char *who, bool reading)
 {
     if (r->real_bytes != 8 && r->real_bytes != 4)
         return set_error(NEMO_HIP_ERR_ARG, "%s: real_bytes = %d (8 or 4)", who, r->real_bytes);
     if (!g_c->dom.ln_bt_av)
         return set_error(NEMO_HIP_ERR_UNSUPPORTED,
                          "%s: ln_bt_av = F (the barotropic *_e levels of ts_rst) is not supported", who);
-    for (const auto &v : rst_vars(r))
-        if (!v.ptr) return set_error(NEMO_HIP_ERR_ARG, "%s: %s is NULL", who, v.name);
+    for (const auto &v : rst_vars(r, reading))
+        if (!v.ptr && !v.opt) return set_error(NEMO_HIP_ERR_ARG, "%s: %s is NULL", who, v.name);
     return 0;
 }
 
@@ -1618,7 +1625,7 @@ int nemo_hip_rst_write(const char *dir, int kt, const nemo_hip_rst_t *r)
     NEED_DOMAIN();
     NEED_NOT_CAPTURING("nemo_hip_rst_write");
     NEED_PTR(dir); NEED_PTR(r);
-    int rc = rst_check(r, "rst_write");
+    int rc = rst_check(r, "rst_write", false);
     if (rc) return rc;
     if (mkdir(dir, 0755) != 0 && errno != EEXIST)
         return set_error(NEMO_HIP_ERR_IO, "rst_write: cannot create %s: %s", dir, strerror(errno));
@@ -1631,7 +1638,9 @@ int nemo_hip_rst_write(const char *dir, int kt, const nemo_hip_rst_t *r)
              kt, g_c->hdom.rdt, d.jpi, d.jpj, d.jpk, g_c->hdom.jperio);
     std::string man = head;
     std::vector<char> buf;
-    const auto vars = rst_vars(r);
+    std::vector<RstVar> vars;
+    for (const auto &v : rst_vars(r))
+        if (v.ptr) vars.push_back(v);
     for (size_t q = 0; q < vars.size(); q++) {
         const RstVar &v = vars[q];
         const size_t bytes = v.n * (size_t)v.width;
@@ -1673,7 +1682,7 @@ int nemo_hip_rst_read(const char *dir, nemo_hip_rst_t *r, int *kt, int *neuler)
     NEED_DOMAIN();
     NEED_NOT_CAPTURING("nemo_hip_rst_read");
     NEED_PTR(dir); NEED_PTR(r); NEED_PTR(kt); NEED_PTR(neuler);
-    int rc = rst_check(r, "rst_read");
+    int rc = rst_check(r, "rst_read", true);
     if (rc) return rc;
     if (r->un_b && (!r->vn_b || !r->ub_b || !r->vb_b || !r->r1_hu_n || !r->r1_hv_n))
         return set_error(NEMO_HIP_ERR_ARG, "rst_read: un_b given without vn_b, ub_b, vb_b, r1_hu_n and r1_hv_n");
@@ -1705,11 +1714,14 @@ int nemo_hip_rst_read(const char *dir, nemo_hip_rst_t *r, int *kt, int *neuler)
     HIPCHK(hipStreamSynchronize(g_stream));          // no queued work still uses the destinations
     const bool dev = (g_ptr_mode == NEMO_HIP_PTR_DEVICE);
     std::vector<char> raw, conv;
-    for (const auto &v : rst_vars(r)) {
-        if ((v.before && !have_before) || (v.ts && !read_ts)) continue;
+    r->kqsr_b = 0;
+    for (const auto &v : rst_vars(r, true)) {
+        if ((v.before && !have_before) || (v.ts && !read_ts) || !v.ptr) continue;
         const std::string dt = json_dtype(man, v.name);
         if (dt.empty()) {
-            if (!strcmp(v.name, "rhop")) continue;   // optional (restart.F90:288-292)
+            // optional: rhop (restart.F90:288-292), the solar heat content
+            // and fraction (traqsr.F90:136, 424: absent means an Euler start)
+            if (!strcmp(v.name, "rhop") || v.opt) continue;
             return set_error(NEMO_HIP_ERR_IO, "rst_read: %s has no variable %s", dir, v.name);
         }
         const int fw = dt == "f8" ? 8 : dt == "f4" ? 4 : 0;
@@ -1735,6 +1747,7 @@ int nemo_hip_rst_read(const char *dir, nemo_hip_rst_t *r, int *kt, int *neuler)
         const size_t bytes = v.n * (size_t)v.width;
         if (dev) HIPCHK(hipMemcpy(v.ptr, src, bytes, hipMemcpyHostToDevice));
         else memcpy(v.ptr, src, bytes);
+        if (!strcmp(v.name, "qsr_hc_b")) r->kqsr_b = 1;
     }
     const size_t n2 = d.n2, n3 = d.n3, w = (size_t)r->real_bytes;
     auto copy = [&](void *dst, const void *src, size_t bytes) -> int {

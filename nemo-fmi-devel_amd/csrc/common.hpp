@@ -55,7 +55,7 @@ struct DevDomain {
     int ln_traldf_lap, ln_traldf_msc, ln_zdfddm;
     int n_dynadv, nn_dynkeg, nvor_scheme, nn_een_e3f;
     int ln_drgimp, ln_dynspg_ts, ln_bt_fw, ln_bt_av, nn_bt_flt, nn_baro;
-    int neuler, nit000;
+    int neuler, nit000, ln_traqsr;
     double rdt, rn_atfp, rau0, grav, rn_scal_load;
     // masks as level intervals, DOM/dommsk.F90
     KRange kT, kU, kV, kF;

@@ -513,7 +513,7 @@ int launch_stp_ctl(const double *sshn, const T *un, const T *tsn, double *buf, i
 size_t stp_ctl_buffer_bytes() { return ((size_t)STP_NB * 6 + 8) * sizeof(double) + 4 * sizeof(uint32_t); }
 
 // ---------------------------------------------------------------------------
-// tra_sbc, TRA/trasbc.F90:108-160 (linear free surface, ln_traqsr = F): one
+// tra_sbc, TRA/trasbc.F90:97-160 (linear free surface): one
 // thread per (i,j); every point swaps or resets its trends, the points of
 // 2..jpim1 x 2..jpj get the new trend and update level 1 of tsa
 // ---------------------------------------------------------------------------
@@ -527,9 +527,14 @@ KERNEL void k_tra_sbc(DevDomain d, int first, double *__restrict__ qns, double *
     const uint32_t ij = IJ32(i, j);
     const size_t n2 = d.n2, n3 = d.n3;
     const double rcp = 3991.86795711963, r1_rau0 = 1.0 / d.rau0, r1_rau0_rcp = 1.0 / (d.rau0 * rcp);
-    const double q = qns[ij] + qsr[ij];
-    qns[ij] = q;
-    qsr[ij] = 0.0;
+    // lines 97-100: without penetrative solar flux the whole of qsr enters
+    // the surface level through qns; with it, tra_qsr takes qsr
+    double q = qns[ij];
+    if (!d.ln_traqsr) {
+        q = q + qsr[ij];
+        qns[ij] = q;
+        qsr[ij] = 0.0;
+    }
     double bt, bs, ct, cs;
     if (first) {
         bt = bs = ct = cs = 0.0;

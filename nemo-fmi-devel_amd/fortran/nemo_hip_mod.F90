@@ -53,7 +53,7 @@ MODULE nemo_hip_mod
       INTEGER(C_INT32_T) :: n_dynadv, nn_dynkeg, nvor_scheme, nn_een_e3f
       INTEGER(C_INT32_T) :: ln_drgimp
       INTEGER(C_INT32_T) :: ln_dynspg_ts, ln_bt_fw, ln_bt_av, nn_bt_flt, nn_baro
-      INTEGER(C_INT32_T) :: neuler, nit000, reserved0
+      INTEGER(C_INT32_T) :: neuler, nit000, ln_traqsr
       REAL(C_DOUBLE)     :: rdt, rn_atfp, rau0, grav, rn_bt_alpha, rn_scal_load
       TYPE(C_PTR) :: k_top, k_bot
       TYPE(C_PTR) :: e1t, e2t, e1u, e2u, e1v, e2v, e1f, e2f
@@ -79,9 +79,10 @@ MODULE nemo_hip_mod
    END TYPE nemo_hip_spg_ts_t
 
    TYPE, BIND(C) :: nemo_hip_rst_t
-      INTEGER(C_INT32_T) :: real_bytes, reserved0
+      INTEGER(C_INT32_T) :: real_bytes, kqsr_b
       TYPE(C_PTR) :: ub, vb, tsb, un, vn, tsn, rhop, sshb, sshn, ub2_b, vb2_b, un_bf, vn_bf
       TYPE(C_PTR) :: un_b, vn_b, ub_b, vb_b, r1_hu_n, r1_hv_n
+      TYPE(C_PTR) :: qsr_hc, qsr_hc_b, fraqsr_1lev
    END TYPE nemo_hip_rst_t
 
    !! module arrays bound at initialisation
@@ -621,7 +622,8 @@ CONTAINS
       !! the restart variables from the bound oce and dynspg_ts arrays
       TYPE(C_PTR), INTENT(in) :: prhop
       TYPE(nemo_hip_rst_t) :: r
-      r%real_bytes = 8 ; r%reserved0 = 0
+      r%real_bytes = 8 ; r%kqsr_b = 0
+      r%qsr_hc = C_NULL_PTR ; r%qsr_hc_b = C_NULL_PTR ; r%fraqsr_1lev = C_NULL_PTR
       r%ub = p_ub ; r%vb = p_vb ; r%tsb = p_tsb ; r%un = p_un ; r%vn = p_vn ; r%tsn = p_tsn ; r%rhop = prhop
       r%sshb = p_sshb ; r%sshn = p_sshn
       r%ub2_b = spg_state%ub2_b ; r%vb2_b = spg_state%vb2_b ; r%un_bf = spg_state%un_bf ; r%vn_bf = spg_state%vn_bf
@@ -629,26 +631,41 @@ CONTAINS
       r%r1_hu_n = spg_state%r1_hu_n ; r%r1_hv_n = spg_state%r1_hv_n
    END FUNCTION rst_state
 
-   SUBROUTINE rst_write( kt, cdir, prhop )
+   SUBROUTINE rst_write( kt, cdir, prhop, pqsr_hc, pfraqsr_1lev )
       !! IOM/restart.F90:133 with ts_rst WRITE: the bound before and now fields,
-      !! sea levels, barotropic time-filter fields and prhop into directory cdir
+      !! sea levels, barotropic time-filter fields and prhop into directory cdir;
+      !! with tra_qsr, its qsr_hc (as qsr_hc_b) and fraqsr_1lev (traqsr.F90:291-292)
       INTEGER, INTENT(in) :: kt
       CHARACTER(len=*), INTENT(in) :: cdir
       REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(in) :: prhop
-      CALL check( c_rst_write( cstr(TRIM(cdir)), INT(kt, C_INT), rst_state( C_LOC(prhop) ) ), 'rst_write' )
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(in), OPTIONAL :: pqsr_hc
+      REAL(wp), DIMENSION(:,:)  , TARGET, CONTIGUOUS, INTENT(in), OPTIONAL :: pfraqsr_1lev
+      TYPE(nemo_hip_rst_t) :: r
+      r = rst_state( C_LOC(prhop) )
+      IF( PRESENT(pqsr_hc) )        r%qsr_hc = C_LOC(pqsr_hc)
+      IF( PRESENT(pfraqsr_1lev) )   r%fraqsr_1lev = C_LOC(pfraqsr_1lev)
+      CALL check( c_rst_write( cstr(TRIM(cdir)), INT(kt, C_INT), r ), 'rst_write' )
    END SUBROUTINE rst_write
 
-   SUBROUTINE rst_read( cdir, prhop, kt, kneuler )
+   SUBROUTINE rst_read( cdir, prhop, kt, kneuler, pqsr_hc_b, pfraqsr_1lev, kqsr_b )
       !! IOM/restart.F90:240 with ts_rst READ and the depth means of
-      !! DOM/istate.F90:159-178; kneuler = 0 for an Euler restart
+      !! DOM/istate.F90:159-178; kneuler = 0 for an Euler restart; with
+      !! tra_qsr, qsr_hc_b and fraqsr_1lev (traqsr.F90:136-139, 424-425) and
+      !! kqsr_b = 1 when the file held qsr_hc_b (tra_qsr's argument at nit000)
       CHARACTER(len=*), INTENT(in) :: cdir
       REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(inout) :: prhop
       INTEGER, INTENT(out) :: kt, kneuler
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(inout), OPTIONAL :: pqsr_hc_b
+      REAL(wp), DIMENSION(:,:)  , TARGET, CONTIGUOUS, INTENT(inout), OPTIONAL :: pfraqsr_1lev
+      INTEGER, INTENT(out), OPTIONAL :: kqsr_b
       TYPE(nemo_hip_rst_t) :: r
       INTEGER(C_INT) :: ikt, ine
       r = rst_state( C_LOC(prhop) )
+      IF( PRESENT(pqsr_hc_b) )      r%qsr_hc_b = C_LOC(pqsr_hc_b)
+      IF( PRESENT(pfraqsr_1lev) )   r%fraqsr_1lev = C_LOC(pfraqsr_1lev)
       CALL check( c_rst_read( cstr(TRIM(cdir)), r, ikt, ine ), 'rst_read' )
       kt = ikt ; kneuler = ine
+      IF( PRESENT(kqsr_b) )   kqsr_b = r%kqsr_b
    END SUBROUTINE rst_read
 
    SUBROUTINE zdf_mxl( kt )

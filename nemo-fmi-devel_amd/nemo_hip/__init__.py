@@ -339,9 +339,11 @@ class NemoHip:
                 rn_si1=23.0, sp=False):
         """tra_qsr, two bands (namelist_ref's rn_abs, rn_si0, rn_si1 by default)"""
         fn = self.L.nemo_hip_tra_qsr_sp if sp else self.L.nemo_hip_tra_qsr
+        dt = np.float32 if sp else np.float64
         p = self._p
         self._check(fn(C.c_int(kt), C.c_int(kqsr_b), C.c_int(nksr), C.c_double(rn_abs), C.c_double(rn_si0),
-                       C.c_double(rn_si1), p(qsr), p(gdepw), p(qsr_hc), p(qsr_hc_b), p(tsa), p(fraqsr_1lev)))
+                       C.c_double(rn_si1), p(qsr), p(gdepw, dt), p(qsr_hc), p(qsr_hc_b), p(tsa, dt),
+                       p(fraqsr_1lev)))
 
     def tra_sbc(self, kt, qns, qsr, sfx, emp, tsn, tsa, sbc_tsc, sbc_tsc_b, sp=False):
         fn = self.L.nemo_hip_tra_sbc_sp if sp else self.L.nemo_hip_tra_sbc
@@ -431,9 +433,9 @@ class NemoHip:
         for n in abi.RST_FIELDS:
             a = fields.get(n)
             if a is not None:
-                two_d = n in ("sshb", "sshn", "ub2_b", "vb2_b", "un_bf", "vn_bf", "un_b", "vn_b", "ub_b", "vb_b",
-                              "r1_hu_n", "r1_hv_n")
-                setattr(r, n, self._p(a, np.float64 if two_d else dt3).value)
+                fp64 = n in ("sshb", "sshn", "ub2_b", "vb2_b", "un_bf", "vn_bf", "un_b", "vn_b", "ub_b", "vb_b",
+                             "r1_hu_n", "r1_hv_n", "qsr_hc", "qsr_hc_b", "fraqsr_1lev")
+                setattr(r, n, self._p(a, np.float64 if fp64 else dt3).value)
         return r
 
     def rst_write(self, path, kt, sp=False, **fields):
@@ -442,10 +444,12 @@ class NemoHip:
         self._check(self.L.nemo_hip_rst_write(str(path).encode(), C.c_int(kt), C.byref(r)))
 
     def rst_read(self, path, sp=False, **fields):
-        """rst_read: fills the fields; returns (kt, neuler)."""
+        """rst_read: fills the fields; returns (kt, neuler).  self.kqsr_b is
+        then 1 when the file held qsr_hc_b (and it was asked for), else 0."""
         r = self._rst(fields, sp)
         kt, ne = C.c_int(), C.c_int()
         self._check(self.L.nemo_hip_rst_read(str(path).encode(), C.byref(r), C.byref(kt), C.byref(ne)))
+        self.kqsr_b = r.kqsr_b
         return kt.value, ne.value
 
     def zdf_mxl(self, kt, rn2b, gdept, gdepw, nlb10, nmln, hmlp, hmlpt, rho_c=0.01, sp=False):

@@ -31,7 +31,7 @@ DOMAIN_INT_FIELDS = [
     "n_dynadv", "nn_dynkeg", "nvor_scheme", "nn_een_e3f",
     "ln_drgimp",
     "ln_dynspg_ts", "ln_bt_fw", "ln_bt_av", "nn_bt_flt", "nn_baro",
-    "neuler", "nit000", "reserved0",
+    "neuler", "nit000", "ln_traqsr",
 ]
 DOMAIN_REAL_FIELDS = ["rdt", "rn_atfp", "rau0", "grav", "rn_bt_alpha", "rn_scal_load"]
 DOMAIN_KIDX_FIELDS = ["k_top", "k_bot"]
@@ -103,8 +103,8 @@ def eos80():
 
 # restart state, nemo_hip_rst_t (IOM/restart.F90:133-309, DYN/dynspg_ts.F90:1391-1463)
 RST_FIELDS = ["ub", "vb", "tsb", "un", "vn", "tsn", "rhop", "sshb", "sshn", "ub2_b", "vb2_b", "un_bf", "vn_bf",
-              "un_b", "vn_b", "ub_b", "vb_b", "r1_hu_n", "r1_hv_n"]
+              "un_b", "vn_b", "ub_b", "vb_b", "r1_hu_n", "r1_hv_n", "qsr_hc", "qsr_hc_b", "fraqsr_1lev"]
 
 
 class RstStruct(C.Structure):
-    _fields_ = [("real_bytes", C.c_int32), ("reserved0", C.c_int32)] + [(n, C.c_void_p) for n in RST_FIELDS]
+    _fields_ = [("real_bytes", C.c_int32), ("kqsr_b", C.c_int32)] + [(n, C.c_void_p) for n in RST_FIELDS]

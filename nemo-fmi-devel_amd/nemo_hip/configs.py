@@ -46,8 +46,7 @@ class Domain:
         s = abi.DomainStruct()
         s.jpi, s.jpj, s.jpk, s.jperio = self.jpi, self.jpj, self.jpk, self.jperio
         for n in abi.DOMAIN_INT_FIELDS[4:]:
-            if n != "reserved0":
-                setattr(s, n, int(self.opts[n]))
+            setattr(s, n, int(self.opts.get(n, 0)))
         for n in abi.DOMAIN_REAL_FIELDS:
             setattr(s, n, float(self.opts[n]))
         for n in abi.DOMAIN_KIDX_FIELDS:
@@ -74,7 +73,7 @@ DEFAULT_OPTS = dict(
     ln_traldf_lap=1, ln_traldf_msc=0, ln_zdfddm=0, ln_zad_Aimp=0,
     n_dynadv=abi.NP_VEC_C2, nn_dynkeg=0, nvor_scheme=abi.NP_ENE, nn_een_e3f=1,
     ln_drgimp=1, ln_dynspg_ts=1, ln_bt_fw=1, ln_bt_av=1, nn_bt_flt=1, nn_baro=30,
-    neuler=0, nit000=1,
+    neuler=0, nit000=1, ln_traqsr=0,
     rdt=1800.0, rn_atfp=0.1, rau0=RAU0, grav=GRAV, rn_bt_alpha=0.0, rn_scal_load=0.0,
 )
 

@@ -2361,11 +2361,12 @@ int ora_tra_sbc(int kt, double *qns, double *qsr, const double *sfx, const doubl
     const double rcp = 3991.86795711963, r1_rau0 = 1.0 / D.rau0, r1_rau0_rcp = 1.0 / (D.rau0 * rcp);
     const size_t nn = (size_t)jpi * jpj;
     double zfact;
-    for (j = 1; j <= jpj; j++)
-        for (i = 1; i <= jpi; i++) {
-            A2(qns, i, j) = A2(qns, i, j) + A2(qsr, i, j);
-            A2(qsr, i, j) = 0.0;
-        }
+    if (!D.ln_traqsr) /* trasbc.F90:97-100 */
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A2(qns, i, j) = A2(qns, i, j) + A2(qsr, i, j);
+                A2(qsr, i, j) = 0.0;
+            }
     if (kt == D.nit000) {
         zfact = 1.0;
         for (size_t q = 0; q < 2 * nn; q++) {
@@ -2889,7 +2890,9 @@ int ora_abi_layout(size_t *out, int n)
                   offsetof(nemo_hip_domain_t, e3vw_n), sizeof(nemo_hip_spg_ts_t),
                   offsetof(nemo_hip_spg_ts_t, un_b), offsetof(nemo_hip_spg_ts_t, vn_bf), sizeof(nemo_hip_eos_t),
                   offsetof(nemo_hip_eos_t, rn_a0), offsetof(nemo_hip_eos_t, rn_nu),
-                  sizeof(nemo_hip_rst_t), offsetof(nemo_hip_rst_t, rhop), offsetof(nemo_hip_rst_t, r1_hv_n)};
+                  sizeof(nemo_hip_rst_t), offsetof(nemo_hip_rst_t, rhop), offsetof(nemo_hip_rst_t, r1_hv_n),
+                  offsetof(nemo_hip_rst_t, fraqsr_1lev), offsetof(nemo_hip_domain_t, ln_traqsr),
+                  offsetof(nemo_hip_rst_t, kqsr_b)};
     int m = (int)(sizeof v / sizeof v[0]);
     for (int q = 0; q < n && q < m; q++) out[q] = v[q];
     return m;

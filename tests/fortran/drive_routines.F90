@@ -48,7 +48,7 @@ PROGRAM drive_routines
    sdom%n_dynadv = ih(13) ; sdom%nn_dynkeg = ih(14) ; sdom%nvor_scheme = ih(15) ; sdom%nn_een_e3f = ih(16)
    sdom%ln_drgimp = ih(17)
    sdom%ln_dynspg_ts = ih(18) ; sdom%ln_bt_fw = ih(19) ; sdom%ln_bt_av = ih(20) ; sdom%nn_bt_flt = ih(21)
-   sdom%nn_baro = ih(22) ; sdom%neuler = ih(23) ; sdom%nit000 = ih(24) ; sdom%reserved0 = 0
+   sdom%nn_baro = ih(22) ; sdom%neuler = ih(23) ; sdom%nit000 = ih(24) ; sdom%ln_traqsr = ih(25)
    sdom%rdt = rh(1) ; sdom%rn_atfp = rh(2) ; sdom%rau0 = rh(3) ; sdom%grav = rh(4)
    sdom%rn_bt_alpha = rh(5) ; sdom%rn_scal_load = rh(6)
    sdom%k_top = C_LOC( ktop ) ; sdom%k_bot = C_LOC( kbot )

@@ -23,13 +23,14 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_struct_layout_matches_c():
-    out = (C.c_size_t * 16)()
-    n = oracle.lib().ora_abi_layout(out, 16)
+    out = (C.c_size_t * 32)()
+    n = oracle.lib().ora_abi_layout(out, 32)
     got = list(out[:n])
     D, S, E, R = abi.DomainStruct, abi.SpgTsStruct, abi.EosStruct, abi.RstStruct
     want = [C.sizeof(D), D.rdt.offset, D.k_top.offset, D.e1t.offset, D.e3t_n.offset, D.e3vw_n.offset,
             C.sizeof(S), S.un_b.offset, S.vn_bf.offset, C.sizeof(E), E.rn_a0.offset, E.rn_nu.offset,
-            C.sizeof(R), R.rhop.offset, R.r1_hv_n.offset]
+            C.sizeof(R), R.rhop.offset, R.r1_hv_n.offset, R.fraqsr_1lev.offset, D.ln_traqsr.offset,
+            R.kqsr_b.offset]
     assert got == want
 
 

@@ -22,8 +22,7 @@ DRIVER = os.path.join(HERE, "fortran", "drive_routines")
 def write_input(path, d, S, kt, p2dt, zun, zvn, zwn):
     """The stream file drive_routines reads: header, domain arrays in
     nemo_hip_domain_t order, then the fields (all Fortran order)."""
-    ih = [d.jpi, d.jpj, d.jpk, d.jperio] + [int(d.opts[n]) if n != "reserved0" else 0
-                                              for n in abi.DOMAIN_INT_FIELDS[4:]]
+    ih = [d.jpi, d.jpj, d.jpk, d.jperio] + [int(d.opts.get(n, 0)) for n in abi.DOMAIN_INT_FIELDS[4:]]
     rh = [float(d.opts[n]) for n in abi.DOMAIN_REAL_FIELDS]
     f = lambda a, dt=np.float64: np.asarray(a, dtype=dt).ravel(order="F").tobytes()
     with open(path, "wb") as fh:

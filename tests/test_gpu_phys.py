@@ -319,3 +319,64 @@ def test_tra_qsr(hip, case, sp):
         check(host(gfr, e.s2), fr, tol=SP_TOL if sp else 1e-12, what=f"tra_qsr {case} kt={kt} fraqsr_1lev")
     with pytest.raises(nemo_hip.NemoHipError, match="nksr"):
         hip.tra_qsr(2, 0, gq, gw, ghc, ghb, gta, gfr, sp=sp)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sp", [False, True])
+@pytest.mark.parametrize("mode", ["device", "host"])
+def test_tra_sbc_then_qsr(hip, sp, mode):
+    """step.F90's order, tra_sbc then tra_qsr, with ln_traqsr = 1 registered:
+    tra_sbc leaves qsr to tra_qsr (trasbc.F90:97-100), so the solar heat is
+    spread over the levels down to nksr instead of entering level 1 through
+    qns.  Checked against the oracle run in the same order, in device- and
+    host-pointer mode and both storage forms."""
+    e = Env(hip, "basin", opts=dict(ln_traqsr=1))
+    rng = np.random.default_rng(79)
+    ts = tracers(rng, e, sp)
+    gdepw = configs.depths(e.d)[1]
+    if sp:
+        gdepw = np.asfortranarray(gdepw.astype(np.float32).astype(np.float64))
+    nksr = e.o.qsr_nksr(gdepw)
+    assert nksr > 1
+    fields = {n: np.asfortranarray(rng.standard_normal(e.s2) * sc) for n, sc in
+              (("qns", 100.0), ("sfx", 1e-3), ("emp", 1e-5))}
+    fields["qsr"] = np.asfortranarray(rng.uniform(0.0, 300.0, e.s2))
+    o = {n: v.copy(order="F") for n, v in fields.items()}
+    o.update(sbc=np.zeros(e.s2 + (2,), order="F"), sbc_b=np.zeros(e.s2 + (2,), order="F"),
+             hc=np.zeros(e.s3, order="F"), hb=np.zeros(e.s3, order="F"), fr=np.zeros(e.s2, order="F"))
+    tsa = rnd(rng, e.s4, sp)
+    wta = tsa.copy(order="F")
+    fdt = np.float32 if sp else np.float64
+    if mode == "device":
+        gd = {n: dev(v) for n, v in o.items()}
+        gta, gts, gw = g(tsa, sp), g(ts, sp), g(gdepw, sp)
+    else:
+        gd = {n: v.copy(order="F") for n, v in o.items()}
+        gta = np.asfortranarray(tsa.astype(fdt))
+        gts, gw = np.asfortranarray(ts.astype(fdt)), np.asfortranarray(gdepw.astype(fdt))
+    if mode == "host":
+        hip.set_pointer_mode(NEMO_HIP_PTR_HOST)
+    try:
+        _sbc_qsr_steps(hip, e, mode, sp, o, fields, ts, gdepw, nksr, wta, gd, gta, gts, gw)
+    finally:
+        hip.set_pointer_mode(NEMO_HIP_PTR_DEVICE)
+    assert np.abs(o["hc"][:, :, 1:nksr]).max() > 0.0     # the heat reached below level 1
+
+
+def _sbc_qsr_steps(hip, e, mode, sp, o, fields, ts, gdepw, nksr, wta, gd, gta, gts, gw):
+    for kt in (1, 2, 3):
+        e.o.tra_sbc(kt, o["qns"], o["qsr"], o["sfx"], o["emp"], ts, wta, o["sbc"], o["sbc_b"])
+        e.o.tra_qsr(kt, nksr, o["qsr"], gdepw, o["hc"], o["hb"], wta, o["fr"])
+        hip.tra_sbc(kt, gd["qns"], gd["qsr"], gd["sfx"], gd["emp"], gts, gta, gd["sbc"], gd["sbc_b"], sp=sp)
+        hip.tra_qsr(kt, nksr, gd["qsr"], gw, gd["hc"], gd["hb"], gta, gd["fr"], sp=sp)
+        if mode == "device":
+            sync(hip)
+            got = lambda a, s: host(a, s)
+        else:
+            got = lambda a, s: np.asarray(a, dtype=np.float64).reshape(s, order="F")
+        cmp(got(gta, e.s4), wta, sp, f"tra_sbc+tra_qsr kt={kt} tsa")
+        # qsr survives tra_sbc, and qns is not given the solar part
+        check(got(gd["qsr"], e.s2), fields["qsr"], what="qsr kept")
+        check(got(gd["qns"], e.s2), fields["qns"], what="qns without qsr")
+        check(got(gd["hc"], e.s3), o["hc"], tol=SP_TOL if sp else 1e-12, what=f"qsr_hc kt={kt}")
+        check(got(gd["fr"], e.s2), o["fr"], tol=SP_TOL if sp else 1e-12, what=f"fraqsr_1lev kt={kt}")

@@ -146,3 +146,45 @@ def test_restart_errors(hip, tmp_path):
     hip.register(restarted(d, ln_bt_av=0))
     with pytest.raises(nemo_hip.NemoHipError, match="ln_bt_av"):
         hip.rst_write(tmp_path / "d", 3, **F)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("mode", ["device", "host"])
+def test_restart_solar_flux_variables(hip, mode, tmp_path):
+    """tra_qsr's restart variables (traqsr.F90:287-292, 136-139, 424-425):
+    rst_write stores qsr_hc under the name qsr_hc_b and fraqsr_1lev;
+    rst_read reads them into qsr_hc_b and fraqsr_1lev and reports
+    kqsr_b = 1; a file without them leaves both untouched with kqsr_b = 0"""
+    d = CASES["cyc7"]()
+    hip.register(d)
+    rng = np.random.default_rng(6)
+    n2, n3 = d.jpi * d.jpj, d.jpi * d.jpj * d.jpk
+    F = {k: rng.standard_normal(n3) for k in RST_3D}
+    F.update({k: rng.standard_normal(2 * n3) for k in RST_4D})
+    F.update({k: rng.standard_normal(n2) for k in RST_2D})
+    hc, fr = rng.standard_normal(n3), rng.uniform(0, 1, n2)
+    if mode == "device":
+        to = lambda a: torch.from_numpy(a).to("cuda")
+        back = lambda t: t.cpu().numpy()
+    else:
+        to = lambda a: a.copy()
+        back = lambda a: a
+        hip.set_pointer_mode(NEMO_HIP_PTR_HOST)
+    try:
+        G = {k: to(v) for k, v in F.items()}
+        hip.rst_write(tmp_path / "q", 7, qsr_hc=to(hc), fraqsr_1lev=to(fr), **G)
+        man = json.load(open(tmp_path / "q" / "manifest.json"))
+        assert {"qsr_hc_b", "fraqsr_1lev"} <= {v["name"] for v in man["variables"]}
+        hb, fr2 = to(np.full(n3, np.nan)), to(np.full(n2, np.nan))
+        H = {k: to(np.full_like(v, np.nan)) for k, v in F.items()}
+        assert hip.rst_read(tmp_path / "q", qsr_hc_b=hb, fraqsr_1lev=fr2, **H) == (7, 1)
+        assert hip.kqsr_b == 1
+        assert np.array_equal(back(hb), hc) and np.array_equal(back(fr2), fr)
+        # a restart without them: an Euler start of tra_qsr
+        hip.rst_write(tmp_path / "p", 7, **G)
+        hb2, fr3 = to(np.full(n3, 3.0)), to(np.full(n2, 4.0))
+        assert hip.rst_read(tmp_path / "p", qsr_hc_b=hb2, fraqsr_1lev=fr3, **H) == (7, 1)
+        assert hip.kqsr_b == 0
+        assert np.all(back(hb2) == 3.0) and np.all(back(fr3) == 4.0)
+    finally:
+        hip.set_pointer_mode(NEMO_HIP_PTR_DEVICE)

@@ -99,3 +99,51 @@ def test_time_stepping():
     want = np.zeros_like(e3t)
     want[1:-1, 1:-1, :nksr] = 0.5 * (1e-5 + hc1[1:-1, 1:-1, :nksr]) / e3t[1:-1, 1:-1, :nksr]
     assert np.array_equal(hb3, B) and np.allclose(ta3[..., 0], want, rtol=1e-14, atol=0.0)
+
+
+def test_sbc_then_qsr_with_ln_traqsr():
+    """ln_traqsr = 1 (trasbc.F90:97-100): tra_sbc leaves qns and qsr as
+    they are and its level-1 heat trend is that of qns alone; tra_sbc then
+    tra_qsr put the column's heat content to qns + Qsr (1 - I(bottom)/Qsr),
+    in units of rau0 rcp, where the merged run (ln_traqsr = 0) puts all of
+    qns + qsr into level 1."""
+    d0 = configs.basin(16, 14, 31, seed=3, land_frac=0.0, depth_min=5000.0, depth_max=5000.0)
+    d1 = configs.basin(16, 14, 31, seed=3, land_frac=0.0, depth_min=5000.0, depth_max=5000.0,
+                       opts=dict(ln_traqsr=1))
+    gdepw = configs.depths(d1)[1]
+    rng = np.random.default_rng(12)
+    s2, s3 = (d1.jpi, d1.jpj), (d1.jpi, d1.jpj, d1.jpk)
+    qns = np.asfortranarray(rng.uniform(-100.0, 100.0, s2))
+    qsr = np.asfortranarray(rng.uniform(50.0, 300.0, s2))
+    zero2 = np.zeros(s2, order="F")
+    ts = np.zeros(s3 + (2,), order="F")
+    ts[..., 0], ts[..., 1] = 10.0, 35.0
+
+    def sbc(d, q, s):
+        o = Oracle(d)                            # the oracle holds one registered domain
+        a, b = q.copy(order="F"), s.copy(order="F")
+        tsa = np.zeros(s3 + (2,), order="F")
+        o.tra_sbc(1, a, b, zero2.copy(order="F"), zero2.copy(order="F"), ts, tsa,
+                  np.zeros(s2 + (2,), order="F"), np.zeros(s2 + (2,), order="F"))
+        return a, b, tsa
+
+    a1, b1, t1 = sbc(d1, qns, qsr)
+    assert np.array_equal(a1, qns) and np.array_equal(b1, qsr)
+    _, _, t_qns = sbc(d0, qns, zero2)           # merged run with no solar part
+    assert np.array_equal(t1, t_qns)
+    a0, b0, t0 = sbc(d0, qns, qsr)               # merged: qns + qsr at level 1
+    assert np.allclose(a0, qns + qsr, rtol=0, atol=0) and not b0.any()
+    o1 = Oracle(d1)
+    nksr = o1.qsr_nksr(gdepw)
+    fr = np.zeros(s2, order="F")
+    o1.tra_qsr(1, nksr, b1, gdepw, np.zeros(s3, order="F"), np.zeros(s3, order="F"), t1, fr)
+    e3t = d1.arrays["e3t_n"]
+    heat = (t1[..., 0] * e3t).sum(axis=2) * d1.opts["rau0"] * RCP
+    want = qns + qsr * (1.0 - profile(gdepw[:, :, nksr]))
+    want0 = (t0[..., 0] * e3t).sum(axis=2) * d0.opts["rau0"] * RCP
+    wet = d1.arrays["k_bot"] > 0                 # wet columns (a land column takes no solar heat)
+    wet[0, :] = wet[-1, :] = False
+    wet[:, 0] = wet[:, -1] = False
+    assert wet.sum() > 100
+    assert np.allclose(heat[wet], want[wet], rtol=1e-12, atol=1e-9)
+    assert np.allclose(want0[wet], (qns + qsr)[wet], rtol=1e-12, atol=1e-9)
