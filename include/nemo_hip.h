@@ -486,6 +486,46 @@ int nemo_hip_stp_ctl(int kt, const double *sshn, const double *un, const double 
 int nemo_hip_stp_ctl_sp(int kt, const double *sshn, const float *un, const float *tsn,
                         double zmax[6], int32_t loc[11], int *kindic);
 
+/* glob_sum and glob_sum_full (lib_fortran_generic.h90 FUNCTION_GLOBSUM,
+   lib_fortran.F90:300-332 DDPDD): the compensated (double-double) sum of
+   ptab(i,j,k) * mask(i,j) over levels 1..nk of a (jpi,jpj,nk) array, mask =
+   tmask_i (kind 0, glob_sum) or tmask_h (kind 1, glob_sum_full; both from
+   DOM/dommsk.F90:213-238).  The terms and the sum are FP64 in both storage
+   forms.  psum: one double in the pointer space of the call (in device mode
+   a device address, and the call is asynchronous on the library stream). */
+int nemo_hip_glob_sum(const double *ptab, int nk, int kind, double *psum);
+int nemo_hip_glob_sum_sp(const float *ptab, int nk, int kind, double *psum);
+
+/* dia_hsb, DIA/diahsb.F90:60-239 (linear free surface, no ice-shelf cavity,
+   runoff or geothermal flux): the heat, salt and volume budget of the run
+   from thirteen compensated global sums, all FP64 (also in the precision
+   form).  nemo_hip_dia_hsb_init is dia_hsb_init with dia_hsb_rst('READ')
+   without a restart (:278-307, :411-412): it writes the initial-state
+   arrays from tsn and sshn, sets surf_tot and zeroes the accumulated forcing.
+   nemo_hip_dia_hsb adds the step's forcing to frc_* and fills sums[] (in the
+   order of gsum.hip's table) and the diagnostics iom_put writes (:221-233).
+   The reference also masks tsn and tsb in place (:88-89); this call leaves
+   them as given and masks the terms instead, which gives the same sums.
+   Pointers follow the pointer mode; the scalars are host memory and the
+   calls synchronise. */
+typedef struct nemo_hip_hsb {
+    int32_t real_bytes;                    /* 8: tsn, tsb, wn double; 4: float */
+    int32_t reserved0;
+    const void *tsn, *tsb;                 /* (jpi,jpj,jpk,2)                  */
+    const void *wn;                        /* (jpi,jpj,jpk)                    */
+    const double *sshn, *emp, *rnf, *qsr;  /* (jpi,jpj); qsr only with ln_traqsr */
+    const double *sbc_tsc;                 /* (jpi,jpj,2)                      */
+    double *surf_ini, *ssh_ini, *ssh_hc_loc_ini, *ssh_sc_loc_ini;   /* (jpi,jpj)      */
+    double *e3t_ini, *hc_loc_ini, *sc_loc_ini;                       /* (jpi,jpj,jpk)  */
+    double surf_tot;                       /* set by init                      */
+    double frc_v, frc_t, frc_s, frc_wn_t, frc_wn_s;   /* accumulated forcing   */
+    double sums[13];                       /* out: the call's global sums      */
+    double bgfrcvol, bgfrctem, bgfrchfx, bgfrcsal, bgtemper, bgsaline;
+    double bgheatco, bgheatfx, bgsaltco, bgvolssh, bgmistem, bgmissal;   /* out */
+} nemo_hip_hsb_t;
+int nemo_hip_dia_hsb_init(nemo_hip_hsb_t *h);
+int nemo_hip_dia_hsb(int kt, nemo_hip_hsb_t *h);
+
 /* ------------------------------------------------------------------------ */
 /* time-step closure around the hot path (linear free surface)              */
 /* ------------------------------------------------------------------------ */

@@ -43,6 +43,8 @@ struct Ctx {
     SeosPar eos{};
     int neos = NEMO_NP_SEOS;
     double *stp_buf = nullptr;                          // stp_ctl partial maxima and results
+    double *gsum_buf = nullptr;                         // glob_sum partial pairs and total
+    double *hsb_buf = nullptr;                          // dia_hsb partial pairs and sums
 };
 static std::vector<Ctx *> g_ctxs;
 static Ctx *g_c = nullptr;   // the current context
@@ -167,6 +169,7 @@ static void free_domain()
     for (auto &s : g_c->scratch3) s = nullptr;
     for (auto &s : g_c->scratch2) s = nullptr;
     g_c->stp_buf = nullptr;
+    g_c->gsum_buf = g_c->hsb_buf = nullptr;
     g_c->have_domain = false;
 }
 
@@ -488,6 +491,8 @@ static int register_domain(const nemo_hip_domain_t *h)
         if ((rc = dev_alloc((void **)&g_c->scratch2[s], n2 * sizeof(double)))) return rc;
     for (int s = 0; s < NH_NSCRATCH2; s++) HIPCHK(hipMemset(g_c->scratch2[s], 0, n2 * sizeof(double)));
     if ((rc = dev_alloc((void **)&g_c->stp_buf, stp_ctl_buffer_bytes()))) return rc;
+    if ((rc = dev_alloc((void **)&g_c->gsum_buf, glob_sum_scratch_bytes()))) return rc;
+    if ((rc = dev_alloc((void **)&g_c->hsb_buf, dia_hsb_scratch_bytes() + 16 * sizeof(double)))) return rc;
     HIPCHK(hipDeviceSynchronize());
     g_c->dom = d;
     g_c->have_domain = true;
@@ -1360,6 +1365,115 @@ static int do_tra_qsr(int kt, int kqsr_b, int nksr, double rn_abs, double rn_si0
     return S.finish(rc);
 }
 
+// glob_sum / glob_sum_full: the compensated sum into psum (one double)
+template <typename T>
+static int do_glob_sum(const T *ptab, int nk, int kind, double *psum)
+{
+    NEED_DOMAIN();
+    NEED_PTR(ptab); NEED_PTR(psum);
+    const DevDomain &d = g_c->dom;
+    if (nk < 1 || nk > d.jpk) return set_error(NEMO_HIP_ERR_ARG, "glob_sum: nk = %d outside 1..jpk", nk);
+    if (kind != 0 && kind != 1) return set_error(NEMO_HIP_ERR_ARG, "glob_sum: kind = %d (0 tmask_i, 1 tmask_h)", kind);
+    Stager S;
+    const T *p = S.in(ptab, d.n2 * (size_t)nk);
+    double *o = S.out(psum, 1);
+    int rc = S.status ? S.status : launch_glob_sum<T>(p, nk, kind, g_c->gsum_buf, o);
+    return S.finish(rc);
+}
+
+// dia_hsb: the thirteen sums on the device, the budget on the host
+static const double HSB_RCP = 3991.86795711963;   // TRA/eosbn2.F90:1245
+template <typename T>
+static int do_dia_hsb_init(nemo_hip_hsb_t *h)
+{
+    NEED_DOMAIN();
+    NEED_NOT_CAPTURING("dia_hsb_init");
+    NEED_PTR(h);
+    NEED_PTR(h->tsn); NEED_PTR(h->sshn); NEED_PTR(h->surf_ini); NEED_PTR(h->ssh_ini); NEED_PTR(h->ssh_hc_loc_ini);
+    NEED_PTR(h->ssh_sc_loc_ini); NEED_PTR(h->e3t_ini); NEED_PTR(h->hc_loc_ini); NEED_PTR(h->sc_loc_ini);
+    const size_t n2 = g_c->dom.n2, n3 = g_c->dom.n3;
+    Stager S;
+    const double *sn = S.in(h->sshn, n2);
+    const T *ts = S.in((const T *)h->tsn, 2 * n3);
+    double *si = S.out(h->surf_ini, n2), *shi = S.out(h->ssh_ini, n2), *shc = S.out(h->ssh_hc_loc_ini, n2);
+    double *ssc = S.out(h->ssh_sc_loc_ini, n2), *e3i = S.out(h->e3t_ini, n3), *hci = S.out(h->hc_loc_ini, n3);
+    double *sci = S.out(h->sc_loc_ini, n3);
+    double *tot = g_c->hsb_buf + dia_hsb_scratch_bytes() / sizeof(double);
+    int rc = S.status ? S.status : launch_dia_hsb_init<T>(sn, ts, si, shi, shc, ssc, e3i, hci, sci);
+    // surf_tot = glob_sum(surf), diahsb.F90:412
+    if (!rc) rc = launch_glob_sum<double>(si, 1, 0, g_c->gsum_buf, tot);
+    if ((rc = S.finish(rc))) return rc;
+    HIPCHK(hipMemcpyAsync(&h->surf_tot, tot, sizeof(double), hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    h->frc_v = h->frc_t = h->frc_s = h->frc_wn_t = h->frc_wn_s = 0.0;   // diahsb.F90:291-305
+    return NEMO_HIP_OK;
+}
+
+template <typename T>
+static int do_dia_hsb(int kt, nemo_hip_hsb_t *h)
+{
+    NEED_DOMAIN();
+    NEED_NOT_CAPTURING("dia_hsb");
+    NEED_PTR(h);
+    NEED_PTR(h->tsn); NEED_PTR(h->tsb); NEED_PTR(h->wn); NEED_PTR(h->sshn); NEED_PTR(h->emp); NEED_PTR(h->rnf);
+    NEED_PTR(h->sbc_tsc); NEED_PTR(h->surf_ini); NEED_PTR(h->ssh_ini); NEED_PTR(h->ssh_hc_loc_ini);
+    NEED_PTR(h->ssh_sc_loc_ini); NEED_PTR(h->e3t_ini); NEED_PTR(h->hc_loc_ini); NEED_PTR(h->sc_loc_ini);
+    const int traqsr = g_c->dom.ln_traqsr;
+    if (traqsr) NEED_PTR(h->qsr);
+    const size_t n2 = g_c->dom.n2, n3 = g_c->dom.n3;
+    Stager S;
+    const double *em = S.in(h->emp, n2), *rn = S.in(h->rnf, n2), *sb = S.in(h->sbc_tsc, 2 * n2);
+    const double *qs = traqsr ? S.in(h->qsr, n2) : em;
+    const double *sn = S.in(h->sshn, n2);
+    const T *tn = S.in((const T *)h->tsn, 2 * n3), *tb = S.in((const T *)h->tsb, 2 * n3);
+    const T *w = S.in((const T *)h->wn, n3);
+    const double *si = S.in(h->surf_ini, n2), *shi = S.in(h->ssh_ini, n2), *shc = S.in(h->ssh_hc_loc_ini, n2);
+    const double *ssc = S.in(h->ssh_sc_loc_ini, n2), *e3i = S.in(h->e3t_ini, n3), *hci = S.in(h->hc_loc_ini, n3);
+    const double *sci = S.in(h->sc_loc_ini, n3);
+    double *sums = g_c->hsb_buf + dia_hsb_scratch_bytes() / sizeof(double);
+    int rc = S.status ? S.status
+                      : launch_dia_hsb<T>(traqsr, em, rn, sb, qs, sn, tn, tb, w, si, shi, shc, ssc, e3i, hci, sci,
+                                          g_c->hsb_buf, sums);
+    if ((rc = S.finish(rc))) return rc;
+    double z[13];
+    HIPCHK(hipMemcpyAsync(z, sums, sizeof z, hipMemcpyDeviceToHost, g_stream));
+    HIPCHK(hipStreamSynchronize(g_stream));
+    for (int q = 0; q < 13; q++) h->sums[q] = z[q];
+    // diahsb.F90:93-233, linear free surface
+    const double rau0 = g_c->hdom.rau0, rdt = g_c->hdom.rdt;
+    const double r1_rau0 = 1.0 / rau0, r1_rau0_rcp = 1.0 / (rau0 * HSB_RCP);
+    const double z_frc_trd_v = r1_rau0 * z[0];
+    double z_frc_trd_t = z[1];
+    const double z_frc_trd_s = z[2];
+    if (traqsr) z_frc_trd_t = z_frc_trd_t + r1_rau0_rcp * z[3];
+    const double z_wn_trd_t = -z[4], z_wn_trd_s = -z[5];
+    h->frc_v = h->frc_v + z_frc_trd_v * rdt;
+    h->frc_t = h->frc_t + z_frc_trd_t * rdt;
+    h->frc_s = h->frc_s + z_frc_trd_s * rdt;
+    h->frc_wn_t = h->frc_wn_t + z_wn_trd_t * rdt;
+    h->frc_wn_s = h->frc_wn_s + z_wn_trd_s * rdt;
+    const double zdiff_v1 = z[6] - h->frc_v;
+    const double z_ssh_hc = z[7], z_ssh_sc = z[8];
+    const double zdiff_hc = z[10] - h->frc_t, zdiff_sc = z[11] - h->frc_s;
+    const double zdiff_hc1 = zdiff_hc + z_ssh_hc, zdiff_sc1 = zdiff_sc + z_ssh_sc;
+    const double zerr_hc1 = z_ssh_hc - h->frc_wn_t, zerr_sc1 = z_ssh_sc - h->frc_wn_s;
+    const double zvol_tot = z[12];
+    const double ztime = h->surf_tot * (double)kt * rdt;
+    h->bgfrcvol = h->frc_v * 1.e-9;
+    h->bgfrctem = h->frc_t * rau0 * HSB_RCP * 1.e-20;
+    h->bgfrchfx = h->frc_t * rau0 * HSB_RCP / ztime;
+    h->bgfrcsal = h->frc_s * 1.e-9;
+    h->bgtemper = zdiff_hc1 / zvol_tot;
+    h->bgsaline = zdiff_sc1 / zvol_tot;
+    h->bgheatco = zdiff_hc1 * 1.e-20 * rau0 * HSB_RCP;
+    h->bgheatfx = zdiff_hc1 * rau0 * HSB_RCP / ztime;
+    h->bgsaltco = zdiff_sc1 * 1.e-9;
+    h->bgvolssh = zdiff_v1 * 1.e-9;
+    h->bgmistem = zerr_hc1 / zvol_tot;
+    h->bgmissal = zerr_sc1 / zvol_tot;
+    return NEMO_HIP_OK;
+}
+
 // stp_ctl: maxima on the device, the tests and the message on the host
 template <typename T>
 static int do_stp_ctl(int kt, const double *sshn, const T *un, const T *tsn, double zmax[6], int32_t loc[11],
@@ -1943,6 +2057,29 @@ int nemo_hip_stp_ctl(int kt, const double *sshn, const double *un, const double 
                      int32_t loc[11], int *kindic)
 {
     return do_stp_ctl<double>(kt, sshn, un, tsn, zmax, loc, kindic);
+}
+
+int nemo_hip_glob_sum(const double *ptab, int nk, int kind, double *psum)
+{
+    return do_glob_sum<double>(ptab, nk, kind, psum);
+}
+int nemo_hip_glob_sum_sp(const float *ptab, int nk, int kind, double *psum)
+{
+    return do_glob_sum<float>(ptab, nk, kind, psum);
+}
+int nemo_hip_dia_hsb_init(nemo_hip_hsb_t *h)
+{
+    if (!h) return set_error(NEMO_HIP_ERR_ARG, "dia_hsb_init: NULL argument");
+    if (h->real_bytes == 8) return do_dia_hsb_init<double>(h);
+    if (h->real_bytes == 4) return do_dia_hsb_init<float>(h);
+    return set_error(NEMO_HIP_ERR_ARG, "dia_hsb_init: real_bytes = %d (8 or 4)", h->real_bytes);
+}
+int nemo_hip_dia_hsb(int kt, nemo_hip_hsb_t *h)
+{
+    if (!h) return set_error(NEMO_HIP_ERR_ARG, "dia_hsb: NULL argument");
+    if (h->real_bytes == 8) return do_dia_hsb<double>(kt, h);
+    if (h->real_bytes == 4) return do_dia_hsb<float>(kt, h);
+    return set_error(NEMO_HIP_ERR_ARG, "dia_hsb: real_bytes = %d (8 or 4)", h->real_bytes);
 }
 
 int nemo_hip_tra_qsr(int kt, int kqsr_b, int nksr, double rn_abs, double rn_si0, double rn_si1, const double *qsr,

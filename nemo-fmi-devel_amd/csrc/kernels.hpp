@@ -87,3 +87,16 @@ int launch_tra_qsr(int mode, int nksr, double rn_abs, double rn_si0, double rn_s
 template <typename T>
 int launch_tra_sbc(int first, double *qns, double *qsr, const double *sfx, const double *emp, const T *tsn, T *tsa,
                    double *sbc, double *sbc_b);
+
+// compensated global sums (gsum.hip)
+template <typename T> int launch_glob_sum(const T *p, int nk, int kind, double *scratch, double *out);
+size_t glob_sum_scratch_bytes();
+template <typename T>
+int launch_dia_hsb(int ln_traqsr, const double *emp, const double *rnf, const double *sbc, const double *qsr,
+                   const double *sshn, const T *tsn, const T *tsb, const T *wn, const double *surf_ini,
+                   const double *ssh_ini, const double *ssh_hc, const double *ssh_sc, const double *e3t_ini,
+                   const double *hc_ini, const double *sc_ini, double *scratch, double *sums);
+template <typename T>
+int launch_dia_hsb_init(const double *sshn, const T *tsn, double *surf_ini, double *ssh_ini, double *ssh_hc,
+                        double *ssh_sc, double *e3t_ini, double *hc_ini, double *sc_ini);
+size_t dia_hsb_scratch_bytes();

@@ -352,6 +352,43 @@ class NemoHip:
         self._check(fn(C.c_int(kt), q(qns), q(qsr), q(sfx), q(emp), q(tsn, dt), q(tsa, dt), q(sbc_tsc),
                        q(sbc_tsc_b)))
 
+    # compensated global sums -----------------------------------------------------
+    def glob_sum(self, ptab, nk, kind=0, out=None, sp=False):
+        """glob_sum (kind 0) / glob_sum_full (kind 1) of ptab(:,:,1:nk).  Device
+        mode: out is a one-element FP64 device tensor (returned; the call is
+        asynchronous).  Host mode: returns the sum."""
+        fn = self.L.nemo_hip_glob_sum_sp if sp else self.L.nemo_hip_glob_sum
+        dt = np.float32 if sp else np.float64
+        if self.mode == NEMO_HIP_PTR_DEVICE:
+            self._check(fn(self._p(ptab, dt), C.c_int(nk), C.c_int(kind), self._p(out)))
+            return out
+        r = np.zeros(1)
+        self._check(fn(self._p(ptab, dt), C.c_int(nk), C.c_int(kind), self._p(r)))
+        return float(r[0])
+
+    def _hsb(self, fields, sp, state=None):
+        h = abi.HsbStruct() if state is None else state
+        h.real_bytes = 4 if sp else 8
+        dt3 = np.float32 if sp else np.float64
+        for n in abi.HSB_PTR_FIELDS:
+            a = fields.get(n)
+            if a is not None:
+                setattr(h, n, self._p(a, dt3 if n in ("tsn", "tsb", "wn") else np.float64).value)
+        return h
+
+    def dia_hsb_init(self, sp=False, **fields):
+        """dia_hsb_init: fills the *_ini arrays given; returns the nemo_hip_hsb_t
+        state (surf_tot set, frc_* zero) to pass to dia_hsb."""
+        h = self._hsb(fields, sp)
+        self._check(self.L.nemo_hip_dia_hsb_init(C.byref(h)))
+        return h
+
+    def dia_hsb(self, kt, state, sp=False, **fields):
+        """dia_hsb: updates state (frc_*, sums, diagnostics) in place; returns it"""
+        h = self._hsb(fields, sp, state)
+        self._check(self.L.nemo_hip_dia_hsb(C.c_int(kt), C.byref(h)))
+        return h
+
     # run control ----------------------------------------------------------------
     def stp_ctl(self, kt, sshn, un, tsn, sp=False):
         """stp_ctl: returns (status, zmax[6], loc[11], kindic, message).  status

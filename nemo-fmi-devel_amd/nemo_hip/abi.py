@@ -108,3 +108,19 @@ RST_FIELDS = ["ub", "vb", "tsb", "un", "vn", "tsn", "rhop", "sshb", "sshn", "ub2
 
 class RstStruct(C.Structure):
     _fields_ = [("real_bytes", C.c_int32), ("kqsr_b", C.c_int32)] + [(n, C.c_void_p) for n in RST_FIELDS]
+
+
+# dia_hsb state and results (nemo_hip_hsb_t)
+HSB_PTR_FIELDS = ["tsn", "tsb", "wn", "sshn", "emp", "rnf", "qsr", "sbc_tsc", "surf_ini", "ssh_ini",
+                  "ssh_hc_loc_ini", "ssh_sc_loc_ini", "e3t_ini", "hc_loc_ini", "sc_loc_ini"]
+HSB_SCALARS = ["surf_tot", "frc_v", "frc_t", "frc_s", "frc_wn_t", "frc_wn_s"]
+HSB_DIAG = ["bgfrcvol", "bgfrctem", "bgfrchfx", "bgfrcsal", "bgtemper", "bgsaline", "bgheatco", "bgheatfx",
+            "bgsaltco", "bgvolssh", "bgmistem", "bgmissal"]
+
+
+class HsbStruct(C.Structure):
+    _fields_ = ([("real_bytes", C.c_int32), ("reserved0", C.c_int32)]
+                + [(n, C.c_void_p) for n in HSB_PTR_FIELDS]
+                + [(n, C.c_double) for n in HSB_SCALARS]
+                + [("sums", C.c_double * 13)]
+                + [(n, C.c_double) for n in HSB_DIAG])

@@ -2883,6 +2883,122 @@ int ora_lbc_adj(double *p, int nk, char nat)
     return 0;
 }
 
+/* ---------------------------------------------------------------------- */
+/* compensated global sums: DDPDD (lib_fortran.F90:300-332) accumulated in */
+/* the loop order of FUNCTION_GLOBSUM (lib_fortran_generic.h90): k, j, i.  */
+/* tmask_h: 0 on the outer ring (halo width 1, one domain), tmask_i =     */
+/* ssmask * tmask_h (DOM/dommsk.F90:213-238).                              */
+/* ---------------------------------------------------------------------- */
+static void ddpdd(double ar, double ai, double *br, double *bi)
+{
+    const double zt1 = ar + *br;
+    const double zerr = zt1 - ar;
+    const double zt2 = ((*br - zerr) + (ar - (zt1 - zerr))) + ai + *bi;
+    *br = zt1 + zt2;
+    *bi = zt2 - ((zt1 + zt2) - zt1);
+}
+static double tmask_h(int i, int j) { return (i >= 2 && i <= jpim1 && j >= 2 && j <= jpjm1) ? 1.0 : 0.0; }
+static double tmask_i(int i, int j) { return A2(ssmask, i, j) * tmask_h(i, j); }
+
+/* glob_sum (kind 0) / glob_sum_full (kind 1) of ptab(:,:,1:nk) */
+double ora_glob_sum(const double *ptab, int nk, int kind)
+{
+    int i, j, k;
+    double cr = 0.0, ci = 0.0;
+    for (k = 1; k <= nk; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                const double ztmp = A3(ptab, i, j, k) * (kind ? tmask_h(i, j) : tmask_i(i, j));
+                ddpdd(ztmp, 0.0, &cr, &ci);
+            }
+    return cr;
+}
+
+/* dia_hsb_init + dia_hsb_rst('READ') without a restart, DIA/diahsb.F90:
+   278-307, 411-412; returns surf_tot */
+double ora_dia_hsb_init(const double *tsn, const double *sshn, double *surf_ini, double *ssh_ini,
+                        double *ssh_hc_loc_ini, double *ssh_sc_loc_ini, double *e3t_ini, double *hc_loc_ini,
+                        double *sc_loc_ini)
+{
+    int i, j, k;
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) {
+            A2(surf_ini, i, j) = e1e2t(i, j) * tmask_i(i, j);
+            A2(ssh_ini, i, j) = A2(sshn, i, j);
+            A2(ssh_hc_loc_ini, i, j) = A4(tsn, i, j, 1, 1) * A2(sshn, i, j);
+            A2(ssh_sc_loc_ini, i, j) = A4(tsn, i, j, 1, 2) * A2(sshn, i, j);
+        }
+    for (k = 1; k <= jpk; k++)
+        for (j = 1; j <= jpj; j++)
+            for (i = 1; i <= jpi; i++) {
+                A3(e3t_ini, i, j, k) = e3t_n(i, j, k) * TMASK(i, j, k);
+                A3(hc_loc_ini, i, j, k) = A4(tsn, i, j, k, 1) * e3t_n(i, j, k) * TMASK(i, j, k);
+                A3(sc_loc_ini, i, j, k) = A4(tsn, i, j, k, 2) * e3t_n(i, j, k) * TMASK(i, j, k);
+            }
+    return ora_glob_sum(surf_ini, 1, 0);   /* surf = surf_ini here */
+}
+
+/* the thirteen global sums of dia_hsb (DIA/diahsb.F90:88-189, linear free
+   surface, no cavity, runoff or geothermal flux), each in its own sequential
+   DDPDD pass as the reference forms them: the in-place masking of tsn and
+   tsb (:88-89) on copies, the 2D and 3D work arrays as written, then
+   glob_sum (tmask_i) or glob_sum_full (tmask_h) */
+int ora_dia_hsb_sums(const double *tsn_in, const double *tsb_in, const double *wn, const double *sshn,
+                     const double *emp, const double *rnf, const double *qsr, const double *sbc_tsc,
+                     const double *surf_ini, const double *ssh_ini, const double *ssh_hc_loc_ini,
+                     const double *ssh_sc_loc_ini, const double *e3t_ini, const double *hc_loc_ini,
+                     const double *sc_loc_ini, double *sums)
+{
+    int i, j, k, n;
+    const size_t nn = n3() * 2;
+    double *tsn = malloc(nn * sizeof(double)), *tsb = malloc(nn * sizeof(double));
+    double *surf = z2(), *z2d = z2(), *zwrk = z3();
+    memcpy(tsn, tsn_in, nn * sizeof(double));
+    memcpy(tsb, tsb_in, nn * sizeof(double));
+    for (n = 1; n <= 2; n++)
+        for (k = 1; k <= jpk; k++)
+            for (j = 1; j <= jpj; j++)
+                for (i = 1; i <= jpi; i++) {
+                    A4(tsn, i, j, k, n) = A4(tsn, i, j, k, n) * TMASK(i, j, k);
+                    A4(tsb, i, j, k, n) = A4(tsb, i, j, k, n) * TMASK(i, j, k);
+                }
+    for (j = 1; j <= jpj; j++)
+        for (i = 1; i <= jpi; i++) A2(surf, i, j) = e1e2t(i, j) * tmask_i(i, j);
+#define SUM2(dst, expr, kind)                                         \
+    do {                                                              \
+        for (j = 1; j <= jpj; j++)                                    \
+            for (i = 1; i <= jpi; i++) A2(z2d, i, j) = (expr);        \
+        dst = ora_glob_sum(z2d, 1, kind);                             \
+    } while (0)
+#define SUM3(dst, expr)                                               \
+    do {                                                              \
+        for (k = 1; k <= jpkm1; k++)                                  \
+            for (j = 1; j <= jpj; j++)                                \
+                for (i = 1; i <= jpi; i++) A3(zwrk, i, j, k) = (expr); \
+        dst = ora_glob_sum(zwrk, jpkm1, 1);                           \
+    } while (0)
+    SUM2(sums[0], -(A2(emp, i, j) - A2(rnf, i, j) + 0.0) * A2(surf, i, j), 0);
+    SUM2(sums[1], A2(sbc_tsc, i, j) * A2(surf, i, j), 0);
+    SUM2(sums[2], sbc_tsc[n2() + ((size_t)j - 1) * jpi + i - 1] * A2(surf, i, j), 0);
+    sums[3] = 0.0;
+    if (D.ln_traqsr) SUM2(sums[3], A2(qsr, i, j) * A2(surf, i, j), 0);
+    SUM2(sums[4], A2(surf, i, j) * A3(wn, i, j, 1) * A4(tsb, i, j, 1, 1), 0);
+    SUM2(sums[5], A2(surf, i, j) * A3(wn, i, j, 1) * A4(tsb, i, j, 1, 2), 0);
+    SUM2(sums[6], A2(surf, i, j) * A2(sshn, i, j) - A2(surf_ini, i, j) * A2(ssh_ini, i, j), 1);
+    SUM2(sums[7], A2(surf, i, j) * (A4(tsn, i, j, 1, 1) * A2(sshn, i, j) - A2(ssh_hc_loc_ini, i, j)), 1);
+    SUM2(sums[8], A2(surf, i, j) * (A4(tsn, i, j, 1, 2) * A2(sshn, i, j) - A2(ssh_sc_loc_ini, i, j)), 1);
+    SUM3(sums[9], (A2(surf, i, j) * e3t_n(i, j, k) - A2(surf_ini, i, j) * A3(e3t_ini, i, j, k)) * TMASK(i, j, k));
+    SUM3(sums[10], (A2(surf, i, j) * e3t_n(i, j, k) * A4(tsn, i, j, k, 1) -
+                    A2(surf_ini, i, j) * A3(hc_loc_ini, i, j, k)) * TMASK(i, j, k));
+    SUM3(sums[11], (A2(surf, i, j) * e3t_n(i, j, k) * A4(tsn, i, j, k, 2) -
+                    A2(surf_ini, i, j) * A3(sc_loc_ini, i, j, k)) * TMASK(i, j, k));
+    SUM3(sums[12], A2(surf, i, j) * e3t_n(i, j, k) * TMASK(i, j, k));
+#undef SUM2
+#undef SUM3
+    free(tsn); free(tsb); free(surf); free(z2d); free(zwrk);
+    return 0;
+}
+
 int ora_abi_layout(size_t *out, int n)
 {
     size_t v[] = {sizeof(nemo_hip_domain_t), offsetof(nemo_hip_domain_t, rdt), offsetof(nemo_hip_domain_t, k_top),

@@ -229,6 +229,34 @@ class Oracle:
         st = self.L.ora_stp_ctl(C.c_int(kt), _p(sshn), _p(un), _p(tsn), _p(zmax), _p(loc), C.byref(kind))
         return st, zmax, loc, kind.value
 
+    def glob_sum(self, ptab, nk, kind=0):
+        """glob_sum (kind 0, tmask_i) / glob_sum_full (kind 1, tmask_h) of ptab(:,:,1:nk)"""
+        f = self.L.ora_glob_sum
+        f.restype = C.c_double
+        return float(f(_p(ptab), C.c_int(nk), C.c_int(kind)))
+
+    def dia_hsb_init(self, tsn, sshn):
+        """dia_hsb_init without a restart: (surf_tot, dict of the initial-state arrays)"""
+        d = self.dom
+        s2, s3 = (d.jpi, d.jpj), (d.jpi, d.jpj, d.jpk)
+        a = {n: np.zeros(s2, order="F") for n in ("surf_ini", "ssh_ini", "ssh_hc_loc_ini", "ssh_sc_loc_ini")}
+        a.update({n: np.zeros(s3, order="F") for n in ("e3t_ini", "hc_loc_ini", "sc_loc_ini")})
+        f = self.L.ora_dia_hsb_init
+        f.restype = C.c_double
+        tot = f(_p(tsn), _p(sshn), _p(a["surf_ini"]), _p(a["ssh_ini"]), _p(a["ssh_hc_loc_ini"]),
+                _p(a["ssh_sc_loc_ini"]), _p(a["e3t_ini"]), _p(a["hc_loc_ini"]), _p(a["sc_loc_ini"]))
+        return float(tot), a
+
+    def dia_hsb_sums(self, tsn, tsb, wn, sshn, emp, rnf, qsr, sbc_tsc, ini):
+        """the thirteen global sums of one dia_hsb call"""
+        sums = np.zeros(13)
+        q = qsr if qsr is not None else emp
+        _chk(self.L.ora_dia_hsb_sums(_p(tsn), _p(tsb), _p(wn), _p(sshn), _p(emp), _p(rnf), _p(q), _p(sbc_tsc),
+                                     _p(ini["surf_ini"]), _p(ini["ssh_ini"]), _p(ini["ssh_hc_loc_ini"]),
+                                     _p(ini["ssh_sc_loc_ini"]), _p(ini["e3t_ini"]), _p(ini["hc_loc_ini"]),
+                                     _p(ini["sc_loc_ini"]), _p(sums)), "dia_hsb_sums")
+        return sums
+
     def qsr_nksr(self, gdepw0, prldex=23.0, pqsr_frc=100.0):
         """trc_oce_ext_lev(prldex, pqsr_frc): nksr of the two-band tra_qsr"""
         return int(self.L.ora_qsr_nksr(C.c_double(prldex), C.c_double(pqsr_frc), _p(gdepw0)))
