@@ -57,7 +57,11 @@ def parse():
     ap.add_argument("--cpu-sample", default="512x512x75")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--ensemble", type=int, default=0, metavar="MEMBERS",
+    ap.add_argument("--cpu-workers", type=int, default=16, metavar="N",
+                    help="processes of the all-cores CPU baseline (one sub-domain each, as NEMO's MPI ranks; "
+                         "capped at the cores this process may use)")
+    ap.add_argument("--cpu-worker-sample", default="256x256x75")
+    ap.add_argument("--ensemble", type=int, default=24, metavar="MEMBERS",
                     help="also step an ensemble of MEMBERS domains (i x j drawn from 256x320 .. 768x640, 56 levels, "
                          "FP32 fused step, one library context each), eagerly and as one recorded HIP graph per "
                          "step: reported apart, never the headline value")
@@ -568,11 +572,13 @@ def kernel_times(h):
         "dyn_spg_ts": ["k_spg_cor1", "k_spg_cor2", "k_spg_init", "k_spg_p1a", "k_spg_p1b", "k_spg_s1", "k_spg_s2", "k_spg_s3",
                        "k_spg_p3a", "k_spg_p3b"],
         "tra_adv_trp": ["k_tra_adv_trp"],
-        "tra_adv_fct": ["k_fct_h", "k_fct_v", "k_fct_lim", "k_fct_beta", "k_fct_final", "k_fct_low", "k_fct_ad"],
+        "tra_adv_fct": ["k_fct_fused", "k_fct_cpt_fwd", "k_fct_h", "k_fct_v", "k_fct_lim", "k_fct_beta",
+                        "k_fct_final", "k_fct_low", "k_fct_ad"],
         "tra_ldf_iso": ["k_ldf_iso_coef", "k_tra_ldf_iso", "k_fill3", "k_copy3"], "tra_zdf": ["k_tra_zdf", "k_clamp_ring"],
         "lbc_lnk": ["k_lbc"],
         "step_dyn": ["k_step_dyn1", "k_step_dyn2"],
-        "step_tra": ["k_fct_h", "k_fct_v", "k_fct_lim", "k_fct_beta", "k_fct_final", "k_zero_ring", "k_tra_zdf"],
+        "step_tra": ["k_fct_fused", "k_fct_cpt_fwd", "k_fct_h", "k_fct_v", "k_fct_lim", "k_fct_beta", "k_fct_final",
+                     "k_zero_ring", "k_tra_zdf"],
         "div_hor": ["k_div_hor"], "ssh_nxt": ["k_ssh_nxt"], "wzv": ["k_wzv"], "ssh_swp": ["k_ssh_swp"],
         "tra_nxt": ["k_tra_nxt"], "dyn_nxt": ["k_dyn_nxt1", "k_dyn_nxt2"],
         "ldf_slp": ["k_slp_mxl", "k_slp_uv_raw", "k_slp_uv_filt", "k_slp_w_raw", "k_slp_w_filt"],
@@ -585,9 +591,50 @@ def kernel_times(h):
     return out
 
 
-def cpu_baseline(sample, steps, seed):
-    """The oracle (one core, -O2, no FMA) on a bounded sub-domain of the same
-    physics: cells/s of the same step sequence."""
+def cpu_info():
+    """CPU model and the number of cores this process may use."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count()
+    return model, cores, os.cpu_count()
+
+
+def _cpu_worker(args):
+    """One all-cores baseline process: the -O3 oracle stepping its own
+    sub-domain (NEMO runs one MPI rank per core; the halo exchanges it would
+    add are left out, in the CPU's favour)."""
+    sample, steps, seed = args
+    os.environ["NEMO_ORACLE_O3"] = "1"
+    for p in (ROOT, os.path.join(ROOT, "nemo-fmi-devel_amd")):
+        if p not in sys.path:
+            sys.path.insert(0, p)
+    return cpu_baseline(sample, steps, seed, timed_only=True)
+
+
+def cpu_allcores(sample, steps, seed, workers):
+    """Aggregate cells/s of `workers` processes, each the -O3 oracle on its
+    own sub-domain of the same physics, started together."""
+    import multiprocessing as mp
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(workers) as pool:
+        times = pool.map(_cpu_worker, [(sample, steps, seed + 100 * w) for w in range(workers)])
+    jpi, jpj, jpk = [int(x) for x in sample.split("x")]
+    cells = jpi * jpj * jpk * steps * workers
+    return cells / max(times), max(times)
+
+
+def cpu_baseline(sample, steps, seed, timed_only=False):
+    """The oracle (one core, no FMA) on a bounded sub-domain of the same
+    physics: cells/s of the same step sequence (timed_only: the seconds)."""
     import numpy as np
     from nemo_hip import configs
     from oracle import oracle as ora
@@ -620,10 +667,14 @@ def cpu_baseline(sample, steps, seed):
                       S["wslpi"], S["wslpj"], ah, akz)
         o.tra_zdf(kt, S["tsb"], tsa, S["avt"], S["avs"], 1, ah, akz)
     t = time.perf_counter() - t0
+    if timed_only:
+        return t
     cells = jpi * jpj * jpk
+    model, cores, ncpu = cpu_info()
     return {"value": cells * steps / t, "unit": "grid-cell updates/s", "cores": 1, "kind": "port",
             "sample": f"{jpi}x{jpj}x{jpk} sub-domain of the same seeded physics, {steps} steps, "
-                      f"{t:.1f} s, plain C oracle -O2 -ffp-contract=off, 1 thread"}
+                      f"{t:.1f} s, plain C oracle -O2 -ffp-contract=off, 1 thread",
+            "cpu_model": model, "cores_available": cores, "nproc": ncpu}
 
 
 TRAFFIC_TABLE = os.path.join(ROOT, "profiles", "hbm_traffic.md")
@@ -668,6 +719,34 @@ def stream_copy(mib=2048, reps=10):
     del a, b
     torch.cuda.empty_cache()
     return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
+def stream_copy_hand(h, mib=2048, reps=10):
+    """The same copy with the library's hand-written streaming kernel
+    (16-byte loads and stores, grid-stride), best over a few grid sizes;
+    timed with HIP events on the library stream."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    ext = torch.cuda.ExternalStream(h.stream())
+    best = 0.0
+    for blocks in (1024, 2048, 4096, 8192):
+        with torch.cuda.stream(ext):
+            for _ in range(2):
+                h.stream_copy(a, b, n * 4, blocks)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(ext)
+            for _ in range(reps):
+                h.stream_copy(a, b, n * 4, blocks)
+            e1.record(ext)
+            e1.synchronize()
+        best = max(best, 2 * n * 4 / (e0.elapsed_time(e1) / reps * 1e-3) / 1e9)
+    assert torch.equal(a, b)
+    del a, b
+    torch.cuda.empty_cache()
+    return best
 
 
 def copy_calibration(mib):
@@ -766,6 +845,7 @@ def main():
                 "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": alg_bytes / (avg_ms * 1e-3) / HBM_PEAK,
                 "traffic": traffic_per_launch(kname, head, jpi, jpj, jpk), "avg_launch_ms": avg_ms,
                 "launches_timed": live["launches"], "algorithmic_bytes_per_launch": alg_bytes}
+        roof["traffic_over_algorithmic"] = roof["traffic"] / alg_bytes if roof["traffic"] else None
     host_ms = None
     if a.host_mode:
         host_ms = time_host_mode(h, d, S, a.host_mode)
@@ -788,7 +868,8 @@ def main():
         log(f"ensemble of {a.ensemble} (FP32 fused, {e_cells:.3e} cells): {e_ms:.1f} ms/step call by call, "
             f"{g_ms:.1f} ms/step from one graph, {e_cells / (g_ms * 1e-3):.3e} cell-updates/s")
     copy_gbs = stream_copy()
-    log(f"device copy (2 GiB, read + write): {copy_gbs:.0f} GB/s")
+    copy_hand = stream_copy_hand(h)
+    log(f"device copy (2 GiB, read + write): torch {copy_gbs:.0f} GB/s, hand-written kernel {copy_hand:.0f} GB/s")
     kt_, kb_ = d.arrays["k_top"], d.arrays["k_bot"]
     wet_cells = int(((kb_ - kt_ + 1) * (kt_ > 0)).sum())
     cpu = None
@@ -796,6 +877,16 @@ def main():
         t0 = time.perf_counter()
         cpu = cpu_baseline(a.cpu_sample, a.cpu_steps, a.seed)
         log(f"cpu baseline {cpu['value']:.3e} cells/s ({time.perf_counter() - t0:.1f} s incl. setup)")
+        nw = max(1, min(a.cpu_workers, cpu["cores_available"] or 1))
+        if nw > 1:
+            t0 = time.perf_counter()
+            v, tmax = cpu_allcores(a.cpu_worker_sample, a.cpu_steps, a.seed, nw)
+            cpu["all_cores"] = {"value": v, "unit": "grid-cell updates/s", "cores": nw, "kind": "port",
+                                "sample": f"{nw} processes, each a {a.cpu_worker_sample} sub-domain of the same "
+                                          f"physics for {a.cpu_steps} steps, plain C oracle -O3 "
+                                          f"-ffp-contract=off, slowest process {tmax:.1f} s (one process per "
+                                          f"core as NEMO's MPI ranks, halo exchanges not counted)"}
+            log(f"cpu baseline, {nw} processes: {v:.3e} cells/s ({time.perf_counter() - t0:.1f} s incl. setup)")
     h_ms = res[head]["ms_per_step"]
     line = {
         "metric": METRIC,
@@ -812,6 +903,7 @@ def main():
         "wet_cells": wet_cells,
         "wet_cell_updates_per_s": wet_cells / (h_ms * 1e-3),
         "stream_copy_GBps": copy_gbs,
+        "stream_copy_hand_GBps": copy_hand,
         "host_mode_fp64_ms_per_step": host_ms,
         "ensemble": ens,
         "adjoint": adj,
@@ -829,9 +921,11 @@ def main():
 
 
 # Algorithmic bytes per launch of each library kernel, for the roofline of the
-# dominant one and for the traffic table: every 3D array the kernel reads or
-# writes, its inputs and outputs between kernels included, counts once per
-# interior cell at storage width w (masks, 1-D e3 tables and 2D metrics 0);
+# dominant one and for the traffic table: every 3D array of the reference
+# routine's inputs and outputs that the kernel reads or writes counts once per
+# interior cell at storage width w; arrays that only pass between the kernels
+# of one routine (FCT's zwi, zwx, zwy, zwz, betas and compact-scheme sweep,
+# the Thomas pivots) count 0, and so do masks, 1-D e3 tables and 2D metrics;
 # the barotropic sub-steps count their 2D FP64 arrays.  Arguments: w, jpi,
 # jpj, jpk and the form ("fp64", "fp32", "fp64_fused", "fp32_fused").
 def _b(n_arrays_interior):
@@ -842,32 +936,29 @@ def _fused(f):
     return f.endswith("_fused")
 
 
-def _iso_nt(f):
-    # tra_ldf_iso covers a tracer pair per launch in both storage forms
-    return 2
-
-
 KERNEL_BYTES = {
     "k_tra_zdf": _b(8),          # r avt ah_wslp2 tb*2 ta*2, w ta*2
-    # r ahtu ahtv uslp vslp wslpi wslpj ah_wslp2 akz + (tb, ta) per tracer, w ta per tracer
-    "k_tra_ldf_iso": lambda w, i, j, k, f: (8 + 3 * _iso_nt(f)) * w * (i - 2) * (j - 2) * (k - 1),
+    # r ahtu ahtv uslp vslp wslpi wslpj ah_wslp2 akz tb*2 ta*2, w ta*2
+    "k_tra_ldf_iso": _b(14),
     "k_ldf_iso_coef": _b(5),     # r ahtu ahtv wslpi wslpj, w ah_wslp2
     "k_dyn_zdf": _b(7),          # r ub vb ua va avm, w ua va
     "k_dyn_vor": _b(6),          # r un vn ua va, w ua va
     "k_dyn_keg_zad": _b(7),      # r un vn wn ua va, w ua va
-    "k_fct_final": _b(7),        # r zwx zwy zwz zbetup zbetdo ta, w ta
-    "k_fct_beta": _b(7),         # r tb zwi zwx zwy zwz, w zbetup zbetdo
-    "k_fct_lim": _b(14),         # two tracers: r tb zwi zwx zwy zwz ta, w ta
-    "k_fct_ad": _b(8),           # r pun pvn pwn tb tn, w zwx zwy zwz
+    # FCT, two tracers per launch: r pun pvn pwn (un vn wn in the fused step)
+    # tb*2 tn*2 ta*2 (no ta read in the fused step: the trend starts at zero),
+    # w ta*2
+    "k_fct_fused": lambda w, i, j, k, f: (9 if _fused(f) else 11) * w * (i - 2) * (j - 2) * (k - 1),
+    "k_fct_cpt_fwd": _b(2),      # r tn*2 (its sweep is a temporary)
+    "k_fct_h": lambda w, i, j, k, f: (9 if _fused(f) else 11) * w * (i - 2) * (j - 2) * (k - 1),
+    "k_fct_v": _b(5),            # r pwn tb*2 tn*2
+    "k_fct_lim": _b(6),          # two tracers: r tb ta, w ta
+    "k_fct_final": _b(2),        # r ta, w ta
+    "k_fct_beta": _b(1),         # r tb
+    "k_fct_ad": _b(5),           # r pun pvn pwn tb tn
     "k_fct_low": _b(7),          # r pun pvn pwn tb ta, w ta zwi
     "k_spg_p1a": _b(6),          # r ua va un vn, w ua va
     "k_spg_p3b": _b(8),          # r ua va un vn, w ua va un vn
     "k_tra_adv_trp": lambda w, i, j, k, f: 6 * w * i * j * (k - 1),   # r un vn wn, w zun zvn zwn
-    # FCT flux pass, two tracers per launch: r pun pvn pwn tb*2 tn*2 ta*2,
-    # w ta*2 zwi*2 zwx*2 zwy*2; in the fused step the transports are formed
-    # from un vn wn and the trend starts from zero (no ta read)
-    "k_fct_h": lambda w, i, j, k, f: (15 if _fused(f) else 17) * w * (i - 2) * (j - 2) * (k - 1),
-    "k_fct_v": _b(7),            # r pwn tb*2 tn*2, w zwz*2
     # barotropic sub-step kernels (2D, FP64 in both forms; ENE scheme)
     # S3: reads zwx zwy zwz zCdU_u zCdU_v un_e vn_e zu_frc zv_frc zsshp2_e
     # ub_e(-1) vb_e(-1) hu_n hv_n r1_hu_n r1_hv_n r1_e1u r1_e2v e2u e1v r1_e2u

@@ -34,6 +34,7 @@
 #ifndef NEMO_HIP_H
 #define NEMO_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -281,6 +282,18 @@ int nemo_hip_lbc_lnk(double *ptab, int nk, char cd_nat, double psgn, double pval
 int nemo_hip_tra_adv_trp(const double *un, const double *vn, const double *wn,
                          double *zun, double *zvn, double *zwn);
 
+/* tra_adv, TRA/traadv.F90:77-175 with the FCT scheme (nn_fct_h, nn_fct_v in
+   {2,4}): r2dt of kt (lines 95-97, kept by the library as the module keeps
+   it), the effective transports of un, vn, wn (lines 100-124, formed where
+   the fluxes use them, not stored) and tra_adv_fct (line 150) adding the
+   trend of the kjpt tracers to pta */
+int nemo_hip_tra_adv(int kt, const double *un, const double *vn, const double *wn,
+                     const double *ptb, const double *ptn, double *pta, int kjpt,
+                     int kn_fct_h, int kn_fct_v);
+int nemo_hip_tra_adv_sp(int kt, const float *un, const float *vn, const float *wn,
+                        const float *ptb, const float *ptn, float *pta, int kjpt,
+                        int kn_fct_h, int kn_fct_v);
+
 /* FCT advection, TRA/traadv_fct.F90:54-80 (kn_fct_h, kn_fct_v in {2,4}) */
 int nemo_hip_tra_adv_fct(int kt, int kit000, const char *cdtype, double p2dt,
                          const double *pun, const double *pvn, const double *pwn,
@@ -485,6 +498,22 @@ int nemo_hip_stp_ctl(int kt, const double *sshn, const double *un, const double 
                      double zmax[6], int32_t loc[11], int *kindic);
 int nemo_hip_stp_ctl_sp(int kt, const double *sshn, const float *un, const float *tsn,
                         double zmax[6], int32_t loc[11], int *kindic);
+
+/* A streaming device-to-device copy of bytes (a multiple of 16, both
+   addresses 16-byte aligned) with `blocks` workgroups, on the library stream
+   (device pointers, asynchronous).  No counterpart in the reference: the
+   benchmark's measured bandwidth ceiling for a streaming kernel. */
+int nemo_hip_stream_copy(const void *src, void *dst, size_t bytes, int blocks);
+
+/* Tuning switches (no counterpart in the reference): the kernel variants
+   and tile shapes a caller or the project's autotuner may choose, process
+   wide.  Every choice gives the same results bit for bit.  Keys:
+     "fct_fused"  1 (default): tra_adv_fct on closed domains as one kernel
+                  per tile and tracer (intermediates on chip); 0: the
+                  flux / limiter chain through HBM scratch.
+   Unknown keys and values are errors. */
+int nemo_hip_tune_set(const char *key, int value);
+int nemo_hip_tune_get(const char *key, int *value);
 
 /* glob_sum and glob_sum_full (lib_fortran_generic.h90 FUNCTION_GLOBSUM,
    lib_fortran.F90:300-332 DDPDD): the compensated (double-double) sum of

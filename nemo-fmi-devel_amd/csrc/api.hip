@@ -94,6 +94,10 @@ int set_error(int code, const char *fmt, ...)
 hipStream_t stream() { return g_stream; }
 const DevDomain &dom() { return g_c->dom; }
 bool capturing() { return g_capturing; }
+// tuning switches, process-wide; names and defaults
+static const char *g_tune_names[T_NTUNE] = {"fct_fused"};
+static int g_tune[T_NTUNE] = {1};
+int tune(Tune which) { return g_tune[which]; }
 int *spg_rot_u() { return g_c->spg_rot_u; }
 int *spg_rot_s() { return g_c->spg_rot_s; }
 void spg_ts_reset()
@@ -885,6 +889,26 @@ static int do_tra_adv_fct(int kt, int kit000, const char *cdtype, double p2dt, c
     const T *u = S.in(pun, n3), *v = S.in(pvn, n3), *w = S.in(pwn, n3), *tb = S.in(ptb, n4), *tn = S.in(ptn, n4);
     T *ta = S.inout(pta, n4);
     rc = S.status ? S.status : launch_tra_adv_fct<T>(p2dt, u, v, w, tb, tn, ta, kjpt, kn_fct_h, kn_fct_v);
+    return S.finish(rc);
+}
+
+// tra_adv: r2dt of kt, transports formed where used, FCT
+template <typename T>
+static int do_tra_adv(int kt, const T *un, const T *vn, const T *wn, const T *ptb, const T *ptn, T *pta, int kjpt,
+                      int kn_fct_h, int kn_fct_v)
+{
+    NEED_DOMAIN();
+    NEED_STEADY(kt);
+    NEED_PTR(un); NEED_PTR(vn); NEED_PTR(wn); NEED_PTR(ptb); NEED_PTR(ptn); NEED_PTR(pta);
+    if (kjpt < 1) return set_error(NEMO_HIP_ERR_ARG, "kjpt = %d", kjpt);
+    if ((kn_fct_h != 2 && kn_fct_h != 4) || (kn_fct_v != 2 && kn_fct_v != 4))
+        return set_error(NEMO_HIP_ERR_UNSUPPORTED, "tra_adv: kn_fct_h = %d, kn_fct_v = %d (2 or 4)", kn_fct_h, kn_fct_v);
+    const double r2dt = r2dt_update(g_c->r2dt_tra, kt);
+    const size_t n3 = g_c->dom.n3, n4 = n3 * (size_t)kjpt;
+    Stager S;
+    const T *u = S.in(un, n3), *v = S.in(vn, n3), *w = S.in(wn, n3), *tb = S.in(ptb, n4), *tn = S.in(ptn, n4);
+    T *ta = S.inout(pta, n4);
+    int rc = S.status ? S.status : launch_tra_adv<T>(r2dt, u, v, w, tb, tn, ta, kjpt, kn_fct_h, kn_fct_v);
     return S.finish(rc);
 }
 
@@ -2057,6 +2081,50 @@ int nemo_hip_stp_ctl(int kt, const double *sshn, const double *un, const double 
                      int32_t loc[11], int *kindic)
 {
     return do_stp_ctl<double>(kt, sshn, un, tsn, zmax, loc, kindic);
+}
+
+int nemo_hip_stream_copy(const void *src, void *dst, size_t bytes, int blocks)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    if (!src || !dst) return set_error(NEMO_HIP_ERR_ARG, "stream_copy: NULL pointer");
+    if (bytes % 16 || ((uintptr_t)src | (uintptr_t)dst) % 16)
+        return set_error(NEMO_HIP_ERR_ARG, "stream_copy: sizes and addresses must be multiples of 16 bytes");
+    if (blocks < 1 || blocks > 65536) return set_error(NEMO_HIP_ERR_ARG, "stream_copy: blocks = %d", blocks);
+    return launch_stream_copy(src, dst, bytes, blocks);
+}
+
+int nemo_hip_tune_set(const char *key, int value)
+{
+    if (!key) return set_error(NEMO_HIP_ERR_ARG, "tune_set: NULL key");
+    for (int q = 0; q < T_NTUNE; q++)
+        if (!strcmp(key, g_tune_names[q])) {
+            if (q == T_FCT_FUSED && value != 0 && value != 1)
+                return set_error(NEMO_HIP_ERR_ARG, "tune_set: fct_fused = %d (0 or 1)", value);
+            g_tune[q] = value;
+            return NEMO_HIP_OK;
+        }
+    return set_error(NEMO_HIP_ERR_ARG, "tune_set: unknown key %s", key);
+}
+int nemo_hip_tune_get(const char *key, int *value)
+{
+    if (!key || !value) return set_error(NEMO_HIP_ERR_ARG, "tune_get: NULL argument");
+    for (int q = 0; q < T_NTUNE; q++)
+        if (!strcmp(key, g_tune_names[q])) {
+            *value = g_tune[q];
+            return NEMO_HIP_OK;
+        }
+    return set_error(NEMO_HIP_ERR_ARG, "tune_get: unknown key %s", key);
+}
+
+int nemo_hip_tra_adv(int kt, const double *un, const double *vn, const double *wn, const double *ptb,
+                     const double *ptn, double *pta, int kjpt, int kn_fct_h, int kn_fct_v)
+{
+    return do_tra_adv<double>(kt, un, vn, wn, ptb, ptn, pta, kjpt, kn_fct_h, kn_fct_v);
+}
+int nemo_hip_tra_adv_sp(int kt, const float *un, const float *vn, const float *wn, const float *ptb,
+                        const float *ptn, float *pta, int kjpt, int kn_fct_h, int kn_fct_v)
+{
+    return do_tra_adv<float>(kt, un, vn, wn, ptb, ptn, pta, kjpt, kn_fct_h, kn_fct_v);
 }
 
 int nemo_hip_glob_sum(const double *ptab, int nk, int kind, double *psum)

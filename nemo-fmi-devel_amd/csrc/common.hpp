@@ -277,6 +277,9 @@ bool capturing();              // a graph is being recorded on the library strea
 int *spg_rot_u();              // dyn_spg_ts level buffers of the current context
 int *spg_rot_s();
 bool have_domain();
+// tuning switches (nemo_hip_tune_set): kernel variants and tile shapes
+enum Tune { T_FCT_FUSED, T_NTUNE };
+int tune(Tune which);
 // per-routine timing: when this launch is timed, returns true with the two
 // events its dispatch packet records at kernel start and end
 bool timing_pair(const char *name, const char *kernel, hipEvent_t *start, hipEvent_t *stop);

@@ -275,3 +275,35 @@ int launch_dia_hsb_init(const double *sshn, const T *tsn, double *surf_ini, doub
                                         double *, double *, double *);
 HSB_INST(double)
 HSB_INST(float)
+
+// ---------------------------------------------------------------------------
+// a streaming copy of n 16-byte words, the measured bandwidth ceiling a
+// streaming kernel on this device meets (benchmark utility): each thread
+// moves four words per pass, loads issued before stores, grid-stride
+// ---------------------------------------------------------------------------
+KERNEL void k_stream_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * NH_BLOCK * 4;
+    for (size_t q = (size_t)blockIdx.x * NH_BLOCK * 4 + threadIdx.x; q < n; q += stride) {
+        uint4 a[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const size_t e = q + (size_t)u * NH_BLOCK;
+            a[u] = e < n ? src[e] : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const size_t e = q + (size_t)u * NH_BLOCK;
+            if (e < n) dst[e] = a[u];
+        }
+    }
+}
+
+int launch_stream_copy(const void *src, void *dst, size_t bytes, int blocks)
+{
+    Grid2 g;
+    g.block = dim3(NH_BLOCK, 1, 1);
+    g.grid = dim3((unsigned)blocks, 1, 1);
+    LAUNCH("stream_copy", k_stream_copy, g, (const uint4 *)src, (uint4 *)dst, bytes / 16);
+    return 0;
+}

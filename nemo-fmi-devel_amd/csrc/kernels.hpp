@@ -12,6 +12,9 @@ template <typename T>
 int launch_tra_adv_fct(double p2dt, const T *pun, const T *pvn, const T *pwn, const T *ptb, const T *ptn, T *pta,
                        int kjpt, int kn_fct_h, int kn_fct_v);
 int launch_interp_4th_cpt(const double *tin, double *tout);
+template <typename T>
+int launch_tra_adv(double p2dt, const T *un, const T *vn, const T *wn, const T *ptb, const T *ptn, T *pta, int kjpt,
+                   int kn_fct_h, int kn_fct_v);
 template <typename T> int launch_tra_ldf_lap(const T *pahu, const T *pahv, const T *ptb, T *pta, int kjpt, int kpass);
 template <typename T>
 int launch_tra_ldf_iso(int first, double z2dt, const T *pahu, const T *pahv, const T *ptb, const T *ptbb, T *pta,
@@ -100,3 +103,4 @@ template <typename T>
 int launch_dia_hsb_init(const double *sshn, const T *tsn, double *surf_ini, double *ssh_ini, double *ssh_hc,
                         double *ssh_sc, double *e3t_ini, double *hc_ini, double *sc_ini);
 size_t dia_hsb_scratch_bytes();
+int launch_stream_copy(const void *src, void *dst, size_t bytes, int blocks);

@@ -21,9 +21,8 @@
 // betas/limiter and the tridiagonal eliminations run in FP64 in both forms.
 #include "common.hpp"
 #include "kernels.hpp"
+#include "fct.hpp"
 
-// deepest column the data-independent pivot table of interp_4th_cpt covers
-#define STEP_MAXK 160
 // levels per thread of k_fct_beta: a short k-march reuses the vertical
 // neighbours the thread has loaded, where one level per thread re-fetches
 // them through the fabric.  It pays for the FP64 betas; for FP32 storage one
@@ -309,16 +308,6 @@ KERNEL void k_fct_beta(DevDomain d, double p2dt, const T * __restrict__ pbef, co
     }
 }
 
-// limited fluxes of nonosc (traadv_fct.F90:404-422), FP64
-template <typename T>
-__device__ __forceinline__ double lim_face(double flux, double bdo_here, double bup_next, double bup_here,
-                                           double bdo_next)
-{
-    const double za = fmin2(fmin2(1.0, bdo_here), bup_next);
-    const double zb = fmin2(fmin2(1.0, bup_here), bdo_next);
-    const double zc = (0.5 + nsign(0.5, flux));
-    return flux * (zc * za + (1.0 - zc) * zb);
-}
 // paa at (p,q) after nonosc's limiting and its lbc_lnk('U')
 template <typename T>
 __device__ __forceinline__ double lim_x(const DevDomain &d, const T *paa, const T *bup, const T *bdo, int p, int q, int k)
@@ -1425,80 +1414,6 @@ template int launch_tra_zdf_imp<float>(double, int, const float *, float *, int,
 // p(k) = 4 - 1/p(k-1) from p(3) = 4, every other row has pivot 1, so they
 // come from a host table.
 // ===========================================================================
-template <typename T>
-struct PivotTab {
-    T p[STEP_MAXK + 2];
-};
-
-// (k is uniform across a wave: the level tests select, they do not branch,
-// so the loads of a stencil can be issued together)
-template <typename T>
-__device__ __forceinline__ T trp_u(const DevDomain &d, const T *un, int p, int q, int k)
-{
-    const Metrics<T> &m = met<T>(d);
-    const size_t pq = IJ(p, q);
-    const int kc = k < d.jpk ? k : d.jpk;
-    const T v = m.e2u[pq] * E3(m.e3u, pq, kc) * A3(un, p, q, kc);
-    return k >= d.jpk ? T(0) : v;
-}
-template <typename T>
-__device__ __forceinline__ T trp_v(const DevDomain &d, const T *vn, int p, int q, int k)
-{
-    const Metrics<T> &m = met<T>(d);
-    const size_t pq = IJ(p, q);
-    const int kc = k < d.jpk ? k : d.jpk;
-    const T v = m.e1v[pq] * E3(m.e3v, pq, kc) * A3(vn, p, q, kc);
-    return k >= d.jpk ? T(0) : v;
-}
-template <typename T>
-__device__ __forceinline__ T trp_w(const DevDomain &d, const T *wn, int p, int q, int k)
-{
-    const int kc = k < d.jpk ? k : d.jpk;
-    const T v = met<T>(d).e1e2t[IJ(p, q)] * A3(wn, p, q, kc);
-    return k >= d.jpk ? T(0) : v;
-}
-// interp_4th_cpt pivot of row k in a column whose last wet level is ikb
-template <typename T>
-__device__ __forceinline__ T cpt_piv(const PivotTab<T> &pt, int k, int ikb)
-{
-    return (k >= 3 && k < ikb) ? pt.p[k] : T(1);
-}
-
-template <typename T>
-static PivotTab<T> cpt_pivots(int jpk);
-
-// transports: TRP = 0 reads the arrays of tra_adv (pun, pvn, pwn, zero at
-// jpk); TRP = 1 forms them from un, vn, wn where used (traadv.F90:100-124)
-template <int TRP, typename T>
-__device__ __forceinline__ T tr_u(const DevDomain &d, const T *pu, int p, int q, int k)
-{
-    if (TRP) return trp_u(d, pu, p, q, k);
-    return A3(pu, p, q, k);
-}
-template <int TRP, typename T>
-__device__ __forceinline__ T tr_v(const DevDomain &d, const T *pv, int p, int q, int k)
-{
-    if (TRP) return trp_v(d, pv, p, q, k);
-    return A3(pv, p, q, k);
-}
-template <int TRP, typename T>
-__device__ __forceinline__ T tr_w(const DevDomain &d, const T *pw, int p, int q, int k)
-{
-    if (TRP) return trp_w(d, pw, p, q, k);
-    return A3(pw, p, q, k);
-}
-template <int TRP, typename T>
-__device__ __forceinline__ T tp_upz(const DevDomain &d, const T *pw, const T *tb, int i, int j, int k)
-{
-    // k == 1: linear free surface flux; k >= jpk: zero (selects on the uniform k)
-    const int kc = k < d.jpk ? k : d.jpk, km1 = kc > 1 ? kc - 1 : 1;
-    const T w = tr_w<TRP>(d, pw, i, j, kc);
-    const T tbk = A3(tb, i, j, kc);
-    const T zfp = w + fabs(w), zfm = w - fabs(w);
-    const T gen = T(0.5) * (zfp * tbk + zfm * A3(tb, i, j, km1)) * WMASK(T, i, j, kc);
-    const T sfc = w * tbk;
-    return k == 1 ? sfc : (k >= d.jpk ? T(0) : gen);
-}
 
 // per cell (one thread per (i,j,k), k = blockIdx.z + 1): the horizontal
 // fluxes of the (i+1/2, j) and (i, j+1/2) faces (upstream and antidiffusive,
@@ -1647,6 +1562,14 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
     for (int t0 = 0; t0 < kjpt; t0 += 2) {
         const int nt = kjpt - t0 < 2 ? kjpt - t0 : 2;
         const size_t o = (size_t)t0 * n3;
+        if (!d.iper && !d.jper && nh::tune(nh::T_FCT_FUSED)) {
+            // closed domain: the whole chain in one kernel per tile and tracer
+            // (zwz's slots hold the compact scheme's forward elimination)
+            if ((st = launch_fct_fused<TRP, T>(d, pt, p2dt, nt, pu, pv, pw, ptb + o, ptn + o, pta + o, kn_fct_h,
+                                               kn_fct_v, ta0, zwz, nm)))
+                return st;
+            continue;
+        }
         Grid2 g3 = grid2(d.jpi - 1, d.jpj - 1);
         g3.grid.z = (unsigned)(d.jpk - 1);
 #define FH_LAUNCH2(H, Z, N) \
@@ -1701,7 +1624,7 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
 
 
 template <typename T>
-static PivotTab<T> cpt_pivots(int jpk)
+PivotTab<T> cpt_pivots(int jpk)
 {
     // interp_4th_cpt rows 3.. of a wet column: zwd = 3*1+1, zwi = zws = 1,
     // zwt(k) = zwd - zwi*zws(k-1)/zwt(k-1) with row 2 an identity row
@@ -1718,6 +1641,25 @@ static PivotTab<T> cpt_pivots(int jpk)
     }
     return pt;
 }
+
+// tra_adv (TRA/traadv.F90:77-175, FCT): the transports of un, vn, wn formed
+// where used and tra_adv_fct added to pta
+template <typename T>
+int launch_tra_adv(double p2dt, const T *un, const T *vn, const T *wn, const T *ptb, const T *ptn, T *pta, int kjpt,
+                   int kn_fct_h, int kn_fct_v)
+{
+    const DevDomain &d = nh::dom();
+    if (d.jpk <= STEP_MAXK)
+        return launch_fct_pairs<1, T>(d, p2dt, un, vn, wn, ptb, ptn, pta, kjpt, kn_fct_h, kn_fct_v, 0);
+    T *zun = (T *)nh::scratch3(nh::S_ZUN), *zvn = (T *)nh::scratch3(nh::S_ZVN), *zwn = (T *)nh::scratch3(nh::S_ZWN);
+    int st = launch_tra_adv_trp<T>(un, vn, wn, zun, zvn, zwn);
+    if (st) return st;
+    return launch_tra_adv_fct<T>(p2dt, zun, zvn, zwn, ptb, ptn, pta, kjpt, kn_fct_h, kn_fct_v);
+}
+template int launch_tra_adv<double>(double, const double *, const double *, const double *, const double *,
+                                    const double *, double *, int, int, int);
+template int launch_tra_adv<float>(double, const float *, const float *, const float *, const float *,
+                                   const float *, float *, int, int, int);
 
 template <typename T>
 int launch_step_tra(double p2dt, double z2dt, double p2dt_zdf, const T *un, const T *vn, const T *wn, const T *tsb,
@@ -1740,6 +1682,8 @@ int launch_step_tra(double p2dt, double z2dt, double p2dt_zdf, const T *un, cons
     LAUNCH("step_tra", (k_tra_zdf<T, 2>), gi, d, p2dt_zdf, tsb, tsa, 0, avt, 1, add, zwt, 1);
     return 0;
 }
+template PivotTab<double> cpt_pivots<double>(int);
+template PivotTab<float> cpt_pivots<float>(int);
 template int launch_step_tra<double>(double, double, double, const double *, const double *, const double *,
                                      const double *, const double *, double *, int, int, const double *,
                                      const double *, const double *, const double *, const double *, const double *,

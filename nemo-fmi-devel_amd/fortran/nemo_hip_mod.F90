@@ -21,6 +21,10 @@ MODULE nemo_hip_mod
    !!   tra_sbc        TRA/trasbc.F90:62            stp_ctl        stpctl.F90:55
    !!   zdf_mxl        ZDF/zdfmxl.F90:64            ldf_slp        LDF/ldfslp.F90:83
    !!   rst_write      IOM/restart.F90:133          rst_read       IOM/restart.F90:240
+   !! and the module-global step routines that dispatch to them:
+   !!   tra_adv        TRA/traadv.F90:77            tra_ldf        TRA/traldf.F90:49
+   !!   tra_zdf        TRA/trazdf.F90:46            dyn_adv        DYN/dynadv.F90:55
+   !!   dyn_spg        DYN/dynspg.F90:55
    !! The arrays those routines take from NEMO modules (oce, zdf_oce, ldfslp,
    !! sbc_oce, zdfdrg, dynspg_ts, dom_oce) are bound once, after allocation, with the
    !! nemo_hip_bind_* routines: module arrays never move, so their addresses
@@ -42,6 +46,9 @@ MODULE nemo_hip_mod
    PUBLIC :: eos_insitu, eos_insitu_pot, rab_3d, bn2, hpg_zco, dyn_ldf_lap, tra_sbc, tra_qsr, stp_ctl
    PUBLIC :: nemo_hip_bind_mxl, zdf_mxl, ldf_slp
    PUBLIC :: nemo_hip_rst_t, rst_write, rst_read
+   PUBLIC :: nemo_hip_bind_tra, tra_adv, tra_ldf, tra_zdf, dyn_adv, dyn_spg
+   !! ldftra.F90:79-80: the operators tra_ldf dispatches to
+   INTEGER, PARAMETER, PUBLIC :: np_lap = 10, np_lap_i = 11
    PUBLIC :: nemo_hip_domain_t, nemo_hip_spg_ts_t
    PUBLIC :: nemo_hip_field_alloc, nemo_hip_field_free, nemo_hip_field_push, nemo_hip_field_pull
 
@@ -106,6 +113,9 @@ MODULE nemo_hip_mod
    TYPE(C_PTR), SAVE :: p_qns = C_NULL_PTR, p_qsr = C_NULL_PTR, p_sfx = C_NULL_PTR
    TYPE(C_PTR), SAVE :: p_sbc_tsc = C_NULL_PTR, p_sbc_tsc_b = C_NULL_PTR
    INTEGER, SAVE :: l_ldfslp = 0
+   !! the traadv / ldftra namelist choices and arrays tra_adv and tra_ldf use
+   INTEGER, SAVE :: m_nn_fct_h = 4, m_nn_fct_v = 4, m_nldf_tra = np_lap, m_nit000 = 1
+   TYPE(C_PTR), SAVE :: p_ahtu = C_NULL_PTR, p_ahtv = C_NULL_PTR
    TYPE(nemo_hip_spg_ts_t), SAVE :: spg_state
 
    INTERFACE
@@ -151,6 +161,22 @@ MODULE nemo_hip_mod
          CHARACTER(KIND=C_CHAR), DIMENSION(*) :: cdtype
          REAL(C_DOUBLE), VALUE :: p2dt
          TYPE(C_PTR), VALUE :: pun, pvn, pwn, ptb, ptn, pta
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_tra_adv(kt, un, vn, wn, ptb, ptn, pta, kjpt, kn_fct_h, kn_fct_v)   &
+         &                              BIND(C, name="nemo_hip_tra_adv")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt, kjpt, kn_fct_h, kn_fct_v
+         TYPE(C_PTR), VALUE :: un, vn, wn, ptb, ptn, pta
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_tra_zdf(kt, tsb, tsa, avt, avs, l_ldfslp, ah_wslp2, akz) BIND(C, name="nemo_hip_tra_zdf")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt, l_ldfslp
+         TYPE(C_PTR), VALUE :: tsb, tsa, avt, avs, ah_wslp2, akz
+      END FUNCTION
+      INTEGER(C_INT) FUNCTION c_dyn_adv(kt, un, vn, wn, ua, va) BIND(C, name="nemo_hip_dyn_adv")
+         IMPORT :: C_INT, C_PTR
+         INTEGER(C_INT), VALUE :: kt
+         TYPE(C_PTR), VALUE :: un, vn, wn, ua, va
       END FUNCTION
       INTEGER(C_INT) FUNCTION c_interp_4th_cpt(pin, pout) BIND(C, name="nemo_hip_interp_4th_cpt")
          IMPORT :: C_INT, C_PTR
@@ -364,7 +390,19 @@ CONTAINS
       !! sdom: sizes, switches and C_LOC of the dom_oce arrays (see nemo_hip.h)
       TYPE(nemo_hip_domain_t), INTENT(in) :: sdom
       CALL check( c_domain_register( sdom ), 'nemo_hip_domain_register' )
+      m_nit000 = sdom%nit000
    END SUBROUTINE nemo_hip_domain
+
+   SUBROUTINE nemo_hip_bind_tra( kn_fct_h, kn_fct_v, knldf_tra, pahtu, pahtv )
+      !! the traadv namelist orders (nn_fct_h, nn_fct_v), ldftra's operator
+      !! (nldf_tra: np_lap or np_lap_i) and diffusivities (ahtu, ahtv) that
+      !! tra_adv(kt) and tra_ldf(kt) use; the tracers are those bound with
+      !! nemo_hip_bind_nxt (tsb, tsn, tsa), the velocities with nemo_hip_bind_dyn
+      INTEGER, INTENT(in) :: kn_fct_h, kn_fct_v, knldf_tra
+      REAL(wp), DIMENSION(:,:,:), TARGET, CONTIGUOUS, INTENT(in) :: pahtu, pahtv
+      m_nn_fct_h = kn_fct_h ; m_nn_fct_v = kn_fct_v ; m_nldf_tra = knldf_tra
+      p_ahtu = C_LOC( pahtu ) ; p_ahtv = C_LOC( pahtv )
+   END SUBROUTINE nemo_hip_bind_tra
 
    !!----------------------------------------------------------------------
    !! resident fields (device mode): a device mirror per prognostic array,
@@ -476,6 +514,52 @@ CONTAINS
    !!----------------------------------------------------------------------
    !! drop-in routines
    !!----------------------------------------------------------------------
+   SUBROUTINE tra_adv( kt )
+      !! TRA/traadv.F90:77 with the FCT scheme: r2dt, the effective transports
+      !! (formed on the device where the fluxes use them) and tra_adv_fct on
+      !! (T,S), added to tsa
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_tra_adv( INT(kt,C_INT), p_un, p_vn, p_wn, p_tsb, p_tsn, p_tsa, 2_C_INT,                    &
+         &                   INT(m_nn_fct_h,C_INT), INT(m_nn_fct_v,C_INT) ), 'tra_adv' )
+   END SUBROUTINE tra_adv
+
+   SUBROUTINE tra_ldf( kt )
+      !! TRA/traldf.F90:49: the laplacian operator nldf_tra selects, kpass 1
+      INTEGER, INTENT(in) ::   kt
+      SELECT CASE( m_nldf_tra )
+      CASE( np_lap )
+         CALL check( c_tra_ldf_lap( INT(kt,C_INT), INT(m_nit000,C_INT), cstr('TRA'), p_ahtu, p_ahtv,             &
+            &        p_ahtu, p_ahtu, p_ahtu, p_ahtu, p_tsb, p_tsa, 2_C_INT, 1_C_INT ), 'tra_ldf' )
+      CASE( np_lap_i )
+         CALL check( c_tra_ldf_iso( INT(kt,C_INT), INT(m_nit000,C_INT), cstr('TRA'), p_ahtu, p_ahtv,             &
+            &        p_ahtu, p_ahtu, p_ahtu, p_ahtu, p_tsb, p_tsb, p_tsa, 2_C_INT, 1_C_INT,                         &
+            &        p_uslp, p_vslp, p_wslpi, p_wslpj, p_ah_wslp2, p_akz ), 'tra_ldf' )
+      CASE DEFAULT
+         CALL check( 2_C_INT, 'tra_ldf: only np_lap and np_lap_i are on the device' )
+      END SELECT
+   END SUBROUTINE tra_ldf
+
+   SUBROUTINE tra_zdf( kt )
+      !! TRA/trazdf.F90:46: r2dt, tra_zdf_imp on (T,S) and the reset of negative salinity
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_tra_zdf( INT(kt,C_INT), p_tsb, p_tsa, p_avt, p_avs, INT(l_ldfslp,C_INT), p_ah_wslp2, p_akz ),   &
+         &        'tra_zdf' )
+   END SUBROUTINE tra_zdf
+
+   SUBROUTINE dyn_adv( kt )
+      !! DYN/dynadv.F90:55: vector form (dyn_keg + dyn_zad) or flux form
+      !! (dyn_adv_cen2), as the registered n_dynadv and nn_dynkeg select
+      INTEGER, INTENT(in) ::   kt
+      CALL check( c_dyn_adv( INT(kt,C_INT), p_un, p_vn, p_wn, p_ua, p_va ), 'dyn_adv' )
+   END SUBROUTINE dyn_adv
+
+   SUBROUTINE dyn_spg( kt )
+      !! DYN/dynspg.F90:55 with ln_dynspg_ts and no atmospheric pressure, tide
+      !! or ice loading (the only trend it adds is dyn_spg_ts's)
+      INTEGER, INTENT(in) ::   kt
+      CALL dyn_spg_ts( kt )
+   END SUBROUTINE dyn_spg
+
    SUBROUTINE tra_adv_fct( kt, kit000, cdtype, p2dt, pun, pvn, pwn, ptb, ptn, pta, kjpt, kn_fct_h, kn_fct_v )
       INTEGER                            , INTENT(in   ) ::   kt, kit000, kjpt, kn_fct_h, kn_fct_v
       CHARACTER(len=3)                   , INTENT(in   ) ::   cdtype

@@ -206,6 +206,14 @@ class NemoHip:
         p = lambda a: self._p(a, dt)
         self._check(fn(p(un), p(vn), p(wn), p(zun), p(zvn), p(zwn)))
 
+    def tra_adv(self, kt, un, vn, wn, ptb, ptn, pta, kjpt, kn_fct_h, kn_fct_v, sp=False):
+        """tra_adv (FCT): r2dt of kt, transports of un, vn, wn formed where used"""
+        fn = self.L.nemo_hip_tra_adv_sp if sp else self.L.nemo_hip_tra_adv
+        dt = np.float32 if sp else np.float64
+        p = lambda a: self._p(a, dt)
+        self._check(fn(C.c_int(kt), p(un), p(vn), p(wn), p(ptb), p(ptn), p(pta), C.c_int(kjpt), C.c_int(kn_fct_h),
+                       C.c_int(kn_fct_v)))
+
     def tra_adv_fct(self, kt, kit000, cdtype, p2dt, pun, pvn, pwn, ptb, ptn, pta, kjpt, kn_fct_h, kn_fct_v, sp=False):
         fn = self.L.nemo_hip_tra_adv_fct_sp if sp else self.L.nemo_hip_tra_adv_fct
         dt = np.float32 if sp else np.float64
@@ -351,6 +359,18 @@ class NemoHip:
         q = self._p
         self._check(fn(C.c_int(kt), q(qns), q(qsr), q(sfx), q(emp), q(tsn, dt), q(tsa, dt), q(sbc_tsc),
                        q(sbc_tsc_b)))
+
+    def stream_copy(self, src, dst, nbytes, blocks):
+        self._check(self.L.nemo_hip_stream_copy(self._p(src), self._p(dst), C.c_size_t(nbytes), C.c_int(blocks)))
+
+    # tuning switches -------------------------------------------------------------
+    def tune_set(self, key, value):
+        self._check(self.L.nemo_hip_tune_set(str(key).encode(), C.c_int(int(value))))
+
+    def tune_get(self, key):
+        v = C.c_int()
+        self._check(self.L.nemo_hip_tune_get(str(key).encode(), C.byref(v)))
+        return v.value
 
     # compensated global sums -----------------------------------------------------
     def glob_sum(self, ptab, nk, kind=0, out=None, sp=False):

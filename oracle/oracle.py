@@ -13,7 +13,8 @@ import sys
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# NEMO_ORACLE_O3=1 loads the -O3 build (the timed CPU baseline); the tests use the -O2 one
+LIB = os.path.join(HERE, "liboracle_o3.so" if os.environ.get("NEMO_ORACLE_O3") == "1" else "liboracle.so")
 sys.path.insert(0, os.path.join(os.path.dirname(HERE), "nemo-fmi-devel_amd"))
 from nemo_hip import abi  # noqa: E402
 
@@ -21,7 +22,7 @@ _lib = None
 
 
 def build():
-    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    subprocess.run(["make", "-s", "-C", HERE, os.path.basename(LIB)], check=True)
 
 
 def lib():

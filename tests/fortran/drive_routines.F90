@@ -6,7 +6,10 @@ PROGRAM drive_routines
    !! stream file written by tests/test_gpu_fortran.py (sizes, switches,
    !! k_top/k_bot, the 24 horizontal metrics and 7 vertical scale factors
    !! in nemo_hip_domain_t order, then the fields); the results go to a
-   !! second stream file:  tra_ldf_lap, tra_zdf_imp, tra_adv_fct outputs.
+   !! second stream file:  tra_ldf_lap, tra_zdf_imp, tra_adv_fct outputs, then
+   !! the module-global step routines as step.F90 calls them: tsa from zero
+   !! through tra_adv, tra_ldf, tra_zdf; ua, va from zero through dyn_adv,
+   !! dyn_vor, dyn_zdf; hdivn of div_hor.
    !!
    !! usage: drive_routines <input> <output>
    !!======================================================================
@@ -23,6 +26,8 @@ PROGRAM drive_routines
    REAL(wp), ALLOCATABLE, TARGET :: ahtu(:,:,:), ahtv(:,:,:), avt(:,:,:), avs(:,:,:), avm(:,:,:)
    REAL(wp), ALLOCATABLE, TARGET :: zun(:,:,:), zvn(:,:,:), zwn(:,:,:), zgu(:,:,:)
    REAL(wp), ALLOCATABLE, TARGET :: tsb(:,:,:,:), tsn(:,:,:,:), ta_lap(:,:,:,:), ta_zdf(:,:,:,:), ta_fct(:,:,:,:)
+   REAL(wp), ALLOCATABLE, TARGET :: un(:,:,:), vn(:,:,:), wn(:,:,:), ub(:,:,:), vb(:,:,:), ua(:,:,:), va(:,:,:)
+   REAL(wp), ALLOCATABLE, TARGET :: hdivn(:,:,:), tsa(:,:,:,:), f2(:,:,:)
    CHARACTER(len=1024) :: cin, cout
 
    CALL GET_COMMAND_ARGUMENT( 1, cin )
@@ -36,6 +41,11 @@ PROGRAM drive_routines
    ALLOCATE( tsb(jpi,jpj,jpk,2), tsn(jpi,jpj,jpk,2), ta_lap(jpi,jpj,jpk,2), ta_zdf(jpi,jpj,jpk,2) )
    ALLOCATE( ta_fct(jpi,jpj,jpk,2) )
    READ(iu) ktop, kbot, m2, e3, ahtu, ahtv, avt, avs, avm, zun, zvn, zwn, tsb, tsn, ta_lap, kt, p2dt
+   ALLOCATE( un(jpi,jpj,jpk), vn(jpi,jpj,jpk), wn(jpi,jpj,jpk), ub(jpi,jpj,jpk), vb(jpi,jpj,jpk) )
+   ALLOCATE( ua(jpi,jpj,jpk), va(jpi,jpj,jpk), hdivn(jpi,jpj,jpk), tsa(jpi,jpj,jpk,2), f2(jpi,jpj,18) )
+   ! f2: rCdU_bot utau utau_b vtau vtau_b, then zeros for ua_b va_b and the closure's 2D arrays
+   f2(:,:,:) = 0._wp
+   READ(iu) un, vn, wn, ub, vb, f2(:,:,1:5)
    CLOSE(iu)
    ta_zdf(:,:,:,:) = ta_lap(:,:,:,:)
    ta_fct(:,:,:,:) = ta_lap(:,:,:,:)
@@ -75,10 +85,27 @@ PROGRAM drive_routines
    CALL tra_ldf_lap( kt, 1, 'TRA', ahtu, ahtv, zgu, zgu, zgu, zgu, tsb, ta_lap, 2, 1 )
    CALL tra_zdf_imp( kt, 1, 'TRA', p2dt, tsb, ta_zdf, 2 )
    CALL tra_adv_fct( kt, 1, 'TRA', p2dt, zun, zvn, zwn, tsb, tsn, ta_fct, 2, 4, 4 )
+   ! the module arrays of oce, zdfdrg, sbc_oce, dynspg_ts and ldftra, bound once
+   CALL nemo_hip_bind_dyn( un, vn, wn, ub, vb, ua, va, f2(:,:,1), f2(:,:,6), f2(:,:,7), f2(:,:,2), f2(:,:,3),   &
+      &                    f2(:,:,4), f2(:,:,5) )
+   CALL nemo_hip_bind_nxt( hdivn, f2(:,:,8), f2(:,:,9), f2(:,:,10), f2(:,:,11), f2(:,:,12), tsb, tsn, tsa,       &
+      &                    f2(:,:,13), f2(:,:,14), f2(:,:,15), f2(:,:,16), f2(:,:,17), f2(:,:,18), f2(:,:,8),      &
+      &                    f2(:,:,8) )
+   CALL nemo_hip_bind_tra( 4, 4, np_lap, ahtu, ahtv )
+   ! step.F90:234-257: the tracer trend from zero, and :179-204 the momentum trend
+   tsa(:,:,:,:) = 0._wp
+   CALL tra_adv( kt )
+   CALL tra_ldf( kt )
+   CALL tra_zdf( kt )
+   ua(:,:,:) = 0._wp ; va(:,:,:) = 0._wp
+   CALL dyn_adv( kt )
+   CALL dyn_vor( kt )
+   CALL dyn_zdf( kt )
+   CALL div_hor( kt )
    CALL nemo_hip_stop
 
    OPEN( NEWUNIT=iu, FILE=TRIM(cout), ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE', ACTION='WRITE' )
-   WRITE(iu) ta_lap, ta_zdf, ta_fct
+   WRITE(iu) ta_lap, ta_zdf, ta_fct, tsa, ua, va, hdivn
    CLOSE(iu)
    WRITE(*,'(a)') 'drive_routines: done'
 END PROGRAM drive_routines

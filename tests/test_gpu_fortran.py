@@ -39,6 +39,8 @@ def write_input(path, d, S, kt, p2dt, zun, zvn, zwn):
             fh.write(f(a))
         fh.write(np.asarray([kt], dtype=np.int32).tobytes())
         fh.write(np.asarray([p2dt], dtype=np.float64).tobytes())
+        for n in ("un", "vn", "wn", "ub", "vb", "rCdU_bot", "utau", "utau_b", "vtau", "vtau_b"):
+            fh.write(f(S[n]))
 
 
 @pytest.mark.gpu
@@ -62,9 +64,29 @@ def test_fortran_module_drives_routines(tmp_path, case):
     r = subprocess.run([DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, f"drive_routines failed ({r.returncode}):\n{r.stdout}\n{r.stderr}"
     out = np.fromfile(fout, dtype=np.float64)
-    n4 = int(np.prod(s4))
-    assert out.size == 3 * n4
-    got = [out[q * n4:(q + 1) * n4].reshape(s4, order="F") for q in range(3)]
+    n4, n3 = int(np.prod(s4)), int(np.prod(s3))
+    assert out.size == 4 * n4 + 3 * n3
+    got = [out[q * n4:(q + 1) * n4].reshape(s4, order="F") for q in range(4)]
+    o3 = out[4 * n4:]
+    g3 = [o3[q * n3:(q + 1) * n3].reshape(s3, order="F") for q in range(3)]
     check(got[0], want_lap, what="tra_ldf_lap via Fortran")
     check(got[1], want_zdf, what="tra_zdf_imp via Fortran")
     check(got[2], want_fct, what="tra_adv_fct via Fortran")
+    # the module-global routines: tra_adv(kt), tra_ldf(kt), tra_zdf(kt) from a zero trend (r2dt = 2 rdt at kt 3)
+    tsa = np.zeros(s4, order="F")
+    o.tra_adv_fct(kt, 1, "TRA", p2dt, *z, S["tsb"], S["tsn"], tsa, 2, 4, 4)
+    o.tra_ldf_lap(kt, 1, "TRA", S["ahtu"], S["ahtv"], S["tsb"], tsa, 2, 1)
+    o.tra_zdf(kt, S["tsb"], tsa, S["avt"], S["avs"], 0, np.zeros(s3, order="F"), np.zeros(s3, order="F"))
+    check(got[3], tsa, what="tra_adv + tra_ldf + tra_zdf via Fortran")
+    # dyn_adv(kt), dyn_vor(kt), dyn_zdf(kt) from a zero trend, div_hor(kt)
+    ua, va = np.zeros(s3, order="F"), np.zeros(s3, order="F")
+    o.dyn_adv(kt, S["un"], S["vn"], S["wn"], ua, va)
+    o.dyn_vor(kt, S["un"], S["vn"], ua, va)
+    z2 = np.zeros((d.jpi, d.jpj), order="F")
+    o.dyn_zdf(kt, S["ub"], S["vb"], ua, va, S["avm"], S["rCdU_bot"], z2, z2, S["utau"], S["utau_b"], S["vtau"],
+              S["vtau_b"])
+    check(g3[0], ua, what="dyn_adv + dyn_vor + dyn_zdf ua via Fortran")
+    check(g3[1], va, what="dyn_adv + dyn_vor + dyn_zdf va via Fortran")
+    hd = np.zeros(s3, order="F")
+    o.div_hor(kt, S["un"], S["vn"], hd)
+    check(g3[2], hd, what="div_hor via Fortran")
