@@ -524,6 +524,16 @@ int nemo_hip_tune_get(const char *key, int *value);
    a device address, and the call is asynchronous on the library stream). */
 int nemo_hip_glob_sum(const double *ptab, int nk, int kind, double *psum);
 int nemo_hip_glob_sum_sp(const float *ptab, int nk, int kind, double *psum);
+/* nemo_hip_glob_sum that allocates nothing.  No counterpart in the reference:
+   it is for callers that cannot allocate device memory (code under HIP graph
+   capture, serving loops, harnesses that forbid hipMalloc).  Device-pointer
+   mode only (ptab and psum device addresses; asynchronous on the library
+   stream).  Its partial sums live in a static device array of 16 KiB that
+   every GPU holds until the process ends and that all calls on one GPU
+   share: calls that overlap on one GPU race.  At most
+   NEMO_HIP_GLOB_SUM_NOALLOC_MAX values (jpi*jpj*nk). */
+#define NEMO_HIP_GLOB_SUM_NOALLOC_MAX 2147483647
+int nemo_hip_glob_sum_noalloc(const double *ptab, int nk, int kind, double *psum);
 
 /* dia_hsb, DIA/diahsb.F90:60-239 (linear free surface, no ice-shelf cavity,
    runoff or geothermal flux): the heat, salt and volume budget of the run

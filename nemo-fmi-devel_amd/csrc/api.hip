@@ -559,6 +559,10 @@ int nemo_hip_init(int device)
     if (device < 0 || device >= n) return set_error(NEMO_HIP_ERR_ARG, "device %d out of range (%d devices)", device, n);
     HIPCHK(hipSetDevice(device));
     HIPCHK(hipStreamCreate(&g_stream));
+    {
+        const int rc = glob_sum_scratch_init(device);   // nemo_hip_glob_sum_noalloc's array
+        if (rc) return rc;
+    }
     g_device = device;
     g_ctxs.assign(1, new Ctx());
     g_c = g_ctxs[0];
@@ -2134,6 +2138,23 @@ int nemo_hip_glob_sum(const double *ptab, int nk, int kind, double *psum)
 int nemo_hip_glob_sum_sp(const float *ptab, int nk, int kind, double *psum)
 {
     return do_glob_sum<float>(ptab, nk, kind, psum);
+}
+int nemo_hip_glob_sum_noalloc(const double *ptab, int nk, int kind, double *psum)
+{
+    NEED_DOMAIN();
+    NEED_PTR(ptab); NEED_PTR(psum);
+    if (g_ptr_mode != NEMO_HIP_PTR_DEVICE)
+        return set_error(NEMO_HIP_ERR_STATE, "glob_sum_noalloc: device-pointer mode only (host mode would stage)");
+    const DevDomain &d = g_c->dom;
+    if (nk < 1 || nk > d.jpk) return set_error(NEMO_HIP_ERR_ARG, "glob_sum_noalloc: nk = %d outside 1..jpk", nk);
+    if (kind != 0 && kind != 1)
+        return set_error(NEMO_HIP_ERR_ARG, "glob_sum_noalloc: kind = %d (0 tmask_i, 1 tmask_h)", kind);
+    if (d.n2 * (size_t)nk > (size_t)NEMO_HIP_GLOB_SUM_NOALLOC_MAX)
+        return set_error(NEMO_HIP_ERR_ARG, "glob_sum_noalloc: %zu values, more than the maximum %d", d.n2 * (size_t)nk,
+                         NEMO_HIP_GLOB_SUM_NOALLOC_MAX);
+    double *scr = glob_sum_static_scratch(g_device);
+    if (!scr) return set_error(NEMO_HIP_ERR_STATE, "glob_sum_noalloc: no scratch address for device %d", g_device);
+    return launch_glob_sum<double>(ptab, nk, kind, scr, psum);
 }
 int nemo_hip_dia_hsb_init(nemo_hip_hsb_t *h)
 {

@@ -133,6 +133,34 @@ template int launch_glob_sum<float>(const float *, int, int, double *, double *)
 size_t glob_sum_scratch_bytes() { return GSUM_SCRATCH_DOUBLES * sizeof(double); }
 
 // ---------------------------------------------------------------------------
+// the glob_sum that allocates nothing (nemo_hip_glob_sum_noalloc): its
+// partial pairs live in this file-scope device array, 16 KiB per GPU, whose
+// device address nemo_hip_init takes once per device, with that device
+// current.  Host code uses only that address: the array's name is a host
+// stand-in.  Calls on one GPU share the array, so calls that overlap on one
+// GPU race; nothing in it needs clearing (every launch writes the partials
+// that the final block then reads).
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[GSUM_SCRATCH_DOUBLES];
+#define GSUM_MAXDEV 64
+static double *g_scratch_dev[GSUM_MAXDEV];
+
+int glob_sum_scratch_init(int device)
+{
+    if (device < 0 || device >= GSUM_MAXDEV)
+        return nh::set_error(NEMO_HIP_ERR_ARG, "glob_sum: device %d beyond %d", device, GSUM_MAXDEV);
+    if (g_scratch_dev[device]) return 0;
+    void *ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_dev[device] = (double *)ptr;
+    return 0;
+}
+double *glob_sum_static_scratch(int device)
+{
+    return (device >= 0 && device < GSUM_MAXDEV) ? g_scratch_dev[device] : nullptr;
+}
+
+// ---------------------------------------------------------------------------
 // dia_hsb, DIA/diahsb.F90:88-189 (linear free surface, no ice-shelf cavity,
 // no runoff, no geothermal flux): the thirteen global sums of one call.
 //   0  -(emp - rnf) surf                   glob_sum       (:93)

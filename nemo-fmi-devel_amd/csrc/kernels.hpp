@@ -94,6 +94,8 @@ int launch_tra_sbc(int first, double *qns, double *qsr, const double *sfx, const
 // compensated global sums (gsum.hip)
 template <typename T> int launch_glob_sum(const T *p, int nk, int kind, double *scratch, double *out);
 size_t glob_sum_scratch_bytes();
+int glob_sum_scratch_init(int device);     // the static scratch's address on this device
+double *glob_sum_static_scratch(int device);
 template <typename T>
 int launch_dia_hsb(int ln_traqsr, const double *emp, const double *rnf, const double *sbc, const double *qsr,
                    const double *sshn, const T *tsn, const T *tsb, const T *wn, const double *surf_ini,

@@ -386,6 +386,12 @@ class NemoHip:
         self._check(fn(self._p(ptab, dt), C.c_int(nk), C.c_int(kind), self._p(r)))
         return float(r[0])
 
+    def glob_sum_noalloc(self, ptab, nk, kind, out):
+        """nemo_hip_glob_sum_noalloc: FP64 device arrays only; out is a
+        one-element FP64 device tensor, written asynchronously."""
+        self._check(self.L.nemo_hip_glob_sum_noalloc(self._p(ptab), C.c_int(nk), C.c_int(kind), self._p(out)))
+        return out
+
     def _hsb(self, fields, sp, state=None):
         h = abi.HsbStruct() if state is None else state
         h.real_bytes = 4 if sp else 8
