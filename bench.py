@@ -83,6 +83,8 @@ def parse():
                     help="finite-difference directions for the adjoint gradient check")
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--timing-steps", type=int, default=2)
+    ap.add_argument("--tune", default="", metavar="KEY=V,...",
+                    help="library tuning switches (nemo_hip_tune_set) to set before timing, e.g. spg_persistent=1")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (three times): the reference kernel the "
                          "FETCH_SIZE / WRITE_SIZE counters are scaled by")
@@ -570,15 +572,15 @@ def kernel_times(h):
     kern = {
         "dyn_adv": ["k_dyn_keg_zad"], "dyn_vor": ["k_dyn_vor"], "dyn_zdf": ["k_dyn_zdf"],
         "dyn_spg_ts": ["k_spg_cor1", "k_spg_cor2", "k_spg_init", "k_spg_p1a", "k_spg_p1b", "k_spg_s1", "k_spg_s2", "k_spg_s3",
-                       "k_spg_p3a", "k_spg_p3b"],
+                       "k_spg_loop", "k_spg_p3a", "k_spg_p3b"],
         "tra_adv_trp": ["k_tra_adv_trp"],
         "tra_adv_fct": ["k_fct_fused", "k_fct_cpt_fwd", "k_fct_h", "k_fct_v", "k_fct_lim", "k_fct_beta",
                         "k_fct_final", "k_fct_low", "k_fct_ad"],
-        "tra_ldf_iso": ["k_ldf_iso_coef", "k_tra_ldf_iso", "k_fill3", "k_copy3"], "tra_zdf": ["k_tra_zdf", "k_clamp_ring"],
+        "tra_ldf_iso": ["k_ldf_iso_coef", "k_tra_ldf_iso", "k_fill3", "k_copy3"], "tra_zdf": ["k_tra_zdf", "k_tra_zdf_reg", "k_clamp_ring"],
         "lbc_lnk": ["k_lbc"],
         "step_dyn": ["k_step_dyn1", "k_step_dyn2"],
         "step_tra": ["k_fct_fused", "k_fct_cpt_fwd", "k_fct_h", "k_fct_v", "k_fct_lim", "k_fct_beta", "k_fct_final",
-                     "k_zero_ring", "k_tra_zdf"],
+                     "k_zero_ring", "k_tra_zdf", "k_tra_zdf_reg"],
         "div_hor": ["k_div_hor"], "ssh_nxt": ["k_ssh_nxt"], "wzv": ["k_wzv"], "ssh_swp": ["k_ssh_swp"],
         "tra_nxt": ["k_tra_nxt"], "dyn_nxt": ["k_dyn_nxt1", "k_dyn_nxt2"],
         "ldf_slp": ["k_slp_mxl", "k_slp_uv_raw", "k_slp_uv_filt", "k_slp_w_raw", "k_slp_w_filt"],
@@ -781,6 +783,11 @@ def main():
     log(f"domain {jpi}x{jpj}x{jpk} built in {time.perf_counter() - t0:.1f} s, wet fraction "
         f"{float((d.arrays['k_top'] > 0).mean()):.3f}")
     h = nemo_hip.get(0, nemo_hip.NEMO_HIP_PTR_DEVICE)
+    for kv in [x for x in a.tune.split(",") if x]:
+        key, val = kv.split("=")
+        h.tune_set(key.strip(), int(val))
+        log(f"tuning: {key.strip()} = {int(val)}")
+    tuning = {k: h.tune_get(k) for k in ("fct_fused", "spg_persistent", "zdf_reg")}
     h.register(d)
     t0 = time.perf_counter()
     S = configs.make_state_device(d, seed=a.seed + 1)
@@ -912,6 +919,7 @@ def main():
         "forms": res,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "tuning": tuning,
     }
     if "fp64" in res:
         for f in ("fp32", "fp32_fused", "fp64_fused"):
@@ -938,6 +946,7 @@ def _fused(f):
 
 KERNEL_BYTES = {
     "k_tra_zdf": _b(8),          # r avt ah_wslp2 tb*2 ta*2, w ta*2
+    "k_tra_zdf_reg": _b(8),
     # r ahtu ahtv uslp vslp wslpi wslpj ah_wslp2 akz tb*2 ta*2, w ta*2
     "k_tra_ldf_iso": _b(14),
     "k_ldf_iso_coef": _b(5),     # r ahtu ahtv wslpi wslpj, w ah_wslp2

@@ -511,6 +511,16 @@ int nemo_hip_stream_copy(const void *src, void *dst, size_t bytes, int blocks);
      "fct_fused"  1 (default): tra_adv_fct on closed domains as one kernel
                   per tile and tracer (intermediates on chip); 0: the
                   flux / limiter chain through HBM scratch.
+     "spg_persistent"  1 or 2: dyn_spg_ts's sub-steps in one cooperative
+                  launch with grid barriers (1: one 1024-thread workgroup
+                  per CU; 2: 256-thread workgroups, as many as stay
+                  resident; bounded waits: a barrier that gives up makes
+                  the next nemo_hip_sync fail); 0 (default), or while a
+                  graph is recorded: two launches per sub-step.
+     "zdf_reg"    1: in the FP32 storage form with jpk <= 76, the column
+                  solves of tra_zdf and dyn_zdf keep the forward-
+                  substituted values on chip; 0 (default): written to the
+                  trend array and read back.
    Unknown keys and values are errors. */
 int nemo_hip_tune_set(const char *key, int value);
 int nemo_hip_tune_get(const char *key, int *value);

@@ -6,6 +6,11 @@ hipcc flags:
                                     like the reference Fortran expression
   -Xclang -target-feature -Xclang -packed-fp32-ops
                                     no packed FP32 instructions
+  -mllvm -pragma-unroll-threshold=500000
+                                    `#pragma unroll` unrolls the column solves
+                                    that keep a whole column in registers
+                                    fully (their arrays are then indexed by
+                                    constants and stay out of scratch memory)
 """
 import concurrent.futures as cf
 import os
@@ -19,7 +24,7 @@ LIB = os.path.join(HERE, "libnemo_hip.so")
 SOURCES = ["api.hip", "dom_lbc.hip", "tra.hip", "dyn.hip", "spg.hip", "nxt.hip", "phys.hip", "slp.hip", "adj.hip", "gsum.hip", "fct_fused.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
-         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-pragma-unroll-threshold=500000",
          "-Wall", "-Wno-unused-variable", "-Wno-unused-function", "-I", CSRC]
 
 

@@ -44,6 +44,8 @@ struct Ctx {
     int neos = NEMO_NP_SEOS;
     double *stp_buf = nullptr;                          // stp_ctl partial maxima and results
     double *gsum_buf = nullptr;                         // glob_sum partial pairs and total
+    unsigned *spg_bar = nullptr;                        // dyn_spg_ts grid barrier and status
+    bool spg_check = false;                             // a cooperative phase 2 awaits its status check
     double *hsb_buf = nullptr;                          // dia_hsb partial pairs and sums
 };
 static std::vector<Ctx *> g_ctxs;
@@ -95,11 +97,14 @@ hipStream_t stream() { return g_stream; }
 const DevDomain &dom() { return g_c->dom; }
 bool capturing() { return g_capturing; }
 // tuning switches, process-wide; names and defaults
-static const char *g_tune_names[T_NTUNE] = {"fct_fused"};
-static int g_tune[T_NTUNE] = {1};
+static const char *g_tune_names[T_NTUNE] = {"fct_fused", "spg_persistent", "zdf_reg"};
+static int g_tune[T_NTUNE] = {1, 0, 0};
+static const int g_tune_max[T_NTUNE] = {1, 2, 1};
 int tune(Tune which) { return g_tune[which]; }
 int *spg_rot_u() { return g_c->spg_rot_u; }
 int *spg_rot_s() { return g_c->spg_rot_s; }
+unsigned *spg_barrier() { return g_c->spg_bar; }
+void spg_loop_launched() { g_c->spg_check = true; }
 void spg_ts_reset()
 {
     for (int q = 0; q < 3; q++) g_c->spg_rot_u[q] = q;
@@ -174,6 +179,8 @@ static void free_domain()
     for (auto &s : g_c->scratch2) s = nullptr;
     g_c->stp_buf = nullptr;
     g_c->gsum_buf = g_c->hsb_buf = nullptr;
+    g_c->spg_bar = nullptr;
+    g_c->spg_check = false;
     g_c->have_domain = false;
 }
 
@@ -496,6 +503,8 @@ static int register_domain(const nemo_hip_domain_t *h)
     for (int s = 0; s < NH_NSCRATCH2; s++) HIPCHK(hipMemset(g_c->scratch2[s], 0, n2 * sizeof(double)));
     if ((rc = dev_alloc((void **)&g_c->stp_buf, stp_ctl_buffer_bytes()))) return rc;
     if ((rc = dev_alloc((void **)&g_c->gsum_buf, glob_sum_scratch_bytes()))) return rc;
+    if ((rc = dev_alloc((void **)&g_c->spg_bar, 64))) return rc;
+    HIPCHK(hipMemset(g_c->spg_bar, 0, 64));
     if ((rc = dev_alloc((void **)&g_c->hsb_buf, dia_hsb_scratch_bytes() + 16 * sizeof(double)))) return rc;
     HIPCHK(hipDeviceSynchronize());
     g_c->dom = d;
@@ -624,6 +633,18 @@ int nemo_hip_sync(void)
     if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "not initialised");
     NEED_NOT_CAPTURING("nemo_hip_sync");
     HIPCHK(hipStreamSynchronize(g_stream));
+    // a grid barrier of dyn_spg_ts's cooperative phase 2 that gave up leaves
+    // its status word set: the barotropic results of that call are wrong
+    for (Ctx *c : g_ctxs) {
+        if (!c || !c->spg_check || !c->spg_bar) continue;
+        c->spg_check = false;
+        int flag = 0;
+        HIPCHK(hipMemcpy(&flag, c->spg_bar + 2, sizeof flag, hipMemcpyDeviceToHost));
+        if (flag) {
+            HIPCHK(hipMemset(c->spg_bar, 0, 64));
+            return set_error(NEMO_HIP_ERR_HIP, "dyn_spg_ts: a grid barrier of the barotropic sub-steps timed out");
+        }
+    }
     return NEMO_HIP_OK;
 }
 
@@ -2102,8 +2123,8 @@ int nemo_hip_tune_set(const char *key, int value)
     if (!key) return set_error(NEMO_HIP_ERR_ARG, "tune_set: NULL key");
     for (int q = 0; q < T_NTUNE; q++)
         if (!strcmp(key, g_tune_names[q])) {
-            if (q == T_FCT_FUSED && value != 0 && value != 1)
-                return set_error(NEMO_HIP_ERR_ARG, "tune_set: fct_fused = %d (0 or 1)", value);
+            if (value < 0 || value > g_tune_max[q])
+                return set_error(NEMO_HIP_ERR_ARG, "tune_set: %s = %d (0 to %d)", key, value, g_tune_max[q]);
             g_tune[q] = value;
             return NEMO_HIP_OK;
         }

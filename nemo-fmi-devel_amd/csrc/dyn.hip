@@ -521,15 +521,168 @@ __device__ __forceinline__ void zdf_column(const DevDomain &d, double r2dt, int 
     }
 }
 
-template <typename T>
-KERNEL void k_dyn_zdf(DevDomain d, double r2dt, const T * __restrict__ ub, const T * __restrict__ vb, T * __restrict__ ua, T * __restrict__ va, const T * __restrict__ avm,
+// The same column solve in the FP32 storage form with jpk <= KMAX: the
+// forward-substituted values (rounded to storage, as zdf_column stores
+// them) stay in registers instead of passing through pa, and the FP64
+// pivots are kept at every ZREG_SEG-th level, the others recomputed segment
+// by segment in the back substitution from a second read of avm.  Every
+// loop over levels is unrolled to KMAX so the register array is indexed by
+// constants; the level count only guards.  Results equal zdf_column's bit
+// for bit.
+#define ZREG_SEG 8
+template <int KMAX, typename Pre = PreIdentity>
+__device__ __forceinline__ void zdf_column_reg(const DevDomain &d, double r2dt, int comp, int i, int j,
+                                               const float *__restrict__ pb, float *__restrict__ pa,
+                                               const float *__restrict__ avm, const double *__restrict__ rCdU_bot,
+                                               const double *__restrict__ pa_b, const double *__restrict__ tau,
+                                               const double *__restrict__ tau_b, Pre pre = Pre())
+{
+    const Metrics<float> &m = met<float>(d);
+    const int jpk = d.jpk;
+    const size_t plane = d.n2;
+    const uint32_t ij = IJ32(i, j);
+    const uint32_t ijn = comp ? IJ32(i, j + 1) : IJ32(i + 1, j);
+    const float *e3x = comp ? m.e3v : m.e3u;
+    const float *e3xw = comp ? m.e3vw : m.e3uw;
+    const KRange &kr = comp ? d.kV : d.kU;
+    const int x0 = kr.k0[ij], x1 = kr.k1[ij];
+    auto xmask = [&](int k) { return (k >= x0 && k <= x1) ? 1.0 : 0.0; };
+    auto wxmask = [&](int k) { return (k >= x0 && k <= x1 && (k == 1 || k - 1 >= x0)) ? 1.0 : 0.0; };
+    const double r_vvl = 0.0;
+    const bool rm_b = d.ln_drgimp && d.ln_dynspg_ts;
+    const double ab = rm_b ? pa_b[ij] : 0.0;
+    if (!is_interior(d, i, j)) {
+        for (int k = 1; k <= jpk - 1; k++) {
+            const double xm = xmask(k);
+            double a = ((double)ldp(plane_ptr(pb, plane, k), ij) +
+                        r2dt * (double)pre(k, ldp((const float *)plane_ptr(pa, plane, k), ij))) * xm;
+            if (rm_b) a = (a - ab) * xm;
+            stp(plane_ptr(pa, plane, k), ij, (float)a);
+        }
+        return;
+    }
+    const int ikb = comp ? d.mbkv[ij] : d.mbku[ij];
+    const double csum = rCdU_bot[ijn] + rCdU_bot[ij];
+    const double zdt = r2dt * 0.5;
+    auto zzw = [&](int k, double ze3) {
+        const float *ak = plane_ptr(avm, plane, k);
+        return -(zdt * ((double)ldp(ak, ijn) + (double)ldp(ak, ij)) / (ze3 * (double)E3(e3xw, ij, k)) * wxmask(k));
+    };
+    auto ze3_of = [&](int k) { return (1.0 - r_vvl) * (double)E3(e3x, ij, k) + r_vvl * (double)E3(e3x, ij, k); };
+    // the eliminated diagonal of level k >= 2 from the previous level's
+    // upper diagonal and pivot (lines 208-243 and the forward elimination)
+    auto pivot = [&](int k, double ze3, double zws, double zws_p, double zwd_p) {
+        const double zwi = zzw(k, ze3);
+        double zwd = 1.0 - zwi - zws;
+        if (d.ln_drgimp && k == ikb) zwd = zwd - r2dt * 0.5 * csum / ze3;
+        return zwd - zwi * zws_p / zwd_p;
+    };
+    constexpr int NSEG = (KMAX + ZREG_SEG - 1) / ZREG_SEG;
+    float fw[KMAX];        // level k at index k - 1
+    double ck[NSEG];       // the pivot of level s * ZREG_SEG at index s
+    double zws_prev, zwd_prev, a_prev;
+    {   // k = 1: surface row with the wind stress
+        const double xm = xmask(1);
+        double a = ((double)ldp(pb, ij) + r2dt * (double)pre(1, ldp((const float *)pa, ij))) * xm;
+        if (rm_b) a = (a - ab) * xm;
+        const double ze3 = ze3_of(1);
+        if (rm_b && 1 == ikb) a = a + r2dt * 0.5 * csum * ab / ze3;
+        const double zws = zzw(2, ze3);
+        double zwd = 1.0 - zws;
+        if (d.ln_drgimp && 1 == ikb) zwd = zwd - r2dt * 0.5 * csum / ze3;
+        a = a + r2dt * 0.5 * (tau_b[ij] + tau[ij]) / (ze3 * d.rau0) * xm;
+        fw[0] = (float)a;
+        zws_prev = zws;
+        zwd_prev = zwd;
+        a_prev = a;
+    }
+#pragma unroll
+    for (int k = 2; k < KMAX; k++) {
+        if (k <= jpk - 1) {
+            const double xm = xmask(k);
+            double a = ((double)ldp(plane_ptr(pb, plane, k), ij) +
+                        r2dt * (double)pre(k, ldp((const float *)plane_ptr(pa, plane, k), ij))) * xm;
+            if (rm_b) a = (a - ab) * xm;
+            const double ze3 = ze3_of(k);
+            if (rm_b && k == ikb) a = a + r2dt * 0.5 * csum * ab / ze3;
+            const double zwi = zzw(k, ze3);
+            const double zws = zzw(k + 1, ze3);
+            const double zwd = pivot(k, ze3, zws, zws_prev, zwd_prev);
+            a = a - zwi / zwd_prev * a_prev;
+            if (k % ZREG_SEG == 0) ck[k / ZREG_SEG] = zwd;
+            fw[k - 1] = (float)a;
+            zws_prev = zws;
+            zwd_prev = zwd;
+            a_prev = a;
+        }
+    }
+    double x = a_prev / zwd_prev;
+    stp(plane_ptr(pa, plane, jpk - 1), ij, (float)x);
+    const int kl = jpk - 2;
+#pragma unroll
+    for (int s = NSEG - 1; s >= 0; s--) {
+        const int k0 = s * ZREG_SEG + 1;
+        if (k0 <= kl) {
+            double piv[ZREG_SEG], zsv[ZREG_SEG];
+            double zwd_p = 0.0, zws_p = 0.0;
+            if (s > 0) {
+                zwd_p = ck[s];
+                zws_p = zzw(k0, ze3_of(k0 - 1));
+            }
+#pragma unroll
+            for (int q = 0; q < ZREG_SEG; q++) {
+                const int k = k0 + q;
+                if (k < KMAX && k <= kl) {
+                    const double ze3 = ze3_of(k);
+                    const double zws = zzw(k + 1, ze3);
+                    double zwd;
+                    if (k == 1) {
+                        zwd = 1.0 - zws;
+                        if (d.ln_drgimp && 1 == ikb) zwd = zwd - r2dt * 0.5 * csum / ze3;
+                    } else {
+                        zwd = pivot(k, ze3, zws, zws_p, zwd_p);
+                    }
+                    piv[q] = zwd;
+                    zsv[q] = zws;
+                    zws_p = zws;
+                    zwd_p = zwd;
+                }
+            }
+#pragma unroll
+            for (int q = ZREG_SEG - 1; q >= 0; q--) {
+                const int k = k0 + q;
+                if (k < KMAX && k <= kl) {
+                    x = ((double)fw[k - 1] - zsv[q] * x) / piv[q];
+                    stp(plane_ptr(pa, plane, k), ij, (float)x);
+                }
+            }
+        }
+    }
+}
+// the levels zdf_column_reg holds in registers (jpk <= ZDF_REG_KMAX): the
+// 75 of the full-size configurations
+#define ZDF_REG_KMAX 76
+
+template <typename T, bool REG>
+KERNEL_W(2) void k_dyn_zdf(DevDomain d, double r2dt, const T * __restrict__ ub, const T * __restrict__ vb, T * __restrict__ ua, T * __restrict__ va, const T * __restrict__ avm,
                           const double * __restrict__ rCdU_bot, const double * __restrict__ ua_b, const double * __restrict__ va_b, const double * __restrict__ utau,
                           const double * __restrict__ utau_b, const double * __restrict__ vtau, const double * __restrict__ vtau_b, double * __restrict__ zwd_s)
 {
     THREAD_IJ(1, 1);
     if (i > d.jpi || j > d.jpj) return;
-    zdf_column(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, zwd_s);
-    zdf_column(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, zwd_s);
+    if constexpr (REG) {
+        zdf_column_reg<ZDF_REG_KMAX>(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b);
+        zdf_column_reg<ZDF_REG_KMAX>(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b);
+    } else {
+        zdf_column(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, zwd_s);
+        zdf_column(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, zwd_s);
+    }
+}
+// the register form applies: FP32 storage, jpk <= ZDF_REG_KMAX, switch zdf_reg
+template <typename T>
+static bool zdf_reg(const DevDomain &d)
+{
+    return sizeof(T) == 4 && d.jpk <= ZDF_REG_KMAX && nh::tune(nh::T_ZDF_REG);
 }
 
 template <typename T>
@@ -541,8 +694,15 @@ int launch_dyn_zdf(double r2dt, const T *ub, const T *vb, T *ua, T *va, const T 
     Grid2 g = grid2(d.jpi, d.jpj);
     double *zwd = (double *)nh::scratch3(nh::S_ZWT);
     // forward-substituted values stay in ua/va at storage precision
-    LAUNCH("dyn_zdf", k_dyn_zdf<T>, g, d, r2dt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau, utau_b, vtau, vtau_b,
-           zwd);
+    if constexpr (sizeof(T) == 4) {
+        if (zdf_reg<T>(d)) {
+            LAUNCH("dyn_zdf", (k_dyn_zdf<T, true>), g, d, r2dt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau,
+                   utau_b, vtau, vtau_b, zwd);
+            return 0;
+        }
+    }
+    LAUNCH("dyn_zdf", (k_dyn_zdf<T, false>), g, d, r2dt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau, utau_b,
+           vtau, vtau_b, zwd);
     return 0;
 }
 template int launch_dyn_zdf<double>(double, const double *, const double *, double *, double *, const double *,
@@ -697,8 +857,8 @@ struct PreSpg {
     }
 };
 
-template <typename T>
-KERNEL void k_step_dyn2(DevDomain d, double r2dt, double r1_2dt_b, const T *__restrict__ ub, const T *__restrict__ vb,
+template <typename T, bool REG>
+KERNEL_W(2) void k_step_dyn2(DevDomain d, double r2dt, double r1_2dt_b, const T *__restrict__ ub, const T *__restrict__ vb,
                         T *__restrict__ ua, T *__restrict__ va, T *__restrict__ un, T *__restrict__ vn,
                         const T *__restrict__ avm, const double *__restrict__ rCdU_bot,
                         const double *__restrict__ ua_b, const double *__restrict__ va_b,
@@ -732,8 +892,13 @@ KERNEL void k_step_dyn2(DevDomain d, double r2dt, double r1_2dt_b, const T *__re
     const bool inner = is_interior(d, i, j);
     PreSpg<T> pu{su_s[ij], (ua_b[ij] - ub_b[ij]) * r1_2dt_b, 0.0, d.kU.k0[ij], d.kU.k1[ij], inner};
     PreSpg<T> pv{sv_s[ij], (va_b[ij] - vb_b[ij]) * r1_2dt_b, 0.0, d.kV.k0[ij], d.kV.k1[ij], inner};
-    zdf_column(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, zwd_s, pu);
-    zdf_column(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, zwd_s, pv);
+    if constexpr (REG) {
+        zdf_column_reg<ZDF_REG_KMAX>(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, pu);
+        zdf_column_reg<ZDF_REG_KMAX>(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, pv);
+    } else {
+        zdf_column(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, zwd_s, pu);
+        zdf_column(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, zwd_s, pv);
+    }
 }
 
 template <typename T, int ADV, int KEG, int SCHEME>
@@ -795,9 +960,17 @@ int launch_step_dyn(int kt, double r2dt, const T *ub, const T *vb, const T *wn, 
     if ((rc = launch_dyn_spg_ts<T>(kt, st, 1))) return rc;
     const double z2dt_bf = (kt == d.nit000 && d.neuler == 0) ? d.rdt : 2.0 * d.rdt;
     double *zwd = (double *)nh::scratch3(nh::S_ZWT);
-    LAUNCH("step_dyn", k_step_dyn2<T>, g, d, r2dt, 1.0 / z2dt_bf, ub, vb, ua, va, un, vn, avm, st->rCdU_bot, st->ua_b,
-           st->va_b, st->ub_b, st->vb_b, st->un_adv, st->vn_adv, st->r1_hu_n, st->r1_hv_n, st->un_b, st->vn_b, w[2],
-           w[3], st->utau, utau_b, st->vtau, vtau_b, zwd);
+    if constexpr (sizeof(T) == 4) {
+        if (zdf_reg<T>(d)) {
+            LAUNCH("step_dyn", (k_step_dyn2<T, true>), g, d, r2dt, 1.0 / z2dt_bf, ub, vb, ua, va, un, vn, avm,
+                   st->rCdU_bot, st->ua_b, st->va_b, st->ub_b, st->vb_b, st->un_adv, st->vn_adv, st->r1_hu_n,
+                   st->r1_hv_n, st->un_b, st->vn_b, w[2], w[3], st->utau, utau_b, st->vtau, vtau_b, zwd);
+            return 0;
+        }
+    }
+    LAUNCH("step_dyn", (k_step_dyn2<T, false>), g, d, r2dt, 1.0 / z2dt_bf, ub, vb, ua, va, un, vn, avm,
+           st->rCdU_bot, st->ua_b, st->va_b, st->ub_b, st->vb_b, st->un_adv, st->vn_adv, st->r1_hu_n, st->r1_hv_n,
+           st->un_b, st->vn_b, w[2], w[3], st->utau, utau_b, st->vtau, vtau_b, zwd);
     return 0;
 }
 template int launch_step_dyn<double>(int, double, const double *, const double *, const double *, const double *,

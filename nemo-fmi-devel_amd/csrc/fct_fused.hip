@@ -211,29 +211,24 @@ __global__ __launch_bounds__(NH_BLOCK, FuWaves<T>::value) void k_fct_fused(
         ikb[r] = d.mbkt[of[r]];
     }
 
-    // ---- loads of a level (clamped into 1..jpk-1 for the transports)
-    auto ld_u = [&](int r, int k) {
-        const int kc = min(max(k, 1), jpk - 1);
-        const T raw = ldp(plane_ptr(pu, n2, kc), of[r]);
-        if (TRP) return e2u[r] * E3(m.e3u, of[r], kc) * raw;
-        return raw;
-    };
-    auto ld_v = [&](int r, int k) {
-        const int kc = min(max(k, 1), jpk - 1);
-        const T raw = ldp(plane_ptr(pv, n2, kc), of[r]);
-        if (TRP) return e1v[r] * E3(m.e3v, of[r], kc) * raw;
-        return raw;
-    };
-    auto ld_w = [&](int r, int k) {
-        const int kc = min(max(k, 1), jpk - 1);
-        const T raw = ldp(plane_ptr(pw, n2, kc), of[r]);
-        if (TRP) return e1e2t[r] * raw;
-        return raw;
-    };
+    // ---- loads of a level.  Every global load of the march is issued one
+    // level (tb two) before the level that uses it and only moved between
+    // registers in between: the wait counter of the loads is in order, so a
+    // load issued in a stage and used at once would also wait for every load
+    // of the next level in flight.  The raw values are loaded; the transports
+    // are formed at their level (the same products, in the same order).
+    // Levels are clamped into 1..jpk-1 for the transports, 1..jpk for e3t.
+    auto kcl = [&](int k) { return min(max(k, 1), jpk - 1); };
+    auto ld_raw = [&](const T *p, int r, int k) { return ldp(plane_ptr(p, n2, kcl(k)), of[r]); };
 
     // ---- registers carried down the march
     T tbc[NS], tbx[NS], tbxx[NS], tnc[NS], tnx[NS];
     T uc[NS], ux[NS], vc[NS], vx[NS], wc[NS], wx[NS], zoc[NS], zox[NS], tac[NS], tax[NS];
+    T eux[NS], evx[NS];                 // e3u, e3v of the prefetched transport level (TRP)
+    // e3t at the level in flight (it until the loads of level it are issued,
+    // then it-1), it, it+1, it+2; and in FP64 for the FP32 form
+    T e3x[NS], e3c[NS], e3p1[NS], e3p2[NS];
+    double e3dx[NS], e3dc[NS], e3dp1[NS];
     T upzb[NS], hu[NS], hv[NS];
     T zwi0[NS], zwi1[NS];
     T zx0[NS], zx1[NS], zx2[NS], zy0[NS], zy1[NS], zy2[NS];
@@ -247,9 +242,15 @@ __global__ __launch_bounds__(NH_BLOCK, FuWaves<T>::value) void k_fct_fused(
         tbx[r] = ldp(plane_ptr(tb, n2, k1), of[r]);
         tnc[r] = ldp(plane_ptr(tn, n2, kt), of[r]);
         tnx[r] = ldp(plane_ptr(tn, n2, k1), of[r]);
-        ux[r] = ld_u(r, jpk - 1);
-        vx[r] = ld_v(r, jpk - 1);
-        wx[r] = r * NH_BLOCK < NZR ? ld_w(r, jpk - 1) : T(0);
+        ux[r] = ld_raw(pu, r, jpk - 1);
+        vx[r] = ld_raw(pv, r, jpk - 1);
+        wx[r] = r * NH_BLOCK < NZR ? ld_raw(pw, r, jpk - 1) : T(0);
+        eux[r] = TRP ? E3(m.e3u, of[r], kcl(jpk - 1)) : T(0);
+        evx[r] = TRP ? E3(m.e3v, of[r], kcl(jpk - 1)) : T(0);
+        e3x[r] = E3(m.e3t, of[r], kt);      // the first level of the march
+        e3dx[r] = sizeof(T) == 8 ? 0.0 : E3(d.md.e3t, of[r], kt);
+        e3c[r] = e3p1[r] = e3p2[r] = T(1);
+        e3dc[r] = e3dp1[r] = 1.0;
         zox[r] = (NV4 && r * NH_BLOCK < NBR) ? ldp(plane_ptr(zo, n2, k1), of[r]) : T(0);
         tax[r] = (!TA0 && r * NH_BLOCK < NTL) ? ldp(plane_ptr((const T *)ta, n2, k1), of[r]) : T(0);
         uc[r] = vc[r] = wc[r] = zoc[r] = tac[r] = tbxx[r] = T(0);
@@ -276,18 +277,36 @@ __global__ __launch_bounds__(NH_BLOCK, FuWaves<T>::value) void k_fct_fused(
             const int k2 = max(it - 2, 1), k1 = max(it - 1, 1);
 #pragma unroll
             for (int r = 0; r < NS; r++) {
-                tbxx[r] = ldp(plane_ptr(tb, n2, k2), of[r]);
-                uc[r] = ux[r];
-                vc[r] = vx[r];
-                wc[r] = wx[r];
+                // level it from the registers loaded one level earlier
+                if (TRP) {
+                    uc[r] = e2u[r] * eux[r] * ux[r];
+                    vc[r] = e1v[r] * evx[r] * vx[r];
+                    wc[r] = e1e2t[r] * wx[r];
+                } else {
+                    uc[r] = ux[r];
+                    vc[r] = vx[r];
+                    wc[r] = wx[r];
+                }
                 zoc[r] = zox[r];
                 tac[r] = tax[r];
-                ux[r] = ld_u(r, it - 1);
-                vx[r] = ld_v(r, it - 1);
-                if (r * NH_BLOCK < NZR) wx[r] = ld_w(r, it - 1);
+                // loads of the next levels
+                tbxx[r] = ldp(plane_ptr(tb, n2, k2), of[r]);
+                ux[r] = ld_raw(pu, r, it - 1);
+                vx[r] = ld_raw(pv, r, it - 1);
+                if (TRP) {
+                    eux[r] = E3(m.e3u, of[r], kcl(it - 1));
+                    evx[r] = E3(m.e3v, of[r], kcl(it - 1));
+                }
+                if (r * NH_BLOCK < NZR) wx[r] = ld_raw(pw, r, it - 1);
                 if (NV4 && r * NH_BLOCK < NBR) zox[r] = ldp(plane_ptr(zo, n2, k1), of[r]);
                 // the trend only at the tile's own cells (other tiles write theirs)
                 if (!TA0 && r * NH_BLOCK < NTL) tax[r] = ldp(plane_ptr((const T *)ta, n2, k1), of[r]);
+                e3c[r] = e3x[r];
+                e3x[r] = E3(m.e3t, of[r], k1);
+                if (sizeof(T) == 4) {
+                    e3dc[r] = e3dx[r];
+                    e3dx[r] = E3(d.md.e3t, of[r], k1);
+                }
             }
         }
         __syncthreads();
@@ -323,7 +342,7 @@ __global__ __launch_bounds__(NH_BLOCK, FuWaves<T>::value) void k_fct_fused(
                     if (inI[r]) {
                         const T ztra = -((hu[r] - s_hu[sl[r] - 1] + hv[r] - s_hv[sl[r] - PX] + upzt - upzb[r]) *
                                          r1e[r]);
-                        const T e3 = E3(m.e3t, of[r], it);
+                        const T e3 = e3c[r];
                         const T tm = wet ? T(1) : T(0);
                         zwi = (e3 * tbc[r] + zp2dt * ztra) / e3 * tm;
                         if (vT[r]) tl0[r] = (TA0 ? T(0) : tac[r]) + ztra / e3 * tm;
@@ -449,7 +468,8 @@ __global__ __launch_bounds__(NH_BLOCK, FuWaves<T>::value) void k_fct_fused(
                                         fmax2(0.0, c_kp1) - fmin2(0.0, c_k);
                     const double zneg = fmax2(0.0, a_i) - fmin2(0.0, a_im1) + fmax2(0.0, b_j) - fmin2(0.0, b_jm1) +
                                         fmax2(0.0, c_k) - fmin2(0.0, c_kp1);
-                    const double zbt = e1e2t_d[r] * E3(d.md.e3t, of[r], mB) / p2dt;
+                    const double e3d = sizeof(T) == 8 ? (double)e3p1[r] : e3dp1[r];   // level mB
+                    const double zbt = e1e2t_d[r] * e3d / p2dt;
                     const double paft_k = zwi1[r];
                     const double bu = ((double)zup - paft_k) / (zpos + zrtrn) * zbt;
                     const double bd = (paft_k - (double)zdo) / (zneg + zrtrn) * zbt;
@@ -496,7 +516,7 @@ __global__ __launch_bounds__(NH_BLOCK, FuWaves<T>::value) void k_fct_fused(
                 else lz_n = lim_face<T>(zz2[r], bd_c, s_bup[s2p][c], bu_c, s_bdo[s2p][c]);
                 if (nC == jpk - 1) lz_np1 = lim_face<T>(T(0), 0.0, bu_c, 0.0, bd_c);
                 else lz_np1 = lzc[r];
-                const T e3 = E3(m.e3t, of[r], nC);
+                const T e3 = e3p2[r];     // level nC
                 A3(ta, gi[r], gj[r], nC) =
                     tl2[r] - (lx_i - lx_im1 + ly_j - ly_jm1 + (T)lz_n - (T)lz_np1) * r1e[r] / e3;
                 lzc[r] = lz_n;
@@ -517,6 +537,9 @@ __global__ __launch_bounds__(NH_BLOCK, FuWaves<T>::value) void k_fct_fused(
             zz1[r] = zz0[r];
             tl2[r] = tl1[r];
             tl1[r] = tl0[r];
+            e3p2[r] = e3p1[r];
+            e3p1[r] = e3c[r];
+            e3dp1[r] = e3dc[r];
         }
         {
             const int k2 = max(it - 2, 1);

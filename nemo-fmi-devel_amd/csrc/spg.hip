@@ -379,17 +379,23 @@ __device__ __forceinline__ void spg_new_vel(const DevDomain &d, int i, int j, do
     va = (vn_e[ij] + rdtbt * (zwy + zv_trd + w.zv_frc[ij])) * SSVMASK(ij);
 }
 // S2: after sea level at (i,j), its time sum and zsshp2_e
-KERNEL void k_spg_s2(DevDomain d, double rdtbt, double wgt1, SpgSsh c, double *__restrict__ ssha_to,
-                     double *__restrict__ ssha, SpgWork w)
+__device__ __forceinline__ void spg_s2_point(const DevDomain &d, double rdtbt, double wgt1, const SpgSsh &c,
+                                             double *__restrict__ ssha_to, double *__restrict__ ssha,
+                                             const SpgWork &w, int i, int j)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
     const size_t ij = IJ(i, j);
     const double ssha_e = spg_ssha(d, rdtbt, c, w, i, j);
     ssha_to[ij] = ssha_e;
     // a zero weight leaves the sum (+0 from its reset) as it is
     if (wgt1 != 0.0) ssha[ij] = ssha[ij] + wgt1 * ssha_e;
     w.zsshp2_e[ij] = spg_p2(c, ssha_e, ij);
+}
+KERNEL void k_spg_s2(DevDomain d, double rdtbt, double wgt1, SpgSsh c, double *__restrict__ ssha_to,
+                     double *__restrict__ ssha, SpgWork w)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    spg_s2_point(d, rdtbt, wgt1, c, ssha_to, ssha, w, i, j);
 }
 
 __device__ __forceinline__ void spg_new_vel_p2(const DevDomain &d, int p, int q, double rdtbt, const double *un_e,
@@ -402,14 +408,15 @@ __device__ __forceinline__ void spg_new_vel_p2(const DevDomain &d, int p, int q,
 }
 
 // S3: after velocities and the next sub-step's extrapolation
-KERNEL void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double *__restrict__ un_e,
-                     const double *__restrict__ vn_e, double *__restrict__ ubb_to_a, double *__restrict__ vbb_to_a,
-                     const double *__restrict__ r1_hu_n, const double *__restrict__ r1_hv_n,
-                     const double *__restrict__ hu_n, const double *__restrict__ hv_n, const double *__restrict__ ht_n,
-                     double *__restrict__ ua_b, double *__restrict__ va_b, SpgWork w, SpgNext nx)
+__device__ __forceinline__ void spg_s3_point(const DevDomain &d, double rdtbt, double wgt1,
+                                             const double *__restrict__ un_e, const double *__restrict__ vn_e,
+                                             double *__restrict__ ubb_to_a, double *__restrict__ vbb_to_a,
+                                             const double *__restrict__ r1_hu_n, const double *__restrict__ r1_hv_n,
+                                             const double *__restrict__ hu_n, const double *__restrict__ hv_n,
+                                             const double *__restrict__ ht_n, double *__restrict__ ua_b,
+                                             double *__restrict__ va_b, const SpgWork &w, const SpgNext &nx, int i,
+                                             int j)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
     const size_t ij = IJ(i, j);
     // the after velocities at (i,j) after lbc_lnk('U') / ('V'): evaluated at
     // their source points; on the interior both sources are (i,j) and one
@@ -449,6 +456,115 @@ KERNEL void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double *__res
     nx.zwy[ij] = zwy;
     nx.un_adv[ij] = nx.un_adv[ij] + nx.wgt2 * zwx * m.r1_e2u[ij];
     nx.vn_adv[ij] = nx.vn_adv[ij] + nx.wgt2 * zwy * m.r1_e1v[ij];
+}
+KERNEL void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double *__restrict__ un_e,
+                     const double *__restrict__ vn_e, double *__restrict__ ubb_to_a, double *__restrict__ vbb_to_a,
+                     const double *__restrict__ r1_hu_n, const double *__restrict__ r1_hv_n,
+                     const double *__restrict__ hu_n, const double *__restrict__ hv_n, const double *__restrict__ ht_n,
+                     double *__restrict__ ua_b, double *__restrict__ va_b, SpgWork w, SpgNext nx)
+{
+    THREAD_IJ(1, 1);
+    if (i > d.jpi || j > d.jpj) return;
+    spg_s3_point(d, rdtbt, wgt1, un_e, vn_e, ubb_to_a, vbb_to_a, r1_hu_n, r1_hv_n, hu_n, hv_n, ht_n, ua_b, va_b, w,
+                 nx, i, j);
+}
+
+// ---------------------------------------------------------------------------
+// Phase 2 as one cooperative launch: every sub-step's S2 and S3 over the
+// workgroups' tiles, separated by grid barriers, with the level buffers
+// rotating in the kernel as the host loop rotates them.  Every workgroup is
+// resident (the grid is sized from the occupancy), so a barrier cannot wait
+// for a workgroup that has not started; each wait is still bounded, and a
+// barrier that gives up sets *status, which the library reports.
+// ---------------------------------------------------------------------------
+#define SPG_MAXIT 96
+#define SPG_SPIN_LIMIT (1u << 20)
+struct SpgLoop {
+    int icycle, ll_init, ntx, ntiles;
+    double rdtbt;
+    double wgt1[SPG_MAXIT], wgt2[SPG_MAXIT];
+    double *ube[3], *vbe[3], *sse[4];
+    double *ua_e2[2], *va_e2[2], *zwx2[2], *zwy2[2];
+    int ru0[3], rs0[4];
+    const double *r1_hu_n, *r1_hv_n, *hu_n, *hv_n, *ht_n;
+    double *ua_b, *va_b, *ssha, *un_adv, *vn_adv;
+};
+
+// bar[0]: arrivals, bar[1]: generation.  The last workgroup to arrive resets
+// the count and moves the generation on; the others wait for that.  Device-
+// scope fences around the barrier make the workgroups' stores of one phase
+// visible to the loads of the next in every XCD.
+__device__ __forceinline__ void spg_grid_barrier(unsigned *bar, int *status, unsigned nblocks)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned *count = bar, *gen = bar + 1;
+        const unsigned g = __hip_atomic_load(gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        const unsigned a = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        if (a == nblocks) {
+            __hip_atomic_store(count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            unsigned it = 0;
+            while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                if (++it > SPG_SPIN_LIMIT) {
+                    __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+// NTH threads per workgroup, tiles of 64 x NTH/64 points
+template <int NTH>
+__global__ __launch_bounds__(NTH) void k_spg_loop(DevDomain d, SpgWork w, SpgLoop L, unsigned *bar, int *status)
+{
+    constexpr int TYL = NTH / 64;
+    int ru[3] = {L.ru0[0], L.ru0[1], L.ru0[2]}, rs[4] = {L.rs0[0], L.rs0[1], L.rs0[2], L.rs0[3]};
+    const unsigned nb = gridDim.x;
+    const int tx = (int)(threadIdx.x & 63u), ty = (int)(threadIdx.x >> 6);
+    for (int jn = 1; jn <= L.icycle; jn++) {
+        const int cur = (jn - 1) & 1, nxt = jn & 1;
+        SpgWork wc = w;
+        wc.ua_e = L.ua_e2[cur]; wc.va_e = L.va_e2[cur]; wc.zwx = L.zwx2[cur]; wc.zwy = L.zwy2[cur];
+        double za0, za1, za2, za3;
+        if (jn == 1 && L.ll_init) { za0 = 1.0; za1 = 0.0; za2 = 0.0; za3 = 0.0; }
+        else if (jn == 2 && L.ll_init) { za0 = 1.0833333333333; za1 = -0.1666666666666; za2 = 0.0833333333333; za3 = 0.0; }
+        else { za0 = 0.614; za1 = 0.285; za2 = 0.088; za3 = 0.013; }
+        const SpgSsh c{za0, za1, za2, za3, L.sse[rs[0]], L.sse[rs[1]], L.sse[rs[2]]};
+        const double wgt1 = L.wgt1[jn - 1];
+        for (int t = (int)blockIdx.x; t < L.ntiles; t += (int)nb) {
+            const int i = (t % L.ntx) * 64 + tx + 1, j = (t / L.ntx) * TYL + ty + 1;
+            if (i <= d.jpi && j <= d.jpj) spg_s2_point(d, L.rdtbt, wgt1, c, L.sse[rs[3]], L.ssha, wc, i, j);
+        }
+        spg_grid_barrier(bar, status, nb);
+        SpgNext nx{};
+        nx.on = jn < L.icycle;
+        if (nx.on) {
+            if (jn + 1 < 3 && L.ll_init) { nx.za1 = 1.0; nx.za2 = 0.0; nx.za3 = 0.0; }
+            else { nx.za1 = 1.781105; nx.za2 = -1.06221; nx.za3 = 0.281105; }
+            nx.wgt2 = L.wgt2[jn];
+            nx.ub_before = L.ube[ru[1]];
+            nx.vb_before = L.vbe[ru[1]];
+            nx.ua_e = L.ua_e2[nxt]; nx.va_e = L.va_e2[nxt]; nx.zwx = L.zwx2[nxt]; nx.zwy = L.zwy2[nxt];
+            nx.un_adv = L.un_adv; nx.vn_adv = L.vn_adv;
+        }
+        for (int t = (int)blockIdx.x; t < L.ntiles; t += (int)nb) {
+            const int i = (t % L.ntx) * 64 + tx + 1, j = (t / L.ntx) * TYL + ty + 1;
+            if (i <= d.jpi && j <= d.jpj)
+                spg_s3_point(d, L.rdtbt, wgt1, L.ube[ru[0]], L.vbe[ru[0]], L.ube[ru[2]], L.vbe[ru[2]], L.r1_hu_n,
+                             L.r1_hv_n, L.hu_n, L.hv_n, L.ht_n, L.ua_b, L.va_b, wc, nx, i, j);
+        }
+        if (jn < L.icycle) spg_grid_barrier(bar, status, nb);
+        int q;
+        q = ru[2]; ru[2] = ru[1]; ru[1] = ru[0]; ru[0] = q;
+        q = rs[3]; rs[3] = rs[2]; rs[2] = rs[1]; rs[1] = rs[0]; rs[0] = q;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -565,6 +681,62 @@ int spg_ts_weights(const DevDomain &d, int ll_av, int ll_fw, int *jpit, double *
     return 0;
 }
 
+// phase 2 by one cooperative launch of k_spg_loop: returns false (and
+// launches nothing) when the tuning switch is off, a graph is being
+// recorded, the weights do not fit the kernel's table or the device cannot
+// hold the grid resident; the caller then runs the two-kernel loop.
+// Switch value 1: one 1024-thread workgroup per CU (the fewest workgroups
+// at each barrier); 2: 256-thread workgroups, as many as stay resident.
+static bool spg_loop_coop(const DevDomain &d, const SpgWork &w, const nemo_hip_spg_ts_t *st, int icycle, int ll_init,
+                          double rdtbt, const double *wgt1, const double *wgt2, double *const *ube,
+                          double *const *vbe, double *const *sse, double *const *ua_e2, double *const *va_e2,
+                          double *const *zwx2, double *const *zwy2, const int *ru, const int *rs)
+{
+    const int mode = nh::tune(nh::T_SPG_PERSISTENT);
+    if (!mode || nh::capturing() || icycle > SPG_MAXIT || icycle < 1) return false;
+    const int nth = mode == 1 ? 1024 : NH_BLOCK;
+    const void *kern = mode == 1 ? (const void *)k_spg_loop<1024> : (const void *)k_spg_loop<NH_BLOCK>;
+    unsigned *bar = nh::spg_barrier();
+    int *status = (int *)(bar + 2);
+    int dev = 0, coop = 0, ncu = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev) != hipSuccess || !coop) return false;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return false;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, nth, 0) != hipSuccess || per_cu < 1)
+        return false;
+    if (mode == 1) per_cu = 1;
+    SpgLoop L{};
+    L.icycle = icycle;
+    L.ll_init = ll_init;
+    L.ntx = (d.jpi + 63) / 64;
+    L.ntiles = L.ntx * ((d.jpj + nth / 64 - 1) / (nth / 64));
+    L.rdtbt = rdtbt;
+    for (int q = 0; q < icycle; q++) {
+        L.wgt1[q] = wgt1[q];
+        L.wgt2[q] = wgt2[q];
+    }
+    for (int q = 0; q < 3; q++) { L.ube[q] = ube[q]; L.vbe[q] = vbe[q]; L.ru0[q] = ru[q]; }
+    for (int q = 0; q < 4; q++) { L.sse[q] = sse[q]; L.rs0[q] = rs[q]; }
+    for (int q = 0; q < 2; q++) { L.ua_e2[q] = ua_e2[q]; L.va_e2[q] = va_e2[q]; L.zwx2[q] = zwx2[q]; L.zwy2[q] = zwy2[q]; }
+    L.r1_hu_n = st->r1_hu_n; L.r1_hv_n = st->r1_hv_n; L.hu_n = st->hu_n; L.hv_n = st->hv_n; L.ht_n = st->ht_n;
+    L.ua_b = st->ua_b; L.va_b = st->va_b; L.ssha = st->ssha; L.un_adv = st->un_adv; L.vn_adv = st->vn_adv;
+    const unsigned grid = (unsigned)(per_cu * ncu < L.ntiles ? per_cu * ncu : L.ntiles);
+    DevDomain dd = d;
+    SpgWork ww = w;
+    void *args[] = {(void *)&dd, (void *)&ww, (void *)&L, (void *)&bar, (void *)&status};
+    hipEvent_t ev0, ev1;
+    const bool timed = nh::timing_pair("dyn_spg_ts", "k_spg_loop", &ev0, &ev1);
+    if (timed) (void)hipEventRecord(ev0, nh::stream());
+    const hipError_t e = hipLaunchCooperativeKernel(kern, dim3(grid), dim3(nth), args, 0, nh::stream());
+    if (timed) (void)hipEventRecord(ev1, nh::stream());
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return false;
+    }
+    nh::spg_loop_launched();
+    return true;
+}
+
 template <typename T>
 int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
 {
@@ -630,6 +802,16 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
         ab3(1, za1, za2, za3);
         LAUNCH("dyn_spg_ts", k_spg_s1, g, d, za1, za2, za3, wgt2[0], ube[ru[0]], ube[ru[1]], ube[ru[2]],
                vbe[ru[0]], vbe[ru[1]], vbe[ru[2]], st->hu_n, st->hv_n, st->un_adv, st->vn_adv, w);
+    }
+    if (spg_loop_coop(d, w, st, icycle, ll_init, rdtbt, wgt1, wgt2, ube, vbe, sse, ua_e2, va_e2, zwx2, zwy2, ru,
+                      rs)) {
+        // the kernel rotated the levels icycle times; so do the host's indices
+        for (int jn = 1; jn <= icycle; jn++) {
+            int t;
+            t = ru[2]; ru[2] = ru[1]; ru[1] = ru[0]; ru[0] = t;
+            t = rs[3]; rs[3] = rs[2]; rs[2] = rs[1]; rs[1] = rs[0]; rs[0] = t;
+        }
+        icycle = 0;   // the loop below has nothing left to do
     }
     for (int jn = 1; jn <= icycle; jn++) {
         const int cur = (jn - 1) & 1, nxt = jn & 1;

@@ -1350,6 +1350,165 @@ KERNEL void k_tra_zdf(DevDomain d, double p2dt, const T * __restrict__ ptb, T * 
     }
 }
 
+// FP32 storage with at most KMAX - 1 solved levels: the same solve with the
+// forward-substituted values (rounded to storage, as k_tra_zdf stores them)
+// kept on chip instead of written to pta and read back (the first tracer's
+// in registers, the second's in LDS, one column per thread), and the FP64
+// pivots kept at every REG_SEG-th level, the others recomputed segment by
+// segment in the back substitution from a second read of the diffusivity.
+// Every loop over levels is unrolled to KMAX, so the register array is
+// indexed by constants; the level count only guards.  Results equal
+// k_tra_zdf<float>'s bit for bit.
+#define REG_SEG 8
+template <int NT, int KMAX>
+__global__ __launch_bounds__(NH_BLOCK, 2) void k_tra_zdf_reg(DevDomain d, double p2dt, const float *__restrict__ ptb,
+                                                             float *__restrict__ pta, int jn0,
+                                                             const float *__restrict__ src, int l_ldfslp,
+                                                             const float *__restrict__ add, int clamp_jn)
+{
+    THREAD_IJ(2, 2);
+    if (i > d.jpi - 1 || j > d.jpj - 1) return;
+    const uint32_t ij = IJ32(i, j);
+    const size_t plane = d.n2, n3 = d.n3;
+    const Metrics<float> &m = met<float>(d);
+    const int jpk = d.jpk;
+    const float *tb[NT];
+    float *ta[NT];
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+        tb[n] = ptb + (size_t)(jn0 + n) * n3;
+        ta[n] = pta + (size_t)(jn0 + n) * n3;
+    }
+    const float *addp = l_ldfslp ? add : src;
+    auto coef = [&](int k) -> double {
+        const double sv = (double)ldp(plane_ptr(src, plane, k), ij);
+        const double av = (double)ldp(plane_ptr(addp, plane, k <= jpk - 1 ? k : jpk - 1), ij);
+        double c = sv;
+        if (l_ldfslp && k <= jpk - 1) c = c + av;
+        return k <= 1 ? 0.0 : c;
+    };
+    constexpr int NSEG = (KMAX + REG_SEG - 1) / REG_SEG;
+    // level k of tracer n at index k - 1: tracer 0 in registers, tracer 1 in LDS
+    float fw0[KMAX];
+    __shared__ float s_fw1[NT > 1 ? KMAX : 1][NH_BLOCK];
+    const int lt = (int)(threadIdx.y * blockDim.x + threadIdx.x);
+    auto fw_put = [&](int n, int x, float v) {
+        if (n == 0) fw0[x] = v;
+        else s_fw1[x][lt] = v;
+    };
+    auto fw_get = [&](int n, int x) -> float { return n == 0 ? fw0[x] : s_fw1[x][lt]; };
+    double ck[NSEG];               // the pivot of level s * REG_SEG at index s
+    double prev[NT];
+    double zws_prev, zwt_prev, zc_k;
+    {
+        const double zc_kp1 = coef(2);
+        const double e3t = (double)E3(m.e3t, ij, 1);
+        const double zwi = -(p2dt * 0.0 / (double)E3(m.e3w, ij, 1));
+        const double zws = -(p2dt * zc_kp1 / (double)E3(m.e3w, ij, 2));
+        const double zwd = e3t - zwi - zws;
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            const double v = e3t * (double)ldp(tb[n], ij) + p2dt * e3t * (double)ldp((const float *)ta[n], ij);
+            prev[n] = v;
+            fw_put(n, 0, (float)v);
+        }
+        zws_prev = zws;
+        zwt_prev = zwd;
+        zc_k = zc_kp1;
+    }
+#pragma unroll
+    for (int k = 2; k < KMAX; k++) {
+        if (k <= jpk - 1) {
+            const double zc_kp1 = coef(k + 1);
+            const double e3t = (double)E3(m.e3t, ij, k);
+            const double zwi = -(p2dt * zc_k / (double)E3(m.e3w, ij, k));
+            const double zws = -(p2dt * zc_kp1 / (double)E3(m.e3w, ij, k + 1));
+            const double zwd = e3t - zwi - zws;
+            const double zwt_k = zwd - zwi * zws_prev / zwt_prev;
+            if (k % REG_SEG == 0) ck[k / REG_SEG] = zwt_k;
+#pragma unroll
+            for (int n = 0; n < NT; n++) {
+                const double zrhs = e3t * (double)ldp(plane_ptr(tb[n], plane, k), ij) +
+                                    p2dt * e3t * (double)ldp((const float *)plane_ptr(ta[n], plane, k), ij);
+                const double v = zrhs - zwi / zwt_prev * prev[n];
+                prev[n] = v;
+                fw_put(n, k - 1, (float)v);
+            }
+            zws_prev = zws;
+            zwt_prev = zwt_k;
+            zc_k = zc_kp1;
+        }
+    }
+    {
+        const double tm = TMASK(double, i, j, jpk - 1);
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+            const double v = prev[n] / zwt_prev * tm;
+            prev[n] = v;
+            float out = (float)v;
+            if (jn0 + n == clamp_jn && out < 0.f) out = 0.1f;
+            stp(plane_ptr(ta[n], plane, jpk - 1), ij, out);
+        }
+    }
+    const int t0 = d.kT.k0[ij], t1 = d.kT.k1[ij];
+    const int kl = jpk - 2;
+#pragma unroll
+    for (int s = NSEG - 1; s >= 0; s--) {
+        const int k0 = s * REG_SEG + 1;
+        if (k0 <= kl) {
+            double piv[REG_SEG], zsv[REG_SEG];
+            double zc = coef(k0);
+            double zwt_p = 1.0, zws_p = 0.0;
+            if (s > 0) {
+                zwt_p = ck[s];
+                zws_p = -(p2dt * zc / (double)E3(m.e3w, ij, k0));
+            }
+#pragma unroll
+            for (int q = 0; q < REG_SEG; q++) {
+                const int k = k0 + q;
+                if (k < KMAX && k <= kl) {
+                    const double zc_kp1 = coef(k + 1);
+                    const double e3t = (double)E3(m.e3t, ij, k);
+                    const double zwi = -(p2dt * zc / (double)E3(m.e3w, ij, k));
+                    const double zws = -(p2dt * zc_kp1 / (double)E3(m.e3w, ij, k + 1));
+                    const double zwd = e3t - zwi - zws;
+                    piv[q] = k == 1 ? zwd : zwd - zwi * zws_p / zwt_p;
+                    zsv[q] = zws;
+                    zws_p = zws;
+                    zwt_p = piv[q];
+                    zc = zc_kp1;
+                }
+            }
+#pragma unroll
+            for (int q = REG_SEG - 1; q >= 0; q--) {
+                const int k = k0 + q;
+                if (k < KMAX && k <= kl) {
+                    const double tm = (k >= t0 && k <= t1) ? 1.0 : 0.0;
+#pragma unroll
+                    for (int n = 0; n < NT; n++) {
+                        const double v = ((double)fw_get(n, k - 1) - zsv[q] * prev[n]) / piv[q] * tm;
+                        prev[n] = v;
+                        float out = (float)v;
+                        if (jn0 + n == clamp_jn && out < 0.f) out = 0.1f;
+                        stp(plane_ptr(ta[n], plane, k), ij, out);
+                    }
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int n = 0; n < NT; n++) {
+        if (jn0 + n == clamp_jn) {
+            float *sa = plane_ptr(ta[n], plane, jpk);
+            if (ldp((const float *)sa, ij) < 0.f) stp(sa, ij, 0.1f);
+        }
+    }
+}
+
+// the levels k_tra_zdf_reg holds on chip (jpk <= ZDF_REG_KMAX): 75 of the
+// full-size configurations; 76 x 256 FP32 values of LDS per workgroup
+#define ZDF_REG_KMAX 76
+
 // WHERE( tsa(:,:,:,jp_sal) < 0 ) tsa = 0.1 on the columns the solve does not touch
 template <typename T>
 KERNEL void k_clamp_ring(DevDomain d, T * __restrict__ sa)
@@ -1358,6 +1517,22 @@ KERNEL void k_clamp_ring(DevDomain d, T * __restrict__ sa)
     if (i > d.jpi || j > d.jpj || is_interior(d, i, j)) return;
     for (int k = 1; k <= d.jpk; k++)
         if (A3(sa, i, j, k) < T(0)) A3(sa, i, j, k) = T(0.1);
+}
+
+// one solve of NT tracers sharing the matrix: the register form where it
+// applies (FP32 storage, jpk <= ZDF_REG_KMAX, switch zdf_reg), else k_tra_zdf
+template <typename T, int NT>
+static int zdf_group(const char *name, const DevDomain &d, Grid2 g, double p2dt, const T *ptb, T *pta, int jn,
+                     const T *src, int l_ldfslp, const T *add, double *zwt, int clamp_jn)
+{
+    if constexpr (sizeof(T) == 4) {
+        if (d.jpk <= ZDF_REG_KMAX && nh::tune(nh::T_ZDF_REG)) {
+            LAUNCH(name, (k_tra_zdf_reg<NT, ZDF_REG_KMAX>), g, d, p2dt, ptb, pta, jn, src, l_ldfslp, add, clamp_jn);
+            return 0;
+        }
+    }
+    LAUNCH(name, (k_tra_zdf<T, NT>), g, d, p2dt, ptb, pta, jn, src, l_ldfslp, add, zwt, clamp_jn);
+    return 0;
 }
 
 template <typename T>
@@ -1370,17 +1545,20 @@ int launch_tra_zdf_imp(double p2dt, int is_tra, const T *ptb, T *pta, int kjpt, 
     const int clamp_jn = clamp_sal ? 1 : -1;
     // matrix groups: TRA with ln_zdfddm: T on avt, S on avs; otherwise one
     // matrix (avt for TRA, avs for TRC) shared by every tracer
+    int st;
     if (is_tra && d.ln_zdfddm) {
         for (int jn = 0; jn < kjpt; jn++)
-            LAUNCH("tra_zdf", (k_tra_zdf<T, 1>), g, d, p2dt, ptb, pta, jn, jn == 0 ? avt : avs, l_ldfslp, add, zwt,
-                   clamp_jn);
+            if ((st = zdf_group<T, 1>("tra_zdf", d, g, p2dt, ptb, pta, jn, jn == 0 ? avt : avs, l_ldfslp, add, zwt,
+                                      clamp_jn)))
+                return st;
     } else {
         const T *src = is_tra ? avt : avs;
         int jn = 0;
         for (; jn + 2 <= kjpt; jn += 2)
-            LAUNCH("tra_zdf", (k_tra_zdf<T, 2>), g, d, p2dt, ptb, pta, jn, src, l_ldfslp, add, zwt, clamp_jn);
-        if (jn < kjpt)
-            LAUNCH("tra_zdf", (k_tra_zdf<T, 1>), g, d, p2dt, ptb, pta, jn, src, l_ldfslp, add, zwt, clamp_jn);
+            if ((st = zdf_group<T, 2>("tra_zdf", d, g, p2dt, ptb, pta, jn, src, l_ldfslp, add, zwt, clamp_jn)))
+                return st;
+        if (jn < kjpt && (st = zdf_group<T, 1>("tra_zdf", d, g, p2dt, ptb, pta, jn, src, l_ldfslp, add, zwt, clamp_jn)))
+            return st;
     }
     if (clamp_sal) {
         Grid2 ga = grid2(d.jpi, d.jpj);
@@ -1679,8 +1857,7 @@ int launch_step_tra(double p2dt, double z2dt, double p2dt_zdf, const T *un, cons
     double *zwt = (double *)nh::scratch3(8);
     const T *add = d.ln_traldf_msc ? (const T *)akz : (const T *)ah_wslp2;
     Grid2 gi = grid2(d.jpi - 2, d.jpj - 2);
-    LAUNCH("step_tra", (k_tra_zdf<T, 2>), gi, d, p2dt_zdf, tsb, tsa, 0, avt, 1, add, zwt, 1);
-    return 0;
+    return zdf_group<T, 2>("step_tra", d, gi, p2dt_zdf, tsb, tsa, 0, avt, 1, add, zwt, 1);
 }
 template PivotTab<double> cpt_pivots<double>(int);
 template PivotTab<float> cpt_pivots<float>(int);

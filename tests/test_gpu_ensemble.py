@@ -1,9 +1,20 @@
 """An ensemble of domains of different sizes (one library context each),
-stepped in FP32 storage by the fused step entry points, recorded once into
-a HIP graph and replayed: every replay must give, bit for bit, what the same
-calls give when issued one by one.  The replays are interleaved with eager
-work and synchronisations, and the recording refuses the calls it cannot
-replay faithfully."""
+stepped by the fused step entry points, recorded once into a HIP graph and
+replayed:
+
+* every replay gives, bit for bit, what the same calls give when issued one
+  by one (FP32 storage); the replays are interleaved with eager work and
+  synchronisations, and the recording refuses the calls it cannot replay
+  faithfully;
+* 24 members of uneven sizes (56 levels, closed and cyclic, each its own
+  seeded land mask, bathymetry and initial state) advanced by whole time
+  steps (the hot path and its closure), the step recorded once for all
+  members: after 8 steps each member equals the oracle's trajectory of the
+  same member, bit for bit in FP64 and within the FP32 storage form's
+  tolerances.
+"""
+import json
+
 import numpy as np
 import pytest
 
@@ -82,5 +93,67 @@ def test_ensemble_graph_replays_match_eager_calls(hip):
             hip.graph_free(g)
         hip.context_select(0)
         del eager, graph
+        for c in ctxs:
+            hip.context_destroy(c)
+
+
+# 24 members: i x j from 24 x 20 to 70 x 58, 56 levels; every third one
+# east-west cyclic
+RNG = np.random.default_rng(20261203)
+UNEVEN = [(int(RNG.integers(24, 71)), int(RNG.integers(20, 59)), 56, 7 if m % 3 == 2 else 0, 200 + m)
+          for m in range(24)]
+# FP32 storage against the FP64 oracle after 8 steps (the trajectory tests'
+# bars, tracers at the 1e-5 relative bar of the precision form)
+SP_TOL = {"tsn": 1e-5, "tsb": 1e-5, "un": 3e-6, "vn": 3e-6, "sshn": 3e-6, "un_b": 2e-6, "vn_b": 2e-6}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("sp", [False, True], ids=["fp64", "fp32"])
+def test_uneven_ensemble_graph_against_oracle(hip, sp):
+    from trajectory import OUT_2D, OUT_3D, OUT_4D, HotStep, oracle_run
+    last = 8
+    ctxs, members, doms, g = [], [], [], None
+    try:
+        for (jpi, jpj, jpk, jperio, seed) in UNEVEN:
+            c = hip.context_create()
+            ctxs.append(c)
+            hip.context_select(c)
+            d = configs.basin(jpi, jpj, jpk, seed=seed, jperio=jperio, opts=dict(rdt=600.0))
+            hip.register(d)
+            S = configs.make_state(d, seed=seed + 1)
+            doms.append((d, S))
+            members.append(HotStep(hip, d, S, sp))
+        for kt in (1, 2):                      # Euler start and first leapfrog: eager
+            for c, m in zip(ctxs, members):
+                hip.context_select(c)
+                m.step(kt)
+        hip.graph_begin()
+        for c, m in zip(ctxs, members):
+            hip.context_select(c)
+            m.step(3)
+        g = hip.graph_end()
+        for kt in range(3, last + 1):
+            hip.graph_launch(g)
+        hip.sync()
+        torch.cuda.synchronize()
+        worst = {}
+        for i, ((d, S), m) in enumerate(zip(doms, members)):
+            got = m.fields()
+            want = oracle_run(d, S, (last,))[last]
+            for k in OUT_3D + OUT_4D + OUT_2D:
+                a, b = got[k], want[k]
+                assert np.all(np.isfinite(a)), f"member {i} {k}: non-finite"
+                if not sp:
+                    assert np.array_equal(a, b), f"member {i} {d.jpi}x{d.jpj} {k}: max |diff| {np.abs(a - b).max():.3e}"
+                elif k in SP_TOL:
+                    e = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+                    worst[k] = max(worst.get(k, 0.0), e)
+                    assert e <= SP_TOL[k], f"member {i} {d.jpi}x{d.jpj} {k}: relative error {e:.2e}"
+        print(json.dumps({"members": len(members), "steps": last, "sp": sp, "max_relerr": worst}))
+    finally:
+        if g is not None:
+            hip.graph_free(g)
+        hip.context_select(0)
+        del members
         for c in ctxs:
             hip.context_destroy(c)

@@ -7,7 +7,9 @@ compared exactly.  FP32-storage form: relative error against the FP64 oracle
 within the tolerance written in each test.
 
 Cases: the 32x22x31 GYRE-like closed basin and small seeded basins with land
-(closed, cyclic E-W, cyclic E-W + N-S) so every lbc_lnk branch is exercised.
+(closed, cyclic E-W, cyclic E-W + N-S) so every lbc_lnk branch is exercised,
+and a 75-level cyclic basin (44x36x75), the depth of the full-size
+configurations, for the column arrays sized by the level count.
 """
 import numpy as np
 import pytest
@@ -22,6 +24,7 @@ CASES = {
     "basin": lambda opts=None: configs.basin(40, 30, 12, seed=3, opts=opts),
     "cyc7": lambda opts=None: configs.basin(40, 30, 12, seed=4, jperio=7, opts=opts),
     "cyc1": lambda opts=None: configs.basin(36, 28, 10, seed=5, jperio=1, opts=opts),
+    "deep75": lambda opts=None: configs.basin(44, 36, 75, seed=6, jperio=7, opts=opts),
 }
 CASE_IDS = list(CASES)
 

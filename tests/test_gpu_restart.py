@@ -4,7 +4,8 @@ DYN/dynspg_ts.F90:1391-1463):
 * a run of 8 steps equals, bit for bit, 4 steps, a restart written at
   step 4, and 4 more steps in a context registered with nit000 = 5 and a
   leapfrog start, the restarted state read into NaN-filled fields and the
-  depth-mean velocities recomputed as DOM/istate.F90 does;
+  depth-mean velocities recomputed as DOM/istate.F90 does; the restarted
+  state at step 8 also matches the oracle's 8-step trajectory (1e-12);
 * the files round-trip every variable exactly, in device and host pointer
   modes and in the FP32 storage form (and a file of one form reads into the
   other, widened or rounded);
@@ -22,7 +23,7 @@ import nemo_hip
 from nemo_hip import NEMO_HIP_PTR_DEVICE, NEMO_HIP_PTR_HOST, configs
 from test_gpu_parity import check
 from test_gpu_trajectory import CASES
-from trajectory import OUT_2D, OUT_3D, OUT_4D, gpu_run, initial
+from trajectory import OUT_2D, OUT_3D, OUT_4D, gpu_run, initial, oracle_run
 
 torch = pytest.importorskip("torch")
 
@@ -76,6 +77,9 @@ def test_restart_continues_bit_for_bit(hip, case, tmp_path):
         check(rest[8][k], full[8][k], what=f"{case} restarted step 8 {k}")
         ident &= bool(np.array_equal(rest[8][k], full[8][k]))
     assert ident, "the restarted run should equal the continuous run bit for bit"
+    want = oracle_run(d, S, (8,))[8]
+    for k in OUT_3D + OUT_4D + OUT_2D:
+        check(rest[8][k], want[k], what=f"{case} restarted step 8 {k} against the oracle's trajectory")
 
 
 @pytest.mark.gpu

@@ -133,26 +133,78 @@ def oracle_run(d, S, marks, physics=False, eos=None):
     return out
 
 
+class HotStep:
+    """One member's device-resident state and the hot-path whole step of
+    gpu_run without physics.  step() issues library calls only (nothing on
+    torch's streams), so an ensemble of members can record one step of each
+    into a HIP graph and replay it.  X0 (device tensors by field name)
+    replaces the initial state where given."""
+
+    def __init__(self, h, d, S, sp=False, X0=None):
+        import torch
+        self.h, self.d, self.sp = h, d, sp
+        dt = torch.float32 if sp else torch.float64
+        self.dt = dt
+        n2, n3 = d.jpi * d.jpj, d.jpi * d.jpj * d.jpk
+        ndim = {k: len(np.shape(v)) for k, v in initial(S).items()}
+        dev = lambda a, t: torch.from_numpy(np.ascontiguousarray(np.asarray(a).ravel(order="F"))).to("cuda", t)
+        X = {k: dev(v, dt if ndim[k] >= 3 else torch.float64) for k, v in initial(S).items()}
+        for k in ("ua_b", "va_b", "ssha", "un_adv", "vn_adv"):
+            X[k] = torch.zeros(n2, dtype=torch.float64, device="cuda")
+        X.update(X0 or {})
+        z = lambda n: torch.zeros(n, dtype=dt, device="cuda")
+        self.z = z
+        self.hdivn, self.ua, self.va, self.tsa, self.ah, self.akz = z(n3), z(n3), z(n3), z(2 * n3), z(n3), z(n3)
+        self.F = {k: X[k] for k in SPG_2D + ["ua_b", "va_b", "ssha", "un_adv", "vn_adv"]}
+        self.F.update(ua=self.ua, va=self.va, un=X["un"], vn=X["vn"])
+        self.X = X
+        s2, s3, s4 = (d.jpi, d.jpj), (d.jpi, d.jpj, d.jpk), (d.jpi, d.jpj, d.jpk, 2)
+        self.shape = {k: (s4 if k in OUT_4D else s3 if k in OUT_3D else s2) for k in OUT_3D + OUT_4D + OUT_2D}
+
+    def open(self, kt):
+        """ssh_nxt and wzv: the first guess of ssha and the now w"""
+        h, X, sp = self.h, self.X, self.sp
+        h.ssh_nxt(kt, X["un"], X["vn"], self.hdivn, X["sshb"], X["emp_b"], X["emp"], X["ssha"], sp=sp)
+        h.wzv(kt, self.hdivn, X["wn"], sp=sp)
+
+    def middle(self, kt):
+        """div_hor and wzv of the corrected now velocities"""
+        self.h.div_hor(kt, self.X["un"], self.X["vn"], self.hdivn, sp=self.sp)
+        self.h.wzv(kt, self.hdivn, self.X["wn"], sp=self.sp)
+
+    def close(self, kt):
+        """tra_nxt, dyn_nxt and ssh_swp"""
+        h, X, sp = self.h, self.X, self.sp
+        h.tra_nxt(kt, X["tsb"], X["tsn"], self.tsa, sp=sp)
+        h.dyn_nxt(kt, X["ub"], X["vb"], X["un"], X["vn"], self.ua, self.va, X["ua_b"], X["va_b"], X["un_adv"],
+                  X["vn_adv"], X["ub_b"], X["vb_b"], X["un_b"], X["vn_b"], X["r1_hu_n"], X["r1_hv_n"], sp=sp)
+        h.ssh_swp(kt, X["sshb"], X["sshn"], X["ssha"])
+
+    def step(self, kt):
+        h, X, sp = self.h, self.X, self.sp
+        self.open(kt)
+        h.step_dyn(kt, X["ub"], X["vb"], X["wn"], X["avm"], X["utau_b"], X["vtau_b"], self.F, sp=sp)
+        self.middle(kt)
+        h.step_tra(kt, X["un"], X["vn"], X["wn"], X["tsb"], X["tsn"], self.tsa, *KN_FCT, X["ahtu"], X["ahtv"],
+                   X["uslp"], X["vslp"], X["wslpi"], X["wslpj"], self.ah, self.akz, X["avt"], sp=sp)
+        self.close(kt)
+
+    def fields(self):
+        """the prognostic fields, FP64 host arrays (call after a synchronisation)"""
+        return {k: self.X[k].cpu().numpy().astype(np.float64).reshape(s, order="F") for k, s in self.shape.items()}
+
+
 def gpu_run(h, d, S, marks, sp=False, physics=False, eos=None, kt0=1, X0=None, keep=None):
     """h: the library handle in device-pointer mode with d registered.
     Steps kt0 .. max(marks); X0 (device tensors by field name) replaces the
     initial state where given; `keep`, a dict, receives the final fields."""
     import torch
     from nemo_hip import abi, configs
-    dt = torch.float32 if sp else torch.float64
-    s2, s3, s4 = (d.jpi, d.jpj), (d.jpi, d.jpj, d.jpk), (d.jpi, d.jpj, d.jpk, 2)
+    st = HotStep(h, d, S, sp, X0)
+    X, F, z, dt = st.X, st.F, st.z, st.dt
+    ua, va, tsa, ah, akz = st.ua, st.va, st.tsa, st.ah, st.akz
     n2, n3 = d.jpi * d.jpj, d.jpi * d.jpj * d.jpk
-    ndim = {k: len(np.shape(v)) for k, v in initial(S).items()}
     dev = lambda a, t: torch.from_numpy(np.ascontiguousarray(np.asarray(a).ravel(order="F"))).to("cuda", t)
-    X = {k: dev(v, dt if ndim[k] >= 3 else torch.float64) for k, v in initial(S).items()}
-    for k in ("ua_b", "va_b", "ssha", "un_adv", "vn_adv"):
-        X[k] = torch.zeros(n2, dtype=torch.float64, device="cuda")
-    X.update(X0 or {})
-    z = lambda n: torch.zeros(n, dtype=dt, device="cuda")
-    hdivn, ua, va, tsa, ah, akz = z(n3), z(n3), z(n3), z(2 * n3), z(n3), z(n3)
-    F = {k: X[k] for k in SPG_2D + ["ua_b", "va_b", "ssha", "un_adv", "vn_adv"]}
-    F.update(ua=ua, va=va, un=X["un"], vn=X["vn"])
-    shape = {k: (s4 if k in OUT_4D else s3 if k in OUT_3D else s2) for k in OUT_3D + OUT_4D + OUT_2D}
     if physics:
         h.eos_register(eos or abi.seos())
         gdept, gdepw = (dev(a, dt) for a in configs.depths(d))
@@ -171,16 +223,16 @@ def gpu_run(h, d, S, marks, sp=False, physics=False, eos=None, kt0=1, X0=None, k
     # torch work (the trend reset) is queued on the library stream, in order with the library's launches
     lib_stream = torch.cuda.ExternalStream(h.stream())
     for kt in range(kt0, max(marks) + 1):
-        if physics:
+        if not physics:
+            st.step(kt)
+        else:
             h.rab_3d(X["tsb"], pab, gdept, sp=sp)
             h.bn2(X["tsb"], pab, rn2b, gdept, gdepw, sp=sp)
             h.zdf_mxl(kt, rn2b, gdept, gdepw, nlb10, nmln, hmlp, hmlpt, sp=sp)
             h.eos_insitu(X["tsb"], rhd, gdept, sp=sp)
             h.ldf_slp(kt, rhd, rn2b, gdept, gdepw, nmln, hmlp, hmlpt, RN_SLPMAX, X["uslp"], X["vslp"], X["wslpi"],
                       X["wslpj"], sp=sp)
-        h.ssh_nxt(kt, X["un"], X["vn"], hdivn, X["sshb"], X["emp_b"], X["emp"], X["ssha"], sp=sp)
-        h.wzv(kt, hdivn, X["wn"], sp=sp)
-        if physics:
+            st.open(kt)
             h.eos_insitu_pot(X["tsn"], rhd, rhop, gdept, sp=sp)
             with torch.cuda.stream(lib_stream):
                 ua.zero_()
@@ -190,14 +242,9 @@ def gpu_run(h, d, S, marks, sp=False, physics=False, eos=None, kt0=1, X0=None, k
             h.dyn_ldf_lap(kt, X["ub"], X["vb"], ua, va, 1, ahmt, ahmf, sp=sp)
             h.hpg_zco(kt, rhd, ua, va, sp=sp)
             h.dyn_spg_ts(kt, F, sp=sp)
-        else:
-            h.step_dyn(kt, X["ub"], X["vb"], X["wn"], X["avm"], X["utau_b"], X["vtau_b"], F, sp=sp)
-        h.div_hor(kt, X["un"], X["vn"], hdivn, sp=sp)
-        h.wzv(kt, hdivn, X["wn"], sp=sp)
-        if physics:
+            st.middle(kt)
             h.dyn_zdf(kt, X["ub"], X["vb"], ua, va, X["avm"], X["rCdU_bot"], X["ua_b"], X["va_b"], X["utau"],
                       X["utau_b"], X["vtau"], X["vtau_b"], sp=sp)
-        if physics:
             with torch.cuda.stream(lib_stream):
                 tsa.zero_()
                 qns.copy_(qns0)
@@ -208,17 +255,11 @@ def gpu_run(h, d, S, marks, sp=False, physics=False, eos=None, kt0=1, X0=None, k
             h.tra_ldf_iso(kt, 1, "TRA", X["ahtu"], X["ahtv"], X["tsb"], X["tsb"], tsa, 2, 1, X["uslp"], X["vslp"],
                           X["wslpi"], X["wslpj"], ah, akz, sp=sp)
             h.tra_zdf(kt, X["tsb"], tsa, X["avt"], X["avt"], 1, ah, akz, sp=sp)
-        else:
-            h.step_tra(kt, X["un"], X["vn"], X["wn"], X["tsb"], X["tsn"], tsa, *KN_FCT, X["ahtu"], X["ahtv"],
-                       X["uslp"], X["vslp"], X["wslpi"], X["wslpj"], ah, akz, X["avt"], sp=sp)
-        h.tra_nxt(kt, X["tsb"], X["tsn"], tsa, sp=sp)
-        h.dyn_nxt(kt, X["ub"], X["vb"], X["un"], X["vn"], ua, va, X["ua_b"], X["va_b"], X["un_adv"], X["vn_adv"],
-                  X["ub_b"], X["vb_b"], X["un_b"], X["vn_b"], X["r1_hu_n"], X["r1_hv_n"], sp=sp)
-        h.ssh_swp(kt, X["sshb"], X["sshn"], X["ssha"])
+            st.close(kt)
         if kt in marks:
             torch.cuda.synchronize()
             h.sync()
-            out[kt] = {k: X[k].cpu().numpy().astype(np.float64).reshape(shape[k], order="F") for k in shape}
+            out[kt] = st.fields()
     if keep is not None:
         keep.update(X)
     return out
