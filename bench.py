@@ -83,6 +83,12 @@ def parse():
                     help="finite-difference directions for the adjoint gradient check")
     ap.add_argument("--seed", type=int, default=20261204)
     ap.add_argument("--timing-steps", type=int, default=2)
+    ap.add_argument("--tuning", default=os.path.join(ROOT, "nemo-fmi-devel_amd", "nemo_hip", "tuning.json"),
+                    metavar="PATH", help="settings of the library's switches per workload (nemo_hip.autotune); the "
+                    "setting saved for a form's workload is applied before that form is timed")
+    ap.add_argument("--autotune", default="", metavar="PATH",
+                    help="first tune the switches on each form's step (nemo_hip.autotune) and save the settings to "
+                         "PATH (then applied, as from --tuning)")
     ap.add_argument("--tune", default="", metavar="KEY=V,...",
                     help="library tuning switches (nemo_hip_tune_set) to set before timing, e.g. spg_persistent=1")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -254,6 +260,28 @@ class Stepper:
         h.tra_zdf(kt, self.tsb, self.tsa, f["avt"], f["avs"], 1, self.ah, self.akz, sp=sp)
 
 
+def autotune_form(h, d, S, form, a, log):
+    """tune the library's switches on one form's step at the bench workload
+    and save the setting to a.autotune under "<form> <size>" """
+    import torch
+    from nemo_hip import autotune
+    st = Stepper(h, d, S, form.startswith("fp32"), form.endswith("_fused"))
+    kt = [2]
+
+    def step():
+        st.step(kt[0])
+        kt[0] += 1
+    with torch.cuda.stream(torch.cuda.ExternalStream(h.stream())):
+        step()
+        step()
+        best, ms, trials = autotune.tune(h, step, steps=3, warmup=1, log=log)
+    autotune.save(a.autotune, f"{form} {a.size}", best, ms, trials,
+                  note="bench.py Stepper.step on the seeded basin, coordinate descent, 3 timed steps per trial")
+    log(f"autotune {form}: {best} {ms:.3f} ms/step")
+    del st
+    torch.cuda.empty_cache()
+
+
 def time_form(h, d, S, sp, fused, steps, warmup, timing_steps, live=False, whole=False, physics=False):
     """Warm up, time every kernel over `timing_steps` steps with HIP events,
     then time `steps` steps with the wall clock.  live=True also brackets the
@@ -347,8 +375,10 @@ def time_adjoint(h, size, steps, ckpt, ndir, seed, kn=(4, 4)):
     `ckpt` steps and each segment recomputed before its backward sweep.
     Checked by a dot-product test of one step and, along `ndir` seeded
     directions, against the tangent linear model it transposes (the forward
-    trajectory's limiter factors at every step); central differences of the
-    nonlinear model (8 directions) measure what freezing the limiter costs."""
+    trajectory's limiter factors at every step); along 8 of them against
+    central differences of that model (the finite-difference check); central
+    differences of the nonlinear model (8 directions) measure what freezing
+    the limiter costs."""
     import numpy as np
     import torch
     from nemo_hip import configs
@@ -461,7 +491,8 @@ def time_adjoint(h, size, steps, ckpt, ndir, seed, kn=(4, 4)):
             interior = torch.zeros((jpk, jpj, jpi), dtype=torch.float64, device="cuda")
             interior[:-1, 1:-1, 1:-1] = 1.0
             interior = interior.reshape(-1)
-            errs, errs_nl, growth = [], [], []
+            errs, errs_nl, errs_fd, growth = [], [], [], []
+            rn = float(torch.linalg.vector_norm(r))
             t3 = time.perf_counter()
             for q in range(ndir):
                 dirn = torch.from_numpy(np.random.default_rng(seed + 100 + q).standard_normal(n3)).to("cuda") * interior
@@ -472,6 +503,14 @@ def time_adjoint(h, size, steps, ckpt, ndir, seed, kn=(4, 4)):
                 errs.append(abs(tl - ad) / max(abs(tl), 1e-300))
                 growth.append(float(torch.linalg.vector_norm(Ld) / torch.linalg.vector_norm(dirn)))
                 if q < 8:
+                    # central differences of the model the adjoint transposes
+                    # (the forward trajectory's factors frozen), with a step
+                    # that moves the final state by 1e-4 of the misfit: J is
+                    # quadratic in the initial state there, so the differences
+                    # carry rounding only
+                    eps = 1e-4 * rn / max(float(torch.linalg.vector_norm(Ld)), 1e-300)
+                    fd = (J(run(T0 + eps * dirn, frozen=frozen)) - J(run(T0 - eps * dirn, frozen=frozen))) / (2 * eps)
+                    errs_fd.append(abs(fd - ad) / max(abs(fd), 1e-300))
                     eps = 1e-3
                     fd_nl = (J(run(T0 + eps * dirn)) - J(run(T0 - eps * dirn))) / (2 * eps)
                     errs_nl.append(abs(fd_nl - ad) / max(abs(fd_nl), 1e-300))
@@ -481,7 +520,7 @@ def time_adjoint(h, size, steps, ckpt, ndir, seed, kn=(4, 4)):
         h.context_select(0)
         torch.cuda.empty_cache()
         h.context_destroy(ctx)
-    errs, errs_nl = np.array(errs), np.array(errs_nl)
+    errs, errs_nl, errs_fd = np.array(errs), np.array(errs_nl), np.array(errs_fd)
     ndir = len(errs)
     return {"workload": f"{jpi}x{jpj}x{jpk} FP64 tracer step (FCT {kn[0]}/{kn[1]} with the limiter frozen, laplacian, implicit "
                         f"diffusion, leapfrog filter), {steps} steps, checkpoints every {ckpt}",
@@ -491,6 +530,9 @@ def time_adjoint(h, size, steps, ckpt, ndir, seed, kn=(4, 4)):
             "tl_relerr_median": float(np.median(errs)) if ndir else None,
             "tl_relerr_max": float(errs.max()) if ndir else None, "check_s": t4 - t3,
             "tl_growth_median": float(np.median(growth)) if ndir else None,
+            "fd_directions": len(errs_fd),
+            "fd_relerr_median": float(np.median(errs_fd)) if len(errs_fd) else None,
+            "fd_relerr_max": float(errs_fd.max()) if len(errs_fd) else None,
             "fd_nonlinear_directions": len(errs_nl),
             "fd_nonlinear_relerr_median": float(np.median(errs_nl)) if len(errs_nl) else None}
 
@@ -503,25 +545,40 @@ def restarted_opts(d):
     return d2
 
 
-def time_ensemble(h, members, steps, seed):
-    """ms per ensemble step, issued call by call and replayed from one
-    recorded graph, and the ensemble's cells."""
+def time_ensemble(h, members, steps, seed, ij_lo=(256, 320), ij_hi=(768, 640), jpk=56):
+    """One ensemble step of `members` domains of uneven sizes (i x j drawn
+    from ij_lo .. ij_hi, jpk levels, FP32 fused step), timed four ways:
+    issued call by call; replayed from one graph of those calls; as a plan
+    (one batched launch per position of the members' launch sequence); and
+    replayed from a graph of the plan.  Returns a dict of ms per step, the
+    launches per step and the ensemble's cells."""
     import numpy as np
     import torch
     from nemo_hip import configs
     rng = np.random.default_rng(seed)
     ctxs, st, cells = [], [], 0
     for m in range(members):
-        jpi = int(rng.integers(256, 769)) // 8 * 8
-        jpj = int(rng.integers(320, 641)) // 8 * 8
+        jpi = int(rng.integers(ij_lo[0], ij_hi[0] + 1)) // 8 * 8
+        jpj = int(rng.integers(ij_lo[1], ij_hi[1] + 1)) // 8 * 8
         c = h.context_create()
         ctxs.append(c)
         h.context_select(c)
-        d = build_case(jpi, jpj, 56, seed + 7 * m)
+        d = build_case(jpi, jpj, jpk, seed + 7 * m)
         h.register(d)
         st.append(Stepper(h, d, configs.make_state_device(d, seed=seed + 7 * m + 1), True, True))
-        cells += jpi * jpj * 56
+        cells += jpi * jpj * jpk
     ext = torch.cuda.ExternalStream(h.stream())
+
+    def timed(fn):
+        fn()
+        h.sync()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        h.sync()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    out = {"members": members, "cells": cells, "jpk": jpk, "ij_range": [list(ij_lo), list(ij_hi)]}
     try:
         with torch.cuda.stream(ext):
             for kt in (1, 2, 3):
@@ -529,33 +586,40 @@ def time_ensemble(h, members, steps, seed):
                     h.context_select(c)
                     s_.step(kt)
             h.sync()
-            t0 = time.perf_counter()
-            for kt in range(4, 4 + steps):
+            kt = [4]
+
+            def eager():
                 for c, s_ in zip(ctxs, st):
                     h.context_select(c)
-                    s_.step(kt)
-            h.sync()
-            eager_ms = (time.perf_counter() - t0) * 1e3 / steps
+                    s_.step(kt[0])
+                kt[0] += 1
+            out["eager_ms_per_step"] = timed(eager)
             h.graph_begin()
-            for c, s_ in zip(ctxs, st):
-                h.context_select(c)
-                s_.step(4 + steps)
+            eager()
             g = h.graph_end()
-            h.graph_launch(g)
-            h.sync()
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                h.graph_launch(g)
-            h.sync()
-            graph_ms = (time.perf_counter() - t0) * 1e3 / steps
+            out["graph_ms_per_step"] = timed(lambda: h.graph_launch(g))
             h.graph_free(g)
+            h.plan_begin()
+            eager()
+            p = h.plan_end()
+            info = h.plan_info(p)
+            out["plan"] = info
+            out["plan_ms_per_step"] = timed(lambda: h.plan_launch(p))
+            h.graph_begin()
+            h.plan_launch(p)
+            g = h.graph_end()
+            out["plan_graph_ms_per_step"] = timed(lambda: h.graph_launch(g))
+            h.graph_free(g)
+            h.plan_free(p)
     finally:
         h.context_select(0)
         del st
         torch.cuda.empty_cache()
         for c in ctxs:
             h.context_destroy(c)
-    return eager_ms, graph_ms, cells
+    best = min(out[k] for k in ("graph_ms_per_step", "plan_ms_per_step", "plan_graph_ms_per_step"))
+    out["cell_updates_per_s"] = cells / (best * 1e-3)
+    return out
 
 
 # the physics step's routines beside the hot path (their timer names)
@@ -783,11 +847,23 @@ def main():
     log(f"domain {jpi}x{jpj}x{jpk} built in {time.perf_counter() - t0:.1f} s, wet fraction "
         f"{float((d.arrays['k_top'] > 0).mean()):.3f}")
     h = nemo_hip.get(0, nemo_hip.NEMO_HIP_PTR_DEVICE)
+    from nemo_hip import autotune
+    defaults = {k: h.tune_get(k) for k in autotune.SWITCHES}
+    overrides = {}
     for kv in [x for x in a.tune.split(",") if x]:
         key, val = kv.split("=")
-        h.tune_set(key.strip(), int(val))
-        log(f"tuning: {key.strip()} = {int(val)}")
-    tuning = {k: h.tune_get(k) for k in ("fct_fused", "spg_persistent", "zdf_reg")}
+        overrides[key.strip()] = int(val)
+    tuning = {}
+
+    def set_tuning(form):
+        """the library defaults, the setting saved for the form's workload,
+        then the --tune overrides"""
+        saved = autotune.load(a.autotune or a.tuning, f"{form} {a.size}") or {}
+        setting = dict(defaults, **saved, **overrides)
+        autotune.apply(h, setting)
+        tuning[form] = setting
+        if saved or overrides:
+            log(f"tuning of {form}: {setting}")
     h.register(d)
     t0 = time.perf_counter()
     S = configs.make_state_device(d, seed=a.seed + 1)
@@ -797,9 +873,13 @@ def main():
     forms = a.forms.split(",")
     head = next((f for f in ("fp32_fused", "fp32") if f in forms), forms[0])
     live = None
+    if a.autotune:
+        for form in forms:
+            autotune_form(h, d, S, form, a, log)
     for form in forms:
         sp = form.startswith("fp32")
         fused = form.endswith("_fused")
+        set_tuning(form)
         ms, finite, kt, lv = time_form(h, d, S, sp, fused, a.steps, a.warmup, a.timing_steps, live=(form == head))
         if form == head:
             live = lv
@@ -819,6 +899,7 @@ def main():
             log(f"   {k:14s} {v:8.3f} ms")
     whole = {}
     for form in [f for f in a.whole_step.split(",") if f]:
+        set_tuning(form)
         ms, finite, kt, _ = time_form(h, d, S, form.startswith("fp32"), True, a.steps, a.warmup, a.timing_steps,
                                       whole=True)
         closure = ("div_hor", "ssh_nxt", "wzv", "ssh_swp", "tra_nxt", "dyn_nxt")
@@ -828,6 +909,7 @@ def main():
                                                        if "/" in k and k.split("/")[0] in closure and v[1] > 0}}
         log(f"whole step {form}: {ms:.2f} ms/step, {cells / (ms * 1e-3):.3e} cell-updates/s, finite={finite}, "
             f"closure {sum(whole[form]['closure_ms_per_step'].values()):.3f} ms")
+    autotune.apply(h, dict(defaults, **overrides))   # the remaining workloads run with the library defaults
     physics = {}
     for form in [f for f in a.physics_step.split(",") if f]:
         ms, finite, kt, _ = time_form(h, d, S, form.startswith("fp32"), False, a.steps, a.warmup, a.timing_steps,
@@ -866,14 +948,21 @@ def main():
             f"{adj['adjoint_ms_per_step']:.2f} ms/step, gradient {adj['gradient_s']:.2f} s; dot product "
             f"{adj['dot_product_relerr']:.1e}; against the tangent linear over {adj['directions']} directions: "
             f"median {adj['tl_relerr_median']:.2e}, max {adj['tl_relerr_max']:.2e} (growth of a perturbation "
-            f"{adj['tl_growth_median']:.2e}); against FD of the nonlinear model over "
-            f"{adj['fd_nonlinear_directions']}: median {adj['fd_nonlinear_relerr_median']:.2e} ({adj['check_s']:.1f} s)")
+            f"{adj['tl_growth_median']:.2e}); against FD of the frozen-limiter model over {adj['fd_directions']}: "
+            f"median {adj['fd_relerr_median']:.2e}, max {adj['fd_relerr_max']:.2e}; against FD of the nonlinear "
+            f"model over {adj['fd_nonlinear_directions']}: median {adj['fd_nonlinear_relerr_median']:.2e} "
+            f"({adj['check_s']:.1f} s)")
     if a.ensemble:
-        e_ms, g_ms, e_cells = time_ensemble(h, a.ensemble, 5, a.seed + 11)
-        ens = {"members": a.ensemble, "cells": e_cells, "eager_ms_per_step": e_ms, "graph_ms_per_step": g_ms,
-               "graph_cell_updates_per_s": e_cells / (g_ms * 1e-3)}
-        log(f"ensemble of {a.ensemble} (FP32 fused, {e_cells:.3e} cells): {e_ms:.1f} ms/step call by call, "
-            f"{g_ms:.1f} ms/step from one graph, {e_cells / (g_ms * 1e-3):.3e} cell-updates/s")
+        ens = {}
+        for name, kw in (("regional", {}), ("small", dict(ij_lo=(48, 48), ij_hi=(128, 128)))):
+            e = time_ensemble(h, a.ensemble, 5, a.seed + 11, **kw)
+            ens[name] = e
+            pi = e["plan"]
+            log(f"ensemble of {a.ensemble} {name} members (FP32 fused, {e['cells']:.3e} cells): "
+                f"{e['eager_ms_per_step']:.1f} ms/step call by call, {e['graph_ms_per_step']:.1f} from one graph, "
+                f"{e['plan_ms_per_step']:.1f} as a plan ({pi['launches']} launches per step, {pi['batched']} of "
+                f"{pi['ops']} positions batched over {pi['groups']} group(s)), {e['plan_graph_ms_per_step']:.1f} "
+                f"from a graph of the plan; {e['cell_updates_per_s']:.3e} cell-updates/s")
     copy_gbs = stream_copy()
     copy_hand = stream_copy_hand(h)
     log(f"device copy (2 GiB, read + write): torch {copy_gbs:.0f} GB/s, hand-written kernel {copy_hand:.0f} GB/s")

@@ -247,6 +247,29 @@ int nemo_hip_graph_end(int *graph);
 int nemo_hip_graph_launch(int graph);
 int nemo_hip_graph_free(int graph);
 
+/* Plans (no counterpart routine in the reference: the batching form of an
+   ensemble).  begin starts tracing: the device-mode calls that follow, in
+   any contexts, are checked and recorded as they would be for a graph but
+   nothing is launched.  end builds the plan: the contexts that traced calls
+   are its members; members whose launch sequences match (the same kernels
+   and workgroup shapes, as members with the same options have) form a
+   group, and every position of a group's sequence becomes ONE launch over
+   the work list of all its members' workgroups, each running with its own
+   member's domain, fields and grid (for the kernels written to batch, which
+   are those of nemo_hip_step_dyn, nemo_hip_step_tra and the time-step
+   closure; any other traced launch replays once per member).  launch issues
+   the plan on the library stream (also while a graph is recorded, so a
+   graph of a plan holds one node per batched launch); info reports the
+   members, groups, launches per replay and how many of them are batched;
+   free releases it.  The rules of graph recording apply to the traced
+   calls, and the fields they name must stay allocated while the plan
+   exists. */
+int nemo_hip_plan_begin(void);
+int nemo_hip_plan_end(int *plan);
+int nemo_hip_plan_launch(int plan);
+int nemo_hip_plan_info(int plan, int *nmembers, int *ngroups, int *nops, int *nbatched, int *nlaunch);
+int nemo_hip_plan_free(int plan);
+
 /* per-routine GPU timers keyed by reference routine name (timing.F90:89-212):
    when enabled every kernel launch is bracketed by HIP events on the
    library's stream and accumulated by name */
@@ -508,9 +531,10 @@ int nemo_hip_stream_copy(const void *src, void *dst, size_t bytes, int blocks);
 /* Tuning switches (no counterpart in the reference): the kernel variants
    and tile shapes a caller or the project's autotuner may choose, process
    wide.  Every choice gives the same results bit for bit.  Keys:
-     "fct_fused"  1 (default): tra_adv_fct on closed domains as one kernel
-                  per tile and tracer (intermediates on chip); 0: the
-                  flux / limiter chain through HBM scratch.
+     "fct_fused"  1 or 2: tra_adv_fct on closed domains as one kernel per
+                  tile and tracer (intermediates on chip), with 256 (1) or
+                  512 (2) threads per tile; 0 (default): the flux / limiter
+                  chain through scratch arrays.
      "spg_persistent"  1 or 2: dyn_spg_ts's sub-steps in one cooperative
                   launch with grid barriers (1: one 1024-thread workgroup
                   per CU; 2: 256-thread workgroups, as many as stay

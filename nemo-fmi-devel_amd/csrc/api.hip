@@ -47,11 +47,38 @@ struct Ctx {
     unsigned *spg_bar = nullptr;                        // dyn_spg_ts grid barrier and status
     bool spg_check = false;                             // a cooperative phase 2 awaits its status check
     double *hsb_buf = nullptr;                          // dia_hsb partial pairs and sums
+    std::vector<PlanRec> trace;                         // the launches traced for a plan
 };
 static std::vector<Ctx *> g_ctxs;
 static Ctx *g_c = nullptr;   // the current context
+// plans: one traced step of every member, replayed as batched launches
+struct PlanOp {
+    std::string name, kname;
+    BatchFn bfn = nullptr;                 // batched: one launch over the members' work list
+    unsigned nblk = 0;
+    dim3 block;
+    int nm = 0;
+    size_t off_start = 0, off_ent = 0;     // byte offsets of start[] and the entries in the plan's table
+    std::vector<std::function<hipError_t(hipStream_t)>> singles;   // otherwise: one launch per member
+};
+struct Plan {
+    std::vector<PlanOp> ops;
+    char *dev = nullptr;                   // the device table of every batched launch
+    int nmembers = 0, ngroups = 0, nbatched = 0, nlaunch = 0;
+};
+static std::vector<Plan *> g_plans;
+static void plan_clear_traces()
+{
+    for (Ctx *c : g_ctxs)
+        if (c) c->trace.clear();
+}
+
 // recording of a graph on the library stream (nemo_hip_graph_begin / end)
 static bool g_capturing = false;
+// tracing of a plan (nemo_hip_plan_begin / end): launches are recorded per
+// context, nothing is issued
+static bool g_planning = false;
+static bool recording() { return g_capturing || g_planning; }
 static std::vector<hipGraphExec_t> g_graphs;
 // staging pool (host-pointer mode)
 static std::vector<std::pair<void *, size_t>> g_pool;
@@ -95,11 +122,17 @@ int set_error(int code, const char *fmt, ...)
 }
 hipStream_t stream() { return g_stream; }
 const DevDomain &dom() { return g_c->dom; }
-bool capturing() { return g_capturing; }
+bool capturing() { return recording(); }
+bool planning() { return g_planning; }
+int plan_add(PlanRec &&r)
+{
+    g_c->trace.push_back(std::move(r));
+    return 0;
+}
 // tuning switches, process-wide; names and defaults
 static const char *g_tune_names[T_NTUNE] = {"fct_fused", "spg_persistent", "zdf_reg"};
-static int g_tune[T_NTUNE] = {1, 0, 0};
-static const int g_tune_max[T_NTUNE] = {1, 2, 1};
+static int g_tune[T_NTUNE] = {0, 0, 0};
+static const int g_tune_max[T_NTUNE] = {2, 2, 1};
 int tune(Tune which) { return g_tune[which]; }
 int *spg_rot_u() { return g_c->spg_rot_u; }
 int *spg_rot_s() { return g_c->spg_rot_s; }
@@ -141,7 +174,7 @@ static std::string kernel_key(const char *name, const char *kernel)
 }
 bool timing_pair(const char *name, const char *kernel, hipEvent_t *start, hipEvent_t *stop)
 {
-    if (!g_timing || g_capturing) return false;
+    if (!g_timing || recording()) return false;
     std::string key = kernel_key(name, kernel);
     if (!g_only.empty() && key != g_only) return false;
     if (g_pend.size() >= 4096) {
@@ -201,8 +234,8 @@ struct Stager {
     void *get(const void *user, size_t bytes, bool in, bool out)
     {
         if (!user || !host || status) return (void *)user;
-        if (g_capturing) {
-            status = set_error(NEMO_HIP_ERR_STATE, "host-pointer calls cannot be recorded into a graph");
+        if (recording()) {
+            status = set_error(NEMO_HIP_ERR_STATE, "host-pointer calls cannot be recorded into a graph or a plan");
             return nullptr;
         }
         if (slot >= g_pool.size()) g_pool.push_back({nullptr, 0});
@@ -534,12 +567,12 @@ using namespace nh;
 // the first steps (Euler step and the switch to leapfrog) are refused
 #define NEED_STEADY(kt)                                                                                   \
     do {                                                                                                  \
-        if (g_capturing && (kt) <= g_c->hdom.nit000 + 1)                                                   \
-            return set_error(NEMO_HIP_ERR_STATE, "kt = %d: the first two steps cannot be recorded into a graph", (kt)); \
+        if (recording() && (kt) <= g_c->hdom.nit000 + 1)                                                   \
+            return set_error(NEMO_HIP_ERR_STATE, "kt = %d: the first two steps cannot be recorded into a graph or a plan", (kt)); \
     } while (0)
 #define NEED_NOT_CAPTURING(who)                                                                           \
     do {                                                                                                  \
-        if (g_capturing) return set_error(NEMO_HIP_ERR_STATE, "%s cannot be called while recording a graph", who); \
+        if (recording()) return set_error(NEMO_HIP_ERR_STATE, "%s cannot be called while recording a graph or a plan", who); \
     } while (0)
 #define NEED_PTR(p)                                                                   \
     do {                                                                              \
@@ -588,10 +621,17 @@ int nemo_hip_finalize(void)
         if (gr) (void)hipGraphDestroy(gr);
         g_capturing = false;
     }
+    g_planning = false;
     (void)hipStreamSynchronize(g_stream);
     for (auto &e : g_graphs)
         if (e) (void)hipGraphExecDestroy(e);
     g_graphs.clear();
+    for (Plan *P : g_plans) {
+        if (!P) continue;
+        if (P->dev) (void)hipFree(P->dev);
+        delete P;
+    }
+    g_plans.clear();
     for (Ctx *c : g_ctxs) {
         if (!c) continue;
         g_c = c;
@@ -766,7 +806,7 @@ int nemo_hip_context_destroy(int ctx)
 int nemo_hip_graph_begin(void)
 {
     if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
-    if (g_capturing) return set_error(NEMO_HIP_ERR_STATE, "a graph is already being recorded");
+    if (recording()) return set_error(NEMO_HIP_ERR_STATE, "a graph or a plan is already being recorded");
     if (g_ptr_mode != NEMO_HIP_PTR_DEVICE) return set_error(NEMO_HIP_ERR_STATE, "graphs record device-pointer calls only");
     HIPCHK(hipStreamSynchronize(g_stream));
     HIPCHK(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
@@ -808,6 +848,164 @@ int nemo_hip_graph_free(int graph)
     HIPCHK(hipStreamSynchronize(g_stream));
     HIPCHK(hipGraphExecDestroy(g_graphs[graph]));
     g_graphs[graph] = nullptr;
+    return NEMO_HIP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plans: one traced step of every member, replayed as batched launches
+// ---------------------------------------------------------------------------
+int nemo_hip_plan_begin(void)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    if (recording()) return set_error(NEMO_HIP_ERR_STATE, "a graph or a plan is already being recorded");
+    if (g_ptr_mode != NEMO_HIP_PTR_DEVICE) return set_error(NEMO_HIP_ERR_STATE, "plans trace device-pointer calls only");
+    plan_clear_traces();
+    g_planning = true;
+    return NEMO_HIP_OK;
+}
+
+int nemo_hip_plan_end(int *plan)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    if (!g_planning) return set_error(NEMO_HIP_ERR_STATE, "no plan is being traced");
+    g_planning = false;
+    if (!plan) {
+        plan_clear_traces();
+        return set_error(NEMO_HIP_ERR_ARG, "plan_end: NULL result pointer");
+    }
+    // members: the contexts that traced launches; members whose launch
+    // sequences match (same kernels, argument types and workgroup shapes)
+    // form a group, and each position of a group's sequence becomes one op
+    std::vector<Ctx *> mem;
+    for (Ctx *c : g_ctxs)
+        if (c && !c->trace.empty()) mem.push_back(c);
+    if (mem.empty()) return set_error(NEMO_HIP_ERR_STATE, "plan_end: no launch was traced");
+    std::vector<std::pair<std::string, std::vector<Ctx *>>> groups;
+    for (Ctx *c : mem) {
+        std::string key;
+        char buf[96];
+        for (const PlanRec &r : c->trace) {
+            snprintf(buf, sizeof buf, "|%p:%u,%u,%u:%zu:", (const void *)r.bfn, r.block.x, r.block.y, r.block.z,
+                     r.ent.size());
+            key += r.kname;
+            key += buf;
+        }
+        size_t gq = 0;
+        while (gq < groups.size() && groups[gq].first != key) gq++;
+        if (gq == groups.size()) groups.push_back({key, {}});
+        groups[gq].second.push_back(c);
+    }
+    Plan *P = new Plan();
+    P->nmembers = (int)mem.size();
+    P->ngroups = (int)groups.size();
+    std::vector<char> tab;
+    auto align = [&](size_t a) {
+        while (tab.size() % a) tab.push_back(0);
+    };
+    for (auto &gr : groups) {
+        const std::vector<Ctx *> &gm = gr.second;
+        const int nm = (int)gm.size();
+        const size_t n = gm[0]->trace.size();
+        for (size_t q = 0; q < n; q++) {
+            const PlanRec &r0 = gm[0]->trace[q];
+            PlanOp op;
+            op.name = r0.routine;
+            op.kname = r0.kname;
+            if (r0.bfn) {
+                std::vector<unsigned> st((size_t)nm + 1);
+                unsigned acc = 0;
+                for (int m = 0; m < nm; m++) {
+                    const PlanRec &r = gm[m]->trace[q];
+                    st[m] = acc;
+                    const unsigned long long nb = (unsigned long long)r.grid.x * r.grid.y * r.grid.z;
+                    const unsigned long long padded = (nb + 7ull) & ~7ull;
+                    if (acc + padded > 0x7fffffffull) {
+                        delete P;
+                        plan_clear_traces();
+                        return set_error(NEMO_HIP_ERR_UNSUPPORTED, "plan_end: %s: more than 2^31 workgroups",
+                                         r0.kname.c_str());
+                    }
+                    acc += (unsigned)padded;
+                }
+                st[nm] = acc;
+                align(16);
+                op.off_start = tab.size();
+                tab.insert(tab.end(), (const char *)st.data(), (const char *)(st.data() + st.size()));
+                align(16);
+                op.off_ent = tab.size();
+                for (int m = 0; m < nm; m++) {
+                    const std::vector<char> &e = gm[m]->trace[q].ent;
+                    tab.insert(tab.end(), e.begin(), e.end());
+                }
+                op.bfn = r0.bfn;
+                op.nblk = acc;
+                op.block = r0.block;
+                op.nm = nm;
+                P->nbatched++;
+                P->nlaunch++;
+            } else {
+                for (int m = 0; m < nm; m++) op.singles.push_back(gm[m]->trace[q].single);
+                P->nlaunch += nm;
+            }
+            P->ops.push_back(std::move(op));
+        }
+    }
+    plan_clear_traces();
+    if (!tab.empty()) {
+        if (hipMalloc(&P->dev, tab.size()) != hipSuccess ||
+            hipMemcpy(P->dev, tab.data(), tab.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            if (P->dev) (void)hipFree(P->dev);
+            delete P;
+            return set_error(NEMO_HIP_ERR_HIP, "plan_end: the plan's table of %zu bytes could not be placed",
+                             tab.size());
+        }
+    }
+    g_plans.push_back(P);
+    *plan = (int)g_plans.size() - 1;
+    return NEMO_HIP_OK;
+}
+
+int nemo_hip_plan_launch(int plan)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    if (g_planning) return set_error(NEMO_HIP_ERR_STATE, "plan_launch while a plan is traced");
+    if (plan < 0 || plan >= (int)g_plans.size() || !g_plans[plan]) return set_error(NEMO_HIP_ERR_ARG, "no plan %d", plan);
+    const Plan *P = g_plans[plan];
+    for (const PlanOp &op : P->ops) {
+        if (op.bfn) {
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (!timing_pair(op.name.c_str(), op.kname.c_str(), &e0, &e1)) e0 = e1 = nullptr;
+            HIPCHK(op.bfn(op.nblk, op.block, g_stream, e0, e1, (const unsigned *)(P->dev + op.off_start), op.nm,
+                          P->dev + op.off_ent));
+        } else {
+            for (const auto &f : op.singles) HIPCHK(f(g_stream));
+        }
+    }
+    return NEMO_HIP_OK;
+}
+
+int nemo_hip_plan_info(int plan, int *nmembers, int *ngroups, int *nops, int *nbatched, int *nlaunch)
+{
+    if (plan < 0 || plan >= (int)g_plans.size() || !g_plans[plan]) return set_error(NEMO_HIP_ERR_ARG, "no plan %d", plan);
+    const Plan *P = g_plans[plan];
+    if (nmembers) *nmembers = P->nmembers;
+    if (ngroups) *ngroups = P->ngroups;
+    if (nops) *nops = (int)P->ops.size();
+    if (nbatched) *nbatched = P->nbatched;
+    if (nlaunch) *nlaunch = P->nlaunch;
+    return NEMO_HIP_OK;
+}
+
+int nemo_hip_plan_free(int plan)
+{
+    if (!g_init) return set_error(NEMO_HIP_ERR_STATE, "nemo_hip_init has not been called");
+    NEED_NOT_CAPTURING("nemo_hip_plan_free");
+    if (plan < 0 || plan >= (int)g_plans.size() || !g_plans[plan]) return set_error(NEMO_HIP_ERR_ARG, "no plan %d", plan);
+    HIPCHK(hipStreamSynchronize(g_stream));
+    Plan *P = g_plans[plan];
+    if (P->dev) HIPCHK(hipFree(P->dev));
+    delete P;
+    g_plans[plan] = nullptr;
     return NEMO_HIP_OK;
 }
 
@@ -962,8 +1160,8 @@ static int do_tra_ldf_iso(int kt, int kit000, const char *cdtype, const T *pahu,
 {
     NEED_DOMAIN();
     NEED_STEADY(kt);
-    if (g_capturing && kt == kit000)
-        return set_error(NEMO_HIP_ERR_STATE, "tra_ldf_iso: the first call (kt = kit000) cannot be recorded into a graph");
+    if (recording() && kt == kit000)
+        return set_error(NEMO_HIP_ERR_STATE, "tra_ldf_iso: the first call (kt = kit000) cannot be recorded into a graph or a plan");
     int is_tra, rc = is_tra_type(cdtype, &is_tra);
     if (rc) return rc;
     NEED_PTR(pahu); NEED_PTR(pahv); NEED_PTR(ptb); NEED_PTR(pta); NEED_PTR(uslp); NEED_PTR(vslp);

@@ -302,8 +302,27 @@ double *scratch2(int which);   // n2 doubles each (dyn_spg_ts work arrays), NH_N
             return nh::set_error(4, "%s:%d: %s: %s", __FILE__, __LINE__, #x, hipGetErrorString(e_)); \
     } while (0)
 
+#include "batch.hpp"
+
+// launch on the library stream (timed when timing is on); while a plan is
+// traced, the launch is recorded for this member instead
 #define LAUNCH(name, kern, g, ...)                                                                    \
     do {                                                                                              \
+        if (nh::planning()) {                                                                         \
+            const Grid2 gp_ = (g);                                                                    \
+            nh::PlanRec r_;                                                                           \
+            r_.routine = name;                                                                        \
+            r_.kname = #kern;                                                                         \
+            r_.grid = gp_.grid;                                                                       \
+            r_.block = gp_.block;                                                                     \
+            r_.single = [=](hipStream_t s_) {                                                         \
+                hipLaunchKernelGGL(kern, gp_.grid, gp_.block, 0, s_, __VA_ARGS__);                    \
+                return hipGetLastError();                                                             \
+            };                                                                                        \
+            const int rc_ = nh::plan_add(std::move(r_));                                              \
+            if (rc_) return rc_;                                                                      \
+            break;                                                                                    \
+        }                                                                                             \
         hipEvent_t ev0_, ev1_;                                                                        \
         if (nh::timing_pair(name, #kern, &ev0_, &ev1_))                                               \
             hipExtLaunchKernelGGL(kern, (g).grid, (g).block, 0, nh::stream(), ev0_, ev1_, 0, __VA_ARGS__); \

@@ -41,31 +41,39 @@ int launch_dom_msk(double *tmask, double *umask, double *vmask, double *fmask, d
 // (barrier) before the north-south fill copies whole rows.
 // ---------------------------------------------------------------------------
 template <typename T>
+struct S_lbc {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, T * __restrict__ p, int nk, char nat, T pval)
+    {
+        const int k = (int)vb_.x + 1;
+        if (k > nk) return;
+        for (int j = (int)threadIdx.x + 1; j <= d.jpj; j += blockDim.x) {
+            if (d.iper) {
+                T a = A3(p, d.jpi - 1, j, k), b = A3(p, 2, j, k);
+                A3(p, 1, j, k) = a;
+                A3(p, d.jpi, j, k) = b;
+            } else {
+                if (nat != 'F') A3(p, 1, j, k) = pval;
+                A3(p, d.jpi, j, k) = pval;
+            }
+        }
+        __syncthreads();
+        for (int i = (int)threadIdx.x + 1; i <= d.jpi; i += blockDim.x) {
+            if (d.jper) {
+                T a = A3(p, i, d.jpj - 1, k), b = A3(p, i, 2, k);
+                A3(p, i, 1, k) = a;
+                A3(p, i, d.jpj, k) = b;
+            } else {
+                if (nat != 'F') A3(p, i, 1, k) = pval;
+                A3(p, i, d.jpj, k) = pval;
+            }
+        }
+    }
+};
+template <typename T>
 KERNEL void k_lbc(DevDomain d, T * __restrict__ p, int nk, char nat, T pval)
 {
-    const int k = (int)blockIdx.x + 1;
-    if (k > nk) return;
-    for (int j = (int)threadIdx.x + 1; j <= d.jpj; j += blockDim.x) {
-        if (d.iper) {
-            T a = A3(p, d.jpi - 1, j, k), b = A3(p, 2, j, k);
-            A3(p, 1, j, k) = a;
-            A3(p, d.jpi, j, k) = b;
-        } else {
-            if (nat != 'F') A3(p, 1, j, k) = pval;
-            A3(p, d.jpi, j, k) = pval;
-        }
-    }
-    __syncthreads();
-    for (int i = (int)threadIdx.x + 1; i <= d.jpi; i += blockDim.x) {
-        if (d.jper) {
-            T a = A3(p, i, d.jpj - 1, k), b = A3(p, i, 2, k);
-            A3(p, i, 1, k) = a;
-            A3(p, i, d.jpj, k) = b;
-        } else {
-            if (nat != 'F') A3(p, i, 1, k) = pval;
-            A3(p, i, d.jpj, k) = pval;
-        }
-    }
+    S_lbc<T>::run(vb_hw(), d, p, nk, nat, pval);
 }
 
 template <typename T>
@@ -75,7 +83,7 @@ int launch_lbc(T *p, int nk, char nat, T pval)
     Grid2 g;
     g.grid = dim3((unsigned)nk, 1, 1);
     g.block = dim3(NH_BLOCK, 1, 1);
-    LAUNCH("lbc_lnk", k_lbc<T>, g, d, p, nk, nat, pval);
+    LAUNCHB("lbc_lnk", k_lbc<T>, (S_lbc<T>), g, d, p, nk, nat, pval);
     return 0;
 }
 template int launch_lbc<double>(double *, int, char, double);

@@ -46,51 +46,59 @@ __device__ __forceinline__ T keg_hw(const DevDomain &d, const T *un, const T *vn
 }
 
 template <typename T>
+struct S_dyn_keg_zad {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, int do_keg, int kscheme, int do_zad, const T * __restrict__ un, const T * __restrict__ vn, const T * __restrict__ wn, T * __restrict__ ua, T * __restrict__ va)
+    {
+        THREAD_IJ_V(vb_, 2, 2);
+        if (i > d.jpi - 1 || j > d.jpj - 1) return;
+        const Metrics<T> &m = met<T>(d);
+        const size_t ij = IJ(i, j);
+        const T e1u = m.e1u[ij], e2v = m.e2v[ij], r1_e1e2u = m.r1_e1e2u[ij], r1_e1e2v = m.r1_e1e2v[ij];
+        const T e1e2t = m.e1e2t[ij], e1e2t_e = m.e1e2t[IJ(i + 1, j)], e1e2t_n = m.e1e2t[IJ(i, j + 1)];
+        T zwuw_k = T(0), zwvw_k = T(0);    // surface flux zero, dynzad.F90:93-94
+        for (int k = 1; k <= d.jpk - 1; k++) {
+            const size_t p = IJK(i, j, k);
+            T a = ua[p], b = va[p];
+            if (do_keg) {
+                T h, he, hn;
+                if (kscheme == 0) {
+                    h = keg_c2(d, un, vn, i, j, k);
+                    he = keg_c2(d, un, vn, i + 1, j, k);
+                    hn = keg_c2(d, un, vn, i, j + 1, k);
+                } else {
+                    h = keg_hw(d, un, vn, i, j, k);
+                    he = keg_hw(d, un, vn, i + 1, j, k);
+                    hn = keg_hw(d, un, vn, i, j + 1, k);
+                }
+                a = a - (he - h) / e1u;
+                b = b - (hn - h) / e2v;
+            }
+            if (do_zad) {
+                T zwuw_kp1 = T(0), zwvw_kp1 = T(0);
+                if (k + 1 <= d.jpk - 1) {
+                    const int kk = k + 1;
+                    const T zww = T(0.25) * e1e2t * A3(wn, i, j, kk);
+                    const T zww_e = T(0.25) * e1e2t_e * A3(wn, i + 1, j, kk);
+                    const T zww_n = T(0.25) * e1e2t_n * A3(wn, i, j + 1, kk);
+                    zwuw_kp1 = (zww_e + zww) * (A3(un, i, j, kk - 1) - A3(un, i, j, kk));
+                    zwvw_kp1 = (zww_n + zww) * (A3(vn, i, j, kk - 1) - A3(vn, i, j, kk));
+                }
+                a = a - (zwuw_k + zwuw_kp1) * r1_e1e2u / E3(m.e3u, ij, k);
+                b = b - (zwvw_k + zwvw_kp1) * r1_e1e2v / E3(m.e3v, ij, k);
+                zwuw_k = zwuw_kp1;
+                zwvw_k = zwvw_kp1;
+            }
+            ua[p] = a;
+            va[p] = b;
+        }
+    }
+};
+template <typename T>
 KERNEL void k_dyn_keg_zad(DevDomain d, int do_keg, int kscheme, int do_zad, const T * __restrict__ un, const T * __restrict__ vn, const T * __restrict__ wn,
                               T * __restrict__ ua, T * __restrict__ va)
 {
-    THREAD_IJ(2, 2);
-    if (i > d.jpi - 1 || j > d.jpj - 1) return;
-    const Metrics<T> &m = met<T>(d);
-    const size_t ij = IJ(i, j);
-    const T e1u = m.e1u[ij], e2v = m.e2v[ij], r1_e1e2u = m.r1_e1e2u[ij], r1_e1e2v = m.r1_e1e2v[ij];
-    const T e1e2t = m.e1e2t[ij], e1e2t_e = m.e1e2t[IJ(i + 1, j)], e1e2t_n = m.e1e2t[IJ(i, j + 1)];
-    T zwuw_k = T(0), zwvw_k = T(0);    // surface flux zero, dynzad.F90:93-94
-    for (int k = 1; k <= d.jpk - 1; k++) {
-        const size_t p = IJK(i, j, k);
-        T a = ua[p], b = va[p];
-        if (do_keg) {
-            T h, he, hn;
-            if (kscheme == 0) {
-                h = keg_c2(d, un, vn, i, j, k);
-                he = keg_c2(d, un, vn, i + 1, j, k);
-                hn = keg_c2(d, un, vn, i, j + 1, k);
-            } else {
-                h = keg_hw(d, un, vn, i, j, k);
-                he = keg_hw(d, un, vn, i + 1, j, k);
-                hn = keg_hw(d, un, vn, i, j + 1, k);
-            }
-            a = a - (he - h) / e1u;
-            b = b - (hn - h) / e2v;
-        }
-        if (do_zad) {
-            T zwuw_kp1 = T(0), zwvw_kp1 = T(0);
-            if (k + 1 <= d.jpk - 1) {
-                const int kk = k + 1;
-                const T zww = T(0.25) * e1e2t * A3(wn, i, j, kk);
-                const T zww_e = T(0.25) * e1e2t_e * A3(wn, i + 1, j, kk);
-                const T zww_n = T(0.25) * e1e2t_n * A3(wn, i, j + 1, kk);
-                zwuw_kp1 = (zww_e + zww) * (A3(un, i, j, kk - 1) - A3(un, i, j, kk));
-                zwvw_kp1 = (zww_n + zww) * (A3(vn, i, j, kk - 1) - A3(vn, i, j, kk));
-            }
-            a = a - (zwuw_k + zwuw_kp1) * r1_e1e2u / E3(m.e3u, ij, k);
-            b = b - (zwvw_k + zwvw_kp1) * r1_e1e2v / E3(m.e3v, ij, k);
-            zwuw_k = zwuw_kp1;
-            zwvw_k = zwvw_kp1;
-        }
-        ua[p] = a;
-        va[p] = b;
-    }
+    S_dyn_keg_zad<T>::run(vb_hw(), d, do_keg, kscheme, do_zad, un, vn, wn, ua, va);
 }
 
 // ---------------------------------------------------------------------------
@@ -117,54 +125,62 @@ __device__ __forceinline__ T cen2_zfw(const DevDomain &d, const T *wn, int p, in
 }
 
 template <typename T>
+struct S_dyn_adv_cen2 {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, const T * __restrict__ un, const T * __restrict__ vn, const T * __restrict__ wn, T * __restrict__ ua, T * __restrict__ va)
+    {
+        THREAD_IJ_V(vb_, 2, 2);
+        if (i > d.jpi - 1 || j > d.jpj - 1) return;
+        const Metrics<T> &m = met<T>(d);
+        const size_t ij = IJ(i, j);
+        const T r1_e1e2u = m.r1_e1e2u[ij], r1_e1e2v = m.r1_e1e2v[ij];
+        const int jpk = d.jpk;
+        // surface fluxes (linear free surface), dynadv_cen2.F90:105-112
+        T zfu_uw_k = T(0.5) * (m.e1e2t[ij] * A3(wn, i, j, 1) + m.e1e2t[IJ(i + 1, j)] * A3(wn, i + 1, j, 1)) * A3(un, i, j, 1);
+        T zfv_vw_k = T(0.5) * (m.e1e2t[ij] * A3(wn, i, j, 1) + m.e1e2t[IJ(i, j + 1)] * A3(wn, i, j + 1, 1)) * A3(vn, i, j, 1);
+        for (int k = 1; k <= jpk - 1; k++) {
+            // horizontal: zfu_t(i+1), zfu_t(i), zfv_f(i,j), zfv_f(i,j-1)
+            const T fu_im1 = cen2_zfu(d, un, i - 1, j, k), fu_i = cen2_zfu(d, un, i, j, k), fu_ip1 = cen2_zfu(d, un, i + 1, j, k);
+            const T fu_jp1 = cen2_zfu(d, un, i, j + 1, k), fu_im1jp1 = cen2_zfu(d, un, i - 1, j + 1, k);
+            const T fv_i = cen2_zfv(d, vn, i, j, k), fv_ip1 = cen2_zfv(d, vn, i + 1, j, k);
+            const T fv_jm1 = cen2_zfv(d, vn, i, j - 1, k), fv_ip1jm1 = cen2_zfv(d, vn, i + 1, j - 1, k);
+            const T fv_jp1 = cen2_zfv(d, vn, i, j + 1, k), fv_im1 = cen2_zfv(d, vn, i - 1, j, k);
+            const T u_i = A3(un, i, j, k), u_ip1 = A3(un, i + 1, j, k), u_im1 = A3(un, i - 1, j, k);
+            const T u_jp1 = A3(un, i, j + 1, k), u_jm1 = A3(un, i, j - 1, k);
+            const T v_j = A3(vn, i, j, k), v_ip1 = A3(vn, i + 1, j, k), v_jp1 = A3(vn, i, j + 1, k);
+            const T v_im1 = A3(vn, i - 1, j, k), v_jm1 = A3(vn, i, j - 1, k);
+            const T zfu_t_ip1 = (fu_i + fu_ip1) * (u_i + u_ip1);
+            const T zfu_t_i = (fu_im1 + fu_i) * (u_im1 + u_i);
+            const T zfv_f_j = (fv_i + fv_ip1) * (u_i + u_jp1);
+            const T zfv_f_jm1 = (fv_jm1 + fv_ip1jm1) * (u_jm1 + u_i);
+            const T zfu_f_i = (fu_i + fu_jp1) * (v_j + v_ip1);
+            const T zfu_f_im1 = (fu_im1 + fu_im1jp1) * (v_im1 + v_j);
+            const T zfv_t_jp1 = (fv_i + fv_jp1) * (v_j + v_jp1);
+            const T zfv_t_j = (fv_jm1 + fv_i) * (v_jm1 + v_j);
+            const size_t p = IJK(i, j, k);
+            T a = ua[p] - (zfu_t_ip1 - zfu_t_i + zfv_f_j - zfv_f_jm1) * r1_e1e2u / E3(m.e3u, ij, k);
+            T b = va[p] - (zfu_f_i - zfu_f_im1 + zfv_t_jp1 - zfv_t_j) * r1_e1e2v / E3(m.e3v, ij, k);
+            // vertical, dynadv_cen2.F90:113-133
+            T zfu_uw_kp1 = T(0), zfv_vw_kp1 = T(0);
+            if (k + 1 <= jpk - 1) {
+                const int kk = k + 1;
+                const T fw = cen2_zfw(d, wn, i, j, kk);
+                zfu_uw_kp1 = (fw + cen2_zfw(d, wn, i + 1, j, kk)) * (A3(un, i, j, kk) + A3(un, i, j, kk - 1));
+                zfv_vw_kp1 = (fw + cen2_zfw(d, wn, i, j + 1, kk)) * (A3(vn, i, j, kk) + A3(vn, i, j, kk - 1));
+            }
+            a = a - (zfu_uw_k - zfu_uw_kp1) * r1_e1e2u / E3(m.e3u, ij, k);
+            b = b - (zfv_vw_k - zfv_vw_kp1) * r1_e1e2v / E3(m.e3v, ij, k);
+            ua[p] = a;
+            va[p] = b;
+            zfu_uw_k = zfu_uw_kp1;
+            zfv_vw_k = zfv_vw_kp1;
+        }
+    }
+};
+template <typename T>
 KERNEL void k_dyn_adv_cen2(DevDomain d, const T * __restrict__ un, const T * __restrict__ vn, const T * __restrict__ wn, T * __restrict__ ua, T * __restrict__ va)
 {
-    THREAD_IJ(2, 2);
-    if (i > d.jpi - 1 || j > d.jpj - 1) return;
-    const Metrics<T> &m = met<T>(d);
-    const size_t ij = IJ(i, j);
-    const T r1_e1e2u = m.r1_e1e2u[ij], r1_e1e2v = m.r1_e1e2v[ij];
-    const int jpk = d.jpk;
-    // surface fluxes (linear free surface), dynadv_cen2.F90:105-112
-    T zfu_uw_k = T(0.5) * (m.e1e2t[ij] * A3(wn, i, j, 1) + m.e1e2t[IJ(i + 1, j)] * A3(wn, i + 1, j, 1)) * A3(un, i, j, 1);
-    T zfv_vw_k = T(0.5) * (m.e1e2t[ij] * A3(wn, i, j, 1) + m.e1e2t[IJ(i, j + 1)] * A3(wn, i, j + 1, 1)) * A3(vn, i, j, 1);
-    for (int k = 1; k <= jpk - 1; k++) {
-        // horizontal: zfu_t(i+1), zfu_t(i), zfv_f(i,j), zfv_f(i,j-1)
-        const T fu_im1 = cen2_zfu(d, un, i - 1, j, k), fu_i = cen2_zfu(d, un, i, j, k), fu_ip1 = cen2_zfu(d, un, i + 1, j, k);
-        const T fu_jp1 = cen2_zfu(d, un, i, j + 1, k), fu_im1jp1 = cen2_zfu(d, un, i - 1, j + 1, k);
-        const T fv_i = cen2_zfv(d, vn, i, j, k), fv_ip1 = cen2_zfv(d, vn, i + 1, j, k);
-        const T fv_jm1 = cen2_zfv(d, vn, i, j - 1, k), fv_ip1jm1 = cen2_zfv(d, vn, i + 1, j - 1, k);
-        const T fv_jp1 = cen2_zfv(d, vn, i, j + 1, k), fv_im1 = cen2_zfv(d, vn, i - 1, j, k);
-        const T u_i = A3(un, i, j, k), u_ip1 = A3(un, i + 1, j, k), u_im1 = A3(un, i - 1, j, k);
-        const T u_jp1 = A3(un, i, j + 1, k), u_jm1 = A3(un, i, j - 1, k);
-        const T v_j = A3(vn, i, j, k), v_ip1 = A3(vn, i + 1, j, k), v_jp1 = A3(vn, i, j + 1, k);
-        const T v_im1 = A3(vn, i - 1, j, k), v_jm1 = A3(vn, i, j - 1, k);
-        const T zfu_t_ip1 = (fu_i + fu_ip1) * (u_i + u_ip1);
-        const T zfu_t_i = (fu_im1 + fu_i) * (u_im1 + u_i);
-        const T zfv_f_j = (fv_i + fv_ip1) * (u_i + u_jp1);
-        const T zfv_f_jm1 = (fv_jm1 + fv_ip1jm1) * (u_jm1 + u_i);
-        const T zfu_f_i = (fu_i + fu_jp1) * (v_j + v_ip1);
-        const T zfu_f_im1 = (fu_im1 + fu_im1jp1) * (v_im1 + v_j);
-        const T zfv_t_jp1 = (fv_i + fv_jp1) * (v_j + v_jp1);
-        const T zfv_t_j = (fv_jm1 + fv_i) * (v_jm1 + v_j);
-        const size_t p = IJK(i, j, k);
-        T a = ua[p] - (zfu_t_ip1 - zfu_t_i + zfv_f_j - zfv_f_jm1) * r1_e1e2u / E3(m.e3u, ij, k);
-        T b = va[p] - (zfu_f_i - zfu_f_im1 + zfv_t_jp1 - zfv_t_j) * r1_e1e2v / E3(m.e3v, ij, k);
-        // vertical, dynadv_cen2.F90:113-133
-        T zfu_uw_kp1 = T(0), zfv_vw_kp1 = T(0);
-        if (k + 1 <= jpk - 1) {
-            const int kk = k + 1;
-            const T fw = cen2_zfw(d, wn, i, j, kk);
-            zfu_uw_kp1 = (fw + cen2_zfw(d, wn, i + 1, j, kk)) * (A3(un, i, j, kk) + A3(un, i, j, kk - 1));
-            zfv_vw_kp1 = (fw + cen2_zfw(d, wn, i, j + 1, kk)) * (A3(vn, i, j, kk) + A3(vn, i, j, kk - 1));
-        }
-        a = a - (zfu_uw_k - zfu_uw_kp1) * r1_e1e2u / E3(m.e3u, ij, k);
-        b = b - (zfv_vw_k - zfv_vw_kp1) * r1_e1e2v / E3(m.e3v, ij, k);
-        ua[p] = a;
-        va[p] = b;
-        zfu_uw_k = zfu_uw_kp1;
-        zfv_vw_k = zfv_vw_kp1;
-    }
+    S_dyn_adv_cen2<T>::run(vb_hw(), d, un, vn, wn, ua, va);
 }
 
 template <typename T>
@@ -174,10 +190,10 @@ int launch_dyn_adv(int which, int kscheme, const T *un, const T *vn, const T *wn
     const DevDomain &d = nh::dom();
     Grid2 g = grid2(d.jpi - 2, d.jpj - 2);
     if (which == 4) {
-        LAUNCH("dyn_adv_cen2", k_dyn_adv_cen2<T>, g, d, un, vn, wn, ua, va);
+        LAUNCHB("dyn_adv_cen2", k_dyn_adv_cen2<T>, (S_dyn_adv_cen2<T>), g, d, un, vn, wn, ua, va);
     } else {
         const char *nm = which == 1 ? "dyn_keg" : which == 2 ? "dyn_zad" : "dyn_adv";
-        LAUNCH(nm, k_dyn_keg_zad<T>, g, d, (which & 1), kscheme, (which & 2) ? 1 : 0, un, vn, wn, ua, va);
+        LAUNCHB(nm, k_dyn_keg_zad<T>, (S_dyn_keg_zad<T>), g, d, (which & 1), kscheme, (which & 2) ? 1 : 0, un, vn, wn, ua, va);
     }
     return 0;
 }
@@ -361,27 +377,35 @@ __device__ __forceinline__ void vor_ent_pt(const DevDomain &d, int kvor, const T
 // instantiation carries only its own stencil, which keeps register use (and so
 // occupancy) at what that scheme needs.
 template <typename T, int SCHEME, int NCOR, int NRVM, int NTOT>
+struct S_dyn_vor {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, const T * __restrict__ un, const T * __restrict__ vn, T * __restrict__ ua, T * __restrict__ va)
+    {
+        THREAD_IJ_V(vb_, 2, 2);
+        if (i > d.jpi - 1 || j > d.jpj - 1) return;
+        for (int k = 1; k <= d.jpk - 1; k++) {
+            const size_t p = IJK(i, j, k);
+            T a = ua[p], b = va[p];
+            switch (SCHEME) {
+            case NEMO_NP_ENE: vor_ene_pt(d, NTOT, un, vn, i, j, k, a, b); break;
+            case NEMO_NP_ENS: vor_ens_pt(d, NTOT, un, vn, i, j, k, a, b); break;
+            case NEMO_NP_MIX:
+                vor_ens_pt(d, NRVM, un, vn, i, j, k, a, b);
+                vor_ene_pt(d, NCOR, un, vn, i, j, k, a, b);
+                break;
+            case NEMO_NP_ENT: vor_ent_pt(d, NTOT, un, vn, i, j, k, a, b); break;
+            default: vor_een_pt(d, SCHEME, NTOT, un, vn, i, j, k, a, b); break;
+            }
+            ua[p] = a;
+            va[p] = b;
+        }
+    }
+};
+template <typename T, int SCHEME, int NCOR, int NRVM, int NTOT>
 KERNEL void k_dyn_vor(DevDomain d, const T * __restrict__ un, const T * __restrict__ vn, T * __restrict__ ua,
                       T * __restrict__ va)
 {
-    THREAD_IJ(2, 2);
-    if (i > d.jpi - 1 || j > d.jpj - 1) return;
-    for (int k = 1; k <= d.jpk - 1; k++) {
-        const size_t p = IJK(i, j, k);
-        T a = ua[p], b = va[p];
-        switch (SCHEME) {
-        case NEMO_NP_ENE: vor_ene_pt(d, NTOT, un, vn, i, j, k, a, b); break;
-        case NEMO_NP_ENS: vor_ens_pt(d, NTOT, un, vn, i, j, k, a, b); break;
-        case NEMO_NP_MIX:
-            vor_ens_pt(d, NRVM, un, vn, i, j, k, a, b);
-            vor_ene_pt(d, NCOR, un, vn, i, j, k, a, b);
-            break;
-        case NEMO_NP_ENT: vor_ent_pt(d, NTOT, un, vn, i, j, k, a, b); break;
-        default: vor_een_pt(d, SCHEME, NTOT, un, vn, i, j, k, a, b); break;
-        }
-        ua[p] = a;
-        va[p] = b;
-    }
+    S_dyn_vor<T, SCHEME, NCOR, NRVM, NTOT>::run(vb_hw(), d, un, vn, ua, va);
 }
 
 template <typename T, int SCHEME>
@@ -391,13 +415,13 @@ static int launch_dyn_vor_s(const DevDomain &d, const T *un, const T *vn, T *ua,
     // (nrvm, ntot) from the momentum advection form, DYN/dynvor.F90:903-939
     switch (d.n_dynadv) {
     case NEMO_NP_LIN_DYN:
-        LAUNCH("dyn_vor", (k_dyn_vor<T, SCHEME, np_COR, np_COR, np_COR>), g, d, un, vn, ua, va);
+        LAUNCHB("dyn_vor", (k_dyn_vor<T, SCHEME, np_COR, np_COR, np_COR>), (S_dyn_vor<T, SCHEME, np_COR, np_COR, np_COR>), g, d, un, vn, ua, va);
         break;
     case NEMO_NP_VEC_C2:
-        LAUNCH("dyn_vor", (k_dyn_vor<T, SCHEME, np_COR, np_RVO, np_CRV>), g, d, un, vn, ua, va);
+        LAUNCHB("dyn_vor", (k_dyn_vor<T, SCHEME, np_COR, np_RVO, np_CRV>), (S_dyn_vor<T, SCHEME, np_COR, np_RVO, np_CRV>), g, d, un, vn, ua, va);
         break;
     default:
-        LAUNCH("dyn_vor", (k_dyn_vor<T, SCHEME, np_COR, np_MET, np_CME>), g, d, un, vn, ua, va);
+        LAUNCHB("dyn_vor", (k_dyn_vor<T, SCHEME, np_COR, np_MET, np_CME>), (S_dyn_vor<T, SCHEME, np_COR, np_MET, np_CME>), g, d, un, vn, ua, va);
         break;
     }
     return 0;
@@ -664,19 +688,27 @@ __device__ __forceinline__ void zdf_column_reg(const DevDomain &d, double r2dt, 
 #define ZDF_REG_KMAX 76
 
 template <typename T, bool REG>
+struct S_dyn_zdf {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 2;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, double r2dt, const T * __restrict__ ub, const T * __restrict__ vb, T * __restrict__ ua, T * __restrict__ va, const T * __restrict__ avm, const double * __restrict__ rCdU_bot, const double * __restrict__ ua_b, const double * __restrict__ va_b, const double * __restrict__ utau, const double * __restrict__ utau_b, const double * __restrict__ vtau, const double * __restrict__ vtau_b, double * __restrict__ zwd_s)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        if constexpr (REG) {
+            zdf_column_reg<ZDF_REG_KMAX>(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b);
+            zdf_column_reg<ZDF_REG_KMAX>(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b);
+        } else {
+            zdf_column(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, zwd_s);
+            zdf_column(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, zwd_s);
+        }
+    }
+};
+template <typename T, bool REG>
 KERNEL_W(2) void k_dyn_zdf(DevDomain d, double r2dt, const T * __restrict__ ub, const T * __restrict__ vb, T * __restrict__ ua, T * __restrict__ va, const T * __restrict__ avm,
                           const double * __restrict__ rCdU_bot, const double * __restrict__ ua_b, const double * __restrict__ va_b, const double * __restrict__ utau,
                           const double * __restrict__ utau_b, const double * __restrict__ vtau, const double * __restrict__ vtau_b, double * __restrict__ zwd_s)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    if constexpr (REG) {
-        zdf_column_reg<ZDF_REG_KMAX>(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b);
-        zdf_column_reg<ZDF_REG_KMAX>(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b);
-    } else {
-        zdf_column(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, zwd_s);
-        zdf_column(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, zwd_s);
-    }
+    S_dyn_zdf<T, REG>::run(vb_hw(), d, r2dt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau, utau_b, vtau, vtau_b, zwd_s);
 }
 // the register form applies: FP32 storage, jpk <= ZDF_REG_KMAX, switch zdf_reg
 template <typename T>
@@ -696,12 +728,12 @@ int launch_dyn_zdf(double r2dt, const T *ub, const T *vb, T *ua, T *va, const T 
     // forward-substituted values stay in ua/va at storage precision
     if constexpr (sizeof(T) == 4) {
         if (zdf_reg<T>(d)) {
-            LAUNCH("dyn_zdf", (k_dyn_zdf<T, true>), g, d, r2dt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau,
+            LAUNCHB("dyn_zdf", (k_dyn_zdf<T, true>), (S_dyn_zdf<T, true>), g, d, r2dt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau,
                    utau_b, vtau, vtau_b, zwd);
             return 0;
         }
     }
-    LAUNCH("dyn_zdf", (k_dyn_zdf<T, false>), g, d, r2dt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau, utau_b,
+    LAUNCHB("dyn_zdf", (k_dyn_zdf<T, false>), (S_dyn_zdf<T, false>), g, d, r2dt, ub, vb, ua, va, avm, rCdU_bot, ua_b, va_b, utau, utau_b,
            vtau, vtau_b, zwd);
     return 0;
 }
@@ -726,6 +758,119 @@ template int launch_dyn_zdf<float>(double, const float *, const float *, float *
 // separate calls bit for bit in either storage form.
 // ---------------------------------------------------------------------------
 template <typename T, int ADV, int KEG, int SCHEME, int NCOR, int NRVM, int NTOT>
+struct S_step_dyn1 {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, const T *__restrict__ un, const T *__restrict__ vn, const T *__restrict__ wn, T *__restrict__ ua, T *__restrict__ va, const double *__restrict__ r1_hu_n, const double *__restrict__ r1_hv_n, const double *__restrict__ un_b, const double *__restrict__ vn_b, double *__restrict__ zu_frc, double *__restrict__ zv_frc, double *__restrict__ su_s, double *__restrict__ sv_s, double *__restrict__ zwxb, double *__restrict__ zwyb)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const size_t ij = IJ(i, j);
+        const int jpk = d.jpk;
+        if (!is_interior(d, i, j)) {
+            for (int k = 1; k <= jpk; k++) {
+                A3(ua, i, j, k) = T(0);
+                A3(va, i, j, k) = T(0);
+            }
+            double su = 0.0, sv = 0.0;
+            su = su * r1_hu_n[ij];
+            sv = sv * r1_hv_n[ij];
+            zu_frc[ij] = su; su_s[ij] = su;
+            zv_frc[ij] = sv; sv_s[ij] = sv;
+            return;
+        }
+        const Metrics<T> &m = met<T>(d);
+        const T e1u = m.e1u[ij], e2v = m.e2v[ij], r1_e1e2u = m.r1_e1e2u[ij], r1_e1e2v = m.r1_e1e2v[ij];
+        const T e1e2t = m.e1e2t[ij], e1e2t_e = m.e1e2t[IJ(i + 1, j)], e1e2t_n = m.e1e2t[IJ(i, j + 1)];
+        // vertical flux carried between levels: zad (dynzad.F90:93-94) or cen2 surface flux (dynadv_cen2.F90:105-112)
+        T cu_k = T(0), cv_k = T(0);
+        if (ADV == NEMO_NP_FLX_C2) {
+            cu_k = T(0.5) * (e1e2t * A3(wn, i, j, 1) + e1e2t_e * A3(wn, i + 1, j, 1)) * A3(un, i, j, 1);
+            cv_k = T(0.5) * (e1e2t * A3(wn, i, j, 1) + e1e2t_n * A3(wn, i, j + 1, 1)) * A3(vn, i, j, 1);
+        }
+        double su = 0.0, sv = 0.0;
+        for (int k = 1; k <= jpk - 1; k++) {
+            T a = T(0), b = T(0);
+            if (ADV == NEMO_NP_VEC_C2) {
+                T h, he, hn;
+                if (KEG == 0) {
+                    h = keg_c2(d, un, vn, i, j, k); he = keg_c2(d, un, vn, i + 1, j, k); hn = keg_c2(d, un, vn, i, j + 1, k);
+                } else {
+                    h = keg_hw(d, un, vn, i, j, k); he = keg_hw(d, un, vn, i + 1, j, k); hn = keg_hw(d, un, vn, i, j + 1, k);
+                }
+                a = a - (he - h) / e1u;
+                b = b - (hn - h) / e2v;
+                T cu_kp1 = T(0), cv_kp1 = T(0);
+                if (k + 1 <= jpk - 1) {
+                    const int kk = k + 1;
+                    const T zww = T(0.25) * e1e2t * A3(wn, i, j, kk);
+                    const T zww_e = T(0.25) * e1e2t_e * A3(wn, i + 1, j, kk);
+                    const T zww_n = T(0.25) * e1e2t_n * A3(wn, i, j + 1, kk);
+                    cu_kp1 = (zww_e + zww) * (A3(un, i, j, kk - 1) - A3(un, i, j, kk));
+                    cv_kp1 = (zww_n + zww) * (A3(vn, i, j, kk - 1) - A3(vn, i, j, kk));
+                }
+                a = a - (cu_k + cu_kp1) * r1_e1e2u / E3(m.e3u, ij, k);
+                b = b - (cv_k + cv_kp1) * r1_e1e2v / E3(m.e3v, ij, k);
+                cu_k = cu_kp1;
+                cv_k = cv_kp1;
+            } else if (ADV == NEMO_NP_FLX_C2) {
+                const T fu_im1 = cen2_zfu(d, un, i - 1, j, k), fu_i = cen2_zfu(d, un, i, j, k), fu_ip1 = cen2_zfu(d, un, i + 1, j, k);
+                const T fu_jp1 = cen2_zfu(d, un, i, j + 1, k), fu_im1jp1 = cen2_zfu(d, un, i - 1, j + 1, k);
+                const T fv_i = cen2_zfv(d, vn, i, j, k), fv_ip1 = cen2_zfv(d, vn, i + 1, j, k);
+                const T fv_jm1 = cen2_zfv(d, vn, i, j - 1, k), fv_ip1jm1 = cen2_zfv(d, vn, i + 1, j - 1, k);
+                const T fv_jp1 = cen2_zfv(d, vn, i, j + 1, k), fv_im1 = cen2_zfv(d, vn, i - 1, j, k);
+                const T u_i = A3(un, i, j, k), u_ip1 = A3(un, i + 1, j, k), u_im1 = A3(un, i - 1, j, k);
+                const T u_jp1 = A3(un, i, j + 1, k), u_jm1 = A3(un, i, j - 1, k);
+                const T v_j = A3(vn, i, j, k), v_ip1 = A3(vn, i + 1, j, k), v_jp1 = A3(vn, i, j + 1, k);
+                const T v_im1 = A3(vn, i - 1, j, k), v_jm1 = A3(vn, i, j - 1, k);
+                const T zfu_t_ip1 = (fu_i + fu_ip1) * (u_i + u_ip1);
+                const T zfu_t_i = (fu_im1 + fu_i) * (u_im1 + u_i);
+                const T zfv_f_j = (fv_i + fv_ip1) * (u_i + u_jp1);
+                const T zfv_f_jm1 = (fv_jm1 + fv_ip1jm1) * (u_jm1 + u_i);
+                const T zfu_f_i = (fu_i + fu_jp1) * (v_j + v_ip1);
+                const T zfu_f_im1 = (fu_im1 + fu_im1jp1) * (v_im1 + v_j);
+                const T zfv_t_jp1 = (fv_i + fv_jp1) * (v_j + v_jp1);
+                const T zfv_t_j = (fv_jm1 + fv_i) * (v_jm1 + v_j);
+                a = a - (zfu_t_ip1 - zfu_t_i + zfv_f_j - zfv_f_jm1) * r1_e1e2u / E3(m.e3u, ij, k);
+                b = b - (zfu_f_i - zfu_f_im1 + zfv_t_jp1 - zfv_t_j) * r1_e1e2v / E3(m.e3v, ij, k);
+                T cu_kp1 = T(0), cv_kp1 = T(0);
+                if (k + 1 <= jpk - 1) {
+                    const int kk = k + 1;
+                    const T fw = cen2_zfw(d, wn, i, j, kk);
+                    cu_kp1 = (fw + cen2_zfw(d, wn, i + 1, j, kk)) * (A3(un, i, j, kk) + A3(un, i, j, kk - 1));
+                    cv_kp1 = (fw + cen2_zfw(d, wn, i, j + 1, kk)) * (A3(vn, i, j, kk) + A3(vn, i, j, kk - 1));
+                }
+                a = a - (cu_k - cu_kp1) * r1_e1e2u / E3(m.e3u, ij, k);
+                b = b - (cv_k - cv_kp1) * r1_e1e2v / E3(m.e3v, ij, k);
+                cu_k = cu_kp1;
+                cv_k = cv_kp1;
+            }
+            switch (SCHEME) {
+            case NEMO_NP_ENE: vor_ene_pt(d, NTOT, un, vn, i, j, k, a, b); break;
+            case NEMO_NP_ENS: vor_ens_pt(d, NTOT, un, vn, i, j, k, a, b); break;
+            case NEMO_NP_MIX:
+                vor_ens_pt(d, NRVM, un, vn, i, j, k, a, b);
+                vor_ene_pt(d, NCOR, un, vn, i, j, k, a, b);
+                break;
+            case NEMO_NP_ENT: vor_ent_pt(d, NTOT, un, vn, i, j, k, a, b); break;
+            default: vor_een_pt(d, SCHEME, NTOT, un, vn, i, j, k, a, b); break;
+            }
+            const size_t p = IJK(i, j, k);
+            ua[p] = a;
+            va[p] = b;
+            su = su + (double)E3(m.e3u, ij, k) * (double)a * UMASK(double, i, j, k);
+            sv = sv + (double)E3(m.e3v, ij, k) * (double)b * VMASK(double, i, j, k);
+        }
+        A3(ua, i, j, jpk) = T(0);
+        A3(va, i, j, jpk) = T(0);
+        su = su * r1_hu_n[ij];
+        sv = sv * r1_hv_n[ij];
+        zu_frc[ij] = su; su_s[ij] = su;
+        zv_frc[ij] = sv; sv_s[ij] = sv;
+        zwxb[ij] = (double)A3(un, i, j, d.mbku[ij]) - un_b[ij];
+        zwyb[ij] = (double)A3(vn, i, j, d.mbkv[ij]) - vn_b[ij];
+    }
+};
+template <typename T, int ADV, int KEG, int SCHEME, int NCOR, int NRVM, int NTOT>
 KERNEL void k_step_dyn1(DevDomain d, const T *__restrict__ un, const T *__restrict__ vn, const T *__restrict__ wn,
                         T *__restrict__ ua, T *__restrict__ va, const double *__restrict__ r1_hu_n,
                         const double *__restrict__ r1_hv_n, const double *__restrict__ un_b,
@@ -733,112 +878,7 @@ KERNEL void k_step_dyn1(DevDomain d, const T *__restrict__ un, const T *__restri
                         double *__restrict__ su_s, double *__restrict__ sv_s, double *__restrict__ zwxb,
                         double *__restrict__ zwyb)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const size_t ij = IJ(i, j);
-    const int jpk = d.jpk;
-    if (!is_interior(d, i, j)) {
-        for (int k = 1; k <= jpk; k++) {
-            A3(ua, i, j, k) = T(0);
-            A3(va, i, j, k) = T(0);
-        }
-        double su = 0.0, sv = 0.0;
-        su = su * r1_hu_n[ij];
-        sv = sv * r1_hv_n[ij];
-        zu_frc[ij] = su; su_s[ij] = su;
-        zv_frc[ij] = sv; sv_s[ij] = sv;
-        return;
-    }
-    const Metrics<T> &m = met<T>(d);
-    const T e1u = m.e1u[ij], e2v = m.e2v[ij], r1_e1e2u = m.r1_e1e2u[ij], r1_e1e2v = m.r1_e1e2v[ij];
-    const T e1e2t = m.e1e2t[ij], e1e2t_e = m.e1e2t[IJ(i + 1, j)], e1e2t_n = m.e1e2t[IJ(i, j + 1)];
-    // vertical flux carried between levels: zad (dynzad.F90:93-94) or cen2 surface flux (dynadv_cen2.F90:105-112)
-    T cu_k = T(0), cv_k = T(0);
-    if (ADV == NEMO_NP_FLX_C2) {
-        cu_k = T(0.5) * (e1e2t * A3(wn, i, j, 1) + e1e2t_e * A3(wn, i + 1, j, 1)) * A3(un, i, j, 1);
-        cv_k = T(0.5) * (e1e2t * A3(wn, i, j, 1) + e1e2t_n * A3(wn, i, j + 1, 1)) * A3(vn, i, j, 1);
-    }
-    double su = 0.0, sv = 0.0;
-    for (int k = 1; k <= jpk - 1; k++) {
-        T a = T(0), b = T(0);
-        if (ADV == NEMO_NP_VEC_C2) {
-            T h, he, hn;
-            if (KEG == 0) {
-                h = keg_c2(d, un, vn, i, j, k); he = keg_c2(d, un, vn, i + 1, j, k); hn = keg_c2(d, un, vn, i, j + 1, k);
-            } else {
-                h = keg_hw(d, un, vn, i, j, k); he = keg_hw(d, un, vn, i + 1, j, k); hn = keg_hw(d, un, vn, i, j + 1, k);
-            }
-            a = a - (he - h) / e1u;
-            b = b - (hn - h) / e2v;
-            T cu_kp1 = T(0), cv_kp1 = T(0);
-            if (k + 1 <= jpk - 1) {
-                const int kk = k + 1;
-                const T zww = T(0.25) * e1e2t * A3(wn, i, j, kk);
-                const T zww_e = T(0.25) * e1e2t_e * A3(wn, i + 1, j, kk);
-                const T zww_n = T(0.25) * e1e2t_n * A3(wn, i, j + 1, kk);
-                cu_kp1 = (zww_e + zww) * (A3(un, i, j, kk - 1) - A3(un, i, j, kk));
-                cv_kp1 = (zww_n + zww) * (A3(vn, i, j, kk - 1) - A3(vn, i, j, kk));
-            }
-            a = a - (cu_k + cu_kp1) * r1_e1e2u / E3(m.e3u, ij, k);
-            b = b - (cv_k + cv_kp1) * r1_e1e2v / E3(m.e3v, ij, k);
-            cu_k = cu_kp1;
-            cv_k = cv_kp1;
-        } else if (ADV == NEMO_NP_FLX_C2) {
-            const T fu_im1 = cen2_zfu(d, un, i - 1, j, k), fu_i = cen2_zfu(d, un, i, j, k), fu_ip1 = cen2_zfu(d, un, i + 1, j, k);
-            const T fu_jp1 = cen2_zfu(d, un, i, j + 1, k), fu_im1jp1 = cen2_zfu(d, un, i - 1, j + 1, k);
-            const T fv_i = cen2_zfv(d, vn, i, j, k), fv_ip1 = cen2_zfv(d, vn, i + 1, j, k);
-            const T fv_jm1 = cen2_zfv(d, vn, i, j - 1, k), fv_ip1jm1 = cen2_zfv(d, vn, i + 1, j - 1, k);
-            const T fv_jp1 = cen2_zfv(d, vn, i, j + 1, k), fv_im1 = cen2_zfv(d, vn, i - 1, j, k);
-            const T u_i = A3(un, i, j, k), u_ip1 = A3(un, i + 1, j, k), u_im1 = A3(un, i - 1, j, k);
-            const T u_jp1 = A3(un, i, j + 1, k), u_jm1 = A3(un, i, j - 1, k);
-            const T v_j = A3(vn, i, j, k), v_ip1 = A3(vn, i + 1, j, k), v_jp1 = A3(vn, i, j + 1, k);
-            const T v_im1 = A3(vn, i - 1, j, k), v_jm1 = A3(vn, i, j - 1, k);
-            const T zfu_t_ip1 = (fu_i + fu_ip1) * (u_i + u_ip1);
-            const T zfu_t_i = (fu_im1 + fu_i) * (u_im1 + u_i);
-            const T zfv_f_j = (fv_i + fv_ip1) * (u_i + u_jp1);
-            const T zfv_f_jm1 = (fv_jm1 + fv_ip1jm1) * (u_jm1 + u_i);
-            const T zfu_f_i = (fu_i + fu_jp1) * (v_j + v_ip1);
-            const T zfu_f_im1 = (fu_im1 + fu_im1jp1) * (v_im1 + v_j);
-            const T zfv_t_jp1 = (fv_i + fv_jp1) * (v_j + v_jp1);
-            const T zfv_t_j = (fv_jm1 + fv_i) * (v_jm1 + v_j);
-            a = a - (zfu_t_ip1 - zfu_t_i + zfv_f_j - zfv_f_jm1) * r1_e1e2u / E3(m.e3u, ij, k);
-            b = b - (zfu_f_i - zfu_f_im1 + zfv_t_jp1 - zfv_t_j) * r1_e1e2v / E3(m.e3v, ij, k);
-            T cu_kp1 = T(0), cv_kp1 = T(0);
-            if (k + 1 <= jpk - 1) {
-                const int kk = k + 1;
-                const T fw = cen2_zfw(d, wn, i, j, kk);
-                cu_kp1 = (fw + cen2_zfw(d, wn, i + 1, j, kk)) * (A3(un, i, j, kk) + A3(un, i, j, kk - 1));
-                cv_kp1 = (fw + cen2_zfw(d, wn, i, j + 1, kk)) * (A3(vn, i, j, kk) + A3(vn, i, j, kk - 1));
-            }
-            a = a - (cu_k - cu_kp1) * r1_e1e2u / E3(m.e3u, ij, k);
-            b = b - (cv_k - cv_kp1) * r1_e1e2v / E3(m.e3v, ij, k);
-            cu_k = cu_kp1;
-            cv_k = cv_kp1;
-        }
-        switch (SCHEME) {
-        case NEMO_NP_ENE: vor_ene_pt(d, NTOT, un, vn, i, j, k, a, b); break;
-        case NEMO_NP_ENS: vor_ens_pt(d, NTOT, un, vn, i, j, k, a, b); break;
-        case NEMO_NP_MIX:
-            vor_ens_pt(d, NRVM, un, vn, i, j, k, a, b);
-            vor_ene_pt(d, NCOR, un, vn, i, j, k, a, b);
-            break;
-        case NEMO_NP_ENT: vor_ent_pt(d, NTOT, un, vn, i, j, k, a, b); break;
-        default: vor_een_pt(d, SCHEME, NTOT, un, vn, i, j, k, a, b); break;
-        }
-        const size_t p = IJK(i, j, k);
-        ua[p] = a;
-        va[p] = b;
-        su = su + (double)E3(m.e3u, ij, k) * (double)a * UMASK(double, i, j, k);
-        sv = sv + (double)E3(m.e3v, ij, k) * (double)b * VMASK(double, i, j, k);
-    }
-    A3(ua, i, j, jpk) = T(0);
-    A3(va, i, j, jpk) = T(0);
-    su = su * r1_hu_n[ij];
-    sv = sv * r1_hv_n[ij];
-    zu_frc[ij] = su; su_s[ij] = su;
-    zv_frc[ij] = sv; sv_s[ij] = sv;
-    zwxb[ij] = (double)A3(un, i, j, d.mbku[ij]) - un_b[ij];
-    zwyb[ij] = (double)A3(vn, i, j, d.mbkv[ij]) - vn_b[ij];
+    S_step_dyn1<T, ADV, KEG, SCHEME, NCOR, NRVM, NTOT>::run(vb_hw(), d, un, vn, wn, ua, va, r1_hu_n, r1_hv_n, un_b, vn_b, zu_frc, zv_frc, su_s, sv_s, zwxb, zwyb);
 }
 
 // ua after dyn_spg_ts: interior columns remove the vertical mean of the
@@ -858,6 +898,41 @@ struct PreSpg {
 };
 
 template <typename T, bool REG>
+struct S_step_dyn2 {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 2;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, double r2dt, double r1_2dt_b, const T *__restrict__ ub, const T *__restrict__ vb, T *__restrict__ ua, T *__restrict__ va, T *__restrict__ un, T *__restrict__ vn, const T *__restrict__ avm, const double *__restrict__ rCdU_bot, const double *__restrict__ ua_b, const double *__restrict__ va_b, const double *__restrict__ ub_b, const double *__restrict__ vb_b, const double *__restrict__ un_adv, const double *__restrict__ vn_adv, const double *__restrict__ r1_hu_n, const double *__restrict__ r1_hv_n, const double *__restrict__ un_b, const double *__restrict__ vn_b, const double *__restrict__ su_s, const double *__restrict__ sv_s, const double *__restrict__ utau, const double *__restrict__ utau_b, const double *__restrict__ vtau, const double *__restrict__ vtau_b, double *__restrict__ zwd_s)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const size_t ij = IJ(i, j);
+        const size_t plane = d.n2;
+        const uint32_t ij32 = IJ32(i, j);
+        // phase 3 on the now velocities, lines 1254-1259
+        {
+            const double cu = un_adv[ij] * r1_hu_n[ij], cv = vn_adv[ij] * r1_hv_n[ij];
+            const double ubn = un_b[ij], vbn = vn_b[ij];
+            const int u0 = d.kU.k0[ij], u1 = d.kU.k1[ij], v0 = d.kV.k0[ij], v1 = d.kV.k1[ij];
+    #pragma unroll 4
+            for (int k = 1; k <= d.jpk - 1; k++) {
+                T *uk = plane_ptr(un, plane, k), *vk = plane_ptr(vn, plane, k);
+                const double um = (k >= u0 && k <= u1) ? 1.0 : 0.0, vm = (k >= v0 && k <= v1) ? 1.0 : 0.0;
+                stp(uk, ij32, (T)(((double)ldp((const T *)uk, ij32) + cu - ubn) * um));
+                stp(vk, ij32, (T)(((double)ldp((const T *)vk, ij32) + cv - vbn) * vm));
+            }
+        }
+        const bool inner = is_interior(d, i, j);
+        PreSpg<T> pu{su_s[ij], (ua_b[ij] - ub_b[ij]) * r1_2dt_b, 0.0, d.kU.k0[ij], d.kU.k1[ij], inner};
+        PreSpg<T> pv{sv_s[ij], (va_b[ij] - vb_b[ij]) * r1_2dt_b, 0.0, d.kV.k0[ij], d.kV.k1[ij], inner};
+        if constexpr (REG) {
+            zdf_column_reg<ZDF_REG_KMAX>(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, pu);
+            zdf_column_reg<ZDF_REG_KMAX>(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, pv);
+        } else {
+            zdf_column(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, zwd_s, pu);
+            zdf_column(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, zwd_s, pv);
+        }
+    }
+};
+template <typename T, bool REG>
 KERNEL_W(2) void k_step_dyn2(DevDomain d, double r2dt, double r1_2dt_b, const T *__restrict__ ub, const T *__restrict__ vb,
                         T *__restrict__ ua, T *__restrict__ va, T *__restrict__ un, T *__restrict__ vn,
                         const T *__restrict__ avm, const double *__restrict__ rCdU_bot,
@@ -871,34 +946,7 @@ KERNEL_W(2) void k_step_dyn2(DevDomain d, double r2dt, double r1_2dt_b, const T 
                         const double *__restrict__ vtau, const double *__restrict__ vtau_b,
                         double *__restrict__ zwd_s)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const size_t ij = IJ(i, j);
-    const size_t plane = d.n2;
-    const uint32_t ij32 = IJ32(i, j);
-    // phase 3 on the now velocities, lines 1254-1259
-    {
-        const double cu = un_adv[ij] * r1_hu_n[ij], cv = vn_adv[ij] * r1_hv_n[ij];
-        const double ubn = un_b[ij], vbn = vn_b[ij];
-        const int u0 = d.kU.k0[ij], u1 = d.kU.k1[ij], v0 = d.kV.k0[ij], v1 = d.kV.k1[ij];
-#pragma unroll 4
-        for (int k = 1; k <= d.jpk - 1; k++) {
-            T *uk = plane_ptr(un, plane, k), *vk = plane_ptr(vn, plane, k);
-            const double um = (k >= u0 && k <= u1) ? 1.0 : 0.0, vm = (k >= v0 && k <= v1) ? 1.0 : 0.0;
-            stp(uk, ij32, (T)(((double)ldp((const T *)uk, ij32) + cu - ubn) * um));
-            stp(vk, ij32, (T)(((double)ldp((const T *)vk, ij32) + cv - vbn) * vm));
-        }
-    }
-    const bool inner = is_interior(d, i, j);
-    PreSpg<T> pu{su_s[ij], (ua_b[ij] - ub_b[ij]) * r1_2dt_b, 0.0, d.kU.k0[ij], d.kU.k1[ij], inner};
-    PreSpg<T> pv{sv_s[ij], (va_b[ij] - vb_b[ij]) * r1_2dt_b, 0.0, d.kV.k0[ij], d.kV.k1[ij], inner};
-    if constexpr (REG) {
-        zdf_column_reg<ZDF_REG_KMAX>(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, pu);
-        zdf_column_reg<ZDF_REG_KMAX>(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, pv);
-    } else {
-        zdf_column(d, r2dt, 0, i, j, ub, ua, avm, rCdU_bot, ua_b, utau, utau_b, zwd_s, pu);
-        zdf_column(d, r2dt, 1, i, j, vb, va, avm, rCdU_bot, va_b, vtau, vtau_b, zwd_s, pv);
-    }
+    S_step_dyn2<T, REG>::run(vb_hw(), d, r2dt, r1_2dt_b, ub, vb, ua, va, un, vn, avm, rCdU_bot, ua_b, va_b, ub_b, vb_b, un_adv, vn_adv, r1_hu_n, r1_hv_n, un_b, vn_b, su_s, sv_s, utau, utau_b, vtau, vtau_b, zwd_s);
 }
 
 template <typename T, int ADV, int KEG, int SCHEME>
@@ -908,15 +956,15 @@ static int launch_step_dyn1_s(const DevDomain &d, Grid2 g, const T *un, const T 
     const double *r1u = st->r1_hu_n, *r1v = st->r1_hv_n, *ub = st->un_b, *vb = st->vn_b;
     switch (ADV) {
     case NEMO_NP_LIN_DYN:
-        LAUNCH("step_dyn", (k_step_dyn1<T, ADV, KEG, SCHEME, np_COR, np_COR, np_COR>), g, d, un, vn, wn, ua, va, r1u,
+        LAUNCHB("step_dyn", (k_step_dyn1<T, ADV, KEG, SCHEME, np_COR, np_COR, np_COR>), (S_step_dyn1<T, ADV, KEG, SCHEME, np_COR, np_COR, np_COR>), g, d, un, vn, wn, ua, va, r1u,
                r1v, ub, vb, w[0], w[1], w[2], w[3], w[4], w[5]);
         break;
     case NEMO_NP_VEC_C2:
-        LAUNCH("step_dyn", (k_step_dyn1<T, ADV, KEG, SCHEME, np_COR, np_RVO, np_CRV>), g, d, un, vn, wn, ua, va, r1u,
+        LAUNCHB("step_dyn", (k_step_dyn1<T, ADV, KEG, SCHEME, np_COR, np_RVO, np_CRV>), (S_step_dyn1<T, ADV, KEG, SCHEME, np_COR, np_RVO, np_CRV>), g, d, un, vn, wn, ua, va, r1u,
                r1v, ub, vb, w[0], w[1], w[2], w[3], w[4], w[5]);
         break;
     default:
-        LAUNCH("step_dyn", (k_step_dyn1<T, ADV, KEG, SCHEME, np_COR, np_MET, np_CME>), g, d, un, vn, wn, ua, va, r1u,
+        LAUNCHB("step_dyn", (k_step_dyn1<T, ADV, KEG, SCHEME, np_COR, np_MET, np_CME>), (S_step_dyn1<T, ADV, KEG, SCHEME, np_COR, np_MET, np_CME>), g, d, un, vn, wn, ua, va, r1u,
                r1v, ub, vb, w[0], w[1], w[2], w[3], w[4], w[5]);
         break;
     }
@@ -962,13 +1010,13 @@ int launch_step_dyn(int kt, double r2dt, const T *ub, const T *vb, const T *wn, 
     double *zwd = (double *)nh::scratch3(nh::S_ZWT);
     if constexpr (sizeof(T) == 4) {
         if (zdf_reg<T>(d)) {
-            LAUNCH("step_dyn", (k_step_dyn2<T, true>), g, d, r2dt, 1.0 / z2dt_bf, ub, vb, ua, va, un, vn, avm,
+            LAUNCHB("step_dyn", (k_step_dyn2<T, true>), (S_step_dyn2<T, true>), g, d, r2dt, 1.0 / z2dt_bf, ub, vb, ua, va, un, vn, avm,
                    st->rCdU_bot, st->ua_b, st->va_b, st->ub_b, st->vb_b, st->un_adv, st->vn_adv, st->r1_hu_n,
                    st->r1_hv_n, st->un_b, st->vn_b, w[2], w[3], st->utau, utau_b, st->vtau, vtau_b, zwd);
             return 0;
         }
     }
-    LAUNCH("step_dyn", (k_step_dyn2<T, false>), g, d, r2dt, 1.0 / z2dt_bf, ub, vb, ua, va, un, vn, avm,
+    LAUNCHB("step_dyn", (k_step_dyn2<T, false>), (S_step_dyn2<T, false>), g, d, r2dt, 1.0 / z2dt_bf, ub, vb, ua, va, un, vn, avm,
            st->rCdU_bot, st->ua_b, st->va_b, st->ub_b, st->vb_b, st->un_adv, st->vn_adv, st->r1_hu_n, st->r1_hv_n,
            st->un_b, st->vn_b, w[2], w[3], st->utau, utau_b, st->vtau, vtau_b, zwd);
     return 0;

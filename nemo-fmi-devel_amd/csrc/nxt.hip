@@ -69,27 +69,35 @@ struct HdivCol {
 // then lbc_lnk('T') of all jpk levels.  One thread per cell.
 // ---------------------------------------------------------------------------
 template <typename T>
-KERNEL void k_div_hor(DevDomain d, const T *__restrict__ un, const T *__restrict__ vn, T *__restrict__ hdivn)
-{
-    THREAD_IJ(1, 1);
-    const int k0 = (int)blockIdx.z * NXT_KB + 1;
-    if (i > d.jpi || j > d.jpj) return;
-    int si, sj;
-    const bool ok = lbc_src(d, 'T', i, j, si, sj);
-    const bool ring = si != i || sj != j || !ok;
-    const HdivCol<T> c(d, un, vn, ok ? si : 2, ok ? sj : 2);
-    const uint32_t ij = IJ32(i, j);
-#pragma unroll
-    for (int q = 0; q < NXT_KB; q++) {
-        const int k = k0 + q;
-        if (k > d.jpk) break;
-        T *h = plane_ptr(hdivn, d.n2, k);
-        if (k == d.jpk) {   // level jpk is not computed: lbc_lnk only
-            if (ring) stp(h, ij, ok ? ldp((const T *)h, c.ij) : T(0));
-        } else {
-            stp(h, ij, ok ? (T)c.at(d, k) : T(0));
+struct S_div_hor {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, const T *__restrict__ un, const T *__restrict__ vn, T *__restrict__ hdivn)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        const int k0 = (int)vb_.z * NXT_KB + 1;
+        if (i > d.jpi || j > d.jpj) return;
+        int si, sj;
+        const bool ok = lbc_src(d, 'T', i, j, si, sj);
+        const bool ring = si != i || sj != j || !ok;
+        const HdivCol<T> c(d, un, vn, ok ? si : 2, ok ? sj : 2);
+        const uint32_t ij = IJ32(i, j);
+    #pragma unroll
+        for (int q = 0; q < NXT_KB; q++) {
+            const int k = k0 + q;
+            if (k > d.jpk) break;
+            T *h = plane_ptr(hdivn, d.n2, k);
+            if (k == d.jpk) {   // level jpk is not computed: lbc_lnk only
+                if (ring) stp(h, ij, ok ? ldp((const T *)h, c.ij) : T(0));
+            } else {
+                stp(h, ij, ok ? (T)c.at(d, k) : T(0));
+            }
         }
     }
+};
+template <typename T>
+KERNEL void k_div_hor(DevDomain d, const T *__restrict__ un, const T *__restrict__ vn, T *__restrict__ hdivn)
+{
+    S_div_hor<T>::run(vb_hw(), d, un, vn, hdivn);
 }
 
 // ---------------------------------------------------------------------------
@@ -98,25 +106,33 @@ KERNEL void k_div_hor(DevDomain d, const T *__restrict__ un, const T *__restrict
 // its hdivn (at its lbc source point on the ring), the integral and ssha.
 // ---------------------------------------------------------------------------
 template <typename T>
+struct S_ssh_nxt {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, double z2dt, double zcoef, const T *__restrict__ un, const T *__restrict__ vn, T *__restrict__ hdivn, const double *__restrict__ sshb, const double *__restrict__ emp_b, const double *__restrict__ emp, double *__restrict__ ssha)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        int si, sj;
+        const bool ok = lbc_src(d, 'T', i, j, si, sj);
+        const uint32_t ij = IJ32(i, j);
+        const HdivCol<T> c(d, un, vn, ok ? si : 2, ok ? sj : 2);
+        double zhdiv = 0.0;
+    #pragma unroll 2
+        for (int k = 1; k <= d.jpk - 1; k++) {
+            const T h = ok ? (T)c.at(d, k) : T(0);
+            stp(plane_ptr(hdivn, d.n2, k), ij, h);
+            zhdiv = zhdiv + E3(d.md.e3t, ij, k) * (double)h;
+        }
+        if (si != i || sj != j || !ok) A3(hdivn, i, j, d.jpk) = ok ? A3(hdivn, si, sj, d.jpk) : T(0);
+        ssha[ij] = (sshb[ij] - z2dt * (zcoef * (emp_b[ij] + emp[ij]) + zhdiv)) * SSMASK(ij);
+    }
+};
+template <typename T>
 KERNEL void k_ssh_nxt(DevDomain d, double z2dt, double zcoef, const T *__restrict__ un, const T *__restrict__ vn,
                       T *__restrict__ hdivn, const double *__restrict__ sshb, const double *__restrict__ emp_b,
                       const double *__restrict__ emp, double *__restrict__ ssha)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    int si, sj;
-    const bool ok = lbc_src(d, 'T', i, j, si, sj);
-    const uint32_t ij = IJ32(i, j);
-    const HdivCol<T> c(d, un, vn, ok ? si : 2, ok ? sj : 2);
-    double zhdiv = 0.0;
-#pragma unroll 2
-    for (int k = 1; k <= d.jpk - 1; k++) {
-        const T h = ok ? (T)c.at(d, k) : T(0);
-        stp(plane_ptr(hdivn, d.n2, k), ij, h);
-        zhdiv = zhdiv + E3(d.md.e3t, ij, k) * (double)h;
-    }
-    if (si != i || sj != j || !ok) A3(hdivn, i, j, d.jpk) = ok ? A3(hdivn, si, sj, d.jpk) : T(0);
-    ssha[ij] = (sshb[ij] - z2dt * (zcoef * (emp_b[ij] + emp[ij]) + zhdiv)) * SSMASK(ij);
+    S_ssh_nxt<T>::run(vb_hw(), d, z2dt, zcoef, un, vn, hdivn, sshb, emp_b, emp, ssha);
 }
 
 // ---------------------------------------------------------------------------
@@ -125,32 +141,47 @@ KERNEL void k_ssh_nxt(DevDomain d, double z2dt, double zcoef, const T *__restric
 // when the scale factors are fixed).  One thread per column.
 // ---------------------------------------------------------------------------
 template <typename T>
+struct S_wzv {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, int first, double z1_2dt, const T *__restrict__ hdivn, T *__restrict__ wn)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const uint32_t ij = IJ32(i, j);
+        if (first) A3(wn, i, j, d.jpk) = T(0);
+        double w = (double)A3(wn, i, j, d.jpk);
+    #pragma unroll 4
+        for (int k = d.jpk - 1; k >= 1; k--) {
+            const double e3 = E3(d.md.e3t, ij, k);
+            w = w - (e3 * (double)A3(hdivn, i, j, k) + z1_2dt * (e3 - e3)) * TMASK(double, i, j, k);
+            A3(wn, i, j, k) = (T)w;
+        }
+    }
+};
+template <typename T>
 KERNEL void k_wzv(DevDomain d, int first, double z1_2dt, const T *__restrict__ hdivn, T *__restrict__ wn)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const uint32_t ij = IJ32(i, j);
-    if (first) A3(wn, i, j, d.jpk) = T(0);
-    double w = (double)A3(wn, i, j, d.jpk);
-#pragma unroll 4
-    for (int k = d.jpk - 1; k >= 1; k--) {
-        const double e3 = E3(d.md.e3t, ij, k);
-        w = w - (e3 * (double)A3(hdivn, i, j, k) + z1_2dt * (e3 - e3)) * TMASK(double, i, j, k);
-        A3(wn, i, j, k) = (T)w;
-    }
+    S_wzv<T>::run(vb_hw(), d, first, z1_2dt, hdivn, wn);
 }
 
 // ---------------------------------------------------------------------------
 // ssh_swp, DYN/sshwzv.F90:216-267 (linear free surface)
 // ---------------------------------------------------------------------------
+struct S_ssh_swp {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, int euler, double atfp, double *__restrict__ sshb, double *__restrict__ sshn, const double *__restrict__ ssha)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const uint32_t ij = IJ32(i, j);
+        if (!euler) sshb[ij] = sshn[ij] + atfp * (sshb[ij] - 2 * sshn[ij] + ssha[ij]);
+        sshn[ij] = ssha[ij];
+    }
+};
 KERNEL void k_ssh_swp(DevDomain d, int euler, double atfp, double *__restrict__ sshb, double *__restrict__ sshn,
                       const double *__restrict__ ssha)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const uint32_t ij = IJ32(i, j);
-    if (!euler) sshb[ij] = sshn[ij] + atfp * (sshb[ij] - 2 * sshn[ij] + ssha[ij]);
-    sshn[ij] = ssha[ij];
+    S_ssh_swp::run(vb_hw(), d, euler, atfp, sshb, sshn, ssha);
 }
 
 // ---------------------------------------------------------------------------
@@ -160,52 +191,60 @@ KERNEL void k_ssh_swp(DevDomain d, int euler, double atfp, double *__restrict__ 
 // (grid.z = jpk * 2).
 // ---------------------------------------------------------------------------
 template <typename T>
+struct S_tra_nxt {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, int euler, double atfp, T *__restrict__ tsb, T *__restrict__ tsn, T *__restrict__ tsa)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        const int k = (int)(vb_.z % (unsigned)d.jpk) + 1;
+        const size_t off = (size_t)(vb_.z / (unsigned)d.jpk) * d.n3;
+        if (i > d.jpi || j > d.jpj) return;
+        T *pb = tsb + off, *pn = tsn + off, *pa = tsa + off;
+        if (!is_interior(d, i, j)) {
+            int si, sj;
+            if (lbc_src(d, 'T', i, j, si, sj)) return;   // written by the thread of (si,sj)
+            A3(pa, i, j, k) = T(0);
+            if (euler) {
+                if (k < d.jpk) A3(pn, i, j, k) = T(0);
+            } else {
+                A3(pb, i, j, k) = T(0);
+                A3(pn, i, j, k) = T(0);
+            }
+            return;
+        }
+        const T ta = A3(pa, i, j, k);
+        T tb, tn;
+        if (euler) {
+            tb = A3(pb, i, j, k);   // only copied to ring points by the leapfrog branch
+            tn = k < d.jpk ? ta : A3(pn, i, j, k);
+            if (k < d.jpk) A3(pn, i, j, k) = tn;
+        } else if (k < d.jpk) {
+            const double ztn = (double)A3(pn, i, j, k);
+            const double ztd = (double)ta - 2.0 * ztn + (double)A3(pb, i, j, k);
+            tb = (T)(ztn + atfp * ztd);
+            tn = ta;
+            A3(pb, i, j, k) = tb;
+            A3(pn, i, j, k) = tn;
+        } else {
+            tb = A3(pb, i, j, k);
+            tn = A3(pn, i, j, k);
+        }
+        lbc_targets(d, i, j, [&](int ti, int tj) {
+            A3(pa, ti, tj, k) = ta;
+            if (euler) {
+                if (k < d.jpk) A3(pn, ti, tj, k) = tn;
+            } else {
+                A3(pb, ti, tj, k) = tb;
+                A3(pn, ti, tj, k) = tn;
+            }
+        });
+    }
+};
+template <typename T>
 KERNEL void k_tra_nxt(DevDomain d, int euler, double atfp, T *__restrict__ tsb, T *__restrict__ tsn,
                       T *__restrict__ tsa)
 {
-    THREAD_IJ(1, 1);
-    const int k = (int)(blockIdx.z % (unsigned)d.jpk) + 1;
-    const size_t off = (size_t)(blockIdx.z / (unsigned)d.jpk) * d.n3;
-    if (i > d.jpi || j > d.jpj) return;
-    T *pb = tsb + off, *pn = tsn + off, *pa = tsa + off;
-    if (!is_interior(d, i, j)) {
-        int si, sj;
-        if (lbc_src(d, 'T', i, j, si, sj)) return;   // written by the thread of (si,sj)
-        A3(pa, i, j, k) = T(0);
-        if (euler) {
-            if (k < d.jpk) A3(pn, i, j, k) = T(0);
-        } else {
-            A3(pb, i, j, k) = T(0);
-            A3(pn, i, j, k) = T(0);
-        }
-        return;
-    }
-    const T ta = A3(pa, i, j, k);
-    T tb, tn;
-    if (euler) {
-        tb = A3(pb, i, j, k);   // only copied to ring points by the leapfrog branch
-        tn = k < d.jpk ? ta : A3(pn, i, j, k);
-        if (k < d.jpk) A3(pn, i, j, k) = tn;
-    } else if (k < d.jpk) {
-        const double ztn = (double)A3(pn, i, j, k);
-        const double ztd = (double)ta - 2.0 * ztn + (double)A3(pb, i, j, k);
-        tb = (T)(ztn + atfp * ztd);
-        tn = ta;
-        A3(pb, i, j, k) = tb;
-        A3(pn, i, j, k) = tn;
-    } else {
-        tb = A3(pb, i, j, k);
-        tn = A3(pn, i, j, k);
-    }
-    lbc_targets(d, i, j, [&](int ti, int tj) {
-        A3(pa, ti, tj, k) = ta;
-        if (euler) {
-            if (k < d.jpk) A3(pn, ti, tj, k) = tn;
-        } else {
-            A3(pb, ti, tj, k) = tb;
-            A3(pn, ti, tj, k) = tn;
-        }
-    });
+    S_tra_nxt<T>::run(vb_hw(), d, euler, atfp, tsb, tsn, tsa);
 }
 
 // ---------------------------------------------------------------------------
@@ -218,59 +257,67 @@ KERNEL void k_tra_nxt(DevDomain d, int euler, double atfp, T *__restrict__ tsb, 
 // advective barotropic velocity (lines 127-133), every column on itself.
 // ---------------------------------------------------------------------------
 template <typename T>
+struct S_dyn_nxt1 {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, int spg_ts, int bt_fw, T *__restrict__ ua, T *__restrict__ va, T *__restrict__ un, T *__restrict__ vn, const double *__restrict__ ua_b, const double *__restrict__ va_b, const double *__restrict__ un_adv, const double *__restrict__ vn_adv, const double *__restrict__ un_b, const double *__restrict__ vn_b, const double *__restrict__ r1_hu, const double *__restrict__ r1_hv)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const Metrics<double> &m = d.md;
+        const uint32_t ij = IJ32(i, j);
+        const int jpk = d.jpk;
+        if (spg_ts && !bt_fw) {
+            const double cu = un_adv[ij] * r1_hu[ij], cv = vn_adv[ij] * r1_hv[ij];
+            for (int k = 1; k <= jpk - 1; k++) {
+                A3(un, i, j, k) = (T)(((double)A3(un, i, j, k) - cu + un_b[ij]) * UMASK(double, i, j, k));
+                A3(vn, i, j, k) = (T)(((double)A3(vn, i, j, k) - cv + vn_b[ij]) * VMASK(double, i, j, k));
+            }
+        }
+        if (!is_interior(d, i, j)) {
+            int si, sj;
+            if (lbc_src(d, 'U', i, j, si, sj)) return;   // written by the thread of (si,sj)
+            for (int k = 1; k <= jpk; k++) {
+                A3(ua, i, j, k) = T(0);
+                A3(va, i, j, k) = T(0);
+            }
+            return;
+        }
+        double zue = 0.0, zve = 0.0;
+        if (spg_ts) {
+            zue = E3(m.e3u, ij, 1) * (double)A3(ua, i, j, 1) * UMASK(double, i, j, 1);
+            zve = E3(m.e3v, ij, 1) * (double)A3(va, i, j, 1) * VMASK(double, i, j, 1);
+    #pragma unroll 4
+            for (int k = 2; k <= jpk - 1; k++) {
+                zue = zue + E3(m.e3u, ij, k) * (double)A3(ua, i, j, k) * UMASK(double, i, j, k);
+                zve = zve + E3(m.e3v, ij, k) * (double)A3(va, i, j, k) * VMASK(double, i, j, k);
+            }
+            zue = zue * r1_hu[ij];
+            zve = zve * r1_hv[ij];
+        }
+    #pragma unroll 4
+        for (int k = 1; k <= jpk; k++) {
+            T a = A3(ua, i, j, k), b = A3(va, i, j, k);
+            if (spg_ts && k < jpk) {
+                a = (T)(((double)a - zue + ua_b[ij]) * UMASK(double, i, j, k));
+                b = (T)(((double)b - zve + va_b[ij]) * VMASK(double, i, j, k));
+                A3(ua, i, j, k) = a;
+                A3(va, i, j, k) = b;
+            }
+            lbc_targets(d, i, j, [&](int ti, int tj) {
+                A3(ua, ti, tj, k) = a;
+                A3(va, ti, tj, k) = b;
+            });
+        }
+    }
+};
+template <typename T>
 KERNEL void k_dyn_nxt1(DevDomain d, int spg_ts, int bt_fw, T *__restrict__ ua, T *__restrict__ va,
                        T *__restrict__ un, T *__restrict__ vn, const double *__restrict__ ua_b,
                        const double *__restrict__ va_b, const double *__restrict__ un_adv,
                        const double *__restrict__ vn_adv, const double *__restrict__ un_b,
                        const double *__restrict__ vn_b, const double *__restrict__ r1_hu, const double *__restrict__ r1_hv)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const Metrics<double> &m = d.md;
-    const uint32_t ij = IJ32(i, j);
-    const int jpk = d.jpk;
-    if (spg_ts && !bt_fw) {
-        const double cu = un_adv[ij] * r1_hu[ij], cv = vn_adv[ij] * r1_hv[ij];
-        for (int k = 1; k <= jpk - 1; k++) {
-            A3(un, i, j, k) = (T)(((double)A3(un, i, j, k) - cu + un_b[ij]) * UMASK(double, i, j, k));
-            A3(vn, i, j, k) = (T)(((double)A3(vn, i, j, k) - cv + vn_b[ij]) * VMASK(double, i, j, k));
-        }
-    }
-    if (!is_interior(d, i, j)) {
-        int si, sj;
-        if (lbc_src(d, 'U', i, j, si, sj)) return;   // written by the thread of (si,sj)
-        for (int k = 1; k <= jpk; k++) {
-            A3(ua, i, j, k) = T(0);
-            A3(va, i, j, k) = T(0);
-        }
-        return;
-    }
-    double zue = 0.0, zve = 0.0;
-    if (spg_ts) {
-        zue = E3(m.e3u, ij, 1) * (double)A3(ua, i, j, 1) * UMASK(double, i, j, 1);
-        zve = E3(m.e3v, ij, 1) * (double)A3(va, i, j, 1) * VMASK(double, i, j, 1);
-#pragma unroll 4
-        for (int k = 2; k <= jpk - 1; k++) {
-            zue = zue + E3(m.e3u, ij, k) * (double)A3(ua, i, j, k) * UMASK(double, i, j, k);
-            zve = zve + E3(m.e3v, ij, k) * (double)A3(va, i, j, k) * VMASK(double, i, j, k);
-        }
-        zue = zue * r1_hu[ij];
-        zve = zve * r1_hv[ij];
-    }
-#pragma unroll 4
-    for (int k = 1; k <= jpk; k++) {
-        T a = A3(ua, i, j, k), b = A3(va, i, j, k);
-        if (spg_ts && k < jpk) {
-            a = (T)(((double)a - zue + ua_b[ij]) * UMASK(double, i, j, k));
-            b = (T)(((double)b - zve + va_b[ij]) * VMASK(double, i, j, k));
-            A3(ua, i, j, k) = a;
-            A3(va, i, j, k) = b;
-        }
-        lbc_targets(d, i, j, [&](int ti, int tj) {
-            A3(ua, ti, tj, k) = a;
-            A3(va, ti, tj, k) = b;
-        });
-    }
+    S_dyn_nxt1<T>::run(vb_hw(), d, spg_ts, bt_fw, ua, va, un, vn, ua_b, va_b, un_adv, vn_adv, un_b, vn_b, r1_hu, r1_hv);
 }
 
 // Part 2 (lines 147-176, 286-297, 299-347): Euler swap or Asselin filter and
@@ -279,64 +326,72 @@ KERNEL void k_dyn_nxt1(DevDomain d, int spg_ts, int bt_fw, T *__restrict__ ua, T
 // only itself.  The depth mean of part 1 multiplies by r1_hu once (zue *
 // r1_hu_a) exactly as lines 119-120 do, and the same holds for line 292.
 template <typename T>
+struct S_dyn_nxt2 {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, int euler, int bt_fix, double atfp, T *__restrict__ ub, T *__restrict__ vb, T *__restrict__ un, T *__restrict__ vn, const T *__restrict__ ua, const T *__restrict__ va, double *__restrict__ ub_b, double *__restrict__ vb_b, double *__restrict__ un_b, double *__restrict__ vn_b, const double *__restrict__ r1_hu, const double *__restrict__ r1_hv)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const Metrics<double> &m = d.md;
+        const uint32_t ij = IJ32(i, j);
+        const int jpkm1 = d.jpk - 1;
+        double sun = 0.0, svn = 0.0, sub = 0.0, svb = 0.0;   // depth sums of un, vn, ub, vb
+    #pragma unroll 2
+        for (int k = 1; k <= jpkm1; k++) {
+            const double a = (double)A3(ua, i, j, k), b = (double)A3(va, i, j, k);
+            const double e3u = E3(m.e3u, ij, k), e3v = E3(m.e3v, ij, k);
+            const double mu = UMASK(double, i, j, k), mv = VMASK(double, i, j, k);
+            double fu, fv;
+            if (euler) {
+                fu = (double)A3(ub, i, j, k);
+                fv = (double)A3(vb, i, j, k);
+            } else {
+                const double n1 = (double)A3(un, i, j, k), n2 = (double)A3(vn, i, j, k);
+                const T zuf = (T)(n1 + atfp * ((double)A3(ub, i, j, k) - 2.0 * n1 + a));
+                const T zvf = (T)(n2 + atfp * ((double)A3(vb, i, j, k) - 2.0 * n2 + b));
+                A3(ub, i, j, k) = zuf;
+                A3(vb, i, j, k) = zvf;
+                fu = (double)zuf;
+                fv = (double)zvf;
+            }
+            A3(un, i, j, k) = A3(ua, i, j, k);
+            A3(vn, i, j, k) = A3(va, i, j, k);
+            const double tu = e3u * (double)A3(ua, i, j, k) * mu, tv = e3v * (double)A3(va, i, j, k) * mv;
+            const double bu = e3u * fu * mu, bv = e3v * fv * mv;
+            sun = k == 1 ? tu : sun + tu;
+            svn = k == 1 ? tv : svn + tv;
+            sub = k == 1 ? bu : sub + bu;
+            svb = k == 1 ? bv : svb + bv;
+        }
+        if (!euler && bt_fix) {
+            // lines 286-297: the depth mean of ub/vb set to the input un_b/vn_b
+            const double cu = sub * r1_hu[ij] - un_b[ij], cv = svb * r1_hv[ij] - vn_b[ij];
+    #pragma unroll 4
+            for (int k = 1; k <= jpkm1; k++) {
+                const double e3u = E3(m.e3u, ij, k), e3v = E3(m.e3v, ij, k);
+                const double mu = UMASK(double, i, j, k), mv = VMASK(double, i, j, k);
+                const T nb = (T)((double)A3(ub, i, j, k) - cu * mu);
+                const T nv = (T)((double)A3(vb, i, j, k) - cv * mv);
+                A3(ub, i, j, k) = nb;
+                A3(vb, i, j, k) = nv;
+                const double bu = e3u * (double)nb * mu, bv = e3v * (double)nv * mv;
+                sub = k == 1 ? bu : sub + bu;
+                svb = k == 1 ? bv : svb + bv;
+            }
+        }
+        un_b[ij] = sun * r1_hu[ij];
+        vn_b[ij] = svn * r1_hv[ij];
+        ub_b[ij] = sub * r1_hu[ij];
+        vb_b[ij] = svb * r1_hv[ij];
+    }
+};
+template <typename T>
 KERNEL void k_dyn_nxt2(DevDomain d, int euler, int bt_fix, double atfp, T *__restrict__ ub, T *__restrict__ vb,
                        T *__restrict__ un, T *__restrict__ vn, const T *__restrict__ ua, const T *__restrict__ va,
                        double *__restrict__ ub_b, double *__restrict__ vb_b, double *__restrict__ un_b,
                        double *__restrict__ vn_b, const double *__restrict__ r1_hu, const double *__restrict__ r1_hv)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const Metrics<double> &m = d.md;
-    const uint32_t ij = IJ32(i, j);
-    const int jpkm1 = d.jpk - 1;
-    double sun = 0.0, svn = 0.0, sub = 0.0, svb = 0.0;   // depth sums of un, vn, ub, vb
-#pragma unroll 2
-    for (int k = 1; k <= jpkm1; k++) {
-        const double a = (double)A3(ua, i, j, k), b = (double)A3(va, i, j, k);
-        const double e3u = E3(m.e3u, ij, k), e3v = E3(m.e3v, ij, k);
-        const double mu = UMASK(double, i, j, k), mv = VMASK(double, i, j, k);
-        double fu, fv;
-        if (euler) {
-            fu = (double)A3(ub, i, j, k);
-            fv = (double)A3(vb, i, j, k);
-        } else {
-            const double n1 = (double)A3(un, i, j, k), n2 = (double)A3(vn, i, j, k);
-            const T zuf = (T)(n1 + atfp * ((double)A3(ub, i, j, k) - 2.0 * n1 + a));
-            const T zvf = (T)(n2 + atfp * ((double)A3(vb, i, j, k) - 2.0 * n2 + b));
-            A3(ub, i, j, k) = zuf;
-            A3(vb, i, j, k) = zvf;
-            fu = (double)zuf;
-            fv = (double)zvf;
-        }
-        A3(un, i, j, k) = A3(ua, i, j, k);
-        A3(vn, i, j, k) = A3(va, i, j, k);
-        const double tu = e3u * (double)A3(ua, i, j, k) * mu, tv = e3v * (double)A3(va, i, j, k) * mv;
-        const double bu = e3u * fu * mu, bv = e3v * fv * mv;
-        sun = k == 1 ? tu : sun + tu;
-        svn = k == 1 ? tv : svn + tv;
-        sub = k == 1 ? bu : sub + bu;
-        svb = k == 1 ? bv : svb + bv;
-    }
-    if (!euler && bt_fix) {
-        // lines 286-297: the depth mean of ub/vb set to the input un_b/vn_b
-        const double cu = sub * r1_hu[ij] - un_b[ij], cv = svb * r1_hv[ij] - vn_b[ij];
-#pragma unroll 4
-        for (int k = 1; k <= jpkm1; k++) {
-            const double e3u = E3(m.e3u, ij, k), e3v = E3(m.e3v, ij, k);
-            const double mu = UMASK(double, i, j, k), mv = VMASK(double, i, j, k);
-            const T nb = (T)((double)A3(ub, i, j, k) - cu * mu);
-            const T nv = (T)((double)A3(vb, i, j, k) - cv * mv);
-            A3(ub, i, j, k) = nb;
-            A3(vb, i, j, k) = nv;
-            const double bu = e3u * (double)nb * mu, bv = e3v * (double)nv * mv;
-            sub = k == 1 ? bu : sub + bu;
-            svb = k == 1 ? bv : svb + bv;
-        }
-    }
-    un_b[ij] = sun * r1_hu[ij];
-    vn_b[ij] = svn * r1_hv[ij];
-    ub_b[ij] = sub * r1_hu[ij];
-    vb_b[ij] = svb * r1_hv[ij];
+    S_dyn_nxt2<T>::run(vb_hw(), d, euler, bt_fix, atfp, ub, vb, un, vn, ua, va, ub_b, vb_b, un_b, vn_b, r1_hu, r1_hv);
 }
 
 // ---------------------------------------------------------------------------
@@ -348,7 +403,7 @@ int launch_div_hor(const T *un, const T *vn, T *hdivn)
     const DevDomain &d = nh::dom();
     Grid2 g = grid2(d.jpi, d.jpj);
     g.grid.z = (unsigned)((d.jpk + NXT_KB - 1) / NXT_KB);
-    LAUNCH("div_hor", k_div_hor<T>, g, d, un, vn, hdivn);
+    LAUNCHB("div_hor", k_div_hor<T>, (S_div_hor<T>), g, d, un, vn, hdivn);
     return 0;
 }
 
@@ -358,7 +413,7 @@ int launch_ssh_nxt(double z2dt, double zcoef, const T *un, const T *vn, T *hdivn
 {
     const DevDomain &d = nh::dom();
     Grid2 g = grid2(d.jpi, d.jpj);
-    LAUNCH("ssh_nxt", k_ssh_nxt<T>, g, d, z2dt, zcoef, un, vn, hdivn, sshb, emp_b, emp, ssha);
+    LAUNCHB("ssh_nxt", k_ssh_nxt<T>, (S_ssh_nxt<T>), g, d, z2dt, zcoef, un, vn, hdivn, sshb, emp_b, emp, ssha);
     return 0;
 }
 
@@ -367,7 +422,7 @@ int launch_wzv(int first, double z1_2dt, const T *hdivn, T *wn)
 {
     const DevDomain &d = nh::dom();
     Grid2 g = grid2(d.jpi, d.jpj);
-    LAUNCH("wzv", k_wzv<T>, g, d, first, z1_2dt, hdivn, wn);
+    LAUNCHB("wzv", k_wzv<T>, (S_wzv<T>), g, d, first, z1_2dt, hdivn, wn);
     return 0;
 }
 
@@ -375,7 +430,7 @@ int launch_ssh_swp(int euler, double *sshb, double *sshn, const double *ssha)
 {
     const DevDomain &d = nh::dom();
     Grid2 g = grid2(d.jpi, d.jpj);
-    LAUNCH("ssh_swp", k_ssh_swp, g, d, euler, d.rn_atfp, sshb, sshn, ssha);
+    LAUNCHB("ssh_swp", k_ssh_swp, (S_ssh_swp), g, d, euler, d.rn_atfp, sshb, sshn, ssha);
     return 0;
 }
 
@@ -385,7 +440,7 @@ int launch_tra_nxt(int euler, T *tsb, T *tsn, T *tsa)
     const DevDomain &d = nh::dom();
     Grid2 g = grid2(d.jpi, d.jpj);
     g.grid.z = (unsigned)(2 * d.jpk);
-    LAUNCH("tra_nxt", k_tra_nxt<T>, g, d, euler, d.rn_atfp, tsb, tsn, tsa);
+    LAUNCHB("tra_nxt", k_tra_nxt<T>, (S_tra_nxt<T>), g, d, euler, d.rn_atfp, tsb, tsn, tsa);
     return 0;
 }
 
@@ -396,9 +451,9 @@ int launch_dyn_nxt(int euler, T *ub, T *vb, T *un, T *vn, T *ua, T *va, const do
 {
     const DevDomain &d = nh::dom();
     Grid2 g = grid2(d.jpi, d.jpj);
-    LAUNCH("dyn_nxt", k_dyn_nxt1<T>, g, d, d.ln_dynspg_ts, d.ln_bt_fw, ua, va, un, vn, ua_b, va_b, un_adv, vn_adv,
+    LAUNCHB("dyn_nxt", k_dyn_nxt1<T>, (S_dyn_nxt1<T>), g, d, d.ln_dynspg_ts, d.ln_bt_fw, ua, va, un, vn, ua_b, va_b, un_adv, vn_adv,
            (const double *)un_b, (const double *)vn_b, r1_hu, r1_hv);
-    LAUNCH("dyn_nxt", k_dyn_nxt2<T>, g, d, euler, d.ln_dynspg_ts && d.ln_bt_fw, d.rn_atfp, ub, vb, un, vn,
+    LAUNCHB("dyn_nxt", k_dyn_nxt2<T>, (S_dyn_nxt2<T>), g, d, euler, d.ln_dynspg_ts && d.ln_bt_fw, d.rn_atfp, ub, vb, un, vn,
            (const T *)ua, (const T *)va, ub_b, vb_b, un_b, vn_b, r1_hu, r1_hv);
     return 0;
 }

@@ -95,95 +95,116 @@ __device__ __forceinline__ void bt_coriolis(const DevDomain &d, int i, int j, co
 // Coriolis pre-computations, lines 241-342 (recomputed every call: with a
 // linear free surface they are constant, so this equals the kt == nit000 set)
 // ---------------------------------------------------------------------------
+struct S_spg_cor1 {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, const double * __restrict__ ht_n, SpgWork w)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const Metrics<double> &m = d.md;
+        const size_t ij = IJ(i, j);
+        switch (d.nvor_scheme) {
+        case NEMO_NP_EEN:
+            if (i <= d.jpi - 1 && j <= d.jpj - 1) {
+                double z;
+                const double s = A2(ht_n, i, j + 1) + A2(ht_n, i + 1, j + 1) + A2(ht_n, i, j) + A2(ht_n, i + 1, j);
+                if (d.nn_een_e3f == 0) {
+                    z = s * 0.25;
+                } else {
+                    z = s / fmax2(1.0, A2(d.ssmask, i, j + 1) + A2(d.ssmask, i + 1, j + 1) + A2(d.ssmask, i, j) +
+                                           A2(d.ssmask, i + 1, j));
+                }
+                if (z != 0.0) z = m.ff_f[ij] / z;
+                w.zhf[ij] = z;      // zwz before lbc_lnk
+            }
+            break;
+        case NEMO_NP_ENE:
+        case NEMO_NP_ENS:
+        case NEMO_NP_MIX:
+            if (j <= d.jpj - 1) {
+                // zhf = 0 * (1 - umask*umask) + sum_k e3f * umask(j) * umask(j+1)
+                double z = 0.0;
+                z = z * (1.0 - UMASK(double, i, j, 1) * UMASK(double, i, j + 1, 1));
+                for (int k = 1; k <= d.jpk - 1; k++)
+                    z = z + E3(m.e3f, ij, k) * UMASK(double, i, j, k) * UMASK(double, i, j + 1, k);
+                w.zhf[ij] = z;
+            }
+            break;
+        default: break;
+        }
+    }
+};
 KERNEL void k_spg_cor1(DevDomain d, const double * __restrict__ ht_n, SpgWork w)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const Metrics<double> &m = d.md;
-    const size_t ij = IJ(i, j);
-    switch (d.nvor_scheme) {
-    case NEMO_NP_EEN:
-        if (i <= d.jpi - 1 && j <= d.jpj - 1) {
-            double z;
-            const double s = A2(ht_n, i, j + 1) + A2(ht_n, i + 1, j + 1) + A2(ht_n, i, j) + A2(ht_n, i + 1, j);
-            if (d.nn_een_e3f == 0) {
-                z = s * 0.25;
-            } else {
-                z = s / fmax2(1.0, A2(d.ssmask, i, j + 1) + A2(d.ssmask, i + 1, j + 1) + A2(d.ssmask, i, j) +
-                                       A2(d.ssmask, i + 1, j));
-            }
-            if (z != 0.0) z = m.ff_f[ij] / z;
-            w.zhf[ij] = z;      // zwz before lbc_lnk
-        }
-        break;
-    case NEMO_NP_ENE:
-    case NEMO_NP_ENS:
-    case NEMO_NP_MIX:
-        if (j <= d.jpj - 1) {
-            // zhf = 0 * (1 - umask*umask) + sum_k e3f * umask(j) * umask(j+1)
-            double z = 0.0;
-            z = z * (1.0 - UMASK(double, i, j, 1) * UMASK(double, i, j + 1, 1));
-            for (int k = 1; k <= d.jpk - 1; k++)
-                z = z + E3(m.e3f, ij, k) * UMASK(double, i, j, k) * UMASK(double, i, j + 1, k);
-            w.zhf[ij] = z;
-        }
-        break;
-    default: break;
-    }
+    S_spg_cor1::run(vb_hw(), d, ht_n, w);
 }
 
+struct S_spg_cor2 {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, const double * __restrict__ ht_n, SpgWork w)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const Metrics<double> &m = d.md;
+        const size_t ij = IJ(i, j);
+        int si, sj;
+        switch (d.nvor_scheme) {
+        case NEMO_NP_ENE:
+        case NEMO_NP_ENS:
+        case NEMO_NP_MIX: {
+            double zhf = lbc_src(d, 'F', i, j, si, sj) ? A2(w.zhf, si, sj) : 0.0;
+            double z = 0.0;
+            if (zhf != 0.0) z = 1.0 / zhf;
+            w.zwz[ij] = m.ff_f[ij] * z;
+            break;
+        }
+        case NEMO_NP_EEN:
+            w.zwz[ij] = lbc_src(d, 'F', i, j, si, sj) ? A2(w.zhf, si, sj) : 0.0;
+            break;
+        case NEMO_NP_EET:
+            if (i >= 2 && j >= 2) {
+                const double ss = d.ssmask[ij];
+                const double z1_ht = ss / (A2(ht_n, i, j) + 1.0 - ss);
+                const double *f = m.ff_f;
+                A2(w.ftne, i, j) = (A2(f, i - 1, j) + A2(f, i, j) + A2(f, i, j - 1)) * z1_ht;
+                A2(w.ftnw, i, j) = (A2(f, i - 1, j - 1) + A2(f, i - 1, j) + A2(f, i, j)) * z1_ht;
+                A2(w.ftse, i, j) = (A2(f, i, j) + A2(f, i, j - 1) + A2(f, i - 1, j - 1)) * z1_ht;
+                A2(w.ftsw, i, j) = (A2(f, i, j - 1) + A2(f, i - 1, j - 1) + A2(f, i - 1, j)) * z1_ht;
+            } else if (i == 1) {
+                A2(w.ftne, i, j) = A2(w.ftnw, i, j) = A2(w.ftse, i, j) = A2(w.ftsw, i, j) = 0.0;
+            }
+            break;
+        default: break;
+        }
+    }
+};
 KERNEL void k_spg_cor2(DevDomain d, const double * __restrict__ ht_n, SpgWork w)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const Metrics<double> &m = d.md;
-    const size_t ij = IJ(i, j);
-    int si, sj;
-    switch (d.nvor_scheme) {
-    case NEMO_NP_ENE:
-    case NEMO_NP_ENS:
-    case NEMO_NP_MIX: {
-        double zhf = lbc_src(d, 'F', i, j, si, sj) ? A2(w.zhf, si, sj) : 0.0;
-        double z = 0.0;
-        if (zhf != 0.0) z = 1.0 / zhf;
-        w.zwz[ij] = m.ff_f[ij] * z;
-        break;
-    }
-    case NEMO_NP_EEN:
-        w.zwz[ij] = lbc_src(d, 'F', i, j, si, sj) ? A2(w.zhf, si, sj) : 0.0;
-        break;
-    case NEMO_NP_EET:
-        if (i >= 2 && j >= 2) {
-            const double ss = d.ssmask[ij];
-            const double z1_ht = ss / (A2(ht_n, i, j) + 1.0 - ss);
-            const double *f = m.ff_f;
-            A2(w.ftne, i, j) = (A2(f, i - 1, j) + A2(f, i, j) + A2(f, i, j - 1)) * z1_ht;
-            A2(w.ftnw, i, j) = (A2(f, i - 1, j - 1) + A2(f, i - 1, j) + A2(f, i, j)) * z1_ht;
-            A2(w.ftse, i, j) = (A2(f, i, j) + A2(f, i, j - 1) + A2(f, i - 1, j - 1)) * z1_ht;
-            A2(w.ftsw, i, j) = (A2(f, i, j - 1) + A2(f, i - 1, j - 1) + A2(f, i - 1, j)) * z1_ht;
-        } else if (i == 1) {
-            A2(w.ftne, i, j) = A2(w.ftnw, i, j) = A2(w.ftse, i, j) = A2(w.ftsw, i, j) = 0.0;
-        }
-        break;
-    default: break;
-    }
+    S_spg_cor2::run(vb_hw(), d, ht_n, w);
 }
 
+struct S_spg_cor3 {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, SpgWork w)
+    {
+        // EEN triads from the lbc-filled zwz, lines 267-275
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        if (i == 1) {
+            A2(w.ftne, i, j) = A2(w.ftnw, i, j) = A2(w.ftse, i, j) = A2(w.ftsw, i, j) = 0.0;
+            return;
+        }
+        if (j < 2) return;
+        const double *z = w.zwz;
+        A2(w.ftne, i, j) = A2(z, i - 1, j) + A2(z, i, j) + A2(z, i, j - 1);
+        A2(w.ftnw, i, j) = A2(z, i - 1, j - 1) + A2(z, i - 1, j) + A2(z, i, j);
+        A2(w.ftse, i, j) = A2(z, i, j) + A2(z, i, j - 1) + A2(z, i - 1, j - 1);
+        A2(w.ftsw, i, j) = A2(z, i, j - 1) + A2(z, i - 1, j - 1) + A2(z, i - 1, j);
+    }
+};
 KERNEL void k_spg_cor3(DevDomain d, SpgWork w)
 {
-    // EEN triads from the lbc-filled zwz, lines 267-275
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    if (i == 1) {
-        A2(w.ftne, i, j) = A2(w.ftnw, i, j) = A2(w.ftse, i, j) = A2(w.ftsw, i, j) = 0.0;
-        return;
-    }
-    if (j < 2) return;
-    const double *z = w.zwz;
-    A2(w.ftne, i, j) = A2(z, i - 1, j) + A2(z, i, j) + A2(z, i, j - 1);
-    A2(w.ftnw, i, j) = A2(z, i - 1, j - 1) + A2(z, i - 1, j) + A2(z, i, j);
-    A2(w.ftse, i, j) = A2(z, i, j) + A2(z, i, j - 1) + A2(z, i - 1, j - 1);
-    A2(w.ftsw, i, j) = A2(z, i, j - 1) + A2(z, i - 1, j - 1) + A2(z, i - 1, j);
+    S_spg_cor3::run(vb_hw(), d, w);
 }
 
 // ---------------------------------------------------------------------------
@@ -191,137 +212,159 @@ KERNEL void k_spg_cor3(DevDomain d, SpgWork w)
 // bottom baroclinic velocities (lines 519-527)
 // ---------------------------------------------------------------------------
 template <typename T>
+struct S_spg_p1a {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, T * __restrict__ ua, T * __restrict__ va, const T * __restrict__ un, const T * __restrict__ vn, const double * __restrict__ r1_hu_n, const double * __restrict__ r1_hv_n, const double * __restrict__ un_b, const double * __restrict__ vn_b, SpgWork w)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const Metrics<T> &m = met<T>(d);
+        const size_t ij = IJ(i, j);
+        const int jpk = d.jpk;
+        double su = 0.0, sv = 0.0;
+        for (int k = 1; k <= jpk - 1; k++) {
+            const size_t p = IJK(i, j, k);
+            su = su + (double)E3(m.e3u, ij, k) * (double)ua[p] * UMASK(double, i, j, k);
+            sv = sv + (double)E3(m.e3v, ij, k) * (double)va[p] * VMASK(double, i, j, k);
+        }
+        su = su * r1_hu_n[ij];
+        sv = sv * r1_hv_n[ij];
+        w.zu_frc[ij] = su;
+        w.zv_frc[ij] = sv;
+        if (!is_interior(d, i, j)) return;
+        for (int k = 1; k <= jpk - 1; k++) {
+            const size_t p = IJK(i, j, k);
+            ua[p] = (T)((double)ua[p] - su * UMASK(double, i, j, k));
+            va[p] = (T)((double)va[p] - sv * VMASK(double, i, j, k));
+        }
+        w.zwxb[ij] = (double)A3(un, i, j, d.mbku[ij]) - un_b[ij];
+        w.zwyb[ij] = (double)A3(vn, i, j, d.mbkv[ij]) - vn_b[ij];
+    }
+};
+template <typename T>
 KERNEL void k_spg_p1a(DevDomain d, T * __restrict__ ua, T * __restrict__ va, const T * __restrict__ un, const T * __restrict__ vn, const double * __restrict__ r1_hu_n,
                           const double * __restrict__ r1_hv_n, const double * __restrict__ un_b, const double * __restrict__ vn_b, SpgWork w)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const Metrics<T> &m = met<T>(d);
-    const size_t ij = IJ(i, j);
-    const int jpk = d.jpk;
-    double su = 0.0, sv = 0.0;
-    for (int k = 1; k <= jpk - 1; k++) {
-        const size_t p = IJK(i, j, k);
-        su = su + (double)E3(m.e3u, ij, k) * (double)ua[p] * UMASK(double, i, j, k);
-        sv = sv + (double)E3(m.e3v, ij, k) * (double)va[p] * VMASK(double, i, j, k);
-    }
-    su = su * r1_hu_n[ij];
-    sv = sv * r1_hv_n[ij];
-    w.zu_frc[ij] = su;
-    w.zv_frc[ij] = sv;
-    if (!is_interior(d, i, j)) return;
-    for (int k = 1; k <= jpk - 1; k++) {
-        const size_t p = IJK(i, j, k);
-        ua[p] = (T)((double)ua[p] - su * UMASK(double, i, j, k));
-        va[p] = (T)((double)va[p] - sv * VMASK(double, i, j, k));
-    }
-    w.zwxb[ij] = (double)A3(un, i, j, d.mbku[ij]) - un_b[ij];
-    w.zwyb[ij] = (double)A3(vn, i, j, d.mbkv[ij]) - vn_b[ij];
+    S_spg_p1a<T>::run(vb_hw(), d, ua, va, un, vn, r1_hu_n, r1_hv_n, un_b, vn_b, w);
 }
 
 // Phase 1b: barotropic forcing (lines 386-639)
+struct S_spg_p1b {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, const double * __restrict__ un_b, const double * __restrict__ vn_b, const double * __restrict__ hu_n, const double * __restrict__ hv_n, const double * __restrict__ r1_hu_n, const double * __restrict__ r1_hv_n, const double * __restrict__ ht_n, const double * __restrict__ rCdU_bot, const double * __restrict__ utau, const double * __restrict__ vtau, const double * __restrict__ emp, const double * __restrict__ rnf, SpgWork w)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const Metrics<double> &m = d.md;
+        const size_t ij = IJ(i, j);
+        const double r1_rau0 = 1.0 / d.rau0, r1_2 = 0.5;
+        // zssh_frc = r1_rau0 * ( emp - rnf + fwfisf ), fwfisf = 0 without cavities
+        w.zssh_frc[ij] = r1_rau0 * (emp[ij] - (rnf ? rnf[ij] : 0.0) + 0.0);
+        if (!is_interior(d, i, j)) return;
+        w.zCdU_u[ij] = r1_2 * (A2(rCdU_bot, i + 1, j) + A2(rCdU_bot, i, j));
+        w.zCdU_v[ij] = r1_2 * (A2(rCdU_bot, i, j + 1) + A2(rCdU_bot, i, j));
+        // now fluxes zwx = un_b * hu_n * e2u at the 6 points the scheme reads
+        double zwx_[2][3], zwy_[3][2];   // zwx(i-1..i, j..j+1) / zwy(i..i+1, j-1..j)
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++) {
+                const int p = i - 1 + a, q = j + b;
+                zwx_[a][b] = A2(un_b, p, q) * A2(hu_n, p, q) * m.e2u[IJ(p, q)];
+                const int p2 = i + a, q2 = j - 1 + b;
+                zwy_[a][b] = A2(vn_b, p2, q2) * A2(hv_n, p2, q2) * m.e1v[IJ(p2, q2)];
+            }
+        double zu_trd, zv_trd;
+        {
+            const double *zwz = w.zwz;
+            switch (d.nvor_scheme) {
+            case NEMO_NP_ENT: {
+                const double r1_4 = 0.25;
+                zu_trd = r1_4 * m.r1_e1e2u[ij] * r1_hu_n[ij] *
+                         (m.e1e2t[IJ(i + 1, j)] * A2(ht_n, i + 1, j) * m.ff_t[IJ(i + 1, j)] * (A2(vn_b, i + 1, j) + A2(vn_b, i + 1, j - 1)) +
+                          m.e1e2t[ij] * A2(ht_n, i, j) * m.ff_t[ij] * (A2(vn_b, i, j) + A2(vn_b, i, j - 1)));
+                zv_trd = -(r1_4 * m.r1_e1e2v[ij] * r1_hv_n[ij] *
+                           (m.e1e2t[IJ(i, j + 1)] * A2(ht_n, i, j + 1) * m.ff_t[IJ(i, j + 1)] * (A2(un_b, i, j + 1) + A2(un_b, i - 1, j + 1)) +
+                            m.e1e2t[ij] * A2(ht_n, i, j) * m.ff_t[ij] * (A2(un_b, i, j) + A2(un_b, i - 1, j))));
+                break;
+            }
+            case NEMO_NP_ENE:
+            case NEMO_NP_MIX: {
+                const double zy1 = (zwy_[0][0] + zwy_[1][0]) * m.r1_e1u[ij];
+                const double zy2 = (zwy_[0][1] + zwy_[1][1]) * m.r1_e1u[ij];
+                const double zx1 = (zwx_[0][0] + zwx_[0][1]) * m.r1_e2v[ij];
+                const double zx2 = (zwx_[1][0] + zwx_[1][1]) * m.r1_e2v[ij];
+                zu_trd = 0.25 * (A2(zwz, i, j - 1) * zy1 + A2(zwz, i, j) * zy2);
+                zv_trd = -(0.25 * (A2(zwz, i - 1, j) * zx1 + A2(zwz, i, j) * zx2));
+                break;
+            }
+            case NEMO_NP_ENS: {
+                const double zy1 = 0.125 * (zwy_[0][0] + zwy_[1][0] + zwy_[0][1] + zwy_[1][1]) * m.r1_e1u[ij];
+                const double zx1 = -(0.125 * (zwx_[0][0] + zwx_[0][1] + zwx_[1][0] + zwx_[1][1]) * m.r1_e2v[ij]);
+                zu_trd = zy1 * (A2(zwz, i, j - 1) + A2(zwz, i, j));
+                zv_trd = zx1 * (A2(zwz, i - 1, j) + A2(zwz, i, j));
+                break;
+            }
+            default: {
+                const double r1_12 = 1.0 / 12.0;
+                zu_trd = r1_12 * m.r1_e1u[ij] *
+                         (A2(w.ftne, i, j) * zwy_[0][1] + A2(w.ftnw, i + 1, j) * zwy_[1][1] + A2(w.ftse, i, j) * zwy_[0][0] +
+                          A2(w.ftsw, i + 1, j) * zwy_[1][0]);
+                zv_trd = -(r1_12 * m.r1_e2v[ij] *
+                           (A2(w.ftsw, i, j + 1) * zwx_[0][1] + A2(w.ftse, i, j + 1) * zwx_[1][1] +
+                            A2(w.ftnw, i, j) * zwx_[0][0] + A2(w.ftne, i, j) * zwx_[1][0]));
+                break;
+            }
+            }
+        }
+        double zu = w.zu_frc[ij], zv = w.zv_frc[ij];
+        zu = zu - zu_trd * d.ssumask[ij];
+        zv = zv - zv_trd * d.ssvmask[ij];
+        zu = zu + r1_hu_n[ij] * r1_2 * (A2(rCdU_bot, i + 1, j) + A2(rCdU_bot, i, j)) * w.zwxb[ij];
+        zv = zv + r1_hv_n[ij] * r1_2 * (A2(rCdU_bot, i, j + 1) + A2(rCdU_bot, i, j)) * w.zwyb[ij];
+        zu = zu + r1_rau0 * utau[ij] * r1_hu_n[ij];
+        zv = zv + r1_rau0 * vtau[ij] * r1_hv_n[ij];
+        w.zu_frc[ij] = zu;
+        w.zv_frc[ij] = zv;
+    }
+};
 KERNEL void k_spg_p1b(DevDomain d, const double * __restrict__ un_b, const double * __restrict__ vn_b, const double * __restrict__ hu_n,
                           const double * __restrict__ hv_n, const double * __restrict__ r1_hu_n, const double * __restrict__ r1_hv_n, const double * __restrict__ ht_n,
                           const double * __restrict__ rCdU_bot, const double * __restrict__ utau, const double * __restrict__ vtau, const double * __restrict__ emp,
                           const double * __restrict__ rnf, SpgWork w)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const Metrics<double> &m = d.md;
-    const size_t ij = IJ(i, j);
-    const double r1_rau0 = 1.0 / d.rau0, r1_2 = 0.5;
-    // zssh_frc = r1_rau0 * ( emp - rnf + fwfisf ), fwfisf = 0 without cavities
-    w.zssh_frc[ij] = r1_rau0 * (emp[ij] - (rnf ? rnf[ij] : 0.0) + 0.0);
-    if (!is_interior(d, i, j)) return;
-    w.zCdU_u[ij] = r1_2 * (A2(rCdU_bot, i + 1, j) + A2(rCdU_bot, i, j));
-    w.zCdU_v[ij] = r1_2 * (A2(rCdU_bot, i, j + 1) + A2(rCdU_bot, i, j));
-    // now fluxes zwx = un_b * hu_n * e2u at the 6 points the scheme reads
-    double zwx_[2][3], zwy_[3][2];   // zwx(i-1..i, j..j+1) / zwy(i..i+1, j-1..j)
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) {
-            const int p = i - 1 + a, q = j + b;
-            zwx_[a][b] = A2(un_b, p, q) * A2(hu_n, p, q) * m.e2u[IJ(p, q)];
-            const int p2 = i + a, q2 = j - 1 + b;
-            zwy_[a][b] = A2(vn_b, p2, q2) * A2(hv_n, p2, q2) * m.e1v[IJ(p2, q2)];
-        }
-    double zu_trd, zv_trd;
-    {
-        const double *zwz = w.zwz;
-        switch (d.nvor_scheme) {
-        case NEMO_NP_ENT: {
-            const double r1_4 = 0.25;
-            zu_trd = r1_4 * m.r1_e1e2u[ij] * r1_hu_n[ij] *
-                     (m.e1e2t[IJ(i + 1, j)] * A2(ht_n, i + 1, j) * m.ff_t[IJ(i + 1, j)] * (A2(vn_b, i + 1, j) + A2(vn_b, i + 1, j - 1)) +
-                      m.e1e2t[ij] * A2(ht_n, i, j) * m.ff_t[ij] * (A2(vn_b, i, j) + A2(vn_b, i, j - 1)));
-            zv_trd = -(r1_4 * m.r1_e1e2v[ij] * r1_hv_n[ij] *
-                       (m.e1e2t[IJ(i, j + 1)] * A2(ht_n, i, j + 1) * m.ff_t[IJ(i, j + 1)] * (A2(un_b, i, j + 1) + A2(un_b, i - 1, j + 1)) +
-                        m.e1e2t[ij] * A2(ht_n, i, j) * m.ff_t[ij] * (A2(un_b, i, j) + A2(un_b, i - 1, j))));
-            break;
-        }
-        case NEMO_NP_ENE:
-        case NEMO_NP_MIX: {
-            const double zy1 = (zwy_[0][0] + zwy_[1][0]) * m.r1_e1u[ij];
-            const double zy2 = (zwy_[0][1] + zwy_[1][1]) * m.r1_e1u[ij];
-            const double zx1 = (zwx_[0][0] + zwx_[0][1]) * m.r1_e2v[ij];
-            const double zx2 = (zwx_[1][0] + zwx_[1][1]) * m.r1_e2v[ij];
-            zu_trd = 0.25 * (A2(zwz, i, j - 1) * zy1 + A2(zwz, i, j) * zy2);
-            zv_trd = -(0.25 * (A2(zwz, i - 1, j) * zx1 + A2(zwz, i, j) * zx2));
-            break;
-        }
-        case NEMO_NP_ENS: {
-            const double zy1 = 0.125 * (zwy_[0][0] + zwy_[1][0] + zwy_[0][1] + zwy_[1][1]) * m.r1_e1u[ij];
-            const double zx1 = -(0.125 * (zwx_[0][0] + zwx_[0][1] + zwx_[1][0] + zwx_[1][1]) * m.r1_e2v[ij]);
-            zu_trd = zy1 * (A2(zwz, i, j - 1) + A2(zwz, i, j));
-            zv_trd = zx1 * (A2(zwz, i - 1, j) + A2(zwz, i, j));
-            break;
-        }
-        default: {
-            const double r1_12 = 1.0 / 12.0;
-            zu_trd = r1_12 * m.r1_e1u[ij] *
-                     (A2(w.ftne, i, j) * zwy_[0][1] + A2(w.ftnw, i + 1, j) * zwy_[1][1] + A2(w.ftse, i, j) * zwy_[0][0] +
-                      A2(w.ftsw, i + 1, j) * zwy_[1][0]);
-            zv_trd = -(r1_12 * m.r1_e2v[ij] *
-                       (A2(w.ftsw, i, j + 1) * zwx_[0][1] + A2(w.ftse, i, j + 1) * zwx_[1][1] +
-                        A2(w.ftnw, i, j) * zwx_[0][0] + A2(w.ftne, i, j) * zwx_[1][0]));
-            break;
-        }
-        }
-    }
-    double zu = w.zu_frc[ij], zv = w.zv_frc[ij];
-    zu = zu - zu_trd * d.ssumask[ij];
-    zv = zv - zv_trd * d.ssvmask[ij];
-    zu = zu + r1_hu_n[ij] * r1_2 * (A2(rCdU_bot, i + 1, j) + A2(rCdU_bot, i, j)) * w.zwxb[ij];
-    zv = zv + r1_hv_n[ij] * r1_2 * (A2(rCdU_bot, i, j + 1) + A2(rCdU_bot, i, j)) * w.zwyb[ij];
-    zu = zu + r1_rau0 * utau[ij] * r1_hu_n[ij];
-    zv = zv + r1_rau0 * vtau[ij] * r1_hv_n[ij];
-    w.zu_frc[ij] = zu;
-    w.zv_frc[ij] = zv;
+    S_spg_p1b::run(vb_hw(), d, un_b, vn_b, hu_n, hv_n, r1_hu_n, r1_hv_n, ht_n, rCdU_bot, utau, vtau, emp, rnf, w);
 }
 
 // ---------------------------------------------------------------------------
 // Phase 2 sub-steps
 // ---------------------------------------------------------------------------
+struct S_spg_s1 {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, double za1, double za2, double za3, double wgt2, const double * __restrict__ un_e, const double * __restrict__ ub_e, const double * __restrict__ ubb_e, const double * __restrict__ vn_e, const double * __restrict__ vb_e, const double * __restrict__ vbb_e, const double * __restrict__ hu_n, const double * __restrict__ hv_n, double * __restrict__ un_adv, double * __restrict__ vn_adv, SpgWork w)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const Metrics<double> &m = d.md;
+        const size_t ij = IJ(i, j);
+        const double ue = za1 * un_e[ij] + za2 * ub_e[ij] + za3 * ubb_e[ij];
+        const double ve = za1 * vn_e[ij] + za2 * vb_e[ij] + za3 * vbb_e[ij];
+        if (d.nvor_scheme == NEMO_NP_ENT) {   // only the t-point Coriolis reads them
+            w.ua_e[ij] = ue;
+            w.va_e[ij] = ve;
+        }
+        const double zwx = m.e2u[ij] * ue * hu_n[ij];
+        const double zwy = m.e1v[ij] * ve * hv_n[ij];
+        w.zwx[ij] = zwx;
+        w.zwy[ij] = zwy;
+        un_adv[ij] = un_adv[ij] + wgt2 * zwx * m.r1_e2u[ij];
+        vn_adv[ij] = vn_adv[ij] + wgt2 * zwy * m.r1_e1v[ij];
+    }
+};
 KERNEL void k_spg_s1(DevDomain d, double za1, double za2, double za3, double wgt2, const double * __restrict__ un_e,
                          const double * __restrict__ ub_e, const double * __restrict__ ubb_e, const double * __restrict__ vn_e, const double * __restrict__ vb_e,
                          const double * __restrict__ vbb_e, const double * __restrict__ hu_n, const double * __restrict__ hv_n, double * __restrict__ un_adv, double * __restrict__ vn_adv,
                          SpgWork w)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const Metrics<double> &m = d.md;
-    const size_t ij = IJ(i, j);
-    const double ue = za1 * un_e[ij] + za2 * ub_e[ij] + za3 * ubb_e[ij];
-    const double ve = za1 * vn_e[ij] + za2 * vb_e[ij] + za3 * vbb_e[ij];
-    if (d.nvor_scheme == NEMO_NP_ENT) {   // only the t-point Coriolis reads them
-        w.ua_e[ij] = ue;
-        w.va_e[ij] = ve;
-    }
-    const double zwx = m.e2u[ij] * ue * hu_n[ij];
-    const double zwy = m.e1v[ij] * ve * hv_n[ij];
-    w.zwx[ij] = zwx;
-    w.zwy[ij] = zwy;
-    un_adv[ij] = un_adv[ij] + wgt2 * zwx * m.r1_e2u[ij];
-    vn_adv[ij] = vn_adv[ij] + wgt2 * zwy * m.r1_e1v[ij];
+    S_spg_s1::run(vb_hw(), d, za1, za2, za3, wgt2, un_e, ub_e, ubb_e, vn_e, vb_e, vbb_e, hu_n, hv_n, un_adv, vn_adv, w);
 }
 
 // the sub-step's sea-level levels and the AB3-AM4 interpolation weights
@@ -390,12 +433,19 @@ __device__ __forceinline__ void spg_s2_point(const DevDomain &d, double rdtbt, d
     if (wgt1 != 0.0) ssha[ij] = ssha[ij] + wgt1 * ssha_e;
     w.zsshp2_e[ij] = spg_p2(c, ssha_e, ij);
 }
+struct S_spg_s2 {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, double rdtbt, double wgt1, SpgSsh c, double *__restrict__ ssha_to, double *__restrict__ ssha, SpgWork w)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        spg_s2_point(d, rdtbt, wgt1, c, ssha_to, ssha, w, i, j);
+    }
+};
 KERNEL void k_spg_s2(DevDomain d, double rdtbt, double wgt1, SpgSsh c, double *__restrict__ ssha_to,
                      double *__restrict__ ssha, SpgWork w)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    spg_s2_point(d, rdtbt, wgt1, c, ssha_to, ssha, w, i, j);
+    S_spg_s2::run(vb_hw(), d, rdtbt, wgt1, c, ssha_to, ssha, w);
 }
 
 __device__ __forceinline__ void spg_new_vel_p2(const DevDomain &d, int p, int q, double rdtbt, const double *un_e,
@@ -457,16 +507,23 @@ __device__ __forceinline__ void spg_s3_point(const DevDomain &d, double rdtbt, d
     nx.un_adv[ij] = nx.un_adv[ij] + nx.wgt2 * zwx * m.r1_e2u[ij];
     nx.vn_adv[ij] = nx.vn_adv[ij] + nx.wgt2 * zwy * m.r1_e1v[ij];
 }
+struct S_spg_s3 {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, double rdtbt, double wgt1, const double *__restrict__ un_e, const double *__restrict__ vn_e, double *__restrict__ ubb_to_a, double *__restrict__ vbb_to_a, const double *__restrict__ r1_hu_n, const double *__restrict__ r1_hv_n, const double *__restrict__ hu_n, const double *__restrict__ hv_n, const double *__restrict__ ht_n, double *__restrict__ ua_b, double *__restrict__ va_b, SpgWork w, SpgNext nx)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        spg_s3_point(d, rdtbt, wgt1, un_e, vn_e, ubb_to_a, vbb_to_a, r1_hu_n, r1_hv_n, hu_n, hv_n, ht_n, ua_b, va_b, w,
+                     nx, i, j);
+    }
+};
 KERNEL void k_spg_s3(DevDomain d, double rdtbt, double wgt1, const double *__restrict__ un_e,
                      const double *__restrict__ vn_e, double *__restrict__ ubb_to_a, double *__restrict__ vbb_to_a,
                      const double *__restrict__ r1_hu_n, const double *__restrict__ r1_hv_n,
                      const double *__restrict__ hu_n, const double *__restrict__ hv_n, const double *__restrict__ ht_n,
                      double *__restrict__ ua_b, double *__restrict__ va_b, SpgWork w, SpgNext nx)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    spg_s3_point(d, rdtbt, wgt1, un_e, vn_e, ubb_to_a, vbb_to_a, r1_hu_n, r1_hv_n, hu_n, hv_n, ht_n, ua_b, va_b, w,
-                 nx, i, j);
+    S_spg_s3::run(vb_hw(), d, rdtbt, wgt1, un_e, vn_e, ubb_to_a, vbb_to_a, r1_hu_n, r1_hv_n, hu_n, hv_n, ht_n, ua_b, va_b, w, nx);
 }
 
 // ---------------------------------------------------------------------------
@@ -570,52 +627,84 @@ __global__ __launch_bounds__(NTH) void k_spg_loop(DevDomain d, SpgWork w, SpgLoo
 // ---------------------------------------------------------------------------
 // Phase 3, lines 1214-1269
 // ---------------------------------------------------------------------------
+struct S_spg_p3a {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, int first_euler, double * __restrict__ un_adv, double * __restrict__ vn_adv, double * __restrict__ ub2_b, double * __restrict__ vb2_b, double * __restrict__ un_bf, double * __restrict__ vn_bf)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const size_t ij = IJ(i, j);
+        const double r1_2 = 0.5, atfp = d.rn_atfp;
+        const double zwx = un_adv[ij], zwy = vn_adv[ij];
+        if (!first_euler) {
+            un_adv[ij] = r1_2 * (ub2_b[ij] + zwx - atfp * un_bf[ij]);
+            vn_adv[ij] = r1_2 * (vb2_b[ij] + zwy - atfp * vn_bf[ij]);
+            un_bf[ij] = atfp * un_bf[ij] + (zwx - ub2_b[ij]);
+            vn_bf[ij] = atfp * vn_bf[ij] + (zwy - vb2_b[ij]);
+        } else {
+            un_bf[ij] = 0.0;
+            vn_bf[ij] = 0.0;
+        }
+        ub2_b[ij] = zwx;
+        vb2_b[ij] = zwy;
+    }
+};
 KERNEL void k_spg_p3a(DevDomain d, int first_euler, double * __restrict__ un_adv, double * __restrict__ vn_adv, double * __restrict__ ub2_b,
                           double * __restrict__ vb2_b, double * __restrict__ un_bf, double * __restrict__ vn_bf)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const size_t ij = IJ(i, j);
-    const double r1_2 = 0.5, atfp = d.rn_atfp;
-    const double zwx = un_adv[ij], zwy = vn_adv[ij];
-    if (!first_euler) {
-        un_adv[ij] = r1_2 * (ub2_b[ij] + zwx - atfp * un_bf[ij]);
-        vn_adv[ij] = r1_2 * (vb2_b[ij] + zwy - atfp * vn_bf[ij]);
-        un_bf[ij] = atfp * un_bf[ij] + (zwx - ub2_b[ij]);
-        vn_bf[ij] = atfp * vn_bf[ij] + (zwy - vb2_b[ij]);
-    } else {
-        un_bf[ij] = 0.0;
-        vn_bf[ij] = 0.0;
-    }
-    ub2_b[ij] = zwx;
-    vb2_b[ij] = zwy;
+    S_spg_p3a::run(vb_hw(), d, first_euler, un_adv, vn_adv, ub2_b, vb2_b, un_bf, vn_bf);
 }
 
+template <typename T>
+struct S_spg_p3b {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, double r1_2dt_b, T * __restrict__ ua, T * __restrict__ va, T * __restrict__ un, T * __restrict__ vn, const double * __restrict__ ua_b, const double * __restrict__ va_b, const double * __restrict__ ub_b, const double * __restrict__ vb_b, const double * __restrict__ un_adv, const double * __restrict__ vn_adv, const double * __restrict__ r1_hu_n, const double * __restrict__ r1_hv_n, const double * __restrict__ un_b, const double * __restrict__ vn_b)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const size_t ij = IJ(i, j);
+        const double du = (ua_b[ij] - ub_b[ij]) * r1_2dt_b, dv = (va_b[ij] - vb_b[ij]) * r1_2dt_b;
+        const double cu = un_adv[ij] * r1_hu_n[ij], cv = vn_adv[ij] * r1_hv_n[ij];
+        const double ubn = un_b[ij], vbn = vn_b[ij];
+        for (int k = 1; k <= d.jpk - 1; k++) {
+            const size_t p = IJK(i, j, k);
+            ua[p] = (T)((double)ua[p] + du);
+            va[p] = (T)((double)va[p] + dv);
+            un[p] = (T)(((double)un[p] + cu - ubn) * UMASK(double, i, j, k));
+            vn[p] = (T)(((double)vn[p] + cv - vbn) * VMASK(double, i, j, k));
+        }
+    }
+};
 template <typename T>
 KERNEL void k_spg_p3b(DevDomain d, double r1_2dt_b, T * __restrict__ ua, T * __restrict__ va, T * __restrict__ un, T * __restrict__ vn, const double * __restrict__ ua_b,
                           const double * __restrict__ va_b, const double * __restrict__ ub_b, const double * __restrict__ vb_b, const double * __restrict__ un_adv,
                           const double * __restrict__ vn_adv, const double * __restrict__ r1_hu_n, const double * __restrict__ r1_hv_n, const double * __restrict__ un_b,
                           const double * __restrict__ vn_b)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const size_t ij = IJ(i, j);
-    const double du = (ua_b[ij] - ub_b[ij]) * r1_2dt_b, dv = (va_b[ij] - vb_b[ij]) * r1_2dt_b;
-    const double cu = un_adv[ij] * r1_hu_n[ij], cv = vn_adv[ij] * r1_hv_n[ij];
-    const double ubn = un_b[ij], vbn = vn_b[ij];
-    for (int k = 1; k <= d.jpk - 1; k++) {
-        const size_t p = IJK(i, j, k);
-        ua[p] = (T)((double)ua[p] + du);
-        va[p] = (T)((double)va[p] + dv);
-        un[p] = (T)(((double)un[p] + cu - ubn) * UMASK(double, i, j, k));
-        vn[p] = (T)(((double)vn[p] + cv - vbn) * VMASK(double, i, j, k));
-    }
+    S_spg_p3b<T>::run(vb_hw(), d, r1_2dt_b, ua, va, un, vn, ua_b, va_b, ub_b, vb_b, un_adv, vn_adv, r1_hu_n, r1_hv_n, un_b, vn_b);
 }
 
 // phase 2 initialisation, lines 665-702, and the reset of the time sums: the
 // now levels from un_b, vn_b, sshn; with ll_init the before levels at zero
 // (one kernel instead of memset/memcpy calls, so the call can be recorded
 // into a graph)
+struct S_spg_init {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 1;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, int ll_init, const double *__restrict__ un_b, const double *__restrict__ vn_b, const double *__restrict__ sshn, double *__restrict__ ub0, double *__restrict__ vb0, double *__restrict__ ss0, double *__restrict__ ub1, double *__restrict__ vb1, double *__restrict__ ss1, double *__restrict__ ub2, double *__restrict__ vb2, double *__restrict__ ss2, double *__restrict__ ua_b, double *__restrict__ va_b, double *__restrict__ ssha, double *__restrict__ un_adv, double *__restrict__ vn_adv)
+    {
+        THREAD_IJ_V(vb_, 1, 1);
+        if (i > d.jpi || j > d.jpj) return;
+        const size_t ij = IJ(i, j);
+        if (ll_init) {
+            ub1[ij] = 0.0; vb1[ij] = 0.0; ss1[ij] = 0.0;
+            ub2[ij] = 0.0; vb2[ij] = 0.0; ss2[ij] = 0.0;
+        }
+        ss0[ij] = sshn[ij];
+        ub0[ij] = un_b[ij];
+        vb0[ij] = vn_b[ij];
+        ua_b[ij] = 0.0; va_b[ij] = 0.0; ssha[ij] = 0.0; un_adv[ij] = 0.0; vn_adv[ij] = 0.0;
+    }
+};
 KERNEL void k_spg_init(DevDomain d, int ll_init, const double *__restrict__ un_b, const double *__restrict__ vn_b,
                        const double *__restrict__ sshn, double *__restrict__ ub0, double *__restrict__ vb0,
                        double *__restrict__ ss0, double *__restrict__ ub1, double *__restrict__ vb1,
@@ -623,17 +712,7 @@ KERNEL void k_spg_init(DevDomain d, int ll_init, const double *__restrict__ un_b
                        double *__restrict__ ss2, double *__restrict__ ua_b, double *__restrict__ va_b,
                        double *__restrict__ ssha, double *__restrict__ un_adv, double *__restrict__ vn_adv)
 {
-    THREAD_IJ(1, 1);
-    if (i > d.jpi || j > d.jpj) return;
-    const size_t ij = IJ(i, j);
-    if (ll_init) {
-        ub1[ij] = 0.0; vb1[ij] = 0.0; ss1[ij] = 0.0;
-        ub2[ij] = 0.0; vb2[ij] = 0.0; ss2[ij] = 0.0;
-    }
-    ss0[ij] = sshn[ij];
-    ub0[ij] = un_b[ij];
-    vb0[ij] = vn_b[ij];
-    ua_b[ij] = 0.0; va_b[ij] = 0.0; ssha[ij] = 0.0; un_adv[ij] = 0.0; vn_adv[ij] = 0.0;
+    S_spg_init::run(vb_hw(), d, ll_init, un_b, vn_b, sshn, ub0, vb0, ss0, ub1, vb1, ss1, ub2, vb2, ss2, ua_b, va_b, ssha, un_adv, vn_adv);
 }
 
 // ---------------------------------------------------------------------------
@@ -763,13 +842,13 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
     if (rc) return rc;
 
     Grid2 g = grid2(d.jpi, d.jpj);
-    LAUNCH("dyn_spg_ts", k_spg_cor1, g, d, st->ht_n, w);
-    LAUNCH("dyn_spg_ts", k_spg_cor2, g, d, st->ht_n, w);
-    if (d.nvor_scheme == NEMO_NP_EEN) LAUNCH("dyn_spg_ts", k_spg_cor3, g, d, w);
+    LAUNCHB("dyn_spg_ts", k_spg_cor1, (S_spg_cor1), g, d, st->ht_n, w);
+    LAUNCHB("dyn_spg_ts", k_spg_cor2, (S_spg_cor2), g, d, st->ht_n, w);
+    if (d.nvor_scheme == NEMO_NP_EEN) LAUNCHB("dyn_spg_ts", k_spg_cor3, (S_spg_cor3), g, d, w);
     if (!fused)
-        LAUNCH("dyn_spg_ts", k_spg_p1a<T>, g, d, ua, va, (const T *)un, (const T *)vn, st->r1_hu_n, st->r1_hv_n,
+        LAUNCHB("dyn_spg_ts", k_spg_p1a<T>, (S_spg_p1a<T>), g, d, ua, va, (const T *)un, (const T *)vn, st->r1_hu_n, st->r1_hv_n,
                st->un_b, st->vn_b, w);
-    LAUNCH("dyn_spg_ts", k_spg_p1b, g, d, st->un_b, st->vn_b, st->hu_n, st->hv_n, st->r1_hu_n, st->r1_hv_n, st->ht_n,
+    LAUNCHB("dyn_spg_ts", k_spg_p1b, (S_spg_p1b), g, d, st->un_b, st->vn_b, st->hu_n, st->hv_n, st->r1_hu_n, st->r1_hv_n, st->ht_n,
            st->rCdU_bot, st->utau, st->vtau, st->emp, st->rnf, w);
 
     // phase 2 initialisation, lines 665-702.  The before levels are set
@@ -785,7 +864,7 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
         return nh::set_error(NEMO_HIP_ERR_STATE,
                              "dyn_spg_ts: ln_bt_av = F carries the barotropic levels between calls; not recordable");
     }
-    LAUNCH("dyn_spg_ts", k_spg_init, g, d, ll_init, st->un_b, st->vn_b, st->sshn, ube[ru[0]], vbe[ru[0]], sse[rs[0]],
+    LAUNCHB("dyn_spg_ts", k_spg_init, (S_spg_init), g, d, ll_init, st->un_b, st->vn_b, st->sshn, ube[ru[0]], vbe[ru[0]], sse[rs[0]],
            ube[ru[1]], vbe[ru[1]], sse[rs[1]], ube[ru[2]], vbe[ru[2]], sse[rs[2]], st->ua_b, st->va_b, st->ssha,
            st->un_adv, st->vn_adv);
 
@@ -800,7 +879,7 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
     {
         double za1, za2, za3;
         ab3(1, za1, za2, za3);
-        LAUNCH("dyn_spg_ts", k_spg_s1, g, d, za1, za2, za3, wgt2[0], ube[ru[0]], ube[ru[1]], ube[ru[2]],
+        LAUNCHB("dyn_spg_ts", k_spg_s1, (S_spg_s1), g, d, za1, za2, za3, wgt2[0], ube[ru[0]], ube[ru[1]], ube[ru[2]],
                vbe[ru[0]], vbe[ru[1]], vbe[ru[2]], st->hu_n, st->hv_n, st->un_adv, st->vn_adv, w);
     }
     if (spg_loop_coop(d, w, st, icycle, ll_init, rdtbt, wgt1, wgt2, ube, vbe, sse, ua_e2, va_e2, zwx2, zwy2, ru,
@@ -822,7 +901,7 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
         else if (jn == 2 && ll_init) { za0 = 1.0833333333333; za1 = -0.1666666666666; za2 = 0.0833333333333; za3 = 0.0; }
         else { za0 = 0.614; za1 = 0.285; za2 = 0.088; za3 = 0.013; }
         const SpgSsh c{za0, za1, za2, za3, sse[rs[0]], sse[rs[1]], sse[rs[2]]};
-        LAUNCH("dyn_spg_ts", k_spg_s2, g, d, rdtbt, wgt1[jn - 1], c, sse[rs[3]], st->ssha, wc);
+        LAUNCHB("dyn_spg_ts", k_spg_s2, (S_spg_s2), g, d, rdtbt, wgt1[jn - 1], c, sse[rs[3]], st->ssha, wc);
         SpgNext nx{};
         nx.on = jn < icycle;
         if (nx.on) {
@@ -834,7 +913,7 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
             nx.ua_e = ua_e2[nxt]; nx.va_e = va_e2[nxt]; nx.zwx = zwx2[nxt]; nx.zwy = zwy2[nxt];
             nx.un_adv = st->un_adv; nx.vn_adv = st->vn_adv;
         }
-        LAUNCH("dyn_spg_ts", k_spg_s3, g, d, rdtbt, wgt1[jn - 1], ube[ru[0]], vbe[ru[0]], ube[ru[2]], vbe[ru[2]],
+        LAUNCHB("dyn_spg_ts", k_spg_s3, (S_spg_s3), g, d, rdtbt, wgt1[jn - 1], ube[ru[0]], vbe[ru[0]], ube[ru[2]], vbe[ru[2]],
                st->r1_hu_n, st->r1_hv_n, st->hu_n, st->hv_n, st->ht_n, st->ua_b, st->va_b, wc, nx);
         // swap: bb <- b, b <- n, n <- a (the after velocities were written
         // over bb; the after sea level into the fourth buffer)
@@ -842,10 +921,10 @@ int launch_dyn_spg_ts(int kt, const nemo_hip_spg_ts_t *st, int fused)
         t = ru[2]; ru[2] = ru[1]; ru[1] = ru[0]; ru[0] = t;
         t = rs[3]; rs[3] = rs[2]; rs[2] = rs[1]; rs[1] = rs[0]; rs[0] = t;
     }
-    LAUNCH("dyn_spg_ts", k_spg_p3a, g, d, (kt == d.nit000 && d.neuler == 0) ? 1 : 0, st->un_adv, st->vn_adv, st->ub2_b,
+    LAUNCHB("dyn_spg_ts", k_spg_p3a, (S_spg_p3a), g, d, (kt == d.nit000 && d.neuler == 0) ? 1 : 0, st->un_adv, st->vn_adv, st->ub2_b,
            st->vb2_b, st->un_bf, st->vn_bf);
     if (!fused)
-        LAUNCH("dyn_spg_ts", k_spg_p3b<T>, g, d, r1_2dt_b, ua, va, un, vn, st->ua_b, st->va_b, st->ub_b, st->vb_b,
+        LAUNCHB("dyn_spg_ts", k_spg_p3b<T>, (S_spg_p3b<T>), g, d, r1_2dt_b, ua, va, un, vn, st->ua_b, st->va_b, st->ub_b, st->vb_b,
                st->un_adv, st->vn_adv, st->r1_hu_n, st->r1_hv_n, st->un_b, st->vn_b);
     return 0;
 }

@@ -147,6 +147,26 @@ class NemoHip:
     def graph_free(self, g):
         self._check(self.L.nemo_hip_graph_free(C.c_int(g)))
 
+    # plans: one traced step of every member, replayed as batched launches
+    def plan_begin(self):
+        self._check(self.L.nemo_hip_plan_begin())
+
+    def plan_end(self):
+        p = C.c_int()
+        self._check(self.L.nemo_hip_plan_end(C.byref(p)))
+        return p.value
+
+    def plan_launch(self, p):
+        self._check(self.L.nemo_hip_plan_launch(C.c_int(p)))
+
+    def plan_info(self, p):
+        v = [C.c_int() for _ in range(5)]
+        self._check(self.L.nemo_hip_plan_info(C.c_int(p), *[C.byref(x) for x in v]))
+        return dict(zip(("members", "groups", "ops", "batched", "launches"), (x.value for x in v)))
+
+    def plan_free(self, p):
+        self._check(self.L.nemo_hip_plan_free(C.c_int(p)))
+
     # resident fields -----------------------------------------------------
     def field_alloc(self, nbytes):
         p = C.c_void_p()

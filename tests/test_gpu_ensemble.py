@@ -6,12 +6,16 @@ replayed:
   by one (FP32 storage); the replays are interleaved with eager work and
   synchronisations, and the recording refuses the calls it cannot replay
   faithfully;
+* a plan (nemo_hip_plan_*) of the same step over closed members and a
+  cyclic one: every position of the closed members' launch sequence is one
+  batched launch over all their workgroups, and plan launches, eager or
+  recorded into a graph, equal the calls issued one by one, bit for bit;
 * 24 members of uneven sizes (56 levels, closed and cyclic, each its own
   seeded land mask, bathymetry and initial state) advanced by whole time
   steps (the hot path and its closure), the step recorded once for all
-  members: after 8 steps each member equals the oracle's trajectory of the
-  same member, bit for bit in FP64 and within the FP32 storage form's
-  tolerances.
+  members into a graph, of the calls or of a plan of them: after 8 steps
+  each member equals the oracle's trajectory of the same member, bit for
+  bit in FP64 and within the FP32 storage form's tolerances.
 """
 import json
 
@@ -97,6 +101,74 @@ def test_ensemble_graph_replays_match_eager_calls(hip):
             hip.context_destroy(c)
 
 
+@pytest.mark.gpu
+def test_ensemble_plan_replays_match_eager_calls(hip):
+    """a plan of the fused FP32 step over closed and cyclic members: each
+    position of the closed members' launch sequence is one batched launch,
+    the cyclic member forms a group of its own; plan launches, eager or
+    recorded into a graph, equal the calls issued one by one"""
+    members = MEMBERS + [(44, 26, 9, 0, 8)]
+    ctxs, eager, batched, plan, g = [], [], [], None, None
+    try:
+        for (jpi, jpj, jpk, jperio, seed) in members:
+            c = hip.context_create()
+            ctxs.append(c)
+            hip.context_select(c)
+            d = configs.basin(jpi, jpj, jpk, seed=seed, jperio=jperio, opts=dict(OPTS))
+            hip.register(d)
+            eager.append(_stepper(hip, d, seed + 100))
+            batched.append(_stepper(hip, d, seed + 100))
+        for kt in (1, 2):
+            for c, a, b in zip(ctxs, eager, batched):
+                hip.context_select(c)
+                a.step(kt)
+                b.step(kt)
+        hip.sync()
+        # what tracing refuses, and an empty plan
+        hip.plan_begin()
+        hip.context_select(ctxs[0])
+        with pytest.raises(nemo_hip.NemoHipError, match="first two steps"):
+            batched[0].step(2)
+        with pytest.raises(nemo_hip.NemoHipError, match="no launch"):
+            hip.plan_end()
+        hip.plan_begin()
+        for c, b in zip(ctxs, batched):
+            hip.context_select(c)
+            b.step(3)
+        plan = hip.plan_end()
+        info = hip.plan_info(plan)
+        assert info["members"] == len(members) and info["groups"] == 2, info
+        assert info["batched"] > 0 and info["launches"] < info["ops"] * len(members), info
+        for kt in (3, 4, 5, 6):
+            for c, a in zip(ctxs, eager):
+                hip.context_select(c)
+                a.step(kt)
+            if kt == 5:       # from here on through a graph of the plan
+                hip.graph_begin()
+                hip.plan_launch(plan)
+                g = hip.graph_end()
+            if kt >= 5:
+                hip.graph_launch(g)
+            else:
+                hip.plan_launch(plan)
+            hip.sync()
+            torch.cuda.synchronize()
+            for m, (a, b) in enumerate(zip(eager, batched)):
+                sa, sb = _state(a), _state(b)
+                for k in sa:
+                    assert np.array_equal(sa[k], sb[k]), f"member {m} field {k} differs after the plan at kt {kt}"
+        print(json.dumps({"plan": info}))
+    finally:
+        if g is not None:
+            hip.graph_free(g)
+        if plan is not None:
+            hip.plan_free(plan)
+        hip.context_select(0)
+        del eager, batched
+        for c in ctxs:
+            hip.context_destroy(c)
+
+
 # 24 members: i x j from 24 x 20 to 70 x 58, 56 levels; every third one
 # east-west cyclic
 RNG = np.random.default_rng(20261203)
@@ -109,10 +181,11 @@ SP_TOL = {"tsn": 1e-5, "tsb": 1e-5, "un": 3e-6, "vn": 3e-6, "sshn": 3e-6, "un_b"
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("sp", [False, True], ids=["fp64", "fp32"])
-def test_uneven_ensemble_graph_against_oracle(hip, sp):
+@pytest.mark.parametrize("mode", ["graph", "plan"])
+def test_uneven_ensemble_graph_against_oracle(hip, sp, mode):
     from trajectory import OUT_2D, OUT_3D, OUT_4D, HotStep, oracle_run
     last = 8
-    ctxs, members, doms, g = [], [], [], None
+    ctxs, members, doms, g, plan, info = [], [], [], None, None, None
     try:
         for (jpi, jpj, jpk, jperio, seed) in UNEVEN:
             c = hip.context_create()
@@ -127,11 +200,23 @@ def test_uneven_ensemble_graph_against_oracle(hip, sp):
             for c, m in zip(ctxs, members):
                 hip.context_select(c)
                 m.step(kt)
-        hip.graph_begin()
-        for c, m in zip(ctxs, members):
-            hip.context_select(c)
-            m.step(3)
-        g = hip.graph_end()
+        if mode == "plan":        # one batched launch per position of the members' step
+            hip.plan_begin()
+            for c, m in zip(ctxs, members):
+                hip.context_select(c)
+                m.step(3)
+            plan = hip.plan_end()
+            info = hip.plan_info(plan)
+            assert info["members"] == len(members) and info["batched"] > 0, info
+            hip.graph_begin()
+            hip.plan_launch(plan)
+            g = hip.graph_end()
+        else:
+            hip.graph_begin()
+            for c, m in zip(ctxs, members):
+                hip.context_select(c)
+                m.step(3)
+            g = hip.graph_end()
         for kt in range(3, last + 1):
             hip.graph_launch(g)
         hip.sync()
@@ -149,10 +234,13 @@ def test_uneven_ensemble_graph_against_oracle(hip, sp):
                     e = float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
                     worst[k] = max(worst.get(k, 0.0), e)
                     assert e <= SP_TOL[k], f"member {i} {d.jpi}x{d.jpj} {k}: relative error {e:.2e}"
-        print(json.dumps({"members": len(members), "steps": last, "sp": sp, "max_relerr": worst}))
+        print(json.dumps({"members": len(members), "steps": last, "sp": sp, "mode": mode, "plan": info,
+                          "max_relerr": worst}))
     finally:
         if g is not None:
             hip.graph_free(g)
+        if plan is not None:
+            hip.plan_free(plan)
         hip.context_select(0)
         del members
         for c in ctxs:

@@ -1,13 +1,14 @@
 """The fused FCT kernel (tra_adv_fct on closed domains in one k-march per
 tile and tracer) against the oracle and against the flux / limiter chain.
 
-* FP64 drop-in tra_adv_fct: bit-identical to the oracle for the four
-  order pairs on basins whose sizes leave partial tiles in i and j, on the
+* FP64 drop-in tra_adv_fct with either workgroup size: bit-identical to
+  the oracle for the four order pairs on basins whose sizes leave partial tiles in i and j, on the
   GYRE-like basin and on a 75-level basin, with 1, 2 and 3 tracers (a pair
   and a single-tracer remainder);
-* both storage forms: the fused kernel equals the chain (tuning switch
-  fct_fused = 0) bit for bit, for the drop-in routine (trend read) and for
-  the fused step (trend from zero, transports formed where used).
+* both storage forms: the fused kernel, with 256 or 512 threads per tile
+  (tuning switch fct_fused = 1, 2), equals the chain (fct_fused = 0) bit
+  for bit, for the drop-in routine (trend read) and for the fused step
+  (trend from zero, transports formed where used).
 """
 import numpy as np
 import pytest
@@ -45,20 +46,33 @@ def setup(hip, name, kjpt=2, seed=5):
     return d, S, o, z, tb, tn, ta
 
 
+def with_mode(hip, mode, fn):
+    default = hip.tune_get("fct_fused")
+    try:
+        hip.tune_set("fct_fused", mode)
+        return fn()
+    finally:
+        hip.tune_set("fct_fused", default)
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", list(BASINS))
 @pytest.mark.parametrize("orders", ORDERS)
-def test_fused_fct_fp64_oracle(hip, name, orders):
+@pytest.mark.parametrize("mode", [1, 2])
+def test_fused_fct_fp64_oracle(hip, name, orders, mode):
     d, S, o, z, tb, tn, ta = setup(hip, name)
-    assert hip.tune_get("fct_fused") == 1
     p2dt = 2.0 * d.opts["rdt"]
     want = ta.copy(order="F")
     o.tra_adv_fct(3, 1, "TRA", p2dt, *z, tb, tn, want, 2, *orders)
-    got = dev(ta)
-    hip.tra_adv_fct(3, 1, "TRA", p2dt, *[dev(a) for a in z], dev(tb), dev(tn), got, 2, *orders)
-    sync(hip)
-    g = host(got, want.shape)
-    assert np.array_equal(g, want), f"{name} {orders}: max |diff| {np.abs(g - want).max():.3e}"
+
+    def run():
+        got = dev(ta)
+        hip.tra_adv_fct(3, 1, "TRA", p2dt, *[dev(a) for a in z], dev(tb), dev(tn), got, 2, *orders)
+        sync(hip)
+        return host(got, want.shape)
+
+    g = with_mode(hip, mode, run)
+    assert np.array_equal(g, want), f"{name} {orders} mode {mode}: max |diff| {np.abs(g - want).max():.3e}"
 
 
 @pytest.mark.gpu
@@ -75,14 +89,12 @@ def test_fused_fct_tracer_counts(hip, kjpt):
 
 
 def both_paths(hip, fn):
-    """fn() with the fused kernel, then with the chain; returns both results"""
-    try:
-        hip.tune_set("fct_fused", 1)
-        a = fn()
-        hip.tune_set("fct_fused", 0)
-        b = fn()
-    finally:
-        hip.tune_set("fct_fused", 1)
+    """fn() with the fused kernel (both workgroup sizes, which must agree),
+    then with the chain; returns the fused and the chain results"""
+    a = with_mode(hip, 1, fn)
+    a2 = with_mode(hip, 2, fn)
+    assert np.array_equal(a, a2, equal_nan=True), "the fused kernel's two workgroup sizes differ"
+    b = with_mode(hip, 0, fn)
     return a, b
 
 
