@@ -12,9 +12,6 @@
 #include "common.hpp"
 #include "kernels.hpp"
 
-// the column arrays of the implicit-diffusion adjoint (levels 1..jpk-1)
-#define NH_MAXK 160
-
 namespace {
 
 __device__ __forceinline__ bool interior(const DevDomain &d, int i, int j)
@@ -207,37 +204,64 @@ KERNEL void k_fct_ad_l4(DevDomain d, FctAd a, double *__restrict__ zl_u, double 
 
 // kn_v = 4: ztw = interp_4th_cpt(tn) enters the vertical flux as w ztw;
 // the transpose of the compact solve (traadv_fct.F90:535-614) on one
-// interior column per thread, the substitutions reversed
-KERNEL void k_fct_ad_v4(DevDomain d, FctAd a, double *__restrict__ tn_ad)
+// interior column per thread, the substitutions reversed, as two passes
+// over the column: top down, the pivots (recomputed as the forward solve
+// forms them) and the reversed back substitution, whose results t(k) and
+// pivots go to two scratch columns; bottom up, the reversed forward
+// substitution and the adjoint of the right-hand side.  Nothing is held
+// per level in the thread, so the kernel needs no private memory.
+KERNEL void k_fct_ad_v4(DevDomain d, FctAd a, double *__restrict__ tn_ad, double *__restrict__ zwt_s,
+                        double *__restrict__ t_s)
 {
     THREAD_IJ(2, 2);
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
     const int K = d.jpk - 1, ikt = A2(d.mikt, i, j) + 1, ikb = A2(d.mbkt, i, j);
-    double zwt[NH_MAXK], zs[NH_MAXK], cc[NH_MAXK], x[NH_MAXK], r[NH_MAXK];
-    for (int k = 2; k <= K; k++) {
+    // the diagonal, the equal sub- and super-diagonals and the right-hand
+    // side coefficient of row k
+    auto row = [&](int k, double &zd, double &zs, double &cc) {
         const double wm = WMASK(double, i, j, k);
-        double zd = 3.0 * wm + 1.0;
-        zs[k] = wm;           // the sub- and super-diagonals are equal
-        cc[k] = 3.0 * wm;     // right-hand side: cc (tn(k) + tn(k-1))
+        zd = 3.0 * wm + 1.0;
+        zs = wm;
+        cc = 3.0 * wm;
         if (k == ikt || k == ikb) {
             zd = 1.0;
-            zs[k] = 0.0;
-            cc[k] = 0.5;
+            zs = 0.0;
+            cc = 0.5;
         }
-        zwt[k] = (k == 2) ? zd : zd - zs[k] * zs[k - 1] / zwt[k - 1];
-        x[k] = A3(a.wn, i, j, k) * (az_face(d, a, i, j, k) * WMASK(double, i, j, k));
-        r[k] = 0.0;
-    }
-    for (int k = 2; k <= K - 1; k++) {   // back substitution reversed
-        const double t = x[k] / zwt[k];
-        r[k] += t;
-        x[k + 1] -= zs[k] * t;
-    }
-    r[K] += x[K] / zwt[K];
-    for (int k = K; k >= 3; k--) r[k - 1] -= zs[k] / zwt[k - 1] * r[k];   // forward substitution reversed
+    };
+    double zwt_p = 0.0, zs_p = 0.0, carry = 0.0;
     for (int k = 2; k <= K; k++) {
-        A3(tn_ad, i, j, k) += cc[k] * r[k];
-        A3(tn_ad, i, j, k - 1) += cc[k] * r[k];
+        double zd, zs, cc;
+        row(k, zd, zs, cc);
+        const double zwt = (k == 2) ? zd : zd - zs * zs_p / zwt_p;
+        double x = A3(a.wn, i, j, k) * (az_face(d, a, i, j, k) * WMASK(double, i, j, k));
+        if (k > 2) x -= carry;
+        const double t = x / zwt;
+        A3(t_s, i, j, k) = t;
+        A3(zwt_s, i, j, k) = zwt;
+        carry = zs * t;
+        zwt_p = zwt;
+        zs_p = zs;
+    }
+    // r(K) = t(K); r(k-1) = t(k-1) - zs(k) / zwt(k-1) r(k); tn_ad(m) gets
+    // cc(m) r(m) and then cc(m+1) r(m+1), in that order
+    double zd, zs_k, cc_k;
+    row(K, zd, zs_k, cc_k);
+    double r_k = A3(t_s, i, j, K);
+    double acc = A3(tn_ad, i, j, K) + cc_k * r_k;   // level K: cc(K) r(K) only
+    for (int k = K; k >= 2; k--) {
+        A3(tn_ad, i, j, k) = acc;
+        const double up = cc_k * r_k;                 // cc(k) r(k), also owed to level k-1
+        if (k == 2) {
+            A3(tn_ad, i, j, 1) = A3(tn_ad, i, j, 1) + up;
+            break;
+        }
+        const double r_km1 = A3(t_s, i, j, k - 1) - zs_k / A3(zwt_s, i, j, k - 1) * r_k;
+        double cc_km1;
+        row(k - 1, zd, zs_k, cc_km1);
+        acc = (A3(tn_ad, i, j, k - 1) + cc_km1 * r_km1) + up;
+        r_k = r_km1;
+        cc_k = cc_km1;
     }
 }
 
@@ -275,38 +299,47 @@ KERNEL void k_ldf_lap_ad(DevDomain d, const double *__restrict__ ahu, const doub
 }
 
 // ---------------------------------------------------------------------------
-// tra_zdf_imp, one column per thread: the elimination reversed
+// tra_zdf_imp, one column per thread: the elimination reversed, as two
+// passes over the column.  Top down: the pivots zwt(k) (trazdf.F90:193-234,
+// recomputed as the forward solve forms them, kept in a scratch column)
+// and the reversed back substitution t(k) = x(k) tmask / zwt(k), x(k+1) -=
+// zws(k) t(k), its t(k) written over ta_ad(k).  Bottom up: the reversed
+// forward substitution r(k-1) = t(k-1) - zwi(k) / zwt(k-1) r(k) and the
+// adjoint of the right-hand side.  No per-level state in the thread.
 // ---------------------------------------------------------------------------
 KERNEL void k_zdf_imp_ad(DevDomain d, double p2dt, const double *__restrict__ avt, double *__restrict__ ta_ad,
-                         double *__restrict__ tb_ad)
+                         double *__restrict__ tb_ad, double *__restrict__ zwt_s)
 {
     THREAD_IJ(2, 2);
     if (i > d.jpi - 1 || j > d.jpj - 1) return;
     const Metrics<double> &m = d.md;
     const uint32_t ij = IJ32(i, j);
     const int K = d.jpk - 1;
-    // pivots zwt(k) and the couplings, recomputed top-down (trazdf.F90:193-234)
-    double zwt[NH_MAXK], zws[NH_MAXK], zwi[NH_MAXK], x[NH_MAXK], r[NH_MAXK];
-    for (int k = 1; k <= K; k++) {
+    auto zwi_of = [&](int k) {
         const double wt_k = (k == 1) ? 0.0 : A3(avt, i, j, k);
-        zwi[k] = -p2dt * wt_k / E3(m.e3w, ij, k);
-        zws[k] = -p2dt * A3(avt, i, j, k + 1) / E3(m.e3w, ij, k + 1);
-        const double zwd = E3(m.e3t, ij, k) - zwi[k] - zws[k];
-        zwt[k] = (k == 1) ? zwd : zwd - zwi[k] * zws[k - 1] / zwt[k - 1];
-        x[k] = A3(ta_ad, i, j, k);
-        r[k] = 0.0;
-    }
-    for (int k = 1; k <= K - 1; k++) {
-        const double t = x[k] * TMASK(double, i, j, k) / zwt[k];
-        r[k] += t;
-        x[k + 1] -= zws[k] * t;
-    }
-    r[K] += x[K] * TMASK(double, i, j, K) / zwt[K];
-    for (int k = K; k >= 2; k--) r[k - 1] -= zwi[k] / zwt[k - 1] * r[k];
+        return -p2dt * wt_k / E3(m.e3w, ij, k);
+    };
+    double zwt_p = 0.0, zws_p = 0.0, carry = 0.0;
     for (int k = 1; k <= K; k++) {
+        const double zwi = zwi_of(k);
+        const double zws = -p2dt * A3(avt, i, j, k + 1) / E3(m.e3w, ij, k + 1);
+        const double zwd = E3(m.e3t, ij, k) - zwi - zws;
+        const double zwt = (k == 1) ? zwd : zwd - zwi * zws_p / zwt_p;
+        double x = A3(ta_ad, i, j, k);
+        if (k > 1) x -= carry;
+        const double t = x * TMASK(double, i, j, k) / zwt;
+        A3(ta_ad, i, j, k) = t;
+        A3(zwt_s, i, j, k) = zwt;
+        carry = zws * t;
+        zwt_p = zwt;
+        zws_p = zws;
+    }
+    double r = A3(ta_ad, i, j, K);
+    for (int k = K; k >= 1; k--) {
         const double e3 = E3(m.e3t, ij, k);
-        A3(tb_ad, i, j, k) += e3 * r[k];
-        A3(ta_ad, i, j, k) = p2dt * e3 * r[k];
+        A3(tb_ad, i, j, k) += e3 * r;
+        A3(ta_ad, i, j, k) = p2dt * e3 * r;
+        if (k > 1) r = A3(ta_ad, i, j, k - 1) - zwi_of(k) / A3(zwt_s, i, j, k - 1) * r;
     }
 }
 
@@ -376,8 +409,6 @@ int launch_fct_ad(const double *pun, const double *pvn, const double *pwn, int k
                   const double *cy, const double *cz, const double *ta_ad, double *tb_ad, double *tn_ad)
 {
     const DevDomain &d = nh::dom();
-    if (kn_v == 4 && d.jpk > NH_MAXK - 1)
-        return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "tra_adv_fct_ad: jpk > %d", NH_MAXK - 1);
     Grid2 g = grid2(d.jpi, d.jpj);
     g.grid.z = (unsigned)(d.jpk - 1);
     double *zl_u = (double *)nh::scratch3(nh::S_ZWI), *zl_v = (double *)nh::scratch3(nh::S_ZWX);
@@ -386,7 +417,8 @@ int launch_fct_ad(const double *pun, const double *pvn, const double *pwn, int k
     LAUNCH("tra_adv_fct_ad", k_fct_ad, g, d, a, tb_ad, tn_ad);
     if (kn_v == 4) {
         Grid2 gc = grid2(d.jpi, d.jpj);
-        LAUNCH("tra_adv_fct_ad", k_fct_ad_v4, gc, d, a, tn_ad);
+        LAUNCH("tra_adv_fct_ad", k_fct_ad_v4, gc, d, a, tn_ad, (double *)nh::scratch3(nh::S_ZWY),
+               (double *)nh::scratch3(nh::S_ZWZ));
     }
     return 0;
 }
@@ -403,9 +435,8 @@ int launch_ldf_lap_ad(const double *pahu, const double *pahv, const double *ta_a
 int launch_zdf_imp_ad(double p2dt, const double *avt, double *ta_ad, double *tb_ad)
 {
     const DevDomain &d = nh::dom();
-    if (d.jpk > NH_MAXK - 1) return nh::set_error(NEMO_HIP_ERR_UNSUPPORTED, "tra_zdf_imp_ad: jpk > %d", NH_MAXK - 1);
     Grid2 g = grid2(d.jpi, d.jpj);
-    LAUNCH("tra_zdf_imp_ad", k_zdf_imp_ad, g, d, p2dt, avt, ta_ad, tb_ad);
+    LAUNCH("tra_zdf_imp_ad", k_zdf_imp_ad, g, d, p2dt, avt, ta_ad, tb_ad, (double *)nh::scratch3(nh::S_ZWT));
     return 0;
 }
 

@@ -404,7 +404,11 @@ def make_state_device(d, seed=1, umax=0.2, divergent=0.2, kz_range=(1e-5, 1e-1),
     """make_state's fields generated with torch on the GPU: every 3D field is a
     tensor of shape (jpk, jpj, jpi), i.e. NEMO's Fortran layout, flattened by
     .reshape(-1).  Horizontal structure: FFT-smoothed noise; vertical
-    structure: two random modes per field (as make_state on large grids)."""
+    structure: two random modes per field (as make_state on large grids).
+    Unlike make_state, the streamfunction is tapered to zero near the coast
+    and the divergent part bounded by a vertical Courant number of 0.5, so
+    the state is within the advection schemes' stability limits at every
+    size."""
     import torch
     g = torch.Generator(device=device)
     g.manual_seed(int(seed))
@@ -467,36 +471,57 @@ def make_state_device(d, seed=1, umax=0.2, divergent=0.2, kz_range=(1e-5, 1e-1),
     tn[1] = (35.0 + sst_noise * randn(jpk, jpj, jpi)) * t
     S["tsn"] = tn
     S["tsb"] = tn + 0.01 * randn(2, jpk, jpj, jpi) * t[None]
-    psi = field3d() * f
+    # The streamfunction is tapered to zero over r = sig/2 cells from the coast
+    # (per level, the mean of the f-mask eroded by 1..r cells) rather than cut
+    # there: a cut leaves jumps of tens of rms speeds along the coast.
+    # Tapered, its velocities stay divergence-free to rounding, coastal cells
+    # included, so w comes from the divergent part alone.
+    def taper(mask, r=max(4, int(round(sig / 2.0)))):
+        e, acc = 1.0 - mask, torch.zeros_like(mask)
+        for _ in range(r):
+            e = torch.nn.functional.max_pool2d(e[:, None], 3, stride=1, padding=1)[:, 0]
+            acc += 1.0 - e
+        return acc / r
+    psi = field3d() * taper(f)
     phi = field3d()
-    un = torch.zeros_like(t)
-    vn = torch.zeros_like(t)
-    un[:, 1:, :] = -(psi[:, 1:, :] - psi[:, :-1, :])
-    vn[:, :, 1:] = psi[:, :, 1:] - psi[:, :, :-1]
-    un[:, :, :-1] += divergent * (phi[:, :, 1:] - phi[:, :, :-1])
-    vn[:, :-1, :] += divergent * (phi[:, 1:, :] - phi[:, :-1, :])
-    un *= u
-    vn *= v
+    e3t = torch.from_numpy(d.e3t_1d).to(device)[:, None, None]
+    e3w = torch.from_numpy(d.e3w_1d).to(device)[:, None, None]
+    e2u, e1v, r1_e1e2t = T2(A["e2u"]), T2(A["e1v"]), T2(A["r1_e1e2t"])
+
+    def velocities(a_div):
+        un = torch.zeros_like(t)
+        vn = torch.zeros_like(t)
+        un[:, 1:, :] = -(psi[:, 1:, :] - psi[:, :-1, :])
+        vn[:, :, 1:] = psi[:, :, 1:] - psi[:, :, :-1]
+        un[:, :, :-1] += a_div * (phi[:, :, 1:] - phi[:, :, :-1])
+        vn[:, :-1, :] += a_div * (phi[:, 1:, :] - phi[:, :-1, :])
+        return un * u, vn * v
+
+    def continuity(un, vn):
+        """wn(k) = wn(k+1) - e3t*hdiv (masked), wn(jpk) = 0 (sshwzv.F90 wzv, linear free surface)"""
+        zu = e2u[None] * e3t * un
+        zv = e1v[None] * e3t * vn
+        hdiv = torch.zeros_like(t)
+        hdiv[:, 1:-1, 1:-1] = ((zu[:, 1:-1, 1:-1] - zu[:, 1:-1, :-2] + zv[:, 1:-1, 1:-1] - zv[:, :-2, 1:-1])
+                               * r1_e1e2t[None, 1:-1, 1:-1] / e3t)
+        inc = (e3t * hdiv * t)[:-1]
+        wn = torch.zeros_like(t)
+        wn[:-1] = torch.flip(torch.cumsum(torch.flip(inc, [0]), 0), [0]) * t[:-1]
+        return wn
+
+    un, vn = velocities(divergent)
     sc = umax / torch.sqrt(torch.mean(un[u > 0] ** 2)).item()
+    # The divergent part is then scaled down, if need be, so that the vertical
+    # Courant number 2 rdt |w| / e3w stays below 0.5 (the thin surface levels
+    # of a stretched grid set it): the upstream and centred fluxes of the FCT
+    # are stable only below 1.
+    p2dt = 2.0 * float(d.opts.get("rdt", 600.0))
+    cw = float((continuity(*velocities(divergent)).abs() * (sc * p2dt) / e3w).max())
+    a_div = divergent * min(1.0, 0.5 / cw) if cw > 0 else divergent
+    un, vn = velocities(a_div)
     un *= sc
     vn *= sc
-    # The masked streamfunction jumps to zero at the coast, and on wide grids
-    # (smoothing scale jpi/16) those jumps reach 60 times the rms speed.  The
-    # cap at 5 umax keeps the advective Courant number of a stepped state
-    # below 0.15 at 10 km and 2 rdt = 1200 s (w follows from continuity below).
-    un.clamp_(-5.0 * umax, 5.0 * umax)
-    vn.clamp_(-5.0 * umax, 5.0 * umax)
-    e3t = torch.from_numpy(d.e3t_1d).to(device)[:, None, None]
-    e2u, e1v, r1_e1e2t = T2(A["e2u"]), T2(A["e1v"]), T2(A["r1_e1e2t"])
-    zu = e2u[None] * e3t * un
-    zv = e1v[None] * e3t * vn
-    hdiv = torch.zeros_like(t)
-    hdiv[:, 1:-1, 1:-1] = ((zu[:, 1:-1, 1:-1] - zu[:, 1:-1, :-2] + zv[:, 1:-1, 1:-1] - zv[:, :-2, 1:-1])
-                           * r1_e1e2t[None, 1:-1, 1:-1] / e3t)
-    # wn(k) = wn(k+1) - e3t*hdiv (masked), wn(jpk) = 0
-    inc = (e3t * hdiv * t)[:-1]
-    wn = torch.zeros_like(t)
-    wn[:-1] = torch.flip(torch.cumsum(torch.flip(inc, [0]), 0), [0]) * t[:-1]
+    wn = continuity(un, vn)
     S["un"], S["vn"], S["wn"] = un, vn, wn
     S["ub"] = un * (1.0 + 0.05 * randn(jpk, jpj, jpi)) * u
     S["vb"] = vn * (1.0 + 0.05 * randn(jpk, jpj, jpi)) * v

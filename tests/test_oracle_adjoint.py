@@ -230,3 +230,43 @@ def test_gradient_matches_finite_differences():
         fd = (jp - jm) / (2 * eps)
         ad = float(np.vdot(g, dirn))
         assert abs(fd - ad) <= 1e-9 * max(abs(fd), 1e-12), (q, fd, ad)
+
+
+def test_nonlinear_step_is_not_differentiable_at_finite_difference_scales():
+    """Why the gradient is checked against the frozen-limiter model: central
+    differences of the nonlinear FCT step (the limiter factors recomputed
+    from each perturbed state) do not settle as the step shrinks from 1e-2
+    to 1e-5 -- nonosc's min/max branches switch within every such step, so
+    no linearisation, frozen or not, can match them to 1e-6 -- while those
+    of the frozen model agree to rounding at every step."""
+    d, S, o, F = setup(0)
+    t, *_ = configs.masks(d)
+    kn = (4, 4)
+    T0 = t1(S, "tsn")
+    o.lbc_lnk(T0, "T")
+    s3 = (d.jpi, d.jpj, d.jpk)
+    cs = []
+    b, n = T0.copy(order="F"), T0.copy(order="F")
+    for _ in range(4):
+        cs.append(coef(o, d, F, b, n, kn=kn))
+        b, n = tl_step(o, d, F, cs[-1], b, n)
+    rng = np.random.default_rng(3)
+    target = n + 0.1 * rng.standard_normal(s3)
+    w = np.zeros(s3)
+    w[1:-1, 1:-1, 0] = t[1:-1, 1:-1, 0]
+
+    def J(T, frozen):
+        b, n = T.copy(order="F"), T.copy(order="F")
+        for c in cs:
+            b, n = tl_step(o, d, F, c if frozen else coef(o, d, F, b, n, kn=kn), b, n)
+        return 0.5 * float(np.sum(w * (n - target) ** 2))
+
+    spread = {True: [], False: []}
+    for q in range(4):
+        dirn = interior(d, np.random.default_rng(100 + q).standard_normal(s3))
+        o.lbc_lnk(dirn, "T")
+        for frozen in (True, False):
+            fd = [(J(T0 + e * dirn, frozen) - J(T0 - e * dirn, frozen)) / (2 * e) for e in (1e-2, 1e-3, 1e-4, 1e-5)]
+            spread[frozen].append((max(fd) - min(fd)) / abs(fd[-1]))
+    assert max(spread[True]) <= 1e-8, spread[True]
+    assert min(spread[False]) >= 1e-2, spread[False]
