@@ -384,13 +384,18 @@ KERNEL void k_fct_final(DevDomain d, const T * __restrict__ paa, const T * __res
 }
 
 // (3)+(4) fused for closed domains: the Zalesak betas and the limited
-// divergence in one k-march per 64x4 tile.  The bounds zbup/zbdo (FP64) of
+// divergence in one k-march per 64xTY tile.  The bounds zbup/zbdo (FP64) of
 // the tile and a 2-point halo, for three levels, and the betas of the tile
 // and a 1-point halo, for two levels, live in LDS, so the betas never go
 // through HBM.  Every global load is issued one level ahead into registers
 // and consumed after the next barrier, so the loads of the march overlap
-// its LDS work.  Every value is the one k_fct_beta and k_fct_final compute
-// (same expressions, same rounding to the storage type), and the ring values
+// its LDS work.  Each plane is fetched once per tile: the antidiffusive
+// fluxes the limited divergence needs are passed on through LDS by the
+// threads that loaded them for the betas, the vertical flux of level k+1
+// stays in a register as the beta's flux of level k, and (FP32 storage) the
+// first guess of the betas comes from the plane fetched for the bounds.
+// Every value is the one k_fct_beta and k_fct_final compute (same
+// expressions, same rounding to the storage type), and the ring values
 // lbc_lnk gives a closed domain (zero betas and limited fluxes) are selected
 // where they apply.
 // k_fct_lim tiles: LIM_TX x TY cells per 64x4 workgroup, so each thread owns
@@ -416,8 +421,15 @@ struct S_fct_lim {
         constexpr int NZ = (ZX * ZY + NH_BLOCK - 1) / NH_BLOCK, NB = (BX * BY + NH_BLOCK - 1) / NH_BLOCK;
         constexpr int WY = NH_BLOCK / LIM_TX, NR = TY / WY;   // workgroup rows, cells per thread
         static_assert(TY % WY == 0, "tile height must be a multiple of the workgroup height");
+        // the first guess through LDS for FP32 storage only: with FP64 the
+        // extra plane would cost a workgroup per CU
+        constexpr bool AFT_LDS = sizeof(T) == 4;
         __shared__ ZB zup_s[3][ZY][ZX], zdo_s[3][ZY][ZX];
         __shared__ T bup_s[2][BY][BX], bdo_s[2][BY][BX];
+        __shared__ T aft_s[AFT_LDS ? 3 : 1][AFT_LDS ? BY : 1][AFT_LDS ? BX : 1];
+        // paa, pbb of level k and pcc of level k+1 over the beta region, for
+        // the limited divergence of level k
+        __shared__ T faa_s[BY][BX], fbb_s[BY][BX], fcc_s[BY][BX];
         // vb_.z selects the tracer of the pair
         const size_t o = (size_t)vb_.z * d.n3;
         const T *pbef = pbef_ + o, *paft = paft_ + o, *paa = paa_ + o, *pbb = pbb_ + o, *pcc = pcc_ + o;
@@ -488,6 +500,10 @@ struct S_fct_lim {
                 }
                 zup_s[sl][zpy[r]][zpx[r]] = u;
                 zdo_s[sl][zpy[r]][zpx[r]] = w;
+                if constexpr (AFT_LDS) {
+                    const int bx_ = zpx[r] - 1, by_ = zpy[r] - 1;
+                    if (bx_ >= 0 && bx_ < BX && by_ >= 0 && by_ < BY) aft_s[sl][by_][bx_] = zzw[r];
+                }
             }
         };
 
@@ -496,6 +512,7 @@ struct S_fct_lim {
         bool bon[NB], bin[NB];
         uint32_t bof[NB];
         T fa0[NB], fa1[NB], fb0[NB], fb1[NB], fc0[NB], fc1[NB], fzw[NB];
+        T fa1_prev[NB], fb1_prev[NB];   // paa, pbb of the level before
         double fe1e2t[NB], fe3t[NB];
     #pragma unroll
         for (int r = 0; r < NB; r++) {
@@ -507,8 +524,12 @@ struct S_fct_lim {
             bin[r] = bon[r] && pi >= 2 && pi <= jpi - 1 && pj >= 2 && pj <= jpj - 1;
             bof[r] = IJ32(min(max(pi, 2), jpi - 1), min(max(pj, 2), jpj - 1));
             fe1e2t[r] = bin[r] ? d.md.e1e2t[bof[r]] : 0.0;
+            fa1_prev[r] = fb1_prev[r] = T(0);
         }
-        auto beta_fetch = [&](int kk) {
+        // first: level kk is the first, so pcc(kk) is fetched too; after
+        // it, pcc(kk) is the pcc(kk+1) fetched for level kk-1 (levels up to
+        // jpk - 1 are fetched unclamped, in order)
+        auto beta_fetch = [&](int kk, bool first) {
             const int kc = min(kk, jpk - 1);
             const T *pa = plane_ptr(paa, d.n2, kc), *pb = plane_ptr(pbb, d.n2, kc), *pc = plane_ptr(pcc, d.n2, kc);
             const T *pc1 = plane_ptr(pcc, d.n2, kc + 1), *pz = plane_ptr(paft, d.n2, kc);
@@ -520,20 +541,27 @@ struct S_fct_lim {
                 fa1[r] = ldp(pa, q);
                 fb0[r] = ldp(pb, q - w);
                 fb1[r] = ldp(pb, q);
-                fc0[r] = ldp(pc, q);
+                fc0[r] = first ? ldp(pc, q) : fc1[r];
                 fc1[r] = ldp(pc1, q);
-                fzw[r] = ldp(pz, q);
+                if constexpr (!AFT_LDS) fzw[r] = ldp(pz, q);
                 fe3t[r] = E3(d.md.e3t, q, kc);
             }
         };
         // nonosc betas, lines 366-399, then the closed-ring lbc_lnk (zero off
         // the interior and at level jpk)
+        // also hands paa, pbb of level kk-1 and pcc of level kk to the
+        // limited divergence of level kk-1 (faa_s, fbb_s, fcc_s)
         auto beta_make = [&](int kk) {
             const int sb = kk & 1;
             const int sa = (kk > 1 ? kk - 1 : 1) % 3, sc = kk % 3, sn = (kk + 1) % 3;
     #pragma unroll
             for (int r = 0; r < NB; r++) {
                 if (!bon[r]) continue;
+                faa_s[bpy[r]][bpx[r]] = fa1_prev[r];
+                fbb_s[bpy[r]][bpx[r]] = fb1_prev[r];
+                fcc_s[bpy[r]][bpx[r]] = fc0[r];
+                fa1_prev[r] = fa1[r];
+                fb1_prev[r] = fb1[r];
                 T bu_t = T(0), bd_t = T(0);
                 if (bin[r] && kk <= jpk - 1) {
                     const int zx = bpx[r] + 1, zy = bpy[r] + 1;
@@ -558,7 +586,9 @@ struct S_fct_lim {
                     const double zneg = fmax2(0.0, a_i) - fmin2(0.0, a_im1) + fmax2(0.0, b_j) - fmin2(0.0, b_jm1) +
                                         fmax2(0.0, c_k) - fmin2(0.0, c_kp1);
                     const double zbt = fe1e2t[r] * fe3t[r] / p2dt;
-                    const double paft_k = fzw[r];
+                    double paft_k;
+                    if constexpr (AFT_LDS) paft_k = aft_s[sc][bpy[r]][bpx[r]];
+                    else paft_k = fzw[r];
                     const double bu = ((double)zup - paft_k) / (zpos + zrtrn) * zbt;
                     const double bd = (paft_k - (double)zdo) / (zneg + zrtrn) * zbt;
                     if (sizeof(T) == 4) {
@@ -578,7 +608,7 @@ struct S_fct_lim {
         int jr[NR];
         bool own[NR];
         uint32_t ijr[NR], fof[NR];
-        T ga0[NR], ga1[NR], gb0[NR], gb1[NR], gc1[NR], gta[NR], ge3t[NR], r1_e1e2t[NR];
+        T gta[NR], ge3t[NR], r1_e1e2t[NR];
         double lz_k[NR];
     #pragma unroll
         for (int r = 0; r < NR; r++) {
@@ -588,21 +618,14 @@ struct S_fct_lim {
             fof[r] = IJ32(min(i, jpi - 1), min(jr[r], jpj - 1));
             r1_e1e2t[r] = own[r] ? m.r1_e1e2t[ijr[r]] : T(0);
             lz_k[r] = own[r] ? (double)A3(pcc, i, jr[r], 1) : 0.0;
-            ga0[r] = ga1[r] = gb0[r] = gb1[r] = gc1[r] = gta[r] = ge3t[r] = T(1);
+            gta[r] = ge3t[r] = T(1);
         }
         auto fin_fetch = [&](int k) {
             const int kc = min(k, jpk - 1);
-            const T *pa = plane_ptr(paa, d.n2, kc), *pb = plane_ptr(pbb, d.n2, kc), *pc1 = plane_ptr(pcc, d.n2, kc + 1);
             const T *pt = plane_ptr((const T *)ta, d.n2, kc);
-            const uint32_t w = (uint32_t)jpi;
     #pragma unroll
             for (int r = 0; r < NR; r++) {
                 const uint32_t q = fof[r];
-                ga0[r] = ldp(pa, q - 1u);
-                ga1[r] = ldp(pa, q);
-                gb0[r] = ldp(pb, q - w);
-                gb1[r] = ldp(pb, q);
-                gc1[r] = ldp(pc1, q);
                 gta[r] = ldp(pt, q);
                 ge3t[r] = E3(m.e3t, q, kc);
             }
@@ -617,13 +640,17 @@ struct S_fct_lim {
                 const int j = jr[r], cy = ty + WY * r + 1;
                 const bool okym = j - 1 >= 2;
                 const double bu_c = bup_s[s0][cy][cx], bd_c = bdo_s[s0][cy][cx];
-                const double lx_i = lim_face<T>(ga1[r], bd_c, bup_s[s0][cy][cx + 1], bu_c, bdo_s[s0][cy][cx + 1]);
+                const double lx_i =
+                    lim_face<T>(faa_s[cy][cx], bd_c, bup_s[s0][cy][cx + 1], bu_c, bdo_s[s0][cy][cx + 1]);
                 const double lx_im1 =
-                    okxm ? lim_face<T>(ga0[r], bdo_s[s0][cy][cx - 1], bu_c, bup_s[s0][cy][cx - 1], bd_c) : 0.0;
-                const double ly_j = lim_face<T>(gb1[r], bd_c, bup_s[s0][cy + 1][cx], bu_c, bdo_s[s0][cy + 1][cx]);
+                    okxm ? lim_face<T>(faa_s[cy][cx - 1], bdo_s[s0][cy][cx - 1], bu_c, bup_s[s0][cy][cx - 1], bd_c)
+                         : 0.0;
+                const double ly_j =
+                    lim_face<T>(fbb_s[cy][cx], bd_c, bup_s[s0][cy + 1][cx], bu_c, bdo_s[s0][cy + 1][cx]);
                 const double ly_jm1 =
-                    okym ? lim_face<T>(gb0[r], bdo_s[s0][cy - 1][cx], bu_c, bup_s[s0][cy - 1][cx], bd_c) : 0.0;
-                const double lz_kp1 = lim_face<T>(gc1[r], bdo_s[s1][cy][cx], bu_c, bup_s[s1][cy][cx], bd_c);
+                    okym ? lim_face<T>(fbb_s[cy - 1][cx], bdo_s[s0][cy - 1][cx], bu_c, bup_s[s0][cy - 1][cx], bd_c)
+                         : 0.0;
+                const double lz_kp1 = lim_face<T>(fcc_s[cy][cx], bdo_s[s1][cy][cx], bu_c, bup_s[s1][cy][cx], bd_c);
                 A3(ta, i, j, k) = gta[r] - ((T)lx_i - (T)lx_im1 + (T)ly_j - (T)ly_jm1 + (T)lz_k[r] - (T)lz_kp1) *
                                                r1_e1e2t[r] / ge3t[r];
                 lz_k[r] = lz_kp1;
@@ -638,10 +665,10 @@ struct S_fct_lim {
         zb_store(2);
         zb_fetch(3);
         zb_store(3);
-        beta_fetch(1);
+        beta_fetch(1, true);
         __syncthreads();
         beta_make(1);
-        beta_fetch(2);
+        beta_fetch(2, false);
         fin_fetch(1);
         zb_fetch(4);
         __syncthreads();
@@ -651,7 +678,7 @@ struct S_fct_lim {
         // level k+3
         for (int k = 1; k <= jpk - 1; k++) {
             beta_make(k + 1);
-            beta_fetch(k + 2);
+            beta_fetch(k + 2, false);
             __syncthreads();
             fin_do(k);
             fin_fetch(k + 1);
