@@ -545,6 +545,11 @@ int nemo_hip_stream_copy(const void *src, void *dst, size_t bytes, int blocks);
                   solves of tra_zdf and dyn_zdf keep the forward-
                   substituted values on chip; 0 (default): written to the
                   trend array and read back.
+     "fct_vlim"   1 or 2: in the chain (fct_fused 0) on closed domains,
+                  the vertical upstream and antidiffusive fluxes, the first
+                  guess and the limiter in one bottom-up march per tile and
+                  tracer (256 threads per 64 x 4 tile (1) or 512 per 64 x 8
+                  (2)); 0 (default): as the kernels of the chain.
    Unknown keys and values are errors. */
 int nemo_hip_tune_set(const char *key, int value);
 int nemo_hip_tune_get(const char *key, int *value);

@@ -21,7 +21,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnemo_hip.so")
-SOURCES = ["api.hip", "dom_lbc.hip", "tra.hip", "dyn.hip", "spg.hip", "nxt.hip", "phys.hip", "slp.hip", "adj.hip", "gsum.hip", "fct_fused.hip"]
+SOURCES = ["api.hip", "dom_lbc.hip", "tra.hip", "dyn.hip", "spg.hip", "nxt.hip", "phys.hip", "slp.hip", "adj.hip", "gsum.hip", "fct_fused.hip", "fct_vlim.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-mllvm", "-pragma-unroll-threshold=500000",

@@ -130,9 +130,9 @@ int plan_add(PlanRec &&r)
     return 0;
 }
 // tuning switches, process-wide; names and defaults
-static const char *g_tune_names[T_NTUNE] = {"fct_fused", "spg_persistent", "zdf_reg"};
-static int g_tune[T_NTUNE] = {0, 0, 0};
-static const int g_tune_max[T_NTUNE] = {2, 2, 1};
+static const char *g_tune_names[T_NTUNE] = {"fct_fused", "spg_persistent", "zdf_reg", "fct_vlim"};
+static int g_tune[T_NTUNE] = {0, 0, 0, 0};
+static const int g_tune_max[T_NTUNE] = {2, 2, 1, 2};
 int tune(Tune which) { return g_tune[which]; }
 int *spg_rot_u() { return g_c->spg_rot_u; }
 int *spg_rot_s() { return g_c->spg_rot_s; }

@@ -280,7 +280,7 @@ unsigned *spg_barrier();       // dyn_spg_ts's grid barrier (2 words) and its st
 void spg_loop_launched();      // a cooperative phase 2 was queued: check its status at the next sync
 bool have_domain();
 // tuning switches (nemo_hip_tune_set): kernel variants and tile shapes
-enum Tune { T_FCT_FUSED, T_SPG_PERSISTENT, T_ZDF_REG, T_NTUNE };
+enum Tune { T_FCT_FUSED, T_SPG_PERSISTENT, T_ZDF_REG, T_FCT_VLIM, T_NTUNE };
 int tune(Tune which);
 // per-routine timing: when this launch is timed, returns true with the two
 // events its dispatch packet records at kernel start and end

@@ -97,6 +97,20 @@ __device__ __forceinline__ double lim_face(double flux, double bdo_here, double 
 
 // tra_adv_fct of nt (1 or 2) tracers on a closed domain in one k-marched
 // kernel per tile and tracer (fct_fused.hip)
+// the compact scheme's forward sweep of nt (1 or 2) tracers into pzo
+// (fct_fused.hip)
+template <typename T>
+int launch_fct_cpt_fwd(const DevDomain &d, const PivotTab<T> &pt, int nt, const T *ptn, T *pzo, const char *nm);
+
+// the vertical fluxes, first guess and limiter of nt (1 or 2) tracers on a
+// closed domain in one bottom-up march (fct_vlim.hip); ps: the horizontal
+// upstream sums of k_fct_h (VERT = 0), pzo: the forward sweep (order 4) or
+// tn (order 2), paa / pbb: zwx / zwy after their lbc_lnk
+template <int TRP, typename T>
+int launch_fct_vlim(const DevDomain &d, double p2dt, int nt, int kn_fct_v, int ta0,
+                    const T *pw, const T *ptb, const T *ps, const T *pzo, const T *paa, const T *pbb, T *pta,
+                    const char *nm);
+
 template <int TRP, typename T>
 int launch_fct_fused(const DevDomain &d, const PivotTab<T> &pt, double p2dt, int nt, const T *pu, const T *pv,
                      const T *pw, const T *ptb, const T *ptn, T *pta, int kn_fct_h, int kn_fct_v, int ta0, T *pzo,

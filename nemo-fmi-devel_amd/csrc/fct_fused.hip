@@ -589,6 +589,19 @@ static int fu_launch(const DevDomain &d, const PivotTab<T> &pt, double p2dt, int
     return 0;
 }
 
+template <typename T>
+int launch_fct_cpt_fwd(const DevDomain &d, const PivotTab<T> &pt, int nt, const T *ptn, T *pzo, const char *nm)
+{
+    Grid2 gi = grid2(d.jpi - 2, d.jpj - 2);
+    if (nt == 2) LAUNCHB(nm, (k_fct_cpt_fwd<T, 2>), (S_fct_cpt_fwd<T, 2>), gi, d, pt, ptn, pzo);
+    else LAUNCHB(nm, (k_fct_cpt_fwd<T, 1>), (S_fct_cpt_fwd<T, 1>), gi, d, pt, ptn, pzo);
+    return 0;
+}
+template int launch_fct_cpt_fwd<double>(const DevDomain &, const PivotTab<double> &, int, const double *, double *,
+                                        const char *);
+template int launch_fct_cpt_fwd<float>(const DevDomain &, const PivotTab<float> &, int, const float *, float *,
+                                       const char *);
+
 // tra_adv_fct of nt (1 or 2) tracers on a closed domain; pzo: scratch for
 // the pre-pass of the order-4 vertical scheme (nt arrays of n3)
 template <int TRP, typename T>

@@ -1721,8 +1721,12 @@ template int launch_tra_zdf_imp<float>(double, int, const float *, float *, int,
 // lines 123-145, 180-221) and, on the interior, the low-order trend and the
 // first guess zwi (158-170)
 // NT: the number of tracers of the launch, a template constant so that the
-// loads of both tracers of a pair are in flight together
-template <int TRP, typename T, int NH4, int TA0, int NT>
+// loads of both tracers of a pair are in flight together.  VERT = 0 leaves
+// the vertical fluxes to k_fct_vlim: the kernel then stores, in zwi's place,
+// the horizontal part s = hu - hum + hv - hvm of the upstream-flux sum (the
+// leading terms of the sum ztra is formed from, in the same order) and
+// leaves ta alone.
+template <int TRP, typename T, int NH4, int TA0, int NT, int VERT = 1>
 struct S_fct_h {
     static constexpr int NTHR = NH_BLOCK, WAVES = 1;
     __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, double p2dt, const T *__restrict__ pu, const T *__restrict__ pv, const T *__restrict__ pw, const T *__restrict__ ptb, const T *__restrict__ ptn, T *__restrict__ pta, T *__restrict__ pzwi, T *__restrict__ pzwx, T *__restrict__ pzwy, int kjpt)
@@ -1764,6 +1768,10 @@ struct S_fct_h {
             if (!inner) continue;
             const T hum = T(0.5) * ((um + fabs(um)) * A3(tb, i - 1, j, k) + (um - fabs(um)) * tbc);
             const T hvm = T(0.5) * ((vm + fabs(vm)) * A3(tb, i, j - 1, k) + (vm - fabs(vm)) * tbc);
+            if (!VERT) {
+                A3(pzwi + t * n3, i, j, k) = hu - hum + hv - hvm;
+                continue;
+            }
             const T ztra = -((hu - hum + hv - hvm + tp_upz<TRP>(d, pw, tb, i, j, k) - tp_upz<TRP>(d, pw, tb, i, j, k + 1)) *
                              r1_e1e2t);
             T *ta = pta + t * n3, *zwi = pzwi + t * n3;
@@ -1774,12 +1782,12 @@ struct S_fct_h {
         }
     }
 };
-template <int TRP, typename T, int NH4, int TA0, int NT>
+template <int TRP, typename T, int NH4, int TA0, int NT, int VERT = 1>
 KERNEL void k_fct_h(DevDomain d, double p2dt, const T *__restrict__ pu, const T *__restrict__ pv,
                     const T *__restrict__ pw, const T *__restrict__ ptb, const T *__restrict__ ptn, T *__restrict__ pta,
                     T *__restrict__ pzwi, T *__restrict__ pzwx, T *__restrict__ pzwy, int kjpt)
 {
-    S_fct_h<TRP, T, NH4, TA0, NT>::run(vb_hw(), d, p2dt, pu, pv, pw, ptb, ptn, pta, pzwi, pzwx, pzwy, kjpt);
+    S_fct_h<TRP, T, NH4, TA0, NT, VERT>::run(vb_hw(), d, p2dt, pu, pv, pw, ptb, ptn, pta, pzwi, pzwx, pzwy, kjpt);
 }
 
 // per interior column: the vertical antidiffusive flux zwz (lines 255-278),
@@ -1897,6 +1905,27 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
         }
         Grid2 g3 = grid2(d.jpi - 1, d.jpj - 1);
         g3.grid.z = (unsigned)(d.jpk - 1);
+        if (!d.iper && !d.jper && nh::tune(nh::T_FCT_VLIM)) {
+            // closed domain: horizontal fluxes, then the compact scheme's
+            // forward sweep, then the vertical fluxes, first guess and limiter
+            // in one bottom-up march (zwi holds s, zwz the forward sweep)
+#define FH_LAUNCH2(H, Z, N) \
+    LAUNCHB(nm, (k_fct_h<TRP, T, H, Z, N, 0>), (S_fct_h<TRP, T, H, Z, N, 0>), g3, d, p2dt, pu, pv, pw, ptb + o, ptn + o, pta + o, zwi, zwx, zwy, nt)
+#define FH_LAUNCH(H, Z) \
+    do { if (nt == 2) FH_LAUNCH2(H, Z, 2); else FH_LAUNCH2(H, Z, 1); } while (0)
+            if (kn_fct_h == 4) FH_LAUNCH(1, 1);
+            else FH_LAUNCH(0, 1);
+#undef FH_LAUNCH
+#undef FH_LAUNCH2
+            if (kn_fct_v == 4 && (st = launch_fct_cpt_fwd<T>(d, pt, nt, ptn + o, zwz, nm))) return st;
+            const int nk = nt * d.jpk;
+            if ((st = launch_lbc<T>(zwx, nk, 'U', T(0)))) return st;
+            if ((st = launch_lbc<T>(zwy, nk, 'V', T(0)))) return st;
+            if ((st = launch_fct_vlim<TRP, T>(d, p2dt, nt, kn_fct_v, ta0, pw, ptb + o, zwi,
+                                              kn_fct_v == 4 ? zwz : ptn + o, zwx, zwy, pta + o, nm)))
+                return st;
+            continue;
+        }
 #define FH_LAUNCH2(H, Z, N) \
     LAUNCHB(nm, (k_fct_h<TRP, T, H, Z, N>), (S_fct_h<TRP, T, H, Z, N>), g3, d, p2dt, pu, pv, pw, ptb + o, ptn + o, pta + o, zwi, zwx, zwy, nt)
 #define FH_LAUNCH(H, Z) \

@@ -5,8 +5,9 @@ variants that give the same bits: the FCT as a chain of kernels or as one
 k-marched kernel per tile with 256 or 512 threads per 64 x TY tile
 (fct_fused 0 / 1 / 2), the barotropic sub-steps as two launches per
 sub-step or one cooperative launch with 1024- or 256-thread workgroups
-(spg_persistent 0 / 1 / 2), and the column solves through the trend array or
-on chip (zdf_reg 0 / 1).  Which is fastest depends on the domain (its size,
+(spg_persistent 0 / 1 / 2), the column solves through the trend array or
+on chip (zdf_reg 0 / 1), and the chain's vertical half as kernels or as one
+bottom-up march with 256 or 512 threads per tile (fct_vlim 0 / 1 / 2).  Which is fastest depends on the domain (its size,
 level count and storage form), so tune() times a caller's own step with each
 candidate, by coordinate descent over the switches (each switch's values
 with the others at their best so far, repeated until nothing changes), and
@@ -23,6 +24,7 @@ SWITCHES = {
     "fct_fused": (0, 1, 2),
     "spg_persistent": (0, 1, 2),
     "zdf_reg": (0, 1),
+    "fct_vlim": (0, 1, 2),
 }
 
 

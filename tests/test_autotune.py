@@ -34,17 +34,18 @@ def run(cost, monkeypatch):
 
 
 def test_descent_finds_the_separable_optimum(monkeypatch):
-    # fastest: fct_fused 2, spg_persistent 0, zdf_reg 1
-    cost = lambda s: 10 + [3, 2, 1][s["fct_fused"]] + [0, 5, 9][s["spg_persistent"]] + [1, 0][s["zdf_reg"]]
+    # fastest: fct_fused 2, spg_persistent 0, zdf_reg 1, fct_vlim 1
+    cost = lambda s: (10 + [3, 2, 1][s["fct_fused"]] + [0, 5, 9][s["spg_persistent"]] + [1, 0][s["zdf_reg"]]
+                      + [2, 0, 4][s["fct_vlim"]])
     h, (best, ms, trials) = run(cost, monkeypatch)
-    assert best == {"fct_fused": 2, "spg_persistent": 0, "zdf_reg": 1}
+    assert best == {"fct_fused": 2, "spg_persistent": 0, "zdf_reg": 1, "fct_vlim": 1}
     assert abs(ms - 11.0) < 1e-9
     assert h.sw == best                      # the best setting is left applied
-    assert len(trials) >= 1 + 2 + 2 + 1
+    assert len(trials) >= 1 + 2 + 2 + 1 + 2
 
 
 def test_descent_keeps_the_start_when_it_is_best(monkeypatch):
-    cost = lambda s: 1.0 + s["fct_fused"] + s["spg_persistent"] + s["zdf_reg"]
+    cost = lambda s: 1.0 + s["fct_fused"] + s["spg_persistent"] + s["zdf_reg"] + s["fct_vlim"]
     h, (best, ms, trials) = run(cost, monkeypatch)
     assert best == {k: 0 for k in autotune.SWITCHES} and abs(ms - 1.0) < 1e-9
 

@@ -102,13 +102,17 @@ def test_ensemble_graph_replays_match_eager_calls(hip):
 
 
 @pytest.mark.gpu
-def test_ensemble_plan_replays_match_eager_calls(hip):
+@pytest.mark.parametrize("vlim", [0, 1])
+def test_ensemble_plan_replays_match_eager_calls(hip, vlim):
     """a plan of the fused FP32 step over closed and cyclic members: each
     position of the closed members' launch sequence is one batched launch,
     the cyclic member forms a group of its own; plan launches, eager or
-    recorded into a graph, equal the calls issued one by one"""
+    recorded into a graph, equal the calls issued one by one (with the FCT
+    chain's vertical half as kernels or as one march, switch fct_vlim)"""
     members = MEMBERS + [(44, 26, 9, 0, 8)]
     ctxs, eager, batched, plan, g = [], [], [], None, None
+    vlim0 = hip.tune_get("fct_vlim")
+    hip.tune_set("fct_vlim", vlim)
     try:
         for (jpi, jpj, jpk, jperio, seed) in members:
             c = hip.context_create()
@@ -159,6 +163,7 @@ def test_ensemble_plan_replays_match_eager_calls(hip):
                     assert np.array_equal(sa[k], sb[k]), f"member {m} field {k} differs after the plan at kt {kt}"
         print(json.dumps({"plan": info}))
     finally:
+        hip.tune_set("fct_vlim", vlim0)
         if g is not None:
             hip.graph_free(g)
         if plan is not None:

@@ -8,7 +8,10 @@ tile and tracer) against the oracle and against the flux / limiter chain.
 * both storage forms: the fused kernel, with 256 or 512 threads per tile
   (tuning switch fct_fused = 1, 2), equals the chain (fct_fused = 0) bit
   for bit, for the drop-in routine (trend read) and for the fused step
-  (trend from zero, transports formed where used).
+  (trend from zero, transports formed where used); so do the chain's
+  forms, its vertical half as the kernels k_fct_v, k_fct_lim (fct_vlim =
+  0) or in one bottom-up march with 256 or 512 threads per tile
+  (fct_vlim = 1, 2).
 """
 import numpy as np
 import pytest
@@ -46,13 +49,19 @@ def setup(hip, name, kjpt=2, seed=5):
     return d, S, o, z, tb, tn, ta
 
 
-def with_mode(hip, mode, fn):
-    default = hip.tune_get("fct_fused")
+def with_switches(hip, sw, fn):
+    defaults = {k: hip.tune_get(k) for k in sw}
     try:
-        hip.tune_set("fct_fused", mode)
+        for k, v in sw.items():
+            hip.tune_set(k, v)
         return fn()
     finally:
-        hip.tune_set("fct_fused", default)
+        for k, v in defaults.items():
+            hip.tune_set(k, v)
+
+
+def with_mode(hip, mode, fn):
+    return with_switches(hip, {"fct_fused": mode}, fn)
 
 
 @pytest.mark.gpu
@@ -76,6 +85,28 @@ def test_fused_fct_fp64_oracle(hip, name, orders, mode):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("name", list(BASINS))
+@pytest.mark.parametrize("orders", ORDERS)
+@pytest.mark.parametrize("vlim", [1, 2])
+def test_chain_march_fp64_oracle(hip, name, orders, vlim):
+    """the chain with its vertical half in one bottom-up march (fct_vlim 1,
+    2), FP64 drop-in tra_adv_fct, against the oracle bit for bit"""
+    d, S, o, z, tb, tn, ta = setup(hip, name)
+    p2dt = 2.0 * d.opts["rdt"]
+    want = ta.copy(order="F")
+    o.tra_adv_fct(3, 1, "TRA", p2dt, *z, tb, tn, want, 2, *orders)
+
+    def run():
+        got = dev(ta)
+        hip.tra_adv_fct(3, 1, "TRA", p2dt, *[dev(a) for a in z], dev(tb), dev(tn), got, 2, *orders)
+        sync(hip)
+        return host(got, want.shape)
+
+    g = with_switches(hip, {"fct_fused": 0, "fct_vlim": vlim}, run)
+    assert np.array_equal(g, want), f"{name} {orders} vlim {vlim}: max |diff| {np.abs(g - want).max():.3e}"
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("kjpt", [1, 3])
 def test_fused_fct_tracer_counts(hip, kjpt):
     d, S, o, z, tb, tn, ta = setup(hip, "odd", kjpt=kjpt)
@@ -90,11 +121,16 @@ def test_fused_fct_tracer_counts(hip, kjpt):
 
 def both_paths(hip, fn):
     """fn() with the fused kernel (both workgroup sizes, which must agree),
-    then with the chain; returns the fused and the chain results"""
+    then with the chain in its two forms (which must agree); returns the
+    fused and the chain results"""
     a = with_mode(hip, 1, fn)
     a2 = with_mode(hip, 2, fn)
     assert np.array_equal(a, a2, equal_nan=True), "the fused kernel's two workgroup sizes differ"
-    b = with_mode(hip, 0, fn)
+    b = with_switches(hip, {"fct_fused": 0, "fct_vlim": 0}, fn)
+    for v in (1, 2):
+        bv = with_switches(hip, {"fct_fused": 0, "fct_vlim": v}, fn)
+        assert np.array_equal(b, bv, equal_nan=True), \
+            f"the chain's march form {v} differs: max |diff| {np.nanmax(np.abs(b - bv)):.3e}"
     return a, b
 
 
