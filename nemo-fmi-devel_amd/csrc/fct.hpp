@@ -46,9 +46,36 @@ __device__ __forceinline__ T cpt_piv(const PivotTab<T> &pt, int k, int ikb)
     return (k >= 3 && k < ikb) ? pt.p[k] : T(1);
 }
 
-// the data-independent pivot table (tra.hip)
+// The pivots of interp_4th_cpt's wet rows do not depend on the data: rows
+// 3.. of a wet column follow p(k) = 4 - 1/p(k-1) from p(3) = 4 (the
+// forward elimination of traadv_fct.F90:577-595 with wmask 1), every other
+// row has pivot 1.  The table is evaluated at compile time (IEEE
+// arithmetic, the operations of the elimination in its order) into constant
+// memory, which the kernels read with a level index.
 template <typename T>
-PivotTab<T> cpt_pivots(int jpk);
+constexpr PivotTab<T> cpt_table()
+{
+    PivotTab<T> pt{};
+    for (int k = 0; k < STEP_MAXK + 2; k++) pt.p[k] = T(1);
+    T zws_prev = T(0), zwt_prev = T(1);
+    for (int k = 3; k <= STEP_MAXK; k++) {
+        const T wm = T(1);
+        const T zwd = T(3) * wm + T(1);
+        const T zwt_k = zwd - wm * zws_prev / zwt_prev;
+        pt.p[k] = zwt_k;
+        zws_prev = wm;
+        zwt_prev = zwt_k;
+    }
+    return pt;
+}
+static __constant__ PivotTab<float> g_cpt_f = cpt_table<float>();
+static __constant__ PivotTab<double> g_cpt_d = cpt_table<double>();
+template <typename T>
+__device__ __forceinline__ const PivotTab<T> &cpt_tab()
+{
+    if constexpr (sizeof(T) == 4) return g_cpt_f;
+    else return g_cpt_d;
+}
 
 // transports: TRP = 0 reads the arrays of tra_adv (pun, pvn, pwn, zero at
 // jpk); TRP = 1 forms them from un, vn, wn where used (traadv.F90:100-124)
@@ -100,7 +127,7 @@ __device__ __forceinline__ double lim_face(double flux, double bdo_here, double 
 // the compact scheme's forward sweep of nt (1 or 2) tracers into pzo
 // (fct_fused.hip)
 template <typename T>
-int launch_fct_cpt_fwd(const DevDomain &d, const PivotTab<T> &pt, int nt, const T *ptn, T *pzo, const char *nm);
+int launch_fct_cpt_fwd(const DevDomain &d, int nt, const T *ptn, T *pzo, const char *nm);
 
 // the vertical fluxes, first guess and limiter of nt (1 or 2) tracers on a
 // closed domain in one bottom-up march (fct_vlim.hip); ps: the horizontal
@@ -112,6 +139,6 @@ int launch_fct_vlim(const DevDomain &d, double p2dt, int nt, int kn_fct_v, int t
                     const char *nm);
 
 template <int TRP, typename T>
-int launch_fct_fused(const DevDomain &d, const PivotTab<T> &pt, double p2dt, int nt, const T *pu, const T *pv,
+int launch_fct_fused(const DevDomain &d, double p2dt, int nt, const T *pu, const T *pv,
                      const T *pw, const T *ptb, const T *ptn, T *pta, int kn_fct_h, int kn_fct_v, int ta0, T *pzo,
                      const char *nm);

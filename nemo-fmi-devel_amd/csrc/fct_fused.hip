@@ -80,7 +80,7 @@ __device__ __forceinline__ void fu_xy(int q, int &x, int &y)
 template <typename T, int NT>
 struct S_fct_cpt_fwd {
     static constexpr int NTHR = NH_BLOCK, WAVES = 1;
-    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, PivotTab<T> pt, const T *__restrict__ ptn, T *__restrict__ pzo)
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, const T *__restrict__ ptn, T *__restrict__ pzo)
     {
         THREAD_IJ_V(vb_, 2, 2);
         if (i > d.jpi - 1 || j > d.jpj - 1) return;
@@ -104,7 +104,7 @@ struct S_fct_cpt_fwd {
                     zwrm = T(3) * wm * (tnk + tn_km1);
                 }
                 if (k == 2) zout = zwrm;
-                else zout = zwrm - zwi_c / cpt_piv(pt, k - 1, ikb) * zout_prev;
+                else zout = zwrm - zwi_c / cpt_piv(cpt_tab<T>(), k - 1, ikb) * zout_prev;
                 A3(zo, i, j, k) = zout;
                 zout_prev = zout;
                 tn_km1 = tnk;
@@ -113,9 +113,9 @@ struct S_fct_cpt_fwd {
     }
 };
 template <typename T, int NT>
-KERNEL void k_fct_cpt_fwd(DevDomain d, PivotTab<T> pt, const T *__restrict__ ptn, T *__restrict__ pzo)
+KERNEL void k_fct_cpt_fwd(DevDomain d, const T *__restrict__ ptn, T *__restrict__ pzo)
 {
-    S_fct_cpt_fwd<T, NT>::run(vb_hw(), d, pt, ptn, pzo);
+    S_fct_cpt_fwd<T, NT>::run(vb_hw(), d, ptn, pzo);
 }
 
 // TRP: transports formed from un, vn, wn (1) or read (0); NH4 / NV4: order 4
@@ -124,7 +124,7 @@ KERNEL void k_fct_cpt_fwd(DevDomain d, PivotTab<T> pt, const T *__restrict__ ptn
 template <int TRP, typename T, int NH4, int NV4, int TA0, int TY, int NTH>
 struct S_fct_fused {
     static constexpr int NTHR = NTH, WAVES = FuWaves<T, NTH>::value;
-    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, PivotTab<T> pt, double p2dt, int ntr, const T *__restrict__ pu, const T *__restrict__ pv, const T *__restrict__ pw, const T *__restrict__ ptb_, const T *__restrict__ ptn_, const T *__restrict__ pzo_, T *__restrict__ pta_)
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, double p2dt, int ntr, const T *__restrict__ pu, const T *__restrict__ pv, const T *__restrict__ pw, const T *__restrict__ ptb_, const T *__restrict__ ptn_, const T *__restrict__ pzo_, T *__restrict__ pta_)
     {
         using ZB = typename std::conditional<sizeof(T) == 4, float, double>::type;
         constexpr int TX = FU_TX;
@@ -434,10 +434,10 @@ struct S_fct_fused {
                             const int kb = ikb[r];
                             T x;
                             if (it == jpk - 1) {
-                                x = zoc[r] / cpt_piv(pt, it, kb);
+                                x = zoc[r] / cpt_piv(cpt_tab<T>(), it, kb);
                             } else {
                                 const T zws = (it == 2 || it == kb) ? T(0) : (wmk(wt[r], it) ? T(1) : T(0));
-                                x = (zoc[r] - zws * xcar[r]) / cpt_piv(pt, it, kb);
+                                x = (zoc[r] - zws * xcar[r]) / cpt_piv(cpt_tab<T>(), it, kb);
                             }
                             xcar[r] = x;
                             if (inI[r]) f = (wc[r] * x - upzt) * (wmk(wt[r], it) ? T(1) : T(0));
@@ -561,16 +561,16 @@ struct S_fct_fused {
 };
 template <int TRP, typename T, int NH4, int NV4, int TA0, int TY, int NTH>
 __global__ __launch_bounds__(NTH, (FuWaves<T, NTH>::value)) void k_fct_fused(
-    DevDomain d, PivotTab<T> pt, double p2dt, int ntr, const T *__restrict__ pu, const T *__restrict__ pv,
+    DevDomain d, double p2dt, int ntr, const T *__restrict__ pu, const T *__restrict__ pv,
     const T *__restrict__ pw, const T *__restrict__ ptb_, const T *__restrict__ ptn_, const T *__restrict__ pzo_,
     T *__restrict__ pta_)
 {
-    S_fct_fused<TRP, T, NH4, NV4, TA0, TY, NTH>::run(vb_hw(), d, pt, p2dt, ntr, pu, pv, pw, ptb_, ptn_, pzo_, pta_);
+    S_fct_fused<TRP, T, NH4, NV4, TA0, TY, NTH>::run(vb_hw(), d, p2dt, ntr, pu, pv, pw, ptb_, ptn_, pzo_, pta_);
 }
 
 // fct_fused switch value 1: 256 threads per workgroup; 2: 512
 template <int TRP, typename T, int NH4, int NV4, int TA0>
-static int fu_launch(const DevDomain &d, const PivotTab<T> &pt, double p2dt, int nt, const T *pu, const T *pv,
+static int fu_launch(const DevDomain &d, double p2dt, int nt, const T *pu, const T *pv,
                      const T *pw, const T *ptb, const T *ptn, const T *pzo, T *pta, const char *nm)
 {
     constexpr int TY = FuTY<T>::value;
@@ -579,51 +579,51 @@ static int fu_launch(const DevDomain &d, const PivotTab<T> &pt, double p2dt, int
     g.grid = dim3((unsigned)(ntx * nty * nt), 1, 1);
     if (nh::tune(nh::T_FCT_FUSED) == 2) {
         g.block = dim3(512, 1, 1);
-        LAUNCHB(nm, (k_fct_fused<TRP, T, NH4, NV4, TA0, TY, 512>), (S_fct_fused<TRP, T, NH4, NV4, TA0, TY, 512>), g, d, pt, p2dt, nt, pu, pv, pw, ptb, ptn, pzo,
+        LAUNCHB(nm, (k_fct_fused<TRP, T, NH4, NV4, TA0, TY, 512>), (S_fct_fused<TRP, T, NH4, NV4, TA0, TY, 512>), g, d, p2dt, nt, pu, pv, pw, ptb, ptn, pzo,
                pta);
     } else {
         g.block = dim3(256, 1, 1);
-        LAUNCHB(nm, (k_fct_fused<TRP, T, NH4, NV4, TA0, TY, 256>), (S_fct_fused<TRP, T, NH4, NV4, TA0, TY, 256>), g, d, pt, p2dt, nt, pu, pv, pw, ptb, ptn, pzo,
+        LAUNCHB(nm, (k_fct_fused<TRP, T, NH4, NV4, TA0, TY, 256>), (S_fct_fused<TRP, T, NH4, NV4, TA0, TY, 256>), g, d, p2dt, nt, pu, pv, pw, ptb, ptn, pzo,
                pta);
     }
     return 0;
 }
 
 template <typename T>
-int launch_fct_cpt_fwd(const DevDomain &d, const PivotTab<T> &pt, int nt, const T *ptn, T *pzo, const char *nm)
+int launch_fct_cpt_fwd(const DevDomain &d, int nt, const T *ptn, T *pzo, const char *nm)
 {
     Grid2 gi = grid2(d.jpi - 2, d.jpj - 2);
-    if (nt == 2) LAUNCHB(nm, (k_fct_cpt_fwd<T, 2>), (S_fct_cpt_fwd<T, 2>), gi, d, pt, ptn, pzo);
-    else LAUNCHB(nm, (k_fct_cpt_fwd<T, 1>), (S_fct_cpt_fwd<T, 1>), gi, d, pt, ptn, pzo);
+    if (nt == 2) LAUNCHB(nm, (k_fct_cpt_fwd<T, 2>), (S_fct_cpt_fwd<T, 2>), gi, d, ptn, pzo);
+    else LAUNCHB(nm, (k_fct_cpt_fwd<T, 1>), (S_fct_cpt_fwd<T, 1>), gi, d, ptn, pzo);
     return 0;
 }
-template int launch_fct_cpt_fwd<double>(const DevDomain &, const PivotTab<double> &, int, const double *, double *,
+template int launch_fct_cpt_fwd<double>(const DevDomain &, int, const double *, double *,
                                         const char *);
-template int launch_fct_cpt_fwd<float>(const DevDomain &, const PivotTab<float> &, int, const float *, float *,
+template int launch_fct_cpt_fwd<float>(const DevDomain &, int, const float *, float *,
                                        const char *);
 
 // tra_adv_fct of nt (1 or 2) tracers on a closed domain; pzo: scratch for
 // the pre-pass of the order-4 vertical scheme (nt arrays of n3)
 template <int TRP, typename T>
-int launch_fct_fused(const DevDomain &d, const PivotTab<T> &pt, double p2dt, int nt, const T *pu, const T *pv,
+int launch_fct_fused(const DevDomain &d, double p2dt, int nt, const T *pu, const T *pv,
                      const T *pw, const T *ptb, const T *ptn, T *pta, int kn_fct_h, int kn_fct_v, int ta0, T *pzo,
                      const char *nm)
 {
     if (kn_fct_v == 4) {
         Grid2 gi = grid2(d.jpi - 2, d.jpj - 2);
-        if (nt == 2) LAUNCHB(nm, (k_fct_cpt_fwd<T, 2>), (S_fct_cpt_fwd<T, 2>), gi, d, pt, ptn, pzo);
-        else LAUNCHB(nm, (k_fct_cpt_fwd<T, 1>), (S_fct_cpt_fwd<T, 1>), gi, d, pt, ptn, pzo);
+        if (nt == 2) LAUNCHB(nm, (k_fct_cpt_fwd<T, 2>), (S_fct_cpt_fwd<T, 2>), gi, d, ptn, pzo);
+        else LAUNCHB(nm, (k_fct_cpt_fwd<T, 1>), (S_fct_cpt_fwd<T, 1>), gi, d, ptn, pzo);
     }
 #define FU_CASE(H, V, Z) \
     if (kn_fct_h == (H ? 4 : 2) && kn_fct_v == (V ? 4 : 2) && !!ta0 == !!Z) \
-        return fu_launch<TRP, T, H, V, Z>(d, pt, p2dt, nt, pu, pv, pw, ptb, ptn, pzo, pta, nm);
+        return fu_launch<TRP, T, H, V, Z>(d, p2dt, nt, pu, pv, pw, ptb, ptn, pzo, pta, nm);
     FU_CASE(1, 1, 1) FU_CASE(1, 1, 0) FU_CASE(1, 0, 1) FU_CASE(1, 0, 0)
     FU_CASE(0, 1, 1) FU_CASE(0, 1, 0) FU_CASE(0, 0, 1) FU_CASE(0, 0, 0)
 #undef FU_CASE
     return nh::set_error(NEMO_HIP_ERR_ARG, "fct: orders %d/%d", kn_fct_h, kn_fct_v);
 }
 #define FU_INST(TRP, T)                                                                                           \
-    template int launch_fct_fused<TRP, T>(const DevDomain &, const PivotTab<T> &, double, int, const T *,         \
+    template int launch_fct_fused<TRP, T>(const DevDomain &, double, int, const T *,                             \
                                           const T *, const T *, const T *, const T *, T *, int, int, int, T *,    \
                                           const char *);
 FU_INST(0, double)

@@ -29,35 +29,6 @@
 
 namespace {
 
-// the pivots of interp_4th_cpt's wet rows, as cpt_pivots forms them on the
-// host (rows 3.. follow p(k) = 4 - 1/p(k-1) from p(3) = 4; the rest are 1),
-// evaluated at compile time into constant memory: the march reads them
-// with a level index, which a table passed as a kernel argument would have
-// to serve from the argument segment
-template <typename T>
-constexpr PivotTab<T> cpt_table()
-{
-    PivotTab<T> pt{};
-    for (int k = 0; k < STEP_MAXK + 2; k++) pt.p[k] = T(1);
-    T zws_prev = T(0), zwt_prev = T(1);
-    for (int k = 3; k <= STEP_MAXK; k++) {
-        const T wm = T(1);
-        const T zwd = T(3) * wm + T(1);
-        const T zwt_k = zwd - wm * zws_prev / zwt_prev;
-        pt.p[k] = zwt_k;
-        zws_prev = wm;
-        zwt_prev = zwt_k;
-    }
-    return pt;
-}
-__constant__ PivotTab<float> g_cpt_f = cpt_table<float>();
-__constant__ PivotTab<double> g_cpt_d = cpt_table<double>();
-template <typename T>
-__device__ __forceinline__ const PivotTab<T> &cpt_tab()
-{
-    if constexpr (sizeof(T) == 4) return g_cpt_f;
-    else return g_cpt_d;
-}
 
 // NTH threads per 64 x TY tile (TY a multiple of NTH / 64)
 template <int TRP, typename T, int NV4, int TA0, int TY, int NTH>

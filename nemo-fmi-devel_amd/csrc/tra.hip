@@ -1713,7 +1713,7 @@ template int launch_tra_zdf_imp<float>(double, int, const float *, float *, int,
 // of separate calls bit for bit in either storage form.  The pivots of
 // interp_4th_cpt do not depend on the data: rows 3..mbkt-1 follow
 // p(k) = 4 - 1/p(k-1) from p(3) = 4, every other row has pivot 1, so they
-// come from a host table.
+// come from a table evaluated at compile time (fct.hpp).
 // ===========================================================================
 
 // per cell (one thread per (i,j,k), k = blockIdx.z + 1): the horizontal
@@ -1795,7 +1795,7 @@ KERNEL void k_fct_h(DevDomain d, double p2dt, const T *__restrict__ pu, const T 
 template <int TRP, typename T, int NV4, int NT>
 struct S_fct_v {
     static constexpr int NTHR = NH_BLOCK, WAVES = 1;
-    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, PivotTab<T> pt, const T *__restrict__ pw, const T *__restrict__ ptb, const T *__restrict__ ptn, T *__restrict__ pzwz, int kjpt)
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, const T *__restrict__ pw, const T *__restrict__ ptb, const T *__restrict__ ptn, T *__restrict__ pzwz, int kjpt)
     {
         (void)kjpt;
         THREAD_IJ_V(vb_, 2, 2);
@@ -1830,18 +1830,18 @@ struct S_fct_v {
                     zwrm = T(3) * wm * (tnk + tn_km1);
                 }
                 if (k == 2) zout = zwrm;
-                else zout = zwrm - zwi_c / cpt_piv(pt, k - 1, ikb) * zout_prev;
+                else zout = zwrm - zwi_c / cpt_piv(cpt_tab<T>(), k - 1, ikb) * zout_prev;
                 A3(zwz, i, j, k) = zout;
                 zout_prev = zout;
                 tn_km1 = tnk;
             }
-            T xnext = zout_prev / cpt_piv(pt, jpk - 1, ikb);
+            T xnext = zout_prev / cpt_piv(cpt_tab<T>(), jpk - 1, ikb);
             A3(zwz, i, j, jpk - 1) = (tr_w<TRP>(d, pw, i, j, jpk - 1) * xnext - tp_upz<TRP>(d, pw, tb, i, j, jpk - 1)) *
                                      WMASK(T, i, j, jpk - 1);
     #pragma unroll 4
             for (int k = jpk - 2; k >= 2; k--) {
                 const T zws = (k == 2 || k == ikb) ? T(0) : WMASK(T, i, j, k);
-                const T x = (A3(zwz, i, j, k) - zws * xnext) / cpt_piv(pt, k, ikb);
+                const T x = (A3(zwz, i, j, k) - zws * xnext) / cpt_piv(cpt_tab<T>(), k, ikb);
                 A3(zwz, i, j, k) = (tr_w<TRP>(d, pw, i, j, k) * x - tp_upz<TRP>(d, pw, tb, i, j, k)) * WMASK(T, i, j, k);
                 xnext = x;
             }
@@ -1849,10 +1849,10 @@ struct S_fct_v {
     }
 };
 template <int TRP, typename T, int NV4, int NT>
-KERNEL void k_fct_v(DevDomain d, PivotTab<T> pt, const T *__restrict__ pw, const T *__restrict__ ptb,
+KERNEL void k_fct_v(DevDomain d, const T *__restrict__ pw, const T *__restrict__ ptb,
                     const T *__restrict__ ptn, T *__restrict__ pzwz, int kjpt)
 {
-    S_fct_v<TRP, T, NV4, NT>::run(vb_hw(), d, pt, pw, ptb, ptn, pzwz, kjpt);
+    S_fct_v<TRP, T, NV4, NT>::run(vb_hw(), d, pw, ptb, ptn, pzwz, kjpt);
 }
 
 // ring and bottom level of a trend that starts from zero (fused step)
@@ -1883,7 +1883,6 @@ template <int TRP, typename T>
 static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const T *pv, const T *pw, const T *ptb,
                             const T *ptn, T *pta, int kjpt, int kn_fct_h, int kn_fct_v, int ta0)
 {
-    const PivotTab<T> pt = cpt_pivots<T>(d.jpk);
     // scratch slot pairs (two tracers of either storage type each)
     T *zwi = (T *)nh::scratch3(0), *zwx = (T *)nh::scratch3(2), *zwy = (T *)nh::scratch3(4);
     T *zwz = (T *)nh::scratch3(6), *bu = (T *)nh::scratch3(8), *bd = (T *)nh::scratch3(10);
@@ -1898,7 +1897,7 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
         if (!d.iper && !d.jper && nh::tune(nh::T_FCT_FUSED)) {
             // closed domain: the whole chain in one kernel per tile and tracer
             // (zwz's slots hold the compact scheme's forward elimination)
-            if ((st = launch_fct_fused<TRP, T>(d, pt, p2dt, nt, pu, pv, pw, ptb + o, ptn + o, pta + o, kn_fct_h,
+            if ((st = launch_fct_fused<TRP, T>(d, p2dt, nt, pu, pv, pw, ptb + o, ptn + o, pta + o, kn_fct_h,
                                                kn_fct_v, ta0, zwz, nm)))
                 return st;
             continue;
@@ -1917,7 +1916,7 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
             else FH_LAUNCH(0, 1);
 #undef FH_LAUNCH
 #undef FH_LAUNCH2
-            if (kn_fct_v == 4 && (st = launch_fct_cpt_fwd<T>(d, pt, nt, ptn + o, zwz, nm))) return st;
+            if (kn_fct_v == 4 && (st = launch_fct_cpt_fwd<T>(d, nt, ptn + o, zwz, nm))) return st;
             const int nk = nt * d.jpk;
             if ((st = launch_lbc<T>(zwx, nk, 'U', T(0)))) return st;
             if ((st = launch_lbc<T>(zwy, nk, 'V', T(0)))) return st;
@@ -1938,11 +1937,11 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
 #undef FH_LAUNCH
 #undef FH_LAUNCH2
         if (kn_fct_v == 4) {
-            if (nt == 2) LAUNCHB(nm, (k_fct_v<TRP, T, 1, 2>), (S_fct_v<TRP, T, 1, 2>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
-            else LAUNCHB(nm, (k_fct_v<TRP, T, 1, 1>), (S_fct_v<TRP, T, 1, 1>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
+            if (nt == 2) LAUNCHB(nm, (k_fct_v<TRP, T, 1, 2>), (S_fct_v<TRP, T, 1, 2>), gi, d, pw, ptb + o, ptn + o, zwz, nt);
+            else LAUNCHB(nm, (k_fct_v<TRP, T, 1, 1>), (S_fct_v<TRP, T, 1, 1>), gi, d, pw, ptb + o, ptn + o, zwz, nt);
         } else {
-            if (nt == 2) LAUNCHB(nm, (k_fct_v<TRP, T, 0, 2>), (S_fct_v<TRP, T, 0, 2>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
-            else LAUNCHB(nm, (k_fct_v<TRP, T, 0, 1>), (S_fct_v<TRP, T, 0, 1>), gi, d, pt, pw, ptb + o, ptn + o, zwz, nt);
+            if (nt == 2) LAUNCHB(nm, (k_fct_v<TRP, T, 0, 2>), (S_fct_v<TRP, T, 0, 2>), gi, d, pw, ptb + o, ptn + o, zwz, nt);
+            else LAUNCHB(nm, (k_fct_v<TRP, T, 0, 1>), (S_fct_v<TRP, T, 0, 1>), gi, d, pw, ptb + o, ptn + o, zwz, nt);
         }
         const int nk = nt * d.jpk;
         if ((st = launch_lbc<T>(zwi, nk, 'T', T(0)))) return st;
@@ -1977,24 +1976,6 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
 }
 
 
-template <typename T>
-PivotTab<T> cpt_pivots(int jpk)
-{
-    // interp_4th_cpt rows 3.. of a wet column: zwd = 3*1+1, zwi = zws = 1,
-    // zwt(k) = zwd - zwi*zws(k-1)/zwt(k-1) with row 2 an identity row
-    PivotTab<T> pt;
-    for (int k = 0; k < STEP_MAXK + 2; k++) pt.p[k] = T(1);
-    T zws_prev = T(0), zwt_prev = T(1);
-    for (int k = 3; k <= jpk && k <= STEP_MAXK; k++) {
-        const T wm = T(1);
-        const T zwd = T(3) * wm + T(1);
-        const T zwt_k = zwd - wm * zws_prev / zwt_prev;
-        pt.p[k] = zwt_k;
-        zws_prev = wm;
-        zwt_prev = zwt_k;
-    }
-    return pt;
-}
 
 // tra_adv (TRA/traadv.F90:77-175, FCT): the transports of un, vn, wn formed
 // where used and tra_adv_fct added to pta
@@ -2035,8 +2016,6 @@ int launch_step_tra(double p2dt, double z2dt, double p2dt_zdf, const T *un, cons
     Grid2 gi = grid2(d.jpi - 2, d.jpj - 2);
     return zdf_group<T, 2>("step_tra", d, gi, p2dt_zdf, tsb, tsa, 0, avt, 1, add, zwt, 1);
 }
-template PivotTab<double> cpt_pivots<double>(int);
-template PivotTab<float> cpt_pivots<float>(int);
 template int launch_step_tra<double>(double, double, double, const double *, const double *, const double *,
                                      const double *, const double *, double *, int, int, const double *,
                                      const double *, const double *, const double *, const double *, const double *,
