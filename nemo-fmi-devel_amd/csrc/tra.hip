@@ -398,17 +398,20 @@ KERNEL void k_fct_final(DevDomain d, const T * __restrict__ paa, const T * __res
 // expressions, same rounding to the storage type), and the ring values
 // lbc_lnk gives a closed domain (zero betas and limited fluxes) are selected
 // where they apply.
-// k_fct_lim tiles: LIM_TX x TY cells per 64x4 workgroup, so each thread owns
-// TY/4 cells of its i column (rows ty, ty+4, ...); taller tiles spread the
-// halo of the bounds (2 cells) and of the betas (1 cell) over more cells and
-// do more work between two barriers of the k-march
+// k_fct_lim tiles: LIM_TX x TY cells per workgroup of LIM_TX x WY threads,
+// so each thread owns TY/WY cells of its i column (rows ty, ty+WY, ...);
+// taller tiles spread the halo of the bounds (2 cells) and of the betas (1
+// cell) over more cells and do more work between two barriers of the march
 #define LIM_TX 64
 template <typename T> struct FctLimTY {
-    static constexpr int value = sizeof(T) == 4 ? 8 : 4;   // 64x8 tiles for FP32, 64x4 for FP64
+    // 64x8 tiles for FP32, 64x4 for FP64, 256 threads each (512 threads per
+    // 64x8 FP32 tile, two workgroups per CU, measured the same)
+    static constexpr int value = sizeof(T) == 4 ? 8 : 4;
+    static constexpr int nth = 256;
 };
-template <typename T, int TY>
+template <typename T, int TY, int NTH = NH_BLOCK>
 struct S_fct_lim {
-    static constexpr int NTHR = NH_BLOCK, WAVES = 3;
+    static constexpr int NTHR = NTH, WAVES = NTH == 512 ? 4 : 3;
     __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, double p2dt, const T *__restrict__ pbef_, const T *__restrict__ paft_, const T *__restrict__ paa_, const T *__restrict__ pbb_, const T *__restrict__ pcc_, T *__restrict__ ta_)
     {
         // bounds: FP64 with the reference's +-1e40 land sentinels; with FP32
@@ -418,8 +421,8 @@ struct S_fct_lim {
         using ZB = typename std::conditional<sizeof(T) == 4, float, double>::type;
         const ZB zsent = sizeof(T) == 4 ? (ZB)INFINITY : (ZB)1.e+40;
         constexpr int ZX = LIM_TX + 4, ZY = TY + 4, BX = LIM_TX + 2, BY = TY + 2;
-        constexpr int NZ = (ZX * ZY + NH_BLOCK - 1) / NH_BLOCK, NB = (BX * BY + NH_BLOCK - 1) / NH_BLOCK;
-        constexpr int WY = NH_BLOCK / LIM_TX, NR = TY / WY;   // workgroup rows, cells per thread
+        constexpr int NZ = (ZX * ZY + NTH - 1) / NTH, NB = (BX * BY + NTH - 1) / NTH;
+        constexpr int WY = NTH / LIM_TX, NR = TY / WY;   // workgroup rows, cells per thread
         static_assert(TY % WY == 0, "tile height must be a multiple of the workgroup height");
         // the first guess through LDS for FP32 storage only: with FP64 the
         // extra plane would cost a workgroup per CU
@@ -454,7 +457,7 @@ struct S_fct_lim {
         uint32_t zkt[NZ], zof[NZ];   // packed tmask interval (loaded once), in-plane offset
     #pragma unroll
         for (int r = 0; r < NZ; r++) {
-            const int q = tid + r * NH_BLOCK;
+            const int q = tid + r * NTH;
             zon[r] = q < ZX * ZY;
             zpx[r] = q % ZX;
             zpy[r] = q / ZX;
@@ -516,7 +519,7 @@ struct S_fct_lim {
         double fe1e2t[NB], fe3t[NB];
     #pragma unroll
         for (int r = 0; r < NB; r++) {
-            const int q = tid + r * NH_BLOCK;
+            const int q = tid + r * NTH;
             bon[r] = q < BX * BY;
             bpx[r] = q % BX;
             bpy[r] = q / BX;
@@ -688,12 +691,12 @@ struct S_fct_lim {
         }
     }
 };
-template <typename T, int TY>
-__global__ __launch_bounds__(NH_BLOCK, 3) void k_fct_lim(DevDomain d, double p2dt, const T *__restrict__ pbef_, const T *__restrict__ paft_,
+template <typename T, int TY, int NTH = NH_BLOCK>
+__global__ __launch_bounds__(NTH, (S_fct_lim<T, TY, NTH>::WAVES)) void k_fct_lim(DevDomain d, double p2dt, const T *__restrict__ pbef_, const T *__restrict__ paft_,
                       const T *__restrict__ paa_, const T *__restrict__ pbb_, const T *__restrict__ pcc_,
                       T *__restrict__ ta_)
 {
-    S_fct_lim<T, TY>::run(vb_hw(), d, p2dt, pbef_, paft_, paa_, pbb_, pcc_, ta_);
+    S_fct_lim<T, TY, NTH>::run(vb_hw(), d, p2dt, pbef_, paft_, paa_, pbb_, pcc_, ta_);
 }
 
 template <typename T>
@@ -1950,11 +1953,12 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
         if ((st = launch_lbc<T>(zwz, nk, 'W', T(0)))) return st;
         if (!d.iper && !d.jper) {
             // closed domain: betas and limited divergence fused, both tracers in one launch
-            constexpr int TY = FctLimTY<T>::value;
-            Grid2 gl = grid2(d.jpi - 2, d.jpj - 2, LIM_TX, NH_BLOCK / LIM_TX);
+            constexpr int TY = FctLimTY<T>::value, NTH = FctLimTY<T>::nth;
+            Grid2 gl = grid2(d.jpi - 2, d.jpj - 2);
+            gl.block = dim3(LIM_TX, NTH / LIM_TX, 1);
             gl.grid.y = (unsigned)((d.jpj - 2 + TY - 1) / TY);
             gl.grid.z = (unsigned)nt;
-            LAUNCHB(nm, (k_fct_lim<T, TY>), (S_fct_lim<T, TY>), gl, d, p2dt, ptb + o, zwi, zwx, zwy, zwz, pta + o);
+            LAUNCHB(nm, (k_fct_lim<T, TY, NTH>), (S_fct_lim<T, TY, NTH>), gl, d, p2dt, ptb + o, zwi, zwx, zwy, zwz, pta + o);
             continue;
         }
         for (int t = 0; t < nt; t++) {
