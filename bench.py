@@ -404,12 +404,15 @@ def time_adjoint(h, size, steps, ckpt, ndir, seed, kn=(4, 4)):
 
             def step(tb, tn, cc, own):
                 """one tracer step from (tb, tn); own: the factors of the state
-                itself (the nonlinear step), else the frozen factors cc"""
-                if own:
-                    h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, tb, tn, *cc, kn=kn)
+                itself (the nonlinear step: tra_adv_fct, which equals the
+                frozen-factor trend with the state's own factors bit for bit),
+                else the frozen factors cc"""
                 ta = A[:n3]
                 ta.zero_()
-                h.tra_adv_fct_tl(p2dt, zun, zvn, zwn, tb, tn, *cc, ta, kn=kn)
+                if own:
+                    h.tra_adv_fct(kt, 1, "TRA", p2dt, zun, zvn, zwn, tb, tn, ta, 1, *kn)
+                else:
+                    h.tra_adv_fct_tl(p2dt, zun, zvn, zwn, tb, tn, *cc, ta, kn=kn)
                 h.tra_ldf_lap(kt, 1, "TRA", ahtu, ahtv, tb, ta, 1, 1)
                 h.tra_zdf_imp(kt, 1, "TRA", p2dt, tb, ta, 1, avt, avt)
                 B[:n3].copy_(tb)
@@ -452,7 +455,7 @@ def time_adjoint(h, size, steps, ckpt, ndir, seed, kn=(4, 4)):
                     cc = [f64() for _ in range(3)]
                     h.tra_adv_fct_coef(p2dt, zun, zvn, zwn, b, n, *cc, kn=kn)
                     seg.append(cc)
-                    b, n = step(b, n, cc, False)
+                    b, n = step(b, n, None, True)
                 for cc in reversed(seg):
                     h.tra_nxt_ad(lb, ln, la)
                     h.tra_zdf_imp_ad(p2dt, avt, la, lb)

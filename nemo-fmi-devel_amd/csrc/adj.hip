@@ -47,21 +47,34 @@ __device__ __forceinline__ void ring_of(const DevDomain &d, int i, int j, F f)
 struct FctAd {
     const double *un, *vn, *wn, *cx, *cy, *cz, *mu;
     const double *zl_u, *zl_v;   // kn_h = 4: the adjoint of zltu / zltv after the lbc fold
+    const double *hi, *lo;       // the cell factors below, formed once per cell (k_fct_ad_cell)
     int kn_h, kn_v;
 };
 
-// mu r1/e3t (hi) and mu r1/e3t tmask (lo) of a cell; zero off the interior
+// mu r1/e3t (hi) and mu r1/e3t tmask (lo) of a cell; zero off the interior.
+// k_fct_ad_cell forms them once per cell; the gathers, which need each
+// cell's pair for the dozen faces around it, read them back.
 __device__ __forceinline__ double cell_hi(const DevDomain &d, const FctAd &a, int i, int j, int k)
 {
     if (!interior(d, i, j) || k < 1 || k > d.jpk - 1) return 0.;
-    const uint32_t ij = IJ32(i, j);
-    return A3(a.mu, i, j, k) * d.md.r1_e1e2t[ij] / E3(d.md.e3t, ij, k);
+    return A3(a.hi, i, j, k);
 }
 __device__ __forceinline__ double cell_lo(const DevDomain &d, const FctAd &a, int i, int j, int k)
 {
     if (!interior(d, i, j) || k < 1 || k > d.jpk - 1) return 0.;
+    return A3(a.lo, i, j, k);
+}
+
+KERNEL void k_fct_ad_cell(DevDomain d, const double *__restrict__ mu, double *__restrict__ hi,
+                          double *__restrict__ lo)
+{
+    THREAD_IJ(2, 2);
+    const int k = (int)blockIdx.z + 1;
+    if (i > d.jpi - 1 || j > d.jpj - 1) return;
     const uint32_t ij = IJ32(i, j);
-    return A3(a.mu, i, j, k) * d.md.r1_e1e2t[ij] / E3(d.md.e3t, ij, k) * TMASK(double, i, j, k);
+    const double h = A3(mu, i, j, k) * d.md.r1_e1e2t[ij] / E3(d.md.e3t, ij, k);
+    A3(hi, i, j, k) = h;
+    A3(lo, i, j, k) = h * TMASK(double, i, j, k);
 }
 
 // adjoint of the limited x-flux at face p (between cells p and p+1) of row j
@@ -412,7 +425,11 @@ int launch_fct_ad(const double *pun, const double *pvn, const double *pwn, int k
     Grid2 g = grid2(d.jpi, d.jpj);
     g.grid.z = (unsigned)(d.jpk - 1);
     double *zl_u = (double *)nh::scratch3(nh::S_ZWI), *zl_v = (double *)nh::scratch3(nh::S_ZWX);
-    const FctAd a{pun, pvn, pwn, cx, cy, cz, ta_ad, zl_u, zl_v, kn_h, kn_v};
+    double *hi = (double *)nh::scratch3(nh::S_ZBETUP), *lo = (double *)nh::scratch3(nh::S_ZBETDO);
+    const FctAd a{pun, pvn, pwn, cx, cy, cz, ta_ad, zl_u, zl_v, hi, lo, kn_h, kn_v};
+    Grid2 gcell = grid2(d.jpi - 2, d.jpj - 2);
+    gcell.grid.z = (unsigned)(d.jpk - 1);
+    LAUNCH("tra_adv_fct_ad", k_fct_ad_cell, gcell, d, ta_ad, hi, lo);
     if (kn_h == 4) LAUNCH("tra_adv_fct_ad", k_fct_ad_l4, g, d, a, zl_u, zl_v);
     LAUNCH("tra_adv_fct_ad", k_fct_ad, g, d, a, tb_ad, tn_ad);
     if (kn_v == 4) {

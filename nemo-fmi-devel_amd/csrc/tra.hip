@@ -32,6 +32,10 @@ template <typename T> struct FctKB { static constexpr int beta = sizeof(T) == 8 
 template <int TRP, typename T>
 static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const T *pv, const T *pw, const T *ptb,
                             const T *ptn, T *pta, int kjpt, int kn_fct_h, int kn_fct_v, int ta0);
+template <typename T>
+static int launch_fct_fluxes1(const DevDomain &d, double p2dt, const T *pu, const T *pv, const T *pw, const T *ptb,
+                              const T *ptn, T *pta, int kn_h, int kn_v, T *zwi, T *zwx, T *zwy, T *zwz,
+                              const char *nm);
 
 // ---------------------------------------------------------------------------
 // effective transport, TRA/traadv.F90:100-124
@@ -807,12 +811,19 @@ int launch_fct_frozen(int mode, double p2dt, const double *pun, const double *pv
     gk.grid.z = (unsigned)((d.jpk - 1 + FctKB<double>::beta - 1) / FctKB<double>::beta);
     int st;
     double *ta = mode == 0 ? dummy : pta;
-    LAUNCH("tra_adv_fct", (k_fct_low<double>), gi, d, p2dt, pun, pvn, pwn, tb, ta, zwi);
-    if ((st = launch_lbc<double>(zwi, d.jpk, 'T', 0.0))) return st;
-    LAUNCH("tra_adv_fct", (k_fct_ad<double>), gh, d, kn_h, kn_v, pun, pvn, pwn, tb, tn, zwx, zwy, zwz, zwt);
-    if ((st = launch_lbc<double>(zwx, d.jpk, 'U', 0.0))) return st;
-    if ((st = launch_lbc<double>(zwy, d.jpk, 'V', 0.0))) return st;
-    if ((st = launch_lbc<double>(zwz, d.jpk, 'W', 0.0))) return st;
+    if (d.jpk <= STEP_MAXK) {
+        // the flux kernels of the tracer-pair chain, for one tracer
+        if ((st = launch_fct_fluxes1<double>(d, p2dt, pun, pvn, pwn, tb, tn, ta, kn_h, kn_v, zwi, zwx, zwy, zwz,
+                                             "tra_adv_fct")))
+            return st;
+    } else {
+        LAUNCH("tra_adv_fct", (k_fct_low<double>), gi, d, p2dt, pun, pvn, pwn, tb, ta, zwi);
+        if ((st = launch_lbc<double>(zwi, d.jpk, 'T', 0.0))) return st;
+        LAUNCH("tra_adv_fct", (k_fct_ad<double>), gh, d, kn_h, kn_v, pun, pvn, pwn, tb, tn, zwx, zwy, zwz, zwt);
+        if ((st = launch_lbc<double>(zwx, d.jpk, 'U', 0.0))) return st;
+        if ((st = launch_lbc<double>(zwy, d.jpk, 'V', 0.0))) return st;
+        if ((st = launch_lbc<double>(zwz, d.jpk, 'W', 0.0))) return st;
+    }
     if (mode == 1) {
         LAUNCH("tra_adv_fct", k_fct_final_frozen, gi, d, zwx, zwy, zwz, cx, cy, cz, pta);
         return 0;
@@ -1880,6 +1891,29 @@ template <typename T>
 KERNEL void k_zero_ring(DevDomain d, T *__restrict__ pta, int ntr)
 {
     S_zero_ring<T>::run(vb_hw(), d, pta, ntr);
+}
+
+// the chain's flux kernels for one tracer with the transports given: pta +=
+// the low-order trend; zwi, zwx, zwy, zwz after their lbc_lnk (the first
+// stage of tra_adv_fct, as the frozen-limiter entry points need it)
+template <typename T>
+static int launch_fct_fluxes1(const DevDomain &d, double p2dt, const T *pu, const T *pv, const T *pw, const T *ptb,
+                              const T *ptn, T *pta, int kn_h, int kn_v, T *zwi, T *zwx, T *zwy, T *zwz,
+                              const char *nm)
+{
+    Grid2 g3 = grid2(d.jpi - 1, d.jpj - 1), gi = grid2(d.jpi - 2, d.jpj - 2);
+    g3.grid.z = (unsigned)(d.jpk - 1);
+    const int nt = 1;
+    if (kn_h == 4) LAUNCHB(nm, (k_fct_h<0, T, 1, 0, 1>), (S_fct_h<0, T, 1, 0, 1>), g3, d, p2dt, pu, pv, pw, ptb, ptn, pta, zwi, zwx, zwy, nt);
+    else LAUNCHB(nm, (k_fct_h<0, T, 0, 0, 1>), (S_fct_h<0, T, 0, 0, 1>), g3, d, p2dt, pu, pv, pw, ptb, ptn, pta, zwi, zwx, zwy, nt);
+    if (kn_v == 4) LAUNCHB(nm, (k_fct_v<0, T, 1, 1>), (S_fct_v<0, T, 1, 1>), gi, d, pw, ptb, ptn, zwz, nt);
+    else LAUNCHB(nm, (k_fct_v<0, T, 0, 1>), (S_fct_v<0, T, 0, 1>), gi, d, pw, ptb, ptn, zwz, nt);
+    int st;
+    if ((st = launch_lbc<T>(zwi, d.jpk, 'T', T(0)))) return st;
+    if ((st = launch_lbc<T>(zwx, d.jpk, 'U', T(0)))) return st;
+    if ((st = launch_lbc<T>(zwy, d.jpk, 'V', T(0)))) return st;
+    if ((st = launch_lbc<T>(zwz, d.jpk, 'W', T(0)))) return st;
+    return 0;
 }
 
 template <int TRP, typename T>
