@@ -1869,6 +1869,73 @@ KERNEL void k_fct_v(DevDomain d, const T *__restrict__ pw, const T *__restrict__
     S_fct_v<TRP, T, NV4, NT>::run(vb_hw(), d, pw, ptb, ptn, pzwz, kjpt);
 }
 
+// k_fct_v for FP32 storage and the order-4 scheme with jpk <= KMAX: the
+// compact scheme's forward-swept values stay in registers (every level loop
+// unrolled to KMAX, so the array is indexed by constants) instead of passing
+// through zwz, which is then written once.  Same operations, same values.
+// One tracer per thread: the workgroup's z coordinate selects it.
+#define FCTV_REG_KMAX 76
+template <int TRP, int KMAX>
+struct S_fct_v_reg {
+    static constexpr int NTHR = NH_BLOCK, WAVES = 2;
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, const float *__restrict__ pw, const float *__restrict__ ptb, const float *__restrict__ ptn, float *__restrict__ pzwz, int kjpt)
+    {
+        (void)kjpt;
+        THREAD_IJ_V(vb_, 2, 2);
+        if (i > d.jpi - 1 || j > d.jpj - 1) return;
+        const size_t n3 = d.n3;
+        const int jpk = d.jpk;
+        const int ikb = d.mbkt[IJ(i, j)];
+        {
+            const size_t t = vb_.z;
+            const float *tb = ptb + t * n3, *tn = ptn + t * n3;
+            float *zwz = pzwz + t * n3;
+            A3(zwz, i, j, 1) = 0.0f;
+            A3(zwz, i, j, jpk) = 0.0f;
+            float zo[KMAX];
+            float zout_prev = 0.0f, tn_km1 = A3(tn, i, j, 1);
+    #pragma unroll
+            for (int k = 2; k < KMAX; k++) {
+                if (k <= jpk - 1) {
+                    const float tnk = A3(tn, i, j, k);
+                    float zwrm, zout, zwi_c = 0.0f;
+                    if (k == 2 || k == ikb) {
+                        zwrm = 0.5f * (tn_km1 + tnk);
+                    } else {
+                        const float wm = WMASK(float, i, j, k);
+                        zwi_c = wm;
+                        zwrm = 3.0f * wm * (tnk + tn_km1);
+                    }
+                    if (k == 2) zout = zwrm;
+                    else zout = zwrm - zwi_c / cpt_piv(cpt_tab<float>(), k - 1, ikb) * zout_prev;
+                    zo[k] = zout;
+                    zout_prev = zout;
+                    tn_km1 = tnk;
+                }
+            }
+            float xnext = zout_prev / cpt_piv(cpt_tab<float>(), jpk - 1, ikb);
+            A3(zwz, i, j, jpk - 1) = (tr_w<TRP>(d, pw, i, j, jpk - 1) * xnext - tp_upz<TRP>(d, pw, tb, i, j, jpk - 1)) *
+                                     WMASK(float, i, j, jpk - 1);
+    #pragma unroll
+            for (int k = KMAX - 2; k >= 2; k--) {
+                if (k <= jpk - 2) {
+                    const float zws = (k == 2 || k == ikb) ? 0.0f : WMASK(float, i, j, k);
+                    const float x = (zo[k] - zws * xnext) / cpt_piv(cpt_tab<float>(), k, ikb);
+                    A3(zwz, i, j, k) =
+                        (tr_w<TRP>(d, pw, i, j, k) * x - tp_upz<TRP>(d, pw, tb, i, j, k)) * WMASK(float, i, j, k);
+                    xnext = x;
+                }
+            }
+        }
+    }
+};
+template <int TRP, int KMAX>
+__global__ __launch_bounds__(NH_BLOCK, 2) void k_fct_v_reg(DevDomain d, const float *__restrict__ pw, const float *__restrict__ ptb,
+                        const float *__restrict__ ptn, float *__restrict__ pzwz, int kjpt)
+{
+    S_fct_v_reg<TRP, KMAX>::run(vb_hw(), d, pw, ptb, ptn, pzwz, kjpt);
+}
+
 // ring and bottom level of a trend that starts from zero (fused step)
 template <typename T>
 struct S_zero_ring {
@@ -1973,7 +2040,17 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
         }
 #undef FH_LAUNCH
 #undef FH_LAUNCH2
-        if (kn_fct_v == 4) {
+        bool vdone = false;
+        if constexpr (sizeof(T) == 4) {
+            if (kn_fct_v == 4 && d.jpk <= FCTV_REG_KMAX) {
+                Grid2 gv = gi;
+                gv.grid.z = (unsigned)nt;
+                LAUNCHB(nm, (k_fct_v_reg<TRP, FCTV_REG_KMAX>), (S_fct_v_reg<TRP, FCTV_REG_KMAX>), gv, d, pw, ptb + o, ptn + o, zwz, nt);
+                vdone = true;
+            }
+        }
+        if (vdone) {
+        } else if (kn_fct_v == 4) {
             if (nt == 2) LAUNCHB(nm, (k_fct_v<TRP, T, 1, 2>), (S_fct_v<TRP, T, 1, 2>), gi, d, pw, ptb + o, ptn + o, zwz, nt);
             else LAUNCHB(nm, (k_fct_v<TRP, T, 1, 1>), (S_fct_v<TRP, T, 1, 1>), gi, d, pw, ptb + o, ptn + o, zwz, nt);
         } else {
