@@ -15,7 +15,9 @@
 //                  extrapolated velocities, fluxes and un_adv sum, pointwise.
 // Only the first extrapolation (k_spg_s1) has a kernel of its own.  (One
 // kernel per sub-step that recomputes zsshp2_e at the east and north
-// neighbours moves as many bytes and runs slower.)  The sub-step levels rotate
+// neighbours moves as many bytes and runs slower; one that shares it through
+// LDS, S2 and S3 separated by a workgroup barrier, runs no faster either:
+// 2.77 ms per step at 1024x1024 against 2.78 for the two kernels.)  The sub-step levels rotate
 // through buffers instead of being copied: three for the velocities, four for
 // the sea level.  All barotropic state and arithmetic is FP64 in both storage
 // forms.  Phase 3 is one 2D kernel and one column kernel.
