@@ -639,12 +639,12 @@ def kernel_times(h):
         "dyn_spg_ts": ["k_spg_cor1", "k_spg_cor2", "k_spg_init", "k_spg_p1a", "k_spg_p1b", "k_spg_s1", "k_spg_s2", "k_spg_s3",
                        "k_spg_loop", "k_spg_p3a", "k_spg_p3b"],
         "tra_adv_trp": ["k_tra_adv_trp"],
-        "tra_adv_fct": ["k_fct_fused", "k_fct_cpt_fwd", "k_fct_h", "k_fct_v", "k_fct_v_reg", "k_fct_lim", "k_fct_vlim",
+        "tra_adv_fct": ["k_fct_fused", "k_fct_cpt_fwd", "k_fct_h", "k_fct_hh", "k_fct_v", "k_fct_vz_reg", "k_fct_lim", "k_fct_vlim",
                         "k_fct_beta", "k_fct_final", "k_fct_low", "k_fct_ad"],
         "tra_ldf_iso": ["k_ldf_iso_coef", "k_tra_ldf_iso", "k_fill3", "k_copy3"], "tra_zdf": ["k_tra_zdf", "k_tra_zdf_reg", "k_clamp_ring"],
         "lbc_lnk": ["k_lbc"],
         "step_dyn": ["k_step_dyn1", "k_step_dyn2"],
-        "step_tra": ["k_fct_fused", "k_fct_cpt_fwd", "k_fct_h", "k_fct_v", "k_fct_v_reg", "k_fct_lim", "k_fct_vlim",
+        "step_tra": ["k_fct_fused", "k_fct_cpt_fwd", "k_fct_h", "k_fct_hh", "k_fct_v", "k_fct_vz_reg", "k_fct_lim", "k_fct_vlim",
                      "k_fct_beta", "k_fct_final", "k_zero_ring", "k_tra_zdf", "k_tra_zdf_reg"],
         "div_hor": ["k_div_hor"], "ssh_nxt": ["k_ssh_nxt"], "wzv": ["k_wzv"], "ssh_swp": ["k_ssh_swp"],
         "tra_nxt": ["k_tra_nxt"], "dyn_nxt": ["k_dyn_nxt1", "k_dyn_nxt2"],
@@ -1050,7 +1050,9 @@ KERNEL_BYTES = {
     "k_fct_cpt_fwd": _b(2),      # r tn*2 (its sweep is a temporary)
     "k_fct_h": lambda w, i, j, k, f: (9 if _fused(f) else 11) * w * (i - 2) * (j - 2) * (k - 1),
     "k_fct_v": _b(5),            # r pwn tb*2 tn*2
-    "k_fct_v_reg": _b(5),
+    "k_fct_hh": _b(6),           # r pun pvn tb*2 tn*2
+    # r pwn tb*2 tn*2 ta*2 (not in the fused step), w ta*2
+    "k_fct_vz_reg": lambda w, i, j, k, f: (7 if _fused(f) else 9) * w * (i - 2) * (j - 2) * (k - 1),
     "k_fct_lim": _b(6),          # two tracers: r tb ta, w ta
     "k_fct_final": _b(2),        # r ta, w ta
     "k_fct_beta": _b(1),         # r tb

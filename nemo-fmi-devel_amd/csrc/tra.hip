@@ -1796,12 +1796,20 @@ struct S_fct_h {
         }
     }
 };
-template <int TRP, typename T, int NH4, int TA0, int NT, int VERT = 1>
+template <int TRP, typename T, int NH4, int TA0, int NT>
 KERNEL void k_fct_h(DevDomain d, double p2dt, const T *__restrict__ pu, const T *__restrict__ pv,
                     const T *__restrict__ pw, const T *__restrict__ ptb, const T *__restrict__ ptn, T *__restrict__ pta,
                     T *__restrict__ pzwi, T *__restrict__ pzwx, T *__restrict__ pzwy, int kjpt)
 {
-    S_fct_h<TRP, T, NH4, TA0, NT, VERT>::run(vb_hw(), d, p2dt, pu, pv, pw, ptb, ptn, pta, pzwi, pzwx, pzwy, kjpt);
+    S_fct_h<TRP, T, NH4, TA0, NT, 1>::run(vb_hw(), d, p2dt, pu, pv, pw, ptb, ptn, pta, pzwi, pzwx, pzwy, kjpt);
+}
+// the horizontal half alone (VERT = 0)
+template <int TRP, typename T, int NH4, int NT>
+KERNEL void k_fct_hh(DevDomain d, double p2dt, const T *__restrict__ pu, const T *__restrict__ pv,
+                     const T *__restrict__ pw, const T *__restrict__ ptb, const T *__restrict__ ptn, T *__restrict__ pta,
+                     T *__restrict__ pzwi, T *__restrict__ pzwx, T *__restrict__ pzwy, int kjpt)
+{
+    S_fct_h<TRP, T, NH4, 1, NT, 0>::run(vb_hw(), d, p2dt, pu, pv, pw, ptb, ptn, pta, pzwi, pzwx, pzwy, kjpt);
 }
 
 // per interior column: the vertical antidiffusive flux zwz (lines 255-278),
@@ -1869,71 +1877,95 @@ KERNEL void k_fct_v(DevDomain d, const T *__restrict__ pw, const T *__restrict__
     S_fct_v<TRP, T, NV4, NT>::run(vb_hw(), d, pw, ptb, ptn, pzwz, kjpt);
 }
 
-// k_fct_v for FP32 storage and the order-4 scheme with jpk <= KMAX: the
-// compact scheme's forward-swept values stay in registers (every level loop
-// unrolled to KMAX, so the array is indexed by constants) instead of passing
-// through zwz, which is then written once.  Same operations, same values.
-// One tracer per thread: the workgroup's z coordinate selects it.
 #define FCTV_REG_KMAX 76
-template <int TRP, int KMAX>
-struct S_fct_v_reg {
+// k_fct_v for FP32 storage and the order-4 scheme with jpk <= KMAX, with
+// the vertical half of k_fct_h as well.  The compact scheme's forward-swept
+// values stay in registers (every level loop unrolled to KMAX, so the array
+// is indexed by constants), so zwz is written once.  After k_fct_hh (k_fct_h
+// with VERT = 0) has left the
+// horizontal upstream sums s in zwi's place, the back substitution, which
+// climbs the column with the vertical upstream flux of each w-level at hand,
+// also forms the low-order trend ztra = -(s + upz(k) - upz(k+1)) r1, adds
+// ztra / e3t to pta and stores the first guess zwi over s -- what k_fct_h
+// with VERT = 1 stores, with the same operations in the same order.  The
+// vertical planes of tb and w are then read once, by this column kernel,
+// instead of from three levels by every cell of k_fct_h.
+template <int TRP, int KMAX, int TA0>
+struct S_fct_vz_reg {
     static constexpr int NTHR = NH_BLOCK, WAVES = 2;
-    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, const float *__restrict__ pw, const float *__restrict__ ptb, const float *__restrict__ ptn, float *__restrict__ pzwz, int kjpt)
+    __device__ __forceinline__ static void run(const VB &vb_, const DevDomain &d, double p2dt, const float *__restrict__ pw, const float *__restrict__ ptb, const float *__restrict__ ptn, float *__restrict__ pta, float *__restrict__ pzwi, float *__restrict__ pzwz)
     {
-        (void)kjpt;
         THREAD_IJ_V(vb_, 2, 2);
         if (i > d.jpi - 1 || j > d.jpj - 1) return;
         const size_t n3 = d.n3;
         const int jpk = d.jpk;
-        const int ikb = d.mbkt[IJ(i, j)];
-        {
-            const size_t t = vb_.z;
-            const float *tb = ptb + t * n3, *tn = ptn + t * n3;
-            float *zwz = pzwz + t * n3;
-            A3(zwz, i, j, 1) = 0.0f;
-            A3(zwz, i, j, jpk) = 0.0f;
-            float zo[KMAX];
-            float zout_prev = 0.0f, tn_km1 = A3(tn, i, j, 1);
+        const size_t ij = IJ(i, j);
+        const int ikb = d.mbkt[ij];
+        const Metrics<float> &m = met<float>(d);
+        const float r1_e1e2t = m.r1_e1e2t[ij];
+        const float zp2dt = (float)p2dt;
+        const size_t t = vb_.z;
+        const float *tb = ptb + t * n3, *tn = ptn + t * n3;
+        float *zwz = pzwz + t * n3, *zwi = pzwi + t * n3, *ta = pta + t * n3;
+        A3(zwz, i, j, 1) = 0.0f;
+        A3(zwz, i, j, jpk) = 0.0f;
+        A3(zwi, i, j, jpk) = 0.0f;
+        float zo[KMAX];
+        float zout_prev = 0.0f, tn_km1 = A3(tn, i, j, 1);
     #pragma unroll
-            for (int k = 2; k < KMAX; k++) {
-                if (k <= jpk - 1) {
-                    const float tnk = A3(tn, i, j, k);
-                    float zwrm, zout, zwi_c = 0.0f;
-                    if (k == 2 || k == ikb) {
-                        zwrm = 0.5f * (tn_km1 + tnk);
-                    } else {
-                        const float wm = WMASK(float, i, j, k);
-                        zwi_c = wm;
-                        zwrm = 3.0f * wm * (tnk + tn_km1);
-                    }
-                    if (k == 2) zout = zwrm;
-                    else zout = zwrm - zwi_c / cpt_piv(cpt_tab<float>(), k - 1, ikb) * zout_prev;
-                    zo[k] = zout;
-                    zout_prev = zout;
-                    tn_km1 = tnk;
+        for (int k = 2; k < KMAX; k++) {
+            if (k <= jpk - 1) {
+                const float tnk = A3(tn, i, j, k);
+                float zwrm, zout, zwi_c = 0.0f;
+                if (k == 2 || k == ikb) {
+                    zwrm = 0.5f * (tn_km1 + tnk);
+                } else {
+                    const float wm = WMASK(float, i, j, k);
+                    zwi_c = wm;
+                    zwrm = 3.0f * wm * (tnk + tn_km1);
                 }
+                if (k == 2) zout = zwrm;
+                else zout = zwrm - zwi_c / cpt_piv(cpt_tab<float>(), k - 1, ikb) * zout_prev;
+                zo[k] = zout;
+                zout_prev = zout;
+                tn_km1 = tnk;
             }
-            float xnext = zout_prev / cpt_piv(cpt_tab<float>(), jpk - 1, ikb);
-            A3(zwz, i, j, jpk - 1) = (tr_w<TRP>(d, pw, i, j, jpk - 1) * xnext - tp_upz<TRP>(d, pw, tb, i, j, jpk - 1)) *
-                                     WMASK(float, i, j, jpk - 1);
+        }
+        float xnext = 0.0f, upz_kp1 = 0.0f;   // upz(jpk) = 0
     #pragma unroll
-            for (int k = KMAX - 2; k >= 2; k--) {
-                if (k <= jpk - 2) {
-                    const float zws = (k == 2 || k == ikb) ? 0.0f : WMASK(float, i, j, k);
-                    const float x = (zo[k] - zws * xnext) / cpt_piv(cpt_tab<float>(), k, ikb);
-                    A3(zwz, i, j, k) =
-                        (tr_w<TRP>(d, pw, i, j, k) * x - tp_upz<TRP>(d, pw, tb, i, j, k)) * WMASK(float, i, j, k);
+        for (int k = KMAX - 1; k >= 1; k--) {
+            if (k <= jpk - 1) {
+                const float upz = tp_upz<TRP>(d, pw, tb, i, j, k);
+                if (k >= 2) {   // zwz, as k_fct_v
+                    const float wm = WMASK(float, i, j, k);
+                    float x;
+                    if (k == jpk - 1) {
+                        x = zo[k] / cpt_piv(cpt_tab<float>(), k, ikb);
+                    } else {
+                        const float zws = (k == 2 || k == ikb) ? 0.0f : wm;
+                        x = (zo[k] - zws * xnext) / cpt_piv(cpt_tab<float>(), k, ikb);
+                    }
+                    A3(zwz, i, j, k) = (tr_w<TRP>(d, pw, i, j, k) * x - upz) * wm;
                     xnext = x;
                 }
+                // the low-order trend and first guess, as k_fct_h
+                const float e3t = E3(m.e3t, ij, k), tm = TMASK(float, i, j, k);
+                const float ztra = -(((A3(zwi, i, j, k) + upz) - upz_kp1) * r1_e1e2t);
+                const float a = TA0 ? 0.0f : A3(ta, i, j, k);
+                A3(ta, i, j, k) = a + ztra / e3t * tm;
+                A3(zwi, i, j, k) = (e3t * A3(tb, i, j, k) + zp2dt * ztra) / e3t * tm;
+                upz_kp1 = upz;
             }
         }
     }
 };
-template <int TRP, int KMAX>
-__global__ __launch_bounds__(NH_BLOCK, 2) void k_fct_v_reg(DevDomain d, const float *__restrict__ pw, const float *__restrict__ ptb,
-                        const float *__restrict__ ptn, float *__restrict__ pzwz, int kjpt)
+template <int TRP, int KMAX, int TA0>
+__global__ __launch_bounds__(NH_BLOCK, 2) void k_fct_vz_reg(DevDomain d, double p2dt, const float *__restrict__ pw,
+                                                          const float *__restrict__ ptb, const float *__restrict__ ptn,
+                                                          float *__restrict__ pta, float *__restrict__ pzwi,
+                                                          float *__restrict__ pzwz)
 {
-    S_fct_v_reg<TRP, KMAX>::run(vb_hw(), d, pw, ptb, ptn, pzwz, kjpt);
+    S_fct_vz_reg<TRP, KMAX, TA0>::run(vb_hw(), d, p2dt, pw, ptb, ptn, pta, pzwi, pzwz);
 }
 
 // ring and bottom level of a trend that starts from zero (fused step)
@@ -2012,13 +2044,10 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
             // closed domain: horizontal fluxes, then the compact scheme's
             // forward sweep, then the vertical fluxes, first guess and limiter
             // in one bottom-up march (zwi holds s, zwz the forward sweep)
-#define FH_LAUNCH2(H, Z, N) \
-    LAUNCHB(nm, (k_fct_h<TRP, T, H, Z, N, 0>), (S_fct_h<TRP, T, H, Z, N, 0>), g3, d, p2dt, pu, pv, pw, ptb + o, ptn + o, pta + o, zwi, zwx, zwy, nt)
-#define FH_LAUNCH(H, Z) \
-    do { if (nt == 2) FH_LAUNCH2(H, Z, 2); else FH_LAUNCH2(H, Z, 1); } while (0)
-            if (kn_fct_h == 4) FH_LAUNCH(1, 1);
-            else FH_LAUNCH(0, 1);
-#undef FH_LAUNCH
+#define FH_LAUNCH2(H, N) \
+    LAUNCHB(nm, (k_fct_hh<TRP, T, H, N>), (S_fct_h<TRP, T, H, 1, N, 0>), g3, d, p2dt, pu, pv, pw, ptb + o, ptn + o, pta + o, zwi, zwx, zwy, nt)
+            if (kn_fct_h == 4) { if (nt == 2) FH_LAUNCH2(1, 2); else FH_LAUNCH2(1, 1); }
+            else { if (nt == 2) FH_LAUNCH2(0, 2); else FH_LAUNCH2(0, 1); }
 #undef FH_LAUNCH2
             if (kn_fct_v == 4 && (st = launch_fct_cpt_fwd<T>(d, nt, ptn + o, zwz, nm))) return st;
             const int nk = nt * d.jpk;
@@ -2029,25 +2058,35 @@ static int launch_fct_pairs(const DevDomain &d, double p2dt, const T *pu, const 
                 return st;
             continue;
         }
+        bool vdone = false;
+        if constexpr (sizeof(T) == 4) {
+            if (kn_fct_v == 4 && d.jpk <= FCTV_REG_KMAX) {
+                // horizontal fluxes per cell, then the vertical fluxes, the
+                // low-order trend and the first guess per column
+#define FH_LAUNCH2(H, N) \
+    LAUNCHB(nm, (k_fct_hh<TRP, T, H, N>), (S_fct_h<TRP, T, H, 1, N, 0>), g3, d, p2dt, pu, pv, pw, ptb + o, ptn + o, pta + o, zwi, zwx, zwy, nt)
+                if (kn_fct_h == 4) { if (nt == 2) FH_LAUNCH2(1, 2); else FH_LAUNCH2(1, 1); }
+                else { if (nt == 2) FH_LAUNCH2(0, 2); else FH_LAUNCH2(0, 1); }
+#undef FH_LAUNCH2
+                Grid2 gv = gi;
+                gv.grid.z = (unsigned)nt;
+                if (ta0) LAUNCHB(nm, (k_fct_vz_reg<TRP, FCTV_REG_KMAX, 1>), (S_fct_vz_reg<TRP, FCTV_REG_KMAX, 1>), gv, d, p2dt, pw, ptb + o, ptn + o, pta + o, zwi, zwz);
+                else LAUNCHB(nm, (k_fct_vz_reg<TRP, FCTV_REG_KMAX, 0>), (S_fct_vz_reg<TRP, FCTV_REG_KMAX, 0>), gv, d, p2dt, pw, ptb + o, ptn + o, pta + o, zwi, zwz);
+                vdone = true;
+            }
+        }
+        if (!vdone) {
 #define FH_LAUNCH2(H, Z, N) \
     LAUNCHB(nm, (k_fct_h<TRP, T, H, Z, N>), (S_fct_h<TRP, T, H, Z, N>), g3, d, p2dt, pu, pv, pw, ptb + o, ptn + o, pta + o, zwi, zwx, zwy, nt)
 #define FH_LAUNCH(H, Z) \
     do { if (nt == 2) FH_LAUNCH2(H, Z, 2); else FH_LAUNCH2(H, Z, 1); } while (0)
-        if (kn_fct_h == 4) {
-            if (ta0) FH_LAUNCH(1, 1); else FH_LAUNCH(1, 0);
-        } else {
-            if (ta0) FH_LAUNCH(0, 1); else FH_LAUNCH(0, 0);
-        }
+            if (kn_fct_h == 4) {
+                if (ta0) FH_LAUNCH(1, 1); else FH_LAUNCH(1, 0);
+            } else {
+                if (ta0) FH_LAUNCH(0, 1); else FH_LAUNCH(0, 0);
+            }
 #undef FH_LAUNCH
 #undef FH_LAUNCH2
-        bool vdone = false;
-        if constexpr (sizeof(T) == 4) {
-            if (kn_fct_v == 4 && d.jpk <= FCTV_REG_KMAX) {
-                Grid2 gv = gi;
-                gv.grid.z = (unsigned)nt;
-                LAUNCHB(nm, (k_fct_v_reg<TRP, FCTV_REG_KMAX>), (S_fct_v_reg<TRP, FCTV_REG_KMAX>), gv, d, pw, ptb + o, ptn + o, zwz, nt);
-                vdone = true;
-            }
         }
         if (vdone) {
         } else if (kn_fct_v == 4) {
