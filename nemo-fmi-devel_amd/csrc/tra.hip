@@ -1911,11 +1911,14 @@ struct S_fct_vz_reg {
         A3(zwz, i, j, jpk) = 0.0f;
         A3(zwi, i, j, jpk) = 0.0f;
         float zo[KMAX];
-        float zout_prev = 0.0f, tn_km1 = A3(tn, i, j, 1);
+        // every level's loads are issued one level ahead, from addresses
+        // within the arrays, so they are in flight across the level guard
+        float zout_prev = 0.0f, tn_km1 = A3(tn, i, j, 1), tn_next = A3(tn, i, j, 2);
     #pragma unroll
         for (int k = 2; k < KMAX; k++) {
             if (k <= jpk - 1) {
-                const float tnk = A3(tn, i, j, k);
+                const float tnk = tn_next;
+                tn_next = A3(tn, i, j, k + 1);   // level k + 1 <= jpk
                 float zwrm, zout, zwi_c = 0.0f;
                 if (k == 2 || k == ikb) {
                     zwrm = 0.5f * (tn_km1 + tnk);
@@ -1931,11 +1934,29 @@ struct S_fct_vz_reg {
                 tn_km1 = tnk;
             }
         }
+        // level k's operands: wn (or pwn), tb(k), tb(k-1), the stored s, ta,
+        // e3t; the climb starts at level jpk - 1
+        const float *wr = pw;
+        const int kf = jpk - 1;
+        float w_c = A3(wr, i, j, kf), tb_c = A3(tb, i, j, kf), tbm_c = A3(tb, i, j, kf > 1 ? kf - 1 : 1);
+        float s_c = A3(zwi, i, j, kf), a_c = TA0 ? 0.0f : A3(ta, i, j, kf), e3_c = E3(m.e3t, ij, kf);
+        const float e1e2t = TRP ? m.e1e2t[ij] : 1.0f;
         float xnext = 0.0f, upz_kp1 = 0.0f;   // upz(jpk) = 0
     #pragma unroll
         for (int k = KMAX - 1; k >= 1; k--) {
             if (k <= jpk - 1) {
-                const float upz = tp_upz<TRP>(d, pw, tb, i, j, k);
+                float w_n = 0.0f, tbm_n = 0.0f, s_n = 0.0f, a_n = 0.0f, e3_n = 0.0f;
+                if (k >= 2) {
+                    w_n = A3(wr, i, j, k - 1);
+                    tbm_n = A3(tb, i, j, k > 2 ? k - 2 : 1);
+                    s_n = A3(zwi, i, j, k - 1);
+                    if (!TA0) a_n = A3(ta, i, j, k - 1);
+                    e3_n = E3(m.e3t, ij, k - 1);
+                }
+                // tp_upz and tr_w of level k, the same operations
+                const float w = TRP ? e1e2t * w_c : w_c;
+                const float zfp = w + fabsf(w), zfm = w - fabsf(w);
+                const float upz = (k == 1) ? w * tb_c : 0.5f * (zfp * tb_c + zfm * tbm_c) * WMASK(float, i, j, k);
                 if (k >= 2) {   // zwz, as k_fct_v
                     const float wm = WMASK(float, i, j, k);
                     float x;
@@ -1945,16 +1966,16 @@ struct S_fct_vz_reg {
                         const float zws = (k == 2 || k == ikb) ? 0.0f : wm;
                         x = (zo[k] - zws * xnext) / cpt_piv(cpt_tab<float>(), k, ikb);
                     }
-                    A3(zwz, i, j, k) = (tr_w<TRP>(d, pw, i, j, k) * x - upz) * wm;
+                    A3(zwz, i, j, k) = (w * x - upz) * wm;
                     xnext = x;
                 }
                 // the low-order trend and first guess, as k_fct_h
-                const float e3t = E3(m.e3t, ij, k), tm = TMASK(float, i, j, k);
-                const float ztra = -(((A3(zwi, i, j, k) + upz) - upz_kp1) * r1_e1e2t);
-                const float a = TA0 ? 0.0f : A3(ta, i, j, k);
-                A3(ta, i, j, k) = a + ztra / e3t * tm;
-                A3(zwi, i, j, k) = (e3t * A3(tb, i, j, k) + zp2dt * ztra) / e3t * tm;
+                const float e3t = e3_c, tm = TMASK(float, i, j, k);
+                const float ztra = -(((s_c + upz) - upz_kp1) * r1_e1e2t);
+                A3(ta, i, j, k) = a_c + ztra / e3t * tm;
+                A3(zwi, i, j, k) = (e3t * tb_c + zp2dt * ztra) / e3t * tm;
                 upz_kp1 = upz;
+                w_c = w_n; tb_c = tbm_c; tbm_c = tbm_n; s_c = s_n; a_c = a_n; e3_c = e3_n;
             }
         }
     }
