@@ -1911,14 +1911,15 @@ struct S_fct_vz_reg {
         A3(zwz, i, j, jpk) = 0.0f;
         A3(zwi, i, j, jpk) = 0.0f;
         float zo[KMAX];
-        // every level's loads are issued one level ahead, from addresses
-        // within the arrays, so they are in flight across the level guard
-        float zout_prev = 0.0f, tn_km1 = A3(tn, i, j, 1), tn_next = A3(tn, i, j, 2);
+        // every level's loads are issued two levels ahead, from addresses
+        // within the arrays, so they are in flight across the level guards
+        float zout_prev = 0.0f, tn_km1 = A3(tn, i, j, 1), tn_n1 = A3(tn, i, j, 2), tn_n2 = A3(tn, i, j, jpk > 2 ? 3 : jpk);
     #pragma unroll
         for (int k = 2; k < KMAX; k++) {
             if (k <= jpk - 1) {
-                const float tnk = tn_next;
-                tn_next = A3(tn, i, j, k + 1);   // level k + 1 <= jpk
+                const float tnk = tn_n1;
+                tn_n1 = tn_n2;
+                tn_n2 = A3(tn, i, j, k + 2 <= jpk ? k + 2 : jpk);
                 float zwrm, zout, zwi_c = 0.0f;
                 if (k == 2 || k == ikb) {
                     zwrm = 0.5f * (tn_km1 + tnk);
@@ -1934,25 +1935,29 @@ struct S_fct_vz_reg {
                 tn_km1 = tnk;
             }
         }
-        // level k's operands: wn (or pwn), tb(k), tb(k-1), the stored s, ta,
-        // e3t; the climb starts at level jpk - 1
+        // level k's operands: wn (or pwn), tb(k), the stored s, ta, e3t (and
+        // tb(k-1), level k-1's); the climb starts at level jpk - 1
+        struct Lv { float w, tb, s, a, e3; };
         const float *wr = pw;
+        const auto lv = [&](int k) {
+            Lv l;
+            l.w = A3(wr, i, j, k);
+            l.tb = A3(tb, i, j, k);
+            l.s = A3(zwi, i, j, k);
+            l.a = TA0 ? 0.0f : A3(ta, i, j, k);
+            l.e3 = E3(m.e3t, ij, k);
+            return l;
+        };
         const int kf = jpk - 1;
-        float w_c = A3(wr, i, j, kf), tb_c = A3(tb, i, j, kf), tbm_c = A3(tb, i, j, kf > 1 ? kf - 1 : 1);
-        float s_c = A3(zwi, i, j, kf), a_c = TA0 ? 0.0f : A3(ta, i, j, kf), e3_c = E3(m.e3t, ij, kf);
+        Lv cur = lv(kf), nx = lv(kf > 1 ? kf - 1 : 1);
         const float e1e2t = TRP ? m.e1e2t[ij] : 1.0f;
         float xnext = 0.0f, upz_kp1 = 0.0f;   // upz(jpk) = 0
     #pragma unroll
         for (int k = KMAX - 1; k >= 1; k--) {
             if (k <= jpk - 1) {
-                float w_n = 0.0f, tbm_n = 0.0f, s_n = 0.0f, a_n = 0.0f, e3_n = 0.0f;
-                if (k >= 2) {
-                    w_n = A3(wr, i, j, k - 1);
-                    tbm_n = A3(tb, i, j, k > 2 ? k - 2 : 1);
-                    s_n = A3(zwi, i, j, k - 1);
-                    if (!TA0) a_n = A3(ta, i, j, k - 1);
-                    e3_n = E3(m.e3t, ij, k - 1);
-                }
+                Lv nn = nx;
+                if (k >= 3) nn = lv(k - 2);
+                const float w_c = cur.w, tb_c = cur.tb, tbm_c = nx.tb, s_c = cur.s, a_c = cur.a, e3_c = cur.e3;
                 // tp_upz and tr_w of level k, the same operations
                 const float w = TRP ? e1e2t * w_c : w_c;
                 const float zfp = w + fabsf(w), zfm = w - fabsf(w);
@@ -1975,7 +1980,8 @@ struct S_fct_vz_reg {
                 A3(ta, i, j, k) = a_c + ztra / e3t * tm;
                 A3(zwi, i, j, k) = (e3t * tb_c + zp2dt * ztra) / e3t * tm;
                 upz_kp1 = upz;
-                w_c = w_n; tb_c = tbm_c; tbm_c = tbm_n; s_c = s_n; a_c = a_n; e3_c = e3_n;
+                cur = nx;
+                nx = nn;
             }
         }
     }
