@@ -3,8 +3,8 @@ tile and tracer) against the oracle and against the flux / limiter chain.
 
 * FP64 drop-in tra_adv_fct with either workgroup size: bit-identical to
   the oracle for the four order pairs on basins whose sizes leave partial tiles in i and j, on the
-  GYRE-like basin and on a 75-level basin, with 1, 2 and 3 tracers (a pair
-  and a single-tracer remainder);
+  GYRE-like basin, on a 6-level, a 75-level and a 90-level basin, with 1, 2
+  and 3 tracers (a pair and a single-tracer remainder);
 * both storage forms: the fused kernel, with 256 or 512 threads per tile
   (tuning switch fct_fused = 1, 2), equals the chain (fct_fused = 0) bit
   for bit, for the drop-in routine (trend read) and for the fused step
@@ -27,6 +27,9 @@ BASINS = {
     "odd": lambda: configs.basin(131, 37, 14, seed=21),
     "deep": lambda: configs.basin(70, 45, 75, seed=22),
     "tiny": lambda: configs.basin(9, 7, 6, seed=23),
+    # deeper than the FP32 register-held column form (76 levels): the FP32
+    # chain takes its per-cell / per-column kernels
+    "deeper": lambda: configs.basin(40, 33, 90, seed=24),
 }
 ORDERS = [(2, 2), (4, 4), (2, 4), (4, 2)]
 
@@ -135,7 +138,7 @@ def both_paths(hip, fn):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["gyre", "odd", "deep"])
+@pytest.mark.parametrize("name", ["gyre", "odd", "deep", "tiny", "deeper"])
 @pytest.mark.parametrize("sp", [False, True])
 @pytest.mark.parametrize("orders", [(4, 4), (2, 2)])
 def test_fused_equals_chain(hip, name, sp, orders):
