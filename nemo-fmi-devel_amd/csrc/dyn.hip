@@ -496,14 +496,28 @@ __device__ __forceinline__ void zdf_column(const DevDomain &d, double r2dt, int 
         const T *ak = plane_ptr(avm, plane, k);
         return -(zdt * ((double)ldp(ak, ijn) + (double)ldp(ak, ij)) / (ze3 * (double)E3(e3xw, ij, k)) * wxmask(k));
     };
+    // zzw from loaded operands: avm at the face's two t-points, e3xw
+    auto zzw_v = [&](int k, double ze3, T a_n, T a_c, T e3w) {
+        return -(zdt * ((double)a_n + (double)a_c) / (ze3 * (double)e3w) * wxmask(k));
+    };
+    auto ld_avm = [&](int k, T &a_n, T &a_c, T &e3w) {   // level k <= jpk
+        const T *ak = plane_ptr(avm, plane, k);
+        a_n = ldp(ak, ijn);
+        a_c = ldp(ak, ij);
+        e3w = E3(e3xw, ij, k);
+    };
     double zws_prev, zwd_prev, a_prev;
+    // the operands of level k (pb, pa, e3x) and of w-level k + 1 (avm, e3xw)
+    // are loaded one level ahead
+    T w1n, w1c, w1e;   // w-level k + 1 of the current level
+    ld_avm(2, w1n, w1c, w1e);
     {   // k = 1: surface row with the wind stress
         const double xm = xmask(1);
         double a = ((double)ldp(pb, ij) + r2dt * (double)pre(1, ldp((const T *)pa, ij))) * xm;
         if (rm_b) a = (a - ab) * xm;
         const double ze3 = (1.0 - r_vvl) * (double)E3(e3x, ij, 1) + r_vvl * (double)E3(e3x, ij, 1);
         if (rm_b && 1 == ikb) a = a + r2dt * 0.5 * csum * ab / ze3;
-        const double zws = zzw(2, ze3);
+        const double zws = zzw_v(2, ze3, w1n, w1c, w1e);
         double zwd = 1.0 - zws;
         if (d.ln_drgimp && 1 == ikb) zwd = zwd - r2dt * 0.5 * csum / ze3;
         a = a + r2dt * 0.5 * (tau_b[ij] + tau[ij]) / (ze3 * d.rau0) * xm;
@@ -513,35 +527,64 @@ __device__ __forceinline__ void zdf_column(const DevDomain &d, double r2dt, int 
         zwd_prev = zwd;
         a_prev = a;
     }
+    {
+        T b_c = ldp(plane_ptr(pb, plane, 2), ij), t_c = ldp((const T *)plane_ptr(pa, plane, 2), ij);
+        T e_c = E3(e3x, ij, 2);
+        T w0n = w1n, w0c = w1c, w0e = w1e;   // w-level k
+        ld_avm(3 <= jpk ? 3 : jpk, w1n, w1c, w1e);
 #pragma unroll 4
-    for (int k = 2; k <= jpk - 1; k++) {
-        const double xm = xmask(k);
-        double a = ((double)ldp(plane_ptr(pb, plane, k), ij) +
-                    r2dt * (double)pre(k, ldp((const T *)plane_ptr(pa, plane, k), ij))) * xm;
-        if (rm_b) a = (a - ab) * xm;
-        const double ze3 = (1.0 - r_vvl) * (double)E3(e3x, ij, k) + r_vvl * (double)E3(e3x, ij, k);
-        if (rm_b && k == ikb) a = a + r2dt * 0.5 * csum * ab / ze3;
-        const double zwi = zzw(k, ze3);
-        const double zws = zzw(k + 1, ze3);
-        double zwd = 1.0 - zwi - zws;
-        if (d.ln_drgimp && k == ikb) zwd = zwd - r2dt * 0.5 * csum / ze3;
-        zwd = zwd - zwi * zws_prev / zwd_prev;
-        a = a - zwi / zwd_prev * a_prev;
-        stp(plane_ptr(zwd_s, plane, k), ij, zwd);
-        stp(plane_ptr(pa, plane, k), ij, (T)a);
-        zws_prev = zws;
-        zwd_prev = zwd;
-        a_prev = a;
+        for (int k = 2; k <= jpk - 1; k++) {
+            // level k + 1's operands (within the arrays: levels <= jpk)
+            const int kn = k + 1, kw = k + 2 <= jpk ? k + 2 : jpk;
+            const T b_n = ldp(plane_ptr(pb, plane, kn), ij), t_n = ldp((const T *)plane_ptr(pa, plane, kn), ij);
+            const T e_n = E3(e3x, ij, kn);
+            T w2n, w2c, w2e;
+            ld_avm(kw, w2n, w2c, w2e);
+            const double xm = xmask(k);
+            double a = ((double)b_c + r2dt * (double)pre(k, t_c)) * xm;
+            if (rm_b) a = (a - ab) * xm;
+            const double ze3 = (1.0 - r_vvl) * (double)e_c + r_vvl * (double)e_c;
+            if (rm_b && k == ikb) a = a + r2dt * 0.5 * csum * ab / ze3;
+            const double zwi = zzw_v(k, ze3, w0n, w0c, w0e);
+            const double zws = zzw_v(k + 1, ze3, w1n, w1c, w1e);
+            double zwd = 1.0 - zwi - zws;
+            if (d.ln_drgimp && k == ikb) zwd = zwd - r2dt * 0.5 * csum / ze3;
+            zwd = zwd - zwi * zws_prev / zwd_prev;
+            a = a - zwi / zwd_prev * a_prev;
+            stp(plane_ptr(zwd_s, plane, k), ij, zwd);
+            stp(plane_ptr(pa, plane, k), ij, (T)a);
+            zws_prev = zws;
+            zwd_prev = zwd;
+            a_prev = a;
+            b_c = b_n; t_c = t_n; e_c = e_n;
+            w0n = w1n; w0c = w1c; w0e = w1e;
+            w1n = w2n; w1c = w2c; w1e = w2e;
+        }
     }
     double x = a_prev / zwd_prev;
     stp(plane_ptr(pa, plane, jpk - 1), ij, (T)x);
+    if (jpk - 2 >= 1) {
+        // level k's e3x, pa and pivot, and w-level k + 1, one level ahead
+        int k = jpk - 2;
+        T e_c = E3(e3x, ij, k), t_c = ldp((const T *)plane_ptr(pa, plane, k), ij);
+        double p_c = ldp((const double *)plane_ptr(zwd_s, plane, k), ij);
+        T wn_, wc_, we_;
+        ld_avm(k + 1, wn_, wc_, we_);
 #pragma unroll 4
-    for (int k = jpk - 2; k >= 1; k--) {
-        const double ze3 = (1.0 - r_vvl) * (double)E3(e3x, ij, k) + r_vvl * (double)E3(e3x, ij, k);
-        const double zws = zzw(k + 1, ze3);
-        const double a = (double)ldp((const T *)plane_ptr(pa, plane, k), ij);
-        x = (a - zws * x) / ldp((const double *)plane_ptr(zwd_s, plane, k), ij);
-        stp(plane_ptr(pa, plane, k), ij, (T)x);
+        for (; k >= 1; k--) {
+            const int kp = k > 1 ? k - 1 : 1;
+            const T e_n = E3(e3x, ij, kp), t_n = ldp((const T *)plane_ptr(pa, plane, kp), ij);
+            const double p_n = ldp((const double *)plane_ptr(zwd_s, plane, kp), ij);
+            T wn2, wc2, we2;
+            ld_avm(kp + 1, wn2, wc2, we2);
+            const double ze3 = (1.0 - r_vvl) * (double)e_c + r_vvl * (double)e_c;
+            const double zws = zzw_v(k + 1, ze3, wn_, wc_, we_);
+            const double a = (double)t_c;
+            x = (a - zws * x) / p_c;
+            stp(plane_ptr(pa, plane, k), ij, (T)x);
+            e_c = e_n; t_c = t_n; p_c = p_n;
+            wn_ = wn2; wc_ = wc2; we_ = we2;
+        }
     }
 }
 
