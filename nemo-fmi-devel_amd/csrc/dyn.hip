@@ -521,7 +521,6 @@ __device__ __forceinline__ void zdf_column(const DevDomain &d, double r2dt, int 
         double zwd = 1.0 - zws;
         if (d.ln_drgimp && 1 == ikb) zwd = zwd - r2dt * 0.5 * csum / ze3;
         a = a + r2dt * 0.5 * (tau_b[ij] + tau[ij]) / (ze3 * d.rau0) * xm;
-        stp(zwd_s, ij, zwd);
         stp(pa, ij, (T)a);
         zws_prev = zws;
         zwd_prev = zwd;
@@ -551,7 +550,7 @@ __device__ __forceinline__ void zdf_column(const DevDomain &d, double r2dt, int 
             if (d.ln_drgimp && k == ikb) zwd = zwd - r2dt * 0.5 * csum / ze3;
             zwd = zwd - zwi * zws_prev / zwd_prev;
             a = a - zwi / zwd_prev * a_prev;
-            stp(plane_ptr(zwd_s, plane, k), ij, zwd);
+            if (k % ZDF_SEG == 0) stp(plane_ptr(zwd_s, plane, k), ij, zwd);   // checkpoint pivots only
             stp(plane_ptr(pa, plane, k), ij, (T)a);
             zws_prev = zws;
             zwd_prev = zwd;
@@ -563,27 +562,59 @@ __device__ __forceinline__ void zdf_column(const DevDomain &d, double r2dt, int 
     }
     double x = a_prev / zwd_prev;
     stp(plane_ptr(pa, plane, jpk - 1), ij, (T)x);
-    if (jpk - 2 >= 1) {
-        // level k's e3x, pa and pivot, and w-level k + 1, one level ahead
-        int k = jpk - 2;
-        T e_c = E3(e3x, ij, k), t_c = ldp((const T *)plane_ptr(pa, plane, k), ij);
-        double p_c = ldp((const double *)plane_ptr(zwd_s, plane, k), ij);
-        T wn_, wc_, we_;
-        ld_avm(k + 1, wn_, wc_, we_);
-#pragma unroll 4
-        for (; k >= 1; k--) {
-            const int kp = k > 1 ? k - 1 : 1;
-            const T e_n = E3(e3x, ij, kp), t_n = ldp((const T *)plane_ptr(pa, plane, kp), ij);
-            const double p_n = ldp((const double *)plane_ptr(zwd_s, plane, kp), ij);
-            T wn2, wc2, we2;
-            ld_avm(kp + 1, wn2, wc2, we2);
-            const double ze3 = (1.0 - r_vvl) * (double)e_c + r_vvl * (double)e_c;
-            const double zws = zzw_v(k + 1, ze3, wn_, wc_, we_);
-            const double a = (double)t_c;
-            x = (a - zws * x) / p_c;
-            stp(plane_ptr(pa, plane, k), ij, (T)x);
-            e_c = e_n; t_c = t_n; p_c = p_n;
-            wn_ = wn2; wc_ = wc2; we_ = we2;
+    // levels jpk-2 .. 1 by segments of ZDF_SEG levels, bottom segment first:
+    // the forward sweep kept the pivot of every ZDF_SEG-th level only, so a
+    // segment first recomputes its pivots from the one above it (the
+    // operations of the forward sweep), then substitutes back.  Each
+    // segment's operands are loaded together, from levels within the arrays.
+    const int kl = jpk - 2;
+    for (int s = (kl + ZDF_SEG - 1) / ZDF_SEG - 1; s >= 0; s--) {
+        const int k0 = s * ZDF_SEG + 1;
+        T e[ZDF_SEG], t[ZDF_SEG], wn[ZDF_SEG], wc[ZDF_SEG], we[ZDF_SEG];   // level k, w-level k + 1
+#pragma unroll
+        for (int q = 0; q < ZDF_SEG; q++) {
+            const int k = k0 + q <= kl ? k0 + q : kl;
+            e[q] = E3(e3x, ij, k);
+            t[q] = ldp((const T *)plane_ptr(pa, plane, k), ij);
+            ld_avm(k + 1, wn[q], wc[q], we[q]);
+        }
+        double zwd_p = 0.0, zws_p = 0.0;
+        T an = T(0), ac = T(0), aw = T(0);   // w-level k0
+        if (s > 0) {
+            zwd_p = ldp((const double *)plane_ptr(zwd_s, plane, k0 - 1), ij);
+            const T ep = E3(e3x, ij, k0 - 1);
+            ld_avm(k0, an, ac, aw);
+            const double ze3p = (1.0 - r_vvl) * (double)ep + r_vvl * (double)ep;
+            zws_p = zzw_v(k0, ze3p, an, ac, aw);
+        }
+        double piv[ZDF_SEG], zsv[ZDF_SEG];
+#pragma unroll
+        for (int q = 0; q < ZDF_SEG; q++) {
+            const int k = k0 + q;
+            const double ze3 = (1.0 - r_vvl) * (double)e[q] + r_vvl * (double)e[q];
+            const double zws = zzw_v(k + 1, ze3, wn[q], wc[q], we[q]);
+            double zwd;
+            if (k == 1) {
+                zwd = 1.0 - zws;
+                if (d.ln_drgimp && 1 == ikb) zwd = zwd - r2dt * 0.5 * csum / ze3;
+            } else {
+                const double zwi = q == 0 ? zzw_v(k, ze3, an, ac, aw) : zzw_v(k, ze3, wn[q - 1], wc[q - 1], we[q - 1]);
+                zwd = 1.0 - zwi - zws;
+                if (d.ln_drgimp && k == ikb) zwd = zwd - r2dt * 0.5 * csum / ze3;
+                zwd = zwd - zwi * zws_p / zwd_p;
+            }
+            piv[q] = zwd;
+            zsv[q] = zws;
+            zws_p = zws;
+            zwd_p = zwd;
+        }
+#pragma unroll
+        for (int q = ZDF_SEG - 1; q >= 0; q--) {
+            const int k = k0 + q;
+            if (k <= kl) {
+                x = ((double)t[q] - zsv[q] * x) / piv[q];
+                stp(plane_ptr(pa, plane, k), ij, (T)x);
+            }
         }
     }
 }
