@@ -1297,12 +1297,15 @@ template int launch_tra_ldf_iso<float>(int, double, const float *, const float *
 // ---------------------------------------------------------------------------
 // tra_zdf_imp (+ the salinity reset of tra_zdf), TRA/trazdf.F90.
 // One thread per interior column: the forward sweep builds the matrix on the
-// fly (FP64), keeps T_k in a scratch column and substitutes every tracer of
-// the group; the backward sweep recomputes the upper diagonal.
+// fly (FP64), keeps the pivot of every ZDF_SEG-th level in a scratch column
+// and substitutes every tracer of the group; the backward sweep recomputes
+// the upper diagonal and, segment by segment from those checkpoints, the
+// other pivots, with the operations of the forward sweep.
 // ---------------------------------------------------------------------------
-// NT tracers share the matrix.  The sweeps are unrolled by four with the
-// pointers declared non-aliasing, so the loads of four levels are issued
-// together ahead of the serial FP64 recurrence.
+// NT tracers share the matrix.  The forward sweep is unrolled by four with
+// the pointers declared non-aliasing, so the loads of four levels are issued
+// together ahead of the serial FP64 recurrence; the backward sweep loads a
+// segment's operands together.
 template <typename T, int NT>
 struct S_tra_zdf {
     static constexpr int NTHR = NH_BLOCK, WAVES = 1;
@@ -1333,11 +1336,8 @@ struct S_tra_zdf {
             if (l_ldfslp && k <= jpk - 1) c = c + av;
             return k <= 1 ? 0.0 : c;
         };
-        // FP64 storage keeps only checkpoint pivots (every ZDF_SEG-th level) and
-        // recomputes the others in the back substitution; with FP32 storage the
-        // kernel is latency-bound and storing every pivot is faster (measured at
-        // 1024x1024x75)
-        constexpr bool ckpt = sizeof(T) == 8;
+        // only checkpoint pivots (every ZDF_SEG-th level) are stored; the back
+        // substitution recomputes the others segment by segment
         double prev[NT];
         double zws_prev, zwt_prev, zc_k;
         // k = 1 (zwi = 0)
@@ -1347,7 +1347,6 @@ struct S_tra_zdf {
             const double zwi = -(p2dt * 0.0 / (double)E3(m.e3w, ij, 1));
             const double zws = -(p2dt * zc_kp1 / (double)E3(m.e3w, ij, 2));
             const double zwd = e3t - zwi - zws;
-            if (!ckpt) stp(zwt, ij, zwd);
     #pragma unroll
             for (int n = 0; n < NT; n++) {
                 const double v = e3t * (double)ldp(tb[n], ij) + p2dt * e3t * (double)ldp((const T *)ta[n], ij);
@@ -1366,7 +1365,7 @@ struct S_tra_zdf {
             const double zws = -(p2dt * zc_kp1 / (double)E3(m.e3w, ij, k + 1));
             const double zwd = e3t - zwi - zws;
             const double zwt_k = zwd - zwi * zws_prev / zwt_prev;
-            if (!ckpt || (k % ZDF_SEG == 0 && k <= jpk - 2)) stp(plane_ptr(zwt, plane, k), ij, zwt_k);
+            if (k % ZDF_SEG == 0 && k <= jpk - 2) stp(plane_ptr(zwt, plane, k), ij, zwt_k);
     #pragma unroll
             for (int n = 0; n < NT; n++) {
                 const double zrhs = e3t * (double)ldp(plane_ptr(tb[n], plane, k), ij) +
@@ -1392,50 +1391,52 @@ struct S_tra_zdf {
             }
         }
         const int t0 = d.kT.k0[ij], t1 = d.kT.k1[ij];
-        if constexpr (!ckpt) {
-    #pragma unroll 4
-            for (int k = jpk - 2; k >= 1; k--) {
-                const double zws = -(p2dt * coef(k + 1) / (double)E3(m.e3w, ij, k + 1));
-                const double zwt_k = ldp((const double *)plane_ptr(zwt, plane, k), ij);
-                const double tm = (k >= t0 && k <= t1) ? 1.0 : 0.0;
-    #pragma unroll
-                for (int n = 0; n < NT; n++) {
-                    const double v = ((double)ldp((const T *)plane_ptr(ta[n], plane, k), ij) - zws * prev[n]) / zwt_k * tm;
-                    prev[n] = v;
-                    T out = (T)v;
-                    if (jn0 + n == clamp_jn && out < T(0)) out = T(0.1);
-                    stp(plane_ptr(ta[n], plane, k), ij, out);
-                }
-            }
-        }
-        // levels jpk-2 .. 1 by segments of ZDF_SEG, bottom segment first; the
-        // lower diagonal of level k equals the upper one of level k-1 (same
-        // expression), so a segment restarts from its checkpoint pivot alone
-        const int kl = ckpt ? jpk - 2 : 0;
+        const int kl = jpk - 2;
+        // each segment's operands are loaded together, from levels within the
+        // arrays (clamped to kl; the values of levels past kl are not used)
+        auto ld_w = [&](int k, T &sv_, T &av_, T &e3w_) {   // w-level k <= jpk
+            sv_ = ldp(plane_ptr(src, plane, k), ij);
+            av_ = ldp(plane_ptr(addp, plane, k <= jpk - 1 ? k : jpk - 1), ij);
+            e3w_ = E3(m.e3w, ij, k);
+        };
+        auto coef_v = [&](int k, T sv_, T av_) -> double {
+            double c = (double)sv_;
+            if (l_ldfslp && k <= jpk - 1) c = c + (double)av_;
+            return k <= 1 ? 0.0 : c;
+        };
         for (int s = (kl + ZDF_SEG - 1) / ZDF_SEG - 1; s >= 0; s--) {
             const int k0 = s * ZDF_SEG + 1;
+            T e3t_[ZDF_SEG], sv_[ZDF_SEG], av_[ZDF_SEG], e3w_[ZDF_SEG], ta_[NT][ZDF_SEG];   // level k, w-level k + 1
+    #pragma unroll
+            for (int q = 0; q < ZDF_SEG; q++) {
+                const int k = k0 + q <= kl ? k0 + q : kl;
+                e3t_[q] = E3(m.e3t, ij, k);
+                ld_w(k + 1, sv_[q], av_[q], e3w_[q]);
+    #pragma unroll
+                for (int n = 0; n < NT; n++) ta_[n][q] = ldp((const T *)plane_ptr(ta[n], plane, k), ij);
+            }
+            T svp, avp, e3wp;   // w-level k0
+            ld_w(k0, svp, avp, e3wp);
             double piv[ZDF_SEG], zsv[ZDF_SEG];
-            double zc = coef(k0);
+            double zc = coef_v(k0, svp, avp);
             double zwt_p = 1.0, zws_p = 0.0;
             if (k0 > 1) {
                 zwt_p = ldp((const double *)plane_ptr(zwt, plane, k0 - 1), ij);
-                zws_p = -(p2dt * zc / (double)E3(m.e3w, ij, k0));
+                zws_p = -(p2dt * zc / (double)e3wp);
             }
     #pragma unroll
             for (int q = 0; q < ZDF_SEG; q++) {
                 const int k = k0 + q;
-                if (k <= kl) {
-                    const double zc_kp1 = coef(k + 1);
-                    const double e3t = (double)E3(m.e3t, ij, k);
-                    const double zwi = -(p2dt * zc / (double)E3(m.e3w, ij, k));
-                    const double zws = -(p2dt * zc_kp1 / (double)E3(m.e3w, ij, k + 1));
-                    const double zwd = e3t - zwi - zws;
-                    piv[q] = k == 1 ? zwd : zwd - zwi * zws_p / zwt_p;
-                    zsv[q] = zws;
-                    zws_p = zws;
-                    zwt_p = piv[q];
-                    zc = zc_kp1;
-                }
+                const double zc_kp1 = coef_v(k + 1, sv_[q], av_[q]);
+                const double e3t = (double)e3t_[q];
+                const double zwi = -(p2dt * zc / (double)(q == 0 ? e3wp : e3w_[q > 0 ? q - 1 : 0]));
+                const double zws = -(p2dt * zc_kp1 / (double)e3w_[q]);
+                const double zwd = e3t - zwi - zws;
+                piv[q] = k == 1 ? zwd : zwd - zwi * zws_p / zwt_p;
+                zsv[q] = zws;
+                zws_p = zws;
+                zwt_p = piv[q];
+                zc = zc_kp1;
             }
     #pragma unroll
             for (int q = ZDF_SEG - 1; q >= 0; q--) {
@@ -1444,8 +1445,7 @@ struct S_tra_zdf {
                     const double tm = (k >= t0 && k <= t1) ? 1.0 : 0.0;
     #pragma unroll
                     for (int n = 0; n < NT; n++) {
-                        const double v =
-                            ((double)ldp((const T *)plane_ptr(ta[n], plane, k), ij) - zsv[q] * prev[n]) / piv[q] * tm;
+                        const double v = ((double)ta_[n][q] - zsv[q] * prev[n]) / piv[q] * tm;
                         prev[n] = v;
                         T out = (T)v;
                         if (jn0 + n == clamp_jn && out < T(0)) out = T(0.1);
