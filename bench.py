@@ -860,7 +860,7 @@ def main():
         """the library defaults, the setting saved for the form's workload,
         then the --tune overrides"""
         saved = autotune.load(a.autotune or a.tuning, f"{form} {a.size}") or {}
-        setting = dict(defaults, **saved, **overrides)
+        setting = {**defaults, **saved, **overrides}
         autotune.apply(h, setting)
         tuning[form] = setting
         if saved or overrides:
