@@ -301,6 +301,36 @@ def test_dyn_zdf(hip, case, kt):
     check(host(va, e.s3), va_o, what="dyn_zdf va")
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("nk", [3, 4, 5, 6])
+def test_column_solves_shallow(hip, nk):
+    """The column solves on shallow basins: the back substitution runs by
+    4-level segments from checkpoint pivots, so 1 to 4 solved levels leave
+    no checkpoint and partial segments; FP64 bit-exact against the oracle."""
+    d = configs.basin(30, 24, nk, seed=9)
+    S = configs.make_state(d, seed=11)
+    o = Oracle(d)
+    hip.register(d)
+    s3, s4 = (d.jpi, d.jpj, d.jpk), (d.jpi, d.jpj, d.jpk, 2)
+    c = lambda n: np.array(S[n], order="F", copy=True)
+    ua_o, va_o = c("ua"), c("va")
+    args = [S[n] for n in ("avm", "rCdU_bot", "ua_b", "va_b", "utau", "utau_b", "vtau", "vtau_b")]
+    o.dyn_zdf(2, S["ub"], S["vb"], ua_o, va_o, *args)
+    ua, va = dev(S["ua"]), dev(S["va"])
+    hip.dyn_zdf(2, dev(S["ub"]), dev(S["vb"]), ua, va, *[dev(a) for a in args])
+    sync(hip)
+    check(host(ua, s3), ua_o, what=f"dyn_zdf ua, {nk} levels")
+    check(host(va, s3), va_o, what=f"dyn_zdf va, {nk} levels")
+    want = c("tsa")
+    avs = np.asfortranarray(S["avt"] * 0.5)
+    ah = np.asfortranarray(np.abs(S["wslpi"]))
+    o.tra_zdf_imp(4, 1, "TRA", 3600.0, S["tsb"], want, 2, S["avt"], avs, 1, ah, ah)
+    ta = dev(S["tsa"])
+    hip.tra_zdf_imp(4, 1, "TRA", 3600.0, dev(S["tsb"]), ta, 2, dev(S["avt"]), dev(avs), 1, dev(ah), dev(ah))
+    sync(hip)
+    check(host(ta, s4), want, what=f"tra_zdf_imp, {nk} levels")
+
+
 SPG_2D = ["un_b", "vn_b", "ub_b", "vb_b", "sshn", "hu_n", "hv_n", "r1_hu_n", "r1_hv_n", "ht_n", "rCdU_bot", "utau",
           "vtau", "emp", "rnf", "ub2_b", "vb2_b", "un_bf", "vn_bf"]
 SPG_OUT = ["ua_b", "va_b", "ssha", "un_adv", "vn_adv", "ub2_b", "vb2_b", "un_bf", "vn_bf"]
